@@ -1,0 +1,182 @@
+"""Seeded synthetic geometries for the flow-and-transport path.
+
+Each generator returns a :class:`Workload`: a one-layer grid (nz = 1) with its
+obstacle mask, per-site mobility and Brinkman occupancy phi. Storage follows
+the reference's site order, site = x * ny + y (cpp/core/flow_system.hpp:64-67),
+so arrays are shaped (nx, ny) with y contiguous. Flow runs along x, from the
+inlet face x = 0 (pressure 1) to the outlet face x = nx (pressure 0).
+
+Mobility closure (python/src/microsimulator/flow.py:191-214, 91-96, 361-384):
+    m = k / (1 + k * C * phi^2 / (1 - phi)^3),   phi capped at 0.9, C = 100,
+with k = H^2/12 the Hele-Shaw permeability of depth H. The one-layer reduction
+(cpp/core/flow_system.hpp:311-316) multiplies mobility by the layer height dz
+to give the column conductance, so a per-site depth H_i maps to mobility
+H_i^3 / (12 dz) / (1 + k_i * drag): conductance = H_i * k_i / (1 + k_i * drag).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NO_FLUX, PERIODIC, FIXED = 0, 1, 2
+DRAG_COEFFICIENT = 100.0
+MAX_VOLUME_FRACTION = 0.9
+
+
+@dataclass
+class Workload:
+    name: str
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    dz: float
+    obstacles: np.ndarray  # uint8 (nx, ny)
+    mobility: np.ndarray  # float32 (nx, ny), zero in solid
+    phi: np.ndarray  # float32 (nx, ny), zero in solid
+    boundary: tuple = (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def sites(self) -> int:
+        return self.nx * self.ny
+
+    @property
+    def fluid(self) -> int:
+        return int((self.obstacles == 0).sum())
+
+    def face_counts(self):
+        return (self.nx + 1) * self.ny, self.nx * (self.ny + 1), 2 * self.nx * self.ny
+
+
+def brinkman_mobility(depth, phi, solid, dz):
+    """Column mobility from depth and occupancy (see module docstring)."""
+    depth = np.asarray(depth, np.float64)
+    phi = np.minimum(np.asarray(phi, np.float64), MAX_VOLUME_FRACTION)
+    k = depth * depth / 12.0
+    drag = DRAG_COEFFICIENT * phi * phi / (1.0 - phi) ** 3
+    m = (depth / dz) * k / (1.0 + k * drag)
+    m = np.where(solid, 0.0, m)
+    return m.astype(np.float32)
+
+
+def gaussian_blobs(nx, ny, count, rng, sigma=(10.0, 30.0), peak=(0.3, 0.7), cap=0.7):
+    """phi = min(cap, sum of `count` Gaussian blobs), each summed over a 4-sigma window."""
+    phi = np.zeros((nx, ny), np.float64)
+    cx = rng.uniform(0, nx, count)
+    cy = rng.uniform(0, ny, count)
+    sg = rng.uniform(sigma[0], sigma[1], count)
+    pk = rng.uniform(peak[0], peak[1], count)
+    for i in range(count):
+        w = int(np.ceil(4 * sg[i]))
+        x0, x1 = max(0, int(cx[i]) - w), min(nx, int(cx[i]) + w + 1)
+        y0, y1 = max(0, int(cy[i]) - w), min(ny, int(cy[i]) + w + 1)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        xs = (np.arange(x0, x1) + 0.5 - cx[i])[:, None]
+        ys = (np.arange(y0, y1) + 0.5 - cy[i])[None, :]
+        phi[x0:x1, y0:y1] += pk[i] * np.exp(-(xs * xs + ys * ys) / (2 * sg[i] * sg[i]))
+    return np.minimum(phi, cap)
+
+
+def place_pillars(nx, ny, count, rng, radius=(8, 15), gap=2, band=None, max_tries=200000):
+    """Rejection-sample non-overlapping circular pillars (centres, radii).
+
+    Pillars keep `gap` cells between each other and stay `band` cells clear of
+    the inlet and outlet faces."""
+    rmax = radius[1]
+    band = rmax + 8 if band is None else band
+    cell = 2 * rmax + gap
+    buckets: dict = {}
+    centres, radii = [], []
+    tries = 0
+    while len(centres) < count and tries < max_tries:
+        tries += 1
+        r = rng.uniform(radius[0], radius[1])
+        x = rng.uniform(band + r, nx - band - r)
+        y = rng.uniform(r + 1, ny - r - 1)
+        bx, by = int(x // cell), int(y // cell)
+        ok = True
+        for i in range(bx - 1, bx + 2):
+            for j in range(by - 1, by + 2):
+                for k in buckets.get((i, j), ()):
+                    ox, oy = centres[k]
+                    if (x - ox) ** 2 + (y - oy) ** 2 < (r + radii[k] + gap) ** 2:
+                        ok = False
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if not ok:
+            continue
+        buckets.setdefault((bx, by), []).append(len(centres))
+        centres.append((x, y))
+        radii.append(r)
+    return np.array(centres, np.float64).reshape(-1, 2), np.array(radii, np.float64)
+
+
+def rasterize_pillars(nx, ny, centres, radii):
+    solid = np.zeros((nx, ny), bool)
+    for (x, y), r in zip(centres, radii):
+        x0, x1 = max(0, int(x - r) - 1), min(nx, int(x + r) + 2)
+        y0, y1 = max(0, int(y - r) - 1), min(ny, int(y + r) + 2)
+        xs = (np.arange(x0, x1) + 0.5 - x)[:, None]
+        ys = (np.arange(y0, y1) + 0.5 - y)[None, :]
+        solid[x0:x1, y0:y1] |= xs * xs + ys * ys <= r * r
+    return solid
+
+
+def channel_trap(nx=256, ny=128, seed=42, depth=10.0, dx=1.0, channel=40, phi_max=0.6):
+    """Straight channel y in [0, channel) over the whole length, plus an 80 x 60
+    trap cavity (scaled with the grid) hanging off it; solid elsewhere. phi ~
+    U[0, phi_max) i.i.d. inside the trap, zero elsewhere. Uniform depth."""
+    rng = np.random.default_rng(seed)
+    sx, sy = nx / 256.0, ny / 128.0
+    ch = int(round(channel * sy))
+    tx0, tx1 = int(round(88 * sx)), int(round(168 * sx))
+    ty0, ty1 = ch, min(ny, ch + int(round(60 * sy)))
+    solid = np.ones((nx, ny), bool)
+    solid[:, :ch] = False
+    solid[tx0:tx1, ty0:ty1] = False
+    phi = np.zeros((nx, ny), np.float64)
+    phi[tx0:tx1, ty0:ty1] = rng.uniform(0.0, phi_max, (tx1 - tx0, ty1 - ty0))
+    mob = brinkman_mobility(np.full((nx, ny), depth), phi, solid, depth)
+    phi = np.where(solid, 0.0, phi).astype(np.float32)
+    return Workload("channel_trap", nx, ny, dx, dx, depth, solid.astype(np.uint8), mob, phi,
+                    meta=dict(seed=seed, depth=depth, trap=(tx0, tx1, ty0, ty1)))
+
+
+def pillar_array(nx=1024, ny=512, seed=7, pillars=300, radius=(8, 15), blobs=40,
+                 depth=(8.0, 12.0), dx=1.0, reference_depth=10.0):
+    """Channel filled with seeded circular pillars, per-site depth U[depth],
+    phi from Gaussian blobs (capped at 0.7)."""
+    rng = np.random.default_rng(seed)
+    centres, radii = place_pillars(nx, ny, pillars, rng, radius=radius)
+    solid = rasterize_pillars(nx, ny, centres, radii)
+    phi = gaussian_blobs(nx, ny, blobs, rng)
+    if np.isscalar(depth):
+        h = np.full((nx, ny), float(depth))
+    else:
+        h = rng.uniform(depth[0], depth[1], (nx, ny))
+    mob = brinkman_mobility(h, phi, solid, reference_depth)
+    phi = np.where(solid, 0.0, phi).astype(np.float32)
+    return Workload("pillar_array", nx, ny, dx, dx, reference_depth, solid.astype(np.uint8), mob,
+                    phi, meta=dict(seed=seed, pillars=len(radii), blobs=blobs))
+
+
+def config(name: str, seed: int | None = None) -> Workload:
+    """The named workloads: c1 (256x128 channel+trap), c2 (1024x512, ~300
+    pillars r~U[8,15], depth U[8,12]), c4 (512x256 channel+trap), c5
+    (4096x2048, ~4k pillars r~U[8,20], 500 blobs, uniform depth 10)."""
+    if name == "c1":
+        return channel_trap(256, 128, seed=42 if seed is None else seed)
+    if name == "c2":
+        return pillar_array(1024, 512, seed=7 if seed is None else seed)
+    if name == "c4":
+        return channel_trap(512, 256, seed=4 if seed is None else seed)
+    if name == "c5":
+        return pillar_array(4096, 2048, seed=5 if seed is None else seed, pillars=4000,
+                            radius=(8, 20), blobs=500, depth=10.0)
+    raise ValueError(f"unknown workload {name}")

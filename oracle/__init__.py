@@ -1,0 +1,403 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg import this package, and only as the checker: the product
+(``microsimulator_amd``) never imports, loads or calls it.
+
+Two libraries are bound when present:
+  * ``oracle/libmsim_oracle.so`` -- the plain-C restatement (msim_oracle.c);
+  * ``oracle/_ref/libmsim_ref.so`` -- the reference's own CPU sources compiled
+    where they lie (built only where the reference tree exists).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libmsim_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmsim_ref.so")
+
+NO_FLUX, PERIODIC, FIXED = 0, 1, 2
+
+STATUS = {
+    0: "ok", 1: "invalid_argument", 2: "runtime_error", 3: "not_converged", 4: "overflow",
+    5: "domain_error", 11: "cg_curvature", 12: "cg_step", 13: "cg_residual", 14: "cg_breakdown",
+    15: "cg_direction",
+}
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+def build(reference: bool = True) -> None:
+    """Compile the oracle (and, when the reference tree exists, oracle/_ref)."""
+    targets = ["all"]
+    if reference and os.path.isdir(os.environ.get("REFERENCE", "/root/reference")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE, *targets], check=True)
+
+
+class _Grid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+                ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+                ("obstacles", C.c_void_p), ("boundary", C.c_uint8 * 6)]
+
+
+class _FlowReport(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("relative_residual", C.c_double),
+                ("true_relative_residual", C.c_double), ("mean_inlet_speed", C.c_float),
+                ("max_speed", C.c_float), ("solved_mean", C.c_float), ("factor", C.c_float)]
+
+
+class _SignalGrid(C.Structure):
+    _fields_ = [("grid", _Grid), ("signal_count", C.c_uint32), ("diffusion", C.c_void_p),
+                ("advection", C.c_void_p), ("boundary_values", C.c_void_p),
+                ("x_faces", C.c_void_p), ("y_faces", C.c_void_p), ("z_faces", C.c_void_p),
+                ("source_rates", C.c_void_p), ("loss_rates", C.c_void_p), ("phi", C.c_void_p),
+                ("monod_vmax", C.c_void_p), ("monod_k", C.c_void_p)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+@dataclass
+class Grid:
+    """A grid as the reference's SignalGridSpec states it (nz = 1 here)."""
+    nx: int
+    ny: int
+    nz: int = 1
+    dx: float = 1.0
+    dy: float = 1.0
+    dz: float = 1.0
+    obstacles: np.ndarray | None = None
+    boundary: tuple = (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX)
+
+    @property
+    def sites(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def face_counts(self):
+        return ((self.nx + 1) * self.ny * self.nz, self.nx * (self.ny + 1) * self.nz,
+                self.nx * self.ny * (self.nz + 1))
+
+    def _c(self, keep: list):
+        g = _Grid()
+        g.nx, g.ny, g.nz = self.nx, self.ny, self.nz
+        g.dx, g.dy, g.dz = self.dx, self.dy, self.dz
+        if self.obstacles is not None:
+            obs = np.ascontiguousarray(self.obstacles, dtype=np.uint8).reshape(-1)
+            keep.append(obs)
+            g.obstacles = obs.ctypes.data
+        for i, k in enumerate(self.boundary):
+            g.boundary[i] = k
+        return g
+
+
+@dataclass
+class FlowReport:
+    iterations: int
+    relative_residual: float
+    true_relative_residual: float
+    mean_inlet_speed: float
+    max_speed: float
+    solved_mean: float
+    factor: float
+
+
+class Oracle:
+    """The plain-C restatement."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build(reference=False)
+        self.lib = C.CDLL(path)
+        L = self.lib
+        P, U, F, Z = C.c_void_p, C.c_uint32, C.c_float, C.c_size_t
+        L.mo_flow_assemble.argtypes = [P, P, U, P, P, P, P, P, P, C.c_char_p, Z]
+        L.mo_flow_apply.argtypes = [P, P, U, P, P, C.c_char_p, Z]
+        L.mo_flow_solve.argtypes = [P, P, U, F, F, U, C.c_int, P, P, P, C.c_char_p, Z]
+        L.mo_adr_validate_step.argtypes = [P, F, C.c_char_p, Z]
+        L.mo_adr_step_f32.argtypes = [P, F, P, C.c_char_p, Z]
+        L.mo_adr_step_f64.argtypes = [P, F, P, P, C.c_char_p, Z]
+        for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_adr_validate_step,
+                   L.mo_adr_step_f32, L.mo_adr_step_f64):
+            fn.restype = C.c_int
+
+    @staticmethod
+    def _check(status, err):
+        if status != 0:
+            raise OracleError(status, err.value.decode(errors="replace"))
+
+    def assemble(self, grid: Grid, mobility, axis: int = 0):
+        keep: list = []
+        g = grid._c(keep)
+        n = grid.nx * grid.ny
+        nf = sum(grid.face_counts())
+        out = {k: np.zeros(n, np.float32) for k in ("conductance", "diagonal", "rhs", "bc")}
+        fc = np.zeros(nf, np.float32)
+        fp = np.zeros(nf, np.float32)
+        mob = _f32(mobility)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_flow_assemble(
+            C.byref(g), _ptr(mob), C.c_uint32(axis), out["conductance"].ctypes.data,
+            out["diagonal"].ctypes.data, out["rhs"].ctypes.data, out["bc"].ctypes.data,
+            fc.ctypes.data, fp.ctypes.data, err, C.c_size_t(512))
+        self._check(st, err)
+        out["face_conductance"] = fc
+        out["face_pressure"] = fp
+        return out
+
+    def apply(self, grid: Grid, mobility, x, axis: int = 0):
+        keep: list = []
+        g = grid._c(keep)
+        xin = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        y = np.zeros(grid.nx * grid.ny, np.float64)
+        mob = _f32(mobility)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_flow_apply(C.byref(g), _ptr(mob), C.c_uint32(axis), xin.ctypes.data,
+                                    y.ctypes.data, err, C.c_size_t(512))
+        self._check(st, err)
+        return y
+
+    def solve_flow(self, grid: Grid, mobility, axis: int = 0, mean_inlet_speed: float = 1.0,
+                   tolerance: float = 1e-6, max_iterations: int = 50000, scale: bool = True):
+        keep: list = []
+        g = grid._c(keep)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        pressure = np.zeros(grid.nx * grid.ny, np.float64)
+        rep = _FlowReport()
+        mob = _f32(mobility)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_flow_solve(
+            C.byref(g), _ptr(mob), C.c_uint32(axis), C.c_float(mean_inlet_speed),
+            C.c_float(tolerance), C.c_uint32(max_iterations), C.c_int(1 if scale else 0),
+            faces.ctypes.data, pressure.ctypes.data, C.byref(rep), err, C.c_size_t(512))
+        self._check(st, err)
+        report = FlowReport(rep.iterations, rep.relative_residual, rep.true_relative_residual,
+                            rep.mean_inlet_speed, rep.max_speed, rep.solved_mean, rep.factor)
+        return faces, pressure, report
+
+    def _signal_grid(self, grid: Grid, keep: list, diffusion, boundary_values=None,
+                     faces=None, advection=None, source=None, loss=None, phi=None,
+                     vmax=None, monod_k=None):
+        sg = _SignalGrid()
+        sg.grid = grid._c(keep)
+        diffusion = _f32(np.atleast_1d(diffusion))
+        S = diffusion.size
+        sg.signal_count = S
+        bv = np.zeros(6 * S, np.float32) if boundary_values is None else _f32(boundary_values)
+        arrays = dict(diffusion=diffusion, boundary_values=bv.reshape(-1))
+        if advection is not None:
+            arrays["advection"] = _f32(advection).reshape(-1)
+        if faces is not None:
+            nfx, nfy, nfz = grid.face_counts()
+            f = _f32(faces).reshape(-1)
+            arrays["x_faces"] = f[:nfx].copy()
+            arrays["y_faces"] = f[nfx:nfx + nfy].copy()
+            arrays["z_faces"] = f[nfx + nfy:].copy()
+        if loss is not None:
+            arrays["loss_rates"] = _f32(loss).reshape(-1)
+            arrays["source_rates"] = (np.zeros_like(arrays["loss_rates"]) if source is None
+                                      else _f32(source).reshape(-1))
+        if phi is not None:
+            arrays["phi"] = _f32(phi).reshape(-1)
+            arrays["monod_vmax"] = _f32(np.broadcast_to(np.asarray(vmax, np.float32), (S,)))
+            arrays["monod_k"] = _f32(np.broadcast_to(np.asarray(monod_k, np.float32), (S,)))
+        for name, arr in arrays.items():
+            arr = np.ascontiguousarray(arr)
+            keep.append(arr)
+            setattr(sg, name, arr.ctypes.data)
+        return sg
+
+    def validate_step(self, grid: Grid, dt: float, **kw):
+        keep: list = []
+        sg = self._signal_grid(grid, keep, **kw)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_adr_validate_step(C.byref(sg), C.c_float(dt), err, C.c_size_t(512))
+        self._check(st, err)
+
+    def adr_step_f32(self, grid: Grid, levels, dt: float, steps: int = 1, **kw):
+        keep: list = []
+        sg = self._signal_grid(grid, keep, **kw)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        err = C.create_string_buffer(512)
+        for _ in range(steps):
+            st = self.lib.mo_adr_step_f32(C.byref(sg), C.c_float(dt), lv.ctypes.data, err,
+                                          C.c_size_t(512))
+            self._check(st, err)
+        return lv
+
+    def adr_step_f64(self, grid: Grid, levels, dt: float, steps: int = 1, **kw):
+        keep: list = []
+        sg = self._signal_grid(grid, keep, **kw)
+        S = sg.signal_count
+        lv = np.array(levels, dtype=np.float64, copy=True).reshape(-1)
+        ledgers = np.zeros((steps, S, 4), np.float64)
+        err = C.create_string_buffer(512)
+        for k in range(steps):
+            st = self.lib.mo_adr_step_f64(C.byref(sg), C.c_float(dt), lv.ctypes.data,
+                                          ledgers[k].ctypes.data, err, C.c_size_t(512))
+            self._check(st, err)
+        return lv, ledgers
+
+
+class Reference:
+    """The reference's own CPU path (oracle/_ref/libmsim_ref.so)."""
+
+    def __init__(self, path: str = REF_SO):
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.ref_flow_create.restype = C.c_void_p
+        L.ref_flow_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                      C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_char_p, C.c_size_t]
+        L.ref_flow_destroy.argtypes = [C.c_void_p]
+        L.ref_flow_arrays.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.ref_flow_pressure.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p,
+                                        C.c_char_p, C.c_size_t]
+        L.ref_flow_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_threads.restype = C.c_uint32
+        P, U, F, Z = C.c_void_p, C.c_uint32, C.c_float, C.c_size_t
+        L.ref_solve_depth_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, P, P, P,
+                                           C.c_char_p, Z]
+        L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, F,
+                                              U, P, P, C.c_char_p, Z]
+        L.ref_set_threads.argtypes = [U]
+
+    @staticmethod
+    def available(path: str = REF_SO) -> bool:
+        return os.path.exists(path)
+
+    def set_threads(self, n: int) -> None:
+        self.lib.ref_set_threads(C.c_uint32(n))
+
+    def threads(self) -> int:
+        return int(self.lib.ref_threads())
+
+    @staticmethod
+    def _kinds(grid: Grid):
+        return np.array(grid.boundary, np.uint8)
+
+    def solve_flow(self, grid: Grid, mobility, axis=0, mean_inlet_speed=1.0, tolerance=1e-6,
+                   max_iterations=50000):
+        kinds = self._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        mob = _f32(mobility)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        it = C.c_uint32()
+        rr = C.c_float()
+        ms = C.c_float()
+        err = C.create_string_buffer(512)
+        st = self.lib.ref_solve_depth_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(mob),
+            C.c_uint32(axis), C.c_float(mean_inlet_speed), C.c_float(tolerance),
+            C.c_uint32(max_iterations), faces.ctypes.data, C.byref(it), C.byref(rr), C.byref(ms),
+            err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        return faces, dict(iterations=it.value, relative_residual=rr.value, max_speed=ms.value)
+
+    def flow_system(self, grid: Grid, mobility, axis=0):
+        return _RefFlow(self, grid, mobility, axis)
+
+    def advance_signal_grid(self, grid: Grid, levels, dt, steps=1, diffusion=(1.0,),
+                            boundary_values=None, faces=None, source=None, loss=None):
+        kinds = self._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        diffusion = _f32(np.atleast_1d(diffusion))
+        S = diffusion.size
+        bv = np.zeros(6 * S, np.float32) if boundary_values is None else _f32(boundary_values)
+        fx = fy = fz = None
+        if faces is not None:
+            nfx, nfy, _ = grid.face_counts()
+            f = _f32(faces).reshape(-1)
+            fx, fy, fz = f[:nfx].copy(), f[nfx:nfx + nfy].copy(), f[nfx + nfy:].copy()
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        src = None
+        if loss is not None:
+            loss = _f32(loss).reshape(-1)
+            src = np.zeros_like(loss) if source is None else _f32(source).reshape(-1)
+        secs = C.c_double()
+        err = C.create_string_buffer(512)
+        st = self.lib.ref_advance_signal_grid(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), C.c_void_p(_ptr(obs)),
+            C.c_void_p(kinds.ctypes.data), C.c_uint32(S), C.c_void_p(diffusion.ctypes.data),
+            C.c_void_p(bv.ctypes.data), C.c_void_p(_ptr(fx)), C.c_void_p(_ptr(fy)),
+            C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)), C.c_void_p(_ptr(loss)), C.c_float(dt),
+            C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.byref(secs), err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        return lv, secs.value
+
+
+class _RefFlow:
+    def __init__(self, ref: Reference, grid: Grid, mobility, axis):
+        self.ref = ref
+        self.grid = grid
+        kinds = Reference._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        self._keep = [kinds, obs, _f32(mobility)]
+        err = C.create_string_buffer(512)
+        self.handle = ref.lib.ref_flow_create(
+            grid.nx, grid.ny, grid.nz, grid.dx, grid.dy, grid.dz, _ptr(obs), kinds.ctypes.data,
+            _ptr(self._keep[2]), axis, err, 512)
+        if not self.handle:
+            raise OracleError(1, err.value.decode(errors="replace"))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.ref.lib.ref_flow_destroy(self.handle)
+            self.handle = None
+
+    def arrays(self):
+        n = self.grid.nx * self.grid.ny
+        nf = (self.grid.nx + 1) * self.grid.ny + self.grid.nx * (self.grid.ny + 1) + 2 * n
+        out = {k: np.zeros(n, np.float32) for k in ("conductance", "diagonal", "rhs", "bc")}
+        fc = np.zeros(nf, np.float32)
+        fp = np.zeros(nf, np.float32)
+        self.ref.lib.ref_flow_arrays(self.handle, out["conductance"].ctypes.data,
+                                     out["diagonal"].ctypes.data, out["rhs"].ctypes.data,
+                                     out["bc"].ctypes.data, fc.ctypes.data, fp.ctypes.data)
+        out["face_conductance"] = fc
+        out["face_pressure"] = fp
+        return out
+
+    def pressure(self, tolerance=1e-6, max_iterations=50000):
+        p = np.zeros(self.grid.nx * self.grid.ny, np.float64)
+        secs = C.c_double()
+        err = C.create_string_buffer(512)
+        st = self.ref.lib.ref_flow_pressure(self.handle, tolerance, max_iterations, p.ctypes.data,
+                                            C.byref(secs), err, 512)
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        return p, secs.value
+
+    def time_iterations(self, iterations: int) -> float:
+        """Wall time of `iterations` Jacobi-PCG iterations (a solve capped there)."""
+        secs = C.c_double()
+        err = C.create_string_buffer(512)
+        self.ref.lib.ref_flow_pressure(self.handle, C.c_float(1e-30), iterations, None,
+                                       C.byref(secs), err, 512)
+        return secs.value
+
+    def apply(self, x):
+        xin = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.zeros_like(xin)
+        self.ref.lib.ref_flow_apply(self.handle, xin.ctypes.data, y.ctypes.data)
+        return y

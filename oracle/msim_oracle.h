@@ -1,0 +1,129 @@
+/*
+ * CPU oracle for the grid flow-and-transport path: a plain-C restatement of
+ * the reference's CPU algorithm, used ONLY as a checker by tests/, by the
+ * smoke check in __graft_entry__.py and by the cpu_baseline leg of bench.py.
+ * Nothing in the product (microsimulator_amd/) links, loads or calls it.
+ *
+ * Every function names the reference file:line it follows. Each float
+ * operation is written in the reference's own order and precision so that,
+ * built with -ffp-contract=off, the results are bit-identical to the
+ * reference's CPU backend (checked against oracle/_ref and tests/golden).
+ *
+ * Parity status per function (what pins it):
+ *   mo_flow_assemble        bit-exact vs reference assembly fixtures + closed forms
+ *   mo_flow_solve           bit-exact vs reference solve_depth_averaged_flow_cpu
+ *                           fixtures; plug flow 5.0, mobility-split ratio 3,
+ *                           dense-Cholesky brute force on tiny grids
+ *   mo_flow_apply           symmetry / positivity; bit-exact vs reference apply
+ *                           through the solve fixtures
+ *   mo_adr_step_f32         bit-exact vs reference advance_signal_grid_cpu
+ *                           fixtures (affine and Monod-as-affine); stencil
+ *                           closed form 0.25/0.5/0.25, reaction closed form 1.75
+ *   mo_adr_step_f64         (no reference counterpart: FP64 state) reduces to
+ *                           mo_adr_step_f32 within FP32 rounding; exact mass
+ *                           ledger identity to 1e-12
+ *   mo_adr_validate_step    the reference's explicit bound, known answers from
+ *                           the reference tests (dt 0.25 ok / 0.51 rejected)
+ */
+#ifndef MSIM_ORACLE_H
+#define MSIM_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the same numbering as the product C-ABI (include/msim_hip.h). */
+enum {
+  MO_OK = 0,
+  MO_INVALID_ARGUMENT = 1,
+  MO_RUNTIME_ERROR = 2,
+  MO_NOT_CONVERGED = 3,
+  MO_OVERFLOW = 4,
+  MO_DOMAIN_ERROR = 5,
+  MO_CG_CURVATURE = 11,
+  MO_CG_STEP = 12,
+  MO_CG_RESIDUAL = 13,
+  MO_CG_BREAKDOWN = 14,
+  MO_CG_DIRECTION = 15
+};
+
+enum { MO_NO_FLUX = 0, MO_PERIODIC = 1, MO_FIXED = 2 };
+
+typedef struct {
+  uint32_t nx, ny, nz;
+  float dx, dy, dz;
+  const uint8_t* obstacles; /* nx*ny*nz, or NULL for all fluid */
+  /* boundary kinds in the order x_lower, x_upper, y_lower, y_upper, z_lower, z_upper */
+  uint8_t boundary[6];
+} mo_grid;
+
+typedef struct {
+  uint32_t iterations;
+  double relative_residual;      /* recursive residual at exit (reference report) */
+  double true_relative_residual; /* ||b - A x|| / ||b|| recomputed in FP64 */
+  float mean_inlet_speed;
+  float max_speed;
+  float solved_mean; /* mean inlet speed of the unscaled field */
+  float factor;      /* scaling factor applied */
+} mo_flow_report;
+
+/* The structured-grid form of the depth-averaged system (nz = 1):
+ * per site: column conductance, diagonal, rhs, boundary conductance;
+ * per face (x|y|z concatenated, reference face order): face conductance and
+ * boundary pressure. Arrays are caller-allocated: N = nx*ny, F = total faces. */
+int mo_flow_assemble(const mo_grid* grid, const float* mobility, uint32_t axis,
+                     float* conductance, float* diagonal, float* rhs, float* boundary_conductance,
+                     float* face_conductance, float* face_pressure, char* err, size_t errlen);
+
+/* y = A x over the whole structured grid (inactive sites give 0), in FP64,
+ * following DepthAveragedFlowSystem::apply's per-site arithmetic. */
+int mo_flow_apply(const mo_grid* grid, const float* mobility, uint32_t axis, const double* x,
+                  double* y, char* err, size_t errlen);
+
+/* Full depth-averaged solve: reduction, assembly, Jacobi-PCG, velocity,
+ * lift and scaling. faces: x|y|z face velocities of the original grid.
+ * pressure: N doubles (zero at inactive sites), may be NULL.
+ * scale = 0 returns the unscaled (Dirichlet 1/0) field. */
+int mo_flow_solve(const mo_grid* grid, const float* mobility, uint32_t axis,
+                  float mean_inlet_speed, float tolerance, uint32_t max_iterations, int scale,
+                  float* faces, double* pressure, mo_flow_report* report, char* err,
+                  size_t errlen);
+
+/* Signal-grid transport description for one explicit step. */
+typedef struct {
+  mo_grid grid;
+  uint32_t signal_count;
+  const float* diffusion;       /* S */
+  const float* advection;       /* 3*S constant velocity, used when faces are NULL */
+  const float* boundary_values; /* 6*S reservoir values of fixed faces (others ignored) */
+  const float* x_faces;         /* velocity field (nx+1)*ny*nz, or NULL */
+  const float* y_faces;         /* nx*(ny+1)*nz */
+  const float* z_faces;         /* nx*ny*(nz+1) */
+  const float* source_rates;    /* affine reaction S*N, or NULL */
+  const float* loss_rates;      /* S*N, or NULL */
+  const float* phi;             /* Monod occupancy N, or NULL (no Monod) */
+  const float* monod_vmax;      /* S */
+  const float* monod_k;         /* S */
+} mo_signal_grid;
+
+/* The explicit stability bound of SignalGrid::validate_step. For Monod the
+ * loss bound is vmax * max(phi) / K. Returns MO_OK or MO_INVALID_ARGUMENT. */
+int mo_adr_validate_step(const mo_signal_grid* sg, float dt, char* err, size_t errlen);
+
+/* One forward-Euler step in FP32, levels S*N in place (reference order). */
+int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err, size_t errlen);
+
+/* One forward-Euler step with FP64 state in face-flux form. ledger (may be
+ * NULL) receives per signal: [mass_before, mass_after, boundary_inflow*dt,
+ * uptake*dt] with mass = sum(c) * voxel volume. */
+int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* ledger, char* err,
+                    size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

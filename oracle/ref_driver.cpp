@@ -1,0 +1,260 @@
+// C entry points over the reference's own CPU path, compiled against the
+// reference sources where they lie (see oracle/Makefile, target `ref`) into
+// oracle/_ref/libmsim_ref.so. Test infrastructure only: tests/ use it to pin
+// the C oracle, tests/golden/make_golden.py writes fixtures from it, and
+// bench.py may time it as the CPU baseline. Nothing in the product loads it.
+//
+// Reference functions called:
+//   solve_depth_averaged_flow_cpu        cpp/cpu/cpu_flow.cpp:338-351
+//   detail::reduce_shallow_flow          cpp/core/flow_system.hpp:1255-1266
+//   detail::axis_flow_boundary           cpp/core/flow_system.hpp:186-201
+//   detail::DepthAveragedFlowSystem      cpp/core/flow_system.hpp:520-748
+//   detail::solve_depth_averaged_pressure cpp/cpu/cpu_flow.cpp:172-175
+//   SignalGrid + advance_signal_grid_cpu cpp/core/signals.cpp:1377-1389
+//   set_cpu_thread_count                 cpp/include/microsimulator/threads.hpp:19
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "core/flow_system.hpp"
+#include "microsimulator/flow.hpp"
+#include "microsimulator/signals.hpp"
+#include "microsimulator/threads.hpp"
+
+namespace ms = microsimulator;
+
+namespace {
+
+// Status numbering shared with oracle/msim_oracle.h.
+constexpr int kOk = 0, kInvalid = 1, kRuntime = 2, kOverflow = 4, kDomain = 5;
+
+int report_error(const std::exception& error, int code, char* err, std::size_t errlen) {
+  if (err != nullptr && errlen > 0) std::snprintf(err, errlen, "%s", error.what());
+  return code;
+}
+
+template <typename Body>
+int guarded(char* err, std::size_t errlen, Body&& body) {
+  try {
+    body();
+    return kOk;
+  } catch (const std::invalid_argument& e) {
+    return report_error(e, kInvalid, err, errlen);
+  } catch (const std::overflow_error& e) {
+    return report_error(e, kOverflow, err, errlen);
+  } catch (const std::domain_error& e) {
+    return report_error(e, kDomain, err, errlen);
+  } catch (const std::exception& e) {
+    return report_error(e, kRuntime, err, errlen);
+  }
+}
+
+ms::GridBoundary boundary_of(std::uint8_t kind, std::uint32_t signals, const float* values) {
+  ms::GridBoundary boundary;
+  boundary.kind = static_cast<ms::GridBoundaryKind>(kind);
+  if (boundary.kind == ms::GridBoundaryKind::fixed) {
+    boundary.values.assign(signals, 0.0F);
+    if (values != nullptr) {
+      for (std::uint32_t s = 0; s < signals; ++s) boundary.values[s] = values[s];
+    }
+  }
+  return boundary;
+}
+
+// boundary_values: 6*S, face-major as in mo_signal_grid (face f, signal s at
+// [6*s + f]); may be null for zero reservoirs.
+ms::SignalGridSpec make_spec(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                             float dy, float dz, const std::uint8_t* obstacles,
+                             const std::uint8_t* kinds, std::uint32_t signals,
+                             const float* boundary_values) {
+  ms::SignalGridSpec spec;
+  spec.signal_count = signals;
+  spec.shape = {.x = nx, .y = ny, .z = nz};
+  spec.spacing = {dx, dy, dz};
+  spec.diffusion.assign(signals, 0.0F);
+  spec.advection.assign(signals, ms::Vec3{});
+  if (obstacles != nullptr) {
+    spec.obstacles.assign(obstacles, obstacles + static_cast<std::size_t>(nx) * ny * nz);
+  }
+  ms::GridBoundary* faces[6] = {&spec.x_lower, &spec.x_upper, &spec.y_lower,
+                                &spec.y_upper, &spec.z_lower, &spec.z_upper};
+  for (int f = 0; f < 6; ++f) {
+    std::vector<float> values(signals, 0.0F);
+    if (boundary_values != nullptr) {
+      for (std::uint32_t s = 0; s < signals; ++s) values[s] = boundary_values[6 * s + f];
+    }
+    *faces[f] = boundary_of(kinds[f], signals, values.data());
+  }
+  return spec;
+}
+
+struct FlowHandle {
+  ms::SignalGridSpec spec;
+  std::vector<float> mobility;
+  std::unique_ptr<ms::detail::ShallowFlowReduction> reduction;
+  std::unique_ptr<ms::detail::FlowBoundary> boundary;
+  std::unique_ptr<ms::detail::DepthAveragedFlowSystem> system;
+};
+
+}  // namespace
+
+extern "C" {
+
+void ref_set_threads(std::uint32_t count) {
+  ms::set_cpu_thread_count(count);
+}
+
+std::uint32_t ref_threads(void) {
+  return static_cast<std::uint32_t>(ms::cpu_thread_count());
+}
+
+// solve_depth_averaged_flow_cpu on a one-signal grid with the given boundary
+// kinds. faces: x|y|z of the grid. The reference refuses tolerances below
+// float epsilon.
+int ref_solve_depth_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
+                         float dz, const std::uint8_t* obstacles, const std::uint8_t* kinds,
+                         const float* mobility, std::uint32_t axis, float mean_inlet_speed,
+                         float tolerance, std::uint32_t max_iterations, float* faces,
+                         std::uint32_t* iterations, float* relative_residual, float* max_speed,
+                         char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> mob;
+    if (mobility != nullptr) mob = std::span<const float>(mobility, n);
+    ms::DepthAveragedFlowParameters parameters;
+    parameters.mean_inlet_speed = mean_inlet_speed;
+    parameters.axis = static_cast<ms::FlowAxis>(axis);
+    parameters.relative_tolerance = tolerance;
+    parameters.max_iterations = max_iterations;
+    const auto result = ms::solve_depth_averaged_flow_cpu(spec, mob, parameters);
+    std::memcpy(faces, result.field.x_faces.data(), result.field.x_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size(), result.field.y_faces.data(),
+                result.field.y_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size() + result.field.y_faces.size(),
+                result.field.z_faces.data(), result.field.z_faces.size() * sizeof(float));
+    *iterations = result.report.iterations;
+    *relative_residual = result.report.relative_residual;
+    *max_speed = result.report.max_speed;
+  });
+}
+
+// Builds the reduced, assembled depth-averaged system once (as the reference
+// does before each solve) and keeps it for repeated solves.
+void* ref_flow_create(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
+                      float dz, const std::uint8_t* obstacles, const std::uint8_t* kinds,
+                      const float* mobility, std::uint32_t axis, char* err, std::size_t errlen) {
+  auto handle = std::make_unique<FlowHandle>();
+  const int status = guarded(err, errlen, [&] {
+    handle->spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    if (mobility != nullptr) handle->mobility.assign(mobility, mobility + n);
+    handle->reduction = std::make_unique<ms::detail::ShallowFlowReduction>(
+        ms::detail::reduce_shallow_flow(handle->spec, handle->mobility,
+                                        static_cast<ms::FlowAxis>(axis)));
+    handle->boundary = std::make_unique<ms::detail::FlowBoundary>(
+        ms::detail::axis_flow_boundary(handle->reduction->reduced_layout(), axis));
+    handle->system = std::make_unique<ms::detail::DepthAveragedFlowSystem>(
+        handle->reduction->grid(), handle->reduction->conductance(), *handle->boundary);
+  });
+  return status == kOk ? handle.release() : nullptr;
+}
+
+void ref_flow_destroy(void* handle) {
+  delete static_cast<FlowHandle*>(handle);
+}
+
+// The assembled per-site arrays of the reduced grid (nz = 1 reduced grid).
+int ref_flow_arrays(void* opaque, float* conductance, float* diagonal, float* rhs,
+                    float* boundary_conductance, float* face_conductance, float* face_pressure) {
+  const auto& h = *static_cast<FlowHandle*>(opaque);
+  const auto& s = *h.system;
+  const auto& c = h.reduction->conductance();
+  std::memcpy(conductance, c.data(), c.size() * sizeof(float));
+  std::memcpy(diagonal, s.diagonal().data(), s.diagonal().size() * sizeof(float));
+  std::memcpy(rhs, s.right_hand_side().data(), s.right_hand_side().size() * sizeof(float));
+  std::memcpy(boundary_conductance, s.boundary_conductance().data(),
+              s.boundary_conductance().size() * sizeof(float));
+  std::memcpy(face_conductance, s.face_conductance().data(),
+              s.face_conductance().size() * sizeof(float));
+  std::memcpy(face_pressure, s.face_pressure().data(), s.face_pressure().size() * sizeof(float));
+  return kOk;
+}
+
+// Jacobi-PCG on the kept system through detail::solve_depth_averaged_pressure
+// (no tolerance floor). pressure: one double per reduced site. seconds: wall
+// time of the solve alone. A solve that stops at max_iterations returns
+// kRuntime with the reference's "did not converge" message.
+int ref_flow_pressure(void* opaque, float tolerance, std::uint32_t max_iterations,
+                      double* pressure, double* seconds, char* err, std::size_t errlen) {
+  auto& h = *static_cast<FlowHandle*>(opaque);
+  const auto start = std::chrono::steady_clock::now();
+  const int status = guarded(err, errlen, [&] {
+    const auto result =
+        ms::detail::solve_depth_averaged_pressure(*h.system, tolerance, max_iterations);
+    if (pressure != nullptr) std::memcpy(pressure, result.data(), result.size() * sizeof(double));
+  });
+  const auto stop = std::chrono::steady_clock::now();
+  if (seconds != nullptr) *seconds = std::chrono::duration<double>(stop - start).count();
+  return status;
+}
+
+// DepthAveragedFlowSystem::apply over the active sites, scattered back to the
+// full reduced grid (zero at inactive sites).
+int ref_flow_apply(void* opaque, const double* input, double* output) {
+  const auto& h = *static_cast<FlowHandle*>(opaque);
+  const auto& active = h.system->active();
+  std::vector<double> in(active.size()), out;
+  for (std::size_t i = 0; i < active.size(); ++i) in[i] = input[active.sites[i]];
+  h.system->apply(in, out);
+  const auto full = h.system->scatter(out);
+  std::memcpy(output, full.data(), full.size() * sizeof(double));
+  return kOk;
+}
+
+// `steps` steps of advance_signal_grid_cpu. levels: S*N in place.
+// velocity faces may be null (no field); source/loss may be null (no
+// reaction). boundary_values: 6*S as in make_spec. seconds: time of the steps.
+int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                            float dy, float dz, const std::uint8_t* obstacles,
+                            const std::uint8_t* kinds, std::uint32_t signals,
+                            const float* diffusion, const float* boundary_values,
+                            const float* x_faces, const float* y_faces, const float* z_faces,
+                            const float* source_rates, const float* loss_rates, float dt,
+                            std::uint32_t steps, float* levels, double* seconds, char* err,
+                            std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    spec.diffusion.assign(diffusion, diffusion + signals);
+    if (x_faces != nullptr) {
+      ms::SignalGridVelocityField field;
+      field.x_faces.assign(x_faces, x_faces + (nx + 1) * static_cast<std::size_t>(ny) * nz);
+      field.y_faces.assign(y_faces, y_faces + static_cast<std::size_t>(nx) * (ny + 1) * nz);
+      field.z_faces.assign(z_faces, z_faces + static_cast<std::size_t>(nx) * ny * (nz + 1));
+      spec.velocity_field = std::move(field);
+    }
+    if (loss_rates != nullptr) {
+      ms::SignalGridAffineReaction reaction;
+      reaction.source_rates.assign(source_rates, source_rates + signals * n);
+      reaction.loss_rates.assign(loss_rates, loss_rates + signals * n);
+      spec.reaction = std::move(reaction);
+    }
+    ms::SignalGrid grid(spec, std::vector<float>(levels, levels + signals * n));
+    const auto start = std::chrono::steady_clock::now();
+    for (std::uint32_t step = 0; step < steps; ++step) {
+      static_cast<void>(ms::advance_signal_grid_cpu(grid, dt));
+    }
+    const auto stop = std::chrono::steady_clock::now();
+    if (seconds != nullptr) *seconds = std::chrono::duration<double>(stop - start).count();
+    const auto out = grid.levels();
+    std::memcpy(levels, out.data(), out.size() * sizeof(float));
+  });
+}
+
+}  // extern "C"
