@@ -1,0 +1,223 @@
+/*
+ * msim_hip.h -- C ABI of the MI355X (gfx950) backend for MicroSimulator's
+ * grid flow-and-transport path: the depth-averaged (Hele-Shaw/Darcy) pressure
+ * solve, face-velocity reconstruction, and the explicit conservative
+ * advection-diffusion-reaction step.
+ *
+ * The entry points are what a `hip` ComputeBackend in the reference would
+ * bind (cpp/include/microsimulator/backend.hpp:21-56). Plain pointers and
+ * sizes only; nothing throws across this boundary. Every call returns an
+ * msim_status; msim_hip_last_error() gives the message of the last failure on
+ * a context. Arrays are caller-owned host memory and are never retained after
+ * a call returns; device memory belongs to the context.
+ *
+ * Storage follows the reference: site = (x*ny + y)*nz + z; face arrays
+ * x: (fx*ny + y)*nz + z, y: (x*(ny+1) + fy)*nz + z, z: (x*ny + y)*(nz+1) + fz
+ * (cpp/core/flow_system.hpp:64-96). Levels are signal-major, s*N + site.
+ *
+ * A context is single-threaded: calls on one context must not overlap.
+ * Different contexts may be used from different threads.
+ */
+#ifndef MSIM_HIP_H
+#define MSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. They map onto the exception types the reference's backends
+ * throw (cpp/cpu/cpu_flow.cpp, cpp/core/signals.cpp, cpp/cuda/cuda_backend.cu):
+ * invalid_argument, runtime_error, overflow_error, domain_error, and the
+ * conjugate-gradient failure kinds 1-5 (cpp/core/conjugate_gradient.hpp:91-98)
+ * offset by 10. */
+typedef enum {
+  MSIM_OK = 0,
+  MSIM_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+  MSIM_RUNTIME_ERROR = 2,    /* std::runtime_error (e.g. outlet unreachable) */
+  MSIM_NOT_CONVERGED = 3,    /* runtime_error "conjugate gradient did not converge" */
+  MSIM_OVERFLOW = 4,         /* std::overflow_error (uint32 index space) */
+  MSIM_DOMAIN_ERROR = 5,     /* negative or non-finite concentration */
+  MSIM_DEVICE_ERROR = 6,     /* a HIP runtime call failed */
+  MSIM_NO_DEVICE = 7,        /* no such device */
+  MSIM_UNSUPPORTED = 8,      /* outside what this backend implements */
+  MSIM_CG_CURVATURE = 11,    /* ConjugateGradientFailure::curvature */
+  MSIM_CG_STEP = 12,         /* ::step */
+  MSIM_CG_RESIDUAL = 13,     /* ::residual */
+  MSIM_CG_BREAKDOWN = 14,    /* ::breakdown */
+  MSIM_CG_DIRECTION = 15     /* ::direction */
+} msim_status;
+
+/* GridBoundaryKind, cpp/include/microsimulator/signals.hpp:17-21. */
+enum { MSIM_NO_FLUX = 0, MSIM_PERIODIC = 1, MSIM_FIXED = 2 };
+
+/* Pressure solver forms. */
+enum {
+  MSIM_SOLVER_JACOBI_FP64 = 0, /* the reference's FP64 Jacobi-PCG, on the device */
+  MSIM_SOLVER_MIXED = 1,       /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+  MSIM_SOLVER_LEARNED = 2      /* learned multilevel preconditioner in flexible CG */
+};
+
+typedef struct msim_ctx msim_ctx;
+
+/* The lattice of a SignalGridSpec (signals.hpp:122-156): shape, spacing,
+ * obstacle mask (uint8 0/1 per site, or NULL) and the six boundary kinds in
+ * the order x_lower, x_upper, y_lower, y_upper, z_lower, z_upper. */
+typedef struct {
+  uint32_t nx, ny, nz;
+  float dx, dy, dz;
+  const uint8_t* obstacles;
+  uint8_t boundary[6];
+} msim_grid;
+
+/* DepthAveragedFlowParameters (cpp/include/microsimulator/flow.hpp:18-28)
+ * plus the controls of the device solver forms. */
+typedef struct {
+  float mean_inlet_speed;      /* target mean speed over open inlet faces */
+  uint32_t axis;               /* FlowAxis: 0 = x, 1 = y */
+  double relative_tolerance;   /* on ||b - Ax|| / ||b|| */
+  uint32_t max_iterations;     /* Krylov iterations, summed over inner solves */
+  uint32_t solver;             /* MSIM_SOLVER_* */
+  uint32_t scale;              /* 1: scale to mean_inlet_speed as the reference does;
+                                  0: return the raw Dirichlet (1 -> 0) field */
+  uint32_t check_interval;     /* iterations between host convergence checks; 0 = 64 */
+  double inner_tolerance;      /* MIXED/LEARNED: inner FP32 relative tolerance; 0 = auto */
+  uint32_t fallback_iterations;/* LEARNED: iteration cap before the Jacobi-PCG fallback */
+  uint32_t profile;            /* 1: time every Krylov kernel launch with events */
+} msim_flow_params;
+
+/* DepthAveragedFlowReport (flow.hpp:30-35) plus what the device forms add. */
+typedef struct {
+  uint32_t iterations;           /* Krylov iterations, all inner solves */
+  uint32_t refinements;          /* FP64 refinement steps (MIXED/LEARNED) */
+  double relative_residual;      /* recursive residual where the solve stopped */
+  double true_relative_residual; /* ||b - Ax|| / ||b|| recomputed in FP64 */
+  float mean_inlet_speed;
+  float max_speed;
+  float solved_mean;             /* mean inlet speed of the unscaled field */
+  float factor;                  /* scale applied to the unscaled field */
+  uint32_t fell_back;            /* LEARNED: 1 when Jacobi-PCG finished the solve */
+  double seconds;                /* device time of the solve and reconstruction */
+  /* With params.profile: summed event time and launch count of the two
+   * Krylov kernels (operator+direction, and update+dots). */
+  double apply_seconds;
+  uint32_t apply_launches;
+  double update_seconds;
+  uint32_t update_launches;
+} msim_flow_report;
+
+/* Device count through hipGetDeviceCount; 0 on error, as
+ * cuda_backend_device_count does (cpp/cuda/cuda_backend.cu:1533-1541).
+ * Replaces backend_device_count(BackendKind::cuda), backend.hpp:62. */
+int msim_hip_device_count(void);
+
+/* Creates a context on one device (one stream, default priority). Replaces
+ * make_cuda_backend(device_index), cpp/include/microsimulator/backend.hpp:60. */
+int msim_hip_create(int device, msim_ctx** out);
+
+/* Frees the context and all device memory it holds. */
+void msim_hip_destroy(msim_ctx* ctx);
+
+/* Message of the last failure on ctx ("" when none). */
+const char* msim_hip_last_error(const msim_ctx* ctx);
+
+/* BackendInfo::device (types.hpp:77-83): the device's name. */
+int msim_hip_device_name(msim_ctx* ctx, char* name, size_t length);
+
+/* ComputeBackend::solve_depth_averaged_flow (backend.hpp:44-46): host arrays
+ * in, the velocity field out. mobility: one float per site or NULL (the
+ * reference's Hele-Shaw default). x/y/z_faces: face velocities. The grid must
+ * be one layer (nz = 1). Equivalent to prepare + solve + download. */
+int msim_hip_solve_depth_averaged_flow(msim_ctx* ctx, const msim_grid* grid,
+                                       const float* mobility, const msim_flow_params* params,
+                                       float* x_faces, float* y_faces, float* z_faces,
+                                       msim_flow_report* report);
+
+/* The same solve split so that the system stays on the device between
+ * solves (as cuda_backend.cu's sync_grid keeps a grid's arrays):
+ *  prepare  -- validates (validate_flow_grid, cpu_flow.cpp:288-336), uploads
+ *              the mask and mobility, assembles the operator on the device
+ *              (flow_system.hpp:522-636);
+ *  solve    -- pressure solve, FP64 true residual, velocity reconstruction,
+ *              lift and scaling, all on the device;
+ *  download -- pressure (N doubles, zero at inactive sites) and/or faces
+ *              (any pointer may be NULL). */
+int msim_hip_flow_prepare(msim_ctx* ctx, const msim_grid* grid, const float* mobility,
+                          uint32_t axis);
+int msim_hip_flow_solve(msim_ctx* ctx, const msim_flow_params* params, msim_flow_report* report);
+int msim_hip_flow_download(msim_ctx* ctx, double* pressure, float* x_faces, float* y_faces,
+                           float* z_faces);
+
+/* DepthAveragedFlowSystem::apply (flow_system.hpp:670-695) on the prepared
+ * system: y = A x for host vectors of N doubles (zero rows at inactive
+ * sites). For checking the device operator. */
+int msim_hip_flow_apply(msim_ctx* ctx, const double* x, double* y);
+
+/* The assembled per-site arrays of the prepared system (N floats each, any
+ * pointer may be NULL): column conductance, Jacobi diagonal, right-hand side
+ * and boundary conductance (flow_system.hpp:549-599). */
+int msim_hip_flow_arrays(msim_ctx* ctx, float* conductance, float* diagonal, float* rhs,
+                         float* boundary_conductance);
+
+/* A signal grid for the explicit transport step: SignalGridSpec's transport
+ * fields (signals.hpp:122-156) flattened. boundary_values: 6*S reservoir
+ * values, face f of signal s at [6*s + f] (uniform per face). Velocity field
+ * faces may be NULL (no advection). Affine reaction (SignalGridAffineReaction,
+ * signals.hpp:109-114) may be NULL. Monod uptake -vmax*phi*c/(K+c), with phi
+ * per site and vmax, K per signal, may be NULL. revision: SignalGrid::revision
+ * (signals.hpp:217-221); the device copies of the spec arrays are rewritten
+ * only when it changes. */
+typedef struct {
+  msim_grid grid;
+  uint32_t signal_count;
+  const float* diffusion;
+  const float* boundary_values;
+  const float* x_faces;
+  const float* y_faces;
+  const float* z_faces;
+  const float* source_rates;
+  const float* loss_rates;
+  const float* phi;
+  const float* monod_vmax;
+  const float* monod_k;
+  uint64_t revision;
+} msim_signal_grid;
+
+/* SignalSolveReport (signals.hpp:57-61). */
+typedef struct {
+  uint32_t converged;
+  uint32_t iterations;
+  float residual_rms;
+} msim_signal_report;
+
+/* ComputeBackend::advance_signal_grid (backend.hpp:30) for the explicit
+ * (forward Euler, upwind) step: levels S*N in place. The step is checked
+ * against the explicit stability bound first (SignalGrid::validate_step,
+ * signals.cpp:1066-1102); a refused or failed step leaves levels unchanged. */
+int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, float dt,
+                                 float* levels, msim_signal_report* report);
+
+/* Device-resident transport for many coupled steps.
+ *  adr_upload   -- spec arrays and initial levels (float when precision is
+ *                  32, double when 64). With use_flow_field != 0 the velocity
+ *                  field is the one the last flow solve left on the device.
+ *  adr_run      -- `steps` explicit steps of dt. ledger (steps*S*4 doubles,
+ *                  or NULL) receives per step and signal: mass before, mass
+ *                  after, boundary supply and uptake, each in FP64 (mass =
+ *                  sum of levels x voxel volume).
+ *  adr_download -- levels in the uploaded precision. */
+int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void* levels,
+                        uint32_t precision, uint32_t use_flow_field);
+int msim_hip_adr_run(msim_ctx* ctx, float dt, uint32_t steps, double* ledger);
+int msim_hip_adr_download(msim_ctx* ctx, void* levels);
+
+/* Device time (seconds) of the kernels of the last adr_run. */
+double msim_hip_adr_seconds(msim_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSIM_HIP_H */

@@ -1,0 +1,508 @@
+"""MI355X backend for MicroSimulator's grid flow-and-transport path.
+
+The Python side mirrors the reference's backend interface for this path
+(``ComputeBackend``, cpp/include/microsimulator/backend.hpp:21-56): a
+:class:`HipBackend` with ``info``, ``supports``, ``solve_depth_averaged_flow``
+(backend.hpp:44-46) and ``advance_signal_grid`` (backend.hpp:30), and the
+device-resident calls that keep a system on the GPU between solves. Every call
+goes through the C ABI of ``libmsim_hip.so`` (include/msim_hip.h), whose
+kernels are hand-written HIP for gfx950. There is no CPU fallback: without the
+library, or without a device, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libmsim_hip.so")
+
+NO_FLUX, PERIODIC, FIXED = 0, 1, 2
+SOLVER_JACOBI_FP64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
+
+
+# --------------------------------------------------------------------------
+# Errors: the exception kinds the reference's backends throw, by status.
+# --------------------------------------------------------------------------
+class MsimError(RuntimeError):
+    status = 2
+
+
+class InvalidArgument(MsimError, ValueError):
+    """std::invalid_argument."""
+    status = 1
+
+
+class NotConverged(MsimError):
+    """runtime_error "... conjugate gradient did not converge"."""
+    status = 3
+
+
+class OverflowIndex(MsimError, OverflowError):
+    status = 4
+
+
+class DomainError(MsimError, ArithmeticError):
+    """A transport step produced a negative or non-finite concentration."""
+    status = 5
+
+
+class DeviceError(MsimError):
+    status = 6
+
+
+class NoDevice(MsimError):
+    status = 7
+
+
+class Unsupported(MsimError, NotImplementedError):
+    status = 8
+
+
+class ConjugateGradientFailure(MsimError):
+    """ConjugateGradientFailure 1-5 (cpp/core/conjugate_gradient.hpp:91-98)."""
+
+    def __init__(self, message, kind):
+        super().__init__(message)
+        self.kind = kind
+
+
+_BY_STATUS = {1: InvalidArgument, 2: MsimError, 3: NotConverged, 4: OverflowIndex,
+              5: DomainError, 6: DeviceError, 7: NoDevice, 8: Unsupported}
+_CG_KINDS = {11: "curvature", 12: "step", 13: "residual", 14: "breakdown", 15: "direction"}
+
+
+# --------------------------------------------------------------------------
+# ctypes mirror of include/msim_hip.h
+# --------------------------------------------------------------------------
+class _Grid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+                ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+                ("obstacles", C.c_void_p), ("boundary", C.c_uint8 * 6)]
+
+
+class _FlowParams(C.Structure):
+    _fields_ = [("mean_inlet_speed", C.c_float), ("axis", C.c_uint32),
+                ("relative_tolerance", C.c_double), ("max_iterations", C.c_uint32),
+                ("solver", C.c_uint32), ("scale", C.c_uint32), ("check_interval", C.c_uint32),
+                ("inner_tolerance", C.c_double), ("fallback_iterations", C.c_uint32),
+                ("profile", C.c_uint32)]
+
+
+class _FlowReport(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("refinements", C.c_uint32),
+                ("relative_residual", C.c_double), ("true_relative_residual", C.c_double),
+                ("mean_inlet_speed", C.c_float), ("max_speed", C.c_float),
+                ("solved_mean", C.c_float), ("factor", C.c_float), ("fell_back", C.c_uint32),
+                ("seconds", C.c_double), ("apply_seconds", C.c_double),
+                ("apply_launches", C.c_uint32), ("update_seconds", C.c_double),
+                ("update_launches", C.c_uint32)]
+
+
+class _SignalGrid(C.Structure):
+    _fields_ = [("grid", _Grid), ("signal_count", C.c_uint32), ("diffusion", C.c_void_p),
+                ("boundary_values", C.c_void_p), ("x_faces", C.c_void_p),
+                ("y_faces", C.c_void_p), ("z_faces", C.c_void_p), ("source_rates", C.c_void_p),
+                ("loss_rates", C.c_void_p), ("phi", C.c_void_p), ("monod_vmax", C.c_void_p),
+                ("monod_k", C.c_void_p), ("revision", C.c_uint64)]
+
+
+class _SignalReport(C.Structure):
+    _fields_ = [("converged", C.c_uint32), ("iterations", C.c_uint32),
+                ("residual_rms", C.c_float)]
+
+
+#: Every entry point include/msim_hip.h declares.
+EXPORTS = (
+    "msim_hip_device_count", "msim_hip_create", "msim_hip_destroy", "msim_hip_last_error",
+    "msim_hip_device_name", "msim_hip_solve_depth_averaged_flow", "msim_hip_flow_prepare",
+    "msim_hip_flow_solve", "msim_hip_flow_download", "msim_hip_flow_apply",
+    "msim_hip_flow_arrays", "msim_hip_advance_signal_grid", "msim_hip_adr_upload",
+    "msim_hip_adr_run", "msim_hip_adr_download", "msim_hip_adr_seconds",
+)
+
+_lib = None
+
+
+def library(path: str = LIBRARY):
+    """Loads libmsim_hip.so (raising if it was not built) and declares the ABI."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m microsimulator_amd.build`")
+    L = C.CDLL(path)
+    P, U, I, D = C.c_void_p, C.c_uint32, C.c_int, C.c_double
+    sig = {
+        "msim_hip_device_count": (I, []),
+        "msim_hip_create": (I, [I, C.POINTER(P)]),
+        "msim_hip_destroy": (None, [P]),
+        "msim_hip_last_error": (C.c_char_p, [P]),
+        "msim_hip_device_name": (I, [P, C.c_char_p, C.c_size_t]),
+        "msim_hip_solve_depth_averaged_flow": (I, [P, P, P, P, P, P, P, P]),
+        "msim_hip_flow_prepare": (I, [P, P, P, U]),
+        "msim_hip_flow_solve": (I, [P, P, P]),
+        "msim_hip_flow_download": (I, [P, P, P, P, P]),
+        "msim_hip_flow_apply": (I, [P, P, P]),
+        "msim_hip_flow_arrays": (I, [P, P, P, P, P]),
+        "msim_hip_advance_signal_grid": (I, [P, P, C.c_float, P, P]),
+        "msim_hip_adr_upload": (I, [P, P, P, U, U]),
+        "msim_hip_adr_run": (I, [P, C.c_float, U, P]),
+        "msim_hip_adr_download": (I, [P, P]),
+        "msim_hip_adr_seconds": (D, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def device_count() -> int:
+    """backend_device_count(BackendKind::hip); 0 when no device or no runtime."""
+    return int(library().msim_hip_device_count())
+
+
+# --------------------------------------------------------------------------
+# Host-side mirror of the reference's types for this path
+# --------------------------------------------------------------------------
+@dataclass
+class Grid:
+    """The lattice of a SignalGridSpec (signals.hpp:122-156), one layer."""
+    nx: int
+    ny: int
+    nz: int = 1
+    dx: float = 1.0
+    dy: float = 1.0
+    dz: float = 1.0
+    obstacles: np.ndarray | None = None
+    boundary: tuple = (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX)
+
+    @property
+    def sites(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def face_counts(self):
+        return ((self.nx + 1) * self.ny * self.nz, self.nx * (self.ny + 1) * self.nz,
+                self.nx * self.ny * (self.nz + 1))
+
+    def _c(self, keep: list) -> _Grid:
+        g = _Grid()
+        g.nx, g.ny, g.nz = self.nx, self.ny, self.nz
+        g.dx, g.dy, g.dz = self.dx, self.dy, self.dz
+        if self.obstacles is not None:
+            obs = np.ascontiguousarray(self.obstacles, dtype=np.uint8).reshape(-1)
+            if obs.size != self.sites:
+                raise InvalidArgument("signal grid obstacle mask must cover every site")
+            keep.append(obs)
+            g.obstacles = obs.ctypes.data
+        for i, k in enumerate(self.boundary):
+            g.boundary[i] = k
+        return g
+
+
+@dataclass
+class DepthAveragedFlowParameters:
+    """DepthAveragedFlowParameters (flow.hpp:18-28) plus the device solver forms."""
+    mean_inlet_speed: float = 1.0
+    axis: int = 0
+    relative_tolerance: float = 1.0e-6
+    max_iterations: int = 50_000
+    solver: int = SOLVER_JACOBI_FP64
+    scale: bool = True
+    check_interval: int = 64
+    inner_tolerance: float = 0.0
+    fallback_iterations: int = 200
+    profile: bool = False
+
+    def _c(self) -> _FlowParams:
+        p = _FlowParams()
+        p.mean_inlet_speed = self.mean_inlet_speed
+        p.axis = self.axis
+        p.relative_tolerance = self.relative_tolerance
+        p.max_iterations = self.max_iterations
+        p.solver = self.solver
+        p.scale = 1 if self.scale else 0
+        p.check_interval = self.check_interval
+        p.inner_tolerance = self.inner_tolerance
+        p.fallback_iterations = self.fallback_iterations
+        p.profile = 1 if self.profile else 0
+        return p
+
+
+@dataclass
+class DepthAveragedFlowReport:
+    iterations: int
+    refinements: int
+    relative_residual: float
+    true_relative_residual: float
+    mean_inlet_speed: float
+    max_speed: float
+    solved_mean: float
+    factor: float
+    fell_back: bool
+    seconds: float
+    apply_seconds: float = 0.0
+    apply_launches: int = 0
+    update_seconds: float = 0.0
+    update_launches: int = 0
+
+    @classmethod
+    def _from(cls, r: _FlowReport):
+        return cls(r.iterations, r.refinements, r.relative_residual, r.true_relative_residual,
+                   r.mean_inlet_speed, r.max_speed, r.solved_mean, r.factor, bool(r.fell_back),
+                   r.seconds, r.apply_seconds, r.apply_launches, r.update_seconds,
+                   r.update_launches)
+
+
+@dataclass
+class VelocityField:
+    """SignalGridVelocityField (signals.hpp:116-120)."""
+    x_faces: np.ndarray
+    y_faces: np.ndarray
+    z_faces: np.ndarray
+
+    def concatenated(self) -> np.ndarray:
+        return np.concatenate([self.x_faces, self.y_faces, self.z_faces])
+
+
+@dataclass
+class DepthAveragedFlowResult:
+    field: VelocityField
+    report: DepthAveragedFlowReport
+
+
+@dataclass
+class SignalGridSpec:
+    """The transport fields of SignalGridSpec (signals.hpp:122-156) this
+    backend reads, plus Monod uptake -vmax*phi*c/(K+c)."""
+    grid: Grid
+    diffusion: np.ndarray
+    boundary_values: np.ndarray | None = None  # 6*S: face f of signal s at [6*s+f]
+    velocity: VelocityField | None = None
+    source_rates: np.ndarray | None = None
+    loss_rates: np.ndarray | None = None
+    phi: np.ndarray | None = None
+    monod_vmax: np.ndarray | None = None
+    monod_k: np.ndarray | None = None
+    revision: int = 0
+
+    @property
+    def signal_count(self) -> int:
+        return int(np.atleast_1d(self.diffusion).size)
+
+    def _c(self, keep: list) -> _SignalGrid:
+        s = _SignalGrid()
+        s.grid = self.grid._c(keep)
+        S = self.signal_count
+        s.signal_count = S
+        arrays = {"diffusion": np.atleast_1d(np.asarray(self.diffusion, np.float32))}
+        if self.boundary_values is not None:
+            arrays["boundary_values"] = np.asarray(self.boundary_values, np.float32).reshape(-1)
+        if self.velocity is not None:
+            arrays["x_faces"] = np.asarray(self.velocity.x_faces, np.float32).reshape(-1)
+            arrays["y_faces"] = np.asarray(self.velocity.y_faces, np.float32).reshape(-1)
+            arrays["z_faces"] = np.asarray(self.velocity.z_faces, np.float32).reshape(-1)
+        if self.loss_rates is not None:
+            arrays["loss_rates"] = np.asarray(self.loss_rates, np.float32).reshape(-1)
+            if self.source_rates is not None:
+                arrays["source_rates"] = np.asarray(self.source_rates, np.float32).reshape(-1)
+        if self.phi is not None:
+            arrays["phi"] = np.asarray(self.phi, np.float32).reshape(-1)
+            arrays["monod_vmax"] = np.broadcast_to(np.asarray(self.monod_vmax, np.float32), (S,))
+            arrays["monod_k"] = np.broadcast_to(np.asarray(self.monod_k, np.float32), (S,))
+        for name, arr in arrays.items():
+            arr = np.ascontiguousarray(arr)
+            keep.append(arr)
+            setattr(s, name, arr.ctypes.data)
+        s.revision = self.revision
+        return s
+
+
+@dataclass
+class SignalSolveReport:
+    """SignalSolveReport (signals.hpp:57-61)."""
+    converged: bool = True
+    iterations: int = 0
+    residual_rms: float = 0.0
+
+
+@dataclass
+class BackendInfo:
+    """BackendInfo (types.hpp:77-83)."""
+    kind: str = "hip"
+    name: str = "hip"
+    device: str = ""
+    device_index: int = 0
+    native: bool = True
+
+
+FEATURES = ("growth", "species", "cell_contacts", "external_constraints", "signals",
+            "coupled_rates", "depth_averaged_flow", "resolved_flow", "contact_dynamics")
+SUPPORTED = frozenset({"signals", "depth_averaged_flow"})
+
+
+class HipBackend:
+    """A `hip` ComputeBackend for the grid flow-and-transport path.
+
+    Only the features in SUPPORTED are implemented; the other ComputeBackend
+    methods raise (no silent fallback, docs/architecture/0001-native-backends.md:33).
+    One context (one device, one stream); calls must not overlap.
+    """
+
+    def __init__(self, device_index: int = 0):
+        self.lib = library()
+        ctx = C.c_void_p()
+        status = self.lib.msim_hip_create(device_index, C.byref(ctx))
+        if status != 0:
+            raise _BY_STATUS.get(status, MsimError)(
+                f"HIP backend device {device_index} is unavailable (status {status})")
+        self.ctx = ctx
+        self.device_index = device_index
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.msim_hip_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- errors -------------------------------------------------------------
+    def _check(self, status: int):
+        if status == 0:
+            return
+        message = (self.lib.msim_hip_last_error(self.ctx) or b"").decode(errors="replace")
+        if status in _CG_KINDS:
+            raise ConjugateGradientFailure(message, _CG_KINDS[status])
+        raise _BY_STATUS.get(status, MsimError)(message)
+
+    # -- ComputeBackend ---------------------------------------------------------
+    def info(self) -> BackendInfo:
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.msim_hip_device_name(self.ctx, buf, 256))
+        return BackendInfo(device=buf.value.decode(), device_index=self.device_index)
+
+    def supports(self, feature: str) -> bool:
+        return feature in SUPPORTED
+
+    def solve_depth_averaged_flow(self, grid: Grid, mobility,
+                                  parameters: DepthAveragedFlowParameters | None = None
+                                  ) -> DepthAveragedFlowResult:
+        """ComputeBackend::solve_depth_averaged_flow (backend.hpp:44-46)."""
+        parameters = parameters or DepthAveragedFlowParameters()
+        keep: list = []
+        g = grid._c(keep)
+        mob = None
+        if mobility is not None:
+            mob = np.ascontiguousarray(mobility, dtype=np.float32).reshape(-1)
+            if mob.size != grid.sites:
+                raise InvalidArgument("flow mobility must hold one value per grid site")
+        nfx, nfy, nfz = grid.face_counts()
+        xf = np.zeros(nfx, np.float32)
+        yf = np.zeros(nfy, np.float32)
+        zf = np.zeros(nfz, np.float32)
+        p = parameters._c()
+        rep = _FlowReport()
+        self._check(self.lib.msim_hip_solve_depth_averaged_flow(
+            self.ctx, C.byref(g), None if mob is None else mob.ctypes.data, C.byref(p),
+            xf.ctypes.data, yf.ctypes.data, zf.ctypes.data, C.byref(rep)))
+        return DepthAveragedFlowResult(VelocityField(xf, yf, zf),
+                                       DepthAveragedFlowReport._from(rep))
+
+    def advance_signal_grid(self, spec: SignalGridSpec, levels, dt: float):
+        """ComputeBackend::advance_signal_grid (backend.hpp:30), forward Euler:
+        returns (new levels, SignalSolveReport); `levels` is not modified."""
+        keep: list = []
+        s = spec._c(keep)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        if lv.size != spec.signal_count * spec.grid.sites:
+            raise InvalidArgument("signal grid level count does not match its specification")
+        rep = _SignalReport()
+        self._check(self.lib.msim_hip_advance_signal_grid(self.ctx, C.byref(s), dt,
+                                                         lv.ctypes.data, C.byref(rep)))
+        return lv, SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
+
+    # -- device-resident flow -------------------------------------------------
+    def flow_prepare(self, grid: Grid, mobility, axis: int = 0):
+        keep: list = []
+        g = grid._c(keep)
+        mob = None if mobility is None else np.ascontiguousarray(mobility, np.float32).reshape(-1)
+        self._grid = grid
+        self._check(self.lib.msim_hip_flow_prepare(
+            self.ctx, C.byref(g), None if mob is None else mob.ctypes.data, axis))
+
+    def flow_solve(self, parameters: DepthAveragedFlowParameters) -> DepthAveragedFlowReport:
+        p = parameters._c()
+        rep = _FlowReport()
+        self._check(self.lib.msim_hip_flow_solve(self.ctx, C.byref(p), C.byref(rep)))
+        return DepthAveragedFlowReport._from(rep)
+
+    def flow_download(self):
+        g = self._grid
+        nfx, nfy, nfz = g.face_counts()
+        pressure = np.zeros(g.nx * g.ny, np.float64)
+        xf, yf, zf = (np.zeros(k, np.float32) for k in (nfx, nfy, nfz))
+        self._check(self.lib.msim_hip_flow_download(self.ctx, pressure.ctypes.data,
+                                                    xf.ctypes.data, yf.ctypes.data,
+                                                    zf.ctypes.data))
+        return pressure, VelocityField(xf, yf, zf)
+
+    def flow_apply(self, x) -> np.ndarray:
+        xin = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.zeros_like(xin)
+        self._check(self.lib.msim_hip_flow_apply(self.ctx, xin.ctypes.data, y.ctypes.data))
+        return y
+
+    def flow_arrays(self) -> dict:
+        n = self._grid.nx * self._grid.ny
+        out = {k: np.zeros(n, np.float32) for k in ("conductance", "diagonal", "rhs", "bc")}
+        self._check(self.lib.msim_hip_flow_arrays(
+            self.ctx, out["conductance"].ctypes.data, out["diagonal"].ctypes.data,
+            out["rhs"].ctypes.data, out["bc"].ctypes.data))
+        return out
+
+    # -- device-resident transport ------------------------------------------------
+    def adr_upload(self, spec: SignalGridSpec, levels, precision: int = 64,
+                   use_flow_field: bool = False):
+        keep: list = []
+        s = spec._c(keep)
+        dtype = np.float64 if precision == 64 else np.float32
+        lv = np.ascontiguousarray(levels, dtype=dtype).reshape(-1)
+        self._adr = (spec.signal_count, spec.grid.sites, dtype)
+        self._check(self.lib.msim_hip_adr_upload(self.ctx, C.byref(s), lv.ctypes.data, precision,
+                                                 1 if use_flow_field else 0))
+
+    def adr_run(self, dt: float, steps: int, ledger: bool = True):
+        S = self._adr[0]
+        led = np.zeros((steps, S, 4), np.float64) if ledger else None
+        self._check(self.lib.msim_hip_adr_run(self.ctx, dt, steps,
+                                              None if led is None else led.ctypes.data))
+        return led
+
+    def adr_download(self) -> np.ndarray:
+        S, n, dtype = self._adr
+        out = np.zeros(S * n, dtype)
+        self._check(self.lib.msim_hip_adr_download(self.ctx, out.ctypes.data))
+        return out
+
+    def adr_seconds(self) -> float:
+        return float(self.lib.msim_hip_adr_seconds(self.ctx))
+
+    # -- the rest of ComputeBackend: not on this backend ------------------------
+    def _unsupported(self, *args, **kwargs):
+        raise Unsupported("the HIP backend implements depth-averaged flow and signal transport only")
+
+    advance_growth = advance_species = advance_coupled = _unsupported
+    find_cell_contacts = find_external_contacts = solve_contact_forces = _unsupported
+    solve_depth_averaged_port_flow = solve_resolved_flow = solve_resolved_port_flow = _unsupported

@@ -1,0 +1,542 @@
+// The C ABI (include/msim_hip.h): argument checks on the host in the
+// reference's order and wording, then the device work in flow.hip / adr.hip.
+// No exception leaves this file: each entry point maps Error and the standard
+// exceptions onto msim_status and keeps the message on the context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "msim_hip.h"
+
+struct msim_ctx {
+  msim::Context impl;
+};
+
+namespace msim {
+namespace {
+
+[[noreturn]] void invalid(const std::string& message) {
+  throw Error(MSIM_INVALID_ARGUMENT, message);
+}
+
+template <typename Body>
+int guarded(msim_ctx* ctx, Body&& body) {
+  if (ctx == nullptr) return MSIM_INVALID_ARGUMENT;
+  try {
+    check(hipSetDevice(ctx->impl.device), "hipSetDevice");
+    body(ctx->impl);
+    ctx->impl.error.clear();
+    return MSIM_OK;
+  } catch (const Error& e) {
+    ctx->impl.error = e.what();
+    return e.code();
+  } catch (const std::bad_alloc&) {
+    ctx->impl.error = "out of host memory";
+    return MSIM_RUNTIME_ERROR;
+  } catch (const std::exception& e) {
+    ctx->impl.error = e.what();
+    return MSIM_RUNTIME_ERROR;
+  }
+}
+
+// SignalGridSpec::validate_lattice and the mask and boundary checks of
+// SignalGridSpec::validate (cpp/core/signals.cpp:584-599, 626-632, 721-731).
+void validate_lattice(const msim_grid& g) {
+  if (g.nx == 0 || g.ny == 0 || g.nz == 0) invalid("signal grid dimensions must be positive");
+  if (!std::isfinite(g.dx) || !std::isfinite(g.dy) || !std::isfinite(g.dz) || g.dx <= 0.0f ||
+      g.dy <= 0.0f || g.dz <= 0.0f) {
+    invalid("signal grid spacing must be finite and positive");
+  }
+  const double sites = static_cast<double>(g.nx) * g.ny * g.nz;
+  if (sites > static_cast<double>(std::numeric_limits<uint32_t>::max())) {
+    throw Error(MSIM_OVERFLOW, "signal grid level count exceeds the uint32 index space");
+  }
+  for (int f = 0; f < 6; ++f) {
+    if (g.boundary[f] > MSIM_FIXED) invalid("unknown signal grid boundary kind");
+  }
+  for (int a = 0; a < 3; ++a) {
+    if ((g.boundary[2 * a] == MSIM_PERIODIC) != (g.boundary[2 * a + 1] == MSIM_PERIODIC)) {
+      invalid("signal grid periodic boundaries must be paired");
+    }
+  }
+  if (g.obstacles != nullptr) {
+    const size_t n = static_cast<size_t>(sites);
+    for (size_t i = 0; i < n; ++i) {
+      if (g.obstacles[i] > 1) invalid("signal grid obstacle mask values must be 0 or 1");
+    }
+  }
+}
+
+// Inlet-to-outlet reachability through sites not in `solid`
+// (validate_flow_grid, cpp/cpu/cpu_flow.cpp:288-336), one layer.
+void check_reachable(const msim_grid& g, const std::vector<uint8_t>& solid, uint32_t axis) {
+  const uint32_t nx = g.nx, ny = g.ny;
+  const size_t n = static_cast<size_t>(nx) * ny;
+  std::vector<uint8_t> visited(n, 0);
+  std::vector<uint32_t> pending;
+  pending.reserve(1024);
+  const uint32_t across = axis == 0 ? ny : nx;
+  for (uint32_t t = 0; t < across; ++t) {
+    const size_t site = axis == 0 ? t : static_cast<size_t>(t) * ny;
+    if (solid[site] == 0) {
+      visited[site] = 1;
+      pending.push_back(static_cast<uint32_t>(site));
+    }
+  }
+  if (pending.empty()) invalid("the flow inlet boundary is entirely blocked");
+  const uint32_t dims[2] = {nx, ny};
+  while (!pending.empty()) {
+    const size_t site = pending.back();
+    pending.pop_back();
+    const uint32_t c[2] = {static_cast<uint32_t>(site / ny), static_cast<uint32_t>(site % ny)};
+    if (c[axis] + 1 == dims[axis]) return;
+    const size_t step[2] = {ny, 1};
+    for (int a = 0; a < 2; ++a) {
+      if (c[a] > 0) {
+        const size_t nb = site - step[a];
+        if (!visited[nb] && solid[nb] == 0) {
+          visited[nb] = 1;
+          pending.push_back(static_cast<uint32_t>(nb));
+        }
+      }
+      if (c[a] + 1 < dims[a]) {
+        const size_t nb = site + step[a];
+        if (!visited[nb] && solid[nb] == 0) {
+          visited[nb] = 1;
+          pending.push_back(static_cast<uint32_t>(nb));
+        }
+      }
+    }
+  }
+  throw Error(MSIM_RUNTIME_ERROR, "the device carries no through-flow: the outlet is unreachable");
+}
+
+// The checks reduce_shallow_flow and the flow system's constructor make
+// before a solve (flow_system.hpp:1255-1266, 260-330; cpu_flow.cpp:288-336).
+void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
+                   std::vector<uint8_t>& solid, std::vector<float>& mob) {
+  validate_lattice(g);
+  if (axis > 2) invalid("unknown flow axis");
+  for (int a = 0; a < 3; ++a) {
+    if (g.boundary[2 * a] == MSIM_PERIODIC || g.boundary[2 * a + 1] == MSIM_PERIODIC) {
+      invalid("native flow solvers do not support periodic boundaries");
+    }
+  }
+  if (g.boundary[2 * axis] != MSIM_FIXED || g.boundary[2 * axis + 1] != MSIM_FIXED) {
+    invalid("native flow-axis boundaries must be fixed to identify inlet and outlet");
+  }
+  if (axis == 2) invalid("shallow flow is depth-integrated along z; flow axis must be x or y");
+  if (g.nz != 1) {
+    throw Error(MSIM_UNSUPPORTED, "the HIP depth-averaged path takes a grid of one layer (nz = 1)");
+  }
+  const size_t n = static_cast<size_t>(g.nx) * g.ny;
+  solid.assign(n, 0);
+  if (g.obstacles != nullptr) std::memcpy(solid.data(), g.obstacles, n);
+  check_reachable(g, solid, axis);
+  mob.assign(n, 1.0f);
+  if (mobility != nullptr) {
+    for (size_t i = 0; i < n; ++i) {
+      if (!std::isfinite(mobility[i]) || mobility[i] < 0.0f) {
+        invalid("flow mobility must be finite and non-negative");
+      }
+    }
+    std::memcpy(mob.data(), mobility, n * sizeof(float));
+  }
+  // Columns without conductance close (ShallowFlowReduction, 326-327); the
+  // reduced grid must still carry flow.
+  std::vector<uint8_t> reduced(solid);
+  for (size_t i = 0; i < n; ++i) {
+    if (reduced[i] == 0 && g.dz * mob[i] == 0.0f) reduced[i] = 1;
+  }
+  check_reachable(g, reduced, axis);
+}
+
+void validate_flow_params(const msim_flow_params& p) {
+  if (p.scale != 0 && (!std::isfinite(p.mean_inlet_speed) || p.mean_inlet_speed == 0.0f)) {
+    invalid("depth-averaged mean inlet speed must be finite and nonzero without a body force");
+  }
+  if (!std::isfinite(p.relative_tolerance) || p.relative_tolerance < 1.0e-15) {
+    invalid("depth-averaged relative tolerance must be finite and at least 1e-15");
+  }
+  if (p.max_iterations == 0) invalid("depth-averaged iteration limit must be positive");
+  if (p.solver > MSIM_SOLVER_LEARNED) invalid("unknown pressure solver");
+}
+
+// The transport fields of SignalGridSpec::validate (signals.cpp:609-700) for
+// what this backend reads, plus the Monod inputs.
+void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
+  validate_lattice(s.grid);
+  if (s.grid.nz != 1) {
+    throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes a grid of one layer (nz = 1)");
+  }
+  if (s.signal_count == 0) invalid("signal grid must contain at least one signal");
+  if (s.signal_count > kMaxSignals) {
+    throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes at most 8 signals");
+  }
+  if (s.diffusion == nullptr) invalid("signal grid transport arrays must match signal count");
+  for (uint32_t k = 0; k < s.signal_count; ++k) {
+    if (!std::isfinite(s.diffusion[k]) || s.diffusion[k] < 0.0f) {
+      invalid("signal diffusion must be finite and non-negative");
+    }
+    for (int f = 0; f < 6; ++f) {
+      if (s.grid.boundary[f] != MSIM_FIXED) continue;
+      const float v = s.boundary_values != nullptr ? s.boundary_values[6 * k + f] : 0.0f;
+      if (!std::isfinite(v) || v < 0.0f) {
+        invalid("fixed signal grid boundary values must be finite and non-negative");
+      }
+    }
+  }
+  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
+  auto solid = [&](size_t i) { return s.grid.obstacles != nullptr && s.grid.obstacles[i] != 0; };
+  if (s.loss_rates != nullptr) {
+    for (size_t i = 0; i < s.signal_count * n; ++i) {
+      const float src = s.source_rates != nullptr ? s.source_rates[i] : 0.0f;
+      if (!std::isfinite(src) || src < 0.0f) {
+        invalid("signal grid affine source rates must be finite and non-negative");
+      }
+      if (!std::isfinite(s.loss_rates[i]) || s.loss_rates[i] < 0.0f) {
+        invalid("signal grid affine loss rates must be finite and non-negative");
+      }
+      if (solid(i % n) && (src != 0.0f || s.loss_rates[i] != 0.0f)) {
+        invalid("signal grid affine reaction must be zero at obstacle sites");
+      }
+    }
+  }
+  if (s.phi != nullptr) {
+    if (s.monod_vmax == nullptr || s.monod_k == nullptr) invalid("Monod uptake needs vmax and K");
+    for (uint32_t k = 0; k < s.signal_count; ++k) {
+      if (!std::isfinite(s.monod_vmax[k]) || s.monod_vmax[k] < 0.0f ||
+          !std::isfinite(s.monod_k[k]) || s.monod_k[k] <= 0.0f) {
+        invalid("Monod vmax must be finite and non-negative and K finite and positive");
+      }
+    }
+    for (size_t i = 0; i < n; ++i) {
+      if (!std::isfinite(s.phi[i]) || s.phi[i] < 0.0f) {
+        invalid("Monod occupancy must be finite and non-negative");
+      }
+    }
+  }
+  if (!device_field && s.x_faces != nullptr) {
+    if (s.y_faces == nullptr) invalid("signal grid velocity field must cover every lattice face");
+    const uint32_t nx = s.grid.nx, ny = s.grid.ny;
+    const size_t nxf = static_cast<size_t>(nx + 1) * ny, nyf = static_cast<size_t>(nx) * (ny + 1);
+    for (size_t i = 0; i < nxf; ++i) {
+      if (!std::isfinite(s.x_faces[i])) invalid("signal grid velocity field values must be finite");
+    }
+    for (size_t i = 0; i < nyf; ++i) {
+      if (!std::isfinite(s.y_faces[i])) invalid("signal grid velocity field values must be finite");
+    }
+    // Closed faces carry no velocity (signals.cpp:654-676).
+    for (uint32_t x = 0; x < nx; ++x) {
+      for (uint32_t y = 0; y < ny; ++y) {
+        const size_t i = static_cast<size_t>(x) * ny + y;
+        const bool here = solid(i);
+        const bool clo_x = x == 0 ? s.grid.boundary[0] == MSIM_NO_FLUX : solid(i - ny);
+        const bool cup_x = x + 1 == nx ? s.grid.boundary[1] == MSIM_NO_FLUX : solid(i + ny);
+        const bool clo_y = y == 0 ? s.grid.boundary[2] == MSIM_NO_FLUX : solid(i - 1);
+        const bool cup_y = y + 1 == ny ? s.grid.boundary[3] == MSIM_NO_FLUX : solid(i + 1);
+        const size_t yf = static_cast<size_t>(x) * (ny + 1) + y;
+        if (nx > 1 && (((clo_x || here) && s.x_faces[i] != 0.0f) ||
+                       ((cup_x || here) && s.x_faces[i + ny] != 0.0f))) {
+          invalid("signal grid velocity field must be zero on closed faces");
+        }
+        if (ny > 1 && (((clo_y || here) && s.y_faces[yf] != 0.0f) ||
+                       ((cup_y || here) && s.y_faces[yf + 1] != 0.0f))) {
+          invalid("signal grid velocity field must be zero on closed faces");
+        }
+      }
+    }
+  }
+}
+
+// validate_signal_levels (signals.cpp:754-784).
+template <typename T>
+void validate_levels(const msim_signal_grid& s, const T* levels) {
+  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
+  int fault = 0;
+  for (size_t i = 0; i < s.signal_count * n; ++i) {
+    const T v = levels[i];
+    if (!std::isfinite(v) || v < 0) {
+      fault = 1;
+      break;
+    }
+    if (s.grid.obstacles != nullptr && v != 0 && s.grid.obstacles[i % n] != 0) fault = 2;
+  }
+  if (fault == 1) invalid("signal grid levels must be finite and non-negative");
+  if (fault == 2) invalid("signal grid levels must be zero at obstacle sites");
+}
+
+// Largest loss rate per signal, for the explicit bound: the affine loss's
+// maximum (max_reaction_loss, signals.cpp:322-330) plus vmax*max(phi)/K.
+void record_losses(Context& ctx, const msim_signal_grid& s) {
+  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
+  float phi_max = 0.0f;
+  if (s.phi != nullptr) {
+    for (size_t i = 0; i < n; ++i) phi_max = std::max(phi_max, s.phi[i]);
+  }
+  for (uint32_t k = 0; k < s.signal_count; ++k) {
+    double loss = 0.0;
+    if (s.loss_rates != nullptr) {
+      loss = static_cast<double>(*std::max_element(s.loss_rates + k * n, s.loss_rates + (k + 1) * n));
+    }
+    if (s.phi != nullptr) {
+      loss += static_cast<double>(s.monod_vmax[k]) * phi_max / static_cast<double>(s.monod_k[k]);
+    }
+    ctx.adr.max_loss[k] = loss;
+  }
+}
+
+// SignalGrid::validate_step (signals.cpp:1066-1102) for the upwind scheme.
+void validate_step(const Context& ctx, float dt) {
+  if (!std::isfinite(dt) || dt < 0.0f) invalid("time step must be finite and non-negative");
+  const auto& P = ctx.adr.params;
+  const uint32_t dims[2] = {P.nx, P.ny};
+  const float spacing[2] = {P.dx, P.dy};
+  for (uint32_t k = 0; k < P.signals; ++k) {
+    double inverse_square_sum = 0.0;
+    double courant_sum = ctx.adr.greatest_outflow;
+    for (int a = 0; a < 2; ++a) {
+      if (dims[a] == 1) continue;
+      const double inverse_spacing = 1.0 / static_cast<double>(spacing[a]);
+      inverse_square_sum += inverse_spacing * inverse_spacing;
+      if (P.has_field == 0) {
+        courant_sum += std::abs(static_cast<double>(P.advection[k][a])) * inverse_spacing;
+      }
+    }
+    const double factor =
+        static_cast<double>(dt) *
+        ((2.0 * static_cast<double>(P.diffusion[k]) * inverse_square_sum) + courant_sum +
+         ctx.adr.max_loss[k]);
+    if (!std::isfinite(factor) || factor > 1.0) {
+      invalid("signal grid time step violates the explicit stability bound");
+    }
+  }
+}
+
+}  // namespace
+}  // namespace msim
+
+using msim::Context;
+using msim::Error;
+
+extern "C" {
+
+int msim_hip_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+  return count;
+}
+
+int msim_hip_create(int device, msim_ctx** out) {
+  if (out == nullptr) return MSIM_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    return MSIM_NO_DEVICE;
+  }
+  auto* ctx = new (std::nothrow) msim_ctx;
+  if (ctx == nullptr) return MSIM_RUNTIME_ERROR;
+  ctx->impl.device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->impl.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->impl.start) != hipSuccess ||
+      hipEventCreate(&ctx->impl.stop) != hipSuccess) {
+    msim_hip_destroy(ctx);
+    return MSIM_DEVICE_ERROR;
+  }
+  hipDeviceProp_t prop{};
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->impl.name = prop.name;
+    if (ctx->impl.name.empty()) ctx->impl.name = prop.gcnArchName;
+  }
+  *out = ctx;
+  return MSIM_OK;
+}
+
+void msim_hip_destroy(msim_ctx* ctx) {
+  if (ctx == nullptr) return;
+  static_cast<void>(hipSetDevice(ctx->impl.device));
+  if (ctx->impl.stream != nullptr) static_cast<void>(hipStreamSynchronize(ctx->impl.stream));
+  if (ctx->impl.flow.host_state != nullptr) static_cast<void>(hipHostFree(ctx->impl.flow.host_state));
+  ctx->impl.flow.host_state = nullptr;
+  for (hipEvent_t e : ctx->impl.probes) static_cast<void>(hipEventDestroy(e));
+  if (ctx->impl.start != nullptr) static_cast<void>(hipEventDestroy(ctx->impl.start));
+  if (ctx->impl.stop != nullptr) static_cast<void>(hipEventDestroy(ctx->impl.stop));
+  if (ctx->impl.stream != nullptr) static_cast<void>(hipStreamDestroy(ctx->impl.stream));
+  delete ctx;
+}
+
+const char* msim_hip_last_error(const msim_ctx* ctx) {
+  return ctx == nullptr ? "null context" : ctx->impl.error.c_str();
+}
+
+int msim_hip_device_name(msim_ctx* ctx, char* name, size_t length) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (name == nullptr || length == 0) msim::invalid("name buffer must not be empty");
+    std::snprintf(name, length, "%s", c.name.c_str());
+  });
+}
+
+int msim_hip_flow_prepare(msim_ctx* ctx, const msim_grid* grid, const float* mobility,
+                          uint32_t axis) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr) msim::invalid("grid must not be null");
+    std::vector<uint8_t> solid;
+    std::vector<float> mob;
+    msim::validate_flow(*grid, mobility, axis, solid, mob);
+    auto& f = c.flow;
+    f.nx = grid->nx;
+    f.ny = grid->ny;
+    f.dx = grid->dx;
+    f.dy = grid->dy;
+    f.dz = grid->dz;
+    f.axis = axis;
+    f.prepared = false;
+    msim::flow_assemble(c, solid.data(), mob.data());
+  });
+}
+
+int msim_hip_flow_solve(msim_ctx* ctx, const msim_flow_params* params, msim_flow_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (params == nullptr || report == nullptr) msim::invalid("parameters must not be null");
+    if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    msim::validate_flow_params(*params);
+    if (params->axis != c.flow.axis) msim::invalid("the flow axis differs from the prepared one");
+    msim::flow_solve(c, *params, *report);
+  });
+}
+
+int msim_hip_flow_download(msim_ctx* ctx, double* pressure, float* x_faces, float* y_faces,
+                           float* z_faces) {
+  return msim::guarded(ctx, [&](Context& c) {
+    auto& f = c.flow;
+    if (!f.solved) msim::invalid("no flow solve to download");
+    if (pressure != nullptr) {
+      msim::check(hipMemcpyAsync(pressure, f.x.data(), f.sites() * sizeof(double),
+                                 hipMemcpyDeviceToHost, c.stream),
+                  "download pressure");
+    }
+    if (x_faces != nullptr) {
+      msim::check(hipMemcpyAsync(x_faces, f.faces.data(), f.x_faces() * sizeof(float),
+                                 hipMemcpyDeviceToHost, c.stream),
+                  "download x faces");
+    }
+    if (y_faces != nullptr) {
+      msim::check(hipMemcpyAsync(y_faces, f.faces.data() + f.x_faces(),
+                                 f.y_faces() * sizeof(float), hipMemcpyDeviceToHost, c.stream),
+                  "download y faces");
+    }
+    msim::check(hipStreamSynchronize(c.stream), "download");
+    if (z_faces != nullptr) std::memset(z_faces, 0, 2 * f.sites() * sizeof(float));
+  });
+}
+
+int msim_hip_solve_depth_averaged_flow(msim_ctx* ctx, const msim_grid* grid,
+                                       const float* mobility, const msim_flow_params* params,
+                                       float* x_faces, float* y_faces, float* z_faces,
+                                       msim_flow_report* report) {
+  if (params == nullptr) return MSIM_INVALID_ARGUMENT;
+  int status = msim_hip_flow_prepare(ctx, grid, mobility, params->axis);
+  if (status != MSIM_OK) return status;
+  status = msim_hip_flow_solve(ctx, params, report);
+  if (status != MSIM_OK) return status;
+  return msim_hip_flow_download(ctx, nullptr, x_faces, y_faces, z_faces);
+}
+
+int msim_hip_flow_apply(msim_ctx* ctx, const double* x, double* y) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    if (x == nullptr || y == nullptr) msim::invalid("vectors must not be null");
+    msim::flow_apply(c, x, y);
+  });
+}
+
+int msim_hip_flow_arrays(msim_ctx* ctx, float* conductance, float* diagonal, float* rhs,
+                         float* boundary_conductance) {
+  return msim::guarded(ctx, [&](Context& c) {
+    auto& f = c.flow;
+    if (!f.prepared) msim::invalid("no flow system has been prepared");
+    const size_t bytes = f.sites() * sizeof(float);
+    const std::pair<float*, float*> copies[] = {{conductance, f.cond.data()},
+                                                {diagonal, f.diag.data()},
+                                                {rhs, f.rhs.data()},
+                                                {boundary_conductance, f.bc.data()}};
+    for (const auto& [dst, src] : copies) {
+      if (dst != nullptr) {
+        msim::check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream), "download");
+      }
+    }
+    msim::check(hipStreamSynchronize(c.stream), "download");
+  });
+}
+
+int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, float dt,
+                                 float* levels, msim_signal_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || levels == nullptr) msim::invalid("grid and levels must not be null");
+    const bool refresh = grid->revision == 0 || grid->revision != c.adr.revision ||
+                         !c.adr.uploaded || c.adr.flow_field;
+    if (refresh) msim::validate_signal_grid(*grid, false);
+    msim::validate_levels(*grid, levels);
+    msim::adr_upload(c, *grid, levels, 32, false);
+    if (refresh) msim::record_losses(c, *grid);
+    msim::validate_step(c, dt);
+    if (report != nullptr) *report = msim_signal_report{1, 0, 0.0f};
+    if (dt == 0.0f) return;
+    msim::adr_run(c, dt, 1, nullptr);
+    msim::adr_download(c, levels);
+  });
+}
+
+int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void* levels,
+                        uint32_t precision, uint32_t use_flow_field) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || levels == nullptr) msim::invalid("grid and levels must not be null");
+    if (precision != 32 && precision != 64) msim::invalid("precision must be 32 or 64");
+    if (use_flow_field != 0) {
+      if (!c.flow.solved) msim::invalid("no flow field on the device");
+      if (c.flow.nx != grid->grid.nx || c.flow.ny != grid->grid.ny) {
+        msim::invalid("the flow field is for another grid");
+      }
+    }
+    msim::validate_signal_grid(*grid, use_flow_field != 0);
+    if (precision == 64) {
+      msim::validate_levels(*grid, static_cast<const double*>(levels));
+    } else {
+      msim::validate_levels(*grid, static_cast<const float*>(levels));
+    }
+    msim::adr_upload(c, *grid, levels, precision, use_flow_field != 0);
+    msim::record_losses(c, *grid);
+  });
+}
+
+int msim_hip_adr_run(msim_ctx* ctx, float dt, uint32_t steps, double* ledger) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.adr.uploaded) msim::invalid("no signal grid on the device");
+    msim::validate_step(c, dt);
+    if (dt == 0.0f || steps == 0) return;
+    msim::adr_run(c, dt, steps, ledger);
+  });
+}
+
+int msim_hip_adr_download(msim_ctx* ctx, void* levels) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.adr.uploaded) msim::invalid("no signal grid on the device");
+    if (levels == nullptr) msim::invalid("levels must not be null");
+    msim::adr_download(c, levels);
+  });
+}
+
+double msim_hip_adr_seconds(msim_ctx* ctx) {
+  return ctx == nullptr ? 0.0 : ctx->impl.adr.seconds;
+}
+
+}  // extern "C"

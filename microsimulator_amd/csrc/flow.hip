@@ -1,0 +1,896 @@
+// Depth-averaged pressure solve on the device: operator assembly, the
+// Jacobi-preconditioned conjugate gradient in FP64 (and in FP32 inside FP64
+// iterative refinement), the FP64 true residual, and face-velocity
+// reconstruction with the reference's lift and scaling.
+//
+// Layout: one layer, site = x*ny + y with y contiguous (as the reference,
+// cpp/core/flow_system.hpp:64-67). The operator is stored on the structured
+// grid, not compacted: per site the conductance of its lower x face (cxl) and
+// lower y face (cyl), its boundary conductance (bc) and its Jacobi diagonal.
+// An inactive site (solid, or fluid without conductance) has every
+// coefficient zero, so its row of A is zero, its preconditioned residual is
+// zero, and it stays exactly zero through the iteration: the structured
+// system is the reference's active system (flow_system.hpp:500-518) with
+// zero rows added, and its sums gain only exact zeros.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace msim {
+namespace {
+
+constexpr int kStream = 256;  // threads per block of the streaming kernels
+
+// cpp/core/flow_system.hpp:171-174.
+__device__ __host__ inline float harmonic(float a, float b) {
+  const float sum = a + b;
+  return sum > 0.0f ? 2.0f * a * b / sum : 0.0f;
+}
+
+// Reduction (flow_system.hpp:276-327) for one layer: the column conductance
+// is the layer height times the mobility; solid sites carry none.
+__global__ void k_conductance(const uint8_t* __restrict__ solid, const float* __restrict__ mobility,
+                              float* __restrict__ cond, size_t n, float dz) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cond[i] = solid[i] != 0 ? 0.0f : dz * mobility[i];
+}
+
+// Assembly (flow_system.hpp:549-599): diagonal, right-hand side and boundary
+// conductance in the reference's float order (x lower, x upper, y lower,
+// y upper), and the lower-face conductances the operator uses.
+__global__ void k_assemble(const float* __restrict__ cond, float* __restrict__ diag,
+                           float* __restrict__ rhs, float* __restrict__ bc,
+                           float* __restrict__ cxl, float* __restrict__ cyl, uint32_t nx,
+                           uint32_t ny, float dx, float dy, uint32_t axis) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(nx) * ny) return;
+  const uint32_t x = static_cast<uint32_t>(i / ny);
+  const uint32_t y = static_cast<uint32_t>(i % ny);
+  const float m = cond[i];
+  cxl[i] = x > 0 ? harmonic(m, cond[i - ny]) : 0.0f;
+  cyl[i] = y > 0 ? harmonic(m, cond[i - 1]) : 0.0f;
+  float d = 0.0f, r = 0.0f, b = 0.0f;
+  if (m != 0.0f) {
+    const float isx = 1.0f / (dx * dx);
+    const float isy = 1.0f / (dy * dy);
+    // x lower
+    if (x > 0) {
+      d += harmonic(m, cond[i - ny]) * isx;
+    } else if (axis == 0) {
+      const float c = 2.0f * m * isx;
+      d += c;
+      b += c;
+      r += c * 1.0f;
+    }
+    // x upper
+    if (x + 1 < nx) {
+      d += harmonic(m, cond[i + ny]) * isx;
+    } else if (axis == 0) {
+      const float c = 2.0f * m * isx;
+      d += c;
+      b += c;
+      r += c * 0.0f;
+    }
+    // y lower
+    if (y > 0) {
+      d += harmonic(m, cond[i - 1]) * isy;
+    } else if (axis == 1) {
+      const float c = 2.0f * m * isy;
+      d += c;
+      b += c;
+      r += c * 1.0f;
+    }
+    // y upper
+    if (y + 1 < ny) {
+      d += harmonic(m, cond[i + 1]) * isy;
+    } else if (axis == 1) {
+      const float c = 2.0f * m * isy;
+      d += c;
+      b += c;
+      r += c * 0.0f;
+    }
+  }
+  diag[i] = d;
+  rhs[i] = r;
+  bc[i] = b;
+}
+
+// ---------------------------------------------------------------------------
+// Marching stencil kernels. A block of TY threads owns TY consecutive y of a
+// strip of `rows` x-rows and walks along x: each thread keeps the operand of
+// its column at x-1, x and x+1 in registers (so every row is read from memory
+// once), and the y neighbours come from an LDS row staged with a one-site
+// halo on each side. Tiles are visited in XCD-contiguous order.
+// ---------------------------------------------------------------------------
+
+struct Tile {
+  uint32_t ty, x0, x1, y;
+  bool valid;
+};
+
+template <int TY>
+__device__ inline Tile tile_of(uint32_t nx, uint32_t ny, uint32_t rows) {
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t nty = (ny + TY - 1) / TY;
+  Tile tile;
+  tile.ty = t % nty;
+  tile.x0 = (t / nty) * rows;
+  tile.x1 = min(nx, tile.x0 + rows);
+  tile.y = tile.ty * TY + threadIdx.x;
+  tile.valid = tile.y < ny;
+  return tile;
+}
+
+// One CG iteration's first half, with the previous iteration's direction
+// update and solution update folded in:
+//   p_new = M^-1 r + beta p_old           (conjugate_gradient.hpp:82, cpu_flow.cpp:92-97)
+//   x    += alpha_prev p_old              (the previous iteration's update, cpu_flow.cpp:84-89)
+//   q     = A p_new                       (DepthAveragedFlowSystem::apply, flow_system.hpp:670-695)
+//   curvature = p_new . q, alpha = rho / curvature (conjugate_gradient.hpp:59-66),
+// the last two in the block that finishes last.
+template <typename T, int TY>
+__global__ __launch_bounds__(TY) void k_cg_apply(
+    const T* __restrict__ r, const float* __restrict__ diag, const T* __restrict__ p_old,
+    T* __restrict__ p_new, T* __restrict__ q, T* __restrict__ xsol, const float* __restrict__ cxl,
+    const float* __restrict__ cyl, const float* __restrict__ bc, uint32_t nx, uint32_t ny,
+    uint32_t rows, T isx, T isy, CgState* st, double* partials) {
+  __shared__ T s_p[TY + 2];
+  __shared__ float s_c[TY + 2];
+  __shared__ double s_red[TY / kWave];
+  __shared__ bool s_last;
+  if (st->done != 0) return;
+  const Tile tl = tile_of<TY>(nx, ny, rows);
+  const uint32_t t = threadIdx.x;
+  const T beta = static_cast<T>(st->beta);
+  const T alpha = static_cast<T>(st->alpha);
+  const bool update_x = xsol != nullptr;
+  // The new direction at a site, and the old direction there.
+  auto operand = [&](size_t idx, T& old) -> T {
+    const float d = diag[idx];
+    const T z = d > 0.0f ? r[idx] / static_cast<T>(d) : static_cast<T>(0);
+    old = p_old[idx];
+    return z + beta * old;
+  };
+  T prev = 0, cur = 0, cur_old = 0, dummy;
+  float cxlo = 0.0f;
+  if (tl.valid && tl.x0 < tl.x1) {
+    const size_t i0 = static_cast<size_t>(tl.x0) * ny + tl.y;
+    if (tl.x0 > 0) prev = operand(i0 - ny, dummy);
+    cur = operand(i0, cur_old);
+    cxlo = cxl[i0];
+  }
+  double acc = 0.0;
+  for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
+    const size_t idx = static_cast<size_t>(xx) * ny + tl.y;
+    T nxt = 0, nxt_old = 0;
+    float cxhi = 0.0f;
+    if (tl.valid && xx + 1 < nx) {
+      nxt = operand(idx + ny, nxt_old);
+      cxhi = cxl[idx + ny];
+    }
+    s_p[t + 1] = tl.valid ? cur : static_cast<T>(0);
+    s_c[t + 1] = tl.valid ? cyl[idx] : 0.0f;
+    if (t == 0) {
+      s_p[0] = tl.ty > 0 ? operand(static_cast<size_t>(xx) * ny + tl.ty * TY - 1, dummy)
+                         : static_cast<T>(0);
+    }
+    if (t == TY - 1) {
+      const uint32_t yu = tl.ty * TY + TY;
+      s_p[TY + 1] = yu < ny ? operand(static_cast<size_t>(xx) * ny + yu, dummy) : static_cast<T>(0);
+      s_c[TY + 1] = yu < ny ? cyl[static_cast<size_t>(xx) * ny + yu] : 0.0f;
+    }
+    __syncthreads();
+    if (tl.valid) {
+      const T pl = s_p[t];
+      const T pu = s_p[t + 2];
+      const float cylo = s_c[t + 1];
+      const float cyhi = s_c[t + 2];
+      T qv = static_cast<T>(bc[idx]) * cur;
+      qv += (static_cast<T>(cxlo) * isx) * (cur - prev);
+      qv += (static_cast<T>(cxhi) * isx) * (cur - nxt);
+      qv += (static_cast<T>(cylo) * isy) * (cur - pl);
+      qv += (static_cast<T>(cyhi) * isy) * (cur - pu);
+      q[idx] = qv;
+      p_new[idx] = cur;
+      if (update_x) xsol[idx] += alpha * cur_old;
+      acc += static_cast<double>(cur) * static_cast<double>(qv);
+    }
+    __syncthreads();
+    prev = cur;
+    cur = nxt;
+    cur_old = nxt_old;
+    cxlo = cxhi;
+  }
+  double part[1] = {block_sum<TY>(acc, s_red)};
+  if (publish_partials<1>(part, partials, &st->ticket[kTicketApply], blockIdx.x, gridDim.x,
+                          &s_last)) {
+    double total[1];
+    sum_partials<TY, 1>(partials, gridDim.x, total, s_red);
+    if (t == 0) {
+      st->ticket[kTicketApply] = 0;
+      const double curvature = total[0];
+      st->curvature = curvature;
+      if (!isfinite(curvature) || curvature <= 0.0) {
+        st->failure = 1;
+        st->done = 1;
+      } else {
+        const double a = st->rho / curvature;
+        if (!isfinite(a)) {
+          st->failure = 2;
+          st->done = 1;
+        } else {
+          st->alpha = a;
+        }
+      }
+    }
+  }
+}
+
+// Second half: r -= alpha q (cpu_flow.cpp:84-89), then r . r for the stopping
+// test and r . M^-1 r for the next direction weight; the last block applies
+// the batched solve's tests (conjugate_gradient.hpp:126-166): converged when
+// r.r <= (tol ||b||)^2, else beta = rho'/rho with the breakdown and direction
+// failure codes.
+template <typename T>
+__global__ __launch_bounds__(kStream) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
+                                                       const float* __restrict__ diag, size_t n,
+                                                       CgState* st, double* partials) {
+  __shared__ double s_red[kStream / kWave];
+  __shared__ bool s_last;
+  if (st->done != 0) return;
+  const T alpha = static_cast<T>(st->alpha);
+  double rr = 0.0, rz = 0.0;
+  const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; i < n; i += stride) {
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const float d = diag[i];
+    const T z = d > 0.0f ? ri / static_cast<T>(d) : static_cast<T>(0);
+    rr += static_cast<double>(ri) * static_cast<double>(ri);
+    rz += static_cast<double>(ri) * static_cast<double>(z);
+  }
+  double part[2] = {block_sum<kStream>(rr, s_red), block_sum<kStream>(rz, s_red)};
+  if (publish_partials<2>(part, partials, &st->ticket[kTicketUpdate], blockIdx.x, gridDim.x,
+                          &s_last)) {
+    double total[2];
+    sum_partials<kStream, 2>(partials, gridDim.x, total, s_red);
+    if (threadIdx.x == 0) {
+      st->ticket[kTicketUpdate] = 0;
+      const double rrv = total[0];
+      const double next_rho = total[1];
+      st->iterations += 1;
+      st->rr = rrv;
+      if (!isfinite(rrv)) {
+        st->failure = 3;
+        st->done = 1;
+      } else if (rrv <= st->threshold) {
+        st->converged = 1;
+        st->done = 1;
+      } else if (!isfinite(next_rho) || st->rho == 0.0) {
+        st->failure = 4;
+        st->done = 1;
+      } else {
+        const double beta = next_rho / st->rho;
+        if (!isfinite(beta)) {
+          st->failure = 5;
+          st->done = 1;
+        } else {
+          st->beta = beta;
+          st->rho = next_rho;
+        }
+      }
+    }
+  }
+}
+
+// x = 0, r = b (or a scaled FP64 residual), p_old = 0; then r.r and r.M^-1 r
+// set up the solve (cpu_flow.cpp:74-80; conjugate_gradient.hpp:49-55).
+template <typename T, typename S>
+__global__ __launch_bounds__(kStream) void k_cg_init(const S* __restrict__ source, double scale,
+                                                     T* __restrict__ r, T* __restrict__ xsol,
+                                                     T* __restrict__ p_old,
+                                                     const float* __restrict__ diag, size_t n,
+                                                     double tolerance, CgState* st,
+                                                     double* partials) {
+  __shared__ double s_red[kStream / kWave];
+  __shared__ bool s_last;
+  double rr = 0.0, rz = 0.0;
+  const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; i < n; i += stride) {
+    const T ri = static_cast<T>(static_cast<double>(source[i]) * scale);
+    r[i] = ri;
+    xsol[i] = static_cast<T>(0);
+    p_old[i] = static_cast<T>(0);
+    const float d = diag[i];
+    const T z = d > 0.0f ? ri / static_cast<T>(d) : static_cast<T>(0);
+    rr += static_cast<double>(ri) * static_cast<double>(ri);
+    rz += static_cast<double>(ri) * static_cast<double>(z);
+  }
+  double part[2] = {block_sum<kStream>(rr, s_red), block_sum<kStream>(rz, s_red)};
+  if (publish_partials<2>(part, partials, &st->ticket[kTicketInit], blockIdx.x, gridDim.x,
+                          &s_last)) {
+    double total[2];
+    sum_partials<kStream, 2>(partials, gridDim.x, total, s_red);
+    if (threadIdx.x == 0) {
+      st->ticket[kTicketInit] = 0;
+      st->rr0 = total[0];
+      st->rr = total[0];
+      st->rho = total[1];
+      const double bound = tolerance * sqrt(total[0]);
+      st->threshold = bound * bound;
+      st->alpha = 0.0;
+      st->beta = 0.0;
+      st->iterations = 0;
+      st->failure = 0;
+      st->converged = total[0] == 0.0 ? 1u : 0u;
+      st->done = total[0] == 0.0 ? 1u : 0u;
+    }
+  }
+}
+
+// x += alpha p for the iteration that converged (its update was deferred).
+template <typename T>
+__global__ void k_cg_finalize(T* __restrict__ xsol, const T* __restrict__ p, size_t n,
+                              const CgState* st) {
+  const T alpha = static_cast<T>(st->alpha);
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    xsol[i] += alpha * p[i];
+}
+
+// FP64 refinement update: x += scale * e.
+__global__ void k_refine(double* __restrict__ xsol, const float* __restrict__ e, double scale,
+                         size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    xsol[i] += scale * static_cast<double>(e[i]);
+}
+
+// FP64 operator on x (flow_system.hpp:670-695): out = A x (mode 0) or
+// out = b - A x (mode 1, the true residual), with ||out||^2 and ||b||^2.
+template <int TY>
+__global__ __launch_bounds__(TY) void k_residual(
+    const double* __restrict__ xin, const float* __restrict__ rhs, double* __restrict__ out,
+    const float* __restrict__ cxl, const float* __restrict__ cyl, const float* __restrict__ bc,
+    uint32_t nx, uint32_t ny, uint32_t rows, double isx, double isy, int mode, CgState* st,
+    double* partials) {
+  __shared__ double s_p[TY + 2];
+  __shared__ float s_c[TY + 2];
+  __shared__ double s_red[TY / kWave];
+  __shared__ bool s_last;
+  const Tile tl = tile_of<TY>(nx, ny, rows);
+  const uint32_t t = threadIdx.x;
+  double prev = 0, cur = 0;
+  float cxlo = 0.0f;
+  if (tl.valid && tl.x0 < tl.x1) {
+    const size_t i0 = static_cast<size_t>(tl.x0) * ny + tl.y;
+    if (tl.x0 > 0) prev = xin[i0 - ny];
+    cur = xin[i0];
+    cxlo = cxl[i0];
+  }
+  double acc = 0.0, bb = 0.0;
+  for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
+    const size_t idx = static_cast<size_t>(xx) * ny + tl.y;
+    double nxt = 0;
+    float cxhi = 0.0f;
+    if (tl.valid && xx + 1 < nx) {
+      nxt = xin[idx + ny];
+      cxhi = cxl[idx + ny];
+    }
+    s_p[t + 1] = tl.valid ? cur : 0.0;
+    s_c[t + 1] = tl.valid ? cyl[idx] : 0.0f;
+    if (t == 0) s_p[0] = tl.ty > 0 ? xin[static_cast<size_t>(xx) * ny + tl.ty * TY - 1] : 0.0;
+    if (t == TY - 1) {
+      const uint32_t yu = tl.ty * TY + TY;
+      s_p[TY + 1] = yu < ny ? xin[static_cast<size_t>(xx) * ny + yu] : 0.0;
+      s_c[TY + 1] = yu < ny ? cyl[static_cast<size_t>(xx) * ny + yu] : 0.0f;
+    }
+    __syncthreads();
+    if (tl.valid) {
+      double qv = static_cast<double>(bc[idx]) * cur;
+      qv += (static_cast<double>(cxlo) * isx) * (cur - prev);
+      qv += (static_cast<double>(cxhi) * isx) * (cur - nxt);
+      qv += (static_cast<double>(s_c[t + 1]) * isy) * (cur - s_p[t]);
+      qv += (static_cast<double>(s_c[t + 2]) * isy) * (cur - s_p[t + 2]);
+      const double b = static_cast<double>(rhs[idx]);
+      const double v = mode == 0 ? qv : b - qv;
+      out[idx] = v;
+      acc += v * v;
+      bb += b * b;
+    }
+    __syncthreads();
+    prev = cur;
+    cur = nxt;
+    cxlo = cxhi;
+  }
+  double part[2] = {block_sum<TY>(acc, s_red), block_sum<TY>(bb, s_red)};
+  if (publish_partials<2>(part, partials, &st->ticket[kTicketResidual], blockIdx.x, gridDim.x,
+                          &s_last)) {
+    double total[2];
+    sum_partials<TY, 2>(partials, gridDim.x, total, s_red);
+    if (t == 0) {
+      st->ticket[kTicketResidual] = 0;
+      st->true_rr = total[0];
+      st->bb = total[1];
+    }
+  }
+}
+
+// Face velocities from the pressure (flow_system.hpp:711-736) lifted to the
+// one-layer grid (424-445): interior faces -harm(m_l, m_u) (p_u - p_l) / h,
+// faces of the flow axis' ends 2m/h toward the boundary pressure held half a
+// spacing outside, faces next to a solid site zero, all divided by the layer
+// height. Accumulates the open-inlet sum and the peak for scale_velocity
+// (1368-1415).
+__global__ __launch_bounds__(kStream) void k_velocity(
+    const double* __restrict__ p, const float* __restrict__ cond, const uint8_t* __restrict__ solid,
+    float* __restrict__ faces, uint32_t nx, uint32_t ny, float dx, float dy, float dz,
+    uint32_t axis, CgState* st, double* partials) {
+  __shared__ double s_red[kStream / kWave];
+  __shared__ bool s_last;
+  const size_t nxf = static_cast<size_t>(nx + 1) * ny;
+  const size_t total = nxf + static_cast<size_t>(nx) * (ny + 1);
+  double inlet = 0.0, count = 0.0, peak = 0.0, bad = 0.0;
+  const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
+  for (size_t f = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; f < total;
+       f += stride) {
+    uint32_t comp, along, dim;
+    size_t lower, upper;
+    if (f < nxf) {
+      comp = 0;
+      const uint32_t fx = static_cast<uint32_t>(f / ny), y = static_cast<uint32_t>(f % ny);
+      along = fx;
+      dim = nx;
+      upper = static_cast<size_t>(fx) * ny + y;
+      lower = upper - ny;  // meaningful only when fx > 0
+    } else {
+      comp = 1;
+      const size_t g = f - nxf;
+      const uint32_t x = static_cast<uint32_t>(g / (ny + 1)), fy = static_cast<uint32_t>(g % (ny + 1));
+      along = fy;
+      dim = ny;
+      upper = static_cast<size_t>(x) * ny + fy;
+      lower = upper - 1;
+    }
+    const bool has_lower = along > 0;
+    const bool has_upper = along < dim;
+    const float sp = comp == 0 ? dx : dy;
+    double value;
+    if (has_lower && has_upper) {
+      const float fm = harmonic(cond[lower], cond[upper]);
+      value = -static_cast<double>(fm) * (p[upper] - p[lower]) / static_cast<double>(sp);
+    } else {
+      const size_t site = has_upper ? upper : lower;
+      const float m = cond[site];
+      float fc = 0.0f, fp = 0.0f;
+      if (comp == axis && m != 0.0f) {
+        fc = (has_upper ? 2.0f : -2.0f) * m / sp;
+        fp = has_upper ? 1.0f : 0.0f;
+      }
+      value = static_cast<double>(fc) * (static_cast<double>(fp) - p[site]);
+    }
+    const float flux = static_cast<float>(value);
+    float v = 0.0f;
+    if (!((has_lower && solid[lower] != 0) || (has_upper && solid[upper] != 0))) v = flux / dz;
+    faces[f] = v;
+    if (!isfinite(v)) bad = 1.0;
+    peak = fmax(peak, static_cast<double>(fabsf(v)));
+    if (comp == axis && along == 0 && solid[upper] == 0 && cond[upper] > 0.0f) {
+      inlet += static_cast<double>(v);
+      count += 1.0;
+    }
+  }
+  double part[4] = {block_sum<kStream>(inlet, s_red), block_sum<kStream>(count, s_red),
+                    block_max<kStream>(peak, s_red), block_max<kStream>(bad, s_red)};
+  if (publish_partials<4>(part, partials, &st->ticket[kTicketVelocity], blockIdx.x, gridDim.x,
+                          &s_last)) {
+    double sums[2], maxes[2];
+    double tmp[4];
+    sum_partials<kStream, 2>(partials, gridDim.x, sums, s_red);
+    max_partials<kStream, 2>(partials + 2 * static_cast<size_t>(gridDim.x), gridDim.x, maxes,
+                             s_red);
+    (void)tmp;
+    if (threadIdx.x == 0) {
+      st->ticket[kTicketVelocity] = 0;
+      st->inlet_sum = sums[0];
+      st->inlet_count = sums[1];
+      st->peak = maxes[0];
+      st->nonfinite = maxes[1] > 0.0 ? 1u : 0u;
+    }
+  }
+}
+
+// scaled = v * factor + 0 (flow_system.hpp:1401-1407) and the scaled peak.
+__global__ __launch_bounds__(kStream) void k_scale(float* __restrict__ faces, size_t total,
+                                                   float factor, CgState* st, double* partials) {
+  __shared__ double s_red[kStream / kWave];
+  __shared__ bool s_last;
+  double peak = 0.0;
+  const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
+  for (size_t f = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; f < total;
+       f += stride) {
+    const float v = faces[f] * factor + 0.0f;
+    faces[f] = v;
+    peak = fmax(peak, static_cast<double>(fabsf(v)));
+  }
+  double part[1] = {block_max<kStream>(peak, s_red)};
+  if (publish_partials<1>(part, partials, &st->ticket[kTicketScale], blockIdx.x, gridDim.x,
+                          &s_last)) {
+    double m[1];
+    max_partials<kStream, 1>(partials, gridDim.x, m, s_red);
+    if (threadIdx.x == 0) {
+      st->ticket[kTicketScale] = 0;
+      st->scaled_peak = m[0];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Host orchestration
+// --------------------------------------------------------------------------
+
+uint32_t stream_blocks(size_t n) {
+  const size_t per_block = static_cast<size_t>(kStream) * 4;
+  const size_t blocks = (n + per_block - 1) / per_block;
+  return static_cast<uint32_t>(std::clamp<size_t>(blocks, 1, 4096));
+}
+
+struct MarchShape {
+  int ty;          // threads per block (64, 128 or 256)
+  uint32_t rows;   // x-rows per block
+  uint32_t blocks;
+};
+
+MarchShape march_shape(uint32_t nx, uint32_t ny) {
+  MarchShape s{};
+  s.ty = ny <= 64 ? 64 : (ny <= 128 ? 128 : 256);
+  const uint32_t nty = (ny + s.ty - 1) / s.ty;
+  // Aim for at least ~2048 blocks (8 per CU over 256 CUs) while keeping a
+  // strip long enough that the two halo rows cost little.
+  uint32_t rows = 32;
+  while (rows > 4 && static_cast<uint64_t>(nty) * ((nx + rows - 1) / rows) < 2048) rows /= 2;
+  s.rows = rows;
+  s.blocks = nty * ((nx + rows - 1) / rows);
+  return s;
+}
+
+template <typename T>
+void launch_apply(Context& ctx, const T* r, const T* p_old, T* p_new, T* q, T* x) {
+  auto& f = ctx.flow;
+  const MarchShape s = march_shape(f.nx, f.ny);
+  const T isx = static_cast<T>(1.0 / (static_cast<double>(f.dx) * f.dx));
+  const T isy = static_cast<T>(1.0 / (static_cast<double>(f.dy) * f.dy));
+  switch (s.ty) {
+    case 64:
+      k_cg_apply<T, 64><<<s.blocks, 64, 0, ctx.stream>>>(r, f.diag.data(), p_old, p_new, q, x,
+                                                         f.cxl.data(), f.cyl.data(), f.bc.data(),
+                                                         f.nx, f.ny, s.rows, isx, isy,
+                                                         f.state.data(), f.partials.data());
+      break;
+    case 128:
+      k_cg_apply<T, 128><<<s.blocks, 128, 0, ctx.stream>>>(
+          r, f.diag.data(), p_old, p_new, q, x, f.cxl.data(), f.cyl.data(), f.bc.data(), f.nx,
+          f.ny, s.rows, isx, isy, f.state.data(), f.partials.data());
+      break;
+    default:
+      k_cg_apply<T, 256><<<s.blocks, 256, 0, ctx.stream>>>(
+          r, f.diag.data(), p_old, p_new, q, x, f.cxl.data(), f.cyl.data(), f.bc.data(), f.nx,
+          f.ny, s.rows, isx, isy, f.state.data(), f.partials.data());
+  }
+  check(hipGetLastError(), "launch k_cg_apply");
+}
+
+void launch_residual(Context& ctx, const double* x, double* out, int mode) {
+  auto& f = ctx.flow;
+  const MarchShape s = march_shape(f.nx, f.ny);
+  const double isx = 1.0 / (static_cast<double>(f.dx) * f.dx);
+  const double isy = 1.0 / (static_cast<double>(f.dy) * f.dy);
+  switch (s.ty) {
+    case 64:
+      k_residual<64><<<s.blocks, 64, 0, ctx.stream>>>(x, f.rhs.data(), out, f.cxl.data(),
+                                                      f.cyl.data(), f.bc.data(), f.nx, f.ny,
+                                                      s.rows, isx, isy, mode, f.state.data(),
+                                                      f.partials.data());
+      break;
+    case 128:
+      k_residual<128><<<s.blocks, 128, 0, ctx.stream>>>(x, f.rhs.data(), out, f.cxl.data(),
+                                                        f.cyl.data(), f.bc.data(), f.nx, f.ny,
+                                                        s.rows, isx, isy, mode, f.state.data(),
+                                                        f.partials.data());
+      break;
+    default:
+      k_residual<256><<<s.blocks, 256, 0, ctx.stream>>>(x, f.rhs.data(), out, f.cxl.data(),
+                                                        f.cyl.data(), f.bc.data(), f.nx, f.ny,
+                                                        s.rows, isx, isy, mode, f.state.data(),
+                                                        f.partials.data());
+  }
+  check(hipGetLastError(), "launch k_residual");
+}
+
+void read_state(Context& ctx) {
+  check(hipMemcpyAsync(ctx.flow.host_state, ctx.flow.state.data(), sizeof(CgState),
+                       hipMemcpyDeviceToHost, ctx.stream),
+        "read solve state");
+  check(hipStreamSynchronize(ctx.stream), "synchronize solve");
+}
+
+int failure_status(uint32_t failure) {
+  return failure == 0 ? MSIM_OK : MSIM_CG_CURVATURE + static_cast<int>(failure) - 1;
+}
+
+const char* failure_text(uint32_t failure) {
+  switch (failure) {
+    case 1: return "encountered non-positive curvature";
+    case 2: return "produced a non-finite step";
+    case 3: return "produced a non-finite residual";
+    case 4: return "encountered a preconditioner breakdown";
+    case 5: return "produced a non-finite direction";
+    default: return "failed";
+  }
+}
+
+// Runs Jacobi-PCG iterations of precision T on the prepared system from the
+// state k_cg_init left, until convergence, failure, or `budget` iterations.
+// Returns the iterations taken. Throws on a CG failure.
+struct KernelTimes {
+  double apply_seconds = 0.0, update_seconds = 0.0;
+  uint32_t apply_launches = 0, update_launches = 0;
+};
+
+template <typename T>
+uint32_t run_cg(Context& ctx, T* r, T* x, T* q, T* pa, T* pb, uint32_t budget, uint32_t interval,
+                bool* converged, KernelTimes* times) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  T* buf[2] = {pa, pb};
+  uint32_t launched = 0;
+  *converged = false;
+  if (times != nullptr) {
+    // Three events per iteration: before the operator kernel, between the two
+    // kernels, after the update kernel.
+    while (ctx.probes.size() < 3 * static_cast<size_t>(interval)) {
+      hipEvent_t e;
+      check(hipEventCreate(&e), "hipEventCreate");
+      ctx.probes.push_back(e);
+    }
+  }
+  read_state(ctx);
+  if (f.host_state->done != 0) {
+    *converged = f.host_state->converged != 0;
+    return 0;
+  }
+  while (launched < budget) {
+    const uint32_t batch = std::min(interval, budget - launched);
+    for (uint32_t j = 0; j < batch; ++j) {
+      const uint32_t it = launched + j + 1;  // 1-based iteration of this solve
+      if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j], ctx.stream), "event");
+      launch_apply<T>(ctx, r, buf[(it + 1) % 2], buf[it % 2], q, x);
+      if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j + 1], ctx.stream), "event");
+      k_cg_update<T><<<stream_blocks(n), kStream, 0, ctx.stream>>>(r, q, f.diag.data(), n,
+                                                                   f.state.data(),
+                                                                   f.partials.data());
+      check(hipGetLastError(), "launch k_cg_update");
+      if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j + 2], ctx.stream), "event");
+    }
+    launched += batch;
+    read_state(ctx);
+    if (times != nullptr) {
+      // Only iterations that did work count (a converged solve's later
+      // launches return at once).
+      const uint32_t before = launched - batch;
+      const uint32_t done_iterations = f.host_state->iterations;
+      const uint32_t worked =
+          std::min(batch, done_iterations > before ? done_iterations - before : 0u);
+      for (uint32_t j = 0; j < worked; ++j) {
+        float a = 0.0f, b = 0.0f;
+        check(hipEventElapsedTime(&a, ctx.probes[3 * j], ctx.probes[3 * j + 1]), "event time");
+        check(hipEventElapsedTime(&b, ctx.probes[3 * j + 1], ctx.probes[3 * j + 2]),
+              "event time");
+        times->apply_seconds += a * 1.0e-3;
+        times->update_seconds += b * 1.0e-3;
+        times->apply_launches += 1;
+        times->update_launches += 1;
+      }
+    }
+    const CgState& s = *f.host_state;
+    if (s.failure != 0) {
+      throw Error(failure_status(s.failure),
+                  std::string("HIP depth-averaged flow conjugate gradient ") +
+                      failure_text(s.failure));
+    }
+    if (s.done != 0) break;
+  }
+  const CgState& s = *f.host_state;
+  if (s.converged != 0 && s.iterations > 0) {
+    k_cg_finalize<T><<<stream_blocks(n), kStream, 0, ctx.stream>>>(x, buf[s.iterations % 2], n,
+                                                                     f.state.data());
+    check(hipGetLastError(), "launch k_cg_finalize");
+  }
+  *converged = s.converged != 0;
+  return s.iterations;
+}
+
+}  // namespace
+
+void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  f.solid.reserve(n);
+  f.mobility.reserve(n);
+  f.cond.reserve(n);
+  for (auto* b : {&f.diag, &f.rhs, &f.bc, &f.cxl, &f.cyl}) b->reserve(n);
+  for (auto* b : {&f.x, &f.r, &f.q, &f.p0, &f.p1, &f.res}) b->reserve(n);
+  f.faces.reserve(f.x_faces() + f.y_faces());
+  // Partial sums: up to 4 values per block of any kernel here.
+  f.partials.reserve(4 * static_cast<size_t>(std::max<uint32_t>(march_shape(f.nx, f.ny).blocks, 4096)));
+  f.state.reserve(1);
+  if (f.host_state == nullptr) {
+    void* ptr = nullptr;
+    check(hipHostMalloc(&ptr, sizeof(CgState), hipHostMallocDefault), "hipHostMalloc");
+    f.host_state = static_cast<CgState*>(ptr);
+  }
+  check(hipMemsetAsync(f.state.data(), 0, sizeof(CgState), ctx.stream), "clear state");
+  check(hipMemcpyAsync(f.solid.data(), solid_host, n, hipMemcpyHostToDevice, ctx.stream),
+        "upload mask");
+  check(hipMemcpyAsync(f.mobility.data(), mobility_host, n * sizeof(float), hipMemcpyHostToDevice,
+                       ctx.stream),
+        "upload mobility");
+  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
+  k_conductance<<<blocks, 256, 0, ctx.stream>>>(f.solid.data(), f.mobility.data(), f.cond.data(),
+                                                n, f.dz);
+  check(hipGetLastError(), "launch k_conductance");
+  k_assemble<<<blocks, 256, 0, ctx.stream>>>(f.cond.data(), f.diag.data(), f.rhs.data(),
+                                             f.bc.data(), f.cxl.data(), f.cyl.data(), f.nx, f.ny,
+                                             f.dx, f.dy, f.axis);
+  check(hipGetLastError(), "launch k_assemble");
+  check(hipStreamSynchronize(ctx.stream), "assemble");
+  f.prepared = true;
+  f.solved = false;
+}
+
+void flow_apply(Context& ctx, const double* x, double* y) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  check(hipMemcpyAsync(f.x.data(), x, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream),
+        "upload vector");
+  launch_residual(ctx, f.x.data(), f.res.data(), 0);
+  check(hipMemcpyAsync(y, f.res.data(), n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+        "download vector");
+  check(hipStreamSynchronize(ctx.stream), "apply");
+  f.solved = false;
+}
+
+void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  const uint32_t interval = params.check_interval == 0 ? 64 : params.check_interval;
+  std::memset(&report, 0, sizeof(report));
+  check(hipEventRecord(ctx.start, ctx.stream), "event");
+  uint32_t iterations = 0;
+  bool converged = false;
+  double relative = 0.0;
+  KernelTimes kernel_times;
+  KernelTimes* times = params.profile != 0 ? &kernel_times : nullptr;
+  if (params.solver == MSIM_SOLVER_JACOBI_FP64) {
+    k_cg_init<double, float><<<stream_blocks(n), kStream, 0, ctx.stream>>>(
+        f.rhs.data(), 1.0, f.r.data(), f.x.data(), f.p0.data(), f.diag.data(), n,
+        params.relative_tolerance, f.state.data(), f.partials.data());
+    check(hipGetLastError(), "launch k_cg_init");
+    iterations = run_cg<double>(ctx, f.r.data(), f.x.data(), f.q.data(), f.p0.data(), f.p1.data(),
+                                params.max_iterations, interval, &converged, times);
+    const CgState& s = *f.host_state;
+    relative = s.rr0 > 0.0 ? std::sqrt(std::max(0.0, s.rr)) / std::sqrt(s.rr0) : 0.0;
+    if (!converged) {
+      throw Error(MSIM_NOT_CONVERGED,
+                  "HIP depth-averaged flow conjugate gradient did not converge: relative " +
+                      std::to_string(relative));
+    }
+    launch_residual(ctx, f.x.data(), f.res.data(), 1);
+    read_state(ctx);
+  } else if (params.solver == MSIM_SOLVER_MIXED) {
+    for (auto* b : {&f.x32, &f.r32, &f.q32, &f.p032, &f.p132}) b->reserve(n);
+    const double inner = params.inner_tolerance > 0.0 ? params.inner_tolerance : 1.0e-4;
+    // x = 0, res = b, ||b||.
+    check(hipMemsetAsync(f.x.data(), 0, n * sizeof(double), ctx.stream), "clear x");
+    launch_residual(ctx, f.x.data(), f.res.data(), 1);
+    read_state(ctx);
+    const double bnorm = std::sqrt(f.host_state->bb);
+    double rnorm = std::sqrt(f.host_state->true_rr);
+    uint32_t refinements = 0;
+    if (bnorm == 0.0) {
+      converged = true;
+    }
+    while (!converged) {
+      if (rnorm <= params.relative_tolerance * bnorm) {
+        converged = true;
+        break;
+      }
+      if (iterations >= params.max_iterations || refinements >= 64) break;
+      // Inner FP32 solve of A e = res / ||res|| to `inner`.
+      k_cg_init<float, double><<<stream_blocks(n), kStream, 0, ctx.stream>>>(
+          f.res.data(), 1.0 / rnorm, f.r32.data(), f.x32.data(), f.p032.data(), f.diag.data(), n,
+          inner, f.state.data(), f.partials.data());
+      check(hipGetLastError(), "launch k_cg_init");
+      bool inner_converged = false;
+      const uint32_t taken = run_cg<float>(ctx, f.r32.data(), f.x32.data(), f.q32.data(),
+                                           f.p032.data(), f.p132.data(),
+                                           params.max_iterations - iterations, interval,
+                                           &inner_converged, times);
+      iterations += taken;
+      ++refinements;
+      k_refine<<<stream_blocks(n), kStream, 0, ctx.stream>>>(f.x.data(), f.x32.data(), rnorm, n);
+      check(hipGetLastError(), "launch k_refine");
+      launch_residual(ctx, f.x.data(), f.res.data(), 1);
+      read_state(ctx);
+      const double next = std::sqrt(f.host_state->true_rr);
+      if (!std::isfinite(next)) {
+        throw Error(MSIM_CG_RESIDUAL, "HIP mixed-precision refinement produced a non-finite residual");
+      }
+      rnorm = next;
+      if (!inner_converged && taken == 0) break;
+    }
+    report.refinements = refinements;
+    relative = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+    if (!converged) {
+      throw Error(MSIM_NOT_CONVERGED,
+                  "HIP mixed-precision depth-averaged flow did not converge: relative " +
+                      std::to_string(relative));
+    }
+  } else {
+    throw Error(MSIM_UNSUPPORTED, "this pressure solver form is not available");
+  }
+  const CgState& s = *f.host_state;
+  report.iterations = iterations;
+  report.relative_residual = relative;
+  report.true_relative_residual = s.bb > 0.0 ? std::sqrt(s.true_rr / s.bb) : 0.0;
+
+  // Velocity reconstruction, lift and scaling.
+  const size_t nfaces = f.x_faces() + f.y_faces();
+  k_velocity<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(
+      f.x.data(), f.cond.data(), f.solid.data(), f.faces.data(), f.nx, f.ny, f.dx, f.dy, f.dz,
+      f.axis, f.state.data(), f.partials.data());
+  check(hipGetLastError(), "launch k_velocity");
+  read_state(ctx);
+  if (s.nonfinite != 0) throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+  if (s.inlet_count == 0.0) throw Error(MSIM_RUNTIME_ERROR, "flow solve has no open inlet faces");
+  const float solved_mean = static_cast<float>(s.inlet_sum / s.inlet_count);
+  const float peak = static_cast<float>(s.peak);
+  if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
+    throw Error(MSIM_RUNTIME_ERROR,
+                "the device carries no through-flow: the outlet is unreachable");
+  }
+  float factor = 1.0f;
+  float max_speed = peak;
+  if (params.scale != 0) {
+    const float driven_mean = static_cast<float>(0.0 / s.inlet_count);
+    factor = (params.mean_inlet_speed - driven_mean) / solved_mean;
+    k_scale<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(f.faces.data(), nfaces, factor,
+                                                              f.state.data(), f.partials.data());
+    check(hipGetLastError(), "launch k_scale");
+    read_state(ctx);
+    max_speed = static_cast<float>(f.host_state->scaled_peak);
+  }
+  check(hipEventRecord(ctx.stop, ctx.stream), "event");
+  check(hipEventSynchronize(ctx.stop), "event sync");
+  float ms = 0.0f;
+  check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+  report.mean_inlet_speed = params.scale != 0 ? params.mean_inlet_speed : solved_mean;
+  report.max_speed = max_speed;
+  report.solved_mean = solved_mean;
+  report.factor = factor;
+  report.seconds = ms * 1.0e-3;
+  report.apply_seconds = kernel_times.apply_seconds;
+  report.apply_launches = kernel_times.apply_launches;
+  report.update_seconds = kernel_times.update_seconds;
+  report.update_launches = kernel_times.update_launches;
+  f.solved = true;
+}
+
+}  // namespace msim

@@ -1,0 +1,183 @@
+// Internal host-side declarations of the HIP backend: device buffers, the
+// context, and the launchers each .hip file provides. Errors inside the
+// library are C++ exceptions (Error) that the C ABI turns into status codes;
+// none crosses the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "msim_hip.h"
+
+namespace msim {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string& message) : std::runtime_error(message), code_(code) {}
+  [[nodiscard]] int code() const noexcept { return code_; }
+
+ private:
+  int code_;
+};
+
+inline void check(hipError_t status, const char* what) {
+  if (status != hipSuccess) {
+    throw Error(MSIM_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(status));
+  }
+}
+
+// Device memory owned by one context. Reallocated only when it must grow.
+template <typename T>
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+
+  void reserve(std::size_t count) {
+    if (count <= capacity_) {
+      count_ = count;
+      return;
+    }
+    release();
+    void* ptr = nullptr;
+    check(hipMalloc(&ptr, (count == 0 ? 1 : count) * sizeof(T)), "hipMalloc");
+    data_ = static_cast<T*>(ptr);
+    capacity_ = count;
+    count_ = count;
+  }
+  void release() {
+    if (data_ != nullptr) static_cast<void>(hipFree(data_));
+    data_ = nullptr;
+    capacity_ = 0;
+    count_ = 0;
+  }
+  [[nodiscard]] T* data() const noexcept { return data_; }
+  [[nodiscard]] std::size_t size() const noexcept { return count_; }
+
+ private:
+  T* data_ = nullptr;
+  std::size_t capacity_ = 0;
+  std::size_t count_ = 0;
+};
+
+// The device-side state of one Krylov solve. One instance lives in device
+// memory; the host reads it back between batches of iterations.
+struct CgState {
+  double rho;        // r . z of the current iterate
+  double alpha;      // step length of the last iteration
+  double beta;       // weight of the previous direction in the next one
+  double rr;         // r . r after the last iteration
+  double curvature;  // p . A p of the last iteration
+  double threshold;  // (tolerance * ||b||)^2
+  double rr0;        // ||b||^2 of this solve
+  double true_rr;    // ||b - A x||^2 from the residual kernel
+  double bb;         // ||b||^2 from the residual kernel
+  double inlet_sum;  // velocity reconstruction: sum over open inlet faces
+  double inlet_count;
+  double peak;       // largest |velocity|
+  double scaled_peak;
+  float factor;      // scaling factor, device copy
+  uint32_t nonfinite;
+  uint32_t iterations;
+  uint32_t converged;
+  uint32_t failure;  // ConjugateGradientFailure (conjugate_gradient.hpp:91-98)
+  uint32_t done;
+  uint32_t ticket[8];
+};
+
+enum Ticket : int {
+  kTicketInit = 0,
+  kTicketApply = 1,
+  kTicketUpdate = 2,
+  kTicketResidual = 3,
+  kTicketVelocity = 4,
+  kTicketScale = 5,
+  kTicketAdr = 6,
+  kTicketMisc = 7,
+};
+
+constexpr int kMaxSignals = 8;
+
+// Everything the prepared depth-averaged system keeps on the device.
+struct FlowDevice {
+  uint32_t nx = 0, ny = 0, axis = 0;
+  float dx = 1, dy = 1, dz = 1;
+  bool prepared = false;
+  bool solved = false;
+  DeviceBuffer<uint8_t> solid;     // original obstacle mask
+  DeviceBuffer<float> mobility;    // uploaded per-site mobility
+  DeviceBuffer<float> cond;        // column conductance (reduction)
+  DeviceBuffer<float> diag, rhs, bc, cxl, cyl;
+  DeviceBuffer<double> x, r, q, p0, p1, res;
+  DeviceBuffer<float> x32, r32, q32, p032, p132;
+  DeviceBuffer<double> partials;
+  DeviceBuffer<CgState> state;
+  DeviceBuffer<float> faces;       // x faces then y faces (z faces are zero)
+  CgState* host_state = nullptr;   // pinned readback
+  [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(nx) * ny; }
+  [[nodiscard]] std::size_t x_faces() const { return static_cast<std::size_t>(nx + 1) * ny; }
+  [[nodiscard]] std::size_t y_faces() const { return static_cast<std::size_t>(nx) * (ny + 1); }
+};
+
+// Transport constants passed to the kernels by value.
+struct AdrParams {
+  uint32_t nx, ny, signals;
+  float dx, dy, dz;
+  uint8_t kind[6];
+  uint8_t has_field, has_affine, has_monod;
+  float diffusion[kMaxSignals];
+  float advection[kMaxSignals][3];
+  float boundary[kMaxSignals][6];
+  float vmax[kMaxSignals];
+  float monod_k[kMaxSignals];
+};
+
+struct AdrDevice {
+  AdrParams params{};
+  uint32_t precision = 32;
+  bool uploaded = false;
+  bool flow_field = false;
+  uint64_t revision = 0;
+  double greatest_outflow = 0.0;
+  double max_loss[kMaxSignals] = {};
+  DeviceBuffer<uint8_t> solid;
+  DeviceBuffer<float> xf, yf, phi, source, loss;
+  DeviceBuffer<uint8_t> levels_a, levels_b;  // S*N of float or double
+  int current = 0;
+  DeviceBuffer<double> partials, ledger, mass;
+  DeviceBuffer<uint32_t> flags;              // [0]: error bits, [1]: first failed step
+  DeviceBuffer<CgState> state;
+  double seconds = 0.0;
+  [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(params.nx) * params.ny; }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t start = nullptr, stop = nullptr;
+  std::vector<hipEvent_t> probes;  // per-launch timing events (profiled solves)
+  std::string error;
+  std::string name;
+  FlowDevice flow;
+  AdrDevice adr;
+};
+
+// flow.hip
+void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host);
+void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report);
+void flow_apply(Context& ctx, const double* x, double* y);
+
+// adr.hip
+void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
+                uint32_t precision, bool use_flow_field);
+void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger);
+void adr_download(Context& ctx, void* levels);
+
+}  // namespace msim

@@ -1,0 +1,117 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own CPU
+path (oracle/_ref/libmsim_ref.so, built from the reference sources by
+`make -C oracle ref`). Run in a container that holds the reference tree:
+
+    python tests/golden/make_golden.py
+
+Fixtures (inputs and the reference's outputs, nothing else):
+  c1_flow.npz      channel + trap 256x128, seed 42: solve_depth_averaged_flow_cpu
+                   at tolerance 1.5e-7 (the reference floor is float epsilon),
+                   mean inlet speed 100; and detail::solve_depth_averaged_pressure
+                   at 1e-10 (no floor there).
+  c1_adr.npz       20 forward-Euler steps of one nutrient over that field, D=500,
+                   inlet 1, Monod (vmax 1, K 0.1) as the affine loss it equals
+                   under forward Euler, recomputed from the levels each step.
+  illcond_flow.npz the reference conformance suite's 64x64 pillar sheet with
+                   mobility 10^U(-3,0) from its LCG (seed 12345)
+                   (tests/conformance/flow_conformance_test.cpp:94-116).
+  small_cases.npz  a 5x8 obstacle case with mobility 0.4+0.1x
+                   (flow_conformance_test.cpp:62-88 pattern, one layer) and a
+                   3-site diffusion line (signal_grid_test.cpp:90-103).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import FIXED, NO_FLUX, Grid, Reference, build  # noqa: E402
+from microsimulator_amd.workloads import config  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def c1():
+    w = config("c1")
+    grid = Grid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    ref = Reference()
+    faces, rep = ref.solve_flow(grid, w.mobility, 0, 100.0, 1.5e-7, 50000)
+    system = ref.flow_system(grid, w.mobility, 0)
+    pressure, _ = system.pressure(1e-10, 100000)
+    arrays = system.arrays()
+    np.savez_compressed(
+        os.path.join(OUT, "c1_flow.npz"), obstacles=np.packbits(w.obstacles.reshape(-1)),
+        mobility=w.mobility.reshape(-1), phi=w.phi.reshape(-1), nx=w.nx, ny=w.ny, dz=w.dz,
+        faces=faces, iterations=rep["iterations"], relative_residual=rep["relative_residual"],
+        max_speed=rep["max_speed"], pressure=pressure, diagonal=arrays["diagonal"],
+        rhs=arrays["rhs"], bc=arrays["bc"])
+    # Transport over the reference's field.
+    S = 1
+    bv = np.zeros(6 * S, np.float32)
+    bv[0] = 1.0
+    D = np.array([500.0], np.float32)
+    vmax, K = np.float32(1.0), np.float32(0.1)
+    dt = np.float32(4.0e-4)
+    phi = w.phi.reshape(-1)
+    levels = np.zeros(w.sites, np.float32)
+    for _ in range(20):
+        loss = ((vmax * phi) / (K + levels)).astype(np.float32)
+        levels, _ = ref.advance_signal_grid(grid, levels, dt, 1, diffusion=D, boundary_values=bv,
+                                            faces=faces, loss=loss)
+    np.savez_compressed(os.path.join(OUT, "c1_adr.npz"), levels=levels, dt=dt, diffusion=D,
+                        boundary_values=bv, vmax=vmax, K=K, steps=20)
+
+
+def illcond():
+    size = 64
+    solid = np.zeros((size, size), np.uint8)
+    for px in range(8, size, 16):
+        for py in range(12, size - 12, 16):
+            solid[px:px + 4, py:py + 4] = 1
+    flat = solid.reshape(-1)
+    mobility = np.zeros(size * size, np.float32)
+    state = 12345
+    for site in range(size * size):
+        state = (state * 1664525 + 1013904223) & 0xFFFFFFFF
+        exponent = np.float32(-3.0) * np.float32(state >> 8) / np.float32(16777216.0)
+        mobility[site] = 0.0 if flat[site] else np.float32(10.0) ** exponent
+    # The conformance case flows along y with fixed y faces.
+    grid = Grid(size, size, 1, 1.0, 1.0, 1.0, solid,
+                boundary=(NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX))
+    ref = Reference()
+    faces, rep = ref.solve_flow(grid, mobility, 1, 1.0, 1e-6, 50000)
+    np.savez_compressed(os.path.join(OUT, "illcond_flow.npz"), obstacles=flat, mobility=mobility,
+                        faces=faces, iterations=rep["iterations"], max_speed=rep["max_speed"])
+
+
+def small():
+    nx, ny = 5, 8
+    solid = np.zeros((nx, ny), np.uint8)
+    solid[2, 3] = 1
+    x = np.arange(nx, dtype=np.float32)[:, None] * np.ones((1, ny), np.float32)
+    mobility = (np.float32(0.4) + np.float32(0.1) * x).astype(np.float32).reshape(-1)
+    grid = Grid(nx, ny, 1, 1.0, 1.0, 1.0, solid,
+                boundary=(NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX))
+    ref = Reference()
+    faces, rep = ref.solve_flow(grid, mobility, 1, 1.0, 1e-6, 50000)
+    line = Grid(3, 1, 1, 1.0, 1.0, 1.0, None,
+                boundary=(NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX))
+    levels, _ = ref.advance_signal_grid(line, np.array([0, 1, 0], np.float32), 0.25, 1,
+                                        diffusion=(1.0,))
+    np.savez_compressed(os.path.join(OUT, "small_cases.npz"), obstacles=solid.reshape(-1),
+                        mobility=mobility, faces=faces, iterations=rep["iterations"],
+                        line_levels=levels)
+
+
+if __name__ == "__main__":
+    build(reference=True)
+    c1()
+    illcond()
+    small()
+    for name in sorted(os.listdir(OUT)):
+        if name.endswith(".npz"):
+            print(name, os.path.getsize(os.path.join(OUT, name)))

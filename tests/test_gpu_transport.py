@@ -1,0 +1,137 @@
+"""The explicit transport step on the GPU through the C ABI.
+
+FP32 steps are bit-exact with the oracle, which is bit-exact with the
+reference (same float operations, no contraction). FP64 steps agree with the
+oracle's FP64 face-flux step to 1e-13 relative and conserve mass to 1e-12
+relative per step.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import microsimulator_amd as msa
+from oracle import Grid as OGrid
+from tests.helpers import FIXED, NO_FLUX, PERIODIC, c1_inputs, golden, split_faces
+
+pytestmark = pytest.mark.gpu
+
+
+def c1_spec(revision=0):
+    c = c1_inputs()
+    d = golden("c1_adr.npz")
+    faces = c["data"]["faces"]
+    xf, yf, zf = split_faces(faces, c["nx"], c["ny"])
+    grid = msa.Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    spec = msa.SignalGridSpec(grid, d["diffusion"], d["boundary_values"],
+                              msa.VelocityField(xf, yf, zf), phi=c["phi"],
+                              monod_vmax=[float(d["vmax"])], monod_k=[float(d["K"])],
+                              revision=revision)
+    return spec, c, d
+
+
+def test_monod_transport_matches_reference_fixture(hip):
+    spec, c, d = c1_spec(revision=101)
+    levels = np.zeros(spec.grid.sites, np.float32)
+    for _ in range(int(d["steps"])):
+        levels, rep = hip.advance_signal_grid(spec, levels, float(d["dt"]))
+    assert rep.converged
+    assert np.array_equal(levels, d["levels"])
+
+
+def test_device_resident_run_equals_single_steps(hip):
+    spec, c, d = c1_spec(revision=102)
+    levels = np.zeros(spec.grid.sites, np.float32)
+    hip.adr_upload(spec, levels, precision=32)
+    hip.adr_run(float(d["dt"]), int(d["steps"]), ledger=False)
+    assert np.array_equal(hip.adr_download(), d["levels"])
+
+
+@pytest.mark.parametrize("kinds", [
+    (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX),
+    (FIXED, NO_FLUX, PERIODIC, PERIODIC, NO_FLUX, NO_FLUX),
+    (PERIODIC, PERIODIC, FIXED, FIXED, NO_FLUX, NO_FLUX),
+])
+def test_affine_transport_is_bit_exact(hip, oracle, kinds):
+    rng = np.random.default_rng(5)
+    nx, ny, S = 40, 70, 3
+    solid = (rng.random(nx * ny) < 0.15).astype(np.uint8)
+    levels = rng.random(S * nx * ny).astype(np.float32)
+    mask = np.tile(solid, S) != 0
+    levels[mask] = 0
+    loss = (rng.random(S * nx * ny) * 0.5).astype(np.float32)
+    source = (rng.random(S * nx * ny) * 0.2).astype(np.float32)
+    loss[mask] = 0
+    source[mask] = 0
+    bv = rng.random(6 * S).astype(np.float32)
+    diffusion = np.array([0.3, 0.05, 0.1], np.float32)
+    grid = msa.Grid(nx, ny, 1, 1.0, 0.5, 1.0, solid, kinds)
+    spec = msa.SignalGridSpec(grid, diffusion, bv, source_rates=source, loss_rates=loss)
+    ours = levels
+    for _ in range(3):
+        ours, _ = hip.advance_signal_grid(spec, ours, 0.05)
+    og = OGrid(nx, ny, 1, 1.0, 0.5, 1.0, solid, kinds)
+    ref = oracle.adr_step_f32(og, levels, 0.05, steps=3, diffusion=diffusion,
+                              boundary_values=bv, source=source, loss=loss)
+    assert np.array_equal(ours, ref)
+
+
+def test_explicit_bound_refuses_and_leaves_levels(hip):
+    line = msa.Grid(3, 1, 1, boundary=(NO_FLUX,) * 6)
+    spec = msa.SignalGridSpec(line, [1.0])
+    levels, _ = hip.advance_signal_grid(spec, [0.0, 1.0, 0.0], 0.25)
+    assert np.allclose(levels, [0.25, 0.5, 0.25], atol=1e-6)
+    with pytest.raises(msa.InvalidArgument, match="explicit stability bound"):
+        hip.advance_signal_grid(spec, levels, 0.51)
+    one = msa.Grid(1, 1, 1, boundary=(NO_FLUX,) * 6)
+    spec = msa.SignalGridSpec(one, [0.0], source_rates=[2.0], loss_rates=[0.5])
+    c, _ = hip.advance_signal_grid(spec, [1.0], 0.5)
+    assert abs(c[0] - 1.75) <= 1e-6
+    with pytest.raises(msa.InvalidArgument):
+        hip.advance_signal_grid(spec, [-1.0], 0.5)
+
+
+def test_fp64_transport_conserves_and_matches_oracle(hip, oracle):
+    spec, c, d = c1_spec(revision=103)
+    hip.adr_upload(spec, np.zeros(spec.grid.sites), precision=64)
+    ledger = hip.adr_run(float(d["dt"]), 20)
+    ours = hip.adr_download()
+    og = OGrid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    ref, ref_ledger = oracle.adr_step_f64(og, np.zeros(spec.grid.sites), float(d["dt"]),
+                                          steps=20, diffusion=d["diffusion"],
+                                          boundary_values=d["boundary_values"],
+                                          faces=c["data"]["faces"], phi=c["phi"],
+                                          vmax=d["vmax"], monod_k=d["K"])
+    assert np.abs(ours - ref).max() <= 1e-13 * np.abs(ref).max()
+    balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+    print("worst per-step imbalance / mass", np.abs(balance).max() / ledger[:, :, 1].max(),
+          flush=True)
+    assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+    assert np.allclose(ledger, ref_ledger, rtol=1e-12, atol=1e-300)
+
+
+def test_coupled_flow_then_transport_on_device(hip):
+    """Flow solve, then three solutes advected by the field left on the device,
+    FP64 state, mass ledger every step."""
+    from microsimulator_amd.workloads import config
+    w = config("c2")
+    g = msa.Grid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    hip.flow_prepare(g, w.mobility, 0)
+    rep = hip.flow_solve(msa.DepthAveragedFlowParameters(mean_inlet_speed=100.0, axis=0,
+                                                         relative_tolerance=1e-8,
+                                                         solver=msa.SOLVER_MIXED))
+    S = 3
+    bv = np.zeros(6 * S, np.float32)
+    bv[0::6] = 1.0
+    diffusion = np.array([100.0, 500.0, 1000.0], np.float32)
+    spec = msa.SignalGridSpec(g, diffusion, bv, phi=w.phi, monod_vmax=[1.0] * S,
+                              monod_k=[0.1] * S)
+    hip.adr_upload(spec, np.zeros(S * g.sites), precision=64, use_flow_field=True)
+    bound = 2 * 1000.0 * 2 + 2 * rep.max_speed * 2 + 7.0 / 0.1 * 1.0
+    dt = 0.5 / bound
+    ledger = hip.adr_run(dt, 50)
+    levels = hip.adr_download().reshape(S, -1)
+    balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+    assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+    assert np.all(levels >= 0) and np.all(levels <= 1.0 + 1e-12)
+    assert np.all(levels[:, w.obstacles.reshape(-1) != 0] == 0)
