@@ -1,0 +1,287 @@
+"""Benchmark: pressure solves/s at relative residual 1e-8 and transport
+cell-updates/s on one MI355X, on the 4096 x 2048 pillar-array channel
+(~4k seeded pillars r ~ U[8,20], phi from 500 seeded Gaussian blobs).
+
+A step is one pressure solve from the assembled system resident in HBM to an
+FP64 true residual ||b - Ax||/||b|| <= 1e-8, including the final true-residual
+check and the velocity reconstruction (lift and scaling). The headline solver
+form is FP32 Jacobi-PCG inside FP64 iterative refinement; the plain FP64
+Jacobi-PCG (the reference's algorithm) is timed beside it on the same GPU.
+Transport: 3 solutes (D = 100, 500, 1000 um^2/s) with Monod uptake, FP64
+state, advected by the solved field, FP64 mass ledger every step.
+
+Prints ONE JSON line. Usage:
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "pressure solves/s at rel-res 1e-8 (learned vs plain PCG); ADR Gcell-upd/s"
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def apply_bytes(precision: int) -> int:
+    v = 8 if precision == 64 else 4
+    reads = 3 * v + 4 * 4  # r, p_old, x (vectors); diag, cxl, cyl, bc (float)
+    writes = 3 * v  # p_new, q, x
+    return reads + writes
+
+
+def update_bytes(precision: int) -> int:
+    v = 8 if precision == 64 else 4
+    return 2 * v + 4 + v  # read r, q, diag; write r
+
+
+def adr_bytes_per_site(signals: int, state_bytes: int) -> int:
+    """Levels in and out per signal, two face velocities, mask, phi."""
+    return signals * 2 * state_bytes + 2 * 4 + 1 + 4
+
+
+def cpu_baseline(w, fp64_iterations: int, threads: int, budget_s: float):
+    """The reference's own FP64 Jacobi-PCG (oracle/_ref, built from its
+    sources) on this host, timed over a bounded number of iterations of the
+    same system; solves/s = 1 / (seconds per iteration x the iterations the
+    same algorithm needs on this system, from the GPU's FP64 solve)."""
+    from oracle import Grid as OGrid
+    from oracle import Reference
+    grid = OGrid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    if Reference.available():
+        ref = Reference()
+        ref.set_threads(threads)
+        system = ref.flow_system(grid, w.mobility, 0)
+        t1 = system.time_iterations(5)
+        k = int(max(10, min(2000, budget_s / max(t1 / 5, 1e-6))))
+        t = system.time_iterations(k)
+        per_iteration = t / k
+        return dict(value=1.0 / (per_iteration * fp64_iterations), unit="solves/s",
+                    cores=ref.threads(), kind="reference",
+                    sample=(f"{k} FP64 Jacobi-PCG iterations of the reference's CPU solver "
+                            f"(detail::solve_depth_averaged_pressure) on the same 4096x2048 "
+                            f"system in {t:.2f} s, scaled by the {fp64_iterations} iterations "
+                            f"the same algorithm takes to 1e-8"),
+                    seconds_per_iteration=per_iteration)
+    from oracle import Oracle, OracleError
+    oracle = Oracle()
+    k = 20
+    t0 = time.perf_counter()
+    try:
+        oracle.solve_flow(grid, w.mobility, 0, 100.0, 1e-30, k)
+    except OracleError:
+        pass
+    t = time.perf_counter() - t0
+    return dict(value=1.0 / (t / k * fp64_iterations), unit="solves/s", cores=1, kind="port",
+                sample=f"{k} iterations of the C oracle (assembly included) in {t:.2f} s",
+                seconds_per_iteration=t / k)
+
+
+def cpu_adr_baseline(w, faces, threads: int, steps: int):
+    """The reference's explicit transport (advance_signal_grid_cpu) for 3
+    solutes with the Monod loss as its affine reaction, `steps` steps."""
+    from oracle import FIXED, NO_FLUX, Reference
+    from oracle import Grid as OGrid
+    if not Reference.available():
+        return None
+    ref = Reference()
+    ref.set_threads(threads)
+    grid = OGrid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles,
+                 boundary=(FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX))
+    S = 3
+    n = w.sites
+    bv = np.zeros(6 * S, np.float32)
+    bv[0::6] = 1.0
+    loss = np.tile((np.float32(1.0) * w.phi.reshape(-1)) / np.float32(0.1 + 0.5), S)
+    levels = np.full(S * n, 0.5, np.float32)
+    levels[np.tile(w.obstacles.reshape(-1), S) != 0] = 0.0
+    _, secs = ref.advance_signal_grid(grid, levels, 1e-6, steps,
+                                      diffusion=np.array([100.0, 500.0, 1000.0], np.float32),
+                                      boundary_values=bv, faces=faces, loss=loss)
+    return dict(value=w.fluid * S * steps / secs / 1e9, unit="Gcell-upd/s",
+                cores=ref.threads(), kind="reference",
+                sample=f"{steps} steps of advance_signal_grid_cpu, 3 solutes, in {secs:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c5")
+    ap.add_argument("--adr-steps", type=int, default=100)
+    ap.add_argument("--fp64-solves", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the path is single-GPU; multi-GPU sharding is design only).", flush=True)
+        sys.exit(2)
+
+    import microsimulator_amd as msa
+    from microsimulator_amd.workloads import config
+
+    t0 = time.perf_counter()
+    w = config(args.workload)
+    log(f"workload {args.workload}: {w.nx}x{w.ny}, fluid {w.fluid}, pillars "
+        f"{w.meta.get('pillars')}, generated in {time.perf_counter() - t0:.1f} s")
+    grid = msa.Grid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    hip = msa.HipBackend(0)
+    log("device", hip.info().device)
+    hip.flow_prepare(grid, w.mobility, 0)
+    tol = 1e-8
+    speed = 100.0
+    mixed = msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
+                                            relative_tolerance=tol, max_iterations=400000,
+                                            solver=msa.SOLVER_MIXED)
+    fp64 = msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
+                                           relative_tolerance=tol, max_iterations=400000,
+                                           solver=msa.SOLVER_JACOBI_FP64)
+
+    for k in range(args.warmup):
+        rep = hip.flow_solve(mixed)
+        log(f"warmup {k}: {rep.iterations} it, {rep.refinements} refinements, "
+            f"{rep.seconds:.3f} s, true {rep.true_relative_residual:.2e}")
+    reports = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        reports.append(hip.flow_solve(mixed))
+    elapsed = time.perf_counter() - t0
+    for k, rep in enumerate(reports):
+        log(f"step {k}: {rep.iterations} it, {rep.refinements} ref, {rep.seconds:.3f} s, "
+            f"true {rep.true_relative_residual:.2e}")
+    solves_per_s = args.steps / elapsed
+    worst_true = max(r.true_relative_residual for r in reports)
+
+    # Plain FP64 Jacobi-PCG, the reference's algorithm, on the same GPU.
+    t0 = time.perf_counter()
+    fp64_reports = [hip.flow_solve(fp64) for _ in range(args.fp64_solves)]
+    fp64_elapsed = time.perf_counter() - t0
+    fp64_rep = fp64_reports[-1]
+    fp64_solves_per_s = args.fp64_solves / fp64_elapsed
+    log(f"fp64 jacobi-pcg: {fp64_rep.iterations} it, {fp64_rep.seconds:.3f} s, "
+        f"true {fp64_rep.true_relative_residual:.2e}")
+
+    # One profiled solve: every Krylov kernel launch timed with HIP events on
+    # the context's stream.
+    prof = hip.flow_solve(msa.DepthAveragedFlowParameters(
+        mean_inlet_speed=speed, axis=0, relative_tolerance=tol, max_iterations=400000,
+        solver=msa.SOLVER_MIXED, profile=True))
+    apply_avg = prof.apply_seconds / max(prof.apply_launches, 1)
+    update_avg = prof.update_seconds / max(prof.update_launches, 1)
+    sites = w.sites
+    apply_traffic = apply_bytes(32) * sites
+    achieved = apply_traffic / apply_avg / 1e9
+    log(f"k_cg_apply<float>: {prof.apply_launches} launches, avg {apply_avg * 1e6:.1f} us, "
+        f"{achieved:.0f} GB/s; k_cg_update<float>: avg {update_avg * 1e6:.1f} us, "
+        f"{update_bytes(32) * sites / update_avg / 1e9:.0f} GB/s")
+    pressure, field = hip.flow_download()
+
+    # Transport over the solved field, FP64 state and ledger.
+    S = 3
+    bv = np.zeros(6 * S, np.float32)
+    bv[0::6] = 1.0
+    diffusion = np.array([100.0, 500.0, 1000.0], np.float32)
+    spec = msa.SignalGridSpec(grid, diffusion, bv, phi=w.phi, monod_vmax=[1.0] * S,
+                              monod_k=[0.1] * S)
+    hip.adr_upload(spec, np.zeros(S * sites), precision=64, use_flow_field=True)
+    # 0.9 of the explicit bound: 2 D sum(1/h^2) + greatest outflow + max loss.
+    outflow_bound = 2.0 * float(np.abs(field.x_faces).max() + np.abs(field.y_faces).max())
+    bound = 2 * 1000.0 * 2 + outflow_bound + 1.0 * float(w.phi.max()) / 0.1
+    dt = 0.9 / bound
+    ledger = hip.adr_run(dt, args.adr_steps)
+    adr_s = hip.adr_seconds()
+    balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+    scale = np.maximum(ledger[:, :, 1], 1e-300)
+    worst_balance = float(np.max(np.abs(balance) / scale))
+    adr_gcups = w.fluid * S * args.adr_steps / adr_s / 1e9
+    adr_gbs = adr_bytes_per_site(S, 8) * sites / (adr_s / args.adr_steps) / 1e9
+    hip.adr_upload(spec, np.zeros(S * sites, np.float32), precision=32, use_flow_field=True)
+    hip.adr_run(dt, args.adr_steps, ledger=False)
+    adr32_s = hip.adr_seconds()
+    adr32_gcups = w.fluid * S * args.adr_steps / adr32_s / 1e9
+    log(f"transport fp64: {adr_gcups:.2f} Gcell-upd/s ({adr_gbs:.0f} GB/s), worst per-step "
+        f"imbalance {worst_balance:.2e}; fp32: {adr32_gcups:.2f} Gcell-upd/s")
+
+    line = {
+        "metric": METRIC,
+        "value": round(solves_per_s, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"pillar-array channel {w.nx}x{w.ny}, {w.meta.get('pillars')} pillars "
+                         f"r~U[8,20], phi from {w.meta.get('blobs')} Gaussian blobs, "
+                         f"Dirichlet 1/0 along x, rel-res 1e-8 (FP64 true residual)"),
+            "solver": "FP32 Jacobi-PCG inside FP64 iterative refinement",
+            "sites": sites,
+            "fluid_sites": w.fluid,
+        },
+        "solve": {
+            "iterations": reports[-1].iterations,
+            "refinements": reports[-1].refinements,
+            "worst_true_relative_residual": worst_true,
+            "device_seconds": reports[-1].seconds,
+        },
+        "fp64_jacobi_pcg": {
+            "solves_per_s": round(fp64_solves_per_s, 5),
+            "iterations": fp64_rep.iterations,
+            "true_relative_residual": fp64_rep.true_relative_residual,
+            "speedup_of_headline": round(solves_per_s / fp64_solves_per_s, 3),
+        },
+        "adr": {
+            "gcell_upd_per_s_fp64": round(adr_gcups, 3),
+            "gcell_upd_per_s_fp32": round(adr32_gcups, 3),
+            "solutes": S,
+            "steps": args.adr_steps,
+            "dt": dt,
+            "worst_step_mass_imbalance": worst_balance,
+            "achieved_gbs_fp64": round(adr_gbs, 1),
+        },
+        "roofline": {
+            "kernel": "k_cg_apply<float> (operator + direction + solution update)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_site": apply_bytes(32),
+            "sites_per_launch": sites,
+            "avg_launch_us": round(apply_avg * 1e6, 2),
+            "launches": prof.apply_launches,
+        },
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        line["cpu_baseline"] = cpu_baseline(w, fp64_rep.iterations, threads, args.cpu_budget)
+        faces = field.concatenated()
+        adr_cpu = cpu_adr_baseline(w, faces, threads, steps=3)
+        if adr_cpu is not None:
+            line["cpu_baseline_adr"] = adr_cpu
+    hip.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
