@@ -124,7 +124,20 @@ def main():
     ap.add_argument("--fp64-solves", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--kernel-iterations", type=int, default=0,
+                    help="profiling run: only N iterations of each solver form, then exit")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="profiling calibration: copy this many MiB device to device, 3 times")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_probe} MiB x 3")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
               "(the path is single-GPU; multi-GPU sharding is design only).", flush=True)
@@ -143,6 +156,16 @@ def main():
     hip.flow_prepare(grid, w.mobility, 0)
     tol = 1e-8
     speed = 100.0
+    if args.kernel_iterations:
+        for solver in (msa.SOLVER_MIXED, msa.SOLVER_JACOBI_FP64):
+            try:
+                hip.flow_solve(msa.DepthAveragedFlowParameters(
+                    mean_inlet_speed=speed, axis=0, relative_tolerance=tol,
+                    max_iterations=args.kernel_iterations, solver=solver))
+            except msa.NotConverged:
+                pass
+        log(f"ran {args.kernel_iterations} iterations of each solver form")
+        return
     mixed = msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
                                             relative_tolerance=tol, max_iterations=400000,
                                             solver=msa.SOLVER_MIXED)

@@ -90,6 +90,8 @@ __device__ inline SiteGeometry geometry(const AdrParams& P, const uint8_t* __res
   return g;
 }
 
+// One FP32 forward-Euler step for S signals; one site per thread.
+template <int S>
 __global__ __launch_bounds__(kAdrBlock) void k_adr_f32(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
@@ -101,15 +103,17 @@ __global__ __launch_bounds__(kAdrBlock) void k_adr_f32(
   if (i >= n) return;
   const uint32_t x = static_cast<uint32_t>(i / P.ny), y = static_cast<uint32_t>(i % P.ny);
   if (solid[i] != 0) {
-    for (uint32_t s = 0; s < P.signals; ++s) out[s * n + i] = in[s * n + i];
+#pragma unroll
+    for (int s = 0; s < S; ++s) out[s * n + i] = in[s * n + i];
     return;
   }
   const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
   const uint32_t dims[2] = {P.nx, P.ny};
   bool bad = false;
   SiteGeometry g = geometry(P, solid, xf, yf, x, y, 0);
-  for (uint32_t s = 0; s < P.signals; ++s) {
-    if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, static_cast<int>(s));
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, s);
     const float* lv = in + s * n;
     const float current = lv[i];
     float lower[2], upper[2];
@@ -152,121 +156,149 @@ __global__ __launch_bounds__(kAdrBlock) void k_adr_f32(
   }
 }
 
-// FP64 face-flux step with its ledger. Partials per block: for each signal,
-// mass after, boundary supply rate, uptake rate.
+// The ledger of a step lives as per-block partial sums (mass after, boundary
+// supply rate, uptake rate, for each signal) in two generations by step
+// parity. Blocks j < 3S of the next step add partial j of the previous step
+// (in a fixed order) into its ledger row, so no grid-wide fence is needed; a
+// last small launch adds the final step's.
+struct LedgerParts {
+  double* parts;   // [2][3 * kMaxSignals][blocks]
+  double* ledger;  // [steps][S][4]
+  uint32_t blocks;
+};
+
+template <int S, int NT>
+__device__ inline void fold_ledger(const AdrParams& P, LedgerParts L, uint32_t step, float dt,
+                                   uint32_t j, double* lds, double* bcast) {
+  // Partial j of `step` into ledger row `step`.
+  const double* v =
+      L.parts + (static_cast<size_t>(step & 1) * (3 * kMaxSignals) + j) * L.blocks;
+  double total = 0.0;
+  for (uint32_t b = threadIdx.x; b < L.blocks; b += NT) total += v[b];
+  total = block_sum<NT>(total, lds);
+  if (threadIdx.x == 0) {
+    const double volume =
+        static_cast<double>(P.dx) * static_cast<double>(P.dy) * static_cast<double>(P.dz);
+    const uint32_t s = j / 3, what = j % 3;
+    double* row = L.ledger + (static_cast<size_t>(step) * S + s) * 4;
+    row[1 + what] = what == 0 ? total * volume : static_cast<double>(dt) * total * volume;
+  }
+  (void)bcast;
+}
+
+// One FP64 step in face-flux form for S signals, grid-stride over sites.
+template <int S>
 __global__ __launch_bounds__(kAdrBlock) void k_adr_f64(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
     const float* __restrict__ loss, const double* __restrict__ in, double* __restrict__ out,
-    float dt, uint32_t* flags, uint32_t step, double* partials, CgState* st, double* ledger,
-    double* mass) {
+    float dt, uint32_t* flags, uint32_t step, LedgerParts L) {
   __shared__ double s_red[kAdrBlock / kWave];
-  __shared__ bool s_last;
+  __shared__ double s_b;
   if (flags[0] != 0) return;
+  if (step > 0 && blockIdx.x < 3 * S) {
+    fold_ledger<S, kAdrBlock>(P, L, step - 1, dt, blockIdx.x, s_red, &s_b);
+  }
   const size_t n = static_cast<size_t>(P.nx) * P.ny;
-  double acc[3 * kMaxSignals];
+  double acc[3 * S];
 #pragma unroll
-  for (int k = 0; k < 3 * kMaxSignals; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 3 * S; ++k) acc[k] = 0.0;
   bool bad = false;
+  const double ih[2] = {1.0 / static_cast<double>(P.dx), 1.0 / static_cast<double>(P.dy)};
   const size_t grid_stride = static_cast<size_t>(gridDim.x) * kAdrBlock;
   for (size_t i = static_cast<size_t>(blockIdx.x) * kAdrBlock + threadIdx.x; i < n;
        i += grid_stride) {
     const uint32_t x = static_cast<uint32_t>(i / P.ny), y = static_cast<uint32_t>(i % P.ny);
     if (solid[i] != 0) {
-      for (uint32_t s = 0; s < P.signals; ++s) out[s * n + i] = in[s * n + i];
-    } else {
-      const double ih[2] = {1.0 / static_cast<double>(P.dx), 1.0 / static_cast<double>(P.dy)};
-      const uint32_t coord[2] = {x, y};
-      const uint32_t dims[2] = {P.nx, P.ny};
-      const size_t stride[2] = {P.ny, 1};
-      SiteGeometry g = geometry(P, solid, xf, yf, x, y, 0);
-      for (uint32_t s = 0; s < P.signals; ++s) {
-        if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, static_cast<int>(s));
-        const double* lv = in + s * n;
-        const double c = lv[i];
-        const double D = static_cast<double>(P.diffusion[s]);
-        double rate = 0.0, supply = 0.0;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          if (dims[a] == 1) continue;
-          double fl = 0.0, fu = 0.0;
-          const double ul = static_cast<double>(g.ulo[a]), uu = static_cast<double>(g.uup[a]);
-          // lower face
-          if (coord[a] == 0) {
-            if (P.kind[2 * a] == MSIM_FIXED) {
-              const double cb = static_cast<double>(P.boundary[s][2 * a]);
-              fl = (ul >= 0.0 ? ul * cb : ul * c) - D * (c - cb) * ih[a];
-              supply += fl * ih[a];
-            }
-          } else if (!g.clo[a]) {
-            const double cl = lv[i - stride[a]];
-            fl = (ul >= 0.0 ? ul * cl : ul * c) - D * (c - cl) * ih[a];
+      for (int s = 0; s < S; ++s) out[s * n + i] = in[s * n + i];
+      continue;
+    }
+    const uint32_t coord[2] = {x, y};
+    const uint32_t dims[2] = {P.nx, P.ny};
+    const size_t stride[2] = {P.ny, 1};
+    SiteGeometry g = geometry(P, solid, xf, yf, x, y, 0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, s);
+      const double* lv = in + s * n;
+      const double c = lv[i];
+      const double D = static_cast<double>(P.diffusion[s]);
+      double rate = 0.0, supply = 0.0;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        if (dims[a] == 1) continue;
+        double fl = 0.0, fu = 0.0;
+        const double ul = static_cast<double>(g.ulo[a]), uu = static_cast<double>(g.uup[a]);
+        // lower face
+        if (coord[a] == 0) {
+          if (P.kind[2 * a] == MSIM_FIXED) {
+            const double cb = static_cast<double>(P.boundary[s][2 * a]);
+            fl = (ul >= 0.0 ? ul * cb : ul * c) - D * (c - cb) * ih[a];
+            supply += fl * ih[a];
           }
-          // upper face
-          if (coord[a] + 1 == dims[a]) {
-            if (P.kind[2 * a + 1] == MSIM_FIXED) {
-              const double cb = static_cast<double>(P.boundary[s][2 * a + 1]);
-              fu = (uu >= 0.0 ? uu * c : uu * cb) - D * (cb - c) * ih[a];
-              supply -= fu * ih[a];
-            }
-          } else if (!g.cup[a]) {
-            const double cu = lv[i + stride[a]];
-            fu = (uu >= 0.0 ? uu * c : uu * cu) - D * (cu - c) * ih[a];
+        } else if (!g.clo[a]) {
+          const double cl = lv[i - stride[a]];
+          fl = (ul >= 0.0 ? ul * cl : ul * c) - D * (c - cl) * ih[a];
+        }
+        // upper face
+        if (coord[a] + 1 == dims[a]) {
+          if (P.kind[2 * a + 1] == MSIM_FIXED) {
+            const double cb = static_cast<double>(P.boundary[s][2 * a + 1]);
+            fu = (uu >= 0.0 ? uu * c : uu * cb) - D * (cb - c) * ih[a];
+            supply -= fu * ih[a];
           }
-          rate -= (fu - fl) * ih[a];
+        } else if (!g.cup[a]) {
+          const double cu = lv[i + stride[a]];
+          fu = (uu >= 0.0 ? uu * c : uu * cu) - D * (cu - c) * ih[a];
         }
-        double lossr = 0.0;
-        const size_t index = s * n + i;
-        if (P.has_affine != 0) {
-          const double src = static_cast<double>(source[index]);
-          rate += src;
-          supply += src;
-          lossr += static_cast<double>(loss[index]);
-        }
-        if (P.has_monod != 0) {
-          lossr += static_cast<double>(P.vmax[s]) * static_cast<double>(phi[i]) /
-                   (static_cast<double>(P.monod_k[s]) + c);
-        }
-        rate -= lossr * c;
-        const double next = c + static_cast<double>(dt) * rate;
-        if (!isfinite(next) || next < 0.0) bad = true;
-        out[index] = next;
-        if (s < kMaxSignals) {
-          acc[3 * s + 0] += next;
-          acc[3 * s + 1] += supply;
-          acc[3 * s + 2] += lossr * c;
-        }
+        rate -= (fu - fl) * ih[a];
       }
+      double lossr = 0.0;
+      const size_t index = s * n + i;
+      if (P.has_affine != 0) {
+        const double src = static_cast<double>(source[index]);
+        rate += src;
+        supply += src;
+        lossr += static_cast<double>(loss[index]);
+      }
+      if (P.has_monod != 0) {
+        lossr += static_cast<double>(P.vmax[s]) * static_cast<double>(phi[i]) /
+                 (static_cast<double>(P.monod_k[s]) + c);
+      }
+      rate -= lossr * c;
+      const double next = c + static_cast<double>(dt) * rate;
+      if (!isfinite(next) || next < 0.0) bad = true;
+      out[index] = next;
+      acc[3 * s + 0] += next;
+      acc[3 * s + 1] += supply;
+      acc[3 * s + 2] += lossr * c;
     }
   }
   if (bad) {
     atomicOr(&flags[0], kFlagNegative);
     atomicMin(&flags[1], step);
   }
-  double part[3 * kMaxSignals];
+  // Solid sites hold their (zero) level; the mass counts only what moved.
 #pragma unroll
-  for (int k = 0; k < 3 * kMaxSignals; ++k) {
-    part[k] = (static_cast<uint32_t>(k / 3) < P.signals) ? block_sum<kAdrBlock>(acc[k], s_red)
-                                                         : 0.0;
-  }
-  if (publish_partials<3 * kMaxSignals>(part, partials, &st->ticket[kTicketAdr], blockIdx.x,
-                                        gridDim.x, &s_last)) {
-    double total[3 * kMaxSignals];
-    sum_partials<kAdrBlock, 3 * kMaxSignals>(partials, gridDim.x, total, s_red);
+  for (int k = 0; k < 3 * S; ++k) {
+    const double part = block_sum<kAdrBlock>(acc[k], s_red);
     if (threadIdx.x == 0) {
-      st->ticket[kTicketAdr] = 0;
-      const double volume =
-          static_cast<double>(P.dx) * static_cast<double>(P.dy) * static_cast<double>(P.dz);
-      for (uint32_t s = 0; s < P.signals && s < kMaxSignals; ++s) {
-        double* row = ledger + (static_cast<size_t>(step) * P.signals + s) * 4;
-        row[0] = mass[s];
-        row[1] = total[3 * s + 0] * volume;
-        row[2] = static_cast<double>(dt) * total[3 * s + 1] * volume;
-        row[3] = static_cast<double>(dt) * total[3 * s + 2] * volume;
-        mass[s] = row[1];
-      }
+      L.parts[(static_cast<size_t>(step & 1) * (3 * kMaxSignals) + k) * L.blocks + blockIdx.x] =
+          part;
     }
   }
+}
+
+// Adds the final step's ledger partials.
+template <int S>
+__global__ __launch_bounds__(kAdrBlock) void k_adr_ledger_tail(AdrParams P, LedgerParts L,
+                                                               uint32_t step, float dt,
+                                                               const uint32_t* flags) {
+  __shared__ double s_red[kAdrBlock / kWave];
+  __shared__ double s_b;
+  if (flags[0] != 0) return;
+  fold_ledger<S, kAdrBlock>(P, L, step, dt, blockIdx.x, s_red, &s_b);
 }
 
 // Sum of levels per signal (FP64), and the greatest outflow rate over fluid
@@ -366,7 +398,8 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.flags.reserve(2);
   a.state.reserve(1);
   a.mass.reserve(kMaxSignals);
-  a.partials.reserve(static_cast<size_t>(3 * kMaxSignals) * 2049);
+  a.partials.reserve(static_cast<size_t>(3 * kMaxSignals) * 2 * 1024 +
+                     static_cast<size_t>(kMaxSignals + 1) * 2048);
   if (!same_spec) {
     a.solid.reserve(n);
     if (grid.grid.obstacles != nullptr) {
@@ -434,13 +467,48 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
         "read survey");
   check(hipStreamSynchronize(ctx.stream), "survey");
   a.greatest_outflow = P.has_field != 0 ? host.peak : 0.0;
+  check(hipMemcpyAsync(a.host_mass, a.mass.data(), sizeof(a.host_mass), hipMemcpyDeviceToHost,
+                       ctx.stream),
+        "read mass");
+  check(hipStreamSynchronize(ctx.stream), "read mass");
   a.uploaded = true;
+}
+
+template <int S>
+void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf) {
+  auto& a = ctx.adr;
+  const auto& P = a.params;
+  const size_t n = a.sites();
+  const uint32_t blocks = site_blocks(n);
+  const uint32_t ledger_blocks = std::min<uint32_t>(blocks, 1024);
+  LedgerParts L{a.partials.data(), a.ledger.data(), ledger_blocks};
+  for (uint32_t k = 0; k < steps; ++k) {
+    uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
+    uint8_t* dst = a.current == 0 ? a.levels_b.data() : a.levels_a.data();
+    if (a.precision == 64) {
+      k_adr_f64<S><<<ledger_blocks, kAdrBlock, 0, ctx.stream>>>(
+          P, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
+          reinterpret_cast<const double*>(src), reinterpret_cast<double*>(dst), dt,
+          a.flags.data(), k, L);
+    } else {
+      k_adr_f32<S><<<blocks, kAdrBlock, 0, ctx.stream>>>(
+          P, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
+          reinterpret_cast<const float*>(src), reinterpret_cast<float*>(dst), dt, a.flags.data(),
+          k);
+    }
+    check(hipGetLastError(), "launch transport step");
+    a.current ^= 1;
+  }
+  if (a.precision == 64 && steps > 0) {
+    k_adr_ledger_tail<S><<<3 * S, kAdrBlock, 0, ctx.stream>>>(P, L, steps - 1, dt,
+                                                             a.flags.data());
+    check(hipGetLastError(), "launch ledger tail");
+  }
 }
 
 void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
   auto& a = ctx.adr;
   const auto& P = a.params;
-  const size_t n = a.sites();
   const float* xf = a.flow_field ? ctx.flow.faces.data() : a.xf.data();
   const float* yf = a.flow_field ? ctx.flow.faces.data() + ctx.flow.x_faces() : a.yf.data();
   const uint32_t init_flags[2] = {0u, 0xffffffffu};
@@ -449,33 +517,24 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
         "clear flags");
   if (a.precision == 64) a.ledger.reserve(static_cast<size_t>(steps) * P.signals * 4 + 1);
   check(hipEventRecord(ctx.start, ctx.stream), "event");
-  const uint32_t blocks = site_blocks(n);
-  const uint32_t ledger_blocks = std::min<uint32_t>(blocks, 2048);
-  check(hipMemsetAsync(a.state.data(), 0, sizeof(CgState), ctx.stream), "clear state");
-  for (uint32_t k = 0; k < steps; ++k) {
-    uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
-    uint8_t* dst = a.current == 0 ? a.levels_b.data() : a.levels_a.data();
-    if (a.precision == 64) {
-      k_adr_f64<<<ledger_blocks, kAdrBlock, 0, ctx.stream>>>(
-          P, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
-          reinterpret_cast<const double*>(src), reinterpret_cast<double*>(dst), dt,
-          a.flags.data(), k, a.partials.data(), a.state.data(), a.ledger.data(), a.mass.data());
-    } else {
-      k_adr_f32<<<blocks, kAdrBlock, 0, ctx.stream>>>(
-          P, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
-          reinterpret_cast<const float*>(src), reinterpret_cast<float*>(dst), dt, a.flags.data(),
-          k);
-    }
-    check(hipGetLastError(), "launch transport step");
-    a.current ^= 1;
+  switch (P.signals) {
+    case 1: launch_steps<1>(ctx, dt, steps, xf, yf); break;
+    case 2: launch_steps<2>(ctx, dt, steps, xf, yf); break;
+    case 3: launch_steps<3>(ctx, dt, steps, xf, yf); break;
+    case 4: launch_steps<4>(ctx, dt, steps, xf, yf); break;
+    case 5: launch_steps<5>(ctx, dt, steps, xf, yf); break;
+    case 6: launch_steps<6>(ctx, dt, steps, xf, yf); break;
+    case 7: launch_steps<7>(ctx, dt, steps, xf, yf); break;
+    default: launch_steps<8>(ctx, dt, steps, xf, yf); break;
   }
   check(hipEventRecord(ctx.stop, ctx.stream), "event");
   uint32_t flags[2] = {0, 0};
   check(hipMemcpyAsync(flags, a.flags.data(), sizeof(flags), hipMemcpyDeviceToHost, ctx.stream),
         "read flags");
-  if (ledger != nullptr && a.precision == 64 && steps > 0) {
-    check(hipMemcpyAsync(ledger, a.ledger.data(), static_cast<size_t>(steps) * P.signals * 4 *
-                                                      sizeof(double),
+  std::vector<double> rows;
+  if (a.precision == 64 && steps > 0) {
+    rows.resize(static_cast<size_t>(steps) * P.signals * 4);
+    check(hipMemcpyAsync(rows.data(), a.ledger.data(), rows.size() * sizeof(double),
                          hipMemcpyDeviceToHost, ctx.stream),
           "read ledger");
   }
@@ -491,6 +550,17 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
     throw Error(MSIM_DOMAIN_ERROR,
                 "signal grid update produced a non-finite or negative concentration (step " +
                     std::to_string(failed) + ")");
+  }
+  if (!rows.empty()) {
+    // Mass before each step: the mass after the previous one.
+    for (uint32_t k = 0; k < steps; ++k) {
+      for (uint32_t s = 0; s < P.signals; ++s) {
+        double* row = rows.data() + (static_cast<size_t>(k) * P.signals + s) * 4;
+        row[0] = a.host_mass[s];
+        a.host_mass[s] = row[1];
+      }
+    }
+    if (ledger != nullptr) std::memcpy(ledger, rows.data(), rows.size() * sizeof(double));
   }
 }
 

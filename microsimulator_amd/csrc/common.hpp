@@ -82,7 +82,10 @@ __device__ inline bool publish_partials(const double (&values)[K], double* parti
 
 // In the last block: sums partials[k*blocks .. k*blocks+blocks) for k < K in
 // a fixed order (thread t takes t, t+NT, ... in order, then the block tree).
-// Result valid in thread 0.
+// Plain loads suffice: the acquire fence publish_partials executed in this
+// block, paired with every other block's release fence through the ticket,
+// makes their partials visible (it invalidates this CU's L1 and its XCD's
+// L2). Result valid in thread 0.
 template <int NT, int K>
 __device__ inline void sum_partials(const double* partials, uint32_t blocks, double (&out)[K],
                                     double* lds) {
@@ -90,8 +93,7 @@ __device__ inline void sum_partials(const double* partials, uint32_t blocks, dou
   for (int k = 0; k < K; ++k) {
     double v = 0.0;
     for (uint32_t b = threadIdx.x; b < blocks; b += NT) {
-      v += __hip_atomic_load(partials + static_cast<size_t>(k) * blocks + b, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+      v += partials[static_cast<size_t>(k) * blocks + b];
     }
     out[k] = block_sum<NT>(v, lds);
   }
@@ -104,8 +106,7 @@ __device__ inline void max_partials(const double* partials, uint32_t blocks, dou
   for (int k = 0; k < K; ++k) {
     double v = 0.0;
     for (uint32_t b = threadIdx.x; b < blocks; b += NT) {
-      v = fmax(v, __hip_atomic_load(partials + static_cast<size_t>(k) * blocks + b,
-                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      v = fmax(v, partials[static_cast<size_t>(k) * blocks + b]);
     }
     out[k] = block_max<NT>(v, lds);
   }

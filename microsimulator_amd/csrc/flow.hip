@@ -107,12 +107,16 @@ __global__ void k_assemble(const float* __restrict__ cond, float* __restrict__ d
 // strip of `rows` x-rows and walks along x: each thread keeps the operand of
 // its column at x-1, x and x+1 in registers (so every row is read from memory
 // once), and the y neighbours come from an LDS row staged with a one-site
-// halo on each side. Tiles are visited in XCD-contiguous order.
+// halo on each side. The loads of row x+2 are issued while row x is computed,
+// so a row's memory latency is hidden behind the row before it. Tiles are
+// visited in XCD-contiguous order, so the halo rows a tile shares with its
+// neighbours meet in one L2.
 // ---------------------------------------------------------------------------
 
 struct Tile {
   uint32_t ty, x0, x1, y;
-  bool valid;
+  bool valid, lo_halo, hi_halo;
+  size_t ylo, yhi;
 };
 
 template <int TY>
@@ -125,180 +129,294 @@ __device__ inline Tile tile_of(uint32_t nx, uint32_t ny, uint32_t rows) {
   tile.x1 = min(nx, tile.x0 + rows);
   tile.y = tile.ty * TY + threadIdx.x;
   tile.valid = tile.y < ny;
+  // Thread 0 also carries the site below the tile, thread TY-1 the one above.
+  tile.lo_halo = threadIdx.x == 0 && tile.ty > 0;
+  tile.hi_halo = threadIdx.x == TY - 1 && tile.ty * TY + TY < ny;
+  tile.ylo = tile.ty * TY - 1;
+  tile.yhi = static_cast<size_t>(tile.ty) * TY + TY;
   return tile;
 }
 
-// One CG iteration's first half, with the previous iteration's direction
-// update and solution update folded in:
-//   p_new = M^-1 r + beta p_old           (conjugate_gradient.hpp:82, cpu_flow.cpp:92-97)
-//   x    += alpha_prev p_old              (the previous iteration's update, cpu_flow.cpp:84-89)
-//   q     = A p_new                       (DepthAveragedFlowSystem::apply, flow_system.hpp:670-695)
-//   curvature = p_new . q, alpha = rho / curvature (conjugate_gradient.hpp:59-66),
-// the last two in the block that finishes last.
+// What one thread loads of one row for the CG operator kernel.
+template <typename T>
+struct CgRow {
+  T r, po, x;
+  float d, cx, cy, b;
+  T lr, lpo;  // site below the tile (thread 0)
+  float ld;
+  T hr, hpo;  // site above the tile (thread TY-1)
+  float hd, hcy;
+};
+
+// Per-block partial sums of the Krylov kernels, two generations of each
+// (indexed by iteration parity): p . A p from the operator kernel (p1 blocks),
+// r . r and r . M^-1 r from the update kernel and the set-up (p2 blocks).
+struct CgPartials {
+  double* pq;
+  double* rz;
+  double* rr;
+  uint32_t p1, p2;
+};
+
+// Sum of `count` partials in a fixed order, the same in every thread of every
+// block that calls it: thread t adds t, t+NT, ... in order, then the block
+// combines waves in order.
+template <int NT>
+__device__ inline double fixed_sum(const double* v, uint32_t count, double* lds, double* bcast) {
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += NT) s += v[i];
+  s = block_sum<NT>(s, lds);
+  if (threadIdx.x == 0) *bcast = s;
+  __syncthreads();
+  const double out = *bcast;
+  __syncthreads();
+  return out;
+}
+
+// One CG iteration k's first half, with iteration k-1's solution update and
+// the direction update folded in (k = it_update + 1):
+//   p_k = M^-1 r_{k-1} + beta_k p_{k-1}   (conjugate_gradient.hpp:82, cpu_flow.cpp:92-97)
+//   x  += alpha_{k-1} p_{k-1}             (cpu_flow.cpp:84-89, deferred one iteration)
+//   q_k = A p_k                           (DepthAveragedFlowSystem::apply, flow_system.hpp:670-695)
+// and this block's part of p_k . q_k. The scalars come from the partial sums
+// the previous kernels left: every block adds them in the same fixed order, so
+// every block holds the same bits and no grid-wide fence is needed.
+//   rho_{k-1} = r.z after iteration k-1 (the set-up's for k = 1),
+//   alpha_{k-1} = rho_{k-2} / (p_{k-1} . q_{k-1}), beta_k = rho_{k-1} / rho_{k-2}.
+// The stopping test of iteration k-1 (r.r <= (tol ||b||)^2, the batched
+// solve's, conjugate_gradient.hpp:126-166) is made here: when it holds the
+// kernel only applies the deferred solution update and records the halt.
 template <typename T, int TY>
 __global__ __launch_bounds__(TY) void k_cg_apply(
     const T* __restrict__ r, const float* __restrict__ diag, const T* __restrict__ p_old,
     T* __restrict__ p_new, T* __restrict__ q, T* __restrict__ xsol, const float* __restrict__ cxl,
     const float* __restrict__ cyl, const float* __restrict__ bc, uint32_t nx, uint32_t ny,
-    uint32_t rows, T isx, T isy, CgState* st, double* partials) {
+    uint32_t rows, T isx, T isy, CgState* st, CgPartials parts) {
   __shared__ T s_p[TY + 2];
   __shared__ float s_c[TY + 2];
   __shared__ double s_red[TY / kWave];
-  __shared__ bool s_last;
-  if (st->done != 0) return;
+  __shared__ double s_b;
+  if (st->failure != 0) return;
+  const uint32_t k = st->it_update + 1;
+  const uint32_t halt = st->halt;
+  // A launch after the one that saw the halt does nothing. (The launch that
+  // records the halt reads either 0 or k-1 here and goes on either way.)
+  if (halt != 0 && halt + 1 < k) return;
   const Tile tl = tile_of<TY>(nx, ny, rows);
   const uint32_t t = threadIdx.x;
-  const T beta = static_cast<T>(st->beta);
-  const T alpha = static_cast<T>(st->alpha);
-  const bool update_x = xsol != nullptr;
-  // The new direction at a site, and the old direction there.
-  auto operand = [&](size_t idx, T& old) -> T {
-    const float d = diag[idx];
-    const T z = d > 0.0f ? r[idx] / static_cast<T>(d) : static_cast<T>(0);
-    old = p_old[idx];
-    return z + beta * old;
-  };
-  T prev = 0, cur = 0, cur_old = 0, dummy;
-  float cxlo = 0.0f;
-  if (tl.valid && tl.x0 < tl.x1) {
-    const size_t i0 = static_cast<size_t>(tl.x0) * ny + tl.y;
-    if (tl.x0 > 0) prev = operand(i0 - ny, dummy);
-    cur = operand(i0, cur_old);
-    cxlo = cxl[i0];
-  }
-  double acc = 0.0;
-  for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
-    const size_t idx = static_cast<size_t>(xx) * ny + tl.y;
-    T nxt = 0, nxt_old = 0;
-    float cxhi = 0.0f;
-    if (tl.valid && xx + 1 < nx) {
-      nxt = operand(idx + ny, nxt_old);
-      cxhi = cxl[idx + ny];
+  const bool lead = blockIdx.x == 0 && t == 0;
+  const double rho_km1 = fixed_sum<TY>(parts.rz + ((k - 1) & 1) * parts.p2, parts.p2, s_red, &s_b);
+  const double rr_km1 = fixed_sum<TY>(parts.rr + ((k - 1) & 1) * parts.p2, parts.p2, s_red, &s_b);
+  double alpha_prev = 0.0, beta = 0.0;
+  bool finish = false;
+  if (k >= 2) {
+    const double rho_km2 = fixed_sum<TY>(parts.rz + (k & 1) * parts.p2, parts.p2, s_red, &s_b);
+    const double pq_km1 = fixed_sum<TY>(parts.pq + ((k - 1) & 1) * parts.p1, parts.p1, s_red, &s_b);
+    alpha_prev = rho_km2 / pq_km1;
+    if (!isfinite(rr_km1)) {
+      if (lead) st->failure = 3;
+      return;
     }
-    s_p[t + 1] = tl.valid ? cur : static_cast<T>(0);
-    s_c[t + 1] = tl.valid ? cyl[idx] : 0.0f;
-    if (t == 0) {
-      s_p[0] = tl.ty > 0 ? operand(static_cast<size_t>(xx) * ny + tl.ty * TY - 1, dummy)
-                         : static_cast<T>(0);
-    }
-    if (t == TY - 1) {
-      const uint32_t yu = tl.ty * TY + TY;
-      s_p[TY + 1] = yu < ny ? operand(static_cast<size_t>(xx) * ny + yu, dummy) : static_cast<T>(0);
-      s_c[TY + 1] = yu < ny ? cyl[static_cast<size_t>(xx) * ny + yu] : 0.0f;
-    }
-    __syncthreads();
-    if (tl.valid) {
-      const T pl = s_p[t];
-      const T pu = s_p[t + 2];
-      const float cylo = s_c[t + 1];
-      const float cyhi = s_c[t + 2];
-      T qv = static_cast<T>(bc[idx]) * cur;
-      qv += (static_cast<T>(cxlo) * isx) * (cur - prev);
-      qv += (static_cast<T>(cxhi) * isx) * (cur - nxt);
-      qv += (static_cast<T>(cylo) * isy) * (cur - pl);
-      qv += (static_cast<T>(cyhi) * isy) * (cur - pu);
-      q[idx] = qv;
-      p_new[idx] = cur;
-      if (update_x) xsol[idx] += alpha * cur_old;
-      acc += static_cast<double>(cur) * static_cast<double>(qv);
-    }
-    __syncthreads();
-    prev = cur;
-    cur = nxt;
-    cur_old = nxt_old;
-    cxlo = cxhi;
-  }
-  double part[1] = {block_sum<TY>(acc, s_red)};
-  if (publish_partials<1>(part, partials, &st->ticket[kTicketApply], blockIdx.x, gridDim.x,
-                          &s_last)) {
-    double total[1];
-    sum_partials<TY, 1>(partials, gridDim.x, total, s_red);
-    if (t == 0) {
-      st->ticket[kTicketApply] = 0;
-      const double curvature = total[0];
-      st->curvature = curvature;
-      if (!isfinite(curvature) || curvature <= 0.0) {
-        st->failure = 1;
-        st->done = 1;
-      } else {
-        const double a = st->rho / curvature;
-        if (!isfinite(a)) {
-          st->failure = 2;
-          st->done = 1;
-        } else {
-          st->alpha = a;
-        }
+    if (rr_km1 <= st->threshold) {
+      finish = true;
+    } else {
+      if (!isfinite(rho_km1) || rho_km2 == 0.0) {
+        if (lead) st->failure = 4;
+        return;
+      }
+      beta = rho_km1 / rho_km2;
+      if (!isfinite(beta)) {
+        if (lead) st->failure = 5;
+        return;
       }
     }
   }
+  if (lead) {
+    st->it_apply = k;
+    st->rr = rr_km1;
+    st->rho = rho_km1;
+    st->alpha = alpha_prev;
+    st->beta = beta;
+    if (finish) st->halt = k - 1;
+  }
+  const T alpha = static_cast<T>(alpha_prev);
+  if (finish) {
+    if (tl.valid) {
+      for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
+        const size_t i = static_cast<size_t>(xx) * ny + tl.y;
+        xsol[i] = xsol[i] + alpha * p_old[i];
+      }
+    }
+    return;
+  }
+  const T betaT = static_cast<T>(beta);
+  const T zero = static_cast<T>(0);
+  // Own rows load everything; the halo rows x0-1 and x1 only the operand
+  // (and x1 its lower-x conductance, the upper one of the strip's last row).
+  auto load = [&](uint32_t xx, bool full) {
+    CgRow<T> w{};
+    if (xx >= nx) return w;
+    const size_t base = static_cast<size_t>(xx) * ny;
+    if (tl.valid) {
+      const size_t i = base + tl.y;
+      w.r = r[i];
+      w.po = p_old[i];
+      w.d = diag[i];
+      w.cx = cxl[i];
+      if (full) {
+        w.x = xsol[i];
+        w.cy = cyl[i];
+        w.b = bc[i];
+      }
+    }
+    if (full && tl.lo_halo) {
+      const size_t i = base + tl.ylo;
+      w.lr = r[i];
+      w.lpo = p_old[i];
+      w.ld = diag[i];
+    }
+    if (full && tl.hi_halo) {
+      const size_t i = base + tl.yhi;
+      w.hr = r[i];
+      w.hpo = p_old[i];
+      w.hd = diag[i];
+      w.hcy = cyl[i];
+    }
+    return w;
+  };
+  auto op = [&](T rv, float d, T po) -> T {
+    const T z = d > 0.0f ? rv / static_cast<T>(d) : zero;
+    return z + betaT * po;
+  };
+  double acc = 0.0;
+  if (tl.x0 < tl.x1) {
+    CgRow<T> cur_row = load(tl.x0, true);
+    CgRow<T> next_row = load(tl.x0 + 1, tl.x0 + 1 < tl.x1);
+    T prev = zero;
+    if (tl.x0 > 0) {
+      const CgRow<T> w = load(tl.x0 - 1, false);
+      prev = op(w.r, w.d, w.po);
+    }
+    T cur = op(cur_row.r, cur_row.d, cur_row.po);
+    for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
+      const CgRow<T> ahead = xx + 2 <= tl.x1 ? load(xx + 2, xx + 2 < tl.x1) : CgRow<T>{};
+      const T nxt = op(next_row.r, next_row.d, next_row.po);
+      const float cxhi = next_row.cx;
+      s_p[t + 1] = tl.valid ? cur : zero;
+      s_c[t + 1] = tl.valid ? cur_row.cy : 0.0f;
+      if (t == 0) s_p[0] = tl.lo_halo ? op(cur_row.lr, cur_row.ld, cur_row.lpo) : zero;
+      if (t == TY - 1) {
+        s_p[TY + 1] = tl.hi_halo ? op(cur_row.hr, cur_row.hd, cur_row.hpo) : zero;
+        s_c[TY + 1] = tl.hi_halo ? cur_row.hcy : 0.0f;
+      }
+      __syncthreads();
+      if (tl.valid) {
+        const size_t idx = static_cast<size_t>(xx) * ny + tl.y;
+        const T pl = s_p[t];
+        const T pu = s_p[t + 2];
+        const float cylo = s_c[t + 1];
+        const float cyhi = s_c[t + 2];
+        T qv = static_cast<T>(cur_row.b) * cur;
+        qv += (static_cast<T>(cur_row.cx) * isx) * (cur - prev);
+        qv += (static_cast<T>(cxhi) * isx) * (cur - nxt);
+        qv += (static_cast<T>(cylo) * isy) * (cur - pl);
+        qv += (static_cast<T>(cyhi) * isy) * (cur - pu);
+        q[idx] = qv;
+        p_new[idx] = cur;
+        xsol[idx] = cur_row.x + alpha * cur_row.po;
+        acc += static_cast<double>(cur) * static_cast<double>(qv);
+      }
+      __syncthreads();
+      prev = cur;
+      cur = nxt;
+      cur_row = next_row;
+      next_row = ahead;
+    }
+  }
+  const double part = block_sum<TY>(acc, s_red);
+  if (t == 0) parts.pq[(k & 1) * parts.p1 + blockIdx.x] = part;
 }
 
-// Second half: r -= alpha q (cpu_flow.cpp:84-89), then r . r for the stopping
-// test and r . M^-1 r for the next direction weight; the last block applies
-// the batched solve's tests (conjugate_gradient.hpp:126-166): converged when
-// r.r <= (tol ||b||)^2, else beta = rho'/rho with the breakdown and direction
-// failure codes.
+// Streaming kernels walk their range in chunks of kUnroll * kStream
+// elements: each thread issues its kUnroll loads before it uses any of them.
+constexpr int kUnroll = 4;
+
+// Iteration k's second half: alpha_k = rho_{k-1} / (p_k . q_k) with the
+// curvature and step failure codes (conjugate_gradient.hpp:58-66), then
+// r -= alpha_k q (cpu_flow.cpp:84-89) and this block's parts of r.r and
+// r.M^-1 r (cpu_flow.cpp:65-69).
 template <typename T>
 __global__ __launch_bounds__(kStream) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
                                                        const float* __restrict__ diag, size_t n,
-                                                       CgState* st, double* partials) {
+                                                       CgState* st, CgPartials parts) {
   __shared__ double s_red[kStream / kWave];
-  __shared__ bool s_last;
-  if (st->done != 0) return;
-  const T alpha = static_cast<T>(st->alpha);
-  double rr = 0.0, rz = 0.0;
-  const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
-  for (size_t i = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; i < n; i += stride) {
-    const T ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const float d = diag[i];
-    const T z = d > 0.0f ? ri / static_cast<T>(d) : static_cast<T>(0);
-    rr += static_cast<double>(ri) * static_cast<double>(ri);
-    rz += static_cast<double>(ri) * static_cast<double>(z);
+  __shared__ double s_b;
+  if (st->failure != 0) return;
+  const uint32_t k = st->it_apply;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (st->halt != 0) {
+    if (lead) st->it_update = k;
+    return;
   }
-  double part[2] = {block_sum<kStream>(rr, s_red), block_sum<kStream>(rz, s_red)};
-  if (publish_partials<2>(part, partials, &st->ticket[kTicketUpdate], blockIdx.x, gridDim.x,
-                          &s_last)) {
-    double total[2];
-    sum_partials<kStream, 2>(partials, gridDim.x, total, s_red);
-    if (threadIdx.x == 0) {
-      st->ticket[kTicketUpdate] = 0;
-      const double rrv = total[0];
-      const double next_rho = total[1];
-      st->iterations += 1;
-      st->rr = rrv;
-      if (!isfinite(rrv)) {
-        st->failure = 3;
-        st->done = 1;
-      } else if (rrv <= st->threshold) {
-        st->converged = 1;
-        st->done = 1;
-      } else if (!isfinite(next_rho) || st->rho == 0.0) {
-        st->failure = 4;
-        st->done = 1;
-      } else {
-        const double beta = next_rho / st->rho;
-        if (!isfinite(beta)) {
-          st->failure = 5;
-          st->done = 1;
-        } else {
-          st->beta = beta;
-          st->rho = next_rho;
-        }
-      }
+  const double rho_km1 =
+      fixed_sum<kStream>(parts.rz + ((k - 1) & 1) * parts.p2, parts.p2, s_red, &s_b);
+  const double curvature = fixed_sum<kStream>(parts.pq + (k & 1) * parts.p1, parts.p1, s_red, &s_b);
+  if (!isfinite(curvature) || curvature <= 0.0) {
+    if (lead) st->failure = 1;
+    return;
+  }
+  const double a = rho_km1 / curvature;
+  if (!isfinite(a)) {
+    if (lead) st->failure = 2;
+    return;
+  }
+  if (lead) {
+    st->it_update = k;
+    st->curvature = curvature;
+  }
+  const T alpha = static_cast<T>(a);
+  double rr = 0.0, rz = 0.0;
+  const size_t chunk = static_cast<size_t>(kStream) * kUnroll;
+  for (size_t base = static_cast<size_t>(blockIdx.x) * chunk + threadIdx.x; base < n;
+       base += static_cast<size_t>(gridDim.x) * chunk) {
+    T rv[kUnroll], qv[kUnroll];
+    float dv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = base + static_cast<size_t>(u) * kStream;
+      const bool in = i < n;
+      rv[u] = in ? r[i] : static_cast<T>(0);
+      qv[u] = in ? q[i] : static_cast<T>(0);
+      dv[u] = in ? diag[i] : 0.0f;
     }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = base + static_cast<size_t>(u) * kStream;
+      const T ri = rv[u] - alpha * qv[u];
+      if (i < n) r[i] = ri;
+      const T z = dv[u] > 0.0f ? ri / static_cast<T>(dv[u]) : static_cast<T>(0);
+      rr += static_cast<double>(ri) * static_cast<double>(ri);
+      rz += static_cast<double>(ri) * static_cast<double>(z);
+    }
+  }
+  const double prr = block_sum<kStream>(rr, s_red);
+  const double prz = block_sum<kStream>(rz, s_red);
+  if (threadIdx.x == 0) {
+    parts.rr[(k & 1) * parts.p2 + blockIdx.x] = prr;
+    parts.rz[(k & 1) * parts.p2 + blockIdx.x] = prz;
   }
 }
 
-// x = 0, r = b (or a scaled FP64 residual), p_old = 0; then r.r and r.M^-1 r
-// set up the solve (cpu_flow.cpp:74-80; conjugate_gradient.hpp:49-55).
+// x = 0, r = b (or a scaled FP64 residual), p_old = 0; r.r and r.M^-1 r into
+// the iteration-0 partials (cpu_flow.cpp:74-80; conjugate_gradient.hpp:49-55);
+// the last block also resets the bookkeeping and sets the threshold.
 template <typename T, typename S>
 __global__ __launch_bounds__(kStream) void k_cg_init(const S* __restrict__ source, double scale,
                                                      T* __restrict__ r, T* __restrict__ xsol,
                                                      T* __restrict__ p_old,
                                                      const float* __restrict__ diag, size_t n,
                                                      double tolerance, CgState* st,
-                                                     double* partials) {
+                                                     CgPartials parts, double* partials) {
   __shared__ double s_red[kStream / kWave];
   __shared__ bool s_last;
   double rr = 0.0, rz = 0.0;
@@ -314,6 +432,10 @@ __global__ __launch_bounds__(kStream) void k_cg_init(const S* __restrict__ sourc
     rz += static_cast<double>(ri) * static_cast<double>(z);
   }
   double part[2] = {block_sum<kStream>(rr, s_red), block_sum<kStream>(rz, s_red)};
+  if (threadIdx.x == 0) {
+    parts.rr[blockIdx.x] = part[0];
+    parts.rz[blockIdx.x] = part[1];
+  }
   if (publish_partials<2>(part, partials, &st->ticket[kTicketInit], blockIdx.x, gridDim.x,
                           &s_last)) {
     double total[2];
@@ -329,20 +451,13 @@ __global__ __launch_bounds__(kStream) void k_cg_init(const S* __restrict__ sourc
       st->beta = 0.0;
       st->iterations = 0;
       st->failure = 0;
+      st->it_update = 0;
+      st->it_apply = 0;
+      st->halt = 0;
       st->converged = total[0] == 0.0 ? 1u : 0u;
       st->done = total[0] == 0.0 ? 1u : 0u;
     }
   }
-}
-
-// x += alpha p for the iteration that converged (its update was deferred).
-template <typename T>
-__global__ void k_cg_finalize(T* __restrict__ xsol, const T* __restrict__ p, size_t n,
-                              const CgState* st) {
-  const T alpha = static_cast<T>(st->alpha);
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    xsol[i] += alpha * p[i];
 }
 
 // FP64 refinement update: x += scale * e.
@@ -352,6 +467,11 @@ __global__ void k_refine(double* __restrict__ xsol, const float* __restrict__ e,
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
     xsol[i] += scale * static_cast<double>(e[i]);
 }
+
+struct ResRow {
+  double x, lx, hx;
+  float cx, cy, b, rhs, hcy;
+};
 
 // FP64 operator on x (flow_system.hpp:670-695): out = A x (mode 0) or
 // out = b - A x (mode 1, the true residual), with ||out||^2 and ||b||^2.
@@ -367,48 +487,62 @@ __global__ __launch_bounds__(TY) void k_residual(
   __shared__ bool s_last;
   const Tile tl = tile_of<TY>(nx, ny, rows);
   const uint32_t t = threadIdx.x;
-  double prev = 0, cur = 0;
-  float cxlo = 0.0f;
-  if (tl.valid && tl.x0 < tl.x1) {
-    const size_t i0 = static_cast<size_t>(tl.x0) * ny + tl.y;
-    if (tl.x0 > 0) prev = xin[i0 - ny];
-    cur = xin[i0];
-    cxlo = cxl[i0];
-  }
-  double acc = 0.0, bb = 0.0;
-  for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
-    const size_t idx = static_cast<size_t>(xx) * ny + tl.y;
-    double nxt = 0;
-    float cxhi = 0.0f;
-    if (tl.valid && xx + 1 < nx) {
-      nxt = xin[idx + ny];
-      cxhi = cxl[idx + ny];
-    }
-    s_p[t + 1] = tl.valid ? cur : 0.0;
-    s_c[t + 1] = tl.valid ? cyl[idx] : 0.0f;
-    if (t == 0) s_p[0] = tl.ty > 0 ? xin[static_cast<size_t>(xx) * ny + tl.ty * TY - 1] : 0.0;
-    if (t == TY - 1) {
-      const uint32_t yu = tl.ty * TY + TY;
-      s_p[TY + 1] = yu < ny ? xin[static_cast<size_t>(xx) * ny + yu] : 0.0;
-      s_c[TY + 1] = yu < ny ? cyl[static_cast<size_t>(xx) * ny + yu] : 0.0f;
-    }
-    __syncthreads();
+  auto load = [&](uint32_t xx, bool full) {
+    ResRow w{};
+    if (xx >= nx) return w;
+    const size_t base = static_cast<size_t>(xx) * ny;
     if (tl.valid) {
-      double qv = static_cast<double>(bc[idx]) * cur;
-      qv += (static_cast<double>(cxlo) * isx) * (cur - prev);
-      qv += (static_cast<double>(cxhi) * isx) * (cur - nxt);
-      qv += (static_cast<double>(s_c[t + 1]) * isy) * (cur - s_p[t]);
-      qv += (static_cast<double>(s_c[t + 2]) * isy) * (cur - s_p[t + 2]);
-      const double b = static_cast<double>(rhs[idx]);
-      const double v = mode == 0 ? qv : b - qv;
-      out[idx] = v;
-      acc += v * v;
-      bb += b * b;
+      const size_t i = base + tl.y;
+      w.x = xin[i];
+      w.cx = cxl[i];
+      if (full) {
+        w.cy = cyl[i];
+        w.b = bc[i];
+        w.rhs = rhs[i];
+      }
     }
-    __syncthreads();
-    prev = cur;
-    cur = nxt;
-    cxlo = cxhi;
+    if (full && tl.lo_halo) w.lx = xin[base + tl.ylo];
+    if (full && tl.hi_halo) {
+      w.hx = xin[base + tl.yhi];
+      w.hcy = cyl[base + tl.yhi];
+    }
+    return w;
+  };
+  double acc = 0.0, bb = 0.0;
+  if (tl.x0 < tl.x1) {
+    ResRow cur_row = load(tl.x0, true);
+    ResRow next_row = load(tl.x0 + 1, tl.x0 + 1 < tl.x1);
+    double prev = tl.x0 > 0 ? load(tl.x0 - 1, false).x : 0.0;
+    for (uint32_t xx = tl.x0; xx < tl.x1; ++xx) {
+      const ResRow ahead = xx + 2 <= tl.x1 ? load(xx + 2, xx + 2 < tl.x1) : ResRow{};
+      const double cur = cur_row.x;
+      const double nxt = next_row.x;
+      s_p[t + 1] = tl.valid ? cur : 0.0;
+      s_c[t + 1] = tl.valid ? cur_row.cy : 0.0f;
+      if (t == 0) s_p[0] = tl.lo_halo ? cur_row.lx : 0.0;
+      if (t == TY - 1) {
+        s_p[TY + 1] = tl.hi_halo ? cur_row.hx : 0.0;
+        s_c[TY + 1] = tl.hi_halo ? cur_row.hcy : 0.0f;
+      }
+      __syncthreads();
+      if (tl.valid) {
+        const size_t idx = static_cast<size_t>(xx) * ny + tl.y;
+        double qv = static_cast<double>(cur_row.b) * cur;
+        qv += (static_cast<double>(cur_row.cx) * isx) * (cur - prev);
+        qv += (static_cast<double>(next_row.cx) * isx) * (cur - nxt);
+        qv += (static_cast<double>(s_c[t + 1]) * isy) * (cur - s_p[t]);
+        qv += (static_cast<double>(s_c[t + 2]) * isy) * (cur - s_p[t + 2]);
+        const double b = static_cast<double>(cur_row.rhs);
+        const double v = mode == 0 ? qv : b - qv;
+        out[idx] = v;
+        acc += v * v;
+        bb += b * b;
+      }
+      __syncthreads();
+      prev = cur;
+      cur_row = next_row;
+      next_row = ahead;
+    }
   }
   double part[2] = {block_sum<TY>(acc, s_red), block_sum<TY>(bb, s_red)};
   if (publish_partials<2>(part, partials, &st->ticket[kTicketResidual], blockIdx.x, gridDim.x,
@@ -537,9 +671,18 @@ __global__ __launch_bounds__(kStream) void k_scale(float* __restrict__ faces, si
 // --------------------------------------------------------------------------
 
 uint32_t stream_blocks(size_t n) {
-  const size_t per_block = static_cast<size_t>(kStream) * 4;
+  const size_t per_block = static_cast<size_t>(kStream) * kUnroll;
   const size_t blocks = (n + per_block - 1) / per_block;
-  return static_cast<uint32_t>(std::clamp<size_t>(blocks, 1, 4096));
+  return static_cast<uint32_t>(std::clamp<size_t>(blocks, 1, 2048));
+}
+
+// Blocks of the Krylov streaming kernels (set-up and update). Every operator
+// block re-adds their partials, so they are kept few: 512 blocks of 256
+// threads (two per CU) with 4-way unrolled loads keep HBM busy.
+uint32_t krylov_blocks(size_t n) {
+  const size_t per_block = static_cast<size_t>(kStream) * kUnroll;
+  const size_t blocks = (n + per_block - 1) / per_block;
+  return static_cast<uint32_t>(std::clamp<size_t>(blocks, 1, 512));
 }
 
 struct MarchShape {
@@ -552,13 +695,24 @@ MarchShape march_shape(uint32_t nx, uint32_t ny) {
   MarchShape s{};
   s.ty = ny <= 64 ? 64 : (ny <= 128 ? 128 : 256);
   const uint32_t nty = (ny + s.ty - 1) / s.ty;
-  // Aim for at least ~2048 blocks (8 per CU over 256 CUs) while keeping a
-  // strip long enough that the two halo rows cost little.
+  // Aim for ~1024 blocks (four per CU) while keeping a strip long enough that
+  // its two halo rows cost little; small grids get shorter strips.
   uint32_t rows = 32;
-  while (rows > 4 && static_cast<uint64_t>(nty) * ((nx + rows - 1) / rows) < 2048) rows /= 2;
+  while (rows > 4 && static_cast<uint64_t>(nty) * ((nx + rows - 1) / rows) < 1024) rows /= 2;
   s.rows = rows;
   s.blocks = nty * ((nx + rows - 1) / rows);
   return s;
+}
+
+CgPartials krylov_partials(Context& ctx) {
+  auto& f = ctx.flow;
+  CgPartials parts{};
+  parts.pq = f.pq.data();
+  parts.rz = f.rz.data();
+  parts.rr = f.rr.data();
+  parts.p1 = march_shape(f.nx, f.ny).blocks;
+  parts.p2 = krylov_blocks(f.sites());
+  return parts;
 }
 
 template <typename T>
@@ -567,24 +721,46 @@ void launch_apply(Context& ctx, const T* r, const T* p_old, T* p_new, T* q, T* x
   const MarchShape s = march_shape(f.nx, f.ny);
   const T isx = static_cast<T>(1.0 / (static_cast<double>(f.dx) * f.dx));
   const T isy = static_cast<T>(1.0 / (static_cast<double>(f.dy) * f.dy));
+  const CgPartials parts = krylov_partials(ctx);
   switch (s.ty) {
     case 64:
       k_cg_apply<T, 64><<<s.blocks, 64, 0, ctx.stream>>>(r, f.diag.data(), p_old, p_new, q, x,
                                                          f.cxl.data(), f.cyl.data(), f.bc.data(),
                                                          f.nx, f.ny, s.rows, isx, isy,
-                                                         f.state.data(), f.partials.data());
+                                                         f.state.data(), parts);
       break;
     case 128:
       k_cg_apply<T, 128><<<s.blocks, 128, 0, ctx.stream>>>(
           r, f.diag.data(), p_old, p_new, q, x, f.cxl.data(), f.cyl.data(), f.bc.data(), f.nx,
-          f.ny, s.rows, isx, isy, f.state.data(), f.partials.data());
+          f.ny, s.rows, isx, isy, f.state.data(), parts);
       break;
     default:
       k_cg_apply<T, 256><<<s.blocks, 256, 0, ctx.stream>>>(
           r, f.diag.data(), p_old, p_new, q, x, f.cxl.data(), f.cyl.data(), f.bc.data(), f.nx,
-          f.ny, s.rows, isx, isy, f.state.data(), f.partials.data());
+          f.ny, s.rows, isx, isy, f.state.data(), parts);
   }
   check(hipGetLastError(), "launch k_cg_apply");
+}
+
+template <typename T>
+void launch_update(Context& ctx, T* r, const T* q) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  k_cg_update<T><<<krylov_blocks(n), kStream, 0, ctx.stream>>>(r, q, f.diag.data(), n,
+                                                               f.state.data(),
+                                                               krylov_partials(ctx));
+  check(hipGetLastError(), "launch k_cg_update");
+}
+
+template <typename T, typename S>
+void launch_init(Context& ctx, const S* source, double scale, T* r, T* x, T* p_old,
+                 double tolerance) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  k_cg_init<T, S><<<krylov_blocks(n), kStream, 0, ctx.stream>>>(
+      source, scale, r, x, p_old, f.diag.data(), n, tolerance, f.state.data(),
+      krylov_partials(ctx), f.partials.data());
+  check(hipGetLastError(), "launch k_cg_init");
 }
 
 void launch_residual(Context& ctx, const double* x, double* out, int mode) {
@@ -636,58 +812,66 @@ const char* failure_text(uint32_t failure) {
   }
 }
 
-// Runs Jacobi-PCG iterations of precision T on the prepared system from the
-// state k_cg_init left, until convergence, failure, or `budget` iterations.
-// Returns the iterations taken. Throws on a CG failure.
 struct KernelTimes {
   double apply_seconds = 0.0, update_seconds = 0.0;
   uint32_t apply_launches = 0, update_launches = 0;
 };
 
+// Runs Jacobi-PCG iterations of precision T on the prepared system from the
+// state launch_init left, until the stopping test holds, a failure, or
+// `budget` iterations (conjugate_gradient.hpp:126-166: the host looks at the
+// device's state once per `interval` iterations). The operator kernel of
+// iteration k+1 makes iteration k's stopping test; one more operator launch
+// after the budget makes the last test. Returns the iterations taken (the
+// iteration that met the test, or the budget). Throws on a CG failure.
 template <typename T>
 uint32_t run_cg(Context& ctx, T* r, T* x, T* q, T* pa, T* pb, uint32_t budget, uint32_t interval,
-                bool* converged, KernelTimes* times) {
+                bool* converged, double* relative, KernelTimes* times) {
   auto& f = ctx.flow;
-  const size_t n = f.sites();
   T* buf[2] = {pa, pb};
-  uint32_t launched = 0;
   *converged = false;
   if (times != nullptr) {
-    // Three events per iteration: before the operator kernel, between the two
-    // kernels, after the update kernel.
-    while (ctx.probes.size() < 3 * static_cast<size_t>(interval)) {
+    while (ctx.probes.size() < 3 * static_cast<size_t>(interval) + 3) {
       hipEvent_t e;
       check(hipEventCreate(&e), "hipEventCreate");
       ctx.probes.push_back(e);
     }
   }
   read_state(ctx);
-  if (f.host_state->done != 0) {
-    *converged = f.host_state->converged != 0;
+  if (f.host_state->done != 0) {  // r.r of the right-hand side is zero
+    *converged = true;
+    *relative = 0.0;
     return 0;
   }
-  while (launched < budget) {
-    const uint32_t batch = std::min(interval, budget - launched);
+  const double rr0 = f.host_state->rr0;
+  uint32_t launched = 0;  // iterations launched
+  bool tested_all = false;
+  while (true) {
+    // A final operator launch tests the last iteration of the budget.
+    const bool last_test = launched >= budget;
+    const uint32_t batch = last_test ? 0 : std::min(interval, budget - launched);
     for (uint32_t j = 0; j < batch; ++j) {
-      const uint32_t it = launched + j + 1;  // 1-based iteration of this solve
+      const uint32_t it = launched + j + 1;  // 1-based iteration
       if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j], ctx.stream), "event");
       launch_apply<T>(ctx, r, buf[(it + 1) % 2], buf[it % 2], q, x);
       if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j + 1], ctx.stream), "event");
-      k_cg_update<T><<<stream_blocks(n), kStream, 0, ctx.stream>>>(r, q, f.diag.data(), n,
-                                                                   f.state.data(),
-                                                                   f.partials.data());
-      check(hipGetLastError(), "launch k_cg_update");
+      launch_update<T>(ctx, r, q);
       if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j + 2], ctx.stream), "event");
     }
+    if (last_test) {
+      const uint32_t it = launched + 1;
+      launch_apply<T>(ctx, r, buf[(it + 1) % 2], buf[it % 2], q, x);
+      tested_all = true;
+    }
+    const uint32_t before = launched;
     launched += batch;
     read_state(ctx);
-    if (times != nullptr) {
-      // Only iterations that did work count (a converged solve's later
-      // launches return at once).
-      const uint32_t before = launched - batch;
-      const uint32_t done_iterations = f.host_state->iterations;
+    const CgState& s = *f.host_state;
+    if (times != nullptr && batch > 0) {
+      // Iterations of this batch that did work (later launches return at once).
+      const uint32_t last_worked = s.halt != 0 ? s.halt : s.it_update;
       const uint32_t worked =
-          std::min(batch, done_iterations > before ? done_iterations - before : 0u);
+          std::min(batch, last_worked > before ? last_worked - before : 0u);
       for (uint32_t j = 0; j < worked; ++j) {
         float a = 0.0f, b = 0.0f;
         check(hipEventElapsedTime(&a, ctx.probes[3 * j], ctx.probes[3 * j + 1]), "event time");
@@ -699,22 +883,18 @@ uint32_t run_cg(Context& ctx, T* r, T* x, T* q, T* pa, T* pb, uint32_t budget, u
         times->update_launches += 1;
       }
     }
-    const CgState& s = *f.host_state;
     if (s.failure != 0) {
       throw Error(failure_status(s.failure),
                   std::string("HIP depth-averaged flow conjugate gradient ") +
                       failure_text(s.failure));
     }
-    if (s.done != 0) break;
+    *relative = rr0 > 0.0 ? std::sqrt(std::max(0.0, s.rr)) / std::sqrt(rr0) : 0.0;
+    if (s.halt != 0) {
+      *converged = true;
+      return s.halt;
+    }
+    if (tested_all) return budget;
   }
-  const CgState& s = *f.host_state;
-  if (s.converged != 0 && s.iterations > 0) {
-    k_cg_finalize<T><<<stream_blocks(n), kStream, 0, ctx.stream>>>(x, buf[s.iterations % 2], n,
-                                                                     f.state.data());
-    check(hipGetLastError(), "launch k_cg_finalize");
-  }
-  *converged = s.converged != 0;
-  return s.iterations;
 }
 
 }  // namespace
@@ -730,6 +910,9 @@ void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobilit
   f.faces.reserve(f.x_faces() + f.y_faces());
   // Partial sums: up to 4 values per block of any kernel here.
   f.partials.reserve(4 * static_cast<size_t>(std::max<uint32_t>(march_shape(f.nx, f.ny).blocks, 4096)));
+  f.pq.reserve(2 * static_cast<size_t>(march_shape(f.nx, f.ny).blocks));
+  f.rz.reserve(2 * static_cast<size_t>(krylov_blocks(n)));
+  f.rr.reserve(2 * static_cast<size_t>(krylov_blocks(n)));
   f.state.reserve(1);
   if (f.host_state == nullptr) {
     void* ptr = nullptr;
@@ -779,14 +962,10 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
   KernelTimes kernel_times;
   KernelTimes* times = params.profile != 0 ? &kernel_times : nullptr;
   if (params.solver == MSIM_SOLVER_JACOBI_FP64) {
-    k_cg_init<double, float><<<stream_blocks(n), kStream, 0, ctx.stream>>>(
-        f.rhs.data(), 1.0, f.r.data(), f.x.data(), f.p0.data(), f.diag.data(), n,
-        params.relative_tolerance, f.state.data(), f.partials.data());
-    check(hipGetLastError(), "launch k_cg_init");
+    launch_init<double, float>(ctx, f.rhs.data(), 1.0, f.r.data(), f.x.data(), f.p0.data(),
+                               params.relative_tolerance);
     iterations = run_cg<double>(ctx, f.r.data(), f.x.data(), f.q.data(), f.p0.data(), f.p1.data(),
-                                params.max_iterations, interval, &converged, times);
-    const CgState& s = *f.host_state;
-    relative = s.rr0 > 0.0 ? std::sqrt(std::max(0.0, s.rr)) / std::sqrt(s.rr0) : 0.0;
+                                params.max_iterations, interval, &converged, &relative, times);
     if (!converged) {
       throw Error(MSIM_NOT_CONVERGED,
                   "HIP depth-averaged flow conjugate gradient did not converge: relative " +
@@ -814,15 +993,14 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
       }
       if (iterations >= params.max_iterations || refinements >= 64) break;
       // Inner FP32 solve of A e = res / ||res|| to `inner`.
-      k_cg_init<float, double><<<stream_blocks(n), kStream, 0, ctx.stream>>>(
-          f.res.data(), 1.0 / rnorm, f.r32.data(), f.x32.data(), f.p032.data(), f.diag.data(), n,
-          inner, f.state.data(), f.partials.data());
-      check(hipGetLastError(), "launch k_cg_init");
+      launch_init<float, double>(ctx, f.res.data(), 1.0 / rnorm, f.r32.data(), f.x32.data(),
+                                 f.p032.data(), inner);
       bool inner_converged = false;
+      double inner_relative = 0.0;
       const uint32_t taken = run_cg<float>(ctx, f.r32.data(), f.x32.data(), f.q32.data(),
                                            f.p032.data(), f.p132.data(),
                                            params.max_iterations - iterations, interval,
-                                           &inner_converged, times);
+                                           &inner_converged, &inner_relative, times);
       iterations += taken;
       ++refinements;
       k_refine<<<stream_blocks(n), kStream, 0, ctx.stream>>>(f.x.data(), f.x32.data(), rnorm, n);

@@ -89,6 +89,13 @@ struct CgState {
   uint32_t converged;
   uint32_t failure;  // ConjugateGradientFailure (conjugate_gradient.hpp:91-98)
   uint32_t done;
+  // Iteration bookkeeping of the Krylov kernels. Each field is written by one
+  // kernel's block 0 and read by the other kernel, never by blocks of the
+  // launch that writes it (halt aside: see k_cg_apply).
+  uint32_t it_update;   // last iteration whose update kernel ran
+  uint32_t it_apply;    // iteration of the last operator kernel
+  uint32_t halt;        // iteration at which r.r met the threshold; 0 while running
+  uint32_t pad;
   uint32_t ticket[8];
 };
 
@@ -118,6 +125,7 @@ struct FlowDevice {
   DeviceBuffer<double> x, r, q, p0, p1, res;
   DeviceBuffer<float> x32, r32, q32, p032, p132;
   DeviceBuffer<double> partials;
+  DeviceBuffer<double> pq, rz, rr; // per-block Krylov partial sums, two generations each
   DeviceBuffer<CgState> state;
   DeviceBuffer<float> faces;       // x faces then y faces (z faces are zero)
   CgState* host_state = nullptr;   // pinned readback
@@ -147,6 +155,7 @@ struct AdrDevice {
   uint64_t revision = 0;
   double greatest_outflow = 0.0;
   double max_loss[kMaxSignals] = {};
+  double host_mass[kMaxSignals] = {};  // mass of each signal now (FP64 runs)
   DeviceBuffer<uint8_t> solid;
   DeviceBuffer<float> xf, yf, phi, source, loss;
   DeviceBuffer<uint8_t> levels_a, levels_b;  // S*N of float or double
