@@ -57,7 +57,8 @@ enum { MSIM_NO_FLUX = 0, MSIM_PERIODIC = 1, MSIM_FIXED = 2 };
 enum {
   MSIM_SOLVER_JACOBI_FP64 = 0, /* the reference's FP64 Jacobi-PCG, on the device */
   MSIM_SOLVER_MIXED = 1,       /* FP32 Jacobi-PCG inside FP64 iterative refinement */
-  MSIM_SOLVER_LEARNED = 2      /* learned multilevel preconditioner in flexible CG */
+  MSIM_SOLVER_LEARNED = 2,     /* learned multilevel preconditioner in flexible CG */
+  MSIM_SOLVER_MULTIGRID = 3    /* the same multilevel cycle without the learned smoother */
 };
 
 typedef struct msim_ctx msim_ctx;
@@ -154,6 +155,11 @@ int msim_hip_flow_download(msim_ctx* ctx, double* pressure, float* x_faces, floa
  * system: y = A x for host vectors of N doubles (zero rows at inactive
  * sites). For checking the device operator. */
 int msim_hip_flow_apply(msim_ctx* ctx, const double* x, double* y);
+
+/* One application of the multilevel preconditioner of the MULTIGRID solver
+ * (no counterpart in the reference) to a host residual of N floats: z = M(r).
+ * For checking the device cycle against a CPU restatement. */
+int msim_hip_flow_precondition(msim_ctx* ctx, const float* r, float* z);
 
 /* The assembled per-site arrays of the prepared system (N floats each, any
  * pointer may be NULL): column conductance, Jacobi diagonal, right-hand side

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "libmsim_hip.so")
 
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
-SOLVER_JACOBI_FP64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
+SOLVER_JACOBI_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_MULTIGRID = 0, 1, 2, 3
 
 
 # --------------------------------------------------------------------------
@@ -120,7 +120,7 @@ EXPORTS = (
     "msim_hip_device_count", "msim_hip_create", "msim_hip_destroy", "msim_hip_last_error",
     "msim_hip_device_name", "msim_hip_solve_depth_averaged_flow", "msim_hip_flow_prepare",
     "msim_hip_flow_solve", "msim_hip_flow_download", "msim_hip_flow_apply",
-    "msim_hip_flow_arrays", "msim_hip_advance_signal_grid", "msim_hip_adr_upload",
+    "msim_hip_flow_arrays", "msim_hip_flow_precondition", "msim_hip_advance_signal_grid", "msim_hip_adr_upload",
     "msim_hip_adr_run", "msim_hip_adr_download", "msim_hip_adr_seconds",
 )
 
@@ -148,6 +148,7 @@ def library(path: str = LIBRARY):
         "msim_hip_flow_download": (I, [P, P, P, P, P]),
         "msim_hip_flow_apply": (I, [P, P, P]),
         "msim_hip_flow_arrays": (I, [P, P, P, P, P]),
+        "msim_hip_flow_precondition": (I, [P, P, P]),
         "msim_hip_advance_signal_grid": (I, [P, P, C.c_float, P, P]),
         "msim_hip_adr_upload": (I, [P, P, P, U, U]),
         "msim_hip_adr_run": (I, [P, C.c_float, U, P]),
@@ -463,6 +464,13 @@ class HipBackend:
         y = np.zeros_like(xin)
         self._check(self.lib.msim_hip_flow_apply(self.ctx, xin.ctypes.data, y.ctypes.data))
         return y
+
+    def flow_precondition(self, r) -> np.ndarray:
+        """One application of the multilevel cycle (FP32) to r."""
+        rin = np.ascontiguousarray(r, np.float32).reshape(-1)
+        z = np.zeros_like(rin)
+        self._check(self.lib.msim_hip_flow_precondition(self.ctx, rin.ctypes.data, z.ctypes.data))
+        return z
 
     def flow_arrays(self) -> dict:
         n = self._grid.nx * self._grid.ny

@@ -167,7 +167,7 @@ void validate_flow_params(const msim_flow_params& p) {
     invalid("depth-averaged relative tolerance must be finite and at least 1e-15");
   }
   if (p.max_iterations == 0) invalid("depth-averaged iteration limit must be positive");
-  if (p.solver > MSIM_SOLVER_LEARNED) invalid("unknown pressure solver");
+  if (p.solver > MSIM_SOLVER_MULTIGRID) invalid("unknown pressure solver");
 }
 
 // The transport fields of SignalGridSpec::validate (signals.cpp:609-700) for
@@ -456,6 +456,15 @@ int msim_hip_flow_apply(msim_ctx* ctx, const double* x, double* y) {
     if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
     if (x == nullptr || y == nullptr) msim::invalid("vectors must not be null");
     msim::flow_apply(c, x, y);
+  });
+}
+
+int msim_hip_flow_precondition(msim_ctx* ctx, const float* r, float* z) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    if (r == nullptr || z == nullptr) msim::invalid("vectors must not be null");
+    msim::mg_precondition(c, r, z);
+    c.flow.solved = false;
   });
 }
 

@@ -21,17 +21,12 @@
 
 #include "common.hpp"
 #include "internal.hpp"
+#include "stencil.hpp"
 
 namespace msim {
 namespace {
 
-constexpr int kStream = 256;  // threads per block of the streaming kernels
 
-// cpp/core/flow_system.hpp:171-174.
-__device__ __host__ inline float harmonic(float a, float b) {
-  const float sum = a + b;
-  return sum > 0.0f ? 2.0f * a * b / sum : 0.0f;
-}
 
 // Reduction (flow_system.hpp:276-327) for one layer: the column conductance
 // is the layer height times the mobility; solid sites carry none.
@@ -113,30 +108,6 @@ __global__ void k_assemble(const float* __restrict__ cond, float* __restrict__ d
 // neighbours meet in one L2.
 // ---------------------------------------------------------------------------
 
-struct Tile {
-  uint32_t ty, x0, x1, y;
-  bool valid, lo_halo, hi_halo;
-  size_t ylo, yhi;
-};
-
-template <int TY>
-__device__ inline Tile tile_of(uint32_t nx, uint32_t ny, uint32_t rows) {
-  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t nty = (ny + TY - 1) / TY;
-  Tile tile;
-  tile.ty = t % nty;
-  tile.x0 = (t / nty) * rows;
-  tile.x1 = min(nx, tile.x0 + rows);
-  tile.y = tile.ty * TY + threadIdx.x;
-  tile.valid = tile.y < ny;
-  // Thread 0 also carries the site below the tile, thread TY-1 the one above.
-  tile.lo_halo = threadIdx.x == 0 && tile.ty > 0;
-  tile.hi_halo = threadIdx.x == TY - 1 && tile.ty * TY + TY < ny;
-  tile.ylo = tile.ty * TY - 1;
-  tile.yhi = static_cast<size_t>(tile.ty) * TY + TY;
-  return tile;
-}
-
 // What one thread loads of one row for the CG operator kernel.
 template <typename T>
 struct CgRow {
@@ -147,31 +118,6 @@ struct CgRow {
   T hr, hpo;  // site above the tile (thread TY-1)
   float hd, hcy;
 };
-
-// Per-block partial sums of the Krylov kernels, two generations of each
-// (indexed by iteration parity): p . A p from the operator kernel (p1 blocks),
-// r . r and r . M^-1 r from the update kernel and the set-up (p2 blocks).
-struct CgPartials {
-  double* pq;
-  double* rz;
-  double* rr;
-  uint32_t p1, p2;
-};
-
-// Sum of `count` partials in a fixed order, the same in every thread of every
-// block that calls it: thread t adds t, t+NT, ... in order, then the block
-// combines waves in order.
-template <int NT>
-__device__ inline double fixed_sum(const double* v, uint32_t count, double* lds, double* bcast) {
-  double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < count; i += NT) s += v[i];
-  s = block_sum<NT>(s, lds);
-  if (threadIdx.x == 0) *bcast = s;
-  __syncthreads();
-  const double out = *bcast;
-  __syncthreads();
-  return out;
-}
 
 // One CG iteration k's first half, with iteration k-1's solution update and
 // the direction update folded in (k = it_update + 1):
@@ -666,6 +612,8 @@ __global__ __launch_bounds__(kStream) void k_scale(float* __restrict__ faces, si
   }
 }
 
+}  // namespace
+
 // --------------------------------------------------------------------------
 // Host orchestration
 // --------------------------------------------------------------------------
@@ -812,11 +760,6 @@ const char* failure_text(uint32_t failure) {
   }
 }
 
-struct KernelTimes {
-  double apply_seconds = 0.0, update_seconds = 0.0;
-  uint32_t apply_launches = 0, update_launches = 0;
-};
-
 // Runs Jacobi-PCG iterations of precision T on the prepared system from the
 // state launch_init left, until the stopping test holds, a failure, or
 // `budget` iterations (conjugate_gradient.hpp:126-166: the host looks at the
@@ -897,8 +840,6 @@ uint32_t run_cg(Context& ctx, T* r, T* x, T* q, T* pa, T* pb, uint32_t budget, u
   }
 }
 
-}  // namespace
-
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
@@ -936,6 +877,7 @@ void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobilit
   check(hipStreamSynchronize(ctx.stream), "assemble");
   f.prepared = true;
   f.solved = false;
+  f.mg.ready = false;
 }
 
 void flow_apply(Context& ctx, const double* x, double* y) {
@@ -1021,6 +963,9 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
                   "HIP mixed-precision depth-averaged flow did not converge: relative " +
                       std::to_string(relative));
     }
+  } else if (params.solver == MSIM_SOLVER_MULTIGRID) {
+    iterations = mg_solve(ctx, params, report, times);
+    relative = report.relative_residual;
   } else {
     throw Error(MSIM_UNSUPPORTED, "this pressure solver form is not available");
   }

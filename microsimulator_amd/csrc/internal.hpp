@@ -8,6 +8,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -112,6 +113,25 @@ enum Ticket : int {
 
 constexpr int kMaxSignals = 8;
 
+// One level of the multilevel preconditioner: a 5-point operator on a
+// (nx, ny) grid of aggregates, as couplings to the lower x and y neighbour
+// (wx, wy), a boundary term (bc) and the inverse diagonal; r and e hold the
+// level's residual and correction during a cycle.
+struct MgLevel {
+  uint32_t nx = 0, ny = 0;
+  DeviceBuffer<float> wx, wy, bc, dinv, r, e;
+  [[nodiscard]] std::size_t cells() const { return static_cast<std::size_t>(nx) * ny; }
+};
+
+struct MgDevice {
+  std::vector<std::unique_ptr<MgLevel>> levels;
+  uint32_t tail = 0;          // first level the single-block kernel handles
+  uint32_t coarse_n = 0;      // cells of the coarsest level
+  DeviceBuffer<float> cinv;   // inverse of the coarsest operator (dense)
+  DeviceBuffer<double> zr, zq, pq, rr;  // flexible-CG partial sums, two generations
+  bool ready = false;
+};
+
 // Everything the prepared depth-averaged system keeps on the device.
 struct FlowDevice {
   uint32_t nx = 0, ny = 0, axis = 0;
@@ -129,6 +149,7 @@ struct FlowDevice {
   DeviceBuffer<CgState> state;
   DeviceBuffer<float> faces;       // x faces then y faces (z faces are zero)
   CgState* host_state = nullptr;   // pinned readback
+  MgDevice mg;
   [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(nx) * ny; }
   [[nodiscard]] std::size_t x_faces() const { return static_cast<std::size_t>(nx + 1) * ny; }
   [[nodiscard]] std::size_t y_faces() const { return static_cast<std::size_t>(nx) * (ny + 1); }
@@ -178,8 +199,30 @@ struct Context {
   AdrDevice adr;
 };
 
+// Summed event time and launch count of the two Krylov kernels of a solve.
+struct KernelTimes {
+  double apply_seconds = 0.0, update_seconds = 0.0;
+  uint32_t apply_launches = 0, update_launches = 0;
+};
+
 // flow.hip
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host);
+uint32_t stream_blocks(size_t n);
+uint32_t krylov_blocks(size_t n);
+void launch_residual(Context& ctx, const double* x, double* out, int mode);
+void read_state(Context& ctx);
+int failure_status(uint32_t failure);
+const char* failure_text(uint32_t failure);
+
+// mg.hip: the multilevel preconditioner and flexible CG.
+void mg_prepare(Context& ctx);
+// One application of the multilevel cycle to a host residual (FP32).
+void mg_precondition(Context& ctx, const float* r_host, float* z_host);
+// FP32 flexible CG preconditioned by the multilevel cycle inside FP64
+// iterative refinement; leaves the FP64 pressure in flow.x and the true
+// residual in the solve state. Returns the Krylov iterations taken.
+uint32_t mg_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
+                  KernelTimes* times);
 void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report);
 void flow_apply(Context& ctx, const double* x, double* y);
 

@@ -200,3 +200,75 @@ def test_error_paths(hip):
     res = hip.solve_depth_averaged_flow(duct, None, params(axis=1, mean_inlet_speed=5.0,
                                                            relative_tolerance=1e-6))
     assert np.all(np.abs(res.field.y_faces - 5.0) <= 1e-5)
+
+
+def _multilevel_reference(oracle, og, mob, nx, ny, dx, dy):
+    from tests import mg_reference as mr
+    a = oracle.assemble(og, mob, 0)
+    m = a["conductance"].reshape(nx, ny).astype(np.float32)
+
+    def harm(p, q):
+        s = p + q
+        return np.where(s > 0, np.float32(2) * p * q / np.where(s > 0, s, 1), 0).astype(np.float32)
+    cxl = np.zeros((nx, ny), np.float32)
+    cxl[1:] = harm(m[1:], m[:-1])
+    cyl = np.zeros((nx, ny), np.float32)
+    cyl[:, 1:] = harm(m[:, 1:], m[:, :-1])
+    return mr.Multilevel(mr.level0(cxl, cyl, a["bc"], nx, ny, dx, dy))
+
+
+@pytest.mark.parametrize("shape", [(256, 128), (301, 77), (1024, 512), (20, 13)])
+def test_multilevel_cycle_matches_cpu_restatement(hip, oracle, shape):
+    """One device cycle (FP32) against the float64 restatement."""
+    nx, ny = shape
+    solid, mob = random_case(13, nx, ny, solid_fraction=0.1)
+    g, og = grids(nx, ny, 1.0, solid, dx=1.0, dy=1.5)
+    hip.flow_prepare(g, mob, 0)
+    ml = _multilevel_reference(oracle, og, mob, nx, ny, 1.0, 1.5)
+    r = np.random.default_rng(2).standard_normal(nx * ny).astype(np.float32)
+    r[solid != 0] = 0.0
+    z = hip.flow_precondition(r).astype(np.float64)
+    ref = ml.cycle(r.astype(np.float64).reshape(nx, ny)).ravel()
+    err = np.abs(z - ref).max() / np.abs(ref).max()
+    print(shape, "levels", len(ml.levels), "relative error", err, flush=True)
+    assert err <= 2e-4
+
+
+@pytest.mark.parametrize("case", ["c1", "illcond"])
+def test_multigrid_solver_reaches_1e8(hip, oracle, case):
+    if case == "c1":
+        c = c1_inputs()
+        g, og = grids(c["nx"], c["ny"], c["dz"], c["obstacles"])
+        mob, axis, speed = c["mobility"], 0, 100.0
+    else:
+        c = illcond_inputs()
+        kinds = (NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX)
+        g, og = grids(64, 64, 1.0, c["obstacles"], kinds)
+        mob, axis, speed = c["mobility"], 1, 1.0
+    hip.flow_prepare(g, mob, axis)
+    rep = hip.flow_solve(params(solver=msa.SOLVER_MULTIGRID, relative_tolerance=1e-8, axis=axis,
+                                mean_inlet_speed=speed))
+    _, field = hip.flow_download()
+    ref_faces, _, ref_rep = oracle.solve_flow(og, mob, axis, speed, 1e-12, 200000)
+    err = np.abs(field.concatenated() - ref_faces).max() / ref_rep.max_speed
+    print(case, "multigrid iterations", rep.iterations, "refinements", rep.refinements,
+          "true", rep.true_relative_residual, "velocity error / max speed", err, flush=True)
+    assert rep.true_relative_residual <= 1e-8
+    assert err <= 1e-5
+    assert rep.iterations <= 150
+
+
+def test_multigrid_full_size(hip):
+    from microsimulator_amd.workloads import config
+    w = config("c2")
+    g = msa.Grid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    hip.flow_prepare(g, w.mobility, 0)
+    fp64 = hip.flow_solve(params(relative_tolerance=1e-8))
+    _, f64 = hip.flow_download()
+    mg = hip.flow_solve(params(solver=msa.SOLVER_MULTIGRID, relative_tolerance=1e-8))
+    _, fmg = hip.flow_download()
+    print("fp64", fp64.iterations, fp64.seconds, "multigrid", mg.iterations, mg.refinements,
+          mg.seconds, flush=True)
+    assert mg.true_relative_residual <= 1e-8
+    assert np.abs(fmg.concatenated() - f64.concatenated()).max() <= 1e-4 * fp64.max_speed
+    assert mg.iterations <= 150
