@@ -5,8 +5,10 @@ cell-updates/s on one MI355X, on the 4096 x 2048 pillar-array channel
 A step is one pressure solve from the assembled system resident in HBM to an
 FP64 true residual ||b - Ax||/||b|| <= 1e-8, including the final true-residual
 check and the velocity reconstruction (lift and scaling). The headline solver
-form is FP32 Jacobi-PCG inside FP64 iterative refinement; the plain FP64
-Jacobi-PCG (the reference's algorithm) is timed beside it on the same GPU.
+form (--solver, default multigrid) is FP32 flexible CG preconditioned by the
+multilevel cycle inside FP64 iterative refinement; the plain FP64 Jacobi-PCG
+(the reference's algorithm) and FP32 Jacobi-PCG with FP64 refinement are timed
+beside it on the same GPU.
 Transport: 3 solutes (D = 100, 500, 1000 um^2/s) with Monod uptake, FP64
 state, advected by the solved field, FP64 mass ledger every step.
 
@@ -32,6 +34,16 @@ METRIC = "pressure solves/s at rel-res 1e-8 (learned vs plain PCG); ADR Gcell-up
 
 def log(*args):
     print(*args, file=sys.stderr, flush=True)
+
+
+# Compulsory HBM bytes per grid site of the multilevel solver's finest-level
+# kernels (FP32 vectors, FP32 coefficients):
+#   k_mg_down   reads r, dinv, wx, wy, bc; writes the coarse residual (1/4 site)
+#   k_mg_up     reads r, dinv, wx, wy, bc, q, the coarse correction (1/4); writes z
+#   k_fcg_apply reads z, p_old, x, cxl, cyl, bc; writes p_new, q, x
+#   k_fcg_update reads r, q; writes r
+MG_BYTES = {"k_mg_down": 5 * 4 + 1, "k_mg_up": 6 * 4 + 1 + 4, "k_fcg_apply": 6 * 4 + 3 * 4,
+            "k_fcg_update": 3 * 4}
 
 
 def apply_bytes(precision: int) -> int:
@@ -117,9 +129,10 @@ def cpu_adr_baseline(w, faces, threads: int, steps: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c5")
+    ap.add_argument("--solver", default="multigrid", choices=("multigrid", "mixed"))
     ap.add_argument("--adr-steps", type=int, default=100)
     ap.add_argument("--fp64-solves", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -156,60 +169,78 @@ def main():
     hip.flow_prepare(grid, w.mobility, 0)
     tol = 1e-8
     speed = 100.0
+    def solver_params(kind, profile=False):
+        solver = {"multigrid": msa.SOLVER_MULTIGRID, "mixed": msa.SOLVER_MIXED,
+                  "fp64": msa.SOLVER_JACOBI_FP64}[kind]
+        return msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
+                                               relative_tolerance=tol, max_iterations=400000,
+                                               solver=solver, profile=profile)
     if args.kernel_iterations:
-        for solver in (msa.SOLVER_MIXED, msa.SOLVER_JACOBI_FP64):
+        for kind in ("multigrid", "mixed", "fp64"):
+            p = solver_params(kind)
+            p.max_iterations = args.kernel_iterations
             try:
-                hip.flow_solve(msa.DepthAveragedFlowParameters(
-                    mean_inlet_speed=speed, axis=0, relative_tolerance=tol,
-                    max_iterations=args.kernel_iterations, solver=solver))
+                hip.flow_solve(p)
             except msa.NotConverged:
                 pass
         log(f"ran {args.kernel_iterations} iterations of each solver form")
         return
-    mixed = msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
-                                            relative_tolerance=tol, max_iterations=400000,
-                                            solver=msa.SOLVER_MIXED)
-    fp64 = msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
-                                           relative_tolerance=tol, max_iterations=400000,
-                                           solver=msa.SOLVER_JACOBI_FP64)
 
+    headline = solver_params(args.solver)
     for k in range(args.warmup):
-        rep = hip.flow_solve(mixed)
+        rep = hip.flow_solve(headline)
         log(f"warmup {k}: {rep.iterations} it, {rep.refinements} refinements, "
-            f"{rep.seconds:.3f} s, true {rep.true_relative_residual:.2e}")
+            f"{rep.seconds:.4f} s, true {rep.true_relative_residual:.2e}")
     reports = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        reports.append(hip.flow_solve(mixed))
+        reports.append(hip.flow_solve(headline))
     elapsed = time.perf_counter() - t0
     for k, rep in enumerate(reports):
-        log(f"step {k}: {rep.iterations} it, {rep.refinements} ref, {rep.seconds:.3f} s, "
+        log(f"step {k}: {rep.iterations} it, {rep.refinements} ref, {rep.seconds:.4f} s, "
             f"true {rep.true_relative_residual:.2e}")
     solves_per_s = args.steps / elapsed
     worst_true = max(r.true_relative_residual for r in reports)
 
-    # Plain FP64 Jacobi-PCG, the reference's algorithm, on the same GPU.
-    t0 = time.perf_counter()
-    fp64_reports = [hip.flow_solve(fp64) for _ in range(args.fp64_solves)]
-    fp64_elapsed = time.perf_counter() - t0
-    fp64_rep = fp64_reports[-1]
-    fp64_solves_per_s = args.fp64_solves / fp64_elapsed
-    log(f"fp64 jacobi-pcg: {fp64_rep.iterations} it, {fp64_rep.seconds:.3f} s, "
-        f"true {fp64_rep.true_relative_residual:.2e}")
+    # The other forms on the same GPU: the reference's algorithm (FP64
+    # Jacobi-PCG) and FP32 Jacobi-PCG with FP64 refinement.
+    others = {}
+    for kind in ("fp64", "mixed"):
+        if kind == args.solver:
+            continue
+        t0 = time.perf_counter()
+        reps = [hip.flow_solve(solver_params(kind)) for _ in range(args.fp64_solves)]
+        others[kind] = (args.fp64_solves / (time.perf_counter() - t0), reps[-1])
+        log(f"{kind}: {reps[-1].iterations} it, {reps[-1].seconds:.3f} s, "
+            f"true {reps[-1].true_relative_residual:.2e}")
+    fp64_solves_per_s, fp64_rep = others["fp64"]
 
-    # One profiled solve: every Krylov kernel launch timed with HIP events on
-    # the context's stream.
-    prof = hip.flow_solve(msa.DepthAveragedFlowParameters(
-        mean_inlet_speed=speed, axis=0, relative_tolerance=tol, max_iterations=400000,
-        solver=msa.SOLVER_MIXED, profile=True))
-    apply_avg = prof.apply_seconds / max(prof.apply_launches, 1)
-    update_avg = prof.update_seconds / max(prof.update_launches, 1)
+    # One profiled solve of the headline form: every kernel of the finest
+    # level timed with HIP events on the context's stream.
+    prof = hip.flow_solve(solver_params(args.solver, profile=True))
     sites = w.sites
-    apply_traffic = apply_bytes(32) * sites
-    achieved = apply_traffic / apply_avg / 1e9
-    log(f"k_cg_apply<float>: {prof.apply_launches} launches, avg {apply_avg * 1e6:.1f} us, "
-        f"{achieved:.0f} GB/s; k_cg_update<float>: avg {update_avg * 1e6:.1f} us, "
-        f"{update_bytes(32) * sites / update_avg / 1e9:.0f} GB/s")
+    if args.solver == "multigrid":
+        c = max(prof.cycles, 1)
+        per_launch = {"k_mg_down": prof.cycle_down_seconds / c,
+                      "k_mg_up": prof.cycle_up_seconds / c,
+                      "k_fcg_apply": prof.apply_seconds / max(prof.apply_launches, 1),
+                      "k_fcg_update": prof.update_seconds / max(prof.update_launches, 1)}
+        rest = prof.cycle_rest_seconds / c
+        for name, sec in per_launch.items():
+            log(f"{name}: avg {sec * 1e6:.1f} us, {MG_BYTES[name] * sites / sec / 1e9:.0f} GB/s")
+        log(f"coarser levels + tail: avg {rest * 1e6:.1f} us per cycle; {prof.cycles} cycles")
+        dominant = max(per_launch, key=per_launch.get)
+        apply_avg = per_launch[dominant]
+        bytes_per_site = MG_BYTES[dominant]
+        kernel_name = dominant
+        launches = prof.cycles
+    else:
+        apply_avg = prof.apply_seconds / max(prof.apply_launches, 1)
+        bytes_per_site = apply_bytes(32)
+        kernel_name = "k_cg_apply<float>"
+        launches = prof.apply_launches
+    achieved = bytes_per_site * sites / apply_avg / 1e9
+    log(f"dominant {kernel_name}: avg {apply_avg * 1e6:.1f} us, {achieved:.0f} GB/s")
     pressure, field = hip.flow_download()
 
     # Transport over the solved field, FP64 state and ledger.
@@ -255,7 +286,9 @@ def main():
             "workload": (f"pillar-array channel {w.nx}x{w.ny}, {w.meta.get('pillars')} pillars "
                          f"r~U[8,20], phi from {w.meta.get('blobs')} Gaussian blobs, "
                          f"Dirichlet 1/0 along x, rel-res 1e-8 (FP64 true residual)"),
-            "solver": "FP32 Jacobi-PCG inside FP64 iterative refinement",
+            "solver": ("FP32 flexible CG with the multilevel preconditioner (2x2 aggregation, "
+                       "V(1,1)) inside FP64 iterative refinement" if args.solver == "multigrid"
+                       else "FP32 Jacobi-PCG inside FP64 iterative refinement"),
             "sites": sites,
             "fluid_sites": w.fluid,
         },
@@ -271,6 +304,9 @@ def main():
             "true_relative_residual": fp64_rep.true_relative_residual,
             "speedup_of_headline": round(solves_per_s / fp64_solves_per_s, 3),
         },
+        "other_forms": {k: {"solves_per_s": round(v[0], 5), "iterations": v[1].iterations,
+                            "true_relative_residual": v[1].true_relative_residual}
+                        for k, v in others.items()},
         "adr": {
             "gcell_upd_per_s_fp64": round(adr_gcups, 3),
             "gcell_upd_per_s_fp32": round(adr32_gcups, 3),
@@ -281,17 +317,17 @@ def main():
             "achieved_gbs_fp64": round(adr_gbs, 1),
         },
         "roofline": {
-            "kernel": "k_cg_apply<float> (operator + direction + solution update)",
+            "kernel": kernel_name,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "bytes_per_site": apply_bytes(32),
+            "bytes_per_site": bytes_per_site,
             "sites_per_launch": sites,
             "avg_launch_us": round(apply_avg * 1e6, 2),
-            "launches": prof.apply_launches,
+            "launches": launches,
         },
     }
     if not args.no_cpu:

@@ -107,6 +107,12 @@ typedef struct {
   uint32_t apply_launches;
   double update_seconds;
   uint32_t update_launches;
+  /* MULTIGRID/LEARNED with params.profile: summed event time of the finest
+   * level's down pass, its up pass, and the rest of the cycle, per cycle. */
+  double cycle_down_seconds;
+  double cycle_up_seconds;
+  double cycle_rest_seconds;
+  uint32_t cycles;
 } msim_flow_report;
 
 /* Device count through hipGetDeviceCount; 0 on error, as

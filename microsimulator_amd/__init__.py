@@ -99,7 +99,9 @@ class _FlowReport(C.Structure):
                 ("solved_mean", C.c_float), ("factor", C.c_float), ("fell_back", C.c_uint32),
                 ("seconds", C.c_double), ("apply_seconds", C.c_double),
                 ("apply_launches", C.c_uint32), ("update_seconds", C.c_double),
-                ("update_launches", C.c_uint32)]
+                ("update_launches", C.c_uint32), ("cycle_down_seconds", C.c_double),
+                ("cycle_up_seconds", C.c_double), ("cycle_rest_seconds", C.c_double),
+                ("cycles", C.c_uint32)]
 
 
 class _SignalGrid(C.Structure):
@@ -251,13 +253,18 @@ class DepthAveragedFlowReport:
     apply_launches: int = 0
     update_seconds: float = 0.0
     update_launches: int = 0
+    cycle_down_seconds: float = 0.0
+    cycle_up_seconds: float = 0.0
+    cycle_rest_seconds: float = 0.0
+    cycles: int = 0
 
     @classmethod
     def _from(cls, r: _FlowReport):
         return cls(r.iterations, r.refinements, r.relative_residual, r.true_relative_residual,
                    r.mean_inlet_speed, r.max_speed, r.solved_mean, r.factor, bool(r.fell_back),
                    r.seconds, r.apply_seconds, r.apply_launches, r.update_seconds,
-                   r.update_launches)
+                   r.update_launches, r.cycle_down_seconds, r.cycle_up_seconds,
+                   r.cycle_rest_seconds, r.cycles)
 
 
 @dataclass

@@ -1013,6 +1013,10 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
   report.apply_launches = kernel_times.apply_launches;
   report.update_seconds = kernel_times.update_seconds;
   report.update_launches = kernel_times.update_launches;
+  report.cycle_down_seconds = kernel_times.down_seconds;
+  report.cycle_up_seconds = kernel_times.up_seconds;
+  report.cycle_rest_seconds = kernel_times.rest_seconds;
+  report.cycles = kernel_times.cycles;
   f.solved = true;
 }
 

@@ -203,6 +203,8 @@ struct Context {
 struct KernelTimes {
   double apply_seconds = 0.0, update_seconds = 0.0;
   uint32_t apply_launches = 0, update_launches = 0;
+  double down_seconds = 0.0, up_seconds = 0.0, rest_seconds = 0.0;
+  uint32_t cycles = 0;
 };
 
 // flow.hip

@@ -738,8 +738,10 @@ FcgPartials fcg_partials(Context& ctx) {
   return p;
 }
 
-// z = M(r): the cycle over all levels.
-void launch_cycle(Context& ctx, float* r, float* z, const float* q) {
+// z = M(r): the cycle over all levels. With probes, events are recorded after
+// the finest down pass (probes[0]) and before the finest up pass (probes[1]).
+void launch_cycle(Context& ctx, float* r, float* z, const float* q,
+                  const hipEvent_t* probes = nullptr) {
   auto& mg = ctx.flow.mg;
   const CgState* st = ctx.flow.state.data();
   const FcgPartials parts = fcg_partials(ctx);
@@ -751,9 +753,12 @@ void launch_cycle(Context& ctx, float* r, float* z, const float* q) {
     k_mg_down<<<tile_grid(L), kMgThreads, 0, ctx.stream>>>(view(L), rl, C.r.data(), C.nx, C.ny,
                                                            st);
     check(hipGetLastError(), "launch k_mg_down");
+    if (l == 0 && probes != nullptr) check(hipEventRecord(probes[0], ctx.stream), "event");
   }
+  if (mg.tail == 0 && probes != nullptr) check(hipEventRecord(probes[0], ctx.stream), "event");
   k_mg_tail<<<1, kTailThreads, 0, ctx.stream>>>(tail_args(ctx, r, z, q, parts), st);
   check(hipGetLastError(), "launch k_mg_tail");
+  if (mg.tail == 0 && probes != nullptr) check(hipEventRecord(probes[1], ctx.stream), "event");
   for (int l = static_cast<int>(mg.tail) - 1; l >= 0; --l) {
     MgLevel& L = *mg.levels[l];
     MgLevel& C = *mg.levels[l + 1];
@@ -761,6 +766,7 @@ void launch_cycle(Context& ctx, float* r, float* z, const float* q) {
     float* out = l == 0 ? z : L.e.data();
     const UpDots dots = (l == 0 && q != nullptr) ? UpDots{q, parts.zr, parts.zq}
                                                  : UpDots{nullptr, nullptr, nullptr};
+    if (l == 0 && probes != nullptr) check(hipEventRecord(probes[1], ctx.stream), "event");
     k_mg_up<<<tile_grid(L), kMgThreads, 0, ctx.stream>>>(view(L), rl, C.e.data(), C.ny, out, st,
                                                          dots);
     check(hipGetLastError(), "launch k_mg_up");
@@ -814,8 +820,9 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
                                                            f.partials.data());
   check(hipGetLastError(), "launch k_fcg_init");
   *converged = false;
+  constexpr int kP = 6;  // events per iteration
   if (times != nullptr) {
-    while (ctx.probes.size() < 3 * static_cast<size_t>(interval) + 3) {
+    while (ctx.probes.size() < kP * static_cast<size_t>(interval) + kP) {
       hipEvent_t e;
       check(hipEventCreate(&e), "hipEventCreate");
       ctx.probes.push_back(e);
@@ -833,13 +840,15 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
     const uint32_t batch = last_test ? 0 : std::min(interval, budget - launched);
     for (uint32_t j = 0; j < batch; ++j) {
       const uint32_t it = launched + j + 1;
-      if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j], ctx.stream), "event");
-      launch_cycle(ctx, r, z, q);
+      const hipEvent_t* pr = times != nullptr ? &ctx.probes[kP * j] : nullptr;
+      if (pr != nullptr) check(hipEventRecord(pr[0], ctx.stream), "event");
+      launch_cycle(ctx, r, z, q, pr != nullptr ? pr + 1 : nullptr);
+      if (pr != nullptr) check(hipEventRecord(pr[3], ctx.stream), "event");
       launch_fcg_apply(ctx, z, buf[(it + 1) % 2], buf[it % 2], q, x);
-      if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j + 1], ctx.stream), "event");
+      if (pr != nullptr) check(hipEventRecord(pr[4], ctx.stream), "event");
       k_fcg_update<<<krylov_blocks(n), kStream, 0, ctx.stream>>>(r, q, n, f.state.data(), parts);
       check(hipGetLastError(), "launch k_fcg_update");
-      if (times != nullptr) check(hipEventRecord(ctx.probes[3 * j + 2], ctx.stream), "event");
+      if (pr != nullptr) check(hipEventRecord(pr[5], ctx.stream), "event");
     }
     if (last_test) {
       const uint32_t it = launched + 1;
@@ -854,12 +863,17 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
       const uint32_t last_worked = s.halt != 0 ? s.halt : s.it_update;
       const uint32_t worked = std::min(batch, last_worked > before ? last_worked - before : 0u);
       for (uint32_t j = 0; j < worked; ++j) {
-        float a = 0.0f, b = 0.0f;
-        check(hipEventElapsedTime(&a, ctx.probes[3 * j], ctx.probes[3 * j + 1]), "event time");
-        check(hipEventElapsedTime(&b, ctx.probes[3 * j + 1], ctx.probes[3 * j + 2]),
-              "event time");
-        times->apply_seconds += a * 1.0e-3;  // cycle + operator
-        times->update_seconds += b * 1.0e-3;
+        const hipEvent_t* pr = &ctx.probes[kP * j];
+        float t[5] = {};
+        for (int e = 0; e < 5; ++e) {
+          check(hipEventElapsedTime(&t[e], pr[e], pr[e + 1]), "event time");
+        }
+        times->down_seconds += t[0] * 1.0e-3;
+        times->rest_seconds += t[1] * 1.0e-3;
+        times->up_seconds += t[2] * 1.0e-3;
+        times->apply_seconds += t[3] * 1.0e-3;
+        times->update_seconds += t[4] * 1.0e-3;
+        times->cycles += 1;
         times->apply_launches += 1;
         times->update_launches += 1;
       }
