@@ -46,6 +46,13 @@ MG_BYTES = {"k_mg_down": 5 * 4 + 1, "k_mg_up": 6 * 4 + 1 + 4, "k_fcg_apply": 6 *
             "k_fcg_update": 3 * 4}
 
 
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (scaled by a 256 MiB copy's readings) on this workload; the commands and
+# figures are in profiles/krylov_kernels_4096x2048_traffic.md.
+TRAFFIC_MB = {"k_fcg_apply": 316.1, "k_mg_up": 370.7, "k_mg_down": 204.9, "k_fcg_update": 101.3,
+              "k_cg_apply<float>": 353.8}
+
+
 def apply_bytes(precision: int) -> int:
     v = 8 if precision == 64 else 4
     reads = 3 * v + 4 * 4  # r, p_old, x (vectors); diag, cxl, cyl, bc (float)
@@ -323,7 +330,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (round(TRAFFIC_MB[kernel_name] * 1e6) if kernel_name in TRAFFIC_MB
+                        else None),
+            "traffic_unit": "bytes per launch",
             "bytes_per_site": bytes_per_site,
             "sites_per_launch": sites,
             "avg_launch_us": round(apply_avg * 1e6, 2),
