@@ -5,10 +5,12 @@ cell-updates/s on one MI355X, on the 4096 x 2048 pillar-array channel
 A step is one pressure solve from the assembled system resident in HBM to an
 FP64 true residual ||b - Ax||/||b|| <= 1e-8, including the final true-residual
 check and the velocity reconstruction (lift and scaling). The headline solver
-form (--solver, default multigrid) is FP32 flexible CG preconditioned by the
-multilevel cycle inside FP64 iterative refinement; the plain FP64 Jacobi-PCG
-(the reference's algorithm) and FP32 Jacobi-PCG with FP64 refinement are timed
-beside it on the same GPU.
+form (--solver, default learned) is FP32 flexible CG preconditioned by the
+multilevel cycle with the learned finest-level smoother (bfloat16 MFMA
+convolutions, weights microsimulator_amd/learned_smoother.npz), inside FP64
+iterative refinement. Timed beside it on the same GPU: the same cycle without
+the network (multigrid), the plain FP64 Jacobi-PCG (the reference's
+algorithm) and FP32 Jacobi-PCG with FP64 refinement.
 Transport: 3 solutes (D = 100, 500, 1000 um^2/s) with Monod uptake, FP64
 state, advected by the solved field, FP64 mass ledger every step.
 
@@ -42,8 +44,14 @@ def log(*args):
 #   k_mg_up     reads r, dinv, wx, wy, bc, q, the coarse correction (1/4); writes z
 #   k_fcg_apply reads z, p_old, x, cxl, cyl, bc; writes p_new, q, x
 #   k_fcg_update reads r, q; writes r
+#   k_learn_up  the same traffic as k_mg_up
 MG_BYTES = {"k_mg_down": 5 * 4 + 1, "k_mg_up": 6 * 4 + 1 + 4, "k_fcg_apply": 6 * 4 + 3 * 4,
-            "k_fcg_update": 3 * 4}
+            "k_fcg_update": 3 * 4, "k_learn_up": 6 * 4 + 1 + 4}
+
+# Algorithmic matrix flops per site of the learned smoother (3x3 convolutions
+# 4 -> 16, 16 -> 16, 16 -> 1, two flops per multiply-add), halo excluded.
+LEARNED_FLOPS = 2 * 9 * (4 * 16 + 16 * 16 + 16 * 1)
+BF16_PEAK_TFLOPS = 2500.0
 
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
@@ -139,7 +147,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c5")
-    ap.add_argument("--solver", default="multigrid", choices=("multigrid", "mixed"))
+    ap.add_argument("--solver", default="learned", choices=("learned", "multigrid", "mixed"))
     ap.add_argument("--adr-steps", type=int, default=100)
     ap.add_argument("--fp64-solves", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -174,16 +182,18 @@ def main():
     hip = msa.HipBackend(0)
     log("device", hip.info().device)
     hip.flow_prepare(grid, w.mobility, 0)
+    from microsimulator_amd import learned
+    hip.learned_set_weights(learned.load_weights())
     tol = 1e-8
     speed = 100.0
     def solver_params(kind, profile=False):
-        solver = {"multigrid": msa.SOLVER_MULTIGRID, "mixed": msa.SOLVER_MIXED,
-                  "fp64": msa.SOLVER_JACOBI_FP64}[kind]
+        solver = {"learned": msa.SOLVER_LEARNED, "multigrid": msa.SOLVER_MULTIGRID,
+                  "mixed": msa.SOLVER_MIXED, "fp64": msa.SOLVER_JACOBI_FP64}[kind]
         return msa.DepthAveragedFlowParameters(mean_inlet_speed=speed, axis=0,
                                                relative_tolerance=tol, max_iterations=400000,
                                                solver=solver, profile=profile)
     if args.kernel_iterations:
-        for kind in ("multigrid", "mixed", "fp64"):
+        for kind in ("learned", "multigrid", "mixed", "fp64"):
             p = solver_params(kind)
             p.max_iterations = args.kernel_iterations
             try:
@@ -209,15 +219,19 @@ def main():
     solves_per_s = args.steps / elapsed
     worst_true = max(r.true_relative_residual for r in reports)
 
-    # The other forms on the same GPU: the reference's algorithm (FP64
-    # Jacobi-PCG) and FP32 Jacobi-PCG with FP64 refinement.
+    # The other forms on the same GPU: the cycle without the network, the
+    # reference's algorithm (FP64 Jacobi-PCG) and FP32 Jacobi-PCG with FP64
+    # refinement.
     others = {}
-    for kind in ("fp64", "mixed"):
+    for kind in ("multigrid", "fp64", "mixed"):
         if kind == args.solver:
             continue
+        count = max(args.steps // 2, 1) if kind == "multigrid" else args.fp64_solves
+        if kind == "multigrid":
+            hip.flow_solve(solver_params(kind))  # warm
         t0 = time.perf_counter()
-        reps = [hip.flow_solve(solver_params(kind)) for _ in range(args.fp64_solves)]
-        others[kind] = (args.fp64_solves / (time.perf_counter() - t0), reps[-1])
+        reps = [hip.flow_solve(solver_params(kind)) for _ in range(count)]
+        others[kind] = (count / (time.perf_counter() - t0), reps[-1])
         log(f"{kind}: {reps[-1].iterations} it, {reps[-1].seconds:.3f} s, "
             f"true {reps[-1].true_relative_residual:.2e}")
     fp64_solves_per_s, fp64_rep = others["fp64"]
@@ -226,16 +240,27 @@ def main():
     # level timed with HIP events on the context's stream.
     prof = hip.flow_solve(solver_params(args.solver, profile=True))
     sites = w.sites
-    if args.solver == "multigrid":
+    mfma = None
+    if args.solver in ("multigrid", "learned"):
         c = max(prof.cycles, 1)
+        up = "k_learn_up" if args.solver == "learned" else "k_mg_up"
         per_launch = {"k_mg_down": prof.cycle_down_seconds / c,
-                      "k_mg_up": prof.cycle_up_seconds / c,
+                      up: prof.cycle_up_seconds / c,
                       "k_fcg_apply": prof.apply_seconds / max(prof.apply_launches, 1),
                       "k_fcg_update": prof.update_seconds / max(prof.update_launches, 1)}
         rest = prof.cycle_rest_seconds / c
         for name, sec in per_launch.items():
             log(f"{name}: avg {sec * 1e6:.1f} us, {MG_BYTES[name] * sites / sec / 1e9:.0f} GB/s")
         log(f"coarser levels + tail: avg {rest * 1e6:.1f} us per cycle; {prof.cycles} cycles")
+        if args.solver == "learned":
+            sec = per_launch["k_learn_up"]
+            tflops = LEARNED_FLOPS * sites / sec / 1e12
+            mfma = {"kernel": "k_learn_up", "bound": "mfma", "achieved": round(tflops, 2),
+                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tflops / BF16_PEAK_TFLOPS, 4),
+                    "flops_per_site": LEARNED_FLOPS, "avg_launch_us": round(sec * 1e6, 2)}
+            log(f"k_learn_up: {tflops:.1f} TFLOP/s of bf16 MFMA ({tflops / BF16_PEAK_TFLOPS:.3f}"
+                f" of dense peak)")
         dominant = max(per_launch, key=per_launch.get)
         apply_avg = per_launch[dominant]
         bytes_per_site = MG_BYTES[dominant]
@@ -293,9 +318,13 @@ def main():
             "workload": (f"pillar-array channel {w.nx}x{w.ny}, {w.meta.get('pillars')} pillars "
                          f"r~U[8,20], phi from {w.meta.get('blobs')} Gaussian blobs, "
                          f"Dirichlet 1/0 along x, rel-res 1e-8 (FP64 true residual)"),
-            "solver": ("FP32 flexible CG with the multilevel preconditioner (2x2 aggregation, "
-                       "V(1,1)) inside FP64 iterative refinement" if args.solver == "multigrid"
-                       else "FP32 Jacobi-PCG inside FP64 iterative refinement"),
+            "solver": {
+                "learned": ("FP32 flexible CG, multilevel preconditioner (2x2 aggregation, "
+                            "V(1,1)) with the learned bf16-MFMA finest-level smoother, inside "
+                            "FP64 iterative refinement; Jacobi-PCG fallback at 200 iterations"),
+                "multigrid": ("FP32 flexible CG with the multilevel preconditioner (2x2 "
+                              "aggregation, V(1,1)) inside FP64 iterative refinement"),
+                "mixed": "FP32 Jacobi-PCG inside FP64 iterative refinement"}[args.solver],
             "sites": sites,
             "fluid_sites": w.fluid,
         },
@@ -304,6 +333,7 @@ def main():
             "refinements": reports[-1].refinements,
             "worst_true_relative_residual": worst_true,
             "device_seconds": reports[-1].seconds,
+            "fell_back": any(r.fell_back for r in reports),
         },
         "fp64_jacobi_pcg": {
             "solves_per_s": round(fp64_solves_per_s, 5),
@@ -339,6 +369,18 @@ def main():
             "launches": launches,
         },
     }
+    if mfma is not None:
+        line["roofline_mfma"] = mfma
+    if args.solver == "learned":
+        with np.load(learned.WEIGHTS_FILE, allow_pickle=False) as f:
+            meta = json.loads(str(f["meta"]))
+        line["learned"] = {
+            "training_pairs": meta.get("total_pairs"), "training_geometries": meta.get("geometries"),
+            "training_steps": meta.get("steps"), "train_device_seconds": meta.get("train_seconds"),
+            "harvest_seconds": meta.get("harvest_seconds"),
+            "iterations_vs_multigrid": [reports[-1].iterations,
+                                        others["multigrid"][1].iterations],
+        }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")

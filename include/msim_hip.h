@@ -228,6 +228,84 @@ int msim_hip_adr_download(msim_ctx* ctx, void* levels);
 /* Device time (seconds) of the kernels of the last adr_run. */
 double msim_hip_adr_seconds(msim_ctx* ctx);
 
+/* ---------------------------------------------------------------------------
+ * Learned smoother of the LEARNED solver (no counterpart in the reference,
+ * whose only preconditioner is Jacobi, cpp/cpu/cpu_flow.cpp:65-69).
+ *
+ * The finest level of the multilevel cycle replaces its damped-Jacobi
+ * post-smoother z = x + 0.8 D^-1 r' by z = x + 0.8 D^-1 r' + s N(r'), where
+ * N is a three-layer 3x3 convolutional network (4 -> 16 -> 16 -> 1 channels,
+ * softsign, no biases) applied on 16 x 64 tiles with a 3-site halo, s is the
+ * tile's largest |D^-1 r'| and the inputs are D^-1 r' / s times (1, the
+ * site's x and y couplings and boundary term over the diagonal). Weights
+ * are MSIM_LEARNED_WEIGHTS floats: W1[9][4][16], W2[9][16][16], W3[9][16]
+ * ([tap][input][output], tap = 3 (dx + 1) + (dy + 1)). They run as bfloat16
+ * with FP32 accumulation on the matrix cores; training keeps FP32 weights.
+ * ------------------------------------------------------------------------- */
+#define MSIM_LEARNED_WEIGHTS 3024
+
+/* Sets / reads the smoother's weights (FP32). */
+int msim_hip_learned_set_weights(msim_ctx* ctx, const float* weights, size_t count);
+int msim_hip_learned_get_weights(msim_ctx* ctx, float* weights, size_t count);
+
+/* One application of the LEARNED solver's cycle to a host residual of N
+ * floats (needs weights and a prepared system). For checking. */
+int msim_hip_learned_precondition(msim_ctx* ctx, const float* r, float* z);
+
+/* The finest level's operator of the prepared system, N floats each (any
+ * pointer may be NULL): coupling to the lower x and y neighbour, boundary
+ * term, inverse diagonal (0 at inactive sites). */
+int msim_hip_learned_operator(msim_ctx* ctx, float* wx, float* wy, float* bc, float* dinv);
+
+/* Harvests training pairs from the prepared system. A flexible-CG solve of
+ * the system (multilevel cycle; with the learned smoother when use_learned
+ * is nonzero) gives one residual r per iteration; up to max_pairs of them,
+ * spread evenly over the solve, are kept. For each, an accurate solve gives
+ * e with A e = r, and the pair is taken after the cycle's pre-smoother and
+ * coarse correction x: r' = r - A x, e' = e - x.
+ * residuals and errors: max_pairs * N floats; *produced: pairs written. */
+int msim_hip_learned_harvest(msim_ctx* ctx, uint32_t max_pairs, uint32_t use_learned,
+                             float* residuals, float* errors, uint32_t* produced);
+
+/* A training set: `geometries` operators of one nx x ny shape, as
+ * operators[(g * 4 + k) * N + site] with k = wx, wy, bc, dinv; `pairs`
+ * residual/error pairs (pairs * N floats each) and the geometry of each. */
+typedef struct {
+  uint32_t nx, ny;
+  uint32_t geometries, pairs;
+  const float* operators;
+  const uint32_t* pair_geometry;
+  const float* residuals;
+  const float* errors;
+} msim_train_set;
+
+/* Adam on the mean over a batch of (e' - z)^T A (e' - z) / e'^T A e',
+ * z = 0.8 D^-1 r' + s N(r'). Batches walk seeded shuffles of the set. */
+typedef struct {
+  uint32_t steps;
+  uint32_t batch;
+  float learning_rate;
+  float beta1, beta2, epsilon;
+  uint64_t seed;
+} msim_train_params;
+
+/* Copies a training set to the device (replacing any earlier one). */
+int msim_hip_learned_upload(msim_ctx* ctx, const msim_train_set* set);
+
+/* Trains the current weights in place on the uploaded set, entirely on the
+ * device. losses: `steps` floats (batch loss before each step) or NULL;
+ * seconds: device time, or NULL. */
+int msim_hip_learned_train(msim_ctx* ctx, const msim_train_params* params, float* losses,
+                           double* seconds);
+
+/* Loss of one uploaded pair and its gradient with respect to the weights
+ * (MSIM_LEARNED_WEIGHTS doubles), from the device's forward and backward
+ * kernels. For checking them against a CPU restatement. */
+int msim_hip_learned_gradients(msim_ctx* ctx, uint32_t pair, double* loss, double* gradient);
+
+/* z = 0.8 D^-1 r' + s N(r') for one uploaded pair (N floats). */
+int msim_hip_learned_apply(msim_ctx* ctx, uint32_t pair, float* z);
+
 #ifdef __cplusplus
 }
 #endif

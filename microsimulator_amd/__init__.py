@@ -112,6 +112,18 @@ class _SignalGrid(C.Structure):
                 ("monod_k", C.c_void_p), ("revision", C.c_uint64)]
 
 
+class _TrainSet(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("geometries", C.c_uint32),
+                ("pairs", C.c_uint32), ("operators", C.c_void_p), ("pair_geometry", C.c_void_p),
+                ("residuals", C.c_void_p), ("errors", C.c_void_p)]
+
+
+class _TrainParams(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("seed", C.c_uint64)]
+
+
 class _SignalReport(C.Structure):
     _fields_ = [("converged", C.c_uint32), ("iterations", C.c_uint32),
                 ("residual_rms", C.c_float)]
@@ -124,7 +136,14 @@ EXPORTS = (
     "msim_hip_flow_solve", "msim_hip_flow_download", "msim_hip_flow_apply",
     "msim_hip_flow_arrays", "msim_hip_flow_precondition", "msim_hip_advance_signal_grid", "msim_hip_adr_upload",
     "msim_hip_adr_run", "msim_hip_adr_download", "msim_hip_adr_seconds",
+    "msim_hip_learned_set_weights", "msim_hip_learned_get_weights",
+    "msim_hip_learned_precondition", "msim_hip_learned_operator", "msim_hip_learned_harvest",
+    "msim_hip_learned_upload", "msim_hip_learned_train", "msim_hip_learned_gradients",
+    "msim_hip_learned_apply",
 )
+
+#: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
+LEARNED_WEIGHTS = 3024
 
 _lib = None
 
@@ -156,6 +175,15 @@ def library(path: str = LIBRARY):
         "msim_hip_adr_run": (I, [P, C.c_float, U, P]),
         "msim_hip_adr_download": (I, [P, P]),
         "msim_hip_adr_seconds": (D, [P]),
+        "msim_hip_learned_set_weights": (I, [P, P, C.c_size_t]),
+        "msim_hip_learned_get_weights": (I, [P, P, C.c_size_t]),
+        "msim_hip_learned_precondition": (I, [P, P, P]),
+        "msim_hip_learned_operator": (I, [P, P, P, P, P]),
+        "msim_hip_learned_harvest": (I, [P, U, U, P, P, P]),
+        "msim_hip_learned_upload": (I, [P, P]),
+        "msim_hip_learned_train": (I, [P, P, P, P]),
+        "msim_hip_learned_gradients": (I, [P, U, P, P]),
+        "msim_hip_learned_apply": (I, [P, U, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -513,6 +541,86 @@ class HipBackend:
 
     def adr_seconds(self) -> float:
         return float(self.lib.msim_hip_adr_seconds(self.ctx))
+
+    # -- learned smoother (no counterpart in the reference) ----------------------
+    def learned_set_weights(self, weights):
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        self._check(self.lib.msim_hip_learned_set_weights(self.ctx, w.ctypes.data, w.size))
+
+    def learned_weights(self) -> np.ndarray:
+        w = np.zeros(LEARNED_WEIGHTS, np.float32)
+        self._check(self.lib.msim_hip_learned_get_weights(self.ctx, w.ctypes.data, w.size))
+        return w
+
+    def learned_precondition(self, r) -> np.ndarray:
+        """One application of the LEARNED solver's cycle (FP32) to r."""
+        rin = np.ascontiguousarray(r, np.float32).reshape(-1)
+        z = np.zeros_like(rin)
+        self._check(self.lib.msim_hip_learned_precondition(self.ctx, rin.ctypes.data,
+                                                           z.ctypes.data))
+        return z
+
+    def learned_operator(self) -> dict:
+        """The finest level's wx, wy, bc and dinv (float32, nx x ny)."""
+        g = self._grid
+        out = {k: np.zeros((g.nx, g.ny), np.float32) for k in ("wx", "wy", "bc", "dinv")}
+        self._check(self.lib.msim_hip_learned_operator(
+            self.ctx, out["wx"].ctypes.data, out["wy"].ctypes.data, out["bc"].ctypes.data,
+            out["dinv"].ctypes.data))
+        return out
+
+    def learned_harvest(self, max_pairs: int, use_learned: bool = False):
+        """Training pairs (r', e') from the prepared system: arrays (k, nx, ny)."""
+        g = self._grid
+        r = np.zeros((max_pairs, g.nx, g.ny), np.float32)
+        e = np.zeros_like(r)
+        produced = C.c_uint32(0)
+        self._check(self.lib.msim_hip_learned_harvest(self.ctx, max_pairs, 1 if use_learned else 0,
+                                                      r.ctypes.data, e.ctypes.data,
+                                                      C.byref(produced)))
+        k = produced.value
+        return r[:k], e[:k]
+
+    def learned_upload(self, operators, pair_geometry, residuals, errors):
+        """operators (G, 4, nx, ny): wx, wy, bc, dinv; pair_geometry (P,);
+        residuals, errors (P, nx, ny)."""
+        ops = np.ascontiguousarray(operators, np.float32)
+        rp = np.ascontiguousarray(residuals, np.float32)
+        ep = np.ascontiguousarray(errors, np.float32)
+        geo = np.ascontiguousarray(pair_geometry, np.uint32)
+        s = _TrainSet()
+        s.nx, s.ny = ops.shape[2], ops.shape[3]
+        s.geometries, s.pairs = ops.shape[0], rp.shape[0]
+        s.operators, s.pair_geometry = ops.ctypes.data, geo.ctypes.data
+        s.residuals, s.errors = rp.ctypes.data, ep.ctypes.data
+        self._check(self.lib.msim_hip_learned_upload(self.ctx, C.byref(s)))
+        self._train_sites = int(s.nx) * int(s.ny)
+
+    def learned_train(self, steps: int, batch: int = 8, learning_rate: float = 1e-3,
+                      betas=(0.9, 0.999), epsilon: float = 1e-8, seed: int = 0):
+        """Adam steps on the uploaded set. Returns (batch losses, device seconds)."""
+        p = _TrainParams(steps, batch, learning_rate, betas[0], betas[1], epsilon, seed)
+        losses = np.zeros(steps, np.float32)
+        seconds = C.c_double(0.0)
+        self._check(self.lib.msim_hip_learned_train(self.ctx, C.byref(p), losses.ctypes.data,
+                                                    C.byref(seconds)))
+        return losses, seconds.value
+
+    def learned_gradients(self, pair: int):
+        """(loss, gradient) of one uploaded pair from the device kernels."""
+        loss = C.c_double(0.0)
+        grad = np.zeros(LEARNED_WEIGHTS, np.float64)
+        self._check(self.lib.msim_hip_learned_gradients(self.ctx, pair, C.byref(loss),
+                                                        grad.ctypes.data))
+        return loss.value, grad
+
+    def learned_apply(self, pair: int) -> np.ndarray:
+        """The smoother's output z for one uploaded pair (flat, float32)."""
+        if not hasattr(self, "_train_sites"):
+            raise InvalidArgument("no training set is uploaded")
+        z = np.zeros(self._train_sites, np.float32)
+        self._check(self.lib.msim_hip_learned_apply(self.ctx, pair, z.ctypes.data))
+        return z
 
     # -- the rest of ComputeBackend: not on this backend ------------------------
     def _unsupported(self, *args, **kwargs):

@@ -463,8 +463,116 @@ int msim_hip_flow_precondition(msim_ctx* ctx, const float* r, float* z) {
   return msim::guarded(ctx, [&](Context& c) {
     if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
     if (r == nullptr || z == nullptr) msim::invalid("vectors must not be null");
-    msim::mg_precondition(c, r, z);
-    c.flow.solved = false;
+    msim::mg_precondition(c, r, z, false);
+  });
+}
+
+int msim_hip_learned_set_weights(msim_ctx* ctx, const float* weights, size_t count) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (weights == nullptr) msim::invalid("weights must not be null");
+    if (count != MSIM_LEARNED_WEIGHTS) {
+      msim::invalid("the learned smoother has " + std::to_string(MSIM_LEARNED_WEIGHTS) +
+                    " weights, got " + std::to_string(count));
+    }
+    for (size_t k = 0; k < count; ++k) {
+      if (!std::isfinite(weights[k])) msim::invalid("weights must be finite");
+    }
+    msim::learned_set_weights(c, weights);
+  });
+}
+
+int msim_hip_learned_get_weights(msim_ctx* ctx, float* weights, size_t count) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (weights == nullptr || count != MSIM_LEARNED_WEIGHTS) {
+      msim::invalid("weights must hold MSIM_LEARNED_WEIGHTS floats");
+    }
+    msim::learned_get_weights(c, weights);
+  });
+}
+
+int msim_hip_learned_precondition(msim_ctx* ctx, const float* r, float* z) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    if (r == nullptr || z == nullptr) msim::invalid("vectors must not be null");
+    if (!c.learned.loaded) msim::invalid("the learned smoother has no weights: set them first");
+    msim::mg_precondition(c, r, z, true);
+  });
+}
+
+int msim_hip_learned_operator(msim_ctx* ctx, float* wx, float* wy, float* bc, float* dinv) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    msim::mg_operator(c, wx, wy, bc, dinv);
+  });
+}
+
+int msim_hip_learned_harvest(msim_ctx* ctx, uint32_t max_pairs, uint32_t use_learned,
+                             float* residuals, float* errors, uint32_t* produced) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    if (residuals == nullptr || errors == nullptr || produced == nullptr) {
+      msim::invalid("output arrays must not be null");
+    }
+    if (max_pairs == 0) msim::invalid("max_pairs must be positive");
+    if (use_learned != 0 && !c.learned.loaded) {
+      msim::invalid("the learned smoother has no weights: set them first");
+    }
+    *produced = 0;
+    *produced = msim::mg_harvest(c, max_pairs, use_learned != 0, residuals, errors);
+  });
+}
+
+int msim_hip_learned_upload(msim_ctx* ctx, const msim_train_set* set) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (set == nullptr) msim::invalid("the training set must not be null");
+    if (set->nx == 0 || set->ny == 0 || set->geometries == 0 || set->pairs == 0) {
+      msim::invalid("the training set is empty");
+    }
+    if (static_cast<uint64_t>(set->nx) * set->ny > (uint64_t{1} << 31)) {
+      throw msim::Error(MSIM_OVERFLOW, "training grids are limited to 2^31 sites");
+    }
+    if (set->operators == nullptr || set->pair_geometry == nullptr ||
+        set->residuals == nullptr || set->errors == nullptr) {
+      msim::invalid("training set arrays must not be null");
+    }
+    if (set->pairs > 65535) msim::invalid("at most 65535 pairs per training set");
+    for (uint32_t p = 0; p < set->pairs; ++p) {
+      if (set->pair_geometry[p] >= set->geometries) {
+        msim::invalid("pair " + std::to_string(p) + " names geometry " +
+                      std::to_string(set->pair_geometry[p]) + " of " +
+                      std::to_string(set->geometries));
+      }
+    }
+    msim::learned_upload(c, *set);
+  });
+}
+
+int msim_hip_learned_train(msim_ctx* ctx, const msim_train_params* params, float* losses,
+                           double* seconds) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (params == nullptr) msim::invalid("parameters must not be null");
+    if (params->steps == 0 || params->batch == 0 || params->batch > 65535) {
+      msim::invalid("steps and batch (1..65535) must be positive");
+    }
+    if (!(params->learning_rate > 0.0f) || !(params->beta1 >= 0.0f && params->beta1 < 1.0f) ||
+        !(params->beta2 >= 0.0f && params->beta2 < 1.0f) || !(params->epsilon > 0.0f)) {
+      msim::invalid("Adam needs learning_rate > 0, 0 <= beta < 1 and epsilon > 0");
+    }
+    msim::learned_train(c, *params, losses, seconds);
+  });
+}
+
+int msim_hip_learned_gradients(msim_ctx* ctx, uint32_t pair, double* loss, double* gradient) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (loss == nullptr || gradient == nullptr) msim::invalid("outputs must not be null");
+    msim::learned_gradients(c, pair, loss, gradient);
+  });
+}
+
+int msim_hip_learned_apply(msim_ctx* ctx, uint32_t pair, float* z) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (z == nullptr) msim::invalid("output must not be null");
+    msim::learned_apply(c, pair, z);
   });
 }
 

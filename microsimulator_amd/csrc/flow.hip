@@ -840,6 +840,19 @@ uint32_t run_cg(Context& ctx, T* r, T* x, T* q, T* pa, T* pb, uint32_t budget, u
   }
 }
 
+uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t interval,
+                      bool* converged, double* relative, KernelTimes* times) {
+  auto& f = ctx.flow;
+  launch_init<double, float>(ctx, f.rhs.data(), 1.0, f.r.data(), f.x.data(), f.p0.data(),
+                             tolerance);
+  const uint32_t iterations = run_cg<double>(ctx, f.r.data(), f.x.data(), f.q.data(),
+                                             f.p0.data(), f.p1.data(), budget, interval,
+                                             converged, relative, times);
+  launch_residual(ctx, f.x.data(), f.res.data(), 1);
+  read_state(ctx);
+  return iterations;
+}
+
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
@@ -904,17 +917,13 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
   KernelTimes kernel_times;
   KernelTimes* times = params.profile != 0 ? &kernel_times : nullptr;
   if (params.solver == MSIM_SOLVER_JACOBI_FP64) {
-    launch_init<double, float>(ctx, f.rhs.data(), 1.0, f.r.data(), f.x.data(), f.p0.data(),
-                               params.relative_tolerance);
-    iterations = run_cg<double>(ctx, f.r.data(), f.x.data(), f.q.data(), f.p0.data(), f.p1.data(),
-                                params.max_iterations, interval, &converged, &relative, times);
+    iterations = jacobi_solve(ctx, params.relative_tolerance, params.max_iterations, interval,
+                              &converged, &relative, times);
     if (!converged) {
       throw Error(MSIM_NOT_CONVERGED,
                   "HIP depth-averaged flow conjugate gradient did not converge: relative " +
                       std::to_string(relative));
     }
-    launch_residual(ctx, f.x.data(), f.res.data(), 1);
-    read_state(ctx);
   } else if (params.solver == MSIM_SOLVER_MIXED) {
     for (auto* b : {&f.x32, &f.r32, &f.q32, &f.p032, &f.p132}) b->reserve(n);
     const double inner = params.inner_tolerance > 0.0 ? params.inner_tolerance : 1.0e-4;
@@ -964,8 +973,48 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
                       std::to_string(relative));
     }
   } else if (params.solver == MSIM_SOLVER_MULTIGRID) {
-    iterations = mg_solve(ctx, params, report, times);
+    iterations = mg_solve(ctx, params, report, times, false, &converged);
     relative = report.relative_residual;
+    if (!converged) {
+      throw Error(MSIM_NOT_CONVERGED,
+                  "HIP multilevel depth-averaged flow did not converge: relative " +
+                      std::to_string(relative));
+    }
+  } else if (params.solver == MSIM_SOLVER_LEARNED) {
+    if (!ctx.learned.loaded) {
+      throw Error(MSIM_INVALID_ARGUMENT, "the learned smoother has no weights: set them first");
+    }
+    // Learned flexible CG up to the cap; Jacobi-PCG (FP64) from x = 0 after.
+    const uint32_t cap = std::min(params.max_iterations, params.fallback_iterations == 0
+                                                             ? 200u
+                                                             : params.fallback_iterations);
+    msim_flow_params capped = params;
+    capped.max_iterations = cap;
+    bool fell = false;
+    try {
+      iterations = mg_solve(ctx, capped, report, times, true, &converged);
+      relative = report.relative_residual;
+      fell = !converged;
+    } catch (const Error& e) {
+      if (e.code() < MSIM_CG_CURVATURE || e.code() > MSIM_CG_DIRECTION) throw;
+      fell = true;
+      iterations = cap;
+    }
+    if (fell) {
+      report.fell_back = 1;
+      uint32_t extra = 0;
+      if (params.max_iterations > iterations) {
+        extra = jacobi_solve(ctx, params.relative_tolerance, params.max_iterations - iterations,
+                             interval, &converged, &relative, times);
+      }
+      iterations += extra;
+      if (!converged) {
+        throw Error(MSIM_NOT_CONVERGED,
+                    "HIP learned depth-averaged flow did not converge after the Jacobi-PCG "
+                    "fallback: relative " +
+                        std::to_string(relative));
+      }
+    }
   } else {
     throw Error(MSIM_UNSUPPORTED, "this pressure solver form is not available");
   }

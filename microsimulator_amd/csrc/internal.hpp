@@ -126,6 +126,8 @@ struct MgLevel {
 struct MgDevice {
   std::vector<std::unique_ptr<MgLevel>> levels;
   uint32_t tail = 0;          // first level the single-block kernel handles
+  bool split_finest = false;  // the finest level runs on the tile kernels even when small
+  bool learned_cycle = false; // the finest up pass is the learned one (its own tile grid)
   uint32_t coarse_n = 0;      // cells of the coarsest level
   DeviceBuffer<float> cinv;   // inverse of the coarsest operator (dense)
   DeviceBuffer<double> zr, zq, pq, rr;  // flexible-CG partial sums, two generations
@@ -188,6 +190,20 @@ struct AdrDevice {
   [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(params.nx) * params.ny; }
 };
 
+// The learned smoother: weights, the packed bfloat16 fragments the kernels
+// read, Adam moments, and the uploaded training set with its scratch.
+struct LearnedDevice {
+  bool loaded = false;
+  uint32_t adam_steps = 0;  // Adam steps since the weights were set
+  DeviceBuffer<float> w, m1, m2;
+  DeviceBuffer<uint16_t> packed;
+  uint32_t nx = 0, ny = 0, geometries = 0, pairs = 0;
+  DeviceBuffer<float> ops, rp, ep;
+  DeviceBuffer<uint32_t> geometry, order;
+  DeviceBuffer<double> energy, lparts, grad;
+  DeviceBuffer<float> z, d, gz, grads, losses;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -197,6 +213,7 @@ struct Context {
   std::string name;
   FlowDevice flow;
   AdrDevice adr;
+  LearnedDevice learned;
 };
 
 // Summed event time and launch count of the two Krylov kernels of a solve.
@@ -218,13 +235,41 @@ const char* failure_text(uint32_t failure);
 
 // mg.hip: the multilevel preconditioner and flexible CG.
 void mg_prepare(Context& ctx);
-// One application of the multilevel cycle to a host residual (FP32).
-void mg_precondition(Context& ctx, const float* r_host, float* z_host);
+// One application of the multilevel cycle (with the learned smoother when
+// `learned`) to a host residual (FP32).
+void mg_precondition(Context& ctx, const float* r_host, float* z_host, bool learned);
 // FP32 flexible CG preconditioned by the multilevel cycle inside FP64
 // iterative refinement; leaves the FP64 pressure in flow.x and the true
-// residual in the solve state. Returns the Krylov iterations taken.
+// residual in the solve state. Returns the Krylov iterations taken;
+// *converged tells whether the FP64 residual met the tolerance.
 uint32_t mg_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
-                  KernelTimes* times);
+                  KernelTimes* times, bool learned, bool* converged);
+// Training pairs from the prepared system (msim_hip_learned_harvest).
+uint32_t mg_harvest(Context& ctx, uint32_t max_pairs, bool learned, float* residuals,
+                    float* errors);
+void mg_operator(Context& ctx, float* wx, float* wy, float* bc, float* dinv);
+
+// learned.hip: the learned smoother and its training.
+void learned_set_weights(Context& ctx, const float* weights);
+void learned_get_weights(Context& ctx, float* weights);
+void learned_upload(Context& ctx, const msim_train_set& set);
+void learned_train(Context& ctx, const msim_train_params& params, float* losses, double* seconds);
+void learned_gradients(Context& ctx, uint32_t pair, double* loss, double* gradient);
+void learned_apply(Context& ctx, uint32_t pair, float* z);
+// The finest level's up pass with the learned post-smoother; with q, the
+// flexible-CG dots z.r and z.q into zr / zq.
+void launch_up_learned(Context& ctx, const MgLevel& L0, const float* r, const float* ec,
+                       uint32_t nyc, float* z, const CgState* st, const float* q, double* zr,
+                       double* zq);
+// Workgroups of the learned up pass on an nx x ny level (one dot partial each).
+uint32_t learned_blocks(uint32_t nx, uint32_t ny);
+// r' = r - A x, e' = e - x on the finest level.
+void learned_make_pairs(Context& ctx, const MgLevel& L0, const float* r, const double* e,
+                        const float* x, float* rp, float* ep);
+// FP64 Jacobi-PCG from x = 0 with at most `budget` iterations (the LEARNED
+// solver's fallback). Returns the iterations taken; *converged as above.
+uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t interval,
+                      bool* converged, double* relative, KernelTimes* times);
 void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report);
 void flow_apply(Context& ctx, const double* x, double* y);
 

@@ -162,56 +162,65 @@ struct UpDots {
 };
 
 // Prolongation and post-smooth: x = omega D^-1 r + c P e_c,
-// out = x + omega D^-1 (r - A x).
+// out = x + omega D^-1 (r - A x). With `pre`, out = x instead (the cycle
+// before post-smoothing). Each workgroup walks a contiguous run of tiles
+// (tile t = (t / tiles_y, t % tiles_y)), so neighbouring tiles share L2 and
+// the flexible-CG dots leave one partial per workgroup.
 __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* __restrict__ r,
                                                       const float* __restrict__ ec, uint32_t nyc,
                                                       float* __restrict__ out, const CgState* st,
-                                                      UpDots dots) {
+                                                      UpDots dots, int pre, uint32_t tiles_y,
+                                                      uint32_t tiles) {
   __shared__ float s_x[kHX * kHY];
   __shared__ double s_red[kMgThreads / kWave];
   if (cycle_stopped(st)) return;
-  const uint32_t fx0 = blockIdx.x * kTX, fy0 = blockIdx.y * kTY;
-  for (int idx = threadIdx.x; idx < kHX * kHY; idx += kMgThreads) {
-    const int64_t fx = static_cast<int64_t>(fx0) + idx / kHY - 1;
-    const int64_t fy = static_cast<int64_t>(fy0) + idx % kHY - 1;
-    float v = 0.0f;
-    if (fx >= 0 && fx < L.nx && fy >= 0 && fy < L.ny) {
-      const size_t i = static_cast<size_t>(fx) * L.ny + static_cast<size_t>(fy);
-      const size_t c = static_cast<size_t>(fx / 2) * nyc + static_cast<size_t>(fy / 2);
-      v = kOmega * L.dinv[i] * r[i] + kCorrection * ec[c];
-    }
-    s_x[idx] = v;
-  }
-  __syncthreads();
+  const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t t0 = blockIdx.x * per, t1 = min(tiles, t0 + per);
   const int lx = threadIdx.x / (kTY / 2), ly = threadIdx.x % (kTY / 2);
   double zr = 0.0, zq = 0.0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t fx0 = (t / tiles_y) * kTX, fy0 = (t % tiles_y) * kTY;
+    for (int idx = threadIdx.x; idx < kHX * kHY; idx += kMgThreads) {
+      const int64_t fx = static_cast<int64_t>(fx0) + idx / kHY - 1;
+      const int64_t fy = static_cast<int64_t>(fy0) + idx % kHY - 1;
+      float v = 0.0f;
+      if (fx >= 0 && fx < L.nx && fy >= 0 && fy < L.ny) {
+        const size_t i = static_cast<size_t>(fx) * L.ny + static_cast<size_t>(fy);
+        const size_t c = static_cast<size_t>(fx / 2) * nyc + static_cast<size_t>(fy / 2);
+        v = kOmega * L.dinv[i] * r[i] + kCorrection * ec[c];
+      }
+      s_x[idx] = v;
+    }
+    __syncthreads();
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < 2; ++a) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const uint32_t fx = fx0 + 2 * lx + a, fy = fy0 + 2 * ly + b;
-      if (fx < L.nx && fy < L.ny) {
-        const size_t i = static_cast<size_t>(fx) * L.ny + fy;
-        const int hx = 2 * lx + a + 1, hy = 2 * ly + b + 1;
-        const float ri = r[i];
-        const float z = s_x[hx * kHY + hy] + kOmega * L.dinv[i] * (ri - tile_ax(L, s_x, hx, hy, i, fx, fy));
-        out[i] = z;
-        if (dots.q != nullptr) {
-          zr += static_cast<double>(z) * static_cast<double>(ri);
-          zq += static_cast<double>(z) * static_cast<double>(dots.q[i]);
+      for (int b = 0; b < 2; ++b) {
+        const uint32_t fx = fx0 + 2 * lx + a, fy = fy0 + 2 * ly + b;
+        if (fx < L.nx && fy < L.ny) {
+          const size_t i = static_cast<size_t>(fx) * L.ny + fy;
+          const int hx = 2 * lx + a + 1, hy = 2 * ly + b + 1;
+          const float ri = r[i];
+          const float z = pre != 0 ? s_x[hx * kHY + hy]
+                                   : s_x[hx * kHY + hy] +
+                                         kOmega * L.dinv[i] * (ri - tile_ax(L, s_x, hx, hy, i, fx, fy));
+          out[i] = z;
+          if (dots.q != nullptr) {
+            zr += static_cast<double>(z) * static_cast<double>(ri);
+            zq += static_cast<double>(z) * static_cast<double>(dots.q[i]);
+          }
         }
       }
     }
+    __syncthreads();
   }
   if (dots.q != nullptr) {
-    const uint32_t blocks = gridDim.x * gridDim.y;
-    const uint32_t block = blockIdx.y * gridDim.x + blockIdx.x;
     const uint32_t k = st->it_update + 1;
     const double a = block_sum<kMgThreads>(zr, s_red);
     const double b = block_sum<kMgThreads>(zq, s_red);
     if (threadIdx.x == 0) {
-      dots.zr[(k & 1) * blocks + block] = a;
-      dots.zq[(k & 1) * blocks + block] = b;
+      dots.zr[(k & 1) * gridDim.x + blockIdx.x] = a;
+      dots.zq[(k & 1) * gridDim.x + blockIdx.x] = b;
     }
   }
 }
@@ -591,12 +600,35 @@ __global__ void k_mg_refine(double* __restrict__ xsol, const float* __restrict__
 // Host side
 // ----------------------------------------------------------------------------
 
+enum CycleMode { kCyclePlain, kCycleLearned, kCyclePreSmooth };
+
+// Residual snapshots taken by a flexible-CG solve (one per iteration).
+struct Snapshots {
+  float* dst;
+  uint32_t cap;
+  uint32_t count;
+};
+
+// First level of the single-workgroup kernel: the finest level stays on the
+// tile kernels when split_finest is set (the learned smoother lives there).
+uint32_t active_tail(const MgDevice& mg) {
+  if (mg.split_finest && mg.tail == 0 && mg.levels.size() > 1) return 1;
+  return mg.tail;
+}
+
 LevelView view(const MgLevel& L) {
   return LevelView{L.wx.data(), L.wy.data(), L.bc.data(), L.dinv.data(), L.nx, L.ny};
 }
 
 dim3 tile_grid(const MgLevel& L) {
   return dim3((L.nx + kTX - 1) / kTX, (L.ny + kTY - 1) / kTY);
+}
+
+// Workgroups of the persistent up pass: about four per CU, or one per tile
+// on small levels.
+uint32_t up_blocks(const MgLevel& L) {
+  const dim3 g = tile_grid(L);
+  return std::min<uint32_t>(g.x * g.y, 1024);
 }
 
 uint32_t fcg_rows(uint32_t nx, uint32_t ny, int ty, uint32_t* blocks) {
@@ -695,10 +727,11 @@ std::vector<float> coarse_inverse(const std::vector<float>& wx, const std::vecto
 
 TailArgs tail_args(Context& ctx, float* r0, float* z0, const float* q0, FcgPartials parts) {
   auto& mg = ctx.flow.mg;
+  const uint32_t tail = active_tail(mg);
   TailArgs T{};
-  T.count = static_cast<uint32_t>(mg.levels.size()) - mg.tail;
+  T.count = static_cast<uint32_t>(mg.levels.size()) - tail;
   for (uint32_t l = 0; l < T.count; ++l) {
-    MgLevel& L = *mg.levels[mg.tail + l];
+    MgLevel& L = *mg.levels[tail + l];
     TailLevel& t = T.lv[l];
     t.wx = L.wx.data();
     t.wy = L.wy.data();
@@ -709,7 +742,7 @@ TailArgs tail_args(Context& ctx, float* r0, float* z0, const float* q0, FcgParti
     t.nx = L.nx;
     t.ny = L.ny;
   }
-  if (mg.tail == 0) {
+  if (tail == 0) {
     T.lv[0].r = r0;
     T.lv[0].e = z0;
     if (q0 != nullptr) T.dots = UpDots{q0, parts.zr, parts.zq};
@@ -727,11 +760,12 @@ FcgPartials fcg_partials(Context& ctx) {
   p.rr = mg.rr.data();
   p.zr = mg.zr.data();
   p.zq = mg.zq.data();
-  if (mg.tail == 0) {
+  if (mg.learned_cycle && active_tail(mg) > 0) {
+    p.p0 = learned_blocks(mg.levels[0]->nx, mg.levels[0]->ny);
+  } else if (active_tail(mg) == 0) {
     p.p0 = 1;
   } else {
-    const dim3 g = tile_grid(*mg.levels[0]);
-    p.p0 = g.x * g.y;
+    p.p0 = up_blocks(*mg.levels[0]);
   }
   fcg_rows(f.nx, f.ny, fcg_ty(f.ny), &p.p1);
   p.p2 = krylov_blocks(f.sites());
@@ -740,13 +774,16 @@ FcgPartials fcg_partials(Context& ctx) {
 
 // z = M(r): the cycle over all levels. With probes, events are recorded after
 // the finest down pass (probes[0]) and before the finest up pass (probes[1]).
-void launch_cycle(Context& ctx, float* r, float* z, const float* q,
-                  const hipEvent_t* probes = nullptr) {
+// kCycleLearned post-smooths the finest level with the learned smoother;
+// kCyclePreSmooth stops the finest level before its post-smoother.
+void launch_cycle(Context& ctx, float* r, float* z, const float* q, const hipEvent_t* probes,
+                  CycleMode mode) {
   auto& mg = ctx.flow.mg;
+  const uint32_t tail = active_tail(mg);
   const CgState* st = ctx.flow.state.data();
   const FcgPartials parts = fcg_partials(ctx);
   // Down passes on the large levels; level 0 reads the flexible-CG residual.
-  for (uint32_t l = 0; l < mg.tail; ++l) {
+  for (uint32_t l = 0; l < tail; ++l) {
     MgLevel& L = *mg.levels[l];
     MgLevel& C = *mg.levels[l + 1];
     const float* rl = l == 0 ? r : L.r.data();
@@ -755,11 +792,11 @@ void launch_cycle(Context& ctx, float* r, float* z, const float* q,
     check(hipGetLastError(), "launch k_mg_down");
     if (l == 0 && probes != nullptr) check(hipEventRecord(probes[0], ctx.stream), "event");
   }
-  if (mg.tail == 0 && probes != nullptr) check(hipEventRecord(probes[0], ctx.stream), "event");
+  if (tail == 0 && probes != nullptr) check(hipEventRecord(probes[0], ctx.stream), "event");
   k_mg_tail<<<1, kTailThreads, 0, ctx.stream>>>(tail_args(ctx, r, z, q, parts), st);
   check(hipGetLastError(), "launch k_mg_tail");
-  if (mg.tail == 0 && probes != nullptr) check(hipEventRecord(probes[1], ctx.stream), "event");
-  for (int l = static_cast<int>(mg.tail) - 1; l >= 0; --l) {
+  if (tail == 0 && probes != nullptr) check(hipEventRecord(probes[1], ctx.stream), "event");
+  for (int l = static_cast<int>(tail) - 1; l >= 0; --l) {
     MgLevel& L = *mg.levels[l];
     MgLevel& C = *mg.levels[l + 1];
     const float* rl = l == 0 ? r : L.r.data();
@@ -767,8 +804,14 @@ void launch_cycle(Context& ctx, float* r, float* z, const float* q,
     const UpDots dots = (l == 0 && q != nullptr) ? UpDots{q, parts.zr, parts.zq}
                                                  : UpDots{nullptr, nullptr, nullptr};
     if (l == 0 && probes != nullptr) check(hipEventRecord(probes[1], ctx.stream), "event");
-    k_mg_up<<<tile_grid(L), kMgThreads, 0, ctx.stream>>>(view(L), rl, C.e.data(), C.ny, out, st,
-                                                         dots);
+    if (l == 0 && mode == kCycleLearned) {
+      launch_up_learned(ctx, L, rl, C.e.data(), C.ny, out, st, dots.q, dots.zr, dots.zq);
+      continue;
+    }
+    const int pre = (l == 0 && mode == kCyclePreSmooth) ? 1 : 0;
+    const dim3 g = tile_grid(L);
+    k_mg_up<<<up_blocks(L), kMgThreads, 0, ctx.stream>>>(view(L), rl, C.e.data(), C.ny, out, st,
+                                                         dots, pre, g.y, g.x * g.y);
     check(hipGetLastError(), "launch k_mg_up");
   }
 }
@@ -805,7 +848,7 @@ void launch_fcg_apply(Context& ctx, const float* z, const float* p_old, float* p
 // iterations. Returns the iterations taken; *converged says whether the
 // stopping test held.
 uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, uint32_t interval,
-                 bool* converged, KernelTimes* times) {
+                 bool* converged, KernelTimes* times, CycleMode mode, Snapshots* snaps) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
   float* r = f.r32.data();
@@ -820,6 +863,17 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
                                                            f.partials.data());
   check(hipGetLastError(), "launch k_fcg_init");
   *converged = false;
+  auto snapshot = [&]() {
+    if (snaps == nullptr || snaps->count >= snaps->cap) return;
+    check(hipMemcpyAsync(snaps->dst + static_cast<size_t>(snaps->count) * n, r, n * sizeof(float),
+                         hipMemcpyDeviceToDevice, ctx.stream),
+          "snapshot residual");
+    ++snaps->count;
+  };
+  if (snaps != nullptr) {
+    interval = 1;
+    snapshot();
+  }
   constexpr int kP = 6;  // events per iteration
   if (times != nullptr) {
     while (ctx.probes.size() < kP * static_cast<size_t>(interval) + kP) {
@@ -842,7 +896,7 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
       const uint32_t it = launched + j + 1;
       const hipEvent_t* pr = times != nullptr ? &ctx.probes[kP * j] : nullptr;
       if (pr != nullptr) check(hipEventRecord(pr[0], ctx.stream), "event");
-      launch_cycle(ctx, r, z, q, pr != nullptr ? pr + 1 : nullptr);
+      launch_cycle(ctx, r, z, q, pr != nullptr ? pr + 1 : nullptr, mode);
       if (pr != nullptr) check(hipEventRecord(pr[3], ctx.stream), "event");
       launch_fcg_apply(ctx, z, buf[(it + 1) % 2], buf[it % 2], q, x);
       if (pr != nullptr) check(hipEventRecord(pr[4], ctx.stream), "event");
@@ -887,6 +941,7 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
       *converged = true;
       return s.halt;
     }
+    if (batch > 0) snapshot();
     if (tested_all) return budget;
   }
 }
@@ -958,38 +1013,48 @@ void mg_prepare(Context& ctx) {
   check(hipMemcpyAsync(mg.cinv.data(), inv.data(), inv.size() * sizeof(float),
                        hipMemcpyHostToDevice, ctx.stream),
         "upload coarse inverse");
-  // Flexible-CG partial sums.
+  // Flexible-CG partial sums, sized for the finest level on the tile kernels.
   const FcgPartials parts = fcg_partials(ctx);
+  const size_t p0 = std::max<size_t>({static_cast<size_t>(up_blocks(*mg.levels[0])),
+                                      learned_blocks(mg.levels[0]->nx, mg.levels[0]->ny), 1});
   mg.pq.reserve(2 * static_cast<size_t>(parts.p1));
   mg.rr.reserve(2 * static_cast<size_t>(parts.p2));
-  mg.zr.reserve(2 * static_cast<size_t>(parts.p0));
-  mg.zq.reserve(2 * static_cast<size_t>(parts.p0));
+  mg.zr.reserve(2 * p0);
+  mg.zq.reserve(2 * p0);
   check(hipStreamSynchronize(ctx.stream), "multilevel set-up");
   mg.ready = true;
 }
 
-void mg_precondition(Context& ctx, const float* r_host, float* z_host) {
+void mg_precondition(Context& ctx, const float* r_host, float* z_host, bool learned) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
   if (!f.mg.ready) mg_prepare(ctx);
+  f.mg.split_finest = learned;
+  f.mg.learned_cycle = learned;
   f.r32.reserve(n);
+  f.p132.reserve(n);
   // A fresh state: the cycle kernels run unless a solve stopped or failed.
   check(hipMemsetAsync(f.state.data(), 0, sizeof(CgState), ctx.stream), "clear state");
   check(hipMemcpyAsync(f.r32.data(), r_host, n * sizeof(float), hipMemcpyHostToDevice,
                        ctx.stream),
         "upload residual");
-  float* z = f.mg.levels[0]->e.data();
-  launch_cycle(ctx, f.r32.data(), z, nullptr);
+  float* z = f.p132.data();
+  launch_cycle(ctx, f.r32.data(), z, nullptr, nullptr, learned ? kCycleLearned : kCyclePlain);
   check(hipMemcpyAsync(z_host, z, n * sizeof(float), hipMemcpyDeviceToHost, ctx.stream),
         "download correction");
   check(hipStreamSynchronize(ctx.stream), "precondition");
+  f.solved = false;
 }
 
-uint32_t mg_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
-                  KernelTimes* times) {
+namespace {
+
+uint32_t solve_refined(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
+                       KernelTimes* times, CycleMode mode, bool* converged, Snapshots* snaps) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
   if (!f.mg.ready) mg_prepare(ctx);
+  f.mg.split_finest = mode != kCyclePlain;
+  f.mg.learned_cycle = mode == kCycleLearned;
   for (auto* b : {&f.x32, &f.r32, &f.q32, &f.p032, &f.p132}) b->reserve(n);
   const double inner = params.inner_tolerance > 0.0 ? params.inner_tolerance : 1.0e-5;
   const uint32_t interval = params.check_interval == 0 ? 8 : params.check_interval;
@@ -999,16 +1064,16 @@ uint32_t mg_solve(Context& ctx, const msim_flow_params& params, msim_flow_report
   const double bnorm = std::sqrt(f.host_state->bb);
   double rnorm = std::sqrt(f.host_state->true_rr);
   uint32_t iterations = 0, refinements = 0;
-  bool converged = bnorm == 0.0;
-  while (!converged) {
+  bool done = bnorm == 0.0;
+  while (!done) {
     if (rnorm <= params.relative_tolerance * bnorm) {
-      converged = true;
+      done = true;
       break;
     }
     if (iterations >= params.max_iterations || refinements >= 64) break;
     bool inner_converged = false;
     const uint32_t taken = run_fcg(ctx, rnorm, inner, params.max_iterations - iterations,
-                                   interval, &inner_converged, times);
+                                   interval, &inner_converged, times, mode, snaps);
     iterations += taken;
     ++refinements;
     k_mg_refine<<<stream_blocks(n), kStream, 0, ctx.stream>>>(f.x.data(), f.x32.data(), rnorm,
@@ -1025,12 +1090,109 @@ uint32_t mg_solve(Context& ctx, const msim_flow_params& params, msim_flow_report
   }
   report.refinements = refinements;
   report.relative_residual = bnorm > 0.0 ? rnorm / bnorm : 0.0;
-  if (!converged) {
-    throw Error(MSIM_NOT_CONVERGED,
-                "HIP multilevel depth-averaged flow did not converge: relative " +
-                    std::to_string(report.relative_residual));
-  }
+  *converged = done;
   return iterations;
+}
+
+}  // namespace
+
+uint32_t mg_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
+                  KernelTimes* times, bool learned, bool* converged) {
+  return solve_refined(ctx, params, report, times, learned ? kCycleLearned : kCyclePlain,
+                       converged, nullptr);
+}
+
+uint32_t mg_harvest(Context& ctx, uint32_t max_pairs, bool learned, float* residuals,
+                    float* errors) {
+  auto& f = ctx.flow;
+  auto& mg = f.mg;
+  const size_t n = f.sites();
+  if (!mg.ready) mg_prepare(ctx);
+  if (mg.levels.size() < 2) {
+    throw Error(MSIM_INVALID_ARGUMENT,
+                "the grid is too small to harvest from: its hierarchy has a single level");
+  }
+  // Up to kSpread residuals per pair kept, so pairs cover the whole solve.
+  constexpr uint32_t kSpread = 4;
+  DeviceBuffer<float> snaps, saved, rp, ep;
+  snaps.reserve(static_cast<size_t>(kSpread) * max_pairs * n);
+  saved.reserve(n);
+  rp.reserve(n);
+  ep.reserve(n);
+  check(hipMemcpyAsync(saved.data(), f.rhs.data(), n * sizeof(float), hipMemcpyDeviceToDevice,
+                       ctx.stream),
+        "save right-hand side");
+  uint32_t count = 0;
+  try {
+    // Residuals of the solver's own iterations.
+    msim_flow_params p{};
+    p.relative_tolerance = 1.0e-8;
+    p.max_iterations = 1000;
+    msim_flow_report rep{};
+    Snapshots sn{snaps.data(), kSpread * max_pairs, 0};
+    bool ok = false;
+    solve_refined(ctx, p, rep, nullptr, learned ? kCycleLearned : kCyclePlain, &ok, &sn);
+    // Residuals spread evenly over the whole solve.
+    count = std::min(sn.count, max_pairs);
+    std::vector<uint32_t> pick(count);
+    for (uint32_t k = 0; k < count; ++k) {
+      pick[k] = count > 1 ? static_cast<uint32_t>(
+                                (static_cast<uint64_t>(k) * (sn.count - 1) + (count - 1) / 2) /
+                                (count - 1))
+                          : 0;
+    }
+    // For each: e = A^-1 r, then the pair after the pre-smoother and the
+    // coarse correction.
+    p.relative_tolerance = 1.0e-10;
+    p.max_iterations = 4000;
+    for (uint32_t k = 0; k < count; ++k) {
+      float* rk = snaps.data() + static_cast<size_t>(pick[k]) * n;
+      check(hipMemcpyAsync(f.rhs.data(), rk, n * sizeof(float), hipMemcpyDeviceToDevice,
+                           ctx.stream),
+            "residual as right-hand side");
+      solve_refined(ctx, p, rep, nullptr, kCyclePlain, &ok, nullptr);
+      if (!ok) throw Error(MSIM_NOT_CONVERGED, "harvest: the error solve did not converge");
+      mg.split_finest = true;
+      mg.learned_cycle = false;
+      check(hipMemsetAsync(f.state.data(), 0, sizeof(CgState), ctx.stream), "clear state");
+      launch_cycle(ctx, rk, f.p132.data(), nullptr, nullptr, kCyclePreSmooth);
+      learned_make_pairs(ctx, *mg.levels[0], rk, f.x.data(), f.p132.data(), rp.data(), ep.data());
+      check(hipMemcpyAsync(residuals + static_cast<size_t>(k) * n, rp.data(), n * sizeof(float),
+                           hipMemcpyDeviceToHost, ctx.stream),
+            "download pair");
+      check(hipMemcpyAsync(errors + static_cast<size_t>(k) * n, ep.data(), n * sizeof(float),
+                           hipMemcpyDeviceToHost, ctx.stream),
+            "download pair");
+    }
+  } catch (...) {
+    static_cast<void>(hipMemcpyAsync(f.rhs.data(), saved.data(), n * sizeof(float),
+                                     hipMemcpyDeviceToDevice, ctx.stream));
+    static_cast<void>(hipStreamSynchronize(ctx.stream));
+    f.solved = false;
+    throw;
+  }
+  check(hipMemcpyAsync(f.rhs.data(), saved.data(), n * sizeof(float), hipMemcpyDeviceToDevice,
+                       ctx.stream),
+        "restore right-hand side");
+  check(hipStreamSynchronize(ctx.stream), "harvest");
+  f.solved = false;
+  return count;
+}
+
+void mg_operator(Context& ctx, float* wx, float* wy, float* bc, float* dinv) {
+  auto& f = ctx.flow;
+  if (!f.mg.ready) mg_prepare(ctx);
+  const MgLevel& L = *f.mg.levels[0];
+  const size_t bytes = L.cells() * sizeof(float);
+  const std::pair<float*, const float*> copies[4] = {
+      {wx, L.wx.data()}, {wy, L.wy.data()}, {bc, L.bc.data()}, {dinv, L.dinv.data()}};
+  for (const auto& c : copies) {
+    if (c.first != nullptr) {
+      check(hipMemcpyAsync(c.first, c.second, bytes, hipMemcpyDeviceToHost, ctx.stream),
+            "download operator");
+    }
+  }
+  check(hipStreamSynchronize(ctx.stream), "operator");
 }
 
 }  // namespace msim
