@@ -26,7 +26,8 @@
 // the pixels in N and (tap, input channel) in K. Lane l supplies
 // A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] and receives
 // D[4 (l / 16) + j][l % 16], so it reads and writes the four channels of one
-// pixel as one 8-byte LDS access. The last layer puts the 9 taps in M,
+// pixel as one 8-byte LDS access (activations kept as four planes of four
+// channels, so a lane group's accesses are adjacent). The last layer puts the 9 taps in M,
 // T[tap][p] = W3[tap] . a2[p], and gathers N[p] = sum_tap T[tap][p + tap].
 //
 // Training runs on the same tiles: forward, a loss kernel giving dL/dz, a
@@ -82,6 +83,11 @@ static_assert(kL2Begin - (kPitch + 1) >= kA1Base && kL2End + kPitch + 1 <= kA1Ba
 static_assert(kL1Begin - (kPitch + 1) >= 0 && kL1End + kPitch + 1 <= kPix,
               "layer 1 reads inside the region");
 static_assert((kL1End - kL1Begin) % 16 == 0 && kL2Pix % 16 == 0, "whole pixel runs");
+// Activations are stored as four planes of [pixel][4 channels] (bf16), one per
+// group of four channels, so the 16 lanes of a lane group touch 16 adjacent
+// 8-byte words: bank-conflict free.
+constexpr int kA1Plane = 4 * kA1Pix;   // shorts per a1 plane
+constexpr int kA2Plane = 4 * kL2Pix;   // shorts per a2 plane
 
 __device__ constexpr int tap_offset(int t) { return (t / 3 - 1) * kPitch + (t % 3 - 1); }
 
@@ -89,6 +95,10 @@ __device__ inline unsigned short bf16_bits(float f) {
   uint32_t u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);
   return static_cast<unsigned short>(u >> 16);
+}
+// The same rounding in one instruction for two values (v_cvt_pk_bf16_f32).
+__device__ inline short to_bf16(float f) {
+  return __builtin_bit_cast(short, static_cast<__bf16>(f));
 }
 __device__ inline float bf16_value(unsigned short b) {
   return __uint_as_float(static_cast<uint32_t>(b) << 16);
@@ -102,7 +112,7 @@ __device__ inline bf16x4 activate(f32x4 acc, bool on) {
   const float keep = on ? 1.0f : 0.0f;
   bf16x4 o;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) o[j] = static_cast<short>(bf16_bits(softsign(acc[j]) * keep));
+  for (int j = 0; j < 4; ++j) o[j] = to_bf16(softsign(acc[j]) * keep);
   return o;
 }
 
@@ -170,14 +180,15 @@ __device__ inline bool tile_pixel(int p) {
 // in = bf16(v, v cx, v cy, v cb) with v = u / s for every region pixel.
 template <int NT>
 __device__ inline void write_inputs(const float4* feat, unsigned short* in, float s) {
+  const float inv = s > 0.0f ? 1.0f / s : 0.0f;
   for (int p = threadIdx.x; p < kPix; p += NT) {
     const float4 f = feat[p];
-    const float v = s > 0.0f ? f.x / s : 0.0f;
+    const float v = f.x * inv;
     bf16x4 w;
-    w[0] = static_cast<short>(bf16_bits(v));
-    w[1] = static_cast<short>(bf16_bits(v * f.y));
-    w[2] = static_cast<short>(bf16_bits(v * f.z));
-    w[3] = static_cast<short>(bf16_bits(v * f.w));
+    w[0] = to_bf16(v);
+    w[1] = to_bf16(v * f.y);
+    w[2] = to_bf16(v * f.z);
+    w[3] = to_bf16(v * f.w);
     store4(in + 4 * p, w);
   }
 }
@@ -192,10 +203,11 @@ __device__ inline void layers12(const unsigned short* in, unsigned short* a1, un
   constexpr int kWaves = NT / kWave;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const int m = lane & 15, g = lane >> 4;
-  // Zero guards of a1 (read only for pixels nothing keeps).
-  for (int k = threadIdx.x; k < 16 * 16 * 2; k += NT) {
-    const int px = k / 16 < 16 ? k / 16 : (kA1Pix - 32) + k / 16;
-    a1[px * 16 + (k & 15)] = 0;
+  // Zero guards of a1 (read only for pixels nothing keeps): 16 pixels at
+  // each end of each plane.
+  for (int k = threadIdx.x; k < 4 * 32 * 4; k += NT) {
+    const int plane = k / 128, e = k % 128, px = e / 4 < 16 ? e / 4 : (kA1Pix - 32) + e / 4;
+    a1[plane * kA1Plane + 4 * px + (e & 3)] = 0;
   }
   for (int mt = wave; mt < kL1Tiles; mt += 2 * kWaves) {
     const bool two = mt + kWaves < kL1Tiles;
@@ -218,8 +230,8 @@ __device__ inline void layers12(const unsigned short* in, unsigned short* a1, un
       c0 = mfma(frag[s], b0[s], c0);
       c1 = mfma(frag[s], b1[s], c1);
     }
-    store4(a1 + 16 * (p0 - kA1Base) + 4 * g, activate(c0, mask[p0] != 0));
-    if (two) store4(a1 + 16 * (p1 - kA1Base) + 4 * g, activate(c1, mask[p1] != 0));
+    store4(a1 + g * kA1Plane + 4 * (p0 - kA1Base), activate(c0, mask[p0] != 0));
+    if (two) store4(a1 + g * kA1Plane + 4 * (p1 - kA1Base), activate(c1, mask[p1] != 0));
   }
   __syncthreads();
   for (int mt = wave; mt < kL2Tiles; mt += 2 * kWaves) {
@@ -229,8 +241,8 @@ __device__ inline void layers12(const unsigned short* in, unsigned short* a1, un
     bf16x4 b0[9], b1[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-      b0[t] = lds4(a1 + 16 * (p0 + tap_offset(t) - kA1Base) + 4 * g);
-      b1[t] = lds4(a1 + 16 * (p1 + tap_offset(t) - kA1Base) + 4 * g);
+      b0[t] = lds4(a1 + g * kA1Plane + 4 * (p0 + tap_offset(t) - kA1Base));
+      b1[t] = lds4(a1 + g * kA1Plane + 4 * (p1 + tap_offset(t) - kA1Base));
     }
     f32x4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -238,8 +250,8 @@ __device__ inline void layers12(const unsigned short* in, unsigned short* a1, un
       c0 = mfma(frag[3 + t], b0[t], c0);
       c1 = mfma(frag[3 + t], b1[t], c1);
     }
-    store4(a2 + 16 * (p0 - kL2Begin) + 4 * g, activate(c0, mask[p0] != 0));
-    if (two) store4(a2 + 16 * (p1 - kL2Begin) + 4 * g, activate(c1, mask[p1] != 0));
+    store4(a2 + g * kA2Plane + 4 * (p0 - kL2Begin), activate(c0, mask[p0] != 0));
+    if (two) store4(a2 + g * kA2Plane + 4 * (p1 - kL2Begin), activate(c1, mask[p1] != 0));
   }
   __syncthreads();
 }
@@ -252,7 +264,7 @@ __device__ inline void layer3(const unsigned short* a2, float* T, const bf16x4* 
   const int m = lane & 15, g = lane >> 4;
   for (int mt = wave; mt < kL2Tiles; mt += kWaves) {
     const int q = 16 * mt + m;
-    const bf16x4 b = lds4(a2 + 16 * q + 4 * g);
+    const bf16x4 b = lds4(a2 + g * kA2Plane + 4 * q);
     const f32x4 acc = mfma(frag[12], b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -300,7 +312,7 @@ struct UpOut {
 // The finest level's up pass with the learned post-smoother:
 // x = omega D^-1 r + c P e_c, r' = r - A x, z = x + omega D^-1 r' + s N(r');
 // with dots z.r and z.q for flexible CG.
-__global__ __launch_bounds__(kThreads) void k_learn_up(Geometry G, const float* __restrict__ r,
+__global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const float* __restrict__ r,
                                                        const float* __restrict__ ec, uint32_t nyc,
                                                        const CgState* st,
                                                        const bf16x4* __restrict__ packed,
@@ -324,42 +336,50 @@ __global__ __launch_bounds__(kThreads) void k_learn_up(Geometry G, const float* 
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
   const uint32_t t0 = blockIdx.x * per, t1 = min(tiles, t0 + per);
   double zr = 0.0, zq = 0.0;
-  for (uint32_t t = t0; t < t1; ++t) {
+  // Every global value the thread's pixels of tile t need, in one round of
+  // loads: r, dinv, e_c, wx (lower, upper), wy (lower, upper), bc. The next
+  // tile's round is issued before this tile's network runs.
+  constexpr int kPer = (kPix + kThreads - 1) / kThreads;
+  auto load_tile = [&](uint32_t t, float (&v)[8][kPer], uint32_t& inside) {
     const int fx0 = static_cast<int>(t / tiles_y) * kTX;
     const int fy0 = static_cast<int>(t % tiles_y) * kTY;
-    // Every global value the thread's pixels need, in one round of loads.
-    constexpr int kPer = (kPix + kThreads - 1) / kThreads;
-    float v_r[kPer], v_d[kPer], v_ec[kPer], v_wxl[kPer], v_wxu[kPer], v_wyl[kPer], v_wyu[kPer],
-        v_b[kPer];
-    uint32_t inside = 0;
+    inside = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int p = static_cast<int>(threadIdx.x) + k * kThreads;
-      v_r[k] = v_d[k] = v_ec[k] = v_wxl[k] = v_wxu[k] = v_wyl[k] = v_wyu[k] = v_b[k] = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) v[a][k] = 0.0f;
       uint32_t gx, gy;
       if (p < kPix && region_site(p, fx0, fy0, G, &gx, &gy)) {
         inside |= 1u << k;
         const size_t i = static_cast<size_t>(gx) * G.ny + gy;
-        v_r[k] = r[i];
-        v_d[k] = G.dinv[i];
-        v_ec[k] = ec[static_cast<size_t>(gx / 2) * nyc + gy / 2];
+        v[0][k] = r[i];
+        v[1][k] = G.dinv[i];
+        v[2][k] = ec[static_cast<size_t>(gx / 2) * nyc + gy / 2];
         if (feature_pixel(p)) {
-          v_wxl[k] = G.wx[i];
-          v_wxu[k] = gx + 1 < G.nx ? G.wx[i + G.ny] : 0.0f;
-          v_wyl[k] = G.wy[i];
-          v_wyu[k] = gy + 1 < G.ny ? G.wy[i + 1] : 0.0f;
-          v_b[k] = G.bc[i];
+          v[3][k] = G.wx[i];
+          v[4][k] = gx + 1 < G.nx ? G.wx[i + G.ny] : 0.0f;
+          v[5][k] = G.wy[i];
+          v[6][k] = gy + 1 < G.ny ? G.wy[i + 1] : 0.0f;
+          v[7][k] = G.bc[i];
         }
       }
     }
+  };
+  float cur[8][kPer], nxt[8][kPer];
+  uint32_t cur_in = 0, nxt_in = 0;
+  if (t0 < t1) load_tile(t0, cur, cur_in);
+  for (uint32_t t = t0; t < t1; ++t) {
+    const int fx0 = static_cast<int>(t / tiles_y) * kTX;
+    const int fy0 = static_cast<int>(t % tiles_y) * kTY;
     // x = omega D^-1 r + c P e_c over the region.
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int p = static_cast<int>(threadIdx.x) + k * kThreads;
       if (p < kPix) {
-        const bool on = (inside >> k) & 1u;
-        s_x[p] = on ? kOmega * v_d[k] * v_r[k] + kCorrection * v_ec[k] : 0.0f;
-        s_mask[p] = on && v_d[k] > 0.0f ? 1 : 0;
+        const bool on = (cur_in >> k) & 1u;
+        s_x[p] = on ? kOmega * cur[1][k] * cur[0][k] + kCorrection * cur[2][k] : 0.0f;
+        s_mask[p] = on && cur[1][k] > 0.0f ? 1 : 0;
       }
     }
     __syncthreads();
@@ -371,22 +391,24 @@ __global__ __launch_bounds__(kThreads) void k_learn_up(Geometry G, const float* 
       const int p = static_cast<int>(threadIdx.x) + k * kThreads;
       if (p >= kPix) continue;
       float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (((inside >> k) & 1u) && feature_pixel(p)) {
+      if (((cur_in >> k) & 1u) && feature_pixel(p)) {
         const float xc = s_x[p];
-        float ax = v_b[k] * xc;
-        ax += v_wxl[k] * (xc - s_x[p - kPitch]);
-        ax += v_wxu[k] * (xc - s_x[p + kPitch]);
-        ax += v_wyl[k] * (xc - s_x[p - 1]);
-        ax += v_wyu[k] * (xc - s_x[p + 1]);
-        const float d = v_d[k];
-        const float res = v_r[k] - ax;
-        f = make_float4(d * res, (v_wxl[k] + v_wxu[k]) * d, (v_wyl[k] + v_wyu[k]) * d,
-                        v_b[k] * d);
+        const float wxl = cur[3][k], wxu = cur[4][k], wyl = cur[5][k], wyu = cur[6][k];
+        const float b = cur[7][k];
+        float ax = b * xc;
+        ax += wxl * (xc - s_x[p - kPitch]);
+        ax += wxu * (xc - s_x[p + kPitch]);
+        ax += wyl * (xc - s_x[p - 1]);
+        ax += wyu * (xc - s_x[p + 1]);
+        const float d = cur[1][k];
+        const float res = cur[0][k] - ax;
+        f = make_float4(d * res, (wxl + wxu) * d, (wyl + wyu) * d, b * d);
         umax = fmaxf(umax, fabsf(f.x));
         if (tile_pixel(p)) s_zc[(p / kPitch - 4) * kTY + p % kPitch - 4] = xc + kOmega * d * res;
       }
       s_feat[p] = f;
     }
+    if (t + 1 < t1) load_tile(t + 1, nxt, nxt_in);
     const float s = block_max_all<kThreads>(umax, s_red);  // has barriers
     write_inputs<kThreads>(s_feat, s_in, s);
     __syncthreads();
@@ -407,6 +429,12 @@ __global__ __launch_bounds__(kThreads) void k_learn_up(Geometry G, const float* 
       }
     }
     __syncthreads();  // the next tile rewrites every shared array
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) cur[a][k] = nxt[a][k];
+    }
+    cur_in = nxt_in;
   }
   if (out.q != nullptr) {
     const uint32_t k = st->it_update + 1;
@@ -628,8 +656,8 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int q = q0 + 4 * g + j;
-        a[j] = static_cast<short>(s_a2[16 * (q - kL2Begin) + m]);
-        b[j] = m < 9 ? static_cast<short>(bf16_bits(s_g[q - tap_offset(m)])) : 0;
+        a[j] = static_cast<short>(s_a2[(m >> 2) * kA2Plane + 4 * (q - kL2Begin) + (m & 3)]);
+        b[j] = m < 9 ? to_bf16(s_g[q - tap_offset(m)]) : 0;
       }
       acc = mfma(a, b, acc);
     }
@@ -642,7 +670,7 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int t = 4 * g + j;
-      wg3[j] = t < 9 ? static_cast<short>(bf16_bits(w[kW1 + kW2 + t * kC1 + m])) : 0;
+      wg3[j] = t < 9 ? to_bf16(w[kW1 + kW2 + t * kC1 + m]) : 0;
     }
     for (int mt = wave; mt < kL2Tiles; mt += kWaves - 1) {
       const int q = kL2Begin + 16 * mt + m;
@@ -650,18 +678,18 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int t = 4 * g + j;
-        b[j] = t < 9 ? static_cast<short>(bf16_bits(s_g[q - tap_offset(t)])) : 0;
+        b[j] = t < 9 ? to_bf16(s_g[q - tap_offset(t)]) : 0;
       }
       const f32x4 acc = mfma(wg3, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
-      const bf16x4 av = lds4(s_a2 + 16 * (q - kL2Begin) + 4 * g);
+      const bf16x4 av = lds4(s_a2 + g * kA2Plane + 4 * (q - kL2Begin));
       const bool on = s_mask[q] != 0;
       bf16x4 o;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const float d = 1.0f - fabsf(bf16_value(static_cast<unsigned short>(av[j])));
-        o[j] = on ? static_cast<short>(bf16_bits(acc[j] * d * d)) : 0;
+        o[j] = on ? to_bf16(acc[j] * d * d) : 0;
       }
-      store4(s_gh2 + 16 * (q - kL2Begin) + 4 * g, o);
+      store4(s_gh2 + g * kA2Plane + 4 * (q - kL2Begin), o);
     }
   }
   __syncthreads();
@@ -675,8 +703,8 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int p = p0 + 4 * g + j;
-        a[j] = static_cast<short>(s_a1[16 * (p + off - kA1Base) + m]);
-        b[j] = static_cast<short>(s_gh2[16 * (p - kL2Begin) + m]);
+        a[j] = static_cast<short>(s_a1[(m >> 2) * kA1Plane + 4 * (p + off - kA1Base) + (m & 3)]);
+        b[j] = static_cast<short>(s_gh2[(m >> 2) * kA2Plane + 4 * (p - kL2Begin) + (m & 3)]);
       }
       acc = mfma(a, b, acc);
     }
@@ -691,7 +719,7 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
     for (int t = 0; t < 9; ++t) {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        wb2[t][j] = static_cast<short>(bf16_bits(w[kW1 + t * kC1 * kC1 + m * kC1 + 4 * g + j]));
+        wb2[t][j] = to_bf16(w[kW1 + t * kC1 * kC1 + m * kC1 + 4 * g + j]);
     }
     for (int mt = wave; mt < kL1Tiles; mt += kWaves) {
       const int q = kL1Begin + 16 * mt + m;
@@ -700,17 +728,17 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
       for (int t = 0; t < 9; ++t) {
         const int src = q - tap_offset(t);
         bf16x4 b = {0, 0, 0, 0};
-        if (src >= kL2Begin && src < kL2End) b = lds4(s_gh2 + 16 * (src - kL2Begin) + 4 * g);
+        if (src >= kL2Begin && src < kL2End) b = lds4(s_gh2 + g * kA2Plane + 4 * (src - kL2Begin));
         acc = mfma(wb2[t], b, acc);
       }
-      unsigned short* a1p = s_a1 + 16 * (q - kA1Base) + 4 * g;
+      unsigned short* a1p = s_a1 + g * kA1Plane + 4 * (q - kA1Base);
       const bf16x4 av = lds4(a1p);
       const bool on = s_mask[q] != 0;
       bf16x4 o;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const float d = 1.0f - fabsf(bf16_value(static_cast<unsigned short>(av[j])));
-        o[j] = on ? static_cast<short>(bf16_bits(acc[j] * d * d)) : 0;
+        o[j] = on ? to_bf16(acc[j] * d * d) : 0;
       }
       store4(a1p, o);
     }
@@ -728,7 +756,7 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int p = p0 + 4 * g + j;
-        a[j] = static_cast<short>(s_a1[16 * (p - kA1Base) + m]);
+        a[j] = static_cast<short>(s_a1[(m >> 2) * kA1Plane + 4 * (p - kA1Base) + (m & 3)]);
         b[j] = t < 9 ? static_cast<short>(s_in[4 * (p + off) + c]) : 0;
       }
       acc = mfma(a, b, acc);

@@ -624,11 +624,11 @@ dim3 tile_grid(const MgLevel& L) {
   return dim3((L.nx + kTX - 1) / kTX, (L.ny + kTY - 1) / kTY);
 }
 
-// Workgroups of the persistent up pass: about four per CU, or one per tile
+// Workgroups of the persistent up pass: about eight per CU, or one per tile
 // on small levels.
 uint32_t up_blocks(const MgLevel& L) {
   const dim3 g = tile_grid(L);
-  return std::min<uint32_t>(g.x * g.y, 1024);
+  return std::min<uint32_t>(g.x * g.y, 2048);
 }
 
 uint32_t fcg_rows(uint32_t nx, uint32_t ny, int ty, uint32_t* blocks) {
