@@ -306,6 +306,85 @@ int msim_hip_learned_gradients(msim_ctx* ctx, uint32_t pair, double* loss, doubl
 /* z = 0.8 D^-1 r' + s N(r') for one uploaded pair (N floats). */
 int msim_hip_learned_apply(msim_ctx* ctx, uint32_t pair, float* z);
 
+/* ---------------------------------------------------------------------------
+ * Batched sweep (no counterpart in the reference, which runs one job per
+ * invocation, docs/architecture/0005-batch-execution.md:30): many independent
+ * one-layer devices of uneven sizes advanced together. Each instance has its
+ * own geometry, mobility, occupancy, inlet pressure (Dirichlet at x = 0, 0 at
+ * x = nx, flow along x, no scaling) and solute diffusivity; the solute has
+ * fixed reservoirs beyond x = 0 (`reservoir`) and x = nx (0), no-flux walls
+ * in y and Monod uptake vmax phi c / (K + c).
+ *
+ * The pressure (FP32 Jacobi-PCG corrections with FP64 dot products inside
+ * FP64 iterative refinement of an FP64 pressure, warm-started, to an FP64
+ * true relative residual) is solved by msim_hip_batch_solve_flow: one launch,
+ * one workgroup per instance. A step is one launch over work items (blocks of
+ * rows, spread over the chip): each checks its rows' FP64 pressure residual,
+ * rebuilds their face velocities and advances them one FP32 forward-Euler
+ * transport step whose sites compute exactly as msim_hip_advance_signal_grid's
+ * do. Runs of steps can be captured in a HIP graph. The launch shape
+ * (workgroup size, sweep row width, sites per work item) is chosen by
+ * msim_hip_batch_autotune, which times each candidate on the uploaded batch.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  uint32_t nx, ny;
+  float dx, dy, dz;
+  const uint8_t* obstacles; /* nx*ny, or NULL */
+  const float* mobility;    /* nx*ny, required */
+  const float* phi;         /* nx*ny occupancy for Monod uptake, or NULL */
+  const float* levels;      /* nx*ny initial solute levels, or NULL (zero) */
+  float inlet_pressure;
+  float diffusion;
+  float reservoir;
+} msim_batch_instance;
+
+typedef struct {
+  float relative_tolerance; /* per-instance FP64 ||b - Ax|| / ||b|| */
+  uint32_t max_iterations;  /* FP32 CG iterations per instance per flow phase */
+  float monod_vmax, monod_k;
+} msim_batch_params;
+
+/* Uploads the instances and assembles their operators (replacing any batch). */
+int msim_hip_batch_upload(msim_ctx* ctx, uint32_t count, const msim_batch_instance* instances,
+                          const msim_batch_params* params);
+
+/* Solves every instance's pressure from its current one and rebuilds its
+ * velocities (one launch). Per-instance outputs (count each, any may be
+ * NULL): FP32 CG iterations, final FP64 true relative residual, flags (1 CG
+ * breakdown, 2 not converged within max_iterations, 4 a negative or
+ * non-finite level). */
+int msim_hip_batch_solve_flow(msim_ctx* ctx, uint32_t* iterations, float* residuals,
+                              uint32_t* flags);
+
+/* Times `steps` steps of each launch shape from the current levels (which it
+ * restores), keeps the fastest and returns its index in *shape. seconds:
+ * per-step time of each candidate (msim_hip_batch_shapes of them), or NULL.
+ * Needs a solved flow. */
+int msim_hip_batch_autotune(msim_ctx* ctx, float dt, uint32_t steps, int* shape,
+                            double* seconds);
+
+/* Number of launch shapes the autotuner tries, and shape k's workgroup size,
+ * sweep row width and sites per work item. */
+int msim_hip_batch_shapes(void);
+int msim_hip_batch_shape(int index, uint32_t* threads, uint32_t* row_width,
+                         uint32_t* item_sites);
+
+/* Per-instance flags as msim_hip_batch_solve_flow describes (count values). */
+int msim_hip_batch_flags(msim_ctx* ctx, uint32_t* flags);
+
+/* `steps` steps of dt from a solved flow. use_graph != 0: captured once in a
+ * HIP graph and replayed. ledger: steps*count*4 doubles per step and instance
+ * (mass before, mass after, uptake over the step -- FP64 sums over fluid
+ * sites times the voxel volume -- and the FP64 true relative residual of the
+ * pressure the step checked), or NULL. seconds: device time, or NULL. */
+int msim_hip_batch_run(msim_ctx* ctx, float dt, uint32_t steps, uint32_t use_graph,
+                       double* ledger, double* seconds);
+
+/* One instance's pressure (nx*ny doubles), x faces ((nx+1)*ny), y faces
+ * (nx*(ny+1)) and levels (nx*ny); any pointer may be NULL. */
+int msim_hip_batch_download(msim_ctx* ctx, uint32_t index, double* pressure, float* x_faces,
+                            float* y_faces, float* levels);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,6 +124,18 @@ class _TrainParams(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+class _BatchInstance(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("dx", C.c_float), ("dy", C.c_float),
+                ("dz", C.c_float), ("obstacles", C.c_void_p), ("mobility", C.c_void_p),
+                ("phi", C.c_void_p), ("levels", C.c_void_p), ("inlet_pressure", C.c_float),
+                ("diffusion", C.c_float), ("reservoir", C.c_float)]
+
+
+class _BatchParams(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_float), ("max_iterations", C.c_uint32),
+                ("monod_vmax", C.c_float), ("monod_k", C.c_float)]
+
+
 class _SignalReport(C.Structure):
     _fields_ = [("converged", C.c_uint32), ("iterations", C.c_uint32),
                 ("residual_rms", C.c_float)]
@@ -139,7 +151,9 @@ EXPORTS = (
     "msim_hip_learned_set_weights", "msim_hip_learned_get_weights",
     "msim_hip_learned_precondition", "msim_hip_learned_operator", "msim_hip_learned_harvest",
     "msim_hip_learned_upload", "msim_hip_learned_train", "msim_hip_learned_gradients",
-    "msim_hip_learned_apply",
+    "msim_hip_learned_apply", "msim_hip_batch_upload", "msim_hip_batch_solve_flow",
+    "msim_hip_batch_autotune", "msim_hip_batch_shapes", "msim_hip_batch_shape",
+    "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -184,6 +198,14 @@ def library(path: str = LIBRARY):
         "msim_hip_learned_train": (I, [P, P, P, P]),
         "msim_hip_learned_gradients": (I, [P, U, P, P]),
         "msim_hip_learned_apply": (I, [P, U, P]),
+        "msim_hip_batch_upload": (I, [P, U, P, P]),
+        "msim_hip_batch_solve_flow": (I, [P, P, P, P]),
+        "msim_hip_batch_autotune": (I, [P, C.c_float, U, P, P]),
+        "msim_hip_batch_shapes": (I, []),
+        "msim_hip_batch_shape": (I, [I, P, P, P]),
+        "msim_hip_batch_flags": (I, [P, P]),
+        "msim_hip_batch_run": (I, [P, C.c_float, U, U, P, P]),
+        "msim_hip_batch_download": (I, [P, U, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -196,6 +218,27 @@ def library(path: str = LIBRARY):
 def device_count() -> int:
     """backend_device_count(BackendKind::hip); 0 when no device or no runtime."""
     return int(library().msim_hip_device_count())
+
+
+@dataclass
+class BatchInstance:
+    """One device of a batched sweep (include/msim_hip.h, msim_batch_instance)."""
+    nx: int
+    ny: int
+    mobility: np.ndarray
+    obstacles: np.ndarray | None = None
+    phi: np.ndarray | None = None
+    levels: np.ndarray | None = None
+    inlet_pressure: float = 1.0
+    diffusion: float = 500.0
+    reservoir: float = 1.0
+    dx: float = 1.0
+    dy: float = 1.0
+    dz: float = 10.0
+
+    @property
+    def sites(self) -> int:
+        return self.nx * self.ny
 
 
 # --------------------------------------------------------------------------
@@ -621,6 +664,89 @@ class HipBackend:
         z = np.zeros(self._train_sites, np.float32)
         self._check(self.lib.msim_hip_learned_apply(self.ctx, pair, z.ctypes.data))
         return z
+
+    # -- batched sweep (no counterpart in the reference) ----------------------
+    def batch_upload(self, instances, relative_tolerance: float = 1e-6,
+                     max_iterations: int = 20000, monod_vmax: float = 1.0,
+                     monod_k: float = 0.1):
+        keep = []
+        arr = (_BatchInstance * len(instances))()
+        for k, inst in enumerate(instances):
+            c = arr[k]
+            c.nx, c.ny, c.dx, c.dy, c.dz = inst.nx, inst.ny, inst.dx, inst.dy, inst.dz
+            for name, dtype in (("obstacles", np.uint8), ("mobility", np.float32),
+                                ("phi", np.float32), ("levels", np.float32)):
+                a = getattr(inst, name)
+                if a is None:
+                    setattr(c, name, None)
+                    continue
+                a = np.ascontiguousarray(a, dtype).reshape(-1)
+                if a.size != inst.sites:
+                    raise InvalidArgument(f"batch instance {k}: {name} must have {inst.sites} values")
+                keep.append(a)
+                setattr(c, name, a.ctypes.data)
+            c.inlet_pressure, c.diffusion, c.reservoir = (inst.inlet_pressure, inst.diffusion,
+                                                          inst.reservoir)
+        p = _BatchParams(relative_tolerance, max_iterations, monod_vmax, monod_k)
+        self._batch = list(instances)
+        self._check(self.lib.msim_hip_batch_upload(self.ctx, len(instances), arr, C.byref(p)))
+
+    def batch_solve_flow(self):
+        """(iterations, relative residuals, flags) per instance."""
+        n = len(self._batch)
+        it = np.zeros(n, np.uint32)
+        res = np.zeros(n, np.float32)
+        fl = np.zeros(n, np.uint32)
+        self._check(self.lib.msim_hip_batch_solve_flow(self.ctx, it.ctypes.data, res.ctypes.data,
+                                                       fl.ctypes.data))
+        return it, res, fl
+
+    @staticmethod
+    def batch_shapes():
+        """The launch shapes the autotuner tries: [(threads, row width, sites per item)]."""
+        lib = library()
+        out = []
+        for k in range(lib.msim_hip_batch_shapes()):
+            t, r, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+            lib.msim_hip_batch_shape(k, C.byref(t), C.byref(r), C.byref(c))
+            out.append((t.value, r.value, c.value))
+        return out
+
+    def batch_autotune(self, dt: float, steps: int = 3):
+        """(index of the chosen shape, per-step seconds of every shape)."""
+        secs = np.zeros(len(self.batch_shapes()), np.float64)
+        k = C.c_int(0)
+        self._check(self.lib.msim_hip_batch_autotune(self.ctx, dt, steps, C.byref(k),
+                                                     secs.ctypes.data))
+        return k.value, secs
+
+    def batch_flags(self) -> np.ndarray:
+        fl = np.zeros(len(self._batch), np.uint32)
+        self._check(self.lib.msim_hip_batch_flags(self.ctx, fl.ctypes.data))
+        return fl
+
+    def batch_run(self, dt: float, steps: int, graph: bool = True, ledger: bool = True):
+        """(ledger [steps, count, 4]: mass before, mass after, uptake, pressure
+        residual -- or None, device seconds)."""
+        n = len(self._batch)
+        led = np.zeros((steps, n, 4), np.float64) if ledger else None
+        secs = C.c_double(0.0)
+        self._check(self.lib.msim_hip_batch_run(self.ctx, dt, steps, 1 if graph else 0,
+                                                None if led is None else led.ctypes.data,
+                                                C.byref(secs)))
+        return led, secs.value
+
+    def batch_download(self, index: int) -> dict:
+        inst = self._batch[index]
+        nx, ny = inst.nx, inst.ny
+        out = dict(pressure=np.zeros(nx * ny, np.float64),
+                   x_faces=np.zeros((nx + 1) * ny, np.float32),
+                   y_faces=np.zeros(nx * (ny + 1), np.float32),
+                   levels=np.zeros(nx * ny, np.float32))
+        self._check(self.lib.msim_hip_batch_download(
+            self.ctx, index, out["pressure"].ctypes.data, out["x_faces"].ctypes.data,
+            out["y_faces"].ctypes.data, out["levels"].ctypes.data))
+        return out
 
     # -- the rest of ComputeBackend: not on this backend ------------------------
     def _unsupported(self, *args, **kwargs):

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <vector>
 
+#include "adr_site.hpp"
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -28,67 +29,6 @@ namespace {
 
 constexpr int kAdrBlock = 256;
 constexpr uint32_t kFlagNegative = 1;
-
-// The exterior value beyond a boundary site (boundary_value, signals.cpp:181-192).
-__device__ inline float exterior(const AdrParams& P, int face, int s, float current,
-                                 float periodic) {
-  const uint8_t kind = P.kind[face];
-  if (kind == MSIM_NO_FLUX) return current;
-  if (kind == MSIM_PERIODIC) return periodic;
-  return P.boundary[s][face];
-}
-
-struct SiteGeometry {
-  bool clo[2], cup[2];      // face_closure (signals.cpp:213-261), x then y
-  float ulo[2], uup[2];     // face velocities (signals.cpp:157-177)
-};
-
-__device__ inline bool solid_at(const uint8_t* solid, size_t i) { return solid[i] != 0; }
-
-__device__ inline SiteGeometry geometry(const AdrParams& P, const uint8_t* __restrict__ solid,
-                                        const float* __restrict__ xf,
-                                        const float* __restrict__ yf, uint32_t x, uint32_t y,
-                                        int s) {
-  const uint32_t nx = P.nx, ny = P.ny;
-  const size_t i = static_cast<size_t>(x) * ny + y;
-  SiteGeometry g;
-  // x axis
-  if (x == 0) {
-    g.clo[0] = P.kind[0] == MSIM_NO_FLUX ||
-               (P.kind[0] == MSIM_PERIODIC && solid_at(solid, static_cast<size_t>(nx - 1) * ny + y));
-  } else {
-    g.clo[0] = solid_at(solid, i - ny);
-  }
-  if (x + 1 == nx) {
-    g.cup[0] = P.kind[1] == MSIM_NO_FLUX || (P.kind[1] == MSIM_PERIODIC && solid_at(solid, y));
-  } else {
-    g.cup[0] = solid_at(solid, i + ny);
-  }
-  // y axis
-  if (y == 0) {
-    g.clo[1] = P.kind[2] == MSIM_NO_FLUX ||
-               (P.kind[2] == MSIM_PERIODIC && solid_at(solid, static_cast<size_t>(x) * ny + ny - 1));
-  } else {
-    g.clo[1] = solid_at(solid, i - 1);
-  }
-  if (y + 1 == ny) {
-    g.cup[1] = P.kind[3] == MSIM_NO_FLUX ||
-               (P.kind[3] == MSIM_PERIODIC && solid_at(solid, static_cast<size_t>(x) * ny));
-  } else {
-    g.cup[1] = solid_at(solid, i + 1);
-  }
-  if (P.has_field != 0) {
-    g.ulo[0] = xf[i];
-    g.uup[0] = xf[i + ny];
-    const size_t yfi = static_cast<size_t>(x) * (ny + 1) + y;
-    g.ulo[1] = yf[yfi];
-    g.uup[1] = yf[yfi + 1];
-  } else {
-    g.ulo[0] = g.uup[0] = P.advection[s][0];
-    g.ulo[1] = g.uup[1] = P.advection[s][1];
-  }
-  return g;
-}
 
 // One FP32 forward-Euler step for S signals; one site per thread.
 template <int S>
@@ -107,46 +47,15 @@ __global__ __launch_bounds__(kAdrBlock) void k_adr_f32(
     for (int s = 0; s < S; ++s) out[s * n + i] = in[s * n + i];
     return;
   }
-  const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
-  const uint32_t dims[2] = {P.nx, P.ny};
   bool bad = false;
   SiteGeometry g = geometry(P, solid, xf, yf, x, y, 0);
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, s);
-    const float* lv = in + s * n;
-    const float current = lv[i];
-    float lower[2], upper[2];
-    lower[0] = x == 0 ? exterior(P, 0, s, current, lv[static_cast<size_t>(P.nx - 1) * P.ny + y])
-                      : lv[i - P.ny];
-    upper[0] = x + 1 == P.nx ? exterior(P, 1, s, current, lv[y]) : lv[i + P.ny];
-    lower[1] = y == 0 ? exterior(P, 2, s, current, lv[static_cast<size_t>(x) * P.ny + P.ny - 1])
-                      : lv[i - 1];
-    upper[1] = y + 1 == P.ny ? exterior(P, 3, s, current, lv[static_cast<size_t>(x) * P.ny])
-                             : lv[i + 1];
-    const float diffusion = P.diffusion[s];
-    float rate = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      if (dims[a] == 1) continue;
-      if (g.clo[a]) lower[a] = current;
-      if (g.cup[a]) upper[a] = current;
-      const float inverse_spacing = ih[a];
-      rate += diffusion * (lower[a] - (2.0f * current) + upper[a]) * inverse_spacing *
-              inverse_spacing;
-      float lower_flux = g.ulo[a] >= 0.0f ? g.ulo[a] * lower[a] : g.ulo[a] * current;
-      float upper_flux = g.uup[a] >= 0.0f ? g.uup[a] * current : g.uup[a] * upper[a];
-      if (g.clo[a]) lower_flux = 0.0f;
-      if (g.cup[a]) upper_flux = 0.0f;
-      rate -= (upper_flux - lower_flux) * inverse_spacing;
-    }
     const size_t index = s * n + i;
-    if (P.has_affine != 0) rate += source[index] - (loss[index] * current);
-    if (P.has_monod != 0) {
-      const float uptake = P.vmax[s] * phi[i] / (P.monod_k[s] + current);
-      rate += 0.0f - (uptake * current);
-    }
-    const float candidate = current + (dt * rate);
+    const float candidate = fe_candidate_f32(P, g, in + s * n, x, y, s, dt,
+                                             P.has_affine != 0 ? source[index] : 0.0f,
+                                             P.has_affine != 0 ? loss[index] : 0.0f, phi);
     if (!isfinite(candidate) || candidate < 0.0f) bad = true;
     out[index] = candidate;
   }

@@ -1,0 +1,120 @@
+// Per-site arithmetic of the explicit FP32 transport step, shared by the
+// single-grid kernel (adr.hip) and the batched sweep (batch.hip) so both
+// compute every site in the same float operations and order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "internal.hpp"
+
+namespace msim {
+
+// The exterior value beyond a boundary site (boundary_value, signals.cpp:181-192).
+__device__ inline float exterior(const AdrParams& P, int face, int s, float current,
+                                 float periodic) {
+  const uint8_t kind = P.kind[face];
+  if (kind == MSIM_NO_FLUX) return current;
+  if (kind == MSIM_PERIODIC) return periodic;
+  return P.boundary[s][face];
+}
+
+struct SiteGeometry {
+  bool clo[2], cup[2];      // face_closure (signals.cpp:213-261), x then y
+  float ulo[2], uup[2];     // face velocities (signals.cpp:157-177)
+};
+
+__device__ inline bool solid_at(const uint8_t* solid, size_t i) { return solid[i] != 0; }
+
+__device__ inline SiteGeometry geometry(const AdrParams& P, const uint8_t* __restrict__ solid,
+                                        const float* __restrict__ xf,
+                                        const float* __restrict__ yf, uint32_t x, uint32_t y,
+                                        int s) {
+  const uint32_t nx = P.nx, ny = P.ny;
+  const size_t i = static_cast<size_t>(x) * ny + y;
+  SiteGeometry g;
+  // x axis
+  if (x == 0) {
+    g.clo[0] = P.kind[0] == MSIM_NO_FLUX ||
+               (P.kind[0] == MSIM_PERIODIC && solid_at(solid, static_cast<size_t>(nx - 1) * ny + y));
+  } else {
+    g.clo[0] = solid_at(solid, i - ny);
+  }
+  if (x + 1 == nx) {
+    g.cup[0] = P.kind[1] == MSIM_NO_FLUX || (P.kind[1] == MSIM_PERIODIC && solid_at(solid, y));
+  } else {
+    g.cup[0] = solid_at(solid, i + ny);
+  }
+  // y axis
+  if (y == 0) {
+    g.clo[1] = P.kind[2] == MSIM_NO_FLUX ||
+               (P.kind[2] == MSIM_PERIODIC && solid_at(solid, static_cast<size_t>(x) * ny + ny - 1));
+  } else {
+    g.clo[1] = solid_at(solid, i - 1);
+  }
+  if (y + 1 == ny) {
+    g.cup[1] = P.kind[3] == MSIM_NO_FLUX ||
+               (P.kind[3] == MSIM_PERIODIC && solid_at(solid, static_cast<size_t>(x) * ny));
+  } else {
+    g.cup[1] = solid_at(solid, i + 1);
+  }
+  if (P.has_field != 0) {
+    g.ulo[0] = xf[i];
+    g.uup[0] = xf[i + ny];
+    const size_t yfi = static_cast<size_t>(x) * (ny + 1) + y;
+    g.ulo[1] = yf[yfi];
+    g.uup[1] = yf[yfi + 1];
+  } else {
+    g.ulo[0] = g.uup[0] = P.advection[s][0];
+    g.ulo[1] = g.uup[1] = P.advection[s][1];
+  }
+  return g;
+}
+
+// Forward-Euler candidate of signal s at site (x, y): the reference's
+// per-site rate in its own float order (cpp/core/signals.cpp:1145-1291
+// transport, 305-320 affine reaction, 1109-1134 forward Euler), with Monod
+// uptake as the affine loss vmax*phi/(K+c) it equals under forward Euler.
+// lv: this signal's levels; source/loss: its affine rates at the site.
+__device__ inline float fe_candidate_f32(const AdrParams& P, const SiteGeometry& g,
+                                         const float* __restrict__ lv, uint32_t x, uint32_t y,
+                                         int s, float dt, float source, float loss,
+                                         const float* __restrict__ phi) {
+  const size_t i = static_cast<size_t>(x) * P.ny + y;
+  const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
+  const uint32_t dims[2] = {P.nx, P.ny};
+  const float current = lv[i];
+  float lower[2], upper[2];
+  lower[0] = x == 0 ? exterior(P, 0, s, current, lv[static_cast<size_t>(P.nx - 1) * P.ny + y])
+                    : lv[i - P.ny];
+  upper[0] = x + 1 == P.nx ? exterior(P, 1, s, current, lv[y]) : lv[i + P.ny];
+  lower[1] = y == 0 ? exterior(P, 2, s, current, lv[static_cast<size_t>(x) * P.ny + P.ny - 1])
+                    : lv[i - 1];
+  upper[1] = y + 1 == P.ny ? exterior(P, 3, s, current, lv[static_cast<size_t>(x) * P.ny])
+                           : lv[i + 1];
+  const float diffusion = P.diffusion[s];
+  float rate = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    if (dims[a] == 1) continue;
+    if (g.clo[a]) lower[a] = current;
+    if (g.cup[a]) upper[a] = current;
+    const float inverse_spacing = ih[a];
+    rate += diffusion * (lower[a] - (2.0f * current) + upper[a]) * inverse_spacing *
+            inverse_spacing;
+    float lower_flux = g.ulo[a] >= 0.0f ? g.ulo[a] * lower[a] : g.ulo[a] * current;
+    float upper_flux = g.uup[a] >= 0.0f ? g.uup[a] * current : g.uup[a] * upper[a];
+    if (g.clo[a]) lower_flux = 0.0f;
+    if (g.cup[a]) upper_flux = 0.0f;
+    rate -= (upper_flux - lower_flux) * inverse_spacing;
+  }
+  if (P.has_affine != 0) rate += source - (loss * current);
+  if (P.has_monod != 0) {
+    const float uptake = P.vmax[s] * phi[i] / (P.monod_k[s] + current);
+    rate += 0.0f - (uptake * current);
+  }
+  return current + (dt * rate);
+}
+
+}  // namespace msim

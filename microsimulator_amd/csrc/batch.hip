@@ -1,0 +1,803 @@
+// Batched sweep: many independent depth-averaged devices of uneven sizes
+// (a parameter sweep), each with its own geometry, mobility, occupancy,
+// inlet pressure and solute diffusivity, advanced together. No counterpart
+// in the reference, which runs one job per invocation
+// (docs/architecture/0005-batch-execution.md:30); each instance follows the
+// reference's single-grid algorithms:
+//   pressure  -- the assembled operator (flow_system.hpp:522-695) solved by
+//                FP32 Jacobi-PCG corrections (cpp/cpu/cpu_flow.cpp:65-140; FP64
+//                dot products) inside FP64 iterative refinement of an FP64
+//                pressure, warm-started, to an FP64 true relative residual;
+//   velocity  -- face velocities and lift (flow_system.hpp:711-736, 424-445)
+//                for the Dirichlet inlet pressure, unscaled;
+//   transport -- one FP32 forward-Euler step of one solute with Monod uptake,
+//                every site in the single-grid kernel's float operations
+//                (adr_site.hpp), with FP64 mass tallies.
+//
+// A step is ONE launch over work items (instance, block of rows) of a few
+// thousand sites, spread over the whole chip. Each item recomputes its rows'
+// FP64 pressure residual (the step's flow check), writes the face velocities
+// it owns, and advances its sites with the face velocities computed where
+// they are needed, so no item waits on another. Per-item FP64 partials
+// (residual, masses, uptake) are added per instance in a fixed order on the
+// host. A run of steps is captured once into a HIP graph and replayed.
+//
+// The pressure solve itself (cold, or when a step's check finds an instance
+// unconverged) is one launch with one workgroup per instance (largest first):
+// the refinement loop needs the instance's dot products, and a workgroup
+// reduces them with barriers only.
+//
+// The launch shapes (workgroup size, sweep row width, item size) are chosen
+// by timing each candidate on the uploaded batch: the autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "adr_site.hpp"
+#include "common.hpp"
+#include "internal.hpp"
+#include "stencil.hpp"
+
+namespace msim {
+namespace {
+
+struct BatchDesc {
+  uint32_t nx, ny;
+  uint64_t site, xface, yface;  // first site / x face / y face of the instance
+  float dx, dy, dz;
+  float inlet;      // pressure at x = 0 (0 at x = nx)
+  float diffusion;  // of the solute
+  float reservoir;  // solute level beyond x = 0 (0 beyond x = nx)
+};
+
+struct BatchItem {
+  uint32_t inst, x0, x1, pad;
+};
+
+struct BatchData {
+  const BatchDesc* desc;
+  const uint32_t* order;
+  const uint8_t* solid;
+  const float* cond;
+  const float* cxl;
+  const float* cyl;
+  const float* bc;
+  const float* dinv;
+  const float* rhs;
+  const float* phi;
+  double* x64;  // pressure
+  float* x;     // FP32 correction of one refinement
+  float* r;
+  float* p;
+  float* q;
+  float* xf;
+  float* yf;
+  float* la;
+  float* lb;
+  uint32_t* iterations;  // [instance]: FP32 CG iterations of the last flow solve
+  float* residual;       // [instance]: FP64 true relative residual after it
+  double* bnorm2;        // [instance]: ||b||^2
+  uint32_t* flags;       // [instance]: 1 CG breakdown, 2 not converged, 4 bad level
+};
+
+struct BatchParams {
+  float tolerance;
+  uint32_t max_iterations;
+  float vmax, monod_k;
+};
+
+// FP64 sums over the workgroup, broadcast to every thread.
+template <int NT, int K>
+__device__ inline void block_sums(double (&v)[K], double* lds) {
+  __shared__ double s_out[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double t = block_sum<NT>(v[k], lds);
+    if (threadIdx.x == 0) s_out[k] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = s_out[k];
+  __syncthreads();
+}
+
+// The operator (flow_system.hpp:670-695) on an FP32 vector at site i.
+__device__ inline float apply_f32(const float* __restrict__ v, const float* __restrict__ cxl,
+                                  const float* __restrict__ cyl, const float* __restrict__ bc,
+                                  uint32_t x, uint32_t y, uint32_t nx, uint32_t ny, size_t i,
+                                  float isx, float isy) {
+  const float c = v[i];
+  float a = bc[i] * c;
+  if (x > 0) a += (cxl[i] * isx) * (c - v[i - ny]);
+  if (x + 1 < nx) a += (cxl[i + ny] * isx) * (c - v[i + ny]);
+  if (y > 0) a += (cyl[i] * isy) * (c - v[i - 1]);
+  if (y + 1 < ny) a += (cyl[i + 1] * isy) * (c - v[i + 1]);
+  return a;
+}
+
+// b - A x in FP64 at site i, b = inlet * (the assembled Dirichlet-1 rhs).
+__device__ inline double residual_f64(const double* __restrict__ x, const float* __restrict__ cxl,
+                                      const float* __restrict__ cyl, const float* __restrict__ bc,
+                                      const float* __restrict__ rhs, float inlet, uint32_t xx,
+                                      uint32_t y, uint32_t nx, uint32_t ny, size_t i, double isx,
+                                      double isy) {
+  const double c = x[i];
+  double a = static_cast<double>(bc[i]) * c;
+  if (xx > 0) a += (static_cast<double>(cxl[i]) * isx) * (c - x[i - ny]);
+  if (xx + 1 < nx) a += (static_cast<double>(cxl[i + ny]) * isx) * (c - x[i + ny]);
+  if (y > 0) a += (static_cast<double>(cyl[i]) * isy) * (c - x[i - 1]);
+  if (y + 1 < ny) a += (static_cast<double>(cyl[i + 1]) * isy) * (c - x[i + 1]);
+  return static_cast<double>(inlet) * static_cast<double>(rhs[i]) - a;
+}
+
+// Face velocities (flow_system.hpp:711-736 with the lift of 424-445) from the
+// FP64 pressure: x face fx (0..nx) of row y, y face fy (0..ny) of column x.
+__device__ inline float face_x(const BatchDesc& I, const double* __restrict__ p,
+                               const float* __restrict__ cond, const uint8_t* __restrict__ solid,
+                               uint32_t fx, uint32_t y) {
+  const uint32_t nx = I.nx, ny = I.ny;
+  const bool has_lower = fx > 0, has_upper = fx < nx;
+  const size_t upper = static_cast<size_t>(fx) * ny + y;
+  const size_t lower = upper - ny;
+  double value;
+  if (has_lower && has_upper) {
+    const float fm = harmonic(cond[lower], cond[upper]);
+    value = -static_cast<double>(fm) * (p[upper] - p[lower]) / static_cast<double>(I.dx);
+  } else {
+    const size_t site = has_upper ? upper : lower;
+    const float m = cond[site];
+    float fc = 0.0f, fp = 0.0f;
+    if (m != 0.0f) {
+      fc = (has_upper ? 2.0f : -2.0f) * m / I.dx;
+      fp = has_upper ? I.inlet : 0.0f;
+    }
+    value = static_cast<double>(fc) * (static_cast<double>(fp) - p[site]);
+  }
+  const float flux = static_cast<float>(value);
+  if ((has_lower && solid[lower] != 0) || (has_upper && solid[upper] != 0)) return 0.0f;
+  return flux / I.dz;
+}
+
+__device__ inline float face_y(const BatchDesc& I, const double* __restrict__ p,
+                               const float* __restrict__ cond, const uint8_t* __restrict__ solid,
+                               uint32_t x, uint32_t fy) {
+  const uint32_t ny = I.ny;
+  const bool has_lower = fy > 0, has_upper = fy < ny;
+  const size_t upper = static_cast<size_t>(x) * ny + fy;
+  const size_t lower = upper - 1;
+  double value;
+  if (has_lower && has_upper) {
+    const float fm = harmonic(cond[lower], cond[upper]);
+    value = -static_cast<double>(fm) * (p[upper] - p[lower]) / static_cast<double>(I.dy);
+  } else {
+    const size_t site = has_upper ? upper : lower;
+    value = static_cast<double>(0.0f) * (static_cast<double>(0.0f) - p[site]);
+  }
+  const float flux = static_cast<float>(value);
+  if ((has_lower && solid[lower] != 0) || (has_upper && solid[upper] != 0)) return 0.0f;
+  return flux / I.dz;
+}
+
+// The pressure solve: one workgroup per instance (order[b], largest first),
+// FP64 refinement around FP32 Jacobi-PCG corrections, then every face
+// velocity of the instance.
+template <int NT, int TY>
+__global__ __launch_bounds__(NT) void k_batch_flow(BatchData D, BatchParams P) {
+  __shared__ double s_red[NT / kWave];
+  constexpr int RX = NT / TY;
+  const uint32_t inst = D.order[blockIdx.x];
+  const BatchDesc I = D.desc[inst];
+  const uint32_t nx = I.nx, ny = I.ny;
+  const size_t o = I.site;
+  const uint32_t tx = threadIdx.x / TY, ty = threadIdx.x % TY;
+  const uint8_t* solid = D.solid + o;
+  const float* cond = D.cond + o;
+  const float* cxl = D.cxl + o;
+  const float* cyl = D.cyl + o;
+  const float* bc = D.bc + o;
+  const float* dinv = D.dinv + o;
+  const float* rhs = D.rhs + o;
+  double* x64 = D.x64 + o;
+  float* xv = D.x + o;
+  float* r = D.r + o;
+  float* p = D.p + o;
+  float* q = D.q + o;
+  const float isx = 1.0f / (I.dx * I.dx), isy = 1.0f / (I.dy * I.dy);
+  auto sweep = [&](auto&& f) {
+    for (uint32_t x0 = 0; x0 < nx; x0 += RX) {
+      const uint32_t x = x0 + tx;
+      if (x >= nx) continue;
+      for (uint32_t y = ty; y < ny; y += TY) f(x, y, static_cast<size_t>(x) * ny + y);
+    }
+  };
+  constexpr uint32_t kRefinements = 16;
+  constexpr double kInner = 1.0e-4;  // relative tolerance of each FP32 correction
+  uint32_t it = 0, failed = 0;
+  double rr64 = 0.0, bb = 0.0, threshold = 0.0;
+  for (uint32_t ref = 0; ref <= kRefinements; ++ref) {
+    // FP64 true residual; its FP32 copy starts the correction e = 0, p = D^-1 r.
+    double s3[3] = {0.0, 0.0, 0.0};  // r.r (FP64), b.b, r.z (FP32 r)
+    sweep([&](uint32_t x, uint32_t y, size_t i) {
+      const double res = residual_f64(x64, cxl, cyl, bc, rhs, I.inlet, x, y, nx, ny, i, isx, isy);
+      const double b = static_cast<double>(I.inlet) * static_cast<double>(rhs[i]);
+      const float ri = static_cast<float>(res);
+      const float zi = dinv[i] * ri;
+      r[i] = ri;
+      p[i] = zi;
+      xv[i] = 0.0f;
+      s3[0] += res * res;
+      s3[1] += b * b;
+      s3[2] += static_cast<double>(ri) * static_cast<double>(zi);
+    });
+    block_sums<NT, 3>(s3, s_red);
+    rr64 = s3[0];
+    bb = s3[1];
+    const double bound = static_cast<double>(P.tolerance) * sqrt(bb);
+    threshold = bound * bound;
+    if (rr64 <= threshold || ref == kRefinements || it >= P.max_iterations) break;
+    double rz = s3[2], rr = rr64;
+    const double inner = kInner * kInner * rr64;
+    while (rr > inner && it < P.max_iterations) {
+      double s1[1] = {0.0};
+      sweep([&](uint32_t x, uint32_t y, size_t i) {
+        const float qi = apply_f32(p, cxl, cyl, bc, x, y, nx, ny, i, isx, isy);
+        q[i] = qi;
+        s1[0] += static_cast<double>(p[i]) * static_cast<double>(qi);
+      });
+      block_sums<NT, 1>(s1, s_red);
+      if (!(s1[0] > 0.0) || !isfinite(s1[0])) {
+        failed = 1;
+        break;
+      }
+      const float alpha = static_cast<float>(rz / s1[0]);
+      double s2[2] = {0.0, 0.0};
+      sweep([&](uint32_t x, uint32_t y, size_t i) {
+        xv[i] = xv[i] + alpha * p[i];
+        const float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const float zi = dinv[i] * ri;
+        s2[0] += static_cast<double>(ri) * static_cast<double>(zi);
+        s2[1] += static_cast<double>(ri) * static_cast<double>(ri);
+      });
+      block_sums<NT, 2>(s2, s_red);
+      const float beta = static_cast<float>(s2[0] / rz);
+      rz = s2[0];
+      rr = s2[1];
+      sweep([&](uint32_t x, uint32_t y, size_t i) { p[i] = dinv[i] * r[i] + beta * p[i]; });
+      __syncthreads();
+      ++it;
+    }
+    if (failed != 0) break;
+    sweep([&](uint32_t x, uint32_t y, size_t i) { x64[i] += static_cast<double>(xv[i]); });
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D.iterations[inst] = it;
+    D.residual[inst] = bb > 0.0 ? static_cast<float>(sqrt(rr64 / bb)) : 0.0f;
+    D.bnorm2[inst] = bb;
+    if (failed != 0) D.flags[inst] |= 1u;
+    if (rr64 > threshold && failed == 0) D.flags[inst] |= 2u;
+  }
+  // Face velocities.
+  float* xf = D.xf + I.xface;
+  float* yf = D.yf + I.yface;
+  for (size_t f = threadIdx.x; f < static_cast<size_t>(nx + 1) * ny; f += NT)
+    xf[f] = face_x(I, x64, cond, solid, static_cast<uint32_t>(f / ny), static_cast<uint32_t>(f % ny));
+  for (size_t f = threadIdx.x; f < static_cast<size_t>(nx) * (ny + 1); f += NT)
+    yf[f] = face_y(I, x64, cond, solid, static_cast<uint32_t>(f / (ny + 1)),
+                   static_cast<uint32_t>(f % (ny + 1)));
+}
+
+// One step for work item blockIdx.x (rows [x0, x1) of an instance): the FP64
+// residual of the rows, their face velocities, and the forward-Euler step of
+// their sites. partials[(step * items + item) * 4 + k]: residual r.r, mass
+// before, mass after, uptake rate (FP64, before the volume factor).
+template <int NT, int TY>
+__global__ __launch_bounds__(NT) void k_batch_step(BatchData D, BatchParams P,
+                                                   const BatchItem* __restrict__ items,
+                                                   double* __restrict__ partials, uint32_t step,
+                                                   float dt) {
+  __shared__ double s_red[NT / kWave];
+  constexpr int RX = NT / TY;
+  const BatchItem item = items[blockIdx.x];
+  const uint32_t inst = item.inst;
+  const BatchDesc I = D.desc[inst];
+  const uint32_t nx = I.nx, ny = I.ny;
+  const size_t o = I.site;
+  const uint8_t* solid = D.solid + o;
+  const float* cond = D.cond + o;
+  const double* x64 = D.x64 + o;
+  float* xf = D.xf + I.xface;
+  float* yf = D.yf + I.yface;
+  const float* in = ((step & 1) == 0 ? D.la : D.lb) + o;
+  float* out = ((step & 1) == 0 ? D.lb : D.la) + o;
+  const float* phi = D.phi != nullptr ? D.phi + o : nullptr;
+  const double isx = 1.0 / (static_cast<double>(I.dx) * I.dx);
+  const double isy = 1.0 / (static_cast<double>(I.dy) * I.dy);
+  AdrParams A{};
+  A.nx = nx;
+  A.ny = ny;
+  A.signals = 1;
+  A.dx = I.dx;
+  A.dy = I.dy;
+  A.dz = I.dz;
+  A.kind[0] = A.kind[1] = MSIM_FIXED;
+  A.kind[2] = A.kind[3] = A.kind[4] = A.kind[5] = MSIM_NO_FLUX;
+  A.has_field = 0;  // velocities are set per site below
+  A.has_affine = 0;
+  A.has_monod = phi != nullptr ? 1 : 0;
+  A.diffusion[0] = I.diffusion;
+  A.boundary[0][0] = I.reservoir;
+  A.vmax[0] = P.vmax;
+  A.monod_k[0] = P.monod_k;
+  const uint32_t tx = threadIdx.x / TY, ty = threadIdx.x % TY;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t bad = 0;
+  for (uint32_t xr = item.x0; xr < item.x1; xr += RX) {
+    const uint32_t x = xr + tx;
+    if (x >= item.x1) continue;
+    for (uint32_t y = ty; y < ny; y += TY) {
+      const size_t i = static_cast<size_t>(x) * ny + y;
+      const double res = residual_f64(x64, D.cxl + o, D.cyl + o, D.bc + o, D.rhs + o, I.inlet, x,
+                                      y, nx, ny, i, isx, isy);
+      acc[0] += res * res;
+      // This site's four faces; the lower x and y faces (and the upper ones
+      // at the grid's end) are this item's to store.
+      const float uxl = face_x(I, x64, cond, solid, x, y);
+      const float uxu = face_x(I, x64, cond, solid, x + 1, y);
+      const float uyl = face_y(I, x64, cond, solid, x, y);
+      const float uyu = face_y(I, x64, cond, solid, x, y + 1);
+      xf[i] = uxl;
+      if (x + 1 == nx) xf[i + ny] = uxu;
+      const size_t yi = static_cast<size_t>(x) * (ny + 1) + y;
+      yf[yi] = uyl;
+      if (y + 1 == ny) yf[yi + 1] = uyu;
+      const float current = in[i];
+      if (solid[i] != 0) {
+        out[i] = current;
+        continue;
+      }
+      SiteGeometry g = geometry(A, solid, xf, yf, x, y, 0);
+      g.ulo[0] = uxl;
+      g.uup[0] = uxu;
+      g.ulo[1] = uyl;
+      g.uup[1] = uyu;
+      const float candidate = fe_candidate_f32(A, g, in, x, y, 0, dt, 0.0f, 0.0f, phi);
+      if (!isfinite(candidate) || candidate < 0.0f) bad = 1;
+      out[i] = candidate;
+      acc[1] += static_cast<double>(current);
+      acc[2] += static_cast<double>(candidate);
+      if (phi != nullptr) {
+        acc[3] += static_cast<double>(P.vmax) * static_cast<double>(phi[i]) *
+                  static_cast<double>(current) / (static_cast<double>(P.monod_k) + current);
+      }
+    }
+  }
+  if (bad) atomicOr(&D.flags[inst], 4u);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double t = block_sum<NT>(acc[k], s_red);
+    if (threadIdx.x == 0) partials[(static_cast<size_t>(step) * gridDim.x + blockIdx.x) * 4 + k] = t;
+  }
+}
+
+__global__ void k_batch_dinv(const float* __restrict__ diag, float* __restrict__ dinv, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dinv[i] = diag[i] > 0.0f ? 1.0f / diag[i] : 0.0f;
+}
+
+using FlowKernel = void (*)(BatchData, BatchParams);
+using StepKernel = void (*)(BatchData, BatchParams, const BatchItem*, double*, uint32_t, float);
+
+struct Shape {
+  int threads, row, chunk;  // workgroup size, sweep row width, sites per item
+  StepKernel step;
+};
+
+const Shape kShapes[] = {
+    {256, 64, 4096, k_batch_step<256, 64>},      {256, 128, 8192, k_batch_step<256, 128>},
+    {256, 64, 16384, k_batch_step<256, 64>},     {512, 64, 8192, k_batch_step<512, 64>},
+    {512, 128, 16384, k_batch_step<512, 128>},   {1024, 128, 16384, k_batch_step<1024, 128>},
+    {1024, 256, 32768, k_batch_step<1024, 256>},
+};
+constexpr int kShapeCount = static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0]));
+constexpr int kDefaultShape = 3;
+constexpr FlowKernel kFlowKernel = k_batch_flow<1024, 128>;
+constexpr int kFlowThreads = 1024;
+
+}  // namespace
+
+// Device state of an uploaded batch.
+struct BatchDevice {
+  uint32_t count = 0;
+  uint64_t sites = 0, xfaces = 0, yfaces = 0;
+  std::vector<BatchDesc> desc;
+  std::vector<uint32_t> order;
+  std::vector<std::vector<BatchItem>> items;  // per shape
+  std::vector<std::unique_ptr<DeviceBuffer<BatchItem>>> d_items;
+  DeviceBuffer<BatchDesc> d_desc;
+  DeviceBuffer<uint32_t> d_order, iterations, flags;
+  DeviceBuffer<uint8_t> solid;
+  DeviceBuffer<float> mobility, cond, cxl, cyl, bc, diag, dinv, rhs, phi, x, r, p, q, xf, yf, la,
+      lb, residual, saved;
+  DeviceBuffer<double> x64, bnorm2, partials;
+  BatchParams params{};
+  bool has_phi = false;
+  bool solved = false;
+  int shape = -1;   // chosen by the autotuner; -1: the default
+  int current = 0;  // 0: levels in la, 1: in lb
+  hipGraphExec_t graph = nullptr;  // the captured run and what it was captured with
+  uint32_t graph_steps = 0;
+  float graph_dt = 0.0f;
+  int graph_shape = -1;
+  BatchData graph_data{};
+  const double* graph_partials = nullptr;
+  ~BatchDevice() {
+    if (graph != nullptr) static_cast<void>(hipGraphExecDestroy(graph));
+  }
+};
+
+namespace {
+
+BatchData batch_data(BatchDevice& B) {
+  return BatchData{B.d_desc.data(),   B.d_order.data(),  B.solid.data(),   B.cond.data(),
+                   B.cxl.data(),      B.cyl.data(),      B.bc.data(),      B.dinv.data(),
+                   B.rhs.data(),      B.has_phi ? B.phi.data() : nullptr,  B.x64.data(),
+                   B.x.data(),        B.r.data(),        B.p.data(),       B.q.data(),
+                   B.xf.data(),       B.yf.data(),       B.la.data(),      B.lb.data(),
+                   B.iterations.data(), B.residual.data(), B.bnorm2.data(), B.flags.data()};
+}
+
+int shape_of(const BatchDevice& B) { return B.shape >= 0 ? B.shape : kDefaultShape; }
+
+void launch_step(Context& ctx, BatchDevice& B, const BatchData& D, int shape, uint32_t step,
+                 float dt) {
+  const Shape& s = kShapes[shape];
+  const uint32_t n = static_cast<uint32_t>(B.items[shape].size());
+  hipLaunchKernelGGL(s.step, dim3(n), dim3(s.threads), 0, ctx.stream, D, B.params,
+                     B.d_items[shape]->data(), B.partials.data(), step, dt);
+  check(hipGetLastError(), "launch k_batch_step");
+}
+
+// Runs `steps` steps from the current levels, in a graph or eagerly.
+void run_steps(Context& ctx, BatchDevice& B, int shape, float dt, uint32_t steps, bool graph) {
+  const BatchData D = batch_data(B);
+  if (B.current != 0) {
+    check(hipMemcpyAsync(B.la.data(), B.lb.data(), B.sites * sizeof(float),
+                         hipMemcpyDeviceToDevice, ctx.stream),
+          "levels to the first buffer");
+    B.current = 0;
+  }
+  size_t most = 0;
+  for (const auto& it : B.items) most = std::max(most, it.size());
+  B.partials.reserve(static_cast<size_t>(steps) * most * 4);
+  if (!graph) {
+    for (uint32_t s = 0; s < steps; ++s) launch_step(ctx, B, D, shape, s, dt);
+  } else {
+    // A captured graph holds every pointer: capture again when any changed.
+    if (B.graph == nullptr || B.graph_steps != steps || B.graph_dt != dt ||
+        B.graph_shape != shape || std::memcmp(&B.graph_data, &D, sizeof(D)) != 0 ||
+        B.graph_partials != B.partials.data()) {
+      if (B.graph != nullptr) {
+        check(hipGraphExecDestroy(B.graph), "destroy graph");
+        B.graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      for (uint32_t s = 0; s < steps; ++s) launch_step(ctx, B, D, shape, s, dt);
+      check(hipStreamEndCapture(ctx.stream, &g), "end capture");
+      const hipError_t e = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
+      static_cast<void>(hipGraphDestroy(g));
+      check(e, "instantiate graph");
+      B.graph_steps = steps;
+      B.graph_dt = dt;
+      B.graph_shape = shape;
+      B.graph_data = D;
+      B.graph_partials = B.partials.data();
+    }
+    check(hipGraphLaunch(B.graph, ctx.stream), "launch graph");
+  }
+  B.current = static_cast<int>(steps & 1);
+}
+
+}  // namespace
+
+BatchDevice& batch_of(Context& ctx) {
+  if (ctx.batch == nullptr) ctx.batch = std::make_shared<BatchDevice>();
+  return *ctx.batch;
+}
+
+void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
+                  const msim_batch_params& params) {
+  ctx.batch.reset();
+  BatchDevice& B = batch_of(ctx);
+  B.count = count;
+  B.desc.resize(count);
+  uint64_t site = 0, xface = 0, yface = 0;
+  B.has_phi = false;
+  for (uint32_t k = 0; k < count; ++k) {
+    const msim_batch_instance& in = inst[k];
+    BatchDesc& d = B.desc[k];
+    d.nx = in.nx;
+    d.ny = in.ny;
+    d.site = site;
+    d.xface = xface;
+    d.yface = yface;
+    d.dx = in.dx;
+    d.dy = in.dy;
+    d.dz = in.dz;
+    d.inlet = in.inlet_pressure;
+    d.diffusion = in.diffusion;
+    d.reservoir = in.reservoir;
+    site += static_cast<uint64_t>(in.nx) * in.ny;
+    xface += static_cast<uint64_t>(in.nx + 1) * in.ny;
+    yface += static_cast<uint64_t>(in.nx) * (in.ny + 1);
+    if (in.phi != nullptr) B.has_phi = true;
+  }
+  B.sites = site;
+  B.xfaces = xface;
+  B.yfaces = yface;
+  // Largest instances first: they are dispatched first.
+  B.order.resize(count);
+  std::iota(B.order.begin(), B.order.end(), 0u);
+  std::stable_sort(B.order.begin(), B.order.end(), [&](uint32_t a, uint32_t b) {
+    return static_cast<uint64_t>(B.desc[a].nx) * B.desc[a].ny >
+           static_cast<uint64_t>(B.desc[b].nx) * B.desc[b].ny;
+  });
+  // Work items of each shape: blocks of whole rows of about `chunk` sites.
+  B.items.assign(kShapeCount, {});
+  B.d_items.clear();
+  for (int s = 0; s < kShapeCount; ++s) {
+    for (uint32_t k : B.order) {
+      const BatchDesc& d = B.desc[k];
+      const uint32_t rows = std::max<uint32_t>(1, static_cast<uint32_t>(kShapes[s].chunk) / d.ny);
+      for (uint32_t x0 = 0; x0 < d.nx; x0 += rows)
+        B.items[s].push_back(BatchItem{k, x0, std::min(d.nx, x0 + rows), 0});
+    }
+    if (B.items[s].size() > 0x7FFFFFFFu) throw Error(MSIM_OVERFLOW, "too many batch work items");
+    auto buf = std::make_unique<DeviceBuffer<BatchItem>>();
+    buf->reserve(B.items[s].size());
+    check(hipMemcpyAsync(buf->data(), B.items[s].data(), B.items[s].size() * sizeof(BatchItem),
+                         hipMemcpyHostToDevice, ctx.stream),
+          "upload work items");
+    B.d_items.push_back(std::move(buf));
+  }
+  const size_t n = site;
+  B.solid.reserve(n);
+  for (auto* b : {&B.mobility, &B.cond, &B.cxl, &B.cyl, &B.bc, &B.diag, &B.dinv, &B.rhs, &B.phi,
+                  &B.x, &B.r, &B.p, &B.q, &B.la, &B.lb, &B.saved})
+    b->reserve(n);
+  B.x64.reserve(n);
+  B.xf.reserve(xface);
+  B.yf.reserve(yface);
+  B.d_desc.reserve(count);
+  B.d_order.reserve(count);
+  B.iterations.reserve(count);
+  B.flags.reserve(count);
+  B.residual.reserve(count);
+  B.bnorm2.reserve(count);
+  check(hipMemcpyAsync(B.d_desc.data(), B.desc.data(), count * sizeof(BatchDesc),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload batch");
+  check(hipMemcpyAsync(B.d_order.data(), B.order.data(), count * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload batch order");
+  check(hipMemsetAsync(B.flags.data(), 0, count * sizeof(uint32_t), ctx.stream), "clear flags");
+  check(hipMemsetAsync(B.x64.data(), 0, n * sizeof(double), ctx.stream), "clear pressure");
+  for (uint32_t k = 0; k < count; ++k) {
+    const msim_batch_instance& in = inst[k];
+    const BatchDesc& d = B.desc[k];
+    const size_t m = static_cast<size_t>(d.nx) * d.ny;
+    if (in.obstacles != nullptr) {
+      check(hipMemcpyAsync(B.solid.data() + d.site, in.obstacles, m, hipMemcpyHostToDevice,
+                           ctx.stream),
+            "upload obstacles");
+    } else {
+      check(hipMemsetAsync(B.solid.data() + d.site, 0, m, ctx.stream), "clear obstacles");
+    }
+    check(hipMemcpyAsync(B.mobility.data() + d.site, in.mobility, m * sizeof(float),
+                         hipMemcpyHostToDevice, ctx.stream),
+          "upload mobility");
+    if (B.has_phi) {
+      if (in.phi != nullptr) {
+        check(hipMemcpyAsync(B.phi.data() + d.site, in.phi, m * sizeof(float),
+                             hipMemcpyHostToDevice, ctx.stream),
+              "upload phi");
+      } else {
+        check(hipMemsetAsync(B.phi.data() + d.site, 0, m * sizeof(float), ctx.stream),
+              "clear phi");
+      }
+    }
+    if (in.levels != nullptr) {
+      check(hipMemcpyAsync(B.la.data() + d.site, in.levels, m * sizeof(float),
+                           hipMemcpyHostToDevice, ctx.stream),
+            "upload levels");
+    } else {
+      check(hipMemsetAsync(B.la.data() + d.site, 0, m * sizeof(float), ctx.stream),
+            "clear levels");
+    }
+    launch_assembly(ctx.stream, B.solid.data() + d.site, B.mobility.data() + d.site,
+                    B.cond.data() + d.site, B.diag.data() + d.site, B.rhs.data() + d.site,
+                    B.bc.data() + d.site, B.cxl.data() + d.site, B.cyl.data() + d.site, d.nx,
+                    d.ny, d.dx, d.dy, d.dz, 0);
+  }
+  k_batch_dinv<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, ctx.stream>>>(
+      B.diag.data(), B.dinv.data(), n);
+  check(hipGetLastError(), "launch k_batch_dinv");
+  B.params.tolerance = params.relative_tolerance;
+  B.params.max_iterations = params.max_iterations;
+  B.params.vmax = params.monod_vmax;
+  B.params.monod_k = params.monod_k;
+  B.current = 0;
+  B.solved = false;
+  check(hipStreamSynchronize(ctx.stream), "batch upload");
+}
+
+// Solves every instance's pressure from its current one and rebuilds its
+// velocities (one launch).
+void batch_solve_flow(Context& ctx, uint32_t* iterations, float* residual, uint32_t* flags) {
+  BatchDevice& B = batch_of(ctx);
+  if (B.count == 0) throw Error(MSIM_INVALID_ARGUMENT, "no batch is uploaded");
+  hipLaunchKernelGGL(kFlowKernel, dim3(B.count), dim3(kFlowThreads), 0, ctx.stream,
+                     batch_data(B), B.params);
+  check(hipGetLastError(), "launch k_batch_flow");
+  B.solved = true;
+  if (iterations != nullptr) {
+    check(hipMemcpyAsync(iterations, B.iterations.data(), B.count * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download iterations");
+  }
+  if (residual != nullptr) {
+    check(hipMemcpyAsync(residual, B.residual.data(), B.count * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download residuals");
+  }
+  if (flags != nullptr) {
+    check(hipMemcpyAsync(flags, B.flags.data(), B.count * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download flags");
+  }
+  check(hipStreamSynchronize(ctx.stream), "batch flow");
+}
+
+int batch_autotune(Context& ctx, float dt, uint32_t steps, double* seconds) {
+  BatchDevice& B = batch_of(ctx);
+  if (B.count == 0) throw Error(MSIM_INVALID_ARGUMENT, "no batch is uploaded");
+  if (!B.solved) throw Error(MSIM_INVALID_ARGUMENT, "solve the batch's flow first");
+  // Every candidate starts from the same levels, and the batch is left as it was.
+  float* levels = B.current == 0 ? B.la.data() : B.lb.data();
+  check(hipMemcpyAsync(B.saved.data(), levels, B.sites * sizeof(float), hipMemcpyDeviceToDevice,
+                       ctx.stream),
+        "save levels");
+  auto restore = [&]() {
+    check(hipMemcpyAsync(B.la.data(), B.saved.data(), B.sites * sizeof(float),
+                         hipMemcpyDeviceToDevice, ctx.stream),
+          "restore levels");
+    B.current = 0;
+  };
+  double best = 0.0;
+  int choice = 0;
+  for (int c = 0; c < kShapeCount; ++c) {
+    restore();
+    run_steps(ctx, B, c, dt, 1, false);  // warm
+    restore();
+    check(hipEventRecord(ctx.start, ctx.stream), "event");
+    run_steps(ctx, B, c, dt, steps, false);
+    check(hipEventRecord(ctx.stop, ctx.stream), "event");
+    check(hipEventSynchronize(ctx.stop), "autotune");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+    const double t = ms * 1.0e-3 / steps;
+    if (seconds != nullptr) seconds[c] = t;
+    if (c == 0 || t < best) {
+      best = t;
+      choice = c;
+    }
+  }
+  restore();
+  check(hipMemsetAsync(B.flags.data(), 0, B.count * sizeof(uint32_t), ctx.stream), "clear flags");
+  check(hipStreamSynchronize(ctx.stream), "autotune");
+  B.shape = choice;
+  return choice;
+}
+
+void batch_shape(int index, int* threads, int* row, int* chunk) {
+  *threads = kShapes[index].threads;
+  *row = kShapes[index].row;
+  *chunk = kShapes[index].chunk;
+}
+
+int batch_shape_count() { return kShapeCount; }
+
+void batch_run(Context& ctx, float dt, uint32_t steps, bool graph, double* ledger,
+               double* seconds) {
+  BatchDevice& B = batch_of(ctx);
+  if (B.count == 0) throw Error(MSIM_INVALID_ARGUMENT, "no batch is uploaded");
+  if (!B.solved) throw Error(MSIM_INVALID_ARGUMENT, "solve the batch's flow first");
+  const int shape = shape_of(B);
+  check(hipEventRecord(ctx.start, ctx.stream), "event");
+  run_steps(ctx, B, shape, dt, steps, graph);
+  check(hipEventRecord(ctx.stop, ctx.stream), "event");
+  check(hipEventSynchronize(ctx.stop), "batch run");
+  float ms = 0.0f;
+  check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+  if (seconds != nullptr) *seconds = ms * 1.0e-3;
+  if (ledger == nullptr) return;
+  // Per-instance rows from the per-item partials, added in item order.
+  const auto& items = B.items[shape];
+  std::vector<double> parts(static_cast<size_t>(steps) * items.size() * 4);
+  std::vector<double> bb(B.count);
+  check(hipMemcpyAsync(parts.data(), B.partials.data(), parts.size() * sizeof(double),
+                       hipMemcpyDeviceToHost, ctx.stream),
+        "download partials");
+  check(hipMemcpyAsync(bb.data(), B.bnorm2.data(), B.count * sizeof(double),
+                       hipMemcpyDeviceToHost, ctx.stream),
+        "download norms");
+  check(hipStreamSynchronize(ctx.stream), "ledger");
+  std::fill(ledger, ledger + static_cast<size_t>(steps) * B.count * 4, 0.0);
+  for (uint32_t s = 0; s < steps; ++s) {
+    for (size_t j = 0; j < items.size(); ++j) {
+      const double* pj = &parts[(static_cast<size_t>(s) * items.size() + j) * 4];
+      double* row = ledger + (static_cast<size_t>(s) * B.count + items[j].inst) * 4;
+      row[0] += pj[1];
+      row[1] += pj[2];
+      row[2] += pj[3];
+      row[3] += pj[0];
+    }
+    for (uint32_t k = 0; k < B.count; ++k) {
+      const BatchDesc& d = B.desc[k];
+      const double volume = static_cast<double>(d.dx) * d.dy * d.dz;
+      double* row = ledger + (static_cast<size_t>(s) * B.count + k) * 4;
+      row[0] *= volume;
+      row[1] *= volume;
+      row[2] *= static_cast<double>(dt) * volume;
+      row[3] = bb[k] > 0.0 ? std::sqrt(row[3] / bb[k]) : 0.0;
+    }
+  }
+}
+
+void batch_flags(Context& ctx, uint32_t* flags) {
+  BatchDevice& B = batch_of(ctx);
+  if (B.count == 0) throw Error(MSIM_INVALID_ARGUMENT, "no batch is uploaded");
+  check(hipMemcpyAsync(flags, B.flags.data(), B.count * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       ctx.stream),
+        "download flags");
+  check(hipStreamSynchronize(ctx.stream), "flags");
+}
+
+void batch_download(Context& ctx, uint32_t index, double* pressure, float* x_faces,
+                    float* y_faces, float* levels) {
+  BatchDevice& B = batch_of(ctx);
+  if (index >= B.count) throw Error(MSIM_INVALID_ARGUMENT, "instance index out of range");
+  const BatchDesc& d = B.desc[index];
+  const size_t m = static_cast<size_t>(d.nx) * d.ny;
+  if (pressure != nullptr) {
+    check(hipMemcpyAsync(pressure, B.x64.data() + d.site, m * sizeof(double),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download pressure");
+  }
+  if (x_faces != nullptr) {
+    check(hipMemcpyAsync(x_faces, B.xf.data() + d.xface,
+                         static_cast<size_t>(d.nx + 1) * d.ny * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download faces");
+  }
+  if (y_faces != nullptr) {
+    check(hipMemcpyAsync(y_faces, B.yf.data() + d.yface,
+                         static_cast<size_t>(d.nx) * (d.ny + 1) * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download faces");
+  }
+  if (levels != nullptr) {
+    const float* src = (B.current == 0 ? B.la.data() : B.lb.data()) + d.site;
+    check(hipMemcpyAsync(levels, src, m * sizeof(float), hipMemcpyDeviceToHost, ctx.stream),
+          "download levels");
+  }
+  check(hipStreamSynchronize(ctx.stream), "batch download");
+}
+
+}  // namespace msim

@@ -657,3 +657,98 @@ double msim_hip_adr_seconds(msim_ctx* ctx) {
 }
 
 }  // extern "C"
+
+int msim_hip_batch_upload(msim_ctx* ctx, uint32_t count, const msim_batch_instance* instances,
+                          const msim_batch_params* params) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (instances == nullptr || params == nullptr) msim::invalid("arguments must not be null");
+    if (count == 0 || count > 65535) msim::invalid("a batch holds 1 to 65535 instances");
+    if (!(params->relative_tolerance > 0.0f) || params->max_iterations == 0) {
+      msim::invalid("the batch needs a positive tolerance and iteration cap");
+    }
+    if (!(params->monod_vmax >= 0.0f) || !(params->monod_k > 0.0f)) {
+      msim::invalid("Monod uptake needs vmax >= 0 and K > 0");
+    }
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < count; ++k) {
+      const msim_batch_instance& in = instances[k];
+      const std::string where = "batch instance " + std::to_string(k) + ": ";
+      if (in.nx < 2 || in.ny < 1) msim::invalid(where + "needs nx >= 2 and ny >= 1");
+      const float h[3] = {in.dx, in.dy, in.dz};
+      for (float v : h) {
+        if (!(v > 0.0f) || !std::isfinite(v)) msim::invalid(where + "spacing must be positive");
+      }
+      if (in.mobility == nullptr) msim::invalid(where + "mobility must not be null");
+      if (!std::isfinite(in.inlet_pressure) || !(in.diffusion >= 0.0f) ||
+          !std::isfinite(in.diffusion) || !(in.reservoir >= 0.0f)) {
+        msim::invalid(where + "inlet pressure, diffusion and reservoir must be finite (>= 0)");
+      }
+      const size_t n = static_cast<size_t>(in.nx) * in.ny;
+      for (size_t i = 0; i < n; ++i) {
+        if (!(in.mobility[i] >= 0.0f) || !std::isfinite(in.mobility[i])) {
+          msim::invalid(where + "mobility must be finite and non-negative");
+        }
+      }
+      total += n;
+    }
+    if (total > (uint64_t{1} << 31)) {
+      throw msim::Error(MSIM_OVERFLOW, "a batch is limited to 2^31 sites");
+    }
+    msim::batch_upload(c, count, instances, *params);
+  });
+}
+
+int msim_hip_batch_solve_flow(msim_ctx* ctx, uint32_t* iterations, float* residuals,
+                              uint32_t* flags) {
+  return msim::guarded(ctx, [&](Context& c) { msim::batch_solve_flow(c, iterations, residuals, flags); });
+}
+
+int msim_hip_batch_autotune(msim_ctx* ctx, float dt, uint32_t steps, int* shape,
+                            double* seconds) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!(dt >= 0.0f) || !std::isfinite(dt) || steps == 0) {
+      msim::invalid("autotuning needs dt >= 0 and steps > 0");
+    }
+    const int k = msim::batch_autotune(c, dt, steps, seconds);
+    if (shape != nullptr) *shape = k;
+  });
+}
+
+int msim_hip_batch_shapes(void) { return msim::batch_shape_count(); }
+
+int msim_hip_batch_shape(int index, uint32_t* threads, uint32_t* row_width,
+                         uint32_t* item_sites) {
+  if (index < 0 || index >= msim::batch_shape_count() || threads == nullptr ||
+      row_width == nullptr || item_sites == nullptr) {
+    return MSIM_INVALID_ARGUMENT;
+  }
+  int t = 0, r = 0, k = 0;
+  msim::batch_shape(index, &t, &r, &k);
+  *threads = static_cast<uint32_t>(t);
+  *row_width = static_cast<uint32_t>(r);
+  *item_sites = static_cast<uint32_t>(k);
+  return MSIM_OK;
+}
+
+int msim_hip_batch_flags(msim_ctx* ctx, uint32_t* flags) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (flags == nullptr) msim::invalid("flags must not be null");
+    msim::batch_flags(c, flags);
+  });
+}
+
+int msim_hip_batch_run(msim_ctx* ctx, float dt, uint32_t steps, uint32_t use_graph,
+                       double* ledger, double* seconds) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!(dt >= 0.0f) || !std::isfinite(dt)) msim::invalid("dt must be finite and >= 0");
+    if (steps == 0) msim::invalid("steps must be positive");
+    msim::batch_run(c, dt, steps, use_graph != 0, ledger, seconds);
+  });
+}
+
+int msim_hip_batch_download(msim_ctx* ctx, uint32_t index, double* pressure, float* x_faces,
+                            float* y_faces, float* levels) {
+  return msim::guarded(ctx, [&](Context& c) {
+    msim::batch_download(c, index, pressure, x_faces, y_faces, levels);
+  });
+}

@@ -853,6 +853,17 @@ uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t 
   return iterations;
 }
 
+void launch_assembly(hipStream_t stream, const uint8_t* solid, const float* mobility, float* cond,
+                     float* diag, float* rhs, float* bc, float* cxl, float* cyl, uint32_t nx,
+                     uint32_t ny, float dx, float dy, float dz, uint32_t axis) {
+  const size_t n = static_cast<size_t>(nx) * ny;
+  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
+  k_conductance<<<blocks, 256, 0, stream>>>(solid, mobility, cond, n, dz);
+  check(hipGetLastError(), "launch k_conductance");
+  k_assemble<<<blocks, 256, 0, stream>>>(cond, diag, rhs, bc, cxl, cyl, nx, ny, dx, dy, axis);
+  check(hipGetLastError(), "launch k_assemble");
+}
+
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
@@ -879,14 +890,9 @@ void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobilit
   check(hipMemcpyAsync(f.mobility.data(), mobility_host, n * sizeof(float), hipMemcpyHostToDevice,
                        ctx.stream),
         "upload mobility");
-  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
-  k_conductance<<<blocks, 256, 0, ctx.stream>>>(f.solid.data(), f.mobility.data(), f.cond.data(),
-                                                n, f.dz);
-  check(hipGetLastError(), "launch k_conductance");
-  k_assemble<<<blocks, 256, 0, ctx.stream>>>(f.cond.data(), f.diag.data(), f.rhs.data(),
-                                             f.bc.data(), f.cxl.data(), f.cyl.data(), f.nx, f.ny,
-                                             f.dx, f.dy, f.axis);
-  check(hipGetLastError(), "launch k_assemble");
+  launch_assembly(ctx.stream, f.solid.data(), f.mobility.data(), f.cond.data(), f.diag.data(),
+                  f.rhs.data(), f.bc.data(), f.cxl.data(), f.cyl.data(), f.nx, f.ny, f.dx, f.dy,
+                  f.dz, f.axis);
   check(hipStreamSynchronize(ctx.stream), "assemble");
   f.prepared = true;
   f.solved = false;

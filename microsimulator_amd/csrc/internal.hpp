@@ -204,6 +204,8 @@ struct LearnedDevice {
   DeviceBuffer<float> z, d, gz, grads, losses;
 };
 
+struct BatchDevice;  // batch.hip
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -214,6 +216,7 @@ struct Context {
   FlowDevice flow;
   AdrDevice adr;
   LearnedDevice learned;
+  std::shared_ptr<BatchDevice> batch;
 };
 
 // Summed event time and launch count of the two Krylov kernels of a solve.
@@ -226,6 +229,11 @@ struct KernelTimes {
 
 // flow.hip
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host);
+// Conductance and assembly of one grid already on the device (Dirichlet 1 at
+// the lower end of `axis`, 0 at the upper end).
+void launch_assembly(hipStream_t stream, const uint8_t* solid, const float* mobility, float* cond,
+                     float* diag, float* rhs, float* bc, float* cxl, float* cyl, uint32_t nx,
+                     uint32_t ny, float dx, float dy, float dz, uint32_t axis);
 uint32_t stream_blocks(size_t n);
 uint32_t krylov_blocks(size_t n);
 void launch_residual(Context& ctx, const double* x, double* out, int mode);
@@ -272,6 +280,19 @@ uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t 
                       bool* converged, double* relative, KernelTimes* times);
 void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report);
 void flow_apply(Context& ctx, const double* x, double* y);
+
+// batch.hip: the batched sweep (msim_hip_batch_*).
+void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
+                  const msim_batch_params& params);
+void batch_solve_flow(Context& ctx, uint32_t* iterations, float* residual, uint32_t* flags);
+int batch_autotune(Context& ctx, float dt, uint32_t steps, double* seconds);
+void batch_shape(int index, int* threads, int* row, int* chunk);
+int batch_shape_count();
+void batch_flags(Context& ctx, uint32_t* flags);
+void batch_run(Context& ctx, float dt, uint32_t steps, bool graph, double* ledger,
+               double* seconds);
+void batch_download(Context& ctx, uint32_t index, double* pressure, float* x_faces,
+                    float* y_faces, float* levels);
 
 // adr.hip
 void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,

@@ -332,9 +332,10 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   if (st->failure != 0 || st->halt != 0) return;
   bf16x4 frag[kFrags];
   load_frags(packed, frag);
-  // A contiguous run of tiles per workgroup (neighbours share L2).
+  // A contiguous run of tiles per workgroup, and a contiguous band of tiles
+  // per XCD, so tiles that share halo rows meet in one L2.
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
-  const uint32_t t0 = blockIdx.x * per, t1 = min(tiles, t0 + per);
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x) * per, t1 = min(tiles, t0 + per);
   double zr = 0.0, zq = 0.0;
   // Every global value the thread's pixels of tile t need, in one round of
   // loads: r, dinv, e_c, wx (lower, upper), wy (lower, upper), bc. The next

@@ -120,12 +120,16 @@ __device__ inline bool cycle_stopped(const CgState* st) {
 }
 
 // Pre-smooth from zero, residual, restriction: r_c = P^T (r - A omega D^-1 r).
+// One workgroup per tile, tiles in XCD-contiguous bands (t = (t / tiles_y,
+// t % tiles_y)).
 __global__ __launch_bounds__(kMgThreads) void k_mg_down(LevelView L, const float* __restrict__ r,
                                                         float* __restrict__ rc, uint32_t nxc,
-                                                        uint32_t nyc, const CgState* st) {
+                                                        uint32_t nyc, const CgState* st,
+                                                        uint32_t tiles_y) {
   __shared__ float s_x[kHX * kHY];
   if (cycle_stopped(st)) return;
-  const uint32_t fx0 = blockIdx.x * kTX, fy0 = blockIdx.y * kTY;
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t fx0 = (t / tiles_y) * kTX, fy0 = (t % tiles_y) * kTY;
   for (int idx = threadIdx.x; idx < kHX * kHY; idx += kMgThreads) {
     const int64_t fx = static_cast<int64_t>(fx0) + idx / kHY - 1;
     const int64_t fy = static_cast<int64_t>(fy0) + idx % kHY - 1;
@@ -164,8 +168,9 @@ struct UpDots {
 // Prolongation and post-smooth: x = omega D^-1 r + c P e_c,
 // out = x + omega D^-1 (r - A x). With `pre`, out = x instead (the cycle
 // before post-smoothing). Each workgroup walks a contiguous run of tiles
-// (tile t = (t / tiles_y, t % tiles_y)), so neighbouring tiles share L2 and
-// the flexible-CG dots leave one partial per workgroup.
+// (tile t = (t / tiles_y, t % tiles_y)) and each XCD a contiguous band, so
+// tiles that share halo rows meet in one L2; the flexible-CG dots leave one
+// partial per workgroup.
 __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* __restrict__ r,
                                                       const float* __restrict__ ec, uint32_t nyc,
                                                       float* __restrict__ out, const CgState* st,
@@ -175,8 +180,7 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* 
   __shared__ double s_red[kMgThreads / kWave];
   if (cycle_stopped(st)) return;
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
-  const uint32_t t0 = blockIdx.x * per, t1 = min(tiles, t0 + per);
-  const int lx = threadIdx.x / (kTY / 2), ly = threadIdx.x % (kTY / 2);
+  const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x) * per, t1 = min(tiles, t0 + per);
   double zr = 0.0, zq = 0.0;
   for (uint32_t t = t0; t < t1; ++t) {
     const uint32_t fx0 = (t / tiles_y) * kTX, fy0 = (t % tiles_y) * kTY;
@@ -192,23 +196,23 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* 
       s_x[idx] = v;
     }
     __syncthreads();
+    // One site per thread in row order: unit-stride loads and stores.
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const uint32_t fx = fx0 + 2 * lx + a, fy = fy0 + 2 * ly + b;
-        if (fx < L.nx && fy < L.ny) {
-          const size_t i = static_cast<size_t>(fx) * L.ny + fy;
-          const int hx = 2 * lx + a + 1, hy = 2 * ly + b + 1;
-          const float ri = r[i];
-          const float z = pre != 0 ? s_x[hx * kHY + hy]
-                                   : s_x[hx * kHY + hy] +
-                                         kOmega * L.dinv[i] * (ri - tile_ax(L, s_x, hx, hy, i, fx, fy));
-          out[i] = z;
-          if (dots.q != nullptr) {
-            zr += static_cast<double>(z) * static_cast<double>(ri);
-            zq += static_cast<double>(z) * static_cast<double>(dots.q[i]);
-          }
+    for (int k = 0; k < (kTX * kTY) / kMgThreads; ++k) {
+      const int idx = static_cast<int>(threadIdx.x) + k * kMgThreads;
+      const int lxs = idx / kTY, lys = idx % kTY;
+      const uint32_t fx = fx0 + lxs, fy = fy0 + lys;
+      if (fx < L.nx && fy < L.ny) {
+        const size_t i = static_cast<size_t>(fx) * L.ny + fy;
+        const int hx = lxs + 1, hy = lys + 1;
+        const float ri = r[i];
+        const float z = pre != 0 ? s_x[hx * kHY + hy]
+                                 : s_x[hx * kHY + hy] +
+                                       kOmega * L.dinv[i] * (ri - tile_ax(L, s_x, hx, hy, i, fx, fy));
+        out[i] = z;
+        if (dots.q != nullptr) {
+          zr += static_cast<double>(z) * static_cast<double>(ri);
+          zq += static_cast<double>(z) * static_cast<double>(dots.q[i]);
         }
       }
     }
@@ -787,8 +791,9 @@ void launch_cycle(Context& ctx, float* r, float* z, const float* q, const hipEve
     MgLevel& L = *mg.levels[l];
     MgLevel& C = *mg.levels[l + 1];
     const float* rl = l == 0 ? r : L.r.data();
-    k_mg_down<<<tile_grid(L), kMgThreads, 0, ctx.stream>>>(view(L), rl, C.r.data(), C.nx, C.ny,
-                                                           st);
+    const dim3 g = tile_grid(L);
+    k_mg_down<<<g.x * g.y, kMgThreads, 0, ctx.stream>>>(view(L), rl, C.r.data(), C.nx, C.ny, st,
+                                                        g.y);
     check(hipGetLastError(), "launch k_mg_down");
     if (l == 0 && probes != nullptr) check(hipEventRecord(probes[0], ctx.stream), "event");
   }

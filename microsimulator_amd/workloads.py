@@ -180,3 +180,33 @@ def config(name: str, seed: int | None = None) -> Workload:
         return pillar_array(4096, 2048, seed=5 if seed is None else seed, pillars=4000,
                             radius=(8, 20), blobs=500, depth=10.0)
     raise ValueError(f"unknown workload {name}")
+
+
+BATCH_SIZES = ((64, 64), (128, 96), (256, 128), (512, 256))
+
+
+def batch_instance(index: int, seed: int = 3):
+    """Instance `index` of the batched sweep: size drawn from BATCH_SIZES,
+    channel+trap (even draws) or pillar-array geometry scaled to the size,
+    inlet pressure U[0.5, 2] and solute diffusivity U[100, 1000], seeded."""
+    import microsimulator_amd as msa
+    rng = np.random.default_rng([seed, index])
+    nx, ny = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+    gseed = int(rng.integers(1 << 30))
+    if rng.random() < 0.5:
+        w = channel_trap(nx, ny, seed=gseed)
+    else:
+        area = nx * ny / (1024 * 512)
+        r_lo = max(3, int(round(8 * ny / 512)))
+        r_hi = max(r_lo + 1, int(round(15 * ny / 512)))
+        w = pillar_array(nx, ny, seed=gseed, pillars=max(2, int(round(300 * area))),
+                         radius=(r_lo, r_hi), blobs=max(2, int(round(40 * area))))
+    return msa.BatchInstance(nx=nx, ny=ny, mobility=w.mobility, obstacles=w.obstacles, phi=w.phi,
+                             inlet_pressure=float(rng.uniform(0.5, 2.0)),
+                             diffusion=float(rng.uniform(100.0, 1000.0)), reservoir=1.0,
+                             dx=w.dx, dy=w.dy, dz=w.dz)
+
+
+def batch_sweep(count: int = 256, seed: int = 3):
+    """The batched sweep: `count` seeded instances of uneven sizes."""
+    return [batch_instance(k, seed) for k in range(count)]
