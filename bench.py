@@ -57,8 +57,8 @@ BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (scaled by a 256 MiB copy's readings) on this workload; the commands and
 # figures are in profiles/krylov_kernels_4096x2048_traffic.md.
-TRAFFIC_MB = {"k_fcg_apply": 316.1, "k_mg_up": 370.7, "k_mg_down": 204.9, "k_fcg_update": 101.3,
-              "k_cg_apply<float>": 353.8}
+TRAFFIC_MB = {"k_learn_up": 499.1, "k_fcg_apply": 315.3, "k_mg_up": 410.6, "k_mg_down": 180.3,
+              "k_fcg_update": 100.9, "k_cg_apply<float>": 353.8}
 
 
 def apply_bytes(precision: int) -> int:
@@ -90,8 +90,9 @@ def cpu_baseline(w, fp64_iterations: int, threads: int, budget_s: float):
         ref = Reference()
         ref.set_threads(threads)
         system = ref.flow_system(grid, w.mobility, 0)
-        t1 = system.time_iterations(5)
-        k = int(max(10, min(2000, budget_s / max(t1 / 5, 1e-6))))
+        system.time_iterations(5)  # thread pool and pages warm
+        t1 = system.time_iterations(20)
+        k = int(max(10, min(8000, budget_s / max(t1 / 20, 1e-6))))
         t = system.time_iterations(k)
         per_iteration = t / k
         return dict(value=1.0 / (per_iteration * fp64_iterations), unit="solves/s",
@@ -141,6 +142,52 @@ def cpu_adr_baseline(w, faces, threads: int, steps: int):
                 sample=f"{steps} steps of advance_signal_grid_cpu, 3 solutes, in {secs:.2f} s")
 
 
+def batch_sweep_bench(msa, hip, steps: int):
+    """The batched sweep: 256 seeded instances drawn from 64x64, 128x96,
+    256x128 and 512x256, each solved (FP64 true residual 1e-8), then `steps`
+    graph-captured flow-check + velocity + FP32 transport steps."""
+    from microsimulator_amd.workloads import batch_sweep
+    inst = batch_sweep(256)
+    fluid = sum(int((i.obstacles == 0).sum()) for i in inst)
+    sites = sum(i.sites for i in inst)
+    hip.batch_upload(inst, relative_tolerance=1e-8, max_iterations=60000)
+    t0 = time.perf_counter()
+    it, res, fl = hip.batch_solve_flow()
+    flow_s = time.perf_counter() - t0
+    worst = 0.0
+    for k, i in enumerate(inst):
+        f = hip.batch_download(k)
+        speed = max(float(np.abs(f["x_faces"]).max()), float(np.abs(f["y_faces"]).max()))
+        worst = max(worst, 2.0 * i.diffusion * (1 / i.dx ** 2 + 1 / i.dy ** 2) + 2.0 * speed / i.dx
+                    + float(i.phi.max()) / 0.1)
+    dt = 0.9 / worst
+    shape, secs = hip.batch_autotune(dt, steps=3)
+    hip.batch_run(dt, steps, graph=True, ledger=False)  # capture and warm
+    led, run_s = hip.batch_run(dt, steps, graph=True)
+    _, eager_s = hip.batch_run(dt, steps, graph=False, ledger=False)
+    flags = hip.batch_flags()
+    log(f"batch: {len(inst)} instances, {sites} sites, flow {flow_s:.3f} s (max {int(it.max())} "
+        f"FP32 iterations), shape {hip.batch_shapes()[shape]}, {steps} graph steps "
+        f"{run_s * 1e3:.2f} ms ({fluid * steps / run_s / 1e9:.2f} Gcell-upd/s), eager "
+        f"{eager_s * 1e3:.2f} ms")
+    return {
+        "instances": len(inst), "sites": sites, "fluid_sites": fluid,
+        "sizes": sorted({f"{i.nx}x{i.ny}" for i in inst}),
+        "flow_solve_seconds": round(flow_s, 4), "flow_max_fp32_iterations": int(it.max()),
+        "flow_max_true_relative_residual": float(res.max()),
+        "autotuned_shape": {"threads": hip.batch_shapes()[shape][0],
+                            "row_width": hip.batch_shapes()[shape][1],
+                            "sites_per_item": hip.batch_shapes()[shape][2]},
+        "autotune_step_us": [round(x * 1e6, 1) for x in secs],
+        "steps": steps, "dt": dt,
+        "graph_ms_per_step": round(run_s * 1e3 / steps, 4),
+        "eager_ms_per_step": round(eager_s * 1e3 / steps, 4),
+        "gcell_upd_per_s": round(fluid * steps / run_s / 1e9, 3),
+        "worst_step_pressure_residual": float(led[:, :, 3].max()),
+        "flags": int(flags.max()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +203,8 @@ def main():
                     help="profiling run: only N iterations of each solver form, then exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="profiling calibration: copy this many MiB device to device, 3 times")
+    ap.add_argument("--batch-steps", type=int, default=50,
+                    help="graph-captured steps of the 256-instance batched sweep (0: skip)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -371,6 +420,8 @@ def main():
     }
     if mfma is not None:
         line["roofline_mfma"] = mfma
+    if args.batch_steps > 0:
+        line["batch"] = batch_sweep_bench(msa, hip, args.batch_steps)
     if args.solver == "learned":
         with np.load(learned.WEIGHTS_FILE, allow_pickle=False) as f:
             meta = json.loads(str(f["meta"]))
