@@ -1,16 +1,16 @@
 // Explicit advection-diffusion-reaction step on the device, one layer.
 //
-// Two kernels share one site-per-thread structure (each thread advances every
-// signal of its site, so the face velocities, mask and occupancy are read once
-// for all signals):
-//  * k_adr_f32: FP32 levels, the reference's per-site rate in its own float
-//    order (cpp/core/signals.cpp:1145-1291 transport, 305-320 affine
-//    reaction, 1109-1134 forward Euler), Monod uptake as the affine loss
+// One marching kernel, k_adr_step, in two arithmetic forms: a block owns a
+// strip of columns and walks x, each thread advancing every signal of its
+// site, so levels, faces, mask and occupancy are read from memory once.
+//  * FP32 levels: the reference's per-site rate in its own float order
+//    (cpp/core/signals.cpp:1145-1291 transport, 305-320 affine reaction,
+//    1109-1134 forward Euler), Monod uptake as the affine loss
 //    vmax*phi/(K+c) it equals under forward Euler.
-//  * k_adr_f64: FP64 levels in face-flux form: every face flux is one
-//    expression of the two sites it joins, so what leaves one site enters the
-//    other bit for bit; the step's mass, boundary supply and uptake are
-//    tallied in FP64 for the conservation ledger.
+//  * FP64 levels in face-flux form: every face flux is one expression of the
+//    two sites it joins, so what leaves one site enters the other bit for
+//    bit; the step's mass, boundary supply and uptake are tallied in FP64 for
+//    the conservation ledger.
 // A site whose update is negative or non-finite raises a flag (the reference
 // throws, signals.cpp:1124-1130); the step is then refused as a whole.
 #include <hip/hip_runtime.h>
@@ -30,41 +30,6 @@ namespace {
 constexpr int kAdrBlock = 256;
 constexpr uint32_t kFlagNegative = 1;
 
-// One FP32 forward-Euler step for S signals; one site per thread.
-template <int S>
-__global__ __launch_bounds__(kAdrBlock) void k_adr_f32(
-    AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
-    const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
-    const float* __restrict__ loss, const float* __restrict__ in, float* __restrict__ out, float dt,
-    uint32_t* flags, uint32_t step) {
-  if (flags[0] != 0) return;  // an earlier step failed: the run stops there
-  const size_t n = static_cast<size_t>(P.nx) * P.ny;
-  const size_t i = static_cast<size_t>(blockIdx.x) * kAdrBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t x = static_cast<uint32_t>(i / P.ny), y = static_cast<uint32_t>(i % P.ny);
-  if (solid[i] != 0) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) out[s * n + i] = in[s * n + i];
-    return;
-  }
-  bool bad = false;
-  SiteGeometry g = geometry(P, solid, xf, yf, x, y, 0);
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, s);
-    const size_t index = s * n + i;
-    const float candidate = fe_candidate_f32(P, g, in + s * n, x, y, s, dt,
-                                             P.has_affine != 0 ? source[index] : 0.0f,
-                                             P.has_affine != 0 ? loss[index] : 0.0f, phi);
-    if (!isfinite(candidate) || candidate < 0.0f) bad = true;
-    out[index] = candidate;
-  }
-  if (bad) {
-    atomicOr(&flags[0], kFlagNegative);
-    atomicMin(&flags[1], step);
-  }
-}
-
 // The ledger of a step lives as per-block partial sums (mass after, boundary
 // supply rate, uptake rate, for each signal) in two generations by step
 // parity. Blocks j < 3S of the next step add partial j of the previous step
@@ -78,7 +43,7 @@ struct LedgerParts {
 
 template <int S, int NT>
 __device__ inline void fold_ledger(const AdrParams& P, LedgerParts L, uint32_t step, float dt,
-                                   uint32_t j, double* lds, double* bcast) {
+                                   uint32_t j, double* lds) {
   // Partial j of `step` into ledger row `step`.
   const double* v =
       L.parts + (static_cast<size_t>(step & 1) * (3 * kMaxSignals) + j) * L.blocks;
@@ -92,109 +57,255 @@ __device__ inline void fold_ledger(const AdrParams& P, LedgerParts L, uint32_t s
     double* row = L.ledger + (static_cast<size_t>(step) * S + s) * 4;
     row[1 + what] = what == 0 ? total * volume : static_cast<double>(dt) * total * volume;
   }
-  (void)bcast;
 }
 
-// One FP64 step in face-flux form for S signals, grid-stride over sites.
-template <int S>
-__global__ __launch_bounds__(kAdrBlock) void k_adr_f64(
+// Sums of K values over a block in one pass: every wave reduces its K values,
+// then thread 0 adds the waves in order. Only thread 0's result is meaningful.
+template <int NT, int K>
+__device__ inline void block_sums(double (&v)[K], double* lds) {
+  constexpr int kWaves = NT / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[k * kWaves + wave] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double t = 0.0;
+      for (int w = 0; w < kWaves; ++w) t += lds[k * kWaves + w];
+      v[k] = t;
+    }
+  }
+}
+
+// What one thread holds of one x-row of its column: the S levels and the
+// mask; for sites the block updates also the faces, occupancy and (with
+// affine reactions) the rates.
+template <typename T, int S, bool kAffine>
+struct AdrRow {
+  T c[S];
+  float src[kAffine ? S : 1], los[kAffine ? S : 1];
+  float xf, ylo, yhi, phi;
+  bool solid;
+};
+
+// One forward-Euler step for S signals, marching: a block owns a strip of
+// TY - 2 columns and `rows` x-rows and walks x, so each level is read from
+// memory once and its x neighbours come from registers, its y neighbours from
+// a row in LDS. The strip's first and last threads only load the columns
+// just outside it (or, on a periodic y boundary, the partner column), so no
+// thread holds more than its own column. The next row's loads are issued one
+// row ahead.
+//  * T = float: the reference's per-site rate in its float order
+//    (fe_update_f32), so every site matches its CPU step bit for bit.
+//  * T = double: face-flux form, each face flux one expression of the two
+//    sites it joins, so what leaves one site enters the other bit for bit;
+//    the step's mass, boundary supply and uptake go to the ledger partials.
+// A site whose update is negative or non-finite raises a flag (the reference
+// throws, signals.cpp:1124-1130); the step is then refused as a whole.
+template <typename T, int S, int TY, bool kAffine>
+__global__ __launch_bounds__(TY) void k_adr_step(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
-    const float* __restrict__ loss, const double* __restrict__ in, double* __restrict__ out,
-    float dt, uint32_t* flags, uint32_t step, LedgerParts L) {
-  __shared__ double s_red[kAdrBlock / kWave];
-  __shared__ double s_b;
-  if (flags[0] != 0) return;
-  if (step > 0 && blockIdx.x < 3 * S) {
-    fold_ledger<S, kAdrBlock>(P, L, step - 1, dt, blockIdx.x, s_red, &s_b);
+    const float* __restrict__ loss, const T* __restrict__ in, T* __restrict__ out, float dt,
+    uint32_t* flags, uint32_t step, uint32_t rows, LedgerParts L) {
+  constexpr bool kF64 = sizeof(T) == 8;
+  constexpr int kWidth = TY - 2;
+  using Row = AdrRow<T, S, kAffine>;
+  __shared__ T s_c[2][S][TY];
+  __shared__ uint8_t s_s[2][TY];
+  __shared__ double s_red[(kF64 ? 3 * S : 1) * (TY / kWave)];
+  if (flags[0] != 0) return;  // an earlier step failed: the run stops there
+  if constexpr (kF64) {
+    if (step > 0 && blockIdx.x < 3 * S) fold_ledger<S, TY>(P, L, step - 1, dt, blockIdx.x, s_red);
   }
-  const size_t n = static_cast<size_t>(P.nx) * P.ny;
-  double acc[3 * S];
-#pragma unroll
-  for (int k = 0; k < 3 * S; ++k) acc[k] = 0.0;
-  bool bad = false;
-  const double ih[2] = {1.0 / static_cast<double>(P.dx), 1.0 / static_cast<double>(P.dy)};
-  const size_t grid_stride = static_cast<size_t>(gridDim.x) * kAdrBlock;
-  for (size_t i = static_cast<size_t>(blockIdx.x) * kAdrBlock + threadIdx.x; i < n;
-       i += grid_stride) {
-    const uint32_t x = static_cast<uint32_t>(i / P.ny), y = static_cast<uint32_t>(i % P.ny);
-    if (solid[i] != 0) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) out[s * n + i] = in[s * n + i];
-      continue;
+  const uint32_t nx = P.nx, ny = P.ny;
+  const size_t n = static_cast<size_t>(nx) * ny;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t strips = (ny + kWidth - 1) / kWidth;
+  const uint32_t x0 = (tile / strips) * rows, x1 = min(nx, x0 + rows);
+  const uint32_t t = threadIdx.x;
+  // Column of this thread: -1 and ny stand for the periodic partner columns.
+  const int col = static_cast<int>((tile % strips) * kWidth + t) - 1;
+  int src_col = col;
+  if (col < 0) src_col = P.kind[2] == MSIM_PERIODIC ? static_cast<int>(ny) - 1 : -1;
+  if (col >= static_cast<int>(ny)) {
+    src_col = col == static_cast<int>(ny) && P.kind[3] == MSIM_PERIODIC ? 0 : -1;
+  }
+  const bool loads = src_col >= 0;
+  const bool owns = t >= 1 && t <= kWidth && col >= 0 && col < static_cast<int>(ny);
+  const uint32_t y = static_cast<uint32_t>(src_col);
+  const bool field = P.has_field != 0, monod = P.has_monod != 0;
+  // Row xx of the walk (xx in [-1, nx]); rows -1 and nx are the periodic
+  // partners when the x boundary is periodic and absent otherwise. `full`
+  // rows are the ones the block updates.
+  auto load = [&](int xx, bool full) {
+    Row w{};
+    int row = xx;
+    if (xx < 0) row = P.kind[0] == MSIM_PERIODIC ? static_cast<int>(nx) - 1 : -1;
+    if (xx >= static_cast<int>(nx)) row = P.kind[1] == MSIM_PERIODIC ? 0 : -1;
+    if (owns && field && xx >= 0 && xx <= static_cast<int>(nx)) {
+      w.xf = xf[static_cast<size_t>(xx) * ny + y];  // lower x face of row xx
     }
-    const uint32_t coord[2] = {x, y};
-    const uint32_t dims[2] = {P.nx, P.ny};
-    const size_t stride[2] = {P.ny, 1};
-    SiteGeometry g = geometry(P, solid, xf, yf, x, y, 0);
+    if (row < 0 || !loads) return w;
+    const size_t i = static_cast<size_t>(row) * ny + y;
+    w.solid = solid[i] != 0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (P.has_field == 0 && s > 0) g = geometry(P, solid, xf, yf, x, y, s);
-      const double* lv = in + s * n;
-      const double c = lv[i];
-      const double D = static_cast<double>(P.diffusion[s]);
-      double rate = 0.0, supply = 0.0;
+    for (int s = 0; s < S; ++s) w.c[s] = in[s * n + i];
+    if (!full || !owns) return w;
+    if (field) {
+      const size_t yfi = static_cast<size_t>(row) * (ny + 1) + y;
+      w.ylo = yf[yfi];
+      w.yhi = yf[yfi + 1];
+    }
+    if (monod) w.phi = phi[i];
+    if constexpr (kAffine) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        if (dims[a] == 1) continue;
-        double fl = 0.0, fu = 0.0;
-        const double ul = static_cast<double>(g.ulo[a]), uu = static_cast<double>(g.uup[a]);
-        // lower face
-        if (coord[a] == 0) {
-          if (P.kind[2 * a] == MSIM_FIXED) {
-            const double cb = static_cast<double>(P.boundary[s][2 * a]);
-            fl = (ul >= 0.0 ? ul * cb : ul * c) - D * (c - cb) * ih[a];
-            supply += fl * ih[a];
+      for (int s = 0; s < S; ++s) {
+        w.src[s] = source[s * n + i];
+        w.los[s] = loss[s * n + i];
+      }
+    }
+    return w;
+  };
+  double acc[kF64 ? 3 * S : 1];
+#pragma unroll
+  for (int k = 0; k < (kF64 ? 3 * S : 1); ++k) acc[k] = 0.0;
+  bool bad = false;
+  if (x0 < x1) {
+    const int xa = static_cast<int>(x0), xb = static_cast<int>(x1);
+    Row prev = load(xa - 1, false);
+    Row cur = load(xa, true);
+    Row next = load(xa + 1, xa + 1 < xb);
+    for (int xx = xa; xx < xb; ++xx) {
+      const Row ahead = xx + 2 <= xb ? load(xx + 2, xx + 2 < xb) : Row{};
+      const int buf = xx & 1;
+#pragma unroll
+      for (int s = 0; s < S; ++s) s_c[buf][s][t] = cur.c[s];
+      s_s[buf][t] = cur.solid ? 1 : 0;
+      __syncthreads();
+      if (owns) {
+        const size_t i = static_cast<size_t>(xx) * ny + y;
+        if (cur.solid) {
+#pragma unroll
+          for (int s = 0; s < S; ++s) out[s * n + i] = cur.c[s];
+        } else {
+          const bool xlo_edge = xx == 0, xhi_edge = xx + 1 == static_cast<int>(nx);
+          const bool ylo_edge = y == 0, yhi_edge = y + 1 == ny;
+          const bool ylo_solid = s_s[buf][t - 1] != 0, yhi_solid = s_s[buf][t + 1] != 0;
+          // face_closure (signals.cpp:213-261).
+          SiteGeometry g;
+          g.clo[0] = xlo_edge ? (P.kind[0] == MSIM_NO_FLUX ||
+                                 (P.kind[0] == MSIM_PERIODIC && prev.solid))
+                              : prev.solid;
+          g.cup[0] = xhi_edge ? (P.kind[1] == MSIM_NO_FLUX ||
+                                 (P.kind[1] == MSIM_PERIODIC && next.solid))
+                              : next.solid;
+          g.clo[1] = ylo_edge ? (P.kind[2] == MSIM_NO_FLUX ||
+                                 (P.kind[2] == MSIM_PERIODIC && ylo_solid))
+                              : ylo_solid;
+          g.cup[1] = yhi_edge ? (P.kind[3] == MSIM_NO_FLUX ||
+                                 (P.kind[3] == MSIM_PERIODIC && yhi_solid))
+                              : yhi_solid;
+#pragma unroll
+          for (int s = 0; s < S; ++s) {
+            if (field) {
+              g.ulo[0] = cur.xf;
+              g.uup[0] = next.xf;
+              g.ulo[1] = cur.ylo;
+              g.uup[1] = cur.yhi;
+            } else {
+              g.ulo[0] = g.uup[0] = P.advection[s][0];
+              g.ulo[1] = g.uup[1] = P.advection[s][1];
+            }
+            const T c = cur.c[s];
+            const T ylo_v = s_c[buf][s][t - 1];
+            const T yhi_v = s_c[buf][s][t + 1];
+            const float src = kAffine ? cur.src[kAffine ? s : 0] : 0.0f;
+            const float los = kAffine ? cur.los[kAffine ? s : 0] : 0.0f;
+            if constexpr (!kF64) {
+              const float l0 = xlo_edge ? exterior(P, 0, s, c, prev.c[s]) : prev.c[s];
+              const float u0 = xhi_edge ? exterior(P, 1, s, c, next.c[s]) : next.c[s];
+              const float l1 = ylo_edge ? exterior(P, 2, s, c, ylo_v) : ylo_v;
+              const float u1 = yhi_edge ? exterior(P, 3, s, c, yhi_v) : yhi_v;
+              const float v = fe_update_f32(P, g, c, l0, u0, l1, u1, s, dt, src, los, cur.phi);
+              if (!isfinite(v) || v < 0.0f) bad = true;
+              out[s * n + i] = v;
+            } else {
+              const double ih[2] = {1.0 / static_cast<double>(P.dx),
+                                    1.0 / static_cast<double>(P.dy)};
+              const uint32_t dims[2] = {nx, ny};
+              const bool lo_edge[2] = {xlo_edge, ylo_edge}, hi_edge[2] = {xhi_edge, yhi_edge};
+              const double nlo[2] = {prev.c[s], ylo_v}, nhi[2] = {next.c[s], yhi_v};
+              const double D = static_cast<double>(P.diffusion[s]);
+              double rate = 0.0, supply = 0.0;
+#pragma unroll
+              for (int a = 0; a < 2; ++a) {
+                if (dims[a] == 1) continue;
+                double fl = 0.0, fu = 0.0;
+                const double ul = static_cast<double>(g.ulo[a]);
+                const double uu = static_cast<double>(g.uup[a]);
+                if (lo_edge[a] && P.kind[2 * a] == MSIM_FIXED) {
+                  const double cb = static_cast<double>(P.boundary[s][2 * a]);
+                  fl = (ul >= 0.0 ? ul * cb : ul * c) - D * (c - cb) * ih[a];
+                  supply += fl * ih[a];
+                } else if (!g.clo[a]) {
+                  const double cl = nlo[a];
+                  fl = (ul >= 0.0 ? ul * cl : ul * c) - D * (c - cl) * ih[a];
+                }
+                if (hi_edge[a] && P.kind[2 * a + 1] == MSIM_FIXED) {
+                  const double cb = static_cast<double>(P.boundary[s][2 * a + 1]);
+                  fu = (uu >= 0.0 ? uu * c : uu * cb) - D * (cb - c) * ih[a];
+                  supply -= fu * ih[a];
+                } else if (!g.cup[a]) {
+                  const double cu = nhi[a];
+                  fu = (uu >= 0.0 ? uu * c : uu * cu) - D * (cu - c) * ih[a];
+                }
+                rate -= (fu - fl) * ih[a];
+              }
+              double lossr = 0.0;
+              if constexpr (kAffine) {
+                rate += static_cast<double>(src);
+                supply += static_cast<double>(src);
+                lossr += static_cast<double>(los);
+              }
+              if (monod) {
+                lossr += static_cast<double>(P.vmax[s]) * static_cast<double>(cur.phi) /
+                         (static_cast<double>(P.monod_k[s]) + c);
+              }
+              rate -= lossr * c;
+              const double v = c + static_cast<double>(dt) * rate;
+              if (!isfinite(v) || v < 0.0) bad = true;
+              out[s * n + i] = v;
+              acc[3 * s + 0] += v;
+              acc[3 * s + 1] += supply;
+              acc[3 * s + 2] += lossr * c;
+            }
           }
-        } else if (!g.clo[a]) {
-          const double cl = lv[i - stride[a]];
-          fl = (ul >= 0.0 ? ul * cl : ul * c) - D * (c - cl) * ih[a];
         }
-        // upper face
-        if (coord[a] + 1 == dims[a]) {
-          if (P.kind[2 * a + 1] == MSIM_FIXED) {
-            const double cb = static_cast<double>(P.boundary[s][2 * a + 1]);
-            fu = (uu >= 0.0 ? uu * c : uu * cb) - D * (cb - c) * ih[a];
-            supply -= fu * ih[a];
-          }
-        } else if (!g.cup[a]) {
-          const double cu = lv[i + stride[a]];
-          fu = (uu >= 0.0 ? uu * c : uu * cu) - D * (cu - c) * ih[a];
-        }
-        rate -= (fu - fl) * ih[a];
       }
-      double lossr = 0.0;
-      const size_t index = s * n + i;
-      if (P.has_affine != 0) {
-        const double src = static_cast<double>(source[index]);
-        rate += src;
-        supply += src;
-        lossr += static_cast<double>(loss[index]);
-      }
-      if (P.has_monod != 0) {
-        lossr += static_cast<double>(P.vmax[s]) * static_cast<double>(phi[i]) /
-                 (static_cast<double>(P.monod_k[s]) + c);
-      }
-      rate -= lossr * c;
-      const double next = c + static_cast<double>(dt) * rate;
-      if (!isfinite(next) || next < 0.0) bad = true;
-      out[index] = next;
-      acc[3 * s + 0] += next;
-      acc[3 * s + 1] += supply;
-      acc[3 * s + 2] += lossr * c;
+      prev = cur;
+      cur = next;
+      next = ahead;
     }
   }
   if (bad) {
     atomicOr(&flags[0], kFlagNegative);
     atomicMin(&flags[1], step);
   }
-  // Solid sites hold their (zero) level; the mass counts only what moved.
-#pragma unroll
-  for (int k = 0; k < 3 * S; ++k) {
-    const double part = block_sum<kAdrBlock>(acc[k], s_red);
+  if constexpr (kF64) {
+    // Solid sites hold their (zero) level; the mass counts only what moved.
+    block_sums<TY, 3 * S>(acc, s_red);
     if (threadIdx.x == 0) {
-      L.parts[(static_cast<size_t>(step & 1) * (3 * kMaxSignals) + k) * L.blocks + blockIdx.x] =
-          part;
+#pragma unroll
+      for (int k = 0; k < 3 * S; ++k) {
+        L.parts[(static_cast<size_t>(step & 1) * (3 * kMaxSignals) + k) * L.blocks + blockIdx.x] =
+            acc[k];
+      }
     }
   }
 }
@@ -205,9 +316,8 @@ __global__ __launch_bounds__(kAdrBlock) void k_adr_ledger_tail(AdrParams P, Ledg
                                                                uint32_t step, float dt,
                                                                const uint32_t* flags) {
   __shared__ double s_red[kAdrBlock / kWave];
-  __shared__ double s_b;
   if (flags[0] != 0) return;
-  fold_ledger<S, kAdrBlock>(P, L, step, dt, blockIdx.x, s_red, &s_b);
+  fold_ledger<S, kAdrBlock>(P, L, step, dt, blockIdx.x, s_red);
 }
 
 // Sum of levels per signal (FP64), and the greatest outflow rate over fluid
@@ -266,6 +376,35 @@ __global__ __launch_bounds__(kAdrBlock) void k_adr_survey(
   }
 }
 
+// Launch shape of the marching step: TY threads per block (64, 128 or 256)
+// for strips of TY - 2 columns, and 16 x-rows per block (fewer on small grids,
+// for about 2048 blocks): on a 4096 x 2048 grid, 8 to 32 rows per block give
+// the same speed within 10%, 16 the best for FP64 levels.
+struct AdrShape {
+  int ty;
+  uint32_t rows, blocks;
+};
+
+AdrShape adr_shape(uint32_t nx, uint32_t ny) {
+  AdrShape s{};
+  // The strip width (TY - 2 columns) that covers the grid's width with the
+  // fewest threads, the wider on a tie.
+  uint64_t best = ~0ull;
+  for (int ty : {256, 128, 64}) {
+    const uint64_t threads = static_cast<uint64_t>((ny + ty - 3) / (ty - 2)) * ty;
+    if (threads < best) {
+      best = threads;
+      s.ty = ty;
+    }
+  }
+  const uint32_t strips = (ny + s.ty - 3) / (s.ty - 2);
+  uint32_t rows = 16;
+  while (rows > 4 && static_cast<uint64_t>(strips) * ((nx + rows - 1) / rows) < 2048) rows /= 2;
+  s.rows = rows;
+  s.blocks = strips * ((nx + rows - 1) / rows);
+  return s;
+}
+
 uint32_t site_blocks(size_t n) { return static_cast<uint32_t>((n + kAdrBlock - 1) / kAdrBlock); }
 
 }  // namespace
@@ -307,8 +446,10 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.flags.reserve(2);
   a.state.reserve(1);
   a.mass.reserve(kMaxSignals);
-  a.partials.reserve(static_cast<size_t>(3 * kMaxSignals) * 2 * 1024 +
-                     static_cast<size_t>(kMaxSignals + 1) * 2048);
+  a.partials.reserve(std::max<size_t>(
+      static_cast<size_t>(3 * kMaxSignals) * 2 *
+          std::max<uint32_t>(adr_shape(P.nx, P.ny).blocks, 3 * kMaxSignals),
+      static_cast<size_t>(kMaxSignals + 1) * 2048));
   if (!same_spec) {
     a.solid.reserve(n);
     if (grid.grid.obstacles != nullptr) {
@@ -383,27 +524,47 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.uploaded = true;
 }
 
+template <typename T, int S, int TY>
+void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
+                 const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L) {
+  auto& a = ctx.adr;
+  if (a.params.has_affine != 0) {
+    k_adr_step<T, S, TY, true><<<sh.blocks, TY, 0, ctx.stream>>>(
+        a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
+        reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
+        sh.rows, L);
+  } else {
+    k_adr_step<T, S, TY, false><<<sh.blocks, TY, 0, ctx.stream>>>(
+        a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
+        reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
+        sh.rows, L);
+  }
+}
+
 template <int S>
 void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf) {
   auto& a = ctx.adr;
   const auto& P = a.params;
-  const size_t n = a.sites();
-  const uint32_t blocks = site_blocks(n);
-  const uint32_t ledger_blocks = std::min<uint32_t>(blocks, 1024);
-  LedgerParts L{a.partials.data(), a.ledger.data(), ledger_blocks};
+  AdrShape sh = adr_shape(P.nx, P.ny);
+  // Blocks j < 3S fold the previous step's ledger partial j: launch at least
+  // that many (blocks past the grid's strips only add zeros).
+  sh.blocks = std::max<uint32_t>(sh.blocks, 3 * S);
+  LedgerParts L{a.partials.data(), a.ledger.data(), sh.blocks};
   for (uint32_t k = 0; k < steps; ++k) {
-    uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
+    const uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
     uint8_t* dst = a.current == 0 ? a.levels_b.data() : a.levels_a.data();
     if (a.precision == 64) {
-      k_adr_f64<S><<<ledger_blocks, kAdrBlock, 0, ctx.stream>>>(
-          P, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
-          reinterpret_cast<const double*>(src), reinterpret_cast<double*>(dst), dt,
-          a.flags.data(), k, L);
+      switch (sh.ty) {
+        case 64: launch_step<double, S, 64>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
+        case 128: launch_step<double, S, 128>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
+        default: launch_step<double, S, 256>(ctx, sh, xf, yf, src, dst, dt, k, L);
+      }
     } else {
-      k_adr_f32<S><<<blocks, kAdrBlock, 0, ctx.stream>>>(
-          P, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
-          reinterpret_cast<const float*>(src), reinterpret_cast<float*>(dst), dt, a.flags.data(),
-          k);
+      switch (sh.ty) {
+        case 64: launch_step<float, S, 64>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
+        case 128: launch_step<float, S, 128>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
+        default: launch_step<float, S, 256>(ctx, sh, xf, yf, src, dst, dt, k, L);
+      }
     }
     check(hipGetLastError(), "launch transport step");
     a.current ^= 1;

@@ -72,27 +72,19 @@ __device__ inline SiteGeometry geometry(const AdrParams& P, const uint8_t* __res
   return g;
 }
 
-// Forward-Euler candidate of signal s at site (x, y): the reference's
-// per-site rate in its own float order (cpp/core/signals.cpp:1145-1291
-// transport, 305-320 affine reaction, 1109-1134 forward Euler), with Monod
-// uptake as the affine loss vmax*phi/(K+c) it equals under forward Euler.
-// lv: this signal's levels; source/loss: its affine rates at the site.
-__device__ inline float fe_candidate_f32(const AdrParams& P, const SiteGeometry& g,
-                                         const float* __restrict__ lv, uint32_t x, uint32_t y,
-                                         int s, float dt, float source, float loss,
-                                         const float* __restrict__ phi) {
-  const size_t i = static_cast<size_t>(x) * P.ny + y;
+// Forward-Euler update of signal s at one site from its neighbour values:
+// the reference's per-site rate in its own float order
+// (cpp/core/signals.cpp:1145-1291 transport, 305-320 affine reaction,
+// 1109-1134 forward Euler), with Monod uptake as the affine loss
+// vmax*phi/(K+c) it equals under forward Euler. lower/upper hold the x and y
+// neighbour values with exterior values already resolved (exterior()); a
+// closed face replaces its neighbour by the site's own value here.
+__device__ inline float fe_update_f32(const AdrParams& P, const SiteGeometry& g, float current,
+                                      float lower0, float upper0, float lower1, float upper1,
+                                      int s, float dt, float source, float loss, float phi) {
   const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
   const uint32_t dims[2] = {P.nx, P.ny};
-  const float current = lv[i];
-  float lower[2], upper[2];
-  lower[0] = x == 0 ? exterior(P, 0, s, current, lv[static_cast<size_t>(P.nx - 1) * P.ny + y])
-                    : lv[i - P.ny];
-  upper[0] = x + 1 == P.nx ? exterior(P, 1, s, current, lv[y]) : lv[i + P.ny];
-  lower[1] = y == 0 ? exterior(P, 2, s, current, lv[static_cast<size_t>(x) * P.ny + P.ny - 1])
-                    : lv[i - 1];
-  upper[1] = y + 1 == P.ny ? exterior(P, 3, s, current, lv[static_cast<size_t>(x) * P.ny])
-                           : lv[i + 1];
+  float lower[2] = {lower0, lower1}, upper[2] = {upper0, upper1};
   const float diffusion = P.diffusion[s];
   float rate = 0.0f;
 #pragma unroll
@@ -111,10 +103,31 @@ __device__ inline float fe_candidate_f32(const AdrParams& P, const SiteGeometry&
   }
   if (P.has_affine != 0) rate += source - (loss * current);
   if (P.has_monod != 0) {
-    const float uptake = P.vmax[s] * phi[i] / (P.monod_k[s] + current);
+    const float uptake = P.vmax[s] * phi / (P.monod_k[s] + current);
     rate += 0.0f - (uptake * current);
   }
   return current + (dt * rate);
+}
+
+// The same update reading the neighbours from the level array lv (one
+// signal's levels) at site (x, y).
+__device__ inline float fe_candidate_f32(const AdrParams& P, const SiteGeometry& g,
+                                         const float* __restrict__ lv, uint32_t x, uint32_t y,
+                                         int s, float dt, float source, float loss,
+                                         const float* __restrict__ phi) {
+  const size_t i = static_cast<size_t>(x) * P.ny + y;
+  const float current = lv[i];
+  const float lower0 =
+      x == 0 ? exterior(P, 0, s, current, lv[static_cast<size_t>(P.nx - 1) * P.ny + y])
+             : lv[i - P.ny];
+  const float upper0 = x + 1 == P.nx ? exterior(P, 1, s, current, lv[y]) : lv[i + P.ny];
+  const float lower1 =
+      y == 0 ? exterior(P, 2, s, current, lv[static_cast<size_t>(x) * P.ny + P.ny - 1])
+             : lv[i - 1];
+  const float upper1 =
+      y + 1 == P.ny ? exterior(P, 3, s, current, lv[static_cast<size_t>(x) * P.ny]) : lv[i + 1];
+  return fe_update_f32(P, g, current, lower0, upper0, lower1, upper1, s, dt, source, loss,
+                       P.has_monod != 0 ? phi[i] : 0.0f);
 }
 
 }  // namespace msim

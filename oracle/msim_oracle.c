@@ -914,10 +914,12 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
           continue;
         }
         const double c = lv[site];
-        double fl[2], fu[2];
+        double fl[2] = {0.0, 0.0}, fu[2] = {0.0, 0.0};
         for (int a = 0; a < 2; ++a) {
           const uint32_t coord = a == 0 ? x : y;
           const uint32_t dim = a == 0 ? g->nx : g->ny;
+          /* An axis of extent one carries no transport (signals.cpp:1231-1247). */
+          if (dim == 1) continue;
           const size_t stride = a == 0 ? ny : 1;
           const double ih = a == 0 ? ihx : ihy;
           float vlo[3], vup[3];
@@ -927,8 +929,19 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
             const double u = lower_side ? (double)vlo[a] : (double)vup[a];
             double flux = 0.0;
             const int at_edge = lower_side ? coord == 0 : coord + 1 == dim;
-            if (at_edge) {
-              const int face = 2 * a + (lower_side ? 0 : 1);
+            const int face = 2 * a + (lower_side ? 0 : 1);
+            if (at_edge && g->boundary[face] == MO_PERIODIC) {
+              /* A periodic face joins the site to its partner on the other
+               * side, as an interior face does (signals.cpp:181-192,
+               * 213-261). */
+              const size_t nb = lower_side ? site + (size_t)(dim - 1) * stride
+                                           : site - (size_t)(dim - 1) * stride;
+              if (!solid_at(g, nb)) {
+                const double cl = lower_side ? lv[nb] : c;
+                const double cu = lower_side ? c : lv[nb];
+                flux = (u >= 0.0 ? u * cl : u * cu) - D * (cu - cl) * ih;
+              }
+            } else if (at_edge) {
               if (g->boundary[face] == MO_FIXED) {
                 const double cb = (double)sg->boundary_values[6 * s + (uint32_t)face];
                 /* Reservoir one full spacing beyond the site. */

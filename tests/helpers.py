@@ -58,3 +58,54 @@ def monod_loss(vmax, phi, K, levels):
     K = np.float32(K)
     return ((vmax * np.asarray(phi, np.float32)) / (K + np.asarray(levels, np.float32))).astype(
         np.float32)
+
+
+TRANSPORT_SHAPES = [(40, 70), (130, 64), (37, 300), (5, 513), (1, 200), (90, 1)]
+TRANSPORT_KINDS = [
+    (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX),
+    (FIXED, NO_FLUX, PERIODIC, PERIODIC, NO_FLUX, NO_FLUX),
+    (PERIODIC, PERIODIC, FIXED, FIXED, NO_FLUX, NO_FLUX),
+    (PERIODIC, PERIODIC, PERIODIC, PERIODIC, NO_FLUX, NO_FLUX),
+]
+
+
+def random_transport_case(nx, ny, S, kinds, seed):
+    """Obstacles, levels in [0.5, 1], affine rates, occupancy and a random face
+    field whose periodic faces agree on both sides."""
+    rng = np.random.default_rng(seed)
+    solid = (rng.random(nx * ny) < 0.12).astype(np.uint8)
+    levels = (0.5 + 0.5 * rng.random(S * nx * ny))
+    levels[np.tile(solid, S) != 0] = 0.0
+    xf = rng.uniform(-1.0, 1.0, (nx + 1) * ny).astype(np.float32)
+    yf = rng.uniform(-1.0, 1.0, nx * (ny + 1)).astype(np.float32)
+    # Faces next to an obstacle or on a no-flux boundary carry nothing; the
+    # two sides of a periodic boundary are one face.
+    sm = solid.reshape(nx, ny) != 0
+    X = xf.reshape(nx + 1, ny)
+    X[:-1][sm] = 0.0
+    X[1:][sm] = 0.0
+    Y = yf.reshape(nx, ny + 1)
+    Y[:, :-1][sm] = 0.0
+    Y[:, 1:][sm] = 0.0
+    for axis, F in ((0, X), (1, Y)):
+        lo = (slice(None), 0) if axis else (0, slice(None))
+        hi = (slice(None), -1) if axis else (-1, slice(None))
+        if kinds[2 * axis] == PERIODIC:
+            both = (F[lo] != 0) & (F[hi] != 0)
+            F[lo] = np.where(both, F[lo], 0.0)
+            F[hi] = F[lo]
+        else:
+            if kinds[2 * axis] == NO_FLUX:
+                F[lo] = 0.0
+            if kinds[2 * axis + 1] == NO_FLUX:
+                F[hi] = 0.0
+    zf = np.zeros(nx * ny * 2, np.float32)
+    loss = (rng.random(S * nx * ny) * 0.5).astype(np.float32)
+    source = (rng.random(S * nx * ny) * 0.2).astype(np.float32)
+    loss[np.tile(solid, S) != 0] = 0.0
+    source[np.tile(solid, S) != 0] = 0.0
+    phi = rng.random(nx * ny).astype(np.float32)
+    bv = rng.random(6 * S).astype(np.float32)
+    diffusion = rng.uniform(0.05, 0.3, S).astype(np.float32)
+    return dict(solid=solid, levels=levels, xf=xf, yf=yf, zf=zf, loss=loss, source=source,
+                phi=phi, bv=bv, diffusion=diffusion)

@@ -12,7 +12,8 @@ import pytest
 
 import microsimulator_amd as msa
 from oracle import Grid as OGrid
-from tests.helpers import FIXED, NO_FLUX, PERIODIC, c1_inputs, golden, split_faces
+from tests.helpers import (FIXED, NO_FLUX, PERIODIC, TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs,
+                           golden, random_transport_case, split_faces)
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +75,46 @@ def test_affine_transport_is_bit_exact(hip, oracle, kinds):
     ref = oracle.adr_step_f32(og, levels, 0.05, steps=3, diffusion=diffusion,
                               boundary_values=bv, source=source, loss=loss)
     assert np.array_equal(ours, ref)
+
+
+# Shapes cover one and several column strips, a last strip of one column,
+# strips of 64, 128 and 256 columns, and axes of extent one.
+SHAPES = TRANSPORT_SHAPES
+KINDS = TRANSPORT_KINDS
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("kinds", KINDS)
+def test_marching_step_on_every_shape(hip, oracle, shape, kinds):
+    """FP32 bit for bit and FP64 to 1e-13 against the oracle, with a face
+    field, affine rates and Monod uptake, on strips of every width."""
+    nx, ny = shape
+    S = 2
+    d = random_transport_case(nx, ny, S, kinds, seed=nx * 1000 + ny)
+    grid = msa.Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], kinds)
+    spec = msa.SignalGridSpec(grid, d["diffusion"], d["bv"],
+                              msa.VelocityField(d["xf"], d["yf"], d["zf"]),
+                              source_rates=d["source"], loss_rates=d["loss"], phi=d["phi"],
+                              monod_vmax=[0.7, 0.3], monod_k=[0.2, 0.5])
+    og = OGrid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], kinds)
+    kw = dict(diffusion=d["diffusion"], boundary_values=d["bv"], source=d["source"],
+              loss=d["loss"], faces=np.concatenate([d["xf"], d["yf"], d["zf"]]), phi=d["phi"],
+              vmax=[0.7, 0.3], monod_k=[0.2, 0.5])
+    dt = 0.02
+    print("case", shape, kinds, flush=True)
+    lv32 = d["levels"].astype(np.float32)
+    hip.adr_upload(spec, lv32, precision=32)
+    hip.adr_run(dt, 3, ledger=False)
+    ref32 = oracle.adr_step_f32(og, lv32, dt, steps=3, **kw)
+    assert np.array_equal(hip.adr_download(), ref32)
+    hip.adr_upload(spec, d["levels"], precision=64)
+    ledger = hip.adr_run(dt, 3)
+    ref64, ref_ledger = oracle.adr_step_f64(og, d["levels"], dt, steps=3, **kw)
+    ours = hip.adr_download()
+    assert np.abs(ours - ref64).max() <= 1e-13 * np.abs(ref64).max()
+    assert np.allclose(ledger, ref_ledger, rtol=1e-12, atol=1e-300)
+    balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+    assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
 
 
 def test_explicit_bound_refuses_and_leaves_levels(hip):

@@ -8,8 +8,9 @@ import numpy as np
 import pytest
 
 from oracle import FIXED, NO_FLUX, PERIODIC, Grid, OracleError
-from tests.helpers import (c1_inputs, divergence, golden, illcond_inputs, monod_loss,
-                           random_case, split_faces)
+from tests.helpers import (TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs, divergence, golden,
+                           illcond_inputs, monod_loss, random_case, random_transport_case,
+                           split_faces)
 
 
 def c1_grid():
@@ -275,6 +276,53 @@ def test_fp64_transport_conserves_and_agrees(oracle):
     assert np.abs(l64 - l32).max() <= 1e-5
     balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
     assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+
+
+def test_fp64_periodic_faces_join_partner_sites(oracle):
+    """On a grid periodic in x and y with a face field crossing both periodic
+    boundaries, the FP64 face-flux step agrees with the FP32 step (bit-exact
+    with the reference, which wraps periodic neighbours) and keeps the mass
+    exactly: nothing enters or leaves."""
+    rng = np.random.default_rng(8)
+    nx, ny = 14, 11
+    solid = (rng.random(nx * ny) < 0.1).astype(np.uint8)
+    kinds = (PERIODIC, PERIODIC, PERIODIC, PERIODIC, NO_FLUX, NO_FLUX)
+    grid = Grid(nx, ny, 1, 1.0, 0.5, 1.0, solid, boundary=kinds)
+    xf = rng.uniform(0.2, 1.0, (nx + 1) * ny).astype(np.float32)  # flow across x = 0
+    xf[nx * ny:] = xf[:ny]
+    yf = rng.uniform(-1.0, 1.0, (nx, ny + 1)).astype(np.float32)
+    yf[:, ny] = yf[:, 0]
+    faces = np.concatenate([xf, yf.reshape(-1), np.zeros(2 * nx * ny, np.float32)])
+    levels = 0.5 + 0.5 * rng.random(nx * ny)
+    levels[solid != 0] = 0.0
+    kw = dict(diffusion=0.1, faces=faces)
+    l32 = oracle.adr_step_f32(grid, levels.astype(np.float32), 0.02, steps=5, **kw)
+    l64, ledger = oracle.adr_step_f64(grid, levels, 0.02, steps=5, **kw)
+    assert np.abs(l64 - l32).max() <= 1e-6
+    assert np.all(ledger[:, :, 2] == 0.0) and np.all(ledger[:, :, 3] == 0.0)
+    assert np.abs(ledger[:, :, 1] - ledger[:, :, 0]).max() <= 1e-13 * ledger[0, 0, 0]
+    # The periodic faces carry mass: closing them changes the result.
+    closed = Grid(nx, ny, 1, 1.0, 0.5, 1.0, solid, boundary=(NO_FLUX,) * 6)
+    lc, _ = oracle.adr_step_f64(closed, levels, 0.02, steps=5, **kw)
+    assert np.abs(lc - l64).max() > 1e-3
+
+
+@pytest.mark.parametrize("kinds", TRANSPORT_KINDS)
+def test_fp64_step_agrees_with_fp32_step(oracle, kinds):
+    """The FP64 face-flux step against the FP32 step (bit-exact with the
+    reference) on every boundary kind, ragged shapes and axes of extent one,
+    with a face field, affine rates and Monod uptake; and its ledger balances."""
+    for nx, ny in TRANSPORT_SHAPES:
+        d = random_transport_case(nx, ny, 2, kinds, seed=nx * 1000 + ny)
+        grid = Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], boundary=kinds)
+        kw = dict(diffusion=d["diffusion"], boundary_values=d["bv"], source=d["source"],
+                  loss=d["loss"], faces=np.concatenate([d["xf"], d["yf"], d["zf"]]),
+                  phi=d["phi"], vmax=[0.7, 0.3], monod_k=[0.2, 0.5])
+        a = oracle.adr_step_f32(grid, d["levels"].astype(np.float32), 0.02, steps=3, **kw)
+        b, ledger = oracle.adr_step_f64(grid, d["levels"], 0.02, steps=3, **kw)
+        assert np.abs(a - b).max() <= 1e-6, (nx, ny)
+        balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+        assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
 
 
 def test_flow_error_paths(oracle):
