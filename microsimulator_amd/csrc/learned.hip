@@ -9,12 +9,14 @@
 // where N works tile by tile (16 x 32 sites, receptive region grown by 3):
 //   u = D^-1 r',  s = max |u| over the region,  v = u / s,
 //   in = [v, v cx, v cy, v cb]      (cx, cy, cb: the site's couplings times dinv)
-//   a1 = softsign(conv(in, W1)) * mask        4 -> 16 channels
-//   a2 = softsign(conv(a1, W2)) * mask        16 -> 16
-//   N  = conv(a2, W3) * mask                  16 -> 1
-// mask = on the grid and dinv > 0. The network is odd and has no biases, so
-// the smoother is homogeneous of degree one in r'; with W3 = 0 it is the
-// damped-Jacobi post-smoother.
+//   a1 = clip(conv(in, W1)) * mask           4 -> 16 channels
+//   a2 = clip(conv(a1, W2)) * mask           16 -> 16
+//   N  = conv(a2, W3) * mask                 16 -> 1
+// clip(h) = min(max(h, -1), 1); mask = on the grid and dinv > 0. The network
+// is odd and has no biases, so the smoother is homogeneous of degree one in
+// r'; with W3 = 0 it is the damped-Jacobi post-smoother. The clipped
+// activation and its mask are one median-of-three per value:
+// med3(h, -mask, mask).
 //
 // One workgroup of 256 threads handles one tile (three fit on a CU). The
 // staged region is the tile grown by 4 (24 x 40 sites, row pitch 40); every
@@ -103,16 +105,13 @@ __device__ inline short to_bf16(float f) {
 __device__ inline float bf16_value(unsigned short b) {
   return __uint_as_float(static_cast<uint32_t>(b) << 16);
 }
-// softsign with the hardware reciprocal (1 ulp); its result is rounded to
-// bfloat16 next, 2^8 times coarser.
-__device__ inline float softsign(float h) { return h * __builtin_amdgcn_rcpf(1.0f + fabsf(h)); }
-
-// bf16(softsign(acc)) of four channels, zero where the pixel is masked.
+// bf16(clip(acc) * mask) of four channels: med3(h, -1, 1) on a kept pixel,
+// med3(h, 0, 0) = 0 on a masked one.
 __device__ inline bf16x4 activate(f32x4 acc, bool on) {
   const float keep = on ? 1.0f : 0.0f;
   bf16x4 o;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) o[j] = to_bf16(softsign(acc[j]) * keep);
+  for (int j = 0; j < 4; ++j) o[j] = to_bf16(__builtin_amdgcn_fmed3f(acc[j], -keep, keep));
   return o;
 }
 
@@ -323,7 +322,6 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   float* s_x = reinterpret_cast<float*>(lds + kOffA);
   unsigned short* s_in = reinterpret_cast<unsigned short*>(lds + kOffA);
   unsigned short* s_a2 = s_in;
-  float4* s_feat = reinterpret_cast<float4*>(lds + kOffB);
   unsigned short* s_a1 = reinterpret_cast<unsigned short*>(lds + kOffB);
   float* s_T = reinterpret_cast<float*>(lds + kOffB);
   float* s_zc = reinterpret_cast<float*>(lds + kOffZc);
@@ -337,81 +335,123 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
   const uint32_t t0 = xcd_tile(blockIdx.x, gridDim.x) * per, t1 = min(tiles, t0 + per);
   double zr = 0.0, zq = 0.0;
-  // Every global value the thread's pixels of tile t need, in one round of
-  // loads: r, dinv, e_c, wx (lower, upper), wy (lower, upper), bc. The next
+  // Staging map: 240 threads cover the 24 x 40 region as 6 rows of 40
+  // columns, four times over (rows r0, r0 + 6, r0 + 12, r0 + 18), so a thread
+  // keeps one column and its four pixels differ by 6 rows.
+  constexpr int kStageRows = 6, kStageThreads = kStageRows * kPitch, kPer = kRows / kStageRows;
+  static_assert(kStageRows * kPer == kRows && kStageThreads <= kThreads, "staging map");
+  const int scol = static_cast<int>(threadIdx.x) % kPitch;
+  const int srow = static_cast<int>(threadIdx.x) / kPitch;
+  const bool stager = threadIdx.x < kStageThreads;
+  const bool feat_col = scol >= 1 && scol < kPitch - 1;
+  // Every global value a thread's pixels of tile t need, in one round of
+  // loads: r, dinv, e_c and, on feature pixels, the operator terms. The next
   // tile's round is issued before this tile's network runs.
-  constexpr int kPer = (kPix + kThreads - 1) / kThreads;
-  auto load_tile = [&](uint32_t t, float (&v)[8][kPer], uint32_t& inside) {
+  struct Stage {
+    float r[kPer], d[kPer], ec[kPer], wxl[kPer], wxu[kPer], wyl[kPer], wyu[kPer], b[kPer];
+    uint32_t on, feat;
+  };
+  auto load_tile = [&](uint32_t t) {
+    Stage v{};
     const int fx0 = static_cast<int>(t / tiles_y) * kTX;
     const int fy0 = static_cast<int>(t % tiles_y) * kTY;
-    inside = 0;
+    const int gy = fy0 - 4 + scol;
+    const bool col_in = stager && gy >= 0 && gy < static_cast<int>(G.ny);
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int p = static_cast<int>(threadIdx.x) + k * kThreads;
-#pragma unroll
-      for (int a = 0; a < 8; ++a) v[a][k] = 0.0f;
-      uint32_t gx, gy;
-      if (p < kPix && region_site(p, fx0, fy0, G, &gx, &gy)) {
-        inside |= 1u << k;
-        const size_t i = static_cast<size_t>(gx) * G.ny + gy;
-        v[0][k] = r[i];
-        v[1][k] = G.dinv[i];
-        v[2][k] = ec[static_cast<size_t>(gx / 2) * nyc + gy / 2];
-        if (feature_pixel(p)) {
-          v[3][k] = G.wx[i];
-          v[4][k] = gx + 1 < G.nx ? G.wx[i + G.ny] : 0.0f;
-          v[5][k] = G.wy[i];
-          v[6][k] = gy + 1 < G.ny ? G.wy[i + 1] : 0.0f;
-          v[7][k] = G.bc[i];
+      const int row = srow + kStageRows * k;
+      const int gx = fx0 - 4 + row;
+      if (col_in && gx >= 0 && gx < static_cast<int>(G.nx)) {
+        v.on |= 1u << k;
+        const size_t i = static_cast<size_t>(gx) * G.ny + static_cast<uint32_t>(gy);
+        v.r[k] = r[i];
+        v.d[k] = G.dinv[i];
+        v.ec[k] = ec[static_cast<size_t>(gx / 2) * nyc + static_cast<uint32_t>(gy) / 2];
+        if (feat_col && row >= 1 && row < kRows - 1) {
+          v.feat |= 1u << k;
+          v.wxl[k] = G.wx[i];
+          v.wxu[k] = static_cast<uint32_t>(gx) + 1 < G.nx ? G.wx[i + G.ny] : 0.0f;
+          v.wyl[k] = G.wy[i];
+          v.wyu[k] = static_cast<uint32_t>(gy) + 1 < G.ny ? G.wy[i + 1] : 0.0f;
+          v.b[k] = G.bc[i];
         }
       }
     }
+    return v;
   };
-  float cur[8][kPer], nxt[8][kPer];
-  uint32_t cur_in = 0, nxt_in = 0;
-  if (t0 < t1) load_tile(t0, cur, cur_in);
+  Stage cur{};
+  if (t0 < t1) cur = load_tile(t0);
   for (uint32_t t = t0; t < t1; ++t) {
     const int fx0 = static_cast<int>(t / tiles_y) * kTX;
     const int fy0 = static_cast<int>(t % tiles_y) * kTY;
     // x = omega D^-1 r + c P e_c over the region.
+    const uint32_t on = cur.on, feat = cur.feat;
+    float xv[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int p = static_cast<int>(threadIdx.x) + k * kThreads;
-      if (p < kPix) {
-        const bool on = (cur_in >> k) & 1u;
-        s_x[p] = on ? kOmega * cur[1][k] * cur[0][k] + kCorrection * cur[2][k] : 0.0f;
-        s_mask[p] = on && cur[1][k] > 0.0f ? 1 : 0;
+      xv[k] = ((on >> k) & 1u) ? kOmega * cur.d[k] * cur.r[k] + kCorrection * cur.ec[k] : 0.0f;
+    }
+    const float* rv = cur.r;
+    const float* dv = cur.d;
+    const float* wxl = cur.wxl;
+    const float* wxu = cur.wxu;
+    const float* wyl = cur.wyl;
+    const float* wyu = cur.wyu;
+    const float* bv = cur.b;
+    if (stager) {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int p = (srow + kStageRows * k) * kPitch + scol;
+        s_x[p] = xv[k];
+        s_mask[p] = ((on >> k) & 1u) && dv[k] > 0.0f ? 1 : 0;
       }
     }
     __syncthreads();
-    // r' = r - A x, u = D^-1 r' and the features on the region grown by 3;
-    // the damped-Jacobi post-smoother in the plain cycle's operation order.
+    // r' = r - A x, u = D^-1 r' and the features; the damped-Jacobi
+    // post-smoother in the plain cycle's operation order.
+    float uv[kPer], cxv[kPer], cyv[kPer], cbv[kPer];
     float umax = 0.0f;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const int p = static_cast<int>(threadIdx.x) + k * kThreads;
-      if (p >= kPix) continue;
-      float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (((cur_in >> k) & 1u) && feature_pixel(p)) {
+      const int row = srow + kStageRows * k;
+      const int p = row * kPitch + scol;
+      uv[k] = cxv[k] = cyv[k] = cbv[k] = 0.0f;
+      if ((feat >> k) & 1u) {
         const float xc = s_x[p];
-        const float wxl = cur[3][k], wxu = cur[4][k], wyl = cur[5][k], wyu = cur[6][k];
-        const float b = cur[7][k];
-        float ax = b * xc;
-        ax += wxl * (xc - s_x[p - kPitch]);
-        ax += wxu * (xc - s_x[p + kPitch]);
-        ax += wyl * (xc - s_x[p - 1]);
-        ax += wyu * (xc - s_x[p + 1]);
-        const float d = cur[1][k];
-        const float res = cur[0][k] - ax;
-        f = make_float4(d * res, (wxl + wxu) * d, (wyl + wyu) * d, b * d);
-        umax = fmaxf(umax, fabsf(f.x));
-        if (tile_pixel(p)) s_zc[(p / kPitch - 4) * kTY + p % kPitch - 4] = xc + kOmega * d * res;
+        float ax = bv[k] * xc;
+        ax += wxl[k] * (xc - s_x[p - kPitch]);
+        ax += wxu[k] * (xc - s_x[p + kPitch]);
+        ax += wyl[k] * (xc - s_x[p - 1]);
+        ax += wyu[k] * (xc - s_x[p + 1]);
+        const float d = dv[k];
+        const float res = rv[k] - ax;
+        uv[k] = d * res;
+        cxv[k] = (wxl[k] + wxu[k]) * d;
+        cyv[k] = (wyl[k] + wyu[k]) * d;
+        cbv[k] = bv[k] * d;
+        umax = fmaxf(umax, fabsf(uv[k]));
+        if (row >= 4 && row < 4 + kTX && scol >= 4 && scol < 4 + kTY) {
+          s_zc[(row - 4) * kTY + scol - 4] = xc + kOmega * d * res;
+        }
       }
-      s_feat[p] = f;
     }
-    if (t + 1 < t1) load_tile(t + 1, nxt, nxt_in);
+    const Stage nxt = t + 1 < t1 ? load_tile(t + 1) : Stage{};
     const float s = block_max_all<kThreads>(umax, s_red);  // has barriers
-    write_inputs<kThreads>(s_feat, s_in, s);
+    // in = bf16(v, v cx, v cy, v cb) with v = u / s; zero off the features.
+    if (stager) {
+      const float inv = s > 0.0f ? 1.0f / s : 0.0f;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int p = (srow + kStageRows * k) * kPitch + scol;
+        const float v = uv[k] * inv;
+        bf16x4 w;
+        w[0] = to_bf16(v);
+        w[1] = to_bf16(v * cxv[k]);
+        w[2] = to_bf16(v * cyv[k]);
+        w[3] = to_bf16(v * cbv[k]);
+        store4(s_in + 4 * p, w);
+      }
+    }
     __syncthreads();
     layers12<kThreads>(s_in, s_a1, s_a2, s_mask, frag);
     layer3<kThreads>(s_a2, s_T, frag);
@@ -430,12 +470,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
       }
     }
     __syncthreads();  // the next tile rewrites every shared array
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) cur[a][k] = nxt[a][k];
-    }
-    cur_in = nxt_in;
+    cur = nxt;
   }
   if (out.q != nullptr) {
     const uint32_t k = st->it_update + 1;
@@ -648,7 +683,7 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
     s_g[p] = v;
   }
   __syncthreads();
-  // dW3 (wave kWaves - 1) and g_h2 = (W3^T g shifted) * softsign'(h2) (the other waves).
+  // dW3 (wave kWaves - 1) and g_h2 = (W3^T g shifted) * clip'(h2) (the other waves).
   if (wave == kWaves - 1) {
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int ks = 0; ks < kL2Tiles; ++ks) {
@@ -687,8 +722,9 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
       bf16x4 o;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const float d = 1.0f - fabsf(bf16_value(static_cast<unsigned short>(av[j])));
-        o[j] = on ? to_bf16(acc[j] * d * d) : 0;
+        // clip'(h) from a = clip(h): 1 inside (-1, 1), 0 where clipped.
+        const bool pass = fabsf(bf16_value(static_cast<unsigned short>(av[j]))) < 1.0f;
+        o[j] = on && pass ? to_bf16(acc[j]) : 0;
       }
       store4(s_gh2 + g * kA2Plane + 4 * (q - kL2Begin), o);
     }
@@ -713,7 +749,7 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
     for (int j = 0; j < 4; ++j) out[kW1 + t * kC1 * kC1 + (4 * g + j) * kC1 + m] = acc[j];
   }
   __syncthreads();
-  // g_h1 = (sum_t W2[t] g_h2 shifted) * softsign'(h1), in place over a1.
+  // g_h1 = (sum_t W2[t] g_h2 shifted) * clip'(h1), in place over a1.
   {
     bf16x4 wb2[9];
 #pragma unroll
@@ -738,8 +774,9 @@ __global__ __launch_bounds__(kBThreads) void k_learn_backward(TrainView V,
       bf16x4 o;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const float d = 1.0f - fabsf(bf16_value(static_cast<unsigned short>(av[j])));
-        o[j] = on ? to_bf16(acc[j] * d * d) : 0;
+        // clip'(h) from a = clip(h): 1 inside (-1, 1), 0 where clipped.
+        const bool pass = fabsf(bf16_value(static_cast<unsigned short>(av[j]))) < 1.0f;
+        o[j] = on && pass ? to_bf16(acc[j]) : 0;
       }
       store4(a1p, o);
     }

@@ -16,14 +16,14 @@ With mask_p = (p on the grid and dinv_p > 0), for a site p of the region:
          cy = (wy_lower + wy_upper) dinv, cb = bc dinv
 Then:
   h1 = conv3x3(in, W1)  on the tile grown by 2  (4 -> 16 channels)
-  a1 = softsign(h1) * mask
+  a1 = clip(h1) * mask
   h2 = conv3x3(a1, W2)  on the tile grown by 1  (16 -> 16)
-  a2 = softsign(h2) * mask
+  a2 = clip(h2) * mask
   n  = conv3x3(a2, W3) * mask  on the tile       (16 -> 1)
   z  = omega u + s n
 
-The network has no biases and an odd activation, so z is odd and homogeneous
-of degree one in r'. Weights are indexed [tap][c_in][c_out], with
+clip(h) = min(max(h, -1), 1). The network has no biases and an odd
+activation, so z is odd and homogeneous of degree one in r'. Weights are indexed [tap][c_in][c_out], with
 tap = (dx + 1) * 3 + (dy + 1). Flat layout (3024 floats): W1 (9x4x16), then
 W2 (9x16x16), then W3 (9x16).
 
@@ -68,8 +68,13 @@ def pack(W1, W2, W3):
     return np.concatenate([np.ravel(W1), np.ravel(W2), np.ravel(W3)])
 
 
-def softsign(h):
-    return h / (1.0 + np.abs(h))
+def clip(h):
+    return np.clip(h, -1.0, 1.0)
+
+
+def clip_slope(h):
+    """Derivative of clip: 1 inside (-1, 1), 0 where the value is clipped."""
+    return (np.abs(h) < 1.0).astype(np.float64)
 
 
 def features(L):
@@ -123,11 +128,11 @@ def tile_forward(L, rp, W, fx0, fy0, bf=False, feats=None):
     m2 = mask[2:-2, 2:-2, None]
     m3 = mask[3:-3, 3:-3]
     h1 = conv(inp, W1, (TX + 4, TY + 4))
-    a1 = np.where(m1, softsign(h1), 0.0)
+    a1 = np.where(m1, clip(h1), 0.0)
     if bf:
         a1 = bf16(a1)
     h2 = conv(a1, W2, (TX + 2, TY + 2))
-    a2 = np.where(m2, softsign(h2), 0.0)
+    a2 = np.where(m2, clip(h2), 0.0)
     if bf:
         a2 = bf16(a2)
     n = np.where(m3, conv(a2, W3, (TX, TY))[..., 0], 0.0)
@@ -212,10 +217,10 @@ def gradients(L, rp, ep, W):
         g[..., 0] *= c["m3"]
         G3 += _corr_grad(c["a2"], g, TX, TY)
         g_a2 = _conv_transpose(g, W3, TX + 2, TY + 2)
-        g_h2 = np.where(c["m2"], g_a2 / (1.0 + np.abs(c["h2"])) ** 2, 0.0)
+        g_h2 = np.where(c["m2"], g_a2 * clip_slope(c["h2"]), 0.0)
         G2 += _corr_grad(c["a1"], g_h2, TX + 2, TY + 2)
         g_a1 = _conv_transpose(g_h2, W2, TX + 4, TY + 4)
-        g_h1 = np.where(c["m1"], g_a1 / (1.0 + np.abs(c["h1"])) ** 2, 0.0)
+        g_h1 = np.where(c["m1"], g_a1 * clip_slope(c["h1"]), 0.0)
         G1 += _corr_grad(c["inp"], g_h1, TX + 4, TY + 4)
     return value, G1, G2, G3
 

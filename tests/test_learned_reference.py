@@ -48,8 +48,23 @@ def case():
     rp = rng.normal(size=(20, 70)) * (L["dinv"] > 0)
     ep = solve(L, rp)
     W1, W2, W3 = lr.initial_weights(1)
+    # Weights three times the initial scale, so that the clip is active on a
+    # part of both hidden layers and the gradient check covers its slope.
     W3 = rng.normal(0.0, 0.05, W3.shape)
-    return L, rp, ep, (W1, W2, W3)
+    return L, rp, ep, (3.0 * W1, 3.0 * W2, W3)
+
+
+def test_fixture_exercises_the_clip(case):
+    L, rp, _, W = case
+    clipped, total = np.zeros(2), np.zeros(2)
+    for fx0, fy0 in lr.tiles(*rp.shape):
+        _, _, c = lr.tile_forward(L, rp, W, fx0, fy0)
+        for k, (h, m) in enumerate(((c["h1"], c["m1"]), (c["h2"], c["m2"]))):
+            keep = np.broadcast_to(m, h.shape)
+            clipped[k] += np.count_nonzero(np.abs(h)[keep] >= 1.0)
+            total[k] += np.count_nonzero(keep)
+    frac = clipped / total
+    assert np.all(frac > 0.01) and np.all(frac < 0.9), frac
 
 
 def test_gradients_match_finite_differences(case):
