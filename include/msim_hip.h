@@ -195,7 +195,14 @@ typedef struct {
   const float* monod_vmax;
   const float* monod_k;
   uint64_t revision;
+  uint32_t advection_scheme; /* MSIM_UPWIND or MSIM_VAN_LEER (SignalAdvectionKind, signals.hpp:41-47) */
 } msim_signal_grid;
+
+/* SignalAdvectionKind (signals.hpp:41-47): first-order upwind, or upwind plus
+ * the van Leer limited correction (signals.cpp:1227-1286), whose explicit
+ * bound halves the advective step (signals.cpp:1089-1094) and whose levels
+ * are settled after each step (settle_step_levels, signals.cpp:979-1001). */
+enum { MSIM_UPWIND = 0, MSIM_VAN_LEER = 1 };
 
 /* SignalSolveReport (signals.hpp:57-61). */
 typedef struct {

@@ -22,6 +22,7 @@ LIBRARY = os.path.join(HERE, "libmsim_hip.so")
 
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
 SOLVER_JACOBI_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_MULTIGRID = 0, 1, 2, 3
+UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind (signals.hpp:41-47)
 
 
 # --------------------------------------------------------------------------
@@ -109,7 +110,8 @@ class _SignalGrid(C.Structure):
                 ("boundary_values", C.c_void_p), ("x_faces", C.c_void_p),
                 ("y_faces", C.c_void_p), ("z_faces", C.c_void_p), ("source_rates", C.c_void_p),
                 ("loss_rates", C.c_void_p), ("phi", C.c_void_p), ("monod_vmax", C.c_void_p),
-                ("monod_k", C.c_void_p), ("revision", C.c_uint64)]
+                ("monod_k", C.c_void_p), ("revision", C.c_uint64),
+                ("advection_scheme", C.c_uint32)]
 
 
 class _TrainSet(C.Structure):
@@ -369,6 +371,7 @@ class SignalGridSpec:
     monod_vmax: np.ndarray | None = None
     monod_k: np.ndarray | None = None
     revision: int = 0
+    advection_scheme: int = 0  # UPWIND or VAN_LEER (SignalAdvectionKind)
 
     @property
     def signal_count(self) -> int:
@@ -399,6 +402,7 @@ class SignalGridSpec:
             keep.append(arr)
             setattr(s, name, arr.ctypes.data)
         s.revision = self.revision
+        s.advection_scheme = self.advection_scheme
         return s
 
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -28,7 +29,8 @@ namespace msim {
 namespace {
 
 constexpr int kAdrBlock = 256;
-constexpr uint32_t kFlagNegative = 1;
+constexpr uint32_t kFlagNegative = 1;  // a non-finite or (upwind) negative update
+constexpr uint32_t kFlagLevels = 2;    // a level left negative after the settle
 
 // The ledger of a step lives as per-block partial sums (mass after, boundary
 // supply rate, uptake rate, for each signal) in two generations by step
@@ -94,27 +96,32 @@ struct AdrRow {
 };
 
 // One forward-Euler step for S signals, marching: a block owns a strip of
-// TY - 2 columns and `rows` x-rows and walks x, so each level is read from
+// TY - 2H columns and `rows` x-rows and walks x, so each level is read from
 // memory once and its x neighbours come from registers, its y neighbours from
-// a row in LDS. The strip's first and last threads only load the columns
-// just outside it (or, on a periodic y boundary, the partner column), so no
-// thread holds more than its own column. The next row's loads are issued one
-// row ahead.
+// a row in LDS. H = 1 for the upwind scheme, 2 under van Leer (whose
+// correction reads the site two away). The strip's first and last H threads
+// only load the columns just outside it (or, on a periodic y boundary, the
+// partner columns), so no thread holds more than its own column. The next
+// row's loads are issued one row ahead.
 //  * T = float: the reference's per-site rate in its float order
 //    (fe_update_f32), so every site matches its CPU step bit for bit.
-//  * T = double: face-flux form, each face flux one expression of the two
-//    sites it joins, so what leaves one site enters the other bit for bit;
-//    the step's mass, boundary supply and uptake go to the ledger partials.
-// A site whose update is negative or non-finite raises a flag (the reference
-// throws, signals.cpp:1124-1130); the step is then refused as a whole.
-template <typename T, int S, int TY, bool kAffine>
+//  * T = double: face-flux form, each face flux (with its limited
+//    correction) one expression of the sites around it, so what leaves one
+//    site enters the other bit for bit; the step's mass, boundary supply and
+//    uptake go to the ledger partials.
+// A site whose update is non-finite, or negative under upwind, raises a flag
+// (the reference throws, signals.cpp:1119-1131); the step is then refused as
+// a whole. Under van Leer each block also leaves the largest level it wrote,
+// for the settle that follows.
+template <typename T, int S, int TY, bool kAffine, bool kLimited>
 __global__ __launch_bounds__(TY) void k_adr_step(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
     const float* __restrict__ loss, const T* __restrict__ in, T* __restrict__ out, float dt,
-    uint32_t* flags, uint32_t step, uint32_t rows, LedgerParts L) {
+    uint32_t* flags, uint32_t step, uint32_t rows, LedgerParts L, double* largest) {
   constexpr bool kF64 = sizeof(T) == 8;
-  constexpr int kWidth = TY - 2;
+  constexpr int kHalo = kLimited ? 2 : 1;
+  constexpr int kWidth = TY - 2 * kHalo;
   using Row = AdrRow<T, S, kAffine>;
   __shared__ T s_c[2][S][TY];
   __shared__ uint8_t s_s[2][TY];
@@ -129,25 +136,27 @@ __global__ __launch_bounds__(TY) void k_adr_step(
   const uint32_t strips = (ny + kWidth - 1) / kWidth;
   const uint32_t x0 = (tile / strips) * rows, x1 = min(nx, x0 + rows);
   const uint32_t t = threadIdx.x;
-  // Column of this thread: -1 and ny stand for the periodic partner columns.
-  const int col = static_cast<int>((tile % strips) * kWidth + t) - 1;
+  const bool xper = P.kind[0] == MSIM_PERIODIC, yper = P.kind[2] == MSIM_PERIODIC;
+  // Column of this thread; columns off the grid stand for the periodic
+  // partner columns.
+  const int col = static_cast<int>((tile % strips) * kWidth + t) - kHalo;
   int src_col = col;
-  if (col < 0) src_col = P.kind[2] == MSIM_PERIODIC ? static_cast<int>(ny) - 1 : -1;
-  if (col >= static_cast<int>(ny)) {
-    src_col = col == static_cast<int>(ny) && P.kind[3] == MSIM_PERIODIC ? 0 : -1;
-  }
+  if (col < 0) src_col = yper ? col + static_cast<int>(ny) : -1;
+  if (col >= static_cast<int>(ny)) src_col = yper ? col - static_cast<int>(ny) : -1;
+  if (src_col < 0 || src_col >= static_cast<int>(ny)) src_col = -1;
   const bool loads = src_col >= 0;
-  const bool owns = t >= 1 && t <= kWidth && col >= 0 && col < static_cast<int>(ny);
+  const bool owns = t >= kHalo && t < kHalo + kWidth && col >= 0 && col < static_cast<int>(ny);
   const uint32_t y = static_cast<uint32_t>(src_col);
   const bool field = P.has_field != 0, monod = P.has_monod != 0;
-  // Row xx of the walk (xx in [-1, nx]); rows -1 and nx are the periodic
-  // partners when the x boundary is periodic and absent otherwise. `full`
-  // rows are the ones the block updates.
+  // Row xx of the walk (xx in [-H, nx + H - 1]); rows off the grid are the
+  // periodic partners when the x boundary is periodic and absent otherwise.
+  // `full` rows are the ones the block updates.
   auto load = [&](int xx, bool full) {
     Row w{};
     int row = xx;
-    if (xx < 0) row = P.kind[0] == MSIM_PERIODIC ? static_cast<int>(nx) - 1 : -1;
-    if (xx >= static_cast<int>(nx)) row = P.kind[1] == MSIM_PERIODIC ? 0 : -1;
+    if (xx < 0) row = xper ? xx + static_cast<int>(nx) : -1;
+    if (xx >= static_cast<int>(nx)) row = xper ? xx - static_cast<int>(nx) : -1;
+    if (row >= static_cast<int>(nx)) row = -1;
     if (owns && field && xx >= 0 && xx <= static_cast<int>(nx)) {
       w.xf = xf[static_cast<size_t>(xx) * ny + y];  // lower x face of row xx
     }
@@ -175,14 +184,20 @@ __global__ __launch_bounds__(TY) void k_adr_step(
   double acc[kF64 ? 3 * S : 1];
 #pragma unroll
   for (int k = 0; k < (kF64 ? 3 * S : 1); ++k) acc[k] = 0.0;
+  T top = T(0);  // largest level written (van Leer)
   bool bad = false;
   if (x0 < x1) {
     const int xa = static_cast<int>(x0), xb = static_cast<int>(x1);
+    // Window of rows xx - 2 .. xx + 2 (the outer two under van Leer only).
+    Row p2 = kLimited ? load(xa - 2, false) : Row{};
     Row prev = load(xa - 1, false);
     Row cur = load(xa, true);
     Row next = load(xa + 1, xa + 1 < xb);
+    Row n2 = kLimited ? load(xa + 2, xa + 2 < xb) : Row{};
     for (int xx = xa; xx < xb; ++xx) {
-      const Row ahead = xx + 2 <= xb ? load(xx + 2, xx + 2 < xb) : Row{};
+      constexpr int kAhead = kLimited ? 3 : 2;
+      const Row ahead =
+          xx + kAhead < xb + kHalo ? load(xx + kAhead, xx + kAhead < xb) : Row{};
       const int buf = xx & 1;
 #pragma unroll
       for (int s = 0; s < S; ++s) s_c[buf][s][t] = cur.c[s];
@@ -192,7 +207,10 @@ __global__ __launch_bounds__(TY) void k_adr_step(
         const size_t i = static_cast<size_t>(xx) * ny + y;
         if (cur.solid) {
 #pragma unroll
-          for (int s = 0; s < S; ++s) out[s * n + i] = cur.c[s];
+          for (int s = 0; s < S; ++s) {
+            out[s * n + i] = cur.c[s];
+            if constexpr (kLimited) top = max(top, cur.c[s]);
+          }
         } else {
           const bool xlo_edge = xx == 0, xhi_edge = xx + 1 == static_cast<int>(nx);
           const bool ylo_edge = y == 0, yhi_edge = y + 1 == ny;
@@ -211,6 +229,19 @@ __global__ __launch_bounds__(TY) void k_adr_step(
           g.cup[1] = yhi_edge ? (P.kind[3] == MSIM_NO_FLUX ||
                                  (P.kind[3] == MSIM_PERIODIC && yhi_solid))
                               : yhi_solid;
+          // Which sites the limited correction may use (signals.cpp:1235-1253).
+          [[maybe_unused]] bool lower_site[2], upper_site[2], far_lo_ok[2], far_hi_ok[2];
+          if constexpr (kLimited) {
+            const bool xp3 = xper && nx >= 3, yp3 = yper && ny >= 3;
+            lower_site[0] = !g.clo[0] && (!xlo_edge || xp3);
+            upper_site[0] = !g.cup[0] && (!xhi_edge || xp3);
+            lower_site[1] = !g.clo[1] && (!ylo_edge || yp3);
+            upper_site[1] = !g.cup[1] && (!yhi_edge || yp3);
+            far_lo_ok[0] = (xx >= 2 || xp3) && !p2.solid;
+            far_hi_ok[0] = (xx + 2 < static_cast<int>(nx) || xp3) && !n2.solid;
+            far_lo_ok[1] = (y >= 2 || yp3) && s_s[buf][t - 2] == 0;
+            far_hi_ok[1] = (y + 2 < ny || yp3) && s_s[buf][t + 2] == 0;
+          }
 #pragma unroll
           for (int s = 0; s < S; ++s) {
             if (field) {
@@ -225,6 +256,13 @@ __global__ __launch_bounds__(TY) void k_adr_step(
             const T c = cur.c[s];
             const T ylo_v = s_c[buf][s][t - 1];
             const T yhi_v = s_c[buf][s][t + 1];
+            [[maybe_unused]] T far_lo[2] = {T(0), T(0)}, far_hi[2] = {T(0), T(0)};
+            if constexpr (kLimited) {
+              far_lo[0] = p2.c[s];
+              far_hi[0] = n2.c[s];
+              far_lo[1] = s_c[buf][s][t - 2];
+              far_hi[1] = s_c[buf][s][t + 2];
+            }
             const float src = kAffine ? cur.src[kAffine ? s : 0] : 0.0f;
             const float los = kAffine ? cur.los[kAffine ? s : 0] : 0.0f;
             if constexpr (!kF64) {
@@ -232,8 +270,25 @@ __global__ __launch_bounds__(TY) void k_adr_step(
               const float u0 = xhi_edge ? exterior(P, 1, s, c, next.c[s]) : next.c[s];
               const float l1 = ylo_edge ? exterior(P, 2, s, c, ylo_v) : ylo_v;
               const float u1 = yhi_edge ? exterior(P, 3, s, c, yhi_v) : yhi_v;
-              const float v = fe_update_f32(P, g, c, l0, u0, l1, u1, s, dt, src, los, cur.phi);
-              if (!isfinite(v) || v < 0.0f) bad = true;
+              float v;
+              if constexpr (kLimited) {
+                Limiter<float> lim;
+#pragma unroll
+                for (int a2 = 0; a2 < 2; ++a2) {
+                  lim.lower_site[a2] = lower_site[a2];
+                  lim.upper_site[a2] = upper_site[a2];
+                  lim.far_lo_ok[a2] = far_lo_ok[a2];
+                  lim.far_hi_ok[a2] = far_hi_ok[a2];
+                  lim.far_lo[a2] = far_lo[a2];
+                  lim.far_hi[a2] = far_hi[a2];
+                }
+                v = fe_update_f32<true>(P, g, c, l0, u0, l1, u1, s, dt, src, los, cur.phi, &lim);
+                if (!isfinite(v)) bad = true;
+                top = max(top, v);
+              } else {
+                v = fe_update_f32(P, g, c, l0, u0, l1, u1, s, dt, src, los, cur.phi);
+                if (!isfinite(v) || v < 0.0f) bad = true;
+              }
               out[s * n + i] = v;
             } else {
               const double ih[2] = {1.0 / static_cast<double>(P.dx),
@@ -256,6 +311,16 @@ __global__ __launch_bounds__(TY) void k_adr_step(
                 } else if (!g.clo[a]) {
                   const double cl = nlo[a];
                   fl = (ul >= 0.0 ? ul * cl : ul * c) - D * (c - cl) * ih[a];
+                  if constexpr (kLimited) {
+                    if (lower_site[a]) {
+                      if (ul >= 0.0) {
+                        if (far_lo_ok[a])
+                          fl += ul * van_leer_increment<double>(cl, c, far_lo[a]);
+                      } else if (upper_site[a]) {
+                        fl += ul * van_leer_increment<double>(c, cl, nhi[a]);
+                      }
+                    }
+                  }
                 }
                 if (hi_edge[a] && P.kind[2 * a + 1] == MSIM_FIXED) {
                   const double cb = static_cast<double>(P.boundary[s][2 * a + 1]);
@@ -264,6 +329,15 @@ __global__ __launch_bounds__(TY) void k_adr_step(
                 } else if (!g.cup[a]) {
                   const double cu = nhi[a];
                   fu = (uu >= 0.0 ? uu * c : uu * cu) - D * (cu - c) * ih[a];
+                  if constexpr (kLimited) {
+                    if (upper_site[a]) {
+                      if (uu >= 0.0) {
+                        if (lower_site[a]) fu += uu * van_leer_increment<double>(c, cu, nlo[a]);
+                      } else if (far_hi_ok[a]) {
+                        fu += uu * van_leer_increment<double>(cu, c, far_hi[a]);
+                      }
+                    }
+                  }
                 }
                 rate -= (fu - fl) * ih[a];
               }
@@ -279,7 +353,8 @@ __global__ __launch_bounds__(TY) void k_adr_step(
               }
               rate -= lossr * c;
               const double v = c + static_cast<double>(dt) * rate;
-              if (!isfinite(v) || v < 0.0) bad = true;
+              if (!isfinite(v) || (v < 0.0 && !kLimited)) bad = true;
+              if constexpr (kLimited) top = max(top, v);
               out[s * n + i] = v;
               acc[3 * s + 0] += v;
               acc[3 * s + 1] += supply;
@@ -288,14 +363,25 @@ __global__ __launch_bounds__(TY) void k_adr_step(
           }
         }
       }
+      if constexpr (kLimited) p2 = prev;
       prev = cur;
       cur = next;
-      next = ahead;
+      if constexpr (kLimited) {
+        next = n2;
+        n2 = ahead;
+      } else {
+        next = ahead;
+      }
     }
   }
   if (bad) {
     atomicOr(&flags[0], kFlagNegative);
     atomicMin(&flags[1], step);
+  }
+  if constexpr (kLimited) {
+    const double m = block_max<TY>(static_cast<double>(top), s_red);
+    if (threadIdx.x == 0) largest[blockIdx.x] = m;
+    __syncthreads();
   }
   if constexpr (kF64) {
     // Solid sites hold their (zero) level; the mass counts only what moved.
@@ -307,6 +393,48 @@ __global__ __launch_bounds__(TY) void k_adr_step(
             acc[k];
       }
     }
+  }
+}
+
+// settle_step_levels then validate_signal_levels (signals.cpp:979-1001,
+// 754-784) after a van Leer step, in place over the S*N levels it wrote:
+// levels below zero by no more than 16 epsilon (of the level type) times the
+// largest level become zero; a level still negative or not finite refuses
+// the step. Every block takes the largest level from the step's block
+// maxima.
+template <typename T>
+__global__ __launch_bounds__(kAdrBlock) void k_adr_settle(T* __restrict__ levels, size_t count,
+                                                          const double* __restrict__ largest,
+                                                          uint32_t parts, uint32_t* flags,
+                                                          uint32_t step) {
+  __shared__ double s_red[kAdrBlock / kWave];
+  __shared__ double s_top;
+  if (flags[0] != 0) return;
+  double m = 0.0;
+  for (uint32_t b = threadIdx.x; b < parts; b += kAdrBlock) m = fmax(m, largest[b]);
+  m = block_max<kAdrBlock>(m, s_red);
+  if (threadIdx.x == 0) s_top = m;
+  __syncthreads();
+  const T top = static_cast<T>(s_top);
+  const T eps = sizeof(T) == 8 ? static_cast<T>(DBL_EPSILON) : static_cast<T>(FLT_EPSILON);
+  const T rounding = T(-16) * eps * top;
+  bool bad = false;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kAdrBlock + threadIdx.x; i < count;
+       i += static_cast<size_t>(gridDim.x) * kAdrBlock) {
+    const T v = levels[i];
+    if (v < T(0)) {
+      if (v >= rounding) {
+        levels[i] = T(0);
+      } else {
+        bad = true;
+      }
+    } else if (!isfinite(v)) {
+      bad = true;
+    }
+  }
+  if (bad) {
+    atomicOr(&flags[0], kFlagLevels);
+    atomicMin(&flags[1], step);
   }
 }
 
@@ -385,24 +513,33 @@ struct AdrShape {
   uint32_t rows, blocks;
 };
 
-AdrShape adr_shape(uint32_t nx, uint32_t ny) {
+AdrShape adr_shape(uint32_t nx, uint32_t ny, int halo) {
   AdrShape s{};
-  // The strip width (TY - 2 columns) that covers the grid's width with the
-  // fewest threads, the wider on a tie.
+  // The strip width (TY - 2 halo columns) that covers the grid's width with
+  // the fewest threads, the wider on a tie.
   uint64_t best = ~0ull;
   for (int ty : {256, 128, 64}) {
-    const uint64_t threads = static_cast<uint64_t>((ny + ty - 3) / (ty - 2)) * ty;
+    const uint32_t width = static_cast<uint32_t>(ty - 2 * halo);
+    const uint64_t threads = static_cast<uint64_t>((ny + width - 1) / width) * ty;
     if (threads < best) {
       best = threads;
       s.ty = ty;
     }
   }
-  const uint32_t strips = (ny + s.ty - 3) / (s.ty - 2);
+  const uint32_t width = static_cast<uint32_t>(s.ty - 2 * halo);
+  const uint32_t strips = (ny + width - 1) / width;
   uint32_t rows = 16;
   while (rows > 4 && static_cast<uint64_t>(strips) * ((nx + rows - 1) / rows) < 2048) rows /= 2;
   s.rows = rows;
   s.blocks = strips * ((nx + rows - 1) / rows);
   return s;
+}
+
+// Blocks of a step launch: at least 3S, since blocks j < 3S fold the
+// previous step's ledger partial j (blocks past the grid's strips only add
+// zeros).
+uint32_t step_blocks(const AdrShape& sh) {
+  return std::max<uint32_t>(sh.blocks, 3 * kMaxSignals);
 }
 
 uint32_t site_blocks(size_t n) { return static_cast<uint32_t>((n + kAdrBlock - 1) / kAdrBlock); }
@@ -436,6 +573,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
     }
   }
   P.has_field = (use_flow_field || grid.x_faces != nullptr) ? 1 : 0;
+  P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
   P.has_affine = grid.loss_rates != nullptr ? 1 : 0;
   P.has_monod = grid.phi != nullptr ? 1 : 0;
   a.precision = precision;
@@ -446,9 +584,11 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.flags.reserve(2);
   a.state.reserve(1);
   a.mass.reserve(kMaxSignals);
+  // Ledger partials [2][3 * kMaxSignals][blocks], then the van Leer block
+  // maxima [blocks]; the survey's partials reuse the front.
+  const uint32_t step_parts = step_blocks(adr_shape(P.nx, P.ny, P.limited != 0 ? 2 : 1));
   a.partials.reserve(std::max<size_t>(
-      static_cast<size_t>(3 * kMaxSignals) * 2 *
-          std::max<uint32_t>(adr_shape(P.nx, P.ny).blocks, 3 * kMaxSignals),
+      static_cast<size_t>(3 * kMaxSignals * 2 + 1) * step_parts,
       static_cast<size_t>(kMaxSignals + 1) * 2048));
   if (!same_spec) {
     a.solid.reserve(n);
@@ -524,20 +664,41 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.uploaded = true;
 }
 
-template <typename T, int S, int TY>
+template <typename T, int S, int TY, bool kLimited>
 void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
-                 const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L) {
+                 const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
+                 double* largest) {
   auto& a = ctx.adr;
   if (a.params.has_affine != 0) {
-    k_adr_step<T, S, TY, true><<<sh.blocks, TY, 0, ctx.stream>>>(
+    k_adr_step<T, S, TY, true, kLimited><<<sh.blocks, TY, 0, ctx.stream>>>(
         a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
         reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
-        sh.rows, L);
+        sh.rows, L, largest);
   } else {
-    k_adr_step<T, S, TY, false><<<sh.blocks, TY, 0, ctx.stream>>>(
+    k_adr_step<T, S, TY, false, kLimited><<<sh.blocks, TY, 0, ctx.stream>>>(
         a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
         reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
-        sh.rows, L);
+        sh.rows, L, largest);
+  }
+  if constexpr (kLimited) {
+    const size_t count = static_cast<size_t>(S) * a.sites();
+    const uint32_t blocks = static_cast<uint32_t>(
+        std::min<size_t>((count + kAdrBlock - 1) / kAdrBlock, 2048));
+    k_adr_settle<T><<<blocks, kAdrBlock, 0, ctx.stream>>>(reinterpret_cast<T*>(dst), count,
+                                                          largest, sh.blocks, a.flags.data(), k);
+  }
+}
+
+template <typename T, int S, bool kLimited>
+void launch_typed(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
+                  const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
+                  double* largest) {
+  switch (sh.ty) {
+    case 64: launch_step<T, S, 64, kLimited>(ctx, sh, xf, yf, src, dst, dt, k, L, largest); break;
+    case 128:
+      launch_step<T, S, 128, kLimited>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      break;
+    default: launch_step<T, S, 256, kLimited>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
   }
 }
 
@@ -545,25 +706,25 @@ template <int S>
 void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf) {
   auto& a = ctx.adr;
   const auto& P = a.params;
-  AdrShape sh = adr_shape(P.nx, P.ny);
-  // Blocks j < 3S fold the previous step's ledger partial j: launch at least
-  // that many (blocks past the grid's strips only add zeros).
-  sh.blocks = std::max<uint32_t>(sh.blocks, 3 * S);
+  const bool limited = P.limited != 0;
+  AdrShape sh = adr_shape(P.nx, P.ny, limited ? 2 : 1);
+  sh.blocks = step_blocks(sh);
   LedgerParts L{a.partials.data(), a.ledger.data(), sh.blocks};
+  double* largest = a.partials.data() + static_cast<size_t>(3 * kMaxSignals * 2) * sh.blocks;
   for (uint32_t k = 0; k < steps; ++k) {
     const uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
     uint8_t* dst = a.current == 0 ? a.levels_b.data() : a.levels_a.data();
     if (a.precision == 64) {
-      switch (sh.ty) {
-        case 64: launch_step<double, S, 64>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
-        case 128: launch_step<double, S, 128>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
-        default: launch_step<double, S, 256>(ctx, sh, xf, yf, src, dst, dt, k, L);
+      if (limited) {
+        launch_typed<double, S, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      } else {
+        launch_typed<double, S, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
       }
     } else {
-      switch (sh.ty) {
-        case 64: launch_step<float, S, 64>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
-        case 128: launch_step<float, S, 128>(ctx, sh, xf, yf, src, dst, dt, k, L); break;
-        default: launch_step<float, S, 256>(ctx, sh, xf, yf, src, dst, dt, k, L);
+      if (limited) {
+        launch_typed<float, S, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      } else {
+        launch_typed<float, S, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
       }
     }
     check(hipGetLastError(), "launch transport step");
@@ -617,9 +778,14 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
     // the starting one : the other).
     const uint32_t failed = flags[1];
     a.current = static_cast<int>((a.current ^ (steps & 1u)) ^ (failed & 1u));
-    throw Error(MSIM_DOMAIN_ERROR,
-                "signal grid update produced a non-finite or negative concentration (step " +
-                    std::to_string(failed) + ")");
+    if ((flags[0] & kFlagNegative) != 0) {
+      throw Error(MSIM_DOMAIN_ERROR,
+                  "signal grid update produced a non-finite or negative concentration (step " +
+                      std::to_string(failed) + ")");
+    }
+    // The settle refused the step (accept_step, signals.cpp:1003-1011).
+    throw Error(MSIM_INVALID_ARGUMENT, "signal grid levels must be finite and non-negative (step " +
+                                           std::to_string(failed) + ")");
   }
   if (!rows.empty()) {
     // Mass before each step: the mass after the previous one.

@@ -72,16 +72,39 @@ __device__ inline SiteGeometry geometry(const AdrParams& P, const uint8_t* __res
   return g;
 }
 
+// van_leer_increment (signals.cpp:202-206).
+template <typename T>
+__device__ inline T van_leer_increment(T upwind, T downwind, T far) {
+  const T behind = upwind - far;
+  const T ahead = downwind - upwind;
+  return behind * ahead > T(0) ? behind * ahead / (behind + ahead) : T(0);
+}
+
+// What the limited (van Leer) correction of one site needs along each axis:
+// whether the lower and upper neighbours are joined fluid sites of the grid
+// (lower_site / upper_site, signals.cpp:1250-1253) and the level two sites
+// away on each side where that site is on the grid and fluid (far(-2) and
+// far(2), signals.cpp:1238-1249).
+template <typename T>
+struct Limiter {
+  bool lower_site[2], upper_site[2];
+  bool far_lo_ok[2], far_hi_ok[2];
+  T far_lo[2], far_hi[2];
+};
+
 // Forward-Euler update of signal s at one site from its neighbour values:
 // the reference's per-site rate in its own float order
-// (cpp/core/signals.cpp:1145-1291 transport, 305-320 affine reaction,
-// 1109-1134 forward Euler), with Monod uptake as the affine loss
-// vmax*phi/(K+c) it equals under forward Euler. lower/upper hold the x and y
-// neighbour values with exterior values already resolved (exterior()); a
-// closed face replaces its neighbour by the site's own value here.
+// (cpp/core/signals.cpp:1145-1291 transport with the van Leer correction when
+// kLimited, 305-320 affine reaction, 1109-1134 forward Euler), with Monod
+// uptake as the affine loss vmax*phi/(K+c) it equals under forward Euler.
+// lower/upper hold the x and y neighbour values with exterior values already
+// resolved (exterior()); a closed face replaces its neighbour by the site's
+// own value here.
+template <bool kLimited = false>
 __device__ inline float fe_update_f32(const AdrParams& P, const SiteGeometry& g, float current,
                                       float lower0, float upper0, float lower1, float upper1,
-                                      int s, float dt, float source, float loss, float phi) {
+                                      int s, float dt, float source, float loss, float phi,
+                                      const Limiter<float>* lim = nullptr) {
   const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
   const uint32_t dims[2] = {P.nx, P.ny};
   float lower[2] = {lower0, lower1}, upper[2] = {upper0, upper1};
@@ -100,6 +123,29 @@ __device__ inline float fe_update_f32(const AdrParams& P, const SiteGeometry& g,
     if (g.clo[a]) lower_flux = 0.0f;
     if (g.cup[a]) upper_flux = 0.0f;
     rate -= (upper_flux - lower_flux) * inverse_spacing;
+    if constexpr (kLimited) {
+      float lower_correction = 0.0f, upper_correction = 0.0f;
+      if (lim->lower_site[a]) {
+        if (g.ulo[a] >= 0.0f) {
+          if (lim->far_lo_ok[a]) {
+            lower_correction =
+                g.ulo[a] * van_leer_increment(lower[a], current, lim->far_lo[a]);
+          }
+        } else if (lim->upper_site[a]) {
+          lower_correction = g.ulo[a] * van_leer_increment(current, lower[a], upper[a]);
+        }
+      }
+      if (lim->upper_site[a]) {
+        if (g.uup[a] >= 0.0f) {
+          if (lim->lower_site[a]) {
+            upper_correction = g.uup[a] * van_leer_increment(current, upper[a], lower[a]);
+          }
+        } else if (lim->far_hi_ok[a]) {
+          upper_correction = g.uup[a] * van_leer_increment(upper[a], current, lim->far_hi[a]);
+        }
+      }
+      rate -= (upper_correction - lower_correction) * inverse_spacing;
+    }
   }
   if (P.has_affine != 0) rate += source - (loss * current);
   if (P.has_monod != 0) {

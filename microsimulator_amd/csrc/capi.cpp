@@ -178,6 +178,9 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
     throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes a grid of one layer (nz = 1)");
   }
   if (s.signal_count == 0) invalid("signal grid must contain at least one signal");
+  if (s.advection_scheme != MSIM_UPWIND && s.advection_scheme != MSIM_VAN_LEER) {
+    invalid("unknown signal advection scheme");  // signals.cpp:726-732
+  }
   if (s.signal_count > kMaxSignals) {
     throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes at most 8 signals");
   }
@@ -294,7 +297,7 @@ void record_losses(Context& ctx, const msim_signal_grid& s) {
   }
 }
 
-// SignalGrid::validate_step (signals.cpp:1066-1102) for the upwind scheme.
+// SignalGrid::validate_step (signals.cpp:1066-1102).
 void validate_step(const Context& ctx, float dt) {
   if (!std::isfinite(dt) || dt < 0.0f) invalid("time step must be finite and non-negative");
   const auto& P = ctx.adr.params;
@@ -311,6 +314,8 @@ void validate_step(const Context& ctx, float dt) {
         courant_sum += std::abs(static_cast<double>(P.advection[k][a])) * inverse_spacing;
       }
     }
+    // A limited face flux carries up to twice the upwind one.
+    if (P.limited != 0) courant_sum *= 2.0;
     const double factor =
         static_cast<double>(dt) *
         ((2.0 * static_cast<double>(P.diffusion[k]) * inverse_square_sum) + courant_sum +

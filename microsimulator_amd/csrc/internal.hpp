@@ -163,6 +163,7 @@ struct AdrParams {
   float dx, dy, dz;
   uint8_t kind[6];
   uint8_t has_field, has_affine, has_monod;
+  uint8_t limited;  // van Leer correction (SignalAdvectionKind::van_leer)
   float diffusion[kMaxSignals];
   float advection[kMaxSignals][3];
   float boundary[kMaxSignals][6];

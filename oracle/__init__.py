@@ -23,6 +23,7 @@ ORACLE_SO = os.path.join(HERE, "libmsim_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libmsim_ref.so")
 
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
+UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind
 
 STATUS = {
     0: "ok", 1: "invalid_argument", 2: "runtime_error", 3: "not_converged", 4: "overflow",
@@ -63,7 +64,8 @@ class _SignalGrid(C.Structure):
                 ("advection", C.c_void_p), ("boundary_values", C.c_void_p),
                 ("x_faces", C.c_void_p), ("y_faces", C.c_void_p), ("z_faces", C.c_void_p),
                 ("source_rates", C.c_void_p), ("loss_rates", C.c_void_p), ("phi", C.c_void_p),
-                ("monod_vmax", C.c_void_p), ("monod_k", C.c_void_p)]
+                ("monod_vmax", C.c_void_p), ("monod_k", C.c_void_p),
+                ("advection_scheme", C.c_uint32)]
 
 
 def _ptr(a):
@@ -193,9 +195,10 @@ class Oracle:
 
     def _signal_grid(self, grid: Grid, keep: list, diffusion, boundary_values=None,
                      faces=None, advection=None, source=None, loss=None, phi=None,
-                     vmax=None, monod_k=None):
+                     vmax=None, monod_k=None, scheme=UPWIND):
         sg = _SignalGrid()
         sg.grid = grid._c(keep)
+        sg.advection_scheme = scheme
         diffusion = _f32(np.atleast_1d(diffusion))
         S = diffusion.size
         sg.signal_count = S
@@ -274,8 +277,8 @@ class Reference:
         P, U, F, Z = C.c_void_p, C.c_uint32, C.c_float, C.c_size_t
         L.ref_solve_depth_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, P, P, P,
                                            C.c_char_p, Z]
-        L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, F,
-                                              U, P, P, C.c_char_p, Z]
+        L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, U,
+                                              F, U, P, P, C.c_char_p, Z]
         L.ref_set_threads.argtypes = [U]
 
     @staticmethod
@@ -316,7 +319,8 @@ class Reference:
         return _RefFlow(self, grid, mobility, axis)
 
     def advance_signal_grid(self, grid: Grid, levels, dt, steps=1, diffusion=(1.0,),
-                            boundary_values=None, faces=None, source=None, loss=None):
+                            boundary_values=None, faces=None, source=None, loss=None,
+                            scheme=UPWIND):
         kinds = self._kinds(grid)
         obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
         diffusion = _f32(np.atleast_1d(diffusion))
@@ -339,8 +343,8 @@ class Reference:
             C.c_float(grid.dy), C.c_float(grid.dz), C.c_void_p(_ptr(obs)),
             C.c_void_p(kinds.ctypes.data), C.c_uint32(S), C.c_void_p(diffusion.ctypes.data),
             C.c_void_p(bv.ctypes.data), C.c_void_p(_ptr(fx)), C.c_void_p(_ptr(fy)),
-            C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)), C.c_void_p(_ptr(loss)), C.c_float(dt),
-            C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.byref(secs), err, C.c_size_t(512))
+            C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)), C.c_void_p(_ptr(loss)),
+            C.c_uint32(scheme), C.c_float(dt), C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.byref(secs), err, C.c_size_t(512))
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
         return lv, secs.value

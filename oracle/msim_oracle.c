@@ -789,6 +789,9 @@ int mo_adr_validate_step(const mo_signal_grid* sg, float dt, char* err, size_t e
       loss = (double)m;
     }
     if (sg->phi != NULL) loss += (double)sg->monod_vmax[s] * max_phi(sg) / (double)sg->monod_k[s];
+    /* A limited face flux carries up to twice the upwind one
+     * (signals.cpp:1089-1094). */
+    if (sg->advection_scheme == MO_VAN_LEER) courant_sum *= 2.0;
     const double factor =
         (double)dt * ((2.0 * (double)sg->diffusion[s] * inverse_square_sum) + courant_sum + loss);
     if (!isfinite(factor) || factor > 1.0)
@@ -806,6 +809,53 @@ static float monod_loss(const mo_signal_grid* sg, uint32_t s, size_t site, float
 
 /* transport_rates_into + operator_rates_into + forward Euler:
  * cpp/core/signals.cpp:1145-1291, 305-320, 1109-1134 (upwind scheme). */
+/* van_leer_increment (signals.cpp:202-206). */
+static float van_leer_increment(float upwind, float downwind, float far) {
+  const float behind = upwind - far;
+  const float ahead = downwind - upwind;
+  return behind * ahead > 0.0f ? behind * ahead / (behind + ahead) : 0.0f;
+}
+
+static double van_leer_increment_f64(double upwind, double downwind, double far) {
+  const double behind = upwind - far;
+  const double ahead = downwind - upwind;
+  return behind * ahead > 0.0 ? behind * ahead / (behind + ahead) : 0.0;
+}
+
+/* The level `offset` sites along `axis` from (x, y, z), or none (returns 0)
+ * where that site is off the grid or solid (signals.cpp:1238-1249). */
+static int far_level(const mo_signal_grid* sg, const float* lvf, const double* lvd, int axis,
+                     uint32_t x, uint32_t y, uint32_t z, int offset, int periodic, double* out) {
+  const mo_grid* g = &sg->grid;
+  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+  uint32_t c[3] = {x, y, z};
+  const int64_t along = (int64_t)c[axis] + offset;
+  const int64_t count = dims[axis];
+  if (!periodic && (along < 0 || along >= count)) return 0;
+  c[axis] = (uint32_t)((along + count) % count);
+  const size_t site = site_of(g, c[0], c[1], c[2]);
+  if (solid_at(g, site)) return 0;
+  *out = lvf != NULL ? (double)lvf[site] : lvd[site];
+  return 1;
+}
+
+/* settle_step_levels then validate_signal_levels (signals.cpp:979-1001,
+ * 754-784) over S*N levels: negatives within 16 epsilon of the largest level
+ * become zero; anything else negative or non-finite is refused. */
+static int settle_f32(float* levels, size_t count, char* err, size_t errlen) {
+  float largest = 0.0f;
+  for (size_t i = 0; i < count; ++i)
+    if (levels[i] > largest) largest = levels[i];
+  const float rounding = -16.0f * FLT_EPSILON * largest;
+  for (size_t i = 0; i < count; ++i)
+    if (levels[i] < 0.0f && levels[i] >= rounding) levels[i] = 0.0f;
+  for (size_t i = 0; i < count; ++i)
+    if (!isfinite(levels[i]) || levels[i] < 0.0f)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "signal grid levels must be finite and non-negative");
+  return MO_OK;
+}
+
 int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err, size_t errlen) {
   int status = validate_grid(&sg->grid, err, errlen);
   if (status != MO_OK) return status;
@@ -862,19 +912,56 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
             if (clo[a]) lower_flux = 0.0f;
             if (cup[a]) upper_flux = 0.0f;
             rate -= (upper_flux - lower_flux) * inverse_spacing;
+            if (sg->advection_scheme != MO_VAN_LEER) continue;
+            /* The limited correction of each face (signals.cpp:1227-1286):
+             * taken only where the upwind, downwind and second-upwind sites
+             * are fluid sites joined by open faces. */
+            const uint32_t coord[3] = {x, y, z};
+            const int periodic = g->boundary[2 * a] == MO_PERIODIC && dims[a] >= 3;
+            const int lower_site = !clo[a] && (coord[a] > 0 || periodic);
+            const int upper_site = !cup[a] && (coord[a] + 1 < dims[a] || periodic);
+            float lower_correction = 0.0f, upper_correction = 0.0f;
+            double beyond = 0.0;
+            if (lower_site) {
+              if (flo[a] >= 0.0f) {
+                if (far_level(sg, lv, NULL, a, x, y, z, -2, periodic, &beyond))
+                  lower_correction = flo[a] * van_leer_increment(lower[a], current, (float)beyond);
+              } else if (upper_site) {
+                lower_correction = flo[a] * van_leer_increment(current, lower[a], upper[a]);
+              }
+            }
+            if (upper_site) {
+              if (fup[a] >= 0.0f) {
+                if (lower_site)
+                  upper_correction = fup[a] * van_leer_increment(current, upper[a], lower[a]);
+              } else if (far_level(sg, lv, NULL, a, x, y, z, 2, periodic, &beyond)) {
+                upper_correction = fup[a] * van_leer_increment(upper[a], current, (float)beyond);
+              }
+            }
+            rate -= (upper_correction - lower_correction) * inverse_spacing;
           }
           const size_t index = (size_t)s * n + site;
           if (sg->source_rates != NULL)
             rate += sg->source_rates[index] - (sg->loss_rates[index] * current);
           if (sg->phi != NULL) rate += 0.0f - (monod_loss(sg, s, site, current) * current);
           const float candidate = current + (dt * rate);
-          if (!isfinite(candidate) || candidate < 0.0f) {
+          /* Under van Leer a level may fall below zero by rounding; the
+           * settle below judges it (signals.cpp:1119-1131). */
+          if (!isfinite(candidate) ||
+              (candidate < 0.0f && sg->advection_scheme != MO_VAN_LEER)) {
             free(updated);
             return fail(err, errlen, MO_DOMAIN_ERROR,
                         "signal grid update produced a non-finite or negative concentration");
           }
           updated[index] = candidate;
         }
+  }
+  if (sg->advection_scheme == MO_VAN_LEER) {
+    status = settle_f32(updated, total, err, errlen);
+    if (status != MO_OK) {
+      free(updated);
+      return status;
+    }
   }
   memcpy(levels, updated, total * sizeof(float));
   free(updated);
@@ -887,6 +974,27 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
  * uniform voxel the per-site rate equals the reference's
  * D(l - 2c + u)/h^2 - (F_u - F_l)/h (signals.cpp:1231-1247) in exact
  * arithmetic. */
+/* The limited correction of the face between sites lo and up (lo below up
+ * along `axis`, the face's velocity u), the same from both sides: u times
+ * van_leer_increment of the upwind, downwind and second-upwind levels, or 0
+ * where the second-upwind site is off the grid or solid (signals.cpp:1227-1286).
+ * lo_off and up_off: the offsets of lo - 1 and up + 1 from the site (x, y). */
+static double face_correction_f64(const mo_signal_grid* sg, const double* lv, int axis,
+                                  uint32_t x, uint32_t y, int lo_off, int up_off, double u,
+                                  double cl, double cu) {
+  const mo_grid* g = &sg->grid;
+  const uint32_t dims[2] = {g->nx, g->ny};
+  const int periodic = g->boundary[2 * axis] == MO_PERIODIC && dims[axis] >= 3;
+  double beyond = 0.0;
+  if (u >= 0.0) {
+    if (far_level(sg, NULL, lv, axis, x, y, 0, lo_off, periodic, &beyond))
+      return u * van_leer_increment_f64(cl, cu, beyond);
+  } else if (far_level(sg, NULL, lv, axis, x, y, 0, up_off, periodic, &beyond)) {
+    return u * van_leer_increment_f64(cu, cl, beyond);
+  }
+  return 0.0;
+}
+
 int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* ledger, char* err,
                     size_t errlen) {
   int status = validate_grid(&sg->grid, err, errlen);
@@ -898,19 +1006,22 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
     return fail(err, errlen, MO_INVALID_ARGUMENT, "the FP64 transport step takes one layer");
   const size_t n = (size_t)g->nx * g->ny;
   const size_t ny = g->ny;
+  const uint32_t S = sg->signal_count;
+  const int limited = sg->advection_scheme == MO_VAN_LEER;
   const double volume = (double)g->dx * g->dy * g->dz;
-  double* out = malloc(n * sizeof(double));
-  for (uint32_t s = 0; s < sg->signal_count; ++s) {
-    double* lv = levels + (size_t)s * n;
+  double* out = malloc(S * n * sizeof(double));
+  double* inflow = calloc(S, sizeof(double));
+  double* uptake = calloc(S, sizeof(double));
+  for (uint32_t s = 0; s < S; ++s) {
+    const double* lv = levels + (size_t)s * n;
+    double* ov = out + (size_t)s * n;
     const double D = (double)sg->diffusion[s];
     const double ihx = 1.0 / (double)g->dx, ihy = 1.0 / (double)g->dy;
-    double mass_before = 0.0, inflow = 0.0, uptake = 0.0, mass_after = 0.0;
-    for (size_t i = 0; i < n; ++i) mass_before += lv[i];
     for (uint32_t x = 0; x < g->nx; ++x)
       for (uint32_t y = 0; y < g->ny; ++y) {
         const size_t site = (size_t)x * ny + y;
         if (solid_at(g, site)) {
-          out[site] = lv[site];
+          ov[site] = lv[site];
           continue;
         }
         const double c = lv[site];
@@ -930,17 +1041,14 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
             double flux = 0.0;
             const int at_edge = lower_side ? coord == 0 : coord + 1 == dim;
             const int face = 2 * a + (lower_side ? 0 : 1);
+            size_t nb = 0;
+            int joined = 0;
             if (at_edge && g->boundary[face] == MO_PERIODIC) {
               /* A periodic face joins the site to its partner on the other
                * side, as an interior face does (signals.cpp:181-192,
                * 213-261). */
-              const size_t nb = lower_side ? site + (size_t)(dim - 1) * stride
-                                           : site - (size_t)(dim - 1) * stride;
-              if (!solid_at(g, nb)) {
-                const double cl = lower_side ? lv[nb] : c;
-                const double cu = lower_side ? c : lv[nb];
-                flux = (u >= 0.0 ? u * cl : u * cu) - D * (cu - cl) * ih;
-              }
+              nb = lower_side ? site + (size_t)(dim - 1) * stride : site - (size_t)(dim - 1) * stride;
+              joined = !solid_at(g, nb);
             } else if (at_edge) {
               if (g->boundary[face] == MO_FIXED) {
                 const double cb = (double)sg->boundary_values[6 * s + (uint32_t)face];
@@ -951,14 +1059,21 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
                   flux = (u >= 0.0 ? u * c : u * cb) - D * (cb - c) * ih;
                 /* Boundary supply into the grid (face area / volume folded in
                  * with ih below). */
-                inflow += (lower_side ? flux : -flux) * ih;
+                inflow[s] += (lower_side ? flux : -flux) * ih;
               }
             } else {
-              const size_t nb = lower_side ? site - stride : site + stride;
-              if (!solid_at(g, nb)) {
-                const double cl = lower_side ? lv[nb] : c;
-                const double cu = lower_side ? c : lv[nb];
-                flux = (u >= 0.0 ? u * cl : u * cu) - D * (cu - cl) * ih;
+              nb = lower_side ? site - stride : site + stride;
+              joined = !solid_at(g, nb);
+            }
+            if (joined) {
+              const double cl = lower_side ? lv[nb] : c;
+              const double cu = lower_side ? c : lv[nb];
+              flux = (u >= 0.0 ? u * cl : u * cu) - D * (cu - cl) * ih;
+              /* A face across a periodic boundary of fewer than three sites
+               * takes no correction (signals.cpp:1235-1257). */
+              if (limited && (!at_edge || dim >= 3)) {
+                flux += lower_side ? face_correction_f64(sg, lv, a, x, y, -2, 1, u, cl, cu)
+                                   : face_correction_f64(sg, lv, a, x, y, -1, 2, u, cl, cu);
               }
             }
             if (lower_side)
@@ -972,32 +1087,61 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
         if (sg->loss_rates != NULL) {
           const size_t index = (size_t)s * n + site;
           rate += (double)sg->source_rates[index];
-          inflow += (double)sg->source_rates[index];
+          inflow[s] += (double)sg->source_rates[index];
           loss += (double)sg->loss_rates[index];
         }
         if (sg->phi != NULL)
           loss += (double)sg->monod_vmax[s] * (double)sg->phi[site] / ((double)sg->monod_k[s] + c);
         rate -= loss * c;
-        uptake += loss * c;
+        uptake[s] += loss * c;
         const double next = c + (double)dt * rate;
-        if (!isfinite(next) || next < 0.0) {
+        if (!isfinite(next) || (next < 0.0 && !limited)) {
           free(out);
+          free(inflow);
+          free(uptake);
           return fail(err, errlen, MO_DOMAIN_ERROR,
                       "signal grid update produced a non-finite or negative concentration");
         }
-        out[site] = next;
+        ov[site] = next;
       }
+  }
+  if (limited) {
+    /* The settle rule of the FP32 step (signals.cpp:979-1001) at FP64
+     * rounding: negatives within 16 epsilon of the largest level become
+     * zero; anything else negative is refused. */
+    double largest = 0.0;
+    for (size_t i = 0; i < S * n; ++i)
+      if (out[i] > largest) largest = out[i];
+    const double rounding = -16.0 * DBL_EPSILON * largest;
+    for (size_t i = 0; i < S * n; ++i) {
+      if (out[i] < 0.0 && out[i] >= rounding) out[i] = 0.0;
+      if (!(out[i] >= 0.0)) {
+        free(out);
+        free(inflow);
+        free(uptake);
+        return fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "signal grid levels must be finite and non-negative");
+      }
+    }
+  }
+  for (uint32_t s = 0; s < S; ++s) {
+    double* lv = levels + (size_t)s * n;
+    const double* ov = out + (size_t)s * n;
+    double mass_before = 0.0, mass_after = 0.0;
     for (size_t i = 0; i < n; ++i) {
-      lv[i] = out[i];
-      mass_after += out[i];
+      mass_before += lv[i];
+      mass_after += ov[i];
+      lv[i] = ov[i];
     }
     if (ledger != NULL) {
       ledger[4 * s + 0] = mass_before * volume;
       ledger[4 * s + 1] = mass_after * volume;
-      ledger[4 * s + 2] = (double)dt * inflow * volume;
-      ledger[4 * s + 3] = (double)dt * uptake * volume;
+      ledger[4 * s + 2] = (double)dt * inflow[s] * volume;
+      ledger[4 * s + 3] = (double)dt * uptake[s] * volume;
     }
   }
   free(out);
+  free(inflow);
+  free(uptake);
   return MO_OK;
 }

@@ -107,13 +107,18 @@ typedef struct {
   const float* phi;             /* Monod occupancy N, or NULL (no Monod) */
   const float* monod_vmax;      /* S */
   const float* monod_k;         /* S */
+  uint32_t advection_scheme;    /* MO_UPWIND or MO_VAN_LEER (SignalAdvectionKind) */
 } mo_signal_grid;
+
+enum { MO_UPWIND = 0, MO_VAN_LEER = 1 };
 
 /* The explicit stability bound of SignalGrid::validate_step. For Monod the
  * loss bound is vmax * max(phi) / K. Returns MO_OK or MO_INVALID_ARGUMENT. */
 int mo_adr_validate_step(const mo_signal_grid* sg, float dt, char* err, size_t errlen);
 
-/* One forward-Euler step in FP32, levels S*N in place (reference order). */
+/* One forward-Euler step in FP32, levels S*N in place (reference order).
+ * Under van Leer the step's levels are settled and validated as
+ * SignalGrid::accept_step does (signals.cpp:979-1011). */
 int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err, size_t errlen);
 
 /* One forward-Euler step with FP64 state in face-flux form. ledger (may be

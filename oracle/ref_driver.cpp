@@ -225,13 +225,15 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
                             const std::uint8_t* kinds, std::uint32_t signals,
                             const float* diffusion, const float* boundary_values,
                             const float* x_faces, const float* y_faces, const float* z_faces,
-                            const float* source_rates, const float* loss_rates, float dt,
-                            std::uint32_t steps, float* levels, double* seconds, char* err,
-                            std::size_t errlen) {
+                            const float* source_rates, const float* loss_rates,
+                            std::uint32_t scheme, float dt, std::uint32_t steps, float* levels,
+                            double* seconds, char* err, std::size_t errlen) {
   return guarded(err, errlen, [&] {
     auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
     const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
     spec.diffusion.assign(diffusion, diffusion + signals);
+    spec.advection_scheme =
+        scheme == 1 ? ms::SignalAdvectionKind::van_leer : ms::SignalAdvectionKind::upwind;
     if (x_faces != nullptr) {
       ms::SignalGridVelocityField field;
       field.x_faces.assign(x_faces, x_faces + (nx + 1) * static_cast<std::size_t>(ny) * nz);

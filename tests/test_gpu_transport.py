@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import microsimulator_amd as msa
+from oracle import VAN_LEER
 from oracle import Grid as OGrid
 from tests.helpers import (FIXED, NO_FLUX, PERIODIC, TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs,
                            golden, random_transport_case, split_faces)
@@ -115,6 +116,76 @@ def test_marching_step_on_every_shape(hip, oracle, shape, kinds):
     assert np.allclose(ledger, ref_ledger, rtol=1e-12, atol=1e-300)
     balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
     assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+
+
+VAN_LEER_SHAPES = TRANSPORT_SHAPES + [(2, 30), (30, 2), (3, 17)]
+
+
+@pytest.mark.parametrize("shape", VAN_LEER_SHAPES)
+@pytest.mark.parametrize("kinds", KINDS)
+def test_van_leer_step_on_every_shape(hip, oracle, shape, kinds):
+    """The limited (van Leer) step: FP32 bit for bit with the oracle (which is
+    bit-exact with the reference's CPU step), FP64 to 1e-13 with a balanced
+    ledger, on every boundary kind, axes of two and three sites and strips of
+    every width."""
+    nx, ny = shape
+    d = random_transport_case(nx, ny, 2, kinds, seed=nx * 1000 + ny)
+    grid = msa.Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], kinds)
+    spec = msa.SignalGridSpec(grid, d["diffusion"], d["bv"],
+                              msa.VelocityField(d["xf"], d["yf"], d["zf"]),
+                              source_rates=d["source"], loss_rates=d["loss"], phi=d["phi"],
+                              monod_vmax=[0.7, 0.3], monod_k=[0.2, 0.5],
+                              advection_scheme=msa.VAN_LEER)
+    og = OGrid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], kinds)
+    kw = dict(diffusion=d["diffusion"], boundary_values=d["bv"], source=d["source"],
+              loss=d["loss"], faces=np.concatenate([d["xf"], d["yf"], d["zf"]]), phi=d["phi"],
+              vmax=[0.7, 0.3], monod_k=[0.2, 0.5], scheme=VAN_LEER)
+    dt = 0.02
+    print("case", shape, kinds, flush=True)
+    lv32 = d["levels"].astype(np.float32)
+    ref32 = oracle.adr_step_f32(og, lv32, dt, steps=3, **kw)
+    hip.adr_upload(spec, lv32, precision=32)
+    hip.adr_run(dt, 3, ledger=False)
+    assert np.array_equal(hip.adr_download(), ref32)
+    one, _ = hip.advance_signal_grid(spec, lv32, dt)
+    assert np.array_equal(one, oracle.adr_step_f32(og, lv32, dt, **kw))
+    hip.adr_upload(spec, d["levels"], precision=64)
+    ledger = hip.adr_run(dt, 3)
+    ref64, ref_ledger = oracle.adr_step_f64(og, d["levels"], dt, steps=3, **kw)
+    ours = hip.adr_download()
+    assert np.abs(ours - ref64).max() <= 1e-13 * np.abs(ref64).max()
+    assert np.allclose(ledger, ref_ledger, rtol=1e-12, atol=1e-300)
+    balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+    assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+
+
+def test_van_leer_front_and_bound(hip, oracle):
+    """A step front carried 40 steps at half the limited bound matches the
+    oracle bit for bit and stays within [0, 1]; a step between the limited
+    and the upwind bounds is refused only under van Leer."""
+    nx, ny = 300, 70
+    lv = np.zeros(nx * ny, np.float32)
+    lv.reshape(nx, ny)[:40] = 1.0
+    xf = np.ones((nx + 1) * ny, np.float32)
+    xf[:ny] = 0.0
+    xf[nx * ny:] = 0.0
+    yf = np.zeros(nx * (ny + 1), np.float32)
+    zf = np.zeros(2 * nx * ny, np.float32)
+    grid = msa.Grid(nx, ny, 1, 1.0, 1.0, 1.0, None, (NO_FLUX,) * 6)
+    spec = msa.SignalGridSpec(grid, [0.0], velocity=msa.VelocityField(xf, yf, zf),
+                              advection_scheme=msa.VAN_LEER)
+    og = OGrid(nx, ny, 1, 1.0, 1.0, 1.0, None, (NO_FLUX,) * 6)
+    ref = oracle.adr_step_f32(og, lv, 0.49, steps=40, scheme=VAN_LEER, diffusion=0.0,
+                              faces=np.concatenate([xf, yf, zf]))
+    hip.adr_upload(spec, lv, precision=32)
+    hip.adr_run(0.49, 40, ledger=False)
+    ours = hip.adr_download()
+    assert np.array_equal(ours, ref)
+    assert ours.min() >= 0.0 and ours.max() <= 1.0
+    with pytest.raises(msa.InvalidArgument, match="explicit stability bound"):
+        hip.advance_signal_grid(spec, lv, 0.51)
+    upwind = msa.SignalGridSpec(grid, [0.0], velocity=msa.VelocityField(xf, yf, zf))
+    hip.advance_signal_grid(upwind, lv, 0.51)
 
 
 def test_explicit_bound_refuses_and_leaves_levels(hip):

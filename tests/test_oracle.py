@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from oracle import FIXED, NO_FLUX, PERIODIC, Grid, OracleError
+from oracle import FIXED, NO_FLUX, PERIODIC, VAN_LEER, Grid, OracleError
 from tests.helpers import (TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs, divergence, golden,
                            illcond_inputs, monod_loss, random_case, random_transport_case,
                            split_faces)
@@ -320,6 +320,92 @@ def test_fp64_step_agrees_with_fp32_step(oracle, kinds):
                   phi=d["phi"], vmax=[0.7, 0.3], monod_k=[0.2, 0.5])
         a = oracle.adr_step_f32(grid, d["levels"].astype(np.float32), 0.02, steps=3, **kw)
         b, ledger = oracle.adr_step_f64(grid, d["levels"], 0.02, steps=3, **kw)
+        assert np.abs(a - b).max() <= 1e-6, (nx, ny)
+        balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+        assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+
+
+VAN_LEER_SHAPES = TRANSPORT_SHAPES + [(2, 30), (30, 2), (3, 17)]
+
+
+def van_leer_kw(d, monod=False):
+    kw = dict(diffusion=d["diffusion"], boundary_values=d["bv"], source=d["source"],
+              loss=d["loss"], faces=np.concatenate([d["xf"], d["yf"], d["zf"]]))
+    if monod:
+        kw.update(phi=d["phi"], vmax=[0.7, 0.3], monod_k=[0.2, 0.5])
+    return kw
+
+
+@pytest.mark.parametrize("kinds", TRANSPORT_KINDS)
+def test_van_leer_transport_matches_reference_live(oracle, reference, kinds):
+    """The limited (van Leer) step, bit for bit with advance_signal_grid_cpu,
+    on every boundary kind, axes of two and three sites (where a periodic
+    axis stops or starts taking corrections) and ragged shapes."""
+    for nx, ny in VAN_LEER_SHAPES:
+        d = random_transport_case(nx, ny, 2, kinds, seed=nx * 1000 + ny)
+        grid = Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], boundary=kinds)
+        lv = d["levels"].astype(np.float32)
+        a = oracle.adr_step_f32(grid, lv, 0.02, steps=3, scheme=VAN_LEER, **van_leer_kw(d))
+        b, _ = reference.advance_signal_grid(grid, lv, 0.02, 3, scheme=VAN_LEER, **van_leer_kw(d))
+        assert np.array_equal(a, b), (nx, ny)
+        upwind = oracle.adr_step_f32(grid, lv, 0.02, steps=3, **van_leer_kw(d))
+        assert np.abs(a - upwind).max() > 1e-4  # the correction is active
+
+
+def test_van_leer_front_matches_reference_live(oracle, reference):
+    """A step front carried at half the limited scheme's bound for 40 steps:
+    levels stay within [0, 1] and match the reference bit for bit."""
+    nx, ny = 60, 8
+    lv = np.zeros(nx * ny, np.float32)
+    lv.reshape(nx, ny)[:15] = 1.0
+    xf = np.ones((nx + 1) * ny, np.float32)
+    xf[:ny] = 0.0
+    xf[nx * ny:] = 0.0
+    faces = np.concatenate([xf, np.zeros(nx * (ny + 1), np.float32),
+                            np.zeros(2 * nx * ny, np.float32)])
+    grid = Grid(nx, ny, 1, 1.0, 1.0, 1.0, None, boundary=(NO_FLUX,) * 6)
+    a = oracle.adr_step_f32(grid, lv, 0.49, steps=40, scheme=VAN_LEER, diffusion=0.0, faces=faces)
+    b, _ = reference.advance_signal_grid(grid, lv, 0.49, 40, scheme=VAN_LEER, diffusion=0.0,
+                                         faces=faces)
+    assert np.array_equal(a, b)
+    assert a.min() >= 0.0 and a.max() <= 1.0
+    # Mass is conserved exactly up to float rounding on a closed domain.
+    assert abs(float(a.astype(np.float64).sum()) - 15 * ny) <= 1e-4
+
+
+def test_van_leer_halves_the_advective_bound(oracle, reference):
+    """dt between the limited and the upwind bounds: only the limited step is
+    refused, by both."""
+    nx, ny = 20, 6
+    xf = np.full((nx + 1) * ny, 2.0, np.float32)
+    xf[:ny] = 0.0
+    xf[nx * ny:] = 0.0
+    faces = np.concatenate([xf, np.zeros(nx * (ny + 1), np.float32),
+                            np.zeros(2 * nx * ny, np.float32)])
+    grid = Grid(nx, ny, 1, 1.0, 1.0, 1.0, None, boundary=(NO_FLUX,) * 6)
+    lv = np.ones(nx * ny, np.float32)
+    dt = 0.4  # upwind bound 0.5, limited 0.25
+    oracle.adr_step_f32(grid, lv, dt, diffusion=0.0, faces=faces)
+    reference.advance_signal_grid(grid, lv, dt, 1, diffusion=0.0, faces=faces)
+    with pytest.raises(OracleError, match="explicit stability bound"):
+        oracle.adr_step_f32(grid, lv, dt, scheme=VAN_LEER, diffusion=0.0, faces=faces)
+    with pytest.raises(OracleError, match="explicit stability bound"):
+        reference.advance_signal_grid(grid, lv, dt, 1, scheme=VAN_LEER, diffusion=0.0,
+                                      faces=faces)
+
+
+@pytest.mark.parametrize("kinds", TRANSPORT_KINDS)
+def test_fp64_van_leer_agrees_and_conserves(oracle, kinds):
+    """The FP64 face-flux form of the limited step: each face's correction is
+    one expression of the sites around it, so the ledger balances; it agrees
+    with the FP32 step (bit-exact with the reference) to FP32 rounding."""
+    for nx, ny in VAN_LEER_SHAPES:
+        d = random_transport_case(nx, ny, 2, kinds, seed=nx * 1000 + ny)
+        grid = Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], boundary=kinds)
+        kw = van_leer_kw(d, monod=True)
+        a = oracle.adr_step_f32(grid, d["levels"].astype(np.float32), 0.02, steps=3,
+                                scheme=VAN_LEER, **kw)
+        b, ledger = oracle.adr_step_f64(grid, d["levels"], 0.02, steps=3, scheme=VAN_LEER, **kw)
         assert np.abs(a - b).max() <= 1e-6, (nx, ny)
         balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
         assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
