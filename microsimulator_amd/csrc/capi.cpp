@@ -76,42 +76,43 @@ void validate_lattice(const msim_grid& g) {
 }
 
 // Inlet-to-outlet reachability through sites not in `solid`
-// (validate_flow_grid, cpp/cpu/cpu_flow.cpp:288-336), one layer.
-void check_reachable(const msim_grid& g, const std::vector<uint8_t>& solid, uint32_t axis) {
-  const uint32_t nx = g.nx, ny = g.ny;
-  const size_t n = static_cast<size_t>(nx) * ny;
+// (validate_flow_grid, cpp/cpu/cpu_flow.cpp:288-336); site (x*ny + y)*nz + z.
+void check_reachable(uint32_t nx, uint32_t ny, uint32_t nz, const std::vector<uint8_t>& solid,
+                     uint32_t axis) {
+  const size_t n = static_cast<size_t>(nx) * ny * nz;
   std::vector<uint8_t> visited(n, 0);
-  std::vector<uint32_t> pending;
+  std::vector<size_t> pending;
   pending.reserve(1024);
-  const uint32_t across = axis == 0 ? ny : nx;
-  for (uint32_t t = 0; t < across; ++t) {
-    const size_t site = axis == 0 ? t : static_cast<size_t>(t) * ny;
-    if (solid[site] == 0) {
+  const uint32_t dims[3] = {nx, ny, nz};
+  const size_t step[3] = {static_cast<size_t>(ny) * nz, nz, 1};
+  auto coord = [&](size_t site, int a) -> uint32_t {
+    return static_cast<uint32_t>((site / step[a]) % dims[a]);
+  };
+  for (size_t site = 0; site < n; ++site) {
+    if (solid[site] == 0 && coord(site, static_cast<int>(axis)) == 0) {
       visited[site] = 1;
-      pending.push_back(static_cast<uint32_t>(site));
+      pending.push_back(site);
     }
   }
   if (pending.empty()) invalid("the flow inlet boundary is entirely blocked");
-  const uint32_t dims[2] = {nx, ny};
   while (!pending.empty()) {
     const size_t site = pending.back();
     pending.pop_back();
-    const uint32_t c[2] = {static_cast<uint32_t>(site / ny), static_cast<uint32_t>(site % ny)};
-    if (c[axis] + 1 == dims[axis]) return;
-    const size_t step[2] = {ny, 1};
-    for (int a = 0; a < 2; ++a) {
-      if (c[a] > 0) {
+    if (coord(site, static_cast<int>(axis)) + 1 == dims[axis]) return;
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t c = coord(site, a);
+      if (c > 0) {
         const size_t nb = site - step[a];
         if (!visited[nb] && solid[nb] == 0) {
           visited[nb] = 1;
-          pending.push_back(static_cast<uint32_t>(nb));
+          pending.push_back(nb);
         }
       }
-      if (c[a] + 1 < dims[a]) {
+      if (c + 1 < dims[a]) {
         const size_t nb = site + step[a];
         if (!visited[nb] && solid[nb] == 0) {
           visited[nb] = 1;
-          pending.push_back(static_cast<uint32_t>(nb));
+          pending.push_back(nb);
         }
       }
     }
@@ -120,9 +121,16 @@ void check_reachable(const msim_grid& g, const std::vector<uint8_t>& solid, uint
 }
 
 // The checks reduce_shallow_flow and the flow system's constructor make
-// before a solve (flow_system.hpp:1255-1266, 260-330; cpu_flow.cpp:288-336).
+// before a solve (flow_system.hpp:1255-1266, 260-330; cpu_flow.cpp:288-336),
+// and the reduction to a grid of columns (ShallowFlowReduction,
+// flow_system.hpp:257-330): `solid` and `mob` are what the device solve on
+// the grid of columns takes (mask and per-site mobility, the conductance
+// being dz times the mobility). One layer keeps the user's mobility; several
+// layers pass each column's conductance with dz = 1, so the device's
+// conductance is that value exactly.
 void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
-                   std::vector<uint8_t>& solid, std::vector<float>& mob) {
+                   std::vector<uint8_t>& solid, std::vector<float>& mob,
+                   FlowDevice::Layers& layers) {
   validate_lattice(g);
   if (axis > 2) invalid("unknown flow axis");
   for (int a = 0; a < 3; ++a) {
@@ -133,30 +141,192 @@ void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
   if (g.boundary[2 * axis] != MSIM_FIXED || g.boundary[2 * axis + 1] != MSIM_FIXED) {
     invalid("native flow-axis boundaries must be fixed to identify inlet and outlet");
   }
+  const size_t voxels = static_cast<size_t>(g.nx) * g.ny * g.nz;
+  std::vector<uint8_t> mask(voxels, 0);
+  if (g.obstacles != nullptr) std::memcpy(mask.data(), g.obstacles, voxels);
+  check_reachable(g.nx, g.ny, g.nz, mask, axis);
   if (axis == 2) invalid("shallow flow is depth-integrated along z; flow axis must be x or y");
-  if (g.nz != 1) {
-    throw Error(MSIM_UNSUPPORTED, "the HIP depth-averaged path takes a grid of one layer (nz = 1)");
-  }
   const size_t n = static_cast<size_t>(g.nx) * g.ny;
-  solid.assign(n, 0);
-  if (g.obstacles != nullptr) std::memcpy(solid.data(), g.obstacles, n);
-  check_reachable(g, solid, axis);
-  mob.assign(n, 1.0f);
-  if (mobility != nullptr) {
-    for (size_t i = 0; i < n; ++i) {
-      if (!std::isfinite(mobility[i]) || mobility[i] < 0.0f) {
+  const uint32_t nz = g.nz;
+  layers = FlowDevice::Layers{};
+  layers.nz = nz;
+  layers.dz = g.dz;
+  // Columns (flow_system.hpp:276-327): contiguous fluid voxels from a floor
+  // common to every column, one mobility per column; height spacing.z per
+  // fluid voxel, conductance height * mobility, or without mobilities
+  // height * (height / max height)^2.
+  std::vector<float> heights(n, 0.0f), conductance(n, 0.0f);
+  std::vector<uint8_t> columns(n, 1);
+  uint32_t common_bottom = nz;
+  float max_height = 0.0f;
+  for (size_t column = 0; column < n; ++column) {
+    bool started = false, ended = false;
+    float value = 1.0f;
+    for (uint32_t z = 0; z < nz; ++z) {
+      const size_t site = column * nz + z;
+      if (mobility != nullptr && (!std::isfinite(mobility[site]) || mobility[site] < 0.0f)) {
         invalid("flow mobility must be finite and non-negative");
       }
+      if (mask[site] != 0) {
+        if (started) ended = true;
+        continue;
+      }
+      if (ended) invalid("shallow flow needs contiguous fluid columns; use resolved flow");
+      const float next = mobility == nullptr ? 1.0f : mobility[site];
+      if (!started) {
+        if (common_bottom == nz) common_bottom = z;
+        if (z != common_bottom) invalid("shallow flow needs a common planar floor; use resolved flow");
+        value = next;
+      } else if (std::abs(next - value) > 2.0e-6f * std::max(std::abs(value), std::abs(next))) {
+        invalid("shallow-flow mobility must be constant through each depth column");
+      }
+      started = true;
+      heights[column] += g.dz;
     }
-    std::memcpy(mob.data(), mobility, n * sizeof(float));
+    if (started) {
+      columns[column] = 0;
+      conductance[column] = heights[column] * value;
+      max_height = std::max(max_height, heights[column]);
+    }
   }
-  // Columns without conductance close (ShallowFlowReduction, 326-327); the
-  // reduced grid must still carry flow.
-  std::vector<uint8_t> reduced(solid);
+  if (mobility == nullptr && max_height > 0.0f) {
+    for (size_t i = 0; i < n; ++i) {
+      const float relative = heights[i] / max_height;
+      conductance[i] *= relative * relative;
+    }
+  }
   for (size_t i = 0; i < n; ++i) {
-    if (reduced[i] == 0 && g.dz * mob[i] == 0.0f) reduced[i] = 1;
+    if (conductance[i] == 0.0f) columns[i] = 1;
   }
-  check_reachable(g, reduced, axis);
+  // The grid of columns must still carry flow.
+  check_reachable(g.nx, g.ny, 1, columns, axis);
+  if (nz == 1) {
+    solid = std::move(mask);
+    mob.assign(n, 1.0f);
+    if (mobility != nullptr) std::memcpy(mob.data(), mobility, n * sizeof(float));
+    return;
+  }
+  solid = columns;
+  mob = conductance;
+  layers.bottom = common_bottom;
+  layers.voxels = std::move(mask);
+  layers.heights = std::move(heights);
+  layers.conductance = std::move(conductance);
+}
+
+// ShallowFlowReduction::lift (flow_system.hpp:424-470): each x and y voxel
+// face between fluid voxels carries its column face's flux over the smaller
+// of the two columns' heights; the vertical faces follow by continuity,
+// integrating the horizontal divergence upward with the column's mean
+// removed so floor and roof stay closed. flux: x|y faces of the grid of
+// columns. Returns x|y|z voxel faces.
+std::vector<float> lift_columns(const FlowDevice& f, const std::vector<float>& flux) {
+  const auto& L = f.layers;
+  const uint32_t nx = f.nx, ny = f.ny, nz = L.nz;
+  const size_t nxf = static_cast<size_t>(nx + 1) * ny * nz;
+  const size_t nyf = static_cast<size_t>(nx) * (ny + 1) * nz;
+  const size_t nzf = static_cast<size_t>(nx) * ny * (nz + 1);
+  std::vector<float> v(nxf + nyf + nzf, 0.0f);
+  auto solid = [&](size_t column, uint32_t z) { return L.voxels[column * nz + z] != 0; };
+  auto xface = [&](uint32_t fx, uint32_t y, uint32_t z) {
+    return (static_cast<size_t>(fx) * ny + y) * nz + z;
+  };
+  auto yface = [&](uint32_t x, uint32_t fy, uint32_t z) {
+    return nxf + (static_cast<size_t>(x) * (ny + 1) + fy) * nz + z;
+  };
+  auto zface = [&](uint32_t x, uint32_t y, uint32_t fz) {
+    return nxf + nyf + (static_cast<size_t>(x) * ny + y) * (nz + 1) + fz;
+  };
+  const size_t cxf = static_cast<size_t>(nx + 1) * ny;  // column x faces
+  for (int comp = 0; comp < 2; ++comp) {
+    const uint32_t fdx = comp == 0 ? nx + 1 : nx, fdy = comp == 0 ? ny : ny + 1;
+    const uint32_t dim = comp == 0 ? nx : ny;
+    for (uint32_t x = 0; x < fdx; ++x) {
+      for (uint32_t y = 0; y < fdy; ++y) {
+        const uint32_t along = comp == 0 ? x : y;
+        const bool has_lower = along > 0, has_upper = along < dim;
+        const size_t up_col = static_cast<size_t>(x) * ny + y;
+        const size_t low_col = comp == 0 ? up_col - ny : up_col - 1;
+        const size_t a_col = has_lower ? low_col : up_col;
+        const size_t b_col = has_upper ? up_col : a_col;
+        const float height = std::min(L.heights[a_col], L.heights[b_col]);
+        const float q = comp == 0 ? flux[static_cast<size_t>(x) * ny + y]
+                                  : flux[cxf + static_cast<size_t>(x) * (ny + 1) + y];
+        for (uint32_t z = 0; z < nz; ++z) {
+          if ((has_lower && solid(low_col, z)) || (has_upper && solid(up_col, z))) continue;
+          if (height > 0.0f) v[comp == 0 ? xface(x, y, z) : yface(x, y, z)] = q / height;
+        }
+      }
+    }
+  }
+  std::vector<double> horizontal;
+  for (uint32_t x = 0; x < nx; ++x) {
+    for (uint32_t y = 0; y < ny; ++y) {
+      const size_t column = static_cast<size_t>(x) * ny + y;
+      horizontal.clear();
+      double sum = 0.0;
+      for (uint32_t z = L.bottom; z < nz; ++z) {
+        if (solid(column, z)) break;
+        const double dxv = (v[xface(x + 1, y, z)] - v[xface(x, y, z)]) / f.dx;
+        const double dyv = (v[yface(x, y + 1, z)] - v[yface(x, y, z)]) / f.dy;
+        horizontal.push_back(dxv + dyv);
+        sum += dxv + dyv;
+      }
+      if (horizontal.empty()) continue;
+      const double mean = sum / static_cast<double>(horizontal.size());
+      double vertical = 0.0;
+      for (size_t k = 0; k + 1 < horizontal.size(); ++k) {
+        vertical -= L.dz * (horizontal[k] - mean);
+        v[zface(x, y, L.bottom + static_cast<uint32_t>(k) + 1)] = static_cast<float>(vertical);
+      }
+    }
+  }
+  return v;
+}
+
+// scale_velocity (flow_system.hpp:1368-1415) over the lifted voxel faces: the
+// open inlet faces are the axis's lower faces of fluid voxels in columns with
+// conductance (open_inlet_faces, 449-459).
+void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_report& report) {
+  auto& v = f.layers.faces;
+  const auto& L = f.layers;
+  const uint32_t nx = f.nx, ny = f.ny, nz = L.nz;
+  float peak = 0.0f;
+  for (const float value : v) {
+    if (!std::isfinite(value)) throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+    peak = std::max(peak, std::abs(value));
+  }
+  const size_t nxf = static_cast<size_t>(nx + 1) * ny * nz;
+  double inlet_sum = 0.0;
+  size_t inlet_count = 0;
+  const uint32_t across = f.axis == 0 ? ny : nx;
+  for (uint32_t t = 0; t < across; ++t) {
+    const size_t column = f.axis == 0 ? t : static_cast<size_t>(t) * ny;
+    for (uint32_t z = 0; z < nz; ++z) {
+      if (L.voxels[column * nz + z] != 0 || !(L.conductance[column] > 0.0f)) continue;
+      const size_t face = f.axis == 0 ? static_cast<size_t>(t) * nz + z
+                                      : nxf + static_cast<size_t>(t) * (ny + 1) * nz + z;
+      inlet_sum += v[face];
+      ++inlet_count;
+    }
+  }
+  if (inlet_count == 0) throw Error(MSIM_RUNTIME_ERROR, "flow solve has no open inlet faces");
+  const float solved_mean = static_cast<float>(inlet_sum / static_cast<double>(inlet_count));
+  if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
+    throw Error(MSIM_RUNTIME_ERROR, "the device carries no through-flow: the outlet is unreachable");
+  }
+  const float driven_mean = static_cast<float>(0.0 / static_cast<double>(inlet_count));
+  const float factor =
+      params.scale != 0 ? (params.mean_inlet_speed - driven_mean) / solved_mean : 1.0f;
+  float scaled_peak = 0.0f;
+  for (float& value : v) {
+    if (params.scale != 0) value = value * factor + 0.0f;
+    scaled_peak = std::max(scaled_peak, std::abs(value));
+  }
+  report.mean_inlet_speed = params.scale != 0 ? params.mean_inlet_speed : solved_mean;
+  report.max_speed = scaled_peak;
+  report.solved_mean = solved_mean;
+  report.factor = factor;
 }
 
 void validate_flow_params(const msim_flow_params& p) {
@@ -396,13 +566,14 @@ int msim_hip_flow_prepare(msim_ctx* ctx, const msim_grid* grid, const float* mob
     if (grid == nullptr) msim::invalid("grid must not be null");
     std::vector<uint8_t> solid;
     std::vector<float> mob;
-    msim::validate_flow(*grid, mobility, axis, solid, mob);
     auto& f = c.flow;
+    msim::validate_flow(*grid, mobility, axis, solid, mob, f.layers);
     f.nx = grid->nx;
     f.ny = grid->ny;
     f.dx = grid->dx;
     f.dy = grid->dy;
-    f.dz = grid->dz;
+    // Several layers: the conductance goes up as the mobility with dz = 1.
+    f.dz = grid->nz == 1 ? grid->dz : 1.0f;
     f.axis = axis;
     f.prepared = false;
     msim::flow_assemble(c, solid.data(), mob.data());
@@ -415,7 +586,24 @@ int msim_hip_flow_solve(msim_ctx* ctx, const msim_flow_params* params, msim_flow
     if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
     msim::validate_flow_params(*params);
     if (params->axis != c.flow.axis) msim::invalid("the flow axis differs from the prepared one");
-    msim::flow_solve(c, *params, *report);
+    auto& f = c.flow;
+    if (f.layers.nz == 1) {
+      msim::flow_solve(c, *params, *report);
+      return;
+    }
+    // Several layers: the device solves the grid of columns and reconstructs
+    // the column flux (unscaled, dz = 1); the host lifts it onto the voxel
+    // faces and scales it to the mean inlet speed.
+    msim_flow_params column = *params;
+    column.scale = 0;
+    msim::flow_solve(c, column, *report);
+    std::vector<float> flux(f.x_faces() + f.y_faces());
+    msim::check(hipMemcpyAsync(flux.data(), f.faces.data(), flux.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, c.stream),
+                "download column flux");
+    msim::check(hipStreamSynchronize(c.stream), "download column flux");
+    f.layers.faces = msim::lift_columns(f, flux);
+    msim::scale_layers(f, *params, *report);
   });
 }
 
@@ -428,6 +616,19 @@ int msim_hip_flow_download(msim_ctx* ctx, double* pressure, float* x_faces, floa
       msim::check(hipMemcpyAsync(pressure, f.x.data(), f.sites() * sizeof(double),
                                  hipMemcpyDeviceToHost, c.stream),
                   "download pressure");
+    }
+    if (f.layers.nz > 1) {
+      // The lifted voxel faces of a grid of several layers.
+      msim::check(hipStreamSynchronize(c.stream), "download");
+      const auto& v = f.layers.faces;
+      const size_t nz = f.layers.nz;
+      const size_t nxf = f.x_faces() * nz, nyf = f.y_faces() * nz;
+      if (x_faces != nullptr) std::memcpy(x_faces, v.data(), nxf * sizeof(float));
+      if (y_faces != nullptr) std::memcpy(y_faces, v.data() + nxf, nyf * sizeof(float));
+      if (z_faces != nullptr) {
+        std::memcpy(z_faces, v.data() + nxf + nyf, (v.size() - nxf - nyf) * sizeof(float));
+      }
+      return;
     }
     if (x_faces != nullptr) {
       msim::check(hipMemcpyAsync(x_faces, f.faces.data(), f.x_faces() * sizeof(float),
@@ -625,6 +826,7 @@ int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void*
     if (precision != 32 && precision != 64) msim::invalid("precision must be 32 or 64");
     if (use_flow_field != 0) {
       if (!c.flow.solved) msim::invalid("no flow field on the device");
+      if (c.flow.layers.nz != 1) msim::invalid("the flow field on the device is of a layered grid");
       if (c.flow.nx != grid->grid.nx || c.flow.ny != grid->grid.ny) {
         msim::invalid("the flow field is for another grid");
       }

@@ -152,6 +152,18 @@ struct FlowDevice {
   DeviceBuffer<float> faces;       // x faces then y faces (z faces are zero)
   CgState* host_state = nullptr;   // pinned readback
   MgDevice mg;
+  // A grid of several layers is solved on its grid of columns (the device
+  // arrays above); the host keeps what lifting the column flux back onto the
+  // voxel faces needs, and the lifted field.
+  struct Layers {
+    uint32_t nz = 1;
+    float dz = 1.0f;                 // voxel spacing along z
+    uint32_t bottom = 0;             // common floor of the fluid columns
+    std::vector<uint8_t> voxels;     // original obstacle mask, nx*ny*nz
+    std::vector<float> heights;      // per column
+    std::vector<float> conductance;  // per column
+    std::vector<float> faces;        // lifted x|y|z voxel faces after a solve
+  } layers;
   [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(nx) * ny; }
   [[nodiscard]] std::size_t x_faces() const { return static_cast<std::size_t>(nx + 1) * ny; }
   [[nodiscard]] std::size_t y_faces() const { return static_cast<std::size_t>(nx) * (ny + 1); }

@@ -179,7 +179,9 @@ typedef struct {
   float* a_diag;
   float* a_rhs;
   float* a_bc;
-  float heights; /* column height (spacing.z) of every fluid column */
+  float* heights;  /* per column: spacing.z times its fluid voxels */
+  uint32_t bottom; /* the common floor of the fluid columns */
+  mo_grid grid2;   /* the grid of columns: nz = 1, reduced_solid as obstacles */
 } system_t;
 
 static void system_free(system_t* s) {
@@ -196,25 +198,25 @@ static void system_free(system_t* s) {
   free(s->a_diag);
   free(s->a_rhs);
   free(s->a_bc);
+  free(s->heights);
   memset(s, 0, sizeof(*s));
 }
 
-/* ShallowFlowReduction (cpp/core/flow_system.hpp:257-330) for nz = 1,
- * reduce_shallow_flow's checks (1255-1266), axis_flow_boundary (186-201) and
- * the DepthAveragedFlowSystem constructor (522-636). */
+/* ShallowFlowReduction (cpp/core/flow_system.hpp:257-330), reduce_shallow_flow's
+ * checks (1255-1266), axis_flow_boundary (186-201) and the
+ * DepthAveragedFlowSystem constructor (522-636) on the grid of columns. */
 static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, system_t* s,
                         char* err, size_t errlen) {
   memset(s, 0, sizeof(*s));
-  if (g->nz != 1)
-    return fail(err, errlen, MO_INVALID_ARGUMENT,
-                "the depth-averaged path takes a grid of one layer (nz = 1)");
   int status = validate_flow_grid(g, g->obstacles, axis, err, errlen);
   if (status != MO_OK) return status;
   if (axis == 2)
     return fail(err, errlen, MO_INVALID_ARGUMENT,
                 "shallow flow is depth-integrated along z; flow axis must be x or y");
   const size_t n = (size_t)g->nx * g->ny;
-  const size_t faces = total_faces(g);
+  s->grid2 = *g;
+  s->grid2.nz = 1;
+  const size_t faces = total_faces(&s->grid2);
   s->n = n;
   s->faces = faces;
   s->mobility = calloc(n, sizeof(float));
@@ -224,40 +226,72 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
   s->bc = calloc(n, sizeof(float));
   s->face_cond = calloc(faces, sizeof(float));
   s->face_pressure = calloc(faces, sizeof(float));
-  float* heights = calloc(n, sizeof(float));
+  s->heights = calloc(n, sizeof(float));
   if (!s->mobility || !s->reduced_solid || !s->diagonal || !s->rhs || !s->bc || !s->face_cond ||
-      !s->face_pressure || !heights) {
-    free(heights);
+      !s->face_pressure || !s->heights) {
     system_free(s);
     return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
   }
-  /* Reduction: flow_system.hpp:276-327. One layer: a fluid column has height
-   * spacing.z and conductance height * mobility; without a mobility array the
-   * conductance is height * (height / max height)^2. */
+  s->grid2.obstacles = s->reduced_solid;
+  /* Reduction: flow_system.hpp:276-327. A column's fluid voxels must be
+   * contiguous, start on a floor common to every column and share one
+   * mobility; its height is spacing.z per fluid voxel and its conductance
+   * height * mobility, or without a mobility array height *
+   * (height / max height)^2. */
+  const uint32_t nz = g->nz;
+  uint32_t common_bottom = nz;
   float max_height = 0.0f;
-  for (size_t i = 0; i < n; ++i) {
-    s->reduced_solid[i] = 1;
-    if (mobility != NULL && (!isfinite(mobility[i]) || mobility[i] < 0.0f)) {
-      free(heights);
-      system_free(s);
-      return fail(err, errlen, MO_INVALID_ARGUMENT, "flow mobility must be finite and non-negative");
+  for (size_t column = 0; column < n; ++column) {
+    s->reduced_solid[column] = 1;
+    int started = 0, ended = 0;
+    float value = 1.0f;
+    for (uint32_t z = 0; z < nz; ++z) {
+      const size_t site = column * nz + z;
+      if (mobility != NULL && (!isfinite(mobility[site]) || mobility[site] < 0.0f)) {
+        system_free(s);
+        return fail(err, errlen, MO_INVALID_ARGUMENT, "flow mobility must be finite and non-negative");
+      }
+      if (solid_at(g, site)) {
+        if (started) ended = 1;
+        continue;
+      }
+      if (ended) {
+        system_free(s);
+        return fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "shallow flow needs contiguous fluid columns; use resolved flow");
+      }
+      const float next = mobility == NULL ? 1.0f : mobility[site];
+      if (!started) {
+        if (common_bottom == nz) common_bottom = z;
+        if (z != common_bottom) {
+          system_free(s);
+          return fail(err, errlen, MO_INVALID_ARGUMENT,
+                      "shallow flow needs a common planar floor; use resolved flow");
+        }
+        value = next;
+      } else if (fabsf(next - value) > 2.0e-6f * fmaxf(fabsf(value), fabsf(next))) {
+        system_free(s);
+        return fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "shallow-flow mobility must be constant through each depth column");
+      }
+      started = 1;
+      s->heights[column] += g->dz;
     }
-    if (solid_at(g, i)) continue;
-    const float value = mobility == NULL ? 1.0f : mobility[i];
-    heights[i] += g->dz;
-    s->reduced_solid[i] = 0;
-    s->mobility[i] = heights[i] * value;
-    if (heights[i] > max_height) max_height = heights[i];
+    if (started) {
+      s->reduced_solid[column] = 0;
+      s->mobility[column] = s->heights[column] * value;
+      if (s->heights[column] > max_height) max_height = s->heights[column];
+    }
   }
+  s->bottom = common_bottom;
   if (mobility == NULL && max_height > 0.0f)
     for (size_t i = 0; i < n; ++i) {
-      const float relative = heights[i] / max_height;
+      const float relative = s->heights[i] / max_height;
       s->mobility[i] *= relative * relative;
     }
   for (size_t i = 0; i < n; ++i)
     if (s->mobility[i] == 0.0f) s->reduced_solid[i] = 1;
-  s->heights = g->dz;
-  free(heights);
+  g = &s->grid2;
   status = validate_flow_grid(g, s->reduced_solid, axis, err, errlen);
   if (status != MO_OK) {
     system_free(s);
@@ -543,9 +577,11 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
   double* full = calloc(s.n, sizeof(double));
   for (size_t i = 0; i < s.count; ++i) full[s.sites[i]] = solution[i];
   free(solution);
-  /* velocity, flow_system.hpp:711-736, over the reduced grid's faces. */
-  const size_t nfaces = s.faces;
-  float* flux = malloc(nfaces * sizeof(float));
+  /* velocity, flow_system.hpp:711-736, over the faces of the grid of
+   * columns: the depth-integrated flux of each column face. */
+  const mo_grid* g2 = &s.grid2;
+  const size_t nfaces2 = s.faces;
+  float* flux = malloc(nfaces2 * sizeof(float));
   const float spacing[3] = {g->dx, g->dy, g->dz};
   const uint32_t dims[3] = {g->nx, g->ny, 1};
   size_t index = 0;
@@ -574,31 +610,70 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
           flux[index] = (float)value;
         }
   }
-  /* lift, flow_system.hpp:424-445 (one layer: no vertical flux). Faces next
-   * to an original solid site stay zero; others divide by the column
-   * height. */
+  /* lift, flow_system.hpp:424-470: each x and y voxel face between fluid
+   * voxels carries its column face's flux over the smaller of the two
+   * columns' heights; then the vertical faces by continuity, integrating the
+   * horizontal divergence upward with the column's mean removed so floor and
+   * roof stay closed. */
+  const uint32_t nz = g->nz;
+  const size_t nfaces = total_faces(g);
   float* velocity = calloc(nfaces, sizeof(float));
-  index = 0;
   for (int comp = 0; comp < 2; ++comp) {
-    uint32_t fd[3] = {dims[0], dims[1], dims[2]};
+    uint32_t fd[2] = {g->nx, g->ny};
     ++fd[comp];
     for (uint32_t x = 0; x < fd[0]; ++x)
-      for (uint32_t y = 0; y < fd[1]; ++y, ++index) {
-        const uint32_t fc[2] = {x, y};
-        const int has_lower = fc[comp] > 0;
-        const int has_upper = fc[comp] < dims[comp];
-        uint32_t lc[2] = {x, y};
-        if (has_lower) --lc[comp];
-        const size_t lower = (size_t)lc[0] * g->ny + lc[1];
-        const size_t upper = (size_t)x * g->ny + y;
-        if ((has_lower && solid_at(g, lower)) || (has_upper && solid_at(g, upper))) continue;
-        const float height = s.heights;
-        if (height > 0.0f) velocity[index] = flux[index] / height;
+      for (uint32_t y = 0; y < fd[1]; ++y)
+        for (uint32_t z = 0; z < nz; ++z) {
+          const uint32_t fc[2] = {x, y};
+          const uint32_t dim = comp == 0 ? g->nx : g->ny;
+          const int has_lower = fc[comp] > 0;
+          const int has_upper = fc[comp] < dim;
+          uint32_t lc[2] = {x, y};
+          if (has_lower) --lc[comp];
+          const size_t low_col = (size_t)lc[0] * g->ny + lc[1];
+          const size_t up_col = (size_t)x * g->ny + y;
+          if ((has_lower && solid_at(g, low_col * nz + z)) ||
+              (has_upper && solid_at(g, up_col * nz + z)))
+            continue;
+          const size_t a_col = has_lower ? low_col : up_col;
+          const size_t b_col = has_upper ? up_col : a_col;
+          const float height = fminf(s.heights[a_col], s.heights[b_col]);
+          if (height > 0.0f)
+            velocity[face_index(g, comp, x, y, z)] =
+                flux[face_index(g2, comp, x, y, 0)] / height;
+        }
+  }
+  if (nz > 1) {
+    double* horizontal = malloc(nz * sizeof(double));
+    for (uint32_t x = 0; x < g->nx; ++x)
+      for (uint32_t y = 0; y < g->ny; ++y) {
+        size_t count = 0;
+        double sum = 0.0;
+        for (uint32_t z = s.bottom; z < nz; ++z) {
+          if (solid_at(g, ((size_t)x * g->ny + y) * nz + z)) break;
+          const double dxv = (velocity[face_index(g, 0, x + 1, y, z)] -
+                              velocity[face_index(g, 0, x, y, z)]) /
+                             g->dx;
+          const double dyv = (velocity[face_index(g, 1, x, y + 1, z)] -
+                              velocity[face_index(g, 1, x, y, z)]) /
+                             g->dy;
+          horizontal[count++] = dxv + dyv;
+          sum += dxv + dyv;
+        }
+        if (count == 0) continue;
+        const double mean = sum / (double)count;
+        double vertical = 0.0;
+        for (size_t k = 0; k + 1 < count; ++k) {
+          vertical -= g->dz * (horizontal[k] - mean);
+          velocity[face_index(g, 2, x, y, s.bottom + (uint32_t)k + 1)] = (float)vertical;
+        }
       }
+    free(horizontal);
   }
   free(flux);
   /* scale_velocity, flow_system.hpp:1368-1415; open inlet faces are the
-   * lower faces of the axis whose site has conductance (420-433). */
+   * axis's lower faces of fluid voxels in columns with conductance
+   * (open_inlet_faces, 449-459). */
   double inlet_sum = 0.0;
   size_t inlet_count = 0;
   float peak = 0.0f;
@@ -612,16 +687,16 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
     const float a = fabsf(velocity[f]);
     if (a > peak) peak = a;
   }
-  const size_t axis_offset = axis == 0 ? 0 : face_count(g, 0);
   const uint32_t across = axis == 0 ? g->ny : g->nx;
-  for (uint32_t t = 0; t < across; ++t) {
-    const size_t site = axis == 0 ? (size_t)t : (size_t)t * g->ny;
-    const size_t f = axis == 0 ? axis_offset + t : axis_offset + (size_t)t * (g->ny + 1);
-    if (!solid_at(g, site) && s.mobility[site] > 0.0f) {
-      inlet_sum += velocity[f];
-      ++inlet_count;
+  for (uint32_t t = 0; t < across; ++t)
+    for (uint32_t z = 0; z < nz; ++z) {
+      const size_t column = axis == 0 ? (size_t)t : (size_t)t * g->ny;
+      const size_t f = axis == 0 ? face_index(g, 0, 0, t, z) : face_index(g, 1, t, 0, z);
+      if (!solid_at(g, column * nz + z) && s.mobility[column] > 0.0f) {
+        inlet_sum += velocity[f];
+        ++inlet_count;
+      }
     }
-  }
   if (inlet_count == 0) {
     free(velocity);
     free(full);

@@ -109,3 +109,34 @@ def random_transport_case(nx, ny, S, kinds, seed):
     diffusion = rng.uniform(0.05, 0.3, S).astype(np.float32)
     return dict(solid=solid, levels=levels, xf=xf, yf=yf, zf=zf, loss=loss, source=source,
                 phi=phi, bv=bv, diffusion=diffusion)
+
+
+def layered_case(nx, ny, nz, seed, mobility=True, axis=0):
+    """A grid of several layers for shallow flow: every column fluid from the
+    floor (z = 0) up to a roof of 1..nz voxels, some columns solid (pillars),
+    the flow axis's two ends clear, and a mobility constant down each column
+    (or none)."""
+    rng = np.random.default_rng(seed)
+    obstacles = np.zeros((nx, ny, nz), np.uint8)
+    roof = rng.integers(1, nz + 1, (nx, ny))
+    for x in range(nx):
+        for y in range(ny):
+            obstacles[x, y, roof[x, y]:] = 1
+    pillars = rng.random((nx, ny)) < 0.1
+    if axis == 0:
+        pillars[0, :] = pillars[-1, :] = False
+    else:
+        pillars[:, 0] = pillars[:, -1] = False
+    obstacles[pillars] = 1
+    mob = None
+    if mobility:
+        column = rng.uniform(0.5, 2.0, (nx, ny)).astype(np.float32)
+        mob = np.repeat(column[:, :, None], nz, axis=2).reshape(-1)
+    boundary = ((FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX) if axis == 0 else
+                (NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX))
+    return obstacles.reshape(-1), mob, boundary
+
+
+LAYERED_CASES = [(12, 9, 3, 1, True), (20, 7, 4, 2, False), (15, 15, 2, 3, True),
+                 (10, 12, 5, 4, False)]
+

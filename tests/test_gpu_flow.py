@@ -16,8 +16,8 @@ import pytest
 
 import microsimulator_amd as msa
 from oracle import Grid as OGrid
-from tests.helpers import (FIXED, NO_FLUX, PERIODIC, c1_inputs, divergence, illcond_inputs,
-                           random_case, split_faces)
+from tests.helpers import (FIXED, LAYERED_CASES, NO_FLUX, PERIODIC, c1_inputs, divergence,
+                           illcond_inputs, layered_case, random_case, split_faces)
 
 pytestmark = pytest.mark.gpu
 
@@ -192,8 +192,11 @@ def test_error_paths(hip):
         hip.solve_depth_averaged_flow(periodic, None, params(axis=1))
     with pytest.raises(msa.InvalidArgument):
         hip.solve_depth_averaged_flow(duct, -np.ones(32, np.float32), params(axis=1))
-    with pytest.raises(msa.Unsupported):
-        hip.solve_depth_averaged_flow(msa.Grid(4, 8, 2), None, params())
+    overhang = np.zeros((4, 8, 3), np.uint8)
+    overhang[2, 3, 1] = 1
+    with pytest.raises(msa.InvalidArgument, match="contiguous fluid columns"):
+        hip.solve_depth_averaged_flow(msa.Grid(4, 8, 3, obstacles=overhang.reshape(-1)), None,
+                                      params())
     with pytest.raises(msa.Unsupported):
         hip.solve_resolved_flow()
     # The context still works after the failures.
@@ -272,3 +275,33 @@ def test_multigrid_full_size(hip):
     assert mg.true_relative_residual <= 1e-8
     assert np.abs(fmg.concatenated() - f64.concatenated()).max() <= 1e-4 * fp64.max_speed
     assert mg.iterations <= 150
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("solver", [msa.SOLVER_JACOBI_FP64, msa.SOLVER_MULTIGRID])
+def test_layered_grids_match_the_oracle(hip, oracle, axis, solver):
+    """Grids of several layers: the device solves the grid of columns, the
+    host lifts the column flux onto the voxel faces (vertical faces by
+    continuity) and scales it. Against the oracle (bit-exact with the
+    reference on these grids): FP64 Jacobi-PCG within 2 iterations and 1e-6 of
+    the largest speed, the multilevel form within 1e-5."""
+    for nx, ny, nz, seed, mob in LAYERED_CASES:
+        obstacles, mobility, boundary = layered_case(nx, ny, nz, seed, mob, axis)
+        g = msa.Grid(nx, ny, nz, 1.0, 0.5, 0.7, obstacles, boundary)
+        og = OGrid(nx, ny, nz, 1.0, 0.5, 0.7, obstacles, boundary)
+        print("case", (nx, ny, nz, seed, mob), "axis", axis, "solver", solver, flush=True)
+        res = hip.solve_depth_averaged_flow(g, mobility, params(
+            axis=axis, mean_inlet_speed=2.0, relative_tolerance=1e-10, solver=solver))
+        ref, _, ref_rep = oracle.solve_flow(og, mobility, axis, 2.0, 1e-10, 200000)
+        got = res.field.concatenated()
+        assert got.size == ref.size
+        err = np.abs(got - ref).max() / ref_rep.max_speed
+        print("  velocity error / max speed", err, "iterations", res.report.iterations,
+              ref_rep.iterations, flush=True)
+        if solver == msa.SOLVER_JACOBI_FP64:
+            assert abs(res.report.iterations - ref_rep.iterations) <= 2
+            assert err <= 1e-6
+        else:
+            assert err <= 1e-5
+        assert res.report.max_speed == pytest.approx(ref_rep.max_speed, rel=1e-5)
+

@@ -8,9 +8,9 @@ import numpy as np
 import pytest
 
 from oracle import FIXED, NO_FLUX, PERIODIC, VAN_LEER, Grid, OracleError
-from tests.helpers import (TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs, divergence, golden,
-                           illcond_inputs, monod_loss, random_case, random_transport_case,
-                           split_faces)
+from tests.helpers import (LAYERED_CASES, TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs, divergence,
+                           golden, illcond_inputs, layered_case, monod_loss, random_case,
+                           random_transport_case, split_faces)
 
 
 def c1_grid():
@@ -409,6 +409,44 @@ def test_fp64_van_leer_agrees_and_conserves(oracle, kinds):
         assert np.abs(a - b).max() <= 1e-6, (nx, ny)
         balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
         assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_layered_shallow_flow_matches_reference_live(oracle, reference, axis):
+    """Grids of several layers: reduction to columns (heights, the (H/Hmax)^2
+    default, conductance), the solve on the grid of columns, the lift onto
+    the voxel faces with vertical faces by continuity, and the scaling over
+    the voxel inlet faces, bit for bit with the reference."""
+    for nx, ny, nz, seed, mob in LAYERED_CASES:
+        obstacles, mobility, boundary = layered_case(nx, ny, nz, seed, mob, axis)
+        grid = Grid(nx, ny, nz, 1.0, 0.5, 0.7, obstacles, boundary=boundary)
+        a, _, rep = oracle.solve_flow(grid, mobility, axis, 2.0, 1e-6, 50000)
+        b, info = reference.solve_flow(grid, mobility, axis, 2.0, 1e-6, 50000)
+        assert np.array_equal(a, b), (nx, ny, nz, axis)
+        assert rep.iterations == info["iterations"] and rep.max_speed == info["max_speed"]
+        nfx, nfy, _ = grid.face_counts()
+        assert np.abs(a[nfx + nfy:]).max() > 0.0  # the vertical faces carry flow
+
+
+def test_layered_shallow_flow_refusals_match_reference(oracle, reference):
+    """An overhang, a raised floor and a mobility that varies down a column
+    are refused with the reference's messages."""
+    nx, ny, nz = 8, 6, 3
+    base = np.zeros((nx, ny, nz), np.uint8)
+    overhang = base.copy()
+    overhang[3, 2, 1] = 1  # fluid above a solid voxel above fluid
+    floor = base.copy()
+    floor[4, 4, 0] = 1  # a column that starts above the common floor
+    varying = np.ones((nx, ny, nz), np.float32)
+    varying[2, 2, 1] = 2.0
+    cases = [(overhang, None, "contiguous fluid columns"), (floor, None, "common planar floor"),
+             (base, varying.reshape(-1), "constant through each depth column")]
+    for obstacles, mobility, message in cases:
+        grid = Grid(nx, ny, nz, 1.0, 1.0, 1.0, obstacles.reshape(-1))
+        with pytest.raises(OracleError, match=message):
+            oracle.solve_flow(grid, mobility, 0, 1.0, 1e-6, 1000)
+        with pytest.raises(OracleError, match=message):
+            reference.solve_flow(grid, mobility, 0, 1.0, 1e-6, 1000)
 
 
 def test_flow_error_paths(oracle):
