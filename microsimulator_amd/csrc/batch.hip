@@ -14,24 +14,29 @@
 //                every site in the single-grid kernel's float operations
 //                (adr_site.hpp), with FP64 mass tallies.
 //
-// A step is ONE launch over work items (instance, block of rows) of a few
-// thousand sites, spread over the whole chip. Each item recomputes its rows'
-// FP64 pressure residual (the step's flow check), writes the face velocities
-// it owns, and advances its sites with the face velocities computed where
-// they are needed, so no item waits on another. Per-item FP64 partials
-// (residual, masses, uptake) are added per instance in a fixed order on the
-// host. A run of steps is captured once into a HIP graph and replayed.
+// A step runs over work items (instance, block of rows) of a few thousand
+// sites, spread over the whole chip, one launch per width class. Each item
+// recomputes its rows' FP64 pressure residual (the step's flow check), writes
+// the face velocities it owns, and advances its sites, so no item waits on
+// another. Instances up to 256 columns wide take a marching kernel: one
+// thread per column walks the item's rows, neighbours from registers and an
+// LDS row, each array read once and each x face computed once; wider ones a
+// general sweep. Per-item FP64 partials (residual, masses, uptake) are added
+// per instance in a fixed order on the host. A run of steps is captured once
+// into a HIP graph and replayed.
 //
 // The pressure solve itself (cold, or when a step's check finds an instance
 // unconverged) is one launch with one workgroup per instance (largest first):
 // the refinement loop needs the instance's dot products, and a workgroup
 // reduces them with barriers only.
 //
-// The launch shapes (workgroup size, sweep row width, item size) are chosen
-// by timing each candidate on the uploaded batch: the autotuner.
+// The launch shapes (sites per item; workgroup size and sweep row width of
+// the general sweep) are chosen by timing each candidate on the uploaded
+// batch: the autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -300,10 +305,10 @@ template <int NT, int TY>
 __global__ __launch_bounds__(NT) void k_batch_step(BatchData D, BatchParams P,
                                                    const BatchItem* __restrict__ items,
                                                    double* __restrict__ partials, uint32_t step,
-                                                   float dt) {
+                                                   float dt, uint32_t base, uint32_t total) {
   __shared__ double s_red[NT / kWave];
   constexpr int RX = NT / TY;
-  const BatchItem item = items[blockIdx.x];
+  const BatchItem item = items[base + blockIdx.x];
   const uint32_t inst = item.inst;
   const BatchDesc I = D.desc[inst];
   const uint32_t nx = I.nx, ny = I.ny;
@@ -381,7 +386,192 @@ __global__ __launch_bounds__(NT) void k_batch_step(BatchData D, BatchParams P,
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const double t = block_sum<NT>(acc[k], s_red);
-    if (threadIdx.x == 0) partials[(static_cast<size_t>(step) * gridDim.x + blockIdx.x) * 4 + k] = t;
+    if (threadIdx.x == 0) {
+      partials[(static_cast<size_t>(step) * total + base + blockIdx.x) * 4 + k] = t;
+    }
+  }
+}
+
+// The same step for an instance at most TY columns wide, marching: work item
+// base + blockIdx.x owns whole rows [x0, x1) of its instance, one thread per
+// column, and walks x, so each array is read once; the x neighbours come
+// from registers, the y neighbours from a row in LDS, and each x face is
+// computed once (the upper face of a row is the next row's lower face).
+// Every site's arithmetic is the general kernel's, so the two agree bit for
+// bit.
+template <int TY>
+__global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
+                                                    const BatchItem* __restrict__ items,
+                                                    double* __restrict__ partials, uint32_t step,
+                                                    float dt, uint32_t base, uint32_t total) {
+  struct Row {
+    double p;
+    float cond, c, cxl, cyl, bc, rhs, phi;
+    bool solid;
+  };
+  __shared__ double s_p[2][TY];
+  __shared__ float s_cond[2][TY], s_c[2][TY], s_cyl[2][TY];
+  __shared__ uint8_t s_s[2][TY];
+  __shared__ double s_red[4 * (TY / kWave)];
+  const BatchItem item = items[base + blockIdx.x];
+  const uint32_t inst = item.inst;
+  const BatchDesc I = D.desc[inst];
+  const uint32_t nx = I.nx, ny = I.ny;
+  const size_t o = I.site;
+  const uint32_t y = threadIdx.x;
+  const bool valid = y < ny;
+  const float* in = ((step & 1) == 0 ? D.la : D.lb) + o;
+  float* out = ((step & 1) == 0 ? D.lb : D.la) + o;
+  const bool monod = D.phi != nullptr;
+  float* xf = D.xf + I.xface;
+  float* yf = D.yf + I.yface;
+  const double isx = 1.0 / (static_cast<double>(I.dx) * I.dx);
+  const double isy = 1.0 / (static_cast<double>(I.dy) * I.dy);
+  AdrParams A{};
+  A.nx = nx;
+  A.ny = ny;
+  A.signals = 1;
+  A.dx = I.dx;
+  A.dy = I.dy;
+  A.dz = I.dz;
+  A.kind[0] = A.kind[1] = MSIM_FIXED;
+  A.kind[2] = A.kind[3] = A.kind[4] = A.kind[5] = MSIM_NO_FLUX;
+  A.has_field = 1;
+  A.has_monod = monod ? 1 : 0;
+  A.diffusion[0] = I.diffusion;
+  A.boundary[0][0] = I.reservoir;
+  A.vmax[0] = P.vmax;
+  A.monod_k[0] = P.monod_k;
+  auto load = [&](int xx, bool full) {
+    Row w{};
+    if (!valid || xx < 0 || xx >= static_cast<int>(nx)) return w;
+    const size_t i = o + static_cast<size_t>(xx) * ny + y;
+    w.p = D.x64[i];
+    w.cond = D.cond[i];
+    w.c = in[i - o];
+    w.solid = D.solid[i] != 0;
+    w.cxl = D.cxl[i];  // also the upper x coupling of the row before
+    if (!full) return w;
+    w.cyl = D.cyl[i];
+    w.bc = D.bc[i];
+    w.rhs = D.rhs[i];
+    if (monod) w.phi = D.phi[i];
+    return w;
+  };
+  // Face x fx of this column from the rows either side (face_x's arithmetic).
+  auto face_x_of = [&](const Row& lo, const Row& up, int fx) -> float {
+    const bool has_lower = fx > 0, has_upper = fx < static_cast<int>(nx);
+    double value;
+    if (has_lower && has_upper) {
+      const float fm = harmonic(lo.cond, up.cond);
+      value = -static_cast<double>(fm) * (up.p - lo.p) / static_cast<double>(I.dx);
+    } else {
+      const Row& site = has_upper ? up : lo;
+      float fc = 0.0f, fp = 0.0f;
+      if (site.cond != 0.0f) {
+        fc = (has_upper ? 2.0f : -2.0f) * site.cond / I.dx;
+        fp = has_upper ? I.inlet : 0.0f;
+      }
+      value = static_cast<double>(fc) * (static_cast<double>(fp) - site.p);
+    }
+    const float flux = static_cast<float>(value);
+    if ((has_lower && lo.solid) || (has_upper && up.solid)) return 0.0f;
+    return flux / I.dz;
+  };
+  // Face y fy of row buf from the LDS row (face_y's arithmetic).
+  auto face_y_of = [&](int buf, uint32_t fy) -> float {
+    const bool has_lower = fy > 0, has_upper = fy < ny;
+    double value;
+    if (has_lower && has_upper) {
+      const float fm = harmonic(s_cond[buf][fy - 1], s_cond[buf][fy]);
+      value = -static_cast<double>(fm) * (s_p[buf][fy] - s_p[buf][fy - 1]) /
+              static_cast<double>(I.dy);
+    } else {
+      const double ps = has_upper ? s_p[buf][fy] : s_p[buf][fy - 1];
+      value = static_cast<double>(0.0f) * (static_cast<double>(0.0f) - ps);
+    }
+    const float flux = static_cast<float>(value);
+    if ((has_lower && s_s[buf][fy - 1] != 0) || (has_upper && s_s[buf][fy] != 0)) return 0.0f;
+    return flux / I.dz;
+  };
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t bad = 0;
+  const int x0 = static_cast<int>(item.x0), x1 = static_cast<int>(item.x1);
+  Row prev = load(x0 - 1, false);
+  Row cur = load(x0, true);
+  Row next = load(x0 + 1, x0 + 1 < x1);
+  float uxl = face_x_of(prev, cur, x0);
+  for (int xx = x0; xx < x1; ++xx) {
+    const Row ahead = xx + 2 <= x1 ? load(xx + 2, xx + 2 < x1) : Row{};
+    const int buf = xx & 1;
+    s_p[buf][y] = cur.p;
+    s_cond[buf][y] = cur.cond;
+    s_c[buf][y] = cur.c;
+    s_cyl[buf][y] = cur.cyl;
+    s_s[buf][y] = cur.solid ? 1 : 0;
+    __syncthreads();
+    const float uxu = face_x_of(cur, next, xx + 1);
+    if (valid) {
+      const size_t i = static_cast<size_t>(xx) * ny + y;
+      // residual_f64's arithmetic.
+      const double c = cur.p;
+      double a = static_cast<double>(cur.bc) * c;
+      if (xx > 0) a += (static_cast<double>(cur.cxl) * isx) * (c - prev.p);
+      if (xx + 1 < static_cast<int>(nx)) a += (static_cast<double>(next.cxl) * isx) * (c - next.p);
+      if (y > 0) a += (static_cast<double>(cur.cyl) * isy) * (c - s_p[buf][y - 1]);
+      if (y + 1 < ny) a += (static_cast<double>(s_cyl[buf][y + 1]) * isy) * (c - s_p[buf][y + 1]);
+      const double res = static_cast<double>(I.inlet) * static_cast<double>(cur.rhs) - a;
+      acc[0] += res * res;
+      const float uyl = face_y_of(buf, y);
+      const float uyu = face_y_of(buf, y + 1);
+      xf[i] = uxl;
+      if (xx + 1 == static_cast<int>(nx)) xf[i + ny] = uxu;
+      const size_t yi = static_cast<size_t>(xx) * (ny + 1) + y;
+      yf[yi] = uyl;
+      if (y + 1 == ny) yf[yi + 1] = uyu;
+      const float current = cur.c;
+      if (cur.solid) {
+        out[i] = current;
+      } else {
+        const bool xlo_edge = xx == 0, xhi_edge = xx + 1 == static_cast<int>(nx);
+        const bool ylo_edge = y == 0, yhi_edge = y + 1 == ny;
+        SiteGeometry g;
+        g.clo[0] = xlo_edge ? false : prev.solid;
+        g.cup[0] = xhi_edge ? false : next.solid;
+        g.clo[1] = ylo_edge ? true : s_s[buf][y - 1] != 0;
+        g.cup[1] = yhi_edge ? true : s_s[buf][y + 1] != 0;
+        g.ulo[0] = uxl;
+        g.uup[0] = uxu;
+        g.ulo[1] = uyl;
+        g.uup[1] = uyu;
+        const float l0 = xlo_edge ? exterior(A, 0, 0, current, 0.0f) : prev.c;
+        const float u0 = xhi_edge ? exterior(A, 1, 0, current, 0.0f) : next.c;
+        const float l1 = ylo_edge ? exterior(A, 2, 0, current, 0.0f) : s_c[buf][y - 1];
+        const float u1 = yhi_edge ? exterior(A, 3, 0, current, 0.0f) : s_c[buf][y + 1];
+        const float candidate =
+            fe_update_f32(A, g, current, l0, u0, l1, u1, 0, dt, 0.0f, 0.0f, cur.phi);
+        if (!isfinite(candidate) || candidate < 0.0f) bad = 1;
+        out[i] = candidate;
+        acc[1] += static_cast<double>(current);
+        acc[2] += static_cast<double>(candidate);
+        if (monod) {
+          acc[3] += static_cast<double>(P.vmax) * static_cast<double>(cur.phi) *
+                    static_cast<double>(current) / (static_cast<double>(P.monod_k) + current);
+        }
+      }
+    }
+    uxl = uxu;
+    prev = cur;
+    cur = next;
+    next = ahead;
+  }
+  if (bad) atomicOr(&D.flags[inst], 4u);
+  block_sums<TY, 4>(acc, s_red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      partials[(static_cast<size_t>(step) * total + base + blockIdx.x) * 4 + k] = acc[k];
+    }
   }
 }
 
@@ -391,21 +581,37 @@ __global__ void k_batch_dinv(const float* __restrict__ diag, float* __restrict__
 }
 
 using FlowKernel = void (*)(BatchData, BatchParams);
-using StepKernel = void (*)(BatchData, BatchParams, const BatchItem*, double*, uint32_t, float);
+using StepKernel = void (*)(BatchData, BatchParams, const BatchItem*, double*, uint32_t, float,
+                            uint32_t, uint32_t);
 
+// A launch shape of the step: sites per work item, and for instances wider
+// than 256 columns (the general kernel) the workgroup size and sweep row
+// width.
 struct Shape {
-  int threads, row, chunk;  // workgroup size, sweep row width, sites per item
+  int threads, row, chunk;
   StepKernel step;
 };
 
 const Shape kShapes[] = {
-    {256, 64, 4096, k_batch_step<256, 64>},      {256, 128, 8192, k_batch_step<256, 128>},
-    {256, 64, 16384, k_batch_step<256, 64>},     {512, 64, 8192, k_batch_step<512, 64>},
-    {512, 128, 16384, k_batch_step<512, 128>},   {1024, 128, 16384, k_batch_step<1024, 128>},
-    {1024, 256, 32768, k_batch_step<1024, 256>},
+    {256, 64, 512, k_batch_step<256, 64>},    {256, 64, 1024, k_batch_step<256, 64>},
+    {256, 64, 2048, k_batch_step<256, 64>},   {256, 64, 4096, k_batch_step<256, 64>},
+    {512, 64, 8192, k_batch_step<512, 64>},   {256, 128, 16384, k_batch_step<256, 128>},
 };
 constexpr int kShapeCount = static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0]));
-constexpr int kDefaultShape = 3;
+constexpr int kDefaultShape = 1;
+// Instance classes by width: marching kernels for at most 64, 128 and 256
+// columns, then the general kernel.
+constexpr int kClasses = 4;
+constexpr uint32_t kClassWidth[kClasses - 1] = {64, 128, 256};
+const StepKernel kMarch[kClasses - 1] = {k_batch_march<64>, k_batch_march<128>,
+                                         k_batch_march<256>};
+
+int class_of(uint32_t ny) {
+  for (int c = 0; c < kClasses - 1; ++c) {
+    if (ny <= kClassWidth[c]) return c;
+  }
+  return kClasses - 1;
+}
 constexpr FlowKernel kFlowKernel = k_batch_flow<1024, 128>;
 constexpr int kFlowThreads = 1024;
 
@@ -417,7 +623,8 @@ struct BatchDevice {
   uint64_t sites = 0, xfaces = 0, yfaces = 0;
   std::vector<BatchDesc> desc;
   std::vector<uint32_t> order;
-  std::vector<std::vector<BatchItem>> items;  // per shape
+  std::vector<std::vector<BatchItem>> items;  // per shape, grouped by instance class
+  std::vector<std::array<uint32_t, 5>> class_begin;  // per shape: first item of each class, end
   std::vector<std::unique_ptr<DeviceBuffer<BatchItem>>> d_items;
   DeviceBuffer<BatchDesc> d_desc;
   DeviceBuffer<uint32_t> d_order, iterations, flags;
@@ -457,10 +664,18 @@ int shape_of(const BatchDevice& B) { return B.shape >= 0 ? B.shape : kDefaultSha
 void launch_step(Context& ctx, BatchDevice& B, const BatchData& D, int shape, uint32_t step,
                  float dt) {
   const Shape& s = kShapes[shape];
-  const uint32_t n = static_cast<uint32_t>(B.items[shape].size());
-  hipLaunchKernelGGL(s.step, dim3(n), dim3(s.threads), 0, ctx.stream, D, B.params,
-                     B.d_items[shape]->data(), B.partials.data(), step, dt);
-  check(hipGetLastError(), "launch k_batch_step");
+  const auto& begin = B.class_begin[shape];
+  const uint32_t total = static_cast<uint32_t>(B.items[shape].size());
+  for (int c = 0; c < kClasses; ++c) {
+    const uint32_t base = begin[c], n = begin[c + 1] - begin[c];
+    if (n == 0) continue;
+    const bool march = c < kClasses - 1;
+    hipLaunchKernelGGL(march ? kMarch[c] : s.step, dim3(n),
+                       dim3(march ? kClassWidth[c] : static_cast<uint32_t>(s.threads)), 0,
+                       ctx.stream, D, B.params, B.d_items[shape]->data(), B.partials.data(), step,
+                       dt, base, total);
+    check(hipGetLastError(), "launch batch step");
+  }
 }
 
 // Runs `steps` steps from the current levels, in a graph or eagerly.
@@ -550,14 +765,21 @@ void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
   });
   // Work items of each shape: blocks of whole rows of about `chunk` sites.
   B.items.assign(kShapeCount, {});
+  B.class_begin.assign(kShapeCount, {});
   B.d_items.clear();
   for (int s = 0; s < kShapeCount; ++s) {
-    for (uint32_t k : B.order) {
-      const BatchDesc& d = B.desc[k];
-      const uint32_t rows = std::max<uint32_t>(1, static_cast<uint32_t>(kShapes[s].chunk) / d.ny);
-      for (uint32_t x0 = 0; x0 < d.nx; x0 += rows)
-        B.items[s].push_back(BatchItem{k, x0, std::min(d.nx, x0 + rows), 0});
+    for (int c = 0; c < kClasses; ++c) {
+      B.class_begin[s][c] = static_cast<uint32_t>(B.items[s].size());
+      for (uint32_t k : B.order) {
+        const BatchDesc& d = B.desc[k];
+        if (class_of(d.ny) != c) continue;
+        const uint32_t rows =
+            std::max<uint32_t>(1, static_cast<uint32_t>(kShapes[s].chunk) / d.ny);
+        for (uint32_t x0 = 0; x0 < d.nx; x0 += rows)
+          B.items[s].push_back(BatchItem{k, x0, std::min(d.nx, x0 + rows), 0});
+      }
     }
+    B.class_begin[s][kClasses] = static_cast<uint32_t>(B.items[s].size());
     if (B.items[s].size() > 0x7FFFFFFFu) throw Error(MSIM_OVERFLOW, "too many batch work items");
     auto buf = std::make_unique<DeviceBuffer<BatchItem>>();
     buf->reserve(B.items[s].size());
