@@ -57,7 +57,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (scaled by a 256 MiB copy's readings) on this workload; the commands and
 # figures are in profiles/krylov_kernels_4096x2048_traffic.md.
-TRAFFIC_MB = {"k_learn_up": 499.1, "k_fcg_apply": 315.3, "k_mg_up": 410.6, "k_mg_down": 180.3,
+TRAFFIC_MB = {"k_learn_up": 511.5, "k_fcg_apply": 315.3, "k_mg_up": 410.6, "k_mg_down": 180.3,
               "k_fcg_update": 100.9, "k_cg_apply<float>": 353.8}
 
 
@@ -249,7 +249,19 @@ def main():
                 hip.flow_solve(p)
             except msa.NotConverged:
                 pass
-        log(f"ran {args.kernel_iterations} iterations of each solver form")
+        # A few transport steps of each form over a solved field.
+        rep = hip.flow_solve(solver_params("multigrid"))
+        S = 3
+        bv = np.zeros(6 * S, np.float32)
+        bv[0::6] = 1.0
+        for scheme, prec in ((msa.UPWIND, 64), (msa.UPWIND, 32), (msa.VAN_LEER, 32)):
+            spec = msa.SignalGridSpec(grid, [100.0, 500.0, 1000.0], bv, phi=w.phi,
+                                      monod_vmax=[1.0] * S, monod_k=[0.1] * S,
+                                      advection_scheme=scheme)
+            hip.adr_upload(spec, np.zeros(S * w.sites), precision=prec, use_flow_field=True)
+            hip.adr_run(1e-6, 5, ledger=prec == 64)
+        log(f"ran {args.kernel_iterations} iterations of each solver form and 5 transport steps "
+            f"of each transport form")
         return
 
     headline = solver_params(args.solver)
@@ -347,8 +359,18 @@ def main():
     hip.adr_run(dt, args.adr_steps, ledger=False)
     adr32_s = hip.adr_seconds()
     adr32_gcups = w.fluid * S * args.adr_steps / adr32_s / 1e9
+    # The limited (van Leer) scheme: half the advective bound, the settle after
+    # each step.
+    limited = msa.SignalGridSpec(grid, diffusion, bv, phi=w.phi, monod_vmax=[1.0] * S,
+                                 monod_k=[0.1] * S, advection_scheme=msa.VAN_LEER)
+    dt_vl = 0.9 / (bound + outflow_bound)
+    hip.adr_upload(limited, np.zeros(S * sites, np.float32), precision=32, use_flow_field=True)
+    hip.adr_run(dt_vl, args.adr_steps, ledger=False)
+    adr_vl_s = hip.adr_seconds()
+    adr_vl_gcups = w.fluid * S * args.adr_steps / adr_vl_s / 1e9
     log(f"transport fp64: {adr_gcups:.2f} Gcell-upd/s ({adr_gbs:.0f} GB/s), worst per-step "
-        f"imbalance {worst_balance:.2e}; fp32: {adr32_gcups:.2f} Gcell-upd/s")
+        f"imbalance {worst_balance:.2e}; fp32: {adr32_gcups:.2f} Gcell-upd/s; fp32 van Leer: "
+        f"{adr_vl_gcups:.2f} Gcell-upd/s")
 
     line = {
         "metric": METRIC,
@@ -396,6 +418,8 @@ def main():
         "adr": {
             "gcell_upd_per_s_fp64": round(adr_gcups, 3),
             "gcell_upd_per_s_fp32": round(adr32_gcups, 3),
+            "gcell_upd_per_s_fp32_van_leer": round(adr_vl_gcups, 3),
+            "dt_van_leer": dt_vl,
             "solutes": S,
             "steps": args.adr_steps,
             "dt": dt,
