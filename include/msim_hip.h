@@ -181,6 +181,18 @@ int msim_hip_flow_arrays(msim_ctx* ctx, float* conductance, float* diagonal, flo
  * per site and vmax, K per signal, may be NULL. revision: SignalGrid::revision
  * (signals.hpp:217-221); the device copies of the spec arrays are rewritten
  * only when it changes. */
+/* GridBoundaryPatch (signals.hpp:72-84): a rectangle of a fixed face that
+ * holds its own reservoir value per signal, replacing the face's values there;
+ * where patches overlap the later one holds (GridBoundary::values_at,
+ * signals.cpp:439-447). In-plane indices: on an x face (y, z), on a y face
+ * (x, z), on a z face (x, y); begin inclusive, end exclusive. */
+typedef struct {
+  uint32_t face; /* 0..5: x_lower, x_upper, y_lower, y_upper, z_lower, z_upper */
+  uint32_t begin[2];
+  uint32_t end[2];
+  const float* values; /* signal_count values */
+} msim_boundary_patch;
+
 typedef struct {
   msim_grid grid;
   uint32_t signal_count;
@@ -196,6 +208,8 @@ typedef struct {
   const float* monod_k;
   uint64_t revision;
   uint32_t advection_scheme; /* MSIM_UPWIND or MSIM_VAN_LEER (SignalAdvectionKind, signals.hpp:41-47) */
+  const msim_boundary_patch* patches; /* reservoir patches of fixed faces, or NULL */
+  uint32_t patch_count;
 } msim_signal_grid;
 
 /* SignalAdvectionKind (signals.hpp:41-47): first-order upwind, or upwind plus

@@ -105,13 +105,19 @@ class _FlowReport(C.Structure):
                 ("cycles", C.c_uint32)]
 
 
+class _BoundaryPatch(C.Structure):
+    _fields_ = [("face", C.c_uint32), ("begin", C.c_uint32 * 2), ("end", C.c_uint32 * 2),
+                ("values", C.c_void_p)]
+
+
 class _SignalGrid(C.Structure):
     _fields_ = [("grid", _Grid), ("signal_count", C.c_uint32), ("diffusion", C.c_void_p),
                 ("boundary_values", C.c_void_p), ("x_faces", C.c_void_p),
                 ("y_faces", C.c_void_p), ("z_faces", C.c_void_p), ("source_rates", C.c_void_p),
                 ("loss_rates", C.c_void_p), ("phi", C.c_void_p), ("monod_vmax", C.c_void_p),
                 ("monod_k", C.c_void_p), ("revision", C.c_uint64),
-                ("advection_scheme", C.c_uint32)]
+                ("advection_scheme", C.c_uint32), ("patches", C.c_void_p),
+                ("patch_count", C.c_uint32)]
 
 
 class _TrainSet(C.Structure):
@@ -372,6 +378,8 @@ class SignalGridSpec:
     monod_k: np.ndarray | None = None
     revision: int = 0
     advection_scheme: int = 0  # UPWIND or VAN_LEER (SignalAdvectionKind)
+    # GridBoundaryPatch: (face, (begin0, begin1), (end0, end1), values per signal)
+    patches: tuple = ()
 
     @property
     def signal_count(self) -> int:
@@ -403,6 +411,18 @@ class SignalGridSpec:
             setattr(s, name, arr.ctypes.data)
         s.revision = self.revision
         s.advection_scheme = self.advection_scheme
+        if len(self.patches):
+            table = (_BoundaryPatch * len(self.patches))()
+            for k, (face, begin, end, values) in enumerate(self.patches):
+                v = np.ascontiguousarray(np.atleast_1d(values), np.float32)
+                keep.append(v)
+                table[k].face = face
+                table[k].begin[0], table[k].begin[1] = begin
+                table[k].end[0], table[k].end[1] = end
+                table[k].values = v.ctypes.data
+            keep.append(table)
+            s.patches = C.cast(table, C.c_void_p)
+            s.patch_count = len(self.patches)
         return s
 
 

@@ -113,7 +113,7 @@ struct AdrRow {
 // (the reference throws, signals.cpp:1119-1131); the step is then refused as
 // a whole. Under van Leer each block also leaves the largest level it wrote,
 // for the settle that follows.
-template <typename T, int S, int TY, bool kAffine, bool kLimited>
+template <typename T, int S, int TY, bool kAffine, bool kLimited, bool kPatched>
 __global__ __launch_bounds__(TY) void k_adr_step(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
@@ -266,10 +266,10 @@ __global__ __launch_bounds__(TY) void k_adr_step(
             const float src = kAffine ? cur.src[kAffine ? s : 0] : 0.0f;
             const float los = kAffine ? cur.los[kAffine ? s : 0] : 0.0f;
             if constexpr (!kF64) {
-              const float l0 = xlo_edge ? exterior(P, 0, s, c, prev.c[s]) : prev.c[s];
-              const float u0 = xhi_edge ? exterior(P, 1, s, c, next.c[s]) : next.c[s];
-              const float l1 = ylo_edge ? exterior(P, 2, s, c, ylo_v) : ylo_v;
-              const float u1 = yhi_edge ? exterior(P, 3, s, c, yhi_v) : yhi_v;
+              const float l0 = xlo_edge ? exterior<kPatched>(P, 0, s, c, prev.c[s], y) : prev.c[s];
+              const float u0 = xhi_edge ? exterior<kPatched>(P, 1, s, c, next.c[s], y) : next.c[s];
+              const float l1 = ylo_edge ? exterior<kPatched>(P, 2, s, c, ylo_v, static_cast<uint32_t>(xx)) : ylo_v;
+              const float u1 = yhi_edge ? exterior<kPatched>(P, 3, s, c, yhi_v, static_cast<uint32_t>(xx)) : yhi_v;
               float v;
               if constexpr (kLimited) {
                 Limiter<float> lim;
@@ -305,7 +305,8 @@ __global__ __launch_bounds__(TY) void k_adr_step(
                 const double ul = static_cast<double>(g.ulo[a]);
                 const double uu = static_cast<double>(g.uup[a]);
                 if (lo_edge[a] && P.kind[2 * a] == MSIM_FIXED) {
-                  const double cb = static_cast<double>(P.boundary[s][2 * a]);
+                  const double cb = static_cast<double>(
+                      reservoir<kPatched>(P, 2 * a, s, a == 0 ? y : static_cast<uint32_t>(xx)));
                   fl = (ul >= 0.0 ? ul * cb : ul * c) - D * (c - cb) * ih[a];
                   supply += fl * ih[a];
                 } else if (!g.clo[a]) {
@@ -323,7 +324,8 @@ __global__ __launch_bounds__(TY) void k_adr_step(
                   }
                 }
                 if (hi_edge[a] && P.kind[2 * a + 1] == MSIM_FIXED) {
-                  const double cb = static_cast<double>(P.boundary[s][2 * a + 1]);
+                  const double cb = static_cast<double>(
+                      reservoir<kPatched>(P, 2 * a + 1, s, a == 0 ? y : static_cast<uint32_t>(xx)));
                   fu = (uu >= 0.0 ? uu * c : uu * cb) - D * (cb - c) * ih[a];
                   supply -= fu * ih[a];
                 } else if (!g.cup[a]) {
@@ -513,12 +515,14 @@ struct AdrShape {
   uint32_t rows, blocks;
 };
 
-AdrShape adr_shape(uint32_t nx, uint32_t ny, int halo) {
+AdrShape adr_shape(uint32_t nx, uint32_t ny, int halo, bool patched) {
   AdrShape s{};
   // The strip width (TY - 2 halo columns) that covers the grid's width with
-  // the fewest threads, the wider on a tie.
+  // the fewest threads, the wider on a tie; 128 columns with reservoir
+  // patches.
   uint64_t best = ~0ull;
   for (int ty : {256, 128, 64}) {
+    if (patched && ty != 128) continue;
     const uint32_t width = static_cast<uint32_t>(ty - 2 * halo);
     const uint64_t threads = static_cast<uint64_t>((ny + width - 1) / width) * ty;
     if (threads < best) {
@@ -574,6 +578,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   }
   P.has_field = (use_flow_field || grid.x_faces != nullptr) ? 1 : 0;
   P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
+  const bool patched = grid.patches != nullptr && grid.patch_count > 0;
   P.has_affine = grid.loss_rates != nullptr ? 1 : 0;
   P.has_monod = grid.phi != nullptr ? 1 : 0;
   a.precision = precision;
@@ -586,11 +591,44 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.mass.reserve(kMaxSignals);
   // Ledger partials [2][3 * kMaxSignals][blocks], then the van Leer block
   // maxima [blocks]; the survey's partials reuse the front.
-  const uint32_t step_parts = step_blocks(adr_shape(P.nx, P.ny, P.limited != 0 ? 2 : 1));
+  const uint32_t step_parts =
+      step_blocks(adr_shape(P.nx, P.ny, P.limited != 0 ? 2 : 1, patched));
   a.partials.reserve(std::max<size_t>(
       static_cast<size_t>(3 * kMaxSignals * 2 + 1) * step_parts,
       static_cast<size_t>(kMaxSignals + 1) * 2048));
   if (!same_spec) {
+    a.has_edge = patched;
+    if (patched) {
+      // GridBoundary::values_at (signals.cpp:439-447) for every boundary site
+      // of the four in-plane faces (second in-plane index z = 0): the last
+      // patch holding the site, else the face's value.
+      const size_t row = 2 * static_cast<size_t>(P.ny) + 2 * static_cast<size_t>(P.nx);
+      std::vector<float> edge(S * row);
+      const uint32_t extent[4] = {P.ny, P.ny, P.nx, P.nx};
+      const size_t offset[4] = {0, P.ny, 2 * static_cast<size_t>(P.ny),
+                                2 * static_cast<size_t>(P.ny) + P.nx};
+      for (uint32_t s = 0; s < S; ++s) {
+        for (int f = 0; f < 4; ++f) {
+          for (uint32_t along = 0; along < extent[f]; ++along) {
+            float v = P.boundary[s][f];
+            for (uint32_t k = grid.patch_count; k-- > 0;) {
+              const msim_boundary_patch& pt = grid.patches[k];
+              if (static_cast<int>(pt.face) == f && along >= pt.begin[0] && along < pt.end[0] &&
+                  0 >= pt.begin[1] && 0 < pt.end[1]) {
+                v = pt.values[s];
+                break;
+              }
+            }
+            edge[s * row + offset[f] + along] = v;
+          }
+        }
+      }
+      a.edge.reserve(edge.size());
+      check(hipMemcpyAsync(a.edge.data(), edge.data(), edge.size() * sizeof(float),
+                           hipMemcpyHostToDevice, ctx.stream),
+            "upload boundary patches");
+      check(hipStreamSynchronize(ctx.stream), "upload boundary patches");
+    }
     a.solid.reserve(n);
     if (grid.grid.obstacles != nullptr) {
       check(hipMemcpyAsync(a.solid.data(), grid.grid.obstacles, n, hipMemcpyHostToDevice,
@@ -634,6 +672,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
       }
     }
   }
+  P.edge = a.has_edge ? a.edge.data() : nullptr;
   a.flow_field = use_flow_field;
   a.revision = grid.revision;
   check(hipMemcpyAsync(a.levels_a.data(), levels, bytes, hipMemcpyHostToDevice, ctx.stream),
@@ -664,18 +703,18 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.uploaded = true;
 }
 
-template <typename T, int S, int TY, bool kLimited>
+template <typename T, int S, int TY, bool kLimited, bool kPatched>
 void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
                  const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
                  double* largest) {
   auto& a = ctx.adr;
   if (a.params.has_affine != 0) {
-    k_adr_step<T, S, TY, true, kLimited><<<sh.blocks, TY, 0, ctx.stream>>>(
+    k_adr_step<T, S, TY, true, kLimited, kPatched><<<sh.blocks, TY, 0, ctx.stream>>>(
         a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
         reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
         sh.rows, L, largest);
   } else {
-    k_adr_step<T, S, TY, false, kLimited><<<sh.blocks, TY, 0, ctx.stream>>>(
+    k_adr_step<T, S, TY, false, kLimited, kPatched><<<sh.blocks, TY, 0, ctx.stream>>>(
         a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
         reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
         sh.rows, L, largest);
@@ -693,12 +732,20 @@ template <typename T, int S, bool kLimited>
 void launch_typed(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
                   const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
                   double* largest) {
+  // Specs with reservoir patches take one strip width (128 columns).
+  if (ctx.adr.params.edge != nullptr) {
+    launch_step<T, S, 128, kLimited, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+    return;
+  }
   switch (sh.ty) {
-    case 64: launch_step<T, S, 64, kLimited>(ctx, sh, xf, yf, src, dst, dt, k, L, largest); break;
-    case 128:
-      launch_step<T, S, 128, kLimited>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+    case 64:
+      launch_step<T, S, 64, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
       break;
-    default: launch_step<T, S, 256, kLimited>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+    case 128:
+      launch_step<T, S, 128, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      break;
+    default:
+      launch_step<T, S, 256, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
   }
 }
 
@@ -707,7 +754,7 @@ void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const
   auto& a = ctx.adr;
   const auto& P = a.params;
   const bool limited = P.limited != 0;
-  AdrShape sh = adr_shape(P.nx, P.ny, limited ? 2 : 1);
+  AdrShape sh = adr_shape(P.nx, P.ny, limited ? 2 : 1, P.edge != nullptr);
   sh.blocks = step_blocks(sh);
   LedgerParts L{a.partials.data(), a.ledger.data(), sh.blocks};
   double* largest = a.partials.data() + static_cast<size_t>(3 * kMaxSignals * 2) * sh.blocks;

@@ -11,13 +11,29 @@
 
 namespace msim {
 
+// The reservoir value of signal s beyond the boundary site at position
+// `along` of a fixed face (y on an x face, x on a y face): the face's uniform
+// value, or with kPatched GridBoundary::values_at (signals.cpp:439-447)
+// resolved per site on the host.
+template <bool kPatched = false>
+__device__ inline float reservoir(const AdrParams& P, int face, int s, uint32_t along) {
+  if constexpr (!kPatched) {
+    return P.boundary[s][face];
+  } else {
+    const uint32_t offset = (face & 1) * (face < 2 ? P.ny : P.nx) + (face < 2 ? 0u : 2u * P.ny) +
+                            static_cast<uint32_t>(s) * (2u * P.ny + 2u * P.nx);
+    return P.edge[offset + along];
+  }
+}
+
 // The exterior value beyond a boundary site (boundary_value, signals.cpp:181-192).
+template <bool kPatched = false>
 __device__ inline float exterior(const AdrParams& P, int face, int s, float current,
-                                 float periodic) {
+                                 float periodic, uint32_t along) {
   const uint8_t kind = P.kind[face];
   if (kind == MSIM_NO_FLUX) return current;
   if (kind == MSIM_PERIODIC) return periodic;
-  return P.boundary[s][face];
+  return reservoir<kPatched>(P, face, s, along);
 }
 
 struct SiteGeometry {
@@ -164,14 +180,14 @@ __device__ inline float fe_candidate_f32(const AdrParams& P, const SiteGeometry&
   const size_t i = static_cast<size_t>(x) * P.ny + y;
   const float current = lv[i];
   const float lower0 =
-      x == 0 ? exterior(P, 0, s, current, lv[static_cast<size_t>(P.nx - 1) * P.ny + y])
+      x == 0 ? exterior(P, 0, s, current, lv[static_cast<size_t>(P.nx - 1) * P.ny + y], y)
              : lv[i - P.ny];
-  const float upper0 = x + 1 == P.nx ? exterior(P, 1, s, current, lv[y]) : lv[i + P.ny];
+  const float upper0 = x + 1 == P.nx ? exterior(P, 1, s, current, lv[y], y) : lv[i + P.ny];
   const float lower1 =
-      y == 0 ? exterior(P, 2, s, current, lv[static_cast<size_t>(x) * P.ny + P.ny - 1])
+      y == 0 ? exterior(P, 2, s, current, lv[static_cast<size_t>(x) * P.ny + P.ny - 1], x)
              : lv[i - 1];
   const float upper1 =
-      y + 1 == P.ny ? exterior(P, 3, s, current, lv[static_cast<size_t>(x) * P.ny]) : lv[i + 1];
+      y + 1 == P.ny ? exterior(P, 3, s, current, lv[static_cast<size_t>(x) * P.ny], x) : lv[i + 1];
   return fe_update_f32(P, g, current, lower0, upper0, lower1, upper1, s, dt, source, loss,
                        P.has_monod != 0 ? phi[i] : 0.0f);
 }

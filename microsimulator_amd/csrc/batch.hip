@@ -544,10 +544,10 @@ __global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
         g.uup[0] = uxu;
         g.ulo[1] = uyl;
         g.uup[1] = uyu;
-        const float l0 = xlo_edge ? exterior(A, 0, 0, current, 0.0f) : prev.c;
-        const float u0 = xhi_edge ? exterior(A, 1, 0, current, 0.0f) : next.c;
-        const float l1 = ylo_edge ? exterior(A, 2, 0, current, 0.0f) : s_c[buf][y - 1];
-        const float u1 = yhi_edge ? exterior(A, 3, 0, current, 0.0f) : s_c[buf][y + 1];
+        const float l0 = xlo_edge ? exterior(A, 0, 0, current, 0.0f, y) : prev.c;
+        const float u0 = xhi_edge ? exterior(A, 1, 0, current, 0.0f, y) : next.c;
+        const float l1 = ylo_edge ? exterior(A, 2, 0, current, 0.0f, static_cast<uint32_t>(xx)) : s_c[buf][y - 1];
+        const float u1 = yhi_edge ? exterior(A, 3, 0, current, 0.0f, static_cast<uint32_t>(xx)) : s_c[buf][y + 1];
         const float candidate =
             fe_update_f32(A, g, current, l0, u0, l1, u1, 0, dt, 0.0f, 0.0f, cur.phi);
         if (!isfinite(candidate) || candidate < 0.0f) bad = 1;

@@ -367,6 +367,31 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
       }
     }
   }
+  // Reservoir patches (GridBoundary::validate and the face extents,
+  // signals.cpp:449-479, 734-744).
+  if (s.patch_count > 0 && s.patches == nullptr) invalid("signal grid boundary patches are missing");
+  for (uint32_t k = 0; k < s.patch_count; ++k) {
+    const msim_boundary_patch& pt = s.patches[k];
+    if (pt.face > 5) invalid("unknown signal grid boundary face");
+    if (s.grid.boundary[pt.face] != MSIM_FIXED) {
+      invalid("non-fixed signal grid boundaries cannot have values");
+    }
+    if (pt.end[0] <= pt.begin[0] || pt.end[1] <= pt.begin[1]) {
+      invalid("signal grid boundary patches must hold at least one site");
+    }
+    if (pt.values == nullptr) invalid("fixed signal grid boundary values must match signal count");
+    for (uint32_t j = 0; j < s.signal_count; ++j) {
+      if (!std::isfinite(pt.values[j]) || pt.values[j] < 0.0f) {
+        invalid("fixed signal grid boundary values must be finite and non-negative");
+      }
+    }
+    const uint32_t axis = pt.face / 2;
+    const uint32_t extent[3][2] = {{s.grid.ny, s.grid.nz}, {s.grid.nx, s.grid.nz},
+                                   {s.grid.nx, s.grid.ny}};
+    if (pt.end[0] > extent[axis][0] || pt.end[1] > extent[axis][1]) {
+      invalid("signal grid boundary patches must lie on their face");
+    }
+  }
   const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
   auto solid = [&](size_t i) { return s.grid.obstacles != nullptr && s.grid.obstacles[i] != 0; };
   if (s.loss_rates != nullptr) {

@@ -176,6 +176,10 @@ struct AdrParams {
   uint8_t kind[6];
   uint8_t has_field, has_affine, has_monod;
   uint8_t limited;  // van Leer correction (SignalAdvectionKind::van_leer)
+  // Reservoir values of the fixed faces site by site where the spec has
+  // boundary patches (device array [S][x_lower ny | x_upper ny | y_lower nx |
+  // y_upper nx]); null: the uniform `boundary` values.
+  const float* edge;
   float diffusion[kMaxSignals];
   float advection[kMaxSignals][3];
   float boundary[kMaxSignals][6];
@@ -193,7 +197,8 @@ struct AdrDevice {
   double max_loss[kMaxSignals] = {};
   double host_mass[kMaxSignals] = {};  // mass of each signal now (FP64 runs)
   DeviceBuffer<uint8_t> solid;
-  DeviceBuffer<float> xf, yf, phi, source, loss;
+  DeviceBuffer<float> xf, yf, phi, source, loss, edge;
+  bool has_edge = false;
   DeviceBuffer<uint8_t> levels_a, levels_b;  // S*N of float or double
   int current = 0;
   DeviceBuffer<double> partials, ledger, mass;

@@ -59,13 +59,19 @@ class _FlowReport(C.Structure):
                 ("max_speed", C.c_float), ("solved_mean", C.c_float), ("factor", C.c_float)]
 
 
+class _BoundaryPatch(C.Structure):
+    _fields_ = [("face", C.c_uint32), ("begin", C.c_uint32 * 2), ("end", C.c_uint32 * 2),
+                ("values", C.c_void_p)]
+
+
 class _SignalGrid(C.Structure):
     _fields_ = [("grid", _Grid), ("signal_count", C.c_uint32), ("diffusion", C.c_void_p),
                 ("advection", C.c_void_p), ("boundary_values", C.c_void_p),
                 ("x_faces", C.c_void_p), ("y_faces", C.c_void_p), ("z_faces", C.c_void_p),
                 ("source_rates", C.c_void_p), ("loss_rates", C.c_void_p), ("phi", C.c_void_p),
                 ("monod_vmax", C.c_void_p), ("monod_k", C.c_void_p),
-                ("advection_scheme", C.c_uint32)]
+                ("advection_scheme", C.c_uint32), ("patches", C.c_void_p),
+                ("patch_count", C.c_uint32)]
 
 
 def _ptr(a):
@@ -195,10 +201,23 @@ class Oracle:
 
     def _signal_grid(self, grid: Grid, keep: list, diffusion, boundary_values=None,
                      faces=None, advection=None, source=None, loss=None, phi=None,
-                     vmax=None, monod_k=None, scheme=UPWIND):
+                     vmax=None, monod_k=None, scheme=UPWIND, patches=()):
+        """patches: (face, (begin0, begin1), (end0, end1), values per signal)."""
         sg = _SignalGrid()
         sg.grid = grid._c(keep)
         sg.advection_scheme = scheme
+        if len(patches):
+            table = (_BoundaryPatch * len(patches))()
+            for k, (face, begin, end, values) in enumerate(patches):
+                v = _f32(np.atleast_1d(values))
+                keep.append(v)
+                table[k].face = face
+                table[k].begin[0], table[k].begin[1] = begin
+                table[k].end[0], table[k].end[1] = end
+                table[k].values = v.ctypes.data
+            keep.append(table)
+            sg.patches = C.cast(table, C.c_void_p)
+            sg.patch_count = len(patches)
         diffusion = _f32(np.atleast_1d(diffusion))
         S = diffusion.size
         sg.signal_count = S
@@ -278,7 +297,7 @@ class Reference:
         L.ref_solve_depth_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, P, P, P,
                                            C.c_char_p, Z]
         L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, U,
-                                              F, U, P, P, C.c_char_p, Z]
+                                              U, P, P, F, U, P, P, C.c_char_p, Z]
         L.ref_set_threads.argtypes = [U]
 
     @staticmethod
@@ -320,7 +339,7 @@ class Reference:
 
     def advance_signal_grid(self, grid: Grid, levels, dt, steps=1, diffusion=(1.0,),
                             boundary_values=None, faces=None, source=None, loss=None,
-                            scheme=UPWIND):
+                            scheme=UPWIND, patches=()):
         kinds = self._kinds(grid)
         obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
         diffusion = _f32(np.atleast_1d(diffusion))
@@ -336,6 +355,10 @@ class Reference:
         if loss is not None:
             loss = _f32(loss).reshape(-1)
             src = np.zeros_like(loss) if source is None else _f32(source).reshape(-1)
+        geometry = np.array([[f, b[0], b[1], e[0], e[1]] for f, b, e, _ in patches],
+                            np.uint32).reshape(-1)
+        pvalues = (np.concatenate([_f32(np.atleast_1d(v)) for *_, v in patches])
+                   if len(patches) else np.zeros(1, np.float32))
         secs = C.c_double()
         err = C.create_string_buffer(512)
         st = self.lib.ref_advance_signal_grid(
@@ -344,7 +367,9 @@ class Reference:
             C.c_void_p(kinds.ctypes.data), C.c_uint32(S), C.c_void_p(diffusion.ctypes.data),
             C.c_void_p(bv.ctypes.data), C.c_void_p(_ptr(fx)), C.c_void_p(_ptr(fy)),
             C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)), C.c_void_p(_ptr(loss)),
-            C.c_uint32(scheme), C.c_float(dt), C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.byref(secs), err, C.c_size_t(512))
+            C.c_uint32(scheme), C.c_uint32(len(patches)),
+            C.c_void_p(_ptr(geometry) if len(patches) else None),
+            C.c_void_p(_ptr(pvalues) if len(patches) else None), C.c_float(dt), C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.byref(secs), err, C.c_size_t(512))
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
         return lv, secs.value

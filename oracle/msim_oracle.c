@@ -800,15 +800,30 @@ static void face_closure(const mo_grid* g, uint32_t x, uint32_t y, uint32_t z, i
 
 /* boundary_value, cpp/core/signals.cpp:181-192 (uniform face values; the
  * per-patch reservoirs of GridBoundary::values_at are not used here). */
+/* GridBoundary::values_at (signals.cpp:439-447): the reservoir value of
+ * signal s at the boundary site with in-plane indices (first, second) of a
+ * fixed face; the last patch that holds the site wins. */
+static float reservoir(const mo_signal_grid* sg, int face, uint32_t s, uint32_t first,
+                       uint32_t second) {
+  for (uint32_t k = sg->patch_count; k-- > 0;) {
+    const mo_boundary_patch* pt = &sg->patches[k];
+    if ((int)pt->face == face && first >= pt->begin[0] && first < pt->end[0] &&
+        second >= pt->begin[1] && second < pt->end[1])
+      return pt->values[s];
+  }
+  return sg->boundary_values[6 * s + (uint32_t)face];
+}
+
+/* boundary_value (signals.cpp:179-192). */
 static float boundary_value(const mo_signal_grid* sg, int face, uint32_t s, float current,
-                            float periodic) {
+                            float periodic, uint32_t first, uint32_t second) {
   switch (sg->grid.boundary[face]) {
     case MO_NO_FLUX:
       return current;
     case MO_PERIODIC:
       return periodic;
     default:
-      return sg->boundary_values[6 * s + (uint32_t)face];
+      return reservoir(sg, face, s, first, second);
   }
 }
 
@@ -962,17 +977,17 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
           face_velocities(sg, s, x, y, z, flo, fup);
           face_closure(g, x, y, z, clo, cup);
           float lower[3], upper[3];
-          lower[0] = x == 0 ? boundary_value(sg, 0, s, current, lv[site_of(g, g->nx - 1, y, z)])
+          lower[0] = x == 0 ? boundary_value(sg, 0, s, current, lv[site_of(g, g->nx - 1, y, z)], y, z)
                             : lv[site_of(g, x - 1, y, z)];
-          upper[0] = x + 1 == g->nx ? boundary_value(sg, 1, s, current, lv[site_of(g, 0, y, z)])
+          upper[0] = x + 1 == g->nx ? boundary_value(sg, 1, s, current, lv[site_of(g, 0, y, z)], y, z)
                                     : lv[site_of(g, x + 1, y, z)];
-          lower[1] = y == 0 ? boundary_value(sg, 2, s, current, lv[site_of(g, x, g->ny - 1, z)])
+          lower[1] = y == 0 ? boundary_value(sg, 2, s, current, lv[site_of(g, x, g->ny - 1, z)], x, z)
                             : lv[site_of(g, x, y - 1, z)];
-          upper[1] = y + 1 == g->ny ? boundary_value(sg, 3, s, current, lv[site_of(g, x, 0, z)])
+          upper[1] = y + 1 == g->ny ? boundary_value(sg, 3, s, current, lv[site_of(g, x, 0, z)], x, z)
                                     : lv[site_of(g, x, y + 1, z)];
-          lower[2] = z == 0 ? boundary_value(sg, 4, s, current, lv[site_of(g, x, y, g->nz - 1)])
+          lower[2] = z == 0 ? boundary_value(sg, 4, s, current, lv[site_of(g, x, y, g->nz - 1)], x, y)
                             : lv[site_of(g, x, y, z - 1)];
-          upper[2] = z + 1 == g->nz ? boundary_value(sg, 5, s, current, lv[site_of(g, x, y, 0)])
+          upper[2] = z + 1 == g->nz ? boundary_value(sg, 5, s, current, lv[site_of(g, x, y, 0)], x, y)
                                     : lv[site_of(g, x, y, z + 1)];
           float rate = 0.0f;
           for (int a = 0; a < 3; ++a) {
@@ -1126,7 +1141,7 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
               joined = !solid_at(g, nb);
             } else if (at_edge) {
               if (g->boundary[face] == MO_FIXED) {
-                const double cb = (double)sg->boundary_values[6 * s + (uint32_t)face];
+                const double cb = (double)reservoir(sg, face, s, a == 0 ? y : x, 0);
                 /* Reservoir one full spacing beyond the site. */
                 if (lower_side)
                   flux = (u >= 0.0 ? u * cb : u * c) - D * (c - cb) * ih;

@@ -92,6 +92,15 @@ int mo_flow_solve(const mo_grid* grid, const float* mobility, uint32_t axis,
                   float* faces, double* pressure, mo_flow_report* report, char* err,
                   size_t errlen);
 
+/* GridBoundaryPatch (signals.hpp:72-84): a rectangle of a fixed face with its
+ * own reservoir value per signal. In-plane indices: on an x face (y, z), on a
+ * y face (x, z), on a z face (x, y); begin inclusive, end exclusive. */
+typedef struct {
+  uint32_t face; /* 0..5: x_lower, x_upper, y_lower, y_upper, z_lower, z_upper */
+  uint32_t begin[2], end[2];
+  const float* values; /* signal_count values */
+} mo_boundary_patch;
+
 /* Signal-grid transport description for one explicit step. */
 typedef struct {
   mo_grid grid;
@@ -108,6 +117,8 @@ typedef struct {
   const float* monod_vmax;      /* S */
   const float* monod_k;         /* S */
   uint32_t advection_scheme;    /* MO_UPWIND or MO_VAN_LEER (SignalAdvectionKind) */
+  const mo_boundary_patch* patches; /* reservoir patches of fixed faces, or NULL */
+  uint32_t patch_count;
 } mo_signal_grid;
 
 enum { MO_UPWIND = 0, MO_VAN_LEER = 1 };

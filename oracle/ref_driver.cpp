@@ -226,10 +226,24 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
                             const float* diffusion, const float* boundary_values,
                             const float* x_faces, const float* y_faces, const float* z_faces,
                             const float* source_rates, const float* loss_rates,
-                            std::uint32_t scheme, float dt, std::uint32_t steps, float* levels,
-                            double* seconds, char* err, std::size_t errlen) {
+                            std::uint32_t scheme, std::uint32_t patch_count,
+                            const std::uint32_t* patch_geometry, const float* patch_values,
+                            float dt, std::uint32_t steps, float* levels, double* seconds,
+                            char* err, std::size_t errlen) {
   return guarded(err, errlen, [&] {
     auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
+    // Reservoir patches: per patch (face, begin[0], begin[1], end[0], end[1])
+    // and `signals` values.
+    ms::GridBoundary* faces[6] = {&spec.x_lower, &spec.x_upper, &spec.y_lower,
+                                  &spec.y_upper, &spec.z_lower, &spec.z_upper};
+    for (std::uint32_t k = 0; k < patch_count; ++k) {
+      const std::uint32_t* g = patch_geometry + 5 * k;
+      ms::GridBoundaryPatch patch;
+      patch.region.begin = {g[1], g[2]};
+      patch.region.end = {g[3], g[4]};
+      patch.values.assign(patch_values + k * signals, patch_values + (k + 1) * signals);
+      faces[g[0]]->patches.push_back(std::move(patch));
+    }
     const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
     spec.diffusion.assign(diffusion, diffusion + signals);
     spec.advection_scheme =

@@ -140,3 +140,27 @@ def layered_case(nx, ny, nz, seed, mobility=True, axis=0):
 LAYERED_CASES = [(12, 9, 3, 1, True), (20, 7, 4, 2, False), (15, 15, 2, 3, True),
                  (10, 12, 5, 4, False)]
 
+
+def random_patches(kinds, nx, ny, S, seed):
+    """Two overlapping reservoir patches on every fixed in-plane face, each
+    with its own value per signal (in-plane indices: y on an x face, x on a
+    y face; the second index is z, of extent one)."""
+    rng = np.random.default_rng(seed)
+    patches = []
+    for face in range(4):
+        if kinds[face] != FIXED:
+            continue
+        extent = ny if face < 2 else nx
+        for _ in range(2):
+            begin = int(rng.integers(0, max(1, extent - 2)))
+            end = int(rng.integers(begin + 1, extent + 1))
+            patches.append((face, (begin, 0), (end, 1), rng.random(S).astype(np.float32)))
+    return patches
+
+
+PATCH_KINDS = [
+    (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX),
+    (FIXED, FIXED, FIXED, FIXED, NO_FLUX, NO_FLUX),
+    (PERIODIC, PERIODIC, FIXED, FIXED, NO_FLUX, NO_FLUX),
+]
+

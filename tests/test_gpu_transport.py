@@ -13,8 +13,9 @@ import pytest
 import microsimulator_amd as msa
 from oracle import VAN_LEER
 from oracle import Grid as OGrid
-from tests.helpers import (FIXED, NO_FLUX, PERIODIC, TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs,
-                           golden, random_transport_case, split_faces)
+from tests.helpers import (FIXED, NO_FLUX, PATCH_KINDS, PERIODIC, TRANSPORT_KINDS,
+                           TRANSPORT_SHAPES, c1_inputs, golden, random_patches,
+                           random_transport_case, split_faces)
 
 pytestmark = pytest.mark.gpu
 
@@ -186,6 +187,51 @@ def test_van_leer_front_and_bound(hip, oracle):
         hip.advance_signal_grid(spec, lv, 0.51)
     upwind = msa.SignalGridSpec(grid, [0.0], velocity=msa.VelocityField(xf, yf, zf))
     hip.advance_signal_grid(upwind, lv, 0.51)
+
+
+@pytest.mark.parametrize("kinds", PATCH_KINDS)
+def test_boundary_patches_on_device(hip, oracle, kinds):
+    """Reservoir patches of fixed faces: FP32 bit for bit with the oracle
+    (bit-exact with the reference's values_at), upwind and van Leer; FP64 to
+    1e-13 with the patches' supply in a balanced ledger."""
+    nx, ny, S = 130, 70, 2
+    d = random_transport_case(nx, ny, S, kinds, seed=5)
+    patches = random_patches(kinds, nx, ny, S, seed=1)
+    og = OGrid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], kinds)
+    grid = msa.Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], kinds)
+    kw = dict(diffusion=d["diffusion"], boundary_values=d["bv"], source=d["source"],
+              loss=d["loss"], faces=np.concatenate([d["xf"], d["yf"], d["zf"]]))
+    for scheme in (msa.UPWIND, msa.VAN_LEER):
+        print("case", kinds, "scheme", scheme, flush=True)
+        spec = msa.SignalGridSpec(grid, d["diffusion"], d["bv"],
+                                  msa.VelocityField(d["xf"], d["yf"], d["zf"]),
+                                  source_rates=d["source"], loss_rates=d["loss"],
+                                  advection_scheme=scheme, patches=patches)
+        lv32 = d["levels"].astype(np.float32)
+        hip.adr_upload(spec, lv32, precision=32)
+        hip.adr_run(0.02, 3, ledger=False)
+        ref32 = oracle.adr_step_f32(og, lv32, 0.02, steps=3, scheme=scheme, patches=patches, **kw)
+        assert np.array_equal(hip.adr_download(), ref32)
+        hip.adr_upload(spec, d["levels"], precision=64)
+        ledger = hip.adr_run(0.02, 3)
+        ref64, ref_ledger = oracle.adr_step_f64(og, d["levels"], 0.02, steps=3, scheme=scheme,
+                                                patches=patches, **kw)
+        assert np.abs(hip.adr_download() - ref64).max() <= 1e-13 * np.abs(ref64).max()
+        assert np.allclose(ledger, ref_ledger, rtol=1e-12, atol=1e-300)
+
+
+def test_boundary_patch_refusals(hip):
+    """The reference's patch checks (GridBoundary::validate and the face
+    extents, signals.cpp:449-479, 734-744), with its messages."""
+    grid = msa.Grid(10, 6, 1, boundary=(FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX))
+    cases = [((2, (0, 0), (3, 1), [0.5]), "non-fixed signal grid boundaries cannot have values"),
+             ((0, (2, 0), (2, 1), [0.5]), "must hold at least one site"),
+             ((0, (0, 0), (3, 1), [-1.0]), "must be finite and non-negative"),
+             ((1, (4, 0), (7, 1), [0.5]), "must lie on their face")]
+    for patch, message in cases:
+        spec = msa.SignalGridSpec(grid, [0.1], [1.0, 0, 0, 0, 0, 0], patches=(patch,))
+        with pytest.raises(msa.InvalidArgument, match=message):
+            hip.advance_signal_grid(spec, np.zeros(60, np.float32), 0.1)
 
 
 def test_explicit_bound_refuses_and_leaves_levels(hip):

@@ -8,9 +8,9 @@ import numpy as np
 import pytest
 
 from oracle import FIXED, NO_FLUX, PERIODIC, VAN_LEER, Grid, OracleError
-from tests.helpers import (LAYERED_CASES, TRANSPORT_KINDS, TRANSPORT_SHAPES, c1_inputs, divergence,
-                           golden, illcond_inputs, layered_case, monod_loss, random_case,
-                           random_transport_case, split_faces)
+from tests.helpers import (LAYERED_CASES, PATCH_KINDS, TRANSPORT_KINDS, TRANSPORT_SHAPES,
+                           c1_inputs, divergence, golden, illcond_inputs, layered_case, monod_loss,
+                           random_case, random_patches, random_transport_case, split_faces)
 
 
 def c1_grid():
@@ -447,6 +447,32 @@ def test_layered_shallow_flow_refusals_match_reference(oracle, reference):
             oracle.solve_flow(grid, mobility, 0, 1.0, 1e-6, 1000)
         with pytest.raises(OracleError, match=message):
             reference.solve_flow(grid, mobility, 0, 1.0, 1e-6, 1000)
+
+
+@pytest.mark.parametrize("kinds", PATCH_KINDS)
+def test_boundary_patches_match_reference_live(oracle, reference, kinds):
+    """Reservoir patches on fixed faces (GridBoundary::values_at: the last
+    patch holding a site wins), upwind and van Leer, bit for bit with the
+    reference; the patches change the result."""
+    nx, ny, S = 30, 20, 2
+    d = random_transport_case(nx, ny, S, kinds, seed=5)
+    grid = Grid(nx, ny, 1, 1.0, 0.5, 1.0, d["solid"], boundary=kinds)
+    patches = random_patches(kinds, nx, ny, S, seed=1)
+    kw = dict(diffusion=d["diffusion"], boundary_values=d["bv"], source=d["source"],
+              loss=d["loss"], faces=np.concatenate([d["xf"], d["yf"], d["zf"]]))
+    lv = d["levels"].astype(np.float32)
+    for scheme in (0, VAN_LEER):
+        a = oracle.adr_step_f32(grid, lv, 0.02, steps=3, scheme=scheme, patches=patches, **kw)
+        b, _ = reference.advance_signal_grid(grid, lv, 0.02, 3, scheme=scheme, patches=patches,
+                                             **kw)
+        assert np.array_equal(a, b)
+        plain = oracle.adr_step_f32(grid, lv, 0.02, steps=3, scheme=scheme, **kw)
+        assert np.abs(a - plain).max() > 1e-3
+        c, ledger = oracle.adr_step_f64(grid, d["levels"], 0.02, steps=3, scheme=scheme,
+                                        patches=patches, **kw)
+        assert np.abs(a - c).max() <= 1e-6
+        balance = ledger[:, :, 1] - ledger[:, :, 0] - ledger[:, :, 2] + ledger[:, :, 3]
+        assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
 
 
 def test_flow_error_paths(oracle):
