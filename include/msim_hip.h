@@ -173,6 +173,74 @@ int msim_hip_flow_precondition(msim_ctx* ctx, const float* r, float* z);
 int msim_hip_flow_arrays(msim_ctx* ctx, float* conductance, float* diagonal, float* rhs,
                          float* boundary_conductance);
 
+/* FlowPortKind (cpp/include/microsimulator/flow.hpp:69-74). */
+enum { MSIM_PORT_SPEED = 0, MSIM_PORT_PRESSURE = 1 };
+
+/* FlowPort (flow.hpp:76-91): an opening over a rectangle of one fixed face.
+ * face: 0..5 in the order of msim_grid.boundary. begin/end: the rectangle's
+ * in-plane indices (y, z on an x face; x, z on a y face; x, y on a z face),
+ * end exclusive. value: for a speed port the mean speed normal to the face
+ * over its open area, positive into the grid; for a pressure port the
+ * pressure held just outside the face. */
+typedef struct {
+  uint32_t face;
+  uint32_t begin[2];
+  uint32_t end[2];
+  uint32_t kind; /* MSIM_PORT_SPEED or MSIM_PORT_PRESSURE */
+  float value;
+} msim_flow_port;
+
+/* FlowPortState (flow.hpp:93-100): the pressure held outside the port, the
+ * volume entering the grid through it per unit time (negative when leaving)
+ * and that flux over its open area. */
+typedef struct {
+  float pressure;
+  float flux;
+  float mean_speed;
+} msim_port_state;
+
+/* DepthAveragedPortFlowParameters (flow.hpp:102-107) plus the device form of
+ * each pressure solve. */
+typedef struct {
+  float relative_tolerance;     /* per pressure solve; at least float epsilon */
+  uint32_t max_iterations;      /* per pressure solve; positive */
+  uint32_t solver;              /* MSIM_SOLVER_*: JACOBI_FP64 is the reference's PCG */
+  uint32_t check_interval;      /* as msim_flow_params; 0 = 64 */
+  double inner_tolerance;       /* MIXED/LEARNED inner tolerance; 0 = auto */
+  uint32_t fallback_iterations; /* LEARNED: cap before the Jacobi-PCG fallback */
+} msim_port_flow_params;
+
+/* DepthAveragedPortFlowReport (flow.hpp:109-114) plus device figures. */
+typedef struct {
+  uint32_t solves;               /* pressure solves combined into the field */
+  uint32_t iterations;           /* Krylov iterations summed over the solves */
+  float relative_residual;       /* largest over the solves */
+  float max_speed;               /* largest |velocity| of the combined field */
+  double true_relative_residual; /* largest FP64 ||b - Ax|| / ||b|| over the solves */
+  double seconds;                /* device time of the solves, lifts and combination */
+} msim_port_flow_report;
+
+/* ComputeBackend::solve_depth_averaged_port_flow (backend.hpp:47-49), as
+ * solve_depth_averaged_port_flow_cpu (cpp/cpu/cpu_flow.cpp:353-375) over
+ * ShallowPortFlow (cpp/core/flow_system.hpp:1268-1311): the ports' checks
+ * (validate_flow_ports, cpu_flow.cpp:193-286, on the grid and on its grid of
+ * columns), one pressure solve on the device per speed port held alone at
+ * pressure one plus one for the pressure ports' own pressures when any is
+ * nonzero, each lifted onto the voxel faces on the device, and the
+ * combination that carries each speed port's stated mean speed
+ * (superpose_port_flows, flow_system.hpp:1139-1252): the small dense system
+ * for the speed ports' pressures is solved on the host, the field combined
+ * on the device. mobility: one float per site or NULL. x/y/z_faces: the
+ * combined face velocities (any may be NULL). states: port_count entries
+ * (may be NULL). The prepared flow system is replaced. */
+int msim_hip_solve_depth_averaged_port_flow(msim_ctx* ctx, const msim_grid* grid,
+                                            const float* mobility, const msim_flow_port* ports,
+                                            uint32_t port_count,
+                                            const msim_port_flow_params* params, float* x_faces,
+                                            float* y_faces, float* z_faces,
+                                            msim_port_flow_report* report,
+                                            msim_port_state* states);
+
 /* A signal grid for the explicit transport step: SignalGridSpec's transport
  * fields (signals.hpp:122-156) flattened. boundary_values: 6*S reservoir
  * values, face f of signal s at [6*s + f] (uniform per face). Velocity field

@@ -23,6 +23,8 @@ LIBRARY = os.path.join(HERE, "libmsim_hip.so")
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
 SOLVER_JACOBI_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_MULTIGRID = 0, 1, 2, 3
 UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind (signals.hpp:41-47)
+SPEED, PRESSURE = 0, 1  # FlowPortKind (flow.hpp:69-74)
+X_LOWER, X_UPPER, Y_LOWER, Y_UPPER, Z_LOWER, Z_UPPER = range(6)  # GridFace
 
 
 # --------------------------------------------------------------------------
@@ -110,6 +112,27 @@ class _BoundaryPatch(C.Structure):
                 ("values", C.c_void_p)]
 
 
+class _FlowPort(C.Structure):
+    _fields_ = [("face", C.c_uint32), ("begin", C.c_uint32 * 2), ("end", C.c_uint32 * 2),
+                ("kind", C.c_uint32), ("value", C.c_float)]
+
+
+class _PortState(C.Structure):
+    _fields_ = [("pressure", C.c_float), ("flux", C.c_float), ("mean_speed", C.c_float)]
+
+
+class _PortFlowParams(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_float), ("max_iterations", C.c_uint32),
+                ("solver", C.c_uint32), ("check_interval", C.c_uint32),
+                ("inner_tolerance", C.c_double), ("fallback_iterations", C.c_uint32)]
+
+
+class _PortFlowReport(C.Structure):
+    _fields_ = [("solves", C.c_uint32), ("iterations", C.c_uint32),
+                ("relative_residual", C.c_float), ("max_speed", C.c_float),
+                ("true_relative_residual", C.c_double), ("seconds", C.c_double)]
+
+
 class _SignalGrid(C.Structure):
     _fields_ = [("grid", _Grid), ("signal_count", C.c_uint32), ("diffusion", C.c_void_p),
                 ("boundary_values", C.c_void_p), ("x_faces", C.c_void_p),
@@ -162,6 +185,7 @@ EXPORTS = (
     "msim_hip_learned_apply", "msim_hip_batch_upload", "msim_hip_batch_solve_flow",
     "msim_hip_batch_autotune", "msim_hip_batch_shapes", "msim_hip_batch_shape",
     "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
+    "msim_hip_solve_depth_averaged_port_flow",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -214,6 +238,7 @@ def library(path: str = LIBRARY):
         "msim_hip_batch_flags": (I, [P, P]),
         "msim_hip_batch_run": (I, [P, C.c_float, U, U, P, P]),
         "msim_hip_batch_download": (I, [P, U, P, P, P, P]),
+        "msim_hip_solve_depth_averaged_port_flow": (I, [P, P, P, P, U, P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -361,6 +386,67 @@ class VelocityField:
 class DepthAveragedFlowResult:
     field: VelocityField
     report: DepthAveragedFlowReport
+
+
+@dataclass
+class FlowPort:
+    """FlowPort (flow.hpp:76-91): an opening over the rectangle
+    [begin, end) of one fixed face's in-plane indices (y, z on an x face;
+    x, z on a y face). A SPEED port carries `value` as its mean speed normal
+    to the face, positive into the grid; a PRESSURE port is held at `value`."""
+    face: int
+    begin: tuple
+    end: tuple
+    kind: int = PRESSURE
+    value: float = 0.0
+
+
+@dataclass
+class FlowPortState:
+    """FlowPortState (flow.hpp:93-100)."""
+    pressure: float
+    flux: float
+    mean_speed: float
+
+
+@dataclass
+class DepthAveragedPortFlowParameters:
+    """DepthAveragedPortFlowParameters (flow.hpp:102-107) plus the device
+    form of each pressure solve."""
+    relative_tolerance: float = 1.0e-6
+    max_iterations: int = 50_000
+    solver: int = SOLVER_JACOBI_FP64
+    check_interval: int = 64
+    inner_tolerance: float = 0.0
+    fallback_iterations: int = 200
+
+    def _c(self) -> _PortFlowParams:
+        p = _PortFlowParams()
+        p.relative_tolerance = self.relative_tolerance
+        p.max_iterations = self.max_iterations
+        p.solver = self.solver
+        p.check_interval = self.check_interval
+        p.inner_tolerance = self.inner_tolerance
+        p.fallback_iterations = self.fallback_iterations
+        return p
+
+
+@dataclass
+class DepthAveragedPortFlowReport:
+    """DepthAveragedPortFlowReport (flow.hpp:109-114) plus device figures."""
+    solves: int
+    iterations: int
+    relative_residual: float
+    max_speed: float
+    true_relative_residual: float
+    seconds: float
+
+
+@dataclass
+class DepthAveragedPortFlowResult:
+    field: VelocityField
+    report: DepthAveragedPortFlowReport
+    ports: list
 
 
 @dataclass
@@ -522,6 +608,43 @@ class HipBackend:
             xf.ctypes.data, yf.ctypes.data, zf.ctypes.data, C.byref(rep)))
         return DepthAveragedFlowResult(VelocityField(xf, yf, zf),
                                        DepthAveragedFlowReport._from(rep))
+
+    def solve_depth_averaged_port_flow(self, grid: Grid, ports, mobility=None,
+                                       parameters: DepthAveragedPortFlowParameters | None = None
+                                       ) -> DepthAveragedPortFlowResult:
+        """ComputeBackend::solve_depth_averaged_port_flow (backend.hpp:47-49)."""
+        parameters = parameters or DepthAveragedPortFlowParameters()
+        keep: list = []
+        g = grid._c(keep)
+        mob = None
+        if mobility is not None:
+            mob = np.ascontiguousarray(mobility, dtype=np.float32).reshape(-1)
+            if mob.size != grid.sites:
+                raise InvalidArgument("flow mobility must hold one value per grid site")
+        arr = (_FlowPort * max(1, len(ports)))()
+        for i, port in enumerate(ports):
+            arr[i].face = port.face
+            arr[i].begin[0], arr[i].begin[1] = port.begin
+            arr[i].end[0], arr[i].end[1] = port.end
+            arr[i].kind = port.kind
+            arr[i].value = port.value
+        states = (_PortState * max(1, len(ports)))()
+        nfx, nfy, nfz = grid.face_counts()
+        xf = np.zeros(nfx, np.float32)
+        yf = np.zeros(nfy, np.float32)
+        zf = np.zeros(nfz, np.float32)
+        p = parameters._c()
+        rep = _PortFlowReport()
+        self._check(self.lib.msim_hip_solve_depth_averaged_port_flow(
+            self.ctx, C.byref(g), None if mob is None else mob.ctypes.data,
+            C.cast(arr, C.c_void_p), len(ports), C.byref(p), xf.ctypes.data, yf.ctypes.data,
+            zf.ctypes.data, C.byref(rep), C.cast(states, C.c_void_p)))
+        report = DepthAveragedPortFlowReport(rep.solves, rep.iterations, rep.relative_residual,
+                                             rep.max_speed, rep.true_relative_residual,
+                                             rep.seconds)
+        return DepthAveragedPortFlowResult(
+            VelocityField(xf, yf, zf), report,
+            [FlowPortState(st.pressure, st.flux, st.mean_speed) for st in states[:len(ports)]])
 
     def advance_signal_grid(self, spec: SignalGridSpec, levels, dt: float):
         """ComputeBackend::advance_signal_grid (backend.hpp:30), forward Euler:
@@ -778,4 +901,4 @@ class HipBackend:
 
     advance_growth = advance_species = advance_coupled = _unsupported
     find_cell_contacts = find_external_contacts = solve_contact_forces = _unsupported
-    solve_depth_averaged_port_flow = solve_resolved_flow = solve_resolved_port_flow = _unsupported
+    solve_resolved_flow = solve_resolved_port_flow = _unsupported

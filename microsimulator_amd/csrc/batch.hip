@@ -845,7 +845,7 @@ void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
     launch_assembly(ctx.stream, B.solid.data() + d.site, B.mobility.data() + d.site,
                     B.cond.data() + d.site, B.diag.data() + d.site, B.rhs.data() + d.site,
                     B.bc.data() + d.site, B.cxl.data() + d.site, B.cyl.data() + d.site, d.nx,
-                    d.ny, d.dx, d.dy, d.dz, 0);
+                    d.ny, d.dx, d.dy, d.dz, FlowEdges{});
   }
   k_batch_dinv<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, ctx.stream>>>(
       B.diag.data(), B.dinv.data(), n);

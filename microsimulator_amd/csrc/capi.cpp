@@ -120,43 +120,24 @@ void check_reachable(uint32_t nx, uint32_t ny, uint32_t nz, const std::vector<ui
   throw Error(MSIM_RUNTIME_ERROR, "the device carries no through-flow: the outlet is unreachable");
 }
 
-// The checks reduce_shallow_flow and the flow system's constructor make
-// before a solve (flow_system.hpp:1255-1266, 260-330; cpu_flow.cpp:288-336),
-// and the reduction to a grid of columns (ShallowFlowReduction,
-// flow_system.hpp:257-330): `solid` and `mob` are what the device solve on
-// the grid of columns takes (mask and per-site mobility, the conductance
-// being dz times the mobility). One layer keeps the user's mobility; several
-// layers pass each column's conductance with dz = 1, so the device's
-// conductance is that value exactly.
-void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
-                   std::vector<uint8_t>& solid, std::vector<float>& mob,
-                   FlowDevice::Layers& layers) {
-  validate_lattice(g);
-  if (axis > 2) invalid("unknown flow axis");
-  for (int a = 0; a < 3; ++a) {
-    if (g.boundary[2 * a] == MSIM_PERIODIC || g.boundary[2 * a + 1] == MSIM_PERIODIC) {
-      invalid("native flow solvers do not support periodic boundaries");
-    }
-  }
-  if (g.boundary[2 * axis] != MSIM_FIXED || g.boundary[2 * axis + 1] != MSIM_FIXED) {
-    invalid("native flow-axis boundaries must be fixed to identify inlet and outlet");
-  }
-  const size_t voxels = static_cast<size_t>(g.nx) * g.ny * g.nz;
-  std::vector<uint8_t> mask(voxels, 0);
-  if (g.obstacles != nullptr) std::memcpy(mask.data(), g.obstacles, voxels);
-  check_reachable(g.nx, g.ny, g.nz, mask, axis);
-  if (axis == 2) invalid("shallow flow is depth-integrated along z; flow axis must be x or y");
+// The grid of columns (ShallowFlowReduction, flow_system.hpp:257-330):
+// contiguous fluid voxels from a floor common to every column, one mobility
+// per column; height spacing.z per fluid voxel, conductance height *
+// mobility, or without mobilities height * (height / max height)^2; a
+// column without conductance is solid.
+struct Columns {
+  std::vector<uint8_t> solid;  // per column
+  std::vector<float> heights, conductance;
+  uint32_t bottom = 0;
+};
+
+Columns reduce_columns(const msim_grid& g, const float* mobility, const std::vector<uint8_t>& mask) {
   const size_t n = static_cast<size_t>(g.nx) * g.ny;
   const uint32_t nz = g.nz;
-  layers = FlowDevice::Layers{};
-  layers.nz = nz;
-  layers.dz = g.dz;
-  // Columns (flow_system.hpp:276-327): contiguous fluid voxels from a floor
-  // common to every column, one mobility per column; height spacing.z per
-  // fluid voxel, conductance height * mobility, or without mobilities
-  // height * (height / max height)^2.
-  std::vector<float> heights(n, 0.0f), conductance(n, 0.0f);
-  std::vector<uint8_t> columns(n, 1);
+  Columns c;
+  c.heights.assign(n, 0.0f);
+  c.conductance.assign(n, 0.0f);
+  c.solid.assign(n, 1);
   uint32_t common_bottom = nz;
   float max_height = 0.0f;
   for (size_t column = 0; column < n; ++column) {
@@ -181,107 +162,295 @@ void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
         invalid("shallow-flow mobility must be constant through each depth column");
       }
       started = true;
-      heights[column] += g.dz;
+      c.heights[column] += g.dz;
     }
     if (started) {
-      columns[column] = 0;
-      conductance[column] = heights[column] * value;
-      max_height = std::max(max_height, heights[column]);
+      c.solid[column] = 0;
+      c.conductance[column] = c.heights[column] * value;
+      max_height = std::max(max_height, c.heights[column]);
     }
   }
   if (mobility == nullptr && max_height > 0.0f) {
     for (size_t i = 0; i < n; ++i) {
-      const float relative = heights[i] / max_height;
-      conductance[i] *= relative * relative;
+      const float relative = c.heights[i] / max_height;
+      c.conductance[i] *= relative * relative;
     }
   }
   for (size_t i = 0; i < n; ++i) {
-    if (conductance[i] == 0.0f) columns[i] = 1;
+    if (c.conductance[i] == 0.0f) c.solid[i] = 1;
   }
-  // The grid of columns must still carry flow.
-  check_reachable(g.nx, g.ny, 1, columns, axis);
-  if (nz == 1) {
+  c.bottom = common_bottom;
+  return c;
+}
+
+// What the device solve on the grid of columns takes: mask and per-site
+// mobility, the device conductance being dz times the mobility. One layer
+// keeps the user's mobility; several layers pass each column's conductance
+// with dz = 1, so the device's conductance is that value exactly, and keep
+// what the lift reads.
+void take_columns(const msim_grid& g, const float* mobility, std::vector<uint8_t>&& mask,
+                  Columns&& c, std::vector<uint8_t>& solid, std::vector<float>& mob,
+                  FlowDevice::Layers& layers) {
+  const size_t n = static_cast<size_t>(g.nx) * g.ny;
+  layers = FlowDevice::Layers{};
+  layers.nz = g.nz;
+  layers.dz = g.dz;
+  if (g.nz == 1) {
     solid = std::move(mask);
     mob.assign(n, 1.0f);
     if (mobility != nullptr) std::memcpy(mob.data(), mobility, n * sizeof(float));
+    layers.heights = std::move(c.heights);
     return;
   }
-  solid = columns;
-  mob = conductance;
-  layers.bottom = common_bottom;
+  solid = c.solid;
+  mob = c.conductance;
+  layers.bottom = c.bottom;
   layers.voxels = std::move(mask);
-  layers.heights = std::move(heights);
-  layers.conductance = std::move(conductance);
+  layers.heights = std::move(c.heights);
+  layers.conductance = std::move(c.conductance);
 }
 
-// ShallowFlowReduction::lift (flow_system.hpp:424-470): each x and y voxel
-// face between fluid voxels carries its column face's flux over the smaller
-// of the two columns' heights; the vertical faces follow by continuity,
-// integrating the horizontal divergence upward with the column's mean
-// removed so floor and roof stay closed. flux: x|y faces of the grid of
-// columns. Returns x|y|z voxel faces.
-std::vector<float> lift_columns(const FlowDevice& f, const std::vector<float>& flux) {
-  const auto& L = f.layers;
-  const uint32_t nx = f.nx, ny = f.ny, nz = L.nz;
-  const size_t nxf = static_cast<size_t>(nx + 1) * ny * nz;
-  const size_t nyf = static_cast<size_t>(nx) * (ny + 1) * nz;
-  const size_t nzf = static_cast<size_t>(nx) * ny * (nz + 1);
-  std::vector<float> v(nxf + nyf + nzf, 0.0f);
-  auto solid = [&](size_t column, uint32_t z) { return L.voxels[column * nz + z] != 0; };
-  auto xface = [&](uint32_t fx, uint32_t y, uint32_t z) {
-    return (static_cast<size_t>(fx) * ny + y) * nz + z;
-  };
-  auto yface = [&](uint32_t x, uint32_t fy, uint32_t z) {
-    return nxf + (static_cast<size_t>(x) * (ny + 1) + fy) * nz + z;
-  };
-  auto zface = [&](uint32_t x, uint32_t y, uint32_t fz) {
-    return nxf + nyf + (static_cast<size_t>(x) * ny + y) * (nz + 1) + fz;
-  };
-  const size_t cxf = static_cast<size_t>(nx + 1) * ny;  // column x faces
-  for (int comp = 0; comp < 2; ++comp) {
-    const uint32_t fdx = comp == 0 ? nx + 1 : nx, fdy = comp == 0 ? ny : ny + 1;
-    const uint32_t dim = comp == 0 ? nx : ny;
-    for (uint32_t x = 0; x < fdx; ++x) {
-      for (uint32_t y = 0; y < fdy; ++y) {
-        const uint32_t along = comp == 0 ? x : y;
-        const bool has_lower = along > 0, has_upper = along < dim;
-        const size_t up_col = static_cast<size_t>(x) * ny + y;
-        const size_t low_col = comp == 0 ? up_col - ny : up_col - 1;
-        const size_t a_col = has_lower ? low_col : up_col;
-        const size_t b_col = has_upper ? up_col : a_col;
-        const float height = std::min(L.heights[a_col], L.heights[b_col]);
-        const float q = comp == 0 ? flux[static_cast<size_t>(x) * ny + y]
-                                  : flux[cxf + static_cast<size_t>(x) * (ny + 1) + y];
-        for (uint32_t z = 0; z < nz; ++z) {
-          if ((has_lower && solid(low_col, z)) || (has_upper && solid(up_col, z))) continue;
-          if (height > 0.0f) v[comp == 0 ? xface(x, y, z) : yface(x, y, z)] = q / height;
+std::vector<uint8_t> obstacle_mask(const msim_grid& g) {
+  const size_t voxels = static_cast<size_t>(g.nx) * g.ny * g.nz;
+  std::vector<uint8_t> mask(voxels, 0);
+  if (g.obstacles != nullptr) std::memcpy(mask.data(), g.obstacles, voxels);
+  return mask;
+}
+
+// The checks reduce_shallow_flow and the flow system's constructor make
+// before a solve (flow_system.hpp:1255-1266, 260-330; cpu_flow.cpp:288-336),
+// and the reduction to a grid of columns.
+void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
+                   std::vector<uint8_t>& solid, std::vector<float>& mob,
+                   FlowDevice::Layers& layers) {
+  validate_lattice(g);
+  if (axis > 2) invalid("unknown flow axis");
+  for (int a = 0; a < 3; ++a) {
+    if (g.boundary[2 * a] == MSIM_PERIODIC || g.boundary[2 * a + 1] == MSIM_PERIODIC) {
+      invalid("native flow solvers do not support periodic boundaries");
+    }
+  }
+  if (g.boundary[2 * axis] != MSIM_FIXED || g.boundary[2 * axis + 1] != MSIM_FIXED) {
+    invalid("native flow-axis boundaries must be fixed to identify inlet and outlet");
+  }
+  std::vector<uint8_t> mask = obstacle_mask(g);
+  check_reachable(g.nx, g.ny, g.nz, mask, axis);
+  if (axis == 2) invalid("shallow flow is depth-integrated along z; flow axis must be x or y");
+  Columns c = reduce_columns(g, mobility, mask);
+  // The grid of columns must still carry flow.
+  check_reachable(g.nx, g.ny, 1, c.solid, axis);
+  take_columns(g, mobility, std::move(mask), std::move(c), solid, mob, layers);
+}
+
+// port_face_geometry (flow_system.hpp:203-219): the axis a face is normal
+// to, whether it is that axis's upper face, and its in-plane axes.
+struct PortGeometry {
+  int component = 0;
+  bool upper = false;
+  int in_plane[2] = {0, 0};
+};
+
+PortGeometry port_geometry(uint32_t face) {
+  if (face / 2 > 2) invalid("unknown flow port face");
+  static constexpr int planes[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+  PortGeometry p;
+  p.component = static_cast<int>(face / 2);
+  p.upper = face % 2 == 1;
+  p.in_plane[0] = planes[p.component][0];
+  p.in_plane[1] = planes[p.component][1];
+  return p;
+}
+
+// for_each_port_face (flow_system.hpp:221-236) on a grid of (nx, ny, nz):
+// body(face index into x|y|z faces, index of the site inside the face), the
+// first in-plane index outer.
+template <typename Body>
+void for_each_port_face(const uint32_t dims[3], const msim_flow_port& port, Body&& body) {
+  const PortGeometry geo = port_geometry(port.face);
+  const size_t nx = dims[0], ny = dims[1], nz = dims[2];
+  const size_t offsets[3] = {0, (nx + 1) * ny * nz, (nx + 1) * ny * nz + nx * (ny + 1) * nz};
+  for (uint32_t first = port.begin[0]; first < port.end[0]; ++first) {
+    for (uint32_t second = port.begin[1]; second < port.end[1]; ++second) {
+      uint32_t face[3] = {0, 0, 0};
+      face[geo.component] = geo.upper ? dims[geo.component] : 0;
+      face[geo.in_plane[0]] = first;
+      face[geo.in_plane[1]] = second;
+      uint32_t site[3] = {face[0], face[1], face[2]};
+      if (geo.upper) --site[geo.component];
+      size_t index;
+      if (geo.component == 0) {
+        index = (face[0] * ny + face[1]) * nz + face[2];
+      } else if (geo.component == 1) {
+        index = (face[0] * (ny + 1) + face[1]) * nz + face[2];
+      } else {
+        index = (face[0] * ny + face[1]) * (nz + 1) + face[2];
+      }
+      body(offsets[geo.component] + index, (site[0] * ny + site[1]) * nz + site[2]);
+    }
+  }
+}
+
+// validate_flow_ports (cpp/cpu/cpu_flow.cpp:193-286) on a grid of `dims`
+// with boundary kinds `kinds` and obstacle mask `solid`.
+void validate_ports(const uint32_t dims[3], const uint8_t kinds[6],
+                    const std::vector<uint8_t>& solid, const msim_flow_port* ports,
+                    uint32_t count) {
+  for (int a = 0; a < 3; ++a) {
+    if (kinds[2 * a] == MSIM_PERIODIC || kinds[2 * a + 1] == MSIM_PERIODIC) {
+      invalid("native flow solvers do not support periodic boundaries");
+    }
+  }
+  if (count == 0) invalid("flow needs at least one port");
+  const size_t nx = dims[0], ny = dims[1], nz = dims[2];
+  const size_t sites = nx * ny * nz;
+  std::vector<uint8_t> claimed((nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1), 0);
+  std::vector<uint8_t> visited(sites, 0);
+  std::vector<size_t> pending;
+  bool holds_pressure = false;
+  for (uint32_t p = 0; p < count; ++p) {
+    const msim_flow_port& port = ports[p];
+    if (port.kind != MSIM_PORT_SPEED && port.kind != MSIM_PORT_PRESSURE) {
+      invalid("unknown flow port kind");
+    }
+    if (!std::isfinite(port.value)) invalid("flow port values must be finite");
+    const PortGeometry geo = port_geometry(port.face);
+    if (kinds[port.face] != MSIM_FIXED) invalid("a flow port must open a fixed boundary");
+    const uint32_t extent[2] = {dims[geo.in_plane[0]], dims[geo.in_plane[1]]};
+    if (port.end[0] <= port.begin[0] || port.end[1] <= port.begin[1] ||
+        port.end[0] > extent[0] || port.end[1] > extent[1]) {
+      invalid("a flow port must span at least one site of its face");
+    }
+    bool open = false;
+    for_each_port_face(dims, port, [&](size_t face, size_t site) {
+      if (claimed[face] != 0) invalid("flow ports must not overlap");
+      claimed[face] = 1;
+      if (solid[site] != 0) return;
+      open = true;
+      if (port.kind == MSIM_PORT_PRESSURE && visited[site] == 0) {
+        visited[site] = 1;
+        pending.push_back(site);
+      }
+    });
+    if (!open) invalid("a flow port is entirely blocked");
+    holds_pressure = holds_pressure || port.kind == MSIM_PORT_PRESSURE;
+  }
+  if (!holds_pressure) invalid("flow needs at least one pressure port");
+  // Something flows: a speed port, or a pressure other than the first
+  // pressure port's.
+  float first = 0.0f;
+  for (uint32_t p = 0; p < count; ++p) {
+    if (ports[p].kind == MSIM_PORT_PRESSURE) {
+      first = ports[p].value;
+      break;
+    }
+  }
+  bool drives = false;
+  for (uint32_t p = 0; p < count; ++p) {
+    drives = drives || ports[p].kind == MSIM_PORT_SPEED || ports[p].value != first;
+  }
+  if (!drives) invalid("flow ports state no speed and hold one pressure, so nothing flows");
+  // A speed port's flow has to leave, or arrive, through a pressure port.
+  const size_t step[3] = {ny * nz, nz, 1};
+  while (!pending.empty()) {
+    const size_t site = pending.back();
+    pending.pop_back();
+    for (int a = 0; a < 3; ++a) {
+      const size_t c = (site / step[a]) % dims[a];
+      if (c > 0) {
+        const size_t nb = site - step[a];
+        if (visited[nb] == 0 && solid[nb] == 0) {
+          visited[nb] = 1;
+          pending.push_back(nb);
+        }
+      }
+      if (c + 1 < dims[a]) {
+        const size_t nb = site + step[a];
+        if (visited[nb] == 0 && solid[nb] == 0) {
+          visited[nb] = 1;
+          pending.push_back(nb);
         }
       }
     }
   }
-  std::vector<double> horizontal;
-  for (uint32_t x = 0; x < nx; ++x) {
-    for (uint32_t y = 0; y < ny; ++y) {
-      const size_t column = static_cast<size_t>(x) * ny + y;
-      horizontal.clear();
-      double sum = 0.0;
-      for (uint32_t z = L.bottom; z < nz; ++z) {
-        if (solid(column, z)) break;
-        const double dxv = (v[xface(x + 1, y, z)] - v[xface(x, y, z)]) / f.dx;
-        const double dyv = (v[yface(x, y + 1, z)] - v[yface(x, y, z)]) / f.dy;
-        horizontal.push_back(dxv + dyv);
-        sum += dxv + dyv;
-      }
-      if (horizontal.empty()) continue;
-      const double mean = sum / static_cast<double>(horizontal.size());
-      double vertical = 0.0;
-      for (size_t k = 0; k + 1 < horizontal.size(); ++k) {
-        vertical -= L.dz * (horizontal[k] - mean);
-        v[zface(x, y, L.bottom + static_cast<uint32_t>(k) + 1)] = static_cast<float>(vertical);
-      }
+  for (uint32_t p = 0; p < count; ++p) {
+    if (ports[p].kind != MSIM_PORT_SPEED) continue;
+    bool reached = false;
+    for_each_port_face(dims, ports[p], [&](size_t, size_t site) {
+      reached = reached || visited[site] != 0;
+    });
+    if (!reached) {
+      throw Error(MSIM_RUNTIME_ERROR, "a speed port carries no flow: no pressure port is reachable");
     }
   }
-  return v;
+}
+
+// ShallowPortFlow's construction (flow_system.hpp:1272-1282): the ports'
+// checks on the grid, the reduction, each port on the grid of columns
+// (reduced_port, 347-372: an x or y face, spanning every fluid voxel of each
+// column it reaches), the ports' checks on the grid of columns, and the faces
+// each port carries flow through.
+PortPlan plan_ports(const msim_grid& g, const float* mobility, const msim_flow_port* ports,
+                    uint32_t count, std::vector<uint8_t>& solid, std::vector<float>& mob,
+                    FlowDevice::Layers& layers) {
+  validate_lattice(g);
+  if (count > 0 && ports == nullptr) invalid("ports must not be null");
+  std::vector<uint8_t> mask = obstacle_mask(g);
+  const uint32_t dims[3] = {g.nx, g.ny, g.nz};
+  validate_ports(dims, g.boundary, mask, ports, count);
+  Columns c = reduce_columns(g, mobility, mask);
+  const uint32_t nz = g.nz;
+  std::vector<msim_flow_port> reduced(ports, ports + count);
+  for (uint32_t p = 0; p < count; ++p) {
+    const PortGeometry geo = port_geometry(ports[p].face);
+    if (geo.component == 2) invalid("a shallow-flow port cannot open a z face; use resolved flow");
+    uint32_t site[3] = {0, 0, 0};
+    if (geo.upper) site[geo.component] = dims[geo.component] - 1;
+    for (uint32_t along = ports[p].begin[0]; along < ports[p].end[0]; ++along) {
+      site[geo.in_plane[0]] = along;
+      for (uint32_t z = c.bottom; z < nz; ++z) {
+        if (mask[(static_cast<size_t>(site[0]) * g.ny + site[1]) * nz + z] != 0) break;
+        if (z < ports[p].begin[1] || z >= ports[p].end[1]) {
+          invalid("a shallow-flow port must span the full depth of its columns; use resolved flow");
+        }
+      }
+    }
+    reduced[p].begin[1] = 0;
+    reduced[p].end[1] = 1;
+  }
+  const uint32_t columns[3] = {g.nx, g.ny, 1};
+  const uint8_t column_kinds[6] = {g.boundary[0], g.boundary[1], g.boundary[2], g.boundary[3],
+                                   MSIM_NO_FLUX, MSIM_NO_FLUX};
+  validate_ports(columns, column_kinds, c.solid, reduced.data(), count);
+  PortPlan plan;
+  plan.ports.assign(ports, ports + count);
+  const size_t slots = 2 * (static_cast<size_t>(g.nx) + g.ny);
+  plan.open.assign(slots, 0);
+  plan.owner.assign(slots, -1);
+  plan.offsets.assign(count + 1, 0);
+  plan.signed_area.assign(count, 0.0);
+  const float spacing[3] = {g.dx, g.dy, g.dz};
+  for (uint32_t p = 0; p < count; ++p) {
+    const PortGeometry geo = port_geometry(ports[p].face);
+    // port_flow_boundary (238-253) on the grid of columns.
+    const size_t base = ports[p].face == 0   ? 0
+                        : ports[p].face == 1 ? g.ny
+                        : ports[p].face == 2 ? 2 * static_cast<size_t>(g.ny)
+                                             : 2 * static_cast<size_t>(g.ny) + g.nx;
+    for (uint32_t along = ports[p].begin[0]; along < ports[p].end[0]; ++along) {
+      plan.open[base + along] = 1;
+      plan.owner[base + along] = static_cast<int32_t>(p);
+    }
+    // open_port_faces (385-393): faces onto fluid voxels of columns with
+    // conductance.
+    for_each_port_face(dims, ports[p], [&](size_t face, size_t site) {
+      if (mask[site] == 0 && c.conductance[site / nz] > 0.0f) plan.faces.push_back(face);
+    });
+    plan.offsets[p + 1] = plan.faces.size();
+    const double area = static_cast<double>(spacing[geo.in_plane[0]]) * spacing[geo.in_plane[1]];
+    plan.signed_area[p] = geo.upper ? -area : area;
+  }
+  take_columns(g, mobility, std::move(mask), std::move(c), solid, mob, layers);
+  return plan;
 }
 
 // scale_velocity (flow_system.hpp:1368-1415) over the lifted voxel faces: the
@@ -600,8 +769,10 @@ int msim_hip_flow_prepare(msim_ctx* ctx, const msim_grid* grid, const float* mob
     // Several layers: the conductance goes up as the mobility with dz = 1.
     f.dz = grid->nz == 1 ? grid->dz : 1.0f;
     f.axis = axis;
+    f.ported = false;
     f.prepared = false;
     msim::flow_assemble(c, solid.data(), mob.data());
+    msim::flow_upload_layers(c);
   });
 }
 
@@ -609,6 +780,7 @@ int msim_hip_flow_solve(msim_ctx* ctx, const msim_flow_params* params, msim_flow
   return msim::guarded(ctx, [&](Context& c) {
     if (params == nullptr || report == nullptr) msim::invalid("parameters must not be null");
     if (!c.flow.prepared) msim::invalid("no flow system has been prepared");
+    if (c.flow.ported) msim::invalid("the prepared flow is driven through ports");
     msim::validate_flow_params(*params);
     if (params->axis != c.flow.axis) msim::invalid("the flow axis differs from the prepared one");
     auto& f = c.flow;
@@ -616,18 +788,20 @@ int msim_hip_flow_solve(msim_ctx* ctx, const msim_flow_params* params, msim_flow
       msim::flow_solve(c, *params, *report);
       return;
     }
-    // Several layers: the device solves the grid of columns and reconstructs
-    // the column flux (unscaled, dz = 1); the host lifts it onto the voxel
-    // faces and scales it to the mean inlet speed.
+    // Several layers: the device solves the grid of columns, reconstructs
+    // the column flux (unscaled, dz = 1) and lifts it onto the voxel faces;
+    // the host scales the lifted field to the mean inlet speed.
     msim_flow_params column = *params;
     column.scale = 0;
     msim::flow_solve(c, column, *report);
-    std::vector<float> flux(f.x_faces() + f.y_faces());
-    msim::check(hipMemcpyAsync(flux.data(), f.faces.data(), flux.size() * sizeof(float),
-                               hipMemcpyDeviceToHost, c.stream),
-                "download column flux");
-    msim::check(hipStreamSynchronize(c.stream), "download column flux");
-    f.layers.faces = msim::lift_columns(f, flux);
+    f.lifted.reserve(f.voxel_faces());
+    msim::flow_lift(c, f.lifted.data());
+    f.layers.faces.resize(f.voxel_faces());
+    msim::check(hipMemcpyAsync(f.layers.faces.data(), f.lifted.data(),
+                               f.layers.faces.size() * sizeof(float), hipMemcpyDeviceToHost,
+                               c.stream),
+                "download lifted faces");
+    msim::check(hipStreamSynchronize(c.stream), "download lifted faces");
     msim::scale_layers(f, *params, *report);
   });
 }
@@ -680,6 +854,52 @@ int msim_hip_solve_depth_averaged_flow(msim_ctx* ctx, const msim_grid* grid,
   status = msim_hip_flow_solve(ctx, params, report);
   if (status != MSIM_OK) return status;
   return msim_hip_flow_download(ctx, nullptr, x_faces, y_faces, z_faces);
+}
+
+int msim_hip_solve_depth_averaged_port_flow(msim_ctx* ctx, const msim_grid* grid,
+                                            const float* mobility, const msim_flow_port* ports,
+                                            uint32_t port_count,
+                                            const msim_port_flow_params* params, float* x_faces,
+                                            float* y_faces, float* z_faces,
+                                            msim_port_flow_report* report,
+                                            msim_port_state* states) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || params == nullptr || report == nullptr) {
+      msim::invalid("grid, parameters and report must not be null");
+    }
+    // DepthAveragedPortFlowParameters::validate (cpu_flow.cpp:187-192).
+    if (!std::isfinite(params->relative_tolerance) ||
+        params->relative_tolerance < std::numeric_limits<float>::epsilon()) {
+      msim::invalid(
+          "depth-averaged relative tolerance must be finite and at least float machine epsilon");
+    }
+    if (params->max_iterations == 0) msim::invalid("depth-averaged iteration limit must be positive");
+    if (params->solver > MSIM_SOLVER_MULTIGRID) msim::invalid("unknown pressure solver");
+    std::vector<uint8_t> solid;
+    std::vector<float> mob;
+    auto& f = c.flow;
+    f.prepared = false;
+    const msim::PortPlan plan =
+        msim::plan_ports(*grid, mobility, ports, port_count, solid, mob, f.layers);
+    f.nx = grid->nx;
+    f.ny = grid->ny;
+    f.dx = grid->dx;
+    f.dy = grid->dy;
+    f.dz = grid->nz == 1 ? grid->dz : 1.0f;
+    f.axis = 0;
+    f.ported = true;
+    const size_t slots = plan.open.size();
+    f.edge_open.reserve(slots);
+    f.edge_pressure.reserve(slots);
+    msim::check(hipMemcpyAsync(f.edge_open.data(), plan.open.data(), slots,
+                               hipMemcpyHostToDevice, c.stream),
+                "upload open port faces");
+    msim::check(hipMemsetAsync(f.edge_pressure.data(), 0, slots * sizeof(float), c.stream),
+                "clear port pressures");
+    msim::flow_assemble(c, solid.data(), mob.data());
+    msim::flow_upload_layers(c);
+    msim::port_flow_solve(c, plan, *params, x_faces, y_faces, z_faces, *report, states);
+  });
 }
 
 int msim_hip_flow_apply(msim_ctx* ctx, const double* x, double* y) {

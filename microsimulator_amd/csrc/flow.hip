@@ -37,13 +37,29 @@ __global__ void k_conductance(const uint8_t* __restrict__ solid, const float* __
   cond[i] = solid[i] != 0 ? 0.0f : dz * mobility[i];
 }
 
+// Whether the exterior face `side` (0 x lower, 1 x upper, 2 y lower, 3 y
+// upper) at in-plane index `along` is open, and the pressure held half a
+// spacing outside it.
+__device__ __forceinline__ bool open_edge(const FlowEdges& e, uint32_t side, uint32_t along,
+                                          uint32_t nx, uint32_t ny, float& pressure) {
+  if (e.open == nullptr) {
+    pressure = (side & 1u) != 0 ? 0.0f : 1.0f;
+    return (side >> 1) == e.axis;
+  }
+  const uint32_t slot =
+      along + (side == 0 ? 0u : side == 1 ? ny : side == 2 ? 2u * ny : 2u * ny + nx);
+  pressure = e.pressure[slot];
+  return e.open[slot] != 0;
+}
+
 // Assembly (flow_system.hpp:549-599): diagonal, right-hand side and boundary
 // conductance in the reference's float order (x lower, x upper, y lower,
-// y upper), and the lower-face conductances the operator uses.
+// y upper), and the lower-face conductances the operator uses. Each open
+// exterior face adds 2m/h^2 toward the pressure held outside it.
 __global__ void k_assemble(const float* __restrict__ cond, float* __restrict__ diag,
                            float* __restrict__ rhs, float* __restrict__ bc,
                            float* __restrict__ cxl, float* __restrict__ cyl, uint32_t nx,
-                           uint32_t ny, float dx, float dy, uint32_t axis) {
+                           uint32_t ny, float dx, float dy, FlowEdges edges) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= static_cast<size_t>(nx) * ny) return;
   const uint32_t x = static_cast<uint32_t>(i / ny);
@@ -55,41 +71,42 @@ __global__ void k_assemble(const float* __restrict__ cond, float* __restrict__ d
   if (m != 0.0f) {
     const float isx = 1.0f / (dx * dx);
     const float isy = 1.0f / (dy * dy);
+    float held = 0.0f;
     // x lower
     if (x > 0) {
       d += harmonic(m, cond[i - ny]) * isx;
-    } else if (axis == 0) {
+    } else if (open_edge(edges, 0, y, nx, ny, held)) {
       const float c = 2.0f * m * isx;
       d += c;
       b += c;
-      r += c * 1.0f;
+      r += c * held;
     }
     // x upper
     if (x + 1 < nx) {
       d += harmonic(m, cond[i + ny]) * isx;
-    } else if (axis == 0) {
+    } else if (open_edge(edges, 1, y, nx, ny, held)) {
       const float c = 2.0f * m * isx;
       d += c;
       b += c;
-      r += c * 0.0f;
+      r += c * held;
     }
     // y lower
     if (y > 0) {
       d += harmonic(m, cond[i - 1]) * isy;
-    } else if (axis == 1) {
+    } else if (open_edge(edges, 2, x, nx, ny, held)) {
       const float c = 2.0f * m * isy;
       d += c;
       b += c;
-      r += c * 1.0f;
+      r += c * held;
     }
     // y upper
     if (y + 1 < ny) {
       d += harmonic(m, cond[i + 1]) * isy;
-    } else if (axis == 1) {
+    } else if (open_edge(edges, 3, x, nx, ny, held)) {
       const float c = 2.0f * m * isy;
       d += c;
       b += c;
-      r += c * 0.0f;
+      r += c * held;
     }
   }
   diag[i] = d;
@@ -512,7 +529,7 @@ __global__ __launch_bounds__(TY) void k_residual(
 __global__ __launch_bounds__(kStream) void k_velocity(
     const double* __restrict__ p, const float* __restrict__ cond, const uint8_t* __restrict__ solid,
     float* __restrict__ faces, uint32_t nx, uint32_t ny, float dx, float dy, float dz,
-    uint32_t axis, CgState* st, double* partials) {
+    FlowEdges edges, CgState* st, double* partials) {
   __shared__ double s_red[kStream / kWave];
   __shared__ bool s_last;
   const size_t nxf = static_cast<size_t>(nx + 1) * ny;
@@ -521,12 +538,13 @@ __global__ __launch_bounds__(kStream) void k_velocity(
   const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
   for (size_t f = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; f < total;
        f += stride) {
-    uint32_t comp, along, dim;
+    uint32_t comp, along, dim, across;
     size_t lower, upper;
     if (f < nxf) {
       comp = 0;
       const uint32_t fx = static_cast<uint32_t>(f / ny), y = static_cast<uint32_t>(f % ny);
       along = fx;
+      across = y;
       dim = nx;
       upper = static_cast<size_t>(fx) * ny + y;
       lower = upper - ny;  // meaningful only when fx > 0
@@ -535,6 +553,7 @@ __global__ __launch_bounds__(kStream) void k_velocity(
       const size_t g = f - nxf;
       const uint32_t x = static_cast<uint32_t>(g / (ny + 1)), fy = static_cast<uint32_t>(g % (ny + 1));
       along = fy;
+      across = x;
       dim = ny;
       upper = static_cast<size_t>(x) * ny + fy;
       lower = upper - 1;
@@ -549,10 +568,10 @@ __global__ __launch_bounds__(kStream) void k_velocity(
     } else {
       const size_t site = has_upper ? upper : lower;
       const float m = cond[site];
-      float fc = 0.0f, fp = 0.0f;
-      if (comp == axis && m != 0.0f) {
+      float fc = 0.0f, fp = 0.0f, held = 0.0f;
+      if (m != 0.0f && open_edge(edges, 2 * comp + (has_upper ? 0u : 1u), across, nx, ny, held)) {
         fc = (has_upper ? 2.0f : -2.0f) * m / sp;
-        fp = has_upper ? 1.0f : 0.0f;
+        fp = held;
       }
       value = static_cast<double>(fc) * (static_cast<double>(fp) - p[site]);
     }
@@ -562,7 +581,7 @@ __global__ __launch_bounds__(kStream) void k_velocity(
     faces[f] = v;
     if (!isfinite(v)) bad = 1.0;
     peak = fmax(peak, static_cast<double>(fabsf(v)));
-    if (comp == axis && along == 0 && solid[upper] == 0 && cond[upper] > 0.0f) {
+    if (comp == edges.axis && along == 0 && solid[upper] == 0 && cond[upper] > 0.0f) {
       inlet += static_cast<double>(v);
       count += 1.0;
     }
@@ -855,13 +874,30 @@ uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t 
 
 void launch_assembly(hipStream_t stream, const uint8_t* solid, const float* mobility, float* cond,
                      float* diag, float* rhs, float* bc, float* cxl, float* cyl, uint32_t nx,
-                     uint32_t ny, float dx, float dy, float dz, uint32_t axis) {
+                     uint32_t ny, float dx, float dy, float dz, FlowEdges edges) {
   const size_t n = static_cast<size_t>(nx) * ny;
   const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
   k_conductance<<<blocks, 256, 0, stream>>>(solid, mobility, cond, n, dz);
   check(hipGetLastError(), "launch k_conductance");
-  k_assemble<<<blocks, 256, 0, stream>>>(cond, diag, rhs, bc, cxl, cyl, nx, ny, dx, dy, axis);
+  k_assemble<<<blocks, 256, 0, stream>>>(cond, diag, rhs, bc, cxl, cyl, nx, ny, dx, dy, edges);
   check(hipGetLastError(), "launch k_assemble");
+}
+
+void flow_hold_pressures(Context& ctx, const float* pressure_host) {
+  auto& f = ctx.flow;
+  if (!f.ported) throw Error(MSIM_INVALID_ARGUMENT, "the prepared flow has no ports");
+  const size_t slots = 2 * (static_cast<size_t>(f.nx) + f.ny);
+  check(hipMemcpyAsync(f.edge_pressure.data(), pressure_host, slots * sizeof(float),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload port pressures");
+  // Only the right-hand side changes: the operator and the multilevel
+  // hierarchy built on it stay valid.
+  const size_t n = f.sites();
+  k_assemble<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, ctx.stream>>>(
+      f.cond.data(), f.diag.data(), f.rhs.data(), f.bc.data(), f.cxl.data(), f.cyl.data(), f.nx,
+      f.ny, f.dx, f.dy, f.edges());
+  check(hipGetLastError(), "launch k_assemble");
+  f.solved = false;
 }
 
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host) {
@@ -892,7 +928,7 @@ void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobilit
         "upload mobility");
   launch_assembly(ctx.stream, f.solid.data(), f.mobility.data(), f.cond.data(), f.diag.data(),
                   f.rhs.data(), f.bc.data(), f.cxl.data(), f.cyl.data(), f.nx, f.ny, f.dx, f.dy,
-                  f.dz, f.axis);
+                  f.dz, f.edges());
   check(hipStreamSynchronize(ctx.stream), "assemble");
   f.prepared = true;
   f.solved = false;
@@ -1033,20 +1069,27 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
   const size_t nfaces = f.x_faces() + f.y_faces();
   k_velocity<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(
       f.x.data(), f.cond.data(), f.solid.data(), f.faces.data(), f.nx, f.ny, f.dx, f.dy, f.dz,
-      f.axis, f.state.data(), f.partials.data());
+      f.edges(), f.state.data(), f.partials.data());
   check(hipGetLastError(), "launch k_velocity");
   read_state(ctx);
-  if (s.nonfinite != 0) throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
-  if (s.inlet_count == 0.0) throw Error(MSIM_RUNTIME_ERROR, "flow solve has no open inlet faces");
-  const float solved_mean = static_cast<float>(s.inlet_sum / s.inlet_count);
+  // A solve of a port-driven flow returns the unscaled field of the held
+  // pressures, which the caller combines with the other solves' fields.
+  float solved_mean = 0.0f;
   const float peak = static_cast<float>(s.peak);
-  if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
-    throw Error(MSIM_RUNTIME_ERROR,
-                "the device carries no through-flow: the outlet is unreachable");
+  if (!f.ported) {
+    if (s.nonfinite != 0) {
+      throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+    }
+    if (s.inlet_count == 0.0) throw Error(MSIM_RUNTIME_ERROR, "flow solve has no open inlet faces");
+    solved_mean = static_cast<float>(s.inlet_sum / s.inlet_count);
+    if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
+      throw Error(MSIM_RUNTIME_ERROR,
+                  "the device carries no through-flow: the outlet is unreachable");
+    }
   }
   float factor = 1.0f;
   float max_speed = peak;
-  if (params.scale != 0) {
+  if (params.scale != 0 && !f.ported) {
     const float driven_mean = static_cast<float>(0.0 / s.inlet_count);
     factor = (params.mean_inlet_speed - driven_mean) / solved_mean;
     k_scale<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(f.faces.data(), nfaces, factor,

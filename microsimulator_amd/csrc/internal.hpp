@@ -134,12 +134,29 @@ struct MgDevice {
   bool ready = false;
 };
 
+// The boundary of a depth-averaged pressure system on its grid of columns.
+// With `open` null: both ends of `axis` open, the lower held at pressure one
+// and the upper at zero (axis_flow_boundary, flow_system.hpp:185-201).
+// Otherwise, per exterior in-plane face, whether it is open and the pressure
+// held half a spacing outside it (port_flow_boundary, 238-253), in device
+// arrays of slots [x_lower ny | x_upper ny | y_lower nx | y_upper nx].
+struct FlowEdges {
+  const uint8_t* open = nullptr;
+  const float* pressure = nullptr;
+  uint32_t axis = 0;
+};
+
 // Everything the prepared depth-averaged system keeps on the device.
 struct FlowDevice {
   uint32_t nx = 0, ny = 0, axis = 0;
   float dx = 1, dy = 1, dz = 1;
   bool prepared = false;
   bool solved = false;
+  // Port-driven flow: the open exterior faces and the pressures held outside
+  // them (FlowEdges slots) instead of the axis boundary.
+  bool ported = false;
+  DeviceBuffer<uint8_t> edge_open;
+  DeviceBuffer<float> edge_pressure;
   DeviceBuffer<uint8_t> solid;     // original obstacle mask
   DeviceBuffer<float> mobility;    // uploaded per-site mobility
   DeviceBuffer<float> cond;        // column conductance (reduction)
@@ -164,9 +181,31 @@ struct FlowDevice {
     std::vector<float> conductance;  // per column
     std::vector<float> faces;        // lifted x|y|z voxel faces after a solve
   } layers;
+  // The device copies of the original mask and the column heights, for the
+  // lift of a column flux onto the voxel faces.
+  DeviceBuffer<uint8_t> voxels;
+  DeviceBuffer<float> heights;
+  DeviceBuffer<float> lifted;       // x|y|z voxel faces of a lifted solve
+  // Port-driven flow: one lifted field per solve, the original grid's port
+  // faces, their values in every field, the combination's weights and the
+  // combined field's largest |velocity| (as float bits).
+  DeviceBuffer<float> port_fields;
+  DeviceBuffer<uint64_t> port_faces;
+  DeviceBuffer<float> port_values;
+  DeviceBuffer<double> port_weights;
+  DeviceBuffer<uint32_t> port_peak;
   [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(nx) * ny; }
   [[nodiscard]] std::size_t x_faces() const { return static_cast<std::size_t>(nx + 1) * ny; }
   [[nodiscard]] std::size_t y_faces() const { return static_cast<std::size_t>(nx) * (ny + 1); }
+  // x|y|z faces of the original grid (nz layers).
+  [[nodiscard]] std::size_t voxel_faces() const {
+    const std::size_t z = layers.nz;
+    return (x_faces() + y_faces()) * z + sites() * (z + 1);
+  }
+  [[nodiscard]] FlowEdges edges() const {
+    if (!ported) return FlowEdges{nullptr, nullptr, axis};
+    return FlowEdges{edge_open.data(), edge_pressure.data(), axis};
+  }
 };
 
 // Transport constants passed to the kernels by value.
@@ -247,11 +286,14 @@ struct KernelTimes {
 
 // flow.hip
 void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobility_host);
-// Conductance and assembly of one grid already on the device (Dirichlet 1 at
-// the lower end of `axis`, 0 at the upper end).
+// Conductance and assembly of one grid already on the device, with the
+// boundary `edges`.
 void launch_assembly(hipStream_t stream, const uint8_t* solid, const float* mobility, float* cond,
                      float* diag, float* rhs, float* bc, float* cxl, float* cyl, uint32_t nx,
-                     uint32_t ny, float dx, float dy, float dz, uint32_t axis);
+                     uint32_t ny, float dx, float dy, float dz, FlowEdges edges);
+// A ported system: holds each open face at the pressure of its slot in
+// `pressure_host` (FlowEdges layout) and reassembles the right-hand side.
+void flow_hold_pressures(Context& ctx, const float* pressure_host);
 uint32_t stream_blocks(size_t n);
 uint32_t krylov_blocks(size_t n);
 void launch_residual(Context& ctx, const double* x, double* out, int mode);
@@ -298,6 +340,32 @@ uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t 
                       bool* converged, double* relative, KernelTimes* times);
 void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report);
 void flow_apply(Context& ctx, const double* x, double* y);
+
+// ports.hip: the lift onto voxel faces and port-driven flow.
+// The solved field on the original grid's x|y|z faces (device `out`, of
+// flow.voxel_faces() floats): one layer copies the face velocities; several
+// layers lift the column flux (ShallowFlowReduction::lift,
+// flow_system.hpp:424-470).
+void flow_lift(Context& ctx, float* out);
+// Uploads the voxel mask and the column heights a lift of several layers reads.
+void flow_upload_layers(Context& ctx);
+// A checked port-driven flow (ShallowPortFlow, flow_system.hpp:1268-1311):
+// the ports; the exterior slots of the grid of columns (FlowEdges layout)
+// that a port opens and the port each belongs to; and per port the original
+// grid's faces it carries flow through (x|y|z face indices in
+// for_each_port_face order, 221-236) with the signed area of one face
+// (make_port_faces, 1124-1130).
+struct PortPlan {
+  std::vector<msim_flow_port> ports;
+  std::vector<uint8_t> open;
+  std::vector<int32_t> owner;
+  std::vector<uint64_t> faces;
+  std::vector<std::size_t> offsets;  // ports.size() + 1 entries
+  std::vector<double> signed_area;
+};
+void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_params& params,
+                     float* x_faces, float* y_faces, float* z_faces, msim_port_flow_report& report,
+                     msim_port_state* states);
 
 // batch.hip: the batched sweep (msim_hip_batch_*).
 void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,

@@ -24,6 +24,7 @@ REF_SO = os.path.join(HERE, "_ref", "libmsim_ref.so")
 
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
 UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind
+SPEED, PRESSURE = 0, 1  # FlowPortKind
 
 STATUS = {
     0: "ok", 1: "invalid_argument", 2: "runtime_error", 3: "not_converged", 4: "overflow",
@@ -62,6 +63,32 @@ class _FlowReport(C.Structure):
 class _BoundaryPatch(C.Structure):
     _fields_ = [("face", C.c_uint32), ("begin", C.c_uint32 * 2), ("end", C.c_uint32 * 2),
                 ("values", C.c_void_p)]
+
+
+class _FlowPort(C.Structure):
+    _fields_ = [("face", C.c_uint32), ("begin", C.c_uint32 * 2), ("end", C.c_uint32 * 2),
+                ("kind", C.c_uint32), ("value", C.c_float)]
+
+
+class _PortState(C.Structure):
+    _fields_ = [("pressure", C.c_float), ("flux", C.c_float), ("mean_speed", C.c_float)]
+
+
+class _PortReport(C.Structure):
+    _fields_ = [("solves", C.c_uint32), ("iterations", C.c_uint32),
+                ("relative_residual", C.c_float), ("max_speed", C.c_float)]
+
+
+def _ports(ports):
+    """(face, (begin0, begin1), (end0, end1), kind, value) tuples as a C array."""
+    arr = (_FlowPort * max(1, len(ports)))()
+    for i, (face, begin, end, kind, value) in enumerate(ports):
+        arr[i].face = face
+        arr[i].begin[0], arr[i].begin[1] = begin
+        arr[i].end[0], arr[i].end[1] = end
+        arr[i].kind = kind
+        arr[i].value = value
+    return arr
 
 
 class _SignalGrid(C.Structure):
@@ -138,6 +165,8 @@ class Oracle:
         L.mo_flow_assemble.argtypes = [P, P, U, P, P, P, P, P, P, C.c_char_p, Z]
         L.mo_flow_apply.argtypes = [P, P, U, P, P, C.c_char_p, Z]
         L.mo_flow_solve.argtypes = [P, P, U, F, F, U, C.c_int, P, P, P, C.c_char_p, Z]
+        L.mo_port_flow_solve.argtypes = [P, P, P, U, F, U, P, P, P, C.c_char_p, Z]
+        L.mo_port_flow_solve.restype = C.c_int
         L.mo_adr_validate_step.argtypes = [P, F, C.c_char_p, Z]
         L.mo_adr_step_f32.argtypes = [P, F, P, C.c_char_p, Z]
         L.mo_adr_step_f64.argtypes = [P, F, P, P, C.c_char_p, Z]
@@ -198,6 +227,28 @@ class Oracle:
         report = FlowReport(rep.iterations, rep.relative_residual, rep.true_relative_residual,
                             rep.mean_inlet_speed, rep.max_speed, rep.solved_mean, rep.factor)
         return faces, pressure, report
+
+    def port_flow(self, grid: Grid, mobility, ports, tolerance: float = 1e-6,
+                  max_iterations: int = 50000):
+        """Port-driven shallow flow: faces (x|y|z), port states (pressure, flux,
+        mean speed per port) and the report."""
+        keep: list = []
+        g = grid._c(keep)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        states = (_PortState * max(1, len(ports)))()
+        rep = _PortReport()
+        mob = _f32(mobility)
+        arr = _ports(ports)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_port_flow_solve(
+            C.byref(g), _ptr(mob), C.cast(arr, C.c_void_p), C.c_uint32(len(ports)),
+            C.c_float(tolerance), C.c_uint32(max_iterations), faces.ctypes.data,
+            C.cast(states, C.c_void_p), C.byref(rep), err, C.c_size_t(512))
+        self._check(st, err)
+        out = np.array([[s.pressure, s.flux, s.mean_speed] for s in states[:len(ports)]],
+                       np.float32).reshape(len(ports), 3)
+        return faces, out, dict(solves=rep.solves, iterations=rep.iterations,
+                                relative_residual=rep.relative_residual, max_speed=rep.max_speed)
 
     def _signal_grid(self, grid: Grid, keep: list, diffusion, boundary_values=None,
                      faces=None, advection=None, source=None, loss=None, phi=None,
@@ -299,6 +350,8 @@ class Reference:
         L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, U,
                                               U, P, P, F, U, P, P, C.c_char_p, Z]
         L.ref_set_threads.argtypes = [U]
+        L.ref_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, P, P, P, P, P, P,
+                                    C.c_char_p, Z]
 
     @staticmethod
     def available(path: str = REF_SO) -> bool:
@@ -333,6 +386,34 @@ class Reference:
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
         return faces, dict(iterations=it.value, relative_residual=rr.value, max_speed=ms.value)
+
+    def port_flow(self, grid: Grid, mobility, ports, tolerance=1e-6, max_iterations=50000):
+        """solve_depth_averaged_port_flow_cpu: faces, port states, report."""
+        kinds = self._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        mob = _f32(mobility)
+        geometry = np.array([[f, b[0], b[1], e[0], e[1], k] for f, b, e, k, _ in ports],
+                            np.uint32).reshape(-1)
+        values = np.array([v for *_, v in ports], np.float32)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        states = np.zeros(3 * max(1, len(ports)), np.float32)
+        counts = np.zeros(2, np.uint32)
+        rr = C.c_float()
+        ms = C.c_float()
+        seconds = C.c_double()
+        err = C.create_string_buffer(512)
+        st = self.lib.ref_port_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(mob),
+            C.c_uint32(len(ports)), _ptr(geometry), _ptr(values), C.c_float(tolerance),
+            C.c_uint32(max_iterations), faces.ctypes.data, states.ctypes.data,
+            counts.ctypes.data, C.byref(rr), C.byref(ms), C.byref(seconds), err,
+            C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        return faces, states[:3 * len(ports)].reshape(len(ports), 3), dict(
+            solves=int(counts[0]), iterations=int(counts[1]), relative_residual=rr.value,
+            max_speed=ms.value, seconds=seconds.value)
 
     def flow_system(self, grid: Grid, mobility, axis=0):
         return _RefFlow(self, grid, mobility, axis)

@@ -202,20 +202,20 @@ static void system_free(system_t* s) {
   memset(s, 0, sizeof(*s));
 }
 
-/* ShallowFlowReduction (cpp/core/flow_system.hpp:257-330), reduce_shallow_flow's
- * checks (1255-1266), axis_flow_boundary (186-201) and the
- * DepthAveragedFlowSystem constructor (522-636) on the grid of columns. */
-static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, system_t* s,
-                        char* err, size_t errlen) {
+/* ShallowFlowReduction (cpp/core/flow_system.hpp:257-330): the grid of
+ * columns. A column's fluid voxels must be contiguous, start on a floor
+ * common to every column and share one mobility; its height is spacing.z per
+ * fluid voxel and its conductance height * mobility, or without a mobility
+ * array height * (height / max height)^2. A column without conductance is
+ * solid in the grid of columns. Allocates everything the assembly fills. */
+static int reduce_columns(const mo_grid* g, const float* mobility, system_t* s, char* err,
+                          size_t errlen) {
   memset(s, 0, sizeof(*s));
-  int status = validate_flow_grid(g, g->obstacles, axis, err, errlen);
-  if (status != MO_OK) return status;
-  if (axis == 2)
-    return fail(err, errlen, MO_INVALID_ARGUMENT,
-                "shallow flow is depth-integrated along z; flow axis must be x or y");
   const size_t n = (size_t)g->nx * g->ny;
   s->grid2 = *g;
   s->grid2.nz = 1;
+  s->grid2.boundary[4] = MO_NO_FLUX;
+  s->grid2.boundary[5] = MO_NO_FLUX;
   const size_t faces = total_faces(&s->grid2);
   s->n = n;
   s->faces = faces;
@@ -233,11 +233,6 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
     return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
   }
   s->grid2.obstacles = s->reduced_solid;
-  /* Reduction: flow_system.hpp:276-327. A column's fluid voxels must be
-   * contiguous, start on a floor common to every column and share one
-   * mobility; its height is spacing.z per fluid voxel and its conductance
-   * height * mobility, or without a mobility array height *
-   * (height / max height)^2. */
   const uint32_t nz = g->nz;
   uint32_t common_bottom = nz;
   float max_height = 0.0f;
@@ -291,16 +286,36 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
     }
   for (size_t i = 0; i < n; ++i)
     if (s->mobility[i] == 0.0f) s->reduced_solid[i] = 1;
-  g = &s->grid2;
-  status = validate_flow_grid(g, s->reduced_solid, axis, err, errlen);
-  if (status != MO_OK) {
-    system_free(s);
-    return status;
-  }
-  /* Assembly: flow_system.hpp:549-599. Solid sites carry no mobility; each
-   * fluid site adds harm(m, m_nb)/h^2 per neighbour and, through an open
-   * exterior face of the flow axis, 2m/h^2 toward the pressure held half a
-   * spacing outside (1 at the lower face, 0 at the upper). */
+  return MO_OK;
+}
+
+/* The DepthAveragedFlowSystem constructor (flow_system.hpp:522-636) on the
+ * grid of columns with the FlowBoundary (177-183) `open` / `pressure` over
+ * its faces: solid sites carry no mobility; each fluid site adds
+ * harm(m, m_nb)/h^2 per neighbour and, through each open exterior face,
+ * 2m/h^2 toward the pressure held half a spacing outside. Then the active
+ * compaction (601-635): sites with a nonzero diagonal in ascending order, six
+ * neighbour slots, lower then upper along x, y, z. May be called again on the
+ * same reduction with another boundary. */
+static int assemble_system(system_t* s, const uint8_t* open_faces, const float* pressures,
+                           char* err, size_t errlen) {
+  const mo_grid* g = &s->grid2;
+  const size_t n = s->n;
+  memset(s->diagonal, 0, n * sizeof(float));
+  memset(s->rhs, 0, n * sizeof(float));
+  memset(s->bc, 0, n * sizeof(float));
+  memset(s->face_cond, 0, s->faces * sizeof(float));
+  memset(s->face_pressure, 0, s->faces * sizeof(float));
+  free(s->sites);
+  free(s->neighbors);
+  free(s->conductance);
+  free(s->a_diag);
+  free(s->a_rhs);
+  free(s->a_bc);
+  s->sites = NULL;
+  s->neighbors = NULL;
+  s->conductance = NULL;
+  s->a_diag = s->a_rhs = s->a_bc = NULL;
   const uint32_t dims[3] = {g->nx, g->ny, 1};
   const float spacing[3] = {g->dx, g->dy, g->dz};
   int open = 0;
@@ -328,10 +343,8 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
         uint32_t fc[3] = {c[0], c[1], c[2]};
         if (off > 0) ++fc[comp];
         const size_t face = face_index(g, comp, fc[0], fc[1], fc[2]);
-        /* axis_flow_boundary: only the two faces of the flow axis open. */
-        const int is_open = (uint32_t)comp == axis && (fc[comp] == 0 || fc[comp] == dims[comp]);
-        if (!is_open) continue;
-        const float pressure = fc[comp] == 0 ? 1.0f : 0.0f;
+        if (!open_faces[face]) continue;
+        const float pressure = pressures[face];
         const float conductance = 2.0f * value * inverse_square;
         s->diagonal[site] += conductance;
         s->bc[site] += conductance;
@@ -342,14 +355,9 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
       }
     }
   }
-  if (!open) {
-    system_free(s);
-    return fail(err, errlen, MO_INVALID_ARGUMENT, "the flow boundary opens onto no fluid");
-  }
-  /* Active compaction, flow_system.hpp:601-635: sites with a nonzero
-   * diagonal in ascending order; six neighbour slots, lower then upper along
-   * x, y, z. */
+  if (!open) return fail(err, errlen, MO_INVALID_ARGUMENT, "the flow boundary opens onto no fluid");
   uint32_t* place = malloc(n * sizeof(uint32_t));
+  if (place == NULL) return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
   size_t count = 0;
   for (size_t site = 0; site < n; ++site) {
     place[site] = NO_NEIGHBOR;
@@ -362,6 +370,10 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
   s->a_diag = malloc((count + 1) * sizeof(float));
   s->a_rhs = malloc((count + 1) * sizeof(float));
   s->a_bc = malloc((count + 1) * sizeof(float));
+  if (!s->sites || !s->neighbors || !s->conductance || !s->a_diag || !s->a_rhs || !s->a_bc) {
+    free(place);
+    return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+  }
   for (size_t site = 0; site < n; ++site) {
     if (place[site] == NO_NEIGHBOR) continue;
     const size_t index = place[site];
@@ -389,6 +401,51 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
   }
   free(place);
   return MO_OK;
+}
+
+/* reduce_shallow_flow (flow_system.hpp:1255-1266) and the system of one
+ * axis: validate_flow_grid on the grid, the reduction, validate_flow_grid on
+ * the grid of columns, and axis_flow_boundary (186-201), which opens both
+ * ends of the axis, the lower at pressure one and the upper at zero. */
+static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, system_t* s,
+                        char* err, size_t errlen) {
+  memset(s, 0, sizeof(*s));
+  int status = validate_flow_grid(g, g->obstacles, axis, err, errlen);
+  if (status != MO_OK) return status;
+  if (axis == 2)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "shallow flow is depth-integrated along z; flow axis must be x or y");
+  status = reduce_columns(g, mobility, s, err, errlen);
+  if (status != MO_OK) return status;
+  status = validate_flow_grid(&s->grid2, s->reduced_solid, axis, err, errlen);
+  if (status != MO_OK) {
+    system_free(s);
+    return status;
+  }
+  uint8_t* open = calloc(s->faces, 1);
+  float* pressure = calloc(s->faces, sizeof(float));
+  if (open == NULL || pressure == NULL) {
+    free(open);
+    free(pressure);
+    system_free(s);
+    return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+  }
+  const mo_grid* g2 = &s->grid2;
+  const uint32_t dims[2] = {g2->nx, g2->ny};
+  const uint32_t across = axis == 0 ? g2->ny : g2->nx;
+  for (uint32_t t = 0; t < across; ++t) {
+    const size_t lower = axis == 0 ? face_index(g2, 0, 0, t, 0) : face_index(g2, 1, t, 0, 0);
+    const size_t upper =
+        axis == 0 ? face_index(g2, 0, dims[0], t, 0) : face_index(g2, 1, t, dims[1], 0);
+    open[lower] = 1;
+    pressure[lower] = 1.0f;
+    open[upper] = 1;
+  }
+  status = assemble_system(s, open, pressure, err, errlen);
+  free(open);
+  free(pressure);
+  if (status != MO_OK) system_free(s);
+  return status;
 }
 
 int mo_flow_assemble(const mo_grid* grid, const float* mobility, uint32_t axis,
@@ -429,8 +486,8 @@ int mo_flow_apply(const mo_grid* grid, const float* mobility, uint32_t axis, con
   system_t s;
   const int status = build_system(grid, mobility, axis, &s, err, errlen);
   if (status != MO_OK) return status;
-  double* in = malloc((s.count + 1) * sizeof(double));
-  double* out = malloc((s.count + 1) * sizeof(double));
+  double* in = calloc(s.count + 1, sizeof(double));
+  double* out = calloc(s.count + 1, sizeof(double));
   for (size_t i = 0; i < s.count; ++i) in[i] = x[s.sites[i]];
   apply_active(&s, grid, in, out);
   for (size_t i = 0; i < s.n; ++i) y[i] = 0.0;
@@ -537,51 +594,53 @@ done:
   return status;
 }
 
-int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
-                  float mean_inlet_speed, float tolerance, uint32_t max_iterations, int scale,
-                  float* faces, double* pressure, mo_flow_report* report, char* err,
-                  size_t errlen) {
-  /* DepthAveragedFlowParameters::validate, cpu_flow.cpp:134-150; the
-   * tolerance floor (float epsilon) is not applied here: the configurations
-   * solve to 1e-8 and 1e-10. */
-  if (scale && (!isfinite(mean_inlet_speed) || mean_inlet_speed == 0.0f))
-    return fail(err, errlen, MO_INVALID_ARGUMENT,
-                "depth-averaged mean inlet speed must be finite and nonzero without a body force");
-  if (!isfinite(tolerance) || tolerance <= 0.0f)
-    return fail(err, errlen, MO_INVALID_ARGUMENT, "depth-averaged relative tolerance must be positive");
-  if (max_iterations == 0)
-    return fail(err, errlen, MO_INVALID_ARGUMENT, "depth-averaged iteration limit must be positive");
-  system_t s;
-  int status = build_system(g, mobility, axis, &s, err, errlen);
-  if (status != MO_OK) return status;
-  double* solution = calloc(s.count + 1, sizeof(double));
-  uint32_t iterations = 0;
-  double relative = 0.0;
-  status = pcg(&s, g, tolerance, max_iterations, solution, &iterations, &relative, err, errlen);
+/* One solve of an assembled system and the field it gives on the original
+ * grid's faces: Jacobi-PCG from zero (cpu_flow.cpp:107-114), scatter
+ * (flow_system.hpp:699-708), velocity over the faces of the grid of columns
+ * (711-736: the depth-integrated flux of each column face) and lift (424-470:
+ * each x and y voxel face between fluid voxels carries its column face's flux
+ * over the smaller of the two columns' heights; then the vertical faces by
+ * continuity, integrating the horizontal divergence upward with the column's
+ * mean removed so floor and roof stay closed). velocity: total_faces(g)
+ * floats; full: the pressure at every column (may be NULL); rr/bb: the FP64
+ * true residual and rhs norms (may be NULL). */
+static int solve_and_lift(const system_t* s, const mo_grid* g, float tolerance,
+                          uint32_t max_iterations, float* velocity, double* full_out,
+                          uint32_t* iterations, double* relative, double* rr_out, double* bb_out,
+                          char* err, size_t errlen) {
+  double* solution = calloc(s->count + 1, sizeof(double));
+  double* full = calloc(s->n, sizeof(double));
+  float* flux = malloc(s->faces * sizeof(float));
+  if (solution == NULL || full == NULL || flux == NULL) {
+    free(solution);
+    free(full);
+    free(flux);
+    return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+  }
+  int status = pcg(s, g, tolerance, max_iterations, solution, iterations, relative, err, errlen);
   if (status != MO_OK) {
     free(solution);
-    system_free(&s);
+    free(full);
+    free(flux);
     return status;
   }
-  /* True residual in FP64 (the reference reports only the recursive one). */
-  double* ax = malloc((s.count + 1) * sizeof(double));
-  apply_active(&s, g, solution, ax);
-  double rr = 0.0, bb = 0.0;
-  for (size_t i = 0; i < s.count; ++i) {
-    const double d = (double)s.a_rhs[i] - ax[i];
-    rr += d * d;
-    bb += (double)s.a_rhs[i] * s.a_rhs[i];
+  if (rr_out != NULL || bb_out != NULL) {
+    /* True residual in FP64 (the reference reports only the recursive one). */
+    double* ax = malloc((s->count + 1) * sizeof(double));
+    apply_active(s, g, solution, ax);
+    double rr = 0.0, bb = 0.0;
+    for (size_t i = 0; i < s->count; ++i) {
+      const double d = (double)s->a_rhs[i] - ax[i];
+      rr += d * d;
+      bb += (double)s->a_rhs[i] * s->a_rhs[i];
+    }
+    free(ax);
+    if (rr_out != NULL) *rr_out = rr;
+    if (bb_out != NULL) *bb_out = bb;
   }
-  free(ax);
-  /* scatter, flow_system.hpp:699-708 */
-  double* full = calloc(s.n, sizeof(double));
-  for (size_t i = 0; i < s.count; ++i) full[s.sites[i]] = solution[i];
+  for (size_t i = 0; i < s->count; ++i) full[s->sites[i]] = solution[i];
   free(solution);
-  /* velocity, flow_system.hpp:711-736, over the faces of the grid of
-   * columns: the depth-integrated flux of each column face. */
-  const mo_grid* g2 = &s.grid2;
-  const size_t nfaces2 = s.faces;
-  float* flux = malloc(nfaces2 * sizeof(float));
+  const mo_grid* g2 = &s->grid2;
   const float spacing[3] = {g->dx, g->dy, g->dz};
   const uint32_t dims[3] = {g->nx, g->ny, 1};
   size_t index = 0;
@@ -601,23 +660,18 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
           const double sp = (double)spacing[comp];
           double value;
           if (has_lower && has_upper) {
-            const float fm = harmonic_mean(s.mobility[lower], s.mobility[upper]);
+            const float fm = harmonic_mean(s->mobility[lower], s->mobility[upper]);
             value = -(double)fm * (full[upper] - full[lower]) / sp;
           } else {
             const size_t site = has_upper ? upper : lower;
-            value = (double)s.face_cond[index] * ((double)s.face_pressure[index] - full[site]);
+            value = (double)s->face_cond[index] * ((double)s->face_pressure[index] - full[site]);
           }
           flux[index] = (float)value;
         }
   }
-  /* lift, flow_system.hpp:424-470: each x and y voxel face between fluid
-   * voxels carries its column face's flux over the smaller of the two
-   * columns' heights; then the vertical faces by continuity, integrating the
-   * horizontal divergence upward with the column's mean removed so floor and
-   * roof stay closed. */
   const uint32_t nz = g->nz;
   const size_t nfaces = total_faces(g);
-  float* velocity = calloc(nfaces, sizeof(float));
+  memset(velocity, 0, nfaces * sizeof(float));
   for (int comp = 0; comp < 2; ++comp) {
     uint32_t fd[2] = {g->nx, g->ny};
     ++fd[comp];
@@ -637,10 +691,9 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
             continue;
           const size_t a_col = has_lower ? low_col : up_col;
           const size_t b_col = has_upper ? up_col : a_col;
-          const float height = fminf(s.heights[a_col], s.heights[b_col]);
+          const float height = fminf(s->heights[a_col], s->heights[b_col]);
           if (height > 0.0f)
-            velocity[face_index(g, comp, x, y, z)] =
-                flux[face_index(g2, comp, x, y, 0)] / height;
+            velocity[face_index(g, comp, x, y, z)] = flux[face_index(g2, comp, x, y, 0)] / height;
         }
   }
   if (nz > 1) {
@@ -649,7 +702,7 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
       for (uint32_t y = 0; y < g->ny; ++y) {
         size_t count = 0;
         double sum = 0.0;
-        for (uint32_t z = s.bottom; z < nz; ++z) {
+        for (uint32_t z = s->bottom; z < nz; ++z) {
           if (solid_at(g, ((size_t)x * g->ny + y) * nz + z)) break;
           const double dxv = (velocity[face_index(g, 0, x + 1, y, z)] -
                               velocity[face_index(g, 0, x, y, z)]) /
@@ -665,12 +718,48 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
         double vertical = 0.0;
         for (size_t k = 0; k + 1 < count; ++k) {
           vertical -= g->dz * (horizontal[k] - mean);
-          velocity[face_index(g, 2, x, y, s.bottom + (uint32_t)k + 1)] = (float)vertical;
+          velocity[face_index(g, 2, x, y, s->bottom + (uint32_t)k + 1)] = (float)vertical;
         }
       }
     free(horizontal);
   }
   free(flux);
+  if (full_out != NULL) memcpy(full_out, full, s->n * sizeof(double));
+  free(full);
+  return MO_OK;
+}
+
+int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
+                  float mean_inlet_speed, float tolerance, uint32_t max_iterations, int scale,
+                  float* faces, double* pressure, mo_flow_report* report, char* err,
+                  size_t errlen) {
+  /* DepthAveragedFlowParameters::validate, cpu_flow.cpp:134-150; the
+   * tolerance floor (float epsilon) is not applied here: the configurations
+   * solve to 1e-8 and 1e-10. */
+  if (scale && (!isfinite(mean_inlet_speed) || mean_inlet_speed == 0.0f))
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "depth-averaged mean inlet speed must be finite and nonzero without a body force");
+  if (!isfinite(tolerance) || tolerance <= 0.0f)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "depth-averaged relative tolerance must be positive");
+  if (max_iterations == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "depth-averaged iteration limit must be positive");
+  system_t s;
+  int status = build_system(g, mobility, axis, &s, err, errlen);
+  if (status != MO_OK) return status;
+  const uint32_t nz = g->nz;
+  const size_t nfaces = total_faces(g);
+  float* velocity = malloc(nfaces * sizeof(float));
+  double* full = malloc(s.n * sizeof(double));
+  uint32_t iterations = 0;
+  double relative = 0.0, rr = 0.0, bb = 0.0;
+  status = solve_and_lift(&s, g, tolerance, max_iterations, velocity, full, &iterations, &relative,
+                          &rr, &bb, err, errlen);
+  if (status != MO_OK) {
+    free(velocity);
+    free(full);
+    system_free(&s);
+    return status;
+  }
   /* scale_velocity, flow_system.hpp:1368-1415; open inlet faces are the
    * axis's lower faces of fluid voxels in columns with conductance
    * (open_inlet_faces, 449-459). */
@@ -734,6 +823,471 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
   free(full);
   system_free(&s);
   return MO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Port-driven flow                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* port_face_geometry (flow_system.hpp:203-219): the axis a face is normal to,
+ * whether it is that axis's upper face, and its in-plane axes. */
+static int port_geometry(uint32_t face, int* component, int* upper, int in_plane[2]) {
+  if (face / 2 > 2) return 0;
+  static const int planes[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+  *component = (int)(face / 2);
+  *upper = (int)(face % 2);
+  in_plane[0] = planes[*component][0];
+  in_plane[1] = planes[*component][1];
+  return 1;
+}
+
+/* face_extent (cpp/core/signals.cpp:533-546). */
+static void face_extent(const mo_grid* g, uint32_t face, uint32_t extent[2]) {
+  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+  int component = 0, upper = 0, in_plane[2] = {0, 0};
+  port_geometry(face, &component, &upper, in_plane);
+  extent[0] = dims[in_plane[0]];
+  extent[1] = dims[in_plane[1]];
+}
+
+/* for_each_port_face (flow_system.hpp:221-236): the exterior faces of a
+ * port's patch, first in-plane index outer, as face index and the index of
+ * the site inside the face. Returns the number written. */
+static size_t port_faces(const mo_grid* g, const mo_flow_port* port, size_t* faces,
+                         size_t* sites) {
+  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+  int component = 0, upper = 0, in_plane[2] = {0, 0};
+  port_geometry(port->face, &component, &upper, in_plane);
+  size_t count = 0;
+  for (uint32_t first = port->begin[0]; first < port->end[0]; ++first)
+    for (uint32_t second = port->begin[1]; second < port->end[1]; ++second) {
+      uint32_t fc[3] = {0, 0, 0};
+      fc[component] = upper ? dims[component] : 0;
+      fc[in_plane[0]] = first;
+      fc[in_plane[1]] = second;
+      uint32_t sc[3] = {fc[0], fc[1], fc[2]};
+      if (upper) --sc[component];
+      faces[count] = face_index(g, component, fc[0], fc[1], fc[2]);
+      sites[count] = ((size_t)sc[0] * g->ny + sc[1]) * g->nz + sc[2];
+      ++count;
+    }
+  return count;
+}
+
+static size_t port_face_count(const mo_flow_port* port) {
+  return (size_t)(port->end[0] - port->begin[0]) * (port->end[1] - port->begin[1]);
+}
+
+/* validate_flow_ports (cpp/cpu/cpu_flow.cpp:193-286) with `solid` as the
+ * grid's obstacle mask. */
+static int validate_ports(const mo_grid* g, const uint8_t* solid, const mo_flow_port* ports,
+                          uint32_t count, char* err, size_t errlen) {
+  int status = validate_grid(g, err, errlen);
+  if (status != MO_OK) return status;
+  for (int a = 0; a < 3; ++a)
+    if (g->boundary[2 * a] == MO_PERIODIC || g->boundary[2 * a + 1] == MO_PERIODIC)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "native flow solvers do not support periodic boundaries");
+  if (count == 0) return fail(err, errlen, MO_INVALID_ARGUMENT, "flow needs at least one port");
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const size_t nf = total_faces(g);
+  uint8_t* claimed = calloc(nf, 1);
+  uint8_t* visited = calloc(n, 1);
+  size_t* stack = malloc((n + 1) * sizeof(size_t));
+  size_t largest = 0;
+  for (uint32_t p = 0; p < count; ++p) {
+    const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+    int component = 0, upper = 0, in_plane[2] = {0, 0};
+    if (ports[p].face / 2 <= 2) {
+      port_geometry(ports[p].face, &component, &upper, in_plane);
+      const size_t span = (size_t)dims[in_plane[0]] * dims[in_plane[1]];
+      if (span > largest) largest = span;
+    }
+  }
+  size_t* faces = malloc((largest + 1) * sizeof(size_t));
+  size_t* sites = malloc((largest + 1) * sizeof(size_t));
+  size_t top = 0;
+  int holds_pressure = 0;
+  status = MO_OK;
+  if (!claimed || !visited || !stack || !faces || !sites) {
+    status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+    goto done;
+  }
+  for (uint32_t p = 0; p < count; ++p) {
+    const mo_flow_port* port = &ports[p];
+    if (port->kind != MO_PORT_SPEED && port->kind != MO_PORT_PRESSURE) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT, "unknown flow port kind");
+      goto done;
+    }
+    if (!isfinite(port->value)) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT, "flow port values must be finite");
+      goto done;
+    }
+    int component = 0, upper = 0, in_plane[2] = {0, 0};
+    if (!port_geometry(port->face, &component, &upper, in_plane)) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT, "unknown flow port face");
+      goto done;
+    }
+    if (g->boundary[port->face] != MO_FIXED) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT, "a flow port must open a fixed boundary");
+      goto done;
+    }
+    uint32_t extent[2];
+    face_extent(g, port->face, extent);
+    if (port->end[0] <= port->begin[0] || port->end[1] <= port->begin[1] ||
+        port->end[0] > extent[0] || port->end[1] > extent[1]) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "a flow port must span at least one site of its face");
+      goto done;
+    }
+    const size_t k = port_faces(g, port, faces, sites);
+    int open = 0;
+    for (size_t i = 0; i < k; ++i) {
+      if (claimed[faces[i]]) {
+        status = fail(err, errlen, MO_INVALID_ARGUMENT, "flow ports must not overlap");
+        goto done;
+      }
+      claimed[faces[i]] = 1;
+      if (solid != NULL && solid[sites[i]]) continue;
+      open = 1;
+      if (port->kind == MO_PORT_PRESSURE && !visited[sites[i]]) {
+        visited[sites[i]] = 1;
+        stack[top++] = sites[i];
+      }
+    }
+    if (!open) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT, "a flow port is entirely blocked");
+      goto done;
+    }
+    holds_pressure = holds_pressure || port->kind == MO_PORT_PRESSURE;
+  }
+  if (!holds_pressure) {
+    status = fail(err, errlen, MO_INVALID_ARGUMENT, "flow needs at least one pressure port");
+    goto done;
+  }
+  /* Something flows: a speed port, or a pressure other than the first
+   * pressure port's. */
+  {
+    float first = 0.0f;
+    for (uint32_t p = 0; p < count; ++p)
+      if (ports[p].kind == MO_PORT_PRESSURE) {
+        first = ports[p].value;
+        break;
+      }
+    int drives = 0;
+    for (uint32_t p = 0; p < count; ++p)
+      if (ports[p].kind == MO_PORT_SPEED || ports[p].value != first) drives = 1;
+    if (!drives) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "flow ports state no speed and hold one pressure, so nothing flows");
+      goto done;
+    }
+  }
+  /* Every speed port reaches a pressure port through fluid. */
+  {
+    const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+    while (top > 0) {
+      const size_t site = stack[--top];
+      const uint32_t c[3] = {(uint32_t)(site / ((size_t)g->ny * g->nz)),
+                             (uint32_t)((site / g->nz) % g->ny), (uint32_t)(site % g->nz)};
+      for (int comp = 0; comp < 3; ++comp)
+        for (int off = -1; off <= 1; off += 2) {
+          uint32_t nc[3] = {c[0], c[1], c[2]};
+          if (off < 0) {
+            if (nc[comp] == 0) continue;
+            --nc[comp];
+          } else {
+            if (nc[comp] + 1 >= dims[comp]) continue;
+            ++nc[comp];
+          }
+          const size_t nb = ((size_t)nc[0] * g->ny + nc[1]) * g->nz + nc[2];
+          if (!visited[nb] && !(solid != NULL && solid[nb])) {
+            visited[nb] = 1;
+            stack[top++] = nb;
+          }
+        }
+    }
+  }
+  for (uint32_t p = 0; p < count; ++p) {
+    if (ports[p].kind != MO_PORT_SPEED) continue;
+    const size_t k = port_faces(g, &ports[p], faces, sites);
+    int reached = 0;
+    for (size_t i = 0; i < k; ++i) reached = reached || visited[sites[i]];
+    if (!reached) {
+      status = fail(err, errlen, MO_RUNTIME_ERROR,
+                    "a speed port carries no flow: no pressure port is reachable");
+      goto done;
+    }
+  }
+done:
+  free(claimed);
+  free(visited);
+  free(stack);
+  free(faces);
+  free(sites);
+  return status;
+}
+
+int mo_port_flow_solve(const mo_grid* g, const float* mobility, const mo_flow_port* ports,
+                       uint32_t count, float tolerance, uint32_t max_iterations, float* out_faces,
+                       mo_port_state* states, mo_port_report* report, char* err, size_t errlen) {
+  /* DepthAveragedPortFlowParameters::validate (cpu_flow.cpp:187-192). */
+  if (!isfinite(tolerance) || tolerance < FLT_EPSILON)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "depth-averaged relative tolerance must be finite and at least float machine "
+                "epsilon");
+  if (max_iterations == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "depth-averaged iteration limit must be positive");
+  /* ShallowPortFlow (flow_system.hpp:1268-1311): the ports on the grid, the
+   * reduction, the ports on the grid of columns, and the faces each port
+   * carries flow through. */
+  int status = validate_ports(g, g->obstacles, ports, count, err, errlen);
+  if (status != MO_OK) return status;
+  system_t s;
+  status = reduce_columns(g, mobility, &s, err, errlen);
+  if (status != MO_OK) return status;
+  const uint32_t nz = g->nz;
+  mo_flow_port* reduced = malloc(count * sizeof(mo_flow_port));
+  size_t total_port_faces = 0;
+  for (uint32_t p = 0; p < count; ++p) total_port_faces += port_face_count(&ports[p]);
+  size_t* open_faces = malloc((total_port_faces + 1) * sizeof(size_t));
+  size_t* offsets = calloc(count + 1, sizeof(size_t));
+  double* signed_area = malloc((count + 1) * sizeof(double));
+  size_t* scratch_faces = malloc((total_port_faces + 1) * sizeof(size_t));
+  size_t* scratch_sites = malloc((total_port_faces + 1) * sizeof(size_t));
+  const size_t nfaces = total_faces(g);
+  uint8_t* open = calloc(s.faces, 1);
+  float* pressure = calloc(s.faces, sizeof(float));
+  float** fields = calloc(count + 1, sizeof(float*));
+  double* velocity = NULL;
+  double** response = NULL;
+  float* pressures = malloc((count + 1) * sizeof(float));
+  float* held = calloc(count + 1, sizeof(float));
+  float* unit = calloc(count + 1, sizeof(float));
+  uint32_t* driven = malloc((count + 1) * sizeof(uint32_t));
+  uint32_t solves = 0, iterations_sum = 0, nfields = 0;
+  float largest_relative = 0.0f;
+  if (!reduced || !open_faces || !offsets || !signed_area || !scratch_faces || !scratch_sites ||
+      !open || !pressure || !fields || !pressures || !held || !unit || !driven) {
+    status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+    goto done;
+  }
+  /* reduced_port (flow_system.hpp:347-372): an x or y face, spanning every
+   * fluid voxel of each column it reaches from the common floor up. */
+  for (uint32_t p = 0; p < count; ++p) {
+    int component = 0, upper = 0, in_plane[2] = {0, 0};
+    port_geometry(ports[p].face, &component, &upper, in_plane);
+    if (component == 2) {
+      status = fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "a shallow-flow port cannot open a z face; use resolved flow");
+      goto done;
+    }
+    uint32_t site[3] = {0, 0, 0};
+    const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+    if (upper) site[component] = dims[component] - 1;
+    for (uint32_t along = ports[p].begin[0]; along < ports[p].end[0]; ++along) {
+      site[in_plane[0]] = along;
+      for (uint32_t z = s.bottom; z < nz; ++z) {
+        if (solid_at(g, ((size_t)site[0] * g->ny + site[1]) * nz + z)) break;
+        if (z < ports[p].begin[1] || z >= ports[p].end[1]) {
+          status = fail(err, errlen, MO_INVALID_ARGUMENT,
+                        "a shallow-flow port must span the full depth of its columns; use "
+                        "resolved flow");
+          goto done;
+        }
+      }
+    }
+    reduced[p] = ports[p];
+    reduced[p].begin[1] = 0;
+    reduced[p].end[1] = 1;
+  }
+  status = validate_ports(&s.grid2, s.reduced_solid, reduced, count, err, errlen);
+  if (status != MO_OK) goto done;
+  /* open_port_faces (385-393) and make_port_faces (1124-1130): the port's
+   * faces onto fluid voxels of columns with conductance; the area of one
+   * face, negative on an upper face. */
+  {
+    const float spacing[3] = {g->dx, g->dy, g->dz};
+    for (uint32_t p = 0; p < count; ++p) {
+      int component = 0, upper = 0, in_plane[2] = {0, 0};
+      port_geometry(ports[p].face, &component, &upper, in_plane);
+      const size_t k = port_faces(g, &ports[p], scratch_faces, scratch_sites);
+      offsets[p + 1] = offsets[p];
+      for (size_t i = 0; i < k; ++i) {
+        const size_t site = scratch_sites[i];
+        if (!solid_at(g, site) && s.mobility[site / nz] > 0.0f)
+          open_faces[offsets[p + 1]++] = scratch_faces[i];
+      }
+      const double area = (double)spacing[in_plane[0]] * spacing[in_plane[1]];
+      signed_area[p] = upper ? -area : area;
+    }
+  }
+  /* port_flow_boundary (238-253) on the grid of columns: every port's faces
+   * open; the pressures are set per solve. */
+  for (uint32_t p = 0; p < count; ++p) {
+    const size_t k = port_faces(&s.grid2, &reduced[p], scratch_faces, scratch_sites);
+    for (size_t i = 0; i < k; ++i) open[scratch_faces[i]] = 1;
+  }
+  /* superpose_port_flows (1139-1252). */
+  uint32_t count_driven = 0;
+  int holds_pressure = 0;
+  for (uint32_t p = 0; p < count; ++p) {
+    if (ports[p].kind == MO_PORT_SPEED) {
+      driven[count_driven++] = p;
+    } else {
+      held[p] = ports[p].value;
+      holds_pressure = holds_pressure || ports[p].value != 0.0f;
+    }
+  }
+#define PORT_SOLVE(values, slot)                                                              \
+  do {                                                                                        \
+    for (uint32_t p = 0; p < count; ++p) {                                                    \
+      const size_t k = port_faces(&s.grid2, &reduced[p], scratch_faces, scratch_sites);       \
+      for (size_t i = 0; i < k; ++i) pressure[scratch_faces[i]] = (values)[p];                \
+    }                                                                                         \
+    status = assemble_system(&s, open, pressure, err, errlen);                                \
+    if (status != MO_OK) goto done;                                                           \
+    fields[slot] = malloc(nfaces * sizeof(float));                                            \
+    if (fields[slot] == NULL) {                                                               \
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");                          \
+      goto done;                                                                              \
+    }                                                                                         \
+    ++nfields;                                                                                \
+    uint32_t its = 0;                                                                         \
+    double rel = 0.0;                                                                         \
+    status = solve_and_lift(&s, g, tolerance, max_iterations, fields[slot], NULL, &its, &rel, \
+                            NULL, NULL, err, errlen);                                         \
+    if (status != MO_OK) goto done;                                                           \
+    iterations_sum += its;                                                                    \
+    if ((float)rel > largest_relative) largest_relative = (float)rel;                         \
+    ++solves;                                                                                 \
+  } while (0)
+  velocity = calloc(nfaces, sizeof(double));
+  if (velocity == NULL) {
+    status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+    goto done;
+  }
+  int have_velocity = 0;
+  if (holds_pressure) {
+    PORT_SOLVE(held, count);
+    for (size_t f = 0; f < nfaces; ++f) velocity[f] += 1.0 * (double)fields[count][f];
+    have_velocity = 1;
+  }
+  for (uint32_t p = 0; p < count; ++p) pressures[p] = held[p];
+  if (count_driven > 0) {
+    response = calloc(count_driven, sizeof(double*));
+    if (response == NULL) {
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+      goto done;
+    }
+    for (uint32_t m = 0; m < count_driven; ++m) {
+      response[m] = calloc(count_driven + 1, sizeof(double));
+      if (response[m] == NULL) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+        goto done;
+      }
+    }
+    for (uint32_t j = 0; j < count_driven; ++j) {
+      for (uint32_t p = 0; p < count; ++p) unit[p] = 0.0f;
+      unit[driven[j]] = 1.0f;
+      PORT_SOLVE(unit, j);
+      for (uint32_t m = 0; m < count_driven; ++m) {
+        const uint32_t q = driven[m];
+        double sum = 0.0;
+        for (size_t i = offsets[q]; i < offsets[q + 1]; ++i) sum += (double)fields[j][open_faces[i]];
+        response[m][j] = sum * signed_area[q];
+      }
+    }
+    for (uint32_t m = 0; m < count_driven; ++m) {
+      const uint32_t q = driven[m];
+      double base = 0.0;
+      if (have_velocity) {
+        double sum = 0.0;
+        for (size_t i = offsets[q]; i < offsets[q + 1]; ++i) sum += velocity[open_faces[i]];
+        base = sum * signed_area[q];
+      }
+      const double open_area = fabs(signed_area[q]) * (double)(offsets[q + 1] - offsets[q]);
+      response[m][count_driven] = (double)ports[q].value * open_area - base;
+    }
+    /* Gaussian elimination with partial pivoting on the augmented matrix. */
+    for (uint32_t column = 0; column < count_driven; ++column) {
+      uint32_t pivot = column;
+      for (uint32_t row = column + 1; row < count_driven; ++row)
+        if (fabs(response[row][column]) > fabs(response[pivot][column])) pivot = row;
+      if (response[pivot][column] == 0.0) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR,
+                      "a speed port carries no flow: no pressure port is reachable");
+        goto done;
+      }
+      double* swap = response[column];
+      response[column] = response[pivot];
+      response[pivot] = swap;
+      for (uint32_t row = 0; row < count_driven; ++row) {
+        if (row == column) continue;
+        const double factor = response[row][column] / response[column][column];
+        for (uint32_t entry = column; entry <= count_driven; ++entry)
+          response[row][entry] -= factor * response[column][entry];
+      }
+    }
+    for (uint32_t j = 0; j < count_driven; ++j) {
+      const double p = response[j][count_driven] / response[j][j];
+      pressures[driven[j]] = (float)p;
+      for (size_t f = 0; f < nfaces; ++f) velocity[f] += p * (double)fields[j][f];
+    }
+  }
+#undef PORT_SOLVE
+  {
+    float max_speed = 0.0f;
+    for (size_t f = 0; f < nfaces; ++f) {
+      out_faces[f] = (float)velocity[f];
+      if (!isfinite(out_faces[f])) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+        goto done;
+      }
+      if (fabsf(out_faces[f]) > max_speed) max_speed = fabsf(out_faces[f]);
+    }
+    for (uint32_t p = 0; p < count; ++p) {
+      double sum = 0.0;
+      for (size_t i = offsets[p]; i < offsets[p + 1]; ++i) sum += velocity[open_faces[i]];
+      const double flux = sum * signed_area[p];
+      const double open_area = fabs(signed_area[p]) * (double)(offsets[p + 1] - offsets[p]);
+      if (states != NULL) {
+        states[p].pressure = pressures[p];
+        states[p].flux = (float)flux;
+        states[p].mean_speed = (float)(flux / open_area);
+      }
+    }
+    if (report != NULL) {
+      report->solves = solves;
+      report->iterations = iterations_sum;
+      report->relative_residual = largest_relative;
+      report->max_speed = max_speed;
+    }
+  }
+done:
+  if (response != NULL) {
+    for (uint32_t m = 0; m < count_driven; ++m) free(response[m]);
+    free(response);
+  }
+  if (fields != NULL)
+    for (uint32_t i = 0; i <= count; ++i) free(fields[i]);
+  free(fields);
+  free(velocity);
+  free(reduced);
+  free(open_faces);
+  free(offsets);
+  free(signed_area);
+  free(scratch_faces);
+  free(scratch_sites);
+  free(open);
+  free(pressure);
+  free(pressures);
+  free(held);
+  free(unit);
+  free(driven);
+  system_free(&s);
+  (void)nfields;
+  return status;
 }
 
 /* ------------------------------------------------------------------------ */

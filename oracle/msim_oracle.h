@@ -24,6 +24,11 @@
  *                           ledger identity to 1e-12
  *   mo_adr_validate_step    the reference's explicit bound, known answers from
  *                           the reference tests (dt 0.25 ok / 0.51 rejected)
+ *   mo_port_flow_solve      bit-exact vs reference solve_depth_averaged_port_flow_cpu
+ *                           (oracle/_ref); the reference tests' known answers
+ *                           (port fluxes 1.5/0.5/-2, pressure-driven duct 0.8,
+ *                           whole-face ports equal to the single-axis solve);
+ *                           the refusals of the reference tests
  */
 #ifndef MSIM_ORACLE_H
 #define MSIM_ORACLE_H
@@ -91,6 +96,37 @@ int mo_flow_solve(const mo_grid* grid, const float* mobility, uint32_t axis,
                   float mean_inlet_speed, float tolerance, uint32_t max_iterations, int scale,
                   float* faces, double* pressure, mo_flow_report* report, char* err,
                   size_t errlen);
+
+/* FlowPortKind and FlowPort (cpp/include/microsimulator/flow.hpp:69-91): an
+ * opening over a rectangle of one fixed face, driven at a mean speed (into
+ * the grid positive) or held at a pressure. In-plane indices as for a
+ * boundary patch: on an x face (y, z), on a y face (x, z), on a z face (x, y). */
+enum { MO_PORT_SPEED = 0, MO_PORT_PRESSURE = 1 };
+typedef struct {
+  uint32_t face; /* 0..5: x_lower, x_upper, y_lower, y_upper, z_lower, z_upper */
+  uint32_t begin[2], end[2];
+  uint32_t kind; /* MO_PORT_SPEED or MO_PORT_PRESSURE */
+  float value;
+} mo_flow_port;
+
+/* FlowPortState (flow.hpp:96-100). */
+typedef struct {
+  float pressure, flux, mean_speed;
+} mo_port_state;
+
+/* DepthAveragedPortFlowReport (flow.hpp:109-114). */
+typedef struct {
+  uint32_t solves, iterations;
+  float relative_residual, max_speed;
+} mo_port_report;
+
+/* solve_depth_averaged_port_flow_cpu (cpp/cpu/cpu_flow.cpp:353-375): shallow
+ * flow driven through ports, as the superposition of one Jacobi-PCG pressure
+ * solve per speed port (and one for the held pressures). faces: x|y|z face
+ * velocities of the original grid; states: one per port. */
+int mo_port_flow_solve(const mo_grid* grid, const float* mobility, const mo_flow_port* ports,
+                       uint32_t count, float tolerance, uint32_t max_iterations, float* faces,
+                       mo_port_state* states, mo_port_report* report, char* err, size_t errlen);
 
 /* GridBoundaryPatch (signals.hpp:72-84): a rectangle of a fixed face with its
  * own reservoir value per signal. In-plane indices: on an x face (y, z), on a

@@ -11,6 +11,7 @@
 //   detail::DepthAveragedFlowSystem      cpp/core/flow_system.hpp:520-748
 //   detail::solve_depth_averaged_pressure cpp/cpu/cpu_flow.cpp:172-175
 //   SignalGrid + advance_signal_grid_cpu cpp/core/signals.cpp:1377-1389
+//   solve_depth_averaged_port_flow_cpu   cpp/cpu/cpu_flow.cpp:353-375
 //   set_cpu_thread_count                 cpp/include/microsimulator/threads.hpp:19
 #include <chrono>
 #include <cstdint>
@@ -270,6 +271,55 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
     if (seconds != nullptr) *seconds = std::chrono::duration<double>(stop - start).count();
     const auto out = grid.levels();
     std::memcpy(levels, out.data(), out.size() * sizeof(float));
+  });
+}
+
+// solve_depth_averaged_port_flow_cpu on a one-signal grid. Ports: per port
+// (face, begin[0], begin[1], end[0], end[1], kind) in port_geometry and its
+// value in port_values. faces: x|y|z of the grid; states: 3 floats per port
+// (pressure, flux, mean speed); report: solves, iterations.
+int ref_port_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
+                  float dz, const std::uint8_t* obstacles, const std::uint8_t* kinds,
+                  const float* mobility, std::uint32_t port_count,
+                  const std::uint32_t* port_geometry, const float* port_values, float tolerance,
+                  std::uint32_t max_iterations, float* faces, float* states,
+                  std::uint32_t* counts, float* relative_residual, float* max_speed,
+                  double* seconds, char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> mob;
+    if (mobility != nullptr) mob = std::span<const float>(mobility, n);
+    std::vector<ms::FlowPort> ports(port_count);
+    for (std::uint32_t k = 0; k < port_count; ++k) {
+      const std::uint32_t* g = port_geometry + 6 * k;
+      ports[k].face = static_cast<ms::GridFace>(g[0]);
+      ports[k].patch.begin = {g[1], g[2]};
+      ports[k].patch.end = {g[3], g[4]};
+      ports[k].kind = static_cast<ms::FlowPortKind>(g[5]);
+      ports[k].value = port_values[k];
+    }
+    ms::DepthAveragedPortFlowParameters parameters;
+    parameters.relative_tolerance = tolerance;
+    parameters.max_iterations = max_iterations;
+    const auto start = std::chrono::steady_clock::now();
+    const auto result = ms::solve_depth_averaged_port_flow_cpu(spec, ports, mob, parameters);
+    const auto stop = std::chrono::steady_clock::now();
+    if (seconds != nullptr) *seconds = std::chrono::duration<double>(stop - start).count();
+    std::memcpy(faces, result.field.x_faces.data(), result.field.x_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size(), result.field.y_faces.data(),
+                result.field.y_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size() + result.field.y_faces.size(),
+                result.field.z_faces.data(), result.field.z_faces.size() * sizeof(float));
+    for (std::uint32_t k = 0; k < port_count; ++k) {
+      states[3 * k] = result.ports[k].pressure;
+      states[3 * k + 1] = result.ports[k].flux;
+      states[3 * k + 2] = result.ports[k].mean_speed;
+    }
+    counts[0] = result.report.solves;
+    counts[1] = result.report.iterations;
+    *relative_residual = result.report.relative_residual;
+    *max_speed = result.report.max_speed;
   });
 }
 

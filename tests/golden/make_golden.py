@@ -18,6 +18,12 @@ Fixtures (inputs and the reference's outputs, nothing else):
   small_cases.npz  a 5x8 obstacle case with mobility 0.4+0.1x
                    (flow_conformance_test.cpp:62-88 pattern, one layer) and a
                    3-site diffusion line (signal_grid_test.cpp:90-103).
+  port_flow.npz    solve_depth_averaged_port_flow_cpu on the conformance
+                   suite's two port cases (flow_conformance_test.cpp:167-271,
+                   built by tests/helpers.py): faces, port states (pressure,
+                   flux, mean speed), solves and iterations.
+
+Usage: python tests/golden/make_golden.py [fixture ...] (default: all).
 """
 from __future__ import annotations
 
@@ -107,11 +113,26 @@ def small():
                         line_levels=levels)
 
 
+def ports():
+    from tests.helpers import depth_port_case, winding_channel_case
+    ref = Reference()
+    out = {}
+    for name, builder in (("depth", depth_port_case), ("winding", winding_channel_case)):
+        g, mobility, case_ports = builder()
+        faces, states, rep = ref.port_flow(Grid(**g), mobility, case_ports)
+        out[f"{name}_faces"] = faces
+        out[f"{name}_states"] = states
+        out[f"{name}_solves"] = rep["solves"]
+        out[f"{name}_iterations"] = rep["iterations"]
+    np.savez_compressed(os.path.join(OUT, "port_flow.npz"), **out)
+
+
+FIXTURES = {"c1": c1, "illcond": illcond, "small": small, "ports": ports}
+
 if __name__ == "__main__":
     build(reference=True)
-    c1()
-    illcond()
-    small()
+    for name in sys.argv[1:] or list(FIXTURES):
+        FIXTURES[name]()
     for name in sorted(os.listdir(OUT)):
         if name.endswith(".npz"):
             print(name, os.path.getsize(os.path.join(OUT, name)))

@@ -164,3 +164,109 @@ PATCH_KINDS = [
     (PERIODIC, PERIODIC, FIXED, FIXED, NO_FLUX, NO_FLUX),
 ]
 
+
+
+# ----------------------------------------------------------------- ports --
+SPEED, PRESSURE = 0, 1
+X_LOWER, X_UPPER, Y_LOWER, Y_UPPER, Z_LOWER, Z_UPPER = range(6)
+SIDES_FIXED = (FIXED, FIXED, FIXED, FIXED, NO_FLUX, NO_FLUX)
+DUCT = (NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX)
+
+
+def port(face, begin, end, depth, kind, value):
+    """A port over [begin, end) of a side face and the depth [0, depth)
+    (tests/cpp/flow_test.cpp:42-49)."""
+    return (face, (begin, 0), (end, depth), kind, value)
+
+
+def depth_port_case():
+    """Two inlets on one face, a third drawing fluid out through a
+    perpendicular face, one outlet, around an obstacle and across varying
+    mobility: 12x10x2, spacing (0.8, 1.25, 0.6)
+    (tests/conformance/flow_conformance_test.cpp:167-205)."""
+    nx, ny, nz = 12, 10, 2
+    obstacles = np.zeros((nx, ny, nz), np.uint8)
+    mobility = np.zeros((nx, ny, nz), np.float32)
+    for x in range(nx):
+        for y in range(ny):
+            obstacles[x, y, :] = 1 if (5 <= x <= 6 and 3 <= y <= 5) else 0
+            mobility[x, y, :] = np.float32(0.4) + np.float32(0.07) * np.float32((x * 3 + y * 5) % 9)
+    grid = dict(nx=nx, ny=ny, nz=nz, dx=0.8, dy=1.25, dz=0.6, obstacles=obstacles.reshape(-1),
+                boundary=SIDES_FIXED)
+    ports = [(Y_LOWER, (0, 0), (3, 2), SPEED, 3.0), (Y_LOWER, (8, 0), (12, 2), SPEED, 1.25),
+             (X_UPPER, (6, 0), (9, 2), SPEED, -0.5), (Y_UPPER, (2, 0), (10, 2), PRESSURE, 0.0)]
+    return grid, mobility.reshape(-1), ports
+
+
+def winding_channel_case():
+    """A grid that is mostly wall with one channel two sites wide winding
+    from an inlet to an outlet: 24x42x1, spacing (1, 1, 2)
+    (flow_conformance_test.cpp:223-249)."""
+    nx, ny = 24, 42
+    obstacles = np.ones((nx, ny), np.uint8)
+    for x in range(1, 23):
+        for y in range(ny):
+            band, run = y // 8, y % 8 < 2
+            turn = x >= 21 if band % 2 == 0 else x <= 2
+            if run or turn:
+                obstacles[x, y] = 0
+    grid = dict(nx=nx, ny=ny, nz=1, dx=1.0, dy=1.0, dz=2.0, obstacles=obstacles.reshape(-1),
+                boundary=DUCT)
+    ports = [(Y_LOWER, (1, 0), (3, 1), SPEED, 2.0), (Y_UPPER, (1, 0), (3, 1), PRESSURE, 0.0)]
+    return grid, None, ports
+
+
+def random_port_case(seed, nx, ny, nz=1, with_mobility=True):
+    """Every side face fixed; two to four ports on distinct side faces (so
+    none overlap), each spanning the full depth, at least one held at a
+    pressure; scattered obstacles (one layer) or columns of random roof
+    height with pillars (several layers); a mobility constant down each
+    column, or none."""
+    rng = np.random.default_rng(seed)
+    if nz == 1:
+        obstacles = (rng.random((nx, ny, 1)) < 0.12).astype(np.uint8)
+    else:
+        obstacles = np.zeros((nx, ny, nz), np.uint8)
+        roof = rng.integers(1, nz + 1, (nx, ny))
+        for x in range(nx):
+            for y in range(ny):
+                obstacles[x, y, roof[x, y]:] = 1
+        obstacles[rng.random((nx, ny)) < 0.08] = 1
+    mobility = None
+    if with_mobility:
+        column = (10.0 ** rng.uniform(-1.0, 0.0, (nx, ny))).astype(np.float32)
+        mobility = np.repeat(column[:, :, None], nz, axis=2).reshape(-1)
+    faces = rng.permutation(4)[: int(rng.integers(2, 5))]
+    ports = []
+    for k, face in enumerate(faces):
+        extent = ny if face < 2 else nx
+        begin = int(rng.integers(0, max(1, extent - 2)))
+        end = int(rng.integers(begin + 1, extent + 1))
+        if k == 0:
+            kind, value = PRESSURE, float(rng.choice([0.0, rng.uniform(-1.0, 2.0)]))
+        elif rng.random() < 0.7:
+            kind, value = SPEED, float(rng.uniform(-2.0, 3.0))
+        else:
+            kind, value = PRESSURE, float(rng.uniform(-1.0, 2.0))
+        ports.append((int(face), (begin, 0), (end, nz), kind, value))
+    grid = dict(nx=nx, ny=ny, nz=nz, dx=float(rng.uniform(0.5, 1.5)),
+                dy=float(rng.uniform(0.5, 1.5)), dz=float(rng.uniform(0.3, 1.0)),
+                obstacles=obstacles.reshape(-1), boundary=SIDES_FIXED)
+    return grid, mobility, ports
+
+
+PORT_CASES = [(1, 20, 17, 1, True), (102, 33, 9, 1, False), (3, 16, 24, 1, True),
+              (4, 14, 11, 3, True), (5, 18, 13, 2, False), (6, 9, 21, 4, True),
+              (7, 40, 31, 1, True), (8, 25, 25, 3, False)]
+
+
+def largest_divergence(nx, ny, nz, dx, dy, dz, obstacles, xf, yf, zf):
+    """The largest divergence of the face velocities at any fluid site
+    (tests/cpp/flow_test.cpp:51-75)."""
+    X = np.asarray(xf, np.float64).reshape(nx + 1, ny, nz)
+    Y = np.asarray(yf, np.float64).reshape(nx, ny + 1, nz)
+    Z = np.asarray(zf, np.float64).reshape(nx, ny, nz + 1)
+    out = ((X[1:] - X[:-1]) / dx + (Y[:, 1:] - Y[:, :-1]) / dy
+           + (Z[:, :, 1:] - Z[:, :, :-1]) / dz)
+    fluid = np.asarray(obstacles).reshape(nx, ny, nz) == 0
+    return float(np.abs(out[fluid]).max()) if fluid.any() else 0.0

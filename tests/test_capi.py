@@ -48,6 +48,15 @@ def test_unsupported_features_are_not_advertised():
     assert set(msa.FEATURES) >= msa.SUPPORTED
 
 
+def test_depth_averaged_entry_points_are_implemented():
+    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49) run the
+    C ABI; only the resolved and cell-mechanics calls are refused."""
+    stub = msa.HipBackend._unsupported
+    assert msa.HipBackend.solve_depth_averaged_flow is not stub
+    assert msa.HipBackend.solve_depth_averaged_port_flow is not stub
+    assert msa.HipBackend.solve_resolved_flow is stub
+
+
 def test_kernels_are_built_for_gfx950_only():
     ensure_built()
     blob = open(msa.LIBRARY, "rb").read()

@@ -280,9 +280,9 @@ def test_multigrid_full_size(hip):
 @pytest.mark.parametrize("axis", [0, 1])
 @pytest.mark.parametrize("solver", [msa.SOLVER_JACOBI_FP64, msa.SOLVER_MULTIGRID])
 def test_layered_grids_match_the_oracle(hip, oracle, axis, solver):
-    """Grids of several layers: the device solves the grid of columns, the
-    host lifts the column flux onto the voxel faces (vertical faces by
-    continuity) and scales it. Against the oracle (bit-exact with the
+    """Grids of several layers: the device solves the grid of columns and
+    lifts the column flux onto the voxel faces (vertical faces by
+    continuity); the host scales it. Against the oracle (bit-exact with the
     reference on these grids): FP64 Jacobi-PCG within 2 iterations and 1e-6 of
     the largest speed, the multilevel form within 1e-5."""
     for nx, ny, nz, seed, mob in LAYERED_CASES:
