@@ -188,6 +188,55 @@ def batch_sweep_bench(msa, hip, steps: int):
     }
 
 
+def port_flow_bench(msa, w, hip, cpu_seconds_per_iteration):
+    """A port-driven flow on the same channel: two speed inlets splitting the
+    lower x face (mean speeds 1 and 2) and the whole upper x face held at
+    pressure 0, so two pressure solves per flow, combined on the device.
+    Timed with the multilevel form and once with the reference's FP64
+    Jacobi-PCG, both at the reference's default tolerance 1e-6. The CPU
+    figure is the reference's seconds per FP64 iteration on this system (from
+    the CPU baseline) times the FP64 iterations the two solves take."""
+    grid = msa.Grid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    half = w.ny // 2
+    ports = [msa.FlowPort(msa.X_LOWER, (0, 0), (half, 1), msa.SPEED, 1.0),
+             msa.FlowPort(msa.X_LOWER, (half, 0), (w.ny, 1), msa.SPEED, 2.0),
+             msa.FlowPort(msa.X_UPPER, (0, 0), (w.ny, 1), msa.PRESSURE, 0.0)]
+    mg = msa.DepthAveragedPortFlowParameters(solver=msa.SOLVER_MULTIGRID)
+    hip.solve_depth_averaged_port_flow(grid, ports, w.mobility, mg)  # warm
+    count = 5
+    t0 = time.perf_counter()
+    runs = [hip.solve_depth_averaged_port_flow(grid, ports, w.mobility, mg) for _ in range(count)]
+    mg_rate = count / (time.perf_counter() - t0)
+    fp64 = msa.DepthAveragedPortFlowParameters(solver=msa.SOLVER_JACOBI_FP64,
+                                               max_iterations=400000)
+    t0 = time.perf_counter()
+    ref_form = hip.solve_depth_averaged_port_flow(grid, ports, w.mobility, fp64)
+    fp64_rate = 1.0 / (time.perf_counter() - t0)
+    last = runs[-1]
+    speeds = [round(p.mean_speed, 6) for p in last.ports]
+    log(f"port flow: multigrid {mg_rate:.2f} flows/s ({last.report.iterations} it over "
+        f"{last.report.solves} solves), fp64 {fp64_rate:.3f} flows/s "
+        f"({ref_form.report.iterations} it), port mean speeds {speeds}")
+    out = {
+        "workload": f"same channel, ports: x_lower halves at speeds 1 and 2, x_upper at p=0",
+        "solves_per_flow": last.report.solves,
+        "flows_per_s_multigrid": round(mg_rate, 3),
+        "multigrid_iterations": last.report.iterations,
+        "multigrid_true_relative_residual": last.report.true_relative_residual,
+        "flows_per_s_fp64_jacobi_pcg": round(fp64_rate, 4),
+        "fp64_iterations": ref_form.report.iterations,
+        "port_mean_speeds": speeds,
+        "port_pressures": [round(p.pressure, 4) for p in last.ports],
+        "max_speed": last.report.max_speed,
+    }
+    if cpu_seconds_per_iteration:
+        cpu_s = cpu_seconds_per_iteration * ref_form.report.iterations
+        out["cpu_reference_flows_per_s"] = round(1.0 / cpu_s, 5)
+        out["cpu_reference_note"] = ("seconds per FP64 iteration of the reference's CPU solver "
+                                     "on this system x the FP64 iterations of both solves")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +252,7 @@ def main():
                     help="profiling run: only N iterations of each solver form, then exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="profiling calibration: copy this many MiB device to device, 3 times")
+    ap.add_argument("--no-ports", action="store_true", help="skip the port-driven flow")
     ap.add_argument("--batch-steps", type=int, default=50,
                     help="graph-captured steps of the 256-instance batched sweep (0: skip)")
     args = ap.parse_args()
@@ -464,6 +514,9 @@ def main():
         adr_cpu = cpu_adr_baseline(w, faces, threads, steps=3)
         if adr_cpu is not None:
             line["cpu_baseline_adr"] = adr_cpu
+    if not args.no_ports:
+        per_iteration = line.get("cpu_baseline", {}).get("seconds_per_iteration")
+        line["port_flow"] = port_flow_bench(msa, w, hip, per_iteration)
     hip.close()
     print(json.dumps(line), flush=True)
 

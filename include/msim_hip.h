@@ -278,7 +278,22 @@ typedef struct {
   uint32_t advection_scheme; /* MSIM_UPWIND or MSIM_VAN_LEER (SignalAdvectionKind, signals.hpp:41-47) */
   const msim_boundary_patch* patches; /* reservoir patches of fixed faces, or NULL */
   uint32_t patch_count;
+  /* SignalIntegrationKind (signals.hpp:33-37) and, for the implicit kinds,
+   * SignalSolveParameters (signals.hpp:49-55; the reference's defaults are
+   * 10000, 1e-7, 1e-5). The implicit kinds hold FP32 levels. */
+  uint32_t integration; /* MSIM_FORWARD_EULER, MSIM_CRANK_NICOLSON or MSIM_BACKWARD_EULER */
+  uint32_t solver_max_iterations;
+  float solver_absolute_tolerance;
+  float solver_relative_tolerance;
 } msim_signal_grid;
+
+/* SignalIntegrationKind (signals.hpp:33-37). The implicit kinds solve each
+ * step by the reference's damped Jacobi iteration
+ * (signal_grid_crank_nicolson_candidate, signals.cpp:1293-1375) under its
+ * stopping rule (solve_signal_jacobi, signal_jacobi.hpp:53-84); Monod uptake
+ * enters them as the loss vmax*phi/(K+c) of the step's starting levels, added
+ * to any affine loss. */
+enum { MSIM_FORWARD_EULER = 0, MSIM_CRANK_NICOLSON = 1, MSIM_BACKWARD_EULER = 2 };
 
 /* SignalAdvectionKind (signals.hpp:41-47): first-order upwind, or upwind plus
  * the van Leer limited correction (signals.cpp:1227-1286), whose explicit
@@ -293,10 +308,13 @@ typedef struct {
   float residual_rms;
 } msim_signal_report;
 
-/* ComputeBackend::advance_signal_grid (backend.hpp:30) for the explicit
- * (forward Euler, upwind) step: levels S*N in place. The step is checked
+/* ComputeBackend::advance_signal_grid (backend.hpp:30, advance_signal_grid_cpu,
+ * signals.cpp:1377-1389): levels S*N in place. A forward-Euler step is checked
  * against the explicit stability bound first (SignalGrid::validate_step,
- * signals.cpp:1066-1102); a refused or failed step leaves levels unchanged. */
+ * signals.cpp:1066-1102); an implicit step that does not converge fails with
+ * MSIM_NOT_CONVERGED and the reference's message. report: the implicit
+ * solve's (converged, iterations, residual RMS); {1, 0, 0} for forward Euler.
+ * A refused or failed step leaves levels unchanged. */
 int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, float dt,
                                  float* levels, msim_signal_report* report);
 
@@ -304,7 +322,8 @@ int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, fl
  *  adr_upload   -- spec arrays and initial levels (float when precision is
  *                  32, double when 64). With use_flow_field != 0 the velocity
  *                  field is the one the last flow solve left on the device.
- *  adr_run      -- `steps` explicit steps of dt. ledger (steps*S*4 doubles,
+ *  adr_run      -- `steps` steps of dt of the spec's integration kind (the
+ *                  implicit kinds need precision 32). ledger (steps*S*4 doubles,
  *                  or NULL) receives per step and signal: mass before, mass
  *                  after, boundary supply and uptake, each in FP64 (mass =
  *                  sum of levels x voxel volume).
@@ -316,6 +335,11 @@ int msim_hip_adr_download(msim_ctx* ctx, void* levels);
 
 /* Device time (seconds) of the kernels of the last adr_run. */
 double msim_hip_adr_seconds(msim_ctx* ctx);
+
+/* The SignalSolveReport (signals.hpp:57-61) of the last implicit step that
+ * adr_run took ({1, 0, 0} after forward-Euler steps). No counterpart in the
+ * reference, whose advance returns it per step. */
+int msim_hip_adr_report(msim_ctx* ctx, msim_signal_report* report);
 
 /* ---------------------------------------------------------------------------
  * Learned smoother of the LEARNED solver (no counterpart in the reference,

@@ -24,6 +24,7 @@ NO_FLUX, PERIODIC, FIXED = 0, 1, 2
 SOLVER_JACOBI_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_MULTIGRID = 0, 1, 2, 3
 UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind (signals.hpp:41-47)
 SPEED, PRESSURE = 0, 1  # FlowPortKind (flow.hpp:69-74)
+FORWARD_EULER, CRANK_NICOLSON, BACKWARD_EULER = 0, 1, 2  # SignalIntegrationKind
 X_LOWER, X_UPPER, Y_LOWER, Y_UPPER, Z_LOWER, Z_UPPER = range(6)  # GridFace
 
 
@@ -140,7 +141,9 @@ class _SignalGrid(C.Structure):
                 ("loss_rates", C.c_void_p), ("phi", C.c_void_p), ("monod_vmax", C.c_void_p),
                 ("monod_k", C.c_void_p), ("revision", C.c_uint64),
                 ("advection_scheme", C.c_uint32), ("patches", C.c_void_p),
-                ("patch_count", C.c_uint32)]
+                ("patch_count", C.c_uint32), ("integration", C.c_uint32),
+                ("solver_max_iterations", C.c_uint32), ("solver_absolute_tolerance", C.c_float),
+                ("solver_relative_tolerance", C.c_float)]
 
 
 class _TrainSet(C.Structure):
@@ -185,7 +188,7 @@ EXPORTS = (
     "msim_hip_learned_apply", "msim_hip_batch_upload", "msim_hip_batch_solve_flow",
     "msim_hip_batch_autotune", "msim_hip_batch_shapes", "msim_hip_batch_shape",
     "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
-    "msim_hip_solve_depth_averaged_port_flow",
+    "msim_hip_solve_depth_averaged_port_flow", "msim_hip_adr_report",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -239,6 +242,7 @@ def library(path: str = LIBRARY):
         "msim_hip_batch_run": (I, [P, C.c_float, U, U, P, P]),
         "msim_hip_batch_download": (I, [P, U, P, P, P, P]),
         "msim_hip_solve_depth_averaged_port_flow": (I, [P, P, P, P, U, P, P, P, P, P, P]),
+        "msim_hip_adr_report": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -466,6 +470,11 @@ class SignalGridSpec:
     advection_scheme: int = 0  # UPWIND or VAN_LEER (SignalAdvectionKind)
     # GridBoundaryPatch: (face, (begin0, begin1), (end0, end1), values per signal)
     patches: tuple = ()
+    # SignalIntegrationKind and SignalSolveParameters (signals.hpp:33-55).
+    integration: int = FORWARD_EULER
+    solver_max_iterations: int = 10_000
+    solver_absolute_tolerance: float = 1.0e-7
+    solver_relative_tolerance: float = 1.0e-5
 
     @property
     def signal_count(self) -> int:
@@ -497,6 +506,10 @@ class SignalGridSpec:
             setattr(s, name, arr.ctypes.data)
         s.revision = self.revision
         s.advection_scheme = self.advection_scheme
+        s.integration = self.integration
+        s.solver_max_iterations = self.solver_max_iterations
+        s.solver_absolute_tolerance = self.solver_absolute_tolerance
+        s.solver_relative_tolerance = self.solver_relative_tolerance
         if len(self.patches):
             table = (_BoundaryPatch * len(self.patches))()
             for k, (face, begin, end, values) in enumerate(self.patches):
@@ -731,6 +744,12 @@ class HipBackend:
 
     def adr_seconds(self) -> float:
         return float(self.lib.msim_hip_adr_seconds(self.ctx))
+
+    def adr_report(self) -> SignalSolveReport:
+        """The report of the last implicit step adr_run took."""
+        rep = _SignalReport()
+        self._check(self.lib.msim_hip_adr_report(self.ctx, C.byref(rep)))
+        return SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
 
     # -- learned smoother (no counterpart in the reference) ----------------------
     def learned_set_weights(self, weights):

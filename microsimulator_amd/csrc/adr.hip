@@ -576,6 +576,11 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
       P.monod_k[s] = grid.monod_k[s];
     }
   }
+  a.integration = grid.integration;
+  a.max_iterations = grid.solver_max_iterations;
+  a.absolute_tolerance = grid.solver_absolute_tolerance;
+  a.relative_tolerance = grid.solver_relative_tolerance;
+  a.last_report = msim_signal_report{1, 0, 0.0f};
   P.has_field = (use_flow_field || grid.x_faces != nullptr) ? 1 : 0;
   P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
   const bool patched = grid.patches != nullptr && grid.patch_count > 0;
@@ -784,6 +789,15 @@ void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const
   }
 }
 
+void launch_settle_f32(Context& ctx, float* levels, size_t count, const double* largest,
+                       uint32_t parts, uint32_t step) {
+  const uint32_t blocks =
+      static_cast<uint32_t>(std::min<size_t>((count + kAdrBlock - 1) / kAdrBlock, 2048));
+  k_adr_settle<float><<<blocks, kAdrBlock, 0, ctx.stream>>>(levels, count, largest, parts,
+                                                            ctx.adr.flags.data(), step);
+  check(hipGetLastError(), "launch k_adr_settle");
+}
+
 void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
   auto& a = ctx.adr;
   const auto& P = a.params;
@@ -793,6 +807,17 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
   check(hipMemcpyAsync(a.flags.data(), init_flags, sizeof(init_flags), hipMemcpyHostToDevice,
                        ctx.stream),
         "clear flags");
+  if (a.integration != MSIM_FORWARD_EULER) {
+    // Implicit steps: FP32 levels, each step's solve judged on the host.
+    check(hipEventRecord(ctx.start, ctx.stream), "event");
+    implicit_run(ctx, dt, steps, xf, yf);
+    check(hipEventRecord(ctx.stop, ctx.stream), "event");
+    check(hipEventSynchronize(ctx.stop), "event sync");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+    a.seconds = ms * 1.0e-3;
+    return;
+  }
   if (a.precision == 64) a.ledger.reserve(static_cast<size_t>(steps) * P.signals * 4 + 1);
   check(hipEventRecord(ctx.start, ctx.stream), "event");
   switch (P.signals) {

@@ -108,19 +108,17 @@ struct Limiter {
   T far_lo[2], far_hi[2];
 };
 
-// Forward-Euler update of signal s at one site from its neighbour values:
-// the reference's per-site rate in its own float order
-// (cpp/core/signals.cpp:1145-1291 transport with the van Leer correction when
-// kLimited, 305-320 affine reaction, 1109-1134 forward Euler), with Monod
-// uptake as the affine loss vmax*phi/(K+c) it equals under forward Euler.
-// lower/upper hold the x and y neighbour values with exterior values already
-// resolved (exterior()); a closed face replaces its neighbour by the site's
-// own value here.
+// The transport rate of signal s at one site from its neighbour values
+// (transport_rates_into, cpp/core/signals.cpp:1145-1291, with the van Leer
+// correction when kLimited) in the reference's float order. lower/upper hold
+// the x and y neighbour values with exterior values already resolved
+// (exterior()); a closed face replaces its neighbour by the site's own value
+// here.
 template <bool kLimited = false>
-__device__ inline float fe_update_f32(const AdrParams& P, const SiteGeometry& g, float current,
-                                      float lower0, float upper0, float lower1, float upper1,
-                                      int s, float dt, float source, float loss, float phi,
-                                      const Limiter<float>* lim = nullptr) {
+__device__ inline float transport_rate_f32(const AdrParams& P, const SiteGeometry& g,
+                                           float current, float lower0, float upper0,
+                                           float lower1, float upper1, int s,
+                                           const Limiter<float>* lim = nullptr) {
   const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
   const uint32_t dims[2] = {P.nx, P.ny};
   float lower[2] = {lower0, lower1}, upper[2] = {upper0, upper1};
@@ -163,12 +161,81 @@ __device__ inline float fe_update_f32(const AdrParams& P, const SiteGeometry& g,
       rate -= (upper_correction - lower_correction) * inverse_spacing;
     }
   }
+  return rate;
+}
+
+// Forward-Euler update of signal s at one site: the transport rate, the
+// affine reaction (signals.cpp:305-320) and forward Euler (1109-1134), with
+// Monod uptake as the affine loss vmax*phi/(K+c) it equals under forward
+// Euler.
+template <bool kLimited = false>
+__device__ inline float fe_update_f32(const AdrParams& P, const SiteGeometry& g, float current,
+                                      float lower0, float upper0, float lower1, float upper1,
+                                      int s, float dt, float source, float loss, float phi,
+                                      const Limiter<float>* lim = nullptr) {
+  float rate = transport_rate_f32<kLimited>(P, g, current, lower0, upper0, lower1, upper1, s, lim);
   if (P.has_affine != 0) rate += source - (loss * current);
   if (P.has_monod != 0) {
     const float uptake = P.vmax[s] * phi / (P.monod_k[s] + current);
     rate += 0.0f - (uptake * current);
   }
   return current + (dt * rate);
+}
+
+// The diagonal of the implicit operator for signal s at a fluid site
+// (signal_operator_diagonal, signals.cpp:263-298): per axis of extent above
+// one, -2 D/h^2, D/h^2 back for each closed face, and the outflow through
+// each open face; less the affine loss.
+__device__ inline float operator_diagonal_f32(const AdrParams& P, const SiteGeometry& g, int s,
+                                              bool reaction, float loss) {
+  const float ih[2] = {1.0f / P.dx, 1.0f / P.dy};
+  const uint32_t dims[2] = {P.nx, P.ny};
+  float diagonal = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    if (dims[a] == 1) continue;
+    const float diffusion_scale = P.diffusion[s] * ih[a] * ih[a];
+    diagonal -= 2.0f * diffusion_scale;
+    if (g.clo[a]) diagonal += diffusion_scale;
+    if (g.cup[a]) diagonal += diffusion_scale;
+    if (!g.cup[a] && g.uup[a] > 0.0f) diagonal -= g.uup[a] * ih[a];
+    if (!g.clo[a] && g.ulo[a] < 0.0f) diagonal += g.ulo[a] * ih[a];
+  }
+  if (reaction) diagonal -= loss;
+  return diagonal;
+}
+
+// Which neighbours the limited correction of the site (x, y) may use, and
+// the levels two sites away, read from one signal's levels lv and the mask
+// (signals.cpp:1233-1253): a face takes the correction only where its upwind,
+// downwind and second-upwind sites are fluid sites joined by open faces; a
+// periodic axis wraps only when it holds at least three sites.
+__device__ inline void site_limiter(const AdrParams& P, const uint8_t* __restrict__ solid,
+                                    const SiteGeometry& g, const float* __restrict__ lv,
+                                    uint32_t x, uint32_t y, Limiter<float>& lim) {
+  const uint32_t nx = P.nx, ny = P.ny;
+  const bool xp3 = P.kind[0] == MSIM_PERIODIC && nx >= 3;
+  const bool yp3 = P.kind[2] == MSIM_PERIODIC && ny >= 3;
+  lim.lower_site[0] = !g.clo[0] && (x > 0 || xp3);
+  lim.upper_site[0] = !g.cup[0] && (x + 1 < nx || xp3);
+  lim.lower_site[1] = !g.clo[1] && (y > 0 || yp3);
+  lim.upper_site[1] = !g.cup[1] && (y + 1 < ny || yp3);
+  auto far = [&](int axis, int offset, float& value) {
+    const int64_t count = axis == 0 ? nx : ny;
+    const bool periodic = axis == 0 ? xp3 : yp3;
+    const int64_t along = static_cast<int64_t>(axis == 0 ? x : y) + offset;
+    value = 0.0f;
+    if (!periodic && (along < 0 || along >= count)) return false;
+    const uint32_t at = static_cast<uint32_t>((along + count) % count);
+    const size_t site = axis == 0 ? static_cast<size_t>(at) * ny + y : static_cast<size_t>(x) * ny + at;
+    if (solid[site] != 0) return false;
+    value = lv[site];
+    return true;
+  };
+  lim.far_lo_ok[0] = far(0, -2, lim.far_lo[0]);
+  lim.far_hi_ok[0] = far(0, 2, lim.far_hi[0]);
+  lim.far_lo_ok[1] = far(1, -2, lim.far_lo[1]);
+  lim.far_hi_ok[1] = far(1, 2, lim.far_hi[1]);
 }
 
 // The same update reading the neighbours from the level array lv (one

@@ -417,10 +417,15 @@ PortPlan plan_ports(const msim_grid& g, const float* mobility, const msim_flow_p
     reduced[p].begin[1] = 0;
     reduced[p].end[1] = 1;
   }
-  const uint32_t columns[3] = {g.nx, g.ny, 1};
-  const uint8_t column_kinds[6] = {g.boundary[0], g.boundary[1], g.boundary[2], g.boundary[3],
-                                   MSIM_NO_FLUX, MSIM_NO_FLUX};
-  validate_ports(columns, column_kinds, c.solid, reduced.data(), count);
+  // On one layer whose every fluid site carries conductance the grid of
+  // columns is the grid itself, and the ports on it are the same ports: the
+  // checks there would repeat the ones above exactly.
+  if (nz > 1 || c.solid != mask) {
+    const uint32_t columns[3] = {g.nx, g.ny, 1};
+    const uint8_t column_kinds[6] = {g.boundary[0], g.boundary[1], g.boundary[2],
+                                     g.boundary[3], MSIM_NO_FLUX,   MSIM_NO_FLUX};
+    validate_ports(columns, column_kinds, c.solid, reduced.data(), count);
+  }
   PortPlan plan;
   plan.ports.assign(ports, ports + count);
   const size_t slots = 2 * (static_cast<size_t>(g.nx) + g.ny);
@@ -517,8 +522,18 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
     throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes a grid of one layer (nz = 1)");
   }
   if (s.signal_count == 0) invalid("signal grid must contain at least one signal");
+  if (s.integration > MSIM_BACKWARD_EULER) {
+    invalid("unknown signal integration kind");  // signals.cpp:718-725
+  }
   if (s.advection_scheme != MSIM_UPWIND && s.advection_scheme != MSIM_VAN_LEER) {
     invalid("unknown signal advection scheme");  // signals.cpp:726-732
+  }
+  // SignalSolveParameters::validate (signals.cpp:482-491).
+  if (s.solver_max_iterations == 0) invalid("signal solver iteration limit must be positive");
+  if (!std::isfinite(s.solver_absolute_tolerance) || s.solver_absolute_tolerance < 0.0f ||
+      !std::isfinite(s.solver_relative_tolerance) || s.solver_relative_tolerance < 0.0f ||
+      (s.solver_absolute_tolerance == 0.0f && s.solver_relative_tolerance == 0.0f)) {
+    invalid("signal solver tolerances must be finite and non-negative");
   }
   if (s.signal_count > kMaxSignals) {
     throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes at most 8 signals");
@@ -664,6 +679,8 @@ void record_losses(Context& ctx, const msim_signal_grid& s) {
 // SignalGrid::validate_step (signals.cpp:1066-1102).
 void validate_step(const Context& ctx, float dt) {
   if (!std::isfinite(dt) || dt < 0.0f) invalid("time step must be finite and non-negative");
+  // The explicit bound binds forward Euler only (signals.cpp:1070-1072).
+  if (ctx.adr.integration != MSIM_FORWARD_EULER) return;
   const auto& P = ctx.adr.params;
   const uint32_t dims[2] = {P.nx, P.ny};
   const float spacing[2] = {P.dx, P.dy};
@@ -1060,6 +1077,7 @@ int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, fl
     if (report != nullptr) *report = msim_signal_report{1, 0, 0.0f};
     if (dt == 0.0f) return;
     msim::adr_run(c, dt, 1, nullptr);
+    if (report != nullptr) *report = c.adr.last_report;
     msim::adr_download(c, levels);
   });
 }
@@ -1069,6 +1087,9 @@ int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void*
   return msim::guarded(ctx, [&](Context& c) {
     if (grid == nullptr || levels == nullptr) msim::invalid("grid and levels must not be null");
     if (precision != 32 && precision != 64) msim::invalid("precision must be 32 or 64");
+    if (precision == 64 && grid->integration != MSIM_FORWARD_EULER) {
+      throw Error(MSIM_UNSUPPORTED, "implicit signal steps hold FP32 levels");
+    }
     if (use_flow_field != 0) {
       if (!c.flow.solved) msim::invalid("no flow field on the device");
       if (c.flow.layers.nz != 1) msim::invalid("the flow field on the device is of a layered grid");
@@ -1106,6 +1127,13 @@ int msim_hip_adr_download(msim_ctx* ctx, void* levels) {
 
 double msim_hip_adr_seconds(msim_ctx* ctx) {
   return ctx == nullptr ? 0.0 : ctx->impl.adr.seconds;
+}
+
+int msim_hip_adr_report(msim_ctx* ctx, msim_signal_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (report == nullptr) msim::invalid("report must not be null");
+    *report = c.adr.last_report;
+  });
 }
 
 }  // extern "C"

@@ -226,8 +226,26 @@ struct AdrParams {
   float monod_k[kMaxSignals];
 };
 
+// The state of one implicit step's Jacobi solve on the device.
+struct ImplicitState {
+  float rhs_rms;    // RMS of the right-hand side
+  float scale;      // the implicit operator's scale
+  float threshold;  // the stopping rule's threshold, set after iterate 0
+  float rms;        // residual RMS of the last iterate judged
+  uint32_t halted;  // the rule was met, or the solve cannot go on
+  uint32_t converged;
+  uint32_t iteration;  // the last iterate judged
+  uint32_t nonfinite;  // an iterate held a non-finite level
+};
+
 struct AdrDevice {
   AdrParams params{};
+  // SignalIntegrationKind and SignalSolveParameters of the uploaded spec;
+  // the report of the last implicit step.
+  uint32_t integration = MSIM_FORWARD_EULER;
+  uint32_t max_iterations = 10000;
+  float absolute_tolerance = 1.0e-7f, relative_tolerance = 1.0e-5f;
+  msim_signal_report last_report{1, 0, 0.0f};
   uint32_t precision = 32;
   bool uploaded = false;
   bool flow_field = false;
@@ -239,6 +257,12 @@ struct AdrDevice {
   DeviceBuffer<float> xf, yf, phi, source, loss, edge;
   bool has_edge = false;
   DeviceBuffer<uint8_t> levels_a, levels_b;  // S*N of float or double
+  // Implicit steps: a third level buffer, the right-hand side, the lagged
+  // Monod loss, per-block partials and the solve's state.
+  DeviceBuffer<uint8_t> levels_c;
+  DeviceBuffer<float> rhs, lagged, implicit_least;
+  DeviceBuffer<double> implicit_sums;
+  DeviceBuffer<ImplicitState> implicit_state;
   int current = 0;
   DeviceBuffer<double> partials, ledger, mass;
   DeviceBuffer<uint32_t> flags;              // [0]: error bits, [1]: first failed step
@@ -343,10 +367,10 @@ void flow_apply(Context& ctx, const double* x, double* y);
 
 // ports.hip: the lift onto voxel faces and port-driven flow.
 // The solved field on the original grid's x|y|z faces (device `out`, of
-// flow.voxel_faces() floats): one layer copies the face velocities; several
-// layers lift the column flux (ShallowFlowReduction::lift,
-// flow_system.hpp:424-470).
-void flow_lift(Context& ctx, float* out);
+// flow.voxel_faces() floats): one layer copies the face velocities (and
+// zeroes the z faces when `with_z`); several layers lift the column flux
+// (ShallowFlowReduction::lift, flow_system.hpp:424-470).
+void flow_lift(Context& ctx, float* out, bool with_z = true);
 // Uploads the voxel mask and the column heights a lift of several layers reads.
 void flow_upload_layers(Context& ctx);
 // A checked port-driven flow (ShallowPortFlow, flow_system.hpp:1268-1311):
@@ -385,5 +409,14 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
                 uint32_t precision, bool use_flow_field);
 void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger);
 void adr_download(Context& ctx, void* levels);
+// The settle and validation of SignalGrid::accept_step (signals.cpp:979-1011)
+// on `count` FP32 levels in place, with the largest level from `parts`
+// per-block maxima (0: no settle, validation only); a failure sets the
+// transport flags with `step`.
+void launch_settle_f32(Context& ctx, float* levels, size_t count, const double* largest,
+                       uint32_t parts, uint32_t step);
+
+// implicit.hip: `steps` implicit steps of dt on the uploaded FP32 levels.
+void implicit_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf);
 
 }  // namespace msim

@@ -170,7 +170,7 @@ void flow_upload_layers(Context& ctx) {
   check(hipStreamSynchronize(ctx.stream), "upload layers");
 }
 
-void flow_lift(Context& ctx, float* out) {
+void flow_lift(Context& ctx, float* out, bool with_z) {
   auto& f = ctx.flow;
   const uint32_t nz = f.layers.nz;
   if (nz == 1) {
@@ -179,8 +179,10 @@ void flow_lift(Context& ctx, float* out) {
     check(hipMemcpyAsync(out, f.faces.data(), planar * sizeof(float), hipMemcpyDeviceToDevice,
                          ctx.stream),
           "copy face velocities");
-    check(hipMemsetAsync(out + planar, 0, 2 * f.sites() * sizeof(float), ctx.stream),
-          "clear z faces");
+    if (with_z) {
+      check(hipMemsetAsync(out + planar, 0, 2 * f.sites() * sizeof(float), ctx.stream),
+            "clear z faces");
+    }
     return;
   }
   const size_t planar = (f.x_faces() + f.y_faces()) * nz;
@@ -212,7 +214,11 @@ void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_pa
   }
   const uint32_t first_driven = holds_pressure ? 1u : 0u;
   const uint32_t J = first_driven + static_cast<uint32_t>(driven.size());
-  const size_t F = f.voxel_faces();
+  // A slot holds every face of the field; on one layer the vertical faces
+  // are zero in every solve (a port cannot open a z face), so a slot holds
+  // the x and y faces only.
+  const size_t nz = f.layers.nz;
+  const size_t F = nz == 1 ? f.x_faces() + f.y_faces() : f.voxel_faces();
   f.port_fields.reserve(static_cast<size_t>(J) * F);
 
   msim_flow_params solve{};
@@ -235,7 +241,7 @@ void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_pa
     flow_hold_pressures(ctx, edge.data());
     msim_flow_report r{};
     flow_solve(ctx, solve, r);
-    flow_lift(ctx, f.port_fields.data() + static_cast<size_t>(slot) * F);
+    flow_lift(ctx, f.port_fields.data() + static_cast<size_t>(slot) * F, nz > 1);
     ++report.solves;
     report.iterations += r.iterations;
     report.relative_residual =
@@ -368,7 +374,6 @@ void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_pa
   }
   float max_speed = 0.0f;
   std::memcpy(&max_speed, &peak_bits, sizeof(float));
-  const size_t nz = f.layers.nz;
   const size_t nxf = f.x_faces() * nz, nyf = f.y_faces() * nz;
   const float* out = f.lifted.data();
   if (x_faces != nullptr) {
@@ -380,7 +385,9 @@ void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_pa
                          ctx.stream),
           "download y faces");
   }
-  if (z_faces != nullptr) {
+  if (z_faces != nullptr && nz == 1) {
+    std::memset(z_faces, 0, 2 * f.sites() * sizeof(float));
+  } else if (z_faces != nullptr) {
     check(hipMemcpyAsync(z_faces, out + nxf + nyf, (F - nxf - nyf) * sizeof(float),
                          hipMemcpyDeviceToHost, ctx.stream),
           "download z faces");

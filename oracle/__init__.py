@@ -25,6 +25,8 @@ REF_SO = os.path.join(HERE, "_ref", "libmsim_ref.so")
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
 UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind
 SPEED, PRESSURE = 0, 1  # FlowPortKind
+FORWARD_EULER, CRANK_NICOLSON, BACKWARD_EULER = 0, 1, 2  # SignalIntegrationKind
+SOLVER_DEFAULTS = (10_000, 1.0e-7, 1.0e-5)  # SignalSolveParameters (signals.hpp:49-55)
 
 STATUS = {
     0: "ok", 1: "invalid_argument", 2: "runtime_error", 3: "not_converged", 4: "overflow",
@@ -98,7 +100,14 @@ class _SignalGrid(C.Structure):
                 ("source_rates", C.c_void_p), ("loss_rates", C.c_void_p), ("phi", C.c_void_p),
                 ("monod_vmax", C.c_void_p), ("monod_k", C.c_void_p),
                 ("advection_scheme", C.c_uint32), ("patches", C.c_void_p),
-                ("patch_count", C.c_uint32)]
+                ("patch_count", C.c_uint32), ("integration", C.c_uint32),
+                ("solver_max_iterations", C.c_uint32), ("solver_absolute_tolerance", C.c_float),
+                ("solver_relative_tolerance", C.c_float)]
+
+
+class _SignalReport(C.Structure):
+    _fields_ = [("converged", C.c_uint32), ("iterations", C.c_uint32),
+                ("residual_rms", C.c_float)]
 
 
 def _ptr(a):
@@ -170,6 +179,8 @@ class Oracle:
         L.mo_adr_validate_step.argtypes = [P, F, C.c_char_p, Z]
         L.mo_adr_step_f32.argtypes = [P, F, P, C.c_char_p, Z]
         L.mo_adr_step_f64.argtypes = [P, F, P, P, C.c_char_p, Z]
+        L.mo_adr_step_implicit_f32.argtypes = [P, F, P, P, C.c_char_p, Z]
+        L.mo_adr_step_implicit_f32.restype = C.c_int
         for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_adr_validate_step,
                    L.mo_adr_step_f32, L.mo_adr_step_f64):
             fn.restype = C.c_int
@@ -252,11 +263,16 @@ class Oracle:
 
     def _signal_grid(self, grid: Grid, keep: list, diffusion, boundary_values=None,
                      faces=None, advection=None, source=None, loss=None, phi=None,
-                     vmax=None, monod_k=None, scheme=UPWIND, patches=()):
-        """patches: (face, (begin0, begin1), (end0, end1), values per signal)."""
+                     vmax=None, monod_k=None, scheme=UPWIND, patches=(),
+                     integration=FORWARD_EULER, solver=SOLVER_DEFAULTS):
+        """patches: (face, (begin0, begin1), (end0, end1), values per signal);
+        solver: (max_iterations, absolute_tolerance, relative_tolerance)."""
         sg = _SignalGrid()
         sg.grid = grid._c(keep)
         sg.advection_scheme = scheme
+        sg.integration = integration
+        (sg.solver_max_iterations, sg.solver_absolute_tolerance,
+         sg.solver_relative_tolerance) = solver
         if len(patches):
             table = (_BoundaryPatch * len(patches))()
             for k, (face, begin, end, values) in enumerate(patches):
@@ -314,6 +330,21 @@ class Oracle:
             self._check(st, err)
         return lv
 
+    def adr_step_implicit(self, grid: Grid, levels, dt: float, steps: int = 1, **kw):
+        """Implicit steps (integration=CRANK_NICOLSON or BACKWARD_EULER in kw):
+        levels after them and the last step's report."""
+        keep: list = []
+        sg = self._signal_grid(grid, keep, **kw)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        rep = _SignalReport()
+        err = C.create_string_buffer(512)
+        for _ in range(steps):
+            st = self.lib.mo_adr_step_implicit_f32(C.byref(sg), C.c_float(dt), lv.ctypes.data,
+                                                   C.byref(rep), err, C.c_size_t(512))
+            self._check(st, err)
+        return lv, dict(converged=bool(rep.converged), iterations=rep.iterations,
+                        residual_rms=rep.residual_rms)
+
     def adr_step_f64(self, grid: Grid, levels, dt: float, steps: int = 1, **kw):
         keep: list = []
         sg = self._signal_grid(grid, keep, **kw)
@@ -348,7 +379,7 @@ class Reference:
         L.ref_solve_depth_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, P, P, P,
                                            C.c_char_p, Z]
         L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, U,
-                                              U, P, P, F, U, P, P, C.c_char_p, Z]
+                                              U, P, P, U, U, F, F, F, U, P, P, P, C.c_char_p, Z]
         L.ref_set_threads.argtypes = [U]
         L.ref_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, P, P, P, P, P, P,
                                     C.c_char_p, Z]
@@ -420,7 +451,10 @@ class Reference:
 
     def advance_signal_grid(self, grid: Grid, levels, dt, steps=1, diffusion=(1.0,),
                             boundary_values=None, faces=None, source=None, loss=None,
-                            scheme=UPWIND, patches=()):
+                            scheme=UPWIND, patches=(), integration=FORWARD_EULER,
+                            solver=SOLVER_DEFAULTS, with_report=False):
+        """advance_signal_grid_cpu `steps` times: levels and seconds (and the
+        last step's report with with_report)."""
         kinds = self._kinds(grid)
         obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
         diffusion = _f32(np.atleast_1d(diffusion))
@@ -441,6 +475,7 @@ class Reference:
         pvalues = (np.concatenate([_f32(np.atleast_1d(v)) for *_, v in patches])
                    if len(patches) else np.zeros(1, np.float32))
         secs = C.c_double()
+        report = np.zeros(3, np.float32)
         err = C.create_string_buffer(512)
         st = self.lib.ref_advance_signal_grid(
             C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
@@ -450,9 +485,15 @@ class Reference:
             C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)), C.c_void_p(_ptr(loss)),
             C.c_uint32(scheme), C.c_uint32(len(patches)),
             C.c_void_p(_ptr(geometry) if len(patches) else None),
-            C.c_void_p(_ptr(pvalues) if len(patches) else None), C.c_float(dt), C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.byref(secs), err, C.c_size_t(512))
+            C.c_void_p(_ptr(pvalues) if len(patches) else None), C.c_uint32(integration),
+            C.c_uint32(solver[0]), C.c_float(solver[1]), C.c_float(solver[2]), C.c_float(dt),
+            C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.c_void_p(report.ctypes.data),
+            C.byref(secs), err, C.c_size_t(512))
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
+        if with_report:
+            return lv, secs.value, dict(converged=bool(report[0]), iterations=int(report[1]),
+                                        residual_rms=float(report[2]))
         return lv, secs.value
 
 

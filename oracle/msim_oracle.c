@@ -1500,7 +1500,81 @@ static int settle_f32(float* levels, size_t count, char* err, size_t errlen) {
   return MO_OK;
 }
 
+/* The transport rate of one fluid level (transport_rates_into,
+ * cpp/core/signals.cpp:1170-1286): diffusion with the boundary values, the
+ * upwind flux of each face and, under van Leer, the limited correction. lv:
+ * the signal's levels. */
+static float transport_rate_at(const mo_signal_grid* sg, const float* lv, uint32_t s, uint32_t x,
+                               uint32_t y, uint32_t z) {
+  const mo_grid* g = &sg->grid;
+  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+  const float spacing[3] = {g->dx, g->dy, g->dz};
+  const size_t site = site_of(g, x, y, z);
+  const float diffusion = sg->diffusion[s];
+  const float current = lv[site];
+  float flo[3], fup[3];
+  int clo[3], cup[3];
+  face_velocities(sg, s, x, y, z, flo, fup);
+  face_closure(g, x, y, z, clo, cup);
+  float lower[3], upper[3];
+  lower[0] = x == 0 ? boundary_value(sg, 0, s, current, lv[site_of(g, g->nx - 1, y, z)], y, z)
+                    : lv[site_of(g, x - 1, y, z)];
+  upper[0] = x + 1 == g->nx ? boundary_value(sg, 1, s, current, lv[site_of(g, 0, y, z)], y, z)
+                            : lv[site_of(g, x + 1, y, z)];
+  lower[1] = y == 0 ? boundary_value(sg, 2, s, current, lv[site_of(g, x, g->ny - 1, z)], x, z)
+                    : lv[site_of(g, x, y - 1, z)];
+  upper[1] = y + 1 == g->ny ? boundary_value(sg, 3, s, current, lv[site_of(g, x, 0, z)], x, z)
+                            : lv[site_of(g, x, y + 1, z)];
+  lower[2] = z == 0 ? boundary_value(sg, 4, s, current, lv[site_of(g, x, y, g->nz - 1)], x, y)
+                    : lv[site_of(g, x, y, z - 1)];
+  upper[2] = z + 1 == g->nz ? boundary_value(sg, 5, s, current, lv[site_of(g, x, y, 0)], x, y)
+                            : lv[site_of(g, x, y, z + 1)];
+  float rate = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] == 1) continue;
+    if (clo[a]) lower[a] = current;
+    if (cup[a]) upper[a] = current;
+    const float inverse_spacing = 1.0f / spacing[a];
+    rate += diffusion * (lower[a] - (2.0f * current) + upper[a]) * inverse_spacing *
+            inverse_spacing;
+    float lower_flux = flo[a] >= 0.0f ? flo[a] * lower[a] : flo[a] * current;
+    float upper_flux = fup[a] >= 0.0f ? fup[a] * current : fup[a] * upper[a];
+    if (clo[a]) lower_flux = 0.0f;
+    if (cup[a]) upper_flux = 0.0f;
+    rate -= (upper_flux - lower_flux) * inverse_spacing;
+    if (sg->advection_scheme != MO_VAN_LEER) continue;
+    /* The limited correction of each face (signals.cpp:1227-1286): taken
+     * only where the upwind, downwind and second-upwind sites are fluid
+     * sites joined by open faces. */
+    const uint32_t coord[3] = {x, y, z};
+    const int periodic = g->boundary[2 * a] == MO_PERIODIC && dims[a] >= 3;
+    const int lower_site = !clo[a] && (coord[a] > 0 || periodic);
+    const int upper_site = !cup[a] && (coord[a] + 1 < dims[a] || periodic);
+    float lower_correction = 0.0f, upper_correction = 0.0f;
+    double beyond = 0.0;
+    if (lower_site) {
+      if (flo[a] >= 0.0f) {
+        if (far_level(sg, lv, NULL, a, x, y, z, -2, periodic, &beyond))
+          lower_correction = flo[a] * van_leer_increment(lower[a], current, (float)beyond);
+      } else if (upper_site) {
+        lower_correction = flo[a] * van_leer_increment(current, lower[a], upper[a]);
+      }
+    }
+    if (upper_site) {
+      if (fup[a] >= 0.0f) {
+        if (lower_site) upper_correction = fup[a] * van_leer_increment(current, upper[a], lower[a]);
+      } else if (far_level(sg, lv, NULL, a, x, y, z, 2, periodic, &beyond)) {
+        upper_correction = fup[a] * van_leer_increment(upper[a], current, (float)beyond);
+      }
+    }
+    rate -= (upper_correction - lower_correction) * inverse_spacing;
+  }
+  return rate;
+}
+
 int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err, size_t errlen) {
+  if (sg->integration != MO_FORWARD_EULER)
+    return mo_adr_step_implicit_f32(sg, dt, levels, NULL, err, errlen);
   int status = validate_grid(&sg->grid, err, errlen);
   if (status != MO_OK) return status;
   status = mo_adr_validate_step(sg, dt, err, errlen);
@@ -1511,8 +1585,6 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
   const size_t total = n * sg->signal_count;
   float* updated = malloc(total * sizeof(float));
   memcpy(updated, levels, total * sizeof(float));
-  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
-  const float spacing[3] = {g->dx, g->dy, g->dz};
   for (uint32_t s = 0; s < sg->signal_count; ++s) {
     const float* lv = levels + (size_t)s * n;
     for (uint32_t x = 0; x < g->nx; ++x)
@@ -1520,70 +1592,12 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
         for (uint32_t z = 0; z < g->nz; ++z) {
           const size_t site = site_of(g, x, y, z);
           if (solid_at(g, site)) continue;
-          const float diffusion = sg->diffusion[s];
           const float current = lv[site];
           if (!isfinite(current)) {
             free(updated);
             return fail(err, errlen, MO_INVALID_ARGUMENT, "signal transport levels must be finite");
           }
-          float flo[3], fup[3];
-          int clo[3], cup[3];
-          face_velocities(sg, s, x, y, z, flo, fup);
-          face_closure(g, x, y, z, clo, cup);
-          float lower[3], upper[3];
-          lower[0] = x == 0 ? boundary_value(sg, 0, s, current, lv[site_of(g, g->nx - 1, y, z)], y, z)
-                            : lv[site_of(g, x - 1, y, z)];
-          upper[0] = x + 1 == g->nx ? boundary_value(sg, 1, s, current, lv[site_of(g, 0, y, z)], y, z)
-                                    : lv[site_of(g, x + 1, y, z)];
-          lower[1] = y == 0 ? boundary_value(sg, 2, s, current, lv[site_of(g, x, g->ny - 1, z)], x, z)
-                            : lv[site_of(g, x, y - 1, z)];
-          upper[1] = y + 1 == g->ny ? boundary_value(sg, 3, s, current, lv[site_of(g, x, 0, z)], x, z)
-                                    : lv[site_of(g, x, y + 1, z)];
-          lower[2] = z == 0 ? boundary_value(sg, 4, s, current, lv[site_of(g, x, y, g->nz - 1)], x, y)
-                            : lv[site_of(g, x, y, z - 1)];
-          upper[2] = z + 1 == g->nz ? boundary_value(sg, 5, s, current, lv[site_of(g, x, y, 0)], x, y)
-                                    : lv[site_of(g, x, y, z + 1)];
-          float rate = 0.0f;
-          for (int a = 0; a < 3; ++a) {
-            if (dims[a] == 1) continue;
-            if (clo[a]) lower[a] = current;
-            if (cup[a]) upper[a] = current;
-            const float inverse_spacing = 1.0f / spacing[a];
-            rate += diffusion * (lower[a] - (2.0f * current) + upper[a]) * inverse_spacing *
-                    inverse_spacing;
-            float lower_flux = flo[a] >= 0.0f ? flo[a] * lower[a] : flo[a] * current;
-            float upper_flux = fup[a] >= 0.0f ? fup[a] * current : fup[a] * upper[a];
-            if (clo[a]) lower_flux = 0.0f;
-            if (cup[a]) upper_flux = 0.0f;
-            rate -= (upper_flux - lower_flux) * inverse_spacing;
-            if (sg->advection_scheme != MO_VAN_LEER) continue;
-            /* The limited correction of each face (signals.cpp:1227-1286):
-             * taken only where the upwind, downwind and second-upwind sites
-             * are fluid sites joined by open faces. */
-            const uint32_t coord[3] = {x, y, z};
-            const int periodic = g->boundary[2 * a] == MO_PERIODIC && dims[a] >= 3;
-            const int lower_site = !clo[a] && (coord[a] > 0 || periodic);
-            const int upper_site = !cup[a] && (coord[a] + 1 < dims[a] || periodic);
-            float lower_correction = 0.0f, upper_correction = 0.0f;
-            double beyond = 0.0;
-            if (lower_site) {
-              if (flo[a] >= 0.0f) {
-                if (far_level(sg, lv, NULL, a, x, y, z, -2, periodic, &beyond))
-                  lower_correction = flo[a] * van_leer_increment(lower[a], current, (float)beyond);
-              } else if (upper_site) {
-                lower_correction = flo[a] * van_leer_increment(current, lower[a], upper[a]);
-              }
-            }
-            if (upper_site) {
-              if (fup[a] >= 0.0f) {
-                if (lower_site)
-                  upper_correction = fup[a] * van_leer_increment(current, upper[a], lower[a]);
-              } else if (far_level(sg, lv, NULL, a, x, y, z, 2, periodic, &beyond)) {
-                upper_correction = fup[a] * van_leer_increment(upper[a], current, (float)beyond);
-              }
-            }
-            rate -= (upper_correction - lower_correction) * inverse_spacing;
-          }
+          float rate = transport_rate_at(sg, lv, s, x, y, z);
           const size_t index = (size_t)s * n + site;
           if (sg->source_rates != NULL)
             rate += sg->source_rates[index] - (sg->loss_rates[index] * current);
@@ -1610,6 +1624,253 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
   memcpy(levels, updated, total * sizeof(float));
   free(updated);
   return MO_OK;
+}
+
+/* The parts of an implicit step the iteration keeps fixed. */
+typedef struct {
+  const mo_signal_grid* sg;
+  size_t n, total, nf;
+  uint32_t* fluid;   /* ascending fluid sites (SignalGridLayout::fluid_sites) */
+  float* source;     /* effective affine source, or NULL without a reaction */
+  float* loss;       /* effective affine loss */
+} implicit_t;
+
+/* operator_rates_into (signals.cpp:305-320): transport plus the affine
+ * reaction of every fluid level of `lv`. Fails where a fluid level is not
+ * finite, as transport_rates_into does (1287-1289). */
+static int implicit_rates(const implicit_t* im, const float* levels, float* rates, char* err,
+                          size_t errlen) {
+  const mo_signal_grid* sg = im->sg;
+  const mo_grid* g = &sg->grid;
+  for (uint32_t s = 0; s < sg->signal_count; ++s) {
+    const float* lv = levels + (size_t)s * im->n;
+    for (size_t k = 0; k < im->nf; ++k) {
+      const size_t site = im->fluid[k];
+      if (!isfinite(lv[site]))
+        return fail(err, errlen, MO_INVALID_ARGUMENT, "signal transport levels must be finite");
+    }
+  }
+  for (uint32_t s = 0; s < sg->signal_count; ++s) {
+    const float* lv = levels + (size_t)s * im->n;
+    for (size_t k = 0; k < im->nf; ++k) {
+      const size_t site = im->fluid[k];
+      const uint32_t x = (uint32_t)(site / ((size_t)g->ny * g->nz));
+      const uint32_t y = (uint32_t)((site / g->nz) % g->ny);
+      const uint32_t z = (uint32_t)(site % g->nz);
+      float rate = transport_rate_at(sg, lv, s, x, y, z);
+      const size_t index = (size_t)s * im->n + site;
+      if (im->loss != NULL) rate += im->source[index] - (im->loss[index] * lv[site]);
+      rates[index] = rate;
+    }
+  }
+  return MO_OK;
+}
+
+/* rms (signals.cpp:332-352): over the fluid levels in their order (signal,
+ * then fluid site), pieces of fluid_grain = 2048 each summed in order and the
+ * pieces combined in order; divided by every level of the grid. */
+static float implicit_rms(const implicit_t* im, const float* values) {
+  const size_t count = (size_t)im->sg->signal_count * im->nf;
+  double sum = 0.0;
+  for (size_t begin = 0; begin < count; begin += 2048) {
+    const size_t end = begin + 2048 < count ? begin + 2048 : count;
+    double piece = 0.0;
+    for (size_t index = begin; index < end; ++index) {
+      const double value =
+          (double)values[(index / im->nf) * im->n + im->fluid[index % im->nf]];
+      piece += value * value;
+    }
+    sum = sum + piece;
+  }
+  return (float)sqrt(sum / (double)im->total);
+}
+
+/* signal_operator_diagonal (signals.cpp:263-298) with the effective loss. */
+static float implicit_diagonal(const implicit_t* im, uint32_t s, size_t site) {
+  const mo_signal_grid* sg = im->sg;
+  const mo_grid* g = &sg->grid;
+  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+  const float spacing[3] = {g->dx, g->dy, g->dz};
+  const uint32_t x = (uint32_t)(site / ((size_t)g->ny * g->nz));
+  const uint32_t y = (uint32_t)((site / g->nz) % g->ny);
+  const uint32_t z = (uint32_t)(site % g->nz);
+  float flo[3], fup[3];
+  int clo[3], cup[3];
+  face_velocities(sg, s, x, y, z, flo, fup);
+  face_closure(g, x, y, z, clo, cup);
+  float diagonal = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] == 1) continue;
+    const float inverse_spacing = 1.0f / spacing[a];
+    const float diffusion_scale = sg->diffusion[s] * inverse_spacing * inverse_spacing;
+    diagonal -= 2.0f * diffusion_scale;
+    if (clo[a]) diagonal += diffusion_scale;
+    if (cup[a]) diagonal += diffusion_scale;
+    if (!cup[a] && fup[a] > 0.0f) diagonal -= fup[a] * inverse_spacing;
+    if (!clo[a] && flo[a] < 0.0f) diagonal += flo[a] * inverse_spacing;
+  }
+  if (im->loss != NULL) diagonal -= im->loss[(size_t)s * im->n + site];
+  return diagonal;
+}
+
+int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
+                             mo_signal_report* report, char* err, size_t errlen) {
+  int status = validate_grid(&sg->grid, err, errlen);
+  if (status != MO_OK) return status;
+  if (sg->integration > MO_BACKWARD_EULER)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "unknown signal integration kind");
+  /* SignalSolveParameters::validate (signals.cpp:482-491). */
+  if (sg->solver_max_iterations == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "signal solver iteration limit must be positive");
+  const float atol = sg->solver_absolute_tolerance, rtol = sg->solver_relative_tolerance;
+  if (!isfinite(atol) || atol < 0.0f || !isfinite(rtol) || rtol < 0.0f ||
+      (atol == 0.0f && rtol == 0.0f))
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "signal solver tolerances must be finite and non-negative");
+  if (!isfinite(dt) || dt < 0.0f)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "time step must be finite and non-negative");
+  mo_signal_report rep = {1, 0, 0.0f};
+  if (report != NULL) *report = rep;
+  if (dt == 0.0f) return MO_OK;
+  const mo_grid* g = &sg->grid;
+  implicit_t im;
+  memset(&im, 0, sizeof(im));
+  im.sg = sg;
+  im.n = (size_t)g->nx * g->ny * g->nz;
+  im.total = im.n * sg->signal_count;
+  im.fluid = malloc((im.n + 1) * sizeof(uint32_t));
+  for (size_t site = 0; site < im.n; ++site)
+    if (!solid_at(g, site)) im.fluid[im.nf++] = (uint32_t)site;
+  const size_t entries = (size_t)sg->signal_count * im.nf;
+  float* rates = calloc(im.total, sizeof(float));
+  float* rhs = calloc(im.total, sizeof(float));
+  float* residual = calloc(im.total, sizeof(float));
+  float* current = malloc(im.total * sizeof(float));
+  float* next = malloc(im.total * sizeof(float));
+  float* diagonal = malloc((entries + 1) * sizeof(float));
+  memcpy(current, levels, im.total * sizeof(float));
+  memcpy(next, levels, im.total * sizeof(float));
+  /* The reaction the iteration holds fixed: the affine arrays, with Monod
+   * uptake as the loss vmax*phi/(K+c) of the starting levels added. */
+  if (sg->loss_rates != NULL || sg->phi != NULL) {
+    im.source = calloc(im.total, sizeof(float));
+    im.loss = calloc(im.total, sizeof(float));
+    for (uint32_t s = 0; s < sg->signal_count; ++s)
+      for (size_t k = 0; k < im.nf; ++k) {
+        const size_t site = im.fluid[k], index = (size_t)s * im.n + site;
+        float loss = sg->loss_rates != NULL ? sg->loss_rates[index] : 0.0f;
+        if (sg->phi != NULL) {
+          const float uptake = monod_loss(sg, s, site, levels[index]);
+          loss = sg->loss_rates != NULL ? loss + uptake : uptake;
+        }
+        im.loss[index] = loss;
+        im.source[index] =
+            sg->loss_rates != NULL && sg->source_rates != NULL ? sg->source_rates[index] : 0.0f;
+      }
+  }
+  /* signal_grid_crank_nicolson_candidate (signals.cpp:1318-1374). */
+  status = implicit_rates(&im, current, rates, err, errlen);
+  if (status != MO_OK) goto done;
+  const int backward = sg->integration == MO_BACKWARD_EULER;
+  const float half_dt = (backward ? 1.0f : 0.5f) * dt;
+  const float explicit_dt = backward ? 0.0f : half_dt;
+  for (uint32_t s = 0; s < sg->signal_count; ++s)
+    for (size_t k = 0; k < im.nf; ++k) {
+      const size_t index = (size_t)s * im.n + im.fluid[k];
+      rhs[index] = current[index] + (explicit_dt * rates[index]) + (dt * 0.0f);
+    }
+  float least = 0.0f;
+  for (uint32_t s = 0; s < sg->signal_count; ++s)
+    for (size_t k = 0; k < im.nf; ++k) {
+      const float d = implicit_diagonal(&im, s, im.fluid[k]);
+      diagonal[(size_t)s * im.nf + k] = d;
+      if (d < least) least = d;
+    }
+  const float scaled = 1.0f - (half_dt * least);
+  const float operator_scale = 1.0f < scaled ? scaled : 1.0f;
+  const float relaxation = sg->advection_scheme == MO_VAN_LEER ? 0.5f : 1.0f;
+  /* residual_rms: the rates of the current iterate, then
+   * rhs - current + half_dt * rates. */
+#define RESIDUAL_RMS(out)                                                         \
+  do {                                                                            \
+    status = implicit_rates(&im, current, rates, err, errlen);                    \
+    if (status != MO_OK) goto done;                                               \
+    for (uint32_t s_ = 0; s_ < sg->signal_count; ++s_)                            \
+      for (size_t k_ = 0; k_ < im.nf; ++k_) {                                     \
+        const size_t i_ = (size_t)s_ * im.n + im.fluid[k_];                       \
+        residual[i_] = rhs[i_] - current[i_] + (half_dt * rates[i_]);            \
+      }                                                                           \
+    (out) = implicit_rms(&im, residual);                                          \
+  } while (0)
+  /* solve_signal_jacobi (signal_jacobi.hpp:53-84). */
+  RESIDUAL_RMS(rep.residual_rms);
+  {
+    const float floor = FLT_EPSILON * implicit_rms(&im, rhs) * operator_scale;
+    const float threshold = (atol < floor ? floor : atol) + (rtol * rep.residual_rms);
+    if (isfinite(rep.residual_rms) && rep.residual_rms <= threshold) {
+      rep.converged = 1;
+    } else {
+      rep.converged = 0;
+      if (isfinite(rep.residual_rms) && isfinite(threshold)) {
+        for (uint32_t iteration = 1; iteration <= sg->solver_max_iterations; ++iteration) {
+          for (uint32_t s = 0; s < sg->signal_count; ++s)
+            for (size_t k = 0; k < im.nf; ++k) {
+              const size_t index = (size_t)s * im.n + im.fluid[k];
+              const float d = diagonal[(size_t)s * im.nf + k];
+              const float remainder = rates[index] - (d * current[index]);
+              const float update =
+                  (rhs[index] + (half_dt * remainder)) / (1.0f - (half_dt * d));
+              next[index] = current[index] + (relaxation * (update - current[index]));
+            }
+          float* swap = current;
+          current = next;
+          next = swap;
+          RESIDUAL_RMS(rep.residual_rms);
+          rep.iterations = iteration;
+          if (!isfinite(rep.residual_rms)) break;
+          if (rep.residual_rms <= threshold) {
+            rep.converged = 1;
+            break;
+          }
+        }
+      }
+    }
+  }
+#undef RESIDUAL_RMS
+  if (report != NULL) *report = rep;
+  if (!rep.converged) {
+    status = fail(err, errlen, MO_NOT_CONVERGED,
+                  "Implicit signal solve did not converge after %u iterations%s", rep.iterations,
+                  sg->advection_scheme == MO_VAN_LEER
+                      ? "; under the van Leer scheme, shorten the step until flow crosses only "
+                        "a few sites in one"
+                      : "");
+    goto done;
+  }
+  /* accept_step: the settle under van Leer, then validate_signal_levels. */
+  if (sg->advection_scheme == MO_VAN_LEER) {
+    status = settle_f32(current, im.total, err, errlen);
+    if (status != MO_OK) goto done;
+  } else {
+    for (size_t i = 0; i < im.total; ++i)
+      if (!isfinite(current[i]) || current[i] < 0.0f) {
+        status = fail(err, errlen, MO_INVALID_ARGUMENT,
+                      "signal grid levels must be finite and non-negative");
+        goto done;
+      }
+  }
+  memcpy(levels, current, im.total * sizeof(float));
+done:
+  free(im.fluid);
+  free(im.source);
+  free(im.loss);
+  free(rates);
+  free(rhs);
+  free(residual);
+  free(current);
+  free(next);
+  free(diagonal);
+  return status;
 }
 
 /* FP64-state step in face-flux form. Each face flux (advective upwind +

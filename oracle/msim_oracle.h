@@ -24,6 +24,10 @@
  *                           ledger identity to 1e-12
  *   mo_adr_validate_step    the reference's explicit bound, known answers from
  *                           the reference tests (dt 0.25 ok / 0.51 rejected)
+ *   mo_adr_step_implicit_f32 bit-exact vs reference advance_signal_grid_cpu under
+ *                           Crank-Nicolson and backward Euler (oracle/_ref): levels,
+ *                           iterations and residual RMS; the reference tests'
+ *                           known answers (2.2, 0.4/0.2/0.4) and non-convergence
  *   mo_port_flow_solve      bit-exact vs reference solve_depth_averaged_port_flow_cpu
  *                           (oracle/_ref); the reference tests' known answers
  *                           (port fluxes 1.5/0.5/-2, pressure-driven duct 0.8,
@@ -155,9 +159,30 @@ typedef struct {
   uint32_t advection_scheme;    /* MO_UPWIND or MO_VAN_LEER (SignalAdvectionKind) */
   const mo_boundary_patch* patches; /* reservoir patches of fixed faces, or NULL */
   uint32_t patch_count;
+  uint32_t integration;         /* MO_FORWARD_EULER, MO_CRANK_NICOLSON or MO_BACKWARD_EULER */
+  uint32_t solver_max_iterations; /* SignalSolveParameters (signals.hpp:49-55) */
+  float solver_absolute_tolerance;
+  float solver_relative_tolerance;
 } mo_signal_grid;
 
 enum { MO_UPWIND = 0, MO_VAN_LEER = 1 };
+enum { MO_FORWARD_EULER = 0, MO_CRANK_NICOLSON = 1, MO_BACKWARD_EULER = 2 };
+
+/* SignalSolveReport (signals.hpp:57-61). */
+typedef struct {
+  uint32_t converged, iterations;
+  float residual_rms;
+} mo_signal_report;
+
+/* One implicit step (Crank-Nicolson or backward Euler) in FP32, levels S*N
+ * in place: signal_grid_crank_nicolson_candidate (signals.cpp:1293-1375)
+ * under solve_signal_jacobi (signal_jacobi.hpp:53-84), then accept_step's
+ * settle and validation (signals.cpp:979-1011). Monod uptake enters as the
+ * loss vmax*phi/(K+c) of the starting levels, added to any affine loss.
+ * report may be NULL. A solve that does not converge returns
+ * MO_NOT_CONVERGED with the reference's message. */
+int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
+                             mo_signal_report* report, char* err, size_t errlen);
 
 /* The explicit stability bound of SignalGrid::validate_step. For Monod the
  * loss bound is vmax * max(phi) / K. Returns MO_OK or MO_INVALID_ARGUMENT. */

@@ -218,7 +218,8 @@ int ref_flow_apply(void* opaque, const double* input, double* output) {
   return kOk;
 }
 
-// `steps` steps of advance_signal_grid_cpu. levels: S*N in place.
+// `steps` steps of advance_signal_grid_cpu. levels: S*N in place. report
+// (may be null): the last step's converged, iterations and residual RMS.
 // velocity faces may be null (no field); source/loss may be null (no
 // reaction). boundary_values: 6*S as in make_spec. seconds: time of the steps.
 int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
@@ -229,7 +230,9 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
                             const float* source_rates, const float* loss_rates,
                             std::uint32_t scheme, std::uint32_t patch_count,
                             const std::uint32_t* patch_geometry, const float* patch_values,
-                            float dt, std::uint32_t steps, float* levels, double* seconds,
+                            std::uint32_t integration, std::uint32_t max_iterations,
+                            float absolute_tolerance, float relative_tolerance, float dt,
+                            std::uint32_t steps, float* levels, float* report, double* seconds,
                             char* err, std::size_t errlen) {
   return guarded(err, errlen, [&] {
     auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
@@ -249,6 +252,10 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
     spec.diffusion.assign(diffusion, diffusion + signals);
     spec.advection_scheme =
         scheme == 1 ? ms::SignalAdvectionKind::van_leer : ms::SignalAdvectionKind::upwind;
+    spec.integration = static_cast<ms::SignalIntegrationKind>(integration);
+    spec.solver.max_iterations = max_iterations;
+    spec.solver.absolute_tolerance = absolute_tolerance;
+    spec.solver.relative_tolerance = relative_tolerance;
     if (x_faces != nullptr) {
       ms::SignalGridVelocityField field;
       field.x_faces.assign(x_faces, x_faces + (nx + 1) * static_cast<std::size_t>(ny) * nz);
@@ -264,8 +271,14 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
     }
     ms::SignalGrid grid(spec, std::vector<float>(levels, levels + signals * n));
     const auto start = std::chrono::steady_clock::now();
+    ms::SignalSolveReport last;
     for (std::uint32_t step = 0; step < steps; ++step) {
-      static_cast<void>(ms::advance_signal_grid_cpu(grid, dt));
+      last = ms::advance_signal_grid_cpu(grid, dt);
+    }
+    if (report != nullptr) {
+      report[0] = last.converged ? 1.0F : 0.0F;
+      report[1] = static_cast<float>(last.iterations);
+      report[2] = last.residual_rms;
     }
     const auto stop = std::chrono::steady_clock::now();
     if (seconds != nullptr) *seconds = std::chrono::duration<double>(stop - start).count();
