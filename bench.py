@@ -142,6 +142,67 @@ def cpu_adr_baseline(w, faces, threads: int, steps: int):
                 sample=f"{steps} steps of advance_signal_grid_cpu, 3 solutes, in {secs:.2f} s")
 
 
+def implicit_bench(msa, hip, grid, w, bv, diffusion, dt, steps: int):
+    """Crank-Nicolson steps of 3 solutes with Monod uptake over the solved
+    field, from zero levels, one adr_run per step so that every step's
+    Jacobi iterations are counted; dt is four times the explicit bound."""
+    S = len(diffusion)
+    spec = msa.SignalGridSpec(grid, diffusion, bv, phi=w.phi, monod_vmax=[1.0] * S,
+                              monod_k=[0.1] * S, integration=msa.CRANK_NICOLSON)
+    hip.adr_upload(spec, np.zeros(S * w.sites, np.float32), precision=32, use_flow_field=True)
+    hip.adr_run(dt, 1, ledger=False)  # warm
+    hip.adr_upload(spec, np.zeros(S * w.sites, np.float32), precision=32, use_flow_field=True)
+    seconds, iterations, first = 0.0, 0, None
+    for _ in range(steps):
+        hip.adr_run(dt, 1, ledger=False)
+        rep = hip.adr_report()
+        seconds += hip.adr_seconds()
+        iterations += rep.iterations
+        first = rep.iterations if first is None else first
+    # Per iteration the fused sweep reads each level, its right-hand side and
+    # lagged loss and writes the next iterate (16 B per level), and reads the
+    # site's two face velocities and mask (9 B).
+    bytes_per_site = 16 * S + 9
+    sweeps = iterations + steps
+    out = {"integration": "crank_nicolson", "solutes": S, "steps": steps, "dt": dt,
+           "jacobi_iterations": iterations, "first_step_iterations": first,
+           "gcell_upd_per_s": round(w.fluid * S * steps / seconds / 1e9, 3),
+           "seconds_per_step": round(seconds / steps, 5),
+           "achieved_gbs_sweeps": round(bytes_per_site * w.sites * sweeps / seconds / 1e9, 1),
+           "bytes_per_site_per_sweep": bytes_per_site}
+    log(f"implicit transport: {out['gcell_upd_per_s']} Gcell-upd/s, {iterations} Jacobi "
+        f"iterations over {steps} steps ({seconds / steps * 1e3:.2f} ms per step)")
+    return out
+
+
+def cpu_implicit_baseline(w, faces, threads: int, dt: float):
+    """One Crank-Nicolson step of the reference (advance_signal_grid_cpu) from
+    zero levels, 3 solutes, the Monod uptake of zero levels as its affine
+    loss: the same first step the device takes."""
+    from oracle import CRANK_NICOLSON, FIXED, NO_FLUX, Reference
+    from oracle import Grid as OGrid
+    if not Reference.available():
+        return None
+    ref = Reference()
+    ref.set_threads(threads)
+    grid = OGrid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles,
+                 boundary=(FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX))
+    S = 3
+    bv = np.zeros(6 * S, np.float32)
+    bv[0::6] = 1.0
+    loss = np.tile((np.float32(1.0) * w.phi.reshape(-1)) / np.float32(0.1), S).astype(np.float32)
+    loss[np.tile(w.obstacles.reshape(-1), S) != 0] = 0.0
+    levels = np.zeros(S * w.sites, np.float32)
+    _, secs, rep = ref.advance_signal_grid(grid, levels, dt, 1,
+                                           diffusion=np.array([100.0, 500.0, 1000.0], np.float32),
+                                           boundary_values=bv, faces=faces, loss=loss,
+                                           integration=CRANK_NICOLSON, with_report=True)
+    return dict(value=w.fluid * S / secs / 1e9, unit="Gcell-upd/s", cores=ref.threads(),
+                kind="reference", iterations=rep["iterations"],
+                sample=f"1 Crank-Nicolson step of advance_signal_grid_cpu, 3 solutes, "
+                       f"{rep['iterations']} Jacobi iterations, in {secs:.2f} s")
+
+
 def batch_sweep_bench(msa, hip, steps: int):
     """The batched sweep: 256 seeded instances drawn from 64x64, 128x96,
     256x128 and 512x256, each solved (FP64 true residual 1e-8), then `steps`
@@ -253,6 +314,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="profiling calibration: copy this many MiB device to device, 3 times")
     ap.add_argument("--no-ports", action="store_true", help="skip the port-driven flow")
+    ap.add_argument("--implicit-steps", type=int, default=5,
+                    help="Crank-Nicolson transport steps to time (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50,
                     help="graph-captured steps of the 256-instance batched sweep (0: skip)")
     args = ap.parse_args()
@@ -310,8 +373,14 @@ def main():
                                       advection_scheme=scheme)
             hip.adr_upload(spec, np.zeros(S * w.sites), precision=prec, use_flow_field=True)
             hip.adr_run(1e-6, 5, ledger=prec == 64)
-        log(f"ran {args.kernel_iterations} iterations of each solver form and 5 transport steps "
-            f"of each transport form")
+        implicit = msa.SignalGridSpec(grid, [100.0, 500.0, 1000.0], bv, phi=w.phi,
+                                      monod_vmax=[1.0] * S, monod_k=[0.1] * S,
+                                      integration=msa.CRANK_NICOLSON)
+        hip.adr_upload(implicit, np.zeros(S * w.sites, np.float32), precision=32,
+                       use_flow_field=True)
+        hip.adr_run(8e-4, 2, ledger=False)
+        log(f"ran {args.kernel_iterations} iterations of each solver form, 5 transport steps "
+            f"of each explicit form and 2 implicit steps")
         return
 
     headline = solver_params(args.solver)
@@ -418,6 +487,9 @@ def main():
     hip.adr_run(dt_vl, args.adr_steps, ledger=False)
     adr_vl_s = hip.adr_seconds()
     adr_vl_gcups = w.fluid * S * args.adr_steps / adr_vl_s / 1e9
+    implicit = None
+    if args.implicit_steps > 0:
+        implicit = implicit_bench(msa, hip, grid, w, bv, diffusion, 4.0 * dt, args.implicit_steps)
     log(f"transport fp64: {adr_gcups:.2f} Gcell-upd/s ({adr_gbs:.0f} GB/s), worst per-step "
         f"imbalance {worst_balance:.2e}; fp32: {adr32_gcups:.2f} Gcell-upd/s; fp32 van Leer: "
         f"{adr_vl_gcups:.2f} Gcell-upd/s")
@@ -492,6 +564,8 @@ def main():
             "launches": launches,
         },
     }
+    if implicit is not None:
+        line["adr_implicit"] = implicit
     if mfma is not None:
         line["roofline_mfma"] = mfma
     if args.batch_steps > 0:
@@ -514,6 +588,10 @@ def main():
         adr_cpu = cpu_adr_baseline(w, faces, threads, steps=3)
         if adr_cpu is not None:
             line["cpu_baseline_adr"] = adr_cpu
+        if implicit is not None:
+            implicit_cpu = cpu_implicit_baseline(w, faces, threads, implicit["dt"])
+            if implicit_cpu is not None:
+                line["cpu_baseline_adr_implicit"] = implicit_cpu
     if not args.no_ports:
         per_iteration = line.get("cpu_baseline", {}).get("seconds_per_iteration")
         line["port_flow"] = port_flow_bench(msa, w, hip, per_iteration)

@@ -581,6 +581,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.absolute_tolerance = grid.solver_absolute_tolerance;
   a.relative_tolerance = grid.solver_relative_tolerance;
   a.last_report = msim_signal_report{1, 0, 0.0f};
+  a.predicted_iterations = kPredictedIterations;
   P.has_field = (use_flow_field || grid.x_faces != nullptr) ? 1 : 0;
   P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
   const bool patched = grid.patches != nullptr && grid.patch_count > 0;

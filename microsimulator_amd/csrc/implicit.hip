@@ -322,9 +322,14 @@ void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, con
     // Iterates 0..max_iterations are judged, a batch at a time.
     ImplicitState host{};
     uint32_t k = 0;
+    // The first batch reaches the iterate at which the previous step stopped
+    // (steps of one run stop at similar iterates), later ones kImplicitBatch
+    // further: a launch after the rule is met costs a few microseconds.
+    uint32_t batch = std::clamp<uint32_t>(a.predicted_iterations + 1, 2, kImplicitBatch);
     for (;;) {
       const uint32_t last = static_cast<uint32_t>(
-          std::min<uint64_t>(static_cast<uint64_t>(k) + kImplicitBatch - 1, a.max_iterations));
+          std::min<uint64_t>(static_cast<uint64_t>(k) + batch - 1, a.max_iterations));
+      batch = kImplicitBatch;
       for (; k <= last; ++k) {
         launch_sweep<S, kLimited, kPatched>(ctx, blocks, xf, yf, iterate(k), work[(k + 1) & 1u],
                                             half_dt, relaxation);
@@ -343,6 +348,7 @@ void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, con
     a.last_report.converged = host.converged;
     a.last_report.iterations = host.halted != 0 ? host.iteration : a.max_iterations;
     a.last_report.residual_rms = host.rms;
+    a.predicted_iterations = a.last_report.iterations;
     if (host.nonfinite != 0) {
       throw Error(MSIM_INVALID_ARGUMENT, "signal transport levels must be finite");
     }

@@ -226,6 +226,9 @@ struct AdrParams {
   float monod_k[kMaxSignals];
 };
 
+// Iterations the first batch of an implicit run's first step enqueues.
+constexpr uint32_t kPredictedIterations = 15;
+
 // The state of one implicit step's Jacobi solve on the device.
 struct ImplicitState {
   float rhs_rms;    // RMS of the right-hand side
@@ -246,6 +249,7 @@ struct AdrDevice {
   uint32_t max_iterations = 10000;
   float absolute_tolerance = 1.0e-7f, relative_tolerance = 1.0e-5f;
   msim_signal_report last_report{1, 0, 0.0f};
+  uint32_t predicted_iterations = kPredictedIterations;  // first batch of the next step
   uint32_t precision = 32;
   bool uploaded = false;
   bool flow_field = false;

@@ -122,14 +122,15 @@ __global__ void k_port_gather(const float* __restrict__ fields, size_t F,
 
 // The combined field (superpose_port_flows, flow_system.hpp:1168-1240): per
 // face, from 0.0, each solve's velocity in FP64 times its weight, added in
-// solve order, then rounded to FP32; the largest |velocity| as float bits
-// (positive floats order as their bits, and a NaN or an infinity compares
-// above every finite value).
+// solve order, then rounded to FP32. Each block leaves the largest |velocity|
+// it wrote as float bits in peak[block] (positive floats order as their
+// bits, and a NaN or an infinity compares above every finite value).
 __global__ __launch_bounds__(256) void k_port_combine(const float* __restrict__ fields, size_t F,
                                                        uint32_t J,
                                                        const double* __restrict__ weights,
                                                        float* __restrict__ out,
                                                        uint32_t* __restrict__ peak) {
+  __shared__ uint32_t s_peak[256 / kWave];
   uint32_t largest = 0;
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t f = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < F;
@@ -145,7 +146,12 @@ __global__ __launch_bounds__(256) void k_port_combine(const float* __restrict__ 
     largest = max(largest, static_cast<uint32_t>(__shfl_xor(static_cast<int>(largest), offset,
                                                             kWave)));
   }
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(peak, largest);
+  if ((threadIdx.x & (kWave - 1)) == 0) s_peak[threadIdx.x / kWave] = largest;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < 256 / kWave; ++w) largest = max(largest, s_peak[w]);
+    peak[blockIdx.x] = largest;
+  }
 }
 
 uint32_t grid_blocks(size_t n, uint32_t threads, uint32_t cap) {
@@ -350,22 +356,23 @@ void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_pa
   }
 
   // The combined field on the device.
+  const uint32_t combine_blocks = grid_blocks(F, 256, 4096);
   f.port_weights.reserve(weights.size());
-  f.port_peak.reserve(1);
+  f.port_peak.reserve(combine_blocks);
   f.lifted.reserve(F);
   check(hipMemcpyAsync(f.port_weights.data(), weights.data(), weights.size() * sizeof(double),
                        hipMemcpyHostToDevice, ctx.stream),
         "upload weights");
-  check(hipMemsetAsync(f.port_peak.data(), 0, sizeof(uint32_t), ctx.stream), "clear peak");
-  k_port_combine<<<grid_blocks(F, 256, 8192), 256, 0, ctx.stream>>>(
+  k_port_combine<<<combine_blocks, 256, 0, ctx.stream>>>(
       f.port_fields.data(), F, J, f.port_weights.data(), f.lifted.data(), f.port_peak.data());
   check(hipGetLastError(), "launch k_port_combine");
   check(hipEventRecord(ctx.stop, ctx.stream), "event");
-  uint32_t peak_bits = 0;
-  check(hipMemcpyAsync(&peak_bits, f.port_peak.data(), sizeof(uint32_t), hipMemcpyDeviceToHost,
-                       ctx.stream),
+  std::vector<uint32_t> block_peaks(combine_blocks);
+  check(hipMemcpyAsync(block_peaks.data(), f.port_peak.data(), combine_blocks * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, ctx.stream),
         "download peak");
   check(hipStreamSynchronize(ctx.stream), "combine");
+  const uint32_t peak_bits = *std::max_element(block_peaks.begin(), block_peaks.end());
   float ms = 0.0f;
   check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
   seconds += ms * 1.0e-3;
