@@ -110,6 +110,33 @@ class _SignalReport(C.Structure):
                 ("residual_rms", C.c_float)]
 
 
+class _RateInstruction(C.Structure):
+    _fields_ = [("operation", C.c_uint32), ("first", C.c_uint32), ("second", C.c_uint32),
+                ("third", C.c_uint32), ("value", C.c_float)]
+
+
+class _RatePlan(C.Structure):
+    _fields_ = [("species_count", C.c_uint32), ("signal_count", C.c_uint32),
+                ("instruction_count", C.c_uint32), ("instructions", C.c_void_p),
+                ("species_outputs", C.c_void_p), ("signal_outputs", C.c_void_p)]
+
+
+class _Cells(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("position_x", C.c_void_p), ("position_y", C.c_void_p),
+                ("position_z", C.c_void_p), ("lengths", C.c_void_p), ("radii", C.c_void_p),
+                ("growth_rates", C.c_void_p), ("cell_types", C.c_void_p)]
+
+
+def _cell_arrays(cells):
+    """cells: dict of position_x/y/z, lengths, radii, growth_rates (float32) and
+    cell_types (int32) arrays of one length."""
+    out = {k: np.ascontiguousarray(cells[k], np.float32)
+           for k in ("position_x", "position_y", "position_z", "lengths", "radii",
+                     "growth_rates")}
+    out["cell_types"] = np.ascontiguousarray(cells["cell_types"], np.int32)
+    return out
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data
 
@@ -180,6 +207,8 @@ class Oracle:
         L.mo_adr_step_f32.argtypes = [P, F, P, C.c_char_p, Z]
         L.mo_adr_step_f64.argtypes = [P, F, P, P, C.c_char_p, Z]
         L.mo_adr_step_implicit_f32.argtypes = [P, F, P, P, C.c_char_p, Z]
+        L.mo_advance_coupled_f32.argtypes = [P, P, P, P, P, P, P, F, P, C.c_char_p, Z]
+        L.mo_advance_coupled_f32.restype = C.c_int
         L.mo_adr_step_implicit_f32.restype = C.c_int
         for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_adr_validate_step,
                    L.mo_adr_step_f32, L.mo_adr_step_f64):
@@ -345,6 +374,44 @@ class Oracle:
         return lv, dict(converged=bool(rep.converged), iterations=rep.iterations,
                         residual_rms=rep.residual_rms)
 
+    def advance_coupled(self, grid: Grid, levels, cells, species, plan, previous_lengths,
+                        dt: float, origin=(0.0, 0.0, 0.0), **kw):
+        """advance_coupled_cpu: grid levels, species levels and the report.
+        plan: (species_count, instructions as (op, first, second, third, value),
+        species_outputs, signal_outputs)."""
+        keep: list = []
+        sg = self._signal_grid(grid, keep, **kw)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        sp = np.array(species, dtype=np.float32, copy=True).reshape(-1)
+        arrays = _cell_arrays(cells)
+        c = _Cells()
+        c.count = arrays["lengths"].size
+        for name, arr in arrays.items():
+            setattr(c, name, arr.ctypes.data)
+        species_count, instructions, species_outputs, signal_outputs = plan
+        table = (_RateInstruction * max(1, len(instructions)))()
+        for k, (op, a, b, d, v) in enumerate(instructions):
+            table[k].operation, table[k].first, table[k].second, table[k].third = op, a, b, d
+            table[k].value = v
+        so = np.ascontiguousarray(species_outputs, np.uint32)
+        go = np.ascontiguousarray(signal_outputs, np.uint32)
+        p = _RatePlan()
+        p.species_count, p.signal_count, p.instruction_count = species_count, go.size, len(instructions)
+        p.instructions = C.cast(table, C.c_void_p)
+        p.species_outputs = so.ctypes.data if so.size else None
+        p.signal_outputs = go.ctypes.data
+        prev = np.ascontiguousarray(previous_lengths, np.float32)
+        org = np.ascontiguousarray(origin, np.float32)
+        rep = _SignalReport()
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_advance_coupled_f32(C.byref(sg), org.ctypes.data, lv.ctypes.data,
+                                             C.byref(c), sp.ctypes.data, C.byref(p),
+                                             prev.ctypes.data, C.c_float(dt), C.byref(rep), err,
+                                             C.c_size_t(512))
+        self._check(st, err)
+        return lv, sp, dict(converged=bool(rep.converged), iterations=rep.iterations,
+                            residual_rms=rep.residual_rms)
+
     def adr_step_f64(self, grid: Grid, levels, dt: float, steps: int = 1, **kw):
         keep: list = []
         sg = self._signal_grid(grid, keep, **kw)
@@ -383,6 +450,9 @@ class Reference:
         L.ref_set_threads.argtypes = [U]
         L.ref_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, P, P, P, P, P, P,
                                     C.c_char_p, Z]
+        L.ref_advance_coupled.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, P, P, P, P, P, U, U,
+                                          U, F, F, U, P, P, P, P, P, P, P, U, P, U, P, P, P, U,
+                                          P, P, F, P, P, C.c_char_p, Z]
 
     @staticmethod
     def available(path: str = REF_SO) -> bool:
@@ -445,6 +515,53 @@ class Reference:
         return faces, states[:3 * len(ports)].reshape(len(ports), 3), dict(
             solves=int(counts[0]), iterations=int(counts[1]), relative_residual=rr.value,
             max_speed=ms.value, seconds=seconds.value)
+
+    def advance_coupled(self, grid: Grid, levels, cells, species, plan, previous_lengths, dt,
+                        origin=(0.0, 0.0, 0.0), diffusion=(1.0,), boundary_values=None,
+                        faces=None, source=None, loss=None, scheme=UPWIND,
+                        integration=FORWARD_EULER, solver=SOLVER_DEFAULTS):
+        """advance_coupled_cpu: grid levels, species levels and the report."""
+        kinds = self._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        diffusion = _f32(np.atleast_1d(diffusion))
+        S = diffusion.size
+        bv = np.zeros(6 * S, np.float32) if boundary_values is None else _f32(boundary_values)
+        fx = fy = fz = None
+        if faces is not None:
+            nfx, nfy, _ = grid.face_counts()
+            f = _f32(faces).reshape(-1)
+            fx, fy, fz = f[:nfx].copy(), f[nfx:nfx + nfy].copy(), f[nfx + nfy:].copy()
+        src = None
+        if loss is not None:
+            loss = _f32(loss).reshape(-1)
+            src = np.zeros_like(loss) if source is None else _f32(source).reshape(-1)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        sp = np.array(species, dtype=np.float32, copy=True).reshape(-1)
+        arrays = _cell_arrays(cells)
+        species_count, instructions, species_outputs, signal_outputs = plan
+        codes = np.array([[op, a, b, d] for op, a, b, d, _ in instructions], np.uint32).reshape(-1)
+        values = np.array([v for *_, v in instructions], np.float32)
+        so = np.ascontiguousarray(species_outputs, np.uint32)
+        go = np.ascontiguousarray(signal_outputs, np.uint32)
+        prev = np.ascontiguousarray(previous_lengths, np.float32)
+        org = np.ascontiguousarray(origin, np.float32)
+        report = np.zeros(3, np.float32)
+        err = C.create_string_buffer(512)
+        st = self.lib.ref_advance_coupled(
+            grid.nx, grid.ny, grid.nz, grid.dx, grid.dy, grid.dz, org.ctypes.data, _ptr(obs),
+            kinds.ctypes.data, S, diffusion.ctypes.data, bv.ctypes.data, _ptr(fx), _ptr(fy),
+            _ptr(fz), _ptr(src), _ptr(loss), scheme, integration, solver[0], solver[1],
+            solver[2], arrays["lengths"].size, arrays["position_x"].ctypes.data,
+            arrays["position_y"].ctypes.data, arrays["position_z"].ctypes.data,
+            arrays["lengths"].ctypes.data, arrays["radii"].ctypes.data,
+            arrays["growth_rates"].ctypes.data, arrays["cell_types"].ctypes.data, species_count,
+            sp.ctypes.data if sp.size else None, len(instructions), codes.ctypes.data,
+            values.ctypes.data, so.ctypes.data if so.size else None, go.size, go.ctypes.data,
+            prev.ctypes.data, dt, lv.ctypes.data, report.ctypes.data, err, 512)
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        return lv, sp, dict(converged=bool(report[0]), iterations=int(report[1]),
+                            residual_rms=float(report[2]))
 
     def flow_system(self, grid: Grid, mobility, axis=0):
         return _RefFlow(self, grid, mobility, axis)

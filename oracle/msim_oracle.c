@@ -1572,6 +1572,42 @@ static float transport_rate_at(const mo_signal_grid* sg, const float* lv, uint32
   return rate;
 }
 
+/* signal_grid_forward_euler_candidate (signals.cpp:1109-1134) after its
+ * step check, dt > 0: the candidate of every fluid level into `updated`
+ * (obstacle levels copied). */
+static int fe_candidate(const mo_signal_grid* sg, float dt, const float* levels, float* updated,
+                        char* err, size_t errlen) {
+  const mo_grid* g = &sg->grid;
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  memcpy(updated, levels, n * sg->signal_count * sizeof(float));
+  for (uint32_t s = 0; s < sg->signal_count; ++s) {
+    const float* lv = levels + (size_t)s * n;
+    for (uint32_t x = 0; x < g->nx; ++x)
+      for (uint32_t y = 0; y < g->ny; ++y)
+        for (uint32_t z = 0; z < g->nz; ++z) {
+          const size_t site = site_of(g, x, y, z);
+          if (solid_at(g, site)) continue;
+          const float current = lv[site];
+          if (!isfinite(current))
+            return fail(err, errlen, MO_INVALID_ARGUMENT, "signal transport levels must be finite");
+          float rate = transport_rate_at(sg, lv, s, x, y, z);
+          const size_t index = (size_t)s * n + site;
+          if (sg->source_rates != NULL)
+            rate += sg->source_rates[index] - (sg->loss_rates[index] * current);
+          if (sg->phi != NULL) rate += 0.0f - (monod_loss(sg, s, site, current) * current);
+          const float candidate = current + (dt * rate);
+          /* Under van Leer a level may fall below zero by rounding; the
+           * settle judges it (signals.cpp:1119-1131). */
+          if (!isfinite(candidate) ||
+              (candidate < 0.0f && sg->advection_scheme != MO_VAN_LEER))
+            return fail(err, errlen, MO_DOMAIN_ERROR,
+                        "signal grid update produced a non-finite or negative concentration");
+          updated[index] = candidate;
+        }
+  }
+  return MO_OK;
+}
+
 int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err, size_t errlen) {
   if (sg->integration != MO_FORWARD_EULER)
     return mo_adr_step_implicit_f32(sg, dt, levels, NULL, err, errlen);
@@ -1581,49 +1617,14 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
   if (status != MO_OK) return status;
   if (dt == 0.0f) return MO_OK;
   const mo_grid* g = &sg->grid;
-  const size_t n = (size_t)g->nx * g->ny * g->nz;
-  const size_t total = n * sg->signal_count;
+  const size_t total = (size_t)g->nx * g->ny * g->nz * sg->signal_count;
   float* updated = malloc(total * sizeof(float));
-  memcpy(updated, levels, total * sizeof(float));
-  for (uint32_t s = 0; s < sg->signal_count; ++s) {
-    const float* lv = levels + (size_t)s * n;
-    for (uint32_t x = 0; x < g->nx; ++x)
-      for (uint32_t y = 0; y < g->ny; ++y)
-        for (uint32_t z = 0; z < g->nz; ++z) {
-          const size_t site = site_of(g, x, y, z);
-          if (solid_at(g, site)) continue;
-          const float current = lv[site];
-          if (!isfinite(current)) {
-            free(updated);
-            return fail(err, errlen, MO_INVALID_ARGUMENT, "signal transport levels must be finite");
-          }
-          float rate = transport_rate_at(sg, lv, s, x, y, z);
-          const size_t index = (size_t)s * n + site;
-          if (sg->source_rates != NULL)
-            rate += sg->source_rates[index] - (sg->loss_rates[index] * current);
-          if (sg->phi != NULL) rate += 0.0f - (monod_loss(sg, s, site, current) * current);
-          const float candidate = current + (dt * rate);
-          /* Under van Leer a level may fall below zero by rounding; the
-           * settle below judges it (signals.cpp:1119-1131). */
-          if (!isfinite(candidate) ||
-              (candidate < 0.0f && sg->advection_scheme != MO_VAN_LEER)) {
-            free(updated);
-            return fail(err, errlen, MO_DOMAIN_ERROR,
-                        "signal grid update produced a non-finite or negative concentration");
-          }
-          updated[index] = candidate;
-        }
-  }
-  if (sg->advection_scheme == MO_VAN_LEER) {
+  status = fe_candidate(sg, dt, levels, updated, err, errlen);
+  if (status == MO_OK && sg->advection_scheme == MO_VAN_LEER)
     status = settle_f32(updated, total, err, errlen);
-    if (status != MO_OK) {
-      free(updated);
-      return status;
-    }
-  }
-  memcpy(levels, updated, total * sizeof(float));
+  if (status == MO_OK) memcpy(levels, updated, total * sizeof(float));
   free(updated);
-  return MO_OK;
+  return status;
 }
 
 /* The parts of an implicit step the iteration keeps fixed. */
@@ -1713,13 +1714,11 @@ static float implicit_diagonal(const implicit_t* im, uint32_t s, size_t site) {
   return diagonal;
 }
 
-int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
-                             mo_signal_report* report, char* err, size_t errlen) {
-  int status = validate_grid(&sg->grid, err, errlen);
-  if (status != MO_OK) return status;
+/* SignalSolveParameters::validate (signals.cpp:482-491) and the
+ * integration kind (718-725). */
+static int validate_solver(const mo_signal_grid* sg, char* err, size_t errlen) {
   if (sg->integration > MO_BACKWARD_EULER)
     return fail(err, errlen, MO_INVALID_ARGUMENT, "unknown signal integration kind");
-  /* SignalSolveParameters::validate (signals.cpp:482-491). */
   if (sg->solver_max_iterations == 0)
     return fail(err, errlen, MO_INVALID_ARGUMENT, "signal solver iteration limit must be positive");
   const float atol = sg->solver_absolute_tolerance, rtol = sg->solver_relative_tolerance;
@@ -1727,11 +1726,30 @@ int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
       (atol == 0.0f && rtol == 0.0f))
     return fail(err, errlen, MO_INVALID_ARGUMENT,
                 "signal solver tolerances must be finite and non-negative");
-  if (!isfinite(dt) || dt < 0.0f)
-    return fail(err, errlen, MO_INVALID_ARGUMENT, "time step must be finite and non-negative");
+  return MO_OK;
+}
+
+/* accept_step's settle and validation (signals.cpp:979-1011). */
+static int accept_levels(const mo_signal_grid* sg, float* levels, size_t total, char* err,
+                         size_t errlen) {
+  if (sg->advection_scheme == MO_VAN_LEER) return settle_f32(levels, total, err, errlen);
+  for (size_t i = 0; i < total; ++i)
+    if (!isfinite(levels[i]) || levels[i] < 0.0f)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "signal grid levels must be finite and non-negative");
+  return MO_OK;
+}
+
+/* signal_grid_crank_nicolson_candidate (signals.cpp:1293-1375) under
+ * solve_signal_jacobi (signal_jacobi.hpp:53-84), dt > 0: the candidate into
+ * `levels` when the solve converges, and its report. sources: S*N source
+ * rates added to the right-hand side as dt * source (NULL: none). */
+static int implicit_candidate(const mo_signal_grid* sg, float dt, const float* sources,
+                              float* levels, mo_signal_report* report, char* err,
+                              size_t errlen) {
+  int status = MO_OK;
+  const float atol = sg->solver_absolute_tolerance, rtol = sg->solver_relative_tolerance;
   mo_signal_report rep = {1, 0, 0.0f};
-  if (report != NULL) *report = rep;
-  if (dt == 0.0f) return MO_OK;
   const mo_grid* g = &sg->grid;
   implicit_t im;
   memset(&im, 0, sizeof(im));
@@ -1777,7 +1795,8 @@ int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
   for (uint32_t s = 0; s < sg->signal_count; ++s)
     for (size_t k = 0; k < im.nf; ++k) {
       const size_t index = (size_t)s * im.n + im.fluid[k];
-      rhs[index] = current[index] + (explicit_dt * rates[index]) + (dt * 0.0f);
+      const float source = sources != NULL ? sources[index] : 0.0f;
+      rhs[index] = current[index] + (explicit_dt * rates[index]) + (dt * source);
     }
   float least = 0.0f;
   for (uint32_t s = 0; s < sg->signal_count; ++s)
@@ -1837,29 +1856,8 @@ int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
     }
   }
 #undef RESIDUAL_RMS
-  if (report != NULL) *report = rep;
-  if (!rep.converged) {
-    status = fail(err, errlen, MO_NOT_CONVERGED,
-                  "Implicit signal solve did not converge after %u iterations%s", rep.iterations,
-                  sg->advection_scheme == MO_VAN_LEER
-                      ? "; under the van Leer scheme, shorten the step until flow crosses only "
-                        "a few sites in one"
-                      : "");
-    goto done;
-  }
-  /* accept_step: the settle under van Leer, then validate_signal_levels. */
-  if (sg->advection_scheme == MO_VAN_LEER) {
-    status = settle_f32(current, im.total, err, errlen);
-    if (status != MO_OK) goto done;
-  } else {
-    for (size_t i = 0; i < im.total; ++i)
-      if (!isfinite(current[i]) || current[i] < 0.0f) {
-        status = fail(err, errlen, MO_INVALID_ARGUMENT,
-                      "signal grid levels must be finite and non-negative");
-        goto done;
-      }
-  }
-  memcpy(levels, current, im.total * sizeof(float));
+  *report = rep;
+  if (rep.converged) memcpy(levels, current, im.total * sizeof(float));
 done:
   free(im.fluid);
   free(im.source);
@@ -1870,6 +1868,44 @@ done:
   free(current);
   free(next);
   free(diagonal);
+  return status;
+}
+
+
+/* The message of an unconverged implicit solve (signal_solve_failure,
+ * signals.cpp:967-977). */
+static int unconverged(const mo_signal_grid* sg, const char* solve, uint32_t iterations, char* err,
+                       size_t errlen) {
+  return fail(err, errlen, MO_NOT_CONVERGED, "%s did not converge after %u iterations%s", solve,
+              iterations,
+              sg->advection_scheme == MO_VAN_LEER
+                  ? "; under the van Leer scheme, shorten the step until flow crosses only a "
+                    "few sites in one"
+                  : "");
+}
+
+int mo_adr_step_implicit_f32(const mo_signal_grid* sg, float dt, float* levels,
+                             mo_signal_report* report, char* err, size_t errlen) {
+  int status = validate_grid(&sg->grid, err, errlen);
+  if (status != MO_OK) return status;
+  status = validate_solver(sg, err, errlen);
+  if (status != MO_OK) return status;
+  if (!isfinite(dt) || dt < 0.0f)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "time step must be finite and non-negative");
+  mo_signal_report rep = {1, 0, 0.0f};
+  if (report != NULL) *report = rep;
+  if (dt == 0.0f) return MO_OK;
+  const size_t total =
+      (size_t)sg->grid.nx * sg->grid.ny * sg->grid.nz * (size_t)sg->signal_count;
+  float* candidate = malloc(total * sizeof(float));
+  memcpy(candidate, levels, total * sizeof(float));
+  status = implicit_candidate(sg, dt, NULL, candidate, &rep, err, errlen);
+  if (report != NULL) *report = rep;
+  if (status == MO_OK && !rep.converged)
+    status = unconverged(sg, "Implicit signal solve", rep.iterations, err, errlen);
+  if (status == MO_OK) status = accept_levels(sg, candidate, total, err, errlen);
+  if (status == MO_OK) memcpy(levels, candidate, total * sizeof(float));
+  free(candidate);
   return status;
 }
 
@@ -2049,4 +2085,370 @@ int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* 
   free(inflow);
   free(uptake);
   return MO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cell-grid exchange                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  uint32_t sites[8];
+  float weights[8];
+  uint32_t count;
+  int entirely_solid;
+} stencil_t;
+
+/* interpolation_axis (signals.cpp:98-127) for a sample inside the grid. */
+static int interpolation_axis(float position, float origin, float spacing, uint32_t dimension,
+                              const char* axis, uint32_t indices[2], float weights[2],
+                              uint32_t* count, char* err, size_t errlen) {
+  if (!isfinite(position))
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "signal sample position must be finite");
+  indices[0] = indices[1] = 0;
+  weights[0] = 1.0f;
+  weights[1] = 0.0f;
+  *count = 1;
+  if (dimension == 1) return MO_OK;
+  const double coordinate = ((double)position - (double)origin) / (double)spacing;
+  const double upper_bound = (double)(dimension - 1);
+  if (coordinate < 0.0 || coordinate > upper_bound)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "signal sample is outside the %s grid bound",
+                axis);
+  const uint32_t lower = (uint32_t)floor(coordinate);
+  if (lower == dimension - 1) {
+    indices[0] = indices[1] = lower;
+    return MO_OK;
+  }
+  const float fraction = (float)(coordinate - (double)lower);
+  indices[0] = lower;
+  indices[1] = lower + 1;
+  weights[0] = 1.0f - fraction;
+  weights[1] = fraction;
+  *count = 2;
+  return MO_OK;
+}
+
+/* signal_grid_stencil (signals.cpp:357-420): the trilinear stencil of a
+ * position; with obstacles only the face-connected fluid component of the
+ * strongest weight keeps its weights, renormalized. */
+static int grid_stencil(const mo_grid* g, const float* origin, float px, float py, float pz,
+                        stencil_t* st, char* err, size_t errlen) {
+  uint32_t xi[2], yi[2], zi[2], xc, yc, zc;
+  float xw[2], yw[2], zw[2];
+  int status = interpolation_axis(px, origin[0], g->dx, g->nx, "x", xi, xw, &xc, err, errlen);
+  if (status == MO_OK)
+    status = interpolation_axis(py, origin[1], g->dy, g->ny, "y", yi, yw, &yc, err, errlen);
+  if (status == MO_OK)
+    status = interpolation_axis(pz, origin[2], g->dz, g->nz, "z", zi, zw, &zc, err, errlen);
+  if (status != MO_OK) return status;
+  memset(st, 0, sizeof(*st));
+  uint32_t coords[8][3];
+  for (uint32_t a = 0; a < xc; ++a)
+    for (uint32_t b = 0; b < yc; ++b)
+      for (uint32_t c = 0; c < zc; ++c) {
+        coords[st->count][0] = xi[a];
+        coords[st->count][1] = yi[b];
+        coords[st->count][2] = zi[c];
+        st->sites[st->count] = (uint32_t)site_of(g, xi[a], yi[b], zi[c]);
+        st->weights[st->count] = xw[a] * yw[b] * zw[c];
+        ++st->count;
+      }
+  if (g->obstacles == NULL) return MO_OK;
+  uint32_t seed = st->count;
+  for (uint32_t i = 0; i < st->count; ++i) {
+    if (solid_at(g, st->sites[i])) st->weights[i] = 0.0f;
+    if (st->weights[i] > 0.0f && (seed == st->count || st->weights[i] > st->weights[seed]))
+      seed = i;
+  }
+  if (seed == st->count) {
+    st->entirely_solid = 1;
+    return MO_OK;
+  }
+  int connected[8] = {0};
+  connected[seed] = 1;
+  for (int grew = 1; grew;) {
+    grew = 0;
+    for (uint32_t i = 0; i < st->count; ++i) {
+      if (st->weights[i] <= 0.0f || connected[i]) continue;
+      for (uint32_t j = 0; j < st->count; ++j) {
+        unsigned distance = 0;
+        for (int axis = 0; axis < 3; ++axis)
+          distance += (unsigned)abs((int)coords[i][axis] - (int)coords[j][axis]);
+        if (connected[j] && distance == 1) {
+          connected[i] = 1;
+          grew = 1;
+          break;
+        }
+      }
+    }
+  }
+  float total = 0.0f;
+  for (uint32_t i = 0; i < st->count; ++i) {
+    if (!connected[i]) st->weights[i] = 0.0f;
+    total += st->weights[i];
+  }
+  for (uint32_t i = 0; i < st->count; ++i) st->weights[i] /= total;
+  return MO_OK;
+}
+
+/* effective_cell_volume and effective_cell_surface_area (species.cpp:134-142),
+ * with the shared constant pi as a float. */
+static float cell_volume(float length, float radius) {
+  return (float)3.14159265358979323846 * radius * radius * (length + 2.0f * radius);
+}
+static float cell_surface(float length, float radius) {
+  return 2.0f * (float)3.14159265358979323846 * radius * (length + 2.0f * radius);
+}
+
+enum {
+  OP_CONSTANT = 0, OP_SPECIES, OP_POSITION_X, OP_POSITION_Y, OP_POSITION_Z, OP_LENGTH,
+  OP_RADIUS, OP_GROWTH_RATE, OP_CELL_TYPE, OP_VOLUME, OP_SURFACE, OP_ADD, OP_SUBTRACT,
+  OP_MULTIPLY, OP_DIVIDE, OP_POWER, OP_MINIMUM, OP_MAXIMUM, OP_NEGATE, OP_EXPONENTIAL,
+  OP_LOGARITHM, OP_LESS, OP_LESS_EQUAL, OP_GREATER, OP_GREATER_EQUAL, OP_EQUAL, OP_SELECT,
+  OP_SIGNAL, OP_VOLUME_CHANGE_RATE
+};
+
+/* CoupledRatePlan::validate (coupled_rates.cpp:60-146). */
+static int validate_plan(const mo_rate_plan* plan, char* err, size_t errlen) {
+  if (plan->signal_count == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "a coupled rate plan needs at least one signal");
+  if (plan->instruction_count == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "a coupled rate plan needs instructions");
+  for (uint32_t index = 0; index < plan->instruction_count; ++index) {
+    const mo_rate_instruction* in = &plan->instructions[index];
+    const char* bad = NULL;
+    switch (in->operation) {
+      case OP_CONSTANT:
+        if (!isfinite(in->value))
+          return fail(err, errlen, MO_INVALID_ARGUMENT, "coupled rate constants must be finite");
+        break;
+      case OP_SPECIES:
+        if (in->first >= plan->species_count)
+          return fail(err, errlen, MO_INVALID_ARGUMENT,
+                      "coupled rate instruction uses an unknown species");
+        break;
+      case OP_SIGNAL:
+        if (in->first >= plan->signal_count)
+          return fail(err, errlen, MO_INVALID_ARGUMENT,
+                      "coupled rate instruction uses an unknown signal");
+        break;
+      case OP_POSITION_X: case OP_POSITION_Y: case OP_POSITION_Z: case OP_LENGTH:
+      case OP_RADIUS: case OP_GROWTH_RATE: case OP_CELL_TYPE: case OP_VOLUME:
+      case OP_VOLUME_CHANGE_RATE: case OP_SURFACE:
+        break;
+      case OP_NEGATE: case OP_EXPONENTIAL: case OP_LOGARITHM:
+        if (in->first >= index) bad = "unary";
+        break;
+      case OP_ADD: case OP_SUBTRACT: case OP_MULTIPLY: case OP_DIVIDE: case OP_POWER:
+      case OP_MINIMUM: case OP_MAXIMUM: case OP_LESS: case OP_LESS_EQUAL: case OP_GREATER:
+      case OP_GREATER_EQUAL: case OP_EQUAL:
+        if (in->first >= index) bad = "first";
+        else if (in->second >= index) bad = "second";
+        break;
+      case OP_SELECT:
+        if (in->first >= index) bad = "condition";
+        else if (in->second >= index) bad = "true";
+        else if (in->third >= index) bad = "false";
+        break;
+      default:
+        return fail(err, errlen, MO_INVALID_ARGUMENT, "coupled rate plan uses an unknown operation");
+    }
+    if (bad != NULL)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "coupled rate %s input must refer to an earlier instruction", bad);
+  }
+  for (uint32_t k = 0; k < plan->species_count; ++k)
+    if (plan->species_outputs[k] >= plan->instruction_count)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "coupled species output uses an unknown instruction");
+  for (uint32_t k = 0; k < plan->signal_count; ++k)
+    if (plan->signal_outputs[k] >= plan->instruction_count)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "coupled signal output uses an unknown instruction");
+  return MO_OK;
+}
+
+/* evaluate_instruction (cpu_coupled.cpp:20-85). */
+static float evaluate(const mo_rate_instruction* in, const float* work, const float* species,
+                      const float* signals, const mo_cells* cells, uint32_t cell,
+                      float volume_change_rate) {
+  switch (in->operation) {
+    case OP_CONSTANT: return in->value;
+    case OP_SPECIES: return species[in->first];
+    case OP_SIGNAL: return signals[in->first];
+    case OP_POSITION_X: return cells->position_x[cell];
+    case OP_POSITION_Y: return cells->position_y[cell];
+    case OP_POSITION_Z: return cells->position_z[cell];
+    case OP_LENGTH: return cells->lengths[cell];
+    case OP_RADIUS: return cells->radii[cell];
+    case OP_GROWTH_RATE: return cells->growth_rates[cell];
+    case OP_CELL_TYPE: return (float)cells->cell_types[cell];
+    case OP_VOLUME_CHANGE_RATE: return volume_change_rate;
+    case OP_VOLUME: return cell_volume(cells->lengths[cell], cells->radii[cell]);
+    case OP_SURFACE: return cell_surface(cells->lengths[cell], cells->radii[cell]);
+    case OP_ADD: return work[in->first] + work[in->second];
+    case OP_SUBTRACT: return work[in->first] - work[in->second];
+    case OP_MULTIPLY: return work[in->first] * work[in->second];
+    case OP_DIVIDE: return work[in->first] / work[in->second];
+    case OP_POWER: return powf(work[in->first], work[in->second]);
+    case OP_MINIMUM: return work[in->second] < work[in->first] ? work[in->second] : work[in->first];
+    case OP_MAXIMUM: return work[in->first] < work[in->second] ? work[in->second] : work[in->first];
+    case OP_NEGATE: return -work[in->first];
+    case OP_EXPONENTIAL: return expf(work[in->first]);
+    case OP_LOGARITHM: return logf(work[in->first]);
+    case OP_LESS: return work[in->first] < work[in->second] ? 1.0f : 0.0f;
+    case OP_LESS_EQUAL: return work[in->first] <= work[in->second] ? 1.0f : 0.0f;
+    case OP_GREATER: return work[in->first] > work[in->second] ? 1.0f : 0.0f;
+    case OP_GREATER_EQUAL: return work[in->first] >= work[in->second] ? 1.0f : 0.0f;
+    case OP_EQUAL: return work[in->first] == work[in->second] ? 1.0f : 0.0f;
+    case OP_SELECT: return work[in->first] != 0.0f ? work[in->second] : work[in->third];
+    default: return NAN;
+  }
+}
+
+int mo_advance_coupled_f32(const mo_signal_grid* sg, const float* origin, float* levels,
+                           const mo_cells* cells, float* species, const mo_rate_plan* plan,
+                           const float* previous_lengths, float dt, mo_signal_report* report,
+                           char* err, size_t errlen) {
+  const mo_grid* g = &sg->grid;
+  /* validate_coupled_step (cpu_coupled.cpp:95-123). */
+  if (!isfinite(dt) || dt < 0.0f)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "coupled time step must be finite and non-negative");
+  for (int a = 0; a < 3; ++a)
+    if (!isfinite(origin[a]))
+      return fail(err, errlen, MO_INVALID_ARGUMENT, "signal grid origin must be finite");
+  /* The world state's invariants (world_state.cpp:14-35). */
+  const uint32_t ncell = cells->count, nsp = plan->species_count;
+  for (uint32_t c = 0; c < ncell; ++c) {
+    const float v[6] = {cells->position_x[c], cells->position_y[c], cells->position_z[c],
+                        cells->lengths[c], cells->radii[c], cells->growth_rates[c]};
+    for (int k = 0; k < 6; ++k)
+      if (!isfinite(v[k])) return fail(err, errlen, MO_INVALID_ARGUMENT, "cell fields must be finite");
+    if (cells->lengths[c] < 0.0f)
+      return fail(err, errlen, MO_INVALID_ARGUMENT, "cell length must be non-negative");
+    if (cells->radii[c] <= 0.0f)
+      return fail(err, errlen, MO_INVALID_ARGUMENT, "cell radius must be positive");
+  }
+  for (size_t i = 0; i < (size_t)ncell * nsp; ++i)
+    if (!isfinite(species[i]))
+      return fail(err, errlen, MO_INVALID_ARGUMENT, "cell species levels must be finite");
+  int status = validate_grid(g, err, errlen);
+  if (status != MO_OK) return status;
+  status = validate_solver(sg, err, errlen);
+  if (status != MO_OK) return status;
+  if (sg->integration == MO_FORWARD_EULER) {
+    status = mo_adr_validate_step(sg, dt, err, errlen);
+    if (status != MO_OK) return status;
+  }
+  status = validate_plan(plan, err, errlen);
+  if (status != MO_OK) return status;
+  if (plan->signal_count != sg->signal_count)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "coupled rate plan and signal grid counts disagree");
+  for (uint32_t c = 0; c < ncell; ++c)
+    if (!isfinite(previous_lengths[c]) || previous_lengths[c] < 0.0f)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "previous cell lengths must be finite and non-negative");
+  stencil_t* stencils = calloc(ncell + 1, sizeof(stencil_t));
+  for (uint32_t c = 0; c < ncell; ++c) {
+    status = grid_stencil(g, origin, cells->position_x[c], cells->position_y[c],
+                          cells->position_z[c], &stencils[c], err, errlen);
+    if (status == MO_OK && stencils[c].entirely_solid)
+      status = fail(err, errlen, MO_INVALID_ARGUMENT,
+                    "signal sample position is inside a grid obstacle");
+    if (status != MO_OK) {
+      free(stencils);
+      return status;
+    }
+  }
+  /* advance_coupled_cpu (cpu_coupled.cpp:125-231). */
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const uint32_t S = plan->signal_count;
+  const size_t total = n * S;
+  const float volume = g->dx * g->dy * g->dz;
+  float* next_species = malloc(((size_t)ncell * nsp + 1) * sizeof(float));
+  memcpy(next_species, species, (size_t)ncell * nsp * sizeof(float));
+  float* exchange = calloc((size_t)ncell * S + 1, sizeof(float));
+  float* work = malloc((plan->instruction_count + 1) * sizeof(float));
+  float* sampled = malloc((S + 1) * sizeof(float));
+  float* sources = calloc(total, sizeof(float));
+  float* next_grid = malloc(total * sizeof(float));
+  mo_signal_report rep = {1, 0, 0.0f};
+  for (uint32_t c = 0; c < ncell && status == MO_OK; ++c) {
+    const stencil_t* st = &stencils[c];
+    for (uint32_t s = 0; s < S; ++s) sampled[s] = 0.0f;
+    for (uint32_t e = 0; e < st->count; ++e)
+      for (uint32_t s = 0; s < S; ++s)
+        sampled[s] += st->weights[e] * levels[(size_t)s * n + st->sites[e]];
+    const float previous_volume = cell_volume(previous_lengths[c], cells->radii[c]);
+    const float current_volume = cell_volume(cells->lengths[c], cells->radii[c]);
+    const float dilution = previous_volume / current_volume;
+    float* cell_species = next_species + (size_t)c * nsp;
+    for (uint32_t k = 0; k < nsp; ++k) cell_species[k] *= dilution;
+    const float volume_change_rate =
+        dt == 0.0f ? 0.0f : (current_volume - previous_volume) / dt;
+    for (uint32_t index = 0; index < plan->instruction_count; ++index) {
+      work[index] = evaluate(&plan->instructions[index], work, cell_species, sampled, cells, c,
+                             volume_change_rate);
+      if (!isfinite(work[index])) {
+        status = fail(err, errlen, MO_DOMAIN_ERROR,
+                      "coupled rate instruction %u produced a non-finite value", index);
+        break;
+      }
+    }
+    if (status != MO_OK) break;
+    for (uint32_t k = 0; k < nsp; ++k) {
+      cell_species[k] += dt * work[plan->species_outputs[k]];
+      if (!isfinite(cell_species[k])) {
+        status = fail(err, errlen, MO_DOMAIN_ERROR,
+                      "coupled species update produced a non-finite level");
+        break;
+      }
+    }
+    for (uint32_t s = 0; s < S; ++s)
+      exchange[(size_t)c * S + s] = work[plan->signal_outputs[s]] / volume;
+  }
+  /* The sources, summed into the sites one cell at a time. */
+  if (status == MO_OK) {
+    for (uint32_t c = 0; c < ncell; ++c) {
+      const stencil_t* st = &stencils[c];
+      for (uint32_t s = 0; s < S; ++s) {
+        const float rate = exchange[(size_t)c * S + s];
+        for (uint32_t e = 0; e < st->count; ++e)
+          sources[(size_t)s * n + st->sites[e]] += st->weights[e] * rate;
+      }
+    }
+    memcpy(next_grid, levels, total * sizeof(float));
+    if (sg->integration != MO_FORWARD_EULER) {
+      /* signal_grid_crank_nicolson_candidate with the sources (signals.cpp:
+       * 1295-1310): they must be finite, and dt = 0 keeps the levels. */
+      for (size_t i = 0; i < total; ++i)
+        if (!isfinite(sources[i])) {
+          status = fail(err, errlen, MO_INVALID_ARGUMENT, "signal source rates must be finite");
+          break;
+        }
+      if (status == MO_OK && dt > 0.0f)
+        status = implicit_candidate(sg, dt, sources, next_grid, &rep, err, errlen);
+      if (status == MO_OK && !rep.converged)
+        status = unconverged(sg, "Implicit coupled signal solve", rep.iterations, err, errlen);
+    } else {
+      if (dt > 0.0f) status = fe_candidate(sg, dt, levels, next_grid, err, errlen);
+      if (status == MO_OK)
+        for (size_t i = 0; i < total; ++i) next_grid[i] += dt * sources[i];
+    }
+    if (report != NULL) *report = rep;
+    if (status == MO_OK) status = accept_levels(sg, next_grid, total, err, errlen);
+    if (status == MO_OK) {
+      memcpy(levels, next_grid, total * sizeof(float));
+      memcpy(species, next_species, (size_t)ncell * nsp * sizeof(float));
+    }
+  }
+  free(stencils);
+  free(next_species);
+  free(exchange);
+  free(work);
+  free(sampled);
+  free(sources);
+  free(next_grid);
+  return status;
 }

@@ -28,6 +28,10 @@
  *                           Crank-Nicolson and backward Euler (oracle/_ref): levels,
  *                           iterations and residual RMS; the reference tests'
  *                           known answers (2.2, 0.4/0.2/0.4) and non-convergence
+ *   mo_advance_coupled_f32  bit-exact vs reference advance_coupled_cpu (oracle/_ref):
+ *                           grid levels, species levels and report, forward Euler
+ *                           and Crank-Nicolson, every rate operation; the reference
+ *                           tests' closed forms and refusals
  *   mo_port_flow_solve      bit-exact vs reference solve_depth_averaged_port_flow_cpu
  *                           (oracle/_ref); the reference tests' known answers
  *                           (port fluxes 1.5/0.5/-2, pressure-driven duct 0.8,
@@ -198,6 +202,41 @@ int mo_adr_step_f32(const mo_signal_grid* sg, float dt, float* levels, char* err
  * uptake*dt] with mass = sum(c) * voxel volume. */
 int mo_adr_step_f64(const mo_signal_grid* sg, float dt, double* levels, double* ledger, char* err,
                     size_t errlen);
+
+/* RateInstruction (species.hpp:18-24) with RateOp's codes
+ * (cpp/include/microsimulator/rate_ops.def). */
+typedef struct {
+  uint32_t operation, first, second, third;
+  float value;
+} mo_rate_instruction;
+
+/* CoupledRatePlan (coupled_rates.hpp:16-41). */
+typedef struct {
+  uint32_t species_count, signal_count, instruction_count;
+  const mo_rate_instruction* instructions;
+  const uint32_t* species_outputs; /* species_count instruction indices */
+  const uint32_t* signal_outputs;  /* signal_count instruction indices */
+} mo_rate_plan;
+
+/* The cells of a WorldState the coupled step reads (world_state.hpp:58-84). */
+typedef struct {
+  uint32_t count;
+  const float *position_x, *position_y, *position_z, *lengths, *radii, *growth_rates;
+  const int32_t* cell_types;
+} mo_cells;
+
+/* advance_coupled_cpu (cpp/cpu/cpu_coupled.cpp:125-231) with its checks
+ * (validate_coupled_step, 95-123): every cell samples the grid through its
+ * stencil (signal_grid_stencil, signals.cpp:357-420), dilutes and advances
+ * its species by the plan, and exchanges with the grid; the sources are
+ * summed into the sites cell by cell and the grid takes a step of its
+ * integration kind with them. levels S*N and species count*species_count in
+ * place; origin: the grid's origin (x, y, z). report (may be NULL): the
+ * implicit solve's. A refused step changes neither. */
+int mo_advance_coupled_f32(const mo_signal_grid* sg, const float* origin, float* levels,
+                           const mo_cells* cells, float* species, const mo_rate_plan* plan,
+                           const float* previous_lengths, float dt, mo_signal_report* report,
+                           char* err, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@
 //   detail::solve_depth_averaged_pressure cpp/cpu/cpu_flow.cpp:172-175
 //   SignalGrid + advance_signal_grid_cpu cpp/core/signals.cpp:1377-1389
 //   solve_depth_averaged_port_flow_cpu   cpp/cpu/cpu_flow.cpp:353-375
+//   WorldState + CoupledRatePlan + advance_coupled_cpu cpp/cpu/cpu_coupled.cpp:125-231
 //   set_cpu_thread_count                 cpp/include/microsimulator/threads.hpp:19
 #include <chrono>
 #include <cstdint>
@@ -23,9 +24,11 @@
 #include <vector>
 
 #include "core/flow_system.hpp"
+#include "microsimulator/coupled_rates.hpp"
 #include "microsimulator/flow.hpp"
 #include "microsimulator/signals.hpp"
 #include "microsimulator/threads.hpp"
+#include "microsimulator/world_state.hpp"
 
 namespace ms = microsimulator;
 
@@ -50,6 +53,9 @@ int guarded(char* err, std::size_t errlen, Body&& body) {
     return report_error(e, kOverflow, err, errlen);
   } catch (const std::domain_error& e) {
     return report_error(e, kDomain, err, errlen);
+  } catch (const std::logic_error& e) {
+    // out_of_range (a sample outside the grid) and the other argument errors.
+    return report_error(e, kInvalid, err, errlen);
   } catch (const std::exception& e) {
     return report_error(e, kRuntime, err, errlen);
   }
@@ -333,6 +339,91 @@ int ref_port_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx
     counts[1] = result.report.iterations;
     *relative_residual = result.report.relative_residual;
     *max_speed = result.report.max_speed;
+  });
+}
+
+// advance_coupled_cpu on a one-layer-or-more grid: the spec as
+// ref_advance_signal_grid builds it (with `origin`), a WorldState of `cells`
+// cells (positions, lengths, radii, growth rates, types, species levels
+// cells*species) and a CoupledRatePlan (instructions as 4 uint32 (operation,
+// first, second, third) plus a float value each) of plan_signals signals.
+// levels and species in place;
+// report: converged, iterations, residual RMS.
+int ref_advance_coupled(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
+                        float dz, const float* origin, const std::uint8_t* obstacles,
+                        const std::uint8_t* kinds, std::uint32_t signals, const float* diffusion,
+                        const float* boundary_values, const float* x_faces, const float* y_faces,
+                        const float* z_faces, const float* source_rates, const float* loss_rates,
+                        std::uint32_t scheme, std::uint32_t integration,
+                        std::uint32_t max_iterations, float absolute_tolerance,
+                        float relative_tolerance, std::uint32_t cells, const float* position_x,
+                        const float* position_y, const float* position_z, const float* lengths,
+                        const float* radii, const float* growth_rates,
+                        const std::int32_t* cell_types, std::uint32_t species_count,
+                        float* species, std::uint32_t instruction_count,
+                        const std::uint32_t* instruction_codes, const float* instruction_values,
+                        const std::uint32_t* species_outputs, std::uint32_t plan_signals,
+                        const std::uint32_t* signal_outputs, const float* previous_lengths,
+                        float dt, float* levels, float* report,
+                        char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
+    spec.origin = {origin[0], origin[1], origin[2]};
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    spec.diffusion.assign(diffusion, diffusion + signals);
+    spec.advection_scheme =
+        scheme == 1 ? ms::SignalAdvectionKind::van_leer : ms::SignalAdvectionKind::upwind;
+    spec.integration = static_cast<ms::SignalIntegrationKind>(integration);
+    spec.solver.max_iterations = max_iterations;
+    spec.solver.absolute_tolerance = absolute_tolerance;
+    spec.solver.relative_tolerance = relative_tolerance;
+    if (x_faces != nullptr) {
+      ms::SignalGridVelocityField field;
+      field.x_faces.assign(x_faces, x_faces + (nx + 1) * static_cast<std::size_t>(ny) * nz);
+      field.y_faces.assign(y_faces, y_faces + static_cast<std::size_t>(nx) * (ny + 1) * nz);
+      field.z_faces.assign(z_faces, z_faces + static_cast<std::size_t>(nx) * ny * (nz + 1));
+      spec.velocity_field = std::move(field);
+    }
+    if (loss_rates != nullptr) {
+      ms::SignalGridAffineReaction reaction;
+      reaction.source_rates.assign(source_rates, source_rates + signals * n);
+      reaction.loss_rates.assign(loss_rates, loss_rates + signals * n);
+      spec.reaction = std::move(reaction);
+    }
+    ms::SignalGrid grid(spec, std::vector<float>(levels, levels + signals * n));
+    ms::WorldState state(cells, species_count);
+    for (std::uint32_t c = 0; c < cells; ++c) {
+      ms::CellInit cell;
+      cell.position = {position_x[c], position_y[c], position_z[c]};
+      cell.length = lengths[c];
+      cell.radius = radii[c];
+      cell.growth_rate = growth_rates[c];
+      cell.cell_type = cell_types[c];
+      cell.species.assign(species + static_cast<std::size_t>(c) * species_count,
+                          species + static_cast<std::size_t>(c + 1) * species_count);
+      static_cast<void>(state.add_cell(cell));
+    }
+    std::vector<ms::RateInstruction> instructions(instruction_count);
+    for (std::uint32_t k = 0; k < instruction_count; ++k) {
+      instructions[k].operation = static_cast<ms::RateOp>(instruction_codes[4 * k]);
+      instructions[k].first = instruction_codes[4 * k + 1];
+      instructions[k].second = instruction_codes[4 * k + 2];
+      instructions[k].third = instruction_codes[4 * k + 3];
+      instructions[k].value = instruction_values[k];
+    }
+    const ms::CoupledRatePlan plan(
+        species_count, plan_signals, std::move(instructions),
+        std::vector<std::uint32_t>(species_outputs, species_outputs + species_count),
+        std::vector<std::uint32_t>(signal_outputs, signal_outputs + plan_signals));
+    const auto result = ms::advance_coupled_cpu(
+        state, grid, plan, std::span<const float>(previous_lengths, cells), dt);
+    const auto out = grid.levels();
+    std::memcpy(levels, out.data(), out.size() * sizeof(float));
+    const auto held = state.species_state();
+    std::memcpy(species, held.levels.data(), held.levels.size() * sizeof(float));
+    report[0] = result.converged ? 1.0F : 0.0F;
+    report[1] = static_cast<float>(result.iterations);
+    report[2] = result.residual_rms;
   });
 }
 

@@ -270,3 +270,86 @@ def largest_divergence(nx, ny, nz, dx, dy, dz, obstacles, xf, yf, zf):
            + (Z[:, :, 1:] - Z[:, :, :-1]) / dz)
     fluid = np.asarray(obstacles).reshape(nx, ny, nz) == 0
     return float(np.abs(out[fluid]).max()) if fluid.any() else 0.0
+
+
+# ---------------------------------------------------------------- coupled --
+# RateOp codes (cpp/include/microsimulator/rate_ops.def).
+(OP_CONSTANT, OP_SPECIES, OP_POSITION_X, OP_POSITION_Y, OP_POSITION_Z, OP_LENGTH, OP_RADIUS,
+ OP_GROWTH_RATE, OP_CELL_TYPE, OP_VOLUME, OP_SURFACE, OP_ADD, OP_SUBTRACT, OP_MULTIPLY,
+ OP_DIVIDE, OP_POWER, OP_MINIMUM, OP_MAXIMUM, OP_NEGATE, OP_EXPONENTIAL, OP_LOGARITHM, OP_LESS,
+ OP_LESS_EQUAL, OP_GREATER, OP_GREATER_EQUAL, OP_EQUAL, OP_SELECT, OP_SIGNAL,
+ OP_VOLUME_CHANGE_RATE) = range(29)
+
+
+def uptake_plan(S, K, transcendental=True):
+    """A coupled rate plan for S signals and K species that uses every
+    operation: Monod uptake of signal 0 scaled by the cell's volume (its
+    negative is the grid exchange), species production and decay, and
+    position-, type- and size-dependent branches. Without `transcendental`
+    the exponential, logarithm and power are left out."""
+    ins = []
+
+    def op(code, a=0, b=0, c=0, v=0.0):
+        ins.append((code, a, b, c, v))
+        return len(ins) - 1
+
+    c0 = op(OP_SIGNAL, 0)
+    half = op(OP_CONSTANT, v=0.25)
+    uptake = op(OP_DIVIDE, c0, op(OP_ADD, c0, half))
+    vmax = op(OP_CONSTANT, v=0.8)
+    rate = op(OP_MULTIPLY, uptake, vmax)
+    volume = op(OP_VOLUME)
+    taken = op(OP_MULTIPLY, op(OP_MULTIPLY, rate, volume), op(OP_CONSTANT, v=0.02))
+    exchange0 = op(OP_NEGATE, taken)
+    growth = op(OP_GROWTH_RATE)
+    px, py = op(OP_POSITION_X), op(OP_POSITION_Y)
+    side = op(OP_LESS, px, py)
+    branch = op(OP_SELECT, side, rate, growth)
+    kind = op(OP_CELL_TYPE)
+    two = op(OP_CONSTANT, v=2.0)
+    typed = op(OP_GREATER_EQUAL, kind, two)
+    surface = op(OP_SURFACE)
+    length, radius = op(OP_LENGTH), op(OP_RADIUS)
+    slender = op(OP_MAXIMUM, op(OP_DIVIDE, length, radius), op(OP_MINIMUM, surface, two))
+    change = op(OP_VOLUME_CHANGE_RATE)
+    pz = op(OP_POSITION_Z)
+    flags = op(OP_ADD, op(OP_LESS_EQUAL, pz, px), op(OP_EQUAL, typed, op(OP_GREATER, px, pz)))
+    shape = op(OP_MULTIPLY, slender, op(OP_ADD, flags, change))
+    if transcendental:
+        shape = op(OP_ADD, shape, op(OP_EXPONENTIAL, op(OP_NEGATE, growth)))
+        shape = op(OP_ADD, shape, op(OP_LOGARITHM, op(OP_ADD, volume, two)))
+        shape = op(OP_ADD, shape, op(OP_POWER, two, half))
+    species_out = []
+    for k in range(K):
+        level = op(OP_SPECIES, k)
+        decay = op(OP_MULTIPLY, level, op(OP_CONSTANT, v=0.1 * (k + 1)))
+        species_out.append(op(OP_SUBTRACT, op(OP_ADD, branch, op(OP_MULTIPLY, shape,
+                                                                  op(OP_CONSTANT, v=0.01))),
+                              decay))
+    signal_out = [exchange0]
+    for s in range(1, S):
+        signal_out.append(op(OP_MULTIPLY, op(OP_SIGNAL, s), op(OP_CONSTANT, v=0.05 * s)))
+    return K, ins, species_out, signal_out
+
+
+def random_cells(seed, nx, ny, solid, count, dx=1.0, dy=1.0, origin=(0.0, 0.0, 0.0)):
+    """Cells at random positions inside the grid whose trilinear stencils
+    touch at least one fluid site."""
+    rng = np.random.default_rng(seed)
+    solid = np.asarray(solid).reshape(nx, ny)
+    xs, ys = [], []
+    while len(xs) < count:
+        x = rng.uniform(0.0, (nx - 1) * dx) if nx > 1 else 0.0
+        y = rng.uniform(0.0, (ny - 1) * dy) if ny > 1 else 0.0
+        i0, j0 = int(x // dx) if nx > 1 else 0, int(y // dy) if ny > 1 else 0
+        near = solid[i0:i0 + 2, j0:j0 + 2]
+        if (near == 0).any():
+            xs.append(x + origin[0])
+            ys.append(y + origin[1])
+    lengths = rng.uniform(1.0, 4.0, count).astype(np.float32)
+    return dict(position_x=np.array(xs, np.float32), position_y=np.array(ys, np.float32),
+                position_z=np.full(count, origin[2], np.float32), lengths=lengths,
+                radii=rng.uniform(0.3, 0.8, count).astype(np.float32),
+                growth_rates=rng.uniform(0.5, 1.5, count).astype(np.float32),
+                cell_types=rng.integers(0, 4, count).astype(np.int32)), \
+        (lengths * rng.uniform(0.85, 1.0, count)).astype(np.float32)
