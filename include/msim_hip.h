@@ -1,8 +1,9 @@
 /*
  * msim_hip.h -- C ABI of the MI355X (gfx950) backend for MicroSimulator's
  * grid flow-and-transport path: the depth-averaged (Hele-Shaw/Darcy) pressure
- * solve, face-velocity reconstruction, and the explicit conservative
- * advection-diffusion-reaction step.
+ * solve (axis- or port-driven), face-velocity reconstruction, the conservative
+ * advection-diffusion-reaction step (explicit or implicit), and the coupled
+ * cell/grid exchange around it.
  *
  * The entry points are what a `hip` ComputeBackend in the reference would
  * bind (cpp/include/microsimulator/backend.hpp:21-56). Plain pointers and
@@ -340,6 +341,103 @@ double msim_hip_adr_seconds(msim_ctx* ctx);
  * adr_run took ({1, 0, 0} after forward-Euler steps). No counterpart in the
  * reference, whose advance returns it per step. */
 int msim_hip_adr_report(msim_ctx* ctx, msim_signal_report* report);
+
+/* ---------------------------------------------------------------------------
+ * Coupled cell/grid exchange (ComputeBackend::advance_coupled, backend.hpp:31-34;
+ * advance_coupled_cpu, cpp/cpu/cpu_coupled.cpp:125-231), one layer, FP32
+ * levels, at most 8 signals and 256 rate instructions.
+ *
+ * Each cell samples the grid's starting levels through its trilinear stencil
+ * (signal_grid_stencil, signals.cpp:357-420: with obstacles only the fluid
+ * corners face-connected to the strongest one, renormalised), dilutes its
+ * species by previous/current effective volume (pi r^2 (L + 2r),
+ * species.cpp:134-137), runs the rate plan, adds dt times its species
+ * outputs, and returns each signal output divided by the voxel volume through
+ * the same stencil as a per-site source, summed over cells in cell order. The
+ * grid then steps with those sources: forward Euler adds dt * source to the
+ * explicit candidate, the implicit kinds add it to the right-hand side
+ * (signals.cpp:1295-1310). The species are committed only when the grid step
+ * is accepted. */
+
+/* RateOp (cpp/include/microsimulator/species.hpp:12-16, codes of rate_ops.def). */
+enum {
+  MSIM_RATE_CONSTANT = 0, MSIM_RATE_SPECIES, MSIM_RATE_POSITION_X, MSIM_RATE_POSITION_Y,
+  MSIM_RATE_POSITION_Z, MSIM_RATE_LENGTH, MSIM_RATE_RADIUS, MSIM_RATE_GROWTH_RATE,
+  MSIM_RATE_CELL_TYPE, MSIM_RATE_VOLUME, MSIM_RATE_SURFACE_AREA, MSIM_RATE_ADD,
+  MSIM_RATE_SUBTRACT, MSIM_RATE_MULTIPLY, MSIM_RATE_DIVIDE, MSIM_RATE_POWER, MSIM_RATE_MINIMUM,
+  MSIM_RATE_MAXIMUM, MSIM_RATE_NEGATE, MSIM_RATE_EXPONENTIAL, MSIM_RATE_LOGARITHM,
+  MSIM_RATE_LESS, MSIM_RATE_LESS_EQUAL, MSIM_RATE_GREATER, MSIM_RATE_GREATER_EQUAL,
+  MSIM_RATE_EQUAL, MSIM_RATE_SELECT, MSIM_RATE_SIGNAL, MSIM_RATE_VOLUME_CHANGE_RATE
+};
+
+/* RateInstruction (species.hpp:18-24): inputs are earlier instructions'
+ * values (first, second, third), a species or signal index (first), or the
+ * constant `value`. */
+typedef struct {
+  uint32_t operation;
+  uint32_t first;
+  uint32_t second;
+  uint32_t third;
+  float value;
+} msim_rate_instruction;
+
+/* CoupledRatePlan (coupled_rates.hpp; checks of CoupledRatePlan::validate,
+ * cpp/core/coupled_rates.cpp:60-146). */
+typedef struct {
+  uint32_t species_count;
+  uint32_t signal_count;
+  uint32_t instruction_count;
+  const msim_rate_instruction* instructions;
+  const uint32_t* species_outputs; /* species_count instruction indices */
+  const uint32_t* signal_outputs;  /* signal_count instruction indices */
+} msim_rate_plan;
+
+/* The cells of a WorldState (world_state.hpp: geometry and attributes), one
+ * array entry per cell. Species levels are a separate count x species_count
+ * array, cell-major (WorldState::species_state). */
+typedef struct {
+  uint32_t count;
+  const float* position_x;
+  const float* position_y;
+  const float* position_z;
+  const float* lengths;
+  const float* radii;
+  const float* growth_rates;
+  const int32_t* cell_types;
+} msim_cells;
+
+/* ComputeBackend::advance_coupled: one coupled step of dt. origin: the grid's
+ * origin (SignalGridSpec::origin). levels (S*N) and species
+ * (count*species_count) in place; previous_lengths: each cell's length before
+ * this step's growth. The reference's checks and messages, in its order
+ * (validate_coupled_step, cpu_coupled.cpp:90-123). report as
+ * msim_hip_advance_signal_grid's. A refused or failed step leaves levels and
+ * species unchanged. */
+int msim_hip_advance_coupled(msim_ctx* ctx, const msim_signal_grid* grid, const float* origin,
+                             float* levels, const msim_cells* cells, float* species,
+                             const msim_rate_plan* plan, const float* previous_lengths,
+                             float dt, msim_signal_report* report);
+
+/* Device-resident coupled steps (no counterpart in the reference, which holds
+ * the world state on the host): the same step as msim_hip_advance_coupled,
+ * many times over cells that stay in place.
+ *  coupled_upload   -- grid spec, levels, cells, species and plan, checked as
+ *                      msim_hip_advance_coupled checks them.
+ *  coupled_run      -- `steps` coupled steps of dt; the first step's previous
+ *                      lengths are previous_lengths (NULL: the cells' own
+ *                      lengths), later steps' the cells' lengths. Stops at the
+ *                      first refused step, whose starting state is kept.
+ *  coupled_download -- levels (S*N) and species (count*species_count); either
+ *                      may be NULL. */
+int msim_hip_coupled_upload(msim_ctx* ctx, const msim_signal_grid* grid, const float* origin,
+                            const float* levels, const msim_cells* cells, const float* species,
+                            const msim_rate_plan* plan);
+int msim_hip_coupled_run(msim_ctx* ctx, const float* previous_lengths, float dt, uint32_t steps,
+                         msim_signal_report* report);
+int msim_hip_coupled_download(msim_ctx* ctx, float* levels, float* species);
+
+/* Device time (seconds) of the kernels of the last coupled_run. */
+double msim_hip_coupled_seconds(msim_ctx* ctx);
 
 /* ---------------------------------------------------------------------------
  * Learned smoother of the LEARNED solver (no counterpart in the reference,

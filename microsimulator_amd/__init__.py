@@ -3,7 +3,8 @@
 The Python side mirrors the reference's backend interface for this path
 (``ComputeBackend``, cpp/include/microsimulator/backend.hpp:21-56): a
 :class:`HipBackend` with ``info``, ``supports``, ``solve_depth_averaged_flow``
-(backend.hpp:44-46) and ``advance_signal_grid`` (backend.hpp:30), and the
+(backend.hpp:44-46), ``solve_depth_averaged_port_flow`` (47-49),
+``advance_signal_grid`` (30) and ``advance_coupled`` (31-34), and the
 device-resident calls that keep a system on the GPU between solves. Every call
 goes through the C ABI of ``libmsim_hip.so`` (include/msim_hip.h), whose
 kernels are hand-written HIP for gfx950. There is no CPU fallback: without the
@@ -175,6 +176,32 @@ class _SignalReport(C.Structure):
                 ("residual_rms", C.c_float)]
 
 
+class _RateInstruction(C.Structure):
+    _fields_ = [("operation", C.c_uint32), ("first", C.c_uint32), ("second", C.c_uint32),
+                ("third", C.c_uint32), ("value", C.c_float)]
+
+
+class _RatePlan(C.Structure):
+    _fields_ = [("species_count", C.c_uint32), ("signal_count", C.c_uint32),
+                ("instruction_count", C.c_uint32), ("instructions", C.c_void_p),
+                ("species_outputs", C.c_void_p), ("signal_outputs", C.c_void_p)]
+
+
+class _Cells(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("position_x", C.c_void_p), ("position_y", C.c_void_p),
+                ("position_z", C.c_void_p), ("lengths", C.c_void_p), ("radii", C.c_void_p),
+                ("growth_rates", C.c_void_p), ("cell_types", C.c_void_p)]
+
+
+#: RateOp (cpp/include/microsimulator/species.hpp:12-16, rate_ops.def), in its order.
+RATE_OPS = ("constant", "species", "position_x", "position_y", "position_z", "length", "radius",
+            "growth_rate", "cell_type", "volume", "surface_area", "add", "subtract", "multiply",
+            "divide", "power", "minimum", "maximum", "negate", "exponential", "logarithm", "less",
+            "less_equal", "greater", "greater_equal", "equal", "select", "signal",
+            "volume_change_rate")
+RateOp = type("RateOp", (), {name.upper(): code for code, name in enumerate(RATE_OPS)})
+
+
 #: Every entry point include/msim_hip.h declares.
 EXPORTS = (
     "msim_hip_device_count", "msim_hip_create", "msim_hip_destroy", "msim_hip_last_error",
@@ -189,6 +216,8 @@ EXPORTS = (
     "msim_hip_batch_autotune", "msim_hip_batch_shapes", "msim_hip_batch_shape",
     "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
     "msim_hip_solve_depth_averaged_port_flow", "msim_hip_adr_report",
+    "msim_hip_advance_coupled", "msim_hip_coupled_upload", "msim_hip_coupled_run",
+    "msim_hip_coupled_download", "msim_hip_coupled_seconds",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -243,6 +272,11 @@ def library(path: str = LIBRARY):
         "msim_hip_batch_download": (I, [P, U, P, P, P, P]),
         "msim_hip_solve_depth_averaged_port_flow": (I, [P, P, P, P, U, P, P, P, P, P, P]),
         "msim_hip_adr_report": (I, [P, P]),
+        "msim_hip_advance_coupled": (I, [P, P, P, P, P, P, P, P, C.c_float, P]),
+        "msim_hip_coupled_upload": (I, [P, P, P, P, P, P, P]),
+        "msim_hip_coupled_run": (I, [P, P, C.c_float, U, P]),
+        "msim_hip_coupled_download": (I, [P, P, P]),
+        "msim_hip_coupled_seconds": (D, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -526,6 +560,77 @@ class SignalGridSpec:
 
 
 @dataclass
+class RateInstruction:
+    """RateInstruction (species.hpp:18-24): operation (a RateOp code), inputs
+    first/second/third, constant value."""
+    operation: int
+    first: int = 0
+    second: int = 0
+    third: int = 0
+    value: float = 0.0
+
+
+@dataclass
+class CoupledRatePlan:
+    """CoupledRatePlan (coupled_rates.hpp): instructions as RateInstruction or
+    (operation, first, second, third, value) tuples."""
+    species_count: int
+    signal_count: int
+    instructions: list
+    species_outputs: list
+    signal_outputs: list
+
+    @staticmethod
+    def of(plan) -> "CoupledRatePlan":
+        """A plan from itself or from (species_count, instructions,
+        species_outputs, signal_outputs)."""
+        if isinstance(plan, CoupledRatePlan):
+            return plan
+        species_count, instructions, species_outputs, signal_outputs = plan
+        return CoupledRatePlan(species_count, len(signal_outputs), list(instructions),
+                               list(species_outputs), list(signal_outputs))
+
+    def _c(self, keep: list) -> _RatePlan:
+        table = (_RateInstruction * max(1, len(self.instructions)))()
+        for k, ins in enumerate(self.instructions):
+            if isinstance(ins, RateInstruction):
+                ins = (ins.operation, ins.first, ins.second, ins.third, ins.value)
+            op, a, b, d, v = ins
+            table[k].operation, table[k].first, table[k].second, table[k].third = op, a, b, d
+            table[k].value = v
+        so = np.ascontiguousarray(self.species_outputs, np.uint32).reshape(-1)
+        go = np.ascontiguousarray(self.signal_outputs, np.uint32).reshape(-1)
+        keep += [table, so, go]
+        p = _RatePlan()
+        p.species_count, p.signal_count = self.species_count, self.signal_count
+        p.instruction_count = len(self.instructions)
+        p.instructions = C.cast(table, C.c_void_p)
+        p.species_outputs = so.ctypes.data if so.size else None
+        p.signal_outputs = go.ctypes.data if go.size else None
+        return p
+
+
+_CELL_FIELDS = ("position_x", "position_y", "position_z", "lengths", "radii", "growth_rates")
+
+
+def _cells_c(cells, keep: list) -> _Cells:
+    """cells: a mapping (or object) of position_x/y/z, lengths, radii,
+    growth_rates (float) and cell_types (int) arrays of one length."""
+    get = cells.__getitem__ if isinstance(cells, dict) else lambda k: getattr(cells, k)
+    arrays = {k: np.ascontiguousarray(get(k), np.float32).reshape(-1) for k in _CELL_FIELDS}
+    arrays["cell_types"] = np.ascontiguousarray(get("cell_types"), np.int32).reshape(-1)
+    count = arrays["lengths"].size
+    if any(a.size != count for a in arrays.values()):
+        raise InvalidArgument("cell arrays must have one length")
+    keep += list(arrays.values())
+    c = _Cells()
+    c.count = count
+    for name, arr in arrays.items():
+        setattr(c, name, arr.ctypes.data if count else None)
+    return c
+
+
+@dataclass
 class SignalSolveReport:
     """SignalSolveReport (signals.hpp:57-61)."""
     converged: bool = True
@@ -545,7 +650,7 @@ class BackendInfo:
 
 FEATURES = ("growth", "species", "cell_contacts", "external_constraints", "signals",
             "coupled_rates", "depth_averaged_flow", "resolved_flow", "contact_dynamics")
-SUPPORTED = frozenset({"signals", "depth_averaged_flow"})
+SUPPORTED = frozenset({"signals", "coupled_rates", "depth_averaged_flow"})
 
 
 class HipBackend:
@@ -671,6 +776,75 @@ class HipBackend:
         self._check(self.lib.msim_hip_advance_signal_grid(self.ctx, C.byref(s), dt,
                                                          lv.ctypes.data, C.byref(rep)))
         return lv, SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
+
+    def advance_coupled(self, spec: SignalGridSpec, levels, cells, species, plan,
+                        previous_lengths, dt: float, origin=(0.0, 0.0, 0.0)):
+        """ComputeBackend::advance_coupled (backend.hpp:31-34): returns (new
+        levels, new species, SignalSolveReport); the inputs are not modified.
+        species: cells x species_count, cell-major."""
+        keep: list = []
+        s = spec._c(keep)
+        p = CoupledRatePlan.of(plan)._c(keep)
+        c = _cells_c(cells, keep)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        if lv.size != spec.signal_count * spec.grid.sites:
+            raise InvalidArgument("signal grid level count does not match its specification")
+        sp = np.array(species, dtype=np.float32, copy=True).reshape(-1)
+        if sp.size != c.count * p.species_count:
+            raise InvalidArgument("coupled rate plan and world state species counts disagree")
+        prev = np.ascontiguousarray(previous_lengths, np.float32).reshape(-1)
+        if prev.size != c.count:
+            raise InvalidArgument("previous cell lengths and world state cell counts disagree")
+        org = np.ascontiguousarray(origin, np.float32).reshape(3)
+        rep = _SignalReport()
+        self._check(self.lib.msim_hip_advance_coupled(
+            self.ctx, C.byref(s), org.ctypes.data, lv.ctypes.data, C.byref(c),
+            sp.ctypes.data if sp.size else None, C.byref(p), prev.ctypes.data if prev.size else None,
+            dt, C.byref(rep)))
+        return lv, sp, SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
+
+    # -- device-resident coupled steps (no counterpart in the reference) ---------
+    def coupled_upload(self, spec: SignalGridSpec, levels, cells, species, plan,
+                       origin=(0.0, 0.0, 0.0)):
+        keep: list = []
+        s = spec._c(keep)
+        p = CoupledRatePlan.of(plan)._c(keep)
+        c = _cells_c(cells, keep)
+        lv = np.ascontiguousarray(levels, np.float32).reshape(-1)
+        if lv.size != spec.signal_count * spec.grid.sites:
+            raise InvalidArgument("signal grid level count does not match its specification")
+        sp = np.ascontiguousarray(species, np.float32).reshape(-1)
+        if sp.size != c.count * p.species_count:
+            raise InvalidArgument("coupled rate plan and world state species counts disagree")
+        org = np.ascontiguousarray(origin, np.float32).reshape(3)
+        self._coupled = (spec.signal_count * spec.grid.sites, c.count, p.species_count)
+        self._adr = (spec.signal_count, spec.grid.sites, np.float32)
+        self._check(self.lib.msim_hip_coupled_upload(
+            self.ctx, C.byref(s), org.ctypes.data, lv.ctypes.data, C.byref(c),
+            sp.ctypes.data if sp.size else None, C.byref(p)))
+
+    def coupled_run(self, dt: float, steps: int = 1, previous_lengths=None) -> SignalSolveReport:
+        prev = None
+        if previous_lengths is not None:
+            prev = np.ascontiguousarray(previous_lengths, np.float32).reshape(-1)
+            if prev.size != self._coupled[1]:
+                raise InvalidArgument("previous cell lengths and world state cell counts disagree")
+        rep = _SignalReport()
+        self._check(self.lib.msim_hip_coupled_run(
+            self.ctx, None if prev is None or prev.size == 0 else prev.ctypes.data, dt, steps,
+            C.byref(rep)))
+        return SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
+
+    def coupled_download(self):
+        total, cells, species = self._coupled
+        lv = np.zeros(total, np.float32)
+        sp = np.zeros(cells * species, np.float32)
+        self._check(self.lib.msim_hip_coupled_download(self.ctx, lv.ctypes.data,
+                                                       sp.ctypes.data if sp.size else None))
+        return lv, sp
+
+    def coupled_seconds(self) -> float:
+        return float(self.lib.msim_hip_coupled_seconds(self.ctx))
 
     # -- device-resident flow -------------------------------------------------
     def flow_prepare(self, grid: Grid, mobility, axis: int = 0):
@@ -916,8 +1090,9 @@ class HipBackend:
 
     # -- the rest of ComputeBackend: not on this backend ------------------------
     def _unsupported(self, *args, **kwargs):
-        raise Unsupported("the HIP backend implements depth-averaged flow and signal transport only")
+        raise Unsupported("the HIP backend implements depth-averaged flow, signal transport and "
+                          "the coupled cell/grid exchange only")
 
-    advance_growth = advance_species = advance_coupled = _unsupported
+    advance_growth = advance_species = _unsupported
     find_cell_contacts = find_external_contacts = solve_contact_forces = _unsupported
     solve_resolved_flow = solve_resolved_port_flow = _unsupported

@@ -29,8 +29,6 @@ namespace msim {
 namespace {
 
 constexpr int kAdrBlock = 256;
-constexpr uint32_t kFlagNegative = 1;  // a non-finite or (upwind) negative update
-constexpr uint32_t kFlagLevels = 2;    // a level left negative after the settle
 
 // The ledger of a step lives as per-block partial sums (mass after, boundary
 // supply rate, uptake rate, for each signal) in two generations by step
@@ -712,7 +710,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
 template <typename T, int S, int TY, bool kLimited, bool kPatched>
 void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
                  const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
-                 double* largest) {
+                 double* largest, bool settle) {
   auto& a = ctx.adr;
   if (a.params.has_affine != 0) {
     k_adr_step<T, S, TY, true, kLimited, kPatched><<<sh.blocks, TY, 0, ctx.stream>>>(
@@ -725,7 +723,7 @@ void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float*
         reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
         sh.rows, L, largest);
   }
-  if constexpr (kLimited) {
+  if (kLimited && settle) {
     const size_t count = static_cast<size_t>(S) * a.sites();
     const uint32_t blocks = static_cast<uint32_t>(
         std::min<size_t>((count + kAdrBlock - 1) / kAdrBlock, 2048));
@@ -737,21 +735,43 @@ void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float*
 template <typename T, int S, bool kLimited>
 void launch_typed(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
                   const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
-                  double* largest) {
+                  double* largest, bool settle = true) {
   // Specs with reservoir patches take one strip width (128 columns).
   if (ctx.adr.params.edge != nullptr) {
-    launch_step<T, S, 128, kLimited, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+    launch_step<T, S, 128, kLimited, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest, settle);
     return;
   }
   switch (sh.ty) {
     case 64:
-      launch_step<T, S, 64, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      launch_step<T, S, 64, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest,
+                                             settle);
       break;
     case 128:
-      launch_step<T, S, 128, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      launch_step<T, S, 128, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest,
+                                              settle);
       break;
     default:
-      launch_step<T, S, 256, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+      launch_step<T, S, 256, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest,
+                                              settle);
+  }
+}
+
+// The explicit candidate of one FP32 step, unsettled.
+template <int S>
+void launch_candidate(Context& ctx, float dt, const float* in, float* out, uint32_t step,
+                      const float* xf, const float* yf) {
+  auto& a = ctx.adr;
+  const bool limited = a.params.limited != 0;
+  AdrShape sh = adr_shape(a.params.nx, a.params.ny, limited ? 2 : 1, a.params.edge != nullptr);
+  sh.blocks = step_blocks(sh);
+  LedgerParts L{a.partials.data(), a.ledger.data(), sh.blocks};
+  double* largest = a.partials.data() + static_cast<size_t>(3 * kMaxSignals * 2) * sh.blocks;
+  const auto* src = reinterpret_cast<const uint8_t*>(in);
+  auto* dst = reinterpret_cast<uint8_t*>(out);
+  if (limited) {
+    launch_typed<float, S, true>(ctx, sh, xf, yf, src, dst, dt, step, L, largest, false);
+  } else {
+    launch_typed<float, S, false>(ctx, sh, xf, yf, src, dst, dt, step, L, largest, false);
   }
 }
 
@@ -788,6 +808,21 @@ void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const
                                                              a.flags.data());
     check(hipGetLastError(), "launch ledger tail");
   }
+}
+
+void fe_candidate_f32(Context& ctx, float dt, const float* in, float* out, uint32_t step,
+                      const float* xf, const float* yf) {
+  switch (ctx.adr.params.signals) {
+    case 1: launch_candidate<1>(ctx, dt, in, out, step, xf, yf); break;
+    case 2: launch_candidate<2>(ctx, dt, in, out, step, xf, yf); break;
+    case 3: launch_candidate<3>(ctx, dt, in, out, step, xf, yf); break;
+    case 4: launch_candidate<4>(ctx, dt, in, out, step, xf, yf); break;
+    case 5: launch_candidate<5>(ctx, dt, in, out, step, xf, yf); break;
+    case 6: launch_candidate<6>(ctx, dt, in, out, step, xf, yf); break;
+    case 7: launch_candidate<7>(ctx, dt, in, out, step, xf, yf); break;
+    default: launch_candidate<8>(ctx, dt, in, out, step, xf, yf); break;
+  }
+  check(hipGetLastError(), "launch transport candidate");
 }
 
 void launch_settle_f32(Context& ctx, float* levels, size_t count, const double* largest,

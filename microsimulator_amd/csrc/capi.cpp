@@ -707,6 +707,198 @@ void validate_step(const Context& ctx, float dt) {
   }
 }
 
+// The world state's invariants (WorldState::validate, world_state.cpp:14-35)
+// for what a coupled step reads, and its species levels.
+void validate_cells(const msim_cells& cells, uint32_t species_count, const float* species) {
+  if (cells.count > 0 &&
+      (cells.position_x == nullptr || cells.position_y == nullptr ||
+       cells.position_z == nullptr || cells.lengths == nullptr || cells.radii == nullptr ||
+       cells.growth_rates == nullptr || cells.cell_types == nullptr)) {
+    invalid("cell arrays must not be null");
+  }
+  for (uint32_t c = 0; c < cells.count; ++c) {
+    const float v[6] = {cells.position_x[c], cells.position_y[c], cells.position_z[c],
+                        cells.lengths[c],    cells.radii[c],      cells.growth_rates[c]};
+    for (float x : v) {
+      if (!std::isfinite(x)) invalid("cell fields must be finite");
+    }
+    if (cells.lengths[c] < 0.0f) invalid("cell length must be non-negative");
+    if (cells.radii[c] <= 0.0f) invalid("cell radius must be positive");
+  }
+  const size_t levels = static_cast<size_t>(cells.count) * species_count;
+  if (levels > 0 && species == nullptr) invalid("species levels must not be null");
+  for (size_t i = 0; i < levels; ++i) {
+    if (!std::isfinite(species[i])) invalid("cell species levels must be finite");
+  }
+}
+
+// CoupledRatePlan::validate (coupled_rates.cpp:60-146).
+void validate_rate_plan(const msim_rate_plan& plan) {
+  if (plan.signal_count == 0) invalid("a coupled rate plan needs at least one signal");
+  if (plan.instruction_count == 0) invalid("a coupled rate plan needs instructions");
+  if (plan.instructions == nullptr || plan.signal_outputs == nullptr ||
+      (plan.species_count > 0 && plan.species_outputs == nullptr)) {
+    invalid("coupled rate plan arrays must not be null");
+  }
+  for (uint32_t index = 0; index < plan.instruction_count; ++index) {
+    const msim_rate_instruction& in = plan.instructions[index];
+    const char* bad = nullptr;
+    switch (in.operation) {
+      case MSIM_RATE_CONSTANT:
+        if (!std::isfinite(in.value)) invalid("coupled rate constants must be finite");
+        break;
+      case MSIM_RATE_SPECIES:
+        if (in.first >= plan.species_count) {
+          invalid("coupled rate instruction uses an unknown species");
+        }
+        break;
+      case MSIM_RATE_SIGNAL:
+        if (in.first >= plan.signal_count) invalid("coupled rate instruction uses an unknown signal");
+        break;
+      case MSIM_RATE_POSITION_X: case MSIM_RATE_POSITION_Y: case MSIM_RATE_POSITION_Z:
+      case MSIM_RATE_LENGTH: case MSIM_RATE_RADIUS: case MSIM_RATE_GROWTH_RATE:
+      case MSIM_RATE_CELL_TYPE: case MSIM_RATE_VOLUME: case MSIM_RATE_VOLUME_CHANGE_RATE:
+      case MSIM_RATE_SURFACE_AREA:
+        break;
+      case MSIM_RATE_NEGATE: case MSIM_RATE_EXPONENTIAL: case MSIM_RATE_LOGARITHM:
+        if (in.first >= index) bad = "unary";
+        break;
+      case MSIM_RATE_ADD: case MSIM_RATE_SUBTRACT: case MSIM_RATE_MULTIPLY: case MSIM_RATE_DIVIDE:
+      case MSIM_RATE_POWER: case MSIM_RATE_MINIMUM: case MSIM_RATE_MAXIMUM: case MSIM_RATE_LESS:
+      case MSIM_RATE_LESS_EQUAL: case MSIM_RATE_GREATER: case MSIM_RATE_GREATER_EQUAL:
+      case MSIM_RATE_EQUAL:
+        if (in.first >= index) bad = "first";
+        else if (in.second >= index) bad = "second";
+        break;
+      case MSIM_RATE_SELECT:
+        if (in.first >= index) bad = "condition";
+        else if (in.second >= index) bad = "true";
+        else if (in.third >= index) bad = "false";
+        break;
+      default:
+        invalid("coupled rate plan uses an unknown operation");
+    }
+    if (bad != nullptr) {
+      invalid(std::string("coupled rate ") + bad + " input must refer to an earlier instruction");
+    }
+  }
+  for (uint32_t k = 0; k < plan.species_count; ++k) {
+    if (plan.species_outputs[k] >= plan.instruction_count) {
+      invalid("coupled species output uses an unknown instruction");
+    }
+  }
+  for (uint32_t k = 0; k < plan.signal_count; ++k) {
+    if (plan.signal_outputs[k] >= plan.instruction_count) {
+      invalid("coupled signal output uses an unknown instruction");
+    }
+  }
+}
+
+// interpolation_axis's bound (signals.cpp:98-127): false outside the grid;
+// else the lower site.
+bool inside_axis(float position, float origin, float spacing, uint32_t dimension,
+                 uint32_t* lower) {
+  *lower = 0;
+  if (dimension == 1) return true;
+  const double coordinate =
+      (static_cast<double>(position) - static_cast<double>(origin)) / static_cast<double>(spacing);
+  if (coordinate < 0.0 || coordinate > static_cast<double>(dimension - 1)) return false;
+  *lower = static_cast<uint32_t>(std::floor(coordinate));
+  return true;
+}
+
+// The stencil checks of validate_coupled_step (cpu_coupled.cpp:112-122), in
+// cell order: each position inside the grid, and not with every corner of
+// positive weight solid.
+void validate_positions(const msim_signal_grid& g, const float* origin, const msim_cells& cells) {
+  const uint32_t nx = g.grid.nx, ny = g.grid.ny;
+  for (uint32_t c = 0; c < cells.count; ++c) {
+    uint32_t x0, y0;
+    if (!inside_axis(cells.position_x[c], origin[0], g.grid.dx, nx, &x0)) {
+      invalid("signal sample is outside the x grid bound");
+    }
+    if (!inside_axis(cells.position_y[c], origin[1], g.grid.dy, ny, &y0)) {
+      invalid("signal sample is outside the y grid bound");
+    }
+    if (g.grid.obstacles == nullptr) continue;
+    // Weights along an axis: 1 - f and f with f the fraction past the lower
+    // site (one site, weight 1, on the grid's upper edge or an axis of one).
+    const auto weights = [&](float position, float o, float spacing, uint32_t dimension,
+                             uint32_t lower, uint32_t count, float* w) {
+      w[0] = 1.0f;
+      w[1] = 0.0f;
+      if (dimension == 1 || count == 1 || lower == dimension - 1) return;
+      const double coordinate =
+          (static_cast<double>(position) - static_cast<double>(o)) / static_cast<double>(spacing);
+      const float fraction = static_cast<float>(coordinate - static_cast<double>(lower));
+      w[0] = 1.0f - fraction;
+      w[1] = fraction;
+    };
+    float wx[2], wy[2];
+    const uint32_t nxc = (nx == 1 || x0 == nx - 1) ? 1 : 2, nyc = (ny == 1 || y0 == ny - 1) ? 1 : 2;
+    weights(cells.position_x[c], origin[0], g.grid.dx, nx, x0, nxc, wx);
+    weights(cells.position_y[c], origin[1], g.grid.dy, ny, y0, nyc, wy);
+    bool fluid = false;
+    for (uint32_t a = 0; a < nxc; ++a) {
+      for (uint32_t b = 0; b < nyc; ++b) {
+        const size_t site = static_cast<size_t>(x0 + a) * ny + y0 + b;
+        if (g.grid.obstacles[site] == 0 && wx[a] * wy[b] > 0.0f) fluid = true;
+      }
+    }
+    if (!fluid) invalid("signal sample position is inside a grid obstacle");
+  }
+}
+
+// Everything validate_coupled_step checks (cpu_coupled.cpp:90-123) but the
+// time step, the explicit bound and the previous lengths, in its order, with
+// the grid's own checks (those of constructing it) first.
+void validate_coupled_inputs(const msim_signal_grid& g, const float* origin, const float* levels,
+                             const msim_cells& cells, const float* species,
+                             const msim_rate_plan& plan) {
+  for (int a = 0; a < 3; ++a) {
+    if (!std::isfinite(origin[a])) invalid("signal grid origin must be finite");
+  }
+  validate_cells(cells, plan.species_count, species);
+  validate_signal_grid(g, false);
+  validate_levels(g, levels);
+}
+
+// The plan-dependent checks of validate_coupled_step (cpu_coupled.cpp:97-104)
+// and this backend's limit on the plan.
+void validate_coupled_plan(const msim_signal_grid& g, const msim_rate_plan& plan) {
+  validate_rate_plan(plan);
+  if (plan.signal_count != g.signal_count) {
+    invalid("coupled rate plan and signal grid counts disagree");
+  }
+}
+
+void validate_previous(const msim_cells& cells, const float* previous) {
+  if (cells.count > 0 && previous == nullptr) invalid("previous lengths must not be null");
+  for (uint32_t c = 0; c < cells.count; ++c) {
+    if (!std::isfinite(previous[c]) || previous[c] < 0.0f) {
+      invalid("previous cell lengths must be finite and non-negative");
+    }
+  }
+}
+
+void check_coupled_limits(const msim_rate_plan& plan, const msim_cells& cells) {
+  if (plan.instruction_count > 256) {
+    throw Error(MSIM_UNSUPPORTED, "the HIP coupled step takes at most 256 rate instructions");
+  }
+  if (cells.count > (1u << 29)) {
+    throw Error(MSIM_OVERFLOW, "the HIP coupled step takes at most 2^29 cells");
+  }
+}
+
+void upload_coupled(Context& c, const msim_signal_grid& g, const float* origin,
+                    const float* levels, const msim_cells& cells, const float* species,
+                    const msim_rate_plan& plan) {
+  check_coupled_limits(plan, cells);
+  adr_upload(c, g, levels, 32, false);
+  record_losses(c, g);
+  coupled_upload(c, origin, cells, species, plan, g.grid.obstacles != nullptr);
+}
+
 }  // namespace
 }  // namespace msim
 
@@ -1134,6 +1326,82 @@ int msim_hip_adr_report(msim_ctx* ctx, msim_signal_report* report) {
     if (report == nullptr) msim::invalid("report must not be null");
     *report = c.adr.last_report;
   });
+}
+
+int msim_hip_advance_coupled(msim_ctx* ctx, const msim_signal_grid* grid, const float* origin,
+                             float* levels, const msim_cells* cells, float* species,
+                             const msim_rate_plan* plan, const float* previous_lengths,
+                             float dt, msim_signal_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || origin == nullptr || levels == nullptr || cells == nullptr ||
+        plan == nullptr) {
+      msim::invalid("grid, origin, levels, cells and plan must not be null");
+    }
+    // validate_coupled_step (cpu_coupled.cpp:90-123), in its order.
+    if (!std::isfinite(dt) || dt < 0.0f) {
+      msim::invalid("coupled time step must be finite and non-negative");
+    }
+    msim::validate_coupled_inputs(*grid, origin, levels, *cells, species, *plan);
+    c.coupled.uploaded = false;
+    msim::adr_upload(c, *grid, levels, 32, false);
+    msim::record_losses(c, *grid);
+    msim::validate_step(c, dt);
+    msim::validate_coupled_plan(*grid, *plan);
+    msim::validate_previous(*cells, previous_lengths);
+    msim::validate_positions(*grid, origin, *cells);
+    msim::check_coupled_limits(*plan, *cells);
+    msim::coupled_upload(c, origin, *cells, species, *plan, grid->grid.obstacles != nullptr);
+    msim::coupled_run(c, previous_lengths, dt, 1);
+    if (report != nullptr) *report = c.adr.last_report;
+    msim::coupled_download(c, levels, species);
+  });
+}
+
+int msim_hip_coupled_upload(msim_ctx* ctx, const msim_signal_grid* grid, const float* origin,
+                            const float* levels, const msim_cells* cells, const float* species,
+                            const msim_rate_plan* plan) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || origin == nullptr || levels == nullptr || cells == nullptr ||
+        plan == nullptr) {
+      msim::invalid("grid, origin, levels, cells and plan must not be null");
+    }
+    c.coupled.uploaded = false;
+    msim::validate_coupled_inputs(*grid, origin, levels, *cells, species, *plan);
+    msim::validate_coupled_plan(*grid, *plan);
+    msim::validate_positions(*grid, origin, *cells);
+    msim::upload_coupled(c, *grid, origin, levels, *cells, species, *plan);
+  });
+}
+
+int msim_hip_coupled_run(msim_ctx* ctx, const float* previous_lengths, float dt, uint32_t steps,
+                         msim_signal_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.coupled.uploaded || !c.adr.uploaded) msim::invalid("no coupled state on the device");
+    if (!std::isfinite(dt) || dt < 0.0f) {
+      msim::invalid("coupled time step must be finite and non-negative");
+    }
+    msim::validate_step(c, dt);
+    if (previous_lengths != nullptr) {
+      for (uint32_t k = 0; k < c.coupled.cells; ++k) {
+        if (!std::isfinite(previous_lengths[k]) || previous_lengths[k] < 0.0f) {
+          msim::invalid("previous cell lengths must be finite and non-negative");
+        }
+      }
+    }
+    if (steps > 0) msim::coupled_run(c, previous_lengths, dt, steps);
+    if (report != nullptr) *report = c.adr.last_report;
+  });
+}
+
+int msim_hip_coupled_download(msim_ctx* ctx, float* levels, float* species) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (!c.coupled.uploaded || !c.adr.uploaded) msim::invalid("no coupled state on the device");
+    msim::coupled_download(c, levels, species);
+  });
+}
+
+double msim_hip_coupled_seconds(msim_ctx* ctx) {
+  return ctx == nullptr ? 0.0 : ctx->impl.coupled.seconds;
 }
 
 }  // extern "C"

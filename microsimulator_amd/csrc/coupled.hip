@@ -1,0 +1,725 @@
+// Coupled cell/grid exchange on the device (advance_coupled_cpu,
+// cpp/cpu/cpu_coupled.cpp:125-231), one layer, FP32 levels.
+//
+// A step is these launches:
+//  * k_coupled_cells, one thread per cell: the cell's bilinear stencil (with
+//    obstacles, only the corners face-connected to the strongest fluid one,
+//    renormalised), the signals it samples from the step's starting levels,
+//    the dilution of its species, the rate plan run by an interpreter whose
+//    working values live in LDS (a column per thread), the species update,
+//    the exchange rate of each signal, and one (site, entry) pair per stencil
+//    corner;
+//  * a stable radix sort of the pairs by site, so each site's entries stand
+//    in cell order, and k_coupled_runs, which finds each site's run of them;
+//  * forward Euler: the explicit candidate (adr.hip), then k_coupled_commit,
+//    which adds dt * source to each level, the source summed over the site's
+//    run in cell order exactly as the reference adds it, and judges the
+//    levels (accept_step); the implicit kinds: k_coupled_sources writes the
+//    sources out for the solve's right-hand side.
+// Every kernel returns at once when an earlier failure set the transport
+// flags, so a run stops at its first refused step; the commit and source
+// kernels clear each site's run after reading it, so the runs array is all
+// zero between steps.
+#include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <bit>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace msim {
+namespace {
+
+constexpr int kSiteBlock = 256;
+constexpr uint32_t kCorners = 4;  // a one-layer stencil holds at most 2 x 2 sites
+constexpr unsigned long long kNoFailure = ~0ull;
+constexpr uint32_t kSpeciesFailure = 0xffffffffu;  // low word of a species-update failure
+constexpr uint32_t kStencilFailure = 0xfffffffeu;  // a position outside the grid (checked on the host)
+
+struct CellParams {
+  uint32_t nx, ny;
+  float dx, dy;
+  float ox, oy;
+  float volume;  // dx * dy * dz
+  float dt;
+  uint32_t cells, species, signals, instructions;
+  uint32_t has_mask;
+  uint32_t step;
+};
+
+// interpolation_axis (signals.cpp:98-127) for a position already known to lie
+// inside the grid.
+struct Axis {
+  uint32_t index[2];
+  float weight[2];
+  uint32_t count;
+  bool inside;
+};
+
+__device__ inline Axis axis_of(float position, float origin, float spacing, uint32_t dimension) {
+  Axis a{{0, 0}, {1.0f, 0.0f}, 1, true};
+  if (dimension == 1) return a;
+  const double coordinate =
+      (static_cast<double>(position) - static_cast<double>(origin)) / static_cast<double>(spacing);
+  if (!(coordinate >= 0.0 && coordinate <= static_cast<double>(dimension - 1))) {
+    a.inside = false;
+    return a;
+  }
+  const uint32_t lower = static_cast<uint32_t>(floor(coordinate));
+  if (lower == dimension - 1) {
+    a.index[0] = a.index[1] = lower;
+    return a;
+  }
+  const float fraction = static_cast<float>(coordinate - static_cast<double>(lower));
+  a.index[0] = lower;
+  a.index[1] = lower + 1;
+  a.weight[0] = 1.0f - fraction;
+  a.weight[1] = fraction;
+  a.count = 2;
+  return a;
+}
+
+// effective_cell_volume and effective_cell_surface_area (species.cpp:134-142).
+constexpr float kPi = static_cast<float>(3.14159265358979323846);
+__device__ inline float cell_volume(float length, float radius) {
+  return kPi * radius * radius * (length + 2.0f * radius);
+}
+__device__ inline float cell_surface(float length, float radius) {
+  return 2.0f * kPi * radius * (length + 2.0f * radius);
+}
+
+__device__ inline float pick(const float (&v)[kMaxSignals], uint32_t index) {
+  float out = v[0];
+#pragma unroll
+  for (uint32_t k = 1; k < kMaxSignals; ++k) out = index == k ? v[k] : out;
+  return out;
+}
+
+__device__ inline void fail_cell(unsigned long long* failure, uint32_t* flags, uint32_t cell,
+                                 uint32_t what, uint32_t step) {
+  atomicMin(failure, (static_cast<unsigned long long>(cell) << 32) | what);
+  atomicOr(&flags[0], kFlagCell);
+  atomicMin(&flags[1], step);
+}
+
+// One thread per cell (cpu_coupled.cpp:142-197 for one cell). The rate plan's
+// values sit in LDS, column threadIdx.x of an [instructions][blockDim] array.
+// The exponential, logarithm and power are evaluated in FP64 and rounded to
+// float.
+__global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
+                                const float* __restrict__ levels, const float* __restrict__ px,
+                                const float* __restrict__ py, const float* __restrict__ pz,
+                                const float* __restrict__ lengths, const float* __restrict__ radii,
+                                const float* __restrict__ growth, const int32_t* __restrict__ types,
+                                const float* __restrict__ previous,
+                                const msim_rate_instruction* __restrict__ plan,
+                                const uint32_t* __restrict__ outputs,
+                                const float* __restrict__ species_in, float* species_out,
+                                float* __restrict__ exchange, uint32_t* __restrict__ keys,
+                                uint32_t* __restrict__ entries, float* __restrict__ weights,
+                                unsigned long long* failure, uint32_t* flags) {
+  extern __shared__ float s_work[];
+  if (flags[0] != 0) return;  // an earlier step failed: the run stops there
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.cells) return;
+  const size_t n = static_cast<size_t>(P.nx) * P.ny;
+  // The stencil (signal_grid_stencil, signals.cpp:357-420): x-major corners.
+  const Axis ax = axis_of(px[c], P.ox, P.dx, P.nx);
+  const Axis ay = axis_of(py[c], P.oy, P.dy, P.ny);
+  if (!ax.inside || !ay.inside) {
+    fail_cell(failure, flags, c, kStencilFailure, P.step);
+    return;
+  }
+  uint32_t site[kCorners], cx[kCorners], cy[kCorners];
+  float w[kCorners];
+  const uint32_t count = ax.count * ay.count;
+#pragma unroll
+  for (uint32_t e = 0; e < kCorners; ++e) {
+    // Corner e is (e / yc, e % yc) for the y axis's yc sites.
+    const uint32_t a = e < count ? e / ay.count : 0, b = e < count ? e % ay.count : 0;
+    const uint32_t xi = a != 0 ? ax.index[1] : ax.index[0];
+    const uint32_t yi = b != 0 ? ay.index[1] : ay.index[0];
+    cx[e] = xi;
+    cy[e] = yi;
+    site[e] = xi * P.ny + yi;
+    w[e] = e < count ? (a != 0 ? ax.weight[1] : ax.weight[0]) * (b != 0 ? ay.weight[1] : ay.weight[0])
+                     : 0.0f;
+  }
+  if (P.has_mask != 0) {
+    uint32_t seed = count;
+    float best = 0.0f;
+#pragma unroll
+    for (uint32_t e = 0; e < kCorners; ++e) {
+      if (e < count) {
+        if (solid[site[e]] != 0) w[e] = 0.0f;
+        if (w[e] > 0.0f && (seed == count || w[e] > best)) {
+          seed = e;
+          best = w[e];
+        }
+      }
+    }
+    if (seed == count) {
+      fail_cell(failure, flags, c, kStencilFailure, P.step);
+      return;
+    }
+    bool connected[kCorners] = {false, false, false, false};
+#pragma unroll
+    for (uint32_t e = 0; e < kCorners; ++e) connected[e] = e == seed;
+    for (bool grew = true; grew;) {
+      grew = false;
+#pragma unroll
+      for (uint32_t i = 0; i < kCorners; ++i) {
+        if (i >= count || w[i] <= 0.0f || connected[i]) continue;
+#pragma unroll
+        for (uint32_t j = 0; j < kCorners; ++j) {
+          if (j >= count || !connected[j] || connected[i]) continue;
+          const uint32_t d = (cx[i] > cx[j] ? cx[i] - cx[j] : cx[j] - cx[i]) +
+                             (cy[i] > cy[j] ? cy[i] - cy[j] : cy[j] - cy[i]);
+          if (d == 1) {
+            connected[i] = true;
+            grew = true;
+          }
+        }
+      }
+    }
+    float total = 0.0f;
+#pragma unroll
+    for (uint32_t e = 0; e < kCorners; ++e) {
+      if (e < count) {
+        if (!connected[e]) w[e] = 0.0f;
+        total += w[e];
+      }
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < kCorners; ++e) {
+      if (e < count) w[e] /= total;
+    }
+  }
+  // The sampled signals (cpu_coupled.cpp:150-160).
+  const uint32_t S = P.signals;
+  float sampled[kMaxSignals];
+#pragma unroll
+  for (uint32_t s = 0; s < kMaxSignals; ++s) {
+    float v = 0.0f;
+    if (s < S) {
+#pragma unroll
+      for (uint32_t e = 0; e < kCorners; ++e) {
+        if (e < count) v += w[e] * levels[s * n + site[e]];
+      }
+    }
+    sampled[s] = v;
+  }
+  // Dilution by the growth since the previous step (162-170).
+  const float length = lengths[c], radius = radii[c];
+  const float previous_volume = cell_volume(previous[c], radius);
+  const float current_volume = cell_volume(length, radius);
+  const float dilution = previous_volume / current_volume;
+  const uint32_t K = P.species;
+  const size_t base = static_cast<size_t>(c) * K;
+  for (uint32_t k = 0; k < K; ++k) species_out[base + k] = species_in[base + k] * dilution;
+  const float volume_change_rate =
+      P.dt == 0.0f ? 0.0f : (current_volume - previous_volume) / P.dt;
+  // The rate plan (evaluate_instruction, cpu_coupled.cpp:20-85).
+  float* work = s_work + threadIdx.x;
+  const uint32_t T = blockDim.x;
+  for (uint32_t index = 0; index < P.instructions; ++index) {
+    const msim_rate_instruction in = plan[index];
+    const float a = in.operation >= MSIM_RATE_ADD && in.operation <= MSIM_RATE_SELECT
+                        ? work[in.first * T]
+                        : 0.0f;
+    const float b = in.operation >= MSIM_RATE_ADD && in.operation <= MSIM_RATE_SELECT &&
+                            in.operation != MSIM_RATE_NEGATE &&
+                            in.operation != MSIM_RATE_EXPONENTIAL &&
+                            in.operation != MSIM_RATE_LOGARITHM
+                        ? work[in.second * T]
+                        : 0.0f;
+    float v;
+    switch (in.operation) {
+      case MSIM_RATE_CONSTANT: v = in.value; break;
+      case MSIM_RATE_SPECIES: v = species_out[base + in.first]; break;
+      case MSIM_RATE_SIGNAL: v = pick(sampled, in.first); break;
+      case MSIM_RATE_POSITION_X: v = px[c]; break;
+      case MSIM_RATE_POSITION_Y: v = py[c]; break;
+      case MSIM_RATE_POSITION_Z: v = pz[c]; break;
+      case MSIM_RATE_LENGTH: v = length; break;
+      case MSIM_RATE_RADIUS: v = radius; break;
+      case MSIM_RATE_GROWTH_RATE: v = growth[c]; break;
+      case MSIM_RATE_CELL_TYPE: v = static_cast<float>(types[c]); break;
+      case MSIM_RATE_VOLUME_CHANGE_RATE: v = volume_change_rate; break;
+      case MSIM_RATE_VOLUME: v = current_volume; break;
+      case MSIM_RATE_SURFACE_AREA: v = cell_surface(length, radius); break;
+      case MSIM_RATE_ADD: v = a + b; break;
+      case MSIM_RATE_SUBTRACT: v = a - b; break;
+      case MSIM_RATE_MULTIPLY: v = a * b; break;
+      case MSIM_RATE_DIVIDE: v = a / b; break;
+      case MSIM_RATE_POWER:
+        v = static_cast<float>(pow(static_cast<double>(a), static_cast<double>(b)));
+        break;
+      case MSIM_RATE_MINIMUM: v = b < a ? b : a; break;
+      case MSIM_RATE_MAXIMUM: v = a < b ? b : a; break;
+      case MSIM_RATE_NEGATE: v = -a; break;
+      case MSIM_RATE_EXPONENTIAL: v = static_cast<float>(exp(static_cast<double>(a))); break;
+      case MSIM_RATE_LOGARITHM: v = static_cast<float>(log(static_cast<double>(a))); break;
+      case MSIM_RATE_LESS: v = a < b ? 1.0f : 0.0f; break;
+      case MSIM_RATE_LESS_EQUAL: v = a <= b ? 1.0f : 0.0f; break;
+      case MSIM_RATE_GREATER: v = a > b ? 1.0f : 0.0f; break;
+      case MSIM_RATE_GREATER_EQUAL: v = a >= b ? 1.0f : 0.0f; break;
+      case MSIM_RATE_EQUAL: v = a == b ? 1.0f : 0.0f; break;
+      default: v = a != 0.0f ? b : work[in.third * T]; break;  // MSIM_RATE_SELECT
+    }
+    work[index * T] = v;
+    if (!isfinite(v)) {
+      fail_cell(failure, flags, c, index, P.step);
+      return;
+    }
+  }
+  // The species update (cpu_coupled.cpp:180-187) and the exchange (188-191).
+  for (uint32_t k = 0; k < K; ++k) {
+    const float v = species_out[base + k] + P.dt * work[outputs[k] * T];
+    if (!isfinite(v)) {
+      fail_cell(failure, flags, c, kSpeciesFailure, P.step);
+      return;
+    }
+    species_out[base + k] = v;
+  }
+  for (uint32_t s = 0; s < S; ++s) {
+    exchange[static_cast<size_t>(c) * S + s] = work[outputs[K + s] * T] / P.volume;
+  }
+#pragma unroll
+  for (uint32_t e = 0; e < kCorners; ++e) {
+    const size_t j = static_cast<size_t>(c) * kCorners + e;
+    keys[j] = e < count ? site[e] : static_cast<uint32_t>(n);
+    entries[j] = static_cast<uint32_t>(j);
+    weights[j] = e < count ? w[e] : 0.0f;
+  }
+}
+
+// Each site's run [begin, end) in the sorted pairs; sites without pairs keep
+// the zero run their last reader left.
+__global__ __launch_bounds__(kSiteBlock) void k_coupled_runs(const uint32_t* __restrict__ keys,
+                                                             size_t pairs, uint32_t sites,
+                                                             uint32_t* __restrict__ runs,
+                                                             const uint32_t* flags) {
+  if (flags[0] != 0) return;
+  const size_t j = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
+  if (j >= pairs) return;
+  const uint32_t k = keys[j];
+  if (k >= sites) return;  // an unused corner
+  if (j == 0 || keys[j - 1] != k) runs[2 * static_cast<size_t>(k)] = static_cast<uint32_t>(j);
+  if (j + 1 == pairs || keys[j + 1] != k) runs[2 * static_cast<size_t>(k) + 1] = static_cast<uint32_t>(j + 1);
+}
+
+// The sources of one site: per signal the sum over its run, in cell order,
+// of weight * exchange (cpu_coupled.cpp:198-207), from 0. Clears the run.
+template <int S>
+__device__ inline void site_sources(size_t site, uint32_t* __restrict__ runs,
+                                    const uint32_t* __restrict__ entries,
+                                    const float* __restrict__ weights,
+                                    const float* __restrict__ exchange, float (&src)[S]) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) src[s] = 0.0f;
+  const uint32_t begin = runs[2 * site], end = runs[2 * site + 1];
+  if (end == 0) return;
+  for (uint32_t j = begin; j < end; ++j) {
+    const uint32_t entry = entries[j];
+    const float w = weights[entry];
+    const float* ex = exchange + static_cast<size_t>(entry / kCorners) * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) src[s] += w * ex[s];
+  }
+  runs[2 * site] = 0;
+  runs[2 * site + 1] = 0;
+}
+
+// Forward Euler's commit: level + dt * source for every level of the site,
+// obstacle sites included (cpu_coupled.cpp:216-222), then the levels judged
+// as accept_step does: under upwind each must be finite and non-negative;
+// under van Leer the block's largest level goes to `largest` for the settle.
+// Writes nothing once a failure is flagged, but still clears the runs.
+template <int S, bool kLimited>
+__global__ __launch_bounds__(kSiteBlock) void k_coupled_commit(
+    size_t n, const float* in, float* out, float dt, uint32_t* __restrict__ runs,
+    const uint32_t* __restrict__ entries, const float* __restrict__ weights,
+    const float* __restrict__ exchange, double* __restrict__ largest, uint32_t* flags,
+    uint32_t step) {
+  __shared__ double s_red[kSiteBlock / kWave];
+  const bool failed = flags[0] != 0;
+  const size_t site = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
+  double top = 0.0;
+  bool bad = false;
+  if (site < n) {
+    float src[S];
+    site_sources<S>(site, runs, entries, weights, exchange, src);
+    if (!failed) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const float v = in[s * n + site] + dt * src[s];
+        out[s * n + site] = v;
+        if constexpr (kLimited) {
+          top = fmax(top, static_cast<double>(v));
+        } else {
+          bad = bad || !isfinite(v) || v < 0.0f;
+        }
+      }
+    }
+  }
+  if constexpr (kLimited) {
+    top = block_max<kSiteBlock>(top, s_red);
+    if (threadIdx.x == 0) largest[blockIdx.x] = top;
+  } else if (bad) {
+    atomicOr(&flags[0], kFlagLevels);
+    atomicMin(&flags[1], step);
+  }
+}
+
+// The implicit kinds' sources, S*N (a non-finite one refuses the step,
+// signals.cpp:1295-1310). Clears the runs.
+template <int S>
+__global__ __launch_bounds__(kSiteBlock) void k_coupled_sources(
+    size_t n, float* __restrict__ sources, uint32_t* __restrict__ runs,
+    const uint32_t* __restrict__ entries, const float* __restrict__ weights,
+    const float* __restrict__ exchange, uint32_t* flags, uint32_t step) {
+  const size_t site = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
+  if (site >= n) return;
+  float src[S];
+  site_sources<S>(site, runs, entries, weights, exchange, src);
+  bool bad = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    sources[s * n + site] = src[s];
+    bad = bad || !isfinite(src[s]);
+  }
+  if (bad) {
+    atomicOr(&flags[0], kFlagSources);
+    atomicMin(&flags[1], step);
+  }
+}
+
+__global__ __launch_bounds__(kSiteBlock) void k_fill_u32(uint32_t* __restrict__ out, size_t count,
+                                                         uint32_t value) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
+  if (i < count) out[i] = value;
+}
+
+__global__ void k_fill_u64(unsigned long long* out, unsigned long long value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = value;
+}
+
+uint32_t blocks_of(size_t count) {
+  return static_cast<uint32_t>((count + kSiteBlock - 1) / kSiteBlock);
+}
+
+// Threads per block of the cell kernel: as many as fit the plan's values in
+// 64 KiB of LDS.
+uint32_t cell_threads(uint32_t instructions) {
+  if (instructions <= 64) return 256;
+  if (instructions <= 128) return 128;
+  return 64;
+}
+
+template <int S>
+void launch_commit(Context& ctx, bool limited, const float* in, float* out, float dt,
+                   uint32_t step) {
+  auto& c = ctx.coupled;
+  const size_t n = ctx.adr.sites();
+  if (limited) {
+    k_coupled_commit<S, true><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+        n, in, out, dt, c.runs.data(), c.sorted_entries.data(), c.weights.data(),
+        c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
+  } else {
+    k_coupled_commit<S, false><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+        n, in, out, dt, c.runs.data(), c.sorted_entries.data(), c.weights.data(),
+        c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
+  }
+}
+
+template <int S>
+void launch_sources(Context& ctx, uint32_t step) {
+  auto& c = ctx.coupled;
+  const size_t n = ctx.adr.sites();
+  k_coupled_sources<S><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+      n, c.sources.data(), c.runs.data(), c.sorted_entries.data(), c.weights.data(),
+      c.exchange.data(), ctx.adr.flags.data(), step);
+}
+
+void commit(Context& ctx, bool limited, const float* in, float* out, float dt, uint32_t step) {
+  switch (ctx.adr.params.signals) {
+    case 1: launch_commit<1>(ctx, limited, in, out, dt, step); break;
+    case 2: launch_commit<2>(ctx, limited, in, out, dt, step); break;
+    case 3: launch_commit<3>(ctx, limited, in, out, dt, step); break;
+    case 4: launch_commit<4>(ctx, limited, in, out, dt, step); break;
+    case 5: launch_commit<5>(ctx, limited, in, out, dt, step); break;
+    case 6: launch_commit<6>(ctx, limited, in, out, dt, step); break;
+    case 7: launch_commit<7>(ctx, limited, in, out, dt, step); break;
+    default: launch_commit<8>(ctx, limited, in, out, dt, step); break;
+  }
+  check(hipGetLastError(), "launch k_coupled_commit");
+}
+
+void sources(Context& ctx, uint32_t step) {
+  switch (ctx.adr.params.signals) {
+    case 1: launch_sources<1>(ctx, step); break;
+    case 2: launch_sources<2>(ctx, step); break;
+    case 3: launch_sources<3>(ctx, step); break;
+    case 4: launch_sources<4>(ctx, step); break;
+    case 5: launch_sources<5>(ctx, step); break;
+    case 6: launch_sources<6>(ctx, step); break;
+    case 7: launch_sources<7>(ctx, step); break;
+    default: launch_sources<8>(ctx, step); break;
+  }
+  check(hipGetLastError(), "launch k_coupled_sources");
+}
+
+// The cell kernel, the pair sort and the runs of one step.
+void cell_phase(Context& ctx, const float* levels, const float* previous, float dt,
+                uint32_t step) {
+  auto& c = ctx.coupled;
+  auto& a = ctx.adr;
+  const size_t n = a.sites();
+  CellParams P{};
+  P.nx = a.params.nx;
+  P.ny = a.params.ny;
+  P.dx = a.params.dx;
+  P.dy = a.params.dy;
+  P.ox = c.origin[0];
+  P.oy = c.origin[1];
+  P.volume = a.params.dx * a.params.dy * a.params.dz;
+  P.dt = dt;
+  P.cells = c.cells;
+  P.species = c.species;
+  P.signals = c.signals;
+  P.instructions = c.instructions;
+  P.has_mask = c.has_mask ? 1u : 0u;
+  P.step = step;
+  const float* species_in = c.species_current == 0 ? c.species_a.data() : c.species_b.data();
+  float* species_out = c.species_current == 0 ? c.species_b.data() : c.species_a.data();
+  if (c.cells > 0) {
+    const uint32_t threads = cell_threads(c.instructions);
+    const size_t lds = static_cast<size_t>(c.instructions) * threads * sizeof(float);
+    k_coupled_cells<<<(c.cells + threads - 1) / threads, threads, lds, ctx.stream>>>(
+        P, a.solid.data(), levels, c.position_x.data(), c.position_y.data(),
+        c.position_z.data(), c.lengths.data(), c.radii.data(), c.growth.data(), c.types.data(),
+        previous, c.plan.data(), c.outputs.data(), species_in, species_out, c.exchange.data(),
+        c.keys.data(), c.entries.data(), c.weights.data(), c.failure.data(), a.flags.data());
+    check(hipGetLastError(), "launch k_coupled_cells");
+    const size_t pairs = static_cast<size_t>(c.cells) * kCorners;
+    const int end_bit = std::max(1, static_cast<int>(std::bit_width(static_cast<uint64_t>(n))));
+    size_t bytes = 0;
+    check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
+                                             c.entries.data(), c.sorted_entries.data(),
+                                             static_cast<int>(pairs), 0, end_bit, ctx.stream),
+          "size coupled sort");
+    if (bytes > c.sort_scratch.size()) c.sort_scratch.reserve(bytes);
+    bytes = c.sort_scratch.size();
+    check(hipcub::DeviceRadixSort::SortPairs(c.sort_scratch.data(), bytes, c.keys.data(),
+                                             c.sorted_keys.data(), c.entries.data(),
+                                             c.sorted_entries.data(), static_cast<int>(pairs), 0,
+                                             end_bit, ctx.stream),
+          "sort coupled pairs");
+    k_coupled_runs<<<blocks_of(pairs), kSiteBlock, 0, ctx.stream>>>(
+        c.sorted_keys.data(), pairs, static_cast<uint32_t>(n), c.runs.data(), a.flags.data());
+    check(hipGetLastError(), "launch k_coupled_runs");
+  }
+}
+
+// The failure the flags and the failure word name, as the reference reports it.
+[[noreturn]] void throw_failure(uint32_t flag_bits, unsigned long long failure, bool limited) {
+  if ((flag_bits & kFlagCell) != 0) {
+    const uint32_t what = static_cast<uint32_t>(failure & 0xffffffffull);
+    if (what == kSpeciesFailure) {
+      throw Error(MSIM_DOMAIN_ERROR, "coupled species update produced a non-finite level");
+    }
+    if (what == kStencilFailure) {
+      throw Error(MSIM_INVALID_ARGUMENT, "signal sample position is outside the grid");
+    }
+    throw Error(MSIM_DOMAIN_ERROR,
+                "coupled rate instruction " + std::to_string(what) + " produced a non-finite value");
+  }
+  if ((flag_bits & kFlagSources) != 0) {
+    throw Error(MSIM_INVALID_ARGUMENT, "signal source rates must be finite");
+  }
+  if ((flag_bits & kFlagNegative) != 0) {
+    throw Error(MSIM_DOMAIN_ERROR,
+                "signal grid update produced a non-finite or negative concentration");
+  }
+  static_cast<void>(limited);
+  throw Error(MSIM_INVALID_ARGUMENT, "signal grid levels must be finite and non-negative");
+}
+
+}  // namespace
+
+void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
+                    const float* species, const msim_rate_plan& plan, bool has_mask) {
+  auto& c = ctx.coupled;
+  const auto& a = ctx.adr;
+  c.uploaded = false;
+  c.cells = cells.count;
+  c.species = plan.species_count;
+  c.signals = plan.signal_count;
+  c.instructions = plan.instruction_count;
+  c.has_mask = has_mask;
+  std::memcpy(c.origin, origin, sizeof(c.origin));
+  const size_t count = cells.count, n = a.sites();
+  const auto put = [&](DeviceBuffer<float>& buffer, const float* host, size_t items) {
+    buffer.reserve(items);
+    if (items > 0) {
+      check(hipMemcpyAsync(buffer.data(), host, items * sizeof(float), hipMemcpyHostToDevice,
+                           ctx.stream),
+            "upload cells");
+    }
+  };
+  put(c.position_x, cells.position_x, count);
+  put(c.position_y, cells.position_y, count);
+  put(c.position_z, cells.position_z, count);
+  put(c.lengths, cells.lengths, count);
+  put(c.radii, cells.radii, count);
+  put(c.growth, cells.growth_rates, count);
+  c.types.reserve(count);
+  if (count > 0) {
+    check(hipMemcpyAsync(c.types.data(), cells.cell_types, count * sizeof(int32_t),
+                         hipMemcpyHostToDevice, ctx.stream),
+          "upload cells");
+  }
+  c.previous.reserve(count);
+  put(c.species_a, species, count * plan.species_count);
+  c.species_b.reserve(count * plan.species_count);
+  c.species_current = 0;
+  c.plan.reserve(plan.instruction_count);
+  check(hipMemcpyAsync(c.plan.data(), plan.instructions,
+                       plan.instruction_count * sizeof(msim_rate_instruction),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload plan");
+  std::vector<uint32_t> outputs(plan.species_outputs, plan.species_outputs + plan.species_count);
+  outputs.insert(outputs.end(), plan.signal_outputs, plan.signal_outputs + plan.signal_count);
+  c.outputs.reserve(outputs.size());
+  check(hipMemcpyAsync(c.outputs.data(), outputs.data(), outputs.size() * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload plan");
+  const size_t pairs = count * kCorners;
+  for (auto* b : {&c.keys, &c.entries, &c.sorted_keys, &c.sorted_entries}) b->reserve(pairs);
+  c.weights.reserve(pairs);
+  c.exchange.reserve(count * plan.signal_count);
+  c.sources.reserve(static_cast<size_t>(plan.signal_count) * n);
+  c.runs.reserve(2 * n);
+  c.largest.reserve(blocks_of(n));
+  c.failure.reserve(1);
+  size_t bytes = 0;
+  check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
+                                           c.entries.data(), c.sorted_entries.data(),
+                                           static_cast<int>(std::max<size_t>(pairs, 1)), 0, 32,
+                                           ctx.stream),
+        "size coupled sort");
+  c.sort_scratch.reserve(bytes);
+  k_fill_u32<<<blocks_of(2 * n), kSiteBlock, 0, ctx.stream>>>(c.runs.data(), 2 * n, 0u);
+  check(hipGetLastError(), "launch k_fill_u32");
+  check(hipStreamSynchronize(ctx.stream), "upload coupled state");
+  c.uploaded = true;
+}
+
+void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t steps) {
+  auto& c = ctx.coupled;
+  auto& a = ctx.adr;
+  const size_t n = a.sites();
+  const size_t count = static_cast<size_t>(a.params.signals) * n;
+  const bool limited = a.params.limited != 0;
+  const bool implicit = a.integration != MSIM_FORWARD_EULER;
+  const float* xf = a.flow_field ? ctx.flow.faces.data() : a.xf.data();
+  const float* yf = a.flow_field ? ctx.flow.faces.data() + ctx.flow.x_faces() : a.yf.data();
+  // The first step's previous lengths; later steps' are the cells' own.
+  if (c.cells > 0) {
+    if (previous_lengths != nullptr) {
+      check(hipMemcpyAsync(c.previous.data(), previous_lengths, c.cells * sizeof(float),
+                           hipMemcpyHostToDevice, ctx.stream),
+            "upload previous lengths");
+    } else {
+      check(hipMemcpyAsync(c.previous.data(), c.lengths.data(), c.cells * sizeof(float),
+                           hipMemcpyDeviceToDevice, ctx.stream),
+            "copy lengths");
+    }
+  }
+  const uint32_t init_flags[2] = {0u, 0xffffffffu};
+  check(hipMemcpyAsync(a.flags.data(), init_flags, sizeof(init_flags), hipMemcpyHostToDevice,
+                       ctx.stream),
+        "clear flags");
+  k_fill_u64<<<1, 64, 0, ctx.stream>>>(c.failure.data(), kNoFailure);
+  check(hipGetLastError(), "launch k_fill_u64");
+  a.last_report = msim_signal_report{1, 0, 0.0f};
+  const int species_start = c.species_current, levels_start = a.current;
+  check(hipEventRecord(ctx.start, ctx.stream), "event");
+  uint32_t flag_bits[2] = {0, 0};
+  unsigned long long failure = kNoFailure;
+  const auto read_flags = [&]() {
+    check(hipMemcpyAsync(flag_bits, a.flags.data(), sizeof(flag_bits), hipMemcpyDeviceToHost,
+                         ctx.stream),
+          "read flags");
+    check(hipMemcpyAsync(&failure, c.failure.data(), sizeof(failure), hipMemcpyDeviceToHost,
+                         ctx.stream),
+          "read failure");
+    check(hipStreamSynchronize(ctx.stream), "coupled step");
+  };
+  for (uint32_t step = 0; step < steps; ++step) {
+    float* level_bufs[2] = {reinterpret_cast<float*>(a.levels_a.data()),
+                            reinterpret_cast<float*>(a.levels_b.data())};
+    float* old = level_bufs[a.current];
+    float* next = level_bufs[a.current ^ 1];
+    if (step == 1 && c.cells > 0) {
+      check(hipMemcpyAsync(c.previous.data(), c.lengths.data(), c.cells * sizeof(float),
+                           hipMemcpyDeviceToDevice, ctx.stream),
+            "copy lengths");
+    }
+    cell_phase(ctx, old, c.previous.data(), dt, step);
+    if (!implicit) {
+      if (dt > 0.0f) {
+        fe_candidate_f32(ctx, dt, old, next, step, xf, yf);
+        commit(ctx, limited, next, next, dt, step);
+      } else {
+        commit(ctx, limited, old, next, dt, step);
+      }
+      if (limited) launch_settle_f32(ctx, next, count, c.largest.data(), blocks_of(n), step);
+      a.current ^= 1;
+      c.species_current ^= 1;
+      continue;
+    }
+    // The implicit kinds: the sources, judged on the host before the solve.
+    sources(ctx, step);
+    read_flags();
+    if (flag_bits[0] != 0) throw_failure(flag_bits[0], failure, limited);
+    if (dt > 0.0f) {
+      implicit_run(ctx, dt, 1, xf, yf, c.sources.data(), "Implicit coupled signal solve");
+    }
+    c.species_current ^= 1;
+  }
+  check(hipEventRecord(ctx.stop, ctx.stream), "event");
+  read_flags();
+  float ms = 0.0f;
+  check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+  c.seconds = ms * 1.0e-3;
+  if (flag_bits[0] != 0) {
+    // Every step from the failed one on did nothing: the state is the one
+    // that step started from.
+    const uint32_t failed = flag_bits[1];
+    a.current = levels_start ^ static_cast<int>(failed & 1u);
+    c.species_current = species_start ^ static_cast<int>(failed & 1u);
+    throw_failure(flag_bits[0], failure, limited);
+  }
+}
+
+void coupled_download(Context& ctx, float* levels, float* species) {
+  auto& c = ctx.coupled;
+  if (levels != nullptr) adr_download(ctx, levels);
+  if (species != nullptr && c.cells > 0 && c.species > 0) {
+    const float* src = c.species_current == 0 ? c.species_a.data() : c.species_b.data();
+    check(hipMemcpyAsync(species, src, static_cast<size_t>(c.cells) * c.species * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download species");
+    check(hipStreamSynchronize(ctx.stream), "download species");
+  }
+}
+
+}  // namespace msim

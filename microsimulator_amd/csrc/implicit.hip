@@ -80,16 +80,17 @@ __device__ inline float site_rate(const AdrParams& P, const uint8_t* __restrict_
 // The step's fixed parts from the levels it starts from (signals.cpp:
 // 1320-1343): with Monod uptake the lagged loss vmax*phi/(K+c) (added to any
 // affine loss) into `lagged`; the right-hand side
-// old + explicit_dt * rate(old) + dt * 0; and per block the sum of squares of
+// old + explicit_dt * rate(old) + dt * source, with the coupled step's
+// sources (zero without); and per block the sum of squares of
 // the right-hand side (for its RMS) and the least operator diagonal (for the
 // operator scale).
 template <int S, bool kLimited, bool kPatched>
 __global__ __launch_bounds__(kImplicitBlock) void k_implicit_setup(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
-    const float* __restrict__ loss, const float* __restrict__ old, float* __restrict__ rhs,
-    float* __restrict__ lagged, float explicit_dt, float dt, double* __restrict__ sums,
-    float* __restrict__ least) {
+    const float* __restrict__ loss, const float* __restrict__ coupled,
+    const float* __restrict__ old, float* __restrict__ rhs, float* __restrict__ lagged,
+    float explicit_dt, float dt, double* __restrict__ sums, float* __restrict__ least) {
   __shared__ double s_red[kImplicitBlock / kWave];
   __shared__ float s_min[kImplicitBlock / kWave];
   const uint32_t nx = P.nx, ny = P.ny;
@@ -115,7 +116,8 @@ __global__ __launch_bounds__(kImplicitBlock) void k_implicit_setup(
       }
       const bool reaction = affine || monod;
       if (reaction) rate += (affine ? source[s * n + i] : 0.0f) - (L * v.c);
-      const float r = (v.c + (explicit_dt * rate)) + (dt * 0.0f);
+      const float r = (v.c + (explicit_dt * rate)) + (dt * (coupled != nullptr ? coupled[s * n + i]
+                                                                              : 0.0f));
       rhs[s * n + i] = r;
       acc += static_cast<double>(r) * static_cast<double>(r);
       low = fminf(low, operator_diagonal_f32(P, v.g, s, reaction, L));
@@ -269,11 +271,11 @@ __global__ __launch_bounds__(kImplicitBlock) void k_level_max(const float* __res
 
 template <int S, bool kLimited, bool kPatched>
 void launch_setup(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
-                  const float* old, float explicit_dt, float dt) {
+                  const float* sources, const float* old, float explicit_dt, float dt) {
   auto& a = ctx.adr;
   k_implicit_setup<S, kLimited, kPatched><<<blocks, kImplicitBlock, 0, ctx.stream>>>(
-      a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(), old,
-      a.rhs.data(), a.lagged.data(), explicit_dt, dt, a.implicit_sums.data(),
+      a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(), sources,
+      old, a.rhs.data(), a.lagged.data(), explicit_dt, dt, a.implicit_sums.data(),
       a.implicit_least.data());
 }
 
@@ -288,7 +290,8 @@ void launch_sweep(Context& ctx, uint32_t blocks, const float* xf, const float* y
 }
 
 template <int S, bool kLimited, bool kPatched>
-void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf) {
+void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf,
+                    const float* sources, const char* label) {
   auto& a = ctx.adr;
   const size_t n = a.sites();
   const size_t count = static_cast<size_t>(S) * n;
@@ -313,7 +316,7 @@ void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, con
             "copy levels");
     }
     auto iterate = [&](uint32_t k) { return k == 0 ? old : work[k & 1u]; };
-    launch_setup<S, kLimited, kPatched>(ctx, blocks, xf, yf, old, explicit_dt, dt);
+    launch_setup<S, kLimited, kPatched>(ctx, blocks, xf, yf, sources, old, explicit_dt, dt);
     check(hipGetLastError(), "launch k_implicit_setup");
     k_implicit_begin<<<1, kImplicitBlock, 0, ctx.stream>>>(
         a.implicit_sums.data(), a.implicit_least.data(), blocks, levels, half_dt,
@@ -353,7 +356,7 @@ void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, con
       throw Error(MSIM_INVALID_ARGUMENT, "signal transport levels must be finite");
     }
     if (host.converged == 0) {
-      std::string message = "Implicit signal solve did not converge after " +
+      std::string message = std::string(label) + " did not converge after " +
                             std::to_string(a.last_report.iterations) + " iterations";
       if (kLimited) {
         message +=
@@ -389,21 +392,23 @@ void implicit_steps(Context& ctx, float dt, uint32_t steps, const float* xf, con
 }
 
 template <int S>
-void implicit_typed(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf) {
+void implicit_typed(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf,
+                    const float* sources, const char* label) {
   const auto& P = ctx.adr.params;
   const bool limited = P.limited != 0, patched = P.edge != nullptr;
   if (limited) {
-    if (patched) implicit_steps<S, true, true>(ctx, dt, steps, xf, yf);
-    else implicit_steps<S, true, false>(ctx, dt, steps, xf, yf);
+    if (patched) implicit_steps<S, true, true>(ctx, dt, steps, xf, yf, sources, label);
+    else implicit_steps<S, true, false>(ctx, dt, steps, xf, yf, sources, label);
   } else {
-    if (patched) implicit_steps<S, false, true>(ctx, dt, steps, xf, yf);
-    else implicit_steps<S, false, false>(ctx, dt, steps, xf, yf);
+    if (patched) implicit_steps<S, false, true>(ctx, dt, steps, xf, yf, sources, label);
+    else implicit_steps<S, false, false>(ctx, dt, steps, xf, yf, sources, label);
   }
 }
 
 }  // namespace
 
-void implicit_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf) {
+void implicit_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf,
+                  const float* sources, const char* label) {
   auto& a = ctx.adr;
   const size_t n = a.sites();
   const uint32_t blocks = static_cast<uint32_t>((n + kImplicitBlock - 1) / kImplicitBlock);
@@ -414,14 +419,14 @@ void implicit_run(Context& ctx, float dt, uint32_t steps, const float* xf, const
   a.implicit_least.reserve(blocks);
   a.implicit_state.reserve(1);
   switch (a.params.signals) {
-    case 1: implicit_typed<1>(ctx, dt, steps, xf, yf); break;
-    case 2: implicit_typed<2>(ctx, dt, steps, xf, yf); break;
-    case 3: implicit_typed<3>(ctx, dt, steps, xf, yf); break;
-    case 4: implicit_typed<4>(ctx, dt, steps, xf, yf); break;
-    case 5: implicit_typed<5>(ctx, dt, steps, xf, yf); break;
-    case 6: implicit_typed<6>(ctx, dt, steps, xf, yf); break;
-    case 7: implicit_typed<7>(ctx, dt, steps, xf, yf); break;
-    default: implicit_typed<8>(ctx, dt, steps, xf, yf); break;
+    case 1: implicit_typed<1>(ctx, dt, steps, xf, yf, sources, label); break;
+    case 2: implicit_typed<2>(ctx, dt, steps, xf, yf, sources, label); break;
+    case 3: implicit_typed<3>(ctx, dt, steps, xf, yf, sources, label); break;
+    case 4: implicit_typed<4>(ctx, dt, steps, xf, yf, sources, label); break;
+    case 5: implicit_typed<5>(ctx, dt, steps, xf, yf, sources, label); break;
+    case 6: implicit_typed<6>(ctx, dt, steps, xf, yf, sources, label); break;
+    case 7: implicit_typed<7>(ctx, dt, steps, xf, yf, sources, label); break;
+    default: implicit_typed<8>(ctx, dt, steps, xf, yf, sources, label); break;
   }
 }
 

@@ -113,6 +113,12 @@ enum Ticket : int {
 
 constexpr int kMaxSignals = 8;
 
+// Bits of the transport flags (AdrDevice::flags[0]).
+constexpr uint32_t kFlagNegative = 1;  // a non-finite or (upwind) negative explicit update
+constexpr uint32_t kFlagLevels = 2;    // a level negative or not finite when the step is judged
+constexpr uint32_t kFlagCell = 4;      // a coupled cell's rate or species update not finite
+constexpr uint32_t kFlagSources = 8;   // a coupled source rate not finite
+
 // One level of the multilevel preconditioner: a 5-point operator on a
 // (nx, ny) grid of aggregates, as couplings to the lower x and y neighbour
 // (wx, wy), a boundary term (bc) and the inverse diagonal; r and e hold the
@@ -289,6 +295,29 @@ struct LearnedDevice {
   DeviceBuffer<float> z, d, gz, grads, losses;
 };
 
+// Coupled cell/grid exchange (coupled.hip): the cells, their species in two
+// generations, the rate plan, and the step's scratch: per cell four stencil
+// corners as (site, entry) pairs with their weights, the pairs sorted by site,
+// each site's run of sorted pairs, the exchange rates and the sources.
+struct CoupledDevice {
+  bool uploaded = false;
+  bool has_mask = false;  // the grid carries an obstacle mask (stencils renormalise)
+  uint32_t cells = 0, species = 0, signals = 0, instructions = 0;
+  float origin[3] = {};
+  DeviceBuffer<float> position_x, position_y, position_z, lengths, radii, growth, previous;
+  DeviceBuffer<int32_t> types;
+  DeviceBuffer<float> species_a, species_b;
+  int species_current = 0;
+  DeviceBuffer<msim_rate_instruction> plan;
+  DeviceBuffer<uint32_t> outputs;  // species outputs, then signal outputs
+  DeviceBuffer<uint32_t> keys, entries, sorted_keys, sorted_entries, runs;
+  DeviceBuffer<float> weights, exchange, sources;
+  DeviceBuffer<unsigned long long> failure;  // (cell << 32 | instruction) of the first failure
+  DeviceBuffer<double> largest;
+  DeviceBuffer<uint8_t> sort_scratch;
+  double seconds = 0.0;
+};
+
 struct BatchDevice;  // batch.hip
 
 struct Context {
@@ -301,6 +330,7 @@ struct Context {
   FlowDevice flow;
   AdrDevice adr;
   LearnedDevice learned;
+  CoupledDevice coupled;
   std::shared_ptr<BatchDevice> batch;
 };
 
@@ -420,7 +450,23 @@ void adr_download(Context& ctx, void* levels);
 void launch_settle_f32(Context& ctx, float* levels, size_t count, const double* largest,
                        uint32_t parts, uint32_t step);
 
+// The explicit candidate of one FP32 forward-Euler step from `in` into `out`,
+// not settled; a non-finite or (upwind) negative update sets the transport
+// flags with `step`.
+void fe_candidate_f32(Context& ctx, float dt, const float* in, float* out, uint32_t step,
+                      const float* xf, const float* yf);
+
 // implicit.hip: `steps` implicit steps of dt on the uploaded FP32 levels.
-void implicit_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf);
+// sources (device, S*N, or null): rates added to each step's right-hand side as
+// dt * source; label: the solve's name in the not-converged message.
+void implicit_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf,
+                  const float* sources = nullptr, const char* label = "Implicit signal solve");
+
+// coupled.hip: device-resident coupled steps (msim_hip_coupled_*). The grid
+// must be uploaded (adr_upload, FP32) first.
+void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
+                    const float* species, const msim_rate_plan& plan, bool has_mask);
+void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t steps);
+void coupled_download(Context& ctx, float* levels, float* species);
 
 }  // namespace msim

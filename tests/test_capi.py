@@ -44,22 +44,38 @@ def test_python_mirror_binds_the_abi():
 
 
 def test_unsupported_features_are_not_advertised():
-    assert msa.SUPPORTED == {"signals", "depth_averaged_flow"}
+    assert msa.SUPPORTED == {"signals", "coupled_rates", "depth_averaged_flow"}
     assert set(msa.FEATURES) >= msa.SUPPORTED
 
 
 def test_depth_averaged_entry_points_are_implemented():
-    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49) run the
-    C ABI; only the resolved and cell-mechanics calls are refused."""
+    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49) and the
+    coupled step (31-34) run the C ABI; only the resolved and cell-mechanics
+    calls are refused."""
     stub = msa.HipBackend._unsupported
     assert msa.HipBackend.solve_depth_averaged_flow is not stub
     assert msa.HipBackend.solve_depth_averaged_port_flow is not stub
+    assert msa.HipBackend.advance_coupled is not stub
     assert msa.HipBackend.solve_resolved_flow is stub
+    assert msa.HipBackend.advance_species is stub
+
+
+def test_rate_ops_follow_the_reference_order():
+    """RateOp's codes (coupled_rates.hpp) as the C ABI's MSIM_RATE_* and the
+    oracle number them."""
+    from tests.helpers import OP_SELECT, OP_SIGNAL, OP_SURFACE, OP_VOLUME_CHANGE_RATE
+    assert msa.RateOp.SELECT == OP_SELECT == 26
+    assert msa.RateOp.SIGNAL == OP_SIGNAL == 27
+    assert msa.RateOp.SURFACE_AREA == OP_SURFACE == 10
+    assert msa.RateOp.VOLUME_CHANGE_RATE == OP_VOLUME_CHANGE_RATE == 28
+    header = open(os.path.join(ROOT, "include", "msim_hip.h")).read()
+    assert "MSIM_RATE_VOLUME_CHANGE_RATE" in header
 
 
 def test_kernels_are_built_for_gfx950_only():
+    """Every code object in the offload bundle targets gfx950 (the sort's
+    header library names other targets in host strings, which is not code)."""
     ensure_built()
     blob = open(msa.LIBRARY, "rb").read()
-    assert b"gfx950" in blob
-    for other in (b"gfx942", b"gfx90a", b"gfx1100"):
-        assert other not in blob
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa-[-\w:+]*", blob))
+    assert targets == {b"amdgcn-amd-amdhsa--gfx950"}
