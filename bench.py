@@ -203,6 +203,69 @@ def cpu_implicit_baseline(w, faces, threads: int, dt: float):
                        f"{rep['iterations']} Jacobi iterations, in {secs:.2f} s")
 
 
+def coupled_setup(w, count):
+    """The coupled workload on the channel: nutrient at 1 in the fluid (AHL and
+    by-product at 0), a colony of `count` cells, LuxI and GFP at 1."""
+    from microsimulator_amd.workloads import colony, quorum_plan
+    cells, previous = colony(w, count)
+    S = 3
+    levels = np.zeros(S * w.sites, np.float32)
+    levels[:w.sites] = np.where(w.obstacles.reshape(-1) != 0, 0.0, 1.0)
+    species = np.ones(2 * count, np.float32)
+    return cells, previous, quorum_plan(), levels, species
+
+
+def coupled_bench(msa, hip, grid, w, field, bv, diffusion, dt, steps: int, count: int):
+    """Device-resident coupled steps (advance_coupled) over the solved field:
+    the colony samples the three solutes, runs its rate plan, updates its two
+    species and returns its exchange through the stencils; the grid steps by
+    forward Euler at dt, then a few by Crank-Nicolson at 4 dt."""
+    cells, previous, plan, levels, species = coupled_setup(w, count)
+    out = {"cells": count, "solutes": 3, "species": 2, "instructions": len(plan[1])}
+    for name, integration, step_dt, n in (("forward_euler", msa.FORWARD_EULER, dt, steps),
+                                          ("crank_nicolson", msa.CRANK_NICOLSON, 4.0 * dt,
+                                           max(steps // 5, 1))):
+        spec = msa.SignalGridSpec(grid, diffusion, bv, field, integration=integration)
+        hip.coupled_upload(spec, levels, cells, species, plan)
+        hip.coupled_run(step_dt, 1, previous)  # warm
+        hip.coupled_upload(spec, levels, cells, species, plan)
+        rep = hip.coupled_run(step_dt, n, previous)
+        secs = hip.coupled_seconds()
+        out[name] = {"steps": n, "dt": step_dt, "seconds_per_step": round(secs / n, 6),
+                     "steps_per_s": round(n / secs, 2),
+                     "mcell_per_s": round(count * n / secs / 1e6, 1),
+                     "grid_gcell_upd_per_s": round(w.fluid * 3 * n / secs / 1e9, 3)}
+        if integration != msa.FORWARD_EULER:
+            out[name]["last_step_iterations"] = rep.iterations
+        log(f"coupled {name}: {count} cells, {secs / n * 1e3:.3f} ms per step "
+            f"({count * n / secs / 1e6:.0f} Mcell/s)")
+    return out
+
+
+def cpu_coupled_baseline(w, faces, threads: int, dt: float, count: int):
+    """One forward-Euler coupled step of the reference (advance_coupled_cpu)
+    on the same colony and grid, timed around the call only."""
+    from oracle import FIXED, NO_FLUX, Reference
+    from oracle import Grid as OGrid
+    if not Reference.available():
+        return None
+    ref = Reference()
+    ref.set_threads(threads)
+    cells, previous, plan, levels, species = coupled_setup(w, count)
+    grid = OGrid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles,
+                 boundary=(FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX))
+    bv = np.zeros(18, np.float32)
+    bv[0::6] = 1.0
+    ref.advance_coupled(grid, levels, cells, species, plan, previous, dt,
+                        diffusion=np.array([100.0, 500.0, 1000.0], np.float32),
+                        boundary_values=bv, faces=faces)
+    secs = ref.last_seconds
+    return dict(value=round(1.0 / secs, 4), unit="coupled steps/s", cores=ref.threads(),
+                kind="reference", mcell_per_s=round(count / secs / 1e6, 3),
+                sample=f"1 forward-Euler step of advance_coupled_cpu, {count} cells, 3 solutes, "
+                       f"in {secs:.2f} s")
+
+
 def batch_sweep_bench(msa, hip, steps: int):
     """The batched sweep: 256 seeded instances drawn from 64x64, 128x96,
     256x128 and 512x256, each solved (FP64 true residual 1e-8), then `steps`
@@ -316,6 +379,9 @@ def main():
     ap.add_argument("--no-ports", action="store_true", help="skip the port-driven flow")
     ap.add_argument("--implicit-steps", type=int, default=5,
                     help="Crank-Nicolson transport steps to time (0: skip)")
+    ap.add_argument("--coupled-steps", type=int, default=20,
+                    help="device-resident coupled steps to time (0: skip)")
+    ap.add_argument("--coupled-cells", type=int, default=1 << 20)
     ap.add_argument("--batch-steps", type=int, default=50,
                     help="graph-captured steps of the 256-instance batched sweep (0: skip)")
     args = ap.parse_args()
@@ -379,8 +445,14 @@ def main():
         hip.adr_upload(implicit, np.zeros(S * w.sites, np.float32), precision=32,
                        use_flow_field=True)
         hip.adr_run(8e-4, 2, ledger=False)
+        if args.coupled_steps > 0:
+            _, field_k = hip.flow_download()
+            cells, previous, plan, levels, species = coupled_setup(w, args.coupled_cells)
+            coupled_spec = msa.SignalGridSpec(grid, [100.0, 500.0, 1000.0], bv, field_k)
+            hip.coupled_upload(coupled_spec, levels, cells, species, plan)
+            hip.coupled_run(1e-6, 3, previous)
         log(f"ran {args.kernel_iterations} iterations of each solver form, 5 transport steps "
-            f"of each explicit form and 2 implicit steps")
+            f"of each explicit form, 2 implicit steps and 3 coupled steps")
         return
 
     headline = solver_params(args.solver)
@@ -490,6 +562,10 @@ def main():
     implicit = None
     if args.implicit_steps > 0:
         implicit = implicit_bench(msa, hip, grid, w, bv, diffusion, 4.0 * dt, args.implicit_steps)
+    coupled = None
+    if args.coupled_steps > 0:
+        coupled = coupled_bench(msa, hip, grid, w, field, bv, diffusion, dt, args.coupled_steps,
+                                args.coupled_cells)
     log(f"transport fp64: {adr_gcups:.2f} Gcell-upd/s ({adr_gbs:.0f} GB/s), worst per-step "
         f"imbalance {worst_balance:.2e}; fp32: {adr32_gcups:.2f} Gcell-upd/s; fp32 van Leer: "
         f"{adr_vl_gcups:.2f} Gcell-upd/s")
@@ -566,6 +642,8 @@ def main():
     }
     if implicit is not None:
         line["adr_implicit"] = implicit
+    if coupled is not None:
+        line["coupled"] = coupled
     if mfma is not None:
         line["roofline_mfma"] = mfma
     if args.batch_steps > 0:
@@ -592,6 +670,10 @@ def main():
             implicit_cpu = cpu_implicit_baseline(w, faces, threads, implicit["dt"])
             if implicit_cpu is not None:
                 line["cpu_baseline_adr_implicit"] = implicit_cpu
+        if coupled is not None:
+            coupled_cpu = cpu_coupled_baseline(w, faces, threads, dt, args.coupled_cells)
+            if coupled_cpu is not None:
+                line["cpu_baseline_coupled"] = coupled_cpu
     if not args.no_ports:
         per_iteration = line.get("cpu_baseline", {}).get("seconds_per_iteration")
         line["port_flow"] = port_flow_bench(msa, w, hip, per_iteration)

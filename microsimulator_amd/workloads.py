@@ -210,3 +210,66 @@ def batch_instance(index: int, seed: int = 3):
 def batch_sweep(count: int = 256, seed: int = 3):
     """The batched sweep: `count` seeded instances of uneven sizes."""
     return [batch_instance(k, seed) for k in range(count)]
+
+
+def colony(w: Workload, count: int, seed: int = 11):
+    """`count` cells at seeded positions whose bilinear stencils lie wholly in
+    fluid, lengths U[1.5, 3.5] and radius 0.5 (same units as the spacing),
+    growth rates U[0.5, 1.5], four cell types; and previous lengths 0.95-1 of
+    the current ones. Returns (cells, previous_lengths), cells as the arrays
+    of a WorldState's geometry and attributes."""
+    rng = np.random.default_rng(seed)
+    solid = w.obstacles.reshape(w.nx, w.ny) != 0
+    open_voxel = ~(solid[:-1, :-1] | solid[1:, :-1] | solid[:-1, 1:] | solid[1:, 1:])
+    xs, ys = [], []
+    need = count
+    while need > 0:
+        x = rng.uniform(0.0, w.nx - 1, 2 * need)
+        y = rng.uniform(0.0, w.ny - 1, 2 * need)
+        keep = open_voxel[np.minimum(x.astype(np.int64), w.nx - 2),
+                          np.minimum(y.astype(np.int64), w.ny - 2)]
+        xs.append(x[keep][:need])
+        ys.append(y[keep][:need])
+        need -= int(min(keep.sum(), need))
+    x = np.concatenate(xs)[:count] * w.dx
+    y = np.concatenate(ys)[:count] * w.dy
+    lengths = rng.uniform(1.5, 3.5, count).astype(np.float32)
+    cells = dict(position_x=x.astype(np.float32), position_y=y.astype(np.float32),
+                 position_z=np.zeros(count, np.float32), lengths=lengths,
+                 radii=np.full(count, 0.5, np.float32),
+                 growth_rates=rng.uniform(0.5, 1.5, count).astype(np.float32),
+                 cell_types=rng.integers(0, 4, count).astype(np.int32))
+    return cells, (lengths * rng.uniform(0.95, 1.0, count)).astype(np.float32)
+
+
+def quorum_plan():
+    """A coupled rate plan (species LuxI and GFP; signals nutrient, AHL and a
+    by-product) of arithmetic operations: Monod uptake of the nutrient scaled
+    by cell volume, AHL released in proportion to LuxI, the by-product a tenth
+    of the uptake, LuxI induced by AHL (a + b c/(Kd + c)) and GFP made at the
+    growth rate times LuxI, each with first-order decay. Returns
+    (species_count, instructions, species_outputs, signal_outputs) with
+    instructions as (RateOp code, first, second, third, value)."""
+    CONSTANT, SPECIES, GROWTH_RATE, VOLUME = 0, 1, 7, 9
+    ADD, SUBTRACT, MULTIPLY, DIVIDE, NEGATE, SIGNAL = 11, 12, 13, 14, 18, 27
+    ins = []
+
+    def op(code, a=0, b=0, v=0.0):
+        ins.append((code, a, b, 0, v))
+        return len(ins) - 1
+
+    nutrient = op(SIGNAL, 0)
+    monod = op(DIVIDE, nutrient, op(ADD, nutrient, op(CONSTANT, v=0.1)))
+    volume = op(VOLUME)
+    uptake = op(MULTIPLY, op(MULTIPLY, monod, op(CONSTANT, v=1.0)), volume)
+    luxi = op(SPECIES, 0)
+    release = op(MULTIPLY, op(MULTIPLY, luxi, op(CONSTANT, v=0.05)), volume)
+    byproduct = op(MULTIPLY, uptake, op(CONSTANT, v=0.1))
+    ahl = op(SIGNAL, 1)
+    hill = op(DIVIDE, ahl, op(ADD, ahl, op(CONSTANT, v=0.5)))
+    decay = op(CONSTANT, v=0.2)
+    induced = op(ADD, op(MULTIPLY, hill, op(CONSTANT, v=2.0)), op(CONSTANT, v=0.1))
+    d_luxi = op(SUBTRACT, induced, op(MULTIPLY, luxi, decay))
+    gfp = op(SPECIES, 1)
+    d_gfp = op(SUBTRACT, op(MULTIPLY, op(GROWTH_RATE), luxi), op(MULTIPLY, gfp, decay))
+    return 2, ins, [d_luxi, d_gfp], [op(NEGATE, uptake), release, byproduct]

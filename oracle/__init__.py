@@ -545,7 +545,7 @@ class Reference:
         go = np.ascontiguousarray(signal_outputs, np.uint32)
         prev = np.ascontiguousarray(previous_lengths, np.float32)
         org = np.ascontiguousarray(origin, np.float32)
-        report = np.zeros(3, np.float32)
+        report = np.zeros(4, np.float32)
         err = C.create_string_buffer(512)
         st = self.lib.ref_advance_coupled(
             grid.nx, grid.ny, grid.nz, grid.dx, grid.dy, grid.dz, org.ctypes.data, _ptr(obs),
@@ -560,6 +560,7 @@ class Reference:
             prev.ctypes.data, dt, lv.ctypes.data, report.ctypes.data, err, 512)
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
+        self.last_seconds = float(report[3])
         return lv, sp, dict(converged=bool(report[0]), iterations=int(report[1]),
                             residual_rms=float(report[2]))
 

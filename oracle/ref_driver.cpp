@@ -348,7 +348,7 @@ int ref_port_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx
 // cells*species) and a CoupledRatePlan (instructions as 4 uint32 (operation,
 // first, second, third) plus a float value each) of plan_signals signals.
 // levels and species in place;
-// report: converged, iterations, residual RMS.
+// report: converged, iterations, residual RMS, seconds of advance_coupled_cpu.
 int ref_advance_coupled(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
                         float dz, const float* origin, const std::uint8_t* obstacles,
                         const std::uint8_t* kinds, std::uint32_t signals, const float* diffusion,
@@ -415,8 +415,10 @@ int ref_advance_coupled(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, fl
         species_count, plan_signals, std::move(instructions),
         std::vector<std::uint32_t>(species_outputs, species_outputs + species_count),
         std::vector<std::uint32_t>(signal_outputs, signal_outputs + plan_signals));
+    const auto start = std::chrono::steady_clock::now();
     const auto result = ms::advance_coupled_cpu(
         state, grid, plan, std::span<const float>(previous_lengths, cells), dt);
+    const auto stop = std::chrono::steady_clock::now();
     const auto out = grid.levels();
     std::memcpy(levels, out.data(), out.size() * sizeof(float));
     const auto held = state.species_state();
@@ -424,6 +426,7 @@ int ref_advance_coupled(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, fl
     report[0] = result.converged ? 1.0F : 0.0F;
     report[1] = static_cast<float>(result.iterations);
     report[2] = result.residual_rms;
+    report[3] = std::chrono::duration<float>(stop - start).count();
   });
 }
 
