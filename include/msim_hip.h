@@ -439,6 +439,32 @@ int msim_hip_coupled_download(msim_ctx* ctx, float* levels, float* species);
 /* Device time (seconds) of the kernels of the last coupled_run. */
 double msim_hip_coupled_seconds(msim_ctx* ctx);
 
+/* colony_volume_fraction (python/src/microsimulator/flow.py:282-305): each
+ * cell's biomass volume pi r^2 (L + 2r) (biomass.py:6-14) spread over the
+ * voxel integrals of a tent kernel of radius averaging_radius around it,
+ * renormalised over the face-connected fluid component of its strongest voxel
+ * (_deposition_weights, flow.py:224-279), divided by the voxel volume: density
+ * (N doubles, site order). Cells outside the grid's extent are omitted. Only
+ * positions, lengths and radii of `cells` are read (growth_rates and
+ * cell_types may be NULL). The voxels' sums are FP64 atomics: they agree with
+ * the reference's cell-ordered sums to FP64 rounding. Any number of layers. */
+int msim_hip_colony_volume_fraction(msim_ctx* ctx, const msim_grid* grid, const float* origin,
+                                    const msim_cells* cells, double averaging_radius,
+                                    double* density);
+
+/* colony_mobility (flow.py:334-382): base / (1 + base * drag) at each fluid
+ * site (0 at solids), drag = C f^2 / (1 - f)^3 with f the column's mean
+ * volume fraction over its fluid voxels, capped at max_volume_fraction.
+ * base_count 0: base[0] is one value for every site; otherwise base holds
+ * base_count values, one per site. mobility: N doubles. */
+int msim_hip_colony_mobility(msim_ctx* ctx, const msim_grid* grid, const float* origin,
+                             const msim_cells* cells, const double* base, uint32_t base_count,
+                             double drag_coefficient, double max_volume_fraction,
+                             double averaging_radius, double* mobility);
+
+/* Device time (seconds) of the last deposit's kernels. */
+double msim_hip_colony_seconds(msim_ctx* ctx);
+
 /* ---------------------------------------------------------------------------
  * Learned smoother of the LEARNED solver (no counterpart in the reference,
  * whose only preconditioner is Jacobi, cpp/cpu/cpu_flow.cpp:65-69).

@@ -217,7 +217,8 @@ EXPORTS = (
     "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
     "msim_hip_solve_depth_averaged_port_flow", "msim_hip_adr_report",
     "msim_hip_advance_coupled", "msim_hip_coupled_upload", "msim_hip_coupled_run",
-    "msim_hip_coupled_download", "msim_hip_coupled_seconds",
+    "msim_hip_coupled_download", "msim_hip_coupled_seconds", "msim_hip_colony_volume_fraction",
+    "msim_hip_colony_mobility", "msim_hip_colony_seconds",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -277,6 +278,9 @@ def library(path: str = LIBRARY):
         "msim_hip_coupled_run": (I, [P, P, C.c_float, U, P]),
         "msim_hip_coupled_download": (I, [P, P, P]),
         "msim_hip_coupled_seconds": (D, [P]),
+        "msim_hip_colony_volume_fraction": (I, [P, P, P, P, D, P]),
+        "msim_hip_colony_mobility": (I, [P, P, P, P, P, U, D, D, D, P]),
+        "msim_hip_colony_seconds": (D, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -615,10 +619,18 @@ _CELL_FIELDS = ("position_x", "position_y", "position_z", "lengths", "radii", "g
 
 def _cells_c(cells, keep: list) -> _Cells:
     """cells: a mapping (or object) of position_x/y/z, lengths, radii,
-    growth_rates (float) and cell_types (int) arrays of one length."""
-    get = cells.__getitem__ if isinstance(cells, dict) else lambda k: getattr(cells, k)
-    arrays = {k: np.ascontiguousarray(get(k), np.float32).reshape(-1) for k in _CELL_FIELDS}
-    arrays["cell_types"] = np.ascontiguousarray(get("cell_types"), np.int32).reshape(-1)
+    growth_rates (float) and cell_types (int) arrays of one length; growth
+    rates and types default to zero where absent."""
+    def get(k):
+        if isinstance(cells, dict):
+            return cells[k] if k in cells else None
+        return getattr(cells, k, None)
+    count = len(get("lengths"))
+    arrays = {k: np.ascontiguousarray(np.zeros(count) if get(k) is None else get(k),
+                                      np.float32).reshape(-1) for k in _CELL_FIELDS}
+    types = get("cell_types")
+    arrays["cell_types"] = np.ascontiguousarray(np.zeros(count) if types is None else types,
+                                                np.int32).reshape(-1)
     count = arrays["lengths"].size
     if any(a.size != count for a in arrays.values()):
         raise InvalidArgument("cell arrays must have one length")
@@ -802,6 +814,47 @@ class HipBackend:
             sp.ctypes.data if sp.size else None, C.byref(p), prev.ctypes.data if prev.size else None,
             dt, C.byref(rep)))
         return lv, sp, SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
+
+    # -- colony volume fraction and mobility (python/src/microsimulator/flow.py) --
+    def colony_volume_fraction(self, grid: Grid, cells, averaging_radius: float = 4.0,
+                               origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """colony_volume_fraction (flow.py:282-305): biomass volume per voxel
+        volume, an (nx, ny, nz) float64 array."""
+        keep: list = []
+        g = grid._c(keep)
+        c = _cells_c(cells, keep)
+        org = np.ascontiguousarray(origin, np.float32).reshape(3)
+        out = np.zeros(grid.sites, np.float64)
+        self._check(self.lib.msim_hip_colony_volume_fraction(
+            self.ctx, C.byref(g), org.ctypes.data, C.byref(c), float(averaging_radius),
+            out.ctypes.data))
+        return out.reshape(grid.nx, grid.ny, grid.nz)
+
+    def colony_mobility(self, grid: Grid, cells, base=1.0, drag_coefficient: float = 100.0,
+                        max_volume_fraction: float = 0.9, averaging_radius: float = 4.0,
+                        origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+        """colony_mobility (flow.py:334-356): one float64 per site; base is one
+        value or one per site."""
+        keep: list = []
+        g = grid._c(keep)
+        c = _cells_c(cells, keep)
+        org = np.ascontiguousarray(origin, np.float32).reshape(3)
+        scalar = np.ndim(base) == 0
+        b = np.ascontiguousarray(np.atleast_1d(base), np.float64).reshape(-1)
+        if b.size == 0:
+            b = np.zeros(1, np.float64)
+            count = 1 << 31
+        else:
+            count = 0 if scalar else b.size
+        out = np.zeros(grid.sites, np.float64)
+        self._check(self.lib.msim_hip_colony_mobility(
+            self.ctx, C.byref(g), org.ctypes.data, C.byref(c), b.ctypes.data, count,
+            float(drag_coefficient), float(max_volume_fraction), float(averaging_radius),
+            out.ctypes.data))
+        return out
+
+    def colony_seconds(self) -> float:
+        return float(self.lib.msim_hip_colony_seconds(self.ctx))
 
     # -- device-resident coupled steps (no counterpart in the reference) ---------
     def coupled_upload(self, spec: SignalGridSpec, levels, cells, species, plan,

@@ -899,6 +899,93 @@ void upload_coupled(Context& c, const msim_signal_grid& g, const float* origin,
   coupled_upload(c, origin, cells, species, plan, g.grid.obstacles != nullptr);
 }
 
+
+// The host's part of colony_volume_fraction's checks (flow.py:282-305,
+// 224-241; biomass.py:12-13): the radius, and per cell in order a finite
+// position and, for a cell inside the grid's extent, a valid length and
+// radius. Returns the first refused cell (cells.count when none) and its
+// message.
+uint32_t colony_host_checks(const msim_grid& g, const float* origin, const msim_cells& cells,
+                            double radius, std::string* message) {
+  if (!std::isfinite(radius) || radius <= 0.0) invalid("averaging radius must be finite and positive");
+  validate_lattice(g);
+  for (int a = 0; a < 3; ++a) {
+    if (!std::isfinite(origin[a])) invalid("signal grid origin must be finite");
+  }
+  if (cells.count > 0 && (cells.position_x == nullptr || cells.position_y == nullptr ||
+                          cells.position_z == nullptr || cells.lengths == nullptr ||
+                          cells.radii == nullptr)) {
+    invalid("cell arrays must not be null");
+  }
+  const uint32_t dims[3] = {g.nx, g.ny, g.nz};
+  const double spacing[3] = {g.dx, g.dy, g.dz};
+  for (uint32_t c = 0; c < cells.count; ++c) {
+    const double p[3] = {cells.position_x[c], cells.position_y[c], cells.position_z[c]};
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) {
+      *message = "deposited positions must be finite";
+      return c;
+    }
+    bool outside = false;
+    for (int a = 0; a < 3; ++a) {
+      const double o = origin[a], h = spacing[a];
+      if (p[a] < o - h / 2.0 || p[a] >= o + (static_cast<double>(dims[a]) - 0.5) * h) {
+        outside = true;
+      }
+    }
+    const double length = cells.lengths[c], r = cells.radii[c];
+    if (!outside && (!std::isfinite(length) || length < 0 || !std::isfinite(r) || r <= 0)) {
+      *message = "length must be finite and nonnegative; radius finite and positive";
+      return c;
+    }
+  }
+  return cells.count;
+}
+
+// The deposit with the reference's first failure: the least cell the host or
+// the device refuses, the host's message winning for its own cell only when
+// the device found none before it (a cell's support is judged before its size).
+void colony_run(Context& ctx, const msim_grid& g, const float* origin, const msim_cells& cells,
+                double radius, const double* base, uint32_t base_count, double drag, double cap,
+                double* density, double* mobility, const std::string& pending) {
+  std::string message;
+  const uint32_t refused = colony_host_checks(g, origin, cells, radius, &message);
+  const uint32_t axis = [&] {
+    uint32_t total = 0;
+    const uint32_t dims[3] = {g.nx, g.ny, g.nz};
+    const double spacing[3] = {g.dx, g.dy, g.dz};
+    for (int a = 0; a < 3; ++a) {
+      total += static_cast<uint32_t>(std::min(std::floor(2.0 * radius / spacing[a]) + 4.0,
+                                              static_cast<double>(dims[a])));
+    }
+    return total;
+  }();
+  const uint32_t region = colony_region_cap(g, radius);
+  if (region > 7000 || (static_cast<size_t>(axis) + region) * 8 + region > 65536) {
+    throw Error(MSIM_UNSUPPORTED,
+                "the HIP deposit takes kernel supports of at most 64 KiB of LDS (about 7000 "
+                "voxels): shorten the averaging radius relative to the spacing");
+  }
+  msim_cells head = cells;
+  // A refused size is judged after the cell's support, so that cell's
+  // support is still deposited (and may fail first); a refused position is not.
+  const bool size_refused = refused < cells.count && message[0] == 'l';
+  head.count = refused + (size_refused ? 1u : 0u);
+  // With a refused mobility argument only the deposit's own failures can
+  // come first: deposit into the density, then report.
+  const bool mobility_ok = pending.empty();
+  std::vector<double> scratch;
+  double* density_out = density;
+  if (mobility != nullptr && !mobility_ok) {
+    scratch.resize(static_cast<size_t>(g.nx) * g.ny * g.nz);
+    density_out = scratch.data();
+  }
+  const uint32_t failed =
+      colony_deposit(ctx, g, origin, head, radius, base, base_count, drag, cap, density_out,
+                     mobility_ok ? mobility : nullptr);
+  if (failed != 0xffffffffu) invalid("biomass deposition has no connected fluid support");
+  if (refused < cells.count) invalid(message);
+  if (!mobility_ok) invalid(pending);
+}
 }  // namespace
 }  // namespace msim
 
@@ -1398,6 +1485,53 @@ int msim_hip_coupled_download(msim_ctx* ctx, float* levels, float* species) {
     if (!c.coupled.uploaded || !c.adr.uploaded) msim::invalid("no coupled state on the device");
     msim::coupled_download(c, levels, species);
   });
+}
+
+int msim_hip_colony_volume_fraction(msim_ctx* ctx, const msim_grid* grid, const float* origin,
+                                    const msim_cells* cells, double averaging_radius,
+                                    double* density) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || origin == nullptr || cells == nullptr || density == nullptr) {
+      msim::invalid("grid, origin, cells and density must not be null");
+    }
+    msim::colony_run(c, *grid, origin, *cells, averaging_radius, nullptr, 0, 0.0, 0.0, density,
+                     nullptr, std::string());
+  });
+}
+
+int msim_hip_colony_mobility(msim_ctx* ctx, const msim_grid* grid, const float* origin,
+                             const msim_cells* cells, const double* base, uint32_t base_count,
+                             double drag_coefficient, double max_volume_fraction,
+                             double averaging_radius, double* mobility) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || origin == nullptr || cells == nullptr || base == nullptr ||
+        mobility == nullptr) {
+      msim::invalid("grid, origin, cells, base and mobility must not be null");
+    }
+    // _darcy_coefficients' checks (flow.py:359-374) and the drag's (91-96),
+    // reported after the deposit's own.
+    std::string pending;
+    const size_t n = static_cast<size_t>(grid->nx) * grid->ny * grid->nz;
+    if (!(max_volume_fraction > 0.0 && max_volume_fraction < 1.0)) {
+      pending = "maximum volume fraction must lie strictly between zero and one";
+    } else if (base_count == 0 && (!std::isfinite(base[0]) || base[0] <= 0.0)) {
+      pending = "base mobility must be finite and positive";
+    } else if (base_count != 0 && base_count != n) {
+      pending = "base mobility must hold one value per grid site";
+    } else if (base_count != 0 &&
+               !std::all_of(base, base + n, [](double b) { return std::isfinite(b) && b >= 0.0; })) {
+      pending = "base mobility values must be finite and non-negative";
+    } else if (!std::isfinite(drag_coefficient) || drag_coefficient < 0.0) {
+      pending = "drag coefficient must be finite and non-negative";
+    }
+    msim::colony_run(c, *grid, origin, *cells, averaging_radius, base,
+                     base_count == 0 ? 1u : base_count, drag_coefficient, max_volume_fraction,
+                     nullptr, mobility, pending);
+  });
+}
+
+double msim_hip_colony_seconds(msim_ctx* ctx) {
+  return ctx == nullptr ? 0.0 : ctx->impl.colony.seconds;
 }
 
 double msim_hip_coupled_seconds(msim_ctx* ctx) {

@@ -318,6 +318,17 @@ struct CoupledDevice {
   double seconds = 0.0;
 };
 
+// Biomass deposition (colony.hip): the cells' positions and sizes, the mask,
+// the deposited volume (FP64, one per site), the output field, and the least
+// failing cell (and an internal bound check).
+struct ColonyDevice {
+  DeviceBuffer<float> position_x, position_y, position_z, lengths, radii;
+  DeviceBuffer<uint8_t> solid;
+  DeviceBuffer<double> volume, out, base;
+  DeviceBuffer<unsigned int> failure;
+  double seconds = 0.0;
+};
+
 struct BatchDevice;  // batch.hip
 
 struct Context {
@@ -331,6 +342,7 @@ struct Context {
   AdrDevice adr;
   LearnedDevice learned;
   CoupledDevice coupled;
+  ColonyDevice colony;
   std::shared_ptr<BatchDevice> batch;
 };
 
@@ -468,5 +480,16 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
                     const float* species, const msim_rate_plan& plan, bool has_mask);
 void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t steps);
 void coupled_download(Context& ctx, float* levels, float* species);
+
+// colony.hip: the voxels one cell's deposit support can span (an upper bound
+// from the averaging radius and the spacing), and the deposit itself: the
+// density (volume fraction, FP64 per site) into density_host, or with
+// mobility_host the colony mobility (base: 1 or N values). Returns the least
+// cell whose support has no connected fluid, or 0xffffffff.
+uint32_t colony_region_cap(const msim_grid& grid, double radius);
+uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin,
+                        const msim_cells& cells, double radius, const double* base,
+                        uint32_t base_count, double drag_coefficient, double cap,
+                        double* density_host, double* mobility_host);
 
 }  // namespace msim
