@@ -239,7 +239,36 @@ def coupled_bench(msa, hip, grid, w, field, bv, diffusion, dt, steps: int, count
             out[name]["last_step_iterations"] = rep.iterations
         log(f"coupled {name}: {count} cells, {secs / n * 1e3:.3f} ms per step "
             f"({count * n / secs / 1e6:.0f} Mcell/s)")
+    # The colony's mobility (biomass deposition with the tent kernel of radius
+    # 4, connected-fluid restriction, Kozeny-Carman drag): the flow input the
+    # next pressure solve would take.
+    hip.colony_mobility(grid, cells, base=1.0)  # warm
+    t0 = time.perf_counter()
+    hip.colony_mobility(grid, cells, base=1.0)
+    call = time.perf_counter() - t0
+    secs = hip.colony_seconds()
+    out["colony_mobility"] = {"averaging_radius": 4.0, "device_seconds": round(secs, 6),
+                              "mcell_per_s": round(count / secs / 1e6, 1),
+                              "call_seconds_with_transfers": round(call, 4)}
+    log(f"colony mobility: {count} cells, {secs * 1e3:.3f} ms on the device "
+        f"({call * 1e3:.1f} ms per call with transfers)")
     return out
+
+
+def cpu_colony_baseline(w, count: int, sample: int = 20000):
+    """The numpy restatement of the reference's colony_volume_fraction (its
+    Python loop over cells) on a sample of the colony, single-threaded."""
+    from oracle import colony as ref
+    from microsimulator_amd.workloads import colony
+    cells, _ = colony(w, count)
+    part = {k: v[:sample] for k, v in cells.items()}
+    t0 = time.perf_counter()
+    ref.colony_volume_fraction((w.nx, w.ny, 1), (0.0, 0.0, 0.0), (w.dx, w.dy, w.dz), w.obstacles,
+                               part)
+    secs = time.perf_counter() - t0
+    return dict(value=round(sample / secs / 1e6, 5), unit="Mcell/s", cores=1, kind="port",
+                sample=f"colony_volume_fraction of {sample} of the {count} cells (numpy "
+                       f"restatement of the reference's Python loop) in {secs:.2f} s")
 
 
 def cpu_coupled_baseline(w, faces, threads: int, dt: float, count: int):
@@ -674,6 +703,7 @@ def main():
             coupled_cpu = cpu_coupled_baseline(w, faces, threads, dt, args.coupled_cells)
             if coupled_cpu is not None:
                 line["cpu_baseline_coupled"] = coupled_cpu
+            line["cpu_baseline_colony"] = cpu_colony_baseline(w, args.coupled_cells)
     if not args.no_ports:
         per_iteration = line.get("cpu_baseline", {}).get("seconds_per_iteration")
         line["port_flow"] = port_flow_bench(msa, w, hip, per_iteration)

@@ -345,7 +345,8 @@ int msim_hip_adr_report(msim_ctx* ctx, msim_signal_report* report);
 /* ---------------------------------------------------------------------------
  * Coupled cell/grid exchange (ComputeBackend::advance_coupled, backend.hpp:31-34;
  * advance_coupled_cpu, cpp/cpu/cpu_coupled.cpp:125-231), one layer, FP32
- * levels, at most 8 signals and 256 rate instructions.
+ * levels, at most 8 signals, and at most 256 rate instructions, signals and
+ * species together.
  *
  * Each cell samples the grid's starting levels through its trilinear stencil
  * (signal_grid_stencil, signals.cpp:357-420: with obstacles only the fluid

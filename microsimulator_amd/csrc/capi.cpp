@@ -882,8 +882,11 @@ void validate_previous(const msim_cells& cells, const float* previous) {
 }
 
 void check_coupled_limits(const msim_rate_plan& plan, const msim_cells& cells) {
-  if (plan.instruction_count > 256) {
-    throw Error(MSIM_UNSUPPORTED, "the HIP coupled step takes at most 256 rate instructions");
+  if (static_cast<uint64_t>(plan.instruction_count) + plan.signal_count + plan.species_count >
+      256) {
+    throw Error(MSIM_UNSUPPORTED,
+                "the HIP coupled step takes at most 256 rate instructions, signals and species "
+                "together");
   }
   if (cells.count > (1u << 29)) {
     throw Error(MSIM_OVERFLOW, "the HIP coupled step takes at most 2^29 cells");

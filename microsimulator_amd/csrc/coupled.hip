@@ -94,11 +94,14 @@ __device__ inline float cell_surface(float length, float radius) {
   return 2.0f * kPi * radius * (length + 2.0f * radius);
 }
 
-__device__ inline float pick(const float (&v)[kMaxSignals], uint32_t index) {
-  float out = v[0];
-#pragma unroll
-  for (uint32_t k = 1; k < kMaxSignals; ++k) out = index == k ? v[k] : out;
-  return out;
+// exp, log and pow in FP64, rounded to float. Kept out of line: their FP64
+// code would otherwise hold registers across the whole interpreter loop.
+__device__ __attribute__((noinline)) float transcendental(uint32_t operation, float a, float b) {
+  if (operation == MSIM_RATE_POWER) {
+    return static_cast<float>(pow(static_cast<double>(a), static_cast<double>(b)));
+  }
+  if (operation == MSIM_RATE_EXPONENTIAL) return static_cast<float>(exp(static_cast<double>(a)));
+  return static_cast<float>(log(static_cast<double>(a)));
 }
 
 __device__ inline void fail_cell(unsigned long long* failure, uint32_t* flags, uint32_t cell,
@@ -109,7 +112,8 @@ __device__ inline void fail_cell(unsigned long long* failure, uint32_t* flags, u
 }
 
 // One thread per cell (cpu_coupled.cpp:142-197 for one cell). The rate plan's
-// values sit in LDS, column threadIdx.x of an [instructions][blockDim] array.
+// values, the samples and the diluted species sit in LDS, column threadIdx.x
+// of a [slots][blockDim] array.
 // The exponential, logarithm and power are evaluated in FP64 and rounded to
 // float.
 __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
@@ -201,33 +205,32 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
       if (e < count) w[e] /= total;
     }
   }
+  // The thread's LDS column: the plan's values, then the sampled signals,
+  // then the diluted species.
+  float* work = s_work + threadIdx.x;
+  const uint32_t T = blockDim.x;
+  const uint32_t S = P.signals, K = P.species;
+  float* sampled = work + static_cast<size_t>(P.instructions) * T;
+  float* held = sampled + static_cast<size_t>(S) * T;
   // The sampled signals (cpu_coupled.cpp:150-160).
-  const uint32_t S = P.signals;
-  float sampled[kMaxSignals];
-#pragma unroll
-  for (uint32_t s = 0; s < kMaxSignals; ++s) {
+  for (uint32_t s = 0; s < S; ++s) {
     float v = 0.0f;
-    if (s < S) {
 #pragma unroll
-      for (uint32_t e = 0; e < kCorners; ++e) {
-        if (e < count) v += w[e] * levels[s * n + site[e]];
-      }
+    for (uint32_t e = 0; e < kCorners; ++e) {
+      if (e < count) v += w[e] * levels[s * n + site[e]];
     }
-    sampled[s] = v;
+    sampled[s * T] = v;
   }
   // Dilution by the growth since the previous step (162-170).
   const float length = lengths[c], radius = radii[c];
   const float previous_volume = cell_volume(previous[c], radius);
   const float current_volume = cell_volume(length, radius);
   const float dilution = previous_volume / current_volume;
-  const uint32_t K = P.species;
   const size_t base = static_cast<size_t>(c) * K;
-  for (uint32_t k = 0; k < K; ++k) species_out[base + k] = species_in[base + k] * dilution;
+  for (uint32_t k = 0; k < K; ++k) held[k * T] = species_in[base + k] * dilution;
   const float volume_change_rate =
       P.dt == 0.0f ? 0.0f : (current_volume - previous_volume) / P.dt;
   // The rate plan (evaluate_instruction, cpu_coupled.cpp:20-85).
-  float* work = s_work + threadIdx.x;
-  const uint32_t T = blockDim.x;
   for (uint32_t index = 0; index < P.instructions; ++index) {
     const msim_rate_instruction in = plan[index];
     const float a = in.operation >= MSIM_RATE_ADD && in.operation <= MSIM_RATE_SELECT
@@ -242,8 +245,8 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
     float v;
     switch (in.operation) {
       case MSIM_RATE_CONSTANT: v = in.value; break;
-      case MSIM_RATE_SPECIES: v = species_out[base + in.first]; break;
-      case MSIM_RATE_SIGNAL: v = pick(sampled, in.first); break;
+      case MSIM_RATE_SPECIES: v = held[in.first * T]; break;
+      case MSIM_RATE_SIGNAL: v = sampled[in.first * T]; break;
       case MSIM_RATE_POSITION_X: v = px[c]; break;
       case MSIM_RATE_POSITION_Y: v = py[c]; break;
       case MSIM_RATE_POSITION_Z: v = pz[c]; break;
@@ -258,14 +261,12 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
       case MSIM_RATE_SUBTRACT: v = a - b; break;
       case MSIM_RATE_MULTIPLY: v = a * b; break;
       case MSIM_RATE_DIVIDE: v = a / b; break;
-      case MSIM_RATE_POWER:
-        v = static_cast<float>(pow(static_cast<double>(a), static_cast<double>(b)));
-        break;
+      case MSIM_RATE_POWER: v = transcendental(in.operation, a, b); break;
       case MSIM_RATE_MINIMUM: v = b < a ? b : a; break;
       case MSIM_RATE_MAXIMUM: v = a < b ? b : a; break;
       case MSIM_RATE_NEGATE: v = -a; break;
-      case MSIM_RATE_EXPONENTIAL: v = static_cast<float>(exp(static_cast<double>(a))); break;
-      case MSIM_RATE_LOGARITHM: v = static_cast<float>(log(static_cast<double>(a))); break;
+      case MSIM_RATE_EXPONENTIAL: v = transcendental(in.operation, a, b); break;
+      case MSIM_RATE_LOGARITHM: v = transcendental(in.operation, a, b); break;
       case MSIM_RATE_LESS: v = a < b ? 1.0f : 0.0f; break;
       case MSIM_RATE_LESS_EQUAL: v = a <= b ? 1.0f : 0.0f; break;
       case MSIM_RATE_GREATER: v = a > b ? 1.0f : 0.0f; break;
@@ -281,7 +282,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
   }
   // The species update (cpu_coupled.cpp:180-187) and the exchange (188-191).
   for (uint32_t k = 0; k < K; ++k) {
-    const float v = species_out[base + k] + P.dt * work[outputs[k] * T];
+    const float v = held[k * T] + P.dt * work[outputs[k] * T];
     if (!isfinite(v)) {
       fail_cell(failure, flags, c, kSpeciesFailure, P.step);
       return;
@@ -415,11 +416,11 @@ uint32_t blocks_of(size_t count) {
   return static_cast<uint32_t>((count + kSiteBlock - 1) / kSiteBlock);
 }
 
-// Threads per block of the cell kernel: as many as fit the plan's values in
-// 64 KiB of LDS.
-uint32_t cell_threads(uint32_t instructions) {
-  if (instructions <= 64) return 256;
-  if (instructions <= 128) return 128;
+// Threads per block of the cell kernel: as many as fit each thread's LDS
+// column (the plan's values, the samples and the species) in 64 KiB.
+uint32_t cell_threads(uint32_t slots) {
+  if (slots <= 64) return 256;
+  if (slots <= 128) return 128;
   return 64;
 }
 
@@ -500,8 +501,9 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
   const float* species_in = c.species_current == 0 ? c.species_a.data() : c.species_b.data();
   float* species_out = c.species_current == 0 ? c.species_b.data() : c.species_a.data();
   if (c.cells > 0) {
-    const uint32_t threads = cell_threads(c.instructions);
-    const size_t lds = static_cast<size_t>(c.instructions) * threads * sizeof(float);
+    const uint32_t slots = c.instructions + c.signals + c.species;
+    const uint32_t threads = cell_threads(slots);
+    const size_t lds = static_cast<size_t>(slots) * threads * sizeof(float);
     k_coupled_cells<<<(c.cells + threads - 1) / threads, threads, lds, ctx.stream>>>(
         P, a.solid.data(), levels, c.position_x.data(), c.position_y.data(),
         c.position_z.data(), c.lengths.data(), c.radii.data(), c.growth.data(), c.types.data(),
