@@ -16,6 +16,12 @@
 //    run in cell order exactly as the reference adds it, and judges the
 //    levels (accept_step); the implicit kinds: k_coupled_sources writes the
 //    sources out for the solve's right-hand side.
+// The device keeps the cells in spatial order (by the site of their stencil's
+// lower corner, computed at upload), so the cell kernel's samples of the grid
+// and the commit's reads of the exchange rates fall on nearby lines; `order`
+// maps a slot to the cell's index in the caller's order. That index places
+// the cell's pairs in the sort's input, so the stable sort still leaves each
+// site's entries in the caller's cell order, and it names a failing cell.
 // Every kernel returns at once when an earlier failure set the transport
 // flags, so a run stops at its first refused step; the commit and source
 // kernels clear each site's run after reading it, so the runs array is all
@@ -122,6 +128,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
                                 const float* __restrict__ lengths, const float* __restrict__ radii,
                                 const float* __restrict__ growth, const int32_t* __restrict__ types,
                                 const float* __restrict__ previous,
+                                const uint32_t* __restrict__ order,
                                 const msim_rate_instruction* __restrict__ plan,
                                 const uint32_t* __restrict__ outputs,
                                 const float* __restrict__ species_in, float* species_out,
@@ -130,14 +137,15 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
                                 unsigned long long* failure, uint32_t* flags) {
   extern __shared__ float s_work[];
   if (flags[0] != 0) return;  // an earlier step failed: the run stops there
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;  // the cell's slot
   if (c >= P.cells) return;
+  const uint32_t caller = order[c];  // its index in the caller's order
   const size_t n = static_cast<size_t>(P.nx) * P.ny;
   // The stencil (signal_grid_stencil, signals.cpp:357-420): x-major corners.
   const Axis ax = axis_of(px[c], P.ox, P.dx, P.nx);
   const Axis ay = axis_of(py[c], P.oy, P.dy, P.ny);
   if (!ax.inside || !ay.inside) {
-    fail_cell(failure, flags, c, kStencilFailure, P.step);
+    fail_cell(failure, flags, caller, kStencilFailure, P.step);
     return;
   }
   uint32_t site[kCorners], cx[kCorners], cy[kCorners];
@@ -169,7 +177,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
       }
     }
     if (seed == count) {
-      fail_cell(failure, flags, c, kStencilFailure, P.step);
+      fail_cell(failure, flags, caller, kStencilFailure, P.step);
       return;
     }
     bool connected[kCorners] = {false, false, false, false};
@@ -276,7 +284,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
     }
     work[index * T] = v;
     if (!isfinite(v)) {
-      fail_cell(failure, flags, c, index, P.step);
+      fail_cell(failure, flags, caller, index, P.step);
       return;
     }
   }
@@ -284,7 +292,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
   for (uint32_t k = 0; k < K; ++k) {
     const float v = held[k * T] + P.dt * work[outputs[k] * T];
     if (!isfinite(v)) {
-      fail_cell(failure, flags, c, kSpeciesFailure, P.step);
+      fail_cell(failure, flags, caller, kSpeciesFailure, P.step);
       return;
     }
     species_out[base + k] = v;
@@ -292,12 +300,14 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
   for (uint32_t s = 0; s < S; ++s) {
     exchange[static_cast<size_t>(c) * S + s] = work[outputs[K + s] * T] / P.volume;
   }
+  // The pairs: at the caller's index in the sort's input, naming the slot.
 #pragma unroll
   for (uint32_t e = 0; e < kCorners; ++e) {
-    const size_t j = static_cast<size_t>(c) * kCorners + e;
+    const size_t j = static_cast<size_t>(caller) * kCorners + e;
+    const size_t slot = static_cast<size_t>(c) * kCorners + e;
     keys[j] = e < count ? site[e] : static_cast<uint32_t>(n);
-    entries[j] = static_cast<uint32_t>(j);
-    weights[j] = e < count ? w[e] : 0.0f;
+    entries[j] = static_cast<uint32_t>(slot);
+    weights[slot] = e < count ? w[e] : 0.0f;
   }
 }
 
@@ -399,6 +409,41 @@ __global__ __launch_bounds__(kSiteBlock) void k_coupled_sources(
   if (bad) {
     atomicOr(&flags[0], kFlagSources);
     atomicMin(&flags[1], step);
+  }
+}
+
+// The site of each cell's stencil corner below it (clamped into the grid):
+// the key of the spatial order, and the cell's index.
+__global__ __launch_bounds__(kSiteBlock) void k_cell_keys(uint32_t nx, uint32_t ny, float dx,
+                                                          float dy, float ox, float oy,
+                                                          const float* __restrict__ px,
+                                                          const float* __restrict__ py,
+                                                          uint32_t cells, uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ index) {
+  const uint32_t c = blockIdx.x * kSiteBlock + threadIdx.x;
+  if (c >= cells) return;
+  const double fx = floor((static_cast<double>(px[c]) - ox) / dx);
+  const double fy = floor((static_cast<double>(py[c]) - oy) / dy);
+  const uint32_t x = static_cast<uint32_t>(fmin(fmax(fx, 0.0), static_cast<double>(nx - 1)));
+  const uint32_t y = static_cast<uint32_t>(fmin(fmax(fy, 0.0), static_cast<double>(ny - 1)));
+  keys[c] = x * ny + y;
+  index[c] = c;
+}
+
+// out[slot * width + k] = in[order[slot] * width + k] (to the spatial order),
+// or the reverse with `back` (to the caller's order).
+__global__ __launch_bounds__(kSiteBlock) void k_permute(const uint32_t* __restrict__ order,
+                                                        size_t count, uint32_t width, bool back,
+                                                        const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
+  if (i >= count * width) return;
+  const size_t slot = i / width, k = i % width;
+  const size_t caller = static_cast<size_t>(order[slot]) * width + k;
+  if (back) {
+    out[caller] = in[i];
+  } else {
+    out[i] = in[caller];
   }
 }
 
@@ -507,7 +552,8 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
     k_coupled_cells<<<(c.cells + threads - 1) / threads, threads, lds, ctx.stream>>>(
         P, a.solid.data(), levels, c.position_x.data(), c.position_y.data(),
         c.position_z.data(), c.lengths.data(), c.radii.data(), c.growth.data(), c.types.data(),
-        previous, c.plan.data(), c.outputs.data(), species_in, species_out, c.exchange.data(),
+        previous, c.order.data(), c.plan.data(), c.outputs.data(), species_in, species_out,
+        c.exchange.data(),
         c.keys.data(), c.entries.data(), c.weights.data(), c.failure.data(), a.flags.data());
     check(hipGetLastError(), "launch k_coupled_cells");
     const size_t pairs = static_cast<size_t>(c.cells) * kCorners;
@@ -568,29 +614,70 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   c.has_mask = has_mask;
   std::memcpy(c.origin, origin, sizeof(c.origin));
   const size_t count = cells.count, n = a.sites();
-  const auto put = [&](DeviceBuffer<float>& buffer, const float* host, size_t items) {
-    buffer.reserve(items);
-    if (items > 0) {
-      check(hipMemcpyAsync(buffer.data(), host, items * sizeof(float), hipMemcpyHostToDevice,
-                           ctx.stream),
-            "upload cells");
-    }
-  };
-  put(c.position_x, cells.position_x, count);
-  put(c.position_y, cells.position_y, count);
-  put(c.position_z, cells.position_z, count);
-  put(c.lengths, cells.lengths, count);
-  put(c.radii, cells.radii, count);
-  put(c.growth, cells.growth_rates, count);
-  c.types.reserve(count);
-  if (count > 0) {
-    check(hipMemcpyAsync(c.types.data(), cells.cell_types, count * sizeof(int32_t),
-                         hipMemcpyHostToDevice, ctx.stream),
+  const uint32_t K = plan.species_count;
+  const size_t pairs = count * kCorners;
+  for (auto* b : {&c.keys, &c.entries, &c.sorted_keys, &c.sorted_entries}) b->reserve(pairs);
+  c.order.reserve(count);
+  c.staging.reserve(std::max<size_t>(2 * count, count * K));
+  size_t bytes = 0;
+  check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
+                                           c.entries.data(), c.sorted_entries.data(),
+                                           static_cast<int>(std::max<size_t>(pairs, 1)), 0, 32,
+                                           ctx.stream),
+        "size coupled sort");
+  c.sort_scratch.reserve(bytes);
+  auto* staging = reinterpret_cast<uint32_t*>(c.staging.data());
+  const auto stage = [&](const void* host, size_t items, size_t at) {
+    check(hipMemcpyAsync(staging + at, host, items * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         ctx.stream),
           "upload cells");
+  };
+  // The spatial order: a stable sort of the cells by their lower corner's site.
+  if (count > 0) {
+    stage(cells.position_x, count, 0);
+    stage(cells.position_y, count, count);
+    k_cell_keys<<<blocks_of(count), kSiteBlock, 0, ctx.stream>>>(
+        a.params.nx, a.params.ny, a.params.dx, a.params.dy, origin[0], origin[1],
+        c.staging.data(), c.staging.data() + count, static_cast<uint32_t>(count), c.keys.data(),
+        c.entries.data());
+    check(hipGetLastError(), "launch k_cell_keys");
+    const int end_bit = std::max(1, static_cast<int>(std::bit_width(static_cast<uint64_t>(n))));
+    size_t need = 0;
+    check(hipcub::DeviceRadixSort::SortPairs(nullptr, need, c.keys.data(), c.sorted_keys.data(),
+                                             c.entries.data(), c.order.data(),
+                                             static_cast<int>(count), 0, end_bit, ctx.stream),
+          "size cell order");
+    if (need > c.sort_scratch.size()) c.sort_scratch.reserve(need);
+    need = c.sort_scratch.size();
+    check(hipcub::DeviceRadixSort::SortPairs(c.sort_scratch.data(), need, c.keys.data(),
+                                             c.sorted_keys.data(), c.entries.data(),
+                                             c.order.data(), static_cast<int>(count), 0, end_bit,
+                                             ctx.stream),
+          "sort cell order");
   }
+  // Every per-cell array to the spatial order.
+  const auto place = [&](auto& buffer, const void* host, uint32_t width) {
+    buffer.reserve(count * width);
+    if (count == 0 || width == 0) return;
+    const size_t items = count * width;
+    const bool positions = host == cells.position_x || host == cells.position_y;
+    const size_t at = host == cells.position_y ? count : 0;
+    if (!positions) stage(host, items, 0);
+    k_permute<<<blocks_of(items), kSiteBlock, 0, ctx.stream>>>(
+        c.order.data(), count, width, false, staging + at,
+        reinterpret_cast<uint32_t*>(buffer.data()));
+    check(hipGetLastError(), "launch k_permute");
+  };
+  place(c.position_x, cells.position_x, 1);
+  place(c.position_y, cells.position_y, 1);
+  place(c.position_z, cells.position_z, 1);
+  place(c.lengths, cells.lengths, 1);
+  place(c.radii, cells.radii, 1);
+  place(c.growth, cells.growth_rates, 1);
+  place(c.types, cells.cell_types, 1);
+  place(c.species_a, species, K);
   c.previous.reserve(count);
-  put(c.species_a, species, count * plan.species_count);
-  c.species_b.reserve(count * plan.species_count);
+  c.species_b.reserve(count * K);
   c.species_current = 0;
   c.plan.reserve(plan.instruction_count);
   check(hipMemcpyAsync(c.plan.data(), plan.instructions,
@@ -603,21 +690,12 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   check(hipMemcpyAsync(c.outputs.data(), outputs.data(), outputs.size() * sizeof(uint32_t),
                        hipMemcpyHostToDevice, ctx.stream),
         "upload plan");
-  const size_t pairs = count * kCorners;
-  for (auto* b : {&c.keys, &c.entries, &c.sorted_keys, &c.sorted_entries}) b->reserve(pairs);
   c.weights.reserve(pairs);
   c.exchange.reserve(count * plan.signal_count);
   c.sources.reserve(static_cast<size_t>(plan.signal_count) * n);
   c.runs.reserve(2 * n);
   c.largest.reserve(blocks_of(n));
   c.failure.reserve(1);
-  size_t bytes = 0;
-  check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
-                                           c.entries.data(), c.sorted_entries.data(),
-                                           static_cast<int>(std::max<size_t>(pairs, 1)), 0, 32,
-                                           ctx.stream),
-        "size coupled sort");
-  c.sort_scratch.reserve(bytes);
   k_fill_u32<<<blocks_of(2 * n), kSiteBlock, 0, ctx.stream>>>(c.runs.data(), 2 * n, 0u);
   check(hipGetLastError(), "launch k_fill_u32");
   check(hipStreamSynchronize(ctx.stream), "upload coupled state");
@@ -636,9 +714,13 @@ void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t
   // The first step's previous lengths; later steps' are the cells' own.
   if (c.cells > 0) {
     if (previous_lengths != nullptr) {
-      check(hipMemcpyAsync(c.previous.data(), previous_lengths, c.cells * sizeof(float),
+      check(hipMemcpyAsync(c.staging.data(), previous_lengths, c.cells * sizeof(float),
                            hipMemcpyHostToDevice, ctx.stream),
             "upload previous lengths");
+      k_permute<<<blocks_of(c.cells), kSiteBlock, 0, ctx.stream>>>(
+          c.order.data(), c.cells, 1, false, reinterpret_cast<const uint32_t*>(c.staging.data()),
+          reinterpret_cast<uint32_t*>(c.previous.data()));
+      check(hipGetLastError(), "launch k_permute");
     } else {
       check(hipMemcpyAsync(c.previous.data(), c.lengths.data(), c.cells * sizeof(float),
                            hipMemcpyDeviceToDevice, ctx.stream),
@@ -717,7 +799,12 @@ void coupled_download(Context& ctx, float* levels, float* species) {
   if (levels != nullptr) adr_download(ctx, levels);
   if (species != nullptr && c.cells > 0 && c.species > 0) {
     const float* src = c.species_current == 0 ? c.species_a.data() : c.species_b.data();
-    check(hipMemcpyAsync(species, src, static_cast<size_t>(c.cells) * c.species * sizeof(float),
+    k_permute<<<blocks_of(static_cast<size_t>(c.cells) * c.species), kSiteBlock, 0, ctx.stream>>>(
+        c.order.data(), c.cells, c.species, true, reinterpret_cast<const uint32_t*>(src),
+        reinterpret_cast<uint32_t*>(c.staging.data()));
+    check(hipGetLastError(), "launch k_permute");
+    check(hipMemcpyAsync(species, c.staging.data(),
+                         static_cast<size_t>(c.cells) * c.species * sizeof(float),
                          hipMemcpyDeviceToHost, ctx.stream),
           "download species");
     check(hipStreamSynchronize(ctx.stream), "download species");

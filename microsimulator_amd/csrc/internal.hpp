@@ -295,8 +295,8 @@ struct LearnedDevice {
   DeviceBuffer<float> z, d, gz, grads, losses;
 };
 
-// Coupled cell/grid exchange (coupled.hip): the cells, their species in two
-// generations, the rate plan, and the step's scratch: per cell four stencil
+// Coupled cell/grid exchange (coupled.hip): the cells in spatial order, their
+// species in two generations, the rate plan, and the step's scratch: per cell four stencil
 // corners as (site, entry) pairs with their weights, the pairs sorted by site,
 // each site's run of sorted pairs, the exchange rates and the sources.
 struct CoupledDevice {
@@ -310,6 +310,8 @@ struct CoupledDevice {
   int species_current = 0;
   DeviceBuffer<msim_rate_instruction> plan;
   DeviceBuffer<uint32_t> outputs;  // species outputs, then signal outputs
+  DeviceBuffer<uint32_t> order;    // slot -> the cell's index in the caller's order
+  DeviceBuffer<float> staging;      // host arrays on their way to or from the slots
   DeviceBuffer<uint32_t> keys, entries, sorted_keys, sorted_entries, runs;
   DeviceBuffer<float> weights, exchange, sources;
   DeviceBuffer<unsigned long long> failure;  // (cell << 32 | instruction) of the first failure
