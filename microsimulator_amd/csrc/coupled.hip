@@ -1,33 +1,32 @@
 // Coupled cell/grid exchange on the device (advance_coupled_cpu,
 // cpp/cpu/cpu_coupled.cpp:125-231), one layer, FP32 levels.
 //
+// The device keeps the cells in spatial order: a stable sort at upload by
+// the site of each cell's lower stencil corner, with `order` mapping a slot
+// to the cell's index in the caller's order and `offsets` giving each lower
+// corner's run of slots. Cells do not move between the steps of a run.
+//
 // A step is these launches:
-//  * k_coupled_cells, one thread per cell: the cell's bilinear stencil (with
+//  * k_coupled_cells, one thread per slot: the cell's bilinear stencil (with
 //    obstacles, only the corners face-connected to the strongest fluid one,
 //    renormalised), the signals it samples from the step's starting levels,
 //    the dilution of its species, the rate plan run by an interpreter whose
 //    working values live in LDS (a column per thread), the species update,
-//    the exchange rate of each signal, and one (site, entry) pair per stencil
-//    corner;
-//  * a stable radix sort of the pairs by site, so each site's entries stand
-//    in cell order, and k_coupled_runs, which finds each site's run of them;
+//    the exchange rate of each signal, and the weights of its 2 x 2 corners;
 //  * forward Euler: the explicit candidate (adr.hip), then k_coupled_commit,
-//    which adds dt * source to each level, the source summed over the site's
-//    run in cell order exactly as the reference adds it, and judges the
-//    levels (accept_step); the implicit kinds: k_coupled_sources writes the
-//    sources out for the solve's right-hand side.
-// The device keeps the cells in spatial order (by the site of their stencil's
-// lower corner, computed at upload), so the cell kernel's samples of the grid
-// and the commit's reads of the exchange rates fall on nearby lines; `order`
-// maps a slot to the cell's index in the caller's order. That index places
-// the cell's pairs in the sort's input, so the stable sort still leaves each
-// site's entries in the caller's cell order, and it names a failing cell.
+//    which adds dt * source to each level and judges the levels
+//    (accept_step); the implicit kinds: k_coupled_sources writes the sources
+//    out for the solve's right-hand side.
+// A site's source is the reference's float sum in the caller's cell order
+// (cpu_coupled.cpp:198-207). The cells that reach site (x, y) are those whose
+// lower corner is (x, y), (x, y - 1), (x - 1, y) or (x - 1, y - 1): four runs
+// of slots, each in caller order (the stable sort kept it), merged by the
+// caller's index as the sum goes.
 // Every kernel returns at once when an earlier failure set the transport
-// flags, so a run stops at its first refused step; the commit and source
-// kernels clear each site's run after reading it, so the runs array is all
-// zero between steps.
+// flags, so a run stops at its first refused step.
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
 #include <bit>
@@ -132,8 +131,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
                                 const msim_rate_instruction* __restrict__ plan,
                                 const uint32_t* __restrict__ outputs,
                                 const float* __restrict__ species_in, float* species_out,
-                                float* __restrict__ exchange, uint32_t* __restrict__ keys,
-                                uint32_t* __restrict__ entries, float* __restrict__ weights,
+                                float* __restrict__ exchange, float4* __restrict__ weights,
                                 unsigned long long* failure, uint32_t* flags) {
   extern __shared__ float s_work[];
   if (flags[0] != 0) return;  // an earlier step failed: the run stops there
@@ -300,74 +298,96 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
   for (uint32_t s = 0; s < S; ++s) {
     exchange[static_cast<size_t>(c) * S + s] = work[outputs[K + s] * T] / P.volume;
   }
-  // The pairs: at the caller's index in the sort's input, naming the slot.
+  // The corners' weights at (a, b) = (0, 0), (0, 1), (1, 0), (1, 1); -1 where
+  // the stencil has no such corner (an axis of one site, or the upper edge).
+  float corner[kCorners];
 #pragma unroll
-  for (uint32_t e = 0; e < kCorners; ++e) {
-    const size_t j = static_cast<size_t>(caller) * kCorners + e;
-    const size_t slot = static_cast<size_t>(c) * kCorners + e;
-    keys[j] = e < count ? site[e] : static_cast<uint32_t>(n);
-    entries[j] = static_cast<uint32_t>(slot);
-    weights[slot] = e < count ? w[e] : 0.0f;
+  for (uint32_t f = 0; f < kCorners; ++f) {
+    const uint32_t fa = f >> 1, fb = f & 1;
+    const uint32_t e = fa * ay.count + fb;
+    const float pick = e == 0 ? w[0] : (e == 1 ? w[1] : (e == 2 ? w[2] : w[3]));
+    corner[f] = fa < ax.count && fb < ay.count ? pick : -1.0f;
   }
+  weights[c] = make_float4(corner[0], corner[1], corner[2], corner[3]);
 }
 
-// Each site's run [begin, end) in the sorted pairs; sites without pairs keep
-// the zero run their last reader left.
-__global__ __launch_bounds__(kSiteBlock) void k_coupled_runs(const uint32_t* __restrict__ keys,
-                                                             size_t pairs, uint32_t sites,
-                                                             uint32_t* __restrict__ runs,
-                                                             const uint32_t* flags) {
-  if (flags[0] != 0) return;
-  const size_t j = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
-  if (j >= pairs) return;
-  const uint32_t k = keys[j];
-  if (k >= sites) return;  // an unused corner
-  if (j == 0 || keys[j - 1] != k) runs[2 * static_cast<size_t>(k)] = static_cast<uint32_t>(j);
-  if (j + 1 == pairs || keys[j + 1] != k) runs[2 * static_cast<size_t>(k) + 1] = static_cast<uint32_t>(j + 1);
-}
-
-// The sources of one site: per signal the sum over its run, in cell order,
-// of weight * exchange (cpu_coupled.cpp:198-207), from 0. Clears the run.
+// The sources of site (x, y): per signal, from 0, the sum of weight *
+// exchange over the cells whose stencil reaches the site, in the caller's
+// cell order (cpu_coupled.cpp:198-207). Corner (da, db) of the cells with
+// lower corner (x - da, y - db) is this site; the four runs are merged by the
+// caller's index. A corner the stencil lacks (weight -1) adds nothing; a
+// present corner of weight 0 adds its product, as the reference does.
 template <int S>
-__device__ inline void site_sources(size_t site, uint32_t* __restrict__ runs,
-                                    const uint32_t* __restrict__ entries,
+__device__ inline void site_sources(uint32_t x, uint32_t y, uint32_t ny,
+                                    const uint32_t* __restrict__ offsets,
+                                    const uint32_t* __restrict__ order,
                                     const float* __restrict__ weights,
                                     const float* __restrict__ exchange, float (&src)[S]) {
 #pragma unroll
   for (int s = 0; s < S; ++s) src[s] = 0.0f;
-  const uint32_t begin = runs[2 * site], end = runs[2 * site + 1];
-  if (end == 0) return;
-  for (uint32_t j = begin; j < end; ++j) {
-    const uint32_t entry = entries[j];
-    const float w = weights[entry];
-    const float* ex = exchange + static_cast<size_t>(entry / kCorners) * S;
+  uint32_t at[4], end[4], head[4];
 #pragma unroll
-    for (int s = 0; s < S; ++s) src[s] += w * ex[s];
+  for (int g = 0; g < 4; ++g) {
+    const uint32_t da = g >> 1, db = g & 1;
+    at[g] = end[g] = 0;
+    if (x >= da && y >= db) {
+      const size_t k = static_cast<size_t>(x - da) * ny + (y - db);
+      at[g] = offsets[k];
+      end[g] = offsets[k + 1];
+    }
+    head[g] = at[g] < end[g] ? order[at[g]] : 0xffffffffu;
   }
-  runs[2 * site] = 0;
-  runs[2 * site + 1] = 0;
+  for (;;) {
+    int pick = -1;
+    uint32_t least = 0xffffffffu;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (at[g] < end[g] && head[g] < least) {
+        least = head[g];
+        pick = g;
+      }
+    }
+    if (pick < 0) break;
+    uint32_t slot = 0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (g == pick) {
+        slot = at[g]++;
+        head[g] = at[g] < end[g] ? order[at[g]] : 0xffffffffu;
+      }
+    }
+    // Lower corner (x - da, y - db) reaches (x, y) with its corner (da, db),
+    // whose weight is the cell's entry g = 2 da + db.
+    const float w = weights[static_cast<size_t>(slot) * kCorners + pick];
+    if (w >= 0.0f) {
+      const float* ex = exchange + static_cast<size_t>(slot) * S;
+#pragma unroll
+      for (int s = 0; s < S; ++s) src[s] += w * ex[s];
+    }
+  }
 }
 
 // Forward Euler's commit: level + dt * source for every level of the site,
 // obstacle sites included (cpu_coupled.cpp:216-222), then the levels judged
 // as accept_step does: under upwind each must be finite and non-negative;
 // under van Leer the block's largest level goes to `largest` for the settle.
-// Writes nothing once a failure is flagged, but still clears the runs.
+// Writes nothing once a failure is flagged.
 template <int S, bool kLimited>
 __global__ __launch_bounds__(kSiteBlock) void k_coupled_commit(
-    size_t n, const float* in, float* out, float dt, uint32_t* __restrict__ runs,
-    const uint32_t* __restrict__ entries, const float* __restrict__ weights,
-    const float* __restrict__ exchange, double* __restrict__ largest, uint32_t* flags,
-    uint32_t step) {
+    size_t n, uint32_t ny, const float* in, float* out, float dt,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ order,
+    const float* __restrict__ weights, const float* __restrict__ exchange,
+    double* __restrict__ largest, uint32_t* flags, uint32_t step) {
   __shared__ double s_red[kSiteBlock / kWave];
   const bool failed = flags[0] != 0;
   const size_t site = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
   double top = 0.0;
   bool bad = false;
-  if (site < n) {
+  if (site < n && !failed) {
     float src[S];
-    site_sources<S>(site, runs, entries, weights, exchange, src);
-    if (!failed) {
+    site_sources<S>(static_cast<uint32_t>(site / ny), static_cast<uint32_t>(site % ny), ny,
+                    offsets, order, weights, exchange, src);
+    {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const float v = in[s * n + site] + dt * src[s];
@@ -390,16 +410,17 @@ __global__ __launch_bounds__(kSiteBlock) void k_coupled_commit(
 }
 
 // The implicit kinds' sources, S*N (a non-finite one refuses the step,
-// signals.cpp:1295-1310). Clears the runs.
+// signals.cpp:1295-1310).
 template <int S>
 __global__ __launch_bounds__(kSiteBlock) void k_coupled_sources(
-    size_t n, float* __restrict__ sources, uint32_t* __restrict__ runs,
-    const uint32_t* __restrict__ entries, const float* __restrict__ weights,
+    size_t n, uint32_t ny, float* __restrict__ sources, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ order, const float* __restrict__ weights,
     const float* __restrict__ exchange, uint32_t* flags, uint32_t step) {
   const size_t site = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
   if (site >= n) return;
   float src[S];
-  site_sources<S>(site, runs, entries, weights, exchange, src);
+  site_sources<S>(static_cast<uint32_t>(site / ny), static_cast<uint32_t>(site % ny), ny, offsets,
+                  order, weights, exchange, src);
   bool bad = false;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -447,6 +468,14 @@ __global__ __launch_bounds__(kSiteBlock) void k_permute(const uint32_t* __restri
   }
 }
 
+// Cells per lower corner (integer counts: their order does not matter).
+__global__ __launch_bounds__(kSiteBlock) void k_count_sites(const uint32_t* __restrict__ keys,
+                                                            uint32_t cells,
+                                                            uint32_t* __restrict__ counts) {
+  const uint32_t p = blockIdx.x * kSiteBlock + threadIdx.x;
+  if (p < cells) atomicAdd(&counts[keys[p]], 1u);
+}
+
 __global__ __launch_bounds__(kSiteBlock) void k_fill_u32(uint32_t* __restrict__ out, size_t count,
                                                          uint32_t value) {
   const size_t i = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
@@ -474,13 +503,14 @@ void launch_commit(Context& ctx, bool limited, const float* in, float* out, floa
                    uint32_t step) {
   auto& c = ctx.coupled;
   const size_t n = ctx.adr.sites();
+  const uint32_t ny = ctx.adr.params.ny;
   if (limited) {
     k_coupled_commit<S, true><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
-        n, in, out, dt, c.runs.data(), c.sorted_entries.data(), c.weights.data(),
+        n, ny, in, out, dt, c.offsets.data(), c.order.data(), c.weights.data(),
         c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
   } else {
     k_coupled_commit<S, false><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
-        n, in, out, dt, c.runs.data(), c.sorted_entries.data(), c.weights.data(),
+        n, ny, in, out, dt, c.offsets.data(), c.order.data(), c.weights.data(),
         c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
   }
 }
@@ -490,7 +520,7 @@ void launch_sources(Context& ctx, uint32_t step) {
   auto& c = ctx.coupled;
   const size_t n = ctx.adr.sites();
   k_coupled_sources<S><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
-      n, c.sources.data(), c.runs.data(), c.sorted_entries.data(), c.weights.data(),
+      n, ctx.adr.params.ny, c.sources.data(), c.offsets.data(), c.order.data(), c.weights.data(),
       c.exchange.data(), ctx.adr.flags.data(), step);
 }
 
@@ -522,7 +552,7 @@ void sources(Context& ctx, uint32_t step) {
   check(hipGetLastError(), "launch k_coupled_sources");
 }
 
-// The cell kernel, the pair sort and the runs of one step.
+// The cell kernel of one step.
 void cell_phase(Context& ctx, const float* levels, const float* previous, float dt,
                 uint32_t step) {
   auto& c = ctx.coupled;
@@ -553,26 +583,9 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
         P, a.solid.data(), levels, c.position_x.data(), c.position_y.data(),
         c.position_z.data(), c.lengths.data(), c.radii.data(), c.growth.data(), c.types.data(),
         previous, c.order.data(), c.plan.data(), c.outputs.data(), species_in, species_out,
-        c.exchange.data(),
-        c.keys.data(), c.entries.data(), c.weights.data(), c.failure.data(), a.flags.data());
+        c.exchange.data(), reinterpret_cast<float4*>(c.weights.data()), c.failure.data(),
+        a.flags.data());
     check(hipGetLastError(), "launch k_coupled_cells");
-    const size_t pairs = static_cast<size_t>(c.cells) * kCorners;
-    const int end_bit = std::max(1, static_cast<int>(std::bit_width(static_cast<uint64_t>(n))));
-    size_t bytes = 0;
-    check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
-                                             c.entries.data(), c.sorted_entries.data(),
-                                             static_cast<int>(pairs), 0, end_bit, ctx.stream),
-          "size coupled sort");
-    if (bytes > c.sort_scratch.size()) c.sort_scratch.reserve(bytes);
-    bytes = c.sort_scratch.size();
-    check(hipcub::DeviceRadixSort::SortPairs(c.sort_scratch.data(), bytes, c.keys.data(),
-                                             c.sorted_keys.data(), c.entries.data(),
-                                             c.sorted_entries.data(), static_cast<int>(pairs), 0,
-                                             end_bit, ctx.stream),
-          "sort coupled pairs");
-    k_coupled_runs<<<blocks_of(pairs), kSiteBlock, 0, ctx.stream>>>(
-        c.sorted_keys.data(), pairs, static_cast<uint32_t>(n), c.runs.data(), a.flags.data());
-    check(hipGetLastError(), "launch k_coupled_runs");
   }
 }
 
@@ -615,17 +628,23 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   std::memcpy(c.origin, origin, sizeof(c.origin));
   const size_t count = cells.count, n = a.sites();
   const uint32_t K = plan.species_count;
-  const size_t pairs = count * kCorners;
-  for (auto* b : {&c.keys, &c.entries, &c.sorted_keys, &c.sorted_entries}) b->reserve(pairs);
+  // keys holds the sort's keys, then the counts per site (N + 1).
+  c.keys.reserve(std::max<size_t>(count, n + 1));
+  for (auto* b : {&c.entries, &c.sorted_keys}) b->reserve(std::max<size_t>(count, 1));
   c.order.reserve(count);
+  c.offsets.reserve(n + 1);
   c.staging.reserve(std::max<size_t>(2 * count, count * K));
   size_t bytes = 0;
   check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
-                                           c.entries.data(), c.sorted_entries.data(),
-                                           static_cast<int>(std::max<size_t>(pairs, 1)), 0, 32,
+                                           c.entries.data(), c.order.data(),
+                                           static_cast<int>(std::max<size_t>(count, 1)), 0, 32,
                                            ctx.stream),
         "size coupled sort");
-  c.sort_scratch.reserve(bytes);
+  size_t scan = 0;
+  check(hipcub::DeviceScan::ExclusiveSum(nullptr, scan, c.keys.data(), c.offsets.data(),
+                                         static_cast<int>(n + 1), ctx.stream),
+        "size offsets scan");
+  c.sort_scratch.reserve(std::max(bytes, scan));
   auto* staging = reinterpret_cast<uint32_t*>(c.staging.data());
   const auto stage = [&](const void* host, size_t items, size_t at) {
     check(hipMemcpyAsync(staging + at, host, items * sizeof(uint32_t), hipMemcpyHostToDevice,
@@ -655,6 +674,18 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
                                              ctx.stream),
           "sort cell order");
   }
+  // Each lower corner's run of slots: counts per site, then their exclusive sum.
+  k_fill_u32<<<blocks_of(n + 1), kSiteBlock, 0, ctx.stream>>>(c.keys.data(), n + 1, 0u);
+  check(hipGetLastError(), "launch k_fill_u32");
+  if (count > 0) {
+    k_count_sites<<<blocks_of(count), kSiteBlock, 0, ctx.stream>>>(
+        c.sorted_keys.data(), static_cast<uint32_t>(count), c.keys.data());
+    check(hipGetLastError(), "launch k_count_sites");
+  }
+  size_t scan_bytes = c.sort_scratch.size();
+  check(hipcub::DeviceScan::ExclusiveSum(c.sort_scratch.data(), scan_bytes, c.keys.data(),
+                                         c.offsets.data(), static_cast<int>(n + 1), ctx.stream),
+        "scan offsets");
   // Every per-cell array to the spatial order.
   const auto place = [&](auto& buffer, const void* host, uint32_t width) {
     buffer.reserve(count * width);
@@ -690,14 +721,11 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   check(hipMemcpyAsync(c.outputs.data(), outputs.data(), outputs.size() * sizeof(uint32_t),
                        hipMemcpyHostToDevice, ctx.stream),
         "upload plan");
-  c.weights.reserve(pairs);
+  c.weights.reserve(count * kCorners);
   c.exchange.reserve(count * plan.signal_count);
   c.sources.reserve(static_cast<size_t>(plan.signal_count) * n);
-  c.runs.reserve(2 * n);
   c.largest.reserve(blocks_of(n));
   c.failure.reserve(1);
-  k_fill_u32<<<blocks_of(2 * n), kSiteBlock, 0, ctx.stream>>>(c.runs.data(), 2 * n, 0u);
-  check(hipGetLastError(), "launch k_fill_u32");
   check(hipStreamSynchronize(ctx.stream), "upload coupled state");
   c.uploaded = true;
 }

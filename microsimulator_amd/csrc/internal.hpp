@@ -296,9 +296,9 @@ struct LearnedDevice {
 };
 
 // Coupled cell/grid exchange (coupled.hip): the cells in spatial order, their
-// species in two generations, the rate plan, and the step's scratch: per cell four stencil
-// corners as (site, entry) pairs with their weights, the pairs sorted by site,
-// each site's run of sorted pairs, the exchange rates and the sources.
+// species in two generations, the rate plan, and the step's scratch: per cell
+// the weights of its 2 x 2 stencil corners and its exchange rates, and the
+// implicit kinds' sources.
 struct CoupledDevice {
   bool uploaded = false;
   bool has_mask = false;  // the grid carries an obstacle mask (stencils renormalise)
@@ -311,8 +311,9 @@ struct CoupledDevice {
   DeviceBuffer<msim_rate_instruction> plan;
   DeviceBuffer<uint32_t> outputs;  // species outputs, then signal outputs
   DeviceBuffer<uint32_t> order;    // slot -> the cell's index in the caller's order
+  DeviceBuffer<uint32_t> offsets;  // N + 1: the slots whose lower corner is site k
   DeviceBuffer<float> staging;      // host arrays on their way to or from the slots
-  DeviceBuffer<uint32_t> keys, entries, sorted_keys, sorted_entries, runs;
+  DeviceBuffer<uint32_t> keys, entries, sorted_keys;  // the upload's sort
   DeviceBuffer<float> weights, exchange, sources;
   DeviceBuffer<unsigned long long> failure;  // (cell << 32 | instruction) of the first failure
   DeviceBuffer<double> largest;
