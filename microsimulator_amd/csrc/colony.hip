@@ -6,7 +6,8 @@
 // (the separable per-axis integrals in LDS, their product per voxel). With an
 // obstacle mask it zeroes solid voxels, finds the strongest voxel (the first
 // in the support's x, y, z order) and keeps only the face-connected fluid
-// component around it, by label propagation in LDS to a fixpoint. It then
+// component around it: on one layer as per-row bit masks grown by shifts and
+// wave shuffles, otherwise by label propagation in LDS, to a fixpoint. It then
 // adds the cell's biomass volume times the normalised weights into the grid
 // with FP64 atomics. k_colony_finish divides by the voxel volume and, for the
 // mobility, applies the Kozeny-Carman drag of each column's mean volume
@@ -127,7 +128,39 @@ __global__ __launch_bounds__(kDepositThreads) void k_colony_deposit(
         seed = v;
       }
     }
-    if (P.has_mask != 0) {
+    if (P.has_mask != 0 && count[2] == 1 && count[0] <= kWave && count[1] <= 64) {
+      // The face-connected component of the strongest voxel (flow.py:262-276)
+      // on one layer: lane i holds row i's fluid and connected voxels as bits
+      // and the component grows by shifts within rows and shuffles across
+      // them until no lane changes.
+      wave_argmax(best, seed);
+      __syncthreads();
+      const uint32_t rows = count[0], cols = count[1];
+      uint64_t fluid = 0, connected = 0;
+      if (tid < rows) {
+        for (uint32_t j = 0; j < cols; ++j) {
+          if (s_kernel[tid * cols + j] > 0.0) fluid |= 1ull << j;
+        }
+        if (seed / cols == tid) connected = fluid & (1ull << (seed % cols));
+      }
+      for (;;) {
+        const uint64_t below = __shfl_up(connected, 1, kWave);
+        const uint64_t above = __shfl_down(connected, 1, kWave);
+        uint64_t grown = connected | (connected << 1) | (connected >> 1);
+        if (tid >= 1) grown |= below;
+        if (tid + 1 < rows) grown |= above;
+        grown &= fluid;
+        const bool changed = grown != connected;
+        connected = grown;
+        if (!__any(changed)) break;
+      }
+      if (tid < rows) {
+        for (uint32_t j = 0; j < cols; ++j) {
+          if (((connected >> j) & 1ull) == 0) s_kernel[tid * cols + j] = 0.0;
+        }
+      }
+      __syncthreads();
+    } else if (P.has_mask != 0) {
       // The face-connected component of the strongest voxel (flow.py:262-276).
       wave_argmax(best, seed);
       for (uint32_t v = tid; v < region; v += kDepositThreads) {
