@@ -99,7 +99,54 @@ __global__ __launch_bounds__(kImplicitBlock) void k_implicit_setup(
   const bool affine = P.has_affine != 0, monod = P.has_monod != 0;
   double acc = 0.0;
   float low = 0.0f;
-  if (i < n && solid[i] == 0) {
+  if (!kLimited && i < n && solid[i] == 0) {
+    // Upwind: every value this site reads is loaded before any is used (as
+    // in the sweep); the arithmetic is the general path's below.
+    const uint32_t x = static_cast<uint32_t>(i / ny), y = static_cast<uint32_t>(i % ny);
+    const size_t row = static_cast<size_t>(x) * ny;
+    const size_t at[4] = {x == 0 ? static_cast<size_t>(nx - 1) * ny + y : i - ny,
+                          x + 1 == nx ? static_cast<size_t>(y) : i + ny,
+                          y == 0 ? row + ny - 1 : i - 1, y + 1 == ny ? row : i + 1};
+    float c[S], nb[S][4], L0[S], q[S];
+    const float occupancy = monod ? phi[i] : 0.0f;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const float* lv = old + s * n;
+      c[s] = lv[i];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) nb[s][k] = lv[at[k]];
+      L0[s] = affine ? loss[s * n + i] : 0.0f;
+      q[s] = affine ? source[s * n + i] : 0.0f;
+    }
+    const SiteGeometry base = geometry(P, solid, xf, yf, x, y, 0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      SiteGeometry g = base;
+      if (P.has_field == 0) {
+        g.ulo[0] = g.uup[0] = P.advection[s][0];
+        g.ulo[1] = g.uup[1] = P.advection[s][1];
+      }
+      const float v = c[s];
+      const float l0 = x == 0 ? exterior<kPatched>(P, 0, s, v, nb[s][0], y) : nb[s][0];
+      const float u0 = x + 1 == nx ? exterior<kPatched>(P, 1, s, v, nb[s][1], y) : nb[s][1];
+      const float l1 = y == 0 ? exterior<kPatched>(P, 2, s, v, nb[s][2], x) : nb[s][2];
+      const float u1 = y + 1 == ny ? exterior<kPatched>(P, 3, s, v, nb[s][3], x) : nb[s][3];
+      float rate = transport_rate_f32<false>(P, g, v, l0, u0, l1, u1, s);
+      float L = L0[s];
+      if (monod) {
+        const float uptake = P.vmax[s] * occupancy / (P.monod_k[s] + v);
+        L = affine ? L + uptake : uptake;
+        lagged[s * n + i] = L;
+      }
+      const bool reaction = affine || monod;
+      if (reaction) rate += q[s] - (L * v);
+      const float r = (v + (explicit_dt * rate)) +
+                      (dt * (coupled != nullptr ? coupled[s * n + i] : 0.0f));
+      rhs[s * n + i] = r;
+      acc += static_cast<double>(r) * static_cast<double>(r);
+      low = fminf(low, operator_diagonal_f32(P, g, s, reaction, L));
+    }
+  } else if (i < n && solid[i] == 0) {
     const uint32_t x = static_cast<uint32_t>(i / ny), y = static_cast<uint32_t>(i % ny);
     const SiteGeometry base = geometry(P, solid, xf, yf, x, y, 0);
 #pragma unroll
@@ -155,7 +202,52 @@ __global__ __launch_bounds__(kImplicitBlock) void k_implicit_sweep(
   const bool affine = P.has_affine != 0, reaction = affine || P.has_monod != 0;
   double acc = 0.0;
   bool bad = false;
-  if (i < n && solid[i] == 0) {
+  if constexpr (!kLimited) {
+    // Upwind: every level this site reads is loaded before any is used, so a
+    // wave has all of its loads in flight at once; then the same arithmetic
+    // as the general path below.
+    if (i < n && solid[i] == 0) {
+      const uint32_t x = static_cast<uint32_t>(i / ny), y = static_cast<uint32_t>(i % ny);
+      const size_t row = static_cast<size_t>(x) * ny;
+      const size_t at[4] = {x == 0 ? static_cast<size_t>(nx - 1) * ny + y : i - ny,
+                            x + 1 == nx ? static_cast<size_t>(y) : i + ny,
+                            y == 0 ? row + ny - 1 : i - 1, y + 1 == ny ? row : i + 1};
+      float c[S], nb[S][4], b[S], L[S], q[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const float* lv = cur + s * n;
+        c[s] = lv[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nb[s][k] = lv[at[k]];
+        b[s] = rhs[s * n + i];
+        L[s] = reaction ? loss[s * n + i] : 0.0f;
+        q[s] = affine ? source[s * n + i] : 0.0f;
+      }
+      const SiteGeometry base = geometry(P, solid, xf, yf, x, y, 0);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        SiteGeometry g = base;
+        if (P.has_field == 0) {
+          g.ulo[0] = g.uup[0] = P.advection[s][0];
+          g.ulo[1] = g.uup[1] = P.advection[s][1];
+        }
+        const float v = c[s];
+        if (!isfinite(v)) bad = true;
+        const float l0 = x == 0 ? exterior<kPatched>(P, 0, s, v, nb[s][0], y) : nb[s][0];
+        const float u0 = x + 1 == nx ? exterior<kPatched>(P, 1, s, v, nb[s][1], y) : nb[s][1];
+        const float l1 = y == 0 ? exterior<kPatched>(P, 2, s, v, nb[s][2], x) : nb[s][2];
+        const float u1 = y + 1 == ny ? exterior<kPatched>(P, 3, s, v, nb[s][3], x) : nb[s][3];
+        float rate = transport_rate_f32<false>(P, g, v, l0, u0, l1, u1, s);
+        if (reaction) rate += q[s] - (L[s] * v);
+        const float r = (b[s] - v) + (half_dt * rate);
+        acc += static_cast<double>(r) * static_cast<double>(r);
+        const float diagonal = operator_diagonal_f32(P, g, s, reaction, L[s]);
+        const float remainder = rate - (diagonal * v);
+        const float update = (b[s] + (half_dt * remainder)) / (1.0f - (half_dt * diagonal));
+        nxt[s * n + i] = v + (relaxation * (update - v));
+      }
+    }
+  } else if (i < n && solid[i] == 0) {
     const uint32_t x = static_cast<uint32_t>(i / ny), y = static_cast<uint32_t>(i % ny);
     const SiteGeometry base = geometry(P, solid, xf, yf, x, y, 0);
 #pragma unroll
