@@ -550,6 +550,8 @@ uint32_t site_blocks(size_t n) { return static_cast<uint32_t>((n + kAdrBlock - 1
 
 void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
                 uint32_t precision, bool use_flow_field) {
+  // Coupled state belongs to the grid it was uploaded with.
+  ctx.coupled.uploaded = false;
   auto& a = ctx.adr;
   auto& P = a.params;
   const size_t n = static_cast<size_t>(grid.grid.nx) * grid.grid.ny;

@@ -1393,6 +1393,13 @@ int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void*
 int msim_hip_adr_run(msim_ctx* ctx, float dt, uint32_t steps, double* ledger) {
   return msim::guarded(ctx, [&](Context& c) {
     if (!c.adr.uploaded) msim::invalid("no signal grid on the device");
+    // A transport uploaded over the device's flow field reads it in place:
+    // a later flow on another grid would leave it reading the wrong field.
+    if (c.adr.flow_field &&
+        (!c.flow.solved || c.flow.layers.nz != 1 || c.flow.nx != c.adr.params.nx ||
+         c.flow.ny != c.adr.params.ny)) {
+      msim::invalid("the flow field on the device is no longer the transport's");
+    }
     msim::validate_step(c, dt);
     if (dt == 0.0f || steps == 0) return;
     msim::adr_run(c, dt, steps, ledger);

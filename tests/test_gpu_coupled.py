@@ -313,3 +313,21 @@ def test_refused_step_in_a_run_keeps_its_starting_state(hip, oracle):
     assert str(got.value) == message
     levels, sp = hip.coupled_download()
     assert np.array_equal(levels, states[failed][0]) and np.array_equal(sp, states[failed][1])
+
+
+def test_a_new_grid_drops_the_coupled_state(hip):
+    """Uploading another signal grid leaves no coupled state to run on: the
+    run is refused instead of reading state sized for the old grid."""
+    kinds = TRANSPORT_KINDS[0]
+    c, cells, prev, plan, species = random_case(1, 20, 15, 2, kinds, msa.FORWARD_EULER, False)
+    grid = msa.Grid(20, 15, 1, 1.0, 0.8, 1.0, c["solid"], kinds)
+    hip.coupled_upload(spec_of(grid, c, msa.UPWIND, msa.FORWARD_EULER), c["levels"], cells,
+                       species, plan)
+    hip.coupled_run(0.02, 1, prev)
+    big = random_transport_case(64, 48, 2, kinds, 3)
+    other = msa.Grid(64, 48, 1, 1.0, 1.0, 1.0, big["solid"], kinds)
+    hip.adr_upload(spec_of(other, big, msa.UPWIND, msa.FORWARD_EULER),
+                   big["levels"].astype(np.float32), precision=32)
+    with pytest.raises(msa.InvalidArgument) as info:
+        hip.coupled_run(0.02, 1)
+    assert str(info.value) == "no coupled state on the device"
