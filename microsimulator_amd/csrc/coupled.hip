@@ -590,7 +590,7 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
 }
 
 // The failure the flags and the failure word name, as the reference reports it.
-[[noreturn]] void throw_failure(uint32_t flag_bits, unsigned long long failure, bool limited) {
+[[noreturn]] void throw_failure(uint32_t flag_bits, unsigned long long failure) {
   if ((flag_bits & kFlagCell) != 0) {
     const uint32_t what = static_cast<uint32_t>(failure & 0xffffffffull);
     if (what == kSpeciesFailure) {
@@ -609,7 +609,6 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
     throw Error(MSIM_DOMAIN_ERROR,
                 "signal grid update produced a non-finite or negative concentration");
   }
-  static_cast<void>(limited);
   throw Error(MSIM_INVALID_ARGUMENT, "signal grid levels must be finite and non-negative");
 }
 
@@ -801,7 +800,7 @@ void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t
     // The implicit kinds: the sources, judged on the host before the solve.
     sources(ctx, step);
     read_flags();
-    if (flag_bits[0] != 0) throw_failure(flag_bits[0], failure, limited);
+    if (flag_bits[0] != 0) throw_failure(flag_bits[0], failure);
     if (dt > 0.0f) {
       implicit_run(ctx, dt, 1, xf, yf, c.sources.data(), "Implicit coupled signal solve");
     }
@@ -818,7 +817,7 @@ void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t
     const uint32_t failed = flag_bits[1];
     a.current = levels_start ^ static_cast<int>(failed & 1u);
     c.species_current = species_start ^ static_cast<int>(failed & 1u);
-    throw_failure(flag_bits[0], failure, limited);
+    throw_failure(flag_bits[0], failure);
   }
 }
 
