@@ -15,7 +15,8 @@
 // aggregation under-corrects by about a factor of two). Each level is two
 // fused passes: `down` (pre-smooth, residual, restriction) and `up`
 // (prolongation, post-smooth), each staging a 16 x 64 tile plus halo in LDS.
-// Levels of <= 16384 cells run in one single-workgroup kernel.
+// Levels of <= 4096 cells run in one single-workgroup kernel (measured at
+// 4096x2048: 16384 cost 130 us of coarse work per cycle, 4096 109 us).
 //
 // Outer solver: flexible CG (Notay's FCG(1): beta_k = -(z.q_{k-1}) /
 // (p_{k-1}.q_{k-1})) in FP32 with FP64 sums, inside the FP64 iterative
@@ -41,7 +42,7 @@ constexpr int kHY = kTY + 2;
 constexpr int kMgThreads = (kTX / 2) * (kTY / 2);  // one thread per 2x2 aggregate
 constexpr int kTailThreads = 1024;
 constexpr uint32_t kCoarseCells = 256;  // densely inverted
-constexpr uint32_t kTailCells = 16384;  // levels handled by the single-block kernel
+constexpr uint32_t kTailCells = 4096;  // levels handled by the single-block kernel
 constexpr uint32_t kMaxLevels = 24;
 constexpr float kOmega = 0.8f;
 constexpr float kCorrection = 1.8f;
