@@ -3,10 +3,10 @@
 the CPU oracle, which is bit-exact with the reference's advance_coupled_cpu.
 
 Tolerances. The stencils, samples, dilution, every arithmetic and comparison
-instruction, the per-site source sums (in cell order, after a stable sort of
-the stencil pairs) and the transport are the reference's float operations in
-its order, so grid levels, species and the implicit report are compared bit
-for bit. The exponential, logarithm and power are not: the device evaluates
+instruction, the per-site source sums (in the caller's cell order, merged from
+each site's lower-corner runs of the spatially ordered cells) and the
+transport are the reference's float operations in its order, so grid levels,
+species and the implicit report are compared bit for bit. The exponential, logarithm and power are not: the device evaluates
 them in FP64 and rounds to float, the reference calls the C library's float
 functions (within one ulp of each other), so plans that use them hold species
 to a relative 1e-5 (the grid stays exact wherever the exchange does not use
