@@ -906,10 +906,11 @@ void upload_coupled(Context& c, const msim_signal_grid& g, const float* origin,
 // The host's part of colony_volume_fraction's checks (flow.py:282-305,
 // 224-241; biomass.py:12-13): the radius, and per cell in order a finite
 // position and, for a cell inside the grid's extent, a valid length and
-// radius. Returns the first refused cell (cells.count when none) and its
-// message.
+// radius. Returns the first refused cell (cells.count when none), its message
+// and whether its size (not its position) was refused.
 uint32_t colony_host_checks(const msim_grid& g, const float* origin, const msim_cells& cells,
-                            double radius, std::string* message) {
+                            double radius, std::string* message, bool* size_refused) {
+  *size_refused = false;
   if (!std::isfinite(radius) || radius <= 0.0) invalid("averaging radius must be finite and positive");
   validate_lattice(g);
   for (int a = 0; a < 3; ++a) {
@@ -938,6 +939,7 @@ uint32_t colony_host_checks(const msim_grid& g, const float* origin, const msim_
     const double length = cells.lengths[c], r = cells.radii[c];
     if (!outside && (!std::isfinite(length) || length < 0 || !std::isfinite(r) || r <= 0)) {
       *message = "length must be finite and nonnegative; radius finite and positive";
+      *size_refused = true;
       return c;
     }
   }
@@ -951,7 +953,8 @@ void colony_run(Context& ctx, const msim_grid& g, const float* origin, const msi
                 double radius, const double* base, uint32_t base_count, double drag, double cap,
                 double* density, double* mobility, const std::string& pending) {
   std::string message;
-  const uint32_t refused = colony_host_checks(g, origin, cells, radius, &message);
+  bool size_refused = false;
+  const uint32_t refused = colony_host_checks(g, origin, cells, radius, &message, &size_refused);
   const uint32_t axis = [&] {
     uint32_t total = 0;
     const uint32_t dims[3] = {g.nx, g.ny, g.nz};
@@ -971,7 +974,6 @@ void colony_run(Context& ctx, const msim_grid& g, const float* origin, const msi
   msim_cells head = cells;
   // A refused size is judged after the cell's support, so that cell's
   // support is still deposited (and may fail first); a refused position is not.
-  const bool size_refused = refused < cells.count && message[0] == 'l';
   head.count = refused + (size_refused ? 1u : 0u);
   // With a refused mobility argument only the deposit's own failures can
   // come first: deposit into the density, then report.
