@@ -248,24 +248,79 @@ __global__ __launch_bounds__(kImplicitBlock) void k_implicit_sweep(
       }
     }
   } else if (i < n && solid[i] == 0) {
+    // van Leer: the same, with the four sites two away (site_limiter's, in
+    // adr_site.hpp) loaded up front as well.
     const uint32_t x = static_cast<uint32_t>(i / ny), y = static_cast<uint32_t>(i % ny);
-    const SiteGeometry base = geometry(P, solid, xf, yf, x, y, 0);
+    const size_t row = static_cast<size_t>(x) * ny;
+    const size_t at[4] = {x == 0 ? static_cast<size_t>(nx - 1) * ny + y : i - ny,
+                          x + 1 == nx ? static_cast<size_t>(y) : i + ny,
+                          y == 0 ? row + ny - 1 : i - 1, y + 1 == ny ? row : i + 1};
+    const bool xp3 = P.kind[0] == MSIM_PERIODIC && nx >= 3;
+    const bool yp3 = P.kind[2] == MSIM_PERIODIC && ny >= 3;
+    // Far sites: x - 2, x + 2, y - 2, y + 2 (wrapped on a periodic axis of at
+    // least three sites, absent past a boundary otherwise).
+    size_t far_at[4];
+    bool far_in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int axis = k >> 1;
+      const int64_t count = axis == 0 ? nx : ny;
+      const int64_t along = static_cast<int64_t>(axis == 0 ? x : y) + ((k & 1) ? 2 : -2);
+      far_in[k] = (axis == 0 ? xp3 : yp3) || (along >= 0 && along < count);
+      const uint32_t wrapped = static_cast<uint32_t>(((along % count) + count) % count);
+      far_at[k] = axis == 0 ? static_cast<size_t>(wrapped) * ny + y : row + wrapped;
+    }
+    bool far_open[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) far_open[k] = far_in[k] && solid[far_at[k]] == 0;
+    float c[S], nb[S][4], fv[S][4], b[S], L[S], q[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const float* lv = cur + s * n;
-      SiteView v;
-      site_view<kPatched>(P, solid, xf, yf, lv, x, y, s, base, v);
-      if (!isfinite(v.c)) bad = true;
-      float rate = site_rate<kLimited, kPatched>(P, solid, lv, x, y, s, v);
-      const float L = reaction ? loss[s * n + i] : 0.0f;
-      if (reaction) rate += (affine ? source[s * n + i] : 0.0f) - (L * v.c);
-      const float b = rhs[s * n + i];
-      const float r = (b - v.c) + (half_dt * rate);
+      c[s] = lv[i];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        nb[s][k] = lv[at[k]];
+        fv[s][k] = far_in[k] ? lv[far_at[k]] : 0.0f;
+      }
+      b[s] = rhs[s * n + i];
+      L[s] = reaction ? loss[s * n + i] : 0.0f;
+      q[s] = affine ? source[s * n + i] : 0.0f;
+    }
+    const SiteGeometry base = geometry(P, solid, xf, yf, x, y, 0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      SiteGeometry g = base;
+      if (P.has_field == 0) {
+        g.ulo[0] = g.uup[0] = P.advection[s][0];
+        g.ulo[1] = g.uup[1] = P.advection[s][1];
+      }
+      const float v = c[s];
+      if (!isfinite(v)) bad = true;
+      const float l0 = x == 0 ? exterior<kPatched>(P, 0, s, v, nb[s][0], y) : nb[s][0];
+      const float u0 = x + 1 == nx ? exterior<kPatched>(P, 1, s, v, nb[s][1], y) : nb[s][1];
+      const float l1 = y == 0 ? exterior<kPatched>(P, 2, s, v, nb[s][2], x) : nb[s][2];
+      const float u1 = y + 1 == ny ? exterior<kPatched>(P, 3, s, v, nb[s][3], x) : nb[s][3];
+      Limiter<float> lim;
+      lim.lower_site[0] = !g.clo[0] && (x > 0 || xp3);
+      lim.upper_site[0] = !g.cup[0] && (x + 1 < nx || xp3);
+      lim.lower_site[1] = !g.clo[1] && (y > 0 || yp3);
+      lim.upper_site[1] = !g.cup[1] && (y + 1 < ny || yp3);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        lim.far_lo_ok[a] = far_open[2 * a];
+        lim.far_hi_ok[a] = far_open[2 * a + 1];
+        lim.far_lo[a] = far_open[2 * a] ? fv[s][2 * a] : 0.0f;
+        lim.far_hi[a] = far_open[2 * a + 1] ? fv[s][2 * a + 1] : 0.0f;
+      }
+      float rate = transport_rate_f32<true>(P, g, v, l0, u0, l1, u1, s, &lim);
+      if (reaction) rate += q[s] - (L[s] * v);
+      const float r = (b[s] - v) + (half_dt * rate);
       acc += static_cast<double>(r) * static_cast<double>(r);
-      const float diagonal = operator_diagonal_f32(P, v.g, s, reaction, L);
-      const float remainder = rate - (diagonal * v.c);
-      const float update = (b + (half_dt * remainder)) / (1.0f - (half_dt * diagonal));
-      nxt[s * n + i] = v.c + (relaxation * (update - v.c));
+      const float diagonal = operator_diagonal_f32(P, g, s, reaction, L[s]);
+      const float remainder = rate - (diagonal * v);
+      const float update = (b[s] + (half_dt * remainder)) / (1.0f - (half_dt * diagonal));
+      nxt[s * n + i] = v + (relaxation * (update - v));
     }
   }
   const double total = block_sum<kImplicitBlock>(acc, s_red);
