@@ -22,6 +22,12 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    suite's two port cases (flow_conformance_test.cpp:167-271,
                    built by tests/helpers.py): faces, port states (pressure,
                    flux, mean speed), solves and iterations.
+  coupled.npz      advance_coupled_cpu on a one-layer version of the
+                   conformance suite's coupled colony (513 cells, 3 species,
+                   2 signals; coupled_rates_conformance_test.cpp:18-118, built
+                   by tests/helpers.py), one step of dt 0.05 under each
+                   integration kind and scheme: its inputs, the grid levels,
+                   species and the implicit report.
 
 Usage: python tests/golden/make_golden.py [fixture ...] (default: all).
 """
@@ -127,7 +133,27 @@ def ports():
     np.savez_compressed(os.path.join(OUT, "port_flow.npz"), **out)
 
 
-FIXTURES = {"c1": c1, "illcond": illcond, "small": small, "ports": ports}
+def coupled():
+    from tests.helpers import coupled_colony_case
+    ref = Reference()
+    g, levels, cells, species, plan, previous, transport = coupled_colony_case()
+    out = {"levels": levels, "species": species, "previous": previous,
+           "obstacles": g["obstacles"], **{f"cell_{k}": v for k, v in cells.items()},
+           **{f"transport_{k}": v for k, v in transport.items()}}
+    for integration in (0, 1, 2):
+        for scheme in (0, 1):
+            lv, sp, rep = ref.advance_coupled(Grid(**g), levels, cells, species, plan, previous,
+                                              0.05, origin=(-3.0, -2.0, 0.0), scheme=scheme,
+                                              integration=integration, **transport)
+            key = f"i{integration}s{scheme}"
+            out[f"{key}_levels"] = lv
+            out[f"{key}_species"] = sp
+            out[f"{key}_report"] = np.array([rep["converged"], rep["iterations"],
+                                             rep["residual_rms"]], np.float64)
+    np.savez_compressed(os.path.join(OUT, "coupled.npz"), **out)
+
+
+FIXTURES = {"c1": c1, "illcond": illcond, "small": small, "ports": ports, "coupled": coupled}
 
 if __name__ == "__main__":
     build(reference=True)

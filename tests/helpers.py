@@ -353,3 +353,72 @@ def random_cells(seed, nx, ny, solid, count, dx=1.0, dy=1.0, origin=(0.0, 0.0, 0
                 growth_rates=rng.uniform(0.5, 1.5, count).astype(np.float32),
                 cell_types=rng.integers(0, 4, count).astype(np.int32)), \
         (lengths * rng.uniform(0.85, 1.0, count)).astype(np.float32)
+
+
+def coupled_colony_case(masked=True):
+    """A one-layer version of the reference conformance suite's coupled colony
+    (tests/conformance/coupled_rates_conformance_test.cpp:18-118): a 24x18
+    grid of spacing (0.75, 1.25) and origin (-3, -2), periodic along x, two
+    signals with affine rates, uniform advection as face velocities, 513
+    cells of three species spread over the grid by its formula, and its
+    13-instruction plan. Returns (grid kwargs, levels, cells, species, plan,
+    previous lengths, transport kwargs)."""
+    nx, ny = 24, 18
+    f32 = np.float32
+    solid = np.zeros(nx * ny, np.uint8)
+    if masked:
+        for x, y in ((11, 8), (5, 3), (17, 12), (6, 14)):
+            solid[x * ny + y] = 1
+    kinds = (PERIODIC, PERIODIC, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX)
+    grid = dict(nx=nx, ny=ny, nz=1, dx=0.75, dy=1.25, dz=2.0, obstacles=solid, boundary=kinds)
+    n = nx * ny
+    index = np.arange(2 * n)
+    source = (f32(0.004) * (index % 7).astype(f32)).astype(f32)
+    loss = (f32(0.003) * (index % 11).astype(f32)).astype(f32)
+    levels = (1.0 + f32(0.002) * (index % 113).astype(f32)).astype(f32)
+    dead = np.tile(solid, 2) != 0
+    source[dead] = 0.0
+    loss[dead] = 0.0
+    levels[dead] = 0.0
+    xf = np.full((nx + 1) * ny, 0.05, f32).reshape(nx + 1, ny)
+    yf = np.full(nx * (ny + 1), -0.02, f32).reshape(nx, ny + 1)
+    yf[:, 0] = 0.0
+    yf[:, -1] = 0.0
+    sm = solid.reshape(nx, ny) != 0
+    xf[:-1][sm] = 0.0
+    xf[1:][sm] = 0.0
+    yf[:, :-1][sm] = 0.0
+    yf[:, 1:][sm] = 0.0
+    both = (xf[0] != 0) & (xf[-1] != 0)
+    xf[0] = np.where(both, xf[0], 0.0)
+    xf[-1] = xf[0]
+    faces = np.concatenate([xf.reshape(-1), yf.reshape(-1), np.zeros(2 * n, f32)])
+    k = np.arange(513)
+    fx = (k % 17).astype(f32) / f32(16)
+    fy = ((k * 7) % 19).astype(f32) / f32(18)
+    px = (f32(-3.0) + fx * f32(0.75) * f32(nx - 1)).astype(f32)
+    py = (f32(-2.0) + fy * f32(1.25) * f32(ny - 1)).astype(f32)
+    # Cells whose stencil would be wholly solid move to the next lattice line.
+    for c in range(513):
+        i0 = int(np.floor((float(px[c]) + 3.0) / 0.75))
+        j0 = int(np.floor((float(py[c]) + 2.0) / 1.25))
+        near = [(i, j) for i in (i0, min(i0 + 1, nx - 1)) for j in (j0, min(j0 + 1, ny - 1))]
+        if all(solid[i * ny + j] for i, j in near):
+            py[c] = f32(-2.0) + f32(1.25) * f32(min(j0 + 2, ny - 1))
+    cells = dict(position_x=px, position_y=py, position_z=np.zeros(513, f32),
+                 lengths=(f32(1.5) + f32(0.01) * (k % 31).astype(f32)).astype(f32),
+                 radii=(f32(0.3) + f32(0.005) * (k % 7).astype(f32)).astype(f32),
+                 growth_rates=(f32(0.01) * (k % 5).astype(f32)).astype(f32),
+                 cell_types=(k % 4).astype(np.int32))
+    species = np.stack([0.5 + f32(0.01) * (k % 29), 0.75 + f32(0.02) * (k % 17),
+                        1.0 + f32(0.015) * (k % 13)], 1).astype(f32).reshape(-1)
+    ins = [(OP_SPECIES, 0, 0, 0, 0.0), (OP_SPECIES, 1, 0, 0, 0.0), (OP_SPECIES, 2, 0, 0, 0.0),
+           (OP_SIGNAL, 0, 0, 0, 0.0), (OP_SIGNAL, 1, 0, 0, 0.0), (OP_CONSTANT, 0, 0, 0, 0.02),
+           (OP_CONSTANT, 0, 0, 0, -0.01), (OP_MULTIPLY, 3, 5, 0, 0.0), (OP_ADD, 0, 7, 0, 0.0),
+           (OP_MULTIPLY, 1, 6, 0, 0.0), (OP_ADD, 2, 9, 0, 0.0), (OP_MULTIPLY, 0, 5, 0, 0.0),
+           (OP_MULTIPLY, 4, 6, 0, 0.0)]
+    plan = (3, ins, [8, 9, 10], [11, 12])
+    previous = (cells["lengths"] * f32(0.97)).astype(f32)
+    transport = dict(diffusion=np.array([0.08, 0.03], f32), boundary_values=np.zeros(12, f32),
+                     faces=faces, source=source, loss=loss)
+    return grid, levels, cells, species, plan, previous, transport

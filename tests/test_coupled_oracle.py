@@ -362,3 +362,27 @@ def test_cell_inside_an_obstacle_is_refused(oracle, reference):
     a, b = both(oracle, reference, grid, levels, cell, [1.0], plan, cell["lengths"], 0.1,
                 diffusion=(0.1,))
     assert a == b == (1, "signal sample position is inside a grid obstacle")
+
+
+def test_oracle_matches_the_committed_fixture(oracle):
+    """tests/golden/coupled.npz holds the reference's advance_coupled_cpu on a
+    one-layer version of its conformance colony: the oracle reproduces it bit
+    for bit under every integration and scheme."""
+    import os
+    from tests.helpers import coupled_colony_case
+    path = os.path.join(os.path.dirname(__file__), "golden", "coupled.npz")
+    g, levels, cells, species, plan, previous, transport = coupled_colony_case()
+    with np.load(path, allow_pickle=False) as f:
+        fx = {k: f[k] for k in f.files}
+    assert np.array_equal(fx["levels"], levels) and np.array_equal(fx["species"], species)
+    assert all(np.array_equal(fx[f"cell_{k}"], v) for k, v in cells.items())
+    for integration in INTEGRATIONS:
+        for scheme in (UPWIND, VAN_LEER):
+            key = f"i{integration}s{scheme}"
+            lv, sp, rep = oracle.advance_coupled(Grid(**g), levels, cells, species, plan, previous,
+                                                 0.05, origin=(-3.0, -2.0, 0.0), scheme=scheme,
+                                                 integration=integration, **transport)
+            assert np.array_equal(lv, fx[f"{key}_levels"]), key
+            assert np.array_equal(sp, fx[f"{key}_species"]), key
+            assert [rep["converged"], rep["iterations"]] == list(fx[f"{key}_report"][:2]), key
+            assert np.float32(rep["residual_rms"]) == np.float32(fx[f"{key}_report"][2]), key

@@ -331,3 +331,35 @@ def test_a_new_grid_drops_the_coupled_state(hip):
     with pytest.raises(msa.InvalidArgument) as info:
         hip.coupled_run(0.02, 1)
     assert str(info.value) == "no coupled state on the device"
+
+
+def test_device_matches_the_committed_fixture(hip):
+    """The reference's own outputs (tests/golden/coupled.npz) for a one-layer
+    version of its conformance colony, every integration and scheme: levels
+    and species bit for bit, the implicit report as the oracle's."""
+    import os
+    from tests.helpers import coupled_colony_case
+    path = os.path.join(os.path.dirname(__file__), "golden", "coupled.npz")
+    g, levels, cells, species, plan, previous, transport = coupled_colony_case()
+    with np.load(path, allow_pickle=False) as f:
+        fx = {k: f[k] for k in f.files}
+    grid = msa.Grid(g["nx"], g["ny"], 1, g["dx"], g["dy"], g["dz"], g["obstacles"], g["boundary"])
+    nfx, nfy = (g["nx"] + 1) * g["ny"], g["nx"] * (g["ny"] + 1)
+    faces = transport["faces"]
+    velocity = msa.VelocityField(faces[:nfx], faces[nfx:nfx + nfy], faces[nfx + nfy:])
+    for integration in INTEGRATIONS:
+        for scheme in (msa.UPWIND, msa.VAN_LEER):
+            key = f"i{integration}s{scheme}"
+            print("case", key, flush=True)
+            spec = msa.SignalGridSpec(grid, transport["diffusion"], transport["boundary_values"],
+                                      velocity, source_rates=transport["source"],
+                                      loss_rates=transport["loss"], advection_scheme=scheme,
+                                      integration=integration)
+            lv, sp, rep = hip.advance_coupled(spec, levels, cells, species, plan, previous, 0.05,
+                                              (-3.0, -2.0, 0.0))
+            assert np.array_equal(lv, fx[f"{key}_levels"]), key
+            assert np.array_equal(sp, fx[f"{key}_species"]), key
+            assert rep.converged == bool(fx[f"{key}_report"][0])
+            assert rep.iterations == int(fx[f"{key}_report"][1])
+            ref_rms = float(fx[f"{key}_report"][2])
+            assert abs(rep.residual_rms - ref_rms) <= 2.0 ** -23 * abs(ref_rms)
