@@ -153,7 +153,37 @@ def coupled():
     np.savez_compressed(os.path.join(OUT, "coupled.npz"), **out)
 
 
-FIXTURES = {"c1": c1, "illcond": illcond, "small": small, "ports": ports, "coupled": coupled}
+def implicit():
+    """Crank-Nicolson and backward Euler, upwind and van Leer, on the channel
+    + trap 256x128 over the reference's own flow field: three steps of ten
+    times the forward-Euler dt from the forward-Euler fixture's levels, Monod
+    uptake given as the lagged loss vmax*phi/(K+c) of each step's start."""
+    from tests.helpers import c1_inputs
+    c = c1_inputs()
+    d = np.load(os.path.join(OUT, "c1_adr.npz"))
+    grid = Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    ref = Reference()
+    vmax, K = np.float32(d["vmax"]), np.float32(d["K"])
+    dt = np.float32(10 * d["dt"])
+    out = {"dt": dt, "steps": 3}
+    for integration in (1, 2):
+        for scheme in (0, 1):
+            levels = d["levels"].astype(np.float32)
+            for _ in range(3):
+                loss = ((vmax * c["phi"]) / (K + levels)).astype(np.float32)
+                levels, _, rep = ref.advance_signal_grid(
+                    grid, levels, dt, 1, diffusion=d["diffusion"],
+                    boundary_values=d["boundary_values"], faces=c["data"]["faces"], loss=loss,
+                    scheme=scheme, integration=integration, with_report=True)
+            key = f"i{integration}s{scheme}"
+            out[f"{key}_levels"] = levels
+            out[f"{key}_report"] = np.array([rep["converged"], rep["iterations"],
+                                             rep["residual_rms"]], np.float64)
+    np.savez_compressed(os.path.join(OUT, "c1_implicit.npz"), **out)
+
+
+FIXTURES = {"c1": c1, "illcond": illcond, "small": small, "ports": ports, "coupled": coupled,
+            "implicit": implicit}
 
 if __name__ == "__main__":
     build(reference=True)

@@ -168,3 +168,30 @@ def test_device_resident_implicit_steps_match_the_oracle(hip, oracle, integratio
     print("iterations of the last step", rep.iterations, ref["iterations"], flush=True)
     assert np.array_equal(ours, ref_levels)
     same_report(rep, ref)
+
+
+@pytest.mark.parametrize("integration", IMPLICIT)
+@pytest.mark.parametrize("scheme", [msa.UPWIND, msa.VAN_LEER])
+def test_device_matches_the_committed_fixture(hip, scheme, integration):
+    """Three implicit steps with Monod uptake on the channel + trap 256x128
+    over the reference's flow field, against the reference's vectors in
+    tests/golden/c1_implicit.npz: levels and iteration count exactly, the
+    residual RMS to one float rounding."""
+    from tests.helpers import golden
+    c = c1_inputs()
+    adr, d = golden("c1_adr.npz"), golden("c1_implicit.npz")
+    xf, yf, zf = split_faces(c["data"]["faces"], c["nx"], c["ny"])
+    grid = msa.Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    spec = msa.SignalGridSpec(grid, adr["diffusion"], adr["boundary_values"],
+                              msa.VelocityField(xf, yf, zf), phi=c["phi"],
+                              monod_vmax=[float(adr["vmax"])], monod_k=[float(adr["K"])],
+                              advection_scheme=scheme, integration=integration)
+    print("case", scheme, integration, flush=True)
+    levels = adr["levels"].astype(np.float32)
+    for _ in range(int(d["steps"])):
+        levels, rep = hip.advance_signal_grid(spec, levels, float(d["dt"]))
+    key = f"i{integration}s{scheme}"
+    assert np.array_equal(levels, d[f"{key}_levels"])
+    converged, iterations, rms = d[f"{key}_report"]
+    same_report(rep, dict(converged=bool(converged), iterations=int(iterations),
+                          residual_rms=float(rms)))

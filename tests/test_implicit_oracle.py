@@ -183,3 +183,24 @@ def test_three_dimensional_backward_euler_matches_reference_live(oracle, referen
                                               boundary_values=bv, integration=BACKWARD_EULER,
                                               with_report=True)
     assert np.array_equal(a, b) and rep == ref and rep["iterations"] < 200
+
+
+@pytest.mark.parametrize("integration", IMPLICIT)
+@pytest.mark.parametrize("scheme", [UPWIND, VAN_LEER])
+def test_implicit_steps_match_the_committed_fixture(oracle, scheme, integration):
+    """The channel + trap 256x128 over the reference's flow field, three
+    implicit steps with Monod uptake from the forward-Euler fixture's levels:
+    levels and the last report equal the reference's vectors in
+    tests/golden/c1_implicit.npz (made by tests/golden/make_golden.py)."""
+    from tests.helpers import c1_inputs, golden
+    c = c1_inputs()
+    adr, d = golden("c1_adr.npz"), golden("c1_implicit.npz")
+    grid = Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    levels, rep = oracle.adr_step_implicit(
+        grid, adr["levels"], float(d["dt"]), steps=int(d["steps"]), diffusion=adr["diffusion"],
+        boundary_values=adr["boundary_values"], faces=c["data"]["faces"], phi=c["phi"],
+        vmax=adr["vmax"], monod_k=adr["K"], scheme=scheme, integration=integration)
+    key = f"i{integration}s{scheme}"
+    assert np.array_equal(levels, d[f"{key}_levels"])
+    assert [rep["converged"], rep["iterations"], rep["residual_rms"]] == \
+        d[f"{key}_report"].tolist()
