@@ -28,6 +28,11 @@ Fixtures (inputs and the reference's outputs, nothing else):
                    by tests/helpers.py), one step of dt 0.05 under each
                    integration kind and scheme: its inputs, the grid levels,
                    species and the implicit report.
+  c1_implicit.npz  three Crank-Nicolson and backward-Euler steps (upwind and van
+                   Leer) of ten times c1_adr's dt over the c1 field from
+                   c1_adr's levels, Monod as the lagged loss; levels and the
+                   last report. Also three forward-Euler van Leer steps at
+                   c1_adr's dt (i0s1_levels).
 
 Usage: python tests/golden/make_golden.py [fixture ...] (default: all).
 """
@@ -157,7 +162,8 @@ def implicit():
     """Crank-Nicolson and backward Euler, upwind and van Leer, on the channel
     + trap 256x128 over the reference's own flow field: three steps of ten
     times the forward-Euler dt from the forward-Euler fixture's levels, Monod
-    uptake given as the lagged loss vmax*phi/(K+c) of each step's start."""
+    uptake given as the lagged loss vmax*phi/(K+c) of each step's start;
+    and three forward-Euler van Leer steps at the forward-Euler dt."""
     from tests.helpers import c1_inputs
     c = c1_inputs()
     d = np.load(os.path.join(OUT, "c1_adr.npz"))
@@ -179,6 +185,14 @@ def implicit():
             out[f"{key}_levels"] = levels
             out[f"{key}_report"] = np.array([rep["converged"], rep["iterations"],
                                              rep["residual_rms"]], np.float64)
+    # Forward Euler under van Leer, at the forward-Euler fixture's dt.
+    levels = d["levels"].astype(np.float32)
+    for _ in range(3):
+        loss = ((vmax * c["phi"]) / (K + levels)).astype(np.float32)
+        levels, _ = ref.advance_signal_grid(
+            grid, levels, np.float32(d["dt"]), 1, diffusion=d["diffusion"],
+            boundary_values=d["boundary_values"], faces=c["data"]["faces"], loss=loss, scheme=1)
+    out["i0s1_levels"] = levels
     np.savez_compressed(os.path.join(OUT, "c1_implicit.npz"), **out)
 
 

@@ -195,3 +195,22 @@ def test_device_matches_the_committed_fixture(hip, scheme, integration):
     converged, iterations, rms = d[f"{key}_report"]
     same_report(rep, dict(converged=bool(converged), iterations=int(iterations),
                           residual_rms=float(rms)))
+
+
+def test_forward_euler_van_leer_matches_the_committed_fixture(hip):
+    """Three forward-Euler van Leer steps with Monod on the same case, at the
+    forward-Euler fixture's dt: the reference's levels bit for bit."""
+    from tests.helpers import golden
+    c = c1_inputs()
+    adr, d = golden("c1_adr.npz"), golden("c1_implicit.npz")
+    xf, yf, zf = split_faces(c["data"]["faces"], c["nx"], c["ny"])
+    grid = msa.Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    spec = msa.SignalGridSpec(grid, adr["diffusion"], adr["boundary_values"],
+                              msa.VelocityField(xf, yf, zf), phi=c["phi"],
+                              monod_vmax=[float(adr["vmax"])], monod_k=[float(adr["K"])],
+                              advection_scheme=msa.VAN_LEER)
+    levels = adr["levels"].astype(np.float32)
+    for _ in range(3):
+        levels, rep = hip.advance_signal_grid(spec, levels, float(adr["dt"]))
+    assert rep.converged
+    assert np.array_equal(levels, d["i0s1_levels"])

@@ -204,3 +204,17 @@ def test_implicit_steps_match_the_committed_fixture(oracle, scheme, integration)
     assert np.array_equal(levels, d[f"{key}_levels"])
     assert [rep["converged"], rep["iterations"], rep["residual_rms"]] == \
         d[f"{key}_report"].tolist()
+
+
+def test_forward_euler_van_leer_matches_the_committed_fixture(oracle):
+    """Three forward-Euler van Leer steps with Monod on the same case, at the
+    forward-Euler fixture's dt: the reference's levels bit for bit."""
+    from tests.helpers import c1_inputs, golden
+    c = c1_inputs()
+    adr, d = golden("c1_adr.npz"), golden("c1_implicit.npz")
+    grid = Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    levels = oracle.adr_step_f32(
+        grid, adr["levels"], float(adr["dt"]), steps=3, diffusion=adr["diffusion"],
+        boundary_values=adr["boundary_values"], faces=c["data"]["faces"], phi=c["phi"],
+        vmax=adr["vmax"], monod_k=adr["K"], scheme=VAN_LEER)
+    assert np.array_equal(levels, d["i0s1_levels"])
