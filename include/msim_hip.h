@@ -77,7 +77,8 @@ typedef struct {
 /* DepthAveragedFlowParameters (cpp/include/microsimulator/flow.hpp:18-28)
  * plus the controls of the device solver forms. */
 typedef struct {
-  float mean_inlet_speed;      /* target mean speed over open inlet faces */
+  float mean_inlet_speed;      /* target mean speed over open inlet faces (may be 0 with a
+                                  body force) */
   uint32_t axis;               /* FlowAxis: 0 = x, 1 = y */
   double relative_tolerance;   /* on ||b - Ax|| / ||b|| */
   uint32_t max_iterations;     /* Krylov iterations, summed over inner solves */
@@ -88,6 +89,16 @@ typedef struct {
   double inner_tolerance;      /* MIXED/LEARNED: inner FP32 relative tolerance; 0 = auto */
   uint32_t fallback_iterations;/* LEARNED: iteration cap before the Jacobi-PCG fallback */
   uint32_t profile;            /* 1: time every Krylov kernel launch with events */
+  /* DepthAveragedFlowParameters::body_force (flow.hpp:24-25): the in-plane body
+   * force per column in pressure-gradient units, (fx, fy) pairs, x-major,
+   * y-minor; NULL for none. A forced flow solves the system a second time with
+   * every boundary pressure zero and the body flux's divergence as its
+   * right-hand side (solve_shallow_axis_flow, flow_system.hpp:1420-1447); the
+   * driven field is added after scaling, which then meets the mean inlet speed
+   * with both (scale_velocity, 1368-1415). Iterations add over both solves and
+   * the residuals are the larger. Needs scale = 1. */
+  const float* body_force;
+  size_t body_force_columns;   /* pairs in body_force: the grid's nx*ny */
 } msim_flow_params;
 
 /* DepthAveragedFlowReport (flow.hpp:30-35) plus what the device forms add. */
@@ -327,7 +338,11 @@ int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, fl
  *                  implicit kinds need precision 32). ledger (steps*S*4 doubles,
  *                  or NULL) receives per step and signal: mass before, mass
  *                  after, boundary supply and uptake, each in FP64 (mass =
- *                  sum of levels x voxel volume).
+ *                  sum of levels x voxel volume). A run refused at step k
+ *                  keeps the levels after steps 0..k-1 and fills their
+ *                  ledger rows; the next run's ledger carries on from them.
+ *                  A field read in place is refused once any later flow solve
+ *                  or preparation has run on the context.
  *  adr_download -- levels in the uploaded precision. */
 int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void* levels,
                         uint32_t precision, uint32_t use_flow_field);

@@ -94,7 +94,8 @@ class _FlowParams(C.Structure):
                 ("relative_tolerance", C.c_double), ("max_iterations", C.c_uint32),
                 ("solver", C.c_uint32), ("scale", C.c_uint32), ("check_interval", C.c_uint32),
                 ("inner_tolerance", C.c_double), ("fallback_iterations", C.c_uint32),
-                ("profile", C.c_uint32)]
+                ("profile", C.c_uint32), ("body_force", C.c_void_p),
+                ("body_force_columns", C.c_size_t)]
 
 
 class _FlowReport(C.Structure):
@@ -367,9 +368,19 @@ class DepthAveragedFlowParameters:
     inner_tolerance: float = 0.0
     fallback_iterations: int = 200
     profile: bool = False
+    # In-plane body force per column, (fx, fy) in pressure-gradient units
+    # (x-major, y-minor), or None.
+    body_force: object = None
 
-    def _c(self) -> _FlowParams:
+    def _c(self, keep: list | None = None) -> _FlowParams:
         p = _FlowParams()
+        if self.body_force is not None:
+            force = np.ascontiguousarray(self.body_force, np.float32).reshape(-1, 2)
+            if keep is None:
+                raise ValueError("a body force needs a keep list for its array")
+            keep.append(force)
+            p.body_force = force.ctypes.data
+            p.body_force_columns = force.shape[0]
         p.mean_inlet_speed = self.mean_inlet_speed
         p.axis = self.axis
         p.relative_tolerance = self.relative_tolerance
@@ -523,21 +534,43 @@ class SignalGridSpec:
         s.grid = self.grid._c(keep)
         S = self.signal_count
         s.signal_count = S
+        n = self.grid.sites
         arrays = {"diffusion": np.atleast_1d(np.asarray(self.diffusion, np.float32))}
+        # The C ABI carries no lengths: every array is checked here, with the
+        # reference's messages where it has one (signals.cpp:452, 604, 612;
+        # SignalGridAffineReaction::validate, 493-497).
         if self.boundary_values is not None:
             arrays["boundary_values"] = np.asarray(self.boundary_values, np.float32).reshape(-1)
+            if arrays["boundary_values"].size != 6 * S:
+                raise InvalidArgument("fixed signal grid boundary values must match signal count")
         if self.velocity is not None:
-            arrays["x_faces"] = np.asarray(self.velocity.x_faces, np.float32).reshape(-1)
-            arrays["y_faces"] = np.asarray(self.velocity.y_faces, np.float32).reshape(-1)
-            arrays["z_faces"] = np.asarray(self.velocity.z_faces, np.float32).reshape(-1)
+            nfx, nfy, nfz = self.grid.face_counts()
+            for name, arr, count in (("x_faces", self.velocity.x_faces, nfx),
+                                     ("y_faces", self.velocity.y_faces, nfy),
+                                     ("z_faces", self.velocity.z_faces, nfz)):
+                arrays[name] = np.asarray(arr, np.float32).reshape(-1)
+                if arrays[name].size != count:
+                    raise InvalidArgument(
+                        "signal grid velocity field must cover every lattice face")
+        if (self.loss_rates is None) != (self.source_rates is None):
+            raise InvalidArgument(
+                "signal grid affine reaction arrays must match the grid level count")
         if self.loss_rates is not None:
             arrays["loss_rates"] = np.asarray(self.loss_rates, np.float32).reshape(-1)
-            if self.source_rates is not None:
-                arrays["source_rates"] = np.asarray(self.source_rates, np.float32).reshape(-1)
+            arrays["source_rates"] = np.asarray(self.source_rates, np.float32).reshape(-1)
+            if arrays["loss_rates"].size != S * n or arrays["source_rates"].size != S * n:
+                raise InvalidArgument(
+                    "signal grid affine reaction arrays must match the grid level count")
         if self.phi is not None:
             arrays["phi"] = np.asarray(self.phi, np.float32).reshape(-1)
-            arrays["monod_vmax"] = np.broadcast_to(np.asarray(self.monod_vmax, np.float32), (S,))
-            arrays["monod_k"] = np.broadcast_to(np.asarray(self.monod_k, np.float32), (S,))
+            if arrays["phi"].size != n:
+                raise InvalidArgument("Monod occupancy must hold one value per grid site")
+            try:
+                arrays["monod_vmax"] = np.broadcast_to(
+                    np.asarray(self.monod_vmax, np.float32), (S,))
+                arrays["monod_k"] = np.broadcast_to(np.asarray(self.monod_k, np.float32), (S,))
+            except ValueError:
+                raise InvalidArgument("Monod vmax and K must hold one value per signal") from None
         for name, arr in arrays.items():
             arr = np.ascontiguousarray(arr)
             keep.append(arr)
@@ -552,6 +585,9 @@ class SignalGridSpec:
             table = (_BoundaryPatch * len(self.patches))()
             for k, (face, begin, end, values) in enumerate(self.patches):
                 v = np.ascontiguousarray(np.atleast_1d(values), np.float32)
+                if v.size != S:
+                    raise InvalidArgument(
+                        "fixed signal grid boundary values must match signal count")
                 keep.append(v)
                 table[k].face = face
                 table[k].begin[0], table[k].begin[1] = begin
@@ -731,7 +767,7 @@ class HipBackend:
         xf = np.zeros(nfx, np.float32)
         yf = np.zeros(nfy, np.float32)
         zf = np.zeros(nfz, np.float32)
-        p = parameters._c()
+        p = parameters._c(keep)
         rep = _FlowReport()
         self._check(self.lib.msim_hip_solve_depth_averaged_flow(
             self.ctx, C.byref(g), None if mob is None else mob.ctypes.data, C.byref(p),
@@ -909,7 +945,8 @@ class HipBackend:
             self.ctx, C.byref(g), None if mob is None else mob.ctypes.data, axis))
 
     def flow_solve(self, parameters: DepthAveragedFlowParameters) -> DepthAveragedFlowReport:
-        p = parameters._c()
+        keep: list = []
+        p = parameters._c(keep)
         rep = _FlowReport()
         self._check(self.lib.msim_hip_flow_solve(self.ctx, C.byref(p), C.byref(rep)))
         return DepthAveragedFlowReport._from(rep)

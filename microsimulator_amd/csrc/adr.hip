@@ -124,10 +124,14 @@ __global__ __launch_bounds__(TY) void k_adr_step(
   __shared__ T s_c[2][S][TY];
   __shared__ uint8_t s_s[2][TY];
   __shared__ double s_red[(kF64 ? 3 * S : 1) * (TY / kWave)];
-  if (flags[0] != 0) return;  // an earlier step failed: the run stops there
+  // An earlier step failed: the run stops there. (flags[1] holds the first
+  // failed step; a failure of this step may be seen here half-published, and
+  // then the previous step's ledger row is still folded, as it is valid.)
+  if (flags[0] != 0 && flags[1] < step) return;
   if constexpr (kF64) {
     if (step > 0 && blockIdx.x < 3 * S) fold_ledger<S, TY>(P, L, step - 1, dt, blockIdx.x, s_red);
   }
+  if (flags[0] != 0) return;  // this step already failed elsewhere
   const uint32_t nx = P.nx, ny = P.ny;
   const size_t n = static_cast<size_t>(nx) * ny;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -548,6 +552,36 @@ uint32_t site_blocks(size_t n) { return static_cast<uint32_t>((n + kAdrBlock - 1
 
 }  // namespace
 
+// The kernels' parameters of a signal grid (one layer): shape, spacing,
+// boundary kinds and values, diffusion, Monod constants, the scheme and
+// which optional arrays it has. The velocity field and patches are set by
+// the caller.
+AdrParams transport_params(const msim_signal_grid& grid) {
+  AdrParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.nx = grid.grid.nx;
+  P.ny = grid.grid.ny;
+  P.signals = grid.signal_count;
+  P.dx = grid.grid.dx;
+  P.dy = grid.grid.dy;
+  P.dz = grid.grid.dz;
+  for (int f = 0; f < 6; ++f) P.kind[f] = grid.grid.boundary[f];
+  for (uint32_t s = 0; s < grid.signal_count; ++s) {
+    P.diffusion[s] = grid.diffusion[s];
+    for (int f = 0; f < 6; ++f)
+      P.boundary[s][f] = grid.boundary_values != nullptr ? grid.boundary_values[6 * s + f] : 0.0f;
+    if (grid.monod_vmax != nullptr) {
+      P.vmax[s] = grid.monod_vmax[s];
+      P.monod_k[s] = grid.monod_k[s];
+    }
+  }
+  P.has_field = grid.x_faces != nullptr ? 1 : 0;
+  P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
+  P.has_affine = grid.loss_rates != nullptr ? 1 : 0;
+  P.has_monod = grid.phi != nullptr ? 1 : 0;
+  return P;
+}
+
 void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
                 uint32_t precision, bool use_flow_field) {
   // Coupled state belongs to the grid it was uploaded with.
@@ -559,23 +593,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   const bool same_spec = a.uploaded && grid.revision != 0 && grid.revision == a.revision &&
                          P.nx == grid.grid.nx && P.ny == grid.grid.ny && P.signals == S &&
                          !use_flow_field && !a.flow_field;
-  std::memset(&P, 0, sizeof(P));
-  P.nx = grid.grid.nx;
-  P.ny = grid.grid.ny;
-  P.signals = S;
-  P.dx = grid.grid.dx;
-  P.dy = grid.grid.dy;
-  P.dz = grid.grid.dz;
-  for (int f = 0; f < 6; ++f) P.kind[f] = grid.grid.boundary[f];
-  for (uint32_t s = 0; s < S; ++s) {
-    P.diffusion[s] = grid.diffusion[s];
-    for (int f = 0; f < 6; ++f)
-      P.boundary[s][f] = grid.boundary_values != nullptr ? grid.boundary_values[6 * s + f] : 0.0f;
-    if (grid.monod_vmax != nullptr) {
-      P.vmax[s] = grid.monod_vmax[s];
-      P.monod_k[s] = grid.monod_k[s];
-    }
-  }
+  P = transport_params(grid);
   a.integration = grid.integration;
   a.max_iterations = grid.solver_max_iterations;
   a.absolute_tolerance = grid.solver_absolute_tolerance;
@@ -583,10 +601,7 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.last_report = msim_signal_report{1, 0, 0.0f};
   a.predicted_iterations = kPredictedIterations;
   P.has_field = (use_flow_field || grid.x_faces != nullptr) ? 1 : 0;
-  P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
   const bool patched = grid.patches != nullptr && grid.patch_count > 0;
-  P.has_affine = grid.loss_rates != nullptr ? 1 : 0;
-  P.has_monod = grid.phi != nullptr ? 1 : 0;
   a.precision = precision;
   const size_t bytes = S * n * (precision == 64 ? sizeof(double) : sizeof(float));
   a.levels_a.reserve(bytes);
@@ -668,18 +683,14 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
       check(hipMemcpyAsync(a.loss.data(), grid.loss_rates, S * n * sizeof(float),
                            hipMemcpyHostToDevice, ctx.stream),
             "upload loss");
-      if (grid.source_rates != nullptr) {
-        check(hipMemcpyAsync(a.source.data(), grid.source_rates, S * n * sizeof(float),
-                             hipMemcpyHostToDevice, ctx.stream),
-              "upload source");
-      } else {
-        check(hipMemsetAsync(a.source.data(), 0, S * n * sizeof(float), ctx.stream),
-              "clear source");
-      }
+      check(hipMemcpyAsync(a.source.data(), grid.source_rates, S * n * sizeof(float),
+                           hipMemcpyHostToDevice, ctx.stream),
+            "upload source");
     }
   }
   P.edge = a.has_edge ? a.edge.data() : nullptr;
   a.flow_field = use_flow_field;
+  a.flow_generation = use_flow_field ? ctx.flow.generation : 0;
   a.revision = grid.revision;
   check(hipMemcpyAsync(a.levels_a.data(), levels, bytes, hipMemcpyHostToDevice, ctx.stream),
         "upload levels");
@@ -709,51 +720,65 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   a.uploaded = true;
 }
 
-template <typename T, int S, int TY, bool kLimited, bool kPatched>
-void launch_step(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
-                 const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
-                 double* largest, bool settle) {
+// The device arrays one transport step reads and writes besides the levels:
+// the context's buffers, or the static scratch of the allocation-free step.
+struct AdrView {
+  const AdrParams* P;
+  const uint8_t* solid;
+  const float *phi, *source, *loss;
+  uint32_t* flags;
+};
+
+AdrView view_of(Context& ctx) {
   auto& a = ctx.adr;
-  if (a.params.has_affine != 0) {
-    k_adr_step<T, S, TY, true, kLimited, kPatched><<<sh.blocks, TY, 0, ctx.stream>>>(
-        a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
-        reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
-        sh.rows, L, largest);
+  return AdrView{&a.params, a.solid.data(), a.phi.data(), a.source.data(), a.loss.data(),
+                 a.flags.data()};
+}
+
+template <typename T, int S, int TY, bool kLimited, bool kPatched>
+void launch_step(hipStream_t stream, const AdrView& v, const AdrShape& sh, const float* xf,
+                 const float* yf, const uint8_t* src, uint8_t* dst, float dt, uint32_t k,
+                 LedgerParts L, double* largest, bool settle) {
+  const AdrParams& P = *v.P;
+  if (P.has_affine != 0) {
+    k_adr_step<T, S, TY, true, kLimited, kPatched><<<sh.blocks, TY, 0, stream>>>(
+        P, v.solid, xf, yf, v.phi, v.source, v.loss, reinterpret_cast<const T*>(src),
+        reinterpret_cast<T*>(dst), dt, v.flags, k, sh.rows, L, largest);
   } else {
-    k_adr_step<T, S, TY, false, kLimited, kPatched><<<sh.blocks, TY, 0, ctx.stream>>>(
-        a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(),
-        reinterpret_cast<const T*>(src), reinterpret_cast<T*>(dst), dt, a.flags.data(), k,
-        sh.rows, L, largest);
+    k_adr_step<T, S, TY, false, kLimited, kPatched><<<sh.blocks, TY, 0, stream>>>(
+        P, v.solid, xf, yf, v.phi, v.source, v.loss, reinterpret_cast<const T*>(src),
+        reinterpret_cast<T*>(dst), dt, v.flags, k, sh.rows, L, largest);
   }
   if (kLimited && settle) {
-    const size_t count = static_cast<size_t>(S) * a.sites();
+    const size_t count = static_cast<size_t>(S) * P.nx * P.ny;
     const uint32_t blocks = static_cast<uint32_t>(
         std::min<size_t>((count + kAdrBlock - 1) / kAdrBlock, 2048));
-    k_adr_settle<T><<<blocks, kAdrBlock, 0, ctx.stream>>>(reinterpret_cast<T*>(dst), count,
-                                                          largest, sh.blocks, a.flags.data(), k);
+    k_adr_settle<T><<<blocks, kAdrBlock, 0, stream>>>(reinterpret_cast<T*>(dst), count, largest,
+                                                      sh.blocks, v.flags, k);
   }
 }
 
 template <typename T, int S, bool kLimited>
-void launch_typed(Context& ctx, const AdrShape& sh, const float* xf, const float* yf,
-                  const uint8_t* src, uint8_t* dst, float dt, uint32_t k, LedgerParts L,
-                  double* largest, bool settle = true) {
+void launch_typed(hipStream_t stream, const AdrView& v, const AdrShape& sh, const float* xf,
+                  const float* yf, const uint8_t* src, uint8_t* dst, float dt, uint32_t k,
+                  LedgerParts L, double* largest, bool settle = true) {
   // Specs with reservoir patches take one strip width (128 columns).
-  if (ctx.adr.params.edge != nullptr) {
-    launch_step<T, S, 128, kLimited, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest, settle);
+  if (v.P->edge != nullptr) {
+    launch_step<T, S, 128, kLimited, true>(stream, v, sh, xf, yf, src, dst, dt, k, L, largest,
+                                           settle);
     return;
   }
   switch (sh.ty) {
     case 64:
-      launch_step<T, S, 64, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest,
+      launch_step<T, S, 64, kLimited, false>(stream, v, sh, xf, yf, src, dst, dt, k, L, largest,
                                              settle);
       break;
     case 128:
-      launch_step<T, S, 128, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest,
+      launch_step<T, S, 128, kLimited, false>(stream, v, sh, xf, yf, src, dst, dt, k, L, largest,
                                               settle);
       break;
     default:
-      launch_step<T, S, 256, kLimited, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest,
+      launch_step<T, S, 256, kLimited, false>(stream, v, sh, xf, yf, src, dst, dt, k, L, largest,
                                               settle);
   }
 }
@@ -770,10 +795,12 @@ void launch_candidate(Context& ctx, float dt, const float* in, float* out, uint3
   double* largest = a.partials.data() + static_cast<size_t>(3 * kMaxSignals * 2) * sh.blocks;
   const auto* src = reinterpret_cast<const uint8_t*>(in);
   auto* dst = reinterpret_cast<uint8_t*>(out);
+  const AdrView v = view_of(ctx);
   if (limited) {
-    launch_typed<float, S, true>(ctx, sh, xf, yf, src, dst, dt, step, L, largest, false);
+    launch_typed<float, S, true>(ctx.stream, v, sh, xf, yf, src, dst, dt, step, L, largest, false);
   } else {
-    launch_typed<float, S, false>(ctx, sh, xf, yf, src, dst, dt, step, L, largest, false);
+    launch_typed<float, S, false>(ctx.stream, v, sh, xf, yf, src, dst, dt, step, L, largest,
+                                  false);
   }
 }
 
@@ -786,20 +813,21 @@ void launch_steps(Context& ctx, float dt, uint32_t steps, const float* xf, const
   sh.blocks = step_blocks(sh);
   LedgerParts L{a.partials.data(), a.ledger.data(), sh.blocks};
   double* largest = a.partials.data() + static_cast<size_t>(3 * kMaxSignals * 2) * sh.blocks;
+  const AdrView v = view_of(ctx);
   for (uint32_t k = 0; k < steps; ++k) {
     const uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
     uint8_t* dst = a.current == 0 ? a.levels_b.data() : a.levels_a.data();
     if (a.precision == 64) {
       if (limited) {
-        launch_typed<double, S, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+        launch_typed<double, S, true>(ctx.stream, v, sh, xf, yf, src, dst, dt, k, L, largest);
       } else {
-        launch_typed<double, S, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+        launch_typed<double, S, false>(ctx.stream, v, sh, xf, yf, src, dst, dt, k, L, largest);
       }
     } else {
       if (limited) {
-        launch_typed<float, S, true>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+        launch_typed<float, S, true>(ctx.stream, v, sh, xf, yf, src, dst, dt, k, L, largest);
       } else {
-        launch_typed<float, S, false>(ctx, sh, xf, yf, src, dst, dt, k, L, largest);
+        launch_typed<float, S, false>(ctx.stream, v, sh, xf, yf, src, dst, dt, k, L, largest);
       }
     }
     check(hipGetLastError(), "launch transport step");
@@ -888,6 +916,22 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
     // the starting one : the other).
     const uint32_t failed = flags[1];
     a.current = static_cast<int>((a.current ^ (steps & 1u)) ^ (failed & 1u));
+    // The steps before it stand: their ledger rows carry the mass on, and go
+    // to the caller.
+    if (!rows.empty()) {
+      const uint32_t kept = std::min(failed, steps);
+      for (uint32_t k = 0; k < kept; ++k) {
+        for (uint32_t s = 0; s < P.signals; ++s) {
+          double* row = rows.data() + (static_cast<size_t>(k) * P.signals + s) * 4;
+          row[0] = a.host_mass[s];
+          a.host_mass[s] = row[1];
+        }
+      }
+      if (ledger != nullptr && kept > 0) {
+        std::memcpy(ledger, rows.data(),
+                    static_cast<size_t>(kept) * P.signals * 4 * sizeof(double));
+      }
+    }
     if ((flags[0] & kFlagNegative) != 0) {
       throw Error(MSIM_DOMAIN_ERROR,
                   "signal grid update produced a non-finite or negative concentration (step " +

@@ -461,7 +461,8 @@ PortPlan plan_ports(const msim_grid& g, const float* mobility, const msim_flow_p
 // scale_velocity (flow_system.hpp:1368-1415) over the lifted voxel faces: the
 // open inlet faces are the axis's lower faces of fluid voxels in columns with
 // conductance (open_inlet_faces, 449-459).
-void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_report& report) {
+void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_report& report,
+                  const std::vector<float>* driven) {
   auto& v = f.layers.faces;
   const auto& L = f.layers;
   const uint32_t nx = f.nx, ny = f.ny, nz = L.nz;
@@ -471,7 +472,7 @@ void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_repor
     peak = std::max(peak, std::abs(value));
   }
   const size_t nxf = static_cast<size_t>(nx + 1) * ny * nz;
-  double inlet_sum = 0.0;
+  double inlet_sum = 0.0, driven_sum = 0.0;
   size_t inlet_count = 0;
   const uint32_t across = f.axis == 0 ? ny : nx;
   for (uint32_t t = 0; t < across; ++t) {
@@ -481,6 +482,7 @@ void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_repor
       const size_t face = f.axis == 0 ? static_cast<size_t>(t) * nz + z
                                       : nxf + static_cast<size_t>(t) * (ny + 1) * nz + z;
       inlet_sum += v[face];
+      if (driven != nullptr) driven_sum += (*driven)[face];
       ++inlet_count;
     }
   }
@@ -489,12 +491,14 @@ void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_repor
   if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
     throw Error(MSIM_RUNTIME_ERROR, "the device carries no through-flow: the outlet is unreachable");
   }
-  const float driven_mean = static_cast<float>(0.0 / static_cast<double>(inlet_count));
+  const float driven_mean = static_cast<float>(driven_sum / static_cast<double>(inlet_count));
   const float factor =
       params.scale != 0 ? (params.mean_inlet_speed - driven_mean) / solved_mean : 1.0f;
   float scaled_peak = 0.0f;
-  for (float& value : v) {
-    if (params.scale != 0) value = value * factor + 0.0f;
+  for (size_t i = 0; i < v.size(); ++i) {
+    float& value = v[i];
+    if (params.scale != 0) value = value * factor + (driven != nullptr ? (*driven)[i] : 0.0f);
+    if (!std::isfinite(value)) throw Error(MSIM_RUNTIME_ERROR, "non-finite combined flow velocity");
     scaled_peak = std::max(scaled_peak, std::abs(value));
   }
   report.mean_inlet_speed = params.scale != 0 ? params.mean_inlet_speed : solved_mean;
@@ -504,8 +508,12 @@ void scale_layers(FlowDevice& f, const msim_flow_params& params, msim_flow_repor
 }
 
 void validate_flow_params(const msim_flow_params& p) {
-  if (p.scale != 0 && (!std::isfinite(p.mean_inlet_speed) || p.mean_inlet_speed == 0.0f)) {
+  if (p.scale != 0 && (!std::isfinite(p.mean_inlet_speed) ||
+                       (p.mean_inlet_speed == 0.0f && p.body_force == nullptr))) {
     invalid("depth-averaged mean inlet speed must be finite and nonzero without a body force");
+  }
+  if (p.body_force != nullptr && p.scale == 0) {
+    invalid("a body-forced flow is returned scaled: set scale");
   }
   if (!std::isfinite(p.relative_tolerance) || p.relative_tolerance < 1.0e-15) {
     invalid("depth-averaged relative tolerance must be finite and at least 1e-15");
@@ -578,9 +586,14 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
   }
   const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
   auto solid = [&](size_t i) { return s.grid.obstacles != nullptr && s.grid.obstacles[i] != 0; };
+  // SignalGridAffineReaction::validate (signals.cpp:493-497): both arrays or
+  // neither.
+  if ((s.loss_rates == nullptr) != (s.source_rates == nullptr)) {
+    invalid("signal grid affine reaction arrays must match the grid level count");
+  }
   if (s.loss_rates != nullptr) {
     for (size_t i = 0; i < s.signal_count * n; ++i) {
-      const float src = s.source_rates != nullptr ? s.source_rates[i] : 0.0f;
+      const float src = s.source_rates[i];
       if (!std::isfinite(src) || src < 0.0f) {
         invalid("signal grid affine source rates must be finite and non-negative");
       }
@@ -656,9 +669,11 @@ void validate_levels(const msim_signal_grid& s, const T* levels) {
   if (fault == 2) invalid("signal grid levels must be zero at obstacle sites");
 }
 
+}  // namespace
+
 // Largest loss rate per signal, for the explicit bound: the affine loss's
 // maximum (max_reaction_loss, signals.cpp:322-330) plus vmax*max(phi)/K.
-void record_losses(Context& ctx, const msim_signal_grid& s) {
+void max_losses(const msim_signal_grid& s, double* loss_out) {
   const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
   float phi_max = 0.0f;
   if (s.phi != nullptr) {
@@ -672,21 +687,21 @@ void record_losses(Context& ctx, const msim_signal_grid& s) {
     if (s.phi != nullptr) {
       loss += static_cast<double>(s.monod_vmax[k]) * phi_max / static_cast<double>(s.monod_k[k]);
     }
-    ctx.adr.max_loss[k] = loss;
+    loss_out[k] = loss;
   }
 }
 
 // SignalGrid::validate_step (signals.cpp:1066-1102).
-void validate_step(const Context& ctx, float dt) {
+void validate_explicit_bound(const AdrParams& P, uint32_t integration, double greatest_outflow,
+                             const double* max_loss, float dt) {
   if (!std::isfinite(dt) || dt < 0.0f) invalid("time step must be finite and non-negative");
   // The explicit bound binds forward Euler only (signals.cpp:1070-1072).
-  if (ctx.adr.integration != MSIM_FORWARD_EULER) return;
-  const auto& P = ctx.adr.params;
+  if (integration != MSIM_FORWARD_EULER) return;
   const uint32_t dims[2] = {P.nx, P.ny};
   const float spacing[2] = {P.dx, P.dy};
   for (uint32_t k = 0; k < P.signals; ++k) {
     double inverse_square_sum = 0.0;
-    double courant_sum = ctx.adr.greatest_outflow;
+    double courant_sum = greatest_outflow;
     for (int a = 0; a < 2; ++a) {
       if (dims[a] == 1) continue;
       const double inverse_spacing = 1.0 / static_cast<double>(spacing[a]);
@@ -700,11 +715,31 @@ void validate_step(const Context& ctx, float dt) {
     const double factor =
         static_cast<double>(dt) *
         ((2.0 * static_cast<double>(P.diffusion[k]) * inverse_square_sum) + courant_sum +
-         ctx.adr.max_loss[k]);
+         max_loss[k]);
     if (!std::isfinite(factor) || factor > 1.0) {
       invalid("signal grid time step violates the explicit stability bound");
     }
   }
+}
+
+namespace {
+
+void record_losses(Context& ctx, const msim_signal_grid& s) { max_losses(s, ctx.adr.max_loss); }
+
+// A transport uploaded over the device's flow field reads it in place: any
+// later flow solve or preparation (another grid, speed, mobility or mask)
+// leaves it reading a field its explicit bound and mask were not taken from.
+void check_flow_field(const Context& c) {
+  if (c.adr.flow_field &&
+      (!c.flow.solved || c.flow.generation != c.adr.flow_generation ||
+       c.flow.layers.nz != 1 || c.flow.nx != c.adr.params.nx || c.flow.ny != c.adr.params.ny)) {
+    invalid("the flow field on the device is no longer the transport's");
+  }
+}
+
+void validate_step(const Context& ctx, float dt) {
+  validate_explicit_bound(ctx.adr.params, ctx.adr.integration, ctx.adr.greatest_outflow,
+                          ctx.adr.max_loss, dt);
 }
 
 // The world state's invariants (WorldState::validate, world_state.cpp:14-35)
@@ -1102,8 +1137,17 @@ int msim_hip_flow_solve(msim_ctx* ctx, const msim_flow_params* params, msim_flow
                                f.layers.faces.size() * sizeof(float), hipMemcpyDeviceToHost,
                                c.stream),
                 "download lifted faces");
+    std::vector<float> driven;
+    if (f.forced) {
+      // The driven column flux, lifted as the pressure-driven one is.
+      driven.resize(f.voxel_faces());
+      msim::flow_lift(c, f.lifted.data(), true, f.driven.data());
+      msim::check(hipMemcpyAsync(driven.data(), f.lifted.data(), driven.size() * sizeof(float),
+                                 hipMemcpyDeviceToHost, c.stream),
+                  "download driven faces");
+    }
     msim::check(hipStreamSynchronize(c.stream), "download lifted faces");
-    msim::scale_layers(f, *params, *report);
+    msim::scale_layers(f, *params, *report, f.forced ? &driven : nullptr);
   });
 }
 
@@ -1397,11 +1441,7 @@ int msim_hip_adr_run(msim_ctx* ctx, float dt, uint32_t steps, double* ledger) {
     if (!c.adr.uploaded) msim::invalid("no signal grid on the device");
     // A transport uploaded over the device's flow field reads it in place:
     // a later flow on another grid would leave it reading the wrong field.
-    if (c.adr.flow_field &&
-        (!c.flow.solved || c.flow.layers.nz != 1 || c.flow.nx != c.adr.params.nx ||
-         c.flow.ny != c.adr.params.ny)) {
-      msim::invalid("the flow field on the device is no longer the transport's");
-    }
+    msim::check_flow_field(c);
     msim::validate_step(c, dt);
     if (dt == 0.0f || steps == 0) return;
     msim::adr_run(c, dt, steps, ledger);
@@ -1476,6 +1516,7 @@ int msim_hip_coupled_run(msim_ctx* ctx, const float* previous_lengths, float dt,
                          msim_signal_report* report) {
   return msim::guarded(ctx, [&](Context& c) {
     if (!c.coupled.uploaded || !c.adr.uploaded) msim::invalid("no coupled state on the device");
+    msim::check_flow_field(c);
     if (!std::isfinite(dt) || dt < 0.0f) {
       msim::invalid("coupled time step must be finite and non-negative");
     }

@@ -43,13 +43,77 @@ __global__ void k_conductance(const uint8_t* __restrict__ solid, const float* __
 __device__ __forceinline__ bool open_edge(const FlowEdges& e, uint32_t side, uint32_t along,
                                           uint32_t nx, uint32_t ny, float& pressure) {
   if (e.open == nullptr) {
-    pressure = (side & 1u) != 0 ? 0.0f : 1.0f;
+    pressure = (side & 1u) != 0 || e.held_zero != 0 ? 0.0f : 1.0f;
     return (side >> 1) == e.axis;
   }
   const uint32_t slot =
       along + (side == 0 ? 0u : side == 1 ? ny : side == 2 ? 2u * ny : 2u * ny + nx);
-  pressure = e.pressure[slot];
+  pressure = e.held_zero != 0 ? 0.0f : e.pressure[slot];
   return e.open[slot] != 0;
+}
+
+// ShallowFlowReduction::body_flux (flow_system.hpp:395-420) per face of the
+// grid of columns (x faces, then y faces): the body force integrated across
+// each half cell in series, harm(m_l, m_u) (f_l/2 + f_u/2) on an interior
+// face, m f through an open exterior face, nothing elsewhere. force: (fx, fy)
+// per column.
+__global__ __launch_bounds__(256) void k_body_flux(const float* __restrict__ cond,
+                                                   const float2* __restrict__ force,
+                                                   float* __restrict__ flux, uint32_t nx,
+                                                   uint32_t ny, FlowEdges edges) {
+  const size_t nxf = static_cast<size_t>(nx + 1) * ny;
+  const size_t total = nxf + static_cast<size_t>(nx) * (ny + 1);
+  const size_t f = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= total) return;
+  uint32_t along, dim, across;
+  size_t lower, upper;
+  const bool xface = f < nxf;
+  if (xface) {
+    along = static_cast<uint32_t>(f / ny);
+    across = static_cast<uint32_t>(f % ny);
+    dim = nx;
+    upper = static_cast<size_t>(along) * ny + across;
+    lower = upper - ny;
+  } else {
+    const size_t g = f - nxf;
+    across = static_cast<uint32_t>(g / (ny + 1));
+    along = static_cast<uint32_t>(g % (ny + 1));
+    dim = ny;
+    upper = static_cast<size_t>(across) * ny + along;
+    lower = upper - 1;
+  }
+  float value = 0.0f;
+  float held = 0.0f;
+  if (along > 0 && along < dim) {
+    const float fl = xface ? force[lower].x : force[lower].y;
+    const float fu = xface ? force[upper].x : force[upper].y;
+    value = harmonic(cond[lower], cond[upper]) * (0.5f * fl + 0.5f * fu);
+  } else if (open_edge(edges, (xface ? 0u : 2u) + (along == 0 ? 0u : 1u), across, nx, ny,
+                       held)) {
+    const size_t site = along == 0 ? upper : lower;
+    value = cond[site] * (xface ? force[site].x : force[site].y);
+  }
+  flux[f] = value;
+}
+
+// The right-hand side of the forced system (flow_system.hpp:569-576 with
+// every boundary pressure zero): per site with conductance, the divergence
+// of the body flux, x then y, in float.
+__global__ __launch_bounds__(256) void k_forced_rhs(const float* __restrict__ cond,
+                                                    const float* __restrict__ flux,
+                                                    float* __restrict__ rhs, uint32_t nx,
+                                                    uint32_t ny, float dx, float dy) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(nx) * ny) return;
+  float r = 0.0f;
+  if (cond[i] != 0.0f) {
+    const uint32_t x = static_cast<uint32_t>(i / ny), y = static_cast<uint32_t>(i % ny);
+    const size_t nxf = static_cast<size_t>(nx + 1) * ny;
+    const size_t yf = nxf + static_cast<size_t>(x) * (ny + 1) + y;
+    r += (flux[i] - flux[i + ny]) / dx;
+    r += (flux[yf] - flux[yf + 1]) / dy;
+  }
+  rhs[i] = r;
 }
 
 // Assembly (flow_system.hpp:549-599): diagonal, right-hand side and boundary
@@ -431,6 +495,14 @@ __global__ void k_refine(double* __restrict__ xsol, const float* __restrict__ e,
     xsol[i] += scale * static_cast<double>(e[i]);
 }
 
+// FP64 correction: x += e.
+__global__ void k_add_correction(double* __restrict__ xsol, const double* __restrict__ e,
+                                 size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    xsol[i] += e[i];
+}
+
 struct ResRow {
   double x, lx, hx;
   float cx, cy, b, rhs, hcy;
@@ -529,7 +601,7 @@ __global__ __launch_bounds__(TY) void k_residual(
 __global__ __launch_bounds__(kStream) void k_velocity(
     const double* __restrict__ p, const float* __restrict__ cond, const uint8_t* __restrict__ solid,
     float* __restrict__ faces, uint32_t nx, uint32_t ny, float dx, float dy, float dz,
-    FlowEdges edges, CgState* st, double* partials) {
+    FlowEdges edges, const float* __restrict__ body_flux, CgState* st, double* partials) {
   __shared__ double s_red[kStream / kWave];
   __shared__ bool s_last;
   const size_t nxf = static_cast<size_t>(nx + 1) * ny;
@@ -575,7 +647,9 @@ __global__ __launch_bounds__(kStream) void k_velocity(
       }
       value = static_cast<double>(fc) * (static_cast<double>(fp) - p[site]);
     }
-    const float flux = static_cast<float>(value);
+    float flux = static_cast<float>(value);
+    // A forced solve's correction adds to the body flux (flow_system.hpp:1438-1439).
+    if (body_flux != nullptr) flux = body_flux[f] + flux;
     float v = 0.0f;
     if (!((has_lower && solid[lower] != 0) || (has_upper && solid[upper] != 0))) v = flux / dz;
     faces[f] = v;
@@ -606,27 +680,31 @@ __global__ __launch_bounds__(kStream) void k_velocity(
   }
 }
 
-// scaled = v * factor + 0 (flow_system.hpp:1401-1407) and the scaled peak.
+// scaled = v * factor + driven (flow_system.hpp:1401-1407; driven is zero
+// without a body force), the scaled peak, and whether any is not finite.
 __global__ __launch_bounds__(kStream) void k_scale(float* __restrict__ faces, size_t total,
-                                                   float factor, CgState* st, double* partials) {
+                                                   float factor, const float* __restrict__ driven,
+                                                   CgState* st, double* partials) {
   __shared__ double s_red[kStream / kWave];
   __shared__ bool s_last;
-  double peak = 0.0;
+  double peak = 0.0, bad = 0.0;
   const size_t stride = static_cast<size_t>(gridDim.x) * kStream;
   for (size_t f = static_cast<size_t>(blockIdx.x) * kStream + threadIdx.x; f < total;
        f += stride) {
-    const float v = faces[f] * factor + 0.0f;
+    const float v = faces[f] * factor + (driven != nullptr ? driven[f] : 0.0f);
     faces[f] = v;
+    if (!isfinite(v)) bad = 1.0;
     peak = fmax(peak, static_cast<double>(fabsf(v)));
   }
-  double part[1] = {block_max<kStream>(peak, s_red)};
-  if (publish_partials<1>(part, partials, &st->ticket[kTicketScale], blockIdx.x, gridDim.x,
+  double part[2] = {block_max<kStream>(peak, s_red), block_max<kStream>(bad, s_red)};
+  if (publish_partials<2>(part, partials, &st->ticket[kTicketScale], blockIdx.x, gridDim.x,
                           &s_last)) {
-    double m[1];
-    max_partials<kStream, 1>(partials, gridDim.x, m, s_red);
+    double m[2];
+    max_partials<kStream, 2>(partials, gridDim.x, m, s_red);
     if (threadIdx.x == 0) {
       st->ticket[kTicketScale] = 0;
       st->scaled_peak = m[0];
+      st->nonfinite = m[1] > 0.0 ? 1u : 0u;
     }
   }
 }
@@ -872,6 +950,56 @@ uint32_t jacobi_solve(Context& ctx, double tolerance, uint32_t budget, uint32_t 
   return iterations;
 }
 
+// FP64 Jacobi-PCG continued from the solution x already in f.x (the
+// learned solver's fallback): each pass solves A e = b - A x by the
+// reference's iteration (conjugate_gradient.hpp:126-166) from e = 0, to the
+// fraction of ||b - A x|| that would meet the target if the recursive residual
+// held, adds e to x and recomputes the FP64 true residual, which alone decides
+// convergence. A non-finite iterate restarts from x = 0.
+uint32_t jacobi_continue(Context& ctx, double tolerance, uint32_t budget, uint32_t interval,
+                         bool* converged, double* relative, KernelTimes* times) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  f.xc.reserve(n);
+  launch_residual(ctx, f.x.data(), f.res.data(), 1);
+  read_state(ctx);
+  const double bnorm = std::sqrt(f.host_state->bb);
+  double rnorm = std::sqrt(f.host_state->true_rr);
+  if (!std::isfinite(rnorm) || rnorm > bnorm) {
+    check(hipMemsetAsync(f.x.data(), 0, n * sizeof(double), ctx.stream), "clear x");
+    launch_residual(ctx, f.x.data(), f.res.data(), 1);
+    read_state(ctx);
+    rnorm = std::sqrt(f.host_state->true_rr);
+  }
+  uint32_t iterations = 0;
+  *converged = bnorm == 0.0 || rnorm <= tolerance * bnorm;
+  for (uint32_t pass = 0; !*converged && iterations < budget && pass < 16; ++pass) {
+    // Half the needed reduction as the recursive target: the true residual
+    // of an FP64 solve stays within a small factor of the recursive one.
+    const double inner = std::max(0.5 * tolerance * bnorm / rnorm, 1.0e-15);
+    launch_init<double, double>(ctx, f.res.data(), 1.0, f.r.data(), f.xc.data(), f.p0.data(),
+                                inner);
+    bool inner_converged = false;
+    double inner_relative = 0.0;
+    const uint32_t taken = run_cg<double>(ctx, f.r.data(), f.xc.data(), f.q.data(), f.p0.data(),
+                                          f.p1.data(), budget - iterations, interval,
+                                          &inner_converged, &inner_relative, times);
+    iterations += taken;
+    k_add_correction<<<stream_blocks(n), kStream, 0, ctx.stream>>>(f.x.data(), f.xc.data(), n);
+    check(hipGetLastError(), "launch k_add_correction");
+    launch_residual(ctx, f.x.data(), f.res.data(), 1);
+    read_state(ctx);
+    rnorm = std::sqrt(f.host_state->true_rr);
+    if (!std::isfinite(rnorm)) {
+      throw Error(MSIM_CG_RESIDUAL, "HIP Jacobi-PCG fallback produced a non-finite residual");
+    }
+    *converged = rnorm <= tolerance * bnorm;
+    if (!inner_converged && taken == 0) break;
+  }
+  *relative = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+  return iterations;
+}
+
 void launch_assembly(hipStream_t stream, const uint8_t* solid, const float* mobility, float* cond,
                      float* diag, float* rhs, float* bc, float* cxl, float* cyl, uint32_t nx,
                      uint32_t ny, float dx, float dy, float dz, FlowEdges edges) {
@@ -932,6 +1060,7 @@ void flow_assemble(Context& ctx, const uint8_t* solid_host, const float* mobilit
   check(hipStreamSynchronize(ctx.stream), "assemble");
   f.prepared = true;
   f.solved = false;
+  ++f.generation;
   f.mg.ready = false;
 }
 
@@ -947,17 +1076,20 @@ void flow_apply(Context& ctx, const double* x, double* y) {
   f.solved = false;
 }
 
-void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report) {
+namespace {
+
+// One pressure solve of the prepared system's right-hand side by the chosen
+// form, from x = 0, leaving the solution in f.x and its FP64 true residual in
+// the state: iterations, both residuals, refinements and whether the learned
+// solver fell back go to `report`.
+void solve_pressure(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
+                    KernelTimes* times) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
   const uint32_t interval = params.check_interval == 0 ? 64 : params.check_interval;
-  std::memset(&report, 0, sizeof(report));
-  check(hipEventRecord(ctx.start, ctx.stream), "event");
   uint32_t iterations = 0;
   bool converged = false;
   double relative = 0.0;
-  KernelTimes kernel_times;
-  KernelTimes* times = params.profile != 0 ? &kernel_times : nullptr;
   if (params.solver == MSIM_SOLVER_JACOBI_FP64) {
     iterations = jacobi_solve(ctx, params.relative_tolerance, params.max_iterations, interval,
                               &converged, &relative, times);
@@ -1026,7 +1158,8 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
     if (!ctx.learned.loaded) {
       throw Error(MSIM_INVALID_ARGUMENT, "the learned smoother has no weights: set them first");
     }
-    // Learned flexible CG up to the cap; Jacobi-PCG (FP64) from x = 0 after.
+    // Learned flexible CG up to the cap; FP64 Jacobi-PCG after, continued
+    // from the flexible-CG iterate and judged by the FP64 true residual.
     const uint32_t cap = std::min(params.max_iterations, params.fallback_iterations == 0
                                                              ? 200u
                                                              : params.fallback_iterations);
@@ -1046,8 +1179,9 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
       report.fell_back = 1;
       uint32_t extra = 0;
       if (params.max_iterations > iterations) {
-        extra = jacobi_solve(ctx, params.relative_tolerance, params.max_iterations - iterations,
-                             interval, &converged, &relative, times);
+        extra = jacobi_continue(ctx, params.relative_tolerance,
+                                params.max_iterations - iterations, interval, &converged,
+                                &relative, times);
       }
       iterations += extra;
       if (!converged) {
@@ -1064,18 +1198,106 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
   report.iterations = iterations;
   report.relative_residual = relative;
   report.true_relative_residual = s.bb > 0.0 ? std::sqrt(s.true_rr / s.bb) : 0.0;
+}
+
+// The host's checks of a body force (ShallowFlowReduction::body_flux,
+// flow_system.hpp:398-407), made after the pressure-driven solve as the
+// reference makes them.
+void check_body_force(const FlowDevice& f, const msim_flow_params& params) {
+  if (params.body_force_columns != f.sites()) {
+    throw Error(MSIM_INVALID_ARGUMENT, "shallow flow needs one in-plane body force per column");
+  }
+  for (size_t i = 0; i < 2 * f.sites(); ++i) {
+    if (!std::isfinite(params.body_force[i])) {
+      throw Error(MSIM_INVALID_ARGUMENT, "shallow-flow body force must be finite");
+    }
+  }
+}
+
+// The forced solve (solve_shallow_axis_flow, flow_system.hpp:1433-1441): the
+// body flux, the same operator with every boundary pressure zero and the
+// flux's divergence as its right-hand side, solved by the same form; its
+// velocity added to the body flux and lifted is the driven field (f.driven),
+// whose inlet sum the state then holds. The pressure-driven solve's
+// right-hand side and pressure are kept.
+void solve_forced(Context& ctx, const msim_flow_params& params, msim_flow_report& report,
+                  KernelTimes* times) {
+  auto& f = ctx.flow;
+  const size_t n = f.sites();
+  const size_t nfaces = f.x_faces() + f.y_faces();
+  f.force.reserve(2 * n);
+  f.body_flux.reserve(nfaces);
+  f.driven.reserve(nfaces);
+  f.rhs_saved.reserve(n);
+  f.x_saved.reserve(n);
+  check(hipMemcpyAsync(f.force.data(), params.body_force, 2 * n * sizeof(float),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload body force");
+  k_body_flux<<<static_cast<uint32_t>((nfaces + 255) / 256), 256, 0, ctx.stream>>>(
+      f.cond.data(), reinterpret_cast<const float2*>(f.force.data()), f.body_flux.data(), f.nx,
+      f.ny, f.edges());
+  check(hipGetLastError(), "launch k_body_flux");
+  check(hipMemcpyAsync(f.rhs_saved.data(), f.rhs.data(), n * sizeof(float),
+                       hipMemcpyDeviceToDevice, ctx.stream),
+        "keep right-hand side");
+  check(hipMemcpyAsync(f.x_saved.data(), f.x.data(), n * sizeof(double), hipMemcpyDeviceToDevice,
+                       ctx.stream),
+        "keep pressure");
+  k_forced_rhs<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, ctx.stream>>>(
+      f.cond.data(), f.body_flux.data(), f.rhs.data(), f.nx, f.ny, f.dx, f.dy);
+  check(hipGetLastError(), "launch k_forced_rhs");
+  auto restore = [&] {
+    check(hipMemcpyAsync(f.rhs.data(), f.rhs_saved.data(), n * sizeof(float),
+                         hipMemcpyDeviceToDevice, ctx.stream),
+          "restore right-hand side");
+  };
+  try {
+    solve_pressure(ctx, params, report, times);
+  } catch (...) {
+    restore();
+    static_cast<void>(hipStreamSynchronize(ctx.stream));
+    throw;
+  }
+  restore();
+  FlowEdges held = f.edges();
+  held.held_zero = 1;
+  k_velocity<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(
+      f.x.data(), f.cond.data(), f.solid.data(), f.driven.data(), f.nx, f.ny, f.dx, f.dy, f.dz,
+      held, f.body_flux.data(), f.state.data(), f.partials.data());
+  check(hipGetLastError(), "launch k_velocity");
+  check(hipMemcpyAsync(f.x.data(), f.x_saved.data(), n * sizeof(double), hipMemcpyDeviceToDevice,
+                       ctx.stream),
+        "restore pressure");
+  read_state(ctx);
+}
+
+}  // namespace
+
+void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& report) {
+  auto& f = ctx.flow;
+  // The field is rewritten from here on, whatever the outcome.
+  ++f.generation;
+  f.solved = false;
+  f.forced = false;
+  std::memset(&report, 0, sizeof(report));
+  check(hipEventRecord(ctx.start, ctx.stream), "event");
+  KernelTimes kernel_times;
+  KernelTimes* times = params.profile != 0 ? &kernel_times : nullptr;
+  solve_pressure(ctx, params, report, times);
+  const CgState& s = *f.host_state;
 
   // Velocity reconstruction, lift and scaling.
   const size_t nfaces = f.x_faces() + f.y_faces();
   k_velocity<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(
       f.x.data(), f.cond.data(), f.solid.data(), f.faces.data(), f.nx, f.ny, f.dx, f.dy, f.dz,
-      f.edges(), f.state.data(), f.partials.data());
+      f.edges(), nullptr, f.state.data(), f.partials.data());
   check(hipGetLastError(), "launch k_velocity");
   read_state(ctx);
   // A solve of a port-driven flow returns the unscaled field of the held
   // pressures, which the caller combines with the other solves' fields.
   float solved_mean = 0.0f;
   const float peak = static_cast<float>(s.peak);
+  const double inlet_count = s.inlet_count;
   if (!f.ported) {
     if (s.nonfinite != 0) {
       throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
@@ -1087,15 +1309,33 @@ void flow_solve(Context& ctx, const msim_flow_params& params, msim_flow_report& 
                   "the device carries no through-flow: the outlet is unreachable");
     }
   }
+  double driven_sum = 0.0;
+  if (params.body_force != nullptr && !f.ported) {
+    check_body_force(f, params);
+    msim_flow_report second{};
+    solve_forced(ctx, params, second, times);
+    driven_sum = s.inlet_sum;
+    report.iterations += second.iterations;
+    report.refinements += second.refinements;
+    report.relative_residual = std::max(report.relative_residual, second.relative_residual);
+    report.true_relative_residual =
+        std::max(report.true_relative_residual, second.true_relative_residual);
+    report.fell_back |= second.fell_back;
+    f.forced = true;
+  }
   float factor = 1.0f;
   float max_speed = peak;
   if (params.scale != 0 && !f.ported) {
-    const float driven_mean = static_cast<float>(0.0 / s.inlet_count);
+    const float driven_mean = static_cast<float>(driven_sum / inlet_count);
     factor = (params.mean_inlet_speed - driven_mean) / solved_mean;
-    k_scale<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(f.faces.data(), nfaces, factor,
-                                                              f.state.data(), f.partials.data());
+    k_scale<<<stream_blocks(nfaces), kStream, 0, ctx.stream>>>(
+        f.faces.data(), nfaces, factor, f.forced ? f.driven.data() : nullptr, f.state.data(),
+        f.partials.data());
     check(hipGetLastError(), "launch k_scale");
     read_state(ctx);
+    if (f.host_state->nonfinite != 0) {
+      throw Error(MSIM_RUNTIME_ERROR, "non-finite combined flow velocity");
+    }
     max_speed = static_cast<float>(f.host_state->scaled_peak);
   }
   check(hipEventRecord(ctx.stop, ctx.stream), "event");

@@ -150,6 +150,7 @@ struct FlowEdges {
   const uint8_t* open = nullptr;
   const float* pressure = nullptr;
   uint32_t axis = 0;
+  uint32_t held_zero = 0;  // every open face held at pressure zero (a forced solve)
 };
 
 // Everything the prepared depth-averaged system keeps on the device.
@@ -158,6 +159,9 @@ struct FlowDevice {
   float dx = 1, dy = 1, dz = 1;
   bool prepared = false;
   bool solved = false;
+  // Counts flow solves and preparations: a transport that reads the field in
+  // place records the count it was uploaded at, and is refused once it moves.
+  uint64_t generation = 0;
   // Port-driven flow: the open exterior faces and the pressures held outside
   // them (FlowEdges slots) instead of the axis boundary.
   bool ported = false;
@@ -168,6 +172,13 @@ struct FlowDevice {
   DeviceBuffer<float> cond;        // column conductance (reduction)
   DeviceBuffer<float> diag, rhs, bc, cxl, cyl;
   DeviceBuffer<double> x, r, q, p0, p1, res;
+  DeviceBuffer<double> xc;         // FP64 correction of the learned solver's fallback
+  // A body-forced solve: the force per column (fx, fy), the body flux per
+  // face, the driven field, and the pressure-driven solve's right-hand side
+  // and pressure kept across the forced solve.
+  DeviceBuffer<float> force, body_flux, driven, rhs_saved;
+  DeviceBuffer<double> x_saved;
+  bool forced = false;             // the last solve carried a body force
   DeviceBuffer<float> x32, r32, q32, p032, p132;
   DeviceBuffer<double> partials;
   DeviceBuffer<double> pq, rz, rr; // per-block Krylov partial sums, two generations each
@@ -259,6 +270,7 @@ struct AdrDevice {
   uint32_t precision = 32;
   bool uploaded = false;
   bool flow_field = false;
+  uint64_t flow_generation = 0;  // FlowDevice::generation of the field read in place
   uint64_t revision = 0;
   double greatest_outflow = 0.0;
   double max_loss[kMaxSignals] = {};
@@ -418,8 +430,9 @@ void flow_apply(Context& ctx, const double* x, double* y);
 // The solved field on the original grid's x|y|z faces (device `out`, of
 // flow.voxel_faces() floats): one layer copies the face velocities (and
 // zeroes the z faces when `with_z`); several layers lift the column flux
-// (ShallowFlowReduction::lift, flow_system.hpp:424-470).
-void flow_lift(Context& ctx, float* out, bool with_z = true);
+// (ShallowFlowReduction::lift, flow_system.hpp:424-470). `source`: the
+// column field to lift in place of the solve's (the driven field).
+void flow_lift(Context& ctx, float* out, bool with_z = true, const float* source = nullptr);
 // Uploads the voxel mask and the column heights a lift of several layers reads.
 void flow_upload_layers(Context& ctx);
 // A checked port-driven flow (ShallowPortFlow, flow_system.hpp:1268-1311):
@@ -454,6 +467,9 @@ void batch_download(Context& ctx, uint32_t index, double* pressure, float* x_fac
                     float* y_faces, float* levels);
 
 // adr.hip
+// The kernels' parameters of a one-layer signal grid (velocity field and
+// patches aside).
+AdrParams transport_params(const msim_signal_grid& grid);
 void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
                 uint32_t precision, bool use_flow_field);
 void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger);
@@ -470,6 +486,12 @@ void launch_settle_f32(Context& ctx, float* levels, size_t count, const double* 
 // flags with `step`.
 void fe_candidate_f32(Context& ctx, float dt, const float* in, float* out, uint32_t step,
                       const float* xf, const float* yf);
+
+// capi.cpp: the largest loss rate per signal of a spec, and the explicit
+// bound (SignalGrid::validate_step, signals.cpp:1066-1102).
+void max_losses(const msim_signal_grid& s, double* loss_out);
+void validate_explicit_bound(const AdrParams& P, uint32_t integration, double greatest_outflow,
+                             const double* max_loss, float dt);
 
 // implicit.hip: `steps` implicit steps of dt on the uploaded FP32 levels.
 // sources (device, S*N, or null): rates added to each step's right-hand side as

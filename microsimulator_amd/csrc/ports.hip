@@ -176,13 +176,14 @@ void flow_upload_layers(Context& ctx) {
   check(hipStreamSynchronize(ctx.stream), "upload layers");
 }
 
-void flow_lift(Context& ctx, float* out, bool with_z) {
+void flow_lift(Context& ctx, float* out, bool with_z, const float* source) {
   auto& f = ctx.flow;
   const uint32_t nz = f.layers.nz;
+  const float* faces = source != nullptr ? source : f.faces.data();
   if (nz == 1) {
     // One layer: the face velocities are the field; no vertical faces flow.
     const size_t planar = f.x_faces() + f.y_faces();
-    check(hipMemcpyAsync(out, f.faces.data(), planar * sizeof(float), hipMemcpyDeviceToDevice,
+    check(hipMemcpyAsync(out, faces, planar * sizeof(float), hipMemcpyDeviceToDevice,
                          ctx.stream),
           "copy face velocities");
     if (with_z) {
@@ -193,7 +194,7 @@ void flow_lift(Context& ctx, float* out, bool with_z) {
   }
   const size_t planar = (f.x_faces() + f.y_faces()) * nz;
   k_lift_faces<<<grid_blocks(planar, kLiftThreads, 0xffffffffu), kLiftThreads, 0, ctx.stream>>>(
-      f.faces.data(), f.voxels.data(), f.heights.data(), out, f.nx, f.ny, nz);
+      faces, f.voxels.data(), f.heights.data(), out, f.nx, f.ny, nz);
   check(hipGetLastError(), "launch k_lift_faces");
   k_lift_columns<<<grid_blocks(f.sites(), kLiftThreads, 0xffffffffu), kLiftThreads, 0,
                    ctx.stream>>>(f.voxels.data(), out, f.nx, f.ny, nz, f.layers.bottom, f.dx,

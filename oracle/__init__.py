@@ -201,6 +201,8 @@ class Oracle:
         L.mo_flow_assemble.argtypes = [P, P, U, P, P, P, P, P, P, C.c_char_p, Z]
         L.mo_flow_apply.argtypes = [P, P, U, P, P, C.c_char_p, Z]
         L.mo_flow_solve.argtypes = [P, P, U, F, F, U, C.c_int, P, P, P, C.c_char_p, Z]
+        L.mo_flow_solve_forced.argtypes = [P, P, U, F, F, U, C.c_int, P, Z, P, P, P, C.c_char_p,
+                                           Z]
         L.mo_port_flow_solve.argtypes = [P, P, P, U, F, U, P, P, P, C.c_char_p, Z]
         L.mo_port_flow_solve.restype = C.c_int
         L.mo_adr_validate_step.argtypes = [P, F, C.c_char_p, Z]
@@ -210,7 +212,8 @@ class Oracle:
         L.mo_advance_coupled_f32.argtypes = [P, P, P, P, P, P, P, F, P, C.c_char_p, Z]
         L.mo_advance_coupled_f32.restype = C.c_int
         L.mo_adr_step_implicit_f32.restype = C.c_int
-        for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_adr_validate_step,
+        for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_flow_solve_forced,
+                   L.mo_adr_validate_step,
                    L.mo_adr_step_f32, L.mo_adr_step_f64):
             fn.restype = C.c_int
 
@@ -251,17 +254,23 @@ class Oracle:
         return y
 
     def solve_flow(self, grid: Grid, mobility, axis: int = 0, mean_inlet_speed: float = 1.0,
-                   tolerance: float = 1e-6, max_iterations: int = 50000, scale: bool = True):
+                   tolerance: float = 1e-6, max_iterations: int = 50000, scale: bool = True,
+                   body_force=None):
+        """body_force: (fx, fy) per column (columns x 2), or None."""
         keep: list = []
         g = grid._c(keep)
         faces = np.zeros(sum(grid.face_counts()), np.float32)
         pressure = np.zeros(grid.nx * grid.ny, np.float64)
         rep = _FlowReport()
         mob = _f32(mobility)
+        force = None
+        if body_force is not None:
+            force = np.ascontiguousarray(body_force, np.float32).reshape(-1)
         err = C.create_string_buffer(512)
-        st = self.lib.mo_flow_solve(
+        st = self.lib.mo_flow_solve_forced(
             C.byref(g), _ptr(mob), C.c_uint32(axis), C.c_float(mean_inlet_speed),
             C.c_float(tolerance), C.c_uint32(max_iterations), C.c_int(1 if scale else 0),
+            _ptr(force), C.c_size_t(0 if force is None else force.size // 2),
             faces.ctypes.data, pressure.ctypes.data, C.byref(rep), err, C.c_size_t(512))
         self._check(st, err)
         report = FlowReport(rep.iterations, rep.relative_residual, rep.true_relative_residual,
@@ -445,6 +454,8 @@ class Reference:
         P, U, F, Z = C.c_void_p, C.c_uint32, C.c_float, C.c_size_t
         L.ref_solve_depth_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, P, P, P,
                                            C.c_char_p, Z]
+        L.ref_solve_depth_flow_forced.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, Z, P,
+                                                  P, P, P, C.c_char_p, Z]
         L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, U,
                                               U, P, P, U, U, F, F, F, U, P, P, P, C.c_char_p, Z]
         L.ref_set_threads.argtypes = [U]
@@ -469,7 +480,8 @@ class Reference:
         return np.array(grid.boundary, np.uint8)
 
     def solve_flow(self, grid: Grid, mobility, axis=0, mean_inlet_speed=1.0, tolerance=1e-6,
-                   max_iterations=50000):
+                   max_iterations=50000, body_force=None):
+        """solve_depth_averaged_flow_cpu; body_force: (fx, fy) per column, or None."""
         kinds = self._kinds(grid)
         obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
         mob = _f32(mobility)
@@ -478,12 +490,17 @@ class Reference:
         rr = C.c_float()
         ms = C.c_float()
         err = C.create_string_buffer(512)
-        st = self.lib.ref_solve_depth_flow(
-            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
-            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(mob),
-            C.c_uint32(axis), C.c_float(mean_inlet_speed), C.c_float(tolerance),
-            C.c_uint32(max_iterations), faces.ctypes.data, C.byref(it), C.byref(rr), C.byref(ms),
-            err, C.c_size_t(512))
+        head = (C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz),
+                C.c_float(grid.dx), C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs),
+                kinds.ctypes.data, _ptr(mob), C.c_uint32(axis), C.c_float(mean_inlet_speed),
+                C.c_float(tolerance), C.c_uint32(max_iterations))
+        tail = (faces.ctypes.data, C.byref(it), C.byref(rr), C.byref(ms), err, C.c_size_t(512))
+        if body_force is None:
+            st = self.lib.ref_solve_depth_flow(*head, *tail)
+        else:
+            force = np.ascontiguousarray(body_force, np.float32).reshape(-1)
+            st = self.lib.ref_solve_depth_flow_forced(*head, force.ctypes.data,
+                                                      C.c_size_t(force.size // 2), *tail)
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
         return faces, dict(iterations=it.value, relative_residual=rr.value, max_speed=ms.value)

@@ -298,7 +298,7 @@ static int reduce_columns(const mo_grid* g, const float* mobility, system_t* s, 
  * neighbour slots, lower then upper along x, y, z. May be called again on the
  * same reduction with another boundary. */
 static int assemble_system(system_t* s, const uint8_t* open_faces, const float* pressures,
-                           char* err, size_t errlen) {
+                           const float* body_flux, char* err, size_t errlen) {
   const mo_grid* g = &s->grid2;
   const size_t n = s->n;
   memset(s->diagonal, 0, n * sizeof(float));
@@ -326,6 +326,14 @@ static int assemble_system(system_t* s, const uint8_t* open_faces, const float* 
     for (int comp = 0; comp < 3; ++comp) {
       const float sp = spacing[comp];
       const float inverse_square = 1.0f / (sp * sp);
+      if (body_flux != NULL) {
+        /* The divergence of the imposed body flux (flow_system.hpp:569-576). */
+        uint32_t up[3] = {c[0], c[1], c[2]};
+        ++up[comp];
+        const size_t lo = face_index(g, comp, c[0], c[1], c[2]);
+        const size_t hi = face_index(g, comp, up[0], up[1], up[2]);
+        s->rhs[site] += (body_flux[lo] - body_flux[hi]) / sp;
+      }
       for (int off = -1; off <= 1; off += 2) {
         int has = 0;
         size_t nb = 0;
@@ -441,7 +449,7 @@ static int build_system(const mo_grid* g, const float* mobility, uint32_t axis, 
     pressure[lower] = 1.0f;
     open[upper] = 1;
   }
-  status = assemble_system(s, open, pressure, err, errlen);
+  status = assemble_system(s, open, pressure, NULL, err, errlen);
   free(open);
   free(pressure);
   if (status != MO_OK) system_free(s);
@@ -605,9 +613,9 @@ done:
  * floats; full: the pressure at every column (may be NULL); rr/bb: the FP64
  * true residual and rhs norms (may be NULL). */
 static int solve_and_lift(const system_t* s, const mo_grid* g, float tolerance,
-                          uint32_t max_iterations, float* velocity, double* full_out,
-                          uint32_t* iterations, double* relative, double* rr_out, double* bb_out,
-                          char* err, size_t errlen) {
+                          uint32_t max_iterations, const float* body_flux, float* velocity,
+                          double* full_out, uint32_t* iterations, double* relative, double* rr_out,
+                          double* bb_out, char* err, size_t errlen) {
   double* solution = calloc(s->count + 1, sizeof(double));
   double* full = calloc(s->n, sizeof(double));
   float* flux = malloc(s->faces * sizeof(float));
@@ -667,6 +675,9 @@ static int solve_and_lift(const system_t* s, const mo_grid* g, float tolerance,
             value = (double)s->face_cond[index] * ((double)s->face_pressure[index] - full[site]);
           }
           flux[index] = (float)value;
+          /* A forced solve's correction adds to the body flux
+           * (flow_system.hpp:1438-1439). */
+          if (body_flux != NULL) flux[index] = body_flux[index] + flux[index];
         }
   }
   const uint32_t nz = g->nz;
@@ -729,14 +740,62 @@ static int solve_and_lift(const system_t* s, const mo_grid* g, float tolerance,
   return MO_OK;
 }
 
+/* ShallowFlowReduction::body_flux (flow_system.hpp:395-420): the imposed
+ * pressure-gradient-equivalent body force integrated across each half cell in
+ * series, over the faces of the grid of columns. An interior face carries
+ * harm(m_l, m_u) (f_l/2 + f_u/2) of its axis's component; an open exterior
+ * face m f of its column's; z faces nothing. force: (fx, fy) per column. */
+static void body_flux(const system_t* s, const float* force, const uint8_t* open, float* flux) {
+  const mo_grid* g2 = &s->grid2;
+  memset(flux, 0, s->faces * sizeof(float));
+  const uint32_t dims[2] = {g2->nx, g2->ny};
+  for (int comp = 0; comp < 2; ++comp) {
+    uint32_t fd[2] = {dims[0], dims[1]};
+    ++fd[comp];
+    for (uint32_t x = 0; x < fd[0]; ++x)
+      for (uint32_t y = 0; y < fd[1]; ++y) {
+        const uint32_t fc[2] = {x, y};
+        const size_t face = face_index(g2, comp, x, y, 0);
+        const int has_lower = fc[comp] > 0, has_upper = fc[comp] < dims[comp];
+        uint32_t lc[2] = {x, y};
+        if (has_lower) --lc[comp];
+        const size_t lower = (size_t)lc[0] * g2->ny + lc[1];
+        const size_t upper = (size_t)x * g2->ny + y;
+        if (has_lower && has_upper) {
+          flux[face] = harmonic_mean(s->mobility[lower], s->mobility[upper]) *
+                       (0.5f * force[2 * lower + comp] + 0.5f * force[2 * upper + comp]);
+        } else if (open[face]) {
+          const size_t site = has_lower ? lower : upper;
+          flux[face] = s->mobility[site] * force[2 * site + comp];
+        }
+      }
+  }
+}
+
 int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
                   float mean_inlet_speed, float tolerance, uint32_t max_iterations, int scale,
                   float* faces, double* pressure, mo_flow_report* report, char* err,
                   size_t errlen) {
+  return mo_flow_solve_forced(g, mobility, axis, mean_inlet_speed, tolerance, max_iterations,
+                              scale, NULL, 0, faces, pressure, report, err, errlen);
+}
+
+/* solve_shallow_axis_flow (flow_system.hpp:1420-1447) with the solve of
+ * cpu_flow.cpp:338-351: the pressure-driven solve; with a body force, a
+ * second solve of the same system with every boundary pressure zero and the
+ * body flux's divergence as its right-hand side, whose correction added to
+ * the body flux and lifted is the driven field; then scale_velocity
+ * (1368-1415), which scales only the pressure-driven field so the combined
+ * mean over the open inlet faces is the requested one. Iterations add over
+ * both solves and the residuals are the larger of the two. */
+int mo_flow_solve_forced(const mo_grid* g, const float* mobility, uint32_t axis,
+                         float mean_inlet_speed, float tolerance, uint32_t max_iterations,
+                         int scale, const float* force, size_t force_columns, float* faces,
+                         double* pressure, mo_flow_report* report, char* err, size_t errlen) {
   /* DepthAveragedFlowParameters::validate, cpu_flow.cpp:134-150; the
    * tolerance floor (float epsilon) is not applied here: the configurations
    * solve to 1e-8 and 1e-10. */
-  if (scale && (!isfinite(mean_inlet_speed) || mean_inlet_speed == 0.0f))
+  if (scale && (!isfinite(mean_inlet_speed) || (mean_inlet_speed == 0.0f && force == NULL)))
     return fail(err, errlen, MO_INVALID_ARGUMENT,
                 "depth-averaged mean inlet speed must be finite and nonzero without a body force");
   if (!isfinite(tolerance) || tolerance <= 0.0f)
@@ -749,80 +808,118 @@ int mo_flow_solve(const mo_grid* g, const float* mobility, uint32_t axis,
   const uint32_t nz = g->nz;
   const size_t nfaces = total_faces(g);
   float* velocity = malloc(nfaces * sizeof(float));
+  float* driven = force != NULL ? calloc(nfaces, sizeof(float)) : NULL;
   double* full = malloc(s.n * sizeof(double));
   uint32_t iterations = 0;
   double relative = 0.0, rr = 0.0, bb = 0.0;
-  status = solve_and_lift(&s, g, tolerance, max_iterations, velocity, full, &iterations, &relative,
-                          &rr, &bb, err, errlen);
-  if (status != MO_OK) {
-    free(velocity);
-    free(full);
-    system_free(&s);
-    return status;
-  }
-  /* scale_velocity, flow_system.hpp:1368-1415; open inlet faces are the
-   * axis's lower faces of fluid voxels in columns with conductance
-   * (open_inlet_faces, 449-459). */
-  double inlet_sum = 0.0;
-  size_t inlet_count = 0;
-  float peak = 0.0f;
-  for (size_t f = 0; f < nfaces; ++f) {
-    if (!isfinite(velocity[f])) {
-      free(velocity);
-      free(full);
-      system_free(&s);
-      return fail(err, errlen, MO_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+  float* bflux = NULL;
+  uint8_t* open = NULL;
+  float* zero = NULL;
+#define FORCED_FAIL(code_expr)  \
+  do {                          \
+    status = (code_expr);       \
+    goto finish;                \
+  } while (0)
+  status = solve_and_lift(&s, g, tolerance, max_iterations, NULL, velocity, full, &iterations,
+                          &relative, &rr, &bb, err, errlen);
+  if (status != MO_OK) goto finish;
+  if (force != NULL) {
+    if (force_columns != s.n)
+      FORCED_FAIL(fail(err, errlen, MO_INVALID_ARGUMENT,
+                       "shallow flow needs one in-plane body force per column"));
+    for (size_t i = 0; i < 2 * s.n; ++i)
+      if (!isfinite(force[i]))
+        FORCED_FAIL(fail(err, errlen, MO_INVALID_ARGUMENT, "shallow-flow body force must be finite"));
+    /* axis_flow_boundary's open faces with every pressure zero. */
+    bflux = malloc(s.faces * sizeof(float));
+    open = calloc(s.faces, 1);
+    zero = calloc(s.faces, sizeof(float));
+    if (bflux == NULL || open == NULL || zero == NULL || driven == NULL)
+      FORCED_FAIL(fail(err, errlen, MO_RUNTIME_ERROR, "out of memory"));
+    const mo_grid* g2 = &s.grid2;
+    const uint32_t across = axis == 0 ? g2->ny : g2->nx;
+    for (uint32_t t = 0; t < across; ++t) {
+      open[axis == 0 ? face_index(g2, 0, 0, t, 0) : face_index(g2, 1, t, 0, 0)] = 1;
+      open[axis == 0 ? face_index(g2, 0, g2->nx, t, 0) : face_index(g2, 1, t, g2->ny, 0)] = 1;
     }
-    const float a = fabsf(velocity[f]);
-    if (a > peak) peak = a;
+    body_flux(&s, force, open, bflux);
+    status = assemble_system(&s, open, zero, bflux, err, errlen);
+    if (status != MO_OK) goto finish;
+    uint32_t forced_iterations = 0;
+    double forced_relative = 0.0, frr = 0.0, fbb = 0.0;
+    status = solve_and_lift(&s, g, tolerance, max_iterations, bflux, driven, NULL,
+                            &forced_iterations, &forced_relative, &frr, &fbb, err, errlen);
+    if (status != MO_OK) goto finish;
+    iterations += forced_iterations;
+    if (forced_relative > relative) relative = forced_relative;
+    if (fbb > 0.0 && (bb == 0.0 || frr / fbb > rr / bb)) {
+      rr = frr;
+      bb = fbb;
+    }
   }
-  const uint32_t across = axis == 0 ? g->ny : g->nx;
-  for (uint32_t t = 0; t < across; ++t)
-    for (uint32_t z = 0; z < nz; ++z) {
-      const size_t column = axis == 0 ? (size_t)t : (size_t)t * g->ny;
-      const size_t f = axis == 0 ? face_index(g, 0, 0, t, z) : face_index(g, 1, t, 0, z);
-      if (!solid_at(g, column * nz + z) && s.mobility[column] > 0.0f) {
-        inlet_sum += velocity[f];
-        ++inlet_count;
+  {
+    /* scale_velocity, flow_system.hpp:1368-1415; open inlet faces are the
+     * axis's lower faces of fluid voxels in columns with conductance
+     * (open_inlet_faces, 449-459). */
+    double inlet_sum = 0.0, driven_sum = 0.0;
+    size_t inlet_count = 0;
+    float peak = 0.0f;
+    for (size_t f = 0; f < nfaces; ++f) {
+      if (!isfinite(velocity[f]))
+        FORCED_FAIL(fail(err, errlen, MO_RUNTIME_ERROR, "flow solve produced a non-finite velocity"));
+      const float a = fabsf(velocity[f]);
+      if (a > peak) peak = a;
+    }
+    const uint32_t across = axis == 0 ? g->ny : g->nx;
+    for (uint32_t t = 0; t < across; ++t)
+      for (uint32_t z = 0; z < nz; ++z) {
+        const size_t column = axis == 0 ? (size_t)t : (size_t)t * g->ny;
+        const size_t f = axis == 0 ? face_index(g, 0, 0, t, z) : face_index(g, 1, t, 0, z);
+        if (!solid_at(g, column * nz + z) && s.mobility[column] > 0.0f) {
+          inlet_sum += velocity[f];
+          if (driven != NULL) driven_sum += driven[f];
+          ++inlet_count;
+        }
       }
+    if (inlet_count == 0)
+      FORCED_FAIL(fail(err, errlen, MO_RUNTIME_ERROR, "flow solve has no open inlet faces"));
+    const float solved_mean = (float)(inlet_sum / (double)inlet_count);
+    if (peak == 0.0f || solved_mean <= 1.0e-9f * peak)
+      FORCED_FAIL(fail(err, errlen, MO_RUNTIME_ERROR,
+                       "the device carries no through-flow: the outlet is unreachable"));
+    const float driven_mean = (float)(driven_sum / (double)inlet_count);
+    const float factor = scale ? (mean_inlet_speed - driven_mean) / solved_mean : 1.0f;
+    float scaled_peak = 0.0f;
+    for (size_t f = 0; f < nfaces; ++f) {
+      const float v = scale ? velocity[f] * factor + (driven != NULL ? driven[f] : 0.0f)
+                            : velocity[f];
+      if (!isfinite(v))
+        FORCED_FAIL(fail(err, errlen, MO_RUNTIME_ERROR, "non-finite combined flow velocity"));
+      faces[f] = v;
+      const float a = fabsf(v);
+      if (a > scaled_peak) scaled_peak = a;
     }
-  if (inlet_count == 0) {
-    free(velocity);
-    free(full);
-    system_free(&s);
-    return fail(err, errlen, MO_RUNTIME_ERROR, "flow solve has no open inlet faces");
+    if (pressure != NULL) memcpy(pressure, full, s.n * sizeof(double));
+    if (report != NULL) {
+      report->iterations = iterations;
+      report->relative_residual = relative;
+      report->true_relative_residual = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+      report->mean_inlet_speed = scale ? mean_inlet_speed : solved_mean;
+      report->max_speed = scaled_peak;
+      report->solved_mean = solved_mean;
+      report->factor = factor;
+    }
   }
-  const float solved_mean = (float)(inlet_sum / (double)inlet_count);
-  if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
-    free(velocity);
-    free(full);
-    system_free(&s);
-    return fail(err, errlen, MO_RUNTIME_ERROR,
-                "the device carries no through-flow: the outlet is unreachable");
-  }
-  const float driven_mean = (float)(0.0 / (double)inlet_count);
-  const float factor = scale ? (mean_inlet_speed - driven_mean) / solved_mean : 1.0f;
-  float scaled_peak = 0.0f;
-  for (size_t f = 0; f < nfaces; ++f) {
-    const float v = scale ? velocity[f] * factor + 0.0f : velocity[f];
-    faces[f] = v;
-    const float a = fabsf(v);
-    if (a > scaled_peak) scaled_peak = a;
-  }
-  if (pressure != NULL) memcpy(pressure, full, s.n * sizeof(double));
-  if (report != NULL) {
-    report->iterations = iterations;
-    report->relative_residual = relative;
-    report->true_relative_residual = bb > 0.0 ? sqrt(rr / bb) : 0.0;
-    report->mean_inlet_speed = scale ? mean_inlet_speed : solved_mean;
-    report->max_speed = scaled_peak;
-    report->solved_mean = solved_mean;
-    report->factor = factor;
-  }
+#undef FORCED_FAIL
+finish:
   free(velocity);
+  free(driven);
   free(full);
+  free(bflux);
+  free(open);
+  free(zero);
   system_free(&s);
-  return MO_OK;
+  return status;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1145,7 +1242,7 @@ int mo_port_flow_solve(const mo_grid* g, const float* mobility, const mo_flow_po
       const size_t k = port_faces(&s.grid2, &reduced[p], scratch_faces, scratch_sites);       \
       for (size_t i = 0; i < k; ++i) pressure[scratch_faces[i]] = (values)[p];                \
     }                                                                                         \
-    status = assemble_system(&s, open, pressure, err, errlen);                                \
+    status = assemble_system(&s, open, pressure, NULL, err, errlen);                          \
     if (status != MO_OK) goto done;                                                           \
     fields[slot] = malloc(nfaces * sizeof(float));                                            \
     if (fields[slot] == NULL) {                                                               \
@@ -1155,7 +1252,7 @@ int mo_port_flow_solve(const mo_grid* g, const float* mobility, const mo_flow_po
     ++nfields;                                                                                \
     uint32_t its = 0;                                                                         \
     double rel = 0.0;                                                                         \
-    status = solve_and_lift(&s, g, tolerance, max_iterations, fields[slot], NULL, &its, &rel, \
+    status = solve_and_lift(&s, g, tolerance, max_iterations, NULL, fields[slot], NULL, &its, &rel, \
                             NULL, NULL, err, errlen);                                         \
     if (status != MO_OK) goto done;                                                           \
     iterations_sum += its;                                                                    \

@@ -12,6 +12,7 @@
  * Parity status per function (what pins it):
  *   mo_flow_assemble        bit-exact vs reference assembly fixtures + closed forms
  *   mo_flow_solve           bit-exact vs reference solve_depth_averaged_flow_cpu
+ *   mo_flow_solve_forced    bit-exact vs the same with a body force (live _ref)
  *                           fixtures; plug flow 5.0, mobility-split ratio 3,
  *                           dense-Cholesky brute force on tiny grids
  *   mo_flow_apply           symmetry / positivity; bit-exact vs reference apply
@@ -104,6 +105,14 @@ int mo_flow_solve(const mo_grid* grid, const float* mobility, uint32_t axis,
                   float mean_inlet_speed, float tolerance, uint32_t max_iterations, int scale,
                   float* faces, double* pressure, mo_flow_report* report, char* err,
                   size_t errlen);
+
+/* mo_flow_solve with DepthAveragedFlowParameters::body_force (flow.hpp:24-25):
+ * force holds (fx, fy) per column of the grid (x-major, y-minor), force_columns
+ * of them (nx*ny, or the reference's refusal), or NULL for none. */
+int mo_flow_solve_forced(const mo_grid* grid, const float* mobility, uint32_t axis,
+                         float mean_inlet_speed, float tolerance, uint32_t max_iterations,
+                         int scale, const float* force, size_t force_columns, float* faces,
+                         double* pressure, mo_flow_report* report, char* err, size_t errlen);
 
 /* FlowPortKind and FlowPort (cpp/include/microsimulator/flow.hpp:69-91): an
  * opening over a rectangle of one fixed face, driven at a mean speed (into

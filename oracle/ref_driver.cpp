@@ -151,6 +151,41 @@ int ref_solve_depth_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, f
   });
 }
 
+// The same with DepthAveragedFlowParameters::body_force: force_columns pairs
+// (fx, fy), x-major, y-minor.
+int ref_solve_depth_flow_forced(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                                float dy, float dz, const std::uint8_t* obstacles,
+                                const std::uint8_t* kinds, const float* mobility,
+                                std::uint32_t axis, float mean_inlet_speed, float tolerance,
+                                std::uint32_t max_iterations, const float* force,
+                                std::size_t force_columns, float* faces,
+                                std::uint32_t* iterations, float* relative_residual,
+                                float* max_speed, char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> mob;
+    if (mobility != nullptr) mob = std::span<const float>(mobility, n);
+    ms::DepthAveragedFlowParameters parameters;
+    parameters.mean_inlet_speed = mean_inlet_speed;
+    parameters.axis = static_cast<ms::FlowAxis>(axis);
+    parameters.relative_tolerance = tolerance;
+    parameters.max_iterations = max_iterations;
+    for (std::size_t c = 0; c < force_columns; ++c) {
+      parameters.body_force.push_back({force[2 * c], force[2 * c + 1]});
+    }
+    const auto result = ms::solve_depth_averaged_flow_cpu(spec, mob, parameters);
+    std::memcpy(faces, result.field.x_faces.data(), result.field.x_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size(), result.field.y_faces.data(),
+                result.field.y_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size() + result.field.y_faces.size(),
+                result.field.z_faces.data(), result.field.z_faces.size() * sizeof(float));
+    *iterations = result.report.iterations;
+    *relative_residual = result.report.relative_residual;
+    *max_speed = result.report.max_speed;
+  });
+}
+
 // Builds the reduced, assembled depth-averaged system once (as the reference
 // does before each solve) and keeps it for repeated solves.
 void* ref_flow_create(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,

@@ -22,6 +22,7 @@ from microsimulator_amd import learned
 from microsimulator_amd.workloads import channel_trap
 from tests import learned_reference as lr
 from tests import mg_reference as mr
+from oracle import Grid as OGrid
 from tests.helpers import FIXED, NO_FLUX, c1_inputs, random_case
 
 pytestmark = pytest.mark.gpu
@@ -172,7 +173,7 @@ def test_training_lowers_the_loss(hip, pairs):
     assert np.any(w1[-lr.SIZES[2]:] != 0.0)
 
 
-def test_fallback_to_jacobi_at_the_cap(hip):
+def test_fallback_to_jacobi_at_the_cap(hip, oracle):
     c = c1_inputs()
     g = msa.Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
     hip.flow_prepare(g, c["mobility"], 0)
@@ -185,8 +186,16 @@ def test_fallback_to_jacobi_at_the_cap(hip):
           rep.true_relative_residual, flush=True)
     assert rep.fell_back
     assert rep.iterations > 3
-    # Jacobi-PCG stops on its recursive residual, as the reference's does.
-    assert rep.true_relative_residual <= 2e-8
+    # The fallback continues from the flexible-CG iterate and stops on the
+    # FP64 true residual, the judge of every form.
+    assert rep.true_relative_residual <= 1e-8
+    assert rep.relative_residual == pytest.approx(rep.true_relative_residual, rel=1e-12)
+    # The field is the solution's: against the oracle's FP64 solve.
+    faces = hip.flow_download()[1].concatenated()
+    ref_faces, _, ref_rep = oracle.solve_flow(
+        OGrid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"]), c["mobility"], 0, 100.0,
+        1e-10, 100000)
+    assert np.abs(faces - ref_faces).max() <= 1e-5 * ref_rep.max_speed
     p.fallback_iterations = 200
     rep = hip.flow_solve(p)
     assert not rep.fell_back

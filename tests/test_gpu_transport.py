@@ -293,3 +293,30 @@ def test_coupled_flow_then_transport_on_device(hip):
     assert np.abs(balance).max() <= 1e-12 * ledger[:, :, 1].max()
     assert np.all(levels >= 0) and np.all(levels <= 1.0 + 1e-12)
     assert np.all(levels[:, w.obstacles.reshape(-1) != 0] == 0)
+
+
+def test_transport_over_the_device_field_refused_after_another_solve(hip):
+    """A transport reading the flow field in place is refused once a later
+    solve (same grid, ten times the speed) has rewritten that field: its
+    explicit bound was taken from the old one."""
+    from microsimulator_amd.workloads import channel_trap
+    w = channel_trap(128, 64, seed=3)
+    g = msa.Grid(w.nx, w.ny, 1, w.dx, w.dy, w.dz, w.obstacles)
+    hip.flow_prepare(g, w.mobility, 0)
+    params = msa.DepthAveragedFlowParameters(mean_inlet_speed=10.0, axis=0,
+                                             relative_tolerance=1e-8,
+                                             solver=msa.SOLVER_MULTIGRID)
+    rep = hip.flow_solve(params)
+    bv = np.zeros(6, np.float32)
+    bv[0] = 1.0
+    spec = msa.SignalGridSpec(g, [500.0], bv)
+    hip.adr_upload(spec, np.zeros(g.sites, np.float32), precision=32, use_flow_field=True)
+    dt = 0.5 / (2 * 500.0 * 2 + 2 * rep.max_speed * 2)
+    hip.adr_run(dt, 2, ledger=False)
+    params.mean_inlet_speed = 100.0
+    hip.flow_solve(params)
+    with pytest.raises(msa.InvalidArgument, match="no longer the transport's"):
+        hip.adr_run(dt, 1, ledger=False)
+    # A fresh upload over the new field runs.
+    hip.adr_upload(spec, np.zeros(g.sites, np.float32), precision=32, use_flow_field=True)
+    hip.adr_run(dt / 10, 1, ledger=False)
