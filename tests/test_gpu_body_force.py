@@ -23,6 +23,11 @@ pytestmark = pytest.mark.gpu
 
 
 def case(seed, axis, nx=96, ny=61):
+    """A random sheet whose fluid is one connected region. A pocket of fluid
+    cut off from the boundary makes the forced system only semidefinite
+    there (its right-hand side is the body flux's divergence, not zero), and
+    Jacobi-PCG then breaks down below a few 1e-8, the reference's as well."""
+    from scipy import ndimage
     if axis == 1:
         nx, ny = ny, nx
     solid, mobility = random_case(seed, nx, ny)
@@ -31,6 +36,12 @@ def case(seed, axis, nx=96, ny=61):
         s[0, :] = s[-1, :] = 0
     else:
         s[:, 0] = s[:, -1] = 0
+    labels, count = ndimage.label(s == 0)
+    largest = np.argmax(np.bincount(labels.reshape(-1))[1:]) + 1
+    s[labels != largest] = 1
+    mobility = mobility.reshape(nx, ny)
+    mobility[s != 0] = 0.0
+    mobility = mobility.reshape(-1)
     kinds = (FIXED, FIXED, NO_FLUX, NO_FLUX, NO_FLUX, NO_FLUX) if axis == 0 else \
         (NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX)
     obstacles = s.reshape(-1)
