@@ -330,6 +330,22 @@ typedef struct {
 int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, float dt,
                                  float* levels, msim_signal_report* report);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): msim_hip_advance_signal_grid's forward-Euler step, with the same
+ * arguments, checks, messages and results, that allocates nothing. The spec's
+ * arrays, both level buffers and every partial sum live in one static device
+ * array of msim_hip_static_scratch_bytes() bytes (64 MiB) that each device
+ * holds until the process ends; a grid that needs more is refused with
+ * MSIM_INVALID_ARGUMENT. Forward Euler only, no boundary patches (both
+ * MSIM_UNSUPPORTED). Every call on one device shares that array and rewrites
+ * what it reads: calls from one process are serialised per device, but calls
+ * that overlap on one device from outside that rule (another copy of the
+ * library in the same process) race. */
+int msim_hip_advance_signal_grid_static(msim_ctx* ctx, const msim_signal_grid* grid, float dt,
+                                        float* levels, msim_signal_report* report);
+size_t msim_hip_static_scratch_bytes(void);
+
 /* Device-resident transport for many coupled steps.
  *  adr_upload   -- spec arrays and initial levels (float when precision is
  *                  32, double when 64). With use_flow_field != 0 the velocity

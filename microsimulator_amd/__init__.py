@@ -220,6 +220,7 @@ EXPORTS = (
     "msim_hip_advance_coupled", "msim_hip_coupled_upload", "msim_hip_coupled_run",
     "msim_hip_coupled_download", "msim_hip_coupled_seconds", "msim_hip_colony_volume_fraction",
     "msim_hip_colony_mobility", "msim_hip_colony_seconds",
+    "msim_hip_advance_signal_grid_static", "msim_hip_static_scratch_bytes",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -282,6 +283,8 @@ def library(path: str = LIBRARY):
         "msim_hip_colony_volume_fraction": (I, [P, P, P, P, D, P]),
         "msim_hip_colony_mobility": (I, [P, P, P, P, P, U, D, D, D, P]),
         "msim_hip_colony_seconds": (D, [P]),
+        "msim_hip_advance_signal_grid_static": (I, [P, P, C.c_float, P, P]),
+        "msim_hip_static_scratch_bytes": (C.c_size_t, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -823,6 +826,21 @@ class HipBackend:
         rep = _SignalReport()
         self._check(self.lib.msim_hip_advance_signal_grid(self.ctx, C.byref(s), dt,
                                                          lv.ctypes.data, C.byref(rep)))
+        return lv, SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
+
+    def advance_signal_grid_static(self, spec: SignalGridSpec, levels, dt: float):
+        """advance_signal_grid's forward-Euler step with no device allocation:
+        everything it stages lives in the library's static device scratch
+        (msim_hip_static_scratch_bytes() per device). Returns (new levels,
+        SignalSolveReport); `levels` is not modified."""
+        keep: list = []
+        s = spec._c(keep)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        if lv.size != spec.signal_count * spec.grid.sites:
+            raise InvalidArgument("signal grid level count does not match its specification")
+        rep = _SignalReport()
+        self._check(self.lib.msim_hip_advance_signal_grid_static(self.ctx, C.byref(s), dt,
+                                                                lv.ctypes.data, C.byref(rep)))
         return lv, SignalSolveReport(bool(rep.converged), rep.iterations, rep.residual_rms)
 
     def advance_coupled(self, spec: SignalGridSpec, levels, cells, species, plan,

@@ -19,6 +19,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "adr_site.hpp"
@@ -961,6 +963,204 @@ void adr_download(Context& ctx, void* levels) {
   const uint8_t* src = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
   check(hipMemcpyAsync(levels, src, bytes, hipMemcpyDeviceToHost, ctx.stream), "download levels");
   check(hipStreamSynchronize(ctx.stream), "download levels");
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free step (msim_hip_advance_signal_grid_static)
+// ---------------------------------------------------------------------------
+// Everything one FP32 forward-Euler step needs on the device besides the
+// caller's host arrays lives in one file-scope array: the staged mask, faces,
+// occupancy and reaction rates, both level buffers, the ledger and survey
+// partials, the flags and the survey's state. Each device holds its own copy
+// of the array for the life of the process; host code reaches it only through
+// the address hipGetSymbolAddress gives on that device (the array's name is a
+// host-side stand-in, not a device address).
+
+namespace {
+
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+constexpr int kStaticDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+void* g_scratch[kStaticDevices] = {};
+std::mutex g_scratch_lock[kStaticDevices];
+
+// Carves 256-byte aligned regions out of the scratch array, in order.
+struct ScratchLayout {
+  size_t used = 0;
+  size_t take(size_t bytes) {
+    const size_t at = used;
+    used += (bytes + 255) & ~size_t(255);
+    return at;
+  }
+};
+
+__global__ void k_fill_bytes(uint8_t* __restrict__ out, size_t count, uint8_t value) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    out[i] = value;
+  }
+}
+
+template <int S>
+void launch_static_step(hipStream_t stream, const AdrView& v, const AdrShape& sh, const float* xf,
+                        const float* yf, const uint8_t* src, uint8_t* dst, float dt,
+                        LedgerParts L, double* largest) {
+  if (v.P->limited != 0) {
+    launch_typed<float, S, true>(stream, v, sh, xf, yf, src, dst, dt, 0, L, largest);
+  } else {
+    launch_typed<float, S, false>(stream, v, sh, xf, yf, src, dst, dt, 0, L, largest);
+  }
+}
+
+}  // namespace
+
+size_t static_scratch_bytes() { return kStaticScratchBytes; }
+
+void bind_static_scratch(int device) {
+  if (device < 0 || device >= kStaticDevices) return;
+  std::lock_guard<std::mutex> hold(g_scratch_lock[device]);
+  if (g_scratch[device] != nullptr) return;
+  void* ptr = nullptr;
+  check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  g_scratch[device] = ptr;
+}
+
+void adr_step_static(Context& ctx, const msim_signal_grid& grid, float dt, float* levels) {
+  if (ctx.device < 0 || ctx.device >= kStaticDevices || g_scratch[ctx.device] == nullptr) {
+    throw Error(MSIM_DEVICE_ERROR, "the static scratch of this device is not set up");
+  }
+  if (grid.integration != MSIM_FORWARD_EULER) {
+    throw Error(MSIM_UNSUPPORTED, "the allocation-free transport step takes forward Euler only");
+  }
+  if (grid.patches != nullptr && grid.patch_count > 0) {
+    throw Error(MSIM_UNSUPPORTED,
+                "the allocation-free transport step takes no boundary patches");
+  }
+  AdrParams P = transport_params(grid);
+  const uint32_t S = P.signals;
+  const size_t n = static_cast<size_t>(P.nx) * P.ny;
+  const AdrShape base = adr_shape(P.nx, P.ny, P.limited != 0 ? 2 : 1, false);
+  AdrShape sh = base;
+  sh.blocks = step_blocks(sh);
+  const uint32_t survey_blocks = std::min<uint32_t>(site_blocks(n), 2048);
+  ScratchLayout at;
+  const size_t o_state = at.take(sizeof(CgState));
+  const size_t o_flags = at.take(2 * sizeof(uint32_t));
+  const size_t o_mass = at.take(kMaxSignals * sizeof(double));
+  const size_t o_parts = at.take(
+      std::max<size_t>(static_cast<size_t>(3 * kMaxSignals * 2 + 1) * sh.blocks,
+                       static_cast<size_t>(kMaxSignals + 1) * survey_blocks) *
+      sizeof(double));
+  const size_t o_solid = at.take(n);
+  const size_t nxf = static_cast<size_t>(P.nx + 1) * P.ny, nyf = static_cast<size_t>(P.nx) * (P.ny + 1);
+  const size_t o_xf = P.has_field != 0 ? at.take(nxf * sizeof(float)) : 0;
+  const size_t o_yf = P.has_field != 0 ? at.take(nyf * sizeof(float)) : 0;
+  const size_t o_phi = P.has_monod != 0 ? at.take(n * sizeof(float)) : 0;
+  const size_t o_loss = P.has_affine != 0 ? at.take(S * n * sizeof(float)) : 0;
+  const size_t o_source = P.has_affine != 0 ? at.take(S * n * sizeof(float)) : 0;
+  const size_t o_a = at.take(S * n * sizeof(float));
+  const size_t o_b = at.take(S * n * sizeof(float));
+  if (at.used > kStaticScratchBytes) {
+    throw Error(MSIM_INVALID_ARGUMENT,
+                "the allocation-free transport step needs " + std::to_string(at.used) +
+                    " bytes of device scratch for this grid; its limit is " +
+                    std::to_string(kStaticScratchBytes) + " bytes");
+  }
+  // One call at a time per device in this process: every call shares the
+  // device's array.
+  std::lock_guard<std::mutex> hold(g_scratch_lock[ctx.device]);
+  uint8_t* base_ptr = static_cast<uint8_t*>(g_scratch[ctx.device]);
+  auto* state = reinterpret_cast<CgState*>(base_ptr + o_state);
+  auto* flags = reinterpret_cast<uint32_t*>(base_ptr + o_flags);
+  auto* mass = reinterpret_cast<double*>(base_ptr + o_mass);
+  auto* parts = reinterpret_cast<double*>(base_ptr + o_parts);
+  uint8_t* solid = base_ptr + o_solid;
+  float* xf = P.has_field != 0 ? reinterpret_cast<float*>(base_ptr + o_xf) : nullptr;
+  float* yf = P.has_field != 0 ? reinterpret_cast<float*>(base_ptr + o_yf) : nullptr;
+  float* phi = P.has_monod != 0 ? reinterpret_cast<float*>(base_ptr + o_phi) : nullptr;
+  float* loss = P.has_affine != 0 ? reinterpret_cast<float*>(base_ptr + o_loss) : nullptr;
+  float* source = P.has_affine != 0 ? reinterpret_cast<float*>(base_ptr + o_source) : nullptr;
+  uint8_t* lv_a = base_ptr + o_a;
+  uint8_t* lv_b = base_ptr + o_b;
+  hipStream_t stream = ctx.stream;
+  // The array keeps the last call's contents: the state (with its tickets),
+  // the flags and every staged array are written afresh.
+  const CgState zero_state{};
+  const uint32_t init_flags[2] = {0u, 0xffffffffu};
+  check(hipMemcpyAsync(state, &zero_state, sizeof(CgState), hipMemcpyHostToDevice, stream),
+        "clear state");
+  check(hipMemcpyAsync(flags, init_flags, sizeof(init_flags), hipMemcpyHostToDevice, stream),
+        "clear flags");
+  if (grid.grid.obstacles != nullptr) {
+    check(hipMemcpyAsync(solid, grid.grid.obstacles, n, hipMemcpyHostToDevice, stream),
+          "upload mask");
+  } else {
+    k_fill_bytes<<<std::min<uint32_t>(site_blocks(n), 1024), kAdrBlock, 0, stream>>>(solid, n, 0);
+    check(hipGetLastError(), "launch k_fill_bytes");
+  }
+  if (P.has_field != 0) {
+    check(hipMemcpyAsync(xf, grid.x_faces, nxf * sizeof(float), hipMemcpyHostToDevice, stream),
+          "upload x faces");
+    check(hipMemcpyAsync(yf, grid.y_faces, nyf * sizeof(float), hipMemcpyHostToDevice, stream),
+          "upload y faces");
+  }
+  if (P.has_monod != 0) {
+    check(hipMemcpyAsync(phi, grid.phi, n * sizeof(float), hipMemcpyHostToDevice, stream),
+          "upload phi");
+  }
+  if (P.has_affine != 0) {
+    check(hipMemcpyAsync(loss, grid.loss_rates, S * n * sizeof(float), hipMemcpyHostToDevice,
+                         stream),
+          "upload loss");
+    check(hipMemcpyAsync(source, grid.source_rates, S * n * sizeof(float), hipMemcpyHostToDevice,
+                         stream),
+          "upload source");
+  }
+  check(hipMemcpyAsync(lv_a, levels, S * n * sizeof(float), hipMemcpyHostToDevice, stream),
+        "upload levels");
+  // The greatest outflow for the explicit bound (transport_cache).
+  k_adr_survey<float><<<survey_blocks, kAdrBlock, 0, stream>>>(
+      P, solid, xf, yf, reinterpret_cast<const float*>(lv_a), parts, state, mass);
+  check(hipGetLastError(), "launch k_adr_survey");
+  CgState host{};
+  check(hipMemcpyAsync(&host, state, sizeof(CgState), hipMemcpyDeviceToHost, stream),
+        "read survey");
+  check(hipStreamSynchronize(stream), "survey");
+  double max_loss[kMaxSignals] = {};
+  max_losses(grid, max_loss);
+  validate_explicit_bound(P, MSIM_FORWARD_EULER, P.has_field != 0 ? host.peak : 0.0, max_loss,
+                          dt);
+  if (dt == 0.0f) return;
+  const AdrView v{&P, solid, phi, source, loss, flags};
+  LedgerParts L{parts, nullptr, sh.blocks};
+  double* largest = parts + static_cast<size_t>(3 * kMaxSignals * 2) * sh.blocks;
+  switch (S) {
+    case 1: launch_static_step<1>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    case 2: launch_static_step<2>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    case 3: launch_static_step<3>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    case 4: launch_static_step<4>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    case 5: launch_static_step<5>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    case 6: launch_static_step<6>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    case 7: launch_static_step<7>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+    default: launch_static_step<8>(stream, v, sh, xf, yf, lv_a, lv_b, dt, L, largest); break;
+  }
+  check(hipGetLastError(), "launch transport step");
+  uint32_t result[2] = {0, 0};
+  check(hipMemcpyAsync(result, flags, sizeof(result), hipMemcpyDeviceToHost, stream),
+        "read flags");
+  check(hipStreamSynchronize(stream), "transport step");
+  if ((result[0] & kFlagNegative) != 0) {
+    throw Error(MSIM_DOMAIN_ERROR,
+                "signal grid update produced a non-finite or negative concentration (step 0)");
+  }
+  if (result[0] != 0) {
+    throw Error(MSIM_INVALID_ARGUMENT, "signal grid levels must be finite and non-negative (step 0)");
+  }
+  check(hipMemcpyAsync(levels, lv_b, S * n * sizeof(float), hipMemcpyDeviceToHost, stream),
+        "download levels");
+  check(hipStreamSynchronize(stream), "download levels");
 }
 
 }  // namespace msim

@@ -1057,6 +1057,13 @@ int msim_hip_create(int device, msim_ctx** out) {
     msim_hip_destroy(ctx);
     return MSIM_DEVICE_ERROR;
   }
+  // The allocation-free entry point's scratch address on this device.
+  try {
+    msim::bind_static_scratch(device);
+  } catch (const Error& e) {
+    msim_hip_destroy(ctx);
+    return e.code();
+  }
   hipDeviceProp_t prop{};
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     ctx->impl.name = prop.name;
@@ -1409,6 +1416,19 @@ int msim_hip_advance_signal_grid(msim_ctx* ctx, const msim_signal_grid* grid, fl
     msim::adr_download(c, levels);
   });
 }
+
+int msim_hip_advance_signal_grid_static(msim_ctx* ctx, const msim_signal_grid* grid, float dt,
+                                        float* levels, msim_signal_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || levels == nullptr) msim::invalid("grid and levels must not be null");
+    msim::validate_signal_grid(*grid, false);
+    msim::validate_levels(*grid, levels);
+    if (report != nullptr) *report = msim_signal_report{1, 0, 0.0f};
+    msim::adr_step_static(c, *grid, dt, levels);
+  });
+}
+
+size_t msim_hip_static_scratch_bytes(void) { return msim::static_scratch_bytes(); }
 
 int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void* levels,
                         uint32_t precision, uint32_t use_flow_field) {

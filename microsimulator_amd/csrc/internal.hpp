@@ -487,6 +487,14 @@ void launch_settle_f32(Context& ctx, float* levels, size_t count, const double* 
 void fe_candidate_f32(Context& ctx, float dt, const float* in, float* out, uint32_t step,
                       const float* xf, const float* yf);
 
+// The allocation-free FP32 forward-Euler step: bind_static_scratch takes the
+// device address of adr.hip's static scratch array on `device` (which must be
+// current) once per device; adr_step_static validates the explicit bound and
+// advances `levels` (host, S*N) one step with everything staged in that array.
+void bind_static_scratch(int device);
+size_t static_scratch_bytes();
+void adr_step_static(Context& ctx, const msim_signal_grid& grid, float dt, float* levels);
+
 // capi.cpp: the largest loss rate per signal of a spec, and the explicit
 // bound (SignalGrid::validate_step, signals.cpp:1066-1102).
 void max_losses(const msim_signal_grid& s, double* loss_out);
