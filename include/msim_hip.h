@@ -297,6 +297,10 @@ typedef struct {
   uint32_t solver_max_iterations;
   float solver_absolute_tolerance;
   float solver_relative_tolerance;
+  /* SignalGridSpec::advection (signals.hpp:128): a constant velocity (x, y, z)
+   * per signal, 3*signal_count floats, or NULL for none. Zero with a velocity
+   * field. */
+  const float* advection;
 } msim_signal_grid;
 
 /* SignalIntegrationKind (signals.hpp:33-37). The implicit kinds solve each

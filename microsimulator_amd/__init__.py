@@ -145,7 +145,7 @@ class _SignalGrid(C.Structure):
                 ("advection_scheme", C.c_uint32), ("patches", C.c_void_p),
                 ("patch_count", C.c_uint32), ("integration", C.c_uint32),
                 ("solver_max_iterations", C.c_uint32), ("solver_absolute_tolerance", C.c_float),
-                ("solver_relative_tolerance", C.c_float)]
+                ("solver_relative_tolerance", C.c_float), ("advection", C.c_void_p)]
 
 
 class _TrainSet(C.Structure):
@@ -527,6 +527,8 @@ class SignalGridSpec:
     solver_max_iterations: int = 10_000
     solver_absolute_tolerance: float = 1.0e-7
     solver_relative_tolerance: float = 1.0e-5
+    # SignalGridSpec::advection: a constant velocity (x, y, z) per signal.
+    advection: np.ndarray | None = None
 
     @property
     def signal_count(self) -> int:
@@ -555,6 +557,10 @@ class SignalGridSpec:
                 if arrays[name].size != count:
                     raise InvalidArgument(
                         "signal grid velocity field must cover every lattice face")
+        if self.advection is not None:
+            arrays["advection"] = np.asarray(self.advection, np.float32).reshape(-1)
+            if arrays["advection"].size != 3 * S:
+                raise InvalidArgument("signal grid transport arrays must match signal count")
         if (self.loss_rates is None) != (self.source_rates is None):
             raise InvalidArgument(
                 "signal grid affine reaction arrays must match the grid level count")

@@ -563,6 +563,7 @@ AdrParams transport_params(const msim_signal_grid& grid) {
   std::memset(&P, 0, sizeof(P));
   P.nx = grid.grid.nx;
   P.ny = grid.grid.ny;
+  P.nz = grid.grid.nz;
   P.signals = grid.signal_count;
   P.dx = grid.grid.dx;
   P.dy = grid.grid.dy;
@@ -576,6 +577,8 @@ AdrParams transport_params(const msim_signal_grid& grid) {
       P.vmax[s] = grid.monod_vmax[s];
       P.monod_k[s] = grid.monod_k[s];
     }
+    for (int a = 0; a < 3; ++a)
+      P.advection[s][a] = grid.advection != nullptr ? grid.advection[3 * s + a] : 0.0f;
   }
   P.has_field = grid.x_faces != nullptr ? 1 : 0;
   P.limited = grid.advection_scheme == MSIM_VAN_LEER ? 1 : 0;
@@ -590,11 +593,12 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   ctx.coupled.uploaded = false;
   auto& a = ctx.adr;
   auto& P = a.params;
-  const size_t n = static_cast<size_t>(grid.grid.nx) * grid.grid.ny;
+  const uint32_t nz = grid.grid.nz;
+  const size_t n = static_cast<size_t>(grid.grid.nx) * grid.grid.ny * nz;
   const uint32_t S = grid.signal_count;
   const bool same_spec = a.uploaded && grid.revision != 0 && grid.revision == a.revision &&
-                         P.nx == grid.grid.nx && P.ny == grid.grid.ny && P.signals == S &&
-                         !use_flow_field && !a.flow_field;
+                         P.nx == grid.grid.nx && P.ny == grid.grid.ny && P.nz == nz &&
+                         P.signals == S && !use_flow_field && !a.flow_field;
   P = transport_params(grid);
   a.integration = grid.integration;
   a.max_iterations = grid.solver_max_iterations;
@@ -621,7 +625,41 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
       static_cast<size_t>(kMaxSignals + 1) * 2048));
   if (!same_spec) {
     a.has_edge = patched;
-    if (patched) {
+    if (patched && nz > 1) {
+      // GridBoundary::values_at (signals.cpp:439-447) for every boundary site
+      // of the six faces, in each face's (first, second) order: the last
+      // patch holding the site, else the face's value.
+      const uint32_t extent[6][2] = {{P.ny, nz}, {P.ny, nz}, {P.nx, nz},
+                                     {P.nx, nz}, {P.nx, P.ny}, {P.nx, P.ny}};
+      size_t offset[7] = {0};
+      for (int f = 0; f < 6; ++f)
+        offset[f + 1] = offset[f] + static_cast<size_t>(extent[f][0]) * extent[f][1];
+      const size_t row = offset[6];
+      std::vector<float> edge(S * row);
+      for (uint32_t s = 0; s < S; ++s) {
+        for (int f = 0; f < 6; ++f) {
+          for (uint32_t first = 0; first < extent[f][0]; ++first) {
+            for (uint32_t second = 0; second < extent[f][1]; ++second) {
+              float v = P.boundary[s][f];
+              for (uint32_t k = grid.patch_count; k-- > 0;) {
+                const msim_boundary_patch& pt = grid.patches[k];
+                if (static_cast<int>(pt.face) == f && first >= pt.begin[0] && first < pt.end[0] &&
+                    second >= pt.begin[1] && second < pt.end[1]) {
+                  v = pt.values[s];
+                  break;
+                }
+              }
+              edge[s * row + offset[f] + static_cast<size_t>(first) * extent[f][1] + second] = v;
+            }
+          }
+        }
+      }
+      a.edge.reserve(edge.size());
+      check(hipMemcpyAsync(a.edge.data(), edge.data(), edge.size() * sizeof(float),
+                           hipMemcpyHostToDevice, ctx.stream),
+            "upload boundary patches");
+      check(hipStreamSynchronize(ctx.stream), "upload boundary patches");
+    } else if (patched) {
       // GridBoundary::values_at (signals.cpp:439-447) for every boundary site
       // of the four in-plane faces (second in-plane index z = 0): the last
       // patch holding the site, else the face's value.
@@ -664,14 +702,20 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
       a.xf.release();
       a.yf.release();
     } else if (grid.x_faces != nullptr) {
-      a.xf.reserve(static_cast<size_t>(P.nx + 1) * P.ny);
-      a.yf.reserve(static_cast<size_t>(P.nx) * (P.ny + 1));
+      a.xf.reserve(static_cast<size_t>(P.nx + 1) * P.ny * nz);
+      a.yf.reserve(static_cast<size_t>(P.nx) * (P.ny + 1) * nz);
       check(hipMemcpyAsync(a.xf.data(), grid.x_faces, a.xf.size() * sizeof(float),
                            hipMemcpyHostToDevice, ctx.stream),
             "upload x faces");
       check(hipMemcpyAsync(a.yf.data(), grid.y_faces, a.yf.size() * sizeof(float),
                            hipMemcpyHostToDevice, ctx.stream),
             "upload y faces");
+      if (nz > 1) {
+        a.zf.reserve(static_cast<size_t>(P.nx) * P.ny * (nz + 1));
+        check(hipMemcpyAsync(a.zf.data(), grid.z_faces, a.zf.size() * sizeof(float),
+                             hipMemcpyHostToDevice, ctx.stream),
+              "upload z faces");
+      }
     }
     if (grid.phi != nullptr) {
       a.phi.reserve(n);
@@ -700,7 +744,10 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
   const float* xf = use_flow_field ? ctx.flow.faces.data() : a.xf.data();
   const float* yf = use_flow_field ? ctx.flow.faces.data() + ctx.flow.x_faces() : a.yf.data();
   const uint32_t blocks = std::min<uint32_t>(site_blocks(n), 2048);
-  if (precision == 64) {
+  if (nz > 1) {
+    adr3_survey(ctx, xf, yf, a.zf.data(), reinterpret_cast<const float*>(a.levels_a.data()),
+                a.state.data(), a.partials.data(), a.mass.data());
+  } else if (precision == 64) {
     k_adr_survey<double><<<blocks, kAdrBlock, 0, ctx.stream>>>(
         P, a.solid.data(), xf, yf, reinterpret_cast<const double*>(a.levels_a.data()),
         a.partials.data(), a.state.data(), a.mass.data());
@@ -875,10 +922,15 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
   check(hipMemcpyAsync(a.flags.data(), init_flags, sizeof(init_flags), hipMemcpyHostToDevice,
                        ctx.stream),
         "clear flags");
-  if (a.integration != MSIM_FORWARD_EULER) {
-    // Implicit steps: FP32 levels, each step's solve judged on the host.
+  if (a.integration != MSIM_FORWARD_EULER || P.nz > 1) {
+    // Implicit steps (FP32 levels, each step's solve judged on the host),
+    // and every step on a grid of several layers.
     check(hipEventRecord(ctx.start, ctx.stream), "event");
-    implicit_run(ctx, dt, steps, xf, yf);
+    if (a.integration != MSIM_FORWARD_EULER) {
+      implicit_run(ctx, dt, steps, xf, yf);
+    } else {
+      adr3_run(ctx, dt, steps, xf, yf, a.zf.data());
+    }
     check(hipEventRecord(ctx.stop, ctx.stream), "event");
     check(hipEventSynchronize(ctx.stop), "event sync");
     float ms = 0.0f;
@@ -1037,6 +1089,9 @@ void adr_step_static(Context& ctx, const msim_signal_grid& grid, float dt, float
   if (grid.patches != nullptr && grid.patch_count > 0) {
     throw Error(MSIM_UNSUPPORTED,
                 "the allocation-free transport step takes no boundary patches");
+  }
+  if (grid.grid.nz != 1) {
+    throw Error(MSIM_UNSUPPORTED, "the allocation-free transport step takes a grid of one layer");
   }
   AdrParams P = transport_params(grid);
   const uint32_t S = P.signals;

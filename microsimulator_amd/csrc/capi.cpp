@@ -64,7 +64,7 @@ void validate_lattice(const msim_grid& g) {
   }
   for (int a = 0; a < 3; ++a) {
     if ((g.boundary[2 * a] == MSIM_PERIODIC) != (g.boundary[2 * a + 1] == MSIM_PERIODIC)) {
-      invalid("signal grid periodic boundaries must be paired");
+      invalid(std::string("signal grid periodic ") + "xyz"[a] + " boundaries must be paired");
     }
   }
   if (g.obstacles != nullptr) {
@@ -526,9 +526,6 @@ void validate_flow_params(const msim_flow_params& p) {
 // what this backend reads, plus the Monod inputs.
 void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
   validate_lattice(s.grid);
-  if (s.grid.nz != 1) {
-    throw Error(MSIM_UNSUPPORTED, "the HIP transport step takes a grid of one layer (nz = 1)");
-  }
   if (s.signal_count == 0) invalid("signal grid must contain at least one signal");
   if (s.integration > MSIM_BACKWARD_EULER) {
     invalid("unknown signal integration kind");  // signals.cpp:718-725
@@ -551,6 +548,13 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
     if (!std::isfinite(s.diffusion[k]) || s.diffusion[k] < 0.0f) {
       invalid("signal diffusion must be finite and non-negative");
     }
+  }
+  if (s.advection != nullptr) {
+    for (uint32_t k = 0; k < 3 * s.signal_count; ++k) {
+      if (!std::isfinite(s.advection[k])) invalid("signal advection must be finite");
+    }
+  }
+  for (uint32_t k = 0; k < s.signal_count; ++k) {
     for (int f = 0; f < 6; ++f) {
       if (s.grid.boundary[f] != MSIM_FIXED) continue;
       const float v = s.boundary_values != nullptr ? s.boundary_values[6 * k + f] : 0.0f;
@@ -584,7 +588,8 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
       invalid("signal grid boundary patches must lie on their face");
     }
   }
-  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
+  const uint32_t nx = s.grid.nx, ny = s.grid.ny, nz = s.grid.nz;
+  const size_t n = static_cast<size_t>(nx) * ny * nz;
   auto solid = [&](size_t i) { return s.grid.obstacles != nullptr && s.grid.obstacles[i] != 0; };
   // SignalGridAffineReaction::validate (signals.cpp:493-497): both arrays or
   // neither.
@@ -619,35 +624,102 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
       }
     }
   }
+  const bool field = device_field || s.x_faces != nullptr;
+  if (field && s.advection != nullptr) {
+    for (uint32_t k = 0; k < 3 * s.signal_count; ++k) {
+      if (s.advection[k] != 0.0f) {
+        invalid("signal grid velocity field requires zero constant advection vectors");
+      }
+    }
+  }
   if (!device_field && s.x_faces != nullptr) {
-    if (s.y_faces == nullptr) invalid("signal grid velocity field must cover every lattice face");
-    const uint32_t nx = s.grid.nx, ny = s.grid.ny;
-    const size_t nxf = static_cast<size_t>(nx + 1) * ny, nyf = static_cast<size_t>(nx) * (ny + 1);
+    if (s.y_faces == nullptr || (nz > 1 && s.z_faces == nullptr)) {
+      invalid("signal grid velocity field must cover every lattice face");
+    }
+    const size_t nxf = static_cast<size_t>(nx + 1) * ny * nz;
+    const size_t nyf = static_cast<size_t>(nx) * (ny + 1) * nz;
+    const size_t nzf = static_cast<size_t>(nx) * ny * (nz + 1);
     for (size_t i = 0; i < nxf; ++i) {
       if (!std::isfinite(s.x_faces[i])) invalid("signal grid velocity field values must be finite");
     }
     for (size_t i = 0; i < nyf; ++i) {
       if (!std::isfinite(s.y_faces[i])) invalid("signal grid velocity field values must be finite");
     }
-    // Closed faces carry no velocity (signals.cpp:654-676).
-    for (uint32_t x = 0; x < nx; ++x) {
-      for (uint32_t y = 0; y < ny; ++y) {
-        const size_t i = static_cast<size_t>(x) * ny + y;
-        const bool here = solid(i);
-        const bool clo_x = x == 0 ? s.grid.boundary[0] == MSIM_NO_FLUX : solid(i - ny);
-        const bool cup_x = x + 1 == nx ? s.grid.boundary[1] == MSIM_NO_FLUX : solid(i + ny);
-        const bool clo_y = y == 0 ? s.grid.boundary[2] == MSIM_NO_FLUX : solid(i - 1);
-        const bool cup_y = y + 1 == ny ? s.grid.boundary[3] == MSIM_NO_FLUX : solid(i + 1);
-        const size_t yf = static_cast<size_t>(x) * (ny + 1) + y;
-        if (nx > 1 && (((clo_x || here) && s.x_faces[i] != 0.0f) ||
-                       ((cup_x || here) && s.x_faces[i + ny] != 0.0f))) {
-          invalid("signal grid velocity field must be zero on closed faces");
-        }
-        if (ny > 1 && (((clo_y || here) && s.y_faces[yf] != 0.0f) ||
-                       ((cup_y || here) && s.y_faces[yf + 1] != 0.0f))) {
-          invalid("signal grid velocity field must be zero on closed faces");
+    if (s.z_faces != nullptr) {
+      for (size_t i = 0; i < nzf; ++i) {
+        if (!std::isfinite(s.z_faces[i])) {
+          invalid("signal grid velocity field values must be finite");
         }
       }
+    }
+    auto xface = [&](uint32_t fx, uint32_t y, uint32_t z) {
+      return s.x_faces[(static_cast<size_t>(fx) * ny + y) * nz + z];
+    };
+    auto yface = [&](uint32_t x, uint32_t fy, uint32_t z) {
+      return s.y_faces[(static_cast<size_t>(x) * (ny + 1) + fy) * nz + z];
+    };
+    auto zface = [&](uint32_t x, uint32_t y, uint32_t fz) {
+      return s.z_faces != nullptr ? s.z_faces[(static_cast<size_t>(x) * ny + y) * (nz + 1) + fz]
+                                  : 0.0f;
+    };
+    auto site = [&](uint32_t x, uint32_t y, uint32_t z) {
+      return (static_cast<size_t>(x) * ny + y) * nz + z;
+    };
+    // Closed faces carry no velocity (signals.cpp:654-676).
+    const uint32_t dims[3] = {nx, ny, nz};
+    for (uint32_t x = 0; x < nx; ++x) {
+      for (uint32_t y = 0; y < ny; ++y) {
+        for (uint32_t z = 0; z < nz; ++z) {
+          const uint32_t c[3] = {x, y, z};
+          const bool here = solid(site(x, y, z));
+          for (int a = 0; a < 3; ++a) {
+            if (dims[a] == 1) continue;
+            uint32_t lo[3] = {x, y, z}, hi[3] = {x, y, z};
+            bool clo, cup;
+            if (c[a] == 0) {
+              lo[a] = dims[a] - 1;
+              clo = s.grid.boundary[2 * a] == MSIM_NO_FLUX ||
+                    (s.grid.boundary[2 * a] == MSIM_PERIODIC && solid(site(lo[0], lo[1], lo[2])));
+            } else {
+              lo[a] = c[a] - 1;
+              clo = solid(site(lo[0], lo[1], lo[2]));
+            }
+            if (c[a] + 1 == dims[a]) {
+              hi[a] = 0;
+              cup = s.grid.boundary[2 * a + 1] == MSIM_NO_FLUX ||
+                    (s.grid.boundary[2 * a + 1] == MSIM_PERIODIC && solid(site(hi[0], hi[1], hi[2])));
+            } else {
+              hi[a] = c[a] + 1;
+              cup = solid(site(hi[0], hi[1], hi[2]));
+            }
+            const float lower = a == 0 ? xface(x, y, z) : a == 1 ? yface(x, y, z) : zface(x, y, z);
+            const float upper =
+                a == 0 ? xface(x + 1, y, z) : a == 1 ? yface(x, y + 1, z) : zface(x, y, z + 1);
+            if (((clo || here) && lower != 0.0f) || ((cup || here) && upper != 0.0f)) {
+              invalid("signal grid velocity field must be zero on closed faces");
+            }
+          }
+        }
+      }
+    }
+    // The two sides of a periodic boundary are one face (signals.cpp:678-713).
+    if (s.grid.boundary[0] == MSIM_PERIODIC && nx > 1) {
+      for (uint32_t y = 0; y < ny; ++y)
+        for (uint32_t z = 0; z < nz; ++z)
+          if (xface(0, y, z) != xface(nx, y, z))
+            invalid("signal grid velocity field periodic faces must hold equal values");
+    }
+    if (s.grid.boundary[2] == MSIM_PERIODIC && ny > 1) {
+      for (uint32_t x = 0; x < nx; ++x)
+        for (uint32_t z = 0; z < nz; ++z)
+          if (yface(x, 0, z) != yface(x, ny, z))
+            invalid("signal grid velocity field periodic faces must hold equal values");
+    }
+    if (s.grid.boundary[4] == MSIM_PERIODIC && nz > 1) {
+      for (uint32_t x = 0; x < nx; ++x)
+        for (uint32_t y = 0; y < ny; ++y)
+          if (zface(x, y, 0) != zface(x, y, nz))
+            invalid("signal grid velocity field periodic faces must hold equal values");
     }
   }
 }
@@ -655,7 +727,7 @@ void validate_signal_grid(const msim_signal_grid& s, bool device_field) {
 // validate_signal_levels (signals.cpp:754-784).
 template <typename T>
 void validate_levels(const msim_signal_grid& s, const T* levels) {
-  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
+  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny * s.grid.nz;
   int fault = 0;
   for (size_t i = 0; i < s.signal_count * n; ++i) {
     const T v = levels[i];
@@ -674,7 +746,7 @@ void validate_levels(const msim_signal_grid& s, const T* levels) {
 // Largest loss rate per signal, for the explicit bound: the affine loss's
 // maximum (max_reaction_loss, signals.cpp:322-330) plus vmax*max(phi)/K.
 void max_losses(const msim_signal_grid& s, double* loss_out) {
-  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny;
+  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny * s.grid.nz;
   float phi_max = 0.0f;
   if (s.phi != nullptr) {
     for (size_t i = 0; i < n; ++i) phi_max = std::max(phi_max, s.phi[i]);
@@ -697,12 +769,12 @@ void validate_explicit_bound(const AdrParams& P, uint32_t integration, double gr
   if (!std::isfinite(dt) || dt < 0.0f) invalid("time step must be finite and non-negative");
   // The explicit bound binds forward Euler only (signals.cpp:1070-1072).
   if (integration != MSIM_FORWARD_EULER) return;
-  const uint32_t dims[2] = {P.nx, P.ny};
-  const float spacing[2] = {P.dx, P.dy};
+  const uint32_t dims[3] = {P.nx, P.ny, P.nz == 0 ? 1u : P.nz};
+  const float spacing[3] = {P.dx, P.dy, P.dz};
   for (uint32_t k = 0; k < P.signals; ++k) {
     double inverse_square_sum = 0.0;
     double courant_sum = greatest_outflow;
-    for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < 3; ++a) {
       if (dims[a] == 1) continue;
       const double inverse_spacing = 1.0 / static_cast<double>(spacing[a]);
       inverse_square_sum += inverse_spacing * inverse_spacing;
@@ -895,6 +967,9 @@ void validate_coupled_inputs(const msim_signal_grid& g, const float* origin, con
   }
   validate_cells(cells, plan.species_count, species);
   validate_signal_grid(g, false);
+  if (g.grid.nz != 1) {
+    throw Error(MSIM_UNSUPPORTED, "the HIP coupled step takes a grid of one layer (nz = 1)");
+  }
   validate_levels(g, levels);
 }
 
@@ -1438,10 +1513,13 @@ int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void*
     if (precision == 64 && grid->integration != MSIM_FORWARD_EULER) {
       throw Error(MSIM_UNSUPPORTED, "implicit signal steps hold FP32 levels");
     }
+    if (precision == 64 && grid->grid.nz != 1) {
+      throw Error(MSIM_UNSUPPORTED, "FP64 levels take a grid of one layer (nz = 1)");
+    }
     if (use_flow_field != 0) {
       if (!c.flow.solved) msim::invalid("no flow field on the device");
       if (c.flow.layers.nz != 1) msim::invalid("the flow field on the device is of a layered grid");
-      if (c.flow.nx != grid->grid.nx || c.flow.ny != grid->grid.ny) {
+      if (c.flow.nx != grid->grid.nx || c.flow.ny != grid->grid.ny || grid->grid.nz != 1) {
         msim::invalid("the flow field is for another grid");
       }
     }

@@ -1,5 +1,5 @@
 // Implicit (Crank-Nicolson or backward Euler) transport steps on the device,
-// one layer, FP32 levels: the reference's damped Jacobi iteration
+// FP32 levels (grids of several layers take adr3.hip's set-up and sweep): the reference's damped Jacobi iteration
 // (signal_grid_crank_nicolson_candidate, cpp/core/signals.cpp:1293-1375)
 // under its stopping rule (solve_signal_jacobi,
 // cpp/core/signal_jacobi.hpp:53-84), then the step's settle and validation
@@ -420,6 +420,10 @@ template <int S, bool kLimited, bool kPatched>
 void launch_setup(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
                   const float* sources, const float* old, float explicit_dt, float dt) {
   auto& a = ctx.adr;
+  if (a.params.nz > 1) {
+    implicit3_setup(ctx, blocks, xf, yf, a.zf.data(), sources, old, explicit_dt, dt);
+    return;
+  }
   k_implicit_setup<S, kLimited, kPatched><<<blocks, kImplicitBlock, 0, ctx.stream>>>(
       a.params, a.solid.data(), xf, yf, a.phi.data(), a.source.data(), a.loss.data(), sources,
       old, a.rhs.data(), a.lagged.data(), explicit_dt, dt, a.implicit_sums.data(),
@@ -430,6 +434,10 @@ template <int S, bool kLimited, bool kPatched>
 void launch_sweep(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
                   const float* cur, float* nxt, float half_dt, float relaxation) {
   auto& a = ctx.adr;
+  if (a.params.nz > 1) {
+    implicit3_sweep(ctx, blocks, xf, yf, a.zf.data(), cur, nxt, half_dt, relaxation);
+    return;
+  }
   const float* loss = a.params.has_monod != 0 ? a.lagged.data() : a.loss.data();
   k_implicit_sweep<S, kLimited, kPatched><<<blocks, kImplicitBlock, 0, ctx.stream>>>(
       a.params, a.solid.data(), xf, yf, a.source.data(), loss, a.rhs.data(), cur, nxt, half_dt,

@@ -228,13 +228,15 @@ struct FlowDevice {
 // Transport constants passed to the kernels by value.
 struct AdrParams {
   uint32_t nx, ny, signals;
+  uint32_t nz;  // layers: the one-layer kernels read nx, ny only
   float dx, dy, dz;
   uint8_t kind[6];
   uint8_t has_field, has_affine, has_monod;
   uint8_t limited;  // van Leer correction (SignalAdvectionKind::van_leer)
   // Reservoir values of the fixed faces site by site where the spec has
   // boundary patches (device array [S][x_lower ny | x_upper ny | y_lower nx |
-  // y_upper nx]); null: the uniform `boundary` values.
+  // y_upper nx] on one layer; on several layers every face's sites, see
+  // adr_site3.hpp); null: the uniform `boundary` values.
   const float* edge;
   float diffusion[kMaxSignals];
   float advection[kMaxSignals][3];
@@ -276,7 +278,7 @@ struct AdrDevice {
   double max_loss[kMaxSignals] = {};
   double host_mass[kMaxSignals] = {};  // mass of each signal now (FP64 runs)
   DeviceBuffer<uint8_t> solid;
-  DeviceBuffer<float> xf, yf, phi, source, loss, edge;
+  DeviceBuffer<float> xf, yf, zf, phi, source, loss, edge;
   bool has_edge = false;
   DeviceBuffer<uint8_t> levels_a, levels_b;  // S*N of float or double
   // Implicit steps: a third level buffer, the right-hand side, the lagged
@@ -290,7 +292,9 @@ struct AdrDevice {
   DeviceBuffer<uint32_t> flags;              // [0]: error bits, [1]: first failed step
   DeviceBuffer<CgState> state;
   double seconds = 0.0;
-  [[nodiscard]] std::size_t sites() const { return static_cast<std::size_t>(params.nx) * params.ny; }
+  [[nodiscard]] std::size_t sites() const {
+    return static_cast<std::size_t>(params.nx) * params.ny * (params.nz == 0 ? 1u : params.nz);
+  }
 };
 
 // The learned smoother: weights, the packed bfloat16 fragments the kernels
@@ -500,6 +504,22 @@ void adr_step_static(Context& ctx, const msim_signal_grid& grid, float dt, float
 void max_losses(const msim_signal_grid& s, double* loss_out);
 void validate_explicit_bound(const AdrParams& P, uint32_t integration, double greatest_outflow,
                              const double* max_loss, float dt);
+
+// adr3.hip: transport on a grid of several layers (FP32 levels), per site
+// with the reference's three-axis arithmetic (adr_site3.hpp). The survey of
+// an upload (greatest outflow over the fluid sites, mass per signal);
+// `steps` forward-Euler steps of the uploaded levels; and the implicit
+// step's set-up and sweep kernels for implicit.hip's iteration.
+void adr3_survey(Context& ctx, const float* xf, const float* yf, const float* zf,
+                 const float* levels, CgState* state, double* partials, double* mass);
+void adr3_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf,
+              const float* zf);
+void implicit3_setup(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
+                     const float* zf, const float* sources, const float* old, float explicit_dt,
+                     float dt);
+void implicit3_sweep(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
+                     const float* zf, const float* cur, float* nxt, float half_dt,
+                     float relaxation);
 
 // implicit.hip: `steps` implicit steps of dt on the uploaded FP32 levels.
 // sources (device, S*N, or null): rates added to each step's right-hand side as

@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libmsim_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libmsim_ref.so")
+BINDING_SO = os.path.join(HERE, "_ref", "libmsim_binding.so")
 
 NO_FLUX, PERIODIC, FIXED = 0, 1, 2
 UPWIND, VAN_LEER = 0, 1  # SignalAdvectionKind
@@ -47,6 +48,10 @@ def build(reference: bool = True) -> None:
     targets = ["all"]
     if reference and os.path.isdir(os.environ.get("REFERENCE", "/root/reference")):
         targets.append("ref")
+        # The reference-side hip backend needs the built HIP library.
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "microsimulator_amd",
+                                       "libmsim_hip.so")):
+            targets.append("binding")
     subprocess.run(["make", "-s", "-C", HERE, *targets], check=True)
 
 
@@ -456,8 +461,9 @@ class Reference:
                                            C.c_char_p, Z]
         L.ref_solve_depth_flow_forced.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, Z, P,
                                                   P, P, P, C.c_char_p, Z]
-        L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, U,
-                                              U, P, P, U, U, F, F, F, U, P, P, P, C.c_char_p, Z]
+        L.ref_advance_signal_grid.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, P,
+                                              U, U, P, P, U, U, F, F, F, U, P, P, P, C.c_char_p,
+                                              Z]
         L.ref_set_threads.argtypes = [U]
         L.ref_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, P, P, P, P, P, P,
                                     C.c_char_p, Z]
@@ -584,13 +590,13 @@ class Reference:
     def flow_system(self, grid: Grid, mobility, axis=0):
         return _RefFlow(self, grid, mobility, axis)
 
-    def advance_signal_grid(self, grid: Grid, levels, dt, steps=1, diffusion=(1.0,),
-                            boundary_values=None, faces=None, source=None, loss=None,
-                            scheme=UPWIND, patches=(), integration=FORWARD_EULER,
-                            solver=SOLVER_DEFAULTS, with_report=False):
-        """advance_signal_grid_cpu `steps` times: levels and seconds (and the
-        last step's report with with_report)."""
-        kinds = self._kinds(grid)
+    @staticmethod
+    def _signal_args(grid: Grid, diffusion=(1.0,), boundary_values=None, faces=None,
+                     source=None, loss=None, scheme=UPWIND, patches=(),
+                     integration=FORWARD_EULER, solver=SOLVER_DEFAULTS, advection=None):
+        """The spec arguments of ref_advance_signal_grid (and bind_signal_steps),
+        with the arrays they point into."""
+        kinds = Reference._kinds(grid)
         obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
         diffusion = _f32(np.atleast_1d(diffusion))
         S = diffusion.size
@@ -600,7 +606,6 @@ class Reference:
             nfx, nfy, _ = grid.face_counts()
             f = _f32(faces).reshape(-1)
             fx, fy, fz = f[:nfx].copy(), f[nfx:nfx + nfy].copy(), f[nfx + nfy:].copy()
-        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
         src = None
         if loss is not None:
             loss = _f32(loss).reshape(-1)
@@ -609,27 +614,120 @@ class Reference:
                             np.uint32).reshape(-1)
         pvalues = (np.concatenate([_f32(np.atleast_1d(v)) for *_, v in patches])
                    if len(patches) else np.zeros(1, np.float32))
+        adv = None if advection is None else _f32(advection).reshape(-1)
+        keep = (kinds, obs, diffusion, bv, fx, fy, fz, src, loss, geometry, pvalues, adv)
+        args = (
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), C.c_void_p(_ptr(obs)),
+            C.c_void_p(kinds.ctypes.data), C.c_uint32(S), C.c_void_p(diffusion.ctypes.data),
+            C.c_void_p(_ptr(adv)), C.c_void_p(bv.ctypes.data), C.c_void_p(_ptr(fx)),
+            C.c_void_p(_ptr(fy)), C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)),
+            C.c_void_p(_ptr(loss)), C.c_uint32(scheme), C.c_uint32(len(patches)),
+            C.c_void_p(_ptr(geometry) if len(patches) else None),
+            C.c_void_p(_ptr(pvalues) if len(patches) else None), C.c_uint32(integration),
+            C.c_uint32(solver[0]), C.c_float(solver[1]), C.c_float(solver[2]))
+        return args, keep
+
+    def advance_signal_grid(self, grid: Grid, levels, dt, steps=1, with_report=False, **kw):
+        """advance_signal_grid_cpu `steps` times: levels and seconds (and the
+        last step's report with with_report). kw: the spec as _signal_args
+        takes it (diffusion, boundary_values, faces, source, loss, scheme,
+        patches, integration, solver, advection (x, y, z) per signal)."""
+        args, keep = self._signal_args(grid, **kw)
+        lv = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
         secs = C.c_double()
         report = np.zeros(3, np.float32)
         err = C.create_string_buffer(512)
         st = self.lib.ref_advance_signal_grid(
-            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
-            C.c_float(grid.dy), C.c_float(grid.dz), C.c_void_p(_ptr(obs)),
-            C.c_void_p(kinds.ctypes.data), C.c_uint32(S), C.c_void_p(diffusion.ctypes.data),
-            C.c_void_p(bv.ctypes.data), C.c_void_p(_ptr(fx)), C.c_void_p(_ptr(fy)),
-            C.c_void_p(_ptr(fz)), C.c_void_p(_ptr(src)), C.c_void_p(_ptr(loss)),
-            C.c_uint32(scheme), C.c_uint32(len(patches)),
-            C.c_void_p(_ptr(geometry) if len(patches) else None),
-            C.c_void_p(_ptr(pvalues) if len(patches) else None), C.c_uint32(integration),
-            C.c_uint32(solver[0]), C.c_float(solver[1]), C.c_float(solver[2]), C.c_float(dt),
-            C.c_uint32(steps), C.c_void_p(lv.ctypes.data), C.c_void_p(report.ctypes.data),
-            C.byref(secs), err, C.c_size_t(512))
+            *args, C.c_float(dt), C.c_uint32(steps), C.c_void_p(lv.ctypes.data),
+            C.c_void_p(report.ctypes.data), C.byref(secs), err, C.c_size_t(512))
+        del keep
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
         if with_report:
             return lv, secs.value, dict(converged=bool(report[0]), iterations=int(report[1]),
                                         residual_rms=float(report[2]))
         return lv, secs.value
+
+
+class Binding:
+    """The reference-side hip backend (microsimulator_amd/binding/hip_backend.cpp)
+    compiled against the reference's headers and driven through its
+    ComputeBackend interface, beside the reference's CPU functions on the same
+    inputs (oracle/_ref/libmsim_binding.so). Test infrastructure."""
+
+    def __init__(self, path: str = BINDING_SO):
+        self.lib = C.CDLL(path)
+        L = self.lib
+        P, U, F, Z = C.c_void_p, C.c_uint32, C.c_float, C.c_size_t
+        L.bind_device_count.restype = Z
+        L.bind_create.argtypes = [C.c_char_p, Z, P, C.c_char_p, Z]
+        L.bind_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, Z, P, P, P,
+                                C.c_char_p, Z]
+        L.bind_signal_steps.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, P, U, U,
+                                        P, P, U, U, F, F, F, U, P, P, P, C.c_char_p, Z]
+
+    @staticmethod
+    def available(path: str = BINDING_SO) -> bool:
+        return os.path.exists(path)
+
+    def device_count(self) -> int:
+        return int(self.lib.bind_device_count())
+
+    def create(self):
+        """make_hip_backend(0): its info().name and supports() of signals,
+        coupled_rates, depth_averaged_flow, resolved_flow, and info().native."""
+        name = C.create_string_buffer(64)
+        features = C.c_uint32()
+        err = C.create_string_buffer(512)
+        st = self.lib.bind_create(name, 64, C.byref(features), err, 512)
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        f = features.value
+        return name.value.decode(), dict(signals=bool(f & 1), coupled_rates=bool(f & 2),
+                                         depth_averaged_flow=bool(f & 4),
+                                         resolved_flow=bool(f & 8), native=bool(f & 16))
+
+    def flow(self, grid: Grid, mobility, axis=0, mean_inlet_speed=1.0, tolerance=1e-6,
+             max_iterations=50000, body_force=None):
+        """solve_depth_averaged_flow through the hip backend and through
+        solve_depth_averaged_flow_cpu: (faces_hip, faces_cpu, reports)."""
+        kinds = Reference._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        mob = _f32(mobility)
+        total = sum(grid.face_counts())
+        a = np.zeros(total, np.float32)
+        b = np.zeros(total, np.float32)
+        report = np.zeros(6, np.float32)
+        force = None if body_force is None else _f32(body_force).reshape(-1)
+        err = C.create_string_buffer(512)
+        st = self.lib.bind_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(mob),
+            C.c_uint32(axis), C.c_float(mean_inlet_speed), C.c_float(tolerance),
+            C.c_uint32(max_iterations), _ptr(force),
+            C.c_size_t(0 if force is None else force.size // 2), a.ctypes.data, b.ctypes.data,
+            report.ctypes.data, err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        keys = ("iterations", "relative_residual", "max_speed")
+        return a, b, [dict(zip(keys, report[3 * k:3 * k + 3].tolist())) for k in range(2)]
+
+    def signal_steps(self, grid: Grid, levels, dt, steps=1, **kw):
+        """advance_signal_grid `steps` times through the hip backend and through
+        advance_signal_grid_cpu: (levels_hip, levels_cpu, reports)."""
+        args, keep = Reference._signal_args(grid, **kw)
+        a = np.array(levels, dtype=np.float32, copy=True).reshape(-1)
+        b = np.zeros_like(a)
+        report = np.zeros(6, np.float32)
+        err = C.create_string_buffer(512)
+        st = self.lib.bind_signal_steps(*args, C.c_float(dt), C.c_uint32(steps), a.ctypes.data,
+                                        b.ctypes.data, report.ctypes.data, err, C.c_size_t(512))
+        del keep
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        keys = ("converged", "iterations", "residual_rms")
+        return a, b, [dict(zip(keys, report[3 * k:3 * k + 3].tolist())) for k in range(2)]
 
 
 class _RefFlow:

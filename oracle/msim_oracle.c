@@ -84,7 +84,8 @@ static int validate_grid(const mo_grid* g, char* err, size_t errlen) {
       return fail(err, errlen, MO_INVALID_ARGUMENT, "unknown signal grid boundary kind");
   for (int a = 0; a < 3; ++a)
     if ((g->boundary[2 * a] == MO_PERIODIC) != (g->boundary[2 * a + 1] == MO_PERIODIC))
-      return fail(err, errlen, MO_INVALID_ARGUMENT, "signal grid periodic boundaries must be paired");
+      return fail(err, errlen, MO_INVALID_ARGUMENT, "signal grid periodic %c boundaries must be paired",
+                  "xyz"[a]);
   return MO_OK;
 }
 

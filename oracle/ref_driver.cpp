@@ -100,6 +100,62 @@ ms::SignalGridSpec make_spec(std::uint32_t nx, std::uint32_t ny, std::uint32_t n
   return spec;
 }
 
+// A transport spec from the flat arguments the transport entry points take:
+// patches as per patch (face, begin[0], begin[1], end[0], end[1]) with
+// `signals` values each; advection (x, y, z) per signal or null.
+ms::SignalGridSpec signal_spec(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                               float dy, float dz, const std::uint8_t* obstacles,
+                               const std::uint8_t* kinds, std::uint32_t signals,
+                               const float* diffusion, const float* advection,
+                               const float* boundary_values, const float* x_faces,
+                               const float* y_faces, const float* z_faces,
+                               const float* source_rates, const float* loss_rates,
+                               std::uint32_t scheme, std::uint32_t patch_count,
+                               const std::uint32_t* patch_geometry, const float* patch_values,
+                               std::uint32_t integration, std::uint32_t max_iterations,
+                               float absolute_tolerance, float relative_tolerance) {
+  auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
+  // Reservoir patches: per patch (face, begin[0], begin[1], end[0], end[1])
+  // and `signals` values.
+  ms::GridBoundary* faces[6] = {&spec.x_lower, &spec.x_upper, &spec.y_lower,
+                                &spec.y_upper, &spec.z_lower, &spec.z_upper};
+  for (std::uint32_t k = 0; k < patch_count; ++k) {
+    const std::uint32_t* g = patch_geometry + 5 * k;
+    ms::GridBoundaryPatch patch;
+    patch.region.begin = {g[1], g[2]};
+    patch.region.end = {g[3], g[4]};
+    patch.values.assign(patch_values + k * signals, patch_values + (k + 1) * signals);
+    faces[g[0]]->patches.push_back(std::move(patch));
+  }
+  const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+  spec.diffusion.assign(diffusion, diffusion + signals);
+  if (advection != nullptr) {
+    for (std::uint32_t k = 0; k < signals; ++k) {
+      spec.advection[k] = ms::Vec3{advection[3 * k], advection[3 * k + 1], advection[3 * k + 2]};
+    }
+  }
+  spec.advection_scheme =
+      scheme == 1 ? ms::SignalAdvectionKind::van_leer : ms::SignalAdvectionKind::upwind;
+  spec.integration = static_cast<ms::SignalIntegrationKind>(integration);
+  spec.solver.max_iterations = max_iterations;
+  spec.solver.absolute_tolerance = absolute_tolerance;
+  spec.solver.relative_tolerance = relative_tolerance;
+  if (x_faces != nullptr) {
+    ms::SignalGridVelocityField field;
+    field.x_faces.assign(x_faces, x_faces + (nx + 1) * static_cast<std::size_t>(ny) * nz);
+    field.y_faces.assign(y_faces, y_faces + static_cast<std::size_t>(nx) * (ny + 1) * nz);
+    field.z_faces.assign(z_faces, z_faces + static_cast<std::size_t>(nx) * ny * (nz + 1));
+    spec.velocity_field = std::move(field);
+  }
+  if (loss_rates != nullptr) {
+    ms::SignalGridAffineReaction reaction;
+    reaction.source_rates.assign(source_rates, source_rates + signals * n);
+    reaction.loss_rates.assign(loss_rates, loss_rates + signals * n);
+    spec.reaction = std::move(reaction);
+  }
+  return spec;
+}
+
 struct FlowHandle {
   ms::SignalGridSpec spec;
   std::vector<float> mobility;
@@ -266,7 +322,8 @@ int ref_flow_apply(void* opaque, const double* input, double* output) {
 int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
                             float dy, float dz, const std::uint8_t* obstacles,
                             const std::uint8_t* kinds, std::uint32_t signals,
-                            const float* diffusion, const float* boundary_values,
+                            const float* diffusion, const float* advection,
+                            const float* boundary_values,
                             const float* x_faces, const float* y_faces, const float* z_faces,
                             const float* source_rates, const float* loss_rates,
                             std::uint32_t scheme, std::uint32_t patch_count,
@@ -276,40 +333,12 @@ int ref_advance_signal_grid(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz
                             std::uint32_t steps, float* levels, float* report, double* seconds,
                             char* err, std::size_t errlen) {
   return guarded(err, errlen, [&] {
-    auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, boundary_values);
-    // Reservoir patches: per patch (face, begin[0], begin[1], end[0], end[1])
-    // and `signals` values.
-    ms::GridBoundary* faces[6] = {&spec.x_lower, &spec.x_upper, &spec.y_lower,
-                                  &spec.y_upper, &spec.z_lower, &spec.z_upper};
-    for (std::uint32_t k = 0; k < patch_count; ++k) {
-      const std::uint32_t* g = patch_geometry + 5 * k;
-      ms::GridBoundaryPatch patch;
-      patch.region.begin = {g[1], g[2]};
-      patch.region.end = {g[3], g[4]};
-      patch.values.assign(patch_values + k * signals, patch_values + (k + 1) * signals);
-      faces[g[0]]->patches.push_back(std::move(patch));
-    }
+    const auto spec = signal_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, diffusion,
+                                  advection, boundary_values, x_faces, y_faces, z_faces,
+                                  source_rates, loss_rates, scheme, patch_count, patch_geometry,
+                                  patch_values, integration, max_iterations, absolute_tolerance,
+                                  relative_tolerance);
     const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
-    spec.diffusion.assign(diffusion, diffusion + signals);
-    spec.advection_scheme =
-        scheme == 1 ? ms::SignalAdvectionKind::van_leer : ms::SignalAdvectionKind::upwind;
-    spec.integration = static_cast<ms::SignalIntegrationKind>(integration);
-    spec.solver.max_iterations = max_iterations;
-    spec.solver.absolute_tolerance = absolute_tolerance;
-    spec.solver.relative_tolerance = relative_tolerance;
-    if (x_faces != nullptr) {
-      ms::SignalGridVelocityField field;
-      field.x_faces.assign(x_faces, x_faces + (nx + 1) * static_cast<std::size_t>(ny) * nz);
-      field.y_faces.assign(y_faces, y_faces + static_cast<std::size_t>(nx) * (ny + 1) * nz);
-      field.z_faces.assign(z_faces, z_faces + static_cast<std::size_t>(nx) * ny * (nz + 1));
-      spec.velocity_field = std::move(field);
-    }
-    if (loss_rates != nullptr) {
-      ms::SignalGridAffineReaction reaction;
-      reaction.source_rates.assign(source_rates, source_rates + signals * n);
-      reaction.loss_rates.assign(loss_rates, loss_rates + signals * n);
-      spec.reaction = std::move(reaction);
-    }
     ms::SignalGrid grid(spec, std::vector<float>(levels, levels + signals * n));
     const auto start = std::chrono::steady_clock::now();
     ms::SignalSolveReport last;
@@ -466,3 +495,121 @@ int ref_advance_coupled(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, fl
 }
 
 }  // extern "C"
+
+#ifdef MSIM_WITH_HIP_BINDING
+#include "microsimulator/backend.hpp"
+
+// The reference-side `hip` backend (microsimulator_amd/binding/hip_backend.cpp)
+// driven through the reference's own ComputeBackend interface, beside the
+// reference's CPU functions on the same spec: what the reference's
+// conformance tests would compare with `--backend hip`.
+namespace microsimulator {
+std::unique_ptr<ComputeBackend> make_hip_backend(std::uint32_t device_index);
+std::size_t hip_backend_device_count() noexcept;
+}  // namespace microsimulator
+
+extern "C" {
+
+std::size_t bind_device_count(void) { return ms::hip_backend_device_count(); }
+
+// Makes the backend on device 0 and reads its info and features: name
+// ("hip"), whether it is native, and supports() of signals, coupled_rates,
+// depth_averaged_flow and resolved_flow as bits 0-3.
+int bind_create(char* name, std::size_t name_length, std::uint32_t* features, char* err,
+                std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto backend = ms::make_hip_backend(0);
+    const auto info = backend->info();
+    std::snprintf(name, name_length, "%s", info.name.c_str());
+    *features = (backend->supports(ms::BackendFeature::signals) ? 1u : 0u) |
+                (backend->supports(ms::BackendFeature::coupled_rates) ? 2u : 0u) |
+                (backend->supports(ms::BackendFeature::depth_averaged_flow) ? 4u : 0u) |
+                (backend->supports(ms::BackendFeature::resolved_flow) ? 8u : 0u) |
+                (info.native ? 16u : 0u);
+  });
+}
+
+// solve_depth_averaged_flow through the hip backend (faces_hip) and through
+// solve_depth_averaged_flow_cpu (faces_cpu), on a one-signal grid with the
+// given boundary kinds; force: (fx, fy) per column or null. report:
+// iterations, relative residual, max speed of each (hip first).
+int bind_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy, float dz,
+              const std::uint8_t* obstacles, const std::uint8_t* kinds, const float* mobility,
+              std::uint32_t axis, float mean_inlet_speed, float tolerance,
+              std::uint32_t max_iterations, const float* force, std::size_t force_columns,
+              float* faces_hip, float* faces_cpu, float* report, char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> mob;
+    if (mobility != nullptr) mob = std::span<const float>(mobility, n);
+    ms::DepthAveragedFlowParameters parameters;
+    parameters.mean_inlet_speed = mean_inlet_speed;
+    parameters.axis = static_cast<ms::FlowAxis>(axis);
+    parameters.relative_tolerance = tolerance;
+    parameters.max_iterations = max_iterations;
+    for (std::size_t c = 0; c < force_columns; ++c) {
+      parameters.body_force.push_back({force[2 * c], force[2 * c + 1]});
+    }
+    const auto backend = ms::make_hip_backend(0);
+    const auto hip = backend->solve_depth_averaged_flow(spec, mob, parameters);
+    const auto cpu = ms::solve_depth_averaged_flow_cpu(spec, mob, parameters);
+    float* out[2] = {faces_hip, faces_cpu};
+    const ms::DepthAveragedFlowResult* results[2] = {&hip, &cpu};
+    for (int k = 0; k < 2; ++k) {
+      const auto& field = results[k]->field;
+      std::memcpy(out[k], field.x_faces.data(), field.x_faces.size() * sizeof(float));
+      std::memcpy(out[k] + field.x_faces.size(), field.y_faces.data(),
+                  field.y_faces.size() * sizeof(float));
+      std::memcpy(out[k] + field.x_faces.size() + field.y_faces.size(), field.z_faces.data(),
+                  field.z_faces.size() * sizeof(float));
+      report[3 * k] = static_cast<float>(results[k]->report.iterations);
+      report[3 * k + 1] = results[k]->report.relative_residual;
+      report[3 * k + 2] = results[k]->report.max_speed;
+    }
+  });
+}
+
+// `steps` transport steps through the hip backend's advance_signal_grid
+// (levels in place) and through advance_signal_grid_cpu (levels_cpu), the
+// spec as ref_advance_signal_grid takes it. report: the last step's
+// (converged, iterations, residual RMS) of each (hip first).
+int bind_signal_steps(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
+                      float dz, const std::uint8_t* obstacles, const std::uint8_t* kinds,
+                      std::uint32_t signals, const float* diffusion, const float* advection,
+                      const float* boundary_values, const float* x_faces, const float* y_faces,
+                      const float* z_faces, const float* source_rates, const float* loss_rates,
+                      std::uint32_t scheme, std::uint32_t patch_count,
+                      const std::uint32_t* patch_geometry, const float* patch_values,
+                      std::uint32_t integration, std::uint32_t max_iterations,
+                      float absolute_tolerance, float relative_tolerance, float dt,
+                      std::uint32_t steps, float* levels, float* levels_cpu, float* report,
+                      char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = signal_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, signals, diffusion,
+                                  advection, boundary_values, x_faces, y_faces, z_faces,
+                                  source_rates, loss_rates, scheme, patch_count, patch_geometry,
+                                  patch_values, integration, max_iterations, absolute_tolerance,
+                                  relative_tolerance);
+    const std::size_t count = static_cast<std::size_t>(nx) * ny * nz * signals;
+    ms::SignalGrid on_hip(spec, std::vector<float>(levels, levels + count));
+    ms::SignalGrid on_cpu(spec, std::vector<float>(levels, levels + count));
+    const auto backend = ms::make_hip_backend(0);
+    ms::SignalSolveReport a, b;
+    for (std::uint32_t step = 0; step < steps; ++step) {
+      a = backend->advance_signal_grid(on_hip, dt);
+      b = ms::advance_signal_grid_cpu(on_cpu, dt);
+    }
+    std::memcpy(levels, on_hip.levels().data(), count * sizeof(float));
+    std::memcpy(levels_cpu, on_cpu.levels().data(), count * sizeof(float));
+    const ms::SignalSolveReport* r[2] = {&a, &b};
+    for (int k = 0; k < 2; ++k) {
+      report[3 * k] = r[k]->converged ? 1.0F : 0.0F;
+      report[3 * k + 1] = static_cast<float>(r[k]->iterations);
+      report[3 * k + 2] = r[k]->residual_rms;
+    }
+  });
+}
+
+}  // extern "C"
+#endif  // MSIM_WITH_HIP_BINDING

@@ -422,3 +422,71 @@ def coupled_colony_case(masked=True):
     transport = dict(diffusion=np.array([0.08, 0.03], f32), boundary_values=np.zeros(12, f32),
                      faces=faces, source=source, loss=loss)
     return grid, levels, cells, species, plan, previous, transport
+
+
+def conformance_spec(kind="plain"):
+    """The reference conformance suite's 9 x 7 x 5 grid of two signals
+    (tests/conformance/signal_grid_conformance_test.cpp:14-119): periodic in
+    x, fixed reservoirs in z, constant advection, an affine reaction; `masked`
+    adds an obstacle block and one obstacle on the periodic edge;
+    `velocity_field` replaces the advection by x faces 0.05 + 0.01 y - 0.008 z;
+    `patched` makes every face a fixed reservoir, with patches on x_lower,
+    y_upper and z_lower (each indexed by a different pair of axes) and a
+    uniform field across all six faces. Returns the grid's shape, spacing and
+    kinds, and the keyword arguments of the oracle's and the reference's
+    transport calls (diffusion, advection, boundary_values, faces, source,
+    loss, patches, obstacles) and the levels (make_levels, :121-128)."""
+    nx, ny, nz, S = 9, 7, 5, 2
+    n = nx * ny * nz
+    kinds = [PERIODIC, PERIODIC, NO_FLUX, NO_FLUX, FIXED, FIXED]
+    bv = np.zeros(6 * S, np.float32)
+    bv[0 * 6 + 4], bv[1 * 6 + 4] = 0.25, 0.5
+    bv[0 * 6 + 5], bv[1 * 6 + 5] = 1.0, 0.75
+    advection = np.array([0.1, -0.2, 0.05, -0.05, 0.03, -0.08], np.float32)
+    index = np.arange(S * n)
+    source = (np.float32(0.01) * (index % 3).astype(np.float32)).astype(np.float32)
+    loss = (np.float32(0.005) * (index % 5).astype(np.float32)).astype(np.float32)
+    obstacles = None
+    faces = None
+    patches = ()
+    if kind == "masked":
+        x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+        solid = ((x >= 3) & (x <= 5) & (y >= 2) & (y <= 4) & (z == 2)) | \
+            ((x == 0) & (y == 1) & (z == 1))
+        obstacles = solid.astype(np.uint8).reshape(-1)
+        mask = np.tile(obstacles, S) != 0
+        source[mask] = 0.0
+        loss[mask] = 0.0
+    elif kind == "velocity_field":
+        advection = np.zeros(3 * S, np.float32)
+        fx, y, z = np.meshgrid(np.arange(nx + 1), np.arange(ny), np.arange(nz), indexing="ij")
+        xf = (np.float32(0.05) + np.float32(0.01) * y.astype(np.float32)
+              - np.float32(0.008) * z.astype(np.float32)).astype(np.float32).reshape(-1)
+        faces = np.concatenate([xf, np.zeros(nx * (ny + 1) * nz, np.float32),
+                                np.zeros(nx * ny * (nz + 1), np.float32)])
+    elif kind == "patched":
+        advection = np.zeros(3 * S, np.float32)
+        kinds = [FIXED] * 6
+        for f in range(4):
+            bv[0 * 6 + f], bv[1 * 6 + f] = 0.4, 0.6
+        patches = ((0, (1, 0), (4, 3), [1.5, 0.1]), (0, (3, 2), (7, 5), [0.0, 2.0]),
+                   (3, (0, 1), (5, 4), [0.9, 0.2]),
+                   (4, (2, 0), (9, 2), [1.2, 0.3]), (4, (6, 1), (8, 7), [0.1, 1.1]))
+        faces = np.concatenate([np.full((nx + 1) * ny * nz, 0.06, np.float32),
+                                np.full(nx * (ny + 1) * nz, -0.04, np.float32),
+                                np.full(nx * ny * (nz + 1), 0.05, np.float32)])
+    levels = (np.float32(0.5) + np.float32(0.001) *
+              ((index * 37) % 211).astype(np.float32)).astype(np.float32)
+    if obstacles is not None:
+        levels[np.tile(obstacles, S) != 0] = 0.0
+    kw = dict(diffusion=np.array([0.2, 0.05], np.float32), boundary_values=bv, faces=faces,
+              source=source, loss=loss, patches=patches)
+    if faces is None:
+        kw["advection"] = advection
+    return dict(shape=(nx, ny, nz), spacing=(1.0, 0.75, 1.25), kinds=tuple(kinds),
+                obstacles=obstacles, kw=kw, levels=levels)
+
+
+CONFORMANCE_KINDS = ("plain", "masked", "velocity_field", "patched")
+# The suite's steps: forward Euler at 0.02, the implicit kinds at 0.5 (:267-293).
+CONFORMANCE_STEPS = ((0, 0.02), (1, 0.5), (2, 0.5))
