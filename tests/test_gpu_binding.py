@@ -1,0 +1,65 @@
+"""The reference-side `hip` backend on the GPU, driven through the
+reference's own ComputeBackend interface (oracle/_ref/libmsim_binding.so:
+microsimulator_amd/binding/hip_backend.cpp compiled against the reference's
+headers) beside the reference's CPU functions on the same inputs: what the
+reference's conformance tests check with `--backend hip`.
+- depth-averaged flow (flow_conformance_test.cpp:35-40, 259-263): faces
+  within 8e-4 absolute + relative and iterations within 2, on the channel +
+  trap 256x128 and with a body force;
+- transport (signal_grid_conformance_test.cpp:132-136): levels within 5e-6
+  absolute + relative, on the suite's own 9 x 7 x 5 grids under every
+  integration kind and scheme (the device is bit-exact there)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import Binding, UPWIND, VAN_LEER, Grid
+from tests.helpers import (CONFORMANCE_KINDS, CONFORMANCE_STEPS, c1_inputs, conformance_spec)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def binding():
+    return Binding()
+
+
+def close(a, b, tol):
+    return np.all(np.abs(a - b) <= tol + tol * np.maximum(np.abs(a), np.abs(b)))
+
+
+def test_backend_identity(binding):
+    name, features = binding.create()
+    assert name == "hip" and features["native"]
+    assert features["signals"] and features["coupled_rates"] and features["depth_averaged_flow"]
+    assert not features["resolved_flow"]
+
+
+def test_depth_averaged_flow_through_the_interface(binding):
+    c = c1_inputs()
+    grid = Grid(c["nx"], c["ny"], 1, 1.0, 1.0, c["dz"], c["obstacles"])
+    hip, cpu, (rh, rc) = binding.flow(grid, c["mobility"], 0, 100.0, 1e-6, 100000)
+    print("iterations", rh["iterations"], rc["iterations"], flush=True)
+    assert abs(rh["iterations"] - rc["iterations"]) <= 2
+    assert close(hip, cpu, 8e-4)
+    force = np.random.default_rng(2).uniform(-50, 50, (c["nx"] * c["ny"], 2)).astype(np.float32)
+    hip, cpu, (rh, rc) = binding.flow(grid, c["mobility"], 0, 100.0, 1e-6, 100000,
+                                      body_force=force)
+    assert abs(rh["iterations"] - rc["iterations"]) <= 4
+    assert close(hip, cpu, 8e-4)
+
+
+@pytest.mark.parametrize("kind", CONFORMANCE_KINDS)
+@pytest.mark.parametrize("integration,dt", CONFORMANCE_STEPS)
+@pytest.mark.parametrize("scheme", [UPWIND, VAN_LEER])
+def test_transport_conformance_through_the_interface(binding, kind, integration, dt, scheme):
+    c = conformance_spec(kind)
+    nx, ny, nz = c["shape"]
+    dx, dy, dz = c["spacing"]
+    grid = Grid(nx, ny, nz, dx, dy, dz, c["obstacles"], boundary=c["kinds"])
+    hip, cpu, (rh, rc) = binding.signal_steps(grid, c["levels"], dt, steps=2, scheme=scheme,
+                                              integration=integration, **c["kw"])
+    assert close(hip, cpu, 5e-6)
+    assert np.array_equal(hip, cpu)
+    assert rh["converged"] == rc["converged"] and rh["iterations"] == rc["iterations"]
