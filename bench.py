@@ -341,6 +341,54 @@ def batch_sweep_bench(msa, hip, steps: int):
     }
 
 
+def nutrient_loop_bench(msa, hip, grid, w, flow, steps: int):
+    """The device-resident nutrient loop at the workload: per step the
+    occupancy's Brinkman mobility, the headline pressure solve and velocity,
+    one 3-solute FP64 transport step with Monod uptake and its mass ledger,
+    and the occupancy grown by the uptake."""
+    S = 3
+    solid = w.obstacles.reshape(-1) != 0
+    base = np.where(solid, 0.0, w.dz * w.dz / 12.0).astype(np.float32)
+    hip.flow_prepare(grid, w.mobility, 0)
+    rep = hip.flow_solve(flow)
+    bv = np.zeros(6 * S, np.float32)
+    bv[0::6] = 1.0
+    spec = msa.SignalGridSpec(grid, [100.0, 500.0, 1000.0], bv, phi=w.phi, monod_vmax=[1.0] * S,
+                              monod_k=[0.1] * S)
+    hip.adr_upload(spec, np.zeros(S * w.sites), precision=64, use_flow_field=True)
+    # 0.5 of the explicit bound with the first field; a tenth of the uptake
+    # becomes biomass.
+    dt = 0.5 / (2 * 1000.0 * 2 + 4.0 * rep.max_speed + 1.0 * float(w.phi.max()) / 0.1 * 1.5)
+    t0 = time.perf_counter()
+    ledger, records = hip.nutrient_run(flow, base, dt, steps, 100.0, 0.9, 0.1)
+    elapsed = time.perf_counter() - t0
+    solve_ms = [1e3 * r["solve_seconds"] for r in records]
+    log(f"nutrient loop: {steps} steps in {elapsed:.2f} s ({steps / elapsed:.2f} steps/s), solve "
+        f"{np.mean(solve_ms):.1f} ms/step, worst balance "
+        f"{max(r['mass_balance'] for r in records):.2e}")
+    return {
+        "steps": steps, "steps_per_s": round(steps / elapsed, 3), "dt": dt,
+        "solver": flow.solver, "mean_solve_ms": round(float(np.mean(solve_ms)), 3),
+        "mean_step_ms": round(1e3 * float(np.mean([r["seconds"] for r in records])), 3),
+        "mean_iterations": round(float(np.mean([r["iterations"] for r in records])), 2),
+        "worst_true_relative_residual": max(r["true_relative_residual"] for r in records),
+        "fell_back": any(r["fell_back"] for r in records),
+        "worst_step_mass_balance": max(r["mass_balance"] for r in records),
+        "max_phi_after": records[-1]["max_phi"], "solutes": S,
+    }
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def port_flow_bench(msa, w, hip, cpu_seconds_per_iteration):
     """A port-driven flow on the same channel: two speed inlets splitting the
     lower x face (mean speeds 1 and 2) and the whole upper x face held at
@@ -413,6 +461,9 @@ def main():
     ap.add_argument("--coupled-cells", type=int, default=1 << 20)
     ap.add_argument("--batch-steps", type=int, default=50,
                     help="graph-captured steps of the 256-instance batched sweep (0: skip)")
+    ap.add_argument("--loop-steps", type=int, default=100,
+                    help="steps of the device-resident nutrient loop (solve, velocity, "
+                         "3-solute FP64 transport, occupancy growth) at the workload (0: skip)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -675,22 +726,36 @@ def main():
         line["coupled"] = coupled
     if mfma is not None:
         line["roofline_mfma"] = mfma
+    if args.loop_steps > 0:
+        line["nutrient_loop"] = nutrient_loop_bench(msa, hip, grid, w, headline, args.loop_steps)
+    # The achievable HBM bandwidth of a plain device copy, and the dominant
+    # kernel against it.
+    ceiling = hip.copy_bandwidth(1 << 30, 10)
+    line["roofline"]["copy_ceiling"] = round(ceiling, 1)
+    line["roofline"]["frac_of_copy_ceiling"] = round(achieved / ceiling, 4)
+    log(f"device copy: {ceiling:.0f} GB/s (read + written); dominant kernel "
+        f"{achieved / ceiling:.3f} of it")
     if args.batch_steps > 0:
         line["batch"] = batch_sweep_bench(msa, hip, args.batch_steps)
     if args.solver == "learned":
         with np.load(learned.WEIGHTS_FILE, allow_pickle=False) as f:
             meta = json.loads(str(f["meta"]))
         line["learned"] = {
-            "training_pairs": meta.get("total_pairs"), "training_geometries": meta.get("geometries"),
-            "training_steps": meta.get("steps"), "train_device_seconds": meta.get("train_seconds"),
-            "harvest_seconds": meta.get("harvest_seconds"),
             "iterations_vs_multigrid": [reports[-1].iterations,
                                         others["multigrid"][1].iterations],
+            # Recorded when the committed weights were trained; not timed here.
+            "training_record": {
+                "pairs": meta.get("total_pairs"), "geometries": meta.get("geometries"),
+                "adam_steps": meta.get("steps"), "train_device_seconds": meta.get("train_seconds"),
+                "harvest_seconds": meta.get("harvest_seconds"),
+                "source": "microsimulator_amd/learned_smoother.npz metadata"},
         }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
         line["cpu_baseline"] = cpu_baseline(w, fp64_rep.iterations, threads, args.cpu_budget)
+        line["cpu_baseline"]["cpu_model"] = cpu_model()
+        line["cpu_baseline"]["host_cpus"] = os.cpu_count()
         faces = field.concatenated()
         adr_cpu = cpu_adr_baseline(w, faces, threads, steps=3)
         if adr_cpu is not None:

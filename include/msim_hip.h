@@ -350,6 +350,59 @@ int msim_hip_advance_signal_grid_static(msim_ctx* ctx, const msim_signal_grid* g
                                         float* levels, msim_signal_report* report);
 size_t msim_hip_static_scratch_bytes(void);
 
+/* The nutrient loop of the reference's nutrient study
+ * (scripts/run_nutrient_benchmarks.py:96-120) kept on the device, on a grid of
+ * one layer: per step
+ *  1. the mobility of the occupancy phi by the reference's Brinkman closure,
+ *     base / (1 + base * C * phi^2 / (1 - phi)^3) with phi capped
+ *     (_darcy_coefficients, python/src/microsimulator/flow.py:361-384);
+ *  2. the operator and its multilevel hierarchy rebuilt from it, the pressure
+ *     solve of `flow` (its form, tolerance and scaling) and the velocity;
+ *  3. the explicit bound of the new field, then one FP64 forward-Euler step of
+ *     the uploaded signals over it, with Monod uptake on phi and the step's FP64
+ *     ledger row (as msim_hip_adr_run's);
+ *  4. phi grown by yield x dt x the step's uptake, sum over signals of
+ *     vmax phi c / (K + c) at the levels the step started from.
+ * The reference's script grows cells from the nutrient they took up and
+ * recomputes the colony's mobility before each solve; here the biomass is the
+ * grid's occupancy. Needs msim_hip_flow_prepare (the grid and axis) and
+ * msim_hip_adr_upload (FP64 levels, phi with Monod, no affine reaction,
+ * use_flow_field = 1) first. ledger: steps*S*4 doubles or NULL; per_step:
+ * `steps` records or NULL. A refused step stops the run, keeping the state the
+ * step started from. */
+typedef struct {
+  msim_flow_params flow;       /* the solve of every step */
+  const float* base_mobility;  /* per site: the mobility without biomass (zero at solids) */
+  float drag_coefficient;      /* C (the reference's default 100) */
+  float max_volume_fraction;   /* phi cap inside the drag (0.9) */
+  float yield;                 /* occupancy gained per unit of solute taken up */
+  float dt;                    /* transport step */
+} msim_nutrient_params;
+
+typedef struct {
+  uint32_t iterations;            /* Krylov iterations of the step's pressure solve */
+  uint32_t fell_back;             /* the learned solver's Jacobi-PCG fallback ran */
+  double true_relative_residual;  /* FP64 true residual of the solve */
+  float max_speed;
+  float max_phi;                  /* largest occupancy after the step */
+  double mass_balance;            /* largest |after - before - supply + uptake| / after */
+  double seconds;                 /* device time of the step */
+  double solve_seconds;           /* of which the pressure solve and velocity */
+} msim_nutrient_step;
+
+int msim_hip_nutrient_run(msim_ctx* ctx, const msim_nutrient_params* params, uint32_t steps,
+                          double* ledger, msim_nutrient_step* per_step);
+
+/* No counterpart in the reference: the device's achievable HBM bandwidth, as
+ * GB/s read plus written by `repeats` STREAM-style copies of `bytes` (one
+ * 16-byte load and store per element), timed with HIP events; the ceiling the
+ * stencil kernels are measured against beside the published 8 TB/s. */
+int msim_hip_copy_bandwidth(msim_ctx* ctx, size_t bytes, uint32_t repeats, double* gb_per_s);
+
+/* The Monod occupancy (N floats) the uploaded transport reads: the one uploaded,
+ * or the one a nutrient run grew. */
+int msim_hip_adr_occupancy(msim_ctx* ctx, float* phi);
+
 /* Device-resident transport for many coupled steps.
  *  adr_upload   -- spec arrays and initial levels (float when precision is
  *                  32, double when 64). With use_flow_field != 0 the velocity

@@ -98,6 +98,19 @@ class _FlowParams(C.Structure):
                 ("body_force_columns", C.c_size_t)]
 
 
+class _NutrientParams(C.Structure):
+    _fields_ = [("flow", _FlowParams), ("base_mobility", C.c_void_p),
+                ("drag_coefficient", C.c_float), ("max_volume_fraction", C.c_float),
+                ("yield_", C.c_float), ("dt", C.c_float)]
+
+
+class _NutrientStep(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("fell_back", C.c_uint32),
+                ("true_relative_residual", C.c_double), ("max_speed", C.c_float),
+                ("max_phi", C.c_float), ("mass_balance", C.c_double), ("seconds", C.c_double),
+                ("solve_seconds", C.c_double)]
+
+
 class _FlowReport(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("refinements", C.c_uint32),
                 ("relative_residual", C.c_double), ("true_relative_residual", C.c_double),
@@ -221,6 +234,7 @@ EXPORTS = (
     "msim_hip_coupled_download", "msim_hip_coupled_seconds", "msim_hip_colony_volume_fraction",
     "msim_hip_colony_mobility", "msim_hip_colony_seconds",
     "msim_hip_advance_signal_grid_static", "msim_hip_static_scratch_bytes",
+    "msim_hip_nutrient_run", "msim_hip_adr_occupancy", "msim_hip_copy_bandwidth",
 )
 
 #: Weights of the learned smoother: W1[9][4][16], W2[9][16][16], W3[9][16].
@@ -285,6 +299,9 @@ def library(path: str = LIBRARY):
         "msim_hip_colony_seconds": (D, [P]),
         "msim_hip_advance_signal_grid_static": (I, [P, P, C.c_float, P, P]),
         "msim_hip_static_scratch_bytes": (C.c_size_t, []),
+        "msim_hip_nutrient_run": (I, [P, P, U, P, P]),
+        "msim_hip_adr_occupancy": (I, [P, P]),
+        "msim_hip_copy_bandwidth": (I, [P, C.c_size_t, U, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -1023,6 +1040,44 @@ class HipBackend:
         self._check(self.lib.msim_hip_adr_run(self.ctx, dt, steps,
                                               None if led is None else led.ctypes.data))
         return led
+
+    def nutrient_run(self, flow: DepthAveragedFlowParameters, base_mobility, dt: float,
+                     steps: int, drag_coefficient: float = 100.0,
+                     max_volume_fraction: float = 0.9, yield_: float = 1.0):
+        """The device-resident nutrient loop (msim_hip_nutrient_run) over the
+        prepared flow and the uploaded FP64 transport: returns the ledger
+        (steps x S x 4) and one dict per step."""
+        keep: list = []
+        p = _NutrientParams()
+        p.flow = flow._c(keep)
+        base = np.ascontiguousarray(base_mobility, np.float32).reshape(-1)
+        if base.size != self._grid.nx * self._grid.ny:
+            raise InvalidArgument("flow mobility must hold one value per grid site")
+        p.base_mobility = base.ctypes.data
+        p.drag_coefficient = drag_coefficient
+        p.max_volume_fraction = max_volume_fraction
+        p.yield_ = yield_
+        p.dt = dt
+        S = self._adr[0]
+        ledger = np.zeros((steps, S, 4), np.float64)
+        records = (_NutrientStep * max(1, steps))()
+        self._check(self.lib.msim_hip_nutrient_run(self.ctx, C.byref(p), steps,
+                                                   ledger.ctypes.data,
+                                                   C.cast(records, C.c_void_p)))
+        names = [f[0] for f in _NutrientStep._fields_]
+        return ledger, [{k: getattr(r, k) for k in names} for r in records[:steps]]
+
+    def copy_bandwidth(self, nbytes: int = 1 << 30, repeats: int = 10) -> float:
+        """GB/s (read + written) of a STREAM-style device copy of nbytes."""
+        out = C.c_double()
+        self._check(self.lib.msim_hip_copy_bandwidth(self.ctx, nbytes, repeats, C.byref(out)))
+        return out.value
+
+    def adr_occupancy(self) -> np.ndarray:
+        """The Monod occupancy the uploaded transport reads."""
+        phi = np.zeros(self._adr[1], np.float32)
+        self._check(self.lib.msim_hip_adr_occupancy(self.ctx, phi.ctypes.data))
+        return phi
 
     def adr_download(self) -> np.ndarray:
         S, n, dtype = self._adr

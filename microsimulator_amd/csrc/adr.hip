@@ -1008,6 +1008,38 @@ void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger) {
   }
 }
 
+double adr_refresh_outflow(Context& ctx) {
+  auto& a = ctx.adr;
+  const auto& P = a.params;
+  if (P.has_field == 0) return 0.0;
+  const float* xf = a.flow_field ? ctx.flow.faces.data() : a.xf.data();
+  const float* yf = a.flow_field ? ctx.flow.faces.data() + ctx.flow.x_faces() : a.yf.data();
+  const uint8_t* levels = a.current == 0 ? a.levels_a.data() : a.levels_b.data();
+  const size_t n = a.sites();
+  const uint32_t blocks = std::min<uint32_t>(site_blocks(n), 2048);
+  // The survey's mass goes to a.mass only; the ledger's running mass
+  // (host_mass) is left to the steps' own rows.
+  if (P.nz > 1) {
+    adr3_survey(ctx, xf, yf, a.zf.data(), reinterpret_cast<const float*>(levels),
+                a.state.data(), a.partials.data(), a.mass.data());
+  } else if (a.precision == 64) {
+    k_adr_survey<double><<<blocks, kAdrBlock, 0, ctx.stream>>>(
+        P, a.solid.data(), xf, yf, reinterpret_cast<const double*>(levels), a.partials.data(),
+        a.state.data(), a.mass.data());
+  } else {
+    k_adr_survey<float><<<blocks, kAdrBlock, 0, ctx.stream>>>(
+        P, a.solid.data(), xf, yf, reinterpret_cast<const float*>(levels), a.partials.data(),
+        a.state.data(), a.mass.data());
+  }
+  check(hipGetLastError(), "launch k_adr_survey");
+  CgState host{};
+  check(hipMemcpyAsync(&host, a.state.data(), sizeof(CgState), hipMemcpyDeviceToHost, ctx.stream),
+        "read survey");
+  check(hipStreamSynchronize(ctx.stream), "survey");
+  a.greatest_outflow = host.peak;
+  return host.peak;
+}
+
 void adr_download(Context& ctx, void* levels) {
   auto& a = ctx.adr;
   const size_t bytes =

@@ -796,7 +796,14 @@ void validate_explicit_bound(const AdrParams& P, uint32_t integration, double gr
 
 namespace {
 
-void record_losses(Context& ctx, const msim_signal_grid& s) { max_losses(s, ctx.adr.max_loss); }
+void record_losses(Context& ctx, const msim_signal_grid& s) {
+  max_losses(s, ctx.adr.max_loss);
+  const size_t n = static_cast<size_t>(s.grid.nx) * s.grid.ny * s.grid.nz;
+  ctx.adr.phi_max = 0.0;
+  if (s.phi != nullptr) {
+    for (size_t i = 0; i < n; ++i) ctx.adr.phi_max = std::max(ctx.adr.phi_max, double(s.phi[i]));
+  }
+}
 
 // A transport uploaded over the device's flow field reads it in place: any
 // later flow solve or preparation (another grid, speed, mobility or mask)
@@ -1504,6 +1511,72 @@ int msim_hip_advance_signal_grid_static(msim_ctx* ctx, const msim_signal_grid* g
 }
 
 size_t msim_hip_static_scratch_bytes(void) { return msim::static_scratch_bytes(); }
+
+int msim_hip_nutrient_run(msim_ctx* ctx, const msim_nutrient_params* params, uint32_t steps,
+                          double* ledger, msim_nutrient_step* per_step) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (params == nullptr || params->base_mobility == nullptr) {
+      msim::invalid("parameters and the base mobility must not be null");
+    }
+    auto& f = c.flow;
+    auto& a = c.adr;
+    if (!f.prepared || f.ported || f.layers.nz != 1) {
+      msim::invalid("the nutrient loop needs a prepared axis flow on a grid of one layer");
+    }
+    if (!a.uploaded || a.precision != 64 || !a.flow_field || a.params.has_monod == 0 ||
+        a.params.has_affine != 0 || a.integration != MSIM_FORWARD_EULER ||
+        a.params.nx != f.nx || a.params.ny != f.ny) {
+      msim::invalid(
+          "the nutrient loop needs FP64 forward-Euler transport with Monod uptake and no affine "
+          "reaction, uploaded over the prepared flow's field");
+    }
+    msim::validate_flow_params(params->flow);
+    if (params->flow.scale == 0) msim::invalid("the nutrient loop takes the scaled field");
+    if (params->flow.axis != f.axis) msim::invalid("the flow axis differs from the prepared one");
+    if (!std::isfinite(params->dt) || params->dt <= 0.0f) {
+      msim::invalid("time step must be finite and positive");
+    }
+    if (!std::isfinite(params->drag_coefficient) || params->drag_coefficient < 0.0f) {
+      msim::invalid("drag coefficient must be finite and non-negative");
+    }
+    if (!std::isfinite(params->max_volume_fraction) || params->max_volume_fraction < 0.0f ||
+        params->max_volume_fraction >= 1.0f) {
+      msim::invalid("max volume fraction must be finite and in [0, 1)");
+    }
+    if (!std::isfinite(params->yield) || params->yield < 0.0f) {
+      msim::invalid("yield must be finite and non-negative");
+    }
+    const size_t n = f.sites();
+    for (size_t i = 0; i < n; ++i) {
+      if (!std::isfinite(params->base_mobility[i]) || params->base_mobility[i] < 0.0f) {
+        msim::invalid("flow mobility values must be finite and non-negative");
+      }
+    }
+    msim::nutrient_run(c, *params, steps, ledger, per_step);
+  });
+}
+
+int msim_hip_copy_bandwidth(msim_ctx* ctx, size_t bytes, uint32_t repeats, double* gb_per_s) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (gb_per_s == nullptr || repeats == 0 || bytes < 16) {
+      msim::invalid("copy bandwidth needs an output, repeats and at least 16 bytes");
+    }
+    *gb_per_s = msim::stream_copy_bandwidth(c, bytes, repeats);
+  });
+}
+
+int msim_hip_adr_occupancy(msim_ctx* ctx, float* phi) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (phi == nullptr) msim::invalid("phi must not be null");
+    if (!c.adr.uploaded || c.adr.params.has_monod == 0) {
+      msim::invalid("no transport with Monod uptake on the device");
+    }
+    msim::check(hipMemcpyAsync(phi, c.adr.phi.data(), c.adr.sites() * sizeof(float),
+                               hipMemcpyDeviceToHost, c.stream),
+                "download occupancy");
+    msim::check(hipStreamSynchronize(c.stream), "download occupancy");
+  });
+}
 
 int msim_hip_adr_upload(msim_ctx* ctx, const msim_signal_grid* grid, const void* levels,
                         uint32_t precision, uint32_t use_flow_field) {

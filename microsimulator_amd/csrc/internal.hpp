@@ -179,6 +179,7 @@ struct FlowDevice {
   DeviceBuffer<float> force, body_flux, driven, rhs_saved;
   DeviceBuffer<double> x_saved;
   bool forced = false;             // the last solve carried a body force
+  DeviceBuffer<float> base_mobility;  // the nutrient loop's mobility without biomass
   DeviceBuffer<float> x32, r32, q32, p032, p132;
   DeviceBuffer<double> partials;
   DeviceBuffer<double> pq, rz, rr; // per-block Krylov partial sums, two generations each
@@ -276,6 +277,7 @@ struct AdrDevice {
   uint64_t revision = 0;
   double greatest_outflow = 0.0;
   double max_loss[kMaxSignals] = {};
+  double phi_max = 0.0;  // largest Monod occupancy of the uploaded (or grown) phi
   double host_mass[kMaxSignals] = {};  // mass of each signal now (FP64 runs)
   DeviceBuffer<uint8_t> solid;
   DeviceBuffer<float> xf, yf, zf, phi, source, loss, edge;
@@ -478,6 +480,9 @@ void adr_upload(Context& ctx, const msim_signal_grid& grid, const void* levels,
                 uint32_t precision, bool use_flow_field);
 void adr_run(Context& ctx, float dt, uint32_t steps, double* ledger);
 void adr_download(Context& ctx, void* levels);
+// The greatest outflow over the fluid sites of the field the uploaded
+// transport reads (after that field changed), kept for the explicit bound.
+double adr_refresh_outflow(Context& ctx);
 // The settle and validation of SignalGrid::accept_step (signals.cpp:979-1011)
 // on `count` FP32 levels in place, with the largest level from `parts`
 // per-block maxima (0: no settle, validation only); a failure sets the
@@ -520,6 +525,14 @@ void implicit3_setup(Context& ctx, uint32_t blocks, const float* xf, const float
 void implicit3_sweep(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
                      const float* zf, const float* cur, float* nxt, float half_dt,
                      float relaxation);
+
+// stream.hip: GB/s (read + written) of `repeats` device copies of `bytes`.
+double stream_copy_bandwidth(Context& ctx, size_t bytes, uint32_t repeats);
+
+// nutrient.hip: the nutrient loop (msim_hip_nutrient_run) over the prepared
+// flow and the uploaded FP64 transport.
+void nutrient_run(Context& ctx, const msim_nutrient_params& params, uint32_t steps,
+                  double* ledger, msim_nutrient_step* per_step);
 
 // implicit.hip: `steps` implicit steps of dt on the uploaded FP32 levels.
 // sources (device, S*N, or null): rates added to each step's right-hand side as

@@ -957,10 +957,21 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
 void mg_prepare(Context& ctx) {
   auto& f = ctx.flow;
   auto& mg = f.mg;
+  // Levels of the same shapes keep their device buffers: a system that
+  // changes only its coefficients (a new mobility on the same grid) rebuilds
+  // the hierarchy without allocating.
+  std::vector<std::unique_ptr<MgLevel>> kept = std::move(mg.levels);
   mg.levels.clear();
-  auto level0 = std::make_unique<MgLevel>();
-  level0->nx = f.nx;
-  level0->ny = f.ny;
+  auto take = [&](size_t l, uint32_t nx, uint32_t ny) {
+    if (l < kept.size() && kept[l] && kept[l]->nx == nx && kept[l]->ny == ny) {
+      return std::move(kept[l]);
+    }
+    auto level = std::make_unique<MgLevel>();
+    level->nx = nx;
+    level->ny = ny;
+    return level;
+  };
+  auto level0 = take(0, f.nx, f.ny);
   const size_t n = f.sites();
   for (auto* b : {&level0->wx, &level0->wy, &level0->bc, &level0->dinv, &level0->r, &level0->e})
     b->reserve(n);
@@ -977,9 +988,7 @@ void mg_prepare(Context& ctx) {
   mg.levels.push_back(std::move(level0));
   while (mg.levels.back()->cells() > kCoarseCells && mg.levels.size() < kMaxLevels) {
     MgLevel& F = *mg.levels.back();
-    auto C = std::make_unique<MgLevel>();
-    C->nx = (F.nx + 1) / 2;
-    C->ny = (F.ny + 1) / 2;
+    auto C = take(mg.levels.size(), (F.nx + 1) / 2, (F.ny + 1) / 2);
     const size_t cn = C->cells();
     for (auto* b : {&C->wx, &C->wy, &C->bc, &C->dinv, &C->r, &C->e}) b->reserve(cn);
     const uint32_t cb = static_cast<uint32_t>((cn + 255) / 256);
