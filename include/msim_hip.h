@@ -394,8 +394,9 @@ int msim_hip_nutrient_run(msim_ctx* ctx, const msim_nutrient_params* params, uin
                           double* ledger, msim_nutrient_step* per_step);
 
 /* No counterpart in the reference: the device's achievable HBM bandwidth, as
- * GB/s read plus written by `repeats` STREAM-style copies of `bytes` (one
- * 16-byte load and store per element), timed with HIP events; the ceiling the
+ * GB/s read plus written by `repeats` STREAM-style copies of `bytes` (16-byte
+ * loads and stores, four in flight per thread), timed with HIP events, or by
+ * the runtime's device-to-device copy where that is faster; the ceiling the
  * stencil kernels are measured against beside the published 8 TB/s. */
 int msim_hip_copy_bandwidth(msim_ctx* ctx, size_t bytes, uint32_t repeats, double* gb_per_s);
 

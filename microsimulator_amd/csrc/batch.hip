@@ -25,10 +25,14 @@
 // per instance in a fixed order on the host. A run of steps is captured once
 // into a HIP graph and replayed.
 //
-// The pressure solve itself (cold, or when a step's check finds an instance
-// unconverged) is one launch with one workgroup per instance (largest first):
-// the refinement loop needs the instance's dot products, and a workgroup
-// reduces them with barriers only.
+// The pressure solve itself is one launch with one workgroup per instance
+// (largest first): the refinement loop needs the instance's dot products,
+// and a workgroup reduces them with barriers only. Each FP32 correction is
+// conjugate gradient preconditioned by the instance's own multilevel cycle
+// (2x2 aggregation, V(1,1) damped Jacobi, the coarse correction weighted by
+// 1.8, the coarsest level of <= 64 cells inverted densely): the cycle of the
+// single-grid multilevel form, run level by level inside the workgroup. It is
+// linear and symmetric positive definite, so plain CG applies.
 //
 // The launch shapes (sites per item; workgroup size and sweep row width of
 // the general sweep) are chosen by timing each candidate on the uploaded
@@ -63,6 +67,19 @@ struct BatchItem {
   uint32_t inst, x0, x1, pad;
 };
 
+// The multilevel hierarchy of one instance: level l has nx[l] x ny[l] cells
+// at offset off[l] of the level arrays (level 0 at the instance's own first
+// site); the coarsest level's dense inverse is coarse_n x coarse_n floats at
+// offset cinv of the inverse pool.
+constexpr uint32_t kBatchLevels = 16;
+constexpr uint32_t kBatchCoarse = 64;
+struct BatchLevels {
+  uint32_t count, coarse_n;
+  uint64_t cinv;
+  uint64_t off[kBatchLevels];
+  uint32_t nx[kBatchLevels], ny[kBatchLevels];
+};
+
 struct BatchData {
   const BatchDesc* desc;
   const uint32_t* order;
@@ -71,7 +88,6 @@ struct BatchData {
   const float* cxl;
   const float* cyl;
   const float* bc;
-  const float* dinv;
   const float* rhs;
   const float* phi;
   double* x64;  // pressure
@@ -87,6 +103,15 @@ struct BatchData {
   float* residual;       // [instance]: FP64 true relative residual after it
   double* bnorm2;        // [instance]: ||b||^2
   uint32_t* flags;       // [instance]: 1 CG breakdown, 2 not converged, 4 bad level
+  float* z;              // preconditioned residual
+  const BatchLevels* levels;
+  const float* lwx;      // level couplings to the lower x / y cell (over h^2),
+  const float* lwy;      // boundary term and inverse diagonal
+  const float* lbc;
+  const float* ldinv;
+  float* lr;             // level residual and correction (coarse levels)
+  float* le;
+  const float* cinv;     // dense inverses of the coarsest levels
 };
 
 struct BatchParams {
@@ -187,9 +212,117 @@ __device__ inline float face_y(const BatchDesc& I, const double* __restrict__ p,
   return flux / I.dz;
 }
 
+// One level of an instance's hierarchy as the cycle sees it.
+struct LevelRef {
+  const float* wx;
+  const float* wy;
+  const float* bc;
+  const float* dinv;
+  float* r;
+  float* e;
+  uint32_t nx, ny;
+};
+
+__device__ inline LevelRef level_ref(const BatchData& D, const BatchLevels& V, uint32_t l,
+                                     size_t site) {
+  const size_t o = V.off[l];
+  return LevelRef{D.lwx + o,
+                  D.lwy + o,
+                  D.lbc + o,
+                  D.ldinv + o,
+                  l == 0 ? D.r + site : D.lr + o,
+                  l == 0 ? D.z + site : D.le + o,
+                  V.nx[l],
+                  V.ny[l]};
+}
+
+constexpr float kCycleOmega = 0.8f;
+constexpr float kCycleCorrection = 1.8f;
+
+// (A x)_i on a level, x at cell (gx, gy) given by xat, xc its value at i.
+template <class X>
+__device__ inline float level_ax(const LevelRef& L, size_t i, uint32_t x, uint32_t y, float xc,
+                                 X&& xat) {
+  float a = L.bc[i] * xc;
+  if (x > 0) a += L.wx[i] * (xc - xat(x - 1, y));
+  if (x + 1 < L.nx) a += L.wx[i + L.ny] * (xc - xat(x + 1, y));
+  if (y > 0) a += L.wy[i] * (xc - xat(x, y - 1));
+  if (y + 1 < L.ny) a += L.wy[i + 1] * (xc - xat(x, y + 1));
+  return a;
+}
+
+// z = M r on one instance, by the whole workgroup: per level a down pass
+// (pre-smoothing from zero, x = omega D^-1 r, its residual restricted to the
+// next level), the coarsest level's dense inverse, then per level an up pass
+// (x + 1.8 P e_c, post-smoothed). Returns r.z (FP64) in every thread.
+template <int NT>
+__device__ double batch_cycle(const BatchData& D, const BatchLevels& V, size_t site, double* lds) {
+  const uint32_t K = V.count - 1;
+  for (uint32_t l = 0; l < K; ++l) {
+    const LevelRef F = level_ref(D, V, l, site);
+    const LevelRef C = level_ref(D, V, l + 1, site);
+    auto pre = [&](uint32_t gx, uint32_t gy) {
+      const size_t j = static_cast<size_t>(gx) * F.ny + gy;
+      return kCycleOmega * F.dinv[j] * F.r[j];
+    };
+    const size_t cells = static_cast<size_t>(C.nx) * C.ny;
+    for (size_t c = threadIdx.x; c < cells; c += NT) {
+      const uint32_t X = static_cast<uint32_t>(c / C.ny), Y = static_cast<uint32_t>(c % C.ny);
+      float sum = 0.0f;
+      for (uint32_t a = 0; a < 2; ++a) {
+        for (uint32_t b = 0; b < 2; ++b) {
+          const uint32_t fx = 2 * X + a, fy = 2 * Y + b;
+          if (fx >= F.nx || fy >= F.ny) continue;
+          const size_t i = static_cast<size_t>(fx) * F.ny + fy;
+          const float xc = kCycleOmega * F.dinv[i] * F.r[i];
+          sum += F.r[i] - level_ax(F, i, fx, fy, xc, pre);
+        }
+      }
+      C.r[c] = sum;
+    }
+    __syncthreads();
+  }
+  double rz = 0.0;
+  {
+    const LevelRef B = level_ref(D, V, K, site);
+    const uint32_t n = V.coarse_n;
+    const float* inv = D.cinv + V.cinv;
+    for (uint32_t row = threadIdx.x; row < n; row += NT) {
+      const float* m = inv + static_cast<size_t>(row) * n;
+      float v = 0.0f;
+      for (uint32_t j = 0; j < n; ++j) v += m[j] * B.r[j];
+      B.e[row] = v;
+      if (K == 0) rz += static_cast<double>(v) * static_cast<double>(B.r[row]);
+    }
+    __syncthreads();
+  }
+  for (int l = static_cast<int>(K) - 1; l >= 0; --l) {
+    const LevelRef F = level_ref(D, V, static_cast<uint32_t>(l), site);
+    const LevelRef C = level_ref(D, V, static_cast<uint32_t>(l) + 1, site);
+    auto corrected = [&](uint32_t gx, uint32_t gy) {
+      const size_t j = static_cast<size_t>(gx) * F.ny + gy;
+      return kCycleOmega * F.dinv[j] * F.r[j] +
+             kCycleCorrection * C.e[static_cast<size_t>(gx / 2) * C.ny + gy / 2];
+    };
+    const size_t cells = static_cast<size_t>(F.nx) * F.ny;
+    for (size_t i = threadIdx.x; i < cells; i += NT) {
+      const uint32_t fx = static_cast<uint32_t>(i / F.ny), fy = static_cast<uint32_t>(i % F.ny);
+      const float ri = F.r[i];
+      const float xc = corrected(fx, fy);
+      const float z = xc + kCycleOmega * F.dinv[i] * (ri - level_ax(F, i, fx, fy, xc, corrected));
+      F.e[i] = z;
+      if (l == 0) rz += static_cast<double>(z) * static_cast<double>(ri);
+    }
+    __syncthreads();
+  }
+  double out[1] = {rz};
+  block_sums<NT, 1>(out, lds);
+  return out[0];
+}
+
 // The pressure solve: one workgroup per instance (order[b], largest first),
-// FP64 refinement around FP32 Jacobi-PCG corrections, then every face
-// velocity of the instance.
+// FP64 refinement around FP32 corrections by conjugate gradient with the
+// instance's multilevel cycle, then every face velocity of the instance.
 template <int NT, int TY>
 __global__ __launch_bounds__(NT) void k_batch_flow(BatchData D, BatchParams P) {
   __shared__ double s_red[NT / kWave];
@@ -204,7 +337,6 @@ __global__ __launch_bounds__(NT) void k_batch_flow(BatchData D, BatchParams P) {
   const float* cxl = D.cxl + o;
   const float* cyl = D.cyl + o;
   const float* bc = D.bc + o;
-  const float* dinv = D.dinv + o;
   const float* rhs = D.rhs + o;
   double* x64 = D.x64 + o;
   float* xv = D.x + o;
@@ -221,30 +353,35 @@ __global__ __launch_bounds__(NT) void k_batch_flow(BatchData D, BatchParams P) {
   };
   constexpr uint32_t kRefinements = 16;
   constexpr double kInner = 1.0e-4;  // relative tolerance of each FP32 correction
+  const BatchLevels& V = D.levels[inst];
+  float* z = D.z + o;
   uint32_t it = 0, failed = 0;
   double rr64 = 0.0, bb = 0.0, threshold = 0.0;
   for (uint32_t ref = 0; ref <= kRefinements; ++ref) {
-    // FP64 true residual; its FP32 copy starts the correction e = 0, p = D^-1 r.
-    double s3[3] = {0.0, 0.0, 0.0};  // r.r (FP64), b.b, r.z (FP32 r)
+    // FP64 true residual; its FP32 copy starts the correction e = 0.
+    double s2[2] = {0.0, 0.0};  // r.r (FP64), b.b
     sweep([&](uint32_t x, uint32_t y, size_t i) {
       const double res = residual_f64(x64, cxl, cyl, bc, rhs, I.inlet, x, y, nx, ny, i, isx, isy);
       const double b = static_cast<double>(I.inlet) * static_cast<double>(rhs[i]);
-      const float ri = static_cast<float>(res);
-      const float zi = dinv[i] * ri;
-      r[i] = ri;
-      p[i] = zi;
+      r[i] = static_cast<float>(res);
       xv[i] = 0.0f;
-      s3[0] += res * res;
-      s3[1] += b * b;
-      s3[2] += static_cast<double>(ri) * static_cast<double>(zi);
+      s2[0] += res * res;
+      s2[1] += b * b;
     });
-    block_sums<NT, 3>(s3, s_red);
-    rr64 = s3[0];
-    bb = s3[1];
+    block_sums<NT, 2>(s2, s_red);
+    rr64 = s2[0];
+    bb = s2[1];
     const double bound = static_cast<double>(P.tolerance) * sqrt(bb);
     threshold = bound * bound;
     if (rr64 <= threshold || ref == kRefinements || it >= P.max_iterations) break;
-    double rz = s3[2], rr = rr64;
+    double rz = batch_cycle<NT>(D, V, o, s_red);
+    if (!(rz > 0.0) || !isfinite(rz)) {
+      failed = 1;
+      break;
+    }
+    sweep([&](uint32_t x, uint32_t y, size_t i) { p[i] = z[i]; });
+    __syncthreads();
+    double rr = rr64;
     const double inner = kInner * kInner * rr64;
     while (rr > inner && it < P.max_iterations) {
       double s1[1] = {0.0};
@@ -259,22 +396,26 @@ __global__ __launch_bounds__(NT) void k_batch_flow(BatchData D, BatchParams P) {
         break;
       }
       const float alpha = static_cast<float>(rz / s1[0]);
-      double s2[2] = {0.0, 0.0};
+      double s3[1] = {0.0};
       sweep([&](uint32_t x, uint32_t y, size_t i) {
         xv[i] = xv[i] + alpha * p[i];
         const float ri = r[i] - alpha * q[i];
         r[i] = ri;
-        const float zi = dinv[i] * ri;
-        s2[0] += static_cast<double>(ri) * static_cast<double>(zi);
-        s2[1] += static_cast<double>(ri) * static_cast<double>(ri);
+        s3[0] += static_cast<double>(ri) * static_cast<double>(ri);
       });
-      block_sums<NT, 2>(s2, s_red);
-      const float beta = static_cast<float>(s2[0] / rz);
-      rz = s2[0];
-      rr = s2[1];
-      sweep([&](uint32_t x, uint32_t y, size_t i) { p[i] = dinv[i] * r[i] + beta * p[i]; });
-      __syncthreads();
+      block_sums<NT, 1>(s3, s_red);
+      rr = s3[0];
       ++it;
+      if (rr <= inner) break;
+      const double rz_next = batch_cycle<NT>(D, V, o, s_red);
+      if (!(rz_next > 0.0) || !isfinite(rz_next)) {
+        failed = 1;
+        break;
+      }
+      const float beta = static_cast<float>(rz_next / rz);
+      rz = rz_next;
+      sweep([&](uint32_t x, uint32_t y, size_t i) { p[i] = z[i] + beta * p[i]; });
+      __syncthreads();
     }
     if (failed != 0) break;
     sweep([&](uint32_t x, uint32_t y, size_t i) { x64[i] += static_cast<double>(xv[i]); });
@@ -575,11 +716,6 @@ __global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
   }
 }
 
-__global__ void k_batch_dinv(const float* __restrict__ diag, float* __restrict__ dinv, size_t n) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dinv[i] = diag[i] > 0.0f ? 1.0f / diag[i] : 0.0f;
-}
-
 using FlowKernel = void (*)(BatchData, BatchParams);
 using StepKernel = void (*)(BatchData, BatchParams, const BatchItem*, double*, uint32_t, float,
                             uint32_t, uint32_t);
@@ -629,8 +765,11 @@ struct BatchDevice {
   DeviceBuffer<BatchDesc> d_desc;
   DeviceBuffer<uint32_t> d_order, iterations, flags;
   DeviceBuffer<uint8_t> solid;
-  DeviceBuffer<float> mobility, cond, cxl, cyl, bc, diag, dinv, rhs, phi, x, r, p, q, xf, yf, la,
-      lb, residual, saved;
+  DeviceBuffer<float> mobility, cond, cxl, cyl, bc, diag, rhs, phi, x, r, p, q, z, xf, yf, la, lb,
+      residual, saved;
+  std::vector<BatchLevels> levels;  // per instance
+  DeviceBuffer<BatchLevels> d_levels;
+  DeviceBuffer<float> lwx, lwy, lbc, ldinv, lr, le, cinv;
   DeviceBuffer<double> x64, bnorm2, partials;
   BatchParams params{};
   bool has_phi = false;
@@ -651,12 +790,39 @@ struct BatchDevice {
 namespace {
 
 BatchData batch_data(BatchDevice& B) {
-  return BatchData{B.d_desc.data(),   B.d_order.data(),  B.solid.data(),   B.cond.data(),
-                   B.cxl.data(),      B.cyl.data(),      B.bc.data(),      B.dinv.data(),
-                   B.rhs.data(),      B.has_phi ? B.phi.data() : nullptr,  B.x64.data(),
-                   B.x.data(),        B.r.data(),        B.p.data(),       B.q.data(),
-                   B.xf.data(),       B.yf.data(),       B.la.data(),      B.lb.data(),
-                   B.iterations.data(), B.residual.data(), B.bnorm2.data(), B.flags.data()};
+  BatchData D{};
+  D.desc = B.d_desc.data();
+  D.order = B.d_order.data();
+  D.solid = B.solid.data();
+  D.cond = B.cond.data();
+  D.cxl = B.cxl.data();
+  D.cyl = B.cyl.data();
+  D.bc = B.bc.data();
+  D.rhs = B.rhs.data();
+  D.phi = B.has_phi ? B.phi.data() : nullptr;
+  D.x64 = B.x64.data();
+  D.x = B.x.data();
+  D.r = B.r.data();
+  D.p = B.p.data();
+  D.q = B.q.data();
+  D.xf = B.xf.data();
+  D.yf = B.yf.data();
+  D.la = B.la.data();
+  D.lb = B.lb.data();
+  D.iterations = B.iterations.data();
+  D.residual = B.residual.data();
+  D.bnorm2 = B.bnorm2.data();
+  D.flags = B.flags.data();
+  D.z = B.z.data();
+  D.levels = B.d_levels.data();
+  D.lwx = B.lwx.data();
+  D.lwy = B.lwy.data();
+  D.lbc = B.lbc.data();
+  D.ldinv = B.ldinv.data();
+  D.lr = B.lr.data();
+  D.le = B.le.data();
+  D.cinv = B.cinv.data();
+  return D;
 }
 
 int shape_of(const BatchDevice& B) { return B.shape >= 0 ? B.shape : kDefaultShape; }
@@ -717,6 +883,88 @@ void run_steps(Context& ctx, BatchDevice& B, int shape, float dt, uint32_t steps
     check(hipGraphLaunch(B.graph, ctx.stream), "launch graph");
   }
   B.current = static_cast<int>(steps & 1);
+}
+
+}  // namespace
+
+namespace {
+
+// Each instance's multilevel hierarchy: level shapes and offsets, the level
+// operators on the device (level 0 from the assembled couplings), and the
+// dense inverse of every coarsest level, computed on the host.
+void build_hierarchies(Context& ctx, BatchDevice& B) {
+  B.levels.assign(B.count, BatchLevels{});
+  uint64_t pool = B.sites, inverses = 0;
+  for (uint32_t k = 0; k < B.count; ++k) {
+    const BatchDesc& d = B.desc[k];
+    BatchLevels& L = B.levels[k];
+    uint32_t nx = d.nx, ny = d.ny;
+    L.off[0] = d.site;
+    L.nx[0] = nx;
+    L.ny[0] = ny;
+    L.count = 1;
+    while (static_cast<uint64_t>(nx) * ny > kBatchCoarse) {
+      if (L.count == kBatchLevels) throw Error(MSIM_OVERFLOW, "batch instance needs too many levels");
+      nx = (nx + 1) / 2;
+      ny = (ny + 1) / 2;
+      L.off[L.count] = pool;
+      L.nx[L.count] = nx;
+      L.ny[L.count] = ny;
+      pool += static_cast<uint64_t>(nx) * ny;
+      ++L.count;
+    }
+    L.coarse_n = nx * ny;
+    L.cinv = inverses;
+    inverses += static_cast<uint64_t>(L.coarse_n) * L.coarse_n;
+  }
+  for (auto* b : {&B.lwx, &B.lwy, &B.lbc, &B.ldinv, &B.lr, &B.le}) b->reserve(pool);
+  B.cinv.reserve(std::max<uint64_t>(inverses, 1));
+  B.d_levels.reserve(B.count);
+  check(hipMemcpyAsync(B.d_levels.data(), B.levels.data(), B.count * sizeof(BatchLevels),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload hierarchies");
+  for (uint32_t k = 0; k < B.count; ++k) {
+    const BatchDesc& d = B.desc[k];
+    const BatchLevels& L = B.levels[k];
+    const float isx = 1.0f / (d.dx * d.dx), isy = 1.0f / (d.dy * d.dy);
+    mg_level0_operator(ctx.stream, B.cxl.data() + d.site, B.cyl.data() + d.site,
+                       B.bc.data() + d.site, B.lwx.data() + d.site, B.lwy.data() + d.site,
+                       B.lbc.data() + d.site, B.ldinv.data() + d.site, d.nx, d.ny, isx, isy);
+    for (uint32_t l = 1; l < L.count; ++l) {
+      const uint64_t f = L.off[l - 1], c = L.off[l];
+      mg_coarse_operator(ctx.stream, B.lwx.data() + f, B.lwy.data() + f, B.lbc.data() + f,
+                         L.nx[l - 1], L.ny[l - 1], B.lwx.data() + c, B.lwy.data() + c,
+                         B.lbc.data() + c, B.ldinv.data() + c, L.nx[l], L.ny[l]);
+    }
+  }
+  // The coarsest operators to the host, their inverses back.
+  std::vector<std::vector<float>> wx(B.count), wy(B.count), bc(B.count);
+  for (uint32_t k = 0; k < B.count; ++k) {
+    const BatchLevels& L = B.levels[k];
+    const uint64_t c = L.off[L.count - 1];
+    for (auto* v : {&wx[k], &wy[k], &bc[k]}) v->resize(L.coarse_n);
+    check(hipMemcpyAsync(wx[k].data(), B.lwx.data() + c, L.coarse_n * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download coarse operator");
+    check(hipMemcpyAsync(wy[k].data(), B.lwy.data() + c, L.coarse_n * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download coarse operator");
+    check(hipMemcpyAsync(bc[k].data(), B.lbc.data() + c, L.coarse_n * sizeof(float),
+                         hipMemcpyDeviceToHost, ctx.stream),
+          "download coarse operator");
+  }
+  check(hipStreamSynchronize(ctx.stream), "coarse operators");
+  std::vector<float> all(inverses);
+  for (uint32_t k = 0; k < B.count; ++k) {
+    const BatchLevels& L = B.levels[k];
+    const std::vector<float> inv =
+        mg_coarse_inverse(wx[k], wy[k], bc[k], L.nx[L.count - 1], L.ny[L.count - 1]);
+    std::copy(inv.begin(), inv.end(), all.begin() + static_cast<std::ptrdiff_t>(L.cinv));
+  }
+  check(hipMemcpyAsync(B.cinv.data(), all.data(), all.size() * sizeof(float),
+                       hipMemcpyHostToDevice, ctx.stream),
+        "upload coarse inverses");
+  check(hipStreamSynchronize(ctx.stream), "upload coarse inverses");
 }
 
 }  // namespace
@@ -790,8 +1038,8 @@ void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
   }
   const size_t n = site;
   B.solid.reserve(n);
-  for (auto* b : {&B.mobility, &B.cond, &B.cxl, &B.cyl, &B.bc, &B.diag, &B.dinv, &B.rhs, &B.phi,
-                  &B.x, &B.r, &B.p, &B.q, &B.la, &B.lb, &B.saved})
+  for (auto* b : {&B.mobility, &B.cond, &B.cxl, &B.cyl, &B.bc, &B.diag, &B.rhs, &B.phi, &B.x,
+                  &B.r, &B.p, &B.q, &B.z, &B.la, &B.lb, &B.saved})
     b->reserve(n);
   B.x64.reserve(n);
   B.xf.reserve(xface);
@@ -847,9 +1095,7 @@ void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
                     B.bc.data() + d.site, B.cxl.data() + d.site, B.cyl.data() + d.site, d.nx,
                     d.ny, d.dx, d.dy, d.dz, FlowEdges{});
   }
-  k_batch_dinv<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, ctx.stream>>>(
-      B.diag.data(), B.dinv.data(), n);
-  check(hipGetLastError(), "launch k_batch_dinv");
+  build_hierarchies(ctx, B);
   B.params.tolerance = params.relative_tolerance;
   B.params.max_iterations = params.max_iterations;
   B.params.vmax = params.monod_vmax;

@@ -459,6 +459,19 @@ void port_flow_solve(Context& ctx, const PortPlan& plan, const msim_port_flow_pa
                      float* x_faces, float* y_faces, float* z_faces, msim_port_flow_report& report,
                      msim_port_state* states);
 
+// mg.hip: the multilevel operators of one grid, for callers that keep their
+// own hierarchy (the batched sweep). Couplings are to the lower neighbour,
+// already over the spacing squared; dinv is one over the diagonal (0 where
+// the diagonal is 0). The coarse inverse is dense (nx ny x nx ny floats).
+std::vector<float> mg_coarse_inverse(const std::vector<float>& wx, const std::vector<float>& wy,
+                                     const std::vector<float>& bc, uint32_t nx, uint32_t ny);
+void mg_level0_operator(hipStream_t stream, const float* cxl, const float* cyl, const float* bc,
+                        float* wx, float* wy, float* bcl, float* dinv, uint32_t nx, uint32_t ny,
+                        float isx, float isy);
+void mg_coarse_operator(hipStream_t stream, const float* fwx, const float* fwy, const float* fbc,
+                        uint32_t fnx, uint32_t fny, float* wx, float* wy, float* bc, float* dinv,
+                        uint32_t nx, uint32_t ny);
+
 // batch.hip: the batched sweep (msim_hip_batch_*).
 void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
                   const msim_batch_params& params);

@@ -954,6 +954,34 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
 
 }  // namespace
 
+std::vector<float> mg_coarse_inverse(const std::vector<float>& wx, const std::vector<float>& wy,
+                                     const std::vector<float>& bc, uint32_t nx, uint32_t ny) {
+  return coarse_inverse(wx, wy, bc, nx, ny);
+}
+
+void mg_level0_operator(hipStream_t stream, const float* cxl, const float* cyl, const float* bc,
+                        float* wx, float* wy, float* bcl, float* dinv, uint32_t nx, uint32_t ny,
+                        float isx, float isy) {
+  const size_t n = static_cast<size_t>(nx) * ny;
+  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
+  k_mg_level0<<<blocks, 256, 0, stream>>>(cxl, cyl, bc, wx, wy, bcl, n, isx, isy);
+  check(hipGetLastError(), "launch k_mg_level0");
+  k_mg_dinv<<<blocks, 256, 0, stream>>>(wx, wy, bcl, dinv, nx, ny);
+  check(hipGetLastError(), "launch k_mg_dinv");
+}
+
+void mg_coarse_operator(hipStream_t stream, const float* fwx, const float* fwy, const float* fbc,
+                        uint32_t fnx, uint32_t fny, float* wx, float* wy, float* bc, float* dinv,
+                        uint32_t nx, uint32_t ny) {
+  const size_t n = static_cast<size_t>(nx) * ny;
+  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
+  const LevelView f{fwx, fwy, fbc, nullptr, fnx, fny};
+  k_mg_coarsen<<<blocks, 256, 0, stream>>>(f, wx, wy, bc, nx, ny);
+  check(hipGetLastError(), "launch k_mg_coarsen");
+  k_mg_dinv<<<blocks, 256, 0, stream>>>(wx, wy, bc, dinv, nx, ny);
+  check(hipGetLastError(), "launch k_mg_dinv");
+}
+
 void mg_prepare(Context& ctx) {
   auto& f = ctx.flow;
   auto& mg = f.mg;
