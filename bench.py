@@ -101,7 +101,8 @@ def cpu_baseline(w, fp64_iterations: int, threads: int, budget_s: float):
                             f"(detail::solve_depth_averaged_pressure) on the same 4096x2048 "
                             f"system in {t:.2f} s, scaled by the {fp64_iterations} iterations "
                             f"the same algorithm takes to 1e-8"),
-                    seconds_per_iteration=per_iteration)
+                    seconds_per_iteration=per_iteration,
+                    extrapolation_check=extrapolation_check(ref))
     from oracle import Oracle, OracleError
     oracle = Oracle()
     k = 20
@@ -114,6 +115,30 @@ def cpu_baseline(w, fp64_iterations: int, threads: int, budget_s: float):
     return dict(value=1.0 / (t / k * fp64_iterations), unit="solves/s", cores=1, kind="port",
                 sample=f"{k} iterations of the C oracle (assembly included) in {t:.2f} s",
                 seconds_per_iteration=t / k)
+
+
+def extrapolation_check(ref):
+    """The scaling above checked where a whole solve fits the budget: the
+    reference's complete solve_depth_averaged_flow_cpu of the 1024 x 512
+    pillar array (assembly, every iteration, velocity, scaling) at 1.2e-7,
+    the tightest tolerance it accepts (float epsilon, cpu_flow.cpp:125-130),
+    timed, against a tenth of its iterations timed and scaled the same way."""
+    from oracle import Grid as OGrid
+    from microsimulator_amd.workloads import config
+    w2 = config("c2")
+    grid = OGrid(w2.nx, w2.ny, 1, w2.dx, w2.dy, w2.dz, w2.obstacles)
+    t0 = time.perf_counter()
+    _, rep = ref.solve_flow(grid, w2.mobility, 0, 100.0, 1.2e-7, 400000)
+    whole = time.perf_counter() - t0
+    iterations = rep["iterations"]
+    system = ref.flow_system(grid, w2.mobility, 0)
+    system.time_iterations(5)
+    k = max(10, iterations // 10)
+    sample = system.time_iterations(k)
+    scaled = sample / k * iterations
+    return dict(workload="1024x512 pillar array", tolerance=1.2e-7, iterations=iterations,
+                whole_solve_s=round(whole, 3), sample_iterations=k,
+                scaled_s=round(scaled, 3), whole_over_scaled=round(whole / scaled, 3))
 
 
 def cpu_adr_baseline(w, faces, threads: int, steps: int):

@@ -15,7 +15,7 @@
 //                (adr_site.hpp), with FP64 mass tallies.
 //
 // A step runs over work items (instance, block of rows) of a few thousand
-// sites, spread over the whole chip, one launch per width class. Each item
+// sites, spread over the whole chip, in one launch. Each item
 // recomputes its rows' FP64 pressure residual (the step's flow check), writes
 // the face velocities it owns, and advances its sites, so no item waits on
 // another. Instances up to 256 columns wide take a marching kernel: one
@@ -533,34 +533,52 @@ __global__ __launch_bounds__(NT) void k_batch_step(BatchData D, BatchParams P,
   }
 }
 
-// The same step for an instance at most TY columns wide, marching: work item
-// base + blockIdx.x owns whole rows [x0, x1) of its instance, one thread per
-// column, and walks x, so each array is read once; the x neighbours come
-// from registers, the y neighbours from a row in LDS, and each x face is
-// computed once (the upper face of a row is the next row's lower face).
+// The same step for instances at most 256 columns wide, marching, every
+// width class in one launch. A workgroup of 256 threads holds 256 / W groups
+// of W threads (W = 64, 128 or 256: the width class), each group one work
+// item (rows [x0, x1) of an instance), one thread per column. A group walks
+// x, so each array is read once; the x neighbours come from registers, the y
+// neighbours from the group's segment of an LDS row, and each x face is
+// computed once (the upper face of a row is the next row's lower face). The
+// groups of a workgroup walk the same number of rows (the most any of them
+// has; a group past its own rows idles), so the row barriers are uniform.
 // Every site's arithmetic is the general kernel's, so the two agree bit for
 // bit.
-template <int TY>
-__global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
-                                                    const BatchItem* __restrict__ items,
-                                                    double* __restrict__ partials, uint32_t step,
-                                                    float dt, uint32_t base, uint32_t total) {
+constexpr int kMarchThreads = 256;
+constexpr int kMarchGroups = 4;  // groups per workgroup at the narrowest width
+
+struct MarchBlock {
+  uint32_t width;                 // W
+  uint32_t rows;                  // rows every group walks
+  uint32_t item[kMarchGroups];    // work items; 0xffffffff: none
+};
+
+__global__ __launch_bounds__(kMarchThreads) void k_batch_march(
+    BatchData D, BatchParams P, const BatchItem* __restrict__ items,
+    const MarchBlock* __restrict__ blocks, double* __restrict__ partials, uint32_t step, float dt,
+    uint32_t total) {
   struct Row {
     double p;
     float cond, c, cxl, cyl, bc, rhs, phi;
     bool solid;
   };
-  __shared__ double s_p[2][TY];
-  __shared__ float s_cond[2][TY], s_c[2][TY], s_cyl[2][TY];
-  __shared__ uint8_t s_s[2][TY];
-  __shared__ double s_red[4 * (TY / kWave)];
-  const BatchItem item = items[base + blockIdx.x];
+  __shared__ double s_p[2][kMarchThreads];
+  __shared__ float s_cond[2][kMarchThreads], s_c[2][kMarchThreads], s_cyl[2][kMarchThreads];
+  __shared__ uint8_t s_s[2][kMarchThreads];
+  __shared__ double s_red[4][kMarchThreads / kWave];
+  const MarchBlock blk = blocks[blockIdx.x];
+  const uint32_t W = blk.width;
+  const uint32_t group = threadIdx.x / W;
+  const uint32_t y = threadIdx.x % W;
+  const uint32_t seg = group * W;  // the group's segment of the LDS rows
+  const uint32_t item_index = blk.item[group];
+  const bool has_item = item_index != 0xffffffffu;
+  const BatchItem item = has_item ? items[item_index] : BatchItem{0, 0, 0, 0};
   const uint32_t inst = item.inst;
   const BatchDesc I = D.desc[inst];
   const uint32_t nx = I.nx, ny = I.ny;
   const size_t o = I.site;
-  const uint32_t y = threadIdx.x;
-  const bool valid = y < ny;
+  const bool valid = has_item && y < ny;
   const float* in = ((step & 1) == 0 ? D.la : D.lb) + o;
   float* out = ((step & 1) == 0 ? D.lb : D.la) + o;
   const bool monod = D.phi != nullptr;
@@ -583,9 +601,10 @@ __global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
   A.boundary[0][0] = I.reservoir;
   A.vmax[0] = P.vmax;
   A.monod_k[0] = P.monod_k;
+  const int x0 = static_cast<int>(item.x0), x1 = static_cast<int>(item.x1);
   auto load = [&](int xx, bool full) {
     Row w{};
-    if (!valid || xx < 0 || xx >= static_cast<int>(nx)) return w;
+    if (!valid || xx < 0 || xx >= static_cast<int>(nx) || xx > x1) return w;
     const size_t i = o + static_cast<size_t>(xx) * ny + y;
     w.p = D.x64[i];
     w.cond = D.cond[i];
@@ -619,48 +638,51 @@ __global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
     if ((has_lower && lo.solid) || (has_upper && up.solid)) return 0.0f;
     return flux / I.dz;
   };
-  // Face y fy of row buf from the LDS row (face_y's arithmetic).
+  // Face y fy of row buf from the group's LDS row (face_y's arithmetic).
   auto face_y_of = [&](int buf, uint32_t fy) -> float {
     const bool has_lower = fy > 0, has_upper = fy < ny;
     double value;
     if (has_lower && has_upper) {
-      const float fm = harmonic(s_cond[buf][fy - 1], s_cond[buf][fy]);
-      value = -static_cast<double>(fm) * (s_p[buf][fy] - s_p[buf][fy - 1]) /
+      const float fm = harmonic(s_cond[buf][seg + fy - 1], s_cond[buf][seg + fy]);
+      value = -static_cast<double>(fm) * (s_p[buf][seg + fy] - s_p[buf][seg + fy - 1]) /
               static_cast<double>(I.dy);
     } else {
-      const double ps = has_upper ? s_p[buf][fy] : s_p[buf][fy - 1];
+      const double ps = has_upper ? s_p[buf][seg + fy] : s_p[buf][seg + fy - 1];
       value = static_cast<double>(0.0f) * (static_cast<double>(0.0f) - ps);
     }
     const float flux = static_cast<float>(value);
-    if ((has_lower && s_s[buf][fy - 1] != 0) || (has_upper && s_s[buf][fy] != 0)) return 0.0f;
+    if ((has_lower && s_s[buf][seg + fy - 1] != 0) || (has_upper && s_s[buf][seg + fy] != 0))
+      return 0.0f;
     return flux / I.dz;
   };
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t bad = 0;
-  const int x0 = static_cast<int>(item.x0), x1 = static_cast<int>(item.x1);
   Row prev = load(x0 - 1, false);
   Row cur = load(x0, true);
   Row next = load(x0 + 1, x0 + 1 < x1);
   float uxl = face_x_of(prev, cur, x0);
-  for (int xx = x0; xx < x1; ++xx) {
+  for (uint32_t r = 0; r < blk.rows; ++r) {
+    const int xx = x0 + static_cast<int>(r);
+    const bool live = valid && xx < x1;
     const Row ahead = xx + 2 <= x1 ? load(xx + 2, xx + 2 < x1) : Row{};
-    const int buf = xx & 1;
-    s_p[buf][y] = cur.p;
-    s_cond[buf][y] = cur.cond;
-    s_c[buf][y] = cur.c;
-    s_cyl[buf][y] = cur.cyl;
-    s_s[buf][y] = cur.solid ? 1 : 0;
+    const int buf = static_cast<int>(r & 1);
+    s_p[buf][threadIdx.x] = cur.p;
+    s_cond[buf][threadIdx.x] = cur.cond;
+    s_c[buf][threadIdx.x] = cur.c;
+    s_cyl[buf][threadIdx.x] = cur.cyl;
+    s_s[buf][threadIdx.x] = cur.solid ? 1 : 0;
     __syncthreads();
     const float uxu = face_x_of(cur, next, xx + 1);
-    if (valid) {
+    if (live) {
       const size_t i = static_cast<size_t>(xx) * ny + y;
       // residual_f64's arithmetic.
       const double c = cur.p;
       double a = static_cast<double>(cur.bc) * c;
       if (xx > 0) a += (static_cast<double>(cur.cxl) * isx) * (c - prev.p);
       if (xx + 1 < static_cast<int>(nx)) a += (static_cast<double>(next.cxl) * isx) * (c - next.p);
-      if (y > 0) a += (static_cast<double>(cur.cyl) * isy) * (c - s_p[buf][y - 1]);
-      if (y + 1 < ny) a += (static_cast<double>(s_cyl[buf][y + 1]) * isy) * (c - s_p[buf][y + 1]);
+      if (y > 0) a += (static_cast<double>(cur.cyl) * isy) * (c - s_p[buf][seg + y - 1]);
+      if (y + 1 < ny)
+        a += (static_cast<double>(s_cyl[buf][seg + y + 1]) * isy) * (c - s_p[buf][seg + y + 1]);
       const double res = static_cast<double>(I.inlet) * static_cast<double>(cur.rhs) - a;
       acc[0] += res * res;
       const float uyl = face_y_of(buf, y);
@@ -679,16 +701,18 @@ __global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
         SiteGeometry g;
         g.clo[0] = xlo_edge ? false : prev.solid;
         g.cup[0] = xhi_edge ? false : next.solid;
-        g.clo[1] = ylo_edge ? true : s_s[buf][y - 1] != 0;
-        g.cup[1] = yhi_edge ? true : s_s[buf][y + 1] != 0;
+        g.clo[1] = ylo_edge ? true : s_s[buf][seg + y - 1] != 0;
+        g.cup[1] = yhi_edge ? true : s_s[buf][seg + y + 1] != 0;
         g.ulo[0] = uxl;
         g.uup[0] = uxu;
         g.ulo[1] = uyl;
         g.uup[1] = uyu;
         const float l0 = xlo_edge ? exterior(A, 0, 0, current, 0.0f, y) : prev.c;
         const float u0 = xhi_edge ? exterior(A, 1, 0, current, 0.0f, y) : next.c;
-        const float l1 = ylo_edge ? exterior(A, 2, 0, current, 0.0f, static_cast<uint32_t>(xx)) : s_c[buf][y - 1];
-        const float u1 = yhi_edge ? exterior(A, 3, 0, current, 0.0f, static_cast<uint32_t>(xx)) : s_c[buf][y + 1];
+        const float l1 = ylo_edge ? exterior(A, 2, 0, current, 0.0f, static_cast<uint32_t>(xx))
+                                  : s_c[buf][seg + y - 1];
+        const float u1 = yhi_edge ? exterior(A, 3, 0, current, 0.0f, static_cast<uint32_t>(xx))
+                                  : s_c[buf][seg + y + 1];
         const float candidate =
             fe_update_f32(A, g, current, l0, u0, l1, u1, 0, dt, 0.0f, 0.0f, cur.phi);
         if (!isfinite(candidate) || candidate < 0.0f) bad = 1;
@@ -707,11 +731,21 @@ __global__ __launch_bounds__(TY) void k_batch_march(BatchData D, BatchParams P,
     next = ahead;
   }
   if (bad) atomicOr(&D.flags[inst], 4u);
-  block_sums<TY, 4>(acc, s_red);
-  if (threadIdx.x == 0) {
+  // Per group: wave sums, then the group's waves in order.
+  const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) s_red[k][wave] = v;
+  }
+  __syncthreads();
+  if (has_item && y == 0) {
+    const uint32_t first = seg / kWave, waves = W / kWave;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      partials[(static_cast<size_t>(step) * total + base + blockIdx.x) * 4 + k] = acc[k];
+      double t = 0.0;
+      for (uint32_t w = 0; w < waves; ++w) t += s_red[k][first + w];
+      partials[(static_cast<size_t>(step) * total + item_index) * 4 + k] = t;
     }
   }
 }
@@ -735,12 +769,10 @@ const Shape kShapes[] = {
 };
 constexpr int kShapeCount = static_cast<int>(sizeof(kShapes) / sizeof(kShapes[0]));
 constexpr int kDefaultShape = 1;
-// Instance classes by width: marching kernels for at most 64, 128 and 256
-// columns, then the general kernel.
+// Instance classes by width: the marching kernel's groups of 64, 128 and 256
+// threads for at most that many columns, then the general kernel.
 constexpr int kClasses = 4;
 constexpr uint32_t kClassWidth[kClasses - 1] = {64, 128, 256};
-const StepKernel kMarch[kClasses - 1] = {k_batch_march<64>, k_batch_march<128>,
-                                         k_batch_march<256>};
 
 int class_of(uint32_t ny) {
   for (int c = 0; c < kClasses - 1; ++c) {
@@ -762,6 +794,8 @@ struct BatchDevice {
   std::vector<std::vector<BatchItem>> items;  // per shape, grouped by instance class
   std::vector<std::array<uint32_t, 5>> class_begin;  // per shape: first item of each class, end
   std::vector<std::unique_ptr<DeviceBuffer<BatchItem>>> d_items;
+  std::vector<std::vector<MarchBlock>> march;  // per shape: workgroups of the marching launch
+  std::vector<std::unique_ptr<DeviceBuffer<MarchBlock>>> d_march;
   DeviceBuffer<BatchDesc> d_desc;
   DeviceBuffer<uint32_t> d_order, iterations, flags;
   DeviceBuffer<uint8_t> solid;
@@ -832,14 +866,20 @@ void launch_step(Context& ctx, BatchDevice& B, const BatchData& D, int shape, ui
   const Shape& s = kShapes[shape];
   const auto& begin = B.class_begin[shape];
   const uint32_t total = static_cast<uint32_t>(B.items[shape].size());
-  for (int c = 0; c < kClasses; ++c) {
-    const uint32_t base = begin[c], n = begin[c + 1] - begin[c];
-    if (n == 0) continue;
-    const bool march = c < kClasses - 1;
-    hipLaunchKernelGGL(march ? kMarch[c] : s.step, dim3(n),
-                       dim3(march ? kClassWidth[c] : static_cast<uint32_t>(s.threads)), 0,
-                       ctx.stream, D, B.params, B.d_items[shape]->data(), B.partials.data(), step,
-                       dt, base, total);
+  // Every instance at most 256 columns wide: one launch.
+  const uint32_t marching = static_cast<uint32_t>(B.march[shape].size());
+  if (marching > 0) {
+    k_batch_march<<<marching, kMarchThreads, 0, ctx.stream>>>(
+        D, B.params, B.d_items[shape]->data(), B.d_march[shape]->data(), B.partials.data(), step,
+        dt, total);
+    check(hipGetLastError(), "launch batch step");
+  }
+  // Wider instances: the general sweep.
+  const uint32_t base = begin[kClasses - 1], n = begin[kClasses] - begin[kClasses - 1];
+  if (n > 0) {
+    hipLaunchKernelGGL(s.step, dim3(n), dim3(static_cast<uint32_t>(s.threads)), 0, ctx.stream, D,
+                       B.params, B.d_items[shape]->data(), B.partials.data(), step, dt, base,
+                       total);
     check(hipGetLastError(), "launch batch step");
   }
 }
@@ -1015,6 +1055,8 @@ void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
   B.items.assign(kShapeCount, {});
   B.class_begin.assign(kShapeCount, {});
   B.d_items.clear();
+  B.march.clear();
+  B.d_march.clear();
   for (int s = 0; s < kShapeCount; ++s) {
     for (int c = 0; c < kClasses; ++c) {
       B.class_begin[s][c] = static_cast<uint32_t>(B.items[s].size());
@@ -1029,6 +1071,34 @@ void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
     }
     B.class_begin[s][kClasses] = static_cast<uint32_t>(B.items[s].size());
     if (B.items[s].size() > 0x7FFFFFFFu) throw Error(MSIM_OVERFLOW, "too many batch work items");
+    // Workgroups of the marching launch: 256 / W consecutive items of a
+    // class of width W, walking the most rows any of them has.
+    std::vector<MarchBlock> blocks;
+    for (int c = 0; c < kClasses - 1; ++c) {
+      const uint32_t width = kClassWidth[c];
+      const uint32_t per = static_cast<uint32_t>(kMarchThreads) / width;
+      for (uint32_t j = B.class_begin[s][c]; j < B.class_begin[s][c + 1]; j += per) {
+        MarchBlock b{};
+        b.width = width;
+        for (uint32_t g = 0; g < static_cast<uint32_t>(kMarchGroups); ++g) {
+          const uint32_t it = j + g;
+          if (g < per && it < B.class_begin[s][c + 1]) {
+            b.item[g] = it;
+            b.rows = std::max(b.rows, B.items[s][it].x1 - B.items[s][it].x0);
+          } else {
+            b.item[g] = 0xffffffffu;
+          }
+        }
+        blocks.push_back(b);
+      }
+    }
+    auto mbuf = std::make_unique<DeviceBuffer<MarchBlock>>();
+    mbuf->reserve(std::max<size_t>(blocks.size(), 1));
+    check(hipMemcpyAsync(mbuf->data(), blocks.data(), blocks.size() * sizeof(MarchBlock),
+                         hipMemcpyHostToDevice, ctx.stream),
+          "upload marching workgroups");
+    B.d_march.push_back(std::move(mbuf));
+    B.march.push_back(std::move(blocks));
     auto buf = std::make_unique<DeviceBuffer<BatchItem>>();
     buf->reserve(B.items[s].size());
     check(hipMemcpyAsync(buf->data(), B.items[s].data(), B.items[s].size() * sizeof(BatchItem),
