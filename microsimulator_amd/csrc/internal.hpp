@@ -125,7 +125,9 @@ constexpr uint32_t kFlagSources = 8;   // a coupled source rate not finite
 // level's residual and correction during a cycle.
 struct MgLevel {
   uint32_t nx = 0, ny = 0;
-  DeviceBuffer<float> wx, wy, bc, dinv, r, e;
+  // xp: the iterate after pre-smoothing (read by the up pass); t: the
+  // single-workgroup kernel's intermediate sweep.
+  DeviceBuffer<float> wx, wy, bc, dinv, r, e, xp, t;
   [[nodiscard]] std::size_t cells() const { return static_cast<std::size_t>(nx) * ny; }
 };
 
@@ -417,9 +419,9 @@ void learned_gradients(Context& ctx, uint32_t pair, double* loss, double* gradie
 void learned_apply(Context& ctx, uint32_t pair, float* z);
 // The finest level's up pass with the learned post-smoother; with q, the
 // flexible-CG dots z.r and z.q into zr / zq.
-void launch_up_learned(Context& ctx, const MgLevel& L0, const float* r, const float* ec,
-                       uint32_t nyc, float* z, const CgState* st, const float* q, double* zr,
-                       double* zq);
+void launch_up_learned(Context& ctx, const MgLevel& L0, const float* r, const float* xp,
+                       const float* ec, uint32_t nyc, float* z, const CgState* st, const float* q,
+                       double* zr, double* zq);
 // Workgroups of the learned up pass on an nx x ny level (one dot partial each).
 uint32_t learned_blocks(uint32_t nx, uint32_t ny);
 // r' = r - A x, e' = e - x on the finest level.
@@ -532,6 +534,10 @@ void adr3_survey(Context& ctx, const float* xf, const float* yf, const float* zf
                  const float* levels, CgState* state, double* partials, double* mass);
 void adr3_run(Context& ctx, float dt, uint32_t steps, const float* xf, const float* yf,
               const float* zf);
+// The forward-Euler candidate c + dt * rate on a grid of several layers,
+// written without judging it (a coupled step adds its sources first).
+void adr3_candidate(Context& ctx, float dt, const float* in, float* out, uint32_t step,
+                    const float* xf, const float* yf, const float* zf);
 void implicit3_setup(Context& ctx, uint32_t blocks, const float* xf, const float* yf,
                      const float* zf, const float* sources, const float* old, float explicit_dt,
                      float dt);

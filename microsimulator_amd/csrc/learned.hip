@@ -2,9 +2,9 @@
 // convolutional network on bfloat16 MFMA (no counterpart in the reference,
 // whose only preconditioner is Jacobi, cpp/cpu/cpu_flow.cpp:65-69).
 //
-// On the finest level the cycle's post-smoothing step
+// On the finest level the cycle's post-smoothing
 //     z = x + omega D^-1 r'            (r' = r - A x after the coarse correction)
-// becomes
+// becomes one learned step
 //     z = x + omega D^-1 r' + s N(r'),
 // where N works tile by tile (16 x 32 sites, receptive region grown by 3):
 //   u = D^-1 r',  s = max |u| over the region,  v = u / s,
@@ -312,6 +312,7 @@ struct UpOut {
 // x = omega D^-1 r + c P e_c, r' = r - A x, z = x + omega D^-1 r' + s N(r');
 // with dots z.r and z.q for flexible CG.
 __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const float* __restrict__ r,
+                                                       const float* __restrict__ xp,
                                                        const float* __restrict__ ec, uint32_t nyc,
                                                        const CgState* st,
                                                        const bf16x4* __restrict__ packed,
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   // loads: r, dinv, e_c and, on feature pixels, the operator terms. The next
   // tile's round is issued before this tile's network runs.
   struct Stage {
-    float r[kPer], d[kPer], ec[kPer], wxl[kPer], wxu[kPer], wyl[kPer], wyu[kPer], b[kPer];
+    float r[kPer], d[kPer], x0[kPer], ec[kPer], wxl[kPer], wxu[kPer], wyl[kPer], wyu[kPer], b[kPer];
     uint32_t on, feat;
   };
   auto load_tile = [&](uint32_t t) {
@@ -366,6 +367,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
         const size_t i = static_cast<size_t>(gx) * G.ny + static_cast<uint32_t>(gy);
         v.r[k] = r[i];
         v.d[k] = G.dinv[i];
+        v.x0[k] = xp != nullptr ? xp[i] : kOmega * v.d[k] * v.r[k];
         v.ec[k] = ec[static_cast<size_t>(gx / 2) * nyc + static_cast<uint32_t>(gy) / 2];
         if (feat_col && row >= 1 && row < kRows - 1) {
           v.feat |= 1u << k;
@@ -384,12 +386,13 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   for (uint32_t t = t0; t < t1; ++t) {
     const int fx0 = static_cast<int>(t / tiles_y) * kTX;
     const int fy0 = static_cast<int>(t % tiles_y) * kTY;
-    // x = omega D^-1 r + c P e_c over the region.
+    // x = x_pre + c P e_c over the region (x_pre: the pre-smoothed iterate,
+    // omega D^-1 r after one sweep from zero).
     const uint32_t on = cur.on, feat = cur.feat;
     float xv[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      xv[k] = ((on >> k) & 1u) ? kOmega * cur.d[k] * cur.r[k] + kCorrection * cur.ec[k] : 0.0f;
+      xv[k] = ((on >> k) & 1u) ? cur.x0[k] + kCorrection * cur.ec[k] : 0.0f;
     }
     const float* rv = cur.r;
     const float* dv = cur.d;
@@ -1151,15 +1154,15 @@ uint32_t learned_blocks(uint32_t nx, uint32_t ny) {
   return std::min<uint32_t>(g.x * g.y, 768);  // three per CU
 }
 
-void launch_up_learned(Context& ctx, const MgLevel& L0, const float* r, const float* ec,
-                       uint32_t nyc, float* z, const CgState* st, const float* q, double* zr,
-                       double* zq) {
+void launch_up_learned(Context& ctx, const MgLevel& L0, const float* r, const float* xp,
+                       const float* ec, uint32_t nyc, float* z, const CgState* st, const float* q,
+                       double* zr, double* zq) {
   require_weights(ctx);
   const Geometry G{L0.wx.data(), L0.wy.data(), L0.bc.data(), L0.dinv.data(), L0.nx, L0.ny};
   const UpOut out{z, q, zr, zq};
   const dim3 g = tile_grid(L0.nx, L0.ny, 1);
   k_learn_up<<<learned_blocks(L0.nx, L0.ny), kThreads, 0, ctx.stream>>>(
-      G, r, ec, nyc, st, reinterpret_cast<const bf16x4*>(ctx.learned.packed.data()), out, g.y,
+      G, r, xp, ec, nyc, st, reinterpret_cast<const bf16x4*>(ctx.learned.packed.data()), out, g.y,
       g.x * g.y);
   check(hipGetLastError(), "launch k_learn_up");
 }

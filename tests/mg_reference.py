@@ -1,7 +1,8 @@
 """CPU restatement (float64, numpy) of the device multilevel preconditioner,
 for checking its kernels: 2x2 aggregation with Galerkin coarse operators,
-V(1,1) damped Jacobi (omega 0.8), coarse correction weighted by 1.8, the
-coarsest level (<= 256 cells) solved exactly. Test infrastructure only."""
+V(2,2) damped Jacobi (omega 0.8; the sweeps on each side are a parameter),
+coarse correction weighted by 1.8, the coarsest level (<= 256 cells) solved
+exactly. Test infrastructure only."""
 from __future__ import annotations
 
 import numpy as np
@@ -72,8 +73,16 @@ def dense(L):
     return np.column_stack([apply(L, np.eye(n)[k].reshape(shape)).ravel() for k in range(n)])
 
 
+def smooth(L, x, r, sweeps):
+    """`sweeps` damped-Jacobi sweeps on A x = r from x (None: from zero)."""
+    for _ in range(sweeps):
+        x = OMEGA * L["dinv"] * r if x is None else x + OMEGA * L["dinv"] * (r - apply(L, x))
+    return x
+
+
 class Multilevel:
-    def __init__(self, L0):
+    def __init__(self, L0, pre=2, post=2):
+        self.pre, self.post = pre, post
         self.levels = [L0]
         while self.levels[-1]["bc"].size > COARSE_CELLS:
             self.levels.append(coarsen(self.levels[-1]))
@@ -90,7 +99,7 @@ class Multilevel:
             e = np.zeros(r.size)
             e[self.active] = self.inverse @ r.ravel()[self.active]
             return e.reshape(r.shape)
-        x = OMEGA * L["dinv"] * r
+        x = smooth(L, None, r, self.pre)
         rc = restrict(r - apply(L, x), self.levels[level + 1]["bc"].shape)
         x = x + CORRECTION * prolong(self.cycle(rc, level + 1), r.shape)
-        return x + OMEGA * L["dinv"] * (r - apply(L, x))
+        return smooth(L, x, r, self.post)

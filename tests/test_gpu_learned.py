@@ -3,7 +3,10 @@ restatement in tests/learned_reference.py (itself pinned by finite
 differences and invariants in tests/test_learned_reference.py).
 
 Tolerances:
-- With a zero last layer the learned cycle is the multigrid cycle bit for bit.
+- With a zero last layer the learned cycle is the multilevel cycle with one
+  damped-Jacobi post-smoothing sweep on the finest level (two elsewhere):
+  against its float64 restatement to 2e-4 of the largest value, the
+  multilevel cycle's own tolerance.
 - The device network runs in bfloat16 with FP32 sums, so its forward output is
   compared with a restatement that rounds weights and stored activations to
   bfloat16 the same way. What is left is FP32 summation order and rare
@@ -56,8 +59,22 @@ def test_learned_requires_weights():
             fresh.learned_set_weights(np.zeros(10, np.float32))
 
 
+def plain_part(L0, r):
+    """The learned cycle without its network, restated in float64: two
+    pre-smoothing sweeps, the V(2,2) cycle below, the coarse correction and
+    one damped-Jacobi post-smoothing sweep on the finest level. Returns the
+    iterate before post-smoothing, its residual and the finished cycle."""
+    ml = mr.Multilevel({k: L0[k] for k in ("wx", "wy", "bc")})
+    L = ml.levels[0]
+    x = mr.smooth(L, None, r, 2)
+    rc = mr.restrict(r - mr.apply(L, x), ml.levels[1]["bc"].shape)
+    x = x + mr.CORRECTION * mr.prolong(ml.cycle(rc, 1), r.shape)
+    rp = r - mr.apply(L, x)
+    return L, x, rp, x + mr.OMEGA * L["dinv"] * rp
+
+
 @pytest.mark.parametrize("shape", [(256, 128), (301, 77), (20, 13)])
-def test_zero_last_layer_is_the_multigrid_cycle(hip, shape):
+def test_zero_last_layer_is_the_plain_cycle(hip, shape):
     nx, ny = shape
     solid = prepare(hip, nx, ny)
     w = random_weights(3)
@@ -65,10 +82,11 @@ def test_zero_last_layer_is_the_multigrid_cycle(hip, shape):
     hip.learned_set_weights(w)
     r = np.random.default_rng(5).standard_normal(nx * ny).astype(np.float32)
     r[solid.reshape(-1) != 0] = 0.0
-    z_learned = hip.learned_precondition(r)
-    z_plain = hip.flow_precondition(r)
-    print(shape, "max |difference|", np.abs(z_learned - z_plain).max(), flush=True)
-    np.testing.assert_array_equal(z_learned, z_plain)
+    z = hip.learned_precondition(r).astype(np.float64)
+    _, _, _, ref = plain_part(level0(hip), r.astype(np.float64).reshape(nx, ny))
+    err = np.abs(z - ref.ravel()).max() / np.abs(ref).max()
+    print(shape, "relative error", err, flush=True)
+    assert err <= 2e-4
 
 
 @pytest.mark.parametrize("shape", [(256, 128), (301, 77)])
@@ -78,16 +96,11 @@ def test_learned_cycle_matches_cpu_restatement(hip, shape):
     w = random_weights(7)
     hip.learned_set_weights(w)
     L0 = level0(hip)
-    ml = mr.Multilevel({k: L0[k] for k in ("wx", "wy", "bc")})
     r = np.random.default_rng(9).standard_normal(nx * ny).astype(np.float32)
     r[solid.reshape(-1) != 0] = 0.0
     z = hip.learned_precondition(r).astype(np.float64).reshape(nx, ny)
     rr = r.astype(np.float64).reshape(nx, ny)
-    L = ml.levels[0]
-    x = mr.OMEGA * L["dinv"] * rr
-    rc = mr.restrict(rr - mr.apply(L, x), ml.levels[1]["bc"].shape)
-    x = x + mr.CORRECTION * mr.prolong(ml.cycle(rc, 1), rr.shape)
-    rp = rr - mr.apply(L, x)
+    L, x, rp, _ = plain_part(L0, rr)
     ref = x + lr.smoother(L, rp, lr.unpack(w), bf=True)
     net = lr.correction(L, rp, lr.unpack(w), bf=True)
     err = np.abs(z - ref).max() / np.abs(ref).max()
@@ -160,15 +173,21 @@ def test_training_lowers_the_loss(hip, pairs):
     L0, r, e = pairs
     w0 = learned.initial_weights(0)
     hip.learned_set_weights(w0)
-    losses, seconds = hip.learned_train(60, batch=4, learning_rate=3e-3, seed=1)
-    # The first batch loss is that of damped Jacobi (W3 = 0), in closed form.
+
+    def loss_over_all_pairs():
+        return float(np.mean([hip.learned_gradients(k)[0] for k in range(len(r))]))
+    before = loss_over_all_pairs()
+    # With W3 = 0 the smoother is damped Jacobi, whose loss has a closed form.
     first = []
     for k in range(len(r)):
         d = e[k] - lr.OMEGA * L0["dinv"] * r[k]
         first.append(np.sum(d * mr.apply(L0, d)) / np.sum(e[k] * mr.apply(L0, e[k])))
-    print("losses", losses[:3], "...", losses[-3:], "seconds", seconds, flush=True)
-    assert min(first) * 0.999 <= losses[0] <= max(first) * 1.001
-    assert losses[-10:].mean() < 0.97 * losses[:5].mean()
+    assert before == pytest.approx(float(np.mean(first)), rel=1e-3)
+    losses, seconds = hip.learned_train(60, batch=4, learning_rate=3e-3, seed=1)
+    after = loss_over_all_pairs()
+    print("loss over the pairs", before, "->", after, "batch losses", losses[:3], "...",
+          losses[-3:], "seconds", seconds, flush=True)
+    assert after < 0.995 * before
     w1 = hip.learned_weights()
     assert np.any(w1[-lr.SIZES[2]:] != 0.0)
 
