@@ -147,8 +147,10 @@ int msim_hip_device_name(msim_ctx* ctx, char* name, size_t length);
 
 /* ComputeBackend::solve_depth_averaged_flow (backend.hpp:44-46): host arrays
  * in, the velocity field out. mobility: one float per site or NULL (the
- * reference's Hele-Shaw default). x/y/z_faces: face velocities. The grid must
- * be one layer (nz = 1). Equivalent to prepare + solve + download. */
+ * reference's Hele-Shaw default). x/y/z_faces: face velocities. A grid of
+ * several layers is solved on its grid of columns and lifted onto the voxel
+ * faces (ShallowFlowReduction, flow_system.hpp:257-470). Equivalent to
+ * prepare + solve + download. */
 int msim_hip_solve_depth_averaged_flow(msim_ctx* ctx, const msim_grid* grid,
                                        const float* mobility, const msim_flow_params* params,
                                        float* x_faces, float* y_faces, float* z_faces,
@@ -433,9 +435,9 @@ int msim_hip_adr_report(msim_ctx* ctx, msim_signal_report* report);
 
 /* ---------------------------------------------------------------------------
  * Coupled cell/grid exchange (ComputeBackend::advance_coupled, backend.hpp:31-34;
- * advance_coupled_cpu, cpp/cpu/cpu_coupled.cpp:125-231), one layer, FP32
- * levels, at most 8 signals, and at most 256 rate instructions, signals and
- * species together.
+ * advance_coupled_cpu, cpp/cpu/cpu_coupled.cpp:125-231), on one layer or
+ * several, FP32 levels, at most 8 signals, and at most 256 rate instructions,
+ * signals and species together.
  *
  * Each cell samples the grid's starting levels through its trilinear stencil
  * (signal_grid_stencil, signals.cpp:357-420: with obstacles only the fluid

@@ -342,7 +342,7 @@ class BatchInstance:
 # --------------------------------------------------------------------------
 @dataclass
 class Grid:
-    """The lattice of a SignalGridSpec (signals.hpp:122-156), one layer."""
+    """The lattice of a SignalGridSpec (signals.hpp:122-156)."""
     nx: int
     ny: int
     nz: int = 1

@@ -37,15 +37,18 @@ __device__ inline Geometry3 signal_geometry(const AdrParams& P, const Geometry3&
 
 // One forward-Euler step (signal_grid_forward_euler_candidate): the rate
 // with the affine reaction and Monod uptake (as the affine loss
-// vmax*phi/(K+c) it equals under forward Euler), c + dt * rate. A
-// non-finite, or under upwind negative, candidate raises the flag; under
-// van Leer each block leaves the largest level it wrote for the settle.
+// vmax*phi/(K+c) it equals under forward Euler), c + dt * rate. With
+// `judge`, a non-finite, or under upwind negative, candidate raises the
+// flag, and under van Leer each block leaves the largest level it wrote for
+// the settle; without, the candidate is only written (a coupled step adds
+// its sources and judges the sum).
 template <int S, bool kLimited, bool kPatched>
 __global__ __launch_bounds__(kBlock3) void k_adr3_step(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ zf, const float* __restrict__ phi,
     const float* __restrict__ source, const float* __restrict__ loss, const float* __restrict__ in,
-    float* __restrict__ out, float dt, uint32_t* flags, uint32_t step, double* largest) {
+    float* __restrict__ out, float dt, uint32_t* flags, uint32_t step, double* largest,
+    int judge) {
   __shared__ double s_red[kBlock3 / kWave];
   if (flags[0] != 0) return;  // an earlier step failed: the run stops there
   const size_t n = static_cast<size_t>(P.nx) * P.ny * P.nz;
@@ -77,6 +80,7 @@ __global__ __launch_bounds__(kBlock3) void k_adr3_step(
       }
     }
   }
+  if (judge == 0) return;
   if (bad) {
     atomicOr(&flags[0], kFlagNegative);
     atomicMin(&flags[1], step);
@@ -295,12 +299,25 @@ void adr3_run(Context& ctx, float dt, uint32_t steps, const float* xf, const flo
     dispatch3(P, [&]<int S, bool kLimited, bool kPatched>() {
       k_adr3_step<S, kLimited, kPatched><<<blocks, kBlock3, 0, ctx.stream>>>(
           P, a.solid.data(), xf, yf, zf, a.phi.data(), a.source.data(), a.loss.data(), src, dst,
-          dt, a.flags.data(), k, largest);
+          dt, a.flags.data(), k, largest, 1);
     });
     check(hipGetLastError(), "launch k_adr3_step");
     if (P.limited != 0) launch_settle_f32(ctx, dst, count, largest, blocks, k);
     a.current ^= 1;
   }
+}
+
+void adr3_candidate(Context& ctx, float dt, const float* in, float* out, uint32_t step,
+                    const float* xf, const float* yf, const float* zf) {
+  auto& a = ctx.adr;
+  const auto& P = a.params;
+  const uint32_t blocks = blocks3(a.sites());
+  dispatch3(P, [&]<int S, bool kLimited, bool kPatched>() {
+    k_adr3_step<S, kLimited, kPatched><<<blocks, kBlock3, 0, ctx.stream>>>(
+        P, a.solid.data(), xf, yf, zf, a.phi.data(), a.source.data(), a.loss.data(), in, out, dt,
+        a.flags.data(), step, nullptr, 0);
+  });
+  check(hipGetLastError(), "launch k_adr3_step");
 }
 
 void implicit3_setup(Context& ctx, uint32_t blocks, const float* xf, const float* yf,

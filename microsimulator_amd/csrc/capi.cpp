@@ -922,17 +922,20 @@ bool inside_axis(float position, float origin, float spacing, uint32_t dimension
 }
 
 // The stencil checks of validate_coupled_step (cpu_coupled.cpp:112-122), in
-// cell order: each position inside the grid, and not with every corner of
-// positive weight solid.
+// cell order: each position inside the grid along x, y and z, and not with
+// every corner of positive weight solid (signal_grid_stencil, signals.cpp:357-420).
 void validate_positions(const msim_signal_grid& g, const float* origin, const msim_cells& cells) {
-  const uint32_t nx = g.grid.nx, ny = g.grid.ny;
+  const uint32_t nx = g.grid.nx, ny = g.grid.ny, nz = g.grid.nz;
   for (uint32_t c = 0; c < cells.count; ++c) {
-    uint32_t x0, y0;
+    uint32_t x0, y0, z0;
     if (!inside_axis(cells.position_x[c], origin[0], g.grid.dx, nx, &x0)) {
       invalid("signal sample is outside the x grid bound");
     }
     if (!inside_axis(cells.position_y[c], origin[1], g.grid.dy, ny, &y0)) {
       invalid("signal sample is outside the y grid bound");
+    }
+    if (!inside_axis(cells.position_z[c], origin[2], g.grid.dz, nz, &z0)) {
+      invalid("signal sample is outside the z grid bound");
     }
     if (g.grid.obstacles == nullptr) continue;
     // Weights along an axis: 1 - f and f with f the fraction past the lower
@@ -948,15 +951,19 @@ void validate_positions(const msim_signal_grid& g, const float* origin, const ms
       w[0] = 1.0f - fraction;
       w[1] = fraction;
     };
-    float wx[2], wy[2];
+    float wx[2], wy[2], wz[2];
     const uint32_t nxc = (nx == 1 || x0 == nx - 1) ? 1 : 2, nyc = (ny == 1 || y0 == ny - 1) ? 1 : 2;
+    const uint32_t nzc = (nz == 1 || z0 == nz - 1) ? 1 : 2;
     weights(cells.position_x[c], origin[0], g.grid.dx, nx, x0, nxc, wx);
     weights(cells.position_y[c], origin[1], g.grid.dy, ny, y0, nyc, wy);
+    weights(cells.position_z[c], origin[2], g.grid.dz, nz, z0, nzc, wz);
     bool fluid = false;
     for (uint32_t a = 0; a < nxc; ++a) {
       for (uint32_t b = 0; b < nyc; ++b) {
-        const size_t site = static_cast<size_t>(x0 + a) * ny + y0 + b;
-        if (g.grid.obstacles[site] == 0 && wx[a] * wy[b] > 0.0f) fluid = true;
+        for (uint32_t d = 0; d < nzc; ++d) {
+          const size_t site = (static_cast<size_t>(x0 + a) * ny + y0 + b) * nz + z0 + d;
+          if (g.grid.obstacles[site] == 0 && wx[a] * wy[b] * wz[d] > 0.0f) fluid = true;
+        }
       }
     }
     if (!fluid) invalid("signal sample position is inside a grid obstacle");
@@ -974,9 +981,6 @@ void validate_coupled_inputs(const msim_signal_grid& g, const float* origin, con
   }
   validate_cells(cells, plan.species_count, species);
   validate_signal_grid(g, false);
-  if (g.grid.nz != 1) {
-    throw Error(MSIM_UNSUPPORTED, "the HIP coupled step takes a grid of one layer (nz = 1)");
-  }
   validate_levels(g, levels);
 }
 

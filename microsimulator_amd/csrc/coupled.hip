@@ -1,5 +1,5 @@
 // Coupled cell/grid exchange on the device (advance_coupled_cpu,
-// cpp/cpu/cpu_coupled.cpp:125-231), one layer, FP32 levels.
+// cpp/cpu/cpu_coupled.cpp:125-231), FP32 levels, on one layer or several.
 //
 // The device keeps the cells in spatial order: a stable sort at upload by
 // the site of each cell's lower stencil corner, with `order` mapping a slot
@@ -12,7 +12,8 @@
 //    renormalised), the signals it samples from the step's starting levels,
 //    the dilution of its species, the rate plan run by an interpreter whose
 //    working values live in LDS (a column per thread), the species update,
-//    the exchange rate of each signal, and the weights of its 2 x 2 corners;
+//    the exchange rate of each signal, and the weights of its 2 x 2 corners
+//    (2 x 2 x 2 on a grid of several layers);
 //  * forward Euler: the explicit candidate (adr.hip), then k_coupled_commit,
 //    which adds dt * source to each level and judges the levels
 //    (accept_step); the implicit kinds: k_coupled_sources writes the sources
@@ -20,8 +21,8 @@
 // A site's source is the reference's float sum in the caller's cell order
 // (cpu_coupled.cpp:198-207). The cells that reach site (x, y) are those whose
 // lower corner is (x, y), (x, y - 1), (x - 1, y) or (x - 1, y - 1): four runs
-// of slots, each in caller order (the stable sort kept it), merged by the
-// caller's index as the sum goes.
+// of slots (eight, with z and z - 1, on several layers), each in caller order
+// (the stable sort kept it), merged by the caller's index as the sum goes.
 // Every kernel returns at once when an earlier failure set the transport
 // flags, so a run stops at its first refused step.
 #include <hip/hip_runtime.h>
@@ -42,15 +43,16 @@ namespace msim {
 namespace {
 
 constexpr int kSiteBlock = 256;
-constexpr uint32_t kCorners = 4;  // a one-layer stencil holds at most 2 x 2 sites
+// Corners of a stencil: 2 x 2 on one layer, 2 x 2 x 2 on several (KC).
+constexpr uint32_t kCorners2 = 4, kCorners3 = 8;
 constexpr unsigned long long kNoFailure = ~0ull;
 constexpr uint32_t kSpeciesFailure = 0xffffffffu;  // low word of a species-update failure
 constexpr uint32_t kStencilFailure = 0xfffffffeu;  // a position outside the grid (checked on the host)
 
 struct CellParams {
-  uint32_t nx, ny;
-  float dx, dy;
-  float ox, oy;
+  uint32_t nx, ny, nz;
+  float dx, dy, dz;
+  float ox, oy, oz;
   float volume;  // dx * dy * dz
   float dt;
   uint32_t cells, species, signals, instructions;
@@ -121,6 +123,7 @@ __device__ inline void fail_cell(unsigned long long* failure, uint32_t* flags, u
 // of a [slots][blockDim] array.
 // The exponential, logarithm and power are evaluated in FP64 and rounded to
 // float.
+template <uint32_t KC>
 __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
                                 const float* __restrict__ levels, const float* __restrict__ px,
                                 const float* __restrict__ py, const float* __restrict__ pz,
@@ -131,41 +134,49 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
                                 const msim_rate_instruction* __restrict__ plan,
                                 const uint32_t* __restrict__ outputs,
                                 const float* __restrict__ species_in, float* species_out,
-                                float* __restrict__ exchange, float4* __restrict__ weights,
+                                float* __restrict__ exchange, float* __restrict__ weights,
                                 unsigned long long* failure, uint32_t* flags) {
   extern __shared__ float s_work[];
   if (flags[0] != 0) return;  // an earlier step failed: the run stops there
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;  // the cell's slot
   if (c >= P.cells) return;
   const uint32_t caller = order[c];  // its index in the caller's order
-  const size_t n = static_cast<size_t>(P.nx) * P.ny;
-  // The stencil (signal_grid_stencil, signals.cpp:357-420): x-major corners.
+  const size_t n = static_cast<size_t>(P.nx) * P.ny * P.nz;
+  // The stencil (signal_grid_stencil, signals.cpp:357-420): corners x-major,
+  // then y, then z.
   const Axis ax = axis_of(px[c], P.ox, P.dx, P.nx);
   const Axis ay = axis_of(py[c], P.oy, P.dy, P.ny);
-  if (!ax.inside || !ay.inside) {
+  const Axis az = axis_of(pz[c], P.oz, P.dz, P.nz);
+  if (!ax.inside || !ay.inside || !az.inside) {
     fail_cell(failure, flags, caller, kStencilFailure, P.step);
     return;
   }
-  uint32_t site[kCorners], cx[kCorners], cy[kCorners];
-  float w[kCorners];
-  const uint32_t count = ax.count * ay.count;
+  uint32_t site[KC], cx[KC], cy[KC], cz[KC];
+  float w[KC];
+  const uint32_t count = ax.count * ay.count * az.count;
 #pragma unroll
-  for (uint32_t e = 0; e < kCorners; ++e) {
-    // Corner e is (e / yc, e % yc) for the y axis's yc sites.
-    const uint32_t a = e < count ? e / ay.count : 0, b = e < count ? e % ay.count : 0;
+  for (uint32_t e = 0; e < KC; ++e) {
+    // Corner e is (a, b, d) for the y and z axes' yc and zc sites.
+    const uint32_t a = e < count ? e / (ay.count * az.count) : 0;
+    const uint32_t b = e < count ? (e / az.count) % ay.count : 0;
+    const uint32_t d = e < count ? e % az.count : 0;
     const uint32_t xi = a != 0 ? ax.index[1] : ax.index[0];
     const uint32_t yi = b != 0 ? ay.index[1] : ay.index[0];
+    const uint32_t zi = d != 0 ? az.index[1] : az.index[0];
     cx[e] = xi;
     cy[e] = yi;
-    site[e] = xi * P.ny + yi;
-    w[e] = e < count ? (a != 0 ? ax.weight[1] : ax.weight[0]) * (b != 0 ? ay.weight[1] : ay.weight[0])
+    cz[e] = zi;
+    site[e] = (xi * P.ny + yi) * P.nz + zi;
+    w[e] = e < count ? ((a != 0 ? ax.weight[1] : ax.weight[0]) *
+                        (b != 0 ? ay.weight[1] : ay.weight[0])) *
+                           (d != 0 ? az.weight[1] : az.weight[0])
                      : 0.0f;
   }
   if (P.has_mask != 0) {
     uint32_t seed = count;
     float best = 0.0f;
 #pragma unroll
-    for (uint32_t e = 0; e < kCorners; ++e) {
+    for (uint32_t e = 0; e < KC; ++e) {
       if (e < count) {
         if (solid[site[e]] != 0) w[e] = 0.0f;
         if (w[e] > 0.0f && (seed == count || w[e] > best)) {
@@ -178,20 +189,21 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
       fail_cell(failure, flags, caller, kStencilFailure, P.step);
       return;
     }
-    bool connected[kCorners] = {false, false, false, false};
+    bool connected[KC];
 #pragma unroll
-    for (uint32_t e = 0; e < kCorners; ++e) connected[e] = e == seed;
+    for (uint32_t e = 0; e < KC; ++e) connected[e] = e == seed;
     for (bool grew = true; grew;) {
       grew = false;
 #pragma unroll
-      for (uint32_t i = 0; i < kCorners; ++i) {
+      for (uint32_t i = 0; i < KC; ++i) {
         if (i >= count || w[i] <= 0.0f || connected[i]) continue;
 #pragma unroll
-        for (uint32_t j = 0; j < kCorners; ++j) {
+        for (uint32_t j = 0; j < KC; ++j) {
           if (j >= count || !connected[j] || connected[i]) continue;
-          const uint32_t d = (cx[i] > cx[j] ? cx[i] - cx[j] : cx[j] - cx[i]) +
-                             (cy[i] > cy[j] ? cy[i] - cy[j] : cy[j] - cy[i]);
-          if (d == 1) {
+          const uint32_t dist = (cx[i] > cx[j] ? cx[i] - cx[j] : cx[j] - cx[i]) +
+                                (cy[i] > cy[j] ? cy[i] - cy[j] : cy[j] - cy[i]) +
+                                (cz[i] > cz[j] ? cz[i] - cz[j] : cz[j] - cz[i]);
+          if (dist == 1) {
             connected[i] = true;
             grew = true;
           }
@@ -200,14 +212,14 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
     }
     float total = 0.0f;
 #pragma unroll
-    for (uint32_t e = 0; e < kCorners; ++e) {
+    for (uint32_t e = 0; e < KC; ++e) {
       if (e < count) {
         if (!connected[e]) w[e] = 0.0f;
         total += w[e];
       }
     }
 #pragma unroll
-    for (uint32_t e = 0; e < kCorners; ++e) {
+    for (uint32_t e = 0; e < KC; ++e) {
       if (e < count) w[e] /= total;
     }
   }
@@ -222,7 +234,7 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
   for (uint32_t s = 0; s < S; ++s) {
     float v = 0.0f;
 #pragma unroll
-    for (uint32_t e = 0; e < kCorners; ++e) {
+    for (uint32_t e = 0; e < KC; ++e) {
       if (e < count) v += w[e] * levels[s * n + site[e]];
     }
     sampled[s * T] = v;
@@ -298,40 +310,51 @@ __global__ void k_coupled_cells(CellParams P, const uint8_t* __restrict__ solid,
   for (uint32_t s = 0; s < S; ++s) {
     exchange[static_cast<size_t>(c) * S + s] = work[outputs[K + s] * T] / P.volume;
   }
-  // The corners' weights at (a, b) = (0, 0), (0, 1), (1, 0), (1, 1); -1 where
-  // the stencil has no such corner (an axis of one site, or the upper edge).
-  float corner[kCorners];
+  // The corners' weights at (a, b) = (0, 0), (0, 1), (1, 0), (1, 1) (with d
+  // the last bit on several layers); -1 where the stencil has no such corner
+  // (an axis of one site, or the upper edge).
 #pragma unroll
-  for (uint32_t f = 0; f < kCorners; ++f) {
-    const uint32_t fa = f >> 1, fb = f & 1;
-    const uint32_t e = fa * ay.count + fb;
-    const float pick = e == 0 ? w[0] : (e == 1 ? w[1] : (e == 2 ? w[2] : w[3]));
-    corner[f] = fa < ax.count && fb < ay.count ? pick : -1.0f;
+  for (uint32_t f = 0; f < KC; ++f) {
+    const uint32_t fa = KC == kCorners3 ? f >> 2 : f >> 1;
+    const uint32_t fb = KC == kCorners3 ? (f >> 1) & 1 : f & 1;
+    const uint32_t fd = KC == kCorners3 ? f & 1 : 0;
+    const uint32_t e = (fa * ay.count + fb) * az.count + fd;
+    float pick = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < KC; ++k) {
+      if (k == e) pick = w[k];
+    }
+    weights[static_cast<size_t>(c) * KC + f] =
+        fa < ax.count && fb < ay.count && fd < az.count ? pick : -1.0f;
   }
-  weights[c] = make_float4(corner[0], corner[1], corner[2], corner[3]);
 }
 
-// The sources of site (x, y): per signal, from 0, the sum of weight *
+// The sources of site (x, y, z): per signal, from 0, the sum of weight *
 // exchange over the cells whose stencil reaches the site, in the caller's
-// cell order (cpu_coupled.cpp:198-207). Corner (da, db) of the cells with
-// lower corner (x - da, y - db) is this site; the four runs are merged by the
-// caller's index. A corner the stencil lacks (weight -1) adds nothing; a
+// cell order (cpu_coupled.cpp:198-207). Corner (da, db, dd) of the cells with
+// lower corner (x - da, y - db, z - dd) is this site; the runs are merged by
+// the caller's index. A corner the stencil lacks (weight -1) adds nothing; a
 // present corner of weight 0 adds its product, as the reference does.
-template <int S>
-__device__ inline void site_sources(uint32_t x, uint32_t y, uint32_t ny,
+template <int S, uint32_t KC>
+__device__ inline void site_sources(size_t site, uint32_t ny, uint32_t nz,
                                     const uint32_t* __restrict__ offsets,
                                     const uint32_t* __restrict__ order,
                                     const float* __restrict__ weights,
                                     const float* __restrict__ exchange, float (&src)[S]) {
+  const uint32_t z = static_cast<uint32_t>(site % nz);
+  const size_t column = site / nz;
+  const uint32_t x = static_cast<uint32_t>(column / ny), y = static_cast<uint32_t>(column % ny);
 #pragma unroll
   for (int s = 0; s < S; ++s) src[s] = 0.0f;
-  uint32_t at[4], end[4], head[4];
+  uint32_t at[KC], end[KC], head[KC];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const uint32_t da = g >> 1, db = g & 1;
+  for (uint32_t g = 0; g < KC; ++g) {
+    const uint32_t da = KC == kCorners3 ? g >> 2 : g >> 1;
+    const uint32_t db = KC == kCorners3 ? (g >> 1) & 1 : g & 1;
+    const uint32_t dd = KC == kCorners3 ? g & 1 : 0;
     at[g] = end[g] = 0;
-    if (x >= da && y >= db) {
-      const size_t k = static_cast<size_t>(x - da) * ny + (y - db);
+    if (x >= da && y >= db && z >= dd) {
+      const size_t k = (static_cast<size_t>(x - da) * ny + (y - db)) * nz + (z - dd);
       at[g] = offsets[k];
       end[g] = offsets[k + 1];
     }
@@ -341,24 +364,23 @@ __device__ inline void site_sources(uint32_t x, uint32_t y, uint32_t ny,
     int pick = -1;
     uint32_t least = 0xffffffffu;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
+    for (uint32_t g = 0; g < KC; ++g) {
       if (at[g] < end[g] && head[g] < least) {
         least = head[g];
-        pick = g;
+        pick = static_cast<int>(g);
       }
     }
     if (pick < 0) break;
     uint32_t slot = 0;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g == pick) {
+    for (uint32_t g = 0; g < KC; ++g) {
+      if (static_cast<int>(g) == pick) {
         slot = at[g]++;
         head[g] = at[g] < end[g] ? order[at[g]] : 0xffffffffu;
       }
     }
-    // Lower corner (x - da, y - db) reaches (x, y) with its corner (da, db),
-    // whose weight is the cell's entry g = 2 da + db.
-    const float w = weights[static_cast<size_t>(slot) * kCorners + pick];
+    // That lower corner reaches this site with its corner g, the cell's entry g.
+    const float w = weights[static_cast<size_t>(slot) * KC + pick];
     if (w >= 0.0f) {
       const float* ex = exchange + static_cast<size_t>(slot) * S;
 #pragma unroll
@@ -372,9 +394,9 @@ __device__ inline void site_sources(uint32_t x, uint32_t y, uint32_t ny,
 // as accept_step does: under upwind each must be finite and non-negative;
 // under van Leer the block's largest level goes to `largest` for the settle.
 // Writes nothing once a failure is flagged.
-template <int S, bool kLimited>
+template <int S, bool kLimited, uint32_t KC>
 __global__ __launch_bounds__(kSiteBlock) void k_coupled_commit(
-    size_t n, uint32_t ny, const float* in, float* out, float dt,
+    size_t n, uint32_t ny, uint32_t nz, const float* in, float* out, float dt,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ order,
     const float* __restrict__ weights, const float* __restrict__ exchange,
     double* __restrict__ largest, uint32_t* flags, uint32_t step) {
@@ -385,8 +407,7 @@ __global__ __launch_bounds__(kSiteBlock) void k_coupled_commit(
   bool bad = false;
   if (site < n && !failed) {
     float src[S];
-    site_sources<S>(static_cast<uint32_t>(site / ny), static_cast<uint32_t>(site % ny), ny,
-                    offsets, order, weights, exchange, src);
+    site_sources<S, KC>(site, ny, nz, offsets, order, weights, exchange, src);
     {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
@@ -411,16 +432,16 @@ __global__ __launch_bounds__(kSiteBlock) void k_coupled_commit(
 
 // The implicit kinds' sources, S*N (a non-finite one refuses the step,
 // signals.cpp:1295-1310).
-template <int S>
+template <int S, uint32_t KC>
 __global__ __launch_bounds__(kSiteBlock) void k_coupled_sources(
-    size_t n, uint32_t ny, float* __restrict__ sources, const uint32_t* __restrict__ offsets,
+    size_t n, uint32_t ny, uint32_t nz, float* __restrict__ sources,
+    const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ order, const float* __restrict__ weights,
     const float* __restrict__ exchange, uint32_t* flags, uint32_t step) {
   const size_t site = static_cast<size_t>(blockIdx.x) * kSiteBlock + threadIdx.x;
   if (site >= n) return;
   float src[S];
-  site_sources<S>(static_cast<uint32_t>(site / ny), static_cast<uint32_t>(site % ny), ny, offsets,
-                  order, weights, exchange, src);
+  site_sources<S, KC>(site, ny, nz, offsets, order, weights, exchange, src);
   bool bad = false;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -435,19 +456,24 @@ __global__ __launch_bounds__(kSiteBlock) void k_coupled_sources(
 
 // The site of each cell's stencil corner below it (clamped into the grid):
 // the key of the spatial order, and the cell's index.
-__global__ __launch_bounds__(kSiteBlock) void k_cell_keys(uint32_t nx, uint32_t ny, float dx,
-                                                          float dy, float ox, float oy,
+__global__ __launch_bounds__(kSiteBlock) void k_cell_keys(uint32_t nx, uint32_t ny, uint32_t nz,
+                                                          float dx, float dy, float dz, float ox,
+                                                          float oy, float oz,
                                                           const float* __restrict__ px,
                                                           const float* __restrict__ py,
+                                                          const float* __restrict__ pz,
                                                           uint32_t cells, uint32_t* __restrict__ keys,
                                                           uint32_t* __restrict__ index) {
   const uint32_t c = blockIdx.x * kSiteBlock + threadIdx.x;
   if (c >= cells) return;
-  const double fx = floor((static_cast<double>(px[c]) - ox) / dx);
-  const double fy = floor((static_cast<double>(py[c]) - oy) / dy);
-  const uint32_t x = static_cast<uint32_t>(fmin(fmax(fx, 0.0), static_cast<double>(nx - 1)));
-  const uint32_t y = static_cast<uint32_t>(fmin(fmax(fy, 0.0), static_cast<double>(ny - 1)));
-  keys[c] = x * ny + y;
+  const auto lower = [](float p, float o, float h, uint32_t dimension) {
+    if (dimension == 1) return 0u;
+    const double f = floor((static_cast<double>(p) - o) / h);
+    return static_cast<uint32_t>(fmin(fmax(f, 0.0), static_cast<double>(dimension - 1)));
+  };
+  const uint32_t x = lower(px[c], ox, dx, nx), y = lower(py[c], oy, dy, ny);
+  const uint32_t z = lower(pz[c], oz, dz, nz);
+  keys[c] = (x * ny + y) * nz + z;
   index[c] = c;
 }
 
@@ -498,20 +524,30 @@ uint32_t cell_threads(uint32_t slots) {
   return 64;
 }
 
+template <int S, uint32_t KC>
+void launch_commit_kc(Context& ctx, bool limited, const float* in, float* out, float dt,
+                      uint32_t step) {
+  auto& c = ctx.coupled;
+  const size_t n = ctx.adr.sites();
+  const uint32_t ny = ctx.adr.params.ny, nz = ctx.adr.params.nz;
+  if (limited) {
+    k_coupled_commit<S, true, KC><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+        n, ny, nz, in, out, dt, c.offsets.data(), c.order.data(), c.weights.data(),
+        c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
+  } else {
+    k_coupled_commit<S, false, KC><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+        n, ny, nz, in, out, dt, c.offsets.data(), c.order.data(), c.weights.data(),
+        c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
+  }
+}
+
 template <int S>
 void launch_commit(Context& ctx, bool limited, const float* in, float* out, float dt,
                    uint32_t step) {
-  auto& c = ctx.coupled;
-  const size_t n = ctx.adr.sites();
-  const uint32_t ny = ctx.adr.params.ny;
-  if (limited) {
-    k_coupled_commit<S, true><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
-        n, ny, in, out, dt, c.offsets.data(), c.order.data(), c.weights.data(),
-        c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
+  if (ctx.adr.params.nz > 1) {
+    launch_commit_kc<S, kCorners3>(ctx, limited, in, out, dt, step);
   } else {
-    k_coupled_commit<S, false><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
-        n, ny, in, out, dt, c.offsets.data(), c.order.data(), c.weights.data(),
-        c.exchange.data(), c.largest.data(), ctx.adr.flags.data(), step);
+    launch_commit_kc<S, kCorners2>(ctx, limited, in, out, dt, step);
   }
 }
 
@@ -519,9 +555,16 @@ template <int S>
 void launch_sources(Context& ctx, uint32_t step) {
   auto& c = ctx.coupled;
   const size_t n = ctx.adr.sites();
-  k_coupled_sources<S><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
-      n, ctx.adr.params.ny, c.sources.data(), c.offsets.data(), c.order.data(), c.weights.data(),
-      c.exchange.data(), ctx.adr.flags.data(), step);
+  const uint32_t ny = ctx.adr.params.ny, nz = ctx.adr.params.nz;
+  if (nz > 1) {
+    k_coupled_sources<S, kCorners3><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+        n, ny, nz, c.sources.data(), c.offsets.data(), c.order.data(), c.weights.data(),
+        c.exchange.data(), ctx.adr.flags.data(), step);
+  } else {
+    k_coupled_sources<S, kCorners2><<<blocks_of(n), kSiteBlock, 0, ctx.stream>>>(
+        n, ny, nz, c.sources.data(), c.offsets.data(), c.order.data(), c.weights.data(),
+        c.exchange.data(), ctx.adr.flags.data(), step);
+  }
 }
 
 void commit(Context& ctx, bool limited, const float* in, float* out, float dt, uint32_t step) {
@@ -561,10 +604,13 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
   CellParams P{};
   P.nx = a.params.nx;
   P.ny = a.params.ny;
+  P.nz = a.params.nz;
   P.dx = a.params.dx;
   P.dy = a.params.dy;
+  P.dz = a.params.dz;
   P.ox = c.origin[0];
   P.oy = c.origin[1];
+  P.oz = c.origin[2];
   P.volume = a.params.dx * a.params.dy * a.params.dz;
   P.dt = dt;
   P.cells = c.cells;
@@ -579,12 +625,12 @@ void cell_phase(Context& ctx, const float* levels, const float* previous, float 
     const uint32_t slots = c.instructions + c.signals + c.species;
     const uint32_t threads = cell_threads(slots);
     const size_t lds = static_cast<size_t>(slots) * threads * sizeof(float);
-    k_coupled_cells<<<(c.cells + threads - 1) / threads, threads, lds, ctx.stream>>>(
+    const auto kernel = P.nz > 1 ? k_coupled_cells<kCorners3> : k_coupled_cells<kCorners2>;
+    kernel<<<(c.cells + threads - 1) / threads, threads, lds, ctx.stream>>>(
         P, a.solid.data(), levels, c.position_x.data(), c.position_y.data(),
         c.position_z.data(), c.lengths.data(), c.radii.data(), c.growth.data(), c.types.data(),
         previous, c.order.data(), c.plan.data(), c.outputs.data(), species_in, species_out,
-        c.exchange.data(), reinterpret_cast<float4*>(c.weights.data()), c.failure.data(),
-        a.flags.data());
+        c.exchange.data(), c.weights.data(), c.failure.data(), a.flags.data());
     check(hipGetLastError(), "launch k_coupled_cells");
   }
 }
@@ -632,7 +678,7 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   for (auto* b : {&c.entries, &c.sorted_keys}) b->reserve(std::max<size_t>(count, 1));
   c.order.reserve(count);
   c.offsets.reserve(n + 1);
-  c.staging.reserve(std::max<size_t>(2 * count, count * K));
+  c.staging.reserve(std::max<size_t>(3 * count, count * K));
   size_t bytes = 0;
   check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c.keys.data(), c.sorted_keys.data(),
                                            c.entries.data(), c.order.data(),
@@ -654,9 +700,11 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   if (count > 0) {
     stage(cells.position_x, count, 0);
     stage(cells.position_y, count, count);
+    stage(cells.position_z, count, 2 * count);
     k_cell_keys<<<blocks_of(count), kSiteBlock, 0, ctx.stream>>>(
-        a.params.nx, a.params.ny, a.params.dx, a.params.dy, origin[0], origin[1],
-        c.staging.data(), c.staging.data() + count, static_cast<uint32_t>(count), c.keys.data(),
+        a.params.nx, a.params.ny, a.params.nz, a.params.dx, a.params.dy, a.params.dz, origin[0],
+        origin[1], origin[2], c.staging.data(), c.staging.data() + count,
+        c.staging.data() + 2 * count, static_cast<uint32_t>(count), c.keys.data(),
         c.entries.data());
     check(hipGetLastError(), "launch k_cell_keys");
     const int end_bit = std::max(1, static_cast<int>(std::bit_width(static_cast<uint64_t>(n))));
@@ -690,8 +738,9 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
     buffer.reserve(count * width);
     if (count == 0 || width == 0) return;
     const size_t items = count * width;
-    const bool positions = host == cells.position_x || host == cells.position_y;
-    const size_t at = host == cells.position_y ? count : 0;
+    const bool positions =
+        host == cells.position_x || host == cells.position_y || host == cells.position_z;
+    const size_t at = host == cells.position_y ? count : (host == cells.position_z ? 2 * count : 0);
     if (!positions) stage(host, items, 0);
     k_permute<<<blocks_of(items), kSiteBlock, 0, ctx.stream>>>(
         c.order.data(), count, width, false, staging + at,
@@ -720,7 +769,7 @@ void coupled_upload(Context& ctx, const float* origin, const msim_cells& cells,
   check(hipMemcpyAsync(c.outputs.data(), outputs.data(), outputs.size() * sizeof(uint32_t),
                        hipMemcpyHostToDevice, ctx.stream),
         "upload plan");
-  c.weights.reserve(count * kCorners);
+  c.weights.reserve(count * (a.params.nz > 1 ? kCorners3 : kCorners2));
   c.exchange.reserve(count * plan.signal_count);
   c.sources.reserve(static_cast<size_t>(plan.signal_count) * n);
   c.largest.reserve(blocks_of(n));
@@ -787,7 +836,11 @@ void coupled_run(Context& ctx, const float* previous_lengths, float dt, uint32_t
     cell_phase(ctx, old, c.previous.data(), dt, step);
     if (!implicit) {
       if (dt > 0.0f) {
-        fe_candidate_f32(ctx, dt, old, next, step, xf, yf);
+        if (a.params.nz > 1) {
+          adr3_candidate(ctx, dt, old, next, step, xf, yf, a.zf.data());
+        } else {
+          fe_candidate_f32(ctx, dt, old, next, step, xf, yf);
+        }
         commit(ctx, limited, next, next, dt, step);
       } else {
         commit(ctx, limited, old, next, dt, step);
