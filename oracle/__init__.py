@@ -67,6 +67,19 @@ class _FlowReport(C.Structure):
                 ("max_speed", C.c_float), ("solved_mean", C.c_float), ("factor", C.c_float)]
 
 
+class _ResolvedReport(C.Structure):
+    _fields_ = [("outer_iterations", C.c_uint32), ("inner_iterations", C.c_uint64),
+                ("relative_residual", C.c_float), ("momentum_relative_residual", C.c_float),
+                ("divergence_rms", C.c_float), ("mean_inlet_speed", C.c_float),
+                ("max_speed", C.c_float), ("min_gap_voxels", C.c_uint32),
+                ("solved_mean", C.c_float), ("factor", C.c_float)]
+
+
+RESOLVED_REPORT_FIELDS = ("outer_iterations", "inner_iterations", "relative_residual",
+                          "momentum_relative_residual", "divergence_rms", "mean_inlet_speed",
+                          "max_speed", "min_gap_voxels")
+
+
 class _BoundaryPatch(C.Structure):
     _fields_ = [("face", C.c_uint32), ("begin", C.c_uint32 * 2), ("end", C.c_uint32 * 2),
                 ("values", C.c_void_p)]
@@ -217,6 +230,10 @@ class Oracle:
         L.mo_advance_coupled_f32.argtypes = [P, P, P, P, P, P, P, F, P, C.c_char_p, Z]
         L.mo_advance_coupled_f32.restype = C.c_int
         L.mo_adr_step_implicit_f32.restype = C.c_int
+        L.mo_resolved_flow_solve.argtypes = [P, P, U, F, F, U, F, U, C.c_int, P, P, C.c_char_p, Z]
+        L.mo_resolved_flow_solve.restype = C.c_int
+        L.mo_resolved_operator.argtypes = [P, P, U, P, P, P, P, P, C.c_char_p, Z]
+        L.mo_resolved_operator.restype = C.c_int
         for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_flow_solve_forced,
                    L.mo_adr_validate_step,
                    L.mo_adr_step_f32, L.mo_adr_step_f64):
@@ -281,6 +298,43 @@ class Oracle:
         report = FlowReport(rep.iterations, rep.relative_residual, rep.true_relative_residual,
                             rep.mean_inlet_speed, rep.max_speed, rep.solved_mean, rep.factor)
         return faces, pressure, report
+
+    def resolved_flow(self, grid: Grid, drag=None, axis: int = 1, mean_inlet_speed: float = 1.0,
+                      tolerance: float = 1e-6, max_outer: int = 500,
+                      inner_tolerance: float = 1e-6, max_inner: int = 50000, scale: bool = True):
+        """Resolved Stokes-Brinkman flow (solve_resolved_flow_cpu): faces
+        (x|y|z) and the report as a dict (with the scaling's solved mean and
+        factor)."""
+        keep: list = []
+        g = grid._c(keep)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        rep = _ResolvedReport()
+        d = _f32(drag)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_resolved_flow_solve(
+            C.byref(g), _ptr(d), C.c_uint32(axis), C.c_float(mean_inlet_speed),
+            C.c_float(tolerance), C.c_uint32(max_outer), C.c_float(inner_tolerance),
+            C.c_uint32(max_inner), C.c_int(1 if scale else 0), faces.ctypes.data, C.byref(rep),
+            err, C.c_size_t(512))
+        self._check(st, err)
+        report = {k: getattr(rep, k) for k in RESOLVED_REPORT_FIELDS + ("solved_mean", "factor")}
+        return faces, report
+
+    def resolved_operator(self, grid: Grid, drag, axis: int, u, p):
+        """(A u, G p, D u) of the resolved system (faces, faces, sites)."""
+        keep: list = []
+        g = grid._c(keep)
+        nf, n = sum(grid.face_counts()), grid.nx * grid.ny * grid.nz
+        u = np.ascontiguousarray(u, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        au, gp, du = np.zeros(nf), np.zeros(nf), np.zeros(n)
+        d = _f32(drag)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_resolved_operator(C.byref(g), _ptr(d), C.c_uint32(axis), u.ctypes.data,
+                                           p.ctypes.data, au.ctypes.data, gp.ctypes.data,
+                                           du.ctypes.data, err, C.c_size_t(512))
+        self._check(st, err)
+        return au, gp, du
 
     def port_flow(self, grid: Grid, mobility, ports, tolerance: float = 1e-6,
                   max_iterations: int = 50000):
@@ -465,6 +519,8 @@ class Reference:
                                               U, U, P, P, U, U, F, F, F, U, P, P, P, C.c_char_p,
                                               Z]
         L.ref_set_threads.argtypes = [U]
+        L.ref_solve_resolved_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, F, U, P, P,
+                                              P, P, P, C.c_char_p, Z]
         L.ref_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, P, P, P, P, P, P,
                                     C.c_char_p, Z]
         L.ref_advance_coupled.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, P, P, P, P, P, U, U,
@@ -510,6 +566,34 @@ class Reference:
         if st != 0:
             raise OracleError(st, err.value.decode(errors="replace"))
         return faces, dict(iterations=it.value, relative_residual=rr.value, max_speed=ms.value)
+
+    def resolved_flow(self, grid: Grid, drag=None, axis=1, mean_inlet_speed=1.0, tolerance=1e-6,
+                      max_outer=500, inner_tolerance=1e-6, max_inner=50000):
+        """solve_resolved_flow_cpu: faces (x|y|z) and the report as a dict."""
+        kinds = self._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        d = _f32(drag)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        outer = C.c_uint32()
+        inner = C.c_uint64()
+        values = np.zeros(5, np.float32)
+        gap = C.c_uint32()
+        err = C.create_string_buffer(512)
+        st = self.lib.ref_solve_resolved_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(d),
+            C.c_uint32(axis), C.c_float(mean_inlet_speed), C.c_float(tolerance),
+            C.c_uint32(max_outer), C.c_float(inner_tolerance), C.c_uint32(max_inner),
+            faces.ctypes.data, C.byref(outer), C.byref(inner), values.ctypes.data, C.byref(gap),
+            err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        report = dict(outer_iterations=outer.value, inner_iterations=inner.value,
+                      relative_residual=float(values[0]),
+                      momentum_relative_residual=float(values[1]),
+                      divergence_rms=float(values[2]), mean_inlet_speed=float(values[3]),
+                      max_speed=float(values[4]), min_gap_voxels=gap.value)
+        return faces, report
 
     def port_flow(self, grid: Grid, mobility, ports, tolerance=1e-6, max_iterations=50000):
         """solve_depth_averaged_port_flow_cpu: faces, port states, report."""
@@ -662,6 +746,8 @@ class Binding:
         P, U, F, Z = C.c_void_p, C.c_uint32, C.c_float, C.c_size_t
         L.bind_device_count.restype = Z
         L.bind_create.argtypes = [C.c_char_p, Z, P, C.c_char_p, Z]
+        L.bind_resolved_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, P, P, P, C.c_char_p,
+                                         Z]
         L.bind_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, Z, P, P, P,
                                 C.c_char_p, Z]
         L.bind_signal_steps.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, P, U, U,
@@ -712,6 +798,29 @@ class Binding:
             raise OracleError(st, err.value.decode(errors="replace"))
         keys = ("iterations", "relative_residual", "max_speed")
         return a, b, [dict(zip(keys, report[3 * k:3 * k + 3].tolist())) for k in range(2)]
+
+    def resolved_flow(self, grid: Grid, drag=None, axis=1, mean_inlet_speed=1.0,
+                      tolerance=1e-6):
+        """solve_resolved_flow through the hip backend and through
+        solve_resolved_flow_cpu: (faces_hip, faces_cpu, reports)."""
+        kinds = Reference._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        d = _f32(drag)
+        total = sum(grid.face_counts())
+        a = np.zeros(total, np.float32)
+        b = np.zeros(total, np.float32)
+        report = np.zeros(12, np.float64)
+        err = C.create_string_buffer(512)
+        st = self.lib.bind_resolved_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(d),
+            C.c_uint32(axis), C.c_float(mean_inlet_speed), C.c_float(tolerance), a.ctypes.data,
+            b.ctypes.data, report.ctypes.data, err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        keys = ("outer_iterations", "inner_iterations", "relative_residual", "divergence_rms",
+                "max_speed", "min_gap_voxels")
+        return a, b, [dict(zip(keys, report[6 * k:6 * k + 6].tolist())) for k in range(2)]
 
     def signal_steps(self, grid: Grid, levels, dt, steps=1, **kw):
         """advance_signal_grid `steps` times through the hip backend and through

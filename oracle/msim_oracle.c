@@ -2550,3 +2550,681 @@ int mo_advance_coupled_f32(const mo_signal_grid* sg, const float* origin, float*
   free(next_grid);
   return status;
 }
+
+/* ------------------------------------------------------------------------
+ * Resolved Stokes-Brinkman flow along one axis (solve_resolved_flow_cpu,
+ * cpp/cpu/cpu_flow.cpp:379-469 over detail::solve_resolved_axis_flow,
+ * cpp/core/flow_system.hpp:1449-1481): velocity on every face, pressure at
+ * every site, the momentum block solved by Jacobi-PCG inside flexible GMRES
+ * (cpp/core/flexible_gmres.hpp:33-108), the field scaled to the mean inlet
+ * speed (scale_velocity, flow_system.hpp:1368-1415).
+ * ------------------------------------------------------------------------ */
+
+typedef struct {
+  const mo_grid* g;
+  uint32_t dims[3];
+  float h[3];
+  size_t n, nf, off[3], cnt[3];
+  uint8_t* fluid;
+  float* drag;
+  uint8_t* active;
+  uint8_t* exists;
+  float* face_drag;
+  float* diag;
+  float* force;
+  uint32_t axis;
+} resolved_t;
+
+static void resolved_free(resolved_t* s) {
+  free(s->fluid);
+  free(s->drag);
+  free(s->active);
+  free(s->exists);
+  free(s->face_drag);
+  free(s->diag);
+  free(s->force);
+}
+
+/* FlowGridLayout::face_coordinates (flow_system.hpp:98-110). */
+static int face_coords(const resolved_t* s, size_t index, uint32_t f[3]) {
+  int c = 0;
+  while (c < 2 && index >= s->off[c] + s->cnt[c]) ++c;
+  size_t local = index - s->off[c];
+  uint32_t fd[3] = {s->dims[0], s->dims[1], s->dims[2]};
+  ++fd[c];
+  f[2] = (uint32_t)(local % fd[2]);
+  local /= fd[2];
+  f[1] = (uint32_t)(local % fd[1]);
+  f[0] = (uint32_t)(local / fd[1]);
+  return c;
+}
+
+static size_t rsite(const resolved_t* s, uint32_t x, uint32_t y, uint32_t z) {
+  return ((size_t)x * s->dims[1] + y) * s->dims[2] + z;
+}
+
+static size_t rface(const resolved_t* s, int c, uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t fd[3] = {s->dims[0], s->dims[1], s->dims[2]};
+  ++fd[c];
+  return s->off[c] + ((size_t)x * fd[1] + y) * fd[2] + z;
+}
+
+/* adjacent_site (flow_system.hpp:112-125): the site on `side` of a face, or
+ * -1 off the grid. */
+static long long adjacent(const resolved_t* s, int c, const uint32_t f[3], int side) {
+  uint32_t site[3] = {f[0], f[1], f[2]};
+  if (side < 0) {
+    if (f[c] == 0) return -1;
+    --site[c];
+  } else if (f[c] >= s->dims[c]) {
+    return -1;
+  }
+  return (long long)rsite(s, site[0], site[1], site[2]);
+}
+
+/* neighbor_face (flow_system.hpp:144-160): the same component's face one
+ * step along `axis`, or -1 off the face lattice. */
+static long long neighbor_face(const resolved_t* s, size_t index, int axis, int offset) {
+  uint32_t f[3];
+  const int c = face_coords(s, index, f);
+  uint32_t fd[3] = {s->dims[0], s->dims[1], s->dims[2]};
+  ++fd[c];
+  if (offset < 0) {
+    if (f[axis] == 0) return -1;
+    --f[axis];
+  } else {
+    if (f[axis] + 1 >= fd[axis]) return -1;
+    ++f[axis];
+  }
+  return (long long)rface(s, c, f[0], f[1], f[2]);
+}
+
+/* ResolvedFlowSystem (flow_system.hpp:755-858): the general constructor with
+ * axis_flow_boundary (186-201: the lower end open at pressure 1, the upper at
+ * 0) and no known pressure, then the single-axis forcing (762-781). */
+static int resolved_build(const mo_grid* g, const float* drag, uint32_t axis, resolved_t* s,
+                          char* err, size_t errlen) {
+  memset(s, 0, sizeof(*s));
+  s->g = g;
+  s->axis = axis;
+  s->dims[0] = g->nx;
+  s->dims[1] = g->ny;
+  s->dims[2] = g->nz;
+  s->h[0] = g->dx;
+  s->h[1] = g->dy;
+  s->h[2] = g->dz;
+  s->n = (size_t)g->nx * g->ny * g->nz;
+  for (int c = 0; c < 3; ++c) s->cnt[c] = face_count(g, c);
+  s->off[1] = s->cnt[0];
+  s->off[2] = s->cnt[0] + s->cnt[1];
+  s->nf = s->off[2] + s->cnt[2];
+  s->fluid = malloc(s->n);
+  s->drag = calloc(s->n, sizeof(float));
+  s->active = calloc(s->nf, 1);
+  s->exists = calloc(s->nf, 1);
+  s->face_drag = calloc(s->nf, sizeof(float));
+  s->diag = calloc(s->nf, sizeof(float));
+  s->force = calloc(s->nf, sizeof(float));
+  if (!s->fluid || !s->drag || !s->active || !s->exists || !s->face_drag || !s->diag || !s->force)
+    return fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+  if (drag != NULL) memcpy(s->drag, drag, s->n * sizeof(float));
+  for (size_t site = 0; site < s->n; ++site) {
+    s->fluid[site] = solid_at(g, site) ? 0 : 1;
+    if (!isfinite(s->drag[site]) || s->drag[site] < 0.0f)
+      return fail(err, errlen, MO_INVALID_ARGUMENT,
+                  "resolved-flow drag values must be finite and non-negative");
+    if (s->fluid[site] == 0) s->drag[site] = 0.0f;
+  }
+  for (size_t index = 0; index < s->nf; ++index) {
+    uint32_t f[3];
+    const int c = face_coords(s, index, f);
+    const long long lower = adjacent(s, c, f, -1), upper = adjacent(s, c, f, 1);
+    const int lower_fluid = lower >= 0 && s->fluid[lower];
+    const int upper_fluid = upper >= 0 && s->fluid[upper];
+    s->exists[index] = (lower_fluid || upper_fluid) ? 1 : 0;
+    const int exterior = lower < 0 || upper < 0;
+    /* axis_flow_boundary: the flow axis's exterior faces are open, the lower
+     * end at pressure 1, the upper at 0. */
+    const int open = c == (int)axis && (f[axis] == 0 || f[axis] == s->dims[axis]);
+    const float held = (c == (int)axis && f[axis] == 0) ? 1.0f : 0.0f;
+    const int act = exterior ? (open && (lower_fluid || upper_fluid)) : (lower_fluid && upper_fluid);
+    s->active[index] = act ? 1 : 0;
+    if (!act) continue;
+    float sum = 0.0f, count = 0.0f;
+    if (lower_fluid) {
+      sum += s->drag[lower];
+      count += 1.0f;
+    }
+    if (upper_fluid) {
+      sum += s->drag[upper];
+      count += 1.0f;
+    }
+    s->face_drag[index] = sum / count;
+    s->diag[index] = s->face_drag[index];
+    for (int a = 0; a < 3; ++a)
+      if (s->dims[a] > 1) s->diag[index] += 2.0f / (s->h[a] * s->h[a]);
+    const double ku = upper < 0 ? (double)held : 0.0;
+    const double kl = lower < 0 ? (double)held : 0.0;
+    s->force[index] = (float)(-(ku - kl) / (double)s->h[c]);
+  }
+  /* The single-axis system (762-781): force 1 / ((n + 1) h) on the axis's
+   * active faces, zero elsewhere; an open inlet face must exist. */
+  const float force = 1.0f / (((float)s->dims[axis] + 1.0f) * s->h[axis]);
+  int open_inlet = 0;
+  for (size_t index = 0; index < s->nf; ++index) s->force[index] = 0.0f;
+  for (size_t index = s->off[axis]; index < s->off[axis] + s->cnt[axis]; ++index) {
+    if (!s->active[index]) continue;
+    uint32_t f[3];
+    face_coords(s, index, f);
+    s->force[index] = force;
+    open_inlet = open_inlet || f[axis] == 0;
+  }
+  if (!open_inlet)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "the resolved-flow inlet boundary is entirely blocked");
+  return MO_OK;
+}
+
+/* apply_momentum (flow_system.hpp:881-910). */
+static void resolved_momentum(const resolved_t* s, const double* in, double* out) {
+  for (size_t index = 0; index < s->nf; ++index) {
+    out[index] = 0.0;
+    if (!s->active[index]) continue;
+    uint32_t f[3];
+    const int c = face_coords(s, index, f);
+    double result = (double)s->face_drag[index] * in[index];
+    for (int a = 0; a < 3; ++a) {
+      if (s->dims[a] == 1) continue;
+      const double inv = 1.0 / ((double)s->h[a] * s->h[a]);
+      for (int off = -1; off <= 1; off += 2) {
+        const long long nb = neighbor_face(s, index, a, off);
+        double neighbor;
+        if (a == c) {
+          neighbor = nb >= 0 ? in[nb] : in[index];
+        } else if (nb >= 0 && s->exists[nb]) {
+          neighbor = in[nb];
+        } else {
+          neighbor = -in[index];
+        }
+        result -= (neighbor - in[index]) * inv;
+      }
+    }
+    out[index] = result;
+  }
+}
+
+/* gradient (flow_system.hpp:912-932), into out (faces). */
+static void resolved_gradient(const resolved_t* s, const double* p, double* out) {
+  for (size_t index = 0; index < s->nf; ++index) {
+    out[index] = 0.0;
+    if (!s->active[index]) continue;
+    uint32_t f[3];
+    const int c = face_coords(s, index, f);
+    const long long lower = adjacent(s, c, f, -1), upper = adjacent(s, c, f, 1);
+    const double lv = lower >= 0 && s->fluid[lower] ? p[lower] : 0.0;
+    const double uv = upper >= 0 && s->fluid[upper] ? p[upper] : 0.0;
+    out[index] = (uv - lv) / s->h[c];
+  }
+}
+
+/* divergence (flow_system.hpp:934-956), into out (sites). */
+static void resolved_divergence(const resolved_t* s, const double* u, double* out) {
+  for (size_t site = 0; site < s->n; ++site) {
+    out[site] = 0.0;
+    if (!s->fluid[site]) continue;
+    const uint32_t z = (uint32_t)(site % s->dims[2]);
+    const uint32_t y = (uint32_t)((site / s->dims[2]) % s->dims[1]);
+    const uint32_t x = (uint32_t)(site / ((size_t)s->dims[1] * s->dims[2]));
+    const uint32_t co[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c) {
+      uint32_t up[3] = {co[0], co[1], co[2]};
+      ++up[c];
+      const size_t uf = rface(s, c, up[0], up[1], up[2]);
+      const size_t lf = rface(s, c, co[0], co[1], co[2]);
+      out[site] += (u[uf] - u[lf]) / s->h[c];
+    }
+  }
+}
+
+/* pressure_diagonal (flow_system.hpp:958-978). */
+static void resolved_pressure_diagonal(const resolved_t* s, float* pd) {
+  for (size_t site = 0; site < s->n; ++site) {
+    pd[site] = 0.0f;
+    if (!s->fluid[site]) continue;
+    const uint32_t z = (uint32_t)(site % s->dims[2]);
+    const uint32_t y = (uint32_t)((site / s->dims[2]) % s->dims[1]);
+    const uint32_t x = (uint32_t)(site / ((size_t)s->dims[1] * s->dims[2]));
+    const uint32_t co[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a) {
+      uint32_t up[3] = {co[0], co[1], co[2]};
+      ++up[a];
+      const size_t faces[2] = {rface(s, a, co[0], co[1], co[2]), rface(s, a, up[0], up[1], up[2])};
+      for (int k = 0; k < 2; ++k) {
+        const size_t face = faces[k];
+        if (s->active[face] && s->diag[face] > 0.0f)
+          pd[site] += 1.0f / (s->diag[face] * s->h[a] * s->h[a]);
+      }
+    }
+  }
+}
+
+/* minimum_gap_voxels (flow_system.hpp:998-1041) of a single-axis system. */
+static uint32_t resolved_min_gap(const resolved_t* s) {
+  uint32_t shortest = 0;
+  for (int axis = 0; axis < 3; ++axis) {
+    if (axis == (int)s->axis || s->dims[axis] <= 1) continue;
+    const int fa = (axis + 1) % 3, sa = (axis + 2) % 3;
+    for (uint32_t first = 0; first < s->dims[fa]; ++first) {
+      for (uint32_t second = 0; second < s->dims[sa]; ++second) {
+        uint32_t co[3] = {0, 0, 0};
+        co[fa] = first;
+        co[sa] = second;
+        uint32_t run = 0;
+        int through = 0;
+#define OPEN_FACE(along)                                                              \
+  (co2[0] = co[0], co2[1] = co[1], co2[2] = co[2], co2[axis] = (along),                \
+   s->active[rface(s, axis, co2[0], co2[1], co2[2])] != 0)
+        uint32_t co2[3];
+        for (uint32_t along = 0; along < s->dims[axis]; ++along) {
+          co[axis] = along;
+          if (s->fluid[rsite(s, co[0], co[1], co[2])]) {
+            through = through || (along == 0 && OPEN_FACE(0));
+            ++run;
+          } else {
+            if (run != 0 && !through) shortest = shortest == 0 ? run : (run < shortest ? run : shortest);
+            run = 0;
+            through = 0;
+          }
+        }
+        through = through || (run != 0 && OPEN_FACE(s->dims[axis]));
+        if (run != 0 && !through) shortest = shortest == 0 ? run : (run < shortest ? run : shortest);
+#undef OPEN_FACE
+      }
+    }
+  }
+  return shortest;
+}
+
+/* conjugate_gradient (cpu_flow.cpp:53-103) on the momentum block with the
+ * float-rounded right-hand side and the float diagonal, under
+ * preconditioned_conjugate_gradient (conjugate_gradient.hpp:43-88) with the
+ * label "resolved-flow momentum". */
+static int resolved_cg(const resolved_t* s, const double* rhs_d, float tolerance,
+                       uint32_t max_iterations, double* solution, uint64_t* inner, char* err,
+                       size_t errlen) {
+  const size_t n = s->nf;
+  double* r = malloc((n + 1) * sizeof(double));
+  double* z = calloc(n + 1, sizeof(double));
+  double* p = malloc((n + 1) * sizeof(double));
+  double* q = malloc((n + 1) * sizeof(double));
+  int status = MO_OK;
+  for (size_t i = 0; i < n; ++i) {
+    solution[i] = 0.0;
+    r[i] = (double)(float)rhs_d[i];
+  }
+#define PRECONDITION()                                                  \
+  for (size_t i = 0; i < n; ++i) z[i] = s->diag[i] > 0.0f ? r[i] / s->diag[i] : 0.0;
+  PRECONDITION();
+  for (size_t i = 0; i < n; ++i) p[i] = z[i];
+  uint32_t iterations = 0;
+  const double rhs_norm_squared = dot(r, r, n);
+  if (rhs_norm_squared == 0.0) goto done;
+  {
+    const double rhs_norm = sqrt(rhs_norm_squared);
+    double rho = dot(r, z, n);
+    double relative = 1.0;
+    for (uint32_t iteration = 1; iteration <= max_iterations; ++iteration) {
+      resolved_momentum(s, p, q);
+      const double curvature = dot(p, q, n);
+      if (!isfinite(curvature) || curvature <= 0.0) {
+        status = fail(err, errlen, MO_CG_CURVATURE,
+                      "resolved-flow momentum conjugate gradient encountered non-positive curvature");
+        goto done;
+      }
+      const double alpha = rho / curvature;
+      if (!isfinite(alpha)) {
+        status = fail(err, errlen, MO_CG_STEP,
+                      "resolved-flow momentum conjugate gradient produced a non-finite step");
+        goto done;
+      }
+      for (size_t i = 0; i < n; ++i) {
+        solution[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+      }
+      const double rr = dot(r, r, n);
+      relative = sqrt(rr > 0.0 ? rr : 0.0) / rhs_norm;
+      if (!isfinite(relative)) {
+        status = fail(err, errlen, MO_CG_RESIDUAL,
+                      "resolved-flow momentum conjugate gradient produced a non-finite residual");
+        goto done;
+      }
+      iterations = iteration;
+      if (relative <= (double)tolerance) goto done;
+      PRECONDITION();
+      const double next_rho = dot(r, z, n);
+      if (!isfinite(next_rho) || rho == 0.0) {
+        status = fail(err, errlen, MO_CG_BREAKDOWN,
+                      "resolved-flow momentum conjugate gradient encountered a preconditioner breakdown");
+        goto done;
+      }
+      const double beta = next_rho / rho;
+      if (!isfinite(beta)) {
+        status = fail(err, errlen, MO_CG_DIRECTION,
+                      "resolved-flow momentum conjugate gradient produced a non-finite direction");
+        goto done;
+      }
+      for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+      rho = next_rho;
+    }
+    status = fail(err, errlen, MO_NOT_CONVERGED,
+                  "resolved-flow momentum conjugate gradient did not converge: relative %f",
+                  relative);
+  }
+#undef PRECONDITION
+done:
+  *inner += iterations;
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  return status;
+}
+
+typedef struct {
+  const resolved_t* s;
+  double continuity_scale;
+  float* pd;
+  float inner_tolerance;
+  uint32_t max_inner;
+  uint64_t inner;
+  double* work_u; /* nf */
+  double* work_g; /* nf */
+  double* work_d; /* n */
+} resolved_ops_t;
+
+/* ops.apply (cpu_flow.cpp:414-423): [A G; cs D] [u; p]. */
+static void resolved_apply(resolved_ops_t* o, const double* in, double* out) {
+  const resolved_t* s = o->s;
+  resolved_momentum(s, in, o->work_u);
+  resolved_gradient(s, in + s->nf, o->work_g);
+  resolved_divergence(s, in, o->work_d);
+  for (size_t i = 0; i < s->nf; ++i) out[i] = o->work_u[i] + o->work_g[i];
+  for (size_t i = 0; i < s->n; ++i) out[s->nf + i] = o->continuity_scale * o->work_d[i];
+}
+
+/* ops.precondition (cpu_flow.cpp:424-431): the momentum block by CG, the
+ * pressure by the negative scaled diagonal. */
+static int resolved_precondition(resolved_ops_t* o, const double* in, double* out, char* err,
+                                 size_t errlen) {
+  const resolved_t* s = o->s;
+  const int status =
+      resolved_cg(s, in, o->inner_tolerance, o->max_inner, out, &o->inner, err, errlen);
+  if (status != MO_OK) return status;
+  for (size_t i = 0; i < s->n; ++i)
+    out[s->nf + i] =
+        o->pd[i] > 0.0f ? -in[s->nf + i] / (o->continuity_scale * o->pd[i]) : 0.0;
+  return MO_OK;
+}
+
+/* flexible_gmres (flexible_gmres.hpp:33-108): restarts of 40, twice-modified
+ * Gram-Schmidt, Givens rotations, the true residual tested at each restart. */
+static int resolved_fgmres(resolved_ops_t* o, const double* rhs, double tolerance,
+                           uint32_t max_iterations, double* solution, uint32_t* iterations_out,
+                           double* relative_out, char* err, size_t errlen) {
+  const size_t m = o->s->nf + o->s->n;
+  enum { RESTART = 40 };
+  double* residual = calloc(m, sizeof(double));
+  double* work = calloc(m, sizeof(double));
+  double* basis = calloc((size_t)(RESTART + 1) * m, sizeof(double));
+  double* directions = calloc((size_t)RESTART * m, sizeof(double));
+  int status = MO_OK;
+  for (size_t i = 0; i < m; ++i) solution[i] = 0.0;
+  *iterations_out = 0;
+  *relative_out = 0.0;
+  const double rhs_norm = sqrt(dot(rhs, rhs, m));
+  if (!isfinite(rhs_norm)) {
+    status = fail(err, errlen, MO_RUNTIME_ERROR, "FGMRES non-finite right-hand side");
+    goto done;
+  }
+  if (rhs_norm == 0.0) goto done;
+  uint32_t iterations = 0;
+  while (1) {
+    resolved_apply(o, solution, work);
+    for (size_t i = 0; i < m; ++i) residual[i] = rhs[i];
+    for (size_t i = 0; i < m; ++i) residual[i] += -1.0 * work[i];
+    const double beta = sqrt(dot(residual, residual, m));
+    if (!isfinite(beta)) {
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "FGMRES non-finite true residual");
+      goto done;
+    }
+    if (beta <= tolerance * rhs_norm) {
+      *iterations_out = iterations;
+      *relative_out = beta / rhs_norm;
+      goto done;
+    }
+    if (iterations >= max_iterations) {
+      status = fail(err, errlen, MO_RUNTIME_ERROR,
+                    "resolved-flow FGMRES did not converge: relative residual %f", beta / rhs_norm);
+      goto done;
+    }
+    const uint32_t count = RESTART < max_iterations - iterations ? RESTART : max_iterations - iterations;
+    double h[RESTART + 1][RESTART];
+    double cosine[RESTART], sine[RESTART], g[RESTART + 1];
+    memset(h, 0, sizeof(h));
+    memset(g, 0, sizeof(g));
+    g[0] = beta;
+    for (size_t i = 0; i < m; ++i) basis[i] = 0.0;
+    for (size_t i = 0; i < m; ++i) basis[i] += (1 / beta) * residual[i];
+    uint32_t used = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+      double* direction = directions + (size_t)j * m;
+      status = resolved_precondition(o, basis + (size_t)j * m, direction, err, errlen);
+      if (status != MO_OK) goto done;
+      resolved_apply(o, direction, work);
+      for (int pass = 0; pass < 2; ++pass) {
+        for (uint32_t i = 0; i <= j; ++i) {
+          const double* bi = basis + (size_t)i * m;
+          const double projection = dot(bi, work, m);
+          h[i][j] += projection;
+          for (size_t k = 0; k < m; ++k) work[k] += -projection * bi[k];
+        }
+      }
+      h[j + 1][j] = sqrt(dot(work, work, m));
+      if (!isfinite(h[j + 1][j])) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "FGMRES non-finite Arnoldi vector");
+        goto done;
+      }
+      const int happy = h[j + 1][j] <= 1e-14;
+      double* next = basis + (size_t)(j + 1) * m;
+      for (size_t k = 0; k < m; ++k) next[k] = 0.0;
+      if (!happy)
+        for (size_t k = 0; k < m; ++k) next[k] += (1 / h[j + 1][j]) * work[k];
+      for (uint32_t i = 0; i < j; ++i) {
+        const double upper = cosine[i] * h[i][j] + sine[i] * h[i + 1][j];
+        h[i + 1][j] = -sine[i] * h[i][j] + cosine[i] * h[i + 1][j];
+        h[i][j] = upper;
+      }
+      const double diagonal = hypot(h[j][j], h[j + 1][j]);
+      if (diagonal == 0) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "FGMRES Arnoldi breakdown");
+        goto done;
+      }
+      cosine[j] = h[j][j] / diagonal;
+      sine[j] = h[j + 1][j] / diagonal;
+      h[j][j] = diagonal;
+      h[j + 1][j] = 0;
+      g[j + 1] = -sine[j] * g[j];
+      g[j] *= cosine[j];
+      ++iterations;
+      used = j + 1;
+      if (happy || fabs(g[j + 1]) <= tolerance * rhs_norm) break;
+    }
+    double weights[RESTART];
+    for (uint32_t i = used; i-- > 0;) {
+      double value = g[i];
+      for (uint32_t j = i + 1; j < used; ++j) value -= h[i][j] * weights[j];
+      weights[i] = value / h[i][i];
+    }
+    for (uint32_t i = 0; i < used; ++i) {
+      const double* d = directions + (size_t)i * m;
+      for (size_t k = 0; k < m; ++k) solution[k] += weights[i] * d[k];
+    }
+  }
+done:
+  free(residual);
+  free(work);
+  free(basis);
+  free(directions);
+  return status;
+}
+
+int mo_resolved_flow_solve(const mo_grid* g, const float* drag, uint32_t axis,
+                           float mean_inlet_speed, float tolerance, uint32_t max_outer,
+                           float inner_tolerance, uint32_t max_inner, int scale, float* faces,
+                           mo_resolved_report* report, char* err, size_t errlen) {
+  /* ResolvedFlowParameters::validate (cpu_flow.cpp:152-168). */
+  if (!isfinite(mean_inlet_speed) || mean_inlet_speed == 0.0f)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "resolved-flow mean inlet speed must be finite and nonzero");
+  if (!isfinite(tolerance) || tolerance < FLT_EPSILON)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "resolved-flow outer relative tolerance must be finite and at least float machine "
+                "epsilon");
+  if (!isfinite(inner_tolerance) || inner_tolerance < FLT_EPSILON)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "resolved-flow inner relative tolerance must be finite and at least float machine "
+                "epsilon");
+  if (max_outer == 0 || max_inner == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "resolved-flow iteration limits must be positive");
+  if (axis > 2) return fail(err, errlen, MO_INVALID_ARGUMENT, "unknown resolved-flow axis");
+  int status = validate_flow_grid(g, g->obstacles, axis, err, errlen);
+  if (status != MO_OK) return status;
+  resolved_t s;
+  status = resolved_build(g, drag, axis, &s, err, errlen);
+  if (status != MO_OK) {
+    resolved_free(&s);
+    return status;
+  }
+  const size_t m = s.nf + s.n;
+  float hmin = s.h[0] < s.h[1] ? s.h[0] : s.h[1];
+  if (s.h[2] < hmin) hmin = s.h[2];
+  resolved_ops_t o;
+  o.s = &s;
+  o.continuity_scale = 1.0 / hmin;
+  o.pd = malloc(s.n * sizeof(float));
+  o.inner_tolerance = inner_tolerance;
+  o.max_inner = max_inner;
+  o.inner = 0;
+  o.work_u = malloc(s.nf * sizeof(double));
+  o.work_g = malloc(s.nf * sizeof(double));
+  o.work_d = malloc(s.n * sizeof(double));
+  double* rhs = calloc(m, sizeof(double));
+  double* solution = calloc(m, sizeof(double));
+  double* residual = calloc(m, sizeof(double));
+  float* unscaled = malloc(s.nf * sizeof(float));
+  resolved_pressure_diagonal(&s, o.pd);
+  for (size_t i = 0; i < s.nf; ++i) rhs[i] = (double)s.force[i];
+  uint32_t outer = 0;
+  double relative = 0.0;
+  status = resolved_fgmres(&o, rhs, (double)tolerance, max_outer, solution, &outer, &relative, err,
+                           errlen);
+  if (status == MO_OK) {
+    resolved_apply(&o, solution, residual);
+    double momentum_square = 0.0, force_square = 0.0;
+    for (size_t i = 0; i < s.nf; ++i) {
+      momentum_square += (residual[i] - rhs[i]) * (residual[i] - rhs[i]);
+      force_square += rhs[i] * rhs[i];
+    }
+    resolved_divergence(&s, solution, o.work_d);
+    double divergence_square = 0.0;
+    size_t fluid_count = 0;
+    for (size_t site = 0; site < s.n; ++site) {
+      if (s.fluid[site]) {
+        divergence_square += o.work_d[site] * o.work_d[site];
+        ++fluid_count;
+      }
+    }
+    const double divergence_rms =
+        fluid_count == 0 ? 0.0 : sqrt(divergence_square / (double)fluid_count);
+    for (size_t i = 0; i < s.nf; ++i) unscaled[i] = (float)solution[i];
+    /* scale_velocity (flow_system.hpp:1368-1415) over the open inlet faces
+     * (open_inlet_faces, 980-996). */
+    double inlet_sum = 0.0;
+    size_t inlet_count = 0;
+    float peak = 0.0f;
+    for (size_t i = 0; i < s.nf && status == MO_OK; ++i) {
+      if (!isfinite(unscaled[i]))
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+      const float a = fabsf(unscaled[i]);
+      if (a > peak) peak = a;
+    }
+    for (size_t i = s.off[axis]; i < s.off[axis] + s.cnt[axis] && status == MO_OK; ++i) {
+      uint32_t f[3];
+      face_coords(&s, i, f);
+      if (f[axis] == 0 && s.active[i]) {
+        inlet_sum += unscaled[i];
+        ++inlet_count;
+      }
+    }
+    if (status == MO_OK && inlet_count == 0)
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "flow solve has no open inlet faces");
+    float solved_mean = 0.0f, factor = 1.0f, scaled_peak = 0.0f;
+    if (status == MO_OK) {
+      solved_mean = (float)(inlet_sum / (double)inlet_count);
+      if (peak == 0.0f || solved_mean <= 1.0e-9f * peak)
+        status = fail(err, errlen, MO_RUNTIME_ERROR,
+                      "the device carries no through-flow: the outlet is unreachable");
+    }
+    if (status == MO_OK) {
+      factor = scale ? (mean_inlet_speed - 0.0f) / solved_mean : 1.0f;
+      for (size_t i = 0; i < s.nf && status == MO_OK; ++i) {
+        const float v = unscaled[i] * factor + 0.0f;
+        if (!isfinite(v))
+          status = fail(err, errlen, MO_RUNTIME_ERROR, "non-finite combined flow velocity");
+        faces[i] = v;
+        const float a = fabsf(v);
+        if (a > scaled_peak) scaled_peak = a;
+      }
+    }
+    if (status == MO_OK && report != NULL) {
+      report->outer_iterations = outer;
+      report->inner_iterations = o.inner;
+      report->relative_residual = (float)relative;
+      report->momentum_relative_residual = (float)sqrt(momentum_square / force_square);
+      report->divergence_rms = (float)(divergence_rms * fabsf(factor));
+      report->mean_inlet_speed = mean_inlet_speed;
+      report->max_speed = scaled_peak;
+      report->min_gap_voxels = resolved_min_gap(&s);
+      report->solved_mean = solved_mean;
+      report->factor = factor;
+    }
+  }
+  free(o.pd);
+  free(o.work_u);
+  free(o.work_g);
+  free(o.work_d);
+  free(rhs);
+  free(solution);
+  free(residual);
+  free(unscaled);
+  resolved_free(&s);
+  return status;
+}
+
+/* The resolved system's blocks on given vectors (for tests of the operator):
+ * au = A u (momentum, faces), gp = G p (faces), du = D u (sites). */
+int mo_resolved_operator(const mo_grid* g, const float* drag, uint32_t axis, const double* u,
+                         const double* p, double* au, double* gp, double* du, char* err,
+                         size_t errlen) {
+  if (axis > 2) return fail(err, errlen, MO_INVALID_ARGUMENT, "unknown resolved-flow axis");
+  int status = validate_flow_grid(g, g->obstacles, axis, err, errlen);
+  if (status != MO_OK) return status;
+  resolved_t s;
+  status = resolved_build(g, drag, axis, &s, err, errlen);
+  if (status == MO_OK) {
+    resolved_momentum(&s, u, au);
+    resolved_gradient(&s, p, gp);
+    resolved_divergence(&s, u, du);
+  }
+  resolved_free(&s);
+  return status;
+}

@@ -114,6 +114,38 @@ int mo_flow_solve_forced(const mo_grid* grid, const float* mobility, uint32_t ax
                          int scale, const float* force, size_t force_columns, float* faces,
                          double* pressure, mo_flow_report* report, char* err, size_t errlen);
 
+/* ResolvedFlowReport (cpp/include/microsimulator/flow.hpp:53-62), with the
+ * scaling's solved mean and factor. */
+typedef struct {
+  uint32_t outer_iterations;
+  uint64_t inner_iterations;
+  float relative_residual;
+  float momentum_relative_residual;
+  float divergence_rms;
+  float mean_inlet_speed;
+  float max_speed;
+  uint32_t min_gap_voxels;
+  float solved_mean;
+  float factor;
+} mo_resolved_report;
+
+/* solve_resolved_flow_cpu (cpp/cpu/cpu_flow.cpp:463-467): the resolved
+ * Stokes-Brinkman flow along `axis` (0, 1 or 2), drag one float per site or
+ * NULL (zero), FGMRES to `tolerance` with the momentum block by Jacobi-PCG to
+ * `inner_tolerance`; faces: x|y|z velocities of every face, scaled to
+ * mean_inlet_speed (scale = 0: the unscaled solve). */
+int mo_resolved_flow_solve(const mo_grid* grid, const float* drag, uint32_t axis,
+                           float mean_inlet_speed, float tolerance, uint32_t max_outer,
+                           float inner_tolerance, uint32_t max_inner, int scale, float* faces,
+                           mo_resolved_report* report, char* err, size_t errlen);
+
+/* The resolved system's blocks applied to given vectors (faces u, sites p):
+ * au = A u, gp = G p, du = D u (ResolvedFlowSystem::apply_momentum, gradient,
+ * divergence, flow_system.hpp:881-956). */
+int mo_resolved_operator(const mo_grid* grid, const float* drag, uint32_t axis, const double* u,
+                         const double* p, double* au, double* gp, double* du, char* err,
+                         size_t errlen);
+
 /* FlowPortKind and FlowPort (cpp/include/microsimulator/flow.hpp:69-91): an
  * opening over a rectangle of one fixed face, driven at a mean speed (into
  * the grid positive) or held at a pressure. In-plane indices as for a

@@ -207,6 +207,45 @@ int ref_solve_depth_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, f
   });
 }
 
+// solve_resolved_flow_cpu (cpp/cpu/cpu_flow.cpp:463-467): the resolved
+// Stokes-Brinkman field (x|y|z faces) and its report.
+int ref_solve_resolved_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                            float dy, float dz, const std::uint8_t* obstacles,
+                            const std::uint8_t* kinds, const float* drag, std::uint32_t axis,
+                            float mean_inlet_speed, float tolerance, std::uint32_t max_outer,
+                            float inner_tolerance, std::uint32_t max_inner, float* faces,
+                            std::uint32_t* outer_iterations, std::uint64_t* inner_iterations,
+                            float* report_values, std::uint32_t* min_gap, char* err,
+                            std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> d;
+    if (drag != nullptr) d = std::span<const float>(drag, n);
+    ms::ResolvedFlowParameters parameters;
+    parameters.mean_inlet_speed = mean_inlet_speed;
+    parameters.axis = static_cast<ms::FlowAxis>(axis);
+    parameters.relative_tolerance = tolerance;
+    parameters.max_outer_iterations = max_outer;
+    parameters.inner_relative_tolerance = inner_tolerance;
+    parameters.max_inner_iterations = max_inner;
+    const auto result = ms::solve_resolved_flow_cpu(spec, d, parameters);
+    std::memcpy(faces, result.field.x_faces.data(), result.field.x_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size(), result.field.y_faces.data(),
+                result.field.y_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size() + result.field.y_faces.size(),
+                result.field.z_faces.data(), result.field.z_faces.size() * sizeof(float));
+    *outer_iterations = result.report.outer_iterations;
+    *inner_iterations = result.report.inner_iterations;
+    report_values[0] = result.report.relative_residual;
+    report_values[1] = result.report.momentum_relative_residual;
+    report_values[2] = result.report.divergence_rms;
+    report_values[3] = result.report.mean_inlet_speed;
+    report_values[4] = result.report.max_speed;
+    *min_gap = result.report.min_gap_voxels;
+  });
+}
+
 // The same with DepthAveragedFlowParameters::body_force: force_columns pairs
 // (fx, fy), x-major, y-minor.
 int ref_solve_depth_flow_forced(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
@@ -566,6 +605,48 @@ int bind_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, fl
       report[3 * k] = static_cast<float>(results[k]->report.iterations);
       report[3 * k + 1] = results[k]->report.relative_residual;
       report[3 * k + 2] = results[k]->report.max_speed;
+    }
+  });
+}
+
+// solve_resolved_flow through the hip backend (faces_hip) and through
+// solve_resolved_flow_cpu (faces_cpu). report: outer and inner iterations,
+// relative residual, divergence RMS, max speed and minimum gap of each (hip
+// first).
+int bind_resolved_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx, float dy,
+                       float dz, const std::uint8_t* obstacles, const std::uint8_t* kinds,
+                       const float* drag, std::uint32_t axis, float mean_inlet_speed,
+                       float tolerance, float* faces_hip, float* faces_cpu, double* report,
+                       char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> d;
+    if (drag != nullptr) d = std::span<const float>(drag, n);
+    ms::ResolvedFlowParameters parameters;
+    parameters.mean_inlet_speed = mean_inlet_speed;
+    parameters.axis = static_cast<ms::FlowAxis>(axis);
+    parameters.relative_tolerance = tolerance;
+    parameters.inner_relative_tolerance = tolerance;
+    const auto backend = ms::make_hip_backend(0);
+    const auto hip = backend->solve_resolved_flow(spec, d, parameters);
+    const auto cpu = ms::solve_resolved_flow_cpu(spec, d, parameters);
+    float* out[2] = {faces_hip, faces_cpu};
+    const ms::ResolvedFlowResult* results[2] = {&hip, &cpu};
+    for (int k = 0; k < 2; ++k) {
+      const auto& field = results[k]->field;
+      std::memcpy(out[k], field.x_faces.data(), field.x_faces.size() * sizeof(float));
+      std::memcpy(out[k] + field.x_faces.size(), field.y_faces.data(),
+                  field.y_faces.size() * sizeof(float));
+      std::memcpy(out[k] + field.x_faces.size() + field.y_faces.size(), field.z_faces.data(),
+                  field.z_faces.size() * sizeof(float));
+      const auto& r = results[k]->report;
+      report[6 * k] = r.outer_iterations;
+      report[6 * k + 1] = static_cast<double>(r.inner_iterations);
+      report[6 * k + 2] = r.relative_residual;
+      report[6 * k + 3] = r.divergence_rms;
+      report[6 * k + 4] = r.max_speed;
+      report[6 * k + 5] = r.min_gap_voxels;
     }
   });
 }
