@@ -105,45 +105,59 @@ __global__ void k_mg_coarsen(LevelView f, float* __restrict__ wx, float* __restr
   bc[c] = b;
 }
 
-// A applied to the staged x (row pitch PITCH) at position (hx, hy) of site i.
-template <int PITCH>
-__device__ inline float tile_ax_p(const LevelView& L, const float* s_x, int hx, int hy, size_t i,
-                                  uint32_t fx, uint32_t fy) {
-  const float xc = s_x[hx * PITCH + hy];
-  const float wxl = L.wx[i];
-  const float wxu = fx + 1 < L.nx ? L.wx[i + L.ny] : 0.0f;
-  const float wyl = L.wy[i];
-  const float wyu = fy + 1 < L.ny ? L.wy[i + 1] : 0.0f;
-  float ax = L.bc[i] * xc;
-  ax += wxl * (xc - s_x[(hx - 1) * PITCH + hy]);
-  ax += wxu * (xc - s_x[(hx + 1) * PITCH + hy]);
-  ax += wyl * (xc - s_x[hx * PITCH + hy - 1]);
-  ax += wyu * (xc - s_x[hx * PITCH + hy + 1]);
+// A tile grown by 2 in LDS (kHX2 x kHY2, pitch kHY2): r, the inverse
+// diagonal, the couplings to the lower x and y neighbours and the boundary
+// term, each zero off the grid, and two iterates. Every sweep reads only
+// LDS, so each global value of a tile is loaded once.
+constexpr int kTile2 = kHX2 * kHY2;
+struct TileLds {
+  float r[kTile2], d[kTile2], wx[kTile2], wy[kTile2], bc[kTile2], x[kTile2], y[kTile2];
+};
+
+// Loads the tile's operator and r, and x = init(i, fx, fy) (zero off the grid).
+template <class Init>
+__device__ inline void stage_tile(TileLds& T, const LevelView& L, const float* __restrict__ r,
+                                  uint32_t fx0, uint32_t fy0, Init&& init) {
+  for (int idx = threadIdx.x; idx < kTile2; idx += kMgThreads) {
+    const int64_t fx = static_cast<int64_t>(fx0) + idx / kHY2 - 2;
+    const int64_t fy = static_cast<int64_t>(fy0) + idx % kHY2 - 2;
+    float rv = 0.0f, dv = 0.0f, wxv = 0.0f, wyv = 0.0f, bv = 0.0f, xv = 0.0f;
+    if (fx >= 0 && fx < L.nx && fy >= 0 && fy < L.ny) {
+      const size_t i = static_cast<size_t>(fx) * L.ny + static_cast<size_t>(fy);
+      rv = r[i];
+      dv = L.dinv[i];
+      wxv = L.wx[i];
+      wyv = L.wy[i];
+      bv = L.bc[i];
+      xv = init(i, rv, dv, static_cast<uint32_t>(fx), static_cast<uint32_t>(fy));
+    }
+    T.r[idx] = rv;
+    T.d[idx] = dv;
+    T.wx[idx] = wxv;
+    T.wy[idx] = wyv;
+    T.bc[idx] = bv;
+    T.x[idx] = xv;
+  }
+}
+
+// (A s)_p at LDS position p: the boundary term, then the lower and upper x
+// couplings, then the lower and upper y couplings.
+__device__ inline float lds_ax(const TileLds& T, const float* s, int p) {
+  const float xc = s[p];
+  float ax = T.bc[p] * xc;
+  ax += T.wx[p] * (xc - s[p - kHY2]);
+  ax += T.wx[p + kHY2] * (xc - s[p + kHY2]);
+  ax += T.wy[p] * (xc - s[p - 1]);
+  ax += T.wy[p + 1] * (xc - s[p + 1]);
   return ax;
 }
 
-__device__ inline float tile_ax(const LevelView& L, const float* s_x, int hx, int hy, size_t i,
-                                uint32_t fx, uint32_t fy) {
-  return tile_ax_p<kHY>(L, s_x, hx, hy, i, fx, fy);
-}
-
-// One damped-Jacobi sweep from the iterate staged on the tile grown by 2
-// (s_in, pitch kHY2) onto the tile grown by 1 (s_out, pitch kHY); zero off
-// the grid.
-__device__ inline void tile_sweep(const LevelView& L, const float* __restrict__ r,
-                                  const float* s_in, float* s_out, uint32_t fx0, uint32_t fy0) {
+// One damped-Jacobi sweep out = in + omega D^-1 (r - A in) on the tile grown
+// by 1; off the grid the inverse diagonal and the iterate are zero, so out is.
+__device__ inline void lds_sweep(TileLds& T, const float* in, float* out) {
   for (int idx = threadIdx.x; idx < kHX * kHY; idx += kMgThreads) {
-    const int hx = idx / kHY, hy = idx % kHY;
-    const int64_t fx = static_cast<int64_t>(fx0) + hx - 1;
-    const int64_t fy = static_cast<int64_t>(fy0) + hy - 1;
-    float v = 0.0f;
-    if (fx >= 0 && fx < L.nx && fy >= 0 && fy < L.ny) {
-      const uint32_t gx = static_cast<uint32_t>(fx), gy = static_cast<uint32_t>(fy);
-      const size_t i = static_cast<size_t>(gx) * L.ny + gy;
-      const float xc = s_in[(hx + 1) * kHY2 + hy + 1];
-      v = xc + kOmega * L.dinv[i] * (r[i] - tile_ax_p<kHY2>(L, s_in, hx + 1, hy + 1, i, gx, gy));
-    }
-    s_out[idx] = v;
+    const int p = (idx / kHY + 1) * kHY2 + idx % kHY + 1;
+    out[p] = in[p] + kOmega * T.d[p] * (T.r[p] - lds_ax(T, in, p));
   }
 }
 
@@ -161,23 +175,14 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_down(LevelView L, const float
                                                         float* __restrict__ rc, uint32_t nxc,
                                                         uint32_t nyc, const CgState* st,
                                                         uint32_t tiles_y) {
-  __shared__ float s_x1[kHX2 * kHY2];
-  __shared__ float s_x2[kHX * kHY];
+  __shared__ TileLds T;
   if (cycle_stopped(st)) return;
   const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t fx0 = (t / tiles_y) * kTX, fy0 = (t % tiles_y) * kTY;
-  for (int idx = threadIdx.x; idx < kHX2 * kHY2; idx += kMgThreads) {
-    const int64_t fx = static_cast<int64_t>(fx0) + idx / kHY2 - 2;
-    const int64_t fy = static_cast<int64_t>(fy0) + idx % kHY2 - 2;
-    float v = 0.0f;
-    if (fx >= 0 && fx < L.nx && fy >= 0 && fy < L.ny) {
-      const size_t i = static_cast<size_t>(fx) * L.ny + static_cast<size_t>(fy);
-      v = kOmega * L.dinv[i] * r[i];
-    }
-    s_x1[idx] = v;
-  }
+  stage_tile(T, L, r, fx0, fy0,
+             [](size_t, float rv, float dv, uint32_t, uint32_t) { return kOmega * dv * rv; });
   __syncthreads();
-  tile_sweep(L, r, s_x1, s_x2, fx0, fy0);
+  lds_sweep(T, T.x, T.y);
   __syncthreads();
   const int lx = threadIdx.x / (kTY / 2), ly = threadIdx.x % (kTY / 2);
   float sum = 0.0f;
@@ -188,9 +193,9 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_down(LevelView L, const float
       const uint32_t fx = fx0 + 2 * lx + a, fy = fy0 + 2 * ly + b;
       if (fx < L.nx && fy < L.ny) {
         const size_t i = static_cast<size_t>(fx) * L.ny + fy;
-        const int hx = 2 * lx + a + 1, hy = 2 * ly + b + 1;
-        sum += r[i] - tile_ax(L, s_x2, hx, hy, i, fx, fy);
-        xp[i] = s_x2[hx * kHY + hy];
+        const int p = (2 * lx + a + 2) * kHY2 + 2 * ly + b + 2;
+        sum += T.r[p] - lds_ax(T, T.y, p);
+        xp[i] = T.y[p];
       }
     }
   }
@@ -219,8 +224,7 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* 
                                                       float* __restrict__ out, const CgState* st,
                                                       UpDots dots, int pre, uint32_t tiles_y,
                                                       uint32_t tiles) {
-  __shared__ float s_y[kHX2 * kHY2];
-  __shared__ float s_y1[kHX * kHY];
+  __shared__ TileLds T;
   __shared__ double s_red[kMgThreads / kWave];
   if (cycle_stopped(st)) return;
   const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
@@ -228,21 +232,13 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* 
   double zr = 0.0, zq = 0.0;
   for (uint32_t t = t0; t < t1; ++t) {
     const uint32_t fx0 = (t / tiles_y) * kTX, fy0 = (t % tiles_y) * kTY;
-    for (int idx = threadIdx.x; idx < kHX2 * kHY2; idx += kMgThreads) {
-      const int64_t fx = static_cast<int64_t>(fx0) + idx / kHY2 - 2;
-      const int64_t fy = static_cast<int64_t>(fy0) + idx % kHY2 - 2;
-      float v = 0.0f;
-      if (fx >= 0 && fx < L.nx && fy >= 0 && fy < L.ny) {
-        const size_t i = static_cast<size_t>(fx) * L.ny + static_cast<size_t>(fy);
-        const size_t c = static_cast<size_t>(fx / 2) * nyc + static_cast<size_t>(fy / 2);
-        v = xp[i] + kCorrection * ec[c];
-      }
-      s_y[idx] = v;
-    }
+    stage_tile(T, L, r, fx0, fy0, [&](size_t i, float, float, uint32_t fx, uint32_t fy) {
+      return xp[i] + kCorrection * ec[static_cast<size_t>(fx / 2) * nyc + fy / 2];
+    });
     __syncthreads();
-    if (pre == 0) tile_sweep(L, r, s_y, s_y1, fx0, fy0);
+    if (pre == 0) lds_sweep(T, T.x, T.y);
     __syncthreads();
-    // One site per thread in row order: unit-stride loads and stores.
+    // One site per thread in row order: unit-stride stores.
 #pragma unroll
     for (int k = 0; k < (kTX * kTY) / kMgThreads; ++k) {
       const int idx = static_cast<int>(threadIdx.x) + k * kMgThreads;
@@ -250,11 +246,9 @@ __global__ __launch_bounds__(kMgThreads) void k_mg_up(LevelView L, const float* 
       const uint32_t fx = fx0 + lxs, fy = fy0 + lys;
       if (fx < L.nx && fy < L.ny) {
         const size_t i = static_cast<size_t>(fx) * L.ny + fy;
-        const int hx = lxs + 1, hy = lys + 1;
-        const float ri = r[i];
-        const float z = pre != 0 ? s_y[(lxs + 2) * kHY2 + lys + 2]
-                                 : s_y1[hx * kHY + hy] +
-                                       kOmega * L.dinv[i] * (ri - tile_ax(L, s_y1, hx, hy, i, fx, fy));
+        const int p = (lxs + 2) * kHY2 + lys + 2;
+        const float ri = T.r[p];
+        const float z = pre != 0 ? T.x[p] : T.y[p] + kOmega * T.d[p] * (ri - lds_ax(T, T.y, p));
         out[i] = z;
         if (dots.q != nullptr) {
           zr += static_cast<double>(z) * static_cast<double>(ri);
