@@ -255,6 +255,45 @@ int msim_hip_solve_depth_averaged_port_flow(msim_ctx* ctx, const msim_grid* grid
                                             msim_port_flow_report* report,
                                             msim_port_state* states);
 
+/* ResolvedFlowParameters (cpp/include/microsimulator/flow.hpp:42-51). */
+typedef struct {
+  float mean_inlet_speed;          /* nonzero */
+  uint32_t axis;                   /* 0 x, 1 y, 2 z */
+  float relative_tolerance;        /* FGMRES, >= float epsilon */
+  uint32_t max_outer_iterations;
+  float inner_relative_tolerance;  /* the momentum block's CG, >= float epsilon */
+  uint32_t max_inner_iterations;
+} msim_resolved_params;
+
+/* ResolvedFlowReport (cpp/include/microsimulator/flow.hpp:53-62), and the
+ * device time of the solve. */
+typedef struct {
+  uint32_t outer_iterations;
+  uint64_t inner_iterations;
+  float relative_residual;
+  float momentum_relative_residual;
+  float divergence_rms;
+  float mean_inlet_speed;
+  float max_speed;
+  uint32_t min_gap_voxels;
+  double seconds;
+} msim_resolved_report;
+
+/* ComputeBackend::solve_resolved_flow (backend.hpp:50-52; the reference's
+ * solve_resolved_flow_cpu, cpp/cpu/cpu_flow.cpp:379-467): the resolved
+ * Stokes-Brinkman flow of the voxel grid along an axis, velocity on every
+ * face and pressure at every site, by flexible GMRES (restarts of 40,
+ * cpp/core/flexible_gmres.hpp:33-108) preconditioned by Jacobi-PCG on the
+ * momentum block and the scaled pressure diagonal, all vectors FP64 on the
+ * device; the field scaled to the mean inlet speed. drag: one float per site
+ * (the Brinkman drag) or NULL (zero). x/y/z_faces: the field (any may be
+ * NULL). Errors as the reference's: invalid arguments, a grid that carries no
+ * flow, and the inner or outer solve failing to converge (MSIM_NOT_CONVERGED
+ * or MSIM_RUNTIME_ERROR with its message). */
+int msim_hip_solve_resolved_flow(msim_ctx* ctx, const msim_grid* grid, const float* drag,
+                                 const msim_resolved_params* params, float* x_faces,
+                                 float* y_faces, float* z_faces, msim_resolved_report* report);
+
 /* A signal grid for the explicit transport step: SignalGridSpec's transport
  * fields (signals.hpp:122-156) flattened. boundary_values: 6*S reservoir
  * values, face f of signal s at [6*s + f] (uniform per face). Velocity field

@@ -229,7 +229,8 @@ EXPORTS = (
     "msim_hip_learned_apply", "msim_hip_batch_upload", "msim_hip_batch_solve_flow",
     "msim_hip_batch_autotune", "msim_hip_batch_shapes", "msim_hip_batch_shape",
     "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
-    "msim_hip_solve_depth_averaged_port_flow", "msim_hip_adr_report",
+    "msim_hip_solve_depth_averaged_port_flow", "msim_hip_solve_resolved_flow",
+    "msim_hip_adr_report",
     "msim_hip_advance_coupled", "msim_hip_coupled_upload", "msim_hip_coupled_run",
     "msim_hip_coupled_download", "msim_hip_coupled_seconds", "msim_hip_colony_volume_fraction",
     "msim_hip_colony_mobility", "msim_hip_colony_seconds",
@@ -288,6 +289,7 @@ def library(path: str = LIBRARY):
         "msim_hip_batch_run": (I, [P, C.c_float, U, U, P, P]),
         "msim_hip_batch_download": (I, [P, U, P, P, P, P]),
         "msim_hip_solve_depth_averaged_port_flow": (I, [P, P, P, P, U, P, P, P, P, P, P]),
+        "msim_hip_solve_resolved_flow": (I, [P, P, P, P, P, P, P, P]),
         "msim_hip_adr_report": (I, [P, P]),
         "msim_hip_advance_coupled": (I, [P, P, P, P, P, P, P, P, C.c_float, P]),
         "msim_hip_coupled_upload": (I, [P, P, P, P, P, P, P]),
@@ -442,6 +444,50 @@ class DepthAveragedFlowReport:
                    r.seconds, r.apply_seconds, r.apply_launches, r.update_seconds,
                    r.update_launches, r.cycle_down_seconds, r.cycle_up_seconds,
                    r.cycle_rest_seconds, r.cycles)
+
+
+class _ResolvedParams(C.Structure):
+    _fields_ = [("mean_inlet_speed", C.c_float), ("axis", C.c_uint32),
+                ("relative_tolerance", C.c_float), ("max_outer_iterations", C.c_uint32),
+                ("inner_relative_tolerance", C.c_float), ("max_inner_iterations", C.c_uint32)]
+
+
+class _ResolvedReport(C.Structure):
+    _fields_ = [("outer_iterations", C.c_uint32), ("inner_iterations", C.c_uint64),
+                ("relative_residual", C.c_float), ("momentum_relative_residual", C.c_float),
+                ("divergence_rms", C.c_float), ("mean_inlet_speed", C.c_float),
+                ("max_speed", C.c_float), ("min_gap_voxels", C.c_uint32),
+                ("seconds", C.c_double)]
+
+
+@dataclass
+class ResolvedFlowParameters:
+    """ResolvedFlowParameters (flow.hpp:42-51)."""
+    mean_inlet_speed: float = 1.0
+    axis: int = 1
+    relative_tolerance: float = 1.0e-6
+    max_outer_iterations: int = 500
+    inner_relative_tolerance: float = 1.0e-6
+    max_inner_iterations: int = 50_000
+
+    def _c(self) -> _ResolvedParams:
+        return _ResolvedParams(self.mean_inlet_speed, self.axis, self.relative_tolerance,
+                               self.max_outer_iterations, self.inner_relative_tolerance,
+                               self.max_inner_iterations)
+
+
+@dataclass
+class ResolvedFlowReport:
+    """ResolvedFlowReport (flow.hpp:53-62), and the device seconds."""
+    outer_iterations: int
+    inner_iterations: int
+    relative_residual: float
+    momentum_relative_residual: float
+    divergence_rms: float
+    mean_inlet_speed: float
+    max_speed: float
+    min_gap_voxels: int
+    seconds: float
 
 
 @dataclass
@@ -724,6 +770,9 @@ class BackendInfo:
 
 FEATURES = ("growth", "species", "cell_contacts", "external_constraints", "signals",
             "coupled_rates", "depth_averaged_flow", "resolved_flow", "contact_dynamics")
+# BackendFeature::resolved_flow also covers the resolved port flow, which this
+# backend does not implement, so it is not advertised; solve_resolved_flow
+# itself runs.
 SUPPORTED = frozenset({"signals", "coupled_rates", "depth_averaged_flow"})
 
 
@@ -800,6 +849,34 @@ class HipBackend:
             xf.ctypes.data, yf.ctypes.data, zf.ctypes.data, C.byref(rep)))
         return DepthAveragedFlowResult(VelocityField(xf, yf, zf),
                                        DepthAveragedFlowReport._from(rep))
+
+    def solve_resolved_flow(self, grid: Grid, drag=None,
+                            parameters: ResolvedFlowParameters | None = None):
+        """ComputeBackend::solve_resolved_flow (backend.hpp:50-52): (field,
+        ResolvedFlowReport). drag: one float per site (the Brinkman drag) or
+        None."""
+        parameters = parameters or ResolvedFlowParameters()
+        keep: list = []
+        g = grid._c(keep)
+        d = None
+        if drag is not None:
+            d = np.ascontiguousarray(drag, dtype=np.float32).reshape(-1)
+            if d.size != grid.sites:
+                raise InvalidArgument("resolved-flow drag must hold one value per grid site")
+        nfx, nfy, nfz = grid.face_counts()
+        xf = np.zeros(nfx, np.float32)
+        yf = np.zeros(nfy, np.float32)
+        zf = np.zeros(nfz, np.float32)
+        p = parameters._c()
+        rep = _ResolvedReport()
+        self._check(self.lib.msim_hip_solve_resolved_flow(
+            self.ctx, C.byref(g), None if d is None else d.ctypes.data, C.byref(p),
+            xf.ctypes.data, yf.ctypes.data, zf.ctypes.data, C.byref(rep)))
+        report = ResolvedFlowReport(rep.outer_iterations, rep.inner_iterations,
+                                    rep.relative_residual, rep.momentum_relative_residual,
+                                    rep.divergence_rms, rep.mean_inlet_speed, rep.max_speed,
+                                    rep.min_gap_voxels, rep.seconds)
+        return VelocityField(xf, yf, zf), report
 
     def solve_depth_averaged_port_flow(self, grid: Grid, ports, mobility=None,
                                        parameters: DepthAveragedPortFlowParameters | None = None
@@ -1264,4 +1341,4 @@ class HipBackend:
 
     advance_growth = advance_species = _unsupported
     find_cell_contacts = find_external_contacts = solve_contact_forces = _unsupported
-    solve_resolved_flow = solve_resolved_port_flow = _unsupported
+    solve_resolved_port_flow = _unsupported

@@ -144,6 +144,8 @@ class HipBackend final : public ComputeBackend {
             .native = true};
   }
   [[nodiscard]] bool supports(BackendFeature f) const noexcept override {
+    // resolved_flow also covers solve_resolved_port_flow, which is not built
+    // here; solve_resolved_flow itself runs.
     return f == BackendFeature::signals || f == BackendFeature::coupled_rates ||
            f == BackendFeature::depth_averaged_flow;
   }
@@ -308,9 +310,36 @@ class HipBackend final : public ComputeBackend {
                                           const ContactSolveParameters&) override {
     unsupported();
   }
-  ResolvedFlowResult solve_resolved_flow(const SignalGridSpec&, std::span<const float>,
-                                         const ResolvedFlowParameters&) override {
-    unsupported();
+  ResolvedFlowResult solve_resolved_flow(const SignalGridSpec& spec, std::span<const float> drag,
+                                         const ResolvedFlowParameters& p) override {
+    p.validate();
+    const msim_grid g = grid_of(spec);
+    if (!drag.empty() && drag.size() != spec.site_count()) {
+      throw std::invalid_argument("resolved-flow drag must hold one value per grid site");
+    }
+    const msim_resolved_params params{p.mean_inlet_speed, static_cast<std::uint32_t>(p.axis),
+                                      p.relative_tolerance, p.max_outer_iterations,
+                                      p.inner_relative_tolerance, p.max_inner_iterations};
+    ResolvedFlowResult result;
+    result.field.x_faces.resize(spec.x_face_count());
+    result.field.y_faces.resize(spec.y_face_count());
+    result.field.z_faces.resize(spec.z_face_count());
+    msim_resolved_report report{};
+    if (const int s = msim_hip_solve_resolved_flow(
+            ctx_, &g, drag.empty() ? nullptr : drag.data(), &params, result.field.x_faces.data(),
+            result.field.y_faces.data(), result.field.z_faces.data(), &report);
+        s != MSIM_OK) {
+      rethrow(s, ctx_);
+    }
+    result.report = {.outer_iterations = report.outer_iterations,
+                     .inner_iterations = report.inner_iterations,
+                     .relative_residual = report.relative_residual,
+                     .momentum_relative_residual = report.momentum_relative_residual,
+                     .divergence_rms = report.divergence_rms,
+                     .mean_inlet_speed = report.mean_inlet_speed,
+                     .max_speed = report.max_speed,
+                     .min_gap_voxels = report.min_gap_voxels};
+    return result;
   }
   ResolvedPortFlowResult solve_resolved_port_flow(const SignalGridSpec&,
                                                   std::span<const FlowPort>,

@@ -22,7 +22,7 @@ OUT = os.path.join(HERE, "libmsim_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["flow.hip", "mg.hip", "adr.hip", "adr3.hip", "learned.hip", "batch.hip", "ports.hip", "implicit.hip",
-           "coupled.hip", "colony.hip", "nutrient.hip", "stream.hip", "capi.cpp"]
+           "coupled.hip", "colony.hip", "nutrient.hip", "stream.hip", "resolved.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSIM_OFFLOAD_ARCH", "gfx950")
 

@@ -1294,6 +1294,52 @@ int msim_hip_solve_depth_averaged_flow(msim_ctx* ctx, const msim_grid* grid,
   return msim_hip_flow_download(ctx, nullptr, x_faces, y_faces, z_faces);
 }
 
+int msim_hip_solve_resolved_flow(msim_ctx* ctx, const msim_grid* grid, const float* drag,
+                                 const msim_resolved_params* params, float* x_faces,
+                                 float* y_faces, float* z_faces, msim_resolved_report* report) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || params == nullptr) msim::invalid("grid and parameters must not be null");
+    // ResolvedFlowParameters::validate (cpu_flow.cpp:152-168).
+    if (!std::isfinite(params->mean_inlet_speed) || params->mean_inlet_speed == 0.0f) {
+      msim::invalid("resolved-flow mean inlet speed must be finite and nonzero");
+    }
+    const auto tolerance = [](float t, const char* name) {
+      if (!std::isfinite(t) || t < std::numeric_limits<float>::epsilon()) {
+        msim::invalid(std::string(name) + " must be finite and at least float machine epsilon");
+      }
+    };
+    tolerance(params->relative_tolerance, "resolved-flow outer relative tolerance");
+    tolerance(params->inner_relative_tolerance, "resolved-flow inner relative tolerance");
+    if (params->max_outer_iterations == 0 || params->max_inner_iterations == 0) {
+      msim::invalid("resolved-flow iteration limits must be positive");
+    }
+    if (params->axis > 2) msim::invalid("unknown resolved-flow axis");
+    // validate_flow_grid (cpu_flow.cpp:288-336).
+    msim::validate_lattice(*grid);
+    for (int a = 0; a < 3; ++a) {
+      if (grid->boundary[2 * a] == MSIM_PERIODIC || grid->boundary[2 * a + 1] == MSIM_PERIODIC) {
+        msim::invalid("native flow solvers do not support periodic boundaries");
+      }
+    }
+    const uint32_t axis = params->axis;
+    if (grid->boundary[2 * axis] != MSIM_FIXED || grid->boundary[2 * axis + 1] != MSIM_FIXED) {
+      msim::invalid("native flow-axis boundaries must be fixed to identify inlet and outlet");
+    }
+    const std::vector<uint8_t> mask = msim::obstacle_mask(*grid);
+    msim::check_reachable(grid->nx, grid->ny, grid->nz, mask, axis);
+    const size_t nfx = static_cast<size_t>(grid->nx + 1) * grid->ny * grid->nz;
+    const size_t nfy = static_cast<size_t>(grid->nx) * (grid->ny + 1) * grid->nz;
+    const size_t nfz = static_cast<size_t>(grid->nx) * grid->ny * (grid->nz + 1);
+    std::vector<float> faces(nfx + nfy + nfz);
+    msim_resolved_report rep{};
+    msim::resolved_flow(c, *grid, mask, drag, *params, faces.data(), rep);
+    if (x_faces != nullptr) std::memcpy(x_faces, faces.data(), nfx * sizeof(float));
+    if (y_faces != nullptr) std::memcpy(y_faces, faces.data() + nfx, nfy * sizeof(float));
+    if (z_faces != nullptr) std::memcpy(z_faces, faces.data() + nfx + nfy, nfz * sizeof(float));
+    if (report != nullptr) *report = rep;
+  });
+}
+
 int msim_hip_solve_depth_averaged_port_flow(msim_ctx* ctx, const msim_grid* grid,
                                             const float* mobility, const msim_flow_port* ports,
                                             uint32_t port_count,

@@ -474,6 +474,13 @@ void mg_coarse_operator(hipStream_t stream, const float* fwx, const float* fwy, 
                         uint32_t fnx, uint32_t fny, float* wx, float* wy, float* bc, float* dinv,
                         uint32_t nx, uint32_t ny);
 
+// resolved.hip: the resolved Stokes-Brinkman flow along params.axis of a grid
+// already validated (mask: its obstacles, 0/1 per site); faces_out: every
+// face velocity (x|y|z), scaled.
+void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
+                   const float* drag, const msim_resolved_params& params, float* faces_out,
+                   msim_resolved_report& report);
+
 // batch.hip: the batched sweep (msim_hip_batch_*).
 void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
                   const msim_batch_params& params);

@@ -1,0 +1,946 @@
+// Resolved Stokes-Brinkman flow on the device (msim_hip_solve_resolved_flow):
+// solve_resolved_flow_cpu (cpp/cpu/cpu_flow.cpp:379-467) over
+// ResolvedFlowSystem (cpp/core/flow_system.hpp:755-1077) with the reference's
+// flexible GMRES (cpp/core/flexible_gmres.hpp:33-108).
+//
+// Unknowns: the velocity on every face of the voxel grid (x|y|z faces, the
+// reference's order) and the pressure at every site, FP64 throughout, as the
+// reference computes them. The system [A G; cs D] (momentum with Brinkman
+// drag and the no-slip wall rule, pressure gradient, scaled divergence) is
+// applied by one kernel; the preconditioner is Jacobi-PCG on the momentum
+// block (the float-rounded right-hand side and the float diagonal, as the
+// reference's conjugate_gradient takes them) and the scaled pressure
+// diagonal. Every scalar of both Krylov methods stays on the device: dot
+// products leave fixed-order block partials that a one-workgroup kernel adds
+// and turns into the step (CG's alpha and beta, the stopping test; GMRES's
+// projections and norms), so the vectors never wait on the host. The host
+// reads a few scalars per Arnoldi step (its Hessenberg column) and runs the
+// Givens rotations and the small triangular solve, as the reference does.
+//
+// The system's arrays (active and existing faces, face drag, diagonal,
+// forcing, pressure diagonal) are built on the host from the grid, in the
+// reference's float arithmetic, and uploaded once per solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace msim {
+namespace {
+
+constexpr int kResBlock = 256;
+constexpr int kResChunk = 4096;   // entries per dot-product block
+constexpr uint32_t kRestart = 40;  // flexible_gmres.hpp:46
+
+// The grid as the kernels see it.
+struct ResView {
+  uint32_t dims[3];
+  uint64_t off[3];    // first face of each component
+  uint64_t nf, n;     // faces, sites
+  double h[3];        // spacing
+  double inv[3];      // 1 / h^2, 0 on an axis of one site (no viscous term)
+  double cs;          // continuity scale 1 / min spacing
+  const uint8_t* active;
+  const uint8_t* exists;
+  const uint8_t* fluid;
+  const float* face_drag;
+  const float* diag;
+  const float* pd;    // pressure diagonal
+};
+
+__device__ inline int face_component(const ResView& V, uint64_t f) {
+  return f < V.off[1] ? 0 : (f < V.off[2] ? 1 : 2);
+}
+
+// Face f's coordinates in its component's face lattice.
+__device__ inline void face_xyz(const ResView& V, int c, uint64_t f, uint32_t fd[3],
+                                uint32_t co[3]) {
+  fd[0] = V.dims[0];
+  fd[1] = V.dims[1];
+  fd[2] = V.dims[2];
+  ++fd[c];
+  uint64_t local = f - V.off[c];
+  co[2] = static_cast<uint32_t>(local % fd[2]);
+  local /= fd[2];
+  co[1] = static_cast<uint32_t>(local % fd[1]);
+  co[0] = static_cast<uint32_t>(local / fd[1]);
+}
+
+__device__ inline uint64_t face_at(const ResView& V, int c, const uint32_t fd[3], uint32_t x,
+                                   uint32_t y, uint32_t z) {
+  return V.off[c] + (static_cast<uint64_t>(x) * fd[1] + y) * fd[2] + z;
+}
+
+__device__ inline uint64_t site_at(const ResView& V, uint32_t x, uint32_t y, uint32_t z) {
+  return (static_cast<uint64_t>(x) * V.dims[1] + y) * V.dims[2] + z;
+}
+
+// apply_momentum (flow_system.hpp:881-910) at face f.
+__device__ inline double momentum_at(const ResView& V, const double* __restrict__ u, uint64_t f,
+                                     int c, const uint32_t fd[3], const uint32_t co[3]) {
+  const double uf = u[f];
+  double result = static_cast<double>(V.face_drag[f]) * uf;
+  for (int a = 0; a < 3; ++a) {
+    if (V.dims[a] == 1) continue;
+    for (int off = -1; off <= 1; off += 2) {
+      uint32_t nb[3] = {co[0], co[1], co[2]};
+      bool has = true;
+      if (off < 0) {
+        if (nb[a] == 0) has = false; else --nb[a];
+      } else {
+        if (nb[a] + 1 >= fd[a]) has = false; else ++nb[a];
+      }
+      double neighbor;
+      if (a == c) {
+        neighbor = has ? u[face_at(V, c, fd, nb[0], nb[1], nb[2])] : uf;
+      } else if (has && V.exists[face_at(V, c, fd, nb[0], nb[1], nb[2])] != 0) {
+        neighbor = u[face_at(V, c, fd, nb[0], nb[1], nb[2])];
+      } else {
+        neighbor = -uf;
+      }
+      result -= (neighbor - uf) * V.inv[a];
+    }
+  }
+  return result;
+}
+
+// out = A u on the faces (zero on the system's inactive ones).
+__global__ __launch_bounds__(kResBlock) void k_res_momentum(ResView V, const double* __restrict__ u,
+                                                            double* __restrict__ out) {
+  const uint64_t f = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (f >= V.nf) return;
+  if (V.active[f] == 0) {
+    out[f] = 0.0;
+    return;
+  }
+  const int c = face_component(V, f);
+  uint32_t fd[3], co[3];
+  face_xyz(V, c, f, fd, co);
+  out[f] = momentum_at(V, u, f, c, fd, co);
+}
+
+// out = [A u + G p; cs D u] (cpu_flow.cpp:414-423; gradient and divergence,
+// flow_system.hpp:912-956).
+__global__ __launch_bounds__(kResBlock) void k_res_apply(ResView V, const double* __restrict__ in,
+                                                         double* __restrict__ out) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (k < V.nf) {
+    const uint64_t f = k;
+    if (V.active[f] == 0) {
+      out[f] = 0.0 + 0.0;
+      return;
+    }
+    const int c = face_component(V, f);
+    uint32_t fd[3], co[3];
+    face_xyz(V, c, f, fd, co);
+    const double m = momentum_at(V, in, f, c, fd, co);
+    const double* p = in + V.nf;
+    double lv = 0.0, uv = 0.0;
+    if (co[c] > 0) {
+      uint32_t s[3] = {co[0], co[1], co[2]};
+      --s[c];
+      const uint64_t site = site_at(V, s[0], s[1], s[2]);
+      if (V.fluid[site] != 0) lv = p[site];
+    }
+    if (co[c] < V.dims[c]) {
+      const uint64_t site = site_at(V, co[0], co[1], co[2]);
+      if (V.fluid[site] != 0) uv = p[site];
+    }
+    out[f] = m + (uv - lv) / V.h[c];
+    return;
+  }
+  if (k >= V.nf + V.n) return;
+  const uint64_t site = k - V.nf;
+  double d = 0.0;
+  if (V.fluid[site] != 0) {
+    const uint32_t z = static_cast<uint32_t>(site % V.dims[2]);
+    const uint32_t y = static_cast<uint32_t>((site / V.dims[2]) % V.dims[1]);
+    const uint32_t x = static_cast<uint32_t>(site / (static_cast<uint64_t>(V.dims[1]) * V.dims[2]));
+    const uint32_t co[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c) {
+      uint32_t fd[3] = {V.dims[0], V.dims[1], V.dims[2]};
+      ++fd[c];
+      uint32_t up[3] = {co[0], co[1], co[2]};
+      ++up[c];
+      const double hi = in[face_at(V, c, fd, up[0], up[1], up[2])];
+      const double lo = in[face_at(V, c, fd, co[0], co[1], co[2])];
+      d += (hi - lo) / V.h[c];
+    }
+  }
+  out[k] = V.cs * d;
+}
+
+// Block partials of a . b over chunks of kResChunk entries.
+__global__ __launch_bounds__(kResBlock) void k_res_dot(const double* __restrict__ a,
+                                                       const double* __restrict__ b, uint64_t count,
+                                                       double* __restrict__ partials) {
+  __shared__ double s_red[kResBlock / kWave];
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
+  const uint64_t end = begin + kResChunk < count ? begin + kResChunk : count;
+  double v = 0.0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) v += a[i] * b[i];
+  v = block_sum<kResBlock>(v, s_red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+// The solve's scalars, in device memory.
+struct ResState {
+  // CG
+  double rho, rhs_norm, alpha, beta, relative;
+  uint32_t iterations, done, failure, pad;
+  // GMRES: the last dot's value and the slots the Arnoldi step writes
+  double value;
+};
+
+enum : uint32_t {
+  kCgRun = 0,
+  kCgDone = 1,
+};
+
+// The sum of `blocks` partials in a fixed order, by one workgroup.
+__device__ inline double sum_partials(const double* partials, uint32_t blocks, double* s_red) {
+  double v = 0.0;
+  for (uint32_t i = threadIdx.x; i < blocks; i += kResBlock) v += partials[i];
+  return block_sum<kResBlock>(v, s_red);
+}
+
+// CG set-up after r, z and their partials (r.r, r.z): the reference's
+// initialize, rhs norm and rho (conjugate_gradient.hpp:49-55).
+__global__ __launch_bounds__(kResBlock) void k_cg_begin(const double* partials, uint32_t blocks,
+                                                        ResState* st) {
+  __shared__ double s_red[kResBlock / kWave];
+  const double rr = sum_partials(partials, blocks, s_red);
+  const double rz = sum_partials(partials + blocks, blocks, s_red);
+  if (threadIdx.x != 0) return;
+  st->iterations = 0;
+  st->failure = 0;
+  st->relative = 1.0;
+  if (rr == 0.0) {
+    st->done = kCgDone;
+    st->rhs_norm = 0.0;
+    return;
+  }
+  st->done = kCgRun;
+  st->rhs_norm = sqrt(rr);
+  st->rho = rz;
+}
+
+// alpha = rho / p.q with the curvature and step checks (57-65).
+__global__ __launch_bounds__(kResBlock) void k_cg_alpha(const double* partials, uint32_t blocks,
+                                                        ResState* st) {
+  __shared__ double s_red[kResBlock / kWave];
+  if (st->done != kCgRun) return;
+  const double curvature = sum_partials(partials, blocks, s_red);
+  if (threadIdx.x != 0) return;
+  if (!isfinite(curvature) || curvature <= 0.0) {
+    st->failure = MSIM_CG_CURVATURE;
+    st->done = kCgDone;
+    return;
+  }
+  const double alpha = st->rho / curvature;
+  if (!isfinite(alpha)) {
+    st->failure = MSIM_CG_STEP;
+    st->done = kCgDone;
+    return;
+  }
+  st->alpha = alpha;
+}
+
+// After x += alpha p, r -= alpha q, z = D^-1 r: the relative residual and the
+// stopping test, then beta (66-84).
+__global__ __launch_bounds__(kResBlock) void k_cg_beta(const double* partials, uint32_t blocks,
+                                                       float tolerance, ResState* st) {
+  __shared__ double s_red[kResBlock / kWave];
+  if (st->done != kCgRun) return;
+  const double rr = sum_partials(partials, blocks, s_red);
+  const double rz = sum_partials(partials + blocks, blocks, s_red);
+  if (threadIdx.x != 0) return;
+  const double relative = sqrt(rr > 0.0 ? rr : 0.0) / st->rhs_norm;
+  if (!isfinite(relative)) {
+    st->failure = MSIM_CG_RESIDUAL;
+    st->done = kCgDone;
+    return;
+  }
+  st->iterations += 1;
+  st->relative = relative;
+  if (relative <= static_cast<double>(tolerance)) {
+    st->done = kCgDone;
+    return;
+  }
+  if (!isfinite(rz) || st->rho == 0.0) {
+    st->failure = MSIM_CG_BREAKDOWN;
+    st->done = kCgDone;
+    return;
+  }
+  const double beta = rz / st->rho;
+  if (!isfinite(beta)) {
+    st->failure = MSIM_CG_DIRECTION;
+    st->done = kCgDone;
+    return;
+  }
+  st->beta = beta;
+  st->rho = rz;
+}
+
+// x = 0, r = double(float(rhs)), z = D^-1 r, p = z, and the partials of
+// r.r and r.z.
+__global__ __launch_bounds__(kResBlock) void k_cg_init(ResView V, const double* __restrict__ rhs,
+                                                       double* __restrict__ x,
+                                                       double* __restrict__ r,
+                                                       double* __restrict__ z,
+                                                       double* __restrict__ p,
+                                                       double* __restrict__ partials) {
+  __shared__ double s_red[kResBlock / kWave];
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
+  const uint64_t end = begin + kResChunk < V.nf ? begin + kResChunk : V.nf;
+  double rr = 0.0, rz = 0.0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
+    const double ri = static_cast<double>(static_cast<float>(rhs[i]));
+    const float d = V.diag[i];
+    const double zi = d > 0.0f ? ri / d : 0.0;
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    rr += ri * ri;
+    rz += ri * zi;
+  }
+  rr = block_sum<kResBlock>(rr, s_red);
+  rz = block_sum<kResBlock>(rz, s_red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = rr;
+    partials[gridDim.x + blockIdx.x] = rz;
+  }
+}
+
+// q = A p and the partials of p.q.
+__global__ __launch_bounds__(kResBlock) void k_cg_apply(ResView V, const double* __restrict__ p,
+                                                        double* __restrict__ q,
+                                                        double* __restrict__ partials,
+                                                        const ResState* st) {
+  __shared__ double s_red[kResBlock / kWave];
+  if (st->done != kCgRun) return;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
+  const uint64_t end = begin + kResChunk < V.nf ? begin + kResChunk : V.nf;
+  double pq = 0.0;
+  for (uint64_t f = begin + threadIdx.x; f < end; f += kResBlock) {
+    double v = 0.0;
+    if (V.active[f] != 0) {
+      const int c = face_component(V, f);
+      uint32_t fd[3], co[3];
+      face_xyz(V, c, f, fd, co);
+      v = momentum_at(V, p, f, c, fd, co);
+    }
+    q[f] = v;
+    pq += p[f] * v;
+  }
+  pq = block_sum<kResBlock>(pq, s_red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = pq;
+}
+
+// x += alpha p, r -= alpha q, z = D^-1 r, and the partials of r.r and r.z.
+__global__ __launch_bounds__(kResBlock) void k_cg_update(ResView V, double* __restrict__ x,
+                                                         double* __restrict__ r,
+                                                         double* __restrict__ z,
+                                                         const double* __restrict__ p,
+                                                         const double* __restrict__ q,
+                                                         double* __restrict__ partials,
+                                                         const ResState* st) {
+  __shared__ double s_red[kResBlock / kWave];
+  if (st->done != kCgRun) return;
+  const double alpha = st->alpha;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
+  const uint64_t end = begin + kResChunk < V.nf ? begin + kResChunk : V.nf;
+  double rr = 0.0, rz = 0.0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const float d = V.diag[i];
+    const double zi = d > 0.0f ? ri / d : 0.0;
+    z[i] = zi;
+    rr += ri * ri;
+    rz += ri * zi;
+  }
+  rr = block_sum<kResBlock>(rr, s_red);
+  rz = block_sum<kResBlock>(rz, s_red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = rr;
+    partials[gridDim.x + blockIdx.x] = rz;
+  }
+}
+
+// p = z + beta p.
+__global__ __launch_bounds__(kResBlock) void k_cg_direction(uint64_t n, double* __restrict__ p,
+                                                            const double* __restrict__ z,
+                                                            const ResState* st) {
+  if (st->done != kCgRun) return;
+  const double beta = st->beta;
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// The pressure part of the preconditioner (cpu_flow.cpp:424-431):
+// out = -in / (cs pd) where pd > 0, else 0.
+__global__ __launch_bounds__(kResBlock) void k_res_pressure_precondition(
+    ResView V, const double* __restrict__ in, double* __restrict__ out) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (i >= V.n) return;
+  const float pd = V.pd[i];
+  out[V.nf + i] = pd > 0.0f ? -in[V.nf + i] / (V.cs * static_cast<double>(pd)) : 0.0;
+}
+
+// target += alpha * source with alpha a host value.
+__global__ __launch_bounds__(kResBlock) void k_res_axpy(uint64_t n, double* __restrict__ target,
+                                                        double alpha,
+                                                        const double* __restrict__ source) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (i < n) target[i] += alpha * source[i];
+}
+
+// One Gram-Schmidt projection (flexible_gmres.hpp:70-76): the dot of the
+// basis vector with w from its partials, kept at `slot`, then w -= it * v.
+__global__ __launch_bounds__(kResBlock) void k_res_project_sum(const double* partials,
+                                                               uint32_t blocks, double* slot) {
+  __shared__ double s_red[kResBlock / kWave];
+  const double v = sum_partials(partials, blocks, s_red);
+  if (threadIdx.x == 0) *slot = v;
+}
+
+__global__ __launch_bounds__(kResBlock) void k_res_project(uint64_t n, double* __restrict__ w,
+                                                           const double* __restrict__ v,
+                                                           const double* slot) {
+  const double projection = *slot;
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (i < n) w[i] += -projection * v[i];
+}
+
+// out = 0 + s * in (a basis vector from w or the residual).
+__global__ __launch_bounds__(kResBlock) void k_res_scaled(uint64_t n, double* __restrict__ out,
+                                                          double s, const double* __restrict__ in) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (i < n) out[i] = 0.0 + s * in[i];
+}
+
+// r = rhs + (-1) * w.
+__global__ __launch_bounds__(kResBlock) void k_res_residual(uint64_t n, double* __restrict__ r,
+                                                            const double* __restrict__ rhs,
+                                                            const double* __restrict__ w) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
+  if (i < n) r[i] = rhs[i] + -1.0 * w[i];
+}
+
+uint32_t grid_of(uint64_t n) { return static_cast<uint32_t>((n + kResBlock - 1) / kResBlock); }
+uint32_t chunks_of(uint64_t n) {
+  return static_cast<uint32_t>(std::max<uint64_t>((n + kResChunk - 1) / kResChunk, 1));
+}
+
+// The system, built on the host (flow_system.hpp:755-858, 958-1041).
+struct ResSystem {
+  uint32_t dims[3];
+  float h[3];
+  uint64_t off[3], cnt[3], nf, n;
+  std::vector<uint8_t> fluid, active, exists;
+  std::vector<float> face_drag, diag, force, pd;
+  uint32_t axis;
+};
+
+void face_coords_host(const ResSystem& s, uint64_t index, int* c, uint32_t co[3]) {
+  int k = 0;
+  while (k < 2 && index >= s.off[k] + s.cnt[k]) ++k;
+  uint64_t local = index - s.off[k];
+  uint32_t fd[3] = {s.dims[0], s.dims[1], s.dims[2]};
+  ++fd[k];
+  co[2] = static_cast<uint32_t>(local % fd[2]);
+  local /= fd[2];
+  co[1] = static_cast<uint32_t>(local % fd[1]);
+  co[0] = static_cast<uint32_t>(local / fd[1]);
+  *c = k;
+}
+
+uint64_t face_host(const ResSystem& s, int c, uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t fd[3] = {s.dims[0], s.dims[1], s.dims[2]};
+  ++fd[c];
+  return s.off[c] + (static_cast<uint64_t>(x) * fd[1] + y) * fd[2] + z;
+}
+
+uint64_t site_host(const ResSystem& s, uint32_t x, uint32_t y, uint32_t z) {
+  return (static_cast<uint64_t>(x) * s.dims[1] + y) * s.dims[2] + z;
+}
+
+// The general constructor with axis_flow_boundary (186-201) and no known
+// pressure, then the single-axis forcing (762-781).
+ResSystem build_system(const msim_grid& g, const std::vector<uint8_t>& mask, const float* drag,
+                       uint32_t axis) {
+  ResSystem s{};
+  s.axis = axis;
+  s.dims[0] = g.nx;
+  s.dims[1] = g.ny;
+  s.dims[2] = g.nz;
+  s.h[0] = g.dx;
+  s.h[1] = g.dy;
+  s.h[2] = g.dz;
+  s.n = static_cast<uint64_t>(g.nx) * g.ny * g.nz;
+  s.cnt[0] = static_cast<uint64_t>(g.nx + 1) * g.ny * g.nz;
+  s.cnt[1] = static_cast<uint64_t>(g.nx) * (g.ny + 1) * g.nz;
+  s.cnt[2] = static_cast<uint64_t>(g.nx) * g.ny * (g.nz + 1);
+  s.off[0] = 0;
+  s.off[1] = s.cnt[0];
+  s.off[2] = s.cnt[0] + s.cnt[1];
+  s.nf = s.off[2] + s.cnt[2];
+  s.fluid.assign(s.n, 1);
+  std::vector<float> d(s.n, 0.0f);
+  if (drag != nullptr) std::copy(drag, drag + s.n, d.begin());
+  for (uint64_t site = 0; site < s.n; ++site) {
+    s.fluid[site] = mask[site] != 0 ? 0 : 1;
+    if (!std::isfinite(d[site]) || d[site] < 0.0f) {
+      throw Error(MSIM_INVALID_ARGUMENT, "resolved-flow drag values must be finite and non-negative");
+    }
+    if (s.fluid[site] == 0) d[site] = 0.0f;
+  }
+  s.active.assign(s.nf, 0);
+  s.exists.assign(s.nf, 0);
+  s.face_drag.assign(s.nf, 0.0f);
+  s.diag.assign(s.nf, 0.0f);
+  s.force.assign(s.nf, 0.0f);
+  for (uint64_t index = 0; index < s.nf; ++index) {
+    int c;
+    uint32_t co[3];
+    face_coords_host(s, index, &c, co);
+    int64_t lower = -1, upper = -1;
+    if (co[c] > 0) {
+      uint32_t t[3] = {co[0], co[1], co[2]};
+      --t[c];
+      lower = static_cast<int64_t>(site_host(s, t[0], t[1], t[2]));
+    }
+    if (co[c] < s.dims[c]) upper = static_cast<int64_t>(site_host(s, co[0], co[1], co[2]));
+    const bool lower_fluid = lower >= 0 && s.fluid[lower] != 0;
+    const bool upper_fluid = upper >= 0 && s.fluid[upper] != 0;
+    s.exists[index] = lower_fluid || upper_fluid ? 1 : 0;
+    const bool exterior = lower < 0 || upper < 0;
+    const bool open = c == static_cast<int>(axis) && (co[axis] == 0 || co[axis] == s.dims[axis]);
+    const bool act = exterior ? open && (lower_fluid || upper_fluid) : lower_fluid && upper_fluid;
+    s.active[index] = act ? 1 : 0;
+    if (!act) continue;
+    float sum = 0.0f, count = 0.0f;
+    if (lower_fluid) {
+      sum += d[lower];
+      count += 1.0f;
+    }
+    if (upper_fluid) {
+      sum += d[upper];
+      count += 1.0f;
+    }
+    s.face_drag[index] = sum / count;
+    s.diag[index] = s.face_drag[index];
+    for (int a = 0; a < 3; ++a) {
+      if (s.dims[a] > 1) s.diag[index] += 2.0f / (s.h[a] * s.h[a]);
+    }
+  }
+  const float force = 1.0f / ((static_cast<float>(s.dims[axis]) + 1.0f) * s.h[axis]);
+  bool open_inlet = false;
+  for (uint64_t index = s.off[axis]; index < s.off[axis] + s.cnt[axis]; ++index) {
+    if (s.active[index] == 0) continue;
+    int c;
+    uint32_t co[3];
+    face_coords_host(s, index, &c, co);
+    s.force[index] = force;
+    open_inlet = open_inlet || co[axis] == 0;
+  }
+  if (!open_inlet) {
+    throw Error(MSIM_INVALID_ARGUMENT, "the resolved-flow inlet boundary is entirely blocked");
+  }
+  // pressure_diagonal (958-978).
+  s.pd.assign(s.n, 0.0f);
+  for (uint64_t site = 0; site < s.n; ++site) {
+    if (s.fluid[site] == 0) continue;
+    const uint32_t co[3] = {static_cast<uint32_t>(site / (static_cast<uint64_t>(s.dims[1]) * s.dims[2])),
+                            static_cast<uint32_t>((site / s.dims[2]) % s.dims[1]),
+                            static_cast<uint32_t>(site % s.dims[2])};
+    for (int a = 0; a < 3; ++a) {
+      uint32_t up[3] = {co[0], co[1], co[2]};
+      ++up[a];
+      const uint64_t faces[2] = {face_host(s, a, co[0], co[1], co[2]),
+                                 face_host(s, a, up[0], up[1], up[2])};
+      for (uint64_t face : faces) {
+        if (s.active[face] != 0 && s.diag[face] > 0.0f) {
+          s.pd[site] += 1.0f / (s.diag[face] * s.h[a] * s.h[a]);
+        }
+      }
+    }
+  }
+  return s;
+}
+
+// minimum_gap_voxels of a single-axis system (998-1041).
+uint32_t min_gap(const ResSystem& s) {
+  uint32_t shortest = 0;
+  for (int axis = 0; axis < 3; ++axis) {
+    if (axis == static_cast<int>(s.axis) || s.dims[axis] <= 1) continue;
+    const int fa = (axis + 1) % 3, sa = (axis + 2) % 3;
+    for (uint32_t first = 0; first < s.dims[fa]; ++first) {
+      for (uint32_t second = 0; second < s.dims[sa]; ++second) {
+        uint32_t co[3] = {0, 0, 0};
+        co[fa] = first;
+        co[sa] = second;
+        const auto open_face = [&](uint32_t along) {
+          uint32_t f[3] = {co[0], co[1], co[2]};
+          f[axis] = along;
+          return s.active[face_host(s, axis, f[0], f[1], f[2])] != 0;
+        };
+        uint32_t run = 0;
+        bool through = false;
+        const auto close = [&] {
+          if (run != 0 && !through) shortest = shortest == 0 ? run : std::min(shortest, run);
+          run = 0;
+          through = false;
+        };
+        for (uint32_t along = 0; along < s.dims[axis]; ++along) {
+          co[axis] = along;
+          if (s.fluid[site_host(s, co[0], co[1], co[2])] != 0) {
+            through = through || (along == 0 && open_face(0));
+            ++run;
+          } else {
+            close();
+          }
+        }
+        through = through || (run != 0 && open_face(s.dims[axis]));
+        close();
+      }
+    }
+  }
+  return shortest;
+}
+
+// Device state of one solve.
+struct ResDevice {
+  ResView V{};
+  uint64_t m = 0;
+  DeviceBuffer<uint8_t> active, exists, fluid;
+  DeviceBuffer<float> face_drag, diag, pd;
+  DeviceBuffer<double> rhs, x, r, w, basis, directions, cg_r, cg_z, cg_p, cg_q, partials, slots;
+  DeviceBuffer<ResState> state;
+};
+
+template <typename T>
+void upload(Context& ctx, DeviceBuffer<T>& buf, const std::vector<T>& host, const char* what) {
+  buf.reserve(host.size());
+  check(hipMemcpyAsync(buf.data(), host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice,
+                       ctx.stream),
+        what);
+}
+
+// dot(a, b) of `count` entries into the device scalar `slot`.
+void device_dot(Context& ctx, ResDevice& D, const double* a, const double* b, uint64_t count,
+                double* slot) {
+  const uint32_t blocks = chunks_of(count);
+  k_res_dot<<<blocks, kResBlock, 0, ctx.stream>>>(a, b, count, D.partials.data());
+  k_res_project_sum<<<1, kResBlock, 0, ctx.stream>>>(D.partials.data(), blocks, slot);
+}
+
+double read_scalar(Context& ctx, const double* slot) {
+  double v = 0.0;
+  check(hipMemcpyAsync(&v, slot, sizeof(double), hipMemcpyDeviceToHost, ctx.stream), "read scalar");
+  check(hipStreamSynchronize(ctx.stream), "resolved flow");
+  return v;
+}
+
+// The momentum block by Jacobi-PCG from zero (conjugate_gradient,
+// cpu_flow.cpp:53-103) on rhs (FP64, rounded to float as the reference
+// rounds it): the solution into `out`. Adds the iterations to `inner`.
+void momentum_cg(Context& ctx, ResDevice& D, const double* rhs, double* out, float tolerance,
+                 uint32_t max_iterations, uint64_t& inner) {
+  const ResView& V = D.V;
+  const uint32_t blocks = chunks_of(V.nf);
+  double* parts = D.partials.data();
+  ResState* st = D.state.data();
+  k_cg_init<<<blocks, kResBlock, 0, ctx.stream>>>(V, rhs, out, D.cg_r.data(), D.cg_z.data(),
+                                                  D.cg_p.data(), parts);
+  k_cg_begin<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, st);
+  check(hipGetLastError(), "launch momentum CG");
+  constexpr uint32_t kBatch = 16;
+  uint32_t enqueued = 0;
+  ResState s{};
+  for (;;) {
+    const uint32_t batch = std::min(kBatch, max_iterations - enqueued);
+    for (uint32_t k = 0; k < batch; ++k) {
+      k_cg_apply<<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_p.data(), D.cg_q.data(), parts, st);
+      k_cg_alpha<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, st);
+      k_cg_update<<<blocks, kResBlock, 0, ctx.stream>>>(V, out, D.cg_r.data(), D.cg_z.data(),
+                                                        D.cg_p.data(), D.cg_q.data(), parts, st);
+      k_cg_beta<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, tolerance, st);
+      k_cg_direction<<<grid_of(V.nf), kResBlock, 0, ctx.stream>>>(V.nf, D.cg_p.data(),
+                                                                  D.cg_z.data(), st);
+    }
+    check(hipGetLastError(), "launch momentum CG");
+    enqueued += batch;
+    check(hipMemcpyAsync(&s, st, sizeof(ResState), hipMemcpyDeviceToHost, ctx.stream),
+          "read CG state");
+    check(hipStreamSynchronize(ctx.stream), "momentum CG");
+    if (s.done == kCgDone || enqueued >= max_iterations) break;
+  }
+  inner += s.iterations;
+  const std::string label = "resolved-flow momentum conjugate gradient ";
+  switch (s.failure) {
+    case MSIM_CG_CURVATURE:
+      throw Error(MSIM_CG_CURVATURE, label + "encountered non-positive curvature");
+    case MSIM_CG_STEP: throw Error(MSIM_CG_STEP, label + "produced a non-finite step");
+    case MSIM_CG_RESIDUAL: throw Error(MSIM_CG_RESIDUAL, label + "produced a non-finite residual");
+    case MSIM_CG_BREAKDOWN:
+      throw Error(MSIM_CG_BREAKDOWN, label + "encountered a preconditioner breakdown");
+    case MSIM_CG_DIRECTION:
+      throw Error(MSIM_CG_DIRECTION, label + "produced a non-finite direction");
+    default: break;
+  }
+  if (s.done != kCgDone) {
+    throw Error(MSIM_NOT_CONVERGED,
+                label + "did not converge: relative " + std::to_string(s.relative));
+  }
+}
+
+void apply_system(Context& ctx, ResDevice& D, const double* in, double* out) {
+  k_res_apply<<<grid_of(D.m), kResBlock, 0, ctx.stream>>>(D.V, in, out);
+  check(hipGetLastError(), "launch k_res_apply");
+}
+
+void precondition(Context& ctx, ResDevice& D, const double* in, double* out, float tolerance,
+                  uint32_t max_iterations, uint64_t& inner) {
+  momentum_cg(ctx, D, in, out, tolerance, max_iterations, inner);
+  k_res_pressure_precondition<<<grid_of(D.V.n), kResBlock, 0, ctx.stream>>>(D.V, in, out);
+  check(hipGetLastError(), "launch k_res_pressure_precondition");
+}
+
+}  // namespace
+
+void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
+                   const float* drag, const msim_resolved_params& params, float* faces_out,
+                   msim_resolved_report& report) {
+  const ResSystem S = build_system(g, mask, drag, params.axis);
+  ResDevice D;
+  D.m = S.nf + S.n;
+  ResView& V = D.V;
+  for (int a = 0; a < 3; ++a) {
+    V.dims[a] = S.dims[a];
+    V.off[a] = S.off[a];
+    V.h[a] = static_cast<double>(S.h[a]);
+    V.inv[a] = 1.0 / (static_cast<double>(S.h[a]) * S.h[a]);
+  }
+  V.nf = S.nf;
+  V.n = S.n;
+  V.cs = 1.0 / std::min({S.h[0], S.h[1], S.h[2]});
+  upload(ctx, D.active, S.active, "upload resolved system");
+  upload(ctx, D.exists, S.exists, "upload resolved system");
+  upload(ctx, D.fluid, S.fluid, "upload resolved system");
+  upload(ctx, D.face_drag, S.face_drag, "upload resolved system");
+  upload(ctx, D.diag, S.diag, "upload resolved system");
+  upload(ctx, D.pd, S.pd, "upload resolved system");
+  V.active = D.active.data();
+  V.exists = D.exists.data();
+  V.fluid = D.fluid.data();
+  V.face_drag = D.face_drag.data();
+  V.diag = D.diag.data();
+  V.pd = D.pd.data();
+  const uint64_t m = D.m;
+  std::vector<double> rhs_host(m, 0.0);
+  for (uint64_t i = 0; i < S.nf; ++i) rhs_host[i] = static_cast<double>(S.force[i]);
+  upload(ctx, D.rhs, rhs_host, "upload resolved forcing");
+  for (auto* b : {&D.x, &D.r, &D.w}) b->reserve(m);
+  D.basis.reserve(static_cast<size_t>(kRestart + 1) * m);
+  D.directions.reserve(static_cast<size_t>(kRestart) * m);
+  for (auto* b : {&D.cg_r, &D.cg_z, &D.cg_p, &D.cg_q}) b->reserve(S.nf);
+  D.partials.reserve(2 * static_cast<size_t>(chunks_of(m)));
+  // slots: [0] the last norm, [1 + 2 * i + pass] the Arnoldi projections.
+  D.slots.reserve(2 + 2 * kRestart + 2);
+  D.state.reserve(1);
+  check(hipMemsetAsync(D.x.data(), 0, m * sizeof(double), ctx.stream), "clear solution");
+  check(hipEventRecord(ctx.start, ctx.stream), "event");
+
+  const double tolerance = static_cast<double>(params.relative_tolerance);
+  const uint32_t max_outer = params.max_outer_iterations;
+  uint64_t inner = 0;
+  double* slot = D.slots.data();
+  // flexible_gmres (flexible_gmres.hpp:33-108).
+  device_dot(ctx, D, D.rhs.data(), D.rhs.data(), m, slot);
+  const double rhs_norm = std::sqrt(read_scalar(ctx, slot));
+  if (!std::isfinite(rhs_norm)) throw Error(MSIM_RUNTIME_ERROR, "FGMRES non-finite right-hand side");
+  uint32_t iterations = 0;
+  double relative = 0.0;
+  if (rhs_norm != 0.0) {
+    for (;;) {
+      apply_system(ctx, D, D.x.data(), D.w.data());
+      k_res_residual<<<grid_of(m), kResBlock, 0, ctx.stream>>>(m, D.r.data(), D.rhs.data(),
+                                                               D.w.data());
+      device_dot(ctx, D, D.r.data(), D.r.data(), m, slot);
+      const double beta = std::sqrt(read_scalar(ctx, slot));
+      if (!std::isfinite(beta)) throw Error(MSIM_RUNTIME_ERROR, "FGMRES non-finite true residual");
+      if (beta <= tolerance * rhs_norm) {
+        relative = beta / rhs_norm;
+        break;
+      }
+      if (iterations >= max_outer) {
+        throw Error(MSIM_NOT_CONVERGED, "resolved-flow FGMRES did not converge: relative residual " +
+                                            std::to_string(beta / rhs_norm));
+      }
+      const uint32_t count = std::min(kRestart, max_outer - iterations);
+      std::vector<std::vector<double>> h(count + 1, std::vector<double>(count, 0.0));
+      std::vector<double> cosine(count), sine(count), gv(count + 1, 0.0);
+      gv[0] = beta;
+      double* basis = D.basis.data();
+      double* directions = D.directions.data();
+      k_res_scaled<<<grid_of(m), kResBlock, 0, ctx.stream>>>(m, basis, 1 / beta, D.r.data());
+      uint32_t used = 0;
+      for (uint32_t j = 0; j < count; ++j) {
+        double* direction = directions + static_cast<size_t>(j) * m;
+        const double* vj = basis + static_cast<size_t>(j) * m;
+        precondition(ctx, D, vj, direction, params.inner_relative_tolerance,
+                     params.max_inner_iterations, inner);
+        apply_system(ctx, D, direction, D.w.data());
+        // Twice-modified Gram-Schmidt, every scalar on the device.
+        for (uint32_t pass = 0; pass < 2; ++pass) {
+          for (uint32_t i = 0; i <= j; ++i) {
+            const double* vi = basis + static_cast<size_t>(i) * m;
+            double* s = slot + 1 + 2 * i + pass;
+            device_dot(ctx, D, vi, D.w.data(), m, s);
+            k_res_project<<<grid_of(m), kResBlock, 0, ctx.stream>>>(m, D.w.data(), vi, s);
+          }
+        }
+        device_dot(ctx, D, D.w.data(), D.w.data(), m, slot);
+        std::vector<double> col(2 + 2 * (j + 1));
+        check(hipMemcpyAsync(col.data(), slot, col.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx.stream),
+              "read Arnoldi column");
+        check(hipStreamSynchronize(ctx.stream), "Arnoldi step");
+        for (uint32_t i = 0; i <= j; ++i) {
+          h[i][j] += col[1 + 2 * i];
+          h[i][j] += col[1 + 2 * i + 1];
+        }
+        h[j + 1][j] = std::sqrt(col[0]);
+        if (!std::isfinite(h[j + 1][j])) {
+          throw Error(MSIM_RUNTIME_ERROR, "FGMRES non-finite Arnoldi vector");
+        }
+        const bool happy = h[j + 1][j] <= 1e-14;
+        double* next = basis + static_cast<size_t>(j + 1) * m;
+        k_res_scaled<<<grid_of(m), kResBlock, 0, ctx.stream>>>(
+            m, next, happy ? 0.0 : 1 / h[j + 1][j], D.w.data());
+        for (uint32_t i = 0; i < j; ++i) {
+          const double upper = cosine[i] * h[i][j] + sine[i] * h[i + 1][j];
+          h[i + 1][j] = -sine[i] * h[i][j] + cosine[i] * h[i + 1][j];
+          h[i][j] = upper;
+        }
+        const double diagonal = std::hypot(h[j][j], h[j + 1][j]);
+        if (diagonal == 0) throw Error(MSIM_RUNTIME_ERROR, "FGMRES Arnoldi breakdown");
+        cosine[j] = h[j][j] / diagonal;
+        sine[j] = h[j + 1][j] / diagonal;
+        h[j][j] = diagonal;
+        h[j + 1][j] = 0;
+        gv[j + 1] = -sine[j] * gv[j];
+        gv[j] *= cosine[j];
+        ++iterations;
+        used = j + 1;
+        if (happy || std::abs(gv[j + 1]) <= tolerance * rhs_norm) break;
+      }
+      std::vector<double> weights(used);
+      for (uint32_t i = used; i-- > 0;) {
+        double value = gv[i];
+        for (uint32_t jj = i + 1; jj < used; ++jj) value -= h[i][jj] * weights[jj];
+        weights[i] = value / h[i][i];
+      }
+      for (uint32_t i = 0; i < used; ++i) {
+        k_res_axpy<<<grid_of(m), kResBlock, 0, ctx.stream>>>(
+            m, D.x.data(), weights[i], directions + static_cast<size_t>(i) * m);
+      }
+      check(hipGetLastError(), "launch FGMRES update");
+    }
+  }
+  // The report (cpu_flow.cpp:440-467): momentum residual, divergence.
+  apply_system(ctx, D, D.x.data(), D.w.data());
+  check(hipEventRecord(ctx.stop, ctx.stream), "event");
+  std::vector<double> x(m), w(m);
+  check(hipMemcpyAsync(x.data(), D.x.data(), m * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx.stream),
+        "download resolved solution");
+  check(hipMemcpyAsync(w.data(), D.w.data(), m * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx.stream),
+        "download resolved residual");
+  check(hipStreamSynchronize(ctx.stream), "resolved flow");
+  float ms = 0.0f;
+  check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+  double momentum_square = 0.0, force_square = 0.0;
+  for (uint64_t i = 0; i < S.nf; ++i) {
+    momentum_square += (w[i] - rhs_host[i]) * (w[i] - rhs_host[i]);
+    force_square += rhs_host[i] * rhs_host[i];
+  }
+  // The divergence of the velocity over the fluid sites: the system's last
+  // n rows divided by the continuity scale.
+  double divergence_square = 0.0;
+  uint64_t fluid_count = 0;
+  for (uint64_t site = 0; site < S.n; ++site) {
+    if (S.fluid[site] == 0) continue;
+    double d = 0.0;
+    const uint32_t co[3] = {static_cast<uint32_t>(site / (static_cast<uint64_t>(S.dims[1]) * S.dims[2])),
+                            static_cast<uint32_t>((site / S.dims[2]) % S.dims[1]),
+                            static_cast<uint32_t>(site % S.dims[2])};
+    for (int c = 0; c < 3; ++c) {
+      uint32_t up[3] = {co[0], co[1], co[2]};
+      ++up[c];
+      d += (x[face_host(S, c, up[0], up[1], up[2])] - x[face_host(S, c, co[0], co[1], co[2])]) /
+           static_cast<double>(S.h[c]);
+    }
+    divergence_square += d * d;
+    ++fluid_count;
+  }
+  const double divergence_rms =
+      fluid_count == 0 ? 0.0 : std::sqrt(divergence_square / static_cast<double>(fluid_count));
+  // scale_velocity (flow_system.hpp:1368-1415) over the open inlet faces.
+  std::vector<float> unscaled(S.nf);
+  for (uint64_t i = 0; i < S.nf; ++i) unscaled[i] = static_cast<float>(x[i]);
+  double inlet_sum = 0.0;
+  uint64_t inlet_count = 0;
+  float peak = 0.0f;
+  for (uint64_t i = 0; i < S.nf; ++i) {
+    if (!std::isfinite(unscaled[i])) {
+      throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+    }
+    peak = std::max(peak, std::abs(unscaled[i]));
+  }
+  for (uint64_t i = S.off[S.axis]; i < S.off[S.axis] + S.cnt[S.axis]; ++i) {
+    int c;
+    uint32_t co[3];
+    face_coords_host(S, i, &c, co);
+    if (co[S.axis] == 0 && S.active[i] != 0) {
+      inlet_sum += unscaled[i];
+      ++inlet_count;
+    }
+  }
+  if (inlet_count == 0) throw Error(MSIM_RUNTIME_ERROR, "flow solve has no open inlet faces");
+  const float solved_mean = static_cast<float>(inlet_sum / static_cast<double>(inlet_count));
+  if (peak == 0.0f || solved_mean <= 1.0e-9f * peak) {
+    throw Error(MSIM_RUNTIME_ERROR, "the device carries no through-flow: the outlet is unreachable");
+  }
+  const float factor = (params.mean_inlet_speed - 0.0f) / solved_mean;
+  float scaled_peak = 0.0f;
+  for (uint64_t i = 0; i < S.nf; ++i) {
+    const float v = unscaled[i] * factor + 0.0f;
+    if (!std::isfinite(v)) throw Error(MSIM_RUNTIME_ERROR, "non-finite combined flow velocity");
+    faces_out[i] = v;
+    scaled_peak = std::max(scaled_peak, std::abs(v));
+  }
+  report.outer_iterations = iterations;
+  report.inner_iterations = inner;
+  report.relative_residual = static_cast<float>(relative);
+  report.momentum_relative_residual = static_cast<float>(std::sqrt(momentum_square / force_square));
+  report.divergence_rms = static_cast<float>(divergence_rms * std::abs(factor));
+  report.mean_inlet_speed = params.mean_inlet_speed;
+  report.max_speed = scaled_peak;
+  report.min_gap_voxels = min_gap(S);
+  report.seconds = ms * 1.0e-3;
+}
+
+}  // namespace msim

@@ -49,14 +49,15 @@ def test_unsupported_features_are_not_advertised():
 
 
 def test_depth_averaged_entry_points_are_implemented():
-    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49) and the
-    coupled step (31-34) run the C ABI; only the resolved and cell-mechanics
-    calls are refused."""
+    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49), the
+    resolved axis flow (50-52) and the coupled step (31-34) run the C ABI;
+    the resolved port flow and the cell-mechanics calls are refused."""
     stub = msa.HipBackend._unsupported
     assert msa.HipBackend.solve_depth_averaged_flow is not stub
     assert msa.HipBackend.solve_depth_averaged_port_flow is not stub
     assert msa.HipBackend.advance_coupled is not stub
-    assert msa.HipBackend.solve_resolved_flow is stub
+    assert msa.HipBackend.solve_resolved_flow is not stub
+    assert msa.HipBackend.solve_resolved_port_flow is stub
     assert msa.HipBackend.advance_species is stub
 
 

@@ -50,6 +50,24 @@ def test_depth_averaged_flow_through_the_interface(binding):
     assert close(hip, cpu, 8e-4)
 
 
+def test_resolved_flow_through_the_interface(binding):
+    """run_resolved_case (flow_conformance_test.cpp:142-165) through the
+    backend interface beside solve_resolved_flow_cpu: the suite's
+    compare_fields tolerance, divergence below 5e-5, the same gap."""
+    from tests.test_resolved_oracle import conformance_case, random_case
+    grid, drag = conformance_case()
+    hip, cpu, (rh, rc) = binding.resolved_flow(grid, drag, 1, 2.0, 1e-6)
+    print("outer", rh["outer_iterations"], rc["outer_iterations"], flush=True)
+    assert close(hip, cpu, 8e-4)
+    assert rh["divergence_rms"] < 5e-5
+    assert rh["min_gap_voxels"] == rc["min_gap_voxels"]
+    grid, drag, axis = random_case(5)
+    hip, cpu, (rh, rc) = binding.resolved_flow(grid, drag, axis, 1.5, 1e-6)
+    print("outer", rh["outer_iterations"], rc["outer_iterations"], flush=True)
+    assert close(hip, cpu, 8e-4)
+    assert abs(rh["outer_iterations"] - rc["outer_iterations"]) <= 1
+
+
 @pytest.mark.parametrize("kind", CONFORMANCE_KINDS)
 @pytest.mark.parametrize("integration,dt", CONFORMANCE_STEPS)
 @pytest.mark.parametrize("scheme", [UPWIND, VAN_LEER])
