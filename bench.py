@@ -463,6 +463,74 @@ def port_flow_bench(msa, w, hip, cpu_seconds_per_iteration):
     return out
 
 
+def resolved_channel(msa, nx, ny, nz, seed=17):
+    """A channel along x with 8 % seeded solid voxels (none on the inlet and
+    outlet planes) and seeded Brinkman drag in [0, 2)."""
+    rng = np.random.default_rng(seed)
+    solid = (rng.random((nx, ny, nz)) < 0.08).astype(np.uint8)
+    solid[0] = solid[-1] = 0
+    kinds = (msa.FIXED, msa.FIXED, msa.NO_FLUX, msa.NO_FLUX, msa.NO_FLUX, msa.NO_FLUX)
+    grid = msa.Grid(nx, ny, nz, 1.0, 1.0, 1.0, solid.reshape(-1), kinds)
+    drag = (rng.random(nx * ny * nz) * 2.0).astype(np.float32)
+    return grid, drag
+
+
+def resolved_flow_bench(msa, hip, threads: int, with_cpu: bool):
+    """The resolved Stokes-Brinkman solve (FP64 flexible GMRES over Jacobi-PCG
+    momentum solves, the reference's algorithm) on a 64x32x16 channel, timed
+    on the device and, on the same system, the reference's own CPU solver
+    (oracle/_ref) over a whole solve; and on a 128x64x32 channel on the
+    device only."""
+    out = {}
+    for name, dims, cpu in (("channel_64x32x16", (64, 32, 16), with_cpu),
+                            ("channel_128x64x32", (128, 64, 32), False)):
+        grid, drag = resolved_channel(msa, *dims)
+        # The outer iterations grow with the grid: 404 at 64x32x16.
+        params = msa.ResolvedFlowParameters(mean_inlet_speed=1.0, axis=0,
+                                            max_outer_iterations=4000)
+        field, rep = hip.solve_resolved_flow(grid, drag, params)  # buffers and code warm
+        count = 3 if grid.nx <= 64 else 1
+        t0 = time.perf_counter()
+        for _ in range(count):
+            field, rep = hip.solve_resolved_flow(grid, drag, params)
+        rate = count / (time.perf_counter() - t0)
+        entry = {"workload": f"{dims[0]}x{dims[1]}x{dims[2]} channel along x, 8% solid voxels, "
+                             f"drag in [0, 2), mean inlet speed 1, tolerances 1e-6, "
+                             f"at most 4000 outer iterations",
+                 "faces": int(field.concatenated().size), "solves_per_s": round(rate, 4),
+                 "outer_iterations": rep.outer_iterations,
+                 "inner_iterations": rep.inner_iterations,
+                 "device_seconds": round(rep.seconds, 4),
+                 "relative_residual": rep.relative_residual,
+                 "divergence_rms": rep.divergence_rms}
+        log(f"resolved flow {name}: {rate:.3f} solves/s, {rep.outer_iterations} outer / "
+            f"{rep.inner_iterations} inner iterations, device {rep.seconds:.3f} s")
+        if cpu:
+            from oracle import Grid as OGrid
+            from oracle import Reference
+            if Reference.available():
+                ref = Reference()
+                ref.set_threads(threads)
+                og = OGrid(grid.nx, grid.ny, grid.nz, grid.dx, grid.dy, grid.dz, grid.obstacles,
+                           grid.boundary)
+                t0 = time.perf_counter()
+                faces, rrep = ref.resolved_flow(og, drag, 0, 1.0, 1e-6, 4000, 1e-6, 50000)
+                secs = time.perf_counter() - t0
+                diff = float(np.abs(field.concatenated() - faces).max() / rrep["max_speed"])
+                entry["cpu_reference"] = {
+                    "solves_per_s": round(1.0 / secs, 5), "cores": ref.threads(),
+                    "kind": "reference", "outer_iterations": rrep["outer_iterations"],
+                    "inner_iterations": rrep["inner_iterations"],
+                    "sample": "one whole solve of the same system by the reference's "
+                              "solve_resolved_flow_cpu"}
+                entry["max_field_difference_over_max_speed"] = diff
+                log(f"resolved flow reference CPU: {secs:.2f} s on {ref.threads()} threads, "
+                    f"{rrep['outer_iterations']} outer / {rrep['inner_iterations']} inner, "
+                    f"field difference {diff:.2e}")
+        out[name] = entry
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +547,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="profiling calibration: copy this many MiB device to device, 3 times")
     ap.add_argument("--no-ports", action="store_true", help="skip the port-driven flow")
+    ap.add_argument("--no-resolved", action="store_true",
+                    help="skip the resolved Stokes-Brinkman flow")
     ap.add_argument("--implicit-steps", type=int, default=5,
                     help="Crank-Nicolson transport steps to time (0: skip)")
     ap.add_argument("--coupled-steps", type=int, default=20,
@@ -797,6 +867,9 @@ def main():
     if not args.no_ports:
         per_iteration = line.get("cpu_baseline", {}).get("seconds_per_iteration")
         line["port_flow"] = port_flow_bench(msa, w, hip, per_iteration)
+    if not args.no_resolved:
+        line["resolved_flow"] = resolved_flow_bench(msa, hip, min(16, os.cpu_count() or 1),
+                                                    not args.no_cpu)
     hip.close()
     print(json.dumps(line), flush=True)
 

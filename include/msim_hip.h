@@ -1,7 +1,8 @@
 /*
  * msim_hip.h -- C ABI of the MI355X (gfx950) backend for MicroSimulator's
  * grid flow-and-transport path: the depth-averaged (Hele-Shaw/Darcy) pressure
- * solve (axis- or port-driven), face-velocity reconstruction, the conservative
+ * solve (axis- or port-driven), face-velocity reconstruction, the resolved
+ * Stokes-Brinkman flow (axis- or port-driven), the conservative
  * advection-diffusion-reaction step (explicit or implicit), and the coupled
  * cell/grid exchange around it.
  *
@@ -293,6 +294,50 @@ typedef struct {
 int msim_hip_solve_resolved_flow(msim_ctx* ctx, const msim_grid* grid, const float* drag,
                                  const msim_resolved_params* params, float* x_faces,
                                  float* y_faces, float* z_faces, msim_resolved_report* report);
+
+/* ResolvedPortFlowParameters (cpp/include/microsimulator/flow.hpp:125-132). */
+typedef struct {
+  float relative_tolerance;        /* FGMRES, >= float epsilon */
+  uint32_t max_outer_iterations;   /* per block solve (the reference's default 5000) */
+  float inner_relative_tolerance;  /* the momentum block's CG, >= float epsilon */
+  uint32_t max_inner_iterations;
+} msim_resolved_port_params;
+
+/* ResolvedPortFlowReport (cpp/include/microsimulator/flow.hpp:134-147):
+ * iteration counts summed and residuals the largest over the block solves;
+ * the divergence and gap of the combined field; and the device time of the
+ * block solves. */
+typedef struct {
+  uint32_t solves;
+  uint32_t outer_iterations;
+  uint64_t inner_iterations;
+  float relative_residual;
+  float momentum_relative_residual;
+  float divergence_rms;
+  float max_speed;
+  uint32_t min_gap_voxels;
+  double seconds;
+} msim_resolved_port_report;
+
+/* ComputeBackend::solve_resolved_port_flow (backend.hpp:53-55; the
+ * reference's solve_resolved_port_flow_cpu, cpp/cpu/cpu_flow.cpp:474-479,
+ * over detail::solve_resolved_port_flow, cpp/core/flow_system.hpp:1482-1548):
+ * the resolved Stokes-Brinkman flow driven through ports, the combination of
+ * one block solve on the device per speed port held alone at pressure one
+ * (plus one for the pressure ports' own pressures when any is nonzero) that
+ * carries each speed port's stated mean speed (superpose_port_flows,
+ * 1146-1252). Where shallow flow describes the geometry (shallow_port_pressure,
+ * 1318-1338), each block solve is for the correction to the depth-averaged
+ * pressure of the same ports, itself an FP64 Jacobi-PCG solve on the device.
+ * drag: one float per site or NULL. x/y/z_faces: the combined field (any may
+ * be NULL). states: port_count entries (may be NULL). The context's prepared
+ * depth-averaged flow system is replaced. Errors as the reference's. */
+int msim_hip_solve_resolved_port_flow(msim_ctx* ctx, const msim_grid* grid, const float* drag,
+                                      const msim_flow_port* ports, uint32_t port_count,
+                                      const msim_resolved_port_params* params, float* x_faces,
+                                      float* y_faces, float* z_faces,
+                                      msim_resolved_port_report* report,
+                                      msim_port_state* states);
 
 /* A signal grid for the explicit transport step: SignalGridSpec's transport
  * fields (signals.hpp:122-156) flattened. boundary_values: 6*S reservoir
@@ -592,6 +637,22 @@ int msim_hip_colony_mobility(msim_ctx* ctx, const msim_grid* grid, const float* 
                              const msim_cells* cells, const double* base, uint32_t base_count,
                              double drag_coefficient, double max_volume_fraction,
                              double averaging_radius, double* mobility);
+
+/* colony_flow_coupling (python/src/microsimulator/flow.py:384-424): the
+ * colony's mobility as msim_hip_colony_mobility computes it, and the in-plane
+ * body force per column for drag proportional to the fluid/biomass relative
+ * velocity: each cell's deposited biomass per column times its in-plane
+ * velocity, summed into the column's momentum, times drag / mass (mass the
+ * column's deposited volume; zero force where it has none). velocities: 2
+ * floats per cell (x, y). force: 2 doubles per column (x-major, y-minor), the
+ * pairs DepthAveragedFlowParameters::body_force takes. Checks in the
+ * reference's order: the radius, then per cell a finite velocity before the
+ * deposit's own checks, then the mobility's. */
+int msim_hip_colony_flow_coupling(msim_ctx* ctx, const msim_grid* grid, const float* origin,
+                                  const msim_cells* cells, const float* velocities,
+                                  const double* base, uint32_t base_count,
+                                  double drag_coefficient, double max_volume_fraction,
+                                  double averaging_radius, double* mobility, double* force);
 
 /* Device time (seconds) of the last deposit's kernels. */
 double msim_hip_colony_seconds(msim_ctx* ctx);

@@ -230,10 +230,10 @@ EXPORTS = (
     "msim_hip_batch_autotune", "msim_hip_batch_shapes", "msim_hip_batch_shape",
     "msim_hip_batch_run", "msim_hip_batch_download", "msim_hip_batch_flags",
     "msim_hip_solve_depth_averaged_port_flow", "msim_hip_solve_resolved_flow",
-    "msim_hip_adr_report",
+    "msim_hip_solve_resolved_port_flow", "msim_hip_adr_report",
     "msim_hip_advance_coupled", "msim_hip_coupled_upload", "msim_hip_coupled_run",
     "msim_hip_coupled_download", "msim_hip_coupled_seconds", "msim_hip_colony_volume_fraction",
-    "msim_hip_colony_mobility", "msim_hip_colony_seconds",
+    "msim_hip_colony_mobility", "msim_hip_colony_flow_coupling", "msim_hip_colony_seconds",
     "msim_hip_advance_signal_grid_static", "msim_hip_static_scratch_bytes",
     "msim_hip_nutrient_run", "msim_hip_adr_occupancy", "msim_hip_copy_bandwidth",
 )
@@ -290,6 +290,7 @@ def library(path: str = LIBRARY):
         "msim_hip_batch_download": (I, [P, U, P, P, P, P]),
         "msim_hip_solve_depth_averaged_port_flow": (I, [P, P, P, P, U, P, P, P, P, P, P]),
         "msim_hip_solve_resolved_flow": (I, [P, P, P, P, P, P, P, P]),
+        "msim_hip_solve_resolved_port_flow": (I, [P, P, P, P, U, P, P, P, P, P, P]),
         "msim_hip_adr_report": (I, [P, P]),
         "msim_hip_advance_coupled": (I, [P, P, P, P, P, P, P, P, C.c_float, P]),
         "msim_hip_coupled_upload": (I, [P, P, P, P, P, P, P]),
@@ -298,6 +299,7 @@ def library(path: str = LIBRARY):
         "msim_hip_coupled_seconds": (D, [P]),
         "msim_hip_colony_volume_fraction": (I, [P, P, P, P, D, P]),
         "msim_hip_colony_mobility": (I, [P, P, P, P, P, U, D, D, D, P]),
+        "msim_hip_colony_flow_coupling": (I, [P, P, P, P, P, P, U, D, D, D, P, P]),
         "msim_hip_colony_seconds": (D, [P]),
         "msim_hip_advance_signal_grid_static": (I, [P, P, C.c_float, P, P]),
         "msim_hip_static_scratch_bytes": (C.c_size_t, []),
@@ -490,6 +492,47 @@ class ResolvedFlowReport:
     seconds: float
 
 
+class _ResolvedPortParams(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_float), ("max_outer_iterations", C.c_uint32),
+                ("inner_relative_tolerance", C.c_float), ("max_inner_iterations", C.c_uint32)]
+
+
+class _ResolvedPortReport(C.Structure):
+    _fields_ = [("solves", C.c_uint32), ("outer_iterations", C.c_uint32),
+                ("inner_iterations", C.c_uint64), ("relative_residual", C.c_float),
+                ("momentum_relative_residual", C.c_float), ("divergence_rms", C.c_float),
+                ("max_speed", C.c_float), ("min_gap_voxels", C.c_uint32),
+                ("seconds", C.c_double)]
+
+
+@dataclass
+class ResolvedPortFlowParameters:
+    """ResolvedPortFlowParameters (flow.hpp:125-132)."""
+    relative_tolerance: float = 1.0e-6
+    max_outer_iterations: int = 5000
+    inner_relative_tolerance: float = 1.0e-6
+    max_inner_iterations: int = 50_000
+
+    def _c(self) -> _ResolvedPortParams:
+        return _ResolvedPortParams(self.relative_tolerance, self.max_outer_iterations,
+                                   self.inner_relative_tolerance, self.max_inner_iterations)
+
+
+@dataclass
+class ResolvedPortFlowReport:
+    """ResolvedPortFlowReport (flow.hpp:134-147), and the device seconds of
+    the block solves."""
+    solves: int
+    outer_iterations: int
+    inner_iterations: int
+    relative_residual: float
+    momentum_relative_residual: float
+    divergence_rms: float
+    max_speed: float
+    min_gap_voxels: int
+    seconds: float
+
+
 @dataclass
 class VelocityField:
     """SignalGridVelocityField (signals.hpp:116-120)."""
@@ -565,6 +608,13 @@ class DepthAveragedPortFlowReport:
 class DepthAveragedPortFlowResult:
     field: VelocityField
     report: DepthAveragedPortFlowReport
+    ports: list
+
+
+@dataclass
+class ResolvedPortFlowResult:
+    field: VelocityField
+    report: ResolvedPortFlowReport
     ports: list
 
 
@@ -770,10 +820,7 @@ class BackendInfo:
 
 FEATURES = ("growth", "species", "cell_contacts", "external_constraints", "signals",
             "coupled_rates", "depth_averaged_flow", "resolved_flow", "contact_dynamics")
-# BackendFeature::resolved_flow also covers the resolved port flow, which this
-# backend does not implement, so it is not advertised; solve_resolved_flow
-# itself runs.
-SUPPORTED = frozenset({"signals", "coupled_rates", "depth_averaged_flow"})
+SUPPORTED = frozenset({"signals", "coupled_rates", "depth_averaged_flow", "resolved_flow"})
 
 
 class HipBackend:
@@ -915,6 +962,44 @@ class HipBackend:
             VelocityField(xf, yf, zf), report,
             [FlowPortState(st.pressure, st.flux, st.mean_speed) for st in states[:len(ports)]])
 
+    def solve_resolved_port_flow(self, grid: Grid, ports, drag=None,
+                                 parameters: ResolvedPortFlowParameters | None = None
+                                 ) -> ResolvedPortFlowResult:
+        """ComputeBackend::solve_resolved_port_flow (backend.hpp:53-55)."""
+        parameters = parameters or ResolvedPortFlowParameters()
+        keep: list = []
+        g = grid._c(keep)
+        d = None
+        if drag is not None:
+            d = np.ascontiguousarray(drag, dtype=np.float32).reshape(-1)
+            if d.size != grid.sites:
+                raise InvalidArgument("resolved-flow drag must hold one value per grid site")
+        arr = (_FlowPort * max(1, len(ports)))()
+        for i, port in enumerate(ports):
+            arr[i].face = port.face
+            arr[i].begin[0], arr[i].begin[1] = port.begin
+            arr[i].end[0], arr[i].end[1] = port.end
+            arr[i].kind = port.kind
+            arr[i].value = port.value
+        states = (_PortState * max(1, len(ports)))()
+        nfx, nfy, nfz = grid.face_counts()
+        xf = np.zeros(nfx, np.float32)
+        yf = np.zeros(nfy, np.float32)
+        zf = np.zeros(nfz, np.float32)
+        p = parameters._c()
+        rep = _ResolvedPortReport()
+        self._check(self.lib.msim_hip_solve_resolved_port_flow(
+            self.ctx, C.byref(g), None if d is None else d.ctypes.data,
+            C.cast(arr, C.c_void_p), len(ports), C.byref(p), xf.ctypes.data, yf.ctypes.data,
+            zf.ctypes.data, C.byref(rep), C.cast(states, C.c_void_p)))
+        report = ResolvedPortFlowReport(rep.solves, rep.outer_iterations, rep.inner_iterations,
+                                        rep.relative_residual, rep.momentum_relative_residual,
+                                        rep.divergence_rms, rep.max_speed, rep.min_gap_voxels,
+                                        rep.seconds)
+        return ResolvedPortFlowResult(
+            VelocityField(xf, yf, zf), report,
+            [FlowPortState(st.pressure, st.flux, st.mean_speed) for st in states[:len(ports)]])
+
     def advance_signal_grid(self, spec: SignalGridSpec, levels, dt: float):
         """ComputeBackend::advance_signal_grid (backend.hpp:30), forward Euler:
         returns (new levels, SignalSolveReport); `levels` is not modified."""
@@ -1006,6 +1091,34 @@ class HipBackend:
             float(drag_coefficient), float(max_volume_fraction), float(averaging_radius),
             out.ctypes.data))
         return out
+
+    def colony_flow_coupling(self, grid: Grid, cells, velocities, base=1.0,
+                             drag_coefficient: float = 100.0, max_volume_fraction: float = 0.9,
+                             averaging_radius: float = 4.0, origin=(0.0, 0.0, 0.0)):
+        """colony_flow_coupling (flow.py:384-424): (mobility, one float64 per
+        site; body force, (columns, 2) float64 in-plane pairs). velocities:
+        each cell's in-plane velocity, (cells, 2)."""
+        keep: list = []
+        g = grid._c(keep)
+        c = _cells_c(cells, keep)
+        v = np.ascontiguousarray(velocities, np.float32).reshape(-1)
+        if v.size != 2 * c.count:
+            raise InvalidArgument("cell velocities must hold one in-plane pair per cell")
+        org = np.ascontiguousarray(origin, np.float32).reshape(3)
+        scalar = np.ndim(base) == 0
+        b = np.ascontiguousarray(np.atleast_1d(base), np.float64).reshape(-1)
+        if b.size == 0:
+            b = np.zeros(1, np.float64)
+            count = 1 << 31
+        else:
+            count = 0 if scalar else b.size
+        mobility = np.zeros(grid.sites, np.float64)
+        force = np.zeros((grid.nx * grid.ny, 2), np.float64)
+        self._check(self.lib.msim_hip_colony_flow_coupling(
+            self.ctx, C.byref(g), org.ctypes.data, C.byref(c), v.ctypes.data, b.ctypes.data,
+            count, float(drag_coefficient), float(max_volume_fraction), float(averaging_radius),
+            mobility.ctypes.data, force.ctypes.data))
+        return mobility, force
 
     def colony_seconds(self) -> float:
         return float(self.lib.msim_hip_colony_seconds(self.ctx))
@@ -1336,9 +1449,8 @@ class HipBackend:
 
     # -- the rest of ComputeBackend: not on this backend ------------------------
     def _unsupported(self, *args, **kwargs):
-        raise Unsupported("the HIP backend implements depth-averaged flow, signal transport and "
-                          "the coupled cell/grid exchange only")
+        raise Unsupported("the HIP backend implements depth-averaged and resolved flow, signal "
+                          "transport and the coupled cell/grid exchange only")
 
     advance_growth = advance_species = _unsupported
     find_cell_contacts = find_external_contacts = solve_contact_forces = _unsupported
-    solve_resolved_port_flow = _unsupported

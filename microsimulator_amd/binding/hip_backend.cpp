@@ -144,10 +144,8 @@ class HipBackend final : public ComputeBackend {
             .native = true};
   }
   [[nodiscard]] bool supports(BackendFeature f) const noexcept override {
-    // resolved_flow also covers solve_resolved_port_flow, which is not built
-    // here; solve_resolved_flow itself runs.
     return f == BackendFeature::signals || f == BackendFeature::coupled_rates ||
-           f == BackendFeature::depth_averaged_flow;
+           f == BackendFeature::depth_averaged_flow || f == BackendFeature::resolved_flow;
   }
 
   [[nodiscard]] DepthAveragedFlowResult solve_depth_averaged_flow(
@@ -341,11 +339,52 @@ class HipBackend final : public ComputeBackend {
                      .min_gap_voxels = report.min_gap_voxels};
     return result;
   }
-  ResolvedPortFlowResult solve_resolved_port_flow(const SignalGridSpec&,
-                                                  std::span<const FlowPort>,
-                                                  std::span<const float>,
-                                                  const ResolvedPortFlowParameters&) override {
-    unsupported();
+  ResolvedPortFlowResult solve_resolved_port_flow(const SignalGridSpec& spec,
+                                                  std::span<const FlowPort> ports,
+                                                  std::span<const float> drag,
+                                                  const ResolvedPortFlowParameters& p) override {
+    p.validate();
+    const msim_grid g = grid_of(spec);
+    if (!drag.empty() && drag.size() != spec.site_count()) {
+      throw std::invalid_argument("resolved-flow drag must hold one value per grid site");
+    }
+    std::vector<msim_flow_port> held;
+    for (const auto& port : ports) {
+      held.push_back({static_cast<std::uint32_t>(port.face),
+                      {port.patch.begin[0], port.patch.begin[1]},
+                      {port.patch.end[0], port.patch.end[1]},
+                      port.kind == FlowPortKind::speed ? static_cast<std::uint32_t>(MSIM_PORT_SPEED)
+                                                       : static_cast<std::uint32_t>(MSIM_PORT_PRESSURE),
+                      port.value});
+    }
+    const msim_resolved_port_params params{p.relative_tolerance, p.max_outer_iterations,
+                                           p.inner_relative_tolerance, p.max_inner_iterations};
+    ResolvedPortFlowResult result;
+    result.field.x_faces.resize(spec.x_face_count());
+    result.field.y_faces.resize(spec.y_face_count());
+    result.field.z_faces.resize(spec.z_face_count());
+    std::vector<msim_port_state> states(ports.size());
+    msim_resolved_port_report report{};
+    if (const int s = msim_hip_solve_resolved_port_flow(
+            ctx_, &g, drag.empty() ? nullptr : drag.data(), held.data(),
+            static_cast<std::uint32_t>(held.size()), &params, result.field.x_faces.data(),
+            result.field.y_faces.data(), result.field.z_faces.data(), &report, states.data());
+        s != MSIM_OK) {
+      rethrow(s, ctx_);
+    }
+    result.report = {.solves = report.solves,
+                     .outer_iterations = report.outer_iterations,
+                     .inner_iterations = report.inner_iterations,
+                     .relative_residual = report.relative_residual,
+                     .momentum_relative_residual = report.momentum_relative_residual,
+                     .divergence_rms = report.divergence_rms,
+                     .max_speed = report.max_speed,
+                     .min_gap_voxels = report.min_gap_voxels};
+    for (const auto& st : states) {
+      result.ports.push_back({.pressure = st.pressure, .flux = st.flux,
+                              .mean_speed = st.mean_speed});
+    }
+    return result;
   }
 
  private:

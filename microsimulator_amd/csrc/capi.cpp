@@ -242,54 +242,6 @@ void validate_flow(const msim_grid& g, const float* mobility, uint32_t axis,
   take_columns(g, mobility, std::move(mask), std::move(c), solid, mob, layers);
 }
 
-// port_face_geometry (flow_system.hpp:203-219): the axis a face is normal
-// to, whether it is that axis's upper face, and its in-plane axes.
-struct PortGeometry {
-  int component = 0;
-  bool upper = false;
-  int in_plane[2] = {0, 0};
-};
-
-PortGeometry port_geometry(uint32_t face) {
-  if (face / 2 > 2) invalid("unknown flow port face");
-  static constexpr int planes[3][2] = {{1, 2}, {0, 2}, {0, 1}};
-  PortGeometry p;
-  p.component = static_cast<int>(face / 2);
-  p.upper = face % 2 == 1;
-  p.in_plane[0] = planes[p.component][0];
-  p.in_plane[1] = planes[p.component][1];
-  return p;
-}
-
-// for_each_port_face (flow_system.hpp:221-236) on a grid of (nx, ny, nz):
-// body(face index into x|y|z faces, index of the site inside the face), the
-// first in-plane index outer.
-template <typename Body>
-void for_each_port_face(const uint32_t dims[3], const msim_flow_port& port, Body&& body) {
-  const PortGeometry geo = port_geometry(port.face);
-  const size_t nx = dims[0], ny = dims[1], nz = dims[2];
-  const size_t offsets[3] = {0, (nx + 1) * ny * nz, (nx + 1) * ny * nz + nx * (ny + 1) * nz};
-  for (uint32_t first = port.begin[0]; first < port.end[0]; ++first) {
-    for (uint32_t second = port.begin[1]; second < port.end[1]; ++second) {
-      uint32_t face[3] = {0, 0, 0};
-      face[geo.component] = geo.upper ? dims[geo.component] : 0;
-      face[geo.in_plane[0]] = first;
-      face[geo.in_plane[1]] = second;
-      uint32_t site[3] = {face[0], face[1], face[2]};
-      if (geo.upper) --site[geo.component];
-      size_t index;
-      if (geo.component == 0) {
-        index = (face[0] * ny + face[1]) * nz + face[2];
-      } else if (geo.component == 1) {
-        index = (face[0] * (ny + 1) + face[1]) * nz + face[2];
-      } else {
-        index = (face[0] * ny + face[1]) * (nz + 1) + face[2];
-      }
-      body(offsets[geo.component] + index, (site[0] * ny + site[1]) * nz + site[2]);
-    }
-  }
-}
-
 // validate_flow_ports (cpp/cpu/cpu_flow.cpp:193-286) on a grid of `dims`
 // with boundary kinds `kinds` and obstacle mask `solid`.
 void validate_ports(const uint32_t dims[3], const uint8_t kinds[6],
@@ -391,7 +343,7 @@ void validate_ports(const uint32_t dims[3], const uint8_t kinds[6],
 // each port carries flow through.
 PortPlan plan_ports(const msim_grid& g, const float* mobility, const msim_flow_port* ports,
                     uint32_t count, std::vector<uint8_t>& solid, std::vector<float>& mob,
-                    FlowDevice::Layers& layers) {
+                    FlowDevice::Layers& layers, bool check_reduced = true) {
   validate_lattice(g);
   if (count > 0 && ports == nullptr) invalid("ports must not be null");
   std::vector<uint8_t> mask = obstacle_mask(g);
@@ -420,7 +372,7 @@ PortPlan plan_ports(const msim_grid& g, const float* mobility, const msim_flow_p
   // On one layer whose every fluid site carries conductance the grid of
   // columns is the grid itself, and the ports on it are the same ports: the
   // checks there would repeat the ones above exactly.
-  if (nz > 1 || c.solid != mask) {
+  if (check_reduced && (nz > 1 || c.solid != mask)) {
     const uint32_t columns[3] = {g.nx, g.ny, 1};
     const uint8_t column_kinds[6] = {g.boundary[0], g.boundary[1], g.boundary[2],
                                      g.boundary[3], MSIM_NO_FLUX,   MSIM_NO_FLUX};
@@ -456,6 +408,92 @@ PortPlan plan_ports(const msim_grid& g, const float* mobility, const msim_flow_p
   }
   take_columns(g, mobility, std::move(mask), std::move(c), solid, mob, layers);
   return plan;
+}
+
+// Prepares the context's depth-averaged system for shallow_port_pressure
+// (flow_system.hpp:1318-1338): the reduction with the default mobility, the
+// ports on the grid of columns and the system they bound. Returns false where
+// the reference's construction refuses (its invalid_argument: shallow flow
+// does not describe the geometry, a port in a z face or short of its columns'
+// depth, or a boundary that opens onto no fluid). The reference checks the
+// reduced ports no further, so neither does this.
+bool prepare_shallow_ports(Context& c, const msim_grid& g, const msim_flow_port* ports,
+                           uint32_t count, PortPlan& plan) {
+  std::vector<uint8_t> solid;
+  std::vector<float> mob;
+  auto& f = c.flow;
+  f.prepared = false;
+  try {
+    plan = plan_ports(g, nullptr, ports, count, solid, mob, f.layers, false);
+  } catch (const Error& e) {
+    if (e.code() == MSIM_INVALID_ARGUMENT) return false;
+    throw;
+  }
+  // A column conducts when it is fluid in the grid of columns with a nonzero
+  // conductance; slots [x_lower ny | x_upper ny | y_lower nx | y_upper nx].
+  const size_t nx = g.nx, ny = g.ny;
+  bool open = false;
+  for (size_t slot = 0; slot < plan.open.size() && !open; ++slot) {
+    if (plan.open[slot] == 0) continue;
+    size_t column;
+    if (slot < ny) {
+      column = slot;
+    } else if (slot < 2 * ny) {
+      column = (nx - 1) * ny + (slot - ny);
+    } else if (slot < 2 * ny + nx) {
+      column = (slot - 2 * ny) * ny;
+    } else {
+      column = (slot - 2 * ny - nx) * ny + ny - 1;
+    }
+    open = solid[column] == 0 && mob[column] > 0.0f;
+  }
+  if (!open) return false;
+  f.nx = g.nx;
+  f.ny = g.ny;
+  f.dx = g.dx;
+  f.dy = g.dy;
+  f.dz = g.nz == 1 ? g.dz : 1.0f;
+  f.axis = 0;
+  f.ported = true;
+  const size_t slots = plan.open.size();
+  f.edge_open.reserve(slots);
+  f.edge_pressure.reserve(slots);
+  check(hipMemcpyAsync(f.edge_open.data(), plan.open.data(), slots, hipMemcpyHostToDevice,
+                       c.stream),
+        "upload open port faces");
+  check(hipMemsetAsync(f.edge_pressure.data(), 0, slots * sizeof(float), c.stream),
+        "clear port pressures");
+  flow_assemble(c, solid.data(), mob.data());
+  return true;
+}
+
+// The depth-averaged pressure of the prepared system with each port held at
+// pressures[port]: FP64 Jacobi-PCG to 1e-6 in at most 50000 iterations (the
+// reference's solve_depth_averaged_pressure, cpu_flow.cpp:172-175), each
+// column's pressure on every voxel of the column.
+void shallow_port_pressure(Context& c, const PortPlan& plan, uint32_t nz,
+                           const std::vector<float>& pressures, std::vector<double>& known) {
+  auto& f = c.flow;
+  std::vector<float> edge(plan.open.size(), 0.0f);
+  for (size_t slot = 0; slot < edge.size(); ++slot) {
+    edge[slot] = plan.owner[slot] >= 0 ? pressures[static_cast<size_t>(plan.owner[slot])] : 0.0f;
+  }
+  flow_hold_pressures(c, edge.data());
+  bool converged = false;
+  double relative = 0.0;
+  jacobi_solve(c, static_cast<double>(1.0e-6f), 50000, 64, &converged, &relative, nullptr);
+  if (!converged) {
+    throw Error(MSIM_NOT_CONVERGED,
+                "HIP depth-averaged flow conjugate gradient did not converge: relative " +
+                    std::to_string(relative));
+  }
+  std::vector<double> columns(f.sites());
+  check(hipMemcpyAsync(columns.data(), f.x.data(), columns.size() * sizeof(double),
+                       hipMemcpyDeviceToHost, c.stream),
+        "download depth-averaged pressure");
+  check(hipStreamSynchronize(c.stream), "depth-averaged pressure");
+  known.resize(columns.size() * nz);
+  for (size_t site = 0; site < known.size(); ++site) known[site] = columns[site / nz];
 }
 
 // scale_velocity (flow_system.hpp:1368-1415) over the lifted voxel faces: the
@@ -1030,7 +1068,8 @@ void upload_coupled(Context& c, const msim_signal_grid& g, const float* origin,
 // radius. Returns the first refused cell (cells.count when none), its message
 // and whether its size (not its position) was refused.
 uint32_t colony_host_checks(const msim_grid& g, const float* origin, const msim_cells& cells,
-                            double radius, std::string* message, bool* size_refused) {
+                            double radius, std::string* message, bool* size_refused,
+                            const float* velocities = nullptr) {
   *size_refused = false;
   if (!std::isfinite(radius) || radius <= 0.0) invalid("averaging radius must be finite and positive");
   validate_lattice(g);
@@ -1045,6 +1084,12 @@ uint32_t colony_host_checks(const msim_grid& g, const float* origin, const msim_
   const uint32_t dims[3] = {g.nx, g.ny, g.nz};
   const double spacing[3] = {g.dx, g.dy, g.dz};
   for (uint32_t c = 0; c < cells.count; ++c) {
+    // colony_flow_coupling checks each cell's velocity first (flow.py:409-411).
+    if (velocities != nullptr &&
+        (!std::isfinite(velocities[2 * c]) || !std::isfinite(velocities[2 * c + 1]))) {
+      *message = "in-plane cell velocity must be finite";
+      return c;
+    }
     const double p[3] = {cells.position_x[c], cells.position_y[c], cells.position_z[c]};
     if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) {
       *message = "deposited positions must be finite";
@@ -1072,10 +1117,12 @@ uint32_t colony_host_checks(const msim_grid& g, const float* origin, const msim_
 // the device found none before it (a cell's support is judged before its size).
 void colony_run(Context& ctx, const msim_grid& g, const float* origin, const msim_cells& cells,
                 double radius, const double* base, uint32_t base_count, double drag, double cap,
-                double* density, double* mobility, const std::string& pending) {
+                double* density, double* mobility, const std::string& pending,
+                const float* velocities = nullptr, double* force = nullptr) {
   std::string message;
   bool size_refused = false;
-  const uint32_t refused = colony_host_checks(g, origin, cells, radius, &message, &size_refused);
+  const uint32_t refused =
+      colony_host_checks(g, origin, cells, radius, &message, &size_refused, velocities);
   const uint32_t axis = [&] {
     uint32_t total = 0;
     const uint32_t dims[3] = {g.nx, g.ny, g.nz};
@@ -1107,10 +1154,34 @@ void colony_run(Context& ctx, const msim_grid& g, const float* origin, const msi
   }
   const uint32_t failed =
       colony_deposit(ctx, g, origin, head, radius, base, base_count, drag, cap, density_out,
-                     mobility_ok ? mobility : nullptr);
+                     mobility_ok ? mobility : nullptr, velocities,
+                     mobility_ok ? force : nullptr);
   if (failed != 0xffffffffu) invalid("biomass deposition has no connected fluid support");
   if (refused < cells.count) invalid(message);
   if (!mobility_ok) invalid(pending);
+}
+
+// _darcy_coefficients' checks (flow.py:359-374) and the drag's (91-96), which
+// the reference makes after the deposit: the first failing one's message, or
+// an empty string.
+std::string darcy_pending(const msim_grid& grid, const double* base, uint32_t base_count,
+                          double drag_coefficient, double max_volume_fraction) {
+  const size_t n = static_cast<size_t>(grid.nx) * grid.ny * grid.nz;
+  if (!(max_volume_fraction > 0.0 && max_volume_fraction < 1.0)) {
+    return "maximum volume fraction must lie strictly between zero and one";
+  }
+  if (base_count == 0 && (!std::isfinite(base[0]) || base[0] <= 0.0)) {
+    return "base mobility must be finite and positive";
+  }
+  if (base_count != 0 && base_count != n) return "base mobility must hold one value per grid site";
+  if (base_count != 0 &&
+      !std::all_of(base, base + n, [](double b) { return std::isfinite(b) && b >= 0.0; })) {
+    return "base mobility values must be finite and non-negative";
+  }
+  if (!std::isfinite(drag_coefficient) || drag_coefficient < 0.0) {
+    return "drag coefficient must be finite and non-negative";
+  }
+  return std::string();
 }
 }  // namespace
 }  // namespace msim
@@ -1333,6 +1404,53 @@ int msim_hip_solve_resolved_flow(msim_ctx* ctx, const msim_grid* grid, const flo
     std::vector<float> faces(nfx + nfy + nfz);
     msim_resolved_report rep{};
     msim::resolved_flow(c, *grid, mask, drag, *params, faces.data(), rep);
+    if (x_faces != nullptr) std::memcpy(x_faces, faces.data(), nfx * sizeof(float));
+    if (y_faces != nullptr) std::memcpy(y_faces, faces.data() + nfx, nfy * sizeof(float));
+    if (z_faces != nullptr) std::memcpy(z_faces, faces.data() + nfx + nfy, nfz * sizeof(float));
+    if (report != nullptr) *report = rep;
+  });
+}
+
+int msim_hip_solve_resolved_port_flow(msim_ctx* ctx, const msim_grid* grid, const float* drag,
+                                      const msim_flow_port* ports, uint32_t port_count,
+                                      const msim_resolved_port_params* params, float* x_faces,
+                                      float* y_faces, float* z_faces,
+                                      msim_resolved_port_report* report,
+                                      msim_port_state* states) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || params == nullptr) msim::invalid("grid and parameters must not be null");
+    // ResolvedPortFlowParameters::validate (cpu_flow.cpp:179-185).
+    const auto tolerance = [](float t, const char* name) {
+      if (!std::isfinite(t) || t < std::numeric_limits<float>::epsilon()) {
+        msim::invalid(std::string(name) + " must be finite and at least float machine epsilon");
+      }
+    };
+    tolerance(params->relative_tolerance, "resolved-flow outer relative tolerance");
+    tolerance(params->inner_relative_tolerance, "resolved-flow inner relative tolerance");
+    if (params->max_outer_iterations == 0 || params->max_inner_iterations == 0) {
+      msim::invalid("resolved-flow iteration limits must be positive");
+    }
+    // validate_flow_ports (cpu_flow.cpp:194-286).
+    msim::validate_lattice(*grid);
+    if (port_count > 0 && ports == nullptr) msim::invalid("ports must not be null");
+    const std::vector<uint8_t> mask = msim::obstacle_mask(*grid);
+    const uint32_t dims[3] = {grid->nx, grid->ny, grid->nz};
+    msim::validate_ports(dims, grid->boundary, mask, ports, port_count);
+    msim::PortPlan plan;
+    const bool shallow = msim::prepare_shallow_ports(c, *grid, ports, port_count, plan);
+    const uint32_t nz = grid->nz;
+    const msim::ShallowPressure known = [&](const std::vector<float>& pressures,
+                                            std::vector<double>& out) {
+      out.clear();
+      if (shallow) msim::shallow_port_pressure(c, plan, nz, pressures, out);
+    };
+    const size_t nfx = static_cast<size_t>(grid->nx + 1) * grid->ny * grid->nz;
+    const size_t nfy = static_cast<size_t>(grid->nx) * (grid->ny + 1) * grid->nz;
+    const size_t nfz = static_cast<size_t>(grid->nx) * grid->ny * (grid->nz + 1);
+    std::vector<float> faces(nfx + nfy + nfz);
+    msim_resolved_port_report rep{};
+    msim::resolved_port_flow(c, *grid, mask, drag, ports, port_count, *params, known,
+                             faces.data(), rep, states);
     if (x_faces != nullptr) std::memcpy(x_faces, faces.data(), nfx * sizeof(float));
     if (y_faces != nullptr) std::memcpy(y_faces, faces.data() + nfx, nfy * sizeof(float));
     if (z_faces != nullptr) std::memcpy(z_faces, faces.data() + nfx + nfy, nfz * sizeof(float));
@@ -1782,25 +1900,30 @@ int msim_hip_colony_mobility(msim_ctx* ctx, const msim_grid* grid, const float* 
         mobility == nullptr) {
       msim::invalid("grid, origin, cells, base and mobility must not be null");
     }
-    // _darcy_coefficients' checks (flow.py:359-374) and the drag's (91-96),
-    // reported after the deposit's own.
-    std::string pending;
-    const size_t n = static_cast<size_t>(grid->nx) * grid->ny * grid->nz;
-    if (!(max_volume_fraction > 0.0 && max_volume_fraction < 1.0)) {
-      pending = "maximum volume fraction must lie strictly between zero and one";
-    } else if (base_count == 0 && (!std::isfinite(base[0]) || base[0] <= 0.0)) {
-      pending = "base mobility must be finite and positive";
-    } else if (base_count != 0 && base_count != n) {
-      pending = "base mobility must hold one value per grid site";
-    } else if (base_count != 0 &&
-               !std::all_of(base, base + n, [](double b) { return std::isfinite(b) && b >= 0.0; })) {
-      pending = "base mobility values must be finite and non-negative";
-    } else if (!std::isfinite(drag_coefficient) || drag_coefficient < 0.0) {
-      pending = "drag coefficient must be finite and non-negative";
-    }
+    const std::string pending =
+        msim::darcy_pending(*grid, base, base_count, drag_coefficient, max_volume_fraction);
     msim::colony_run(c, *grid, origin, *cells, averaging_radius, base,
                      base_count == 0 ? 1u : base_count, drag_coefficient, max_volume_fraction,
                      nullptr, mobility, pending);
+  });
+}
+
+int msim_hip_colony_flow_coupling(msim_ctx* ctx, const msim_grid* grid, const float* origin,
+                                  const msim_cells* cells, const float* velocities,
+                                  const double* base, uint32_t base_count,
+                                  double drag_coefficient, double max_volume_fraction,
+                                  double averaging_radius, double* mobility, double* force) {
+  return msim::guarded(ctx, [&](Context& c) {
+    if (grid == nullptr || origin == nullptr || cells == nullptr || base == nullptr ||
+        mobility == nullptr || force == nullptr ||
+        (cells->count > 0 && velocities == nullptr)) {
+      msim::invalid("grid, origin, cells, velocities, base, mobility and force must not be null");
+    }
+    const std::string pending =
+        msim::darcy_pending(*grid, base, base_count, drag_coefficient, max_volume_fraction);
+    msim::colony_run(c, *grid, origin, *cells, averaging_radius, base,
+                     base_count == 0 ? 1u : base_count, drag_coefficient, max_volume_fraction,
+                     nullptr, mobility, pending, velocities, force);
   });
 }
 

@@ -67,10 +67,14 @@ __device__ inline void wave_argmax(double& value, uint32_t& index) {
 
 // One cell per wave, the waves looping over the cells. A cell whose support
 // has no connected fluid records itself in `failure` (the least such cell).
+// With `momentum` (colony_flow_coupling, flow.py:384-424): each voxel's share
+// times the cell's in-plane velocity (vx, vy) is also added into its column's
+// momentum (x, y).
 __global__ __launch_bounds__(kDepositThreads) void k_colony_deposit(
     DepositParams P, const uint8_t* __restrict__ solid, const float* __restrict__ px,
     const float* __restrict__ py, const float* __restrict__ pz, const float* __restrict__ lengths,
-    const float* __restrict__ radii, double* volume, unsigned int* failure) {
+    const float* __restrict__ radii, const float* __restrict__ vx, const float* __restrict__ vy,
+    double* volume, double* momentum, unsigned int* failure) {
   extern __shared__ double s_mem[];
   double* s_axis = s_mem;                   // axis weights: x, then y, then z
   double* s_kernel = s_mem + P.axis_cap;    // the support's weights
@@ -206,13 +210,20 @@ __global__ __launch_bounds__(kDepositThreads) void k_colony_deposit(
     // biomass_volume (biomass.py:6-14), then its share per voxel.
     const double r = static_cast<double>(radii[c]);
     const double amount = 3.14159265358979323846 * r * r * (static_cast<double>(lengths[c]) + 2.0 * r);
+    const double velocity[2] = {momentum != nullptr ? static_cast<double>(vx[c]) : 0.0,
+                                momentum != nullptr ? static_cast<double>(vy[c]) : 0.0};
     for (uint32_t v = tid; v < region; v += kDepositThreads) {
       const double k = s_kernel[v];
       if (k == 0.0) continue;
       const uint32_t i = v / plane, j = (v / count[2]) % count[1], l = v % count[2];
-      const size_t site = (static_cast<size_t>(lo[0] + i) * P.dims[1] + (lo[1] + j)) *
-                              P.dims[2] + (lo[2] + l);
-      unsafeAtomicAdd(&volume[site], amount * (k / total));
+      const size_t column = static_cast<size_t>(lo[0] + i) * P.dims[1] + (lo[1] + j);
+      const size_t site = column * P.dims[2] + (lo[2] + l);
+      const double share = amount * (k / total);
+      unsafeAtomicAdd(&volume[site], share);
+      if (momentum != nullptr) {
+        unsafeAtomicAdd(&momentum[2 * column], share * velocity[0]);
+        unsafeAtomicAdd(&momentum[2 * column + 1], share * velocity[1]);
+      }
     }
     __syncthreads();
   }
@@ -220,11 +231,14 @@ __global__ __launch_bounds__(kDepositThreads) void k_colony_deposit(
 
 // Volume per voxel volume; with `mobility`, per column the mean fraction over
 // its fluid voxels capped at `cap`, the drag C f^2 / (1 - f)^3 and
-// base / (1 + base * drag) at each fluid voxel (flow.py:359-382).
+// base / (1 + base * drag) at each fluid voxel (flow.py:359-382); with
+// `force`, the column's body force momentum * drag / mass, mass the column's
+// deposited volume, zero where it has none (flow.py:421-423).
 __global__ __launch_bounds__(256) void k_colony_finish(
     uint32_t nx, uint32_t ny, uint32_t nz, double voxel, const double* __restrict__ volume,
     double* __restrict__ density, const uint8_t* __restrict__ solid, const double* __restrict__ base,
-    uint32_t per_site_base, double drag_coefficient, double cap, double* __restrict__ mobility) {
+    uint32_t per_site_base, double drag_coefficient, double cap, double* __restrict__ mobility,
+    const double* __restrict__ momentum, double* __restrict__ force) {
   const size_t column = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
   if (column >= static_cast<size_t>(nx) * ny) return;
   const size_t first = column * nz;
@@ -245,6 +259,12 @@ __global__ __launch_bounds__(256) void k_colony_finish(
     const bool open = solid == nullptr || solid[first + z] == 0;
     mobility[first + z] = open ? b / (1.0 + b * drag) : 0.0;
   }
+  if (force == nullptr) return;
+  double mass = 0.0;
+  for (uint32_t z = 0; z < nz; ++z) mass += volume[first + z];
+  const double per_mass = mass > 0.0 ? drag / mass : 0.0;
+  force[2 * column] = momentum[2 * column] * per_mass;
+  force[2 * column + 1] = momentum[2 * column + 1] * per_mass;
 }
 
 __global__ void k_fill_u32_two(unsigned int* out, unsigned int value) {
@@ -272,7 +292,8 @@ uint32_t colony_region_cap(const msim_grid& grid, double radius) {
 uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin,
                         const msim_cells& cells, double radius, const double* base,
                         uint32_t base_count, double drag_coefficient, double cap,
-                        double* density_host, double* mobility_host) {
+                        double* density_host, double* mobility_host, const float* velocities,
+                        double* force_host) {
   auto& d = ctx.colony;
   const size_t n = static_cast<size_t>(grid.nx) * grid.ny * grid.nz;
   const size_t count = cells.count;
@@ -293,6 +314,22 @@ uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin
   if (grid.obstacles != nullptr) {
     check(hipMemcpyAsync(d.solid.data(), grid.obstacles, n, hipMemcpyHostToDevice, ctx.stream),
           "upload mask");
+  }
+  const bool coupling = force_host != nullptr;
+  const size_t columns = static_cast<size_t>(grid.nx) * grid.ny;
+  if (coupling) {
+    std::vector<float> split(2 * count);
+    for (size_t c = 0; c < count; ++c) {
+      split[c] = velocities[2 * c];
+      split[count + c] = velocities[2 * c + 1];
+    }
+    put(d.velocity_x, split.data());
+    put(d.velocity_y, split.data() + count);
+    d.momentum.reserve(2 * columns);
+    d.force.reserve(2 * columns);
+    k_fill_f64<<<static_cast<uint32_t>((2 * columns + 255) / 256), 256, 0, ctx.stream>>>(
+        d.momentum.data(), 2 * columns);
+    check(hipGetLastError(), "launch k_fill_f64");
   }
   d.volume.reserve(n);
   d.out.reserve(n);
@@ -326,7 +363,9 @@ uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin
     const uint32_t blocks = static_cast<uint32_t>(std::min<size_t>(count, 256 * 64));
     k_colony_deposit<<<blocks, kDepositThreads, lds, ctx.stream>>>(
         P, d.solid.data(), d.position_x.data(), d.position_y.data(), d.position_z.data(),
-        d.lengths.data(), d.radii.data(), d.volume.data(), d.failure.data());
+        d.lengths.data(), d.radii.data(), coupling ? d.velocity_x.data() : nullptr,
+        coupling ? d.velocity_y.data() : nullptr, d.volume.data(),
+        coupling ? d.momentum.data() : nullptr, d.failure.data());
     check(hipGetLastError(), "launch k_colony_deposit");
   }
   const double voxel = static_cast<double>(grid.dx * grid.dy * grid.dz);
@@ -337,11 +376,11 @@ uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin
                          ctx.stream),
           "upload base");
   }
-  const size_t columns = static_cast<size_t>(grid.nx) * grid.ny;
   k_colony_finish<<<static_cast<uint32_t>((columns + 255) / 256), 256, 0, ctx.stream>>>(
       grid.nx, grid.ny, grid.nz, voxel, d.volume.data(), want_mobility ? nullptr : d.out.data(),
       grid.obstacles != nullptr ? d.solid.data() : nullptr, d.base.data(), base_count > 1 ? 1u : 0u,
-      drag_coefficient, cap, want_mobility ? d.out.data() : nullptr);
+      drag_coefficient, cap, want_mobility ? d.out.data() : nullptr,
+      coupling ? d.momentum.data() : nullptr, coupling ? d.force.data() : nullptr);
   check(hipGetLastError(), "launch k_colony_finish");
   check(hipEventRecord(ctx.stop, ctx.stream), "event");
   unsigned int failed[2] = {0, 0};
@@ -359,6 +398,11 @@ uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin
     check(hipMemcpyAsync(want_mobility ? mobility_host : density_host, d.out.data(),
                          n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
           "download colony field");
+    if (coupling) {
+      check(hipMemcpyAsync(force_host, d.force.data(), 2 * columns * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx.stream),
+            "download colony force");
+    }
     check(hipStreamSynchronize(ctx.stream), "download colony field");
   }
   return failed[0];

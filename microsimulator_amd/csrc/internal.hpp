@@ -8,6 +8,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -346,8 +347,10 @@ struct CoupledDevice {
 // failing cell (and an internal bound check).
 struct ColonyDevice {
   DeviceBuffer<float> position_x, position_y, position_z, lengths, radii;
+  DeviceBuffer<float> velocity_x, velocity_y;  // in-plane cell velocities (flow coupling)
   DeviceBuffer<uint8_t> solid;
   DeviceBuffer<double> volume, out, base;
+  DeviceBuffer<double> momentum, force;        // per column (x, y) (flow coupling)
   DeviceBuffer<unsigned int> failure;
   double seconds = 0.0;
 };
@@ -474,12 +477,75 @@ void mg_coarse_operator(hipStream_t stream, const float* fwx, const float* fwy, 
                         uint32_t fnx, uint32_t fny, float* wx, float* wy, float* bc, float* dinv,
                         uint32_t nx, uint32_t ny);
 
+// port_face_geometry (flow_system.hpp:203-219): the axis a face is normal
+// to, whether it is that axis's upper face, and its in-plane axes.
+struct PortGeometry {
+  int component = 0;
+  bool upper = false;
+  int in_plane[2] = {0, 0};
+};
+
+inline PortGeometry port_geometry(uint32_t face) {
+  if (face / 2 > 2) throw Error(MSIM_INVALID_ARGUMENT, "unknown flow port face");
+  static constexpr int planes[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+  PortGeometry p;
+  p.component = static_cast<int>(face / 2);
+  p.upper = face % 2 == 1;
+  p.in_plane[0] = planes[p.component][0];
+  p.in_plane[1] = planes[p.component][1];
+  return p;
+}
+
+// for_each_port_face (flow_system.hpp:221-236) on a grid of (nx, ny, nz):
+// body(face index into x|y|z faces, index of the site inside the face), the
+// first in-plane index outer.
+template <typename Body>
+void for_each_port_face(const uint32_t dims[3], const msim_flow_port& port, Body&& body) {
+  const PortGeometry geo = port_geometry(port.face);
+  const size_t nx = dims[0], ny = dims[1], nz = dims[2];
+  const size_t offsets[3] = {0, (nx + 1) * ny * nz, (nx + 1) * ny * nz + nx * (ny + 1) * nz};
+  for (uint32_t first = port.begin[0]; first < port.end[0]; ++first) {
+    for (uint32_t second = port.begin[1]; second < port.end[1]; ++second) {
+      uint32_t face[3] = {0, 0, 0};
+      face[geo.component] = geo.upper ? dims[geo.component] : 0;
+      face[geo.in_plane[0]] = first;
+      face[geo.in_plane[1]] = second;
+      uint32_t site[3] = {face[0], face[1], face[2]};
+      if (geo.upper) --site[geo.component];
+      size_t index;
+      if (geo.component == 0) {
+        index = (face[0] * ny + face[1]) * nz + face[2];
+      } else if (geo.component == 1) {
+        index = (face[0] * (ny + 1) + face[1]) * nz + face[2];
+      } else {
+        index = (face[0] * ny + face[1]) * (nz + 1) + face[2];
+      }
+      body(offsets[geo.component] + index, (site[0] * ny + site[1]) * nz + site[2]);
+    }
+  }
+}
+
 // resolved.hip: the resolved Stokes-Brinkman flow along params.axis of a grid
 // already validated (mask: its obstacles, 0/1 per site); faces_out: every
 // face velocity (x|y|z), scaled.
 void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
                    const float* drag, const msim_resolved_params& params, float* faces_out,
                    msim_resolved_report& report);
+
+// The depth-averaged pressure of the ports held at the given pressures, one
+// per site (each column's on all its voxels), or left empty where shallow flow
+// does not describe the geometry (shallow_port_pressure,
+// flow_system.hpp:1318-1338).
+using ShallowPressure =
+    std::function<void(const std::vector<float>& pressures, std::vector<double>& known)>;
+
+// resolved.hip: the resolved flow driven through ports already validated
+// (validate_flow_ports); faces_out: the combined field (x|y|z).
+void resolved_port_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
+                        const float* drag, const msim_flow_port* ports, uint32_t count,
+                        const msim_resolved_port_params& params, const ShallowPressure& shallow,
+                        float* faces_out, msim_resolved_port_report& report,
+                        msim_port_state* states);
 
 // batch.hip: the batched sweep (msim_hip_batch_*).
 void batch_upload(Context& ctx, uint32_t count, const msim_batch_instance* inst,
@@ -579,9 +645,12 @@ void coupled_download(Context& ctx, float* levels, float* species);
 // mobility_host the colony mobility (base: 1 or N values). Returns the least
 // cell whose support has no connected fluid, or 0xffffffff.
 uint32_t colony_region_cap(const msim_grid& grid, double radius);
+// velocities (2 floats per cell, in-plane x then y) and force_host (2 doubles
+// per column): colony_flow_coupling's body force; both null otherwise.
 uint32_t colony_deposit(Context& ctx, const msim_grid& grid, const float* origin,
                         const msim_cells& cells, double radius, const double* base,
                         uint32_t base_count, double drag_coefficient, double cap,
-                        double* density_host, double* mobility_host);
+                        double* density_host, double* mobility_host,
+                        const float* velocities = nullptr, double* force_host = nullptr);
 
 }  // namespace msim

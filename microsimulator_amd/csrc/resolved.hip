@@ -36,7 +36,6 @@ namespace msim {
 namespace {
 
 constexpr int kResBlock = 256;
-constexpr int kResChunk = 4096;   // entries per dot-product block
 constexpr uint32_t kRestart = 40;  // flexible_gmres.hpp:46
 
 // The grid as the kernels see it.
@@ -47,6 +46,7 @@ struct ResView {
   double h[3];        // spacing
   double inv[3];      // 1 / h^2, 0 on an axis of one site (no viscous term)
   double cs;          // continuity scale 1 / min spacing
+  uint32_t chunk;     // entries per workgroup of the reducing kernels
   const uint8_t* active;
   const uint8_t* exists;
   const uint8_t* fluid;
@@ -59,18 +59,19 @@ __device__ inline int face_component(const ResView& V, uint64_t f) {
   return f < V.off[1] ? 0 : (f < V.off[2] ? 1 : 2);
 }
 
-// Face f's coordinates in its component's face lattice.
+// Face f's coordinates in its component's face lattice (32-bit division:
+// the C ABI refuses grids of 2^32 faces or more).
 __device__ inline void face_xyz(const ResView& V, int c, uint64_t f, uint32_t fd[3],
                                 uint32_t co[3]) {
   fd[0] = V.dims[0];
   fd[1] = V.dims[1];
   fd[2] = V.dims[2];
   ++fd[c];
-  uint64_t local = f - V.off[c];
-  co[2] = static_cast<uint32_t>(local % fd[2]);
+  uint32_t local = static_cast<uint32_t>(f - V.off[c]);
+  co[2] = local % fd[2];
   local /= fd[2];
-  co[1] = static_cast<uint32_t>(local % fd[1]);
-  co[0] = static_cast<uint32_t>(local / fd[1]);
+  co[1] = local % fd[1];
+  co[0] = local / fd[1];
 }
 
 __device__ inline uint64_t face_at(const ResView& V, int c, const uint32_t fd[3], uint32_t x,
@@ -82,10 +83,12 @@ __device__ inline uint64_t site_at(const ResView& V, uint32_t x, uint32_t y, uin
   return (static_cast<uint64_t>(x) * V.dims[1] + y) * V.dims[2] + z;
 }
 
-// apply_momentum (flow_system.hpp:881-910) at face f.
-__device__ inline double momentum_at(const ResView& V, const double* __restrict__ u, uint64_t f,
-                                     int c, const uint32_t fd[3], const uint32_t co[3]) {
-  const double uf = u[f];
+// apply_momentum (flow_system.hpp:881-910) at face f; u(i) is the input at
+// face i.
+template <typename U>
+__device__ inline double momentum_at(const ResView& V, const U& u, uint64_t f, int c,
+                                     const uint32_t fd[3], const uint32_t co[3]) {
+  const double uf = u(f);
   double result = static_cast<double>(V.face_drag[f]) * uf;
   for (int a = 0; a < 3; ++a) {
     if (V.dims[a] == 1) continue;
@@ -99,9 +102,9 @@ __device__ inline double momentum_at(const ResView& V, const double* __restrict_
       }
       double neighbor;
       if (a == c) {
-        neighbor = has ? u[face_at(V, c, fd, nb[0], nb[1], nb[2])] : uf;
+        neighbor = has ? u(face_at(V, c, fd, nb[0], nb[1], nb[2])) : uf;
       } else if (has && V.exists[face_at(V, c, fd, nb[0], nb[1], nb[2])] != 0) {
-        neighbor = u[face_at(V, c, fd, nb[0], nb[1], nb[2])];
+        neighbor = u(face_at(V, c, fd, nb[0], nb[1], nb[2]));
       } else {
         neighbor = -uf;
       }
@@ -109,21 +112,6 @@ __device__ inline double momentum_at(const ResView& V, const double* __restrict_
     }
   }
   return result;
-}
-
-// out = A u on the faces (zero on the system's inactive ones).
-__global__ __launch_bounds__(kResBlock) void k_res_momentum(ResView V, const double* __restrict__ u,
-                                                            double* __restrict__ out) {
-  const uint64_t f = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
-  if (f >= V.nf) return;
-  if (V.active[f] == 0) {
-    out[f] = 0.0;
-    return;
-  }
-  const int c = face_component(V, f);
-  uint32_t fd[3], co[3];
-  face_xyz(V, c, f, fd, co);
-  out[f] = momentum_at(V, u, f, c, fd, co);
 }
 
 // out = [A u + G p; cs D u] (cpu_flow.cpp:414-423; gradient and divergence,
@@ -140,7 +128,7 @@ __global__ __launch_bounds__(kResBlock) void k_res_apply(ResView V, const double
     const int c = face_component(V, f);
     uint32_t fd[3], co[3];
     face_xyz(V, c, f, fd, co);
-    const double m = momentum_at(V, in, f, c, fd, co);
+    const double m = momentum_at(V, [in](uint64_t i) { return in[i]; }, f, c, fd, co);
     const double* p = in + V.nf;
     double lv = 0.0, uv = 0.0;
     if (co[c] > 0) {
@@ -177,129 +165,65 @@ __global__ __launch_bounds__(kResBlock) void k_res_apply(ResView V, const double
   out[k] = V.cs * d;
 }
 
-// Block partials of a . b over chunks of kResChunk entries.
+// Block partials of a . b over chunks of `chunk` entries.
 __global__ __launch_bounds__(kResBlock) void k_res_dot(const double* __restrict__ a,
                                                        const double* __restrict__ b, uint64_t count,
+                                                       uint32_t chunk,
                                                        double* __restrict__ partials) {
   __shared__ double s_red[kResBlock / kWave];
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
-  const uint64_t end = begin + kResChunk < count ? begin + kResChunk : count;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t end = begin + chunk < count ? begin + chunk : count;
   double v = 0.0;
   for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) v += a[i] * b[i];
   v = block_sum<kResBlock>(v, s_red);
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
-// The solve's scalars, in device memory.
+// The solve's scalars, in device memory. The two kernels of a CG iteration
+// each write their own stop flag and failure code and read only the other's,
+// so no launch reads a field that it writes.
 struct ResState {
-  // CG
-  double rho, rhs_norm, alpha, beta, relative;
-  uint32_t iterations, done, failure, pad;
-  // GMRES: the last dot's value and the slots the Arnoldi step writes
-  double value;
+  double rhs_norm, relative;
+  uint32_t iterations, stop1, stop2, failure1, failure2, pad;
 };
 
-enum : uint32_t {
-  kCgRun = 0,
-  kCgDone = 1,
-};
-
-// The sum of `blocks` partials in a fixed order, by one workgroup.
-__device__ inline double sum_partials(const double* partials, uint32_t blocks, double* s_red) {
+// The sum of `blocks` partials in a fixed order, the same in every workgroup
+// (and the same bits as a one-workgroup sum of them).
+__device__ inline double all_sum(const double* partials, uint32_t blocks, double* s_red,
+                                 double* s_b) {
   double v = 0.0;
   for (uint32_t i = threadIdx.x; i < blocks; i += kResBlock) v += partials[i];
-  return block_sum<kResBlock>(v, s_red);
+  v = block_sum<kResBlock>(v, s_red);
+  if (threadIdx.x == 0) *s_b = v;
+  __syncthreads();
+  const double out = *s_b;
+  __syncthreads();
+  return out;
 }
 
-// CG set-up after r, z and their partials (r.r, r.z): the reference's
-// initialize, rhs norm and rho (conjugate_gradient.hpp:49-55).
-__global__ __launch_bounds__(kResBlock) void k_cg_begin(const double* partials, uint32_t blocks,
-                                                        ResState* st) {
-  __shared__ double s_red[kResBlock / kWave];
-  const double rr = sum_partials(partials, blocks, s_red);
-  const double rz = sum_partials(partials + blocks, blocks, s_red);
-  if (threadIdx.x != 0) return;
-  st->iterations = 0;
-  st->failure = 0;
-  st->relative = 1.0;
-  if (rr == 0.0) {
-    st->done = kCgDone;
-    st->rhs_norm = 0.0;
-    return;
-  }
-  st->done = kCgRun;
-  st->rhs_norm = sqrt(rr);
-  st->rho = rz;
-}
+__device__ inline bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
-// alpha = rho / p.q with the curvature and step checks (57-65).
-__global__ __launch_bounds__(kResBlock) void k_cg_alpha(const double* partials, uint32_t blocks,
-                                                        ResState* st) {
-  __shared__ double s_red[kResBlock / kWave];
-  if (st->done != kCgRun) return;
-  const double curvature = sum_partials(partials, blocks, s_red);
-  if (threadIdx.x != 0) return;
-  if (!isfinite(curvature) || curvature <= 0.0) {
-    st->failure = MSIM_CG_CURVATURE;
-    st->done = kCgDone;
-    return;
-  }
-  const double alpha = st->rho / curvature;
-  if (!isfinite(alpha)) {
-    st->failure = MSIM_CG_STEP;
-    st->done = kCgDone;
-    return;
-  }
-  st->alpha = alpha;
-}
+// The CG partials: r.r and r.z of the residual after iteration k in slot
+// (k & 1), p.q of iteration k.
+struct CgParts {
+  double* base;
+  uint32_t blocks;
+  __device__ double* rr(uint32_t k) const { return base + (k & 1) * blocks; }
+  __device__ double* rz(uint32_t k) const { return base + (2 + (k & 1)) * blocks; }
+  __device__ double* pq() const { return base + 4 * blocks; }
+};
 
-// After x += alpha p, r -= alpha q, z = D^-1 r: the relative residual and the
-// stopping test, then beta (66-84).
-__global__ __launch_bounds__(kResBlock) void k_cg_beta(const double* partials, uint32_t blocks,
-                                                       float tolerance, ResState* st) {
-  __shared__ double s_red[kResBlock / kWave];
-  if (st->done != kCgRun) return;
-  const double rr = sum_partials(partials, blocks, s_red);
-  const double rz = sum_partials(partials + blocks, blocks, s_red);
-  if (threadIdx.x != 0) return;
-  const double relative = sqrt(rr > 0.0 ? rr : 0.0) / st->rhs_norm;
-  if (!isfinite(relative)) {
-    st->failure = MSIM_CG_RESIDUAL;
-    st->done = kCgDone;
-    return;
-  }
-  st->iterations += 1;
-  st->relative = relative;
-  if (relative <= static_cast<double>(tolerance)) {
-    st->done = kCgDone;
-    return;
-  }
-  if (!isfinite(rz) || st->rho == 0.0) {
-    st->failure = MSIM_CG_BREAKDOWN;
-    st->done = kCgDone;
-    return;
-  }
-  const double beta = rz / st->rho;
-  if (!isfinite(beta)) {
-    st->failure = MSIM_CG_DIRECTION;
-    st->done = kCgDone;
-    return;
-  }
-  st->beta = beta;
-  st->rho = rz;
-}
-
-// x = 0, r = double(float(rhs)), z = D^-1 r, p = z, and the partials of
-// r.r and r.z.
+// x = 0, r = double(float(rhs)), z = D^-1 r, and the partials of r.r and r.z
+// (the reference's initialize, conjugate_gradient.hpp:49-55); resets the
+// solve's flags.
 __global__ __launch_bounds__(kResBlock) void k_cg_init(ResView V, const double* __restrict__ rhs,
                                                        double* __restrict__ x,
                                                        double* __restrict__ r,
-                                                       double* __restrict__ z,
-                                                       double* __restrict__ p,
-                                                       double* __restrict__ partials) {
+                                                       double* __restrict__ z, CgParts P,
+                                                       ResState* st) {
   __shared__ double s_red[kResBlock / kWave];
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
-  const uint64_t end = begin + kResChunk < V.nf ? begin + kResChunk : V.nf;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * V.chunk;
+  const uint64_t end = begin + V.chunk < V.nf ? begin + V.chunk : V.nf;
   double rr = 0.0, rz = 0.0;
   for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
     const double ri = static_cast<double>(static_cast<float>(rhs[i]));
@@ -308,56 +232,144 @@ __global__ __launch_bounds__(kResBlock) void k_cg_init(ResView V, const double* 
     x[i] = 0.0;
     r[i] = ri;
     z[i] = zi;
-    p[i] = zi;
     rr += ri * ri;
     rz += ri * zi;
   }
   rr = block_sum<kResBlock>(rr, s_red);
   rz = block_sum<kResBlock>(rz, s_red);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = rr;
-    partials[gridDim.x + blockIdx.x] = rz;
+    P.rr(0)[blockIdx.x] = rr;
+    P.rz(0)[blockIdx.x] = rz;
+  }
+  if (lead()) {
+    st->stop1 = st->stop2 = 0;
+    st->failure1 = st->failure2 = 0;
+    st->iterations = 0;
+    st->relative = 1.0;
   }
 }
 
-// q = A p and the partials of p.q.
-__global__ __launch_bounds__(kResBlock) void k_cg_apply(ResView V, const double* __restrict__ p,
-                                                        double* __restrict__ q,
-                                                        double* __restrict__ partials,
-                                                        const ResState* st) {
+// The first half of CG iteration k. For k > 0: the relative residual of
+// iteration k - 1's update and the stopping test, then beta = r.z / rho and
+// the direction p = z + beta p_old (conjugate_gradient.hpp:66-84); for k = 0
+// the right-hand side's norm and p = z. Then q = A p and the partials of p.q.
+// Every workgroup re-adds the partials it needs in the same fixed order, so
+// all hold the same scalars; workgroup 0 records them. A neighbour's p is
+// formed from z and p_old where it is read, so p_old (another buffer) is
+// never written here. `last`: test only (the iteration limit is reached).
+__global__ __launch_bounds__(kResBlock) void k_cg_direction(
+    ResView V, const double* __restrict__ z, const double* __restrict__ p_old,
+    double* __restrict__ p, double* __restrict__ q, CgParts P, uint32_t k, uint32_t last,
+    float tolerance, ResState* st) {
   __shared__ double s_red[kResBlock / kWave];
-  if (st->done != kCgRun) return;
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
-  const uint64_t end = begin + kResChunk < V.nf ? begin + kResChunk : V.nf;
+  __shared__ double s_b;
+  if (st->stop2 != 0) {
+    if (lead()) st->stop1 = 1;
+    return;
+  }
+  double beta = 0.0;
+  const double rr = all_sum(P.rr(k), P.blocks, s_red, &s_b);
+  if (k == 0) {
+    if (rr == 0.0) {
+      if (lead()) {
+        st->rhs_norm = 0.0;
+        st->stop1 = 1;
+      }
+      return;
+    }
+    if (lead()) st->rhs_norm = sqrt(rr);
+  } else {
+    const double relative = sqrt(rr > 0.0 ? rr : 0.0) / st->rhs_norm;
+    if (!isfinite(relative)) {
+      if (lead()) {
+        st->failure1 = MSIM_CG_RESIDUAL;
+        st->stop1 = 1;
+      }
+      return;
+    }
+    if (lead()) {
+      st->iterations = k;
+      st->relative = relative;
+    }
+    if (relative <= static_cast<double>(tolerance)) {
+      if (lead()) st->stop1 = 1;
+      return;
+    }
+    if (last != 0) return;
+    const double rz = all_sum(P.rz(k), P.blocks, s_red, &s_b);
+    const double rho = all_sum(P.rz(k + 1), P.blocks, s_red, &s_b);
+    if (!isfinite(rz) || rho == 0.0) {
+      if (lead()) {
+        st->failure1 = MSIM_CG_BREAKDOWN;
+        st->stop1 = 1;
+      }
+      return;
+    }
+    beta = rz / rho;
+    if (!isfinite(beta)) {
+      if (lead()) {
+        st->failure1 = MSIM_CG_DIRECTION;
+        st->stop1 = 1;
+      }
+      return;
+    }
+  }
+  const bool first = k == 0;
+  const auto dir = [&](uint64_t i) { return first ? z[i] : z[i] + beta * p_old[i]; };
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * V.chunk;
+  const uint64_t end = begin + V.chunk < V.nf ? begin + V.chunk : V.nf;
   double pq = 0.0;
   for (uint64_t f = begin + threadIdx.x; f < end; f += kResBlock) {
+    const double pf = dir(f);
     double v = 0.0;
     if (V.active[f] != 0) {
       const int c = face_component(V, f);
       uint32_t fd[3], co[3];
       face_xyz(V, c, f, fd, co);
-      v = momentum_at(V, p, f, c, fd, co);
+      v = momentum_at(V, dir, f, c, fd, co);
     }
+    p[f] = pf;
     q[f] = v;
-    pq += p[f] * v;
+    pq += pf * v;
   }
   pq = block_sum<kResBlock>(pq, s_red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = pq;
+  if (threadIdx.x == 0) P.pq()[blockIdx.x] = pq;
 }
 
-// x += alpha p, r -= alpha q, z = D^-1 r, and the partials of r.r and r.z.
-__global__ __launch_bounds__(kResBlock) void k_cg_update(ResView V, double* __restrict__ x,
-                                                         double* __restrict__ r,
-                                                         double* __restrict__ z,
-                                                         const double* __restrict__ p,
-                                                         const double* __restrict__ q,
-                                                         double* __restrict__ partials,
-                                                         const ResState* st) {
+// The second half of CG iteration k: alpha = rho / p.q with the curvature and
+// step checks (conjugate_gradient.hpp:57-65), x += alpha p, r -= alpha q,
+// z = D^-1 r, and the partials of r.r and r.z for iteration k + 1.
+__global__ __launch_bounds__(kResBlock) void k_cg_step(ResView V, double* __restrict__ x,
+                                                       double* __restrict__ r,
+                                                       double* __restrict__ z,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ q, CgParts P,
+                                                       uint32_t k, ResState* st) {
   __shared__ double s_red[kResBlock / kWave];
-  if (st->done != kCgRun) return;
-  const double alpha = st->alpha;
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * kResChunk;
-  const uint64_t end = begin + kResChunk < V.nf ? begin + kResChunk : V.nf;
+  __shared__ double s_b;
+  if (st->stop1 != 0) {
+    if (lead()) st->stop2 = 1;
+    return;
+  }
+  const double curvature = all_sum(P.pq(), P.blocks, s_red, &s_b);
+  if (!isfinite(curvature) || curvature <= 0.0) {
+    if (lead()) {
+      st->failure2 = MSIM_CG_CURVATURE;
+      st->stop2 = 1;
+    }
+    return;
+  }
+  const double rho = all_sum(P.rz(k), P.blocks, s_red, &s_b);
+  const double alpha = rho / curvature;
+  if (!isfinite(alpha)) {
+    if (lead()) {
+      st->failure2 = MSIM_CG_STEP;
+      st->stop2 = 1;
+    }
+    return;
+  }
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * V.chunk;
+  const uint64_t end = begin + V.chunk < V.nf ? begin + V.chunk : V.nf;
   double rr = 0.0, rz = 0.0;
   for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
     x[i] += alpha * p[i];
@@ -372,19 +384,9 @@ __global__ __launch_bounds__(kResBlock) void k_cg_update(ResView V, double* __re
   rr = block_sum<kResBlock>(rr, s_red);
   rz = block_sum<kResBlock>(rz, s_red);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = rr;
-    partials[gridDim.x + blockIdx.x] = rz;
+    P.rr(k + 1)[blockIdx.x] = rr;
+    P.rz(k + 1)[blockIdx.x] = rz;
   }
-}
-
-// p = z + beta p.
-__global__ __launch_bounds__(kResBlock) void k_cg_direction(uint64_t n, double* __restrict__ p,
-                                                            const double* __restrict__ z,
-                                                            const ResState* st) {
-  if (st->done != kCgRun) return;
-  const double beta = st->beta;
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
-  if (i < n) p[i] = z[i] + beta * p[i];
 }
 
 // The pressure part of the preconditioner (cpu_flow.cpp:424-431):
@@ -397,29 +399,54 @@ __global__ __launch_bounds__(kResBlock) void k_res_pressure_precondition(
   out[V.nf + i] = pd > 0.0f ? -in[V.nf + i] / (V.cs * static_cast<double>(pd)) : 0.0;
 }
 
-// target += alpha * source with alpha a host value.
-__global__ __launch_bounds__(kResBlock) void k_res_axpy(uint64_t n, double* __restrict__ target,
-                                                        double alpha,
-                                                        const double* __restrict__ source) {
+// x += weights[0] d_0, then += weights[1] d_1, ...: the reference's sequence
+// of axpy updates (flexible_gmres.hpp:97-104), in one pass.
+struct UpdateWeights {
+  double w[kRestart];
+};
+
+__global__ __launch_bounds__(kResBlock) void k_res_update(uint64_t n, double* __restrict__ x,
+                                                          const double* __restrict__ directions,
+                                                          UpdateWeights weights, uint32_t used) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
-  if (i < n) target[i] += alpha * source[i];
+  if (i >= n) return;
+  double v = x[i];
+  for (uint32_t k = 0; k < used; ++k) v += weights.w[k] * directions[k * n + i];
+  x[i] = v;
 }
 
-// One Gram-Schmidt projection (flexible_gmres.hpp:70-76): the dot of the
-// basis vector with w from its partials, kept at `slot`, then w -= it * v.
-__global__ __launch_bounds__(kResBlock) void k_res_project_sum(const double* partials,
-                                                               uint32_t blocks, double* slot) {
+// The one-workgroup sum of a dot product's partials into `slot`.
+__global__ __launch_bounds__(kResBlock) void k_res_sum(const double* partials, uint32_t blocks,
+                                                       double* slot) {
   __shared__ double s_red[kResBlock / kWave];
-  const double v = sum_partials(partials, blocks, s_red);
+  __shared__ double s_b;
+  const double v = all_sum(partials, blocks, s_red, &s_b);
   if (threadIdx.x == 0) *slot = v;
 }
 
-__global__ __launch_bounds__(kResBlock) void k_res_project(uint64_t n, double* __restrict__ w,
-                                                           const double* __restrict__ v,
-                                                           const double* slot) {
-  const double projection = *slot;
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kResBlock + threadIdx.x;
-  if (i < n) w[i] += -projection * v[i];
+// One step of twice-modified Gram-Schmidt (flexible_gmres.hpp:70-76): the
+// projection v.w from the last step's partials (kept at `slot`), w -= it v
+// over this workgroup's chunk, and the partials of the next step's dot,
+// next.w (next = nullptr: w.w), over the same chunk with the updated w.
+__global__ __launch_bounds__(kResBlock) void k_res_mgs(uint64_t n, double* w,
+                                                       const double* __restrict__ v,
+                                                       const double* next, const double* in_parts,
+                                                       uint32_t blocks, uint32_t chunk,
+                                                       double* slot, double* out_parts) {
+  __shared__ double s_red[kResBlock / kWave];
+  __shared__ double s_b;
+  const double projection = all_sum(in_parts, blocks, s_red, &s_b);
+  if (lead()) *slot = projection;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t end = begin + chunk < n ? begin + chunk : n;
+  double acc = 0.0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
+    const double wi = w[i] + -projection * v[i];
+    w[i] = wi;
+    acc += (next != nullptr ? next[i] : wi) * wi;
+  }
+  acc = block_sum<kResBlock>(acc, s_red);
+  if (threadIdx.x == 0) out_parts[blockIdx.x] = acc;
 }
 
 // out = 0 + s * in (a basis vector from w or the residual).
@@ -438,8 +465,17 @@ __global__ __launch_bounds__(kResBlock) void k_res_residual(uint64_t n, double* 
 }
 
 uint32_t grid_of(uint64_t n) { return static_cast<uint32_t>((n + kResBlock - 1) / kResBlock); }
-uint32_t chunks_of(uint64_t n) {
-  return static_cast<uint32_t>(std::max<uint64_t>((n + kResChunk - 1) / kResChunk, 1));
+uint32_t chunks_of(uint64_t n, uint32_t chunk) {
+  return static_cast<uint32_t>(std::max<uint64_t>((n + chunk - 1) / chunk, 1));
+}
+
+// Entries per workgroup of the reducing kernels: one per thread until a
+// launch would need more than 1024 workgroups, then whole multiples of the
+// workgroup, so small systems spread over the chip and large ones keep the
+// partials few.
+uint32_t chunk_for(uint64_t n) {
+  const uint64_t per = (n + kResBlock * 1024ull - 1) / (kResBlock * 1024ull);
+  return kResBlock * static_cast<uint32_t>(std::max<uint64_t>(per, 1));
 }
 
 // The system, built on the host (flow_system.hpp:755-858, 958-1041).
@@ -475,10 +511,13 @@ uint64_t site_host(const ResSystem& s, uint32_t x, uint32_t y, uint32_t z) {
   return (static_cast<uint64_t>(x) * s.dims[1] + y) * s.dims[2] + z;
 }
 
-// The general constructor with axis_flow_boundary (186-201) and no known
-// pressure, then the single-axis forcing (762-781).
+// The general constructor (flow_system.hpp:789-858). axis < 3: with
+// axis_flow_boundary (186-201) and no known pressure, then the single-axis
+// forcing (762-781). axis == 3: the faces `open` flags (port_flow_boundary,
+// 238-253) opened; the forcing, which depends on the pressures held and the
+// known pressure, is formed per solve (port_force).
 ResSystem build_system(const msim_grid& g, const std::vector<uint8_t>& mask, const float* drag,
-                       uint32_t axis) {
+                       uint32_t axis, const std::vector<uint8_t>* open_faces = nullptr) {
   ResSystem s{};
   s.axis = axis;
   s.dims[0] = g.nx;
@@ -525,7 +564,9 @@ ResSystem build_system(const msim_grid& g, const std::vector<uint8_t>& mask, con
     const bool upper_fluid = upper >= 0 && s.fluid[upper] != 0;
     s.exists[index] = lower_fluid || upper_fluid ? 1 : 0;
     const bool exterior = lower < 0 || upper < 0;
-    const bool open = c == static_cast<int>(axis) && (co[axis] == 0 || co[axis] == s.dims[axis]);
+    const bool open = axis < 3 ? c == static_cast<int>(axis) &&
+                                     (co[axis] == 0 || co[axis] == s.dims[axis])
+                               : (*open_faces)[index] != 0;
     const bool act = exterior ? open && (lower_fluid || upper_fluid) : lower_fluid && upper_fluid;
     s.active[index] = act ? 1 : 0;
     if (!act) continue;
@@ -544,18 +585,20 @@ ResSystem build_system(const msim_grid& g, const std::vector<uint8_t>& mask, con
       if (s.dims[a] > 1) s.diag[index] += 2.0f / (s.h[a] * s.h[a]);
     }
   }
-  const float force = 1.0f / ((static_cast<float>(s.dims[axis]) + 1.0f) * s.h[axis]);
-  bool open_inlet = false;
-  for (uint64_t index = s.off[axis]; index < s.off[axis] + s.cnt[axis]; ++index) {
-    if (s.active[index] == 0) continue;
-    int c;
-    uint32_t co[3];
-    face_coords_host(s, index, &c, co);
-    s.force[index] = force;
-    open_inlet = open_inlet || co[axis] == 0;
-  }
-  if (!open_inlet) {
-    throw Error(MSIM_INVALID_ARGUMENT, "the resolved-flow inlet boundary is entirely blocked");
+  if (axis < 3) {
+    const float force = 1.0f / ((static_cast<float>(s.dims[axis]) + 1.0f) * s.h[axis]);
+    bool open_inlet = false;
+    for (uint64_t index = s.off[axis]; index < s.off[axis] + s.cnt[axis]; ++index) {
+      if (s.active[index] == 0) continue;
+      int c;
+      uint32_t co[3];
+      face_coords_host(s, index, &c, co);
+      s.force[index] = force;
+      open_inlet = open_inlet || co[axis] == 0;
+    }
+    if (!open_inlet) {
+      throw Error(MSIM_INVALID_ARGUMENT, "the resolved-flow inlet boundary is entirely blocked");
+    }
   }
   // pressure_diagonal (958-978).
   s.pd.assign(s.n, 0.0f);
@@ -625,8 +668,12 @@ struct ResDevice {
   uint64_t m = 0;
   DeviceBuffer<uint8_t> active, exists, fluid;
   DeviceBuffer<float> face_drag, diag, pd;
-  DeviceBuffer<double> rhs, x, r, w, basis, directions, cg_r, cg_z, cg_p, cg_q, partials, slots;
+  // cg_p holds two direction buffers (iterations alternate between them).
+  DeviceBuffer<double> rhs, x, r, w, basis, directions, cg_r, cg_z, cg_p, cg_q, cg_parts,
+      partials, slots;
   DeviceBuffer<ResState> state;
+  // Iterations the last momentum solve took: the next solve's first batch.
+  uint32_t cg_expect = 16;
 };
 
 template <typename T>
@@ -640,9 +687,9 @@ void upload(Context& ctx, DeviceBuffer<T>& buf, const std::vector<T>& host, cons
 // dot(a, b) of `count` entries into the device scalar `slot`.
 void device_dot(Context& ctx, ResDevice& D, const double* a, const double* b, uint64_t count,
                 double* slot) {
-  const uint32_t blocks = chunks_of(count);
-  k_res_dot<<<blocks, kResBlock, 0, ctx.stream>>>(a, b, count, D.partials.data());
-  k_res_project_sum<<<1, kResBlock, 0, ctx.stream>>>(D.partials.data(), blocks, slot);
+  const uint32_t blocks = chunks_of(count, D.V.chunk);
+  k_res_dot<<<blocks, kResBlock, 0, ctx.stream>>>(a, b, count, D.V.chunk, D.partials.data());
+  k_res_sum<<<1, kResBlock, 0, ctx.stream>>>(D.partials.data(), blocks, slot);
 }
 
 double read_scalar(Context& ctx, const double* slot) {
@@ -654,41 +701,50 @@ double read_scalar(Context& ctx, const double* slot) {
 
 // The momentum block by Jacobi-PCG from zero (conjugate_gradient,
 // cpu_flow.cpp:53-103) on rhs (FP64, rounded to float as the reference
-// rounds it): the solution into `out`. Adds the iterations to `inner`.
+// rounds it): the solution into `out`. Adds the iterations to `inner`. Two
+// launches per iteration. The host reads the state after a first batch as
+// long as the last solve took (plus the launch that tests its last update),
+// then after batches of 8: the solves of one flow take similar counts, so
+// most end in one batch with few launches that find the solve stopped.
 void momentum_cg(Context& ctx, ResDevice& D, const double* rhs, double* out, float tolerance,
                  uint32_t max_iterations, uint64_t& inner) {
   const ResView& V = D.V;
-  const uint32_t blocks = chunks_of(V.nf);
-  double* parts = D.partials.data();
+  const uint32_t blocks = chunks_of(V.nf, V.chunk);
+  const CgParts P{D.cg_parts.data(), blocks};
   ResState* st = D.state.data();
-  k_cg_init<<<blocks, kResBlock, 0, ctx.stream>>>(V, rhs, out, D.cg_r.data(), D.cg_z.data(),
-                                                  D.cg_p.data(), parts);
-  k_cg_begin<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, st);
+  double* const p[2] = {D.cg_p.data(), D.cg_p.data() + V.nf};
+  k_cg_init<<<blocks, kResBlock, 0, ctx.stream>>>(V, rhs, out, D.cg_r.data(), D.cg_z.data(), P,
+                                                  st);
   check(hipGetLastError(), "launch momentum CG");
-  constexpr uint32_t kBatch = 16;
-  uint32_t enqueued = 0;
+  uint32_t k = 0;
+  uint32_t next = std::max(D.cg_expect + 1, 4u);
   ResState s{};
   for (;;) {
-    const uint32_t batch = std::min(kBatch, max_iterations - enqueued);
-    for (uint32_t k = 0; k < batch; ++k) {
-      k_cg_apply<<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_p.data(), D.cg_q.data(), parts, st);
-      k_cg_alpha<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, st);
-      k_cg_update<<<blocks, kResBlock, 0, ctx.stream>>>(V, out, D.cg_r.data(), D.cg_z.data(),
-                                                        D.cg_p.data(), D.cg_q.data(), parts, st);
-      k_cg_beta<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, tolerance, st);
-      k_cg_direction<<<grid_of(V.nf), kResBlock, 0, ctx.stream>>>(V.nf, D.cg_p.data(),
-                                                                  D.cg_z.data(), st);
+    const uint32_t batch = std::min(next, max_iterations - k);
+    next = 8;
+    for (uint32_t i = 0; i < batch; ++i, ++k) {
+      k_cg_direction<<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_z.data(), p[(k + 1) & 1],
+                                                           p[k & 1], D.cg_q.data(), P, k, 0,
+                                                           tolerance, st);
+      k_cg_step<<<blocks, kResBlock, 0, ctx.stream>>>(V, out, D.cg_r.data(), D.cg_z.data(),
+                                                      p[k & 1], D.cg_q.data(), P, k, st);
+    }
+    if (k >= max_iterations) {
+      // The test of the last iteration's update.
+      k_cg_direction<<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_z.data(), p[(k + 1) & 1],
+                                                           p[k & 1], D.cg_q.data(), P, k, 1,
+                                                           tolerance, st);
     }
     check(hipGetLastError(), "launch momentum CG");
-    enqueued += batch;
     check(hipMemcpyAsync(&s, st, sizeof(ResState), hipMemcpyDeviceToHost, ctx.stream),
           "read CG state");
     check(hipStreamSynchronize(ctx.stream), "momentum CG");
-    if (s.done == kCgDone || enqueued >= max_iterations) break;
+    if (s.stop1 != 0 || s.stop2 != 0 || k >= max_iterations) break;
   }
   inner += s.iterations;
+  D.cg_expect = s.iterations;
   const std::string label = "resolved-flow momentum conjugate gradient ";
-  switch (s.failure) {
+  switch (s.failure1 != 0 ? s.failure1 : s.failure2) {
     case MSIM_CG_CURVATURE:
       throw Error(MSIM_CG_CURVATURE, label + "encountered non-positive curvature");
     case MSIM_CG_STEP: throw Error(MSIM_CG_STEP, label + "produced a non-finite step");
@@ -699,7 +755,7 @@ void momentum_cg(Context& ctx, ResDevice& D, const double* rhs, double* out, flo
       throw Error(MSIM_CG_DIRECTION, label + "produced a non-finite direction");
     default: break;
   }
-  if (s.done != kCgDone) {
+  if (s.stop1 == 0 && s.stop2 == 0) {
     throw Error(MSIM_NOT_CONVERGED,
                 label + "did not converge: relative " + std::to_string(s.relative));
   }
@@ -717,13 +773,12 @@ void precondition(Context& ctx, ResDevice& D, const double* in, double* out, flo
   check(hipGetLastError(), "launch k_res_pressure_precondition");
 }
 
-}  // namespace
-
-void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
-                   const float* drag, const msim_resolved_params& params, float* faces_out,
-                   msim_resolved_report& report) {
-  const ResSystem S = build_system(g, mask, drag, params.axis);
-  ResDevice D;
+// The device arrays of a built system, and the buffers of its solves.
+void setup_device(Context& ctx, const ResSystem& S, ResDevice& D) {
+  const uint64_t faces = S.nf;
+  if (faces >= (1ull << 32)) {
+    throw Error(MSIM_INVALID_ARGUMENT, "resolved flow needs fewer than 2^32 faces");
+  }
   D.m = S.nf + S.n;
   ResView& V = D.V;
   for (int a = 0; a < 3; ++a) {
@@ -735,6 +790,7 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
   V.nf = S.nf;
   V.n = S.n;
   V.cs = 1.0 / std::min({S.h[0], S.h[1], S.h[2]});
+  V.chunk = chunk_for(D.m);
   upload(ctx, D.active, S.active, "upload resolved system");
   upload(ctx, D.exists, S.exists, "upload resolved system");
   upload(ctx, D.fluid, S.fluid, "upload resolved system");
@@ -748,30 +804,48 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
   V.diag = D.diag.data();
   V.pd = D.pd.data();
   const uint64_t m = D.m;
-  std::vector<double> rhs_host(m, 0.0);
-  for (uint64_t i = 0; i < S.nf; ++i) rhs_host[i] = static_cast<double>(S.force[i]);
-  upload(ctx, D.rhs, rhs_host, "upload resolved forcing");
-  for (auto* b : {&D.x, &D.r, &D.w}) b->reserve(m);
+  for (auto* b : {&D.rhs, &D.x, &D.r, &D.w}) b->reserve(m);
   D.basis.reserve(static_cast<size_t>(kRestart + 1) * m);
   D.directions.reserve(static_cast<size_t>(kRestart) * m);
-  for (auto* b : {&D.cg_r, &D.cg_z, &D.cg_p, &D.cg_q}) b->reserve(S.nf);
-  D.partials.reserve(2 * static_cast<size_t>(chunks_of(m)));
+  for (auto* b : {&D.cg_r, &D.cg_z, &D.cg_q}) b->reserve(S.nf);
+  D.cg_p.reserve(2 * S.nf);
+  D.cg_parts.reserve(5 * static_cast<size_t>(chunks_of(S.nf, V.chunk)));
+  D.partials.reserve(2 * static_cast<size_t>(chunks_of(m, V.chunk)));
   // slots: [0] the last norm, [1 + 2 * i + pass] the Arnoldi projections.
   D.slots.reserve(2 + 2 * kRestart + 2);
   D.state.reserve(1);
+}
+
+// One block solve (solve_resolved_system, cpu_flow.cpp:379-461): flexible
+// GMRES (flexible_gmres.hpp:33-108) on [A G; cs D] [u; p] = rhs from zero,
+// the solution left in D.x and A applied to it in D.w; rhs (m entries) is
+// uploaded here. Returns the momentum rows' relative residual; adds the
+// device time to *seconds.
+struct BlockSolve {
+  uint32_t outer = 0;
+  uint64_t inner = 0;
+  double relative = 0.0;
+  float momentum_relative = 0.0f;
+};
+
+BlockSolve block_solve(Context& ctx, ResDevice& D, const std::vector<double>& rhs_host,
+                       float relative_tolerance, uint32_t max_outer, float inner_tolerance,
+                       uint32_t max_inner, double* seconds) {
+  const ResView& V = D.V;
+  const uint64_t m = D.m;
+  check(hipMemcpyAsync(D.rhs.data(), rhs_host.data(), m * sizeof(double), hipMemcpyHostToDevice,
+                       ctx.stream),
+        "upload resolved forcing");
   check(hipMemsetAsync(D.x.data(), 0, m * sizeof(double), ctx.stream), "clear solution");
   check(hipEventRecord(ctx.start, ctx.stream), "event");
 
-  const double tolerance = static_cast<double>(params.relative_tolerance);
-  const uint32_t max_outer = params.max_outer_iterations;
-  uint64_t inner = 0;
+  const double tolerance = static_cast<double>(relative_tolerance);
+  BlockSolve out;
   double* slot = D.slots.data();
-  // flexible_gmres (flexible_gmres.hpp:33-108).
   device_dot(ctx, D, D.rhs.data(), D.rhs.data(), m, slot);
   const double rhs_norm = std::sqrt(read_scalar(ctx, slot));
   if (!std::isfinite(rhs_norm)) throw Error(MSIM_RUNTIME_ERROR, "FGMRES non-finite right-hand side");
   uint32_t iterations = 0;
-  double relative = 0.0;
   if (rhs_norm != 0.0) {
     for (;;) {
       apply_system(ctx, D, D.x.data(), D.w.data());
@@ -781,7 +855,7 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
       const double beta = std::sqrt(read_scalar(ctx, slot));
       if (!std::isfinite(beta)) throw Error(MSIM_RUNTIME_ERROR, "FGMRES non-finite true residual");
       if (beta <= tolerance * rhs_norm) {
-        relative = beta / rhs_norm;
+        out.relative = beta / rhs_norm;
         break;
       }
       if (iterations >= max_outer) {
@@ -799,19 +873,28 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
       for (uint32_t j = 0; j < count; ++j) {
         double* direction = directions + static_cast<size_t>(j) * m;
         const double* vj = basis + static_cast<size_t>(j) * m;
-        precondition(ctx, D, vj, direction, params.inner_relative_tolerance,
-                     params.max_inner_iterations, inner);
+        precondition(ctx, D, vj, direction, inner_tolerance, max_inner, out.inner);
         apply_system(ctx, D, direction, D.w.data());
-        // Twice-modified Gram-Schmidt, every scalar on the device.
-        for (uint32_t pass = 0; pass < 2; ++pass) {
-          for (uint32_t i = 0; i <= j; ++i) {
-            const double* vi = basis + static_cast<size_t>(i) * m;
-            double* s = slot + 1 + 2 * i + pass;
-            device_dot(ctx, D, vi, D.w.data(), m, s);
-            k_res_project<<<grid_of(m), kResBlock, 0, ctx.stream>>>(m, D.w.data(), vi, s);
+        // Twice-modified Gram-Schmidt, every scalar on the device: one launch
+        // per projection, each also forming the next dot's partials.
+        {
+          const uint32_t blocks = chunks_of(m, V.chunk);
+          double* parts[2] = {D.partials.data(), D.partials.data() + blocks};
+          k_res_dot<<<blocks, kResBlock, 0, ctx.stream>>>(basis, D.w.data(), m, V.chunk,
+                                                          parts[0]);
+          uint32_t step = 0;
+          for (uint32_t pass = 0; pass < 2; ++pass) {
+            for (uint32_t i = 0; i <= j; ++i, ++step) {
+              const double* vi = basis + static_cast<size_t>(i) * m;
+              const double* next = i < j ? vi + m : (pass == 0 ? basis : nullptr);
+              k_res_mgs<<<blocks, kResBlock, 0, ctx.stream>>>(
+                  m, D.w.data(), vi, next, parts[step & 1], blocks, V.chunk,
+                  slot + 1 + 2 * i + pass, parts[(step + 1) & 1]);
+            }
           }
+          k_res_sum<<<1, kResBlock, 0, ctx.stream>>>(parts[step & 1], blocks, slot);
+          check(hipGetLastError(), "launch Gram-Schmidt");
         }
-        device_dot(ctx, D, D.w.data(), D.w.data(), m, slot);
         std::vector<double> col(2 + 2 * (j + 1));
         check(hipMemcpyAsync(col.data(), slot, col.size() * sizeof(double),
                              hipMemcpyDeviceToHost, ctx.stream),
@@ -852,34 +935,36 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
         for (uint32_t jj = i + 1; jj < used; ++jj) value -= h[i][jj] * weights[jj];
         weights[i] = value / h[i][i];
       }
-      for (uint32_t i = 0; i < used; ++i) {
-        k_res_axpy<<<grid_of(m), kResBlock, 0, ctx.stream>>>(
-            m, D.x.data(), weights[i], directions + static_cast<size_t>(i) * m);
-      }
+      UpdateWeights uw{};
+      for (uint32_t i = 0; i < used; ++i) uw.w[i] = weights[i];
+      k_res_update<<<grid_of(m), kResBlock, 0, ctx.stream>>>(m, D.x.data(), directions, uw, used);
       check(hipGetLastError(), "launch FGMRES update");
     }
   }
-  // The report (cpu_flow.cpp:440-467): momentum residual, divergence.
+  out.outer = iterations;
+  // The momentum rows' residual (cpu_flow.cpp:440-452) from A x in D.w.
   apply_system(ctx, D, D.x.data(), D.w.data());
   check(hipEventRecord(ctx.stop, ctx.stream), "event");
-  std::vector<double> x(m), w(m);
-  check(hipMemcpyAsync(x.data(), D.x.data(), m * sizeof(double), hipMemcpyDeviceToHost,
-                       ctx.stream),
-        "download resolved solution");
-  check(hipMemcpyAsync(w.data(), D.w.data(), m * sizeof(double), hipMemcpyDeviceToHost,
+  std::vector<double> w(D.V.nf);
+  check(hipMemcpyAsync(w.data(), D.w.data(), w.size() * sizeof(double), hipMemcpyDeviceToHost,
                        ctx.stream),
         "download resolved residual");
   check(hipStreamSynchronize(ctx.stream), "resolved flow");
   float ms = 0.0f;
   check(hipEventElapsedTime(&ms, ctx.start, ctx.stop), "event time");
+  *seconds += ms * 1.0e-3;
   double momentum_square = 0.0, force_square = 0.0;
-  for (uint64_t i = 0; i < S.nf; ++i) {
+  for (uint64_t i = 0; i < D.V.nf; ++i) {
     momentum_square += (w[i] - rhs_host[i]) * (w[i] - rhs_host[i]);
     force_square += rhs_host[i] * rhs_host[i];
   }
-  // The divergence of the velocity over the fluid sites: the system's last
-  // n rows divided by the continuity scale.
-  double divergence_square = 0.0;
+  out.momentum_relative = static_cast<float>(std::sqrt(momentum_square / force_square));
+  return out;
+}
+
+// The divergence RMS of a face field over the fluid sites.
+double divergence_rms(const ResSystem& S, const std::vector<double>& u) {
+  double square = 0.0;
   uint64_t fluid_count = 0;
   for (uint64_t site = 0; site < S.n; ++site) {
     if (S.fluid[site] == 0) continue;
@@ -890,14 +975,58 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
     for (int c = 0; c < 3; ++c) {
       uint32_t up[3] = {co[0], co[1], co[2]};
       ++up[c];
-      d += (x[face_host(S, c, up[0], up[1], up[2])] - x[face_host(S, c, co[0], co[1], co[2])]) /
+      d += (u[face_host(S, c, up[0], up[1], up[2])] - u[face_host(S, c, co[0], co[1], co[2])]) /
            static_cast<double>(S.h[c]);
     }
-    divergence_square += d * d;
+    square += d * d;
     ++fluid_count;
   }
-  const double divergence_rms =
-      fluid_count == 0 ? 0.0 : std::sqrt(divergence_square / static_cast<double>(fluid_count));
+  return fluid_count == 0 ? 0.0 : std::sqrt(square / static_cast<double>(fluid_count));
+}
+
+// The forcing of a port system's solve (flow_system.hpp:847-857): on each
+// active face minus the gradient of the known pressure, the pressure held
+// outside an exterior face standing on its missing side.
+std::vector<double> port_force(const ResSystem& S, const std::vector<float>& held,
+                               const std::vector<double>& known) {
+  std::vector<double> rhs(S.nf + S.n, 0.0);
+  for (uint64_t index = 0; index < S.nf; ++index) {
+    if (S.active[index] == 0) continue;
+    int c;
+    uint32_t co[3];
+    face_coords_host(S, index, &c, co);
+    const auto side = [&](bool upper) {
+      if (upper ? co[c] >= S.dims[c] : co[c] == 0) return static_cast<double>(held[index]);
+      uint32_t t[3] = {co[0], co[1], co[2]};
+      if (!upper) --t[c];
+      return known.empty() ? 0.0 : known[site_host(S, t[0], t[1], t[2])];
+    };
+    rhs[index] = static_cast<double>(
+        static_cast<float>(-(side(true) - side(false)) / static_cast<double>(S.h[c])));
+  }
+  return rhs;
+}
+
+}  // namespace
+
+void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
+                   const float* drag, const msim_resolved_params& params, float* faces_out,
+                   msim_resolved_report& report) {
+  const ResSystem S = build_system(g, mask, drag, params.axis);
+  ResDevice D;
+  setup_device(ctx, S, D);
+  std::vector<double> rhs_host(D.m, 0.0);
+  for (uint64_t i = 0; i < S.nf; ++i) rhs_host[i] = static_cast<double>(S.force[i]);
+  double seconds = 0.0;
+  const BlockSolve solved =
+      block_solve(ctx, D, rhs_host, params.relative_tolerance, params.max_outer_iterations,
+                  params.inner_relative_tolerance, params.max_inner_iterations, &seconds);
+  std::vector<double> x(S.nf);
+  check(hipMemcpyAsync(x.data(), D.x.data(), S.nf * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx.stream),
+        "download resolved solution");
+  check(hipStreamSynchronize(ctx.stream), "resolved flow");
+  const double divergence = divergence_rms(S, x);
   // scale_velocity (flow_system.hpp:1368-1415) over the open inlet faces.
   std::vector<float> unscaled(S.nf);
   for (uint64_t i = 0; i < S.nf; ++i) unscaled[i] = static_cast<float>(x[i]);
@@ -932,15 +1061,166 @@ void resolved_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>&
     faces_out[i] = v;
     scaled_peak = std::max(scaled_peak, std::abs(v));
   }
-  report.outer_iterations = iterations;
-  report.inner_iterations = inner;
-  report.relative_residual = static_cast<float>(relative);
-  report.momentum_relative_residual = static_cast<float>(std::sqrt(momentum_square / force_square));
-  report.divergence_rms = static_cast<float>(divergence_rms * std::abs(factor));
+  report.outer_iterations = solved.outer;
+  report.inner_iterations = solved.inner;
+  report.relative_residual = static_cast<float>(solved.relative);
+  report.momentum_relative_residual = solved.momentum_relative;
+  report.divergence_rms = static_cast<float>(divergence * std::abs(factor));
   report.mean_inlet_speed = params.mean_inlet_speed;
   report.max_speed = scaled_peak;
   report.min_gap_voxels = min_gap(S);
-  report.seconds = ms * 1.0e-3;
+  report.seconds = seconds;
+}
+
+void resolved_port_flow(Context& ctx, const msim_grid& g, const std::vector<uint8_t>& mask,
+                        const float* drag, const msim_flow_port* ports, uint32_t count,
+                        const msim_resolved_port_params& params, const ShallowPressure& shallow,
+                        float* faces_out, msim_resolved_port_report& report,
+                        msim_port_state* states) {
+  std::memset(&report, 0, sizeof(report));
+  const uint32_t dims[3] = {g.nx, g.ny, g.nz};
+  const float spacing[3] = {g.dx, g.dy, g.dz};
+  const uint64_t nf = static_cast<uint64_t>(g.nx + 1) * g.ny * g.nz +
+                      static_cast<uint64_t>(g.nx) * (g.ny + 1) * g.nz +
+                      static_cast<uint64_t>(g.nx) * g.ny * (g.nz + 1);
+  // Each port's faces (for_each_port_face, flow_system.hpp:221-236): all of
+  // them open (port_flow_boundary); those onto fluid sites carry its flux
+  // (make_port_faces, 1124-1130).
+  std::vector<uint8_t> open(nf, 0);
+  std::vector<std::vector<uint64_t>> all_faces(count), fluid_faces(count);
+  std::vector<double> signed_area(count);
+  for (uint32_t p = 0; p < count; ++p) {
+    const PortGeometry geo = port_geometry(ports[p].face);
+    for_each_port_face(dims, ports[p], [&](size_t face, size_t site) {
+      open[face] = 1;
+      all_faces[p].push_back(face);
+      if (mask[site] == 0) fluid_faces[p].push_back(face);
+    });
+    const double area = static_cast<double>(spacing[geo.in_plane[0]]) * spacing[geo.in_plane[1]];
+    signed_area[p] = geo.upper ? -area : area;
+  }
+  // The system's arrays do not depend on the pressures held: built once.
+  const ResSystem S = build_system(g, mask, drag, 3, &open);
+  ResDevice D;
+  setup_device(ctx, S, D);
+  double seconds = 0.0;
+  std::vector<float> held_faces(nf, 0.0f);
+  std::vector<double> x(S.nf);
+  // One block solve holding each port at pressures[port]: its field, as
+  // solved, rounded to float (superpose_port_flows' solve).
+  auto solve = [&](const std::vector<float>& pressures) {
+    for (uint32_t p = 0; p < count; ++p) {
+      for (const uint64_t face : all_faces[p]) held_faces[face] = pressures[p];
+    }
+    std::vector<double> known;
+    shallow(pressures, known);
+    const std::vector<double> rhs = port_force(S, held_faces, known);
+    const BlockSolve solved = block_solve(
+        ctx, D, rhs, params.relative_tolerance, params.max_outer_iterations,
+        params.inner_relative_tolerance, params.max_inner_iterations, &seconds);
+    report.outer_iterations += solved.outer;
+    report.inner_iterations += solved.inner;
+    report.relative_residual =
+        std::max(report.relative_residual, static_cast<float>(solved.relative));
+    report.momentum_relative_residual =
+        std::max(report.momentum_relative_residual, solved.momentum_relative);
+    ++report.solves;
+    check(hipMemcpyAsync(x.data(), D.x.data(), S.nf * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx.stream),
+          "download resolved solution");
+    check(hipStreamSynchronize(ctx.stream), "resolved port flow");
+    std::vector<float> field(S.nf);
+    for (uint64_t f = 0; f < S.nf; ++f) field[f] = static_cast<float>(x[f]);
+    return field;
+  };
+  // superpose_port_flows (flow_system.hpp:1146-1252).
+  auto flux = [&](uint32_t q, const auto& velocity) {
+    double sum = 0.0;
+    for (const uint64_t face : fluid_faces[q]) sum += static_cast<double>(velocity[face]);
+    return sum * signed_area[q];
+  };
+  auto open_area = [&](uint32_t q) {
+    return std::abs(signed_area[q]) * static_cast<double>(fluid_faces[q].size());
+  };
+  std::vector<uint32_t> driven;
+  std::vector<float> held(count, 0.0f);
+  bool holds_pressure = false;
+  for (uint32_t p = 0; p < count; ++p) {
+    if (ports[p].kind == MSIM_PORT_SPEED) {
+      driven.push_back(p);
+    } else {
+      held[p] = ports[p].value;
+      holds_pressure = holds_pressure || ports[p].value != 0.0f;
+    }
+  }
+  std::vector<double> velocity;
+  auto add = [&](const std::vector<float>& field, double weight) {
+    if (velocity.empty()) velocity.assign(field.size(), 0.0);
+    for (size_t f = 0; f < field.size(); ++f) velocity[f] += weight * static_cast<double>(field[f]);
+  };
+  if (holds_pressure) add(solve(held), 1.0);
+  std::vector<float> pressures = held;
+  if (!driven.empty()) {
+    const size_t k = driven.size();
+    std::vector<std::vector<float>> fields;
+    std::vector<std::vector<double>> response(k, std::vector<double>(k + 1, 0.0));
+    for (size_t j = 0; j < k; ++j) {
+      std::vector<float> unit(count, 0.0f);
+      unit[driven[j]] = 1.0f;
+      fields.push_back(solve(unit));
+      for (size_t m = 0; m < k; ++m) response[m][j] = flux(driven[m], fields.back());
+    }
+    for (size_t m = 0; m < k; ++m) {
+      const double base = velocity.empty() ? 0.0 : flux(driven[m], velocity);
+      response[m][k] = static_cast<double>(ports[driven[m]].value) * open_area(driven[m]) - base;
+    }
+    // Gaussian elimination with partial pivoting on the augmented matrix.
+    for (size_t column = 0; column < k; ++column) {
+      size_t pivot = column;
+      for (size_t row = column + 1; row < k; ++row) {
+        if (std::abs(response[row][column]) > std::abs(response[pivot][column])) pivot = row;
+      }
+      if (response[pivot][column] == 0.0) {
+        throw Error(MSIM_RUNTIME_ERROR, "a speed port carries no flow: no pressure port is reachable");
+      }
+      std::swap(response[column], response[pivot]);
+      for (size_t row = 0; row < k; ++row) {
+        if (row == column) continue;
+        const double factor = response[row][column] / response[column][column];
+        for (size_t entry = column; entry <= k; ++entry) {
+          response[row][entry] -= factor * response[column][entry];
+        }
+      }
+    }
+    for (size_t j = 0; j < k; ++j) {
+      const double pressure = response[j][k] / response[j][j];
+      pressures[driven[j]] = static_cast<float>(pressure);
+      add(fields[j], pressure);
+    }
+  }
+  float max_speed = 0.0f;
+  std::vector<double> combined(S.nf);
+  for (uint64_t f = 0; f < S.nf; ++f) {
+    const float v = static_cast<float>(velocity[f]);
+    if (!std::isfinite(v)) throw Error(MSIM_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+    faces_out[f] = v;
+    combined[f] = static_cast<double>(v);
+    max_speed = std::max(max_speed, std::abs(v));
+  }
+  if (states != nullptr) {
+    for (uint32_t p = 0; p < count; ++p) {
+      const double f = flux(p, velocity);
+      states[p].pressure = pressures[p];
+      states[p].flux = static_cast<float>(f);
+      states[p].mean_speed = static_cast<float>(f / open_area(p));
+    }
+  }
+  // The divergence and gaps of the combined field do not depend on the
+  // pressures held (flow_system.hpp:1523-1540).
+  report.divergence_rms = static_cast<float>(divergence_rms(S, combined));
+  report.max_speed = max_speed;
+  report.min_gap_voxels = min_gap(S);
+  report.seconds = seconds;
 }
 
 }  // namespace msim

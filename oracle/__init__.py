@@ -94,6 +94,19 @@ class _PortState(C.Structure):
     _fields_ = [("pressure", C.c_float), ("flux", C.c_float), ("mean_speed", C.c_float)]
 
 
+class _ResolvedPortReport(C.Structure):
+    _fields_ = [("solves", C.c_uint32), ("outer_iterations", C.c_uint32),
+                ("inner_iterations", C.c_uint64), ("relative_residual", C.c_float),
+                ("momentum_relative_residual", C.c_float), ("divergence_rms", C.c_float),
+                ("max_speed", C.c_float), ("min_gap_voxels", C.c_uint32),
+                ("shifted_solves", C.c_uint32)]
+
+
+RESOLVED_PORT_REPORT_FIELDS = ("solves", "outer_iterations", "inner_iterations",
+                               "relative_residual", "momentum_relative_residual",
+                               "divergence_rms", "max_speed", "min_gap_voxels")
+
+
 class _PortReport(C.Structure):
     _fields_ = [("solves", C.c_uint32), ("iterations", C.c_uint32),
                 ("relative_residual", C.c_float), ("max_speed", C.c_float)]
@@ -232,6 +245,8 @@ class Oracle:
         L.mo_adr_step_implicit_f32.restype = C.c_int
         L.mo_resolved_flow_solve.argtypes = [P, P, U, F, F, U, F, U, C.c_int, P, P, C.c_char_p, Z]
         L.mo_resolved_flow_solve.restype = C.c_int
+        L.mo_resolved_port_flow_solve.argtypes = [P, P, P, U, F, U, F, U, P, P, P, C.c_char_p, Z]
+        L.mo_resolved_port_flow_solve.restype = C.c_int
         L.mo_resolved_operator.argtypes = [P, P, U, P, P, P, P, P, C.c_char_p, Z]
         L.mo_resolved_operator.restype = C.c_int
         for fn in (L.mo_flow_assemble, L.mo_flow_apply, L.mo_flow_solve, L.mo_flow_solve_forced,
@@ -335,6 +350,32 @@ class Oracle:
                                            du.ctypes.data, err, C.c_size_t(512))
         self._check(st, err)
         return au, gp, du
+
+    def resolved_port_flow(self, grid: Grid, drag, ports, tolerance: float = 1e-6,
+                           max_outer: int = 5000, inner_tolerance: float = 1e-6,
+                           max_inner: int = 50000):
+        """Port-driven resolved flow (solve_resolved_port_flow_cpu): faces
+        (x|y|z), port states (pressure, flux, mean speed per port) and the
+        report as a dict (with the solves the depth-averaged pressure
+        shifted)."""
+        keep: list = []
+        g = grid._c(keep)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        states = (_PortState * max(1, len(ports)))()
+        rep = _ResolvedPortReport()
+        d = _f32(drag)
+        arr = _ports(ports)
+        err = C.create_string_buffer(512)
+        st = self.lib.mo_resolved_port_flow_solve(
+            C.byref(g), _ptr(d), C.cast(arr, C.c_void_p), C.c_uint32(len(ports)),
+            C.c_float(tolerance), C.c_uint32(max_outer), C.c_float(inner_tolerance),
+            C.c_uint32(max_inner), faces.ctypes.data, C.cast(states, C.c_void_p), C.byref(rep),
+            err, C.c_size_t(512))
+        self._check(st, err)
+        out = np.array([[s.pressure, s.flux, s.mean_speed] for s in states[:len(ports)]],
+                       np.float32).reshape(len(ports), 3)
+        report = {k: getattr(rep, k) for k in RESOLVED_PORT_REPORT_FIELDS + ("shifted_solves",)}
+        return faces, out, report
 
     def port_flow(self, grid: Grid, mobility, ports, tolerance: float = 1e-6,
                   max_iterations: int = 50000):
@@ -521,6 +562,9 @@ class Reference:
         L.ref_set_threads.argtypes = [U]
         L.ref_solve_resolved_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, F, U, P, P,
                                               P, P, P, C.c_char_p, Z]
+        L.ref_resolved_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, F, U, P,
+                                             P, P, P, C.c_char_p, Z]
+        L.ref_resolved_port_flow.restype = C.c_int
         L.ref_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, U, P, P, P, P, P, P,
                                     C.c_char_p, Z]
         L.ref_advance_coupled.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, P, P, P, P, P, U, U,
@@ -594,6 +638,36 @@ class Reference:
                       divergence_rms=float(values[2]), mean_inlet_speed=float(values[3]),
                       max_speed=float(values[4]), min_gap_voxels=gap.value)
         return faces, report
+
+    def resolved_port_flow(self, grid: Grid, drag, ports, tolerance=1e-6, max_outer=5000,
+                           inner_tolerance=1e-6, max_inner=50000):
+        """solve_resolved_port_flow_cpu: faces, port states, report."""
+        kinds = self._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        d = _f32(drag)
+        geometry = np.array([[f, b[0], b[1], e[0], e[1], k] for f, b, e, k, _ in ports],
+                            np.uint32).reshape(-1)
+        values = np.array([v for *_, v in ports], np.float32)
+        faces = np.zeros(sum(grid.face_counts()), np.float32)
+        states = np.zeros(3 * max(1, len(ports)), np.float32)
+        counts = np.zeros(4, np.uint64)
+        report_values = np.zeros(4, np.float32)
+        err = C.create_string_buffer(512)
+        st = self.lib.ref_resolved_port_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(d),
+            C.c_uint32(len(ports)), _ptr(geometry), _ptr(values), C.c_float(tolerance),
+            C.c_uint32(max_outer), C.c_float(inner_tolerance), C.c_uint32(max_inner),
+            faces.ctypes.data, states.ctypes.data, counts.ctypes.data,
+            report_values.ctypes.data, err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        report = dict(solves=int(counts[0]), outer_iterations=int(counts[1]),
+                      inner_iterations=int(counts[2]), min_gap_voxels=int(counts[3]),
+                      relative_residual=float(report_values[0]),
+                      momentum_relative_residual=float(report_values[1]),
+                      divergence_rms=float(report_values[2]), max_speed=float(report_values[3]))
+        return faces, states[:3 * len(ports)].reshape(len(ports), 3), report
 
     def port_flow(self, grid: Grid, mobility, ports, tolerance=1e-6, max_iterations=50000):
         """solve_depth_averaged_port_flow_cpu: faces, port states, report."""
@@ -750,6 +824,8 @@ class Binding:
                                          Z]
         L.bind_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, F, F, U, P, Z, P, P, P,
                                 C.c_char_p, Z]
+        L.bind_resolved_port_flow.argtypes = [U, U, U, F, F, F, P, P, P, U, P, P, F, P, P, P,
+                                              P, P, C.c_char_p, Z]
         L.bind_signal_steps.argtypes = [U, U, U, F, F, F, P, P, U, P, P, P, P, P, P, P, P, U, U,
                                         P, P, U, U, F, F, F, U, P, P, P, C.c_char_p, Z]
 
@@ -821,6 +897,36 @@ class Binding:
         keys = ("outer_iterations", "inner_iterations", "relative_residual", "divergence_rms",
                 "max_speed", "min_gap_voxels")
         return a, b, [dict(zip(keys, report[6 * k:6 * k + 6].tolist())) for k in range(2)]
+
+    def resolved_port_flow(self, grid: Grid, drag, ports, tolerance=1e-6):
+        """solve_resolved_port_flow through the hip backend and through
+        solve_resolved_port_flow_cpu: (faces_hip, faces_cpu, states_hip,
+        states_cpu, reports)."""
+        kinds = Reference._kinds(grid)
+        obs = None if grid.obstacles is None else np.ascontiguousarray(grid.obstacles, np.uint8)
+        d = _f32(drag)
+        geometry = np.array([[f, b[0], b[1], e[0], e[1], k] for f, b, e, k, _ in ports],
+                            np.uint32).reshape(-1)
+        values = np.array([v for *_, v in ports], np.float32)
+        total = sum(grid.face_counts())
+        a = np.zeros(total, np.float32)
+        b = np.zeros(total, np.float32)
+        sa = np.zeros(3 * len(ports), np.float32)
+        sb = np.zeros(3 * len(ports), np.float32)
+        report = np.zeros(14, np.float64)
+        err = C.create_string_buffer(512)
+        st = self.lib.bind_resolved_port_flow(
+            C.c_uint32(grid.nx), C.c_uint32(grid.ny), C.c_uint32(grid.nz), C.c_float(grid.dx),
+            C.c_float(grid.dy), C.c_float(grid.dz), _ptr(obs), kinds.ctypes.data, _ptr(d),
+            C.c_uint32(len(ports)), _ptr(geometry), _ptr(values), C.c_float(tolerance),
+            a.ctypes.data, b.ctypes.data, sa.ctypes.data, sb.ctypes.data, report.ctypes.data,
+            err, C.c_size_t(512))
+        if st != 0:
+            raise OracleError(st, err.value.decode(errors="replace"))
+        keys = ("solves", "outer_iterations", "inner_iterations", "relative_residual",
+                "divergence_rms", "max_speed", "min_gap_voxels")
+        return (a, b, sa.reshape(-1, 3), sb.reshape(-1, 3),
+                [dict(zip(keys, report[7 * k:7 * k + 7].tolist())) for k in range(2)])
 
     def signal_steps(self, grid: Grid, levels, dt, steps=1, **kw):
         """advance_signal_grid `steps` times through the hip backend and through

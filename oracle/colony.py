@@ -131,3 +131,62 @@ def colony_mobility(dims, origin, spacing, obstacles, cells, base=1.0, drag_coef
     drag = drag_coefficient * column_fraction * column_fraction / (1.0 - column_fraction) ** 3
     mobility = base_grid / (1.0 + base_grid * drag[:, :, None])
     return np.where(fluid, mobility, 0.0).reshape(-1)
+
+
+def colony_flow_coupling(dims, origin, spacing, obstacles, cells, velocities, base=1.0,
+                         drag_coefficient=100.0, max_volume_fraction=0.9, averaging_radius=4.0):
+    """colony_flow_coupling (flow.py:384-424) with _darcy_coefficients
+    (359-382): the colony's mobility (as colony_mobility) and the in-plane
+    body force per column, momentum * drag / mass, where the momentum is each
+    cell's deposited biomass per column times its in-plane velocity and the
+    mass the column's deposited biomass volume (0 where the column has none).
+    velocities: (cells, 2) in-plane (x, y). Returns (mobility N, force
+    (columns, 2))."""
+    if not math.isfinite(averaging_radius) or averaging_radius <= 0:
+        raise ColonyError("averaging radius must be finite and positive")
+    dims = tuple(int(d) for d in dims)
+    origin = tuple(float(np.float32(o)) for o in origin)
+    spacing = tuple(float(np.float32(h)) for h in spacing)
+    voxel = float(np.float32(np.float32(spacing[0]) * np.float32(spacing[1]))
+                  * np.float32(spacing[2]))
+    volume = np.zeros(dims, dtype=np.float64)
+    momentum = np.zeros((dims[0], dims[1], 2), dtype=np.float64)
+    solid = None if obstacles is None else np.asarray(obstacles, bool).reshape(dims)
+    velocities = np.asarray(velocities, np.float32).reshape(-1, 2)
+    for c in range(len(cells["lengths"])):
+        vx, vy = float(velocities[c, 0]), float(velocities[c, 1])
+        if not math.isfinite(vx) or not math.isfinite(vy):
+            raise ColonyError("in-plane cell velocity must be finite")
+        position = tuple(float(cells[k][c]) for k in ("position_x", "position_y", "position_z"))
+        stencil = deposition_weights(dims, origin, spacing, position, averaging_radius, solid)
+        if stencil is None:
+            continue
+        lows, weights = stencil
+        region = tuple(slice(lo, lo + w) for lo, w in zip(lows, weights.shape))
+        mass = biomass_volume(float(cells["lengths"][c]), float(cells["radii"][c])) * weights
+        volume[region] += mass
+        momentum[region[:2]] += mass.sum(axis=2)[:, :, None] * (vx, vy)
+    density = volume / voxel
+    if not 0 < max_volume_fraction < 1:
+        raise ColonyError("maximum volume fraction must lie strictly between zero and one")
+    if np.ndim(base) == 0:
+        base = float(base)
+        if not math.isfinite(base) or base <= 0.0:
+            raise ColonyError("base mobility must be finite and positive")
+        base_grid = np.full(dims, base)
+    else:
+        if len(base) != dims[0] * dims[1] * dims[2]:
+            raise ColonyError("base mobility must hold one value per grid site")
+        base_grid = np.asarray(base, dtype=np.float64).reshape(dims)
+        if not bool(np.all(np.isfinite(base_grid))) or bool(np.any(base_grid < 0.0)):
+            raise ColonyError("base mobility values must be finite and non-negative")
+    fluid = np.ones(dims, bool) if solid is None else ~solid
+    depth_count = np.maximum(fluid.sum(axis=2), 1)
+    column_fraction = np.minimum(density.sum(axis=2) / depth_count, max_volume_fraction)
+    if not math.isfinite(drag_coefficient) or drag_coefficient < 0.0:
+        raise ColonyError("drag coefficient must be finite and non-negative")
+    drag = drag_coefficient * column_fraction * column_fraction / (1.0 - column_fraction) ** 3
+    mobility = np.where(fluid, base_grid / (1.0 + base_grid * drag[:, :, None]), 0.0)
+    mass = volume.sum(axis=2)
+    force = momentum * np.divide(drag, mass, out=np.zeros_like(drag), where=mass > 0)[:, :, None]
+    return mobility.reshape(-1), force.reshape(-1, 2)

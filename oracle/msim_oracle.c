@@ -2639,11 +2639,16 @@ static long long neighbor_face(const resolved_t* s, size_t index, int axis, int 
   return (long long)rface(s, c, f[0], f[1], f[2]);
 }
 
-/* ResolvedFlowSystem (flow_system.hpp:755-858): the general constructor with
- * axis_flow_boundary (186-201: the lower end open at pressure 1, the upper at
- * 0) and no known pressure, then the single-axis forcing (762-781). */
-static int resolved_build(const mo_grid* g, const float* drag, uint32_t axis, resolved_t* s,
-                          char* err, size_t errlen) {
+/* ResolvedFlowSystem (flow_system.hpp:755-858). axis < 3: the general
+ * constructor with axis_flow_boundary (186-201: the lower end open at
+ * pressure 1, the upper at 0) and no known pressure, then the single-axis
+ * forcing (762-781). axis == 3: the general constructor with the boundary
+ * `open`/`held` (per face: open, and the pressure held outside it) and the
+ * known pressure `known` (one per site, or NULL). */
+static int resolved_build_general(const mo_grid* g, const float* drag, uint32_t axis,
+                                  const uint8_t* open_faces, const float* held_pressure,
+                                  const double* known, resolved_t* s, char* err,
+                                  size_t errlen) {
   memset(s, 0, sizeof(*s));
   s->g = g;
   s->axis = axis;
@@ -2685,8 +2690,15 @@ static int resolved_build(const mo_grid* g, const float* drag, uint32_t axis, re
     const int exterior = lower < 0 || upper < 0;
     /* axis_flow_boundary: the flow axis's exterior faces are open, the lower
      * end at pressure 1, the upper at 0. */
-    const int open = c == (int)axis && (f[axis] == 0 || f[axis] == s->dims[axis]);
-    const float held = (c == (int)axis && f[axis] == 0) ? 1.0f : 0.0f;
+    int open;
+    float held;
+    if (axis < 3) {
+      open = c == (int)axis && (f[axis] == 0 || f[axis] == s->dims[axis]);
+      held = (c == (int)axis && f[axis] == 0) ? 1.0f : 0.0f;
+    } else {
+      open = open_faces[index] != 0;
+      held = held_pressure[index];
+    }
     const int act = exterior ? (open && (lower_fluid || upper_fluid)) : (lower_fluid && upper_fluid);
     s->active[index] = act ? 1 : 0;
     if (!act) continue;
@@ -2703,10 +2715,13 @@ static int resolved_build(const mo_grid* g, const float* drag, uint32_t axis, re
     s->diag[index] = s->face_drag[index];
     for (int a = 0; a < 3; ++a)
       if (s->dims[a] > 1) s->diag[index] += 2.0f / (s->h[a] * s->h[a]);
-    const double ku = upper < 0 ? (double)held : 0.0;
-    const double kl = lower < 0 ? (double)held : 0.0;
+    /* The gradient of the known pressure, the boundary's pressure standing
+     * on the missing side of an exterior face. */
+    const double ku = upper < 0 ? (double)held : (known != NULL ? known[upper] : 0.0);
+    const double kl = lower < 0 ? (double)held : (known != NULL ? known[lower] : 0.0);
     s->force[index] = (float)(-(ku - kl) / (double)s->h[c]);
   }
+  if (axis >= 3) return MO_OK;
   /* The single-axis system (762-781): force 1 / ((n + 1) h) on the axis's
    * active faces, zero elsewhere; an open inlet face must exist. */
   const float force = 1.0f / (((float)s->dims[axis] + 1.0f) * s->h[axis]);
@@ -2722,6 +2737,11 @@ static int resolved_build(const mo_grid* g, const float* drag, uint32_t axis, re
   if (!open_inlet)
     return fail(err, errlen, MO_INVALID_ARGUMENT, "the resolved-flow inlet boundary is entirely blocked");
   return MO_OK;
+}
+
+static int resolved_build(const mo_grid* g, const float* drag, uint32_t axis, resolved_t* s,
+                          char* err, size_t errlen) {
+  return resolved_build_general(g, drag, axis, NULL, NULL, NULL, s, err, errlen);
 }
 
 /* apply_momentum (flow_system.hpp:881-910). */
@@ -3077,6 +3097,53 @@ done:
   return status;
 }
 
+/* solve_resolved_system (cpu_flow.cpp:379-461) on a built system: FGMRES on
+ * [A G; cs D] [u; p] = [force; 0] from zero, preconditioned by the momentum
+ * block's Jacobi-PCG and the scaled pressure diagonal; solution [u; p] (nf +
+ * n), the outer and inner iterations, the relative residual, and the
+ * momentum rows' relative residual. */
+static int resolved_system_solve(const resolved_t* s, float tolerance, uint32_t max_outer,
+                                 float inner_tolerance, uint32_t max_inner, double* solution,
+                                 uint32_t* outer, uint64_t* inner, double* relative,
+                                 float* momentum_relative, char* err, size_t errlen) {
+  const size_t m = s->nf + s->n;
+  float hmin = s->h[0] < s->h[1] ? s->h[0] : s->h[1];
+  if (s->h[2] < hmin) hmin = s->h[2];
+  resolved_ops_t o;
+  o.s = s;
+  o.continuity_scale = 1.0 / hmin;
+  o.pd = malloc(s->n * sizeof(float));
+  o.inner_tolerance = inner_tolerance;
+  o.max_inner = max_inner;
+  o.inner = 0;
+  o.work_u = malloc(s->nf * sizeof(double));
+  o.work_g = malloc(s->nf * sizeof(double));
+  o.work_d = malloc(s->n * sizeof(double));
+  double* rhs = calloc(m, sizeof(double));
+  double* residual = calloc(m, sizeof(double));
+  resolved_pressure_diagonal(s, o.pd);
+  for (size_t i = 0; i < s->nf; ++i) rhs[i] = (double)s->force[i];
+  int status = resolved_fgmres(&o, rhs, (double)tolerance, max_outer, solution, outer, relative,
+                               err, errlen);
+  *inner = o.inner;
+  if (status == MO_OK) {
+    resolved_apply(&o, solution, residual);
+    double momentum_square = 0.0, force_square = 0.0;
+    for (size_t i = 0; i < s->nf; ++i) {
+      momentum_square += (residual[i] - rhs[i]) * (residual[i] - rhs[i]);
+      force_square += rhs[i] * rhs[i];
+    }
+    *momentum_relative = (float)sqrt(momentum_square / force_square);
+  }
+  free(o.pd);
+  free(o.work_u);
+  free(o.work_g);
+  free(o.work_d);
+  free(rhs);
+  free(residual);
+  return status;
+}
+
 int mo_resolved_flow_solve(const mo_grid* g, const float* drag, uint32_t axis,
                            float mean_inlet_speed, float tolerance, uint32_t max_outer,
                            float inner_tolerance, uint32_t max_inner, int scale, float* faces,
@@ -3105,41 +3172,22 @@ int mo_resolved_flow_solve(const mo_grid* g, const float* drag, uint32_t axis,
     return status;
   }
   const size_t m = s.nf + s.n;
-  float hmin = s.h[0] < s.h[1] ? s.h[0] : s.h[1];
-  if (s.h[2] < hmin) hmin = s.h[2];
-  resolved_ops_t o;
-  o.s = &s;
-  o.continuity_scale = 1.0 / hmin;
-  o.pd = malloc(s.n * sizeof(float));
-  o.inner_tolerance = inner_tolerance;
-  o.max_inner = max_inner;
-  o.inner = 0;
-  o.work_u = malloc(s.nf * sizeof(double));
-  o.work_g = malloc(s.nf * sizeof(double));
-  o.work_d = malloc(s.n * sizeof(double));
-  double* rhs = calloc(m, sizeof(double));
   double* solution = calloc(m, sizeof(double));
-  double* residual = calloc(m, sizeof(double));
+  double* divergence = calloc(s.n, sizeof(double));
   float* unscaled = malloc(s.nf * sizeof(float));
-  resolved_pressure_diagonal(&s, o.pd);
-  for (size_t i = 0; i < s.nf; ++i) rhs[i] = (double)s.force[i];
   uint32_t outer = 0;
+  uint64_t inner = 0;
   double relative = 0.0;
-  status = resolved_fgmres(&o, rhs, (double)tolerance, max_outer, solution, &outer, &relative, err,
-                           errlen);
+  float momentum_relative = 0.0f;
+  status = resolved_system_solve(&s, tolerance, max_outer, inner_tolerance, max_inner, solution,
+                                 &outer, &inner, &relative, &momentum_relative, err, errlen);
   if (status == MO_OK) {
-    resolved_apply(&o, solution, residual);
-    double momentum_square = 0.0, force_square = 0.0;
-    for (size_t i = 0; i < s.nf; ++i) {
-      momentum_square += (residual[i] - rhs[i]) * (residual[i] - rhs[i]);
-      force_square += rhs[i] * rhs[i];
-    }
-    resolved_divergence(&s, solution, o.work_d);
+    resolved_divergence(&s, solution, divergence);
     double divergence_square = 0.0;
     size_t fluid_count = 0;
     for (size_t site = 0; site < s.n; ++site) {
       if (s.fluid[site]) {
-        divergence_square += o.work_d[site] * o.work_d[site];
+        divergence_square += divergence[site] * divergence[site];
         ++fluid_count;
       }
     }
@@ -3187,9 +3235,9 @@ int mo_resolved_flow_solve(const mo_grid* g, const float* drag, uint32_t axis,
     }
     if (status == MO_OK && report != NULL) {
       report->outer_iterations = outer;
-      report->inner_iterations = o.inner;
+      report->inner_iterations = inner;
       report->relative_residual = (float)relative;
-      report->momentum_relative_residual = (float)sqrt(momentum_square / force_square);
+      report->momentum_relative_residual = momentum_relative;
       report->divergence_rms = (float)(divergence_rms * fabsf(factor));
       report->mean_inlet_speed = mean_inlet_speed;
       report->max_speed = scaled_peak;
@@ -3198,15 +3246,362 @@ int mo_resolved_flow_solve(const mo_grid* g, const float* drag, uint32_t axis,
       report->factor = factor;
     }
   }
-  free(o.pd);
-  free(o.work_u);
-  free(o.work_g);
-  free(o.work_d);
-  free(rhs);
   free(solution);
-  free(residual);
+  free(divergence);
   free(unscaled);
   resolved_free(&s);
+  return status;
+}
+
+/* shallow_port_pressure (flow_system.hpp:1318-1338): the depth-averaged
+ * pressure that holds each port at pressures[port], at every site (each
+ * column's pressure on all its voxels), from the reduction with the default
+ * mobility, the ports on the grid of columns and a Jacobi-PCG solve to 1e-6
+ * in at most 50000 iterations. *have = 0 where shallow flow does not describe
+ * the geometry (a refusal of the reduction, of a reduced port or of the
+ * system: the reference's invalid_argument); other failures are returned. */
+static int shallow_port_pressure(const mo_grid* g, const mo_flow_port* ports, uint32_t count,
+                                 const float* pressures, double* known, int* have, char* err,
+                                 size_t errlen) {
+  *have = 0;
+  system_t s;
+  int status = reduce_columns(g, NULL, &s, err, errlen);
+  if (status != MO_OK) {
+    system_free(&s);
+    return status == MO_INVALID_ARGUMENT ? MO_OK : status;
+  }
+  const uint32_t nz = g->nz;
+  const uint32_t dims[3] = {g->nx, g->ny, g->nz};
+  size_t largest = 1;
+  for (uint32_t p = 0; p < count; ++p)
+    if (port_face_count(&ports[p]) > largest) largest = port_face_count(&ports[p]);
+  mo_flow_port* reduced = malloc((count + 1) * sizeof(mo_flow_port));
+  size_t* faces = malloc(largest * sizeof(size_t));
+  size_t* sites = malloc(largest * sizeof(size_t));
+  uint8_t* open = calloc(s.faces, 1);
+  float* held = calloc(s.faces, sizeof(float));
+  double* columns = calloc(s.n, sizeof(double));
+  float* lifted = malloc((total_faces(g) + 1) * sizeof(float));
+  int shallow = 1;
+  if (!reduced || !faces || !sites || !open || !held || !columns || !lifted) {
+    status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+    goto done;
+  }
+  /* reduced_port (flow_system.hpp:349-372). */
+  for (uint32_t p = 0; p < count && shallow; ++p) {
+    int component = 0, upper = 0, in_plane[2] = {0, 0};
+    port_geometry(ports[p].face, &component, &upper, in_plane);
+    if (component == 2) {
+      shallow = 0;
+      break;
+    }
+    uint32_t site[3] = {0, 0, 0};
+    if (upper) site[component] = dims[component] - 1;
+    for (uint32_t along = ports[p].begin[0]; along < ports[p].end[0] && shallow; ++along) {
+      site[in_plane[0]] = along;
+      for (uint32_t z = s.bottom; z < nz; ++z) {
+        if (solid_at(g, ((size_t)site[0] * g->ny + site[1]) * nz + z)) break;
+        if (z < ports[p].begin[1] || z >= ports[p].end[1]) {
+          shallow = 0;
+          break;
+        }
+      }
+    }
+    reduced[p] = ports[p];
+    reduced[p].begin[1] = 0;
+    reduced[p].end[1] = 1;
+  }
+  if (!shallow) goto done;
+  /* port_flow_boundary (238-253) on the grid of columns. */
+  for (uint32_t p = 0; p < count; ++p) {
+    const size_t k = port_faces(&s.grid2, &reduced[p], faces, sites);
+    for (size_t i = 0; i < k; ++i) {
+      open[faces[i]] = 1;
+      held[faces[i]] = pressures[p];
+    }
+  }
+  status = assemble_system(&s, open, held, NULL, err, errlen);
+  if (status == MO_INVALID_ARGUMENT) {
+    status = MO_OK;
+    goto done;
+  }
+  if (status != MO_OK) goto done;
+  {
+    uint32_t its = 0;
+    double rel = 0.0;
+    status = solve_and_lift(&s, g, 1.0e-6f, 50000, NULL, lifted, columns, &its, &rel, NULL, NULL,
+                            err, errlen);
+    if (status != MO_OK) goto done;
+  }
+  for (size_t site = 0; site < (size_t)g->nx * g->ny * nz; ++site) known[site] = columns[site / nz];
+  *have = 1;
+done:
+  free(reduced);
+  free(faces);
+  free(sites);
+  free(open);
+  free(held);
+  free(columns);
+  free(lifted);
+  system_free(&s);
+  return status;
+}
+
+int mo_resolved_port_flow_solve(const mo_grid* g, const float* drag, const mo_flow_port* ports,
+                                uint32_t count, float tolerance, uint32_t max_outer,
+                                float inner_tolerance, uint32_t max_inner, float* out_faces,
+                                mo_port_state* states, mo_resolved_port_report* report,
+                                char* err, size_t errlen) {
+  /* ResolvedPortFlowParameters::validate (cpu_flow.cpp:179-185). */
+  if (!isfinite(tolerance) || tolerance < FLT_EPSILON)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "resolved-flow outer relative tolerance must be finite and at least float machine "
+                "epsilon");
+  if (!isfinite(inner_tolerance) || inner_tolerance < FLT_EPSILON)
+    return fail(err, errlen, MO_INVALID_ARGUMENT,
+                "resolved-flow inner relative tolerance must be finite and at least float machine "
+                "epsilon");
+  if (max_outer == 0 || max_inner == 0)
+    return fail(err, errlen, MO_INVALID_ARGUMENT, "resolved-flow iteration limits must be positive");
+  int status = validate_ports(g, g->obstacles, ports, count, err, errlen);
+  if (status != MO_OK) return status;
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const size_t nf = total_faces(g);
+  size_t total_port_faces = 0;
+  for (uint32_t p = 0; p < count; ++p) total_port_faces += port_face_count(&ports[p]);
+  size_t* open_faces = malloc((total_port_faces + 1) * sizeof(size_t));
+  size_t* offsets = calloc(count + 1, sizeof(size_t));
+  double* signed_area = malloc((count + 1) * sizeof(double));
+  size_t* scratch_faces = malloc((total_port_faces + 1) * sizeof(size_t));
+  size_t* scratch_sites = malloc((total_port_faces + 1) * sizeof(size_t));
+  uint8_t* open = calloc(nf, 1);
+  float* held_faces = calloc(nf, sizeof(float));
+  double* known = malloc((n + 1) * sizeof(double));
+  double* solution = malloc((nf + n + 1) * sizeof(double));
+  float** fields = calloc(count + 1, sizeof(float*));
+  double* velocity = calloc(nf, sizeof(double));
+  double** response = NULL;
+  float* pressures = malloc((count + 1) * sizeof(float));
+  float* held = calloc(count + 1, sizeof(float));
+  float* unit = calloc(count + 1, sizeof(float));
+  uint32_t* driven = malloc((count + 1) * sizeof(uint32_t));
+  mo_resolved_port_report rep;
+  memset(&rep, 0, sizeof(rep));
+  resolved_t rest;
+  memset(&rest, 0, sizeof(rest));
+  uint32_t count_driven = 0;
+  if (!open_faces || !offsets || !signed_area || !scratch_faces || !scratch_sites || !open ||
+      !held_faces || !known || !solution || !fields || !velocity || !pressures || !held ||
+      !unit || !driven) {
+    status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+    goto done;
+  }
+  /* make_port_faces (1124-1130) over the faces onto fluid sites; the open
+   * flags of port_flow_boundary (238-253), the same in every solve. */
+  {
+    const float spacing[3] = {g->dx, g->dy, g->dz};
+    for (uint32_t p = 0; p < count; ++p) {
+      int component = 0, upper = 0, in_plane[2] = {0, 0};
+      port_geometry(ports[p].face, &component, &upper, in_plane);
+      const size_t k = port_faces(g, &ports[p], scratch_faces, scratch_sites);
+      offsets[p + 1] = offsets[p];
+      for (size_t i = 0; i < k; ++i) {
+        open[scratch_faces[i]] = 1;
+        if (!solid_at(g, scratch_sites[i])) open_faces[offsets[p + 1]++] = scratch_faces[i];
+      }
+      const double area = (double)spacing[in_plane[0]] * spacing[in_plane[1]];
+      signed_area[p] = upper ? -area : area;
+    }
+  }
+  /* superpose_port_flows (1146-1252); one block solve per call. */
+  int holds_pressure = 0;
+  for (uint32_t p = 0; p < count; ++p) {
+    if (ports[p].kind == MO_PORT_SPEED) {
+      driven[count_driven++] = p;
+    } else {
+      held[p] = ports[p].value;
+      holds_pressure = holds_pressure || ports[p].value != 0.0f;
+    }
+  }
+#define RESOLVED_PORT_SOLVE(values, slot)                                                      \
+  do {                                                                                        \
+    for (uint32_t p = 0; p < count; ++p) {                                                    \
+      const size_t k = port_faces(g, &ports[p], scratch_faces, scratch_sites);                \
+      for (size_t i = 0; i < k; ++i) held_faces[scratch_faces[i]] = (values)[p];             \
+    }                                                                                         \
+    int have = 0;                                                                             \
+    status = shallow_port_pressure(g, ports, count, (values), known, &have, err, errlen);     \
+    if (status != MO_OK) goto done;                                                           \
+    resolved_t sys;                                                                           \
+    status = resolved_build_general(g, drag, 3, open, held_faces, have ? known : NULL, &sys,  \
+                                    err, errlen);                                             \
+    if (status == MO_OK) {                                                                    \
+      uint32_t outer = 0;                                                                     \
+      uint64_t inner = 0;                                                                     \
+      double relative = 0.0;                                                                  \
+      float momentum = 0.0f;                                                                  \
+      for (size_t i = 0; i < nf + n; ++i) solution[i] = 0.0;                                  \
+      status = resolved_system_solve(&sys, tolerance, max_outer, inner_tolerance, max_inner,  \
+                                     solution, &outer, &inner, &relative, &momentum, err,     \
+                                     errlen);                                                 \
+      rep.outer_iterations += outer;                                                          \
+      rep.inner_iterations += inner;                                                          \
+      if ((float)relative > rep.relative_residual) rep.relative_residual = (float)relative;    \
+      if (momentum > rep.momentum_relative_residual) rep.momentum_relative_residual = momentum; \
+      rep.shifted_solves += have ? 1u : 0u;                                                   \
+    }                                                                                         \
+    resolved_free(&sys);                                                                      \
+    if (status != MO_OK) goto done;                                                           \
+    fields[slot] = malloc(nf * sizeof(float));                                                \
+    if (fields[slot] == NULL) {                                                               \
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");                          \
+      goto done;                                                                              \
+    }                                                                                         \
+    for (size_t f = 0; f < nf; ++f) fields[slot][f] = (float)solution[f];                     \
+    ++rep.solves;                                                                             \
+  } while (0)
+  int have_velocity = 0;
+  if (holds_pressure) {
+    RESOLVED_PORT_SOLVE(held, count);
+    for (size_t f = 0; f < nf; ++f) velocity[f] += 1.0 * (double)fields[count][f];
+    have_velocity = 1;
+  }
+  for (uint32_t p = 0; p < count; ++p) pressures[p] = held[p];
+  if (count_driven > 0) {
+    response = calloc(count_driven, sizeof(double*));
+    if (response == NULL) {
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+      goto done;
+    }
+    for (uint32_t m = 0; m < count_driven; ++m) {
+      response[m] = calloc(count_driven + 1, sizeof(double));
+      if (response[m] == NULL) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+        goto done;
+      }
+    }
+    for (uint32_t j = 0; j < count_driven; ++j) {
+      for (uint32_t p = 0; p < count; ++p) unit[p] = 0.0f;
+      unit[driven[j]] = 1.0f;
+      RESOLVED_PORT_SOLVE(unit, j);
+      for (uint32_t m = 0; m < count_driven; ++m) {
+        const uint32_t q = driven[m];
+        double sum = 0.0;
+        for (size_t i = offsets[q]; i < offsets[q + 1]; ++i) sum += (double)fields[j][open_faces[i]];
+        response[m][j] = sum * signed_area[q];
+      }
+    }
+    for (uint32_t m = 0; m < count_driven; ++m) {
+      const uint32_t q = driven[m];
+      double base = 0.0;
+      if (have_velocity) {
+        double sum = 0.0;
+        for (size_t i = offsets[q]; i < offsets[q + 1]; ++i) sum += velocity[open_faces[i]];
+        base = sum * signed_area[q];
+      }
+      const double open_area = fabs(signed_area[q]) * (double)(offsets[q + 1] - offsets[q]);
+      response[m][count_driven] = (double)ports[q].value * open_area - base;
+    }
+    /* Gaussian elimination with partial pivoting on the augmented matrix. */
+    for (uint32_t column = 0; column < count_driven; ++column) {
+      uint32_t pivot = column;
+      for (uint32_t row = column + 1; row < count_driven; ++row)
+        if (fabs(response[row][column]) > fabs(response[pivot][column])) pivot = row;
+      if (response[pivot][column] == 0.0) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR,
+                      "a speed port carries no flow: no pressure port is reachable");
+        goto done;
+      }
+      double* swap = response[column];
+      response[column] = response[pivot];
+      response[pivot] = swap;
+      for (uint32_t row = 0; row < count_driven; ++row) {
+        if (row == column) continue;
+        const double factor = response[row][column] / response[column][column];
+        for (uint32_t entry = column; entry <= count_driven; ++entry)
+          response[row][entry] -= factor * response[column][entry];
+      }
+    }
+    for (uint32_t j = 0; j < count_driven; ++j) {
+      const double p = response[j][count_driven] / response[j][j];
+      pressures[driven[j]] = (float)p;
+      for (size_t f = 0; f < nf; ++f) velocity[f] += p * (double)fields[j][f];
+    }
+  }
+#undef RESOLVED_PORT_SOLVE
+  {
+    float max_speed = 0.0f;
+    for (size_t f = 0; f < nf; ++f) {
+      out_faces[f] = (float)velocity[f];
+      if (!isfinite(out_faces[f])) {
+        status = fail(err, errlen, MO_RUNTIME_ERROR, "flow solve produced a non-finite velocity");
+        goto done;
+      }
+      if (fabsf(out_faces[f]) > max_speed) max_speed = fabsf(out_faces[f]);
+    }
+    for (uint32_t p = 0; p < count; ++p) {
+      double sum = 0.0;
+      for (size_t i = offsets[p]; i < offsets[p + 1]; ++i) sum += velocity[open_faces[i]];
+      const double flux = sum * signed_area[p];
+      const double open_area = fabs(signed_area[p]) * (double)(offsets[p + 1] - offsets[p]);
+      if (states != NULL) {
+        states[p].pressure = pressures[p];
+        states[p].flux = (float)flux;
+        states[p].mean_speed = (float)(flux / open_area);
+      }
+    }
+    rep.max_speed = max_speed;
+  }
+  /* The divergence of the combined field and the gaps, from the system at
+   * rest (every port at pressure zero, no known pressure; 1523-1540). */
+  for (size_t f = 0; f < nf; ++f) held_faces[f] = 0.0f;
+  status = resolved_build_general(g, drag, 3, open, held_faces, NULL, &rest, err, errlen);
+  if (status != MO_OK) goto done;
+  {
+    for (size_t f = 0; f < nf; ++f) solution[f] = (double)out_faces[f];
+    double* divergence = malloc((n + 1) * sizeof(double));
+    if (divergence == NULL) {
+      status = fail(err, errlen, MO_RUNTIME_ERROR, "out of memory");
+      goto done;
+    }
+    resolved_divergence(&rest, solution, divergence);
+    double square_sum = 0.0;
+    size_t fluid_count = 0;
+    for (size_t site = 0; site < n; ++site) {
+      if (rest.fluid[site]) {
+        square_sum += divergence[site] * divergence[site];
+        ++fluid_count;
+      }
+    }
+    free(divergence);
+    rep.divergence_rms =
+        fluid_count == 0 ? 0.0f : (float)sqrt(square_sum / (double)fluid_count);
+    rep.min_gap_voxels = resolved_min_gap(&rest);
+  }
+  if (report != NULL) *report = rep;
+done:
+  if (response != NULL) {
+    for (uint32_t m = 0; m < count_driven; ++m) free(response[m]);
+    free(response);
+  }
+  if (fields != NULL)
+    for (uint32_t i = 0; i <= count; ++i) free(fields[i]);
+  free(fields);
+  free(velocity);
+  free(open_faces);
+  free(offsets);
+  free(signed_area);
+  free(scratch_faces);
+  free(scratch_sites);
+  free(open);
+  free(held_faces);
+  free(known);
+  free(solution);
+  free(pressures);
+  free(held);
+  free(unit);
+  free(driven);
+  resolved_free(&rest);
   return status;
 }
 

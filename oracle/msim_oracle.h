@@ -177,6 +177,34 @@ int mo_port_flow_solve(const mo_grid* grid, const float* mobility, const mo_flow
                        uint32_t count, float tolerance, uint32_t max_iterations, float* faces,
                        mo_port_state* states, mo_port_report* report, char* err, size_t errlen);
 
+/* ResolvedPortFlowReport (cpp/include/microsimulator/flow.hpp). */
+typedef struct {
+  uint32_t solves;
+  uint32_t outer_iterations;
+  uint64_t inner_iterations;
+  float relative_residual;
+  float momentum_relative_residual;
+  float divergence_rms;
+  float max_speed;
+  uint32_t min_gap_voxels;
+  /* Solves whose system was shifted by the depth-averaged pressure
+   * (shallow_port_pressure found a shallow description). */
+  uint32_t shifted_solves;
+} mo_resolved_port_report;
+
+/* solve_resolved_port_flow_cpu (cpp/cpu/cpu_flow.cpp:474-479, over
+ * detail::solve_resolved_port_flow, flow_system.hpp:1482-1548): the resolved
+ * Stokes-Brinkman flow driven through ports, the superposition of one block
+ * solve per speed port (and one for the held pressures), each for the
+ * correction to the depth-averaged pressure of the same ports where shallow
+ * flow describes the geometry. faces: x|y|z velocities; states: one per
+ * port. */
+int mo_resolved_port_flow_solve(const mo_grid* grid, const float* drag, const mo_flow_port* ports,
+                                uint32_t count, float tolerance, uint32_t max_outer,
+                                float inner_tolerance, uint32_t max_inner, float* faces,
+                                mo_port_state* states, mo_resolved_port_report* report,
+                                char* err, size_t errlen);
+
 /* GridBoundaryPatch (signals.hpp:72-84): a rectangle of a fixed face with its
  * own reservoir value per signal. In-plane indices: on an x face (y, z), on a
  * y face (x, z), on a z face (x, y); begin inclusive, end exclusive. */

@@ -445,6 +445,59 @@ int ref_port_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx
   });
 }
 
+// solve_resolved_port_flow_cpu (cpp/cpu/cpu_flow.cpp:474-479) on a one-signal
+// grid, ports as for ref_port_flow. faces: x|y|z; states: 3 floats per port;
+// counts: solves, outer iterations, inner iterations, minimum gap; report:
+// relative residual, momentum relative residual, divergence RMS, max speed.
+int ref_resolved_port_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                           float dy, float dz, const std::uint8_t* obstacles,
+                           const std::uint8_t* kinds, const float* drag,
+                           std::uint32_t port_count, const std::uint32_t* port_geometry,
+                           const float* port_values, float tolerance, std::uint32_t max_outer,
+                           float inner_tolerance, std::uint32_t max_inner, float* faces,
+                           float* states, std::uint64_t* counts, float* report_values, char* err,
+                           std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> d;
+    if (drag != nullptr) d = std::span<const float>(drag, n);
+    std::vector<ms::FlowPort> ports(port_count);
+    for (std::uint32_t k = 0; k < port_count; ++k) {
+      const std::uint32_t* g = port_geometry + 6 * k;
+      ports[k].face = static_cast<ms::GridFace>(g[0]);
+      ports[k].patch.begin = {g[1], g[2]};
+      ports[k].patch.end = {g[3], g[4]};
+      ports[k].kind = static_cast<ms::FlowPortKind>(g[5]);
+      ports[k].value = port_values[k];
+    }
+    ms::ResolvedPortFlowParameters parameters;
+    parameters.relative_tolerance = tolerance;
+    parameters.max_outer_iterations = max_outer;
+    parameters.inner_relative_tolerance = inner_tolerance;
+    parameters.max_inner_iterations = max_inner;
+    const auto result = ms::solve_resolved_port_flow_cpu(spec, ports, d, parameters);
+    std::memcpy(faces, result.field.x_faces.data(), result.field.x_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size(), result.field.y_faces.data(),
+                result.field.y_faces.size() * sizeof(float));
+    std::memcpy(faces + result.field.x_faces.size() + result.field.y_faces.size(),
+                result.field.z_faces.data(), result.field.z_faces.size() * sizeof(float));
+    for (std::uint32_t k = 0; k < port_count; ++k) {
+      states[3 * k] = result.ports[k].pressure;
+      states[3 * k + 1] = result.ports[k].flux;
+      states[3 * k + 2] = result.ports[k].mean_speed;
+    }
+    counts[0] = result.report.solves;
+    counts[1] = result.report.outer_iterations;
+    counts[2] = result.report.inner_iterations;
+    counts[3] = result.report.min_gap_voxels;
+    report_values[0] = result.report.relative_residual;
+    report_values[1] = result.report.momentum_relative_residual;
+    report_values[2] = result.report.divergence_rms;
+    report_values[3] = result.report.max_speed;
+  });
+}
+
 // advance_coupled_cpu on a one-layer-or-more grid: the spec as
 // ref_advance_signal_grid builds it (with `origin`), a WorldState of `cells`
 // cells (positions, lengths, radii, growth rates, types, species levels
@@ -647,6 +700,64 @@ int bind_resolved_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, flo
       report[6 * k + 3] = r.divergence_rms;
       report[6 * k + 4] = r.max_speed;
       report[6 * k + 5] = r.min_gap_voxels;
+    }
+  });
+}
+
+// solve_resolved_port_flow through the hip backend (faces_hip, states_hip)
+// and through solve_resolved_port_flow_cpu (faces_cpu, states_cpu), ports as
+// for ref_port_flow. report per backend (hip first): solves, outer and inner
+// iterations, relative residual, divergence RMS, max speed, minimum gap.
+int bind_resolved_port_flow(std::uint32_t nx, std::uint32_t ny, std::uint32_t nz, float dx,
+                            float dy, float dz, const std::uint8_t* obstacles,
+                            const std::uint8_t* kinds, const float* drag,
+                            std::uint32_t port_count, const std::uint32_t* port_geometry,
+                            const float* port_values, float tolerance, float* faces_hip,
+                            float* faces_cpu, float* states_hip, float* states_cpu,
+                            double* report, char* err, std::size_t errlen) {
+  return guarded(err, errlen, [&] {
+    const auto spec = make_spec(nx, ny, nz, dx, dy, dz, obstacles, kinds, 1, nullptr);
+    const std::size_t n = static_cast<std::size_t>(nx) * ny * nz;
+    std::span<const float> d;
+    if (drag != nullptr) d = std::span<const float>(drag, n);
+    std::vector<ms::FlowPort> ports(port_count);
+    for (std::uint32_t k = 0; k < port_count; ++k) {
+      const std::uint32_t* g = port_geometry + 6 * k;
+      ports[k].face = static_cast<ms::GridFace>(g[0]);
+      ports[k].patch.begin = {g[1], g[2]};
+      ports[k].patch.end = {g[3], g[4]};
+      ports[k].kind = static_cast<ms::FlowPortKind>(g[5]);
+      ports[k].value = port_values[k];
+    }
+    ms::ResolvedPortFlowParameters parameters;
+    parameters.relative_tolerance = tolerance;
+    parameters.inner_relative_tolerance = tolerance;
+    const auto backend = ms::make_hip_backend(0);
+    const auto hip = backend->solve_resolved_port_flow(spec, ports, d, parameters);
+    const auto cpu = ms::solve_resolved_port_flow_cpu(spec, ports, d, parameters);
+    float* out[2] = {faces_hip, faces_cpu};
+    float* st[2] = {states_hip, states_cpu};
+    const ms::ResolvedPortFlowResult* results[2] = {&hip, &cpu};
+    for (int k = 0; k < 2; ++k) {
+      const auto& field = results[k]->field;
+      std::memcpy(out[k], field.x_faces.data(), field.x_faces.size() * sizeof(float));
+      std::memcpy(out[k] + field.x_faces.size(), field.y_faces.data(),
+                  field.y_faces.size() * sizeof(float));
+      std::memcpy(out[k] + field.x_faces.size() + field.y_faces.size(), field.z_faces.data(),
+                  field.z_faces.size() * sizeof(float));
+      for (std::uint32_t p = 0; p < port_count; ++p) {
+        st[k][3 * p] = results[k]->ports[p].pressure;
+        st[k][3 * p + 1] = results[k]->ports[p].flux;
+        st[k][3 * p + 2] = results[k]->ports[p].mean_speed;
+      }
+      const auto& r = results[k]->report;
+      report[7 * k] = r.solves;
+      report[7 * k + 1] = r.outer_iterations;
+      report[7 * k + 2] = static_cast<double>(r.inner_iterations);
+      report[7 * k + 3] = r.relative_residual;
+      report[7 * k + 4] = r.divergence_rms;
+      report[7 * k + 5] = r.max_speed;
+      report[7 * k + 6] = r.min_gap_voxels;
     }
   });
 }

@@ -43,21 +43,21 @@ def test_python_mirror_binds_the_abi():
         assert getattr(lib, name).restype is not None or name == "msim_hip_destroy"
 
 
-def test_unsupported_features_are_not_advertised():
-    assert msa.SUPPORTED == {"signals", "coupled_rates", "depth_averaged_flow"}
+def test_only_implemented_features_are_advertised():
+    assert msa.SUPPORTED == {"signals", "coupled_rates", "depth_averaged_flow", "resolved_flow"}
     assert set(msa.FEATURES) >= msa.SUPPORTED
 
 
-def test_depth_averaged_entry_points_are_implemented():
-    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49), the
-    resolved axis flow (50-52) and the coupled step (31-34) run the C ABI;
-    the resolved port flow and the cell-mechanics calls are refused."""
+def test_flow_entry_points_are_implemented():
+    """Both depth-averaged ComputeBackend calls (backend.hpp:44-49), both
+    resolved ones (50-55) and the coupled step (31-34) run the C ABI; the
+    cell-mechanics calls are refused."""
     stub = msa.HipBackend._unsupported
     assert msa.HipBackend.solve_depth_averaged_flow is not stub
     assert msa.HipBackend.solve_depth_averaged_port_flow is not stub
     assert msa.HipBackend.advance_coupled is not stub
     assert msa.HipBackend.solve_resolved_flow is not stub
-    assert msa.HipBackend.solve_resolved_port_flow is stub
+    assert msa.HipBackend.solve_resolved_port_flow is not stub
     assert msa.HipBackend.advance_species is stub
 
 

@@ -125,3 +125,98 @@ def test_deposition_refusals():
     with pytest.raises(ColonyError, match="radius finite and positive"):
         colony_volume_fraction((3, 3, 1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), None,
                                rods([(1.0, 1.0, 0.0)], radius=0.0))
+
+
+def test_flow_coupling_mobility_is_the_colony_mobility():
+    """colony_flow_coupling's mobility is colony_mobility's for the same
+    colony (flow.py:384-424 calls the same _darcy_coefficients)."""
+    from oracle.colony import colony_flow_coupling
+    rng = np.random.default_rng(2)
+    solid = (rng.random((12, 9, 2)) < 0.1).astype(np.uint8).reshape(-1)
+    cells = rods(np.column_stack([rng.uniform(0, 11, 30), rng.uniform(0, 8, 30),
+                                  rng.uniform(0, 1, 30)]))
+    velocities = rng.uniform(-1, 1, (30, 2))
+    args = ((12, 9, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), solid, cells)
+    mobility, force = colony_flow_coupling(*args, velocities, base=1.5, averaging_radius=2.0)
+    assert np.array_equal(mobility, colony_mobility(*args, base=1.5, averaging_radius=2.0))
+    assert force.shape == (12 * 9, 2)
+
+
+def test_a_common_velocity_gives_drag_times_velocity():
+    """With every cell moving at one velocity V the column's momentum is its
+    biomass times V, so the force is drag * V wherever a column holds
+    biomass and zero elsewhere (flow.py:421-423)."""
+    from oracle.colony import colony_flow_coupling
+    rng = np.random.default_rng(3)
+    dims = (10, 8, 1)
+    cells = rods(np.column_stack([rng.uniform(2, 7, 25), rng.uniform(2, 5, 25), np.zeros(25)]))
+    velocity = (0.75, -1.25)
+    mobility, force = colony_flow_coupling(dims, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), None, cells,
+                                           np.tile(velocity, (25, 1)), base=1.0,
+                                           drag_coefficient=2.0, max_volume_fraction=0.5,
+                                           averaging_radius=1.8)
+    density = colony_volume_fraction(dims, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), None, cells,
+                                     averaging_radius=1.8)
+    fraction = np.minimum(density.sum(axis=2), 0.5).reshape(-1)
+    drag = 2.0 * fraction ** 2 / (1.0 - fraction) ** 3
+    has = density.sum(axis=2).reshape(-1) > 0
+    v = np.array([float(np.float32(velocity[0])), float(np.float32(velocity[1]))])
+    np.testing.assert_allclose(force[has], drag[has, None] * v, rtol=1e-12)
+    assert np.all(force[~has] == 0.0)
+    assert has.sum() > 10
+
+
+def test_flow_coupling_refusals():
+    from oracle.colony import colony_flow_coupling
+    cells = rods([(1.0, 1.0, 0.0), (math.nan, 1.0, 0.0)])
+    args = ((3, 3, 1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), None, cells)
+    with pytest.raises(ColonyError, match="in-plane cell velocity must be finite"):
+        colony_flow_coupling(*args, [(math.inf, 0.0), (0.0, 0.0)])
+    # A cell's velocity is checked before its position (flow.py:409-412).
+    with pytest.raises(ColonyError, match="deposited positions must be finite"):
+        colony_flow_coupling(*args, [(0.0, 0.0), (0.0, 0.0)])
+    with pytest.raises(ColonyError, match="in-plane cell velocity must be finite"):
+        colony_flow_coupling(*args, [(0.0, 0.0), (math.nan, 0.0)])
+    with pytest.raises(ColonyError, match="averaging radius"):
+        colony_flow_coupling(*args, [(0.0, 0.0)] * 2, averaging_radius=0.0)
+
+
+def test_relative_drag_in_a_uniform_duct_from_the_colony(oracle):
+    """The reference's own test of colony_flow_coupling
+    (python/tests/test_flow.py:192-233): one rod of length 3, radius 0.5 at
+    (3, 4, 0) in a 7 x 9 duct along y, base 1, drag coefficient 2, cap 0.5,
+    radius 1.8; mobility and force from the coupling, the flow from the
+    oracle's forced depth-averaged solve (bit-exact with the reference's)."""
+    from oracle import FIXED, NO_FLUX, Grid
+    from oracle.colony import colony_flow_coupling
+    nx, ny = 7, 9
+    grid = Grid(nx, ny, 1, 1.0, 1.0, 1.0, None,
+                boundary=(NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX))
+    cells = rods([(3.0, 4.0, 0.0)], length=3.0)
+
+    def solve(vx, vy, inlet):
+        mobility, force = colony_flow_coupling((nx, ny, 1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+                                               None, cells, [(vx, vy)], base=1.0,
+                                               drag_coefficient=2.0, max_volume_fraction=0.5,
+                                               averaging_radius=1.8)
+        faces, _, _ = oracle.solve_flow(grid, mobility.astype(np.float32), 1, inlet, 1e-6,
+                                        50000, body_force=force.astype(np.float32))
+        x = faces[:(nx + 1) * ny].reshape(nx + 1, ny)
+        y = faces[(nx + 1) * ny:(nx + 1) * ny + nx * (ny + 1)].reshape(nx, ny + 1)
+        np.testing.assert_allclose(y.mean(axis=0), inlet, atol=2e-5)
+        np.testing.assert_allclose(np.diff(x, axis=0) + np.diff(y, axis=1), 0, atol=5e-5)
+        assert np.all(x[[0, -1], :] == 0.0)
+        return x, y
+
+    _, slow = solve(0, 0, 2)
+    assert slow[3, 4] < 1.5 and slow[1, 4] > 2
+    x, matched = solve(0, 2, 2)
+    np.testing.assert_allclose(matched, 2, atol=4e-5)
+    np.testing.assert_allclose(x, 0, atol=4e-5)
+    _, fast = solve(0, 4, 2)
+    np.testing.assert_allclose(fast - matched, matched - slow, atol=5e-5)
+    x, y = solve(3, 0, 0)
+    assert x[3, 4] > 1
+    reverse_x, reverse_y = solve(-3, 0, 0)
+    np.testing.assert_allclose(reverse_x, -x, atol=1e-5)
+    np.testing.assert_allclose(reverse_y, -y, atol=1e-5)

@@ -33,7 +33,7 @@ def test_backend_identity(binding):
     name, features = binding.create()
     assert name == "hip" and features["native"]
     assert features["signals"] and features["coupled_rates"] and features["depth_averaged_flow"]
-    assert not features["resolved_flow"]
+    assert features["resolved_flow"]
 
 
 def test_depth_averaged_flow_through_the_interface(binding):
@@ -66,6 +66,21 @@ def test_resolved_flow_through_the_interface(binding):
     print("outer", rh["outer_iterations"], rc["outer_iterations"], flush=True)
     assert close(hip, cpu, 8e-4)
     assert abs(rh["outer_iterations"] - rc["outer_iterations"]) <= 1
+
+
+def test_resolved_port_flow_through_the_interface(binding):
+    """Two speed inlets over pillars against a pressure outlet through the
+    backend interface beside solve_resolved_port_flow_cpu: the field within
+    the suite's compare_fields tolerance, the speed ports' mean speeds, the
+    solves and the gap."""
+    from tests.test_resolved_port_oracle import port_cases
+    grid, drag, ports = port_cases()["two inlets over pillars"]
+    hip, cpu, sh, sc, (rh, rc) = binding.resolved_port_flow(grid, drag, ports)
+    print("outer", rh["outer_iterations"], rc["outer_iterations"], flush=True)
+    assert close(hip, cpu, 8e-4)
+    assert np.all(np.abs(sh[:2, 2] - sc[:2, 2]) <= 1e-5)
+    assert rh["solves"] == rc["solves"] == 3
+    assert rh["min_gap_voxels"] == rc["min_gap_voxels"]
 
 
 @pytest.mark.parametrize("kind", CONFORMANCE_KINDS)

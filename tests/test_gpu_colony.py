@@ -163,3 +163,86 @@ def test_refusals_match_the_restatement(hip, name, change, message):
         hip.colony_mobility(grid, cells, **kw)
     assert str(theirs.value) == message
     assert str(ours.value) == message
+
+
+@pytest.mark.parametrize("layers", [1, 3])
+def test_flow_coupling_matches_the_restatement(hip, layers):
+    """colony_flow_coupling on the device: 1500 cells with random in-plane
+    velocities on a 64x48 obstacle grid (and three layers of it), mobility
+    and body force against the restatement to 1e-12 of the largest value."""
+    rng = np.random.default_rng(10 + layers)
+    nx, ny = 64, 48
+    solid = (rng.random(nx * ny) < 0.12).astype(np.uint8)
+    cells, _ = random_cells(3, nx, ny, solid, 1500, 1.0, 1.0)
+    cells = {k: cells[k] for k in ("position_x", "position_y", "position_z", "lengths", "radii")}
+    velocities = rng.uniform(-2.0, 2.0, (1500, 2)).astype(np.float32)
+    mask = np.repeat(solid.reshape(nx, ny, 1), layers, axis=2).reshape(-1)
+    if layers > 1:
+        cells = dict(cells, position_z=np.full(1500, 1.0, np.float32))
+    grid = msa.Grid(nx, ny, layers, 1.0, 1.0, 1.0, mask, (NO_FLUX,) * 6)
+    base = rng.uniform(0.5, 2.0, nx * ny * layers)
+    mobility, force = hip.colony_flow_coupling(grid, cells, velocities, base=base,
+                                               drag_coefficient=50.0, averaging_radius=2.5)
+    m_ref, f_ref = ref.colony_flow_coupling((nx, ny, layers), (0.0, 0.0, 0.0),
+                                            (1.0, 1.0, 1.0), mask, cells, velocities, base=base,
+                                            drag_coefficient=50.0, averaging_radius=2.5)
+    assert close(mobility, m_ref)
+    assert close(force, f_ref)
+    assert np.count_nonzero(force) > 100
+
+
+def test_relative_drag_in_a_uniform_duct_from_the_colony(hip):
+    """The reference's own test of colony_flow_coupling with the body-forced
+    flow (python/tests/test_flow.py:192-233), every step on the device: one
+    rod (length 3, radius 0.5) at (3, 4, 0) in a 7 x 9 duct along y, base 1,
+    drag coefficient 2, cap 0.5, radius 1.8; the reference's assertions."""
+    from tests.helpers import FIXED
+    nx, ny = 7, 9
+    grid = msa.Grid(nx, ny, 1, 1.0, 1.0, 1.0, None,
+                    (NO_FLUX, NO_FLUX, FIXED, FIXED, NO_FLUX, NO_FLUX))
+    cells = rods([(3.0, 4.0, 0.0)], length=3.0)
+
+    def solve(vx, vy, inlet):
+        mobility, force = hip.colony_flow_coupling(grid, cells, [(vx, vy)], base=1.0,
+                                                   drag_coefficient=2.0, max_volume_fraction=0.5,
+                                                   averaging_radius=1.8)
+        p = msa.DepthAveragedFlowParameters(mean_inlet_speed=inlet, axis=1,
+                                            body_force=force.astype(np.float32))
+        field = hip.solve_depth_averaged_flow(grid, mobility.astype(np.float32), p).field
+        x = field.x_faces.reshape(nx + 1, ny)
+        y = field.y_faces.reshape(nx, ny + 1)
+        np.testing.assert_allclose(y.mean(axis=0), inlet, atol=2e-5)
+        np.testing.assert_allclose(np.diff(x, axis=0) + np.diff(y, axis=1), 0, atol=5e-5)
+        assert np.all(x[[0, -1], :] == 0.0)
+        return x, y
+
+    _, slow = solve(0, 0, 2)
+    assert slow[3, 4] < 1.5 and slow[1, 4] > 2
+    x, matched = solve(0, 2, 2)
+    np.testing.assert_allclose(matched, 2, atol=4e-5)
+    np.testing.assert_allclose(x, 0, atol=4e-5)
+    _, fast = solve(0, 4, 2)
+    np.testing.assert_allclose(fast - matched, matched - slow, atol=5e-5)
+    x, y = solve(3, 0, 0)
+    assert x[3, 4] > 1
+    reverse_x, reverse_y = solve(-3, 0, 0)
+    np.testing.assert_allclose(reverse_x, -x, atol=1e-5)
+    np.testing.assert_allclose(reverse_y, -y, atol=1e-5)
+
+
+@pytest.mark.parametrize("velocities,message", [
+    ([(math.inf, 0.0), (0.0, 0.0)], "in-plane cell velocity must be finite"),
+    ([(0.0, 0.0), (0.0, math.nan)], "in-plane cell velocity must be finite"),
+])
+def test_flow_coupling_refusals(hip, velocities, message):
+    grid = msa.Grid(5, 5, 1, 1.0, 1.0, 1.0, None, (NO_FLUX,) * 6)
+    cells = rods([(1.0, 1.0, 0.0), (3.0, 3.0, 0.0)])
+    with pytest.raises(ref.ColonyError) as theirs:
+        ref.colony_flow_coupling((5, 5, 1), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), None, cells,
+                                 velocities, averaging_radius=1.0)
+    with pytest.raises(msa.MsimError) as ours:
+        hip.colony_flow_coupling(grid, cells, velocities, averaging_radius=1.0)
+    assert str(theirs.value) == message
+    assert str(ours.value) == message
+    # The context still deposits after a refusal.
+    hip.colony_flow_coupling(grid, cells, [(0.0, 0.0)] * 2, averaging_radius=1.0)

@@ -198,7 +198,7 @@ def test_error_paths(hip):
         hip.solve_depth_averaged_flow(msa.Grid(4, 8, 3, obstacles=overhang.reshape(-1)), None,
                                       params())
     with pytest.raises(msa.Unsupported):
-        hip.solve_resolved_port_flow()
+        hip.advance_species()
     # The context still works after the failures.
     res = hip.solve_depth_averaged_flow(duct, None, params(axis=1, mean_inlet_speed=5.0,
                                                            relative_tolerance=1e-6))
