@@ -47,7 +47,13 @@ struct ResView {
   double inv[3];      // 1 / h^2, 0 on an axis of one site (no viscous term)
   double cs;          // continuity scale 1 / min spacing
   uint32_t chunk;     // entries per workgroup of the reducing kernels
+  uint32_t cg_faces;  // faces per thread of the momentum CG's kernels (1, 2, 4 or 8)
   const uint8_t* active;
+  // Per face, bit 2a + (side > 0): the momentum stencil reads the neighbour
+  // face along axis a on that side (it is on the face lattice and, across
+  // the face's own axis, exists); otherwise it mirrors (flow_system.hpp:
+  // 887-905).
+  const uint8_t* neighbours;
   const uint8_t* exists;
   const uint8_t* fluid;
   const float* face_drag;
@@ -84,32 +90,34 @@ __device__ inline uint64_t site_at(const ResView& V, uint32_t x, uint32_t y, uin
 }
 
 // apply_momentum (flow_system.hpp:881-910) at face f; u(i) is the input at
-// face i.
+// face i. The neighbour bits decide every read up front, so the loads of all
+// six neighbours issue together.
 template <typename U>
 __device__ inline double momentum_at(const ResView& V, const U& u, uint64_t f, int c,
                                      const uint32_t fd[3], const uint32_t co[3]) {
   const double uf = u(f);
+  const uint32_t bits = V.neighbours[f];
+  double value[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int slot = 2 * a + side;
+      if ((bits >> slot) & 1u) {
+        uint32_t nb[3] = {co[0], co[1], co[2]};
+        nb[a] = side == 0 ? nb[a] - 1 : nb[a] + 1;
+        value[slot] = u(face_at(V, c, fd, nb[0], nb[1], nb[2]));
+      } else {
+        value[slot] = a == c ? uf : -uf;
+      }
+    }
+  }
   double result = static_cast<double>(V.face_drag[f]) * uf;
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (V.dims[a] == 1) continue;
-    for (int off = -1; off <= 1; off += 2) {
-      uint32_t nb[3] = {co[0], co[1], co[2]};
-      bool has = true;
-      if (off < 0) {
-        if (nb[a] == 0) has = false; else --nb[a];
-      } else {
-        if (nb[a] + 1 >= fd[a]) has = false; else ++nb[a];
-      }
-      double neighbor;
-      if (a == c) {
-        neighbor = has ? u(face_at(V, c, fd, nb[0], nb[1], nb[2])) : uf;
-      } else if (has && V.exists[face_at(V, c, fd, nb[0], nb[1], nb[2])] != 0) {
-        neighbor = u(face_at(V, c, fd, nb[0], nb[1], nb[2]));
-      } else {
-        neighbor = -uf;
-      }
-      result -= (neighbor - uf) * V.inv[a];
-    }
+    result -= (value[2 * a] - uf) * V.inv[a];
+    result -= (value[2 * a + 1] - uf) * V.inv[a];
   }
   return result;
 }
@@ -184,6 +192,9 @@ __global__ __launch_bounds__(kResBlock) void k_res_dot(const double* __restrict_
 // so no launch reads a field that it writes.
 struct ResState {
   double rhs_norm, relative;
+  // r.z after iteration k's update at rho[k & 1], recorded by the first
+  // kernel of iteration k + 1 (for the second kernel and the next beta).
+  double rho[2];
   uint32_t iterations, stop1, stop2, failure1, failure2, pad;
 };
 
@@ -199,6 +210,39 @@ __device__ inline double all_sum(const double* partials, uint32_t blocks, double
   const double out = *s_b;
   __syncthreads();
   return out;
+}
+
+// all_sum of two partial arrays in one pass: the same sums, bit for bit, in
+// one round of loads instead of two.
+__device__ inline void all_sum2(const double* a, const double* b, uint32_t blocks,
+                                double* s_red, double* s_b, double& sa, double& sb) {
+  double va = 0.0, vb = 0.0;
+  for (uint32_t i = threadIdx.x; i < blocks; i += kResBlock) {
+    va += a[i];
+    vb += b[i];
+  }
+  va = wave_sum(va);
+  vb = wave_sum(vb);
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_red[wave] = va;
+    s_red[kResBlock / kWave + wave] = vb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0.0, tb = 0.0;
+#pragma unroll
+    for (int w = 0; w < kResBlock / kWave; ++w) {
+      ta += s_red[w];
+      tb += s_red[kResBlock / kWave + w];
+    }
+    s_b[0] = ta;
+    s_b[1] = tb;
+  }
+  __syncthreads();
+  sa = s_b[0];
+  sb = s_b[1];
+  __syncthreads();
 }
 
 __device__ inline bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
@@ -222,8 +266,10 @@ __global__ __launch_bounds__(kResBlock) void k_cg_init(ResView V, const double* 
                                                        double* __restrict__ z, CgParts P,
                                                        ResState* st) {
   __shared__ double s_red[kResBlock / kWave];
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * V.chunk;
-  const uint64_t end = begin + V.chunk < V.nf ? begin + V.chunk : V.nf;
+  // The CG kernels' layout: cg_faces faces per thread, kResBlock apart.
+  const uint64_t chunk = static_cast<uint64_t>(V.cg_faces) * kResBlock;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t end = begin + chunk < V.nf ? begin + chunk : V.nf;
   double rr = 0.0, rz = 0.0;
   for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
     const double ri = static_cast<double>(static_cast<float>(rhs[i]));
@@ -257,18 +303,21 @@ __global__ __launch_bounds__(kResBlock) void k_cg_init(ResView V, const double* 
 // all hold the same scalars; workgroup 0 records them. A neighbour's p is
 // formed from z and p_old where it is read, so p_old (another buffer) is
 // never written here. `last`: test only (the iteration limit is reached).
+// Each thread takes F faces, F * kResBlock apart, their loads interleaved.
+template <int F>
 __global__ __launch_bounds__(kResBlock) void k_cg_direction(
     ResView V, const double* __restrict__ z, const double* __restrict__ p_old,
     double* __restrict__ p, double* __restrict__ q, CgParts P, uint32_t k, uint32_t last,
     float tolerance, ResState* st) {
-  __shared__ double s_red[kResBlock / kWave];
-  __shared__ double s_b;
+  __shared__ double s_red[2 * kResBlock / kWave];
+  __shared__ double s_b[2];
   if (st->stop2 != 0) {
     if (lead()) st->stop1 = 1;
     return;
   }
   double beta = 0.0;
-  const double rr = all_sum(P.rr(k), P.blocks, s_red, &s_b);
+  double rr, rz;
+  all_sum2(P.rr(k), P.rz(k), P.blocks, s_red, s_b, rr, rz);
   if (k == 0) {
     if (rr == 0.0) {
       if (lead()) {
@@ -277,7 +326,10 @@ __global__ __launch_bounds__(kResBlock) void k_cg_direction(
       }
       return;
     }
-    if (lead()) st->rhs_norm = sqrt(rr);
+    if (lead()) {
+      st->rhs_norm = sqrt(rr);
+      st->rho[0] = rz;
+    }
   } else {
     const double relative = sqrt(rr > 0.0 ? rr : 0.0) / st->rhs_norm;
     if (!isfinite(relative)) {
@@ -296,8 +348,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_direction(
       return;
     }
     if (last != 0) return;
-    const double rz = all_sum(P.rz(k), P.blocks, s_red, &s_b);
-    const double rho = all_sum(P.rz(k + 1), P.blocks, s_red, &s_b);
+    const double rho = st->rho[(k + 1) & 1];
     if (!isfinite(rz) || rho == 0.0) {
       if (lead()) {
         st->failure1 = MSIM_CG_BREAKDOWN;
@@ -313,24 +364,28 @@ __global__ __launch_bounds__(kResBlock) void k_cg_direction(
       }
       return;
     }
+    if (lead()) st->rho[k & 1] = rz;
   }
   const bool first = k == 0;
   const auto dir = [&](uint64_t i) { return first ? z[i] : z[i] + beta * p_old[i]; };
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * V.chunk;
-  const uint64_t end = begin + V.chunk < V.nf ? begin + V.chunk : V.nf;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * (F * kResBlock);
   double pq = 0.0;
-  for (uint64_t f = begin + threadIdx.x; f < end; f += kResBlock) {
-    const double pf = dir(f);
-    double v = 0.0;
-    if (V.active[f] != 0) {
-      const int c = face_component(V, f);
-      uint32_t fd[3], co[3];
-      face_xyz(V, c, f, fd, co);
-      v = momentum_at(V, dir, f, c, fd, co);
+#pragma unroll
+  for (int u = 0; u < F; ++u) {
+    const uint64_t f = begin + threadIdx.x + static_cast<uint64_t>(u) * kResBlock;
+    if (f < V.nf) {
+      const double pf = dir(f);
+      double v = 0.0;
+      if (V.active[f] != 0) {
+        const int c = face_component(V, f);
+        uint32_t fd[3], co[3];
+        face_xyz(V, c, f, fd, co);
+        v = momentum_at(V, dir, f, c, fd, co);
+      }
+      p[f] = pf;
+      q[f] = v;
+      pq += pf * v;
     }
-    p[f] = pf;
-    q[f] = v;
-    pq += pf * v;
   }
   pq = block_sum<kResBlock>(pq, s_red);
   if (threadIdx.x == 0) P.pq()[blockIdx.x] = pq;
@@ -339,6 +394,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_direction(
 // The second half of CG iteration k: alpha = rho / p.q with the curvature and
 // step checks (conjugate_gradient.hpp:57-65), x += alpha p, r -= alpha q,
 // z = D^-1 r, and the partials of r.r and r.z for iteration k + 1.
+template <int F>
 __global__ __launch_bounds__(kResBlock) void k_cg_step(ResView V, double* __restrict__ x,
                                                        double* __restrict__ r,
                                                        double* __restrict__ z,
@@ -359,8 +415,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_step(ResView V, double* __rest
     }
     return;
   }
-  const double rho = all_sum(P.rz(k), P.blocks, s_red, &s_b);
-  const double alpha = rho / curvature;
+  const double alpha = st->rho[k & 1] / curvature;
   if (!isfinite(alpha)) {
     if (lead()) {
       st->failure2 = MSIM_CG_STEP;
@@ -368,18 +423,21 @@ __global__ __launch_bounds__(kResBlock) void k_cg_step(ResView V, double* __rest
     }
     return;
   }
-  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * V.chunk;
-  const uint64_t end = begin + V.chunk < V.nf ? begin + V.chunk : V.nf;
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * (F * kResBlock);
   double rr = 0.0, rz = 0.0;
-  for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const float d = V.diag[i];
-    const double zi = d > 0.0f ? ri / d : 0.0;
-    z[i] = zi;
-    rr += ri * ri;
-    rz += ri * zi;
+#pragma unroll
+  for (int u = 0; u < F; ++u) {
+    const uint64_t i = begin + threadIdx.x + static_cast<uint64_t>(u) * kResBlock;
+    if (i < V.nf) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      const float d = V.diag[i];
+      const double zi = d > 0.0f ? ri / d : 0.0;
+      z[i] = zi;
+      rr += ri * ri;
+      rz += ri * zi;
+    }
   }
   rr = block_sum<kResBlock>(rr, s_red);
   rz = block_sum<kResBlock>(rz, s_red);
@@ -424,29 +482,91 @@ __global__ __launch_bounds__(kResBlock) void k_res_sum(const double* partials, u
   if (threadIdx.x == 0) *slot = v;
 }
 
-// One step of twice-modified Gram-Schmidt (flexible_gmres.hpp:70-76): the
-// projection v.w from the last step's partials (kept at `slot`), w -= it v
-// over this workgroup's chunk, and the partials of the next step's dot,
-// next.w (next = nullptr: w.w), over the same chunk with the updated w.
-__global__ __launch_bounds__(kResBlock) void k_res_mgs(uint64_t n, double* w,
-                                                       const double* __restrict__ v,
-                                                       const double* next, const double* in_parts,
-                                                       uint32_t blocks, uint32_t chunk,
-                                                       double* slot, double* out_parts) {
-  __shared__ double s_red[kResBlock / kWave];
-  __shared__ double s_b;
-  const double projection = all_sum(in_parts, blocks, s_red, &s_b);
-  if (lead()) *slot = projection;
+// Arnoldi orthogonalisation by classical Gram-Schmidt applied twice (CGS2).
+// The reference orthogonalises by modified Gram-Schmidt applied twice
+// (flexible_gmres.hpp:70-76); both give w minus its projection on the basis
+// and the same coefficients in exact arithmetic, and both are applied twice
+// for the same reason. CGS2 forms all of a pass's dot products in one launch,
+// so a step of j basis vectors takes six launches instead of 2j + 4.
+
+// Partials of v_i . w for i < count over this workgroup's chunk, at
+// partials[i * gridDim.x + block]. V holds the basis vectors, n apart.
+__global__ __launch_bounds__(kResBlock) void k_res_multidot(uint64_t n, uint32_t chunk,
+                                                            const double* __restrict__ V,
+                                                            uint32_t count,
+                                                            const double* __restrict__ w,
+                                                            double* __restrict__ partials) {
+  __shared__ double s_part[kRestart + 1][kResBlock / kWave];
   const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * chunk;
   const uint64_t end = begin + chunk < n ? begin + chunk : n;
-  double acc = 0.0;
-  for (uint64_t i = begin + threadIdx.x; i < end; i += kResBlock) {
-    const double wi = w[i] + -projection * v[i];
-    w[i] = wi;
-    acc += (next != nullptr ? next[i] : wi) * wi;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  for (uint32_t i = 0; i < count; ++i) {
+    const double* vi = V + static_cast<size_t>(i) * n;
+    double acc = 0.0;
+    for (uint64_t e = begin + threadIdx.x; e < end; e += kResBlock) acc += vi[e] * w[e];
+    acc = wave_sum(acc);
+    if (lane == 0) s_part[i][wave] = acc;
   }
-  acc = block_sum<kResBlock>(acc, s_red);
-  if (threadIdx.x == 0) out_parts[blockIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < count; i += kResBlock) {
+    double total = 0.0;
+#pragma unroll
+    for (int k = 0; k < kResBlock / kWave; ++k) total += s_part[i][k];
+    partials[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = total;
+  }
+}
+
+// out[i] = the fixed-order sum of partials[i * blocks + 0 .. blocks) for
+// i < count: one workgroup, each wave taking every fourth coefficient.
+__global__ __launch_bounds__(kResBlock) void k_res_multisum(const double* __restrict__ partials,
+                                                            uint32_t count, uint32_t blocks,
+                                                            double* __restrict__ out) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  for (uint32_t i = wave; i < count; i += kResBlock / kWave) {
+    const double* p = partials + static_cast<size_t>(i) * blocks;
+    double acc = 0.0;
+    for (uint32_t b = lane; b < blocks; b += kWave) acc += p[b];
+    acc = wave_sum(acc);
+    if (lane == 0) out[i] = acc;
+  }
+}
+
+// w -= sum_i h[i] v_i over this workgroup's chunk (in order of i), then the
+// partials of the next pass's dot products with the updated w: v_i . w for
+// i < count (multidot layout), or w . w when `norm` (at partials[block]).
+__global__ __launch_bounds__(kResBlock) void k_res_cgs(uint64_t n, uint32_t chunk,
+                                                       const double* __restrict__ V,
+                                                       uint32_t count, const double* __restrict__ h,
+                                                       double* w, uint32_t norm,
+                                                       double* __restrict__ partials) {
+  __shared__ double s_h[kRestart + 1];
+  __shared__ double s_part[kRestart + 1][kResBlock / kWave];
+  if (threadIdx.x < count) s_h[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  const uint64_t begin = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t end = begin + chunk < n ? begin + chunk : n;
+  for (uint64_t e = begin + threadIdx.x; e < end; e += kResBlock) {
+    double we = w[e];
+    for (uint32_t i = 0; i < count; ++i) we += -s_h[i] * V[static_cast<size_t>(i) * n + e];
+    w[e] = we;
+  }
+  // Each thread reads back only the entries it wrote.
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const uint32_t dots = norm != 0 ? 1u : count;
+  for (uint32_t i = 0; i < dots; ++i) {
+    const double* vi = norm != 0 ? w : V + static_cast<size_t>(i) * n;
+    double acc = 0.0;
+    for (uint64_t e = begin + threadIdx.x; e < end; e += kResBlock) acc += vi[e] * w[e];
+    acc = wave_sum(acc);
+    if (lane == 0) s_part[i][wave] = acc;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < dots; i += kResBlock) {
+    double total = 0.0;
+#pragma unroll
+    for (int k = 0; k < kResBlock / kWave; ++k) total += s_part[i][k];
+    partials[static_cast<size_t>(i) * gridDim.x + blockIdx.x] = total;
+  }
 }
 
 // out = 0 + s * in (a basis vector from w or the residual).
@@ -483,7 +603,7 @@ struct ResSystem {
   uint32_t dims[3];
   float h[3];
   uint64_t off[3], cnt[3], nf, n;
-  std::vector<uint8_t> fluid, active, exists;
+  std::vector<uint8_t> fluid, active, exists, neighbours;
   std::vector<float> face_drag, diag, force, pd;
   uint32_t axis;
 };
@@ -600,6 +720,29 @@ ResSystem build_system(const msim_grid& g, const std::vector<uint8_t>& mask, con
       throw Error(MSIM_INVALID_ARGUMENT, "the resolved-flow inlet boundary is entirely blocked");
     }
   }
+  // The momentum stencil's neighbour bits of each active face.
+  s.neighbours.assign(s.nf, 0);
+  for (uint64_t index = 0; index < s.nf; ++index) {
+    if (s.active[index] == 0) continue;
+    int c;
+    uint32_t co[3];
+    face_coords_host(s, index, &c, co);
+    uint32_t fd[3] = {s.dims[0], s.dims[1], s.dims[2]};
+    ++fd[c];
+    uint8_t bits = 0;
+    for (int a = 0; a < 3; ++a) {
+      for (int side = 0; side < 2; ++side) {
+        const bool has = side == 0 ? co[a] > 0 : co[a] + 1 < fd[a];
+        if (!has) continue;
+        uint32_t nb[3] = {co[0], co[1], co[2]};
+        nb[a] = side == 0 ? nb[a] - 1 : nb[a] + 1;
+        if (a == c || s.exists[face_host(s, c, nb[0], nb[1], nb[2])] != 0) {
+          bits |= static_cast<uint8_t>(1u << (2 * a + side));
+        }
+      }
+    }
+    s.neighbours[index] = bits;
+  }
   // pressure_diagonal (958-978).
   s.pd.assign(s.n, 0.0f);
   for (uint64_t site = 0; site < s.n; ++site) {
@@ -666,7 +809,7 @@ uint32_t min_gap(const ResSystem& s) {
 struct ResDevice {
   ResView V{};
   uint64_t m = 0;
-  DeviceBuffer<uint8_t> active, exists, fluid;
+  DeviceBuffer<uint8_t> active, exists, fluid, neighbours;
   DeviceBuffer<float> face_drag, diag, pd;
   // cg_p holds two direction buffers (iterations alternate between them).
   DeviceBuffer<double> rhs, x, r, w, basis, directions, cg_r, cg_z, cg_p, cg_q, cg_parts,
@@ -709,13 +852,42 @@ double read_scalar(Context& ctx, const double* slot) {
 void momentum_cg(Context& ctx, ResDevice& D, const double* rhs, double* out, float tolerance,
                  uint32_t max_iterations, uint64_t& inner) {
   const ResView& V = D.V;
-  const uint32_t blocks = chunks_of(V.nf, V.chunk);
+  const uint32_t blocks = chunks_of(V.nf, V.cg_faces * kResBlock);
   const CgParts P{D.cg_parts.data(), blocks};
   ResState* st = D.state.data();
   double* const p[2] = {D.cg_p.data(), D.cg_p.data() + V.nf};
   k_cg_init<<<blocks, kResBlock, 0, ctx.stream>>>(V, rhs, out, D.cg_r.data(), D.cg_z.data(), P,
                                                   st);
   check(hipGetLastError(), "launch momentum CG");
+  auto direction = [&](uint32_t k, uint32_t last) {
+    switch (V.cg_faces) {
+#define MSIM_CG_DIRECTION(F)                                                               \
+  case F:                                                                                  \
+    k_cg_direction<F><<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_z.data(), p[(k + 1) & 1], \
+                                                            p[k & 1], D.cg_q.data(), P, k,  \
+                                                            last, tolerance, st);           \
+    break;
+      MSIM_CG_DIRECTION(1)
+      MSIM_CG_DIRECTION(2)
+      MSIM_CG_DIRECTION(4)
+      default: MSIM_CG_DIRECTION(8)
+#undef MSIM_CG_DIRECTION
+    }
+  };
+  auto step = [&](uint32_t k) {
+    switch (V.cg_faces) {
+#define MSIM_CG_STEP(F)                                                                     \
+  case F:                                                                                   \
+    k_cg_step<F><<<blocks, kResBlock, 0, ctx.stream>>>(V, out, D.cg_r.data(), D.cg_z.data(), \
+                                                       p[k & 1], D.cg_q.data(), P, k, st);  \
+    break;
+      MSIM_CG_STEP(1)
+      MSIM_CG_STEP(2)
+      MSIM_CG_STEP(4)
+      default: MSIM_CG_STEP(8)
+#undef MSIM_CG_STEP
+    }
+  };
   uint32_t k = 0;
   uint32_t next = std::max(D.cg_expect + 1, 4u);
   ResState s{};
@@ -723,18 +895,11 @@ void momentum_cg(Context& ctx, ResDevice& D, const double* rhs, double* out, flo
     const uint32_t batch = std::min(next, max_iterations - k);
     next = 8;
     for (uint32_t i = 0; i < batch; ++i, ++k) {
-      k_cg_direction<<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_z.data(), p[(k + 1) & 1],
-                                                           p[k & 1], D.cg_q.data(), P, k, 0,
-                                                           tolerance, st);
-      k_cg_step<<<blocks, kResBlock, 0, ctx.stream>>>(V, out, D.cg_r.data(), D.cg_z.data(),
-                                                      p[k & 1], D.cg_q.data(), P, k, st);
+      direction(k, 0);
+      step(k);
     }
-    if (k >= max_iterations) {
-      // The test of the last iteration's update.
-      k_cg_direction<<<blocks, kResBlock, 0, ctx.stream>>>(V, D.cg_z.data(), p[(k + 1) & 1],
-                                                           p[k & 1], D.cg_q.data(), P, k, 1,
-                                                           tolerance, st);
-    }
+    // The test of the last iteration's update.
+    if (k >= max_iterations) direction(k, 1);
     check(hipGetLastError(), "launch momentum CG");
     check(hipMemcpyAsync(&s, st, sizeof(ResState), hipMemcpyDeviceToHost, ctx.stream),
           "read CG state");
@@ -791,15 +956,23 @@ void setup_device(Context& ctx, const ResSystem& S, ResDevice& D) {
   V.n = S.n;
   V.cs = 1.0 / std::min({S.h[0], S.h[1], S.h[2]});
   V.chunk = chunk_for(D.m);
+  // Faces per thread of the momentum CG: the fewest that keep a launch at
+  // or under 2048 workgroups.
+  V.cg_faces = 1;
+  while (V.cg_faces < 8 && (S.nf + V.cg_faces * kResBlock - 1) / (V.cg_faces * kResBlock) > 2048) {
+    V.cg_faces *= 2;
+  }
   upload(ctx, D.active, S.active, "upload resolved system");
   upload(ctx, D.exists, S.exists, "upload resolved system");
   upload(ctx, D.fluid, S.fluid, "upload resolved system");
+  upload(ctx, D.neighbours, S.neighbours, "upload resolved system");
   upload(ctx, D.face_drag, S.face_drag, "upload resolved system");
   upload(ctx, D.diag, S.diag, "upload resolved system");
   upload(ctx, D.pd, S.pd, "upload resolved system");
   V.active = D.active.data();
   V.exists = D.exists.data();
   V.fluid = D.fluid.data();
+  V.neighbours = D.neighbours.data();
   V.face_drag = D.face_drag.data();
   V.diag = D.diag.data();
   V.pd = D.pd.data();
@@ -809,8 +982,8 @@ void setup_device(Context& ctx, const ResSystem& S, ResDevice& D) {
   D.directions.reserve(static_cast<size_t>(kRestart) * m);
   for (auto* b : {&D.cg_r, &D.cg_z, &D.cg_q}) b->reserve(S.nf);
   D.cg_p.reserve(2 * S.nf);
-  D.cg_parts.reserve(5 * static_cast<size_t>(chunks_of(S.nf, V.chunk)));
-  D.partials.reserve(2 * static_cast<size_t>(chunks_of(m, V.chunk)));
+  D.cg_parts.reserve(5 * static_cast<size_t>(chunks_of(S.nf, V.cg_faces * kResBlock)));
+  D.partials.reserve((kRestart + 1) * static_cast<size_t>(chunks_of(m, V.chunk)));
   // slots: [0] the last norm, [1 + 2 * i + pass] the Arnoldi projections.
   D.slots.reserve(2 + 2 * kRestart + 2);
   D.state.reserve(1);
@@ -875,34 +1048,34 @@ BlockSolve block_solve(Context& ctx, ResDevice& D, const std::vector<double>& rh
         const double* vj = basis + static_cast<size_t>(j) * m;
         precondition(ctx, D, vj, direction, inner_tolerance, max_inner, out.inner);
         apply_system(ctx, D, direction, D.w.data());
-        // Twice-modified Gram-Schmidt, every scalar on the device: one launch
-        // per projection, each also forming the next dot's partials.
+        // Classical Gram-Schmidt twice, every scalar on the device: the
+        // first pass's coefficients at slot[1 ..], the second's at
+        // slot[1 + kRestart ..], the norm of what remains at slot[0].
         {
           const uint32_t blocks = chunks_of(m, V.chunk);
-          double* parts[2] = {D.partials.data(), D.partials.data() + blocks};
-          k_res_dot<<<blocks, kResBlock, 0, ctx.stream>>>(basis, D.w.data(), m, V.chunk,
-                                                          parts[0]);
-          uint32_t step = 0;
-          for (uint32_t pass = 0; pass < 2; ++pass) {
-            for (uint32_t i = 0; i <= j; ++i, ++step) {
-              const double* vi = basis + static_cast<size_t>(i) * m;
-              const double* next = i < j ? vi + m : (pass == 0 ? basis : nullptr);
-              k_res_mgs<<<blocks, kResBlock, 0, ctx.stream>>>(
-                  m, D.w.data(), vi, next, parts[step & 1], blocks, V.chunk,
-                  slot + 1 + 2 * i + pass, parts[(step + 1) & 1]);
-            }
-          }
-          k_res_sum<<<1, kResBlock, 0, ctx.stream>>>(parts[step & 1], blocks, slot);
+          const uint32_t count = j + 1;
+          double* h1 = slot + 1;
+          double* h2 = slot + 1 + kRestart;
+          double* parts = D.partials.data();
+          k_res_multidot<<<blocks, kResBlock, 0, ctx.stream>>>(m, V.chunk, basis, count,
+                                                               D.w.data(), parts);
+          k_res_multisum<<<1, kResBlock, 0, ctx.stream>>>(parts, count, blocks, h1);
+          k_res_cgs<<<blocks, kResBlock, 0, ctx.stream>>>(m, V.chunk, basis, count, h1,
+                                                          D.w.data(), 0, parts);
+          k_res_multisum<<<1, kResBlock, 0, ctx.stream>>>(parts, count, blocks, h2);
+          k_res_cgs<<<blocks, kResBlock, 0, ctx.stream>>>(m, V.chunk, basis, count, h2,
+                                                          D.w.data(), 1, parts);
+          k_res_sum<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, slot);
           check(hipGetLastError(), "launch Gram-Schmidt");
         }
-        std::vector<double> col(2 + 2 * (j + 1));
+        std::vector<double> col(1 + 2 * kRestart);
         check(hipMemcpyAsync(col.data(), slot, col.size() * sizeof(double),
                              hipMemcpyDeviceToHost, ctx.stream),
               "read Arnoldi column");
         check(hipStreamSynchronize(ctx.stream), "Arnoldi step");
         for (uint32_t i = 0; i <= j; ++i) {
-          h[i][j] += col[1 + 2 * i];
-          h[i][j] += col[1 + 2 * i + 1];
+          h[i][j] += col[1 + i];
+          h[i][j] += col[1 + kRestart + i];
         }
         h[j + 1][j] = std::sqrt(col[0]);
         if (!std::isfinite(h[j + 1][j])) {
