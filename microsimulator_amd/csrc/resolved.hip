@@ -516,19 +516,16 @@ __global__ __launch_bounds__(kResBlock) void k_res_multidot(uint64_t n, uint32_t
   }
 }
 
-// out[i] = the fixed-order sum of partials[i * blocks + 0 .. blocks) for
-// i < count: one workgroup, each wave taking every fourth coefficient.
+// out[i] = the fixed-order sum of partials[i * blocks + 0 .. blocks): one
+// workgroup per coefficient i.
 __global__ __launch_bounds__(kResBlock) void k_res_multisum(const double* __restrict__ partials,
-                                                            uint32_t count, uint32_t blocks,
+                                                            uint32_t blocks,
                                                             double* __restrict__ out) {
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  for (uint32_t i = wave; i < count; i += kResBlock / kWave) {
-    const double* p = partials + static_cast<size_t>(i) * blocks;
-    double acc = 0.0;
-    for (uint32_t b = lane; b < blocks; b += kWave) acc += p[b];
-    acc = wave_sum(acc);
-    if (lane == 0) out[i] = acc;
-  }
+  __shared__ double s_red[kResBlock / kWave];
+  __shared__ double s_b;
+  const double v = all_sum(partials + static_cast<size_t>(blockIdx.x) * blocks, blocks, s_red,
+                           &s_b);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
 // w -= sum_i h[i] v_i over this workgroup's chunk (in order of i), then the
@@ -1059,10 +1056,10 @@ BlockSolve block_solve(Context& ctx, ResDevice& D, const std::vector<double>& rh
           double* parts = D.partials.data();
           k_res_multidot<<<blocks, kResBlock, 0, ctx.stream>>>(m, V.chunk, basis, count,
                                                                D.w.data(), parts);
-          k_res_multisum<<<1, kResBlock, 0, ctx.stream>>>(parts, count, blocks, h1);
+          k_res_multisum<<<count, kResBlock, 0, ctx.stream>>>(parts, blocks, h1);
           k_res_cgs<<<blocks, kResBlock, 0, ctx.stream>>>(m, V.chunk, basis, count, h1,
                                                           D.w.data(), 0, parts);
-          k_res_multisum<<<1, kResBlock, 0, ctx.stream>>>(parts, count, blocks, h2);
+          k_res_multisum<<<count, kResBlock, 0, ctx.stream>>>(parts, blocks, h2);
           k_res_cgs<<<blocks, kResBlock, 0, ctx.stream>>>(m, V.chunk, basis, count, h2,
                                                           D.w.data(), 1, parts);
           k_res_sum<<<1, kResBlock, 0, ctx.stream>>>(parts, blocks, slot);
