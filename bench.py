@@ -39,14 +39,16 @@ def log(*args):
 
 
 # Compulsory HBM bytes per grid site of the multilevel solver's finest-level
-# kernels (FP32 vectors, FP32 coefficients):
-#   k_mg_down   reads r, dinv, wx, wy, bc; writes the coarse residual (1/4 site)
-#   k_mg_up     reads r, dinv, wx, wy, bc, q, the coarse correction (1/4); writes z
+# kernels (FP32 vectors, FP32 coefficients; the V(2,2) cycle):
+#   k_mg_down   reads r, dinv, wx, wy, bc; writes the pre-smoothed iterate and
+#               the coarse residual (1/4 site)
+#   k_mg_up     reads r, the pre-smoothed iterate, q, dinv, wx, wy, bc and the
+#               coarse correction (1/4); writes z
 #   k_fcg_apply reads z, p_old, x, cxl, cyl, bc; writes p_new, q, x
 #   k_fcg_update reads r, q; writes r
 #   k_learn_up  the same traffic as k_mg_up
-MG_BYTES = {"k_mg_down": 5 * 4 + 1, "k_mg_up": 6 * 4 + 1 + 4, "k_fcg_apply": 6 * 4 + 3 * 4,
-            "k_fcg_update": 3 * 4, "k_learn_up": 6 * 4 + 1 + 4}
+MG_BYTES = {"k_mg_down": 5 * 4 + 4 + 1, "k_mg_up": 7 * 4 + 1 + 4, "k_fcg_apply": 6 * 4 + 3 * 4,
+            "k_fcg_update": 3 * 4, "k_learn_up": 7 * 4 + 1 + 4}
 
 # Algorithmic matrix flops per site of the learned smoother (3x3 convolutions
 # 4 -> 16, 16 -> 16, 16 -> 1, two flops per multiply-add), halo excluded.
@@ -56,8 +58,8 @@ BF16_PEAK_TFLOPS = 2500.0
 
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (scaled by a 256 MiB copy's readings) on this workload; the commands and
-# figures are in profiles/krylov_kernels_4096x2048_traffic.md.
-TRAFFIC_MB = {"k_learn_up": 511.5, "k_fcg_apply": 315.3, "k_mg_up": 410.6, "k_mg_down": 180.3,
+# figures are in profiles/krylov_and_resolved_kernels_traffic.md.
+TRAFFIC_MB = {"k_learn_up": 754.5, "k_fcg_apply": 315.5, "k_mg_up": 532.5, "k_mg_down": 210.9,
               "k_fcg_update": 100.9, "k_cg_apply<float>": 353.8}
 
 
@@ -528,6 +530,69 @@ def resolved_flow_bench(msa, hip, threads: int, with_cpu: bool):
                     f"{rrep['outer_iterations']} outer / {rrep['inner_iterations']} inner, "
                     f"field difference {diff:.2e}")
         out[name] = entry
+        last = (grid, field)
+    return out, last
+
+
+def layered_transport_bench(msa, hip, grid, field, threads: int, with_cpu: bool, steps: int = 50):
+    """Explicit FP32 transport of 3 solutes with Monod uptake on a grid of
+    several layers, over the resolved flow's field on the same grid (the
+    three-axis kernel k_adr3_step), and the reference's advance_signal_grid_cpu
+    on the same grid and field for a few steps (Monod as the affine loss at
+    level 0.5, the same work per site)."""
+    S = 3
+    n = grid.nx * grid.ny * grid.nz
+    rng = np.random.default_rng(23)
+    solid = np.asarray(grid.obstacles, np.uint8).reshape(-1)
+    phi = (0.6 * rng.random(n)).astype(np.float32)
+    phi[solid != 0] = 0.0
+    bv = np.zeros(6 * S, np.float32)
+    bv[0::6] = 1.0
+    diffusion = np.array([1.0, 5.0, 10.0], np.float32)
+    spec = msa.SignalGridSpec(grid, diffusion, bv, field, phi=phi, monod_vmax=[1.0] * S,
+                              monod_k=[0.1] * S)
+    outflow = 2.0 * sum(float(np.abs(f).max()) for f in (field.x_faces, field.y_faces,
+                                                          field.z_faces))
+    bound = 2.0 * 10.0 * 3 + outflow + float(phi.max()) / 0.1
+    dt = 0.9 / bound
+    levels = np.zeros(S * n, np.float32)
+    hip.adr_upload(spec, levels, precision=32)
+    hip.adr_run(dt, 2, ledger=False)  # warm
+    hip.adr_upload(spec, levels, precision=32)
+    hip.adr_run(dt, steps, ledger=False)
+    secs = hip.adr_seconds()
+    fluid = int((solid == 0).sum())
+    gcups = fluid * S * steps / secs / 1e9
+    bytes_per_site = 8 * S + 3 * 4 + 4 + 1  # levels in and out, three faces, phi, mask
+    out = {"workload": f"{grid.nx}x{grid.ny}x{grid.nz} channel (8% solid voxels), the resolved "
+                       f"flow's field, 3 solutes, D in {{1, 5, 10}}, Monod uptake, FP32, "
+                       f"forward Euler at 0.9 of the explicit bound",
+           "steps": steps, "dt": dt, "gcell_upd_per_s": round(gcups, 3),
+           "achieved_gbs": round(bytes_per_site * n / (secs / steps) / 1e9, 1),
+           "bytes_per_site": bytes_per_site}
+    log(f"layered transport {grid.nx}x{grid.ny}x{grid.nz}: {gcups:.2f} Gcell-upd/s "
+        f"({out['achieved_gbs']:.0f} GB/s)")
+    if with_cpu:
+        from oracle import Grid as OGrid
+        from oracle import Reference
+        if Reference.available():
+            ref = Reference()
+            ref.set_threads(threads)
+            og = OGrid(grid.nx, grid.ny, grid.nz, grid.dx, grid.dy, grid.dz, grid.obstacles,
+                       grid.boundary)
+            loss = np.tile(phi / np.float32(0.1 + 0.5), S)  # zero at obstacles, as phi is
+            faces = field.concatenated()
+            cpu_steps = 3
+            start = np.tile(np.where(solid != 0, 0.0, 0.5).astype(np.float32), S)
+            _, cpu_secs = ref.advance_signal_grid(og, start, dt, cpu_steps, diffusion=diffusion,
+                                                  boundary_values=bv, faces=faces, loss=loss)
+            out["cpu_reference"] = {
+                "gcell_upd_per_s": round(fluid * S * cpu_steps / cpu_secs / 1e9, 4),
+                "cores": ref.threads(), "kind": "reference",
+                "sample": f"{cpu_steps} steps of advance_signal_grid_cpu on the same grid and "
+                          f"field, Monod as the affine loss at level 0.5, in {cpu_secs:.2f} s"}
+            log(f"layered transport reference CPU: "
+                f"{out['cpu_reference']['gcell_upd_per_s']} Gcell-upd/s")
     return out
 
 
@@ -877,8 +942,11 @@ def main():
         per_iteration = line.get("cpu_baseline", {}).get("seconds_per_iteration")
         line["port_flow"] = port_flow_bench(msa, w, hip, per_iteration)
     if not args.no_resolved:
-        line["resolved_flow"] = resolved_flow_bench(msa, hip, min(16, os.cpu_count() or 1),
-                                                    not args.no_cpu)
+        threads = min(16, os.cpu_count() or 1)
+        line["resolved_flow"], (grid3, field3) = resolved_flow_bench(msa, hip, threads,
+                                                                     not args.no_cpu)
+        line["adr_layered"] = layered_transport_bench(msa, hip, grid3, field3, threads,
+                                                      not args.no_cpu)
     hip.close()
     print(json.dumps(line), flush=True)
 
