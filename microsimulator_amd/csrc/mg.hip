@@ -49,6 +49,7 @@ constexpr int kTailThreads = 1024;
 constexpr uint32_t kCoarseCells = 256;  // densely inverted
 constexpr uint32_t kTailCells = 4096;  // levels handled by the single-block kernel
 constexpr uint32_t kMaxLevels = 24;
+static_assert(4 * kCoarseCells <= kTailThreads, "the coarsest solve splits every row in parts");
 constexpr float kOmega = 0.8f;
 constexpr float kCorrection = 1.8f;
 
@@ -284,55 +285,63 @@ struct TailLevel {
 struct TailArgs {
   TailLevel lv[kMaxLevels];
   uint32_t count;      // levels from the first tail level to the coarsest
-  const float* cinv;   // dense inverse of the coarsest operator
+  const float* cinv;   // dense inverse of the coarsest operator, transposed
   uint32_t n;          // its size
   UpDots dots;         // when the tail starts at level 0
 };
 
-// (A x)_i on a tail level, x read from the level array xv.
-__device__ inline float tail_ax(const TailLevel& L, const float* xv, size_t i, uint32_t fx,
-                                uint32_t fy) {
-  const float xc = xv[i];
+// (A x)_i on a tail level at site i = fx ny + fy, with x_j = xv(j); the
+// terms in the order of the tile kernels.
+template <typename X>
+__device__ inline float tail_ax(const TailLevel& L, X xv, uint32_t i, uint32_t fx, uint32_t fy) {
+  const float xc = xv(i);
   float ax = L.bc[i] * xc;
-  if (fx > 0) ax += L.wx[i] * (xc - xv[i - L.ny]);
-  if (fx + 1 < L.nx) ax += L.wx[i + L.ny] * (xc - xv[i + L.ny]);
-  if (fy > 0) ax += L.wy[i] * (xc - xv[i - 1]);
-  if (fy + 1 < L.ny) ax += L.wy[i + 1] * (xc - xv[i + 1]);
+  if (fx > 0) ax += L.wx[i] * (xc - xv(i - L.ny));
+  if (fx + 1 < L.nx) ax += L.wx[i + L.ny] * (xc - xv(i + L.ny));
+  if (fy > 0) ax += L.wy[i] * (xc - xv(i - 1));
+  if (fy + 1 < L.ny) ax += L.wy[i + 1] * (xc - xv(i + 1));
   return ax;
 }
 
-// out = in + omega D^-1 (r - A in) on every cell of a tail level.
-__device__ inline void tail_sweep(const TailLevel& L, const float* in, float* out) {
-  const size_t cells = static_cast<size_t>(L.nx) * L.ny;
-  for (size_t i = threadIdx.x; i < cells; i += kTailThreads) {
-    const uint32_t fx = static_cast<uint32_t>(i / L.ny), fy = static_cast<uint32_t>(i % L.ny);
-    out[i] = in[i] + kOmega * L.dinv[i] * (L.r[i] - tail_ax(L, in, i, fx, fy));
+// out = x + omega D^-1 (r - A x) on every cell of a tail level, x_j = xv(j).
+template <typename X>
+__device__ inline void tail_sweep(const TailLevel& L, X xv, float* out) {
+  const uint32_t cells = L.nx * L.ny;
+  for (uint32_t i = threadIdx.x; i < cells; i += kTailThreads) {
+    const uint32_t fx = i / L.ny, fy = i - fx * L.ny;
+    out[i] = xv(i) + kOmega * L.dinv[i] * (L.r[i] - tail_ax(L, xv, i, fx, fy));
   }
 }
 
 // The cycle on the small levels in one workgroup: down passes, the dense
-// coarsest solve, up passes, with workgroup barriers between the sweeps.
+// coarsest solve, up passes, with workgroup barriers between the phases.
+// The first sweep of each smoothing pair is not stored: the second sweep
+// evaluates it at the sites its stencil reads (from zero going down: omega
+// D^-1 r; going up: the prolonged iterate), so a level costs two phases each
+// way. The coarsest inverse is stored transposed, so the threads of a wave
+// read one row of it together.
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs T, const CgState* st) {
   __shared__ double s_red[kTailThreads / kWave];
   if (cycle_stopped(st)) return;
   for (uint32_t l = 0; l + 1 < T.count; ++l) {
     const TailLevel& F = T.lv[l];
     const TailLevel& Cl = T.lv[l + 1];
-    const size_t fine = static_cast<size_t>(F.nx) * F.ny;
-    for (size_t i = threadIdx.x; i < fine; i += kTailThreads) F.t[i] = kOmega * F.dinv[i] * F.r[i];
+    const float* fr = F.r;
+    const float* fd = F.dinv;
+    tail_sweep(F, [fr, fd](uint32_t j) { return kOmega * fd[j] * fr[j]; }, F.xp);
     __syncthreads();
-    tail_sweep(F, F.t, F.xp);
-    __syncthreads();
-    const size_t cells = static_cast<size_t>(Cl.nx) * Cl.ny;
-    for (size_t c = threadIdx.x; c < cells; c += kTailThreads) {
-      const uint32_t X = static_cast<uint32_t>(c / Cl.ny), Y = static_cast<uint32_t>(c % Cl.ny);
+    const float* fxp = F.xp;
+    const auto xp = [fxp](uint32_t j) { return fxp[j]; };
+    const uint32_t cells = Cl.nx * Cl.ny;
+    for (uint32_t c = threadIdx.x; c < cells; c += kTailThreads) {
+      const uint32_t X = c / Cl.ny, Y = c - X * Cl.ny;
       float sum = 0.0f;
       for (int a = 0; a < 2; ++a) {
         for (int b = 0; b < 2; ++b) {
           const uint32_t fx = 2 * X + a, fy = 2 * Y + b;
           if (fx < F.nx && fy < F.ny) {
-            const size_t i = static_cast<size_t>(fx) * F.ny + fy;
-            sum += F.r[i] - tail_ax(F, F.xp, i, fx, fy);
+            const uint32_t i = fx * F.ny + fy;
+            sum += F.r[i] - tail_ax(F, xp, i, fx, fy);
           }
         }
       }
@@ -342,15 +351,42 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs T, const CgSt
   }
   double zr = 0.0, zq = 0.0;
   {
+    // e = C^-1 r on the coarsest level (n <= kCoarseCells): the columns are
+    // split into `parts` contiguous ranges, one thread per (row, range), the
+    // rows of a wave adjacent, so each step of a thread's range is one
+    // coalesced read of the transposed inverse; the ranges' sums are then
+    // added in range order.
     const TailLevel& B = T.lv[T.count - 1];
     const bool dots = T.count == 1 && T.dots.q != nullptr;
-    for (uint32_t row = threadIdx.x; row < T.n; row += kTailThreads) {
+    __shared__ float s_r[kCoarseCells];
+    __shared__ float s_part[kTailThreads];
+    const uint32_t n = T.n;
+    for (uint32_t j = threadIdx.x; j < n; j += kTailThreads) s_r[j] = B.r[j];
+    __syncthreads();
+    const uint32_t parts = kTailThreads / n;  // n <= kCoarseCells <= kTailThreads / 4
+    const uint32_t span = (n + parts - 1) / parts;
+    if (threadIdx.x < parts * n) {
+      const uint32_t row = threadIdx.x % n, part = threadIdx.x / n;
+      const uint32_t j0 = part * span, j1 = min(n, j0 + span);
       float v = 0.0f;
-      const float* m = T.cinv + static_cast<size_t>(row) * T.n;
-      for (uint32_t j = 0; j < T.n; ++j) v += m[j] * B.r[j];
+      uint32_t j = j0;
+      for (; j + 8 <= j1; j += 8) {
+        float m[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m[k] = T.cinv[static_cast<size_t>(j + k) * n + row];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v += m[k] * s_r[j + k];
+      }
+      for (; j < j1; ++j) v += T.cinv[static_cast<size_t>(j) * n + row] * s_r[j];
+      s_part[part * n + row] = v;
+    }
+    __syncthreads();
+    for (uint32_t row = threadIdx.x; row < n; row += kTailThreads) {
+      float v = 0.0f;
+      for (uint32_t part = 0; part < parts; ++part) v += s_part[part * n + row];
       B.e[row] = v;
       if (dots) {
-        zr += static_cast<double>(v) * static_cast<double>(B.r[row]);
+        zr += static_cast<double>(v) * static_cast<double>(s_r[row]);
         zq += static_cast<double>(v) * static_cast<double>(T.dots.q[row]);
       }
     }
@@ -359,19 +395,24 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs T, const CgSt
   for (int l = static_cast<int>(T.count) - 2; l >= 0; --l) {
     const TailLevel& F = T.lv[l];
     const TailLevel& Cl = T.lv[l + 1];
-    const size_t cells = static_cast<size_t>(F.nx) * F.ny;
+    const uint32_t cells = F.nx * F.ny;
     const bool dots = l == 0 && T.dots.q != nullptr;
-    for (size_t i = threadIdx.x; i < cells; i += kTailThreads) {
-      const uint32_t fx = static_cast<uint32_t>(i / F.ny), fy = static_cast<uint32_t>(i % F.ny);
-      F.t[i] = F.xp[i] + kCorrection * Cl.e[static_cast<size_t>(fx / 2) * Cl.ny + fy / 2];
-    }
+    // The prolonged iterate x_pre + c P e_c, evaluated where it is read.
+    const float* fxp = F.xp;
+    const float* ce = Cl.e;
+    const uint32_t ny = F.ny, cny = Cl.ny;
+    const auto prolonged = [fxp, ce, ny, cny](uint32_t j) {
+      const uint32_t jx = j / ny, jy = j - jx * ny;
+      return fxp[j] + kCorrection * ce[(jx / 2) * cny + jy / 2];
+    };
+    tail_sweep(F, prolonged, F.t);
     __syncthreads();
-    tail_sweep(F, F.t, F.xp);
-    __syncthreads();
-    for (size_t i = threadIdx.x; i < cells; i += kTailThreads) {
-      const uint32_t fx = static_cast<uint32_t>(i / F.ny), fy = static_cast<uint32_t>(i % F.ny);
+    const float* ft = F.t;
+    const auto smoothed = [ft](uint32_t j) { return ft[j]; };
+    for (uint32_t i = threadIdx.x; i < cells; i += kTailThreads) {
+      const uint32_t fx = i / F.ny, fy = i - fx * F.ny;
       const float ri = F.r[i];
-      const float z = F.xp[i] + kOmega * F.dinv[i] * (ri - tail_ax(F, F.xp, i, fx, fy));
+      const float z = F.t[i] + kOmega * F.dinv[i] * (ri - tail_ax(F, smoothed, i, fx, fy));
       F.e[i] = z;
       if (dots) {
         zr += static_cast<double>(z) * static_cast<double>(ri);
@@ -1098,6 +1139,8 @@ void mg_prepare(Context& ctx) {
   // Dense inverse of the coarsest operator.
   MgLevel& B = *mg.levels.back();
   const size_t bn = B.cells();
+  if (bn > kCoarseCells)
+    throw Error(MSIM_INVALID_ARGUMENT, "multilevel: the grid needs more than 24 levels");
   std::vector<float> wx(bn), wy(bn), bc(bn);
   check(hipMemcpyAsync(wx.data(), B.wx.data(), bn * sizeof(float), hipMemcpyDeviceToHost,
                        ctx.stream),
@@ -1110,11 +1153,16 @@ void mg_prepare(Context& ctx) {
         "download coarse operator");
   check(hipStreamSynchronize(ctx.stream), "coarse operator");
   const std::vector<float> inv = coarse_inverse(wx, wy, bc, B.nx, B.ny);
+  // Stored transposed: entry (row, j) at j * n + row.
+  std::vector<float> inv_t(inv.size());
+  for (size_t row = 0; row < bn; ++row)
+    for (size_t j = 0; j < bn; ++j) inv_t[j * bn + row] = inv[row * bn + j];
   mg.coarse_n = static_cast<uint32_t>(bn);
-  mg.cinv.reserve(inv.size());
-  check(hipMemcpyAsync(mg.cinv.data(), inv.data(), inv.size() * sizeof(float),
+  mg.cinv.reserve(inv_t.size());
+  check(hipMemcpyAsync(mg.cinv.data(), inv_t.data(), inv_t.size() * sizeof(float),
                        hipMemcpyHostToDevice, ctx.stream),
         "upload coarse inverse");
+  check(hipStreamSynchronize(ctx.stream), "upload coarse inverse");
   // Flexible-CG partial sums, sized for the finest level on the tile kernels.
   const FcgPartials parts = fcg_partials(ctx);
   const size_t p0 = std::max<size_t>({static_cast<size_t>(up_blocks(*mg.levels[0])),
