@@ -69,7 +69,7 @@ struct BatchItem {
 
 // The multilevel hierarchy of one instance: level l has nx[l] x ny[l] cells
 // at offset off[l] of the level arrays (level 0 at the instance's own first
-// site); the coarsest level's dense inverse is coarse_n x coarse_n floats at
+// site); the coarsest level's dense inverse is coarse_n x coarse_n floats, transposed, at
 // offset cinv of the inverse pool.
 constexpr uint32_t kBatchLevels = 16;
 constexpr uint32_t kBatchCoarse = 64;
@@ -111,7 +111,7 @@ struct BatchData {
   const float* ldinv;
   float* lr;             // level residual and correction (coarse levels)
   float* le;
-  const float* cinv;     // dense inverses of the coarsest levels
+  const float* cinv;     // dense inverses of the coarsest levels, each transposed
 };
 
 struct BatchParams {
@@ -285,14 +285,31 @@ __device__ double batch_cycle(const BatchData& D, const BatchLevels& V, size_t s
   double rz = 0.0;
   {
     const LevelRef B = level_ref(D, V, K, site);
+    // The inverse is stored transposed; each row's columns are split into
+    // NT / n contiguous ranges, one thread each (the rows of a wave adjacent,
+    // so a step reads the inverse coalesced), added in range order.
+    static_assert(NT >= static_cast<int>(kBatchCoarse), "one thread per row at least");
+    __shared__ float s_r[kBatchCoarse];
+    __shared__ float s_part[NT];
     const uint32_t n = V.coarse_n;
     const float* inv = D.cinv + V.cinv;
-    for (uint32_t row = threadIdx.x; row < n; row += NT) {
-      const float* m = inv + static_cast<size_t>(row) * n;
+    for (uint32_t j = threadIdx.x; j < n; j += NT) s_r[j] = B.r[j];
+    __syncthreads();
+    const uint32_t parts = NT / n;
+    const uint32_t span = (n + parts - 1) / parts;
+    if (threadIdx.x < parts * n) {
+      const uint32_t row = threadIdx.x % n, part = threadIdx.x / n;
+      const uint32_t j1 = min(n, part * span + span);
       float v = 0.0f;
-      for (uint32_t j = 0; j < n; ++j) v += m[j] * B.r[j];
+      for (uint32_t j = part * span; j < j1; ++j) v += inv[static_cast<size_t>(j) * n + row] * s_r[j];
+      s_part[part * n + row] = v;
+    }
+    __syncthreads();
+    for (uint32_t row = threadIdx.x; row < n; row += NT) {
+      float v = 0.0f;
+      for (uint32_t part = 0; part < parts; ++part) v += s_part[part * n + row];
       B.e[row] = v;
-      if (K == 0) rz += static_cast<double>(v) * static_cast<double>(B.r[row]);
+      if (K == 0) rz += static_cast<double>(v) * static_cast<double>(s_r[row]);
     }
     __syncthreads();
   }
@@ -999,7 +1016,10 @@ void build_hierarchies(Context& ctx, BatchDevice& B) {
     const BatchLevels& L = B.levels[k];
     const std::vector<float> inv =
         mg_coarse_inverse(wx[k], wy[k], bc[k], L.nx[L.count - 1], L.ny[L.count - 1]);
-    std::copy(inv.begin(), inv.end(), all.begin() + static_cast<std::ptrdiff_t>(L.cinv));
+    // Transposed: entry (row, j) at j * n + row.
+    const size_t n = L.coarse_n;
+    for (size_t row = 0; row < n; ++row)
+      for (size_t j = 0; j < n; ++j) all[L.cinv + j * n + row] = inv[row * n + j];
   }
   check(hipMemcpyAsync(B.cinv.data(), all.data(), all.size() * sizeof(float),
                        hipMemcpyHostToDevice, ctx.stream),

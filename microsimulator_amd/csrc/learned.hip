@@ -340,6 +340,8 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   // columns, four times over (rows r0, r0 + 6, r0 + 12, r0 + 18), so a thread
   // keeps one column and its four pixels differ by 6 rows.
   constexpr int kStageRows = 6, kStageThreads = kStageRows * kPitch, kPer = kRows / kStageRows;
+  constexpr int kOutPer = kTX * kTY / kThreads;  // output pixels per thread
+  static_assert(kOutPer * kThreads == kTX * kTY, "whole output pixels per thread");
   static_assert(kStageRows * kPer == kRows && kStageThreads <= kThreads, "staging map");
   const int scol = static_cast<int>(threadIdx.x) % kPitch;
   const int srow = static_cast<int>(threadIdx.x) / kPitch;
@@ -350,6 +352,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
   // tile's round is issued before this tile's network runs.
   struct Stage {
     float r[kPer], d[kPer], x0[kPer], ec[kPer], wxl[kPer], wxu[kPer], wyl[kPer], wyu[kPer], b[kPer];
+    float qo[kOutPer];  // q at the thread's output pixels
     uint32_t on, feat;
   };
   auto load_tile = [&](uint32_t t) {
@@ -377,6 +380,16 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
           v.wyu[k] = static_cast<uint32_t>(gy) + 1 < G.ny ? G.wy[i + 1] : 0.0f;
           v.b[k] = G.bc[i];
         }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < kOutPer; ++m) {
+      const int k = static_cast<int>(threadIdx.x) + kThreads * m;
+      const uint32_t gx = static_cast<uint32_t>(fx0 + k / kTY);
+      const uint32_t gy = static_cast<uint32_t>(fy0 + k % kTY);
+      if (out.q != nullptr && gx < G.nx && gy < G.ny) {
+        const size_t i = static_cast<size_t>(gx) * G.ny + gy;
+        v.qo[m] = out.q[i];
       }
     }
     return v;
@@ -458,7 +471,9 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
     __syncthreads();
     layers12<kThreads>(s_in, s_a1, s_a2, s_mask, frag);
     layer3<kThreads>(s_a2, s_T, frag);
-    for (int k = threadIdx.x; k < kTX * kTY; k += kThreads) {
+#pragma unroll
+    for (int m = 0; m < kOutPer; ++m) {
+      const int k = static_cast<int>(threadIdx.x) + kThreads * m;
       const int lx = k / kTY, ly = k % kTY;
       const uint32_t gx = static_cast<uint32_t>(fx0 + lx), gy = static_cast<uint32_t>(fy0 + ly);
       if (gx >= G.nx || gy >= G.ny) continue;
@@ -469,7 +484,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_learn_up(Geometry G, const floa
       out.z[i] = z;
       if (out.q != nullptr) {
         zr += static_cast<double>(z) * static_cast<double>(r[i]);
-        zq += static_cast<double>(z) * static_cast<double>(out.q[i]);
+        zq += static_cast<double>(z) * static_cast<double>(cur.qo[m]);
       }
     }
     __syncthreads();  // the next tile rewrites every shared array
