@@ -86,7 +86,9 @@ typedef struct {
   uint32_t solver;             /* MSIM_SOLVER_* */
   uint32_t scale;              /* 1: scale to mean_inlet_speed as the reference does;
                                   0: return the raw Dirichlet (1 -> 0) field */
-  uint32_t check_interval;     /* iterations between host convergence checks; 0 = 64 */
+  uint32_t check_interval;     /* iterations between host convergence checks; 0 = the
+                                  form's own: 64 for Jacobi-PCG (FP64 and MIXED), 8 for
+                                  flexible CG (MULTIGRID, LEARNED) */
   double inner_tolerance;      /* MIXED/LEARNED: inner FP32 relative tolerance; 0 = auto */
   uint32_t fallback_iterations;/* LEARNED: iteration cap before the Jacobi-PCG fallback */
   uint32_t profile;            /* 1: time every Krylov kernel launch with events */
@@ -220,7 +222,7 @@ typedef struct {
   float relative_tolerance;     /* per pressure solve; at least float epsilon */
   uint32_t max_iterations;      /* per pressure solve; positive */
   uint32_t solver;              /* MSIM_SOLVER_*: JACOBI_FP64 is the reference's PCG */
-  uint32_t check_interval;      /* as msim_flow_params; 0 = 64 */
+  uint32_t check_interval;      /* as msim_flow_params */
   double inner_tolerance;       /* MIXED/LEARNED inner tolerance; 0 = auto */
   uint32_t fallback_iterations; /* LEARNED: cap before the Jacobi-PCG fallback */
 } msim_port_flow_params;

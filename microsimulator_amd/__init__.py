@@ -388,7 +388,7 @@ class DepthAveragedFlowParameters:
     max_iterations: int = 50_000
     solver: int = SOLVER_JACOBI_FP64
     scale: bool = True
-    check_interval: int = 64
+    check_interval: int = 0  # 0: 64 for Jacobi-PCG, 8 for flexible CG
     inner_tolerance: float = 0.0
     fallback_iterations: int = 200
     profile: bool = False
@@ -578,7 +578,7 @@ class DepthAveragedPortFlowParameters:
     relative_tolerance: float = 1.0e-6
     max_iterations: int = 50_000
     solver: int = SOLVER_JACOBI_FP64
-    check_interval: int = 64
+    check_interval: int = 0  # 0: 64 for Jacobi-PCG, 8 for flexible CG
     inner_tolerance: float = 0.0
     fallback_iterations: int = 200
 
