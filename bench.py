@@ -59,7 +59,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (scaled by a 256 MiB copy's readings) on this workload; the commands and
 # figures are in profiles/krylov_and_resolved_kernels_traffic.md.
-TRAFFIC_MB = {"k_learn_up": 750.6, "k_fcg_apply": 315.7, "k_mg_up": 529.7, "k_mg_down": 210.9,
+TRAFFIC_MB = {"k_learn_up": 751.4, "k_fcg_apply": 315.7, "k_mg_up": 530.1, "k_mg_down": 210.9,
               "k_fcg_update": 100.9, "k_cg_apply<float>": 353.7}
 
 
