@@ -113,8 +113,17 @@ struct AdrRow {
 // (the reference throws, signals.cpp:1119-1131); the step is then refused as
 // a whole. Under van Leer each block also leaves the largest level it wrote,
 // for the settle that follows.
+// Least waves per SIMD asked of the step kernel: four for the FP64 upwind
+// step without an affine reaction, which fits 128 VGPRs without spills once
+// its ledger sums live in LDS (three waves otherwise, at 131); 1 (no demand)
+// for the forms that would spill.
+template <typename T, int S, bool kAffine, bool kLimited>
+constexpr int step_waves() {
+  return sizeof(T) == 8 && !kAffine && !kLimited && S <= 3 ? 4 : 1;
+}
+
 template <typename T, int S, int TY, bool kAffine, bool kLimited, bool kPatched>
-__global__ __launch_bounds__(TY) void k_adr_step(
+__global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(step_waves<T, S, kAffine, kLimited>()))) void k_adr_step(
     AdrParams P, const uint8_t* __restrict__ solid, const float* __restrict__ xf,
     const float* __restrict__ yf, const float* __restrict__ phi, const float* __restrict__ source,
     const float* __restrict__ loss, const T* __restrict__ in, T* __restrict__ out, float dt,
@@ -185,9 +194,14 @@ __global__ __launch_bounds__(TY) void k_adr_step(
     }
     return w;
   };
-  double acc[kF64 ? 3 * S : 1];
+  // FP64 ledger sums (mass after, supply, uptake per signal), one slot per
+  // thread in LDS: kept in registers they would take 24 VGPRs and hold the
+  // step at three waves per SIMD instead of four.
+  __shared__ double s_acc[kF64 ? 3 * S : 1][TY];
+  if constexpr (kF64) {
 #pragma unroll
-  for (int k = 0; k < (kF64 ? 3 * S : 1); ++k) acc[k] = 0.0;
+    for (int k = 0; k < 3 * S; ++k) s_acc[k][threadIdx.x] = 0.0;
+  }
   T top = T(0);  // largest level written (van Leer)
   bool bad = false;
   if (x0 < x1) {
@@ -362,9 +376,9 @@ __global__ __launch_bounds__(TY) void k_adr_step(
               if (!isfinite(v) || (v < 0.0 && !kLimited)) bad = true;
               if constexpr (kLimited) top = max(top, v);
               out[s * n + i] = v;
-              acc[3 * s + 0] += v;
-              acc[3 * s + 1] += supply;
-              acc[3 * s + 2] += lossr * c;
+              s_acc[3 * s + 0][t] += v;
+              s_acc[3 * s + 1][t] += supply;
+              s_acc[3 * s + 2][t] += lossr * c;
             }
           }
         }
@@ -391,6 +405,9 @@ __global__ __launch_bounds__(TY) void k_adr_step(
   }
   if constexpr (kF64) {
     // Solid sites hold their (zero) level; the mass counts only what moved.
+    double acc[3 * S];
+#pragma unroll
+    for (int k = 0; k < 3 * S; ++k) acc[k] = s_acc[k][threadIdx.x];
     block_sums<TY, 3 * S>(acc, s_red);
     if (threadIdx.x == 0) {
 #pragma unroll
