@@ -140,6 +140,10 @@ struct MgDevice {
   uint32_t coarse_n = 0;      // cells of the coarsest level
   DeviceBuffer<float> cinv;   // inverse of the coarsest operator (dense)
   DeviceBuffer<double> zr, zq, pq, rr;  // flexible-CG partial sums, two generations
+  // Iterations the last inner flexible-CG solve of each (cycle form,
+  // refinement) took: the next solve launches that many plus one before its
+  // first host check (0: no record).
+  uint32_t fcg_expect[3][8] = {};
   bool ready = false;
 };
 

@@ -952,7 +952,8 @@ void launch_fcg_apply(Context& ctx, const float* z, const float* p_old, float* p
 // iterations. Returns the iterations taken; *converged says whether the
 // stopping test held.
 uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, uint32_t interval,
-                 bool* converged, KernelTimes* times, CycleMode mode, Snapshots* snaps) {
+                 uint32_t first, bool* converged, KernelTimes* times, CycleMode mode,
+                 Snapshots* snaps) {
   auto& f = ctx.flow;
   const size_t n = f.sites();
   float* r = f.r32.data();
@@ -978,6 +979,8 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
     interval = 1;
     snapshot();
   }
+  // The event probes cover one batch of `interval` iterations.
+  if (snaps != nullptr || times != nullptr) first = interval;
   constexpr int kP = 6;  // events per iteration
   if (times != nullptr) {
     while (ctx.probes.size() < kP * static_cast<size_t>(interval) + kP) {
@@ -995,7 +998,9 @@ uint32_t run_fcg(Context& ctx, double rnorm, double tolerance, uint32_t budget, 
   bool tested_all = false;
   while (true) {
     const bool last_test = launched >= budget;
-    const uint32_t batch = last_test ? 0 : std::min(interval, budget - launched);
+    const uint32_t batch =
+        last_test ? 0 : std::min(launched == 0 ? std::max(first, interval) : interval,
+                                 budget - launched);
     for (uint32_t j = 0; j < batch; ++j) {
       const uint32_t it = launched + j + 1;
       const hipEvent_t* pr = times != nullptr ? &ctx.probes[kP * j] : nullptr;
@@ -1222,8 +1227,11 @@ uint32_t solve_refined(Context& ctx, const msim_flow_params& params, msim_flow_r
     }
     if (iterations >= params.max_iterations || refinements >= 64) break;
     bool inner_converged = false;
+    uint32_t& expect = f.mg.fcg_expect[mode][std::min(refinements, 7u)];
     const uint32_t taken = run_fcg(ctx, rnorm, inner, params.max_iterations - iterations,
-                                   interval, &inner_converged, times, mode, snaps);
+                                   interval, expect > 0 ? expect + 1 : interval,
+                                   &inner_converged, times, mode, snaps);
+    expect = inner_converged ? taken : 0;
     iterations += taken;
     ++refinements;
     k_mg_refine<<<stream_blocks(n), kStream, 0, ctx.stream>>>(f.x.data(), f.x32.data(), rnorm,
